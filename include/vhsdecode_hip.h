@@ -1,0 +1,215 @@
+/*
+ * VHSDecode.Hip.Native — C ABI of the MI355X (gfx950) per-block RF decode backend.
+ *
+ * Conventions follow the reference's native bridges
+ * (src/VHSDecode.CudaFast.Native/include/vhsdecode_cuda_fast.h:1-180,
+ *  src/VHSDecode.Ipp.Native/include/vhsdecode_ipp.h:1-366):
+ *  - extern "C", cdecl, default visibility; no C++ or torch types cross the ABI;
+ *  - every struct is versioned (_v1) and starts with uint32_t struct_size, which
+ *    the library validates;
+ *  - int32_t status: 0 = OK, negative codes in the -30000 range; exceptions never
+ *    cross the ABI and vhsdecode_hip_get_last_error() returns a thread-local
+ *    message for the last failure on the calling thread;
+ *  - opaque context handles with create/destroy; destroying NULL is a no-op;
+ *  - a context serialises its own calls; different contexts may run concurrently.
+ *
+ * Pointer arguments documented as "device" are HBM addresses on the context's
+ * device (hipMalloc or torch allocations) and the call is asynchronous on the
+ * given stream (NULL = the context's own stream); "host" arguments are ordinary
+ * process memory and the call is synchronous.
+ */
+#ifndef VHSDECODE_HIP_H
+#define VHSDECODE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(_WIN32)
+#define VHSDECODE_HIP_API __declspec(dllexport)
+#else
+#define VHSDECODE_HIP_API __attribute__((visibility("default")))
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VHSDECODE_HIP_ABI_VERSION 1
+
+enum {
+  VHSDECODE_HIP_OK = 0,
+  VHSDECODE_HIP_E_INVALID_ARGUMENT = -30001,
+  VHSDECODE_HIP_E_STRUCT_SIZE = -30002,
+  VHSDECODE_HIP_E_NO_DEVICE = -30003,
+  VHSDECODE_HIP_E_HIP = -30004,
+  VHSDECODE_HIP_E_OUT_OF_MEMORY = -30005,
+  VHSDECODE_HIP_E_UNSUPPORTED = -30006,
+  VHSDECODE_HIP_E_CAPACITY = -30007,
+  VHSDECODE_HIP_E_INTERNAL = -30099
+};
+
+typedef struct vhsdecode_hip_demod_context vhsdecode_hip_demod_context;
+
+/* ---------------------------------------------------------------- runtime
+ * Replaces vhsdecode_cuda_fast_get_abi_version / _get_runtime_info / _get_last_error
+ * (vhsdecode_cuda_fast.h:145-152, vhsdecode_ipp.h:98-116); the managed loader checks
+ * the ABI version together with the exported symbol list
+ * (src/VHSDecode.Core/Dsp/Ipp/IppNativeMethods.cs:8-47, IppRuntime.cs:48-122). */
+typedef struct vhsdecode_hip_runtime_info_v1 {
+  uint32_t struct_size;
+  int32_t device_count;
+  int32_t device_id;
+  int32_t compute_units;
+  uint64_t hbm_bytes;
+  int32_t lds_bytes_per_workgroup;
+  char device_name[128];
+  char gcn_arch[64];
+} vhsdecode_hip_runtime_info_v1;
+
+VHSDECODE_HIP_API int32_t vhsdecode_hip_get_abi_version(void);
+VHSDECODE_HIP_API int32_t vhsdecode_hip_get_runtime_info(int32_t device_id, vhsdecode_hip_runtime_info_v1* info);
+VHSDECODE_HIP_API const char* vhsdecode_hip_get_last_error(void);
+
+/* ---------------------------------------------------------------- block demodulator
+ * The exact/ipp-fast block seam: one call demodulates a run of overlap-save blocks
+ * the way RfDemodulator.DemodulateCore does per block
+ * (src/VHSDecode.Core/Dsp/RfDemodulator.cs:288-918, VHS real-FFT path) plus
+ * RfBlockDecodePipeline.DecodeChromaBurstFloat32 (Decode/RfBlockDecodePipeline.cs:539-573),
+ * and writes the trimmed windows stitched as RfBlockStreamDecoder.CopyTrimmedWindow
+ * does (Decode/RfBlockStreamDecoder.cs:497-549).
+ *
+ * The filter tables are the caller's DecodeFilterSet (Dsp/DecodeFilterSet.cs:9-82):
+ * full-circle complex128 spectra of block_length bins, interleaved (re, im); SOS
+ * sections as [sections][6] = b0 b1 b2 a0 a1 a2. */
+typedef struct vhsdecode_hip_filters_v1 {
+  uint32_t struct_size;
+  int32_t block_length;            /* N; this build supports 32768 */
+  double sample_rate_hz;           /* RF sample rate */
+  const double* rf_video;          /* DecodeFilterSet.RfVideo (RfMtf is all-ones for tape) */
+  const double* video;             /* DecodeFilterSet.Video */
+  const double* video_lowpass05;   /* DecodeFilterSet.VideoLowPass05 */
+  int32_t video_lowpass05_offset;  /* DecodeFilterSet.VideoLowPass05Offset (32) */
+  const double* envelope_sos;      /* DecodeFilterSet.VhsEnvelopeSos */
+  int32_t envelope_sections;
+  const double* chroma_sos;        /* DecodeFilterSet.ChromaBurstSos, may be NULL */
+  int32_t chroma_sections;
+  int32_t chroma_offset;           /* DecodeFilterSet.ChromaOffsetSamples */
+  double diff_demod_max;           /* DiffDemodRepairOptions.MaxValue; <= 0 disables the repair */
+} vhsdecode_hip_filters_v1;
+
+typedef struct vhsdecode_hip_demod_config_v1 {
+  uint32_t struct_size;
+  int32_t device_id;
+  int32_t block_cut;               /* head cut (1024) */
+  int32_t block_cut_end;           /* tail cut (1024) */
+  int32_t max_blocks;              /* largest run of blocks one call may submit */
+  const vhsdecode_hip_filters_v1* filters;
+} vhsdecode_hip_demod_config_v1;
+
+/* Output planes of a stitched span: nblocks * stride samples each
+ * (stride = block_length - block_cut - block_cut_end).  chroma may be NULL. */
+typedef struct vhsdecode_hip_span_v1 {
+  uint32_t struct_size;
+  float* video;
+  float* video_lowpass;            /* sync reference, rolled by -video_lowpass05_offset */
+  float* envelope;
+  float* chroma;
+  uint32_t* block_flags;           /* [nblocks]: bit0 weak RF (an envelope sample == 0), bit1 spike repaired */
+} vhsdecode_hip_span_v1;
+
+VHSDECODE_HIP_API int32_t vhsdecode_hip_demod_create(const vhsdecode_hip_demod_config_v1* config,
+                                                     vhsdecode_hip_demod_context** context);
+VHSDECODE_HIP_API void vhsdecode_hip_demod_destroy(vhsdecode_hip_demod_context* context);
+
+/* Block b (0 <= b < nblocks) reads RF samples [first_sample + b*stride, + block_length)
+ * of `rf` and writes stitched outputs [b*stride, (b+1)*stride).
+ * Device variant: rf and every span pointer are device memory; asynchronous on `stream`. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_demod_span_device(vhsdecode_hip_demod_context* context, const int16_t* rf,
+                                                          int64_t first_sample, int32_t nblocks,
+                                                          const vhsdecode_hip_span_v1* out, void* stream);
+/* Host variant: rf holds rf_count samples in host memory; outputs are host memory. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_demod_span_host(vhsdecode_hip_demod_context* context, const int16_t* rf,
+                                                        int64_t rf_count, int64_t first_sample, int32_t nblocks,
+                                                        const vhsdecode_hip_span_v1* out);
+
+/* ---------------------------------------------------------------- sync pulses
+ * PulseDetection.FindPulses (src/VHSDecode.Core/Dsp/PulseDetection.cs:45-222) with the
+ * VHS bounds of TbcFieldDecodePipeline.FindRawPulses (Decode/TbcFieldDecodePipeline.cs:2375-2384).
+ * sync_ref is device memory; starts/lengths/count are host memory (synchronous).
+ * *count receives the number of pulses found; only min(count, capacity) are stored. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_find_pulses(vhsdecode_hip_demod_context* context, const float* sync_ref,
+                                                    int64_t length, double threshold, int32_t min_length,
+                                                    int32_t max_length, int32_t* starts, int32_t* lengths,
+                                                    int32_t capacity, int32_t* count);
+
+/* ---------------------------------------------------------------- hsync refinement
+ * TbcFieldDecodePipeline.RefineLineLocationsFromHSync, vhs decode type
+ * (Decode/TbcFieldDecodePipeline.cs:4570-4684 with :4807-5052), FP64 on the device. */
+typedef struct vhsdecode_hip_hsync_params_v1 {
+  uint32_t struct_size;
+  double threshold;                /* detected sync threshold (Hz) */
+  double sample_rate_mhz;          /* SyncAnalyzer.SampleRateMHz */
+  double hsync_pulse_us;           /* SyncAnalyzer.HSyncPulseUs */
+  double ire0;                     /* VideoOutputConverter.Ire0 */
+  double hz_ire;                   /* VideoOutputConverter.HzIre */
+  int32_t is_pal;                  /* PAL lines 1-2 are vsync lines too */
+  int32_t use_right_hsync;         /* HSyncRefineOptions.UseRightHSync */
+} vhsdecode_hip_hsync_params_v1;
+
+/* sync_ref: device; locations/filled (in) and refined/filled_out (out): host. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_refine_hsync(vhsdecode_hip_demod_context* context, const float* sync_ref,
+                                                     int64_t length, const double* locations, const uint8_t* filled,
+                                                     int32_t lines, const vhsdecode_hip_hsync_params_v1* params,
+                                                     double* refined, uint8_t* filled_out);
+
+/* ---------------------------------------------------------------- TBC line resampling
+ * TbcLineResampler (Tbc/TbcLineResampler.cs:154-162, 732-878, 997-1011, 1167-1234,
+ * 1495-1525) with the linear line-location interpolator and
+ * VideoOutputConverter.FastMathConversion (Tbc/VideoOutputConverter.cs:74-90). */
+typedef struct vhsdecode_hip_tbc_params_v1 {
+  uint32_t struct_size;
+  int32_t output_line_length;      /* TbcFrameSpec output line length (PAL 1135, NTSC 910) */
+  int32_t first_line;
+  int32_t line_count;
+  double nominal_line_length;      /* <= 0: median of the line lengths, as the reference */
+  double wow_level_smoothing;      /* WowLevelAdjustSmoothing (0 disables) */
+  int32_t convert;                 /* write uint16 output through the converter */
+  double ire0;
+  double hz_ire;
+  int32_t output_zero;
+  double vsync_ire;
+  double output_scale;
+} vhsdecode_hip_tbc_params_v1;
+
+/* Install the 65537 x 16 FP32 windowed-sinc table (the reference's sinc_lut.npz
+ * payload).  Without it the library builds the Kaiser(beta=5) fallback of
+ * TbcLineResampler.BuildKaiserSincLookup (Tbc/TbcLineResampler.cs:1297-1330). */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_set_sinc_table(vhsdecode_hip_demod_context* context, const float* table,
+                                                       int64_t count);
+
+/* source: device (span plane); locations: host; out_u16 / out_f64: host, either may be NULL. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_tbc_resample(vhsdecode_hip_demod_context* context, const float* source,
+                                                     int64_t source_length, const double* locations,
+                                                     int32_t location_count, const vhsdecode_hip_tbc_params_v1* params,
+                                                     uint16_t* out_u16, double* out_f64);
+
+/* Device variant: every pointer is device memory, asynchronous on `stream`.
+ * level_adjust holds one FP64 factor per output line (or per output sample when
+ * per_sample != 0), indexed from line 0. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_tbc_resample_device(vhsdecode_hip_demod_context* context, const float* source,
+                                                            int64_t source_length, const double* locations,
+                                                            const double* level_adjust, int32_t per_sample,
+                                                            const vhsdecode_hip_tbc_params_v1* params,
+                                                            uint16_t* out_u16, double* out_f64, void* stream);
+
+/* Diagnostic: forward 16384-point complex FFTs of `count` transforms (device,
+ * interleaved FP32) with the library's workgroup FFT.  No reference counterpart;
+ * exposed so the transform can be checked in isolation. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_debug_fft16k(vhsdecode_hip_demod_context* context, float* data,
+                                                     int32_t count, int32_t inverse, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VHSDECODE_HIP_H */

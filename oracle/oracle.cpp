@@ -1,0 +1,719 @@
+// Oracle: CPU FP64 restatement of the reference exact backend for the per-block
+// RF decode chain.  TEST INFRASTRUCTURE ONLY: it is linked by tests/, by
+// __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker and
+// the timed CPU baseline.  The shipped library never links or calls it.
+//
+// Each function names the reference file:line it restates (paths relative to the
+// reference checkout).  Compile with -ffp-contract=off: the reference's FP32
+// steps (atan2 polynomial, sosfiltfilt, sinc taps) are sequences of separately
+// rounded operations, and explicit std::fma is used exactly where the reference
+// calls Math.FusedMultiplyAdd / MathF.FusedMultiplyAdd.
+//
+// Pinning (see tests/test_oracle.py):
+//  - fft: against numpy.fft (pocketfft, the library the reference ports).
+//  - demod_block: SHA-256 of the float32 channels of the reference's tape-family
+//    fixture (tests/VHSDecode.Tests/Fixtures/tape-family-v0.4.0.json) and
+//    closed-form tones.
+//  - find_pulses: explicit vectors of sync_pulses_test.cu and a brute-force scan.
+//  - tbc: the three SHA-256 KATs of TbcParallelResamplerTests.cs:76-183.
+//  - sosfiltfilt_f32: scipy.signal.sosfiltfilt in float64 (tolerance) and the
+//    fixture hashes above (bit-exact).
+//  - hsync_refine: generator ground truth (closed-form edges) only; the branch
+//    semantics on irregular signals are parity unpinned.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace {
+
+using cd = std::complex<double>;
+
+// ---------------------------------------------------------------- FFT (FP64)
+// Iterative radix-2 decimation-in-time with twiddles from long-double sin/cos
+// (correctly rounded to within 1 ulp).  Any FP64 FFT within ~1e-15 relative is an
+// acceptable stand-in for the reference's pocketfft port (PocketFftReal.cs:14-24).
+struct Plan {
+  int n = 0;
+  int log2n = 0;
+  std::vector<int> rev;
+  std::vector<cd> tw;  // tw[k] = exp(-2 pi i k / n), k < n/2
+};
+
+std::mutex g_plan_mutex;
+std::map<int, Plan*> g_plans;
+
+const Plan* get_plan(int n) {
+  std::lock_guard<std::mutex> lock(g_plan_mutex);
+  auto it = g_plans.find(n);
+  if (it != g_plans.end()) return it->second;
+  Plan* p = new Plan();
+  p->n = n;
+  while ((1 << p->log2n) < n) p->log2n++;
+  p->rev.resize(n);
+  for (int i = 0; i < n; i++) {
+    int r = 0;
+    for (int b = 0; b < p->log2n; b++) r |= ((i >> b) & 1) << (p->log2n - 1 - b);
+    p->rev[i] = r;
+  }
+  p->tw.resize(n / 2 > 0 ? n / 2 : 1);
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (int k = 0; k < n / 2; k++) {
+    long double a = two_pi * (long double)k / (long double)n;
+    p->tw[k] = cd((double)cosl(a), (double)-sinl(a));
+  }
+  g_plans[n] = p;
+  return p;
+}
+
+// In-place complex FFT; forward (sign -1) unnormalised, inverse divides by n.
+void fft(cd* x, int n, bool inverse) {
+  if (n <= 1) return;
+  const Plan* p = get_plan(n);
+  for (int i = 0; i < n; i++) {
+    int r = p->rev[i];
+    if (r > i) std::swap(x[i], x[r]);
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    int half = len >> 1;
+    int step = n / len;
+    for (int i = 0; i < n; i += len) {
+      for (int j = 0; j < half; j++) {
+        cd w = p->tw[j * step];
+        if (inverse) w = std::conj(w);
+        cd u = x[i + j];
+        cd v = x[i + j + half] * w;
+        x[i + j] = u + v;
+        x[i + j + half] = u - v;
+      }
+    }
+  }
+  if (inverse) {
+    double s = 1.0 / n;
+    for (int i = 0; i < n; i++) x[i] *= s;
+  }
+}
+
+// numpy.fft.rfft: half spectrum (n/2+1 bins) of a real signal.
+void rfft(const double* in, int n, cd* half) {
+  std::vector<cd> buf(n);
+  for (int i = 0; i < n; i++) buf[i] = cd(in[i], 0.0);
+  fft(buf.data(), n, false);
+  for (int k = 0; k <= n / 2; k++) half[k] = buf[k];
+}
+
+// numpy.fft.irfft: ignores the imaginary parts of the DC and Nyquist bins.
+void irfft(const cd* half, int n, double* out) {
+  std::vector<cd> buf(n);
+  buf[0] = cd(half[0].real(), 0.0);
+  buf[n / 2] = cd(half[n / 2].real(), 0.0);
+  for (int k = 1; k < n / 2; k++) {
+    buf[k] = half[k];
+    buf[n - k] = std::conj(half[k]);
+  }
+  fft(buf.data(), n, true);
+  for (int i = 0; i < n; i++) out[i] = buf[i].real();
+}
+
+// NumpyComplexMultiply.ApplyScalar (NumpyComplexMultiply.cs:300-311), FMA form.
+inline cd cmul_fma(cd a, cd b) {
+  return cd(std::fma(a.real(), b.real(), -(a.imag() * b.imag())),
+            std::fma(a.real(), b.imag(), a.imag() * b.real()));
+}
+
+// ---------------------------------------------------------------- FM discriminator
+// PortedMath.VhsRustAtan2Approximation (PortedMath.cs:519-544), FP32.
+float vhs_atan2(float y, float x) {
+  const float min_normal = 1.17549435E-38f;
+  x += std::copysign(min_normal, x);
+  bool swap = std::fabs(x) < std::fabs(y);
+  float input = swap ? x / y : y / x;
+  float sq = input * input;
+  float r = input * (0.99997726f +
+                     (sq * (-0.33262347f +
+                            (sq * (0.19354346f +
+                                   (sq * (-0.11643287f + (sq * (0.05265332f + (sq * -0.01172120f))))))))));
+  const float pi = 3.14159265358979323846f;
+  float half_pi = input >= 0.0f ? pi / 2.0f : -pi / 2.0f;
+  r = swap ? half_pi - r : r;
+  bool xp = x >= 0.0f, yp = y >= 0.0f;
+  if (xp && yp) return r;
+  if (!xp && yp) return pi + r;
+  if (!xp && !yp) return -pi + r;
+  return r;
+}
+
+// PortedMath.VhsRustFrequencyDifference (PortedMath.cs:546-551).
+float vhs_freq_diff(float cur, float prev, float freq) {
+  const float tau = 6.28318530717958647692f;
+  float d = cur - prev;
+  d -= std::floor(d / tau) * tau;
+  return (d * freq) / tau;
+}
+
+// PortedMath.UnwrapHilbertVhsRustApproximation, complex input (PortedMath.cs:336-390).
+void vhs_unwrap(const cd* a, int n, double fs, double* out) {
+  float freq = (float)fs;
+  float prev = vhs_atan2((float)a[0].imag(), (float)a[0].real());
+  out[0] = 0.0;
+  for (int i = 1; i < n; i++) {
+    float cur = vhs_atan2((float)a[i].imag(), (float)a[i].real());
+    out[i] = (double)vhs_freq_diff(cur, prev, freq);
+    prev = cur;
+  }
+}
+
+// RfDemodulator.ReplaceSpikes (RfDemodulator.cs:966-1001).
+void replace_spikes(double* demod, const double* diffed, int n, double max_value) {
+  std::vector<int> to_fix;
+  for (int i = 0; i < n; i++)
+    if (demod[i] > max_value) to_fix.push_back(i);
+  for (int i : to_fix) {
+    int start = std::max(i - 8, 0);
+    int end = std::min(i + 30, n - 1);
+    if (start >= end) continue;
+    double md = -INFINITY, mo = -INFINITY;
+    for (int k = start; k < end; k++) {
+      md = std::max(md, diffed[k]);
+      mo = std::max(mo, demod[k]);
+    }
+    if (md < mo) std::memcpy(demod + start, diffed + start, sizeof(double) * (end - start));
+  }
+}
+
+// ---------------------------------------------------------------- FP32 sosfiltfilt
+struct Sos32 {
+  float b0, b1, b2, a0, a1, a2;
+};
+
+// SosFilter.DefaultPadLength (SosFilter.cs:540-545).
+int default_pad_length(const double* sos, int sections) {
+  int zn = 0, zd = 0;
+  for (int s = 0; s < sections; s++) {
+    if (sos[s * 6 + 2] == 0.0) zn++;
+    if (sos[s * 6 + 5] == 0.0) zd++;
+  }
+  return 3 * ((2 * sections) + 1 - std::min(zn, zd));
+}
+
+// SosFilter.ApplyForwardBackwardFloat32{Core,InPlace} (SosFilter.cs:372-521, 584-833, 924-966, 1034-1063).
+void sosfiltfilt_f32(const double* sos, int sections, const double* input, int n, int pad, float* output) {
+  std::vector<Sos32> s(sections);
+  for (int i = 0; i < sections; i++)
+    s[i] = {(float)sos[i * 6 + 0], (float)sos[i * 6 + 1], (float)sos[i * 6 + 2],
+            (float)sos[i * 6 + 3], (float)sos[i * 6 + 4], (float)sos[i * 6 + 5]};
+  int m = n + 2 * pad;
+  std::vector<float> ext(m);
+  if (pad == 0) {
+    for (int i = 0; i < n; i++) ext[i] = (float)input[i];
+  } else {
+    float first = (float)input[0];
+    for (int i = 0; i < pad; i++) ext[i] = (2.0f * first) - (float)input[pad - i];
+    for (int i = 0; i < n; i++) ext[pad + i] = (float)input[i];
+    float last = (float)input[n - 1];
+    for (int i = 0; i < pad; i++) ext[pad + n + i] = (2.0f * last) - (float)input[n - 2 - i];
+  }
+  // steady-state initial conditions
+  std::vector<float> zi(2 * sections), sz(2 * sections);
+  float scale = 1.0f;
+  for (int i = 0; i < sections; i++) {
+    const Sos32& q = s[i];
+    float t1 = q.b1 - (q.a1 * q.b0);
+    float t2 = q.b2 - (q.a2 * q.b0);
+    float num = (0.0f + t1) + t2;
+    float den = (1.0f + q.a1) + q.a2;
+    float z0 = num / den;
+    float z1 = ((1.0f + q.a1) * z0) - t1;
+    zi[2 * i] = scale * z0;
+    zi[2 * i + 1] = scale * z1;
+    float ndc = ((0.0f + q.b0) + q.b1) + q.b2;
+    float ddc = ((0.0f + q.a0) + q.a1) + q.a2;
+    scale *= ndc / ddc;
+  }
+  auto run = [&](bool backward) {
+    float x0 = backward ? ext[m - 1] : ext[0];
+    for (int i = 0; i < 2 * sections; i++) sz[i] = zi[i] * x0;
+    for (int step = 0; step < m; step++) {
+      int idx = backward ? (m - 1 - step) : step;
+      float v = ext[idx];
+      for (int k = 0; k < sections; k++) {
+        const Sos32& q = s[k];
+        float f = (q.b0 * v) + sz[2 * k];
+        sz[2 * k] = ((q.b1 * v) - (q.a1 * f)) + sz[2 * k + 1];
+        sz[2 * k + 1] = (q.b2 * v) - (q.a2 * f);
+        v = f;
+      }
+      ext[idx] = v;
+    }
+  };
+  run(false);
+  run(true);
+  for (int i = 0; i < n; i++) output[i] = ext[pad + i];
+}
+
+// numpy.roll semantics (FrequencyDomainFilter.RollInPlaceCore, FrequencyDomainFilter.cs:145-161).
+template <typename T>
+void roll(T* v, int n, int shift) {
+  int k = ((shift % n) + n) % n;
+  if (k == 0) return;
+  std::rotate(v, v + (n - k), v + n);
+}
+
+// VhsChromaDecoder.MeanFloat32FastMath(float) (VhsChromaDecoder.cs:3751-3790).
+float mean_f32_fastmath(const float* v, int n) {
+  float acc[32] = {0};
+  for (int block = 0; block < n; block += 32)
+    for (int g = 0; g < 4; g++)
+      for (int lane = 0; lane < 8; lane++) {
+        int idx = block + g * 8 + lane;
+        if (idx < n) acc[g * 8 + lane] += v[idx];
+      }
+  float lanes[8];
+  for (int lane = 0; lane < 8; lane++) {
+    float left = acc[lane] + acc[8 + lane];
+    float right = acc[16 + lane] + acc[24 + lane];
+    lanes[lane] = left + right;
+  }
+  for (int count = 8; count > 1; count /= 2)
+    for (int lane = 0; lane < count / 2; lane++) lanes[lane] += lanes[lane + count / 2];
+  return lanes[0] / (float)n;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_abi_version() { return 1; }
+
+// Complex FFT over interleaved (re, im) doubles.
+void oracle_fft(double* data, int n, int inverse) { fft(reinterpret_cast<cd*>(data), n, inverse != 0); }
+
+float oracle_vhs_atan2(float y, float x) { return vhs_atan2(y, x); }
+
+void oracle_vhs_unwrap(const double* analytic_interleaved, int n, double fs, double* out) {
+  vhs_unwrap(reinterpret_cast<const cd*>(analytic_interleaved), n, fs, out);
+}
+
+void oracle_sosfiltfilt_f32(const double* sos, int sections, const double* input, int n, int pad, float* out) {
+  if (pad < 0) pad = default_pad_length(sos, sections);
+  sosfiltfilt_f32(sos, sections, input, n, pad, out);
+}
+
+// RfDemodulator.DemodulateCore, VHS real-FFT path with the numpy complex analytic
+// signal used by the exact backend (RfDemodulator.cs:288-918; the branch at
+// :395-562 with useNumpyComplexVhsAnalytic, then :621-646, :748-767, :796-821,
+// :858-885) followed by RfBlockDecodePipeline.DecodeChromaBurstFloat32
+// (RfBlockDecodePipeline.cs:539-573) and ShiftChromaAndRemoveDcFloat32InPlace
+// (VhsChromaDecoder.cs:2517-2533).
+//
+// Filters are complex128[n] interleaved.  Outputs are n doubles (envelope holds
+// float32 values), chroma n floats; demod_out (optional) receives DemodRaw after
+// spike repair.  Returns 1 when the weak-RF flag is set, 0 otherwise.
+int oracle_demod_block(const double* input, int n, double fs, const double* rf_video_c,
+                       const double* video_c, const double* vlp05_c, int vlp_offset,
+                       const double* env_sos, int env_sections, const double* chroma_sos,
+                       int chroma_sections, int chroma_offset, double diff_demod_max,
+                       double* video_out, double* vlp_out, double* env_out, float* chroma_out,
+                       double* demod_out) {
+  const cd* rf_video = reinterpret_cast<const cd*>(rf_video_c);
+  const cd* video_f = reinterpret_cast<const cd*>(video_c);
+  const cd* vlp05 = reinterpret_cast<const cd*>(vlp05_c);
+  const int half = n / 2 + 1;
+  std::vector<cd> spec(half), filt(half);
+  rfft(input, n, spec.data());
+  for (int k = 0; k < half; k++) filt[k] = cmul_fma(spec[k], rf_video[k]);  // RfMtf is all-ones for tape
+  std::vector<double> rf_real(n);
+  irfft(filt.data(), n, rf_real.data());
+
+  // numpy complex analytic: fft(input) * rfVideo * rfMtf * hilbert -> ifft
+  std::vector<cd> analytic(n);
+  for (int i = 0; i < n; i++) analytic[i] = cd(input[i], 0.0);
+  fft(analytic.data(), n, false);
+  for (int k = 0; k < n; k++) {
+    cd v = cmul_fma(analytic[k], rf_video[k]);
+    v = cmul_fma(v, cd(1.0, 0.0));
+    double h = (k == 0 || k == n / 2) ? 1.0 : (k < n / 2 ? 2.0 : 0.0);
+    analytic[k] = cd(v.real() * h, v.imag() * h);
+  }
+  fft(analytic.data(), n, true);
+
+  // envelope: |float(x)| rolled by +4, FP32 sosfiltfilt (RfDemodulator.cs:1755-1839)
+  std::vector<double> raw(n);
+  int shift = 4 % n;
+  int split = n - shift;
+  for (int i = 0; i < split; i++) raw[i + shift] = std::fabs((float)rf_real[i]);
+  for (int w = split; w < n; w++) raw[w - split] = std::fabs((float)rf_real[w]);
+  std::vector<float> env(n);
+  sosfiltfilt_f32(env_sos, env_sections, raw.data(), n, default_pad_length(env_sos, env_sections), env.data());
+  int weak = 0;
+  for (int i = 0; i < n; i++) {
+    env_out[i] = (double)env[i];
+    if (env[i] == 0.0f) weak = 1;
+  }
+
+  std::vector<double> demod(n);
+  vhs_unwrap(analytic.data(), n, fs, demod.data());
+  if (diff_demod_max > 0.0 && n > 40) {
+    bool spike = false;
+    for (int i = 20; i < n - 20; i++)
+      if (demod[i] > diff_demod_max) {
+        spike = true;
+        break;
+      }
+    if (spike) {
+      std::vector<cd> diffed(n);
+      for (int i = n - 1; i >= 1; i--) diffed[i] = analytic[i] - analytic[i - 1];
+      diffed[0] = cd(0.0, 0.0);
+      std::vector<double> dd(n);
+      vhs_unwrap(diffed.data(), n, fs, dd.data());
+      replace_spikes(demod.data(), dd.data(), n, diff_demod_max);
+    }
+  }
+  if (demod_out) std::memcpy(demod_out, demod.data(), sizeof(double) * n);
+
+  std::vector<cd> dspec(half), vs(half);
+  rfft(demod.data(), n, dspec.data());
+  for (int k = 0; k < half; k++) vs[k] = cmul_fma(dspec[k], video_f[k]);
+  irfft(vs.data(), n, video_out);
+  for (int k = 0; k < half; k++) vs[k] = cmul_fma(dspec[k], vlp05[k]);
+  irfft(vs.data(), n, vlp_out);
+  if (vlp_offset != 0) roll(vlp_out, n, -vlp_offset);
+
+  if (chroma_out && chroma_sos && chroma_sections > 0) {
+    sosfiltfilt_f32(chroma_sos, chroma_sections, input, n, default_pad_length(chroma_sos, chroma_sections),
+                    chroma_out);
+    roll(chroma_out, n, chroma_offset);
+    float mean = mean_f32_fastmath(chroma_out, n);
+    for (int i = 0; i < n; i++) chroma_out[i] = (float)(chroma_out[i] - mean);
+  }
+  return weak;
+}
+
+// PulseDetection.FindPulsesScalar (PulseDetection.cs:195-222).  Writes up to cap
+// pulses; returns the total number found (may exceed cap).
+int oracle_find_pulses(const double* x, int n, double high, int min_len, int max_len, int* starts, int* lens,
+                       int cap) {
+  if (n <= 0) return 0;
+  bool in_pulse = x[0] <= high;
+  int current = 0;
+  int count = 0;
+  for (int p = 0; p < n; p++) {
+    double v = x[p];
+    if (in_pulse) {
+      if (v > high) {
+        int len = p - current;
+        if (len >= min_len && len <= max_len && current != 0) {
+          if (count < cap) {
+            starts[count] = current;
+            lens[count] = len;
+          }
+          count++;
+        }
+        in_pulse = false;
+      }
+    } else if (v <= high) {
+      current = p;
+      in_pulse = true;
+    }
+  }
+  return count;
+}
+
+// PulseDetection.CalculateZeroCrossing (PulseDetection.cs:17-43, 224-274).
+// Returns 1 and writes *result when a crossing is found.
+int oracle_zero_crossing(const double* data, int n, int start, double target, int edge, int count, int reverse,
+                         double* result) {
+  if (start < 0 || start >= n) return 0;
+  auto forward = [&](const double* d, int len, int s, double* out) -> int {
+    int e = edge;
+    if (e == 0) e = d[s] < target ? 1 : -1;
+    int end = std::min(len, s + count + 1);
+    int loc = -1;
+    for (int i = 1; i < end - s; i++) {
+      double a = d[s + i - 1], b = d[s + i];
+      if (e == 1 ? (a < target && b >= target) : (a > target && b <= target)) {
+        loc = i;
+        break;
+      }
+    }
+    if (loc < 0) return 0;
+    int xi = s + loc;
+    double a = d[xi - 1] - target;
+    double b = d[xi] - target;
+    double y = (b - a) != 0.0 ? -a / (-a + b) : 0.0;
+    *out = xi - 1 + y;
+    return 1;
+  };
+  if (reverse) {
+    std::vector<double> rev(start + 1);
+    for (int i = 0; i <= start; i++) rev[i] = data[start - i];
+    double r;
+    if (!forward(rev.data(), start + 1, 0, &r)) return 0;
+    *result = start - r;
+    return 1;
+  }
+  return forward(data, n, start, result);
+}
+
+// ---------------------------------------------------------------- hsync refine
+// TbcFieldDecodePipeline.RefineLineLocationsFromHSync for vhs (TbcFieldDecodePipeline.cs:4570-4684)
+// with TryRefineLeftHSync (:4870-4948), TryRefineFromRightHSync (:4807-4868),
+// TryConfirmRightHSync (:4950-4987), HSyncAreaLooksValid (:4995-5027), TryMeanRange (:5029-).
+struct Conv {
+  double ire0, hz_ire;
+  double ire_to_hz(double ire) const { return ire0 + hz_ire * ire; }
+};
+
+static inline int round_away(double v) { return (int)std::round(v); }
+
+static bool hsync_area_valid(const double* s, int n, double crossing, int one_us, double end_us, double min_ire,
+                             double max_ire, const Conv& c) {
+  int start = std::clamp(round_away(crossing - 0.75 * one_us), 0, n);
+  int end = std::clamp(round_away(crossing + end_us * one_us), start, n);
+  if (end <= start) return false;
+  double mn = INFINITY, mx = -INFINITY;
+  for (int i = start; i < end; i++) {
+    mn = std::min(mn, s[i]);
+    mx = std::max(mx, s[i]);
+  }
+  return mn >= c.ire_to_hz(min_ire) && mx <= c.ire_to_hz(max_ire);
+}
+
+static bool mean_range(const double* s, int n, double start, double end, double* mean) {
+  int a = std::clamp(round_away(start), 0, n);
+  int b = std::clamp(round_away(end), a, n);
+  if (b <= a) {
+    *mean = 0.0;
+    return false;
+  }
+  double sum = 0.0;
+  for (int i = a; i < b; i++) sum += s[i];
+  *mean = sum / (b - a);
+  return true;
+}
+
+int oracle_refine_hsync(const double* s, int n, const double* locations, const uint8_t* filled_in, int lines,
+                        double threshold, double fs_mhz, double hsync_us, double ire0, double hz_ire, int is_pal,
+                        int use_right, double* refined_out, uint8_t* filled_out) {
+  Conv c{ire0, hz_ire};
+  int one_us = std::max(1, (int)fs_mhz);
+  int normal_len = std::max(1, (int)(hsync_us * fs_mhz));
+  int search_count = std::max(1, one_us * 2);
+  double prev_porch = -1.0;
+  for (int line = 0; line < lines; line++) {
+    refined_out[line] = locations[line];
+    filled_out[line] = filled_in[line];
+  }
+  for (int line = 0; line < lines; line++) {
+    bool vsync_line = (line >= 3 && line <= 6) || (is_pal && line >= 1 && line <= 2);
+    if (vsync_line) {
+      filled_out[line] = 1;
+      continue;
+    }
+    if (!std::isfinite(refined_out[line])) continue;
+    int search_start = round_away(refined_out[line] - one_us);
+    if (search_start < 0 || search_start >= n - 1) continue;
+    double crossing;
+    int has = oracle_zero_crossing(s, n, search_start, threshold, 0, std::min(search_count, n - search_start - 1), 0,
+                                   &crossing);
+    double original = refined_out[line];
+    double right_ref = original;
+    if (has && !filled_out[line]) {
+      // TryRefineLeftHSync
+      double refined_loc = crossing, porch = 0.0;
+      bool preserve = false, left_ok = false;
+      double sync_level;
+      if (hsync_area_valid(s, n, crossing, one_us, 3.5, -65.0, 110.0, c) &&
+          mean_range(s, n, crossing + one_us, crossing + 2.5 * one_us, &sync_level)) {
+        preserve = true;
+        bool porch_ok = true;
+        if (prev_porch > 0.0) {
+          porch = prev_porch;
+        } else if (!mean_range(s, n, crossing - one_us, crossing - 0.5 * one_us, &porch)) {
+          porch_ok = false;
+        }
+        if (porch_ok) {
+          double mid = (porch + sync_level) / 2.0;
+          double rc;
+          int ok = oracle_zero_crossing(s, n, search_start, mid, 0, std::min(400, n - search_start - 1), 0, &rc);
+          bool good = ok && std::fabs(rc - crossing) < one_us / 2.0;
+          if (!good) {
+            if (prev_porch > 0.0) {
+              ok = oracle_zero_crossing(s, n, search_start, (prev_porch + sync_level) / 2.0, 0,
+                                        std::min(400, n - search_start - 1), 0, &rc);
+              good = ok && std::fabs(rc - crossing) < one_us / 2.0;
+              if (good) porch = prev_porch;
+            }
+          }
+          if (good) {
+            refined_loc = rc;
+            left_ok = true;
+          }
+        }
+      }
+      if (left_ok) {
+        refined_out[line] = refined_loc;
+        right_ref = refined_loc;
+        filled_out[line] = filled_in[line];
+        if (porch > 0.0) prev_porch = porch;
+      } else {
+        filled_out[line] = 1;
+        refined_out[line] = original;
+        if (preserve) right_ref = refined_loc;
+      }
+    } else {
+      filled_out[line] = 1;
+      refined_out[line] = original;
+    }
+    if (use_right) {
+      int rs = search_start + normal_len - one_us;
+      if (rs < 0 || rs >= n - 1) continue;
+      if (s[rs] > threshold) continue;
+      int rcount = std::max(1, normal_len * 2);
+      double rcross;
+      if (!oracle_zero_crossing(s, n, rs, threshold, 1, std::min(rcount, n - rs - 1), 0, &rcross)) continue;
+      // TryConfirmRightHSync
+      double rds = rcross - normal_len;
+      double sync_level, rporch;
+      if (!hsync_area_valid(s, n, rds, one_us, 8.0, -65.0, 30.0, c) ||
+          !mean_range(s, n, rds + one_us, rds + 2.5 * one_us, &sync_level) ||
+          !mean_range(s, n, rds + normal_len + one_us, rds + normal_len + 2.0 * one_us, &rporch))
+        continue;
+      double mid = (rporch + sync_level) / 2.0;
+      double rrc;
+      if (!oracle_zero_crossing(s, n, rs, mid, 0, std::min(400, n - rs - 1), 0, &rrc)) continue;
+      if (!(std::fabs(rrc - rcross) < one_us / 2.0)) continue;
+      double sr = (double)(float)fs_mhz;
+      double cand = rcross - normal_len + (2.25 * (sr / 40.0));
+      if (std::fabs(cand - right_ref) >= one_us * 2.0) continue;
+      refined_out[line] = cand;
+      filled_out[line] = 0;
+      if (rporch > 0.0) prev_porch = rporch;
+    }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------- TBC
+static double median(std::vector<double> v) {
+  for (double x : v)
+    if (std::isnan(x)) return x;
+  std::sort(v.begin(), v.end());
+  size_t m = v.size() / 2;
+  return (v.size() & 1) ? v[m] : (v[m - 1] + v[m]) / 2.0;
+}
+
+// TbcLineResampler.SampleSinc (TbcLineResampler.cs:1167-1234); LUT is float[65537*16].
+static double sample_sinc(const double* src, int len, double position, const float* lut) {
+  if (!std::isfinite(position)) return 0.0;
+  const int taps = 16, phases = 65536;
+  float coord = (float)position;
+  int ci = (int)coord;
+  float frac = coord - (float)ci;
+  float pp = frac * (float)phases;
+  int ps = (int)pp;
+  int pe = ps + 1;
+  float alpha = pp - (float)ps;
+  auto wrap = [](int idx, int length) {
+    if (idx < 0) idx += length;
+    return std::clamp(idx, 0, length - 1);
+  };
+  int ws = wrap(ps, phases + 1) * taps;
+  int we = wrap(pe, phases + 1) * taps;
+  int s0 = ci - (taps / 2 - 1);
+  double result = 0.0;
+  bool interior = len >= taps && (unsigned)s0 <= (unsigned)(len - taps);
+  for (int t = 0; t < taps; t++) {
+    float sw = lut[ws + t];
+    float w = std::fma(alpha, lut[we + t] - sw, sw);
+    int si = s0 + t;
+    if (!interior) {
+      if (si < 0) si += len;
+      si = std::clamp(si, 0, len - 1);
+    }
+    float prod = (float)src[si] * w;
+    result = (double)prod + result;
+  }
+  return result;
+}
+
+// TbcLineResampler.PrepareResampling (linear) + ResampleSamples (TbcLineResampler.cs:520-554, 732-878,
+// 997-1011, 1495-1530).  nominal <= 0 uses the median line length.  Writes line_count*out_len doubles
+// (float32 values) into out, and when conv_ire0 is finite also the uint16 conversion
+// (VideoOutputConverter.FastMathConversion.Convert, VideoOutputConverter.cs:74-90) into out_u16.
+int oracle_tbc_resample(const double* src, int src_len, const double* locations, int n_loc, int first_line,
+                        int line_count, int out_len, double nominal, double smoothing, const float* lut,
+                        double* out, int has_conv, double ire0, double hz_ire, int output_zero, double vsync_ire,
+                        double output_scale, uint16_t* out_u16) {
+  if (n_loc < 2 || first_line < 0 || first_line + line_count >= n_loc) return -1;
+  if (nominal <= 0.0) {
+    std::vector<double> l(n_loc - 1);
+    for (int i = 0; i + 1 < n_loc; i++) l[i] = locations[i + 1] - locations[i];
+    nominal = std::max(1.0, median(l));
+  }
+  int prefix = first_line * out_len;
+  int dest_len = out_len * line_count;
+  int scaled = prefix + dest_len;
+  // source positions (LinearLineLocationInterpolator.FillOutputPositions)
+  std::vector<double> pos(dest_len);
+  double scale = nominal / out_len;
+  for (int d = 0; d < dest_len; d++) {
+    int si = prefix + d;
+    int left = si / out_len;
+    double ll = locations[left];
+    double delta = locations[left + 1] - ll;
+    double sp = si * scale;
+    double frac = std::clamp((sp - (left * nominal)) / nominal, 0.0, 1.0);
+    pos[d] = ll + (delta * frac);
+  }
+  // level adjusts (BuildLinearLevelAdjusts)
+  int lines = scaled / out_len;
+  std::vector<double> adj(scaled, 0.0);
+  if (lines > 0) {
+    std::vector<double> f(lines);
+    double inv = 1.0 / nominal;
+    for (int line = 0; line < lines; line++) {
+      double lp = (double)(line * out_len) / out_len;
+      int left = std::clamp((int)std::floor(lp), 0, n_loc - 2);
+      f[line] = (locations[left + 1] * inv) - (locations[left] * inv);
+    }
+    double med = median(f);
+    std::vector<double> dev(lines);
+    for (int i = 0; i < lines; i++) dev[i] = std::fabs(f[i] - med);
+    double mad = median(dev);
+    double thr = mad > 0.0 ? 15.0 * mad : 0.001;
+    for (int line = 0; line < lines; line++) {
+      double v = std::fabs(f[line] - med) > thr ? med : f[line];
+      for (int s = 0; s < out_len; s++) adj[line * out_len + s] = v;
+    }
+    if (smoothing > 0.0) {
+      double alpha = 1.0 / (smoothing * out_len);
+      for (int i = 1; i < scaled; i++) adj[i] = std::fma(adj[i] - adj[i - 1], alpha, adj[i - 1]);
+    }
+  }
+  double conv_inv = 1.0 / hz_ire;
+  double conv_off = (output_zero + 0.5) - (vsync_ire * output_scale);
+  for (int d = 0; d < dest_len; d++) {
+    double r = (double)(float)(sample_sinc(src, src_len, pos[d], lut) * adj[prefix + d]);
+    out[d] = r;
+    if (has_conv && out_u16) {
+      double sd = output_scale * (r - ire0);
+      double v = std::fma(sd, conv_inv, conv_off);
+      uint16_t o;
+      if (v <= 0.0)
+        o = 0;
+      else if (v >= 65535.0)
+        o = 65535;
+      else
+        o = (uint16_t)v;
+      out_u16[d] = o;
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

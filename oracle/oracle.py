@@ -1,0 +1,173 @@
+"""ctypes view of the CPU oracle (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, where it is the checker and the CPU baseline.
+The product library never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u16p = ctypes.POINTER(ctypes.c_uint16)
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.oracle_fft.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
+        L.oracle_vhs_atan2.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.oracle_vhs_atan2.restype = ctypes.c_float
+        L.oracle_vhs_unwrap.argtypes = [_dp, ctypes.c_int, ctypes.c_double, _dp]
+        L.oracle_sosfiltfilt_f32.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _fp]
+        L.oracle_demod_block.argtypes = [_dp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, ctypes.c_int,
+                                         _dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         _dp, _dp, _dp, _fp, _dp]
+        L.oracle_demod_block.restype = ctypes.c_int
+        L.oracle_find_pulses.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                         _ip, _ip, ctypes.c_int]
+        L.oracle_find_pulses.restype = ctypes.c_int
+        L.oracle_zero_crossing.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, _dp]
+        L.oracle_zero_crossing.restype = ctypes.c_int
+        L.oracle_refine_hsync.argtypes = [_dp, ctypes.c_int, _dp, _u8p, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_int, ctypes.c_int, _dp, _u8p]
+        L.oracle_tbc_resample.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_double, ctypes.c_double, _fp, _dp,
+                                          ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_double, _u16p]
+        L.oracle_tbc_resample.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def fft(x: np.ndarray, inverse: bool = False) -> np.ndarray:
+    buf = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    lib().oracle_fft(_p(buf.view(np.float64), _dp), buf.size, 1 if inverse else 0)
+    return buf
+
+
+def vhs_atan2(y: float, x: float) -> float:
+    return lib().oracle_vhs_atan2(y, x)
+
+
+def vhs_unwrap(analytic: np.ndarray, fs: float) -> np.ndarray:
+    a = np.ascontiguousarray(analytic, dtype=np.complex128)
+    out = np.empty(a.size)
+    lib().oracle_vhs_unwrap(_p(a.view(np.float64), _dp), a.size, fs, _p(out, _dp))
+    return out
+
+
+def sosfiltfilt_f32(sos: np.ndarray, x: np.ndarray, pad: int = -1) -> np.ndarray:
+    s = np.ascontiguousarray(sos, dtype=np.float64)
+    xi = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty(xi.size, dtype=np.float32)
+    lib().oracle_sosfiltfilt_f32(_p(s, _dp), s.shape[0], _p(xi, _dp), xi.size, pad, _p(out, _fp))
+    return out
+
+
+class DemodResult:
+    def __init__(self, video, video_lowpass, envelope, chroma, demod, weak):
+        self.video = video
+        self.video_lowpass = video_lowpass
+        self.envelope = envelope
+        self.chroma = chroma
+        self.demod = demod
+        self.weak = weak
+
+
+def demod_block(x: np.ndarray, filters, fs: Optional[float] = None) -> DemodResult:
+    """One block through the exact VHS chain; `filters` is a TapeFilterSet."""
+    xi = np.ascontiguousarray(x, dtype=np.float64)
+    n = xi.size
+    fs = float(filters.sample_rate_hz if fs is None else fs)
+    rv = np.ascontiguousarray(filters.rf_video, dtype=np.complex128)
+    vf = np.ascontiguousarray(filters.video, dtype=np.complex128)
+    lf = np.ascontiguousarray(filters.video_lowpass05, dtype=np.complex128)
+    env = np.ascontiguousarray(filters.envelope_sos, dtype=np.float64)
+    chs = filters.chroma_sos
+    ch = np.ascontiguousarray(chs if chs is not None else np.zeros((0, 6)), dtype=np.float64)
+    video = np.empty(n)
+    vlp = np.empty(n)
+    envelope = np.empty(n)
+    chroma = np.empty(n, dtype=np.float32)
+    demod = np.empty(n)
+    dmax = filters.diff_demod_max if filters.diff_demod_max is not None else -1.0
+    weak = lib().oracle_demod_block(
+        _p(xi, _dp), n, fs, _p(rv.view(np.float64), _dp), _p(vf.view(np.float64), _dp),
+        _p(lf.view(np.float64), _dp), int(filters.video_lowpass05_offset), _p(env, _dp), env.shape[0],
+        _p(ch, _dp), ch.shape[0], int(filters.chroma_offset), float(dmax), _p(video, _dp), _p(vlp, _dp),
+        _p(envelope, _dp), _p(chroma, _fp), _p(demod, _dp))
+    return DemodResult(video, vlp, envelope, chroma if chs is not None else None, demod, bool(weak))
+
+
+def find_pulses(x: np.ndarray, high: float, min_len: int = 0, max_len: int = 5000):
+    xi = np.ascontiguousarray(x, dtype=np.float64)
+    cap = max(16, xi.size // 2 + 1)
+    starts = np.empty(cap, dtype=np.int32)
+    lens = np.empty(cap, dtype=np.int32)
+    count = lib().oracle_find_pulses(_p(xi, _dp), xi.size, high, min_len, max_len, _p(starts, _ip),
+                                     _p(lens, _ip), cap)
+    return starts[:count].copy(), lens[:count].copy()
+
+
+def zero_crossing(data, start, target, edge=0, count=16, reverse=False):
+    d = np.ascontiguousarray(data, dtype=np.float64)
+    out = ctypes.c_double()
+    ok = lib().oracle_zero_crossing(_p(d, _dp), d.size, start, target, edge, count, 1 if reverse else 0,
+                                    ctypes.byref(out))
+    return out.value if ok else None
+
+
+def refine_hsync(sync_ref, locations, filled, threshold, fs_mhz, hsync_us, ire0, hz_ire, is_pal, use_right=True):
+    s = np.ascontiguousarray(sync_ref, dtype=np.float64)
+    loc = np.ascontiguousarray(locations, dtype=np.float64)
+    fil = np.ascontiguousarray(filled, dtype=np.uint8)
+    ro = np.empty_like(loc)
+    fo = np.empty_like(fil)
+    lib().oracle_refine_hsync(_p(s, _dp), s.size, _p(loc, _dp), _p(fil, _u8p), loc.size, threshold, fs_mhz,
+                              hsync_us, ire0, hz_ire, 1 if is_pal else 0, 1 if use_right else 0,
+                              _p(ro, _dp), _p(fo, _u8p))
+    return ro, fo.astype(bool)
+
+
+def tbc_resample(src, locations, first_line, line_count, out_len, lut, nominal=None, smoothing=0.0, conv=None):
+    """conv = (ire0, hz_ire, output_zero, vsync_ire, output_scale) or None."""
+    s = np.ascontiguousarray(src, dtype=np.float64)
+    loc = np.ascontiguousarray(locations, dtype=np.float64)
+    lt = np.ascontiguousarray(lut, dtype=np.float32).reshape(-1)
+    out = np.empty(out_len * line_count)
+    u16 = np.empty(out_len * line_count, dtype=np.uint16)
+    c = conv if conv is not None else (0.0, 1.0, 0, 0.0, 1.0)
+    rc = lib().oracle_tbc_resample(_p(s, _dp), s.size, _p(loc, _dp), loc.size, first_line, line_count, out_len,
+                                   float(nominal) if nominal else 0.0, float(smoothing), _p(lt, _fp), _p(out, _dp),
+                                   1 if conv is not None else 0, float(c[0]), float(c[1]), int(c[2]), float(c[3]),
+                                   float(c[4]), _p(u16, _u16p))
+    if rc != 0:
+        raise ValueError("invalid line range for TBC resampling")
+    return (out, u16) if conv is not None else out

@@ -1,0 +1,460 @@
+// Per-block RF demodulation kernels for gfx950 (VHS real-FFT path).
+//
+// One 1024-thread workgroup demodulates one 32768-sample RF block entirely in
+// registers + LDS: the int16 block is read once from HBM and only the trimmed
+// output window (samples [cut, cut + stride) of each plane) is written, straight
+// into the stitched field-span layout, so the overlap-save stitch of
+// RfBlockStreamDecoder.CopyTrimmedWindow (Decode/RfBlockStreamDecoder.cs:497-549,
+// 1579-1612) costs nothing extra.
+//
+// Chain (Dsp/RfDemodulator.cs:288-918, VHS real path, exact backend semantics):
+//   rfft(x) -> x RfVideo -> analytic (one-sided spectrum, complex inverse as two
+//   16384-point passes for even and odd output samples) -> FP32 polynomial atan2
+//   FM discriminator (PortedMath.cs:519-551) -> envelope: |float(Re)| rolled +4,
+//   FP32 sosfiltfilt (RfDemodulator.cs:1755-1839) -> rfft(demod) -> x Video and
+//   x VideoLowPass05 -> two inverse real FFTs (video, sync reference rolled -32).
+// The chroma plane (FP32 sosfiltfilt of the input with the colour-under band-pass,
+// RfBlockDecodePipeline.cs:539-573) is a second kernel; the diff-demod spike
+// repair (RfDemodulator.cs:2527-2581, :966-1001) is a third, run only by blocks
+// whose demod exceeded the repair threshold.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft16k.hpp"
+#include "sos_scan.hpp"
+#include "vhsd_args.hpp"
+#include "vhsd_device.hpp"
+
+namespace vhsd {
+
+// LDS layout shared by the block kernels (bytes): 128 KiB exchange buffer,
+// 16 KiB twiddle octant, small scratch.
+struct BlockLds {
+  c32 buf[kFftM];
+  float2 oct[kOctantEntries];
+  float scratch[kScratchFloats];
+};
+
+__device__ __forceinline__ void load_octant(BlockLds& s, const float2* __restrict__ oct_g) {
+  for (int u = threadIdx.x; u < kOctantEntries; u += blockDim.x) s.oct[u] = oct_g[u];
+}
+
+// rfft post-processing: Z = FFT16384(x[2m] + i x[2m+1]) held in strided
+// distribution -> X[k] for k = j + 1024 m (and X[16384] returned via xm on thread 0).
+__device__ __forceinline__ void rfft_post(c32* v, BlockLds& s, c32& xm) {
+  const int j = threadIdx.x;
+#pragma unroll
+  for (int m = 0; m < 16; m++) s.buf[j + m * kFftThreads] = v[m];
+  __syncthreads();
+  c32 z0 = s.buf[0];
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    const c32 zk = v[m];
+    const c32 zc = conjc(s.buf[(kFftM - k) & (kFftM - 1)]);
+    const c32 e = add(zk, zc);
+    const c32 o = sub(zk, zc);
+    // X[k] = 0.5*(e - i*W^k*o)
+    const c32 wo = mul(tw32k(s.oct, k), o);
+    v[m] = scale(add(e, mul_mi(wo)), 0.5f);
+  }
+  xm = c32{z0.x - z0.y, 0.0f};
+  __syncthreads();
+}
+
+// Inverse real FFT pre-processing of a half spectrum V[k] (strided, plus V[M] in vm):
+// Y[k] = (V[k] + conj(V[M-k])) + i * (V[k] - conj(V[M-k])) * exp(+2 pi i k / N).
+// numpy.fft.irfft semantics: imaginary parts of V[0] and V[M] are ignored.
+__device__ __forceinline__ void irfft_pre(c32* v, c32 vm, BlockLds& s) {
+  const int j = threadIdx.x;
+#pragma unroll
+  for (int m = 0; m < 16; m++) s.buf[j + m * kFftThreads] = v[m];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    c32 y;
+    if (k == 0) {
+      const float a = v[m].x, b = vm.x;
+      y = c32{a + b, a - b};
+    } else {
+      const c32 vk = v[m];
+      const c32 vc = conjc(s.buf[kFftM - k]);
+      const c32 e = add(vk, vc);
+      const c32 o = mul(sub(vk, vc), conjc(tw32k(s.oct, k)));
+      y = add(e, mul_pi(o));
+    }
+    v[m] = y;
+  }
+  __syncthreads();
+}
+
+// Stage B+: analytic signal from the filtered half spectrum F (strided, F[M] in fm),
+// FM discriminator and real part.  Writes demod (2 per m) and the real part into
+// dem[32] / re[32] laid out as [2m] = even sample 2*(j+1024m), [2m+1] = odd sample.
+__device__ __forceinline__ void analytic_demod(c32* F, c32 fm, BlockLds& s, float fs, float* dem, float* re,
+                                               float* theta_e) {
+  const int j = threadIdx.x;
+  const float inv_n = 1.0f / 32768.0f;
+  c32 v[16];
+  // even output samples
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    v[m] = (k == 0) ? add(F[m], fm) : scale(F[m], 2.0f);
+  }
+  fft16k_inverse(v, s.buf, s.oct);
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const c32 a = scale(v[m], inv_n);
+    re[2 * m] = a.x;
+    theta_e[m] = vhs_atan2(a.y, a.x);
+  }
+  // odd output samples
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    v[m] = (k == 0) ? sub(F[m], fm) : mul(scale(F[m], 2.0f), conjc(tw32k(s.oct, k)));
+  }
+  fft16k_inverse(v, s.buf, s.oct);
+  float* th_o = reinterpret_cast<float*>(s.buf);
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const c32 a = scale(v[m], inv_n);
+    re[2 * m + 1] = a.x;
+    const float t = vhs_atan2(a.y, a.x);
+    dem[2 * m + 1] = vhs_freq_diff(t, theta_e[m], fs);
+    th_o[j + m * kFftThreads] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    // previous odd sample of even sample 2*(j + 1024 m) is odd index (j-1) + 1024 m
+    float prev;
+    if (j > 0)
+      prev = th_o[j - 1 + m * kFftThreads];
+    else
+      prev = (m > 0) ? th_o[kFftThreads - 1 + (m - 1) * kFftThreads] : 0.0f;
+    dem[2 * m] = (j == 0 && m == 0) ? 0.0f : vhs_freq_diff(theta_e[m], prev, fs);
+  }
+  __syncthreads();
+}
+
+// Video and sync-reference planes from the demodulated signal held as
+// dem[2m], dem[2m+1] (samples 2k, 2k+1, k = j + 1024 m).
+__device__ __forceinline__ void video_stage(const float* dem, BlockLds& s, const DemodArgs& a, long long out_base) {
+  const int j = threadIdx.x;
+  c32 D[16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) D[m] = c32{dem[2 * m], dem[2 * m + 1]};
+  fft16k_forward(D, s.buf, s.oct);
+  c32 dm;
+  rfft_post(D, s, dm);
+  const float inv_n = 1.0f / 32768.0f;
+#pragma unroll 1
+  for (int pass = 0; pass < 2; pass++) {
+    const float2* filt = pass == 0 ? a.video : a.vlp05;
+    float* out = pass == 0 ? a.out_video : a.out_vlp;
+    const int shift = pass == 0 ? 0 : a.vlp_offset;
+    c32 v[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const float2 f = filt[j + m * kFftThreads];
+      v[m] = mul(D[m], c32{f.x, f.y});
+    }
+    const float2 fM = filt[kFftM];
+    const c32 vm = mul(dm, c32{fM.x, fM.y});
+    irfft_pre(v, vm, s);
+    fft16k_inverse(v, s.buf, s.oct);
+    // v[m] = N * (y[2k] + i y[2k+1]); output sample n = 2k lands at n - shift (roll by -shift)
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const int k = j + m * kFftThreads;
+      const int n = 2 * k - shift;  // even
+      const int o = n - a.out_begin;  // even: out_begin, shift and out_base are even
+      if (o >= 0 && o < a.out_count) {
+        const float2 val = make_float2(v[m].x * inv_n, v[m].y * inv_n);
+        if (o + 1 < a.out_count)
+          *reinterpret_cast<float2*>(out + out_base + o) = val;
+        else
+          out[out_base + o] = val.x;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
+  const int j = threadIdx.x;
+  const int blk = blockIdx.x;
+  const long long start = a.block_start[blk];
+  const long long out_base = a.out_offset[blk];
+  load_octant(s, a.oct);
+  // ---- stage A: rfft of the int16 block
+  c32 v[16];
+  const int16_t* x = a.rf + start;
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    v[m] = c32{(float)x[2 * k], (float)x[2 * k + 1]};
+  }
+  __syncthreads();
+  fft16k_forward(v, s.buf, s.oct);
+  c32 xm;
+  rfft_post(v, s, xm);
+  // ---- filter by RfVideo (RfMtf is all ones for tape)
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const float2 f = a.rf_video[j + m * kFftThreads];
+    v[m] = mul(v[m], c32{f.x, f.y});
+  }
+  const float2 fM = a.rf_video[kFftM];
+  const c32 fm = mul(xm, c32{fM.x, fM.y});
+  c32 fm_b = c32{0.0f, 0.0f};
+  // X[M] lives in thread 0 only; broadcast through scratch
+  if (j == 0) {
+    s.scratch[0] = fm.x;
+    s.scratch[1] = fm.y;
+  }
+  __syncthreads();
+  fm_b = c32{s.scratch[0], s.scratch[1]};
+  __syncthreads();
+  // ---- stage B: analytic signal + discriminator
+  float dem[32], re[32], th[16];
+  analytic_demod(v, fm_b, s, a.fs, dem, re, th);
+  // spike check over demod[20 .. N-20)
+  bool spike = false;
+  if (a.diff_max > 0.0f) {
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const int n0 = 2 * (j + m * kFftThreads);
+      spike |= (n0 >= 20 && n0 < 32768 - 20 && (double)dem[2 * m] > a.diff_max);
+      spike |= (n0 + 1 >= 20 && n0 + 1 < 32768 - 20 && (double)dem[2 * m + 1] > a.diff_max);
+    }
+  }
+  // ---- envelope: raw[(n + 4) mod N] = |float(re[n])|, FP32 sosfiltfilt
+  float* raw = reinterpret_cast<float*>(s.buf);
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int n0 = 2 * (j + m * kFftThreads);
+    raw[(n0 + 4) & 32767] = fabsf(re[2 * m]);
+    raw[(n0 + 5) & 32767] = fabsf(re[2 * m + 1]);
+  }
+  __syncthreads();
+  const bool weak = sos_filtfilt_block(a.env, raw, 32768, s.scratch, a.out_env + out_base, a.out_begin,
+                                       a.out_count, /*roll=*/0);
+  // ---- flags
+  const unsigned long long any_spike = __ballot(spike);
+  if ((j & 63) == 0 && any_spike) atomicOr(&a.flags[blk], 2u);
+  if (j == 0 && weak) atomicOr(&a.flags[blk], 1u);
+  __syncthreads();
+  // ---- stage C: video and sync reference
+  video_stage(dem, s, a, out_base);
+}
+
+// ------------------------------------------------------------------ chroma
+__global__ void __launch_bounds__(1024) vhs_block_chroma_kernel(ChromaArgs a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
+  const int j = threadIdx.x;
+  const int blk = blockIdx.x;
+  const int16_t* x = a.rf + a.block_start[blk];
+  float* raw = reinterpret_cast<float*>(s.buf);
+  for (int n = j; n < 32768; n += kFftThreads) raw[n] = (float)x[n];
+  __syncthreads();
+  sos_filtfilt_block(a.sos, raw, 32768, s.scratch, a.out_chroma + a.out_offset[blk], a.out_begin, a.out_count,
+                     a.chroma_offset, /*subtract_mean=*/true);
+}
+
+// ------------------------------------------------------------------ spike repair
+__global__ void __launch_bounds__(1024) vhs_block_repair_kernel(RepairArgs r) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
+  const DemodArgs& a = r.d;
+  const int j = threadIdx.x;
+  const int blk = blockIdx.x;
+  if ((a.flags[blk] & 2u) == 0) return;  // uniform across the workgroup
+  const long long start = a.block_start[blk];
+  const long long out_base = a.out_offset[blk];
+  float2* A = r.scratch_analytic + (long long)blk * 32768;
+  float* dm = r.scratch_demod + (long long)blk * 65536;
+  float* dd = dm + 32768;
+  load_octant(s, a.oct);
+  c32 v[16];
+  const int16_t* x = a.rf + start;
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    v[m] = c32{(float)x[2 * k], (float)x[2 * k + 1]};
+  }
+  __syncthreads();
+  fft16k_forward(v, s.buf, s.oct);
+  c32 xm;
+  rfft_post(v, s, xm);
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const float2 f = a.rf_video[j + m * kFftThreads];
+    v[m] = mul(v[m], c32{f.x, f.y});
+  }
+  const float2 fM = a.rf_video[kFftM];
+  if (j == 0) {
+    const c32 fm = mul(xm, c32{fM.x, fM.y});
+    s.scratch[0] = fm.x;
+    s.scratch[1] = fm.y;
+  }
+  __syncthreads();
+  const c32 fm = c32{s.scratch[0], s.scratch[1]};
+  __syncthreads();
+  // analytic signal, materialised
+  const float inv_n = 1.0f / 32768.0f;
+  c32 w[16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    w[m] = (k == 0) ? add(v[m], fm) : scale(v[m], 2.0f);
+  }
+  fft16k_inverse(w, s.buf, s.oct);
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    A[2 * k] = make_float2(w[m].x * inv_n, w[m].y * inv_n);
+  }
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    w[m] = (k == 0) ? sub(v[m], fm) : mul(scale(v[m], 2.0f), conjc(tw32k(s.oct, k)));
+  }
+  fft16k_inverse(w, s.buf, s.oct);
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    A[2 * k + 1] = make_float2(w[m].x * inv_n, w[m].y * inv_n);
+  }
+  __threadfence_block();
+  __syncthreads();
+  // demod and diffed-analytic demod, element-wise (angles recomputed per sample)
+  for (int n = j; n < 32768; n += kFftThreads) {
+    const float2 c = A[n];
+    const float tc = vhs_atan2(c.y, c.x);
+    float2 p = n > 0 ? A[n - 1] : make_float2(0.0f, 0.0f);
+    const float tp = vhs_atan2(p.y, p.x);
+    dm[n] = n == 0 ? 0.0f : vhs_freq_diff(tc, tp, a.fs);
+    // diffed[n] = A[n] - A[n-1], diffed[0] = 0
+    float2 dc = n > 0 ? make_float2(c.x - p.x, c.y - p.y) : make_float2(0.0f, 0.0f);
+    float2 dp = make_float2(0.0f, 0.0f);
+    if (n > 1) {
+      const float2 pp = A[n - 2];
+      dp = make_float2(p.x - pp.x, p.y - pp.y);
+    }
+    const float tdc = vhs_atan2(dc.y, dc.x);
+    const float tdp = vhs_atan2(dp.y, dp.x);
+    dd[n] = n == 0 ? 0.0f : vhs_freq_diff(tdc, tdp, a.fs);
+  }
+  __threadfence_block();
+  __syncthreads();
+  // ReplaceSpikes (RfDemodulator.cs:966-1001): spike indices are collected from
+  // the unmodified demod, then windows [i-8, min(i+30, N-1)) are processed in
+  // ascending order against the progressively repaired demod.  Sequential by
+  // definition; this path only runs for blocks that contain a spike.
+  // sequential repair with an explicit spike list held in LDS scratch space
+  if (j == 0) {
+    const double mx = a.diff_max;
+    int* list = reinterpret_cast<int*>(s.buf);  // up to 32768 indices
+    int count = 0;
+    for (int i = 0; i < 32768; i++)
+      if ((double)dm[i] > mx) list[count++] = i;
+    for (int q = 0; q < count; q++) {
+      const int i = list[q];
+      const int st = i - 8 > 0 ? i - 8 : 0;
+      const int en = i + 30 < 32767 ? i + 30 : 32767;
+      if (st >= en) continue;
+      float md = -INFINITY, mo = -INFINITY;
+      for (int k = st; k < en; k++) {
+        md = fmaxf(md, dd[k]);
+        mo = fmaxf(mo, dm[k]);
+      }
+      if (md < mo)
+        for (int k = st; k < en; k++) dm[k] = dd[k];
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+  float dem[32];
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    dem[2 * m] = dm[2 * k];
+    dem[2 * m + 1] = dm[2 * k + 1];
+  }
+  __syncthreads();
+  video_stage(dem, s, a, out_base);
+}
+
+// ------------------------------------------------------------------ diagnostics
+__global__ void __launch_bounds__(1024) debug_fft_kernel(float2* data, int inverse, const float2* oct_g) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
+  const int j = threadIdx.x;
+  float2* d = data + (long long)blockIdx.x * kFftM;
+  load_octant(s, oct_g);
+  c32 v[16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const float2 t = d[j + m * kFftThreads];
+    v[m] = c32{t.x, t.y};
+  }
+  __syncthreads();
+  if (inverse)
+    fft16k_inverse(v, s.buf, s.oct);
+  else
+    fft16k_forward(v, s.buf, s.oct);
+#pragma unroll
+  for (int m = 0; m < 16; m++) d[j + m * kFftThreads] = make_float2(v[m].x, v[m].y);
+}
+
+}  // namespace vhsd
+
+// ------------------------------------------------------------------ launchers
+extern "C" {
+
+size_t vhsd_block_lds_bytes() { return sizeof(vhsd::BlockLds); }
+
+hipError_t vhsd_launch_block_demod(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream) {
+  const size_t lds = sizeof(vhsd::BlockLds);
+  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_demod_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(vhsd::vhs_block_demod_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
+  return hipGetLastError();
+}
+
+hipError_t vhsd_launch_block_chroma(const vhsd::ChromaArgs* a, int nblocks, hipStream_t stream) {
+  const size_t lds = sizeof(vhsd::BlockLds);
+  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_chroma_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(vhsd::vhs_block_chroma_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
+  return hipGetLastError();
+}
+
+hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const float2* oct, hipStream_t stream) {
+  const size_t lds = sizeof(vhsd::BlockLds);
+  hipError_t e = hipFuncSetAttribute((const void*)vhsd::debug_fft_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(vhsd::debug_fft_kernel, dim3(count), dim3(1024), lds, stream, reinterpret_cast<float2*>(data),
+                     inverse, oct);
+  return hipGetLastError();
+}
+
+hipError_t vhsd_launch_block_repair(const vhsd::RepairArgs* a, int nblocks, hipStream_t stream) {
+  const size_t lds = sizeof(vhsd::BlockLds);
+  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_repair_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(vhsd::vhs_block_repair_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
+  return hipGetLastError();
+}
+
+}  // extern "C"

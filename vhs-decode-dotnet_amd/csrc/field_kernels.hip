@@ -1,0 +1,507 @@
+// Field-level kernels: sync pulse detection, hsync edge refinement and TBC
+// line resampling, all on the stitched field span that the block kernels write.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vhsd_args.hpp"
+
+namespace vhsd {
+
+// ------------------------------------------------------------------ pulses
+// PulseDetection.FindPulsesScalar (Dsp/PulseDetection.cs:195-222): a two-state
+// machine over `value <= high`, NaN samples keep the state.  Each sample is
+// classified independently from the state after the previous sample (the last
+// non-NaN sample at or before it), edges are compacted in order, then pulses
+// pair each end edge with the preceding start edge.
+
+constexpr int kPulseTile = 4096;   // samples per workgroup
+constexpr int kPulseThreads = 256;
+
+__device__ __forceinline__ int in_pulse_at(const float* s, long long i, double high) {
+  // state after processing sample i
+  for (long long k = i; k >= 0; k--) {
+    const float v = s[k];
+    if (v == v) return (double)v <= high ? 1 : 0;
+  }
+  return 0;  // all NaN from the start: the initial state `s[0] <= high` is false
+}
+
+// edge code at sample i >= 1: 0 none, 1 start, 2 end
+__device__ __forceinline__ int edge_at(const float* s, long long i, double high) {
+  const float v = s[i];
+  if (v != v) return 0;
+  const int cur = (double)v <= high ? 1 : 0;
+  const float p = s[i - 1];
+  const int prev = (p == p) ? ((double)p <= high ? 1 : 0) : in_pulse_at(s, i - 1, high);
+  if (!prev && cur) return 1;
+  if (prev && !cur) return 2;
+  return 0;
+}
+
+__global__ void __launch_bounds__(kPulseThreads) pulse_count_kernel(const float* s, long long n, double high,
+                                                                    int* tile_counts) {
+  __shared__ int wave_counts[kPulseThreads / 64];
+  const long long base = (long long)blockIdx.x * kPulseTile;
+  int c = 0;
+  for (int t = threadIdx.x; t < kPulseTile; t += kPulseThreads) {
+    const long long i = base + t;
+    if (i >= 1 && i < n) c += edge_at(s, i, high) != 0;
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & 63) == 0) wave_counts[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < kPulseThreads / 64; w++) tot += wave_counts[w];
+    tile_counts[blockIdx.x] = tot;
+  }
+}
+
+// single workgroup exclusive scan of tile counts (in place); total in *total
+__global__ void __launch_bounds__(1024) exclusive_scan_kernel(int* counts, int n, int* total) {
+  __shared__ int part[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += 1024) {
+    const int i = base + threadIdx.x;
+    const int v = i < n ? counts[i] : 0;
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const int add = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+      __syncthreads();
+      part[threadIdx.x] += add;
+      __syncthreads();
+    }
+    const int incl = part[threadIdx.x];
+    if (i < n) counts[i] = carry + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+// ordered compaction of edges: edge_pos[k], edge_kind[k]
+__global__ void __launch_bounds__(kPulseThreads) pulse_write_kernel(const float* s, long long n, double high,
+                                                                    const int* tile_offsets, long long* edge_pos,
+                                                                    unsigned char* edge_kind, int edge_cap) {
+  __shared__ int wave_counts[kPulseThreads / 64];
+  __shared__ int running;
+  const long long base = (long long)blockIdx.x * kPulseTile;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) running = tile_offsets[blockIdx.x];
+  __syncthreads();
+  for (int t0 = 0; t0 < kPulseTile; t0 += kPulseThreads) {
+    const long long i = base + t0 + threadIdx.x;
+    const int e = (i >= 1 && i < n) ? edge_at(s, i, high) : 0;
+    const unsigned long long m = __ballot(e != 0);
+    if (lane == 0) wave_counts[wave] = __popcll(m);
+    __syncthreads();
+    int before = running;
+    for (int w = 0; w < wave; w++) before += wave_counts[w];
+    if (e != 0) {
+      const int idx = before + __popcll(m & ((1ull << lane) - 1ull));
+      if (idx < edge_cap) {
+        edge_pos[idx] = i;
+        edge_kind[idx] = (unsigned char)e;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int w = 0; w < kPulseThreads / 64; w++) tot += wave_counts[w];
+      running += tot;
+    }
+    __syncthreads();
+  }
+}
+
+// Pair end edges with their start edge, filter by length and start != 0, compact
+// in order.  Single workgroup.  out_count receives the number of pulses found;
+// only the first `capacity` are stored.
+__global__ void __launch_bounds__(1024) pulse_pair_kernel(const long long* edge_pos, const unsigned char* edge_kind,
+                                                          const int* n_edges, int edge_cap, int min_len, int max_len,
+                                                          int* starts, int* lengths, int capacity,
+                                                          int* out_count) {
+  __shared__ int wave_counts[16];
+  __shared__ int running;
+  const int ne = *n_edges < edge_cap ? *n_edges : edge_cap;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) running = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < ne; b0 += 1024) {
+    const int k = b0 + threadIdx.x;
+    bool ok = false;
+    long long st = 0, len = 0;
+    if (k < ne && edge_kind[k] == 2) {
+      const long long e = edge_pos[k];
+      st = (k > 0) ? edge_pos[k - 1] : 0;  // the edge before an end edge is its start
+      len = e - st;
+      ok = len >= min_len && len <= max_len && st != 0;
+    }
+    const unsigned long long m = __ballot(ok);
+    if (lane == 0) wave_counts[wave] = __popcll(m);
+    __syncthreads();
+    int before = running;
+    for (int w = 0; w < wave; w++) before += wave_counts[w];
+    if (ok) {
+      const int idx = before + __popcll(m & ((1ull << lane) - 1ull));
+      if (idx < capacity) {
+        starts[idx] = (int)st;
+        lengths[idx] = (int)len;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int w = 0; w < 16; w++) tot += wave_counts[w];
+      running += tot;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out_count = running;
+}
+
+// ------------------------------------------------------------------ hsync refine
+// TbcFieldDecodePipeline.RefineLineLocationsFromHSync for vhs
+// (Decode/TbcFieldDecodePipeline.cs:4570-4684, helpers :4807-5052) in FP64.
+// Sequential over lines (the porch level carries from line to line); one
+// wavefront per field, lanes cooperate on searches and extrema.  Sums are
+// accumulated left to right on lane 0 so results are bit-identical to the
+// sequential reference on identical input.
+struct WaveCtx {
+  const float* s;
+  long long n;
+  int lane;
+};
+
+__device__ __forceinline__ int iround(double v) { return (int)round(v); }
+
+// CalculateZeroCrossing forward (PulseDetection.cs:224-274)
+__device__ bool zero_crossing(const WaveCtx& w, long long start, double target, int edge, int count, double* out) {
+  if (start < 0 || start >= w.n) return false;
+  int e = edge;
+  if (e == 0) e = ((double)w.s[start] < target) ? 1 : -1;
+  long long end = w.n < start + count + 1 ? w.n : start + count + 1;
+  const long long span = end - start;
+  for (long long b = 1; b < span; b += 64) {
+    const long long i = b + w.lane;
+    bool hit = false;
+    if (i < span) {
+      const double a = (double)w.s[start + i - 1];
+      const double c = (double)w.s[start + i];
+      hit = (e == 1) ? (a < target && c >= target) : (a > target && c <= target);
+    }
+    const unsigned long long m = __ballot(hit);
+    if (m) {
+      const long long x = start + b + (long long)(__ffsll((long long)m) - 1);
+      const double a = (double)w.s[x - 1] - target;
+      const double c = (double)w.s[x] - target;
+      const double y = (c - a) != 0.0 ? -a / (-a + c) : 0.0;
+      *out = (double)(x - 1) + y;
+      return true;
+    }
+  }
+  return false;
+}
+
+__device__ bool area_valid(const WaveCtx& w, double crossing, int one_us, double end_us, double min_hz,
+                           double max_hz) {
+  long long st = iround(crossing - 0.75 * one_us);
+  st = st < 0 ? 0 : (st > w.n ? w.n : st);
+  long long en = iround(crossing + end_us * one_us);
+  en = en < st ? st : (en > w.n ? w.n : en);
+  if (en <= st) return false;
+  double mn = INFINITY, mx = -INFINITY;
+  for (long long i = st + w.lane; i < en; i += 64) {
+    const double v = (double)w.s[i];
+    mn = fmin(mn, v);
+    mx = fmax(mx, v);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mn = fmin(mn, __shfl_xor(mn, o));
+    mx = fmax(mx, __shfl_xor(mx, o));
+  }
+  return mn >= min_hz && mx <= max_hz;
+}
+
+// NaN-aware note: Math.Min/Math.Max propagate NaN; fmin/fmax drop it.  A NaN in
+// the window makes the reference comparison false; mirror that explicitly.
+__device__ bool area_has_nan(const WaveCtx& w, double crossing, int one_us, double end_us) {
+  long long st = iround(crossing - 0.75 * one_us);
+  st = st < 0 ? 0 : (st > w.n ? w.n : st);
+  long long en = iround(crossing + end_us * one_us);
+  en = en < st ? st : (en > w.n ? w.n : en);
+  bool nan = false;
+  for (long long i = st + w.lane; i < en; i += 64) nan |= (w.s[i] != w.s[i]);
+  return __ballot(nan) != 0;
+}
+
+__device__ bool mean_range(const WaveCtx& w, double start, double end, double* mean) {
+  long long a = iround(start);
+  a = a < 0 ? 0 : (a > w.n ? w.n : a);
+  long long b = iround(end);
+  b = b < a ? a : (b > w.n ? w.n : b);
+  if (b <= a) {
+    *mean = 0.0;
+    return false;
+  }
+  double sum = 0.0;
+  if (w.lane == 0)
+    for (long long i = a; i < b; i++) sum += (double)w.s[i];
+  sum = __shfl(sum, 0);
+  *mean = sum / (double)(b - a);
+  return true;
+}
+
+__global__ void __launch_bounds__(64) hsync_refine_kernel(const float* const* spans, const long long* span_len,
+                                                          const double* const* locs, const unsigned char* const* filled,
+                                                          const int* line_counts, HsyncParams p,
+                                                          double* const* refined_out,
+                                                          unsigned char* const* filled_out) {
+  const int f = blockIdx.x;
+  WaveCtx w{spans[f], span_len[f], (int)threadIdx.x};
+  const int lines = line_counts[f];
+  const double* loc = locs[f];
+  const unsigned char* fin = filled[f];
+  double* ro = refined_out[f];
+  unsigned char* fo = filled_out[f];
+  const int one_us = (int)p.fs_mhz > 1 ? (int)p.fs_mhz : 1;
+  const int normal_len = (int)(p.hsync_us * p.fs_mhz) > 1 ? (int)(p.hsync_us * p.fs_mhz) : 1;
+  const int search_count = one_us * 2 > 1 ? one_us * 2 : 1;
+  const long long n = w.n;
+  auto ire_hz = [&](double ire) { return p.ire0 + p.hz_ire * ire; };
+  for (int l = w.lane; l < lines; l += 64) {
+    ro[l] = loc[l];
+    fo[l] = fin[l];
+  }
+  __syncthreads();
+  double prev_porch = -1.0;
+  for (int line = 0; line < lines; line++) {
+    const bool vsync_line = (line >= 3 && line <= 6) || (p.is_pal && line >= 1 && line <= 2);
+    if (vsync_line) {
+      if (w.lane == 0) fo[line] = 1;
+      continue;
+    }
+    const double cur = loc[line];
+    if (!isfinite(cur)) continue;
+    const long long search_start = iround(cur - one_us);
+    if (search_start < 0 || search_start >= n - 1) continue;
+    double crossing = 0.0;
+    const int cnt = (int)(search_count < n - search_start - 1 ? search_count : n - search_start - 1);
+    const bool has = zero_crossing(w, search_start, p.threshold, 0, cnt, &crossing);
+    const double original = cur;
+    double right_ref = original;
+    double out_loc = original;
+    unsigned char out_filled = fin[line];
+    if (has && !fin[line]) {
+      double refined_loc = crossing, porch = 0.0, sync_level = 0.0;
+      bool preserve = false, left_ok = false;
+      const bool valid = area_valid(w, crossing, one_us, 3.5, ire_hz(-65.0), ire_hz(110.0)) &&
+                         !area_has_nan(w, crossing, one_us, 3.5);
+      if (valid && mean_range(w, crossing + one_us, crossing + 2.5 * one_us, &sync_level)) {
+        preserve = true;
+        bool porch_ok = true;
+        if (prev_porch > 0.0)
+          porch = prev_porch;
+        else if (!mean_range(w, crossing - one_us, crossing - 0.5 * one_us, &porch))
+          porch_ok = false;
+        if (porch_ok) {
+          const double mid = (porch + sync_level) / 2.0;
+          const int c4 = (int)(400 < n - search_start - 1 ? 400 : n - search_start - 1);
+          double rc = 0.0;
+          bool good = zero_crossing(w, search_start, mid, 0, c4, &rc) && fabs(rc - crossing) < one_us / 2.0;
+          if (!good && prev_porch > 0.0) {
+            good = zero_crossing(w, search_start, (prev_porch + sync_level) / 2.0, 0, c4, &rc) &&
+                   fabs(rc - crossing) < one_us / 2.0;
+            if (good) porch = prev_porch;
+          }
+          if (good) {
+            refined_loc = rc;
+            left_ok = true;
+          }
+        }
+      }
+      if (left_ok) {
+        out_loc = refined_loc;
+        right_ref = refined_loc;
+        out_filled = fin[line];
+        if (porch > 0.0) prev_porch = porch;
+      } else {
+        out_filled = 1;
+        out_loc = original;
+        if (preserve) right_ref = refined_loc;
+      }
+    } else {
+      out_filled = 1;
+      out_loc = original;
+    }
+    if (p.use_right) {
+      do {
+        const long long rs = search_start + normal_len - one_us;
+        if (rs < 0 || rs >= n - 1) break;
+        if ((double)w.s[rs] > p.threshold) break;
+        const int rcount = normal_len * 2 > 1 ? normal_len * 2 : 1;
+        const int rc_n = (int)(rcount < n - rs - 1 ? rcount : n - rs - 1);
+        double rcross = 0.0;
+        if (!zero_crossing(w, rs, p.threshold, 1, rc_n, &rcross)) break;
+        const double rds = rcross - normal_len;
+        double sync_level = 0.0, rporch = 0.0;
+        if (!(area_valid(w, rds, one_us, 8.0, ire_hz(-65.0), ire_hz(30.0)) && !area_has_nan(w, rds, one_us, 8.0)))
+          break;
+        if (!mean_range(w, rds + one_us, rds + 2.5 * one_us, &sync_level)) break;
+        if (!mean_range(w, rds + normal_len + one_us, rds + normal_len + 2.0 * one_us, &rporch)) break;
+        const double mid = (rporch + sync_level) / 2.0;
+        const int c4 = (int)(400 < n - rs - 1 ? 400 : n - rs - 1);
+        double rrc = 0.0;
+        if (!zero_crossing(w, rs, mid, 0, c4, &rrc)) break;
+        if (!(fabs(rrc - rcross) < one_us / 2.0)) break;
+        const double sr = (double)(float)p.fs_mhz;
+        const double cand = rcross - normal_len + (2.25 * (sr / 40.0));
+        if (fabs(cand - right_ref) >= one_us * 2.0) break;
+        out_loc = cand;
+        out_filled = 0;
+        if (rporch > 0.0) prev_porch = rporch;
+      } while (false);
+    }
+    if (w.lane == 0) {
+      ro[line] = out_loc;
+      fo[line] = out_filled;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ TBC resample
+// TbcLineResampler linear positions + SampleSinc + VideoOutputConverter
+// (Tbc/TbcLineResampler.cs:732-804, 1167-1234, 1495-1525;
+// Tbc/VideoOutputConverter.cs:74-90).  One thread per output sample.
+__device__ __forceinline__ double sample_sinc(const float* src, long long len, double position,
+                                              const float* __restrict__ lut) {
+  if (!isfinite(position)) return 0.0;
+  const int taps = 16, phases = 65536;
+  const float coord = (float)position;
+  const int ci = (int)coord;
+  const float frac = coord - (float)ci;
+  const float pp = frac * (float)phases;
+  const int ps = (int)pp;
+  const int pe = ps + 1;
+  const float alpha = pp - (float)ps;
+  int a0 = ps < 0 ? ps + phases + 1 : ps;
+  a0 = a0 < 0 ? 0 : (a0 > phases ? phases : a0);
+  int a1 = pe < 0 ? pe + phases + 1 : pe;
+  a1 = a1 < 0 ? 0 : (a1 > phases ? phases : a1);
+  const float* ws = lut + a0 * taps;
+  const float* we = lut + a1 * taps;
+  const long long s0 = (long long)ci - (taps / 2 - 1);
+  const bool interior = len >= taps && s0 >= 0 && s0 <= len - taps;
+  double result = 0.0;
+#pragma unroll
+  for (int t = 0; t < taps; t++) {
+    const float sw = ws[t];
+    const float wgt = __builtin_fmaf(alpha, we[t] - sw, sw);
+    long long si = s0 + t;
+    if (!interior) {
+      if (si < 0) si += len;
+      si = si < 0 ? 0 : (si > len - 1 ? len - 1 : si);
+    }
+    float prod;
+    {
+#pragma clang fp contract(off)
+      prod = src[si] * wgt;
+    }
+    result = (double)prod + result;
+  }
+  return result;
+}
+
+__global__ void __launch_bounds__(256) tbc_resample_kernel(const float* src, long long src_len, const double* locs,
+                                                           const double* level_adjust, int level_per_line,
+                                                           TbcParams p, const float* __restrict__ lut,
+                                                           unsigned short* out_u16, double* out_f64) {
+  const long long d = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long dest_len = (long long)p.out_len * p.line_count;
+  if (d >= dest_len) return;
+  const long long prefix = (long long)p.first_line * p.out_len;
+  const long long si = prefix + d;
+  const int left = (int)(si / p.out_len);
+  const double ll = locs[left];
+  const double delta = locs[left + 1] - ll;
+  const double scale = p.nominal / p.out_len;
+  const double sp = (double)si * scale;
+  double frac = (sp - (left * p.nominal)) / p.nominal;
+  frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
+  double pos;
+  {
+#pragma clang fp contract(off)
+    pos = ll + (delta * frac);
+  }
+  const double adj = level_per_line ? level_adjust[left] : level_adjust[si];
+  double r;
+  {
+#pragma clang fp contract(off)
+    r = (double)(float)(sample_sinc(src, src_len, pos, lut) * adj);
+  }
+  if (out_f64) out_f64[d] = r;
+  if (p.has_conv && out_u16) {
+    double sd;
+    {
+#pragma clang fp contract(off)
+      sd = p.out_scale * (r - p.ire0);
+    }
+    const double v = __fma_rn(sd, p.inv_hz_ire, p.conv_offset);
+    unsigned short o;
+    if (v <= 0.0)
+      o = 0;
+    else if (v >= 65535.0)
+      o = 65535;
+    else
+      o = (unsigned short)v;
+    out_u16[d] = o;
+  }
+}
+
+}  // namespace vhsd
+
+extern "C" {
+
+hipError_t vhsd_launch_pulses(const float* s, long long n, double high, int min_len, int max_len, int* tile_counts,
+                              int n_tiles_cap, long long* edge_pos, unsigned char* edge_kind, int edge_cap,
+                              int* counters, int* starts, int* lengths, int capacity, hipStream_t stream) {
+  const int tiles = (int)((n + vhsd::kPulseTile - 1) / vhsd::kPulseTile);
+  if (tiles > n_tiles_cap) return hipErrorInvalidValue;
+  if (tiles == 0) return hipMemsetAsync(counters, 0, 2 * sizeof(int), stream);
+  hipLaunchKernelGGL(vhsd::pulse_count_kernel, dim3(tiles), dim3(vhsd::kPulseThreads), 0, stream, s, n, high,
+                     tile_counts);
+  hipLaunchKernelGGL(vhsd::exclusive_scan_kernel, dim3(1), dim3(1024), 0, stream, tile_counts, tiles, counters);
+  hipLaunchKernelGGL(vhsd::pulse_write_kernel, dim3(tiles), dim3(vhsd::kPulseThreads), 0, stream, s, n, high,
+                     tile_counts, edge_pos, edge_kind, edge_cap);
+  hipLaunchKernelGGL(vhsd::pulse_pair_kernel, dim3(1), dim3(1024), 0, stream, edge_pos, edge_kind, counters,
+                     edge_cap, min_len, max_len, starts, lengths, capacity, counters + 1);
+  return hipGetLastError();
+}
+
+hipError_t vhsd_launch_hsync(const float* const* spans, const long long* span_len, const double* const* locs,
+                             const unsigned char* const* filled, const int* line_counts, const vhsd::HsyncParams* p,
+                             double* const* refined_out, unsigned char* const* filled_out, int fields,
+                             hipStream_t stream) {
+  hipLaunchKernelGGL(vhsd::hsync_refine_kernel, dim3(fields), dim3(64), 0, stream, spans, span_len, locs, filled,
+                     line_counts, *p, refined_out, filled_out);
+  return hipGetLastError();
+}
+
+hipError_t vhsd_launch_tbc(const float* src, long long src_len, const double* locs, const double* level_adjust,
+                           int level_per_line, const vhsd::TbcParams* p, const float* lut, unsigned short* out_u16,
+                           double* out_f64, hipStream_t stream) {
+  const long long dest = (long long)p->out_len * p->line_count;
+  if (dest <= 0) return hipSuccess;
+  const int blocks = (int)((dest + 255) / 256);
+  hipLaunchKernelGGL(vhsd::tbc_resample_kernel, dim3(blocks), dim3(256), 0, stream, src, src_len, locs, level_adjust,
+                     level_per_line, *p, lut, out_u16, out_f64);
+  return hipGetLastError();
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// Kernel argument blocks shared by the host orchestration and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sos_scan.hpp"
+
+namespace vhsd {
+
+struct DemodArgs {
+  const int16_t* rf;             // RF samples (device)
+  const long long* block_start;  // input sample offset of each block
+  const long long* out_offset;   // stitched output offset of each block's first kept sample
+  int out_begin;                 // first kept sample inside the block (block cut)
+  int out_count;                 // kept samples per block (stride)
+  const float2* rf_video;        // [M+1] half spectra
+  const float2* video;
+  const float2* vlp05;
+  const float2* oct;             // twiddle octant [kOctantEntries]
+  int vlp_offset;                // sync reference roll (samples)
+  float fs;                      // sample rate as FP32 (the discriminator's frequency)
+  double diff_max;               // spike-repair threshold; <= 0 disables
+  SosScanArgs env;               // envelope filter
+  float* out_video;
+  float* out_vlp;
+  float* out_env;
+  unsigned* flags;               // per block: bit0 weak RF, bit1 spike repair needed
+};
+
+struct ChromaArgs {
+  const int16_t* rf;
+  const long long* block_start;
+  const long long* out_offset;
+  int out_begin;
+  int out_count;
+  int chroma_offset;  // roll (samples)
+  SosScanArgs sos;
+  float* out_chroma;
+};
+
+struct RepairArgs {
+  DemodArgs d;
+  float2* scratch_analytic;  // [nblocks][32768] complex
+  float* scratch_demod;      // [nblocks][65536]: demod, diffed demod
+};
+
+struct HsyncParams {
+  double threshold;
+  double fs_mhz;       // sample rate in MHz (SyncAnalyzer.SampleRateMHz)
+  double hsync_us;     // HSyncPulseUs
+  double ire0;
+  double hz_ire;
+  int is_pal;
+  int use_right;
+};
+
+struct TbcParams {
+  int out_len;         // output line length
+  int first_line;
+  int line_count;
+  double nominal;      // nominal input line length
+  int has_conv;
+  double ire0;
+  double out_scale;
+  double inv_hz_ire;
+  double conv_offset;  // (outputZero + 0.5) - vsync_ire * outputScale
+};
+
+}  // namespace vhsd
+
+extern "C" {
+size_t vhsd_block_lds_bytes();
+hipError_t vhsd_launch_block_demod(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
+hipError_t vhsd_launch_block_chroma(const vhsd::ChromaArgs* a, int nblocks, hipStream_t stream);
+hipError_t vhsd_launch_block_repair(const vhsd::RepairArgs* a, int nblocks, hipStream_t stream);
+hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const float2* oct, hipStream_t stream);
+hipError_t vhsd_launch_pulses(const float* s, long long n, double high, int min_len, int max_len, int* tile_counts,
+                              int n_tiles_cap, long long* edge_pos, unsigned char* edge_kind, int edge_cap,
+                              int* counters, int* starts, int* lengths, int capacity, hipStream_t stream);
+hipError_t vhsd_launch_hsync(const float* const* spans, const long long* span_len, const double* const* locs,
+                             const unsigned char* const* filled, const int* line_counts, const vhsd::HsyncParams* p,
+                             double* const* refined_out, unsigned char* const* filled_out, int fields,
+                             hipStream_t stream);
+hipError_t vhsd_launch_tbc(const float* src, long long src_len, const double* locs, const double* level_adjust,
+                           int level_per_line, const vhsd::TbcParams* p, const float* lut, unsigned short* out_u16,
+                           double* out_f64, hipStream_t stream);
+}

@@ -1,0 +1,42 @@
+// Device helpers shared by the decode kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace vhsd {
+
+constexpr int kScratchFloats = 512;
+
+// PortedMath.VhsRustAtan2Approximation (PortedMath.cs:519-544): FP32 polynomial
+// atan2, evaluated with separately rounded operations (no contraction) so that
+// identical inputs give bit-identical angles to the reference.
+__device__ __forceinline__ float vhs_atan2(float y, float x) {
+#pragma clang fp contract(off)
+  const float min_normal = 1.17549435E-38f;
+  x += copysignf(min_normal, x);
+  const bool swap = fabsf(x) < fabsf(y);
+  const float input = swap ? x / y : y / x;
+  const float sq = input * input;
+  float r = input * (0.99997726f +
+                     (sq * (-0.33262347f +
+                            (sq * (0.19354346f +
+                                   (sq * (-0.11643287f + (sq * (0.05265332f + (sq * -0.01172120f))))))))));
+  const float pi = 3.14159265358979323846f;
+  const float half_pi = input >= 0.0f ? pi / 2.0f : -pi / 2.0f;
+  r = swap ? half_pi - r : r;
+  const bool xp = x >= 0.0f, yp = y >= 0.0f;
+  if (xp && yp) return r;
+  if (!xp && yp) return pi + r;
+  if (!xp && !yp) return -pi + r;
+  return r;
+}
+
+// PortedMath.VhsRustFrequencyDifference (PortedMath.cs:546-551).
+__device__ __forceinline__ float vhs_freq_diff(float cur, float prev, float freq) {
+#pragma clang fp contract(off)
+  const float tau = 6.28318530717958647692f;
+  float d = cur - prev;
+  d -= floorf(d / tau) * tau;
+  return (d * freq) / tau;
+}
+
+}  // namespace vhsd

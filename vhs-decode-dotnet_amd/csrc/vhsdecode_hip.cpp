@@ -1,0 +1,736 @@
+// Host side of VHSDecode.Hip.Native: context management, filter-table staging,
+// launch orchestration and the C ABI declared in include/vhsdecode_hip.h.
+#include "vhsdecode_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "fft16k.hpp"
+#include "sos_scan.hpp"
+
+#include "vhsd_args.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int32_t fail(int32_t code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define VHSD_HIP(call)                                                                             \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess) return fail(VHSDECODE_HIP_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// SosFilter.DefaultPadLength (Dsp/SosFilter.cs:540-545)
+int default_pad_length(const double* sos, int sections) {
+  int zn = 0, zd = 0;
+  for (int s = 0; s < sections; s++) {
+    if (sos[s * 6 + 2] == 0.0) zn++;
+    if (sos[s * 6 + 5] == 0.0) zd++;
+  }
+  return 3 * ((2 * sections) + 1 - std::min(zn, zd));
+}
+
+// FP32 steady-state initial conditions exactly as
+// SosFilter.SteadyStateInitialConditionsFloat32 (Dsp/SosFilter.cs:924-951).
+#pragma clang fp contract(off)
+void steady_state_f32(const float (*c)[6], int sections, float* zi) {
+  float scale = 1.0f;
+  for (int i = 0; i < sections; i++) {
+    const float b0 = c[i][0], b1 = c[i][1], b2 = c[i][2], a0 = c[i][3], a1 = c[i][4], a2 = c[i][5];
+    float t1 = b1 - (a1 * b0);
+    float t2 = b2 - (a2 * b0);
+    float num = (0.0f + t1) + t2;
+    float den = (1.0f + a1) + a2;
+    float z0 = num / den;
+    float z1 = ((1.0f + a1) * z0) - t1;
+    zi[2 * i] = scale * z0;
+    zi[2 * i + 1] = scale * z1;
+    float ndc = ((0.0f + b0) + b1) + b2;
+    float ddc = ((0.0f + a0) + a1) + a2;
+    scale *= ndc / ddc;
+  }
+}
+
+// Build SosScanArgs for an N-sample block: FP32 coefficients, zi, and the
+// per-level state-transition matrices A^(C 2^d) (FP64 products, stored FP32).
+int32_t prepare_sos(const double* sos, int sections, int n, vhsd::SosScanArgs* a, std::vector<float>* trans) {
+  if (sections < 1 || sections > vhsd::kMaxSections)
+    return fail(VHSDECODE_HIP_E_UNSUPPORTED, "SOS filters of 1 to 4 sections are supported");
+  std::memset(a, 0, sizeof(*a));
+  a->sections = sections;
+  a->pad = default_pad_length(sos, sections);
+  const int L = n + 2 * a->pad;
+  a->chunk = (L + vhsd::kScanThreads - 1) / vhsd::kScanThreads;
+  if (a->chunk > vhsd::kChunk || a->pad > 64) return fail(VHSDECODE_HIP_E_UNSUPPORTED, "SOS padding too long");
+  float c32[vhsd::kMaxSections][6];
+  for (int s = 0; s < sections; s++) {
+    for (int q = 0; q < 6; q++) c32[s][q] = (float)sos[s * 6 + q];
+    a->coef[s][0] = c32[s][0];
+    a->coef[s][1] = c32[s][1];
+    a->coef[s][2] = c32[s][2];
+    a->coef[s][3] = c32[s][4];
+    a->coef[s][4] = c32[s][5];
+  }
+  steady_state_f32(c32, sections, a->zi);
+  const int W = 2 * sections;
+  // one-step transition with zero input, evaluated in FP64 on the FP32 coefficients
+  std::vector<double> A(W * W, 0.0);
+  for (int col = 0; col < W; col++) {
+    std::vector<double> z(W, 0.0);
+    z[col] = 1.0;
+    double v = 0.0;
+    for (int k = 0; k < sections; k++) {
+      const double b0 = c32[k][0], b1 = c32[k][1], b2 = c32[k][2], a1 = c32[k][4], a2 = c32[k][5];
+      const double f = b0 * v + z[2 * k];
+      const double z1 = b1 * v - a1 * f + z[2 * k + 1];
+      const double z2 = b2 * v - a2 * f;
+      z[2 * k] = z1;
+      z[2 * k + 1] = z2;
+      v = f;
+    }
+    for (int r = 0; r < W; r++) A[r * W + col] = z[r];
+  }
+  auto matmul = [W](const std::vector<double>& x, const std::vector<double>& y) {
+    std::vector<double> o(W * W, 0.0);
+    for (int r = 0; r < W; r++)
+      for (int k = 0; k < W; k++)
+        for (int c = 0; c < W; c++) o[r * W + c] += x[r * W + k] * y[k * W + c];
+    return o;
+  };
+  std::vector<double> P(W * W, 0.0);
+  for (int r = 0; r < W; r++) P[r * W + r] = 1.0;
+  std::vector<double> base = A;
+  int e = a->chunk;
+  while (e > 0) {
+    if (e & 1) P = matmul(P, base);
+    base = matmul(base, base);
+    e >>= 1;
+  }
+  trans->assign((size_t)vhsd::kScanLevels * W * W, 0.0f);
+  std::vector<double> Q = P;
+  for (int d = 0; d < vhsd::kScanLevels; d++) {
+    for (int i = 0; i < W * W; i++) (*trans)[(size_t)d * W * W + i] = (float)Q[i];
+    Q = matmul(Q, Q);
+  }
+  return VHSDECODE_HIP_OK;
+}
+#pragma clang fp contract(on)
+
+std::vector<float2> octant_table() {
+  std::vector<float2> t(vhsd::kOctantEntries);
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (int u = 0; u < vhsd::kOctantEntries; u++) {
+    long double a = two_pi * (long double)u / 16384.0L;
+    t[u] = make_float2((float)cosl(a), (float)sinl(a));
+  }
+  return t;
+}
+
+// TbcLineResampler.BuildKaiserSincLookup (Tbc/TbcLineResampler.cs:1297-1330)
+double bessel_i0(double x) {
+  double sum = 1.0, y = (x * x) / 4.0, term = 1.0;
+  for (int k = 1; k < 100; k++) {
+    term *= y / (double)(k * k);
+    double next = sum + term;
+    if (next == sum) break;
+    sum = next;
+  }
+  return sum;
+}
+
+std::vector<float> kaiser_sinc_table() {
+  const int taps = 16, phases = 65536, half = 8;
+  const double beta = 5.0, i0b = bessel_i0(beta);
+  std::vector<float> lut((size_t)(phases + 1) * taps);
+  double w[16];
+  for (int p = 0; p < phases; p++) {
+    double ph = (double)p / phases, sum = 0.0;
+    for (int t = 0; t < taps; t++) {
+      double x = (half - 1 - t) + ph;
+      double s = x == 0.0 ? 1.0 : std::sin(M_PI * x) / (M_PI * x);
+      double r = x / half;
+      double k = (r < -1.0 || r > 1.0) ? 0.0 : bessel_i0(beta * std::sqrt(1.0 - r * r)) / i0b;
+      w[t] = s * k;
+      sum += w[t];
+    }
+    for (int t = 0; t < taps; t++) lut[(size_t)p * taps + t] = (float)(w[t] / sum);
+  }
+  for (int t = 0; t < taps; t++) lut[(size_t)phases * taps + t] = lut[(size_t)(phases - 1) * taps + t];
+  return lut;
+}
+
+double median(std::vector<double> v) {
+  if (v.empty()) return NAN;
+  for (double x : v)
+    if (std::isnan(x)) return x;
+  std::sort(v.begin(), v.end());
+  size_t m = v.size() / 2;
+  return (v.size() & 1) ? v[m] : (v[m - 1] + v[m]) / 2.0;
+}
+
+}  // namespace
+
+struct vhsdecode_hip_demod_context {
+  std::mutex mu;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n = 0, cut = 0, cut_end = 0, stride = 0, max_blocks = 0;
+  double fs = 0.0;
+  int vlp_offset = 0, chroma_offset = 0;
+  double diff_max = 0.0;
+  bool has_chroma = false;
+  vhsd::SosScanArgs env{}, chroma{};
+  DevBuf<float2> rfv, vid, vlp, oct;
+  DevBuf<float> env_trans, chroma_trans;
+  DevBuf<long long> block_start, out_offset;
+  DevBuf<unsigned> flags;
+  DevBuf<float2> rep_a;
+  DevBuf<float> rep_dm;
+  // pulse scratch
+  DevBuf<int> tile_counts, counters, p_starts, p_lens;
+  DevBuf<long long> edge_pos;
+  DevBuf<unsigned char> edge_kind;
+  // hsync scratch
+  DevBuf<double> h_loc, h_ref;
+  DevBuf<unsigned char> h_fil, h_fo;
+  DevBuf<unsigned char> h_ptrs;
+  // tbc
+  DevBuf<float> lut;
+  bool lut_ready = false;
+  DevBuf<double> t_loc, t_adj, t_f64;
+  DevBuf<unsigned short> t_u16;
+  // host-variant staging
+  DevBuf<int16_t> s_rf;
+  DevBuf<float> s_planes;
+  DevBuf<unsigned> s_flags;
+
+  ~vhsdecode_hip_demod_context() {
+    (void)hipSetDevice(device);
+    rfv.release(); vid.release(); vlp.release(); oct.release();
+    env_trans.release(); chroma_trans.release(); block_start.release(); out_offset.release(); flags.release();
+    rep_a.release(); rep_dm.release(); tile_counts.release(); counters.release(); p_starts.release();
+    p_lens.release(); edge_pos.release(); edge_kind.release(); h_loc.release(); h_ref.release();
+    h_fil.release(); h_fo.release(); h_ptrs.release(); lut.release(); t_loc.release(); t_adj.release();
+    t_f64.release(); t_u16.release(); s_rf.release(); s_planes.release(); s_flags.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+hipStream_t pick_stream(vhsdecode_hip_demod_context* c, void* stream) {
+  return stream ? reinterpret_cast<hipStream_t>(stream) : c->stream;
+}
+
+int32_t upload_half(DevBuf<float2>& dst, const double* full, int n) {
+  const int m = n / 2 + 1;
+  std::vector<float2> h(m);
+  for (int k = 0; k < m; k++) h[k] = make_float2((float)full[2 * k], (float)full[2 * k + 1]);
+  VHSD_HIP(dst.ensure(m));
+  VHSD_HIP(hipMemcpy(dst.p, h.data(), sizeof(float2) * m, hipMemcpyHostToDevice));
+  return VHSDECODE_HIP_OK;
+}
+
+int32_t ensure_lut(vhsdecode_hip_demod_context* c) {
+  if (c->lut_ready) return VHSDECODE_HIP_OK;
+  std::vector<float> t = kaiser_sinc_table();
+  VHSD_HIP(c->lut.ensure(t.size()));
+  VHSD_HIP(hipMemcpy(c->lut.p, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice));
+  c->lut_ready = true;
+  return VHSDECODE_HIP_OK;
+}
+
+int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int64_t first_sample, int32_t nblocks,
+                          const vhsdecode_hip_span_v1* out, hipStream_t st) {
+  if (nblocks <= 0) return VHSDECODE_HIP_OK;
+  if (nblocks > c->max_blocks)
+    return fail(VHSDECODE_HIP_E_CAPACITY, "nblocks exceeds the context's max_blocks");
+  if (!rf || !out || !out->video || !out->video_lowpass || !out->envelope)
+    return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "rf and the video, video_lowpass and envelope planes are required");
+  if (first_sample < 0 || (first_sample & 1))
+    return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "first_sample must be even and non-negative");
+  std::vector<long long> starts(nblocks), offs(nblocks);
+  for (int b = 0; b < nblocks; b++) {
+    starts[b] = first_sample + (long long)b * c->stride;
+    offs[b] = (long long)b * c->stride;
+  }
+  VHSD_HIP(hipMemcpyAsync(c->block_start.p, starts.data(), sizeof(long long) * nblocks, hipMemcpyHostToDevice, st));
+  VHSD_HIP(hipMemcpyAsync(c->out_offset.p, offs.data(), sizeof(long long) * nblocks, hipMemcpyHostToDevice, st));
+  unsigned* flags = out->block_flags ? out->block_flags : c->flags.p;
+  VHSD_HIP(hipMemsetAsync(flags, 0, sizeof(unsigned) * nblocks, st));
+  vhsd::DemodArgs a{};
+  a.rf = rf;
+  a.block_start = c->block_start.p;
+  a.out_offset = c->out_offset.p;
+  a.out_begin = c->cut;
+  a.out_count = c->stride;
+  a.rf_video = c->rfv.p;
+  a.video = c->vid.p;
+  a.vlp05 = c->vlp.p;
+  a.oct = c->oct.p;
+  a.vlp_offset = c->vlp_offset;
+  a.fs = (float)c->fs;
+  a.diff_max = c->diff_max;
+  a.env = c->env;
+  a.env.trans = c->env_trans.p;
+  a.out_video = out->video;
+  a.out_vlp = out->video_lowpass;
+  a.out_env = out->envelope;
+  a.flags = flags;
+  VHSD_HIP(vhsd_launch_block_demod(&a, nblocks, st));
+  if (c->diff_max > 0.0) {
+    VHSD_HIP(c->rep_a.ensure((size_t)nblocks * 32768));
+    VHSD_HIP(c->rep_dm.ensure((size_t)nblocks * 65536));
+    vhsd::RepairArgs r{a, c->rep_a.p, c->rep_dm.p};
+    VHSD_HIP(vhsd_launch_block_repair(&r, nblocks, st));
+  }
+  if (c->has_chroma && out->chroma) {
+    vhsd::ChromaArgs ch{};
+    ch.rf = rf;
+    ch.block_start = c->block_start.p;
+    ch.out_offset = c->out_offset.p;
+    ch.out_begin = c->cut;
+    ch.out_count = c->stride;
+    ch.chroma_offset = c->chroma_offset;
+    ch.sos = c->chroma;
+    ch.sos.trans = c->chroma_trans.p;
+    ch.out_chroma = out->chroma;
+    VHSD_HIP(vhsd_launch_block_chroma(&ch, nblocks, st));
+  }
+  return VHSDECODE_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t vhsdecode_hip_get_abi_version(void) { return VHSDECODE_HIP_ABI_VERSION; }
+
+const char* vhsdecode_hip_get_last_error(void) { return g_last_error.c_str(); }
+
+int32_t vhsdecode_hip_get_runtime_info(int32_t device_id, vhsdecode_hip_runtime_info_v1* info) {
+  try {
+    if (!info) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "info is NULL");
+    if (info->struct_size != sizeof(vhsdecode_hip_runtime_info_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "runtime_info struct_size mismatch");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+      info->device_count = 0;
+      return fail(VHSDECODE_HIP_E_NO_DEVICE, "no HIP device is available");
+    }
+    if (device_id < 0 || device_id >= count) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "device_id out of range");
+    hipDeviceProp_t prop;
+    VHSD_HIP(hipGetDeviceProperties(&prop, device_id));
+    info->device_count = count;
+    info->device_id = device_id;
+    info->compute_units = prop.multiProcessorCount;
+    info->hbm_bytes = prop.totalGlobalMem;
+    info->lds_bytes_per_workgroup = (int32_t)prop.sharedMemPerBlock;
+    std::snprintf(info->device_name, sizeof(info->device_name), "%s", prop.name);
+    std::snprintf(info->gcn_arch, sizeof(info->gcn_arch), "%s", prop.gcnArchName);
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_demod_create(const vhsdecode_hip_demod_config_v1* cfg, vhsdecode_hip_demod_context** out) {
+  try {
+    if (!cfg || !out) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "config and context must not be NULL");
+    *out = nullptr;
+    if (cfg->struct_size != sizeof(vhsdecode_hip_demod_config_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "demod_config struct_size mismatch");
+    const vhsdecode_hip_filters_v1* f = cfg->filters;
+    if (!f || f->struct_size != sizeof(vhsdecode_hip_filters_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "filters missing or struct_size mismatch");
+    if (f->block_length != 32768)
+      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "this build demodulates 32768-sample blocks");
+    if (!f->rf_video || !f->video || !f->video_lowpass05 || !f->envelope_sos || f->envelope_sections <= 0)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "rf_video, video, video_lowpass05 and envelope_sos are required");
+    if (!(f->sample_rate_hz > 0.0)) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "sample_rate_hz must be positive");
+    if (cfg->block_cut < 0 || cfg->block_cut_end < 0 || cfg->block_cut + cfg->block_cut_end >= f->block_length ||
+        (cfg->block_cut & 1) || (cfg->block_cut_end & 1))
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "block cuts must be even and leave decoded samples");
+    if (cfg->max_blocks <= 0) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "max_blocks must be positive");
+    if ((f->video_lowpass05_offset & 1) || f->video_lowpass05_offset < 0 ||
+        f->video_lowpass05_offset > cfg->block_cut_end)
+      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "video_lowpass05_offset must be even and within the tail cut");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+      return fail(VHSDECODE_HIP_E_NO_DEVICE, "no HIP device is available");
+    if (cfg->device_id < 0 || cfg->device_id >= count)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "device_id out of range");
+    VHSD_HIP(hipSetDevice(cfg->device_id));
+    auto* c = new (std::nothrow) vhsdecode_hip_demod_context();
+    if (!c) return fail(VHSDECODE_HIP_E_OUT_OF_MEMORY, "context allocation failed");
+    c->device = cfg->device_id;
+    c->n = f->block_length;
+    c->cut = cfg->block_cut;
+    c->cut_end = cfg->block_cut_end;
+    c->stride = c->n - c->cut - c->cut_end;
+    c->max_blocks = cfg->max_blocks;
+    c->fs = f->sample_rate_hz;
+    c->vlp_offset = f->video_lowpass05_offset;
+    c->chroma_offset = f->chroma_offset;
+    c->diff_max = f->diff_demod_max;
+    auto bail = [&](int32_t code) {
+      delete c;
+      return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+      return bail(fail(VHSDECODE_HIP_E_HIP, "hipStreamCreate failed"));
+    int32_t rc;
+    if ((rc = upload_half(c->rfv, f->rf_video, c->n)) != 0) return bail(rc);
+    if ((rc = upload_half(c->vid, f->video, c->n)) != 0) return bail(rc);
+    if ((rc = upload_half(c->vlp, f->video_lowpass05, c->n)) != 0) return bail(rc);
+    std::vector<float2> oct = octant_table();
+    if (c->oct.ensure(oct.size()) != hipSuccess ||
+        hipMemcpy(c->oct.p, oct.data(), sizeof(float2) * oct.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(VHSDECODE_HIP_E_HIP, "twiddle upload failed"));
+    std::vector<float> tr;
+    if ((rc = prepare_sos(f->envelope_sos, f->envelope_sections, c->n, &c->env, &tr)) != 0) return bail(rc);
+    if (c->env_trans.ensure(tr.size()) != hipSuccess ||
+        hipMemcpy(c->env_trans.p, tr.data(), sizeof(float) * tr.size(), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(VHSDECODE_HIP_E_HIP, "envelope table upload failed"));
+    if (f->chroma_sos && f->chroma_sections > 0) {
+      if ((rc = prepare_sos(f->chroma_sos, f->chroma_sections, c->n, &c->chroma, &tr)) != 0) return bail(rc);
+      if (c->chroma_trans.ensure(tr.size()) != hipSuccess ||
+          hipMemcpy(c->chroma_trans.p, tr.data(), sizeof(float) * tr.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(VHSDECODE_HIP_E_HIP, "chroma table upload failed"));
+      c->has_chroma = true;
+    }
+    if (c->block_start.ensure(c->max_blocks) != hipSuccess || c->out_offset.ensure(c->max_blocks) != hipSuccess ||
+        c->flags.ensure(c->max_blocks) != hipSuccess)
+      return bail(fail(VHSDECODE_HIP_E_OUT_OF_MEMORY, "descriptor allocation failed"));
+    *out = c;
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+void vhsdecode_hip_demod_destroy(vhsdecode_hip_demod_context* c) {
+  if (!c) return;
+  {
+    std::lock_guard<std::mutex> lock(c->mu);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+  }
+  delete c;
+}
+
+int32_t vhsdecode_hip_demod_span_device(vhsdecode_hip_demod_context* c, const int16_t* rf, int64_t first_sample,
+                                        int32_t nblocks, const vhsdecode_hip_span_v1* out, void* stream) {
+  try {
+    if (!c) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "context is NULL");
+    if (!out || out->struct_size != sizeof(vhsdecode_hip_span_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "span struct_size mismatch");
+    std::lock_guard<std::mutex> lock(c->mu);
+    VHSD_HIP(hipSetDevice(c->device));
+    return demod_span_locked(c, rf, first_sample, nblocks, out, pick_stream(c, stream));
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_demod_span_host(vhsdecode_hip_demod_context* c, const int16_t* rf, int64_t rf_count,
+                                      int64_t first_sample, int32_t nblocks, const vhsdecode_hip_span_v1* out) {
+  try {
+    if (!c) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "context is NULL");
+    if (!out || out->struct_size != sizeof(vhsdecode_hip_span_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "span struct_size mismatch");
+    if (!rf || nblocks <= 0 || first_sample < 0) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid span request");
+    std::lock_guard<std::mutex> lock(c->mu);
+    VHSD_HIP(hipSetDevice(c->device));
+    const int64_t need = first_sample + (int64_t)(nblocks - 1) * c->stride + c->n;
+    if (need > rf_count) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "rf buffer is shorter than the requested blocks");
+    const int64_t in_count = need - first_sample;
+    const size_t plane = (size_t)nblocks * c->stride;
+    VHSD_HIP(c->s_rf.ensure((size_t)in_count));
+    VHSD_HIP(c->s_planes.ensure(plane * 4));
+    VHSD_HIP(c->s_flags.ensure((size_t)nblocks));
+    hipStream_t st = c->stream;
+    VHSD_HIP(hipMemcpyAsync(c->s_rf.p, rf + first_sample, sizeof(int16_t) * in_count, hipMemcpyHostToDevice, st));
+    vhsdecode_hip_span_v1 dev{};
+    dev.struct_size = sizeof(dev);
+    dev.video = c->s_planes.p;
+    dev.video_lowpass = c->s_planes.p + plane;
+    dev.envelope = c->s_planes.p + 2 * plane;
+    dev.chroma = out->chroma ? c->s_planes.p + 3 * plane : nullptr;
+    dev.block_flags = c->s_flags.p;
+    int32_t rc = demod_span_locked(c, c->s_rf.p, 0, nblocks, &dev, st);
+    if (rc != 0) return rc;
+    if (out->video)
+      VHSD_HIP(hipMemcpyAsync(out->video, dev.video, sizeof(float) * plane, hipMemcpyDeviceToHost, st));
+    if (out->video_lowpass)
+      VHSD_HIP(hipMemcpyAsync(out->video_lowpass, dev.video_lowpass, sizeof(float) * plane, hipMemcpyDeviceToHost, st));
+    if (out->envelope)
+      VHSD_HIP(hipMemcpyAsync(out->envelope, dev.envelope, sizeof(float) * plane, hipMemcpyDeviceToHost, st));
+    if (out->chroma && dev.chroma)
+      VHSD_HIP(hipMemcpyAsync(out->chroma, dev.chroma, sizeof(float) * plane, hipMemcpyDeviceToHost, st));
+    if (out->block_flags)
+      VHSD_HIP(hipMemcpyAsync(out->block_flags, dev.block_flags, sizeof(unsigned) * nblocks, hipMemcpyDeviceToHost, st));
+    VHSD_HIP(hipStreamSynchronize(st));
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_find_pulses(vhsdecode_hip_demod_context* c, const float* sync_ref, int64_t length,
+                                  double threshold, int32_t min_length, int32_t max_length, int32_t* starts,
+                                  int32_t* lengths, int32_t capacity, int32_t* count) {
+  try {
+    if (!c || !sync_ref || !count || capacity < 0 || length < 0 || ((!starts || !lengths) && capacity > 0))
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid pulse request");
+    std::lock_guard<std::mutex> lock(c->mu);
+    VHSD_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int tiles = (int)((length + 4095) / 4096);
+    const size_t edge_cap = (size_t)(length / 2 + 2);
+    VHSD_HIP(c->tile_counts.ensure(std::max(tiles, 1)));
+    VHSD_HIP(c->counters.ensure(2));
+    VHSD_HIP(c->edge_pos.ensure(edge_cap));
+    VHSD_HIP(c->edge_kind.ensure(edge_cap));
+    VHSD_HIP(c->p_starts.ensure(std::max(capacity, 1)));
+    VHSD_HIP(c->p_lens.ensure(std::max(capacity, 1)));
+    VHSD_HIP(hipMemsetAsync(c->counters.p, 0, 2 * sizeof(int), st));
+    VHSD_HIP(vhsd_launch_pulses(sync_ref, length, threshold, min_length, max_length, c->tile_counts.p, tiles,
+                                c->edge_pos.p, c->edge_kind.p, (int)edge_cap, c->counters.p, c->p_starts.p,
+                                c->p_lens.p, capacity, st));
+    int host_counters[2] = {0, 0};
+    VHSD_HIP(hipMemcpyAsync(host_counters, c->counters.p, sizeof(host_counters), hipMemcpyDeviceToHost, st));
+    VHSD_HIP(hipStreamSynchronize(st));
+    const int n = std::min(host_counters[1], capacity);
+    if (n > 0) {
+      VHSD_HIP(hipMemcpy(starts, c->p_starts.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+      VHSD_HIP(hipMemcpy(lengths, c->p_lens.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    }
+    *count = host_counters[1];
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_refine_hsync(vhsdecode_hip_demod_context* c, const float* sync_ref, int64_t length,
+                                   const double* locations, const uint8_t* filled, int32_t lines,
+                                   const vhsdecode_hip_hsync_params_v1* params, double* refined, uint8_t* filled_out) {
+  try {
+    if (!c || !sync_ref || !locations || !filled || !params || !refined || !filled_out || lines < 0)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid hsync request");
+    if (params->struct_size != sizeof(vhsdecode_hip_hsync_params_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "hsync_params struct_size mismatch");
+    if (lines == 0) return VHSDECODE_HIP_OK;
+    std::lock_guard<std::mutex> lock(c->mu);
+    VHSD_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    VHSD_HIP(c->h_loc.ensure(lines));
+    VHSD_HIP(c->h_ref.ensure(lines));
+    VHSD_HIP(c->h_fil.ensure(lines));
+    VHSD_HIP(c->h_fo.ensure(lines));
+    VHSD_HIP(c->h_ptrs.ensure(6 * sizeof(void*) + 2 * sizeof(long long)));
+    VHSD_HIP(hipMemcpyAsync(c->h_loc.p, locations, sizeof(double) * lines, hipMemcpyHostToDevice, st));
+    VHSD_HIP(hipMemcpyAsync(c->h_fil.p, filled, lines, hipMemcpyHostToDevice, st));
+    // pointer tables for the batched kernel (one field here)
+    struct Table {
+      const float* span;
+      const double* loc;
+      const unsigned char* fil;
+      double* ref;
+      unsigned char* fo;
+      long long len;
+      int lines;
+      int pad;
+    } t{sync_ref, c->h_loc.p, c->h_fil.p, c->h_ref.p, c->h_fo.p, (long long)length, lines, 0};
+    static_assert(sizeof(Table) <= 6 * sizeof(void*) + 2 * sizeof(long long), "table size");
+    VHSD_HIP(hipMemcpyAsync(c->h_ptrs.p, &t, sizeof(t), hipMemcpyHostToDevice, st));
+    Table* dt = reinterpret_cast<Table*>(c->h_ptrs.p);
+    vhsd::HsyncParams p{params->threshold, params->sample_rate_mhz, params->hsync_pulse_us, params->ire0,
+                        params->hz_ire, params->is_pal, params->use_right_hsync};
+    VHSD_HIP(vhsd_launch_hsync(&dt->span, &dt->len, &dt->loc, &dt->fil, &dt->lines, &p, &dt->ref, &dt->fo, 1, st));
+    VHSD_HIP(hipMemcpyAsync(refined, c->h_ref.p, sizeof(double) * lines, hipMemcpyDeviceToHost, st));
+    VHSD_HIP(hipMemcpyAsync(filled_out, c->h_fo.p, lines, hipMemcpyDeviceToHost, st));
+    VHSD_HIP(hipStreamSynchronize(st));
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_set_sinc_table(vhsdecode_hip_demod_context* c, const float* table, int64_t count) {
+  try {
+    if (!c || !table || count != (int64_t)65537 * 16)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "sinc table must hold 65537 x 16 floats");
+    std::lock_guard<std::mutex> lock(c->mu);
+    VHSD_HIP(hipSetDevice(c->device));
+    VHSD_HIP(c->lut.ensure((size_t)count));
+    VHSD_HIP(hipMemcpy(c->lut.p, table, sizeof(float) * count, hipMemcpyHostToDevice));
+    c->lut_ready = true;
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+static int32_t tbc_params(const vhsdecode_hip_tbc_params_v1* p, double nominal, vhsd::TbcParams* t) {
+  if (p->output_line_length <= 0 || p->first_line < 0 || p->line_count < 0)
+    return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid TBC geometry");
+  t->out_len = p->output_line_length;
+  t->first_line = p->first_line;
+  t->line_count = p->line_count;
+  t->nominal = nominal;
+  t->has_conv = p->convert != 0;
+  t->ire0 = p->ire0;
+  t->out_scale = p->output_scale;
+  t->inv_hz_ire = 1.0 / p->hz_ire;
+  t->conv_offset = (p->output_zero + 0.5) - (p->vsync_ire * p->output_scale);
+  return VHSDECODE_HIP_OK;
+}
+
+int32_t vhsdecode_hip_tbc_resample(vhsdecode_hip_demod_context* c, const float* source, int64_t source_length,
+                                   const double* locations, int32_t location_count,
+                                   const vhsdecode_hip_tbc_params_v1* params, uint16_t* out_u16, double* out_f64) {
+  try {
+    if (!c || !source || !locations || !params || source_length <= 0)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid TBC request");
+    if (params->struct_size != sizeof(vhsdecode_hip_tbc_params_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "tbc_params struct_size mismatch");
+    if (location_count < 2) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "at least two line locations are required");
+    for (int i = 0; i < location_count; i++)
+      if (!std::isfinite(locations[i])) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "line locations must be finite");
+    if (params->first_line < 0 || params->first_line + params->line_count >= location_count)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "first_line/line_count outside the line locations");
+    std::lock_guard<std::mutex> lock(c->mu);
+    VHSD_HIP(hipSetDevice(c->device));
+    int32_t rc = ensure_lut(c);
+    if (rc) return rc;
+    // nominal line length and per-line level adjusts (TbcLineResampler.cs:508-509, 818-878, 997-1011)
+    double nominal = params->nominal_line_length;
+    if (!(nominal > 0.0)) {
+      std::vector<double> l(location_count - 1);
+      for (int i = 0; i + 1 < location_count; i++) l[i] = locations[i + 1] - locations[i];
+      nominal = std::max(1.0, median(l));
+    }
+    const int L = params->output_line_length;
+    const int lines = params->first_line + params->line_count;
+    std::vector<double> f(lines), dev(lines);
+    const double inv = 1.0 / nominal;
+    for (int line = 0; line < lines; line++) {
+      int left = std::clamp(line, 0, location_count - 2);
+      f[line] = (locations[left + 1] * inv) - (locations[left] * inv);
+    }
+    std::vector<double> adj_line(lines, 0.0);
+    if (lines > 0) {
+      double med = median(f);
+      for (int i = 0; i < lines; i++) dev[i] = std::fabs(f[i] - med);
+      double mad = median(dev);
+      double thr = mad > 0.0 ? 15.0 * mad : 0.001;
+      for (int i = 0; i < lines; i++) adj_line[i] = std::fabs(f[i] - med) > thr ? med : f[i];
+    }
+    const bool per_sample = params->wow_level_smoothing > 0.0;
+    std::vector<double> adj;
+    if (per_sample) {
+      adj.assign((size_t)lines * L, 0.0);
+      for (int i = 0; i < lines; i++) std::fill(adj.begin() + (size_t)i * L, adj.begin() + (size_t)(i + 1) * L, adj_line[i]);
+      const double alpha = 1.0 / (params->wow_level_smoothing * L);
+      for (size_t i = 1; i < adj.size(); i++) adj[i] = std::fma(adj[i] - adj[i - 1], alpha, adj[i - 1]);
+    } else {
+      adj = adj_line;
+    }
+    vhsd::TbcParams t{};
+    if ((rc = tbc_params(params, nominal, &t)) != 0) return rc;
+    const size_t dest = (size_t)L * params->line_count;
+    hipStream_t st = c->stream;
+    VHSD_HIP(c->t_loc.ensure(location_count));
+    VHSD_HIP(c->t_adj.ensure(std::max<size_t>(adj.size(), 1)));
+    VHSD_HIP(c->t_u16.ensure(std::max<size_t>(dest, 1)));
+    VHSD_HIP(c->t_f64.ensure(std::max<size_t>(dest, 1)));
+    VHSD_HIP(hipMemcpyAsync(c->t_loc.p, locations, sizeof(double) * location_count, hipMemcpyHostToDevice, st));
+    if (!adj.empty())
+      VHSD_HIP(hipMemcpyAsync(c->t_adj.p, adj.data(), sizeof(double) * adj.size(), hipMemcpyHostToDevice, st));
+    VHSD_HIP(vhsd_launch_tbc(source, source_length, c->t_loc.p, c->t_adj.p, per_sample ? 0 : 1, &t, c->lut.p,
+                             out_u16 ? c->t_u16.p : nullptr, out_f64 ? c->t_f64.p : nullptr, st));
+    if (out_u16 && dest)
+      VHSD_HIP(hipMemcpyAsync(out_u16, c->t_u16.p, sizeof(uint16_t) * dest, hipMemcpyDeviceToHost, st));
+    if (out_f64 && dest)
+      VHSD_HIP(hipMemcpyAsync(out_f64, c->t_f64.p, sizeof(double) * dest, hipMemcpyDeviceToHost, st));
+    VHSD_HIP(hipStreamSynchronize(st));
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_tbc_resample_device(vhsdecode_hip_demod_context* c, const float* source, int64_t source_length,
+                                          const double* locations, const double* level_adjust, int32_t per_sample,
+                                          const vhsdecode_hip_tbc_params_v1* params, uint16_t* out_u16,
+                                          double* out_f64, void* stream) {
+  try {
+    if (!c || !source || !locations || !level_adjust || !params || source_length <= 0)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid TBC request");
+    if (params->struct_size != sizeof(vhsdecode_hip_tbc_params_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "tbc_params struct_size mismatch");
+    if (!(params->nominal_line_length > 0.0))
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "the device variant needs an explicit nominal line length");
+    std::lock_guard<std::mutex> lock(c->mu);
+    VHSD_HIP(hipSetDevice(c->device));
+    int32_t rc = ensure_lut(c);
+    if (rc) return rc;
+    vhsd::TbcParams t{};
+    if ((rc = tbc_params(params, params->nominal_line_length, &t)) != 0) return rc;
+    VHSD_HIP(vhsd_launch_tbc(source, source_length, locations, level_adjust, per_sample ? 0 : 1, &t, c->lut.p,
+                             out_u16, out_f64, pick_stream(c, stream)));
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_debug_fft16k(vhsdecode_hip_demod_context* c, float* data, int32_t count, int32_t inverse,
+                                   void* stream) {
+  try {
+    if (!c || !data || count <= 0) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid FFT request");
+    std::lock_guard<std::mutex> lock(c->mu);
+    VHSD_HIP(hipSetDevice(c->device));
+    VHSD_HIP(vhsd_launch_debug_fft(data, count, inverse, c->oct.p, pick_stream(c, stream)));
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,141 @@
+"""ctypes binding of VHSDecode.Hip.Native (include/vhsdecode_hip.h).
+
+This is the binding a managed caller would write (the reference's P/Invoke
+declarations, src/VHSDecode.Core/Dsp/Ipp/IppNativeMethods.cs:6-120 and
+Dsp/CudaFast/CudaFastNativeRuntime.cs:131-800, are the model).  Loading fails
+loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.environ.get("VHSDECODE_HIP_LIBRARY", os.path.join(_PKG_ROOT, "lib", "libvhsdecode_hip.so"))
+
+ABI_VERSION = 1
+
+REQUIRED_EXPORTS = (
+    "vhsdecode_hip_get_abi_version",
+    "vhsdecode_hip_get_runtime_info",
+    "vhsdecode_hip_get_last_error",
+    "vhsdecode_hip_demod_create",
+    "vhsdecode_hip_demod_destroy",
+    "vhsdecode_hip_demod_span_device",
+    "vhsdecode_hip_demod_span_host",
+    "vhsdecode_hip_find_pulses",
+    "vhsdecode_hip_refine_hsync",
+    "vhsdecode_hip_set_sinc_table",
+    "vhsdecode_hip_tbc_resample",
+    "vhsdecode_hip_tbc_resample_device",
+    "vhsdecode_hip_debug_fft16k",
+)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_i16p = ctypes.POINTER(ctypes.c_int16)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u16p = ctypes.POINTER(ctypes.c_uint16)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class RuntimeInfoV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("device_count", ctypes.c_int32), ("device_id", ctypes.c_int32),
+                ("compute_units", ctypes.c_int32), ("hbm_bytes", ctypes.c_uint64),
+                ("lds_bytes_per_workgroup", ctypes.c_int32), ("device_name", ctypes.c_char * 128),
+                ("gcn_arch", ctypes.c_char * 64)]
+
+
+class FiltersV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("block_length", ctypes.c_int32),
+                ("sample_rate_hz", ctypes.c_double), ("rf_video", _dp), ("video", _dp),
+                ("video_lowpass05", _dp), ("video_lowpass05_offset", ctypes.c_int32), ("envelope_sos", _dp),
+                ("envelope_sections", ctypes.c_int32), ("chroma_sos", _dp), ("chroma_sections", ctypes.c_int32),
+                ("chroma_offset", ctypes.c_int32), ("diff_demod_max", ctypes.c_double)]
+
+
+class DemodConfigV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("device_id", ctypes.c_int32), ("block_cut", ctypes.c_int32),
+                ("block_cut_end", ctypes.c_int32), ("max_blocks", ctypes.c_int32),
+                ("filters", ctypes.POINTER(FiltersV1))]
+
+
+class SpanV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("video", _fp), ("video_lowpass", _fp), ("envelope", _fp),
+                ("chroma", _fp), ("block_flags", _u32p)]
+
+
+class HsyncParamsV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("threshold", ctypes.c_double),
+                ("sample_rate_mhz", ctypes.c_double), ("hsync_pulse_us", ctypes.c_double),
+                ("ire0", ctypes.c_double), ("hz_ire", ctypes.c_double), ("is_pal", ctypes.c_int32),
+                ("use_right_hsync", ctypes.c_int32)]
+
+
+class TbcParamsV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("output_line_length", ctypes.c_int32),
+                ("first_line", ctypes.c_int32), ("line_count", ctypes.c_int32),
+                ("nominal_line_length", ctypes.c_double), ("wow_level_smoothing", ctypes.c_double),
+                ("convert", ctypes.c_int32), ("ire0", ctypes.c_double), ("hz_ire", ctypes.c_double),
+                ("output_zero", ctypes.c_int32), ("vsync_ire", ctypes.c_double), ("output_scale", ctypes.c_double)]
+
+
+_lib = None
+
+
+class HipError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"VHSDecode.Hip.Native status {status}: {message}")
+        self.status = status
+
+
+def load():
+    """Load and bind the native library (raises if it is absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            f"VHSDecode.Hip.Native is not built ({LIB_PATH}); run `make -C vhs-decode-dotnet_amd` "
+            "or __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    for name in REQUIRED_EXPORTS:
+        getattr(L, name)
+    L.vhsdecode_hip_get_abi_version.restype = ctypes.c_int32
+    L.vhsdecode_hip_get_last_error.restype = ctypes.c_char_p
+    L.vhsdecode_hip_get_runtime_info.argtypes = [ctypes.c_int32, ctypes.POINTER(RuntimeInfoV1)]
+    L.vhsdecode_hip_demod_create.argtypes = [ctypes.POINTER(DemodConfigV1), ctypes.POINTER(ctypes.c_void_p)]
+    L.vhsdecode_hip_demod_destroy.argtypes = [ctypes.c_void_p]
+    L.vhsdecode_hip_demod_destroy.restype = None
+    L.vhsdecode_hip_demod_span_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+                                                  ctypes.POINTER(SpanV1), ctypes.c_void_p]
+    L.vhsdecode_hip_demod_span_host.argtypes = [ctypes.c_void_p, _i16p, ctypes.c_int64, ctypes.c_int64,
+                                                ctypes.c_int32, ctypes.POINTER(SpanV1)]
+    L.vhsdecode_hip_find_pulses.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_double,
+                                            ctypes.c_int32, ctypes.c_int32, _i32p, _i32p, ctypes.c_int32, _i32p]
+    L.vhsdecode_hip_refine_hsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, _dp, _u8p,
+                                             ctypes.c_int32, ctypes.POINTER(HsyncParamsV1), _dp, _u8p]
+    L.vhsdecode_hip_set_sinc_table.argtypes = [ctypes.c_void_p, _fp, ctypes.c_int64]
+    L.vhsdecode_hip_tbc_resample.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, _dp, ctypes.c_int32,
+                                             ctypes.POINTER(TbcParamsV1), _u16p, _dp]
+    L.vhsdecode_hip_tbc_resample_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                                    ctypes.POINTER(TbcParamsV1), ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.c_void_p]
+    L.vhsdecode_hip_debug_fft16k.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_void_p]
+    for name in REQUIRED_EXPORTS:
+        fn = getattr(L, name)
+        if name not in ("vhsdecode_hip_get_last_error", "vhsdecode_hip_demod_destroy"):
+            fn.restype = ctypes.c_int32
+    if L.vhsdecode_hip_get_abi_version() != ABI_VERSION:
+        raise HipError(-30006, "ABI version mismatch")
+    _lib = L
+    return L
+
+
+def check(status: int):
+    if status != 0:
+        msg = load().vhsdecode_hip_get_last_error()
+        raise HipError(status, msg.decode("utf-8", "replace") if msg else "")

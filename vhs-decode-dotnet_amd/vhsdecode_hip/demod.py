@@ -1,0 +1,206 @@
+"""Host-side mirror of the reference's block decode interface over the HIP backend.
+
+`HipBlockDemodulator` plays the role of RfBlockDecodePipeline + RfBlockStreamDecoder
+for the VHS real-FFT path (src/VHSDecode.Core/Decode/RfBlockDecodePipeline.cs:246-383,
+Decode/RfBlockStreamDecoder.cs:419-700): it owns a native demod context built from a
+`TapeFilterSet` (the DecodeFilterSet subset) and returns stitched field-span planes
+(`RfDecodedSpan`, RfBlockStreamDecoder.cs:7-24).  Pulse detection, hsync refinement
+and TBC resampling run on the same context.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _lib
+from .filters import TapeFilterSet
+
+BLOCK_CUT = 1024       # DecodeSession.cs:259-260
+BLOCK_CUT_END = 1024
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+@dataclass
+class DecodedSpan:
+    """Stitched demodulated planes of consecutive blocks (RfDecodedSpan)."""
+
+    video: np.ndarray
+    video_lowpass: np.ndarray
+    envelope: np.ndarray
+    chroma: Optional[np.ndarray]
+    block_flags: np.ndarray
+
+    @property
+    def weak_rf(self) -> bool:
+        return bool((self.block_flags & 1).any())
+
+
+class HipBlockDemodulator:
+    def __init__(self, filters: TapeFilterSet, block_cut: int = BLOCK_CUT, block_cut_end: int = BLOCK_CUT_END,
+                 max_blocks: int = 64, device: int = 0, sinc_table: Optional[np.ndarray] = None):
+        L = _lib.load()
+        self.filters = filters
+        self.block_length = filters.block_length
+        self.block_cut = block_cut
+        self.block_cut_end = block_cut_end
+        self.stride = self.block_length - block_cut - block_cut_end
+        self.max_blocks = max_blocks
+        self.device = device
+        # keep FP64 tables alive for the duration of the create call
+        self._rfv = np.ascontiguousarray(filters.rf_video, dtype=np.complex128).view(np.float64)
+        self._vid = np.ascontiguousarray(filters.video, dtype=np.complex128).view(np.float64)
+        self._vlp = np.ascontiguousarray(filters.video_lowpass05, dtype=np.complex128).view(np.float64)
+        self._env = np.ascontiguousarray(filters.envelope_sos, dtype=np.float64)
+        self._chr = None if filters.chroma_sos is None else np.ascontiguousarray(filters.chroma_sos, dtype=np.float64)
+        f = _lib.FiltersV1()
+        f.struct_size = ctypes.sizeof(_lib.FiltersV1)
+        f.block_length = self.block_length
+        f.sample_rate_hz = float(filters.sample_rate_hz)
+        f.rf_video = _ptr(self._rfv, _lib._dp)
+        f.video = _ptr(self._vid, _lib._dp)
+        f.video_lowpass05 = _ptr(self._vlp, _lib._dp)
+        f.video_lowpass05_offset = int(filters.video_lowpass05_offset)
+        f.envelope_sos = _ptr(self._env, _lib._dp)
+        f.envelope_sections = self._env.shape[0]
+        f.chroma_sos = _ptr(self._chr, _lib._dp) if self._chr is not None else None
+        f.chroma_sections = 0 if self._chr is None else self._chr.shape[0]
+        f.chroma_offset = int(filters.chroma_offset)
+        f.diff_demod_max = float(filters.diff_demod_max) if filters.diff_demod_max is not None else -1.0
+        cfg = _lib.DemodConfigV1()
+        cfg.struct_size = ctypes.sizeof(_lib.DemodConfigV1)
+        cfg.device_id = device
+        cfg.block_cut = block_cut
+        cfg.block_cut_end = block_cut_end
+        cfg.max_blocks = max_blocks
+        cfg.filters = ctypes.pointer(f)
+        handle = ctypes.c_void_p()
+        _lib.check(L.vhsdecode_hip_demod_create(ctypes.byref(cfg), ctypes.byref(handle)))
+        self._h = handle
+        self._L = L
+        if sinc_table is not None:
+            self.set_sinc_table(sinc_table)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.vhsdecode_hip_demod_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ------------------------------------------------------------------ blocks
+    def blocks_for(self, rf_len: int, first_sample: int = 0) -> int:
+        return max(0, (rf_len - first_sample - self.block_length) // self.stride + 1)
+
+    def demod_span_host(self, rf: np.ndarray, first_sample: int = 0, nblocks: Optional[int] = None,
+                        chroma: bool = True) -> DecodedSpan:
+        rf = np.ascontiguousarray(rf, dtype=np.int16)
+        if nblocks is None:
+            nblocks = self.blocks_for(rf.size, first_sample)
+        plane = nblocks * self.stride
+        out = {k: np.empty(plane, dtype=np.float32) for k in ("video", "vlp", "env")}
+        ch = np.empty(plane, dtype=np.float32) if (chroma and self._chr is not None) else None
+        flags = np.zeros(nblocks, dtype=np.uint32)
+        span = _lib.SpanV1()
+        span.struct_size = ctypes.sizeof(_lib.SpanV1)
+        span.video = _ptr(out["video"], _lib._fp)
+        span.video_lowpass = _ptr(out["vlp"], _lib._fp)
+        span.envelope = _ptr(out["env"], _lib._fp)
+        span.chroma = _ptr(ch, _lib._fp) if ch is not None else None
+        span.block_flags = _ptr(flags, _lib._u32p)
+        _lib.check(self._L.vhsdecode_hip_demod_span_host(self._h, _ptr(rf, _lib._i16p), rf.size, first_sample,
+                                                         nblocks, ctypes.byref(span)))
+        return DecodedSpan(out["video"], out["vlp"], out["env"], ch, flags)
+
+    def demod_span_device(self, rf_ptr: int, first_sample: int, nblocks: int, video_ptr: int, vlp_ptr: int,
+                          env_ptr: int, chroma_ptr: int = 0, flags_ptr: int = 0, stream: int = 0):
+        """Device-pointer entry (e.g. torch tensors' data_ptr()); asynchronous on `stream`."""
+        span = _lib.SpanV1()
+        span.struct_size = ctypes.sizeof(_lib.SpanV1)
+        span.video = ctypes.cast(ctypes.c_void_p(video_ptr), _lib._fp)
+        span.video_lowpass = ctypes.cast(ctypes.c_void_p(vlp_ptr), _lib._fp)
+        span.envelope = ctypes.cast(ctypes.c_void_p(env_ptr), _lib._fp)
+        span.chroma = ctypes.cast(ctypes.c_void_p(chroma_ptr), _lib._fp) if chroma_ptr else None
+        span.block_flags = ctypes.cast(ctypes.c_void_p(flags_ptr), _lib._u32p) if flags_ptr else None
+        _lib.check(self._L.vhsdecode_hip_demod_span_device(self._h, ctypes.c_void_p(rf_ptr), first_sample, nblocks,
+                                                           ctypes.byref(span), ctypes.c_void_p(stream)))
+
+    # ------------------------------------------------------------------ sync
+    def find_pulses(self, sync_ref_ptr: int, length: int, threshold: float, min_length: int, max_length: int,
+                    capacity: int = 65536) -> Tuple[np.ndarray, np.ndarray]:
+        starts = np.empty(capacity, dtype=np.int32)
+        lens = np.empty(capacity, dtype=np.int32)
+        count = ctypes.c_int32()
+        _lib.check(self._L.vhsdecode_hip_find_pulses(self._h, ctypes.c_void_p(sync_ref_ptr), length, threshold,
+                                                     min_length, max_length, _ptr(starts, _lib._i32p),
+                                                     _ptr(lens, _lib._i32p), capacity, ctypes.byref(count)))
+        n = count.value
+        if n > capacity:
+            raise _lib.HipError(-30007, f"{n} pulses exceed the capacity {capacity}")
+        return starts[:n].copy(), lens[:n].copy()
+
+    def refine_hsync(self, sync_ref_ptr: int, length: int, locations: np.ndarray, filled: np.ndarray,
+                     threshold: float, sample_rate_mhz: float, hsync_pulse_us: float, ire0: float, hz_ire: float,
+                     is_pal: bool, use_right_hsync: bool = True):
+        loc = np.ascontiguousarray(locations, dtype=np.float64)
+        fil = np.ascontiguousarray(filled, dtype=np.uint8)
+        ro = np.empty_like(loc)
+        fo = np.empty_like(fil)
+        p = _lib.HsyncParamsV1(ctypes.sizeof(_lib.HsyncParamsV1), threshold, sample_rate_mhz, hsync_pulse_us, ire0,
+                               hz_ire, 1 if is_pal else 0, 1 if use_right_hsync else 0)
+        _lib.check(self._L.vhsdecode_hip_refine_hsync(self._h, ctypes.c_void_p(sync_ref_ptr), length,
+                                                      _ptr(loc, _lib._dp), _ptr(fil, _lib._u8p), loc.size,
+                                                      ctypes.byref(p), _ptr(ro, _lib._dp), _ptr(fo, _lib._u8p)))
+        return ro, fo.astype(bool)
+
+    # ------------------------------------------------------------------ TBC
+    def set_sinc_table(self, table: np.ndarray):
+        t = np.ascontiguousarray(table, dtype=np.float32).reshape(-1)
+        _lib.check(self._L.vhsdecode_hip_set_sinc_table(self._h, _ptr(t, _lib._fp), t.size))
+
+    def tbc_resample(self, source_ptr: int, source_length: int, locations: np.ndarray, first_line: int,
+                     line_count: int, output_line_length: int, nominal: float = 0.0, smoothing: float = 0.0,
+                     conv: Optional[tuple] = None, want_f64: bool = True):
+        """conv = (ire0, hz_ire, output_zero, vsync_ire, output_scale) or None."""
+        loc = np.ascontiguousarray(locations, dtype=np.float64)
+        dest = output_line_length * line_count
+        u16 = np.empty(dest, dtype=np.uint16) if conv is not None else None
+        f64 = np.empty(dest, dtype=np.float64) if want_f64 else None
+        c = conv if conv is not None else (0.0, 1.0, 0, 0.0, 1.0)
+        p = _lib.TbcParamsV1(ctypes.sizeof(_lib.TbcParamsV1), output_line_length, first_line, line_count,
+                             float(nominal), float(smoothing), 1 if conv is not None else 0, float(c[0]), float(c[1]),
+                             int(c[2]), float(c[3]), float(c[4]))
+        _lib.check(self._L.vhsdecode_hip_tbc_resample(self._h, ctypes.c_void_p(source_ptr), source_length,
+                                                      _ptr(loc, _lib._dp), loc.size, ctypes.byref(p),
+                                                      _ptr(u16, _lib._u16p) if u16 is not None else None,
+                                                      _ptr(f64, _lib._dp) if f64 is not None else None))
+        return u16, f64
+
+    def debug_fft16k(self, data_ptr: int, count: int, inverse: bool = False, stream: int = 0):
+        _lib.check(self._L.vhsdecode_hip_debug_fft16k(self._h, ctypes.c_void_p(data_ptr), count,
+                                                      1 if inverse else 0, ctypes.c_void_p(stream)))
+
+
+def runtime_info(device: int = 0):
+    L = _lib.load()
+    info = _lib.RuntimeInfoV1()
+    info.struct_size = ctypes.sizeof(_lib.RuntimeInfoV1)
+    _lib.check(L.vhsdecode_hip_get_runtime_info(device, ctypes.byref(info)))
+    return {"device_count": info.device_count, "compute_units": info.compute_units, "hbm_bytes": info.hbm_bytes,
+            "lds_bytes_per_workgroup": info.lds_bytes_per_workgroup,
+            "device_name": info.device_name.decode(), "gcn_arch": info.gcn_arch.decode()}
