@@ -28,10 +28,11 @@ ENV_REL = 2e-4
 CHROMA_REL = 2e-3
 
 
-def _torch():
-    import torch
-    assert torch.cuda.is_available(), "GPU tests need a ROCm device"
-    return torch
+def _dev(a):
+    """Upload to HBM through the HIP runtime the library uses."""
+    from vhsdecode_hip.hipmem import DeviceBuffer, device_count
+    assert device_count() > 0, "GPU tests need a ROCm device"
+    return DeviceBuffer.from_array(a)
 
 
 def oracle_span(rf, fl, nblocks, first=0):
@@ -103,12 +104,10 @@ def _thresholds(fmt):
 
 
 def test_pulses_identical(case):
-    torch = _torch()
     thr, mn, mx = _thresholds(case["fmt"])
     vlp = case["gpu"].video_lowpass
-    dev = torch.from_numpy(vlp).cuda()
-    torch.cuda.synchronize()
-    st, ln = case["dem"].find_pulses(dev.data_ptr(), vlp.size, thr, mn, mx)
+    dev = _dev(vlp)
+    st, ln = case["dem"].find_pulses(dev.ptr, vlp.size, thr, mn, mx)
     # exactness of the detector on identical input
     ost, oln = O.find_pulses(vlp.astype(np.float64), thr, mn, mx)
     assert st.tolist() == ost.tolist() and ln.tolist() == oln.tolist()
@@ -127,17 +126,15 @@ def _line_locations(case):
 
 
 def test_hsync_refine_bit_identical(case):
-    torch = _torch()
     fmt = case["fmt"]
     sp = fmt.sysparams
     thr, _, _ = _thresholds(fmt)
     vlp = case["gpu"].video_lowpass
     locs = _line_locations(case) + 25.0
     filled = np.zeros(locs.size, dtype=np.uint8)
-    dev = torch.from_numpy(vlp).cuda()
-    torch.cuda.synchronize()
+    dev = _dev(vlp)
     is_pal = case["system"] == "PAL"
-    ref, fil = case["dem"].refine_hsync(dev.data_ptr(), vlp.size, locs, filled, thr, 40.0, sp["hsyncPulseUS"],
+    ref, fil = case["dem"].refine_hsync(dev.ptr, vlp.size, locs, filled, thr, 40.0, sp["hsyncPulseUS"],
                                         sp["ire0"], sp["hz_ire"], is_pal, True)
     oref, ofil = O.refine_hsync(vlp.astype(np.float64), locs, filled, thr, 40.0, sp["hsyncPulseUS"], sp["ire0"],
                                 sp["hz_ire"], is_pal, True)
@@ -159,7 +156,6 @@ def _conv(fmt):
 
 
 def test_tbc_bit_identical_and_within_one_lsb(case, sinc_lut):
-    torch = _torch()
     fmt = case["fmt"]
     dem = case["dem"]
     dem.set_sinc_table(sinc_lut)
@@ -168,9 +164,8 @@ def test_tbc_bit_identical_and_within_one_lsb(case, sinc_lut):
     out_len = int(fmt.sysparams["outlinelen"])
     lines = locs.size - 1
     conv = _conv(fmt)
-    dev = torch.from_numpy(video).cuda()
-    torch.cuda.synchronize()
-    u16, f64 = dem.tbc_resample(dev.data_ptr(), video.size, locs, 0, lines, out_len, conv=conv)
+    dev = _dev(video)
+    u16, f64 = dem.tbc_resample(dev.ptr, video.size, locs, 0, lines, out_len, conv=conv)
     ou16_same = O.tbc_resample(video.astype(np.float64), locs, 0, lines, out_len, sinc_lut, conv=conv)
     assert np.array_equal(u16, ou16_same[1])
     assert np.array_equal(f64.view(np.uint64), ou16_same[0].view(np.uint64))
@@ -181,14 +176,13 @@ def test_tbc_bit_identical_and_within_one_lsb(case, sinc_lut):
 
 
 def test_fft16k_matches_numpy(case):
-    torch = _torch()
+    from vhsdecode_hip.hipmem import synchronize
     rng = np.random.default_rng(0)
     x = (rng.standard_normal((3, 16384)) + 1j * rng.standard_normal((3, 16384))).astype(np.complex64)
-    dev = torch.from_numpy(x.view(np.float32).copy()).cuda()
-    torch.cuda.synchronize()
-    case["dem"].debug_fft16k(dev.data_ptr(), 3, inverse=False)
-    torch.cuda.synchronize()
-    got = dev.cpu().numpy().view(np.complex64).reshape(3, 16384)
+    dev = _dev(x.view(np.float32).copy())
+    case["dem"].debug_fft16k(dev.ptr, 3, inverse=False)
+    synchronize()
+    got = dev.download(np.float32, 2 * 3 * 16384).view(np.complex64).reshape(3, 16384)
     ref = np.fft.fft(x.astype(np.complex128), axis=1)
     err = np.sqrt(np.mean(np.abs(got - ref) ** 2)) / np.sqrt(np.mean(np.abs(ref) ** 2))
     print(f"fft16k relative RMS error {err:.2e}", flush=True)

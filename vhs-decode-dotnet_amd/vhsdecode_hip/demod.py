@@ -129,7 +129,7 @@ class HipBlockDemodulator:
 
     def demod_span_device(self, rf_ptr: int, first_sample: int, nblocks: int, video_ptr: int, vlp_ptr: int,
                           env_ptr: int, chroma_ptr: int = 0, flags_ptr: int = 0, stream: int = 0):
-        """Device-pointer entry (e.g. torch tensors' data_ptr()); asynchronous on `stream`."""
+        """Device-pointer entry (HBM addresses, e.g. hipmem.DeviceBuffer.ptr); asynchronous on `stream`."""
         span = _lib.SpanV1()
         span.struct_size = ctypes.sizeof(_lib.SpanV1)
         span.video = ctypes.cast(ctypes.c_void_p(video_ptr), _lib._fp)
