@@ -14,7 +14,7 @@
  *  - a context serialises its own calls; different contexts may run concurrently.
  *
  * Pointer arguments documented as "device" are HBM addresses on the context's
- * device (hipMalloc or torch allocations) and the call is asynchronous on the
+ * device (hipMalloc allocations, or any HBM allocation of the same HIP runtime) and the call is asynchronous on the
  * given stream (NULL = the context's own stream); "host" arguments are ordinary
  * process memory and the call is synchronous.
  */
@@ -201,6 +201,67 @@ VHSDECODE_HIP_API int32_t vhsdecode_hip_tbc_resample_device(vhsdecode_hip_demod_
                                                             const double* level_adjust, int32_t per_sample,
                                                             const vhsdecode_hip_tbc_params_v1* params,
                                                             uint16_t* out_u16, double* out_f64, void* stream);
+
+/* ---------------------------------------------------------------- batched field decode
+ * The per-field tail of TbcFieldDecodePipeline.DecodePrepared for a run of fields
+ * (Decode/TbcFieldDecodePipeline.cs:1344-1348 hsync refinement, then the TBC render of
+ * Tbc/TbcFieldRenderer.cs:289-340): one launch refines every field's line locations
+ * (FP64), one computes the per-field wow level adjusts (median/MAD clamp and the
+ * exponential smoothing of TbcLineResampler.cs:818-878, 997-1011) and one resamples
+ * every output pixel to uint16.  Output line 0 of each field starts at its location 0. */
+typedef struct vhsdecode_hip_field_params_v1 {
+  uint32_t struct_size;
+  double threshold;                /* sync threshold (Hz) */
+  double sample_rate_mhz;
+  double hsync_pulse_us;
+  double ire0;
+  double hz_ire;
+  int32_t is_pal;
+  int32_t use_right_hsync;
+  int32_t output_line_length;      /* TbcFrameSpec output line length */
+  int32_t field_lines;             /* output lines per field; each field supplies field_lines + 1 locations */
+  double nominal_line_length;      /* round(line_period * MHz), DecodeSession.cs:408-410 */
+  double wow_level_smoothing;      /* frame_lines / 2 for vhs (DecodeSession.cs:1029-1050); 0 disables */
+  int32_t output_zero;
+  double vsync_ire;
+  double output_scale;
+} vhsdecode_hip_field_params_v1;
+
+/* sync_ref, video: device span planes; line_locations / filled: host
+ * [nfields][field_lines + 1]; out: device [nfields][field_lines * output_line_length];
+ * refined_out: optional host [nfields][field_lines + 1] (makes the call synchronous). */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod_context* context,
+                                                             const float* sync_ref, const float* video,
+                                                             int64_t span_length, const double* line_locations,
+                                                             const uint8_t* filled, int32_t nfields,
+                                                             const vhsdecode_hip_field_params_v1* params,
+                                                             uint16_t* out, double* refined_out, void* stream);
+
+/* ---------------------------------------------------------------- observability
+ * Per-kernel device time, the counterpart of vhsdecode_cuda_fast_result_v1.elapsed_seconds
+ * (vhsdecode_cuda_fast.h:100-106) at kernel granularity.  When enabled, the context
+ * brackets each launch with HIP events on the launch stream; reading the profile
+ * synchronises the recorded events and clears them. */
+enum {
+  VHSDECODE_HIP_KERNEL_BLOCK_DEMOD = 0,
+  VHSDECODE_HIP_KERNEL_SPIKE_REPAIR = 1,
+  VHSDECODE_HIP_KERNEL_CHROMA = 2,
+  VHSDECODE_HIP_KERNEL_PULSES = 3,
+  VHSDECODE_HIP_KERNEL_HSYNC = 4,
+  VHSDECODE_HIP_KERNEL_TBC = 5,
+  VHSDECODE_HIP_KERNEL_COUNT = 6
+};
+
+typedef struct vhsdecode_hip_profile_v1 {
+  uint32_t struct_size;
+  int64_t launches[VHSDECODE_HIP_KERNEL_COUNT];
+  double total_ms[VHSDECODE_HIP_KERNEL_COUNT];
+  int64_t units[VHSDECODE_HIP_KERNEL_COUNT];  /* blocks / samples / fields / pixels processed */
+} vhsdecode_hip_profile_v1;
+
+VHSDECODE_HIP_API int32_t vhsdecode_hip_set_profiling(vhsdecode_hip_demod_context* context, int32_t enable);
+VHSDECODE_HIP_API int32_t vhsdecode_hip_get_profile(vhsdecode_hip_demod_context* context,
+                                                    vhsdecode_hip_profile_v1* profile);
 
 /* Diagnostic: forward 16384-point complex FFTs of `count` transforms (device,
  * interleaved FP32) with the library's workgroup FFT.  No reference counterpart;

@@ -464,6 +464,153 @@ __global__ void __launch_bounds__(256) tbc_resample_kernel(const float* src, lon
   }
 }
 
+// ------------------------------------------------------------------ batched field TBC
+// Per-field wow level adjusts (TbcLineResampler.BuildLinearLevelAdjusts +
+// SmoothLevelAdjusts, Tbc/TbcLineResampler.cs:818-878, 997-1011): per-line
+// derivative factors, median/MAD outlier clamp, then the exponential smoothing
+// y[i] = fma(x[i] - y[i-1], alpha, y[i-1]) over all output samples.  Within a line
+// x is constant, so the smoothed value is x + (y_before - x) * (1 - alpha)^(s+1);
+// the per-line entry values y_before are carried sequentially (one step per line)
+// and the per-sample powers come from a host table.
+constexpr int kLevelThreads = 512;
+
+__device__ double block_median_512(double* buf, int n, double pad) {
+  // bitonic sort of 512 entries (n <= 512 valid, the rest padded with +inf)
+  const int t = threadIdx.x;
+  __syncthreads();
+  buf[t] = t < n ? buf[t] : pad;
+  __syncthreads();
+  for (int k = 2; k <= kLevelThreads; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int ixj = t ^ j;
+      if (ixj > t) {
+        const double a = buf[t], b = buf[ixj];
+        const bool up = (t & k) == 0;
+        if ((a > b) == up) {
+          buf[t] = b;
+          buf[ixj] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int m = n / 2;
+  const double med = (n & 1) ? buf[m] : (buf[m - 1] + buf[m]) / 2.0;
+  __syncthreads();
+  return med;
+}
+
+// line_decay = (1 - alpha)^out_len, the smoothing carry over one whole line; 0 disables smoothing.
+__global__ void __launch_bounds__(kLevelThreads) tbc_levels_kernel(const double* locs, int nloc, int lines,
+                                                                   double nominal, double line_decay, double* line_x,
+                                                                   double* line_y0) {
+  __shared__ double buf[kLevelThreads];
+  __shared__ double fac[kLevelThreads];
+  __shared__ int nan_flag;
+  const int f = blockIdx.x;
+  const int t = threadIdx.x;
+  const double* loc = locs + (long long)f * nloc;
+  const double inv = 1.0 / nominal;
+  if (t == 0) nan_flag = 0;
+  __syncthreads();
+  double v = 0.0;
+  if (t < lines) {
+    int left = t < nloc - 2 ? t : nloc - 2;
+    {
+#pragma clang fp contract(off)
+      v = (loc[left + 1] * inv) - (loc[left] * inv);
+    }
+    if (v != v) atomicOr(&nan_flag, 1);
+    fac[t] = v;
+  }
+  buf[t] = v;
+  const double med = block_median_512(buf, lines, INFINITY);
+  double dv = 0.0;
+  if (t < lines) dv = fabs(fac[t] - med);
+  buf[t] = dv;
+  const double mad = block_median_512(buf, lines, INFINITY);
+  const bool any_nan = nan_flag != 0;
+  const double m0 = any_nan ? NAN : med;
+  const double mad0 = any_nan ? NAN : mad;
+  const double thr = mad0 > 0.0 ? 15.0 * mad0 : 0.001;
+  if (t < lines) {
+    const double x = fabs(fac[t] - m0) > thr ? m0 : fac[t];
+    line_x[(long long)f * lines + t] = x;
+    fac[t] = x;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double* y0 = line_y0 + (long long)f * lines;
+    if (line_decay > 0.0) {
+      // entry value of each line; line 0's first sample is the unsmoothed factor
+      double prev = fac[0];
+      for (int l = 0; l < lines; l++) {
+        y0[l] = prev;
+        prev = fac[l] + (prev - fac[l]) * line_decay;
+      }
+    } else {
+      for (int l = 0; l < lines; l++) y0[l] = fac[l];
+    }
+  }
+}
+
+struct FieldTbcParams {
+  int out_len;
+  int lines;        // output lines per field (first_line = 0)
+  int nloc;         // line locations per field (lines + 1)
+  double nominal;
+  int smooth;       // 1: apply the smoothing powers table
+  double ire0;
+  double out_scale;
+  double inv_hz_ire;
+  double conv_offset;
+};
+
+__global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long long src_len, const double* locs,
+                                                         const double* line_x, const double* line_y0,
+                                                         const double* pow_table, FieldTbcParams p,
+                                                         const float* __restrict__ lut, unsigned short* out) {
+  const int f = blockIdx.y;
+  const long long d = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long dest_len = (long long)p.out_len * p.lines;
+  if (d >= dest_len) return;
+  const double* loc = locs + (long long)f * p.nloc;
+  const int left = (int)(d / p.out_len);
+  const int smp = (int)(d - (long long)left * p.out_len);
+  const double ll = loc[left];
+  const double delta = loc[left + 1] - ll;
+  const double scale = p.nominal / p.out_len;
+  const double sp = (double)d * scale;
+  double frac = (sp - (left * p.nominal)) / p.nominal;
+  frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
+  double pos, adj;
+  const double x = line_x[(long long)f * p.lines + left];
+  {
+#pragma clang fp contract(off)
+    pos = ll + (delta * frac);
+    adj = p.smooth ? x + (line_y0[(long long)f * p.lines + left] - x) * pow_table[smp] : x;
+  }
+  double r;
+  {
+#pragma clang fp contract(off)
+    r = (double)(float)(sample_sinc(src, src_len, pos, lut) * adj);
+  }
+  double sd;
+  {
+#pragma clang fp contract(off)
+    sd = p.out_scale * (r - p.ire0);
+  }
+  const double v = __fma_rn(sd, p.inv_hz_ire, p.conv_offset);
+  unsigned short o;
+  if (v <= 0.0)
+    o = 0;
+  else if (v >= 65535.0)
+    o = 65535;
+  else
+    o = (unsigned short)v;
+  out[(long long)f * dest_len + d] = o;
+}
+
 }  // namespace vhsd
 
 extern "C" {
@@ -490,6 +637,22 @@ hipError_t vhsd_launch_hsync(const float* const* spans, const long long* span_le
                              hipStream_t stream) {
   hipLaunchKernelGGL(vhsd::hsync_refine_kernel, dim3(fields), dim3(64), 0, stream, spans, span_len, locs, filled,
                      line_counts, *p, refined_out, filled_out);
+  return hipGetLastError();
+}
+
+hipError_t vhsd_launch_tbc_fields(const float* src, long long src_len, const double* refined, int nloc, int lines,
+                                  int fields, double nominal, double line_decay, const double* pow_table,
+                                  double* line_x, double* line_y0, int out_len, int smooth, double ire0,
+                                  double out_scale, double inv_hz_ire, double conv_offset, const float* lut,
+                                  unsigned short* out, hipStream_t stream) {
+  if (fields <= 0) return hipSuccess;
+  if (lines > vhsd::kLevelThreads || nloc < 2 || lines > nloc - 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(vhsd::tbc_levels_kernel, dim3(fields), dim3(vhsd::kLevelThreads), 0, stream, refined, nloc,
+                     lines, nominal, smooth ? line_decay : 0.0, line_x, line_y0);
+  vhsd::FieldTbcParams p{out_len, lines, nloc, nominal, smooth, ire0, out_scale, inv_hz_ire, conv_offset};
+  const long long dest = (long long)out_len * lines;
+  hipLaunchKernelGGL(vhsd::tbc_fields_kernel, dim3((unsigned)((dest + 255) / 256), fields), dim3(256), 0, stream, src,
+                     src_len, refined, line_x, line_y0, pow_table, p, lut, out);
   return hipGetLastError();
 }
 

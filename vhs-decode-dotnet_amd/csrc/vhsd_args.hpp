@@ -81,6 +81,11 @@ hipError_t vhsd_launch_hsync(const float* const* spans, const long long* span_le
                              const unsigned char* const* filled, const int* line_counts, const vhsd::HsyncParams* p,
                              double* const* refined_out, unsigned char* const* filled_out, int fields,
                              hipStream_t stream);
+hipError_t vhsd_launch_tbc_fields(const float* src, long long src_len, const double* refined, int nloc, int lines,
+                                  int fields, double nominal, double line_decay, const double* pow_table,
+                                  double* line_x, double* line_y0, int out_len, int smooth, double ire0,
+                                  double out_scale, double inv_hz_ire, double conv_offset, const float* lut,
+                                  unsigned short* out, hipStream_t stream);
 hipError_t vhsd_launch_tbc(const float* src, long long src_len, const double* locs, const double* level_adjust,
                            int level_per_line, const vhsd::TbcParams* p, const float* lut, unsigned short* out_u16,
                            double* out_f64, hipStream_t stream);

@@ -232,10 +232,29 @@ struct vhsdecode_hip_demod_context {
   bool lut_ready = false;
   DevBuf<double> t_loc, t_adj, t_f64;
   DevBuf<unsigned short> t_u16;
+  // batched fields
+  DevBuf<double> f_loc, f_ref, f_x, f_y0, f_pow;
+  DevBuf<unsigned char> f_fil, f_fo, f_tab;
+  double f_pow_alpha = -1.0;
+  int f_pow_len = 0;
+  void* f_stage = nullptr;   // pinned host staging for the location upload
+  size_t f_stage_bytes = 0;
+  hipEvent_t f_stage_done = nullptr;
   // host-variant staging
   DevBuf<int16_t> s_rf;
   DevBuf<float> s_planes;
   DevBuf<unsigned> s_flags;
+  // profiling: event pairs per kernel kind
+  bool profiling = false;
+  struct Rec {
+    int kind;
+    hipEvent_t a, b;
+    long long units;
+  };
+  std::vector<Rec> recs;
+  long long prof_launches[VHSDECODE_HIP_KERNEL_COUNT] = {};
+  double prof_ms[VHSDECODE_HIP_KERNEL_COUNT] = {};
+  long long prof_units[VHSDECODE_HIP_KERNEL_COUNT] = {};
 
   ~vhsdecode_hip_demod_context() {
     (void)hipSetDevice(device);
@@ -245,11 +264,41 @@ struct vhsdecode_hip_demod_context {
     p_lens.release(); edge_pos.release(); edge_kind.release(); h_loc.release(); h_ref.release();
     h_fil.release(); h_fo.release(); h_ptrs.release(); lut.release(); t_loc.release(); t_adj.release();
     t_f64.release(); t_u16.release(); s_rf.release(); s_planes.release(); s_flags.release();
+    f_loc.release(); f_ref.release(); f_x.release(); f_y0.release(); f_pow.release(); f_fil.release();
+    f_fo.release(); f_tab.release();
+    if (f_stage) (void)hipHostFree(f_stage);
+    if (f_stage_done) (void)hipEventDestroy(f_stage_done);
+    for (auto& r : recs) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
 namespace {
+
+// RAII bracket of one launch with HIP events when profiling is enabled.
+struct ProfScope {
+  vhsdecode_hip_demod_context* c;
+  hipStream_t st;
+  int kind;
+  long long units;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(vhsdecode_hip_demod_context* ctx, hipStream_t s, int k, long long u) : c(ctx), st(s), kind(k), units(u) {
+    if (!c->profiling) return;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      a = b = nullptr;
+      return;
+    }
+    (void)hipEventRecord(a, st);
+  }
+  ~ProfScope() {
+    if (!a) return;
+    (void)hipEventRecord(b, st);
+    c->recs.push_back({kind, a, b, units});
+  }
+};
 
 hipStream_t pick_stream(vhsdecode_hip_demod_context* c, void* stream) {
   return stream ? reinterpret_cast<hipStream_t>(stream) : c->stream;
@@ -310,11 +359,15 @@ int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int
   a.out_vlp = out->video_lowpass;
   a.out_env = out->envelope;
   a.flags = flags;
-  VHSD_HIP(vhsd_launch_block_demod(&a, nblocks, st));
+  {
+    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_BLOCK_DEMOD, nblocks);
+    VHSD_HIP(vhsd_launch_block_demod(&a, nblocks, st));
+  }
   if (c->diff_max > 0.0) {
     VHSD_HIP(c->rep_a.ensure((size_t)nblocks * 32768));
     VHSD_HIP(c->rep_dm.ensure((size_t)nblocks * 65536));
     vhsd::RepairArgs r{a, c->rep_a.p, c->rep_dm.p};
+    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_SPIKE_REPAIR, nblocks);
     VHSD_HIP(vhsd_launch_block_repair(&r, nblocks, st));
   }
   if (c->has_chroma && out->chroma) {
@@ -328,6 +381,7 @@ int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int
     ch.sos = c->chroma;
     ch.sos.trans = c->chroma_trans.p;
     ch.out_chroma = out->chroma;
+    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_CHROMA, nblocks);
     VHSD_HIP(vhsd_launch_block_chroma(&ch, nblocks, st));
   }
   return VHSDECODE_HIP_OK;
@@ -528,6 +582,7 @@ int32_t vhsdecode_hip_find_pulses(vhsdecode_hip_demod_context* c, const float* s
     VHSD_HIP(c->p_starts.ensure(std::max(capacity, 1)));
     VHSD_HIP(c->p_lens.ensure(std::max(capacity, 1)));
     VHSD_HIP(hipMemsetAsync(c->counters.p, 0, 2 * sizeof(int), st));
+    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_PULSES, length);
     VHSD_HIP(vhsd_launch_pulses(sync_ref, length, threshold, min_length, max_length, c->tile_counts.p, tiles,
                                 c->edge_pos.p, c->edge_kind.p, (int)edge_cap, c->counters.p, c->p_starts.p,
                                 c->p_lens.p, capacity, st));
@@ -581,7 +636,10 @@ int32_t vhsdecode_hip_refine_hsync(vhsdecode_hip_demod_context* c, const float* 
     Table* dt = reinterpret_cast<Table*>(c->h_ptrs.p);
     vhsd::HsyncParams p{params->threshold, params->sample_rate_mhz, params->hsync_pulse_us, params->ire0,
                         params->hz_ire, params->is_pal, params->use_right_hsync};
-    VHSD_HIP(vhsd_launch_hsync(&dt->span, &dt->len, &dt->loc, &dt->fil, &dt->lines, &p, &dt->ref, &dt->fo, 1, st));
+    {
+      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_HSYNC, 1);
+      VHSD_HIP(vhsd_launch_hsync(&dt->span, &dt->len, &dt->loc, &dt->fil, &dt->lines, &p, &dt->ref, &dt->fo, 1, st));
+    }
     VHSD_HIP(hipMemcpyAsync(refined, c->h_ref.p, sizeof(double) * lines, hipMemcpyDeviceToHost, st));
     VHSD_HIP(hipMemcpyAsync(filled_out, c->h_fo.p, lines, hipMemcpyDeviceToHost, st));
     VHSD_HIP(hipStreamSynchronize(st));
@@ -682,8 +740,11 @@ int32_t vhsdecode_hip_tbc_resample(vhsdecode_hip_demod_context* c, const float* 
     VHSD_HIP(hipMemcpyAsync(c->t_loc.p, locations, sizeof(double) * location_count, hipMemcpyHostToDevice, st));
     if (!adj.empty())
       VHSD_HIP(hipMemcpyAsync(c->t_adj.p, adj.data(), sizeof(double) * adj.size(), hipMemcpyHostToDevice, st));
-    VHSD_HIP(vhsd_launch_tbc(source, source_length, c->t_loc.p, c->t_adj.p, per_sample ? 0 : 1, &t, c->lut.p,
-                             out_u16 ? c->t_u16.p : nullptr, out_f64 ? c->t_f64.p : nullptr, st));
+    {
+      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_TBC, (long long)dest);
+      VHSD_HIP(vhsd_launch_tbc(source, source_length, c->t_loc.p, c->t_adj.p, per_sample ? 0 : 1, &t, c->lut.p,
+                               out_u16 ? c->t_u16.p : nullptr, out_f64 ? c->t_f64.p : nullptr, st));
+    }
     if (out_u16 && dest)
       VHSD_HIP(hipMemcpyAsync(out_u16, c->t_u16.p, sizeof(uint16_t) * dest, hipMemcpyDeviceToHost, st));
     if (out_f64 && dest)
@@ -712,8 +773,162 @@ int32_t vhsdecode_hip_tbc_resample_device(vhsdecode_hip_demod_context* c, const 
     if (rc) return rc;
     vhsd::TbcParams t{};
     if ((rc = tbc_params(params, params->nominal_line_length, &t)) != 0) return rc;
+    hipStream_t st = pick_stream(c, stream);
+    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_TBC, (long long)params->output_line_length * params->line_count);
     VHSD_HIP(vhsd_launch_tbc(source, source_length, locations, level_adjust, per_sample ? 0 : 1, &t, c->lut.p,
-                             out_u16, out_f64, pick_stream(c, stream)));
+                             out_u16, out_f64, st));
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod_context* c, const float* sync_ref, const float* video,
+                                           int64_t span_length, const double* line_locations, const uint8_t* filled,
+                                           int32_t nfields, const vhsdecode_hip_field_params_v1* p, uint16_t* out,
+                                           double* refined_out, void* stream) {
+  try {
+    if (!c || !sync_ref || !video || !line_locations || !filled || !p || !out || nfields < 0 || span_length <= 0)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid field decode request");
+    if (p->struct_size != sizeof(vhsdecode_hip_field_params_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "field_params struct_size mismatch");
+    if (p->field_lines <= 0 || p->field_lines > 511 || p->output_line_length <= 0 || !(p->nominal_line_length > 0.0))
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid field geometry");
+    if (nfields == 0) return VHSDECODE_HIP_OK;
+    const int nloc = p->field_lines + 1;
+    const size_t nl = (size_t)nfields * nloc;
+    for (size_t i = 0; i < nl; i++)
+      if (!std::isfinite(line_locations[i]))
+        return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "line locations must be finite");
+    std::lock_guard<std::mutex> lock(c->mu);
+    VHSD_HIP(hipSetDevice(c->device));
+    int32_t rc = ensure_lut(c);
+    if (rc) return rc;
+    hipStream_t st = pick_stream(c, stream);
+    VHSD_HIP(c->f_loc.ensure(nl));
+    VHSD_HIP(c->f_ref.ensure(nl));
+    VHSD_HIP(c->f_fil.ensure(nl));
+    VHSD_HIP(c->f_fo.ensure(nl));
+    VHSD_HIP(c->f_x.ensure((size_t)nfields * p->field_lines));
+    VHSD_HIP(c->f_y0.ensure((size_t)nfields * p->field_lines));
+    // per-field pointer tables for the batched hsync kernel
+    const size_t tab_bytes = (size_t)nfields * 52;
+    VHSD_HIP(c->f_tab.ensure(tab_bytes));
+    // smoothing powers table: pow[s] = (1 - alpha)^(s + 1), s < output_line_length
+    const bool smooth = p->wow_level_smoothing > 0.0;
+    const double alpha = smooth ? 1.0 / (p->wow_level_smoothing * p->output_line_length) : 0.0;
+    if (smooth && (c->f_pow_alpha != alpha || c->f_pow_len != p->output_line_length)) {
+      std::vector<double> pw(p->output_line_length);
+      for (int s2 = 0; s2 < p->output_line_length; s2++) pw[s2] = std::pow(1.0 - alpha, (double)(s2 + 1));
+      VHSD_HIP(c->f_pow.ensure(pw.size()));
+      VHSD_HIP(hipMemcpy(c->f_pow.p, pw.data(), sizeof(double) * pw.size(), hipMemcpyHostToDevice));
+      c->f_pow_alpha = alpha;
+      c->f_pow_len = p->output_line_length;
+    }
+    const double line_decay = smooth ? std::pow(1.0 - alpha, (double)p->output_line_length) : 0.0;
+    // stage locations, flags and pointer tables in pinned memory, one async upload
+    const size_t bytes = nl * sizeof(double) + nl + tab_bytes + 64;
+    if (c->f_stage_done) VHSD_HIP(hipEventSynchronize(c->f_stage_done));
+    if (bytes > c->f_stage_bytes) {
+      if (c->f_stage) (void)hipHostFree(c->f_stage);
+      c->f_stage = nullptr;
+      VHSD_HIP(hipHostMalloc(&c->f_stage, bytes, hipHostMallocDefault));
+      c->f_stage_bytes = bytes;
+    }
+    if (!c->f_stage_done) VHSD_HIP(hipEventCreateWithFlags(&c->f_stage_done, hipEventDisableTiming));
+    unsigned char* stg = reinterpret_cast<unsigned char*>(c->f_stage);
+    std::memcpy(stg, line_locations, nl * sizeof(double));
+    std::memcpy(stg + nl * sizeof(double), filled, nl);
+    unsigned char* tab = stg + nl * sizeof(double) + nl;
+    tab = reinterpret_cast<unsigned char*>((reinterpret_cast<uintptr_t>(tab) + 15) & ~uintptr_t(15));
+    const size_t tab_off = (size_t)(tab - stg);
+    // table layout (byte offsets, 8-byte slots): spans, lens, locs, fil, ref, fo, then int line counts
+    const size_t o_sp = 0, o_ln = 8 * (size_t)nfields, o_lc = 16 * (size_t)nfields, o_fi = 24 * (size_t)nfields,
+                 o_rf = 32 * (size_t)nfields, o_fo = 40 * (size_t)nfields, o_cnt = 48 * (size_t)nfields;
+    const size_t used_tab = o_cnt + 4 * (size_t)nfields;
+    if (tab_off + used_tab > bytes) return fail(VHSDECODE_HIP_E_INTERNAL, "staging overflow");
+    for (int f = 0; f < nfields; f++) {
+      const void* vp[6] = {sync_ref, nullptr, c->f_loc.p + (size_t)f * nloc, c->f_fil.p + (size_t)f * nloc,
+                           c->f_ref.p + (size_t)f * nloc, c->f_fo.p + (size_t)f * nloc};
+      const long long len = span_length;
+      std::memcpy(tab + o_sp + 8 * (size_t)f, &vp[0], 8);
+      std::memcpy(tab + o_ln + 8 * (size_t)f, &len, 8);
+      std::memcpy(tab + o_lc + 8 * (size_t)f, &vp[2], 8);
+      std::memcpy(tab + o_fi + 8 * (size_t)f, &vp[3], 8);
+      std::memcpy(tab + o_rf + 8 * (size_t)f, &vp[4], 8);
+      std::memcpy(tab + o_fo + 8 * (size_t)f, &vp[5], 8);
+      std::memcpy(tab + o_cnt + 4 * (size_t)f, &nloc, 4);
+    }
+    VHSD_HIP(hipMemcpyAsync(c->f_loc.p, stg, nl * sizeof(double), hipMemcpyHostToDevice, st));
+    VHSD_HIP(hipMemcpyAsync(c->f_fil.p, stg + nl * sizeof(double), nl, hipMemcpyHostToDevice, st));
+    VHSD_HIP(hipMemcpyAsync(c->f_tab.p, tab, used_tab, hipMemcpyHostToDevice, st));
+    VHSD_HIP(hipEventRecord(c->f_stage_done, st));
+    unsigned char* dtab = c->f_tab.p;
+    const float* const* d_spans = reinterpret_cast<const float* const*>(dtab + o_sp);
+    const long long* d_lens = reinterpret_cast<const long long*>(dtab + o_ln);
+    const double* const* d_locs = reinterpret_cast<const double* const*>(dtab + o_lc);
+    const unsigned char* const* d_fils = reinterpret_cast<const unsigned char* const*>(dtab + o_fi);
+    double* const* d_refs = reinterpret_cast<double* const*>(dtab + o_rf);
+    unsigned char* const* d_fos = reinterpret_cast<unsigned char* const*>(dtab + o_fo);
+    const int* d_lcount = reinterpret_cast<const int*>(dtab + o_cnt);
+    vhsd::HsyncParams hp{p->threshold, p->sample_rate_mhz, p->hsync_pulse_us, p->ire0, p->hz_ire, p->is_pal,
+                         p->use_right_hsync};
+    {
+      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_HSYNC, nfields);
+      VHSD_HIP(vhsd_launch_hsync(d_spans, d_lens, d_locs, d_fils, d_lcount, &hp, d_refs, d_fos, nfields, st));
+    }
+    const double out_scale = p->output_scale;
+    const double conv_off = (p->output_zero + 0.5) - (p->vsync_ire * out_scale);
+    {
+      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_TBC, (long long)nfields * p->field_lines * p->output_line_length);
+      VHSD_HIP(vhsd_launch_tbc_fields(video, span_length, c->f_ref.p, nloc, p->field_lines, nfields,
+                                      p->nominal_line_length, line_decay, smooth ? c->f_pow.p : nullptr, c->f_x.p,
+                                      c->f_y0.p, p->output_line_length, smooth ? 1 : 0, p->ire0, out_scale,
+                                      1.0 / p->hz_ire, conv_off, c->lut.p, out, st));
+    }
+    if (refined_out) {
+      VHSD_HIP(hipMemcpyAsync(refined_out, c->f_ref.p, nl * sizeof(double), hipMemcpyDeviceToHost, st));
+      VHSD_HIP(hipStreamSynchronize(st));
+    }
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_set_profiling(vhsdecode_hip_demod_context* c, int32_t enable) {
+  if (!c) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "context is NULL");
+  std::lock_guard<std::mutex> lock(c->mu);
+  c->profiling = enable != 0;
+  return VHSDECODE_HIP_OK;
+}
+
+int32_t vhsdecode_hip_get_profile(vhsdecode_hip_demod_context* c, vhsdecode_hip_profile_v1* p) {
+  try {
+    if (!c || !p) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "context and profile are required");
+    if (p->struct_size != sizeof(vhsdecode_hip_profile_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "profile struct_size mismatch");
+    std::lock_guard<std::mutex> lock(c->mu);
+    VHSD_HIP(hipSetDevice(c->device));
+    for (auto& r : c->recs) {
+      float ms = 0.0f;
+      VHSD_HIP(hipEventSynchronize(r.b));
+      VHSD_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      c->prof_launches[r.kind] += 1;
+      c->prof_ms[r.kind] += ms;
+      c->prof_units[r.kind] += r.units;
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    c->recs.clear();
+    for (int k = 0; k < VHSDECODE_HIP_KERNEL_COUNT; k++) {
+      p->launches[k] = c->prof_launches[k];
+      p->total_ms[k] = c->prof_ms[k];
+      p->units[k] = c->prof_units[k];
+      c->prof_launches[k] = 0;
+      c->prof_ms[k] = 0.0;
+      c->prof_units[k] = 0;
+    }
     return VHSDECODE_HIP_OK;
   } catch (const std::exception& e) {
     return fail(VHSDECODE_HIP_E_INTERNAL, e.what());

@@ -28,8 +28,13 @@ REQUIRED_EXPORTS = (
     "vhsdecode_hip_set_sinc_table",
     "vhsdecode_hip_tbc_resample",
     "vhsdecode_hip_tbc_resample_device",
+    "vhsdecode_hip_decode_fields_device",
+    "vhsdecode_hip_set_profiling",
+    "vhsdecode_hip_get_profile",
     "vhsdecode_hip_debug_fft16k",
 )
+
+KERNEL_NAMES = ("block_demod", "spike_repair", "chroma", "pulses", "hsync", "tbc")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -81,6 +86,21 @@ class TbcParamsV1(ctypes.Structure):
                 ("output_zero", ctypes.c_int32), ("vsync_ire", ctypes.c_double), ("output_scale", ctypes.c_double)]
 
 
+class FieldParamsV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("threshold", ctypes.c_double),
+                ("sample_rate_mhz", ctypes.c_double), ("hsync_pulse_us", ctypes.c_double),
+                ("ire0", ctypes.c_double), ("hz_ire", ctypes.c_double), ("is_pal", ctypes.c_int32),
+                ("use_right_hsync", ctypes.c_int32), ("output_line_length", ctypes.c_int32),
+                ("field_lines", ctypes.c_int32), ("nominal_line_length", ctypes.c_double),
+                ("wow_level_smoothing", ctypes.c_double), ("output_zero", ctypes.c_int32),
+                ("vsync_ire", ctypes.c_double), ("output_scale", ctypes.c_double)]
+
+
+class ProfileV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("launches", ctypes.c_int64 * 6), ("total_ms", ctypes.c_double * 6),
+                ("units", ctypes.c_int64 * 6)]
+
+
 _lib = None
 
 
@@ -123,6 +143,12 @@ def load():
                                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                                     ctypes.POINTER(TbcParamsV1), ctypes.c_void_p, ctypes.c_void_p,
                                                     ctypes.c_void_p]
+    L.vhsdecode_hip_decode_fields_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                     ctypes.c_int64, _dp, _u8p, ctypes.c_int32,
+                                                     ctypes.POINTER(FieldParamsV1), ctypes.c_void_p, _dp,
+                                                     ctypes.c_void_p]
+    L.vhsdecode_hip_set_profiling.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    L.vhsdecode_hip_get_profile.argtypes = [ctypes.c_void_p, ctypes.POINTER(ProfileV1)]
     L.vhsdecode_hip_debug_fft16k.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                              ctypes.c_void_p]
     for name in REQUIRED_EXPORTS:

@@ -191,6 +191,35 @@ class HipBlockDemodulator:
                                                       _ptr(f64, _lib._dp) if f64 is not None else None))
         return u16, f64
 
+    def decode_fields_device(self, sync_ref_ptr: int, video_ptr: int, span_length: int, locations: np.ndarray,
+                             filled: np.ndarray, geom, out_ptr: int, want_refined: bool = False, stream: int = 0):
+        """Batched hsync refinement + level adjusts + TBC for fields [nfields, lines + 1] of line locations;
+        uint16 luma to device memory at out_ptr ([nfields, lines * out_len])."""
+        loc = np.ascontiguousarray(locations, dtype=np.float64)
+        fil = np.ascontiguousarray(filled, dtype=np.uint8)
+        nfields = loc.shape[0]
+        ire0, hz_ire, ozero, vsync, oscale = geom.conv
+        p = _lib.FieldParamsV1(ctypes.sizeof(_lib.FieldParamsV1), geom.threshold, geom.fs_mhz,
+                               geom.hsync / geom.fs_mhz, ire0, hz_ire, 1 if geom.is_pal else 0, 1, geom.out_len,
+                               geom.field_lines, geom.nominal, geom.smoothing, ozero, vsync, oscale)
+        ref = np.empty_like(loc) if want_refined else None
+        _lib.check(self._L.vhsdecode_hip_decode_fields_device(
+            self._h, ctypes.c_void_p(sync_ref_ptr), ctypes.c_void_p(video_ptr), span_length, _ptr(loc, _lib._dp),
+            _ptr(fil, _lib._u8p), nfields, ctypes.byref(p), ctypes.c_void_p(out_ptr),
+            _ptr(ref, _lib._dp) if ref is not None else None, ctypes.c_void_p(stream)))
+        return ref
+
+    def set_profiling(self, enable: bool = True):
+        _lib.check(self._L.vhsdecode_hip_set_profiling(self._h, 1 if enable else 0))
+
+    def take_profile(self) -> dict:
+        """Per-kernel launches, device ms and units since the last call (HIP events)."""
+        p = _lib.ProfileV1()
+        p.struct_size = ctypes.sizeof(_lib.ProfileV1)
+        _lib.check(self._L.vhsdecode_hip_get_profile(self._h, ctypes.byref(p)))
+        return {name: {"launches": p.launches[i], "ms": p.total_ms[i], "units": p.units[i]}
+                for i, name in enumerate(_lib.KERNEL_NAMES)}
+
     def debug_fft16k(self, data_ptr: int, count: int, inverse: bool = False, stream: int = 0):
         _lib.check(self._L.vhsdecode_hip_debug_fft16k(self._h, ctypes.c_void_p(data_ptr), count,
                                                       1 if inverse else 0, ctypes.c_void_p(stream)))
