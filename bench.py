@@ -68,7 +68,8 @@ def cpu_baseline(fmt, tape, fl, nb, lut, seconds: float, threads: int):
         def one_field(f):
             ref, _ = O.refine_hsync(vlp, f.locations, f.filled.astype(np.uint8), g.threshold, g.fs_mhz,
                                     g.hsync / g.fs_mhz, g.conv[0], g.conv[1], g.is_pal, True)
-            return O.tbc_resample(video, ref, 0, g.field_lines, g.out_len, lut, nominal=g.nominal,
+            og = field.field_origin(f.line0, g)
+            return O.tbc_resample(video[og:], ref - og, 0, g.field_lines, g.out_len, lut, nominal=g.nominal,
                                   smoothing=g.smoothing, conv=g.conv)[1]
 
         return len(list(pool.map(one_field, fields)))

@@ -95,6 +95,9 @@ typedef struct vhsdecode_hip_filters_v1 {
   int32_t chroma_sections;
   int32_t chroma_offset;           /* DecodeFilterSet.ChromaOffsetSamples */
   double diff_demod_max;           /* DiffDemodRepairOptions.MaxValue; <= 0 disables the repair */
+  double plane_bias_hz;            /* video and video_lowpass planes hold FP32 (value - plane_bias_hz); an
+                                      FP32 plane at the carrier (~4 MHz) would keep only 1/4-1/2 Hz, offsets
+                                      near a mid-level carrier keep 1/16 Hz.  0 writes absolute Hz. */
 } vhsdecode_hip_filters_v1;
 
 typedef struct vhsdecode_hip_demod_config_v1 {
@@ -107,7 +110,9 @@ typedef struct vhsdecode_hip_demod_config_v1 {
 } vhsdecode_hip_demod_config_v1;
 
 /* Output planes of a stitched span: nblocks * stride samples each
- * (stride = block_length - block_cut - block_cut_end).  chroma may be NULL. */
+ * (stride = block_length - block_cut - block_cut_end).  chroma may be NULL.
+ * video and video_lowpass are offsets from the context's plane_bias_hz; the sync,
+ * hsync and TBC entries below take planes in that form and add the bias in FP64. */
 typedef struct vhsdecode_hip_span_v1 {
   uint32_t struct_size;
   float* video;
@@ -228,12 +233,16 @@ typedef struct vhsdecode_hip_field_params_v1 {
 } vhsdecode_hip_field_params_v1;
 
 /* sync_ref, video: device span planes; line_locations / filled: host
- * [nfields][field_lines + 1]; out: device [nfields][field_lines * output_line_length];
+ * [nfields][field_lines + 1]; origins: host [nfields] or NULL — each field's TBC source
+ * coordinates count from span position origins[f] (the start of the field's read window,
+ * as TbcLineResampler sees it; DecodeReadWindowPlanner.Resolve, Decode/DecodeReadWindowPlanner.cs:37-63);
+ * out: device [nfields][field_lines * output_line_length];
  * refined_out: optional host [nfields][field_lines + 1] (makes the call synchronous). */
 VHSDECODE_HIP_API int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod_context* context,
                                                              const float* sync_ref, const float* video,
                                                              int64_t span_length, const double* line_locations,
-                                                             const uint8_t* filled, int32_t nfields,
+                                                             const uint8_t* filled, const int64_t* origins,
+                                                             int32_t nfields,
                                                              const vhsdecode_hip_field_params_v1* params,
                                                              uint16_t* out, double* refined_out, void* stream);
 

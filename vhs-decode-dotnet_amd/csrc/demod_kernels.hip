@@ -144,9 +144,12 @@ __device__ __forceinline__ void analytic_demod(c32* F, c32 fm, BlockLds& s, floa
 // dem[2m], dem[2m+1] (samples 2k, 2k+1, k = j + 1024 m).
 __device__ __forceinline__ void video_stage(const float* dem, BlockLds& s, const DemodArgs& a, long long out_base) {
   const int j = threadIdx.x;
+  // The ~4 MHz carrier level would dominate the FP32 rounding of the transforms; remove
+  // the plane bias first (exact: both are near the same binade) and restore the DC the
+  // filters pass as a constant correction, leaving offsets from the bias in the planes.
   c32 D[16];
 #pragma unroll
-  for (int m = 0; m < 16; m++) D[m] = c32{dem[2 * m], dem[2 * m + 1]};
+  for (int m = 0; m < 16; m++) D[m] = c32{dem[2 * m] - a.bias, dem[2 * m + 1] - a.bias};
   fft16k_forward(D, s.buf, s.oct);
   c32 dm;
   rfft_post(D, s, dm);
@@ -156,6 +159,7 @@ __device__ __forceinline__ void video_stage(const float* dem, BlockLds& s, const
     const float2* filt = pass == 0 ? a.video : a.vlp05;
     float* out = pass == 0 ? a.out_video : a.out_vlp;
     const int shift = pass == 0 ? 0 : a.vlp_offset;
+    const float corr = pass == 0 ? a.video_corr : a.vlp_corr;
     c32 v[16];
 #pragma unroll
     for (int m = 0; m < 16; m++) {
@@ -173,7 +177,7 @@ __device__ __forceinline__ void video_stage(const float* dem, BlockLds& s, const
       const int n = 2 * k - shift;  // even
       const int o = n - a.out_begin;  // even: out_begin, shift and out_base are even
       if (o >= 0 && o < a.out_count) {
-        const float2 val = make_float2(v[m].x * inv_n, v[m].y * inv_n);
+        const float2 val = make_float2(v[m].x * inv_n + corr, v[m].y * inv_n + corr);
         if (o + 1 < a.out_count)
           *reinterpret_cast<float2*>(out + out_base + o) = val;
         else

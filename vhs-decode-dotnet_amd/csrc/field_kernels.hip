@@ -17,35 +17,35 @@ namespace vhsd {
 constexpr int kPulseTile = 4096;   // samples per workgroup
 constexpr int kPulseThreads = 256;
 
-__device__ __forceinline__ int in_pulse_at(const float* s, long long i, double high) {
+__device__ __forceinline__ int in_pulse_at(const float* s, long long i, double bias, double high) {
   // state after processing sample i
   for (long long k = i; k >= 0; k--) {
     const float v = s[k];
-    if (v == v) return (double)v <= high ? 1 : 0;
+    if (v == v) return ((double)v + bias) <= high ? 1 : 0;
   }
   return 0;  // all NaN from the start: the initial state `s[0] <= high` is false
 }
 
 // edge code at sample i >= 1: 0 none, 1 start, 2 end
-__device__ __forceinline__ int edge_at(const float* s, long long i, double high) {
+__device__ __forceinline__ int edge_at(const float* s, long long i, double bias, double high) {
   const float v = s[i];
   if (v != v) return 0;
-  const int cur = (double)v <= high ? 1 : 0;
+  const int cur = ((double)v + bias) <= high ? 1 : 0;
   const float p = s[i - 1];
-  const int prev = (p == p) ? ((double)p <= high ? 1 : 0) : in_pulse_at(s, i - 1, high);
+  const int prev = (p == p) ? (((double)p + bias) <= high ? 1 : 0) : in_pulse_at(s, i - 1, bias, high);
   if (!prev && cur) return 1;
   if (prev && !cur) return 2;
   return 0;
 }
 
-__global__ void __launch_bounds__(kPulseThreads) pulse_count_kernel(const float* s, long long n, double high,
-                                                                    int* tile_counts) {
+__global__ void __launch_bounds__(kPulseThreads) pulse_count_kernel(const float* s, long long n, double bias,
+                                                                    double high, int* tile_counts) {
   __shared__ int wave_counts[kPulseThreads / 64];
   const long long base = (long long)blockIdx.x * kPulseTile;
   int c = 0;
   for (int t = threadIdx.x; t < kPulseTile; t += kPulseThreads) {
     const long long i = base + t;
-    if (i >= 1 && i < n) c += edge_at(s, i, high) != 0;
+    if (i >= 1 && i < n) c += edge_at(s, i, bias, high) != 0;
   }
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
   if ((threadIdx.x & 63) == 0) wave_counts[threadIdx.x >> 6] = c;
@@ -84,7 +84,8 @@ __global__ void __launch_bounds__(1024) exclusive_scan_kernel(int* counts, int n
 }
 
 // ordered compaction of edges: edge_pos[k], edge_kind[k]
-__global__ void __launch_bounds__(kPulseThreads) pulse_write_kernel(const float* s, long long n, double high,
+__global__ void __launch_bounds__(kPulseThreads) pulse_write_kernel(const float* s, long long n, double bias,
+                                                                    double high,
                                                                     const int* tile_offsets, long long* edge_pos,
                                                                     unsigned char* edge_kind, int edge_cap) {
   __shared__ int wave_counts[kPulseThreads / 64];
@@ -96,7 +97,7 @@ __global__ void __launch_bounds__(kPulseThreads) pulse_write_kernel(const float*
   __syncthreads();
   for (int t0 = 0; t0 < kPulseTile; t0 += kPulseThreads) {
     const long long i = base + t0 + threadIdx.x;
-    const int e = (i >= 1 && i < n) ? edge_at(s, i, high) : 0;
+    const int e = (i >= 1 && i < n) ? edge_at(s, i, bias, high) : 0;
     const unsigned long long m = __ballot(e != 0);
     if (lane == 0) wave_counts[wave] = __popcll(m);
     __syncthreads();
@@ -177,6 +178,8 @@ struct WaveCtx {
   const float* s;
   long long n;
   int lane;
+  double bias;
+  __device__ __forceinline__ double at(long long i) const { return (double)s[i] + bias; }
 };
 
 __device__ __forceinline__ int iround(double v) { return (int)round(v); }
@@ -185,22 +188,22 @@ __device__ __forceinline__ int iround(double v) { return (int)round(v); }
 __device__ bool zero_crossing(const WaveCtx& w, long long start, double target, int edge, int count, double* out) {
   if (start < 0 || start >= w.n) return false;
   int e = edge;
-  if (e == 0) e = ((double)w.s[start] < target) ? 1 : -1;
+  if (e == 0) e = (w.at(start) < target) ? 1 : -1;
   long long end = w.n < start + count + 1 ? w.n : start + count + 1;
   const long long span = end - start;
   for (long long b = 1; b < span; b += 64) {
     const long long i = b + w.lane;
     bool hit = false;
     if (i < span) {
-      const double a = (double)w.s[start + i - 1];
-      const double c = (double)w.s[start + i];
+      const double a = w.at(start + i - 1);
+      const double c = w.at(start + i);
       hit = (e == 1) ? (a < target && c >= target) : (a > target && c <= target);
     }
     const unsigned long long m = __ballot(hit);
     if (m) {
       const long long x = start + b + (long long)(__ffsll((long long)m) - 1);
-      const double a = (double)w.s[x - 1] - target;
-      const double c = (double)w.s[x] - target;
+      const double a = w.at(x - 1) - target;
+      const double c = w.at(x) - target;
       const double y = (c - a) != 0.0 ? -a / (-a + c) : 0.0;
       *out = (double)(x - 1) + y;
       return true;
@@ -218,7 +221,7 @@ __device__ bool area_valid(const WaveCtx& w, double crossing, int one_us, double
   if (en <= st) return false;
   double mn = INFINITY, mx = -INFINITY;
   for (long long i = st + w.lane; i < en; i += 64) {
-    const double v = (double)w.s[i];
+    const double v = w.at(i);
     mn = fmin(mn, v);
     mx = fmax(mx, v);
   }
@@ -252,125 +255,265 @@ __device__ bool mean_range(const WaveCtx& w, double start, double end, double* m
   }
   double sum = 0.0;
   if (w.lane == 0)
-    for (long long i = a; i < b; i++) sum += (double)w.s[i];
+    for (long long i = a; i < b; i++) sum += w.at(i);
   sum = __shfl(sum, 0);
   *mean = sum / (double)(b - a);
   return true;
 }
 
-__global__ void __launch_bounds__(64) hsync_refine_kernel(const float* const* spans, const long long* span_len,
-                                                          const double* const* locs, const unsigned char* const* filled,
-                                                          const int* line_counts, HsyncParams p,
-                                                          double* const* refined_out,
-                                                          unsigned char* const* filled_out) {
-  const int f = blockIdx.x;
-  WaveCtx w{spans[f], span_len[f], (int)threadIdx.x};
-  const int lines = line_counts[f];
-  const double* loc = locs[f];
-  const unsigned char* fin = filled[f];
-  double* ro = refined_out[f];
-  unsigned char* fo = filled_out[f];
-  const int one_us = (int)p.fs_mhz > 1 ? (int)p.fs_mhz : 1;
-  const int normal_len = (int)(p.hsync_us * p.fs_mhz) > 1 ? (int)(p.hsync_us * p.fs_mhz) : 1;
-  const int search_count = one_us * 2 > 1 ? one_us * 2 : 1;
+// Geometry shared by every line of a refinement.
+struct HsyncGeom {
+  int one_us;
+  int normal_len;
+  int search_count;
+};
+
+__device__ __forceinline__ HsyncGeom hsync_geom(const HsyncParams& p) {
+  HsyncGeom g;
+  g.one_us = (int)p.fs_mhz > 1 ? (int)p.fs_mhz : 1;
+  g.normal_len = (int)(p.hsync_us * p.fs_mhz) > 1 ? (int)(p.hsync_us * p.fs_mhz) : 1;
+  g.search_count = g.one_us * 2 > 1 ? g.one_us * 2 : 1;
+  return g;
+}
+
+// Lines the refinement leaves alone (vsync lines are marked filled).  Returns
+// 0 = refine, 1 = vsync line, 2 = other skip.
+__device__ __forceinline__ int line_kind(const HsyncParams& p, const HsyncGeom& g, long long n, int line, double cur,
+                                         long long* search_start) {
+  if ((line >= 3 && line <= 6) || (p.is_pal && line >= 1 && line <= 2)) return 1;
+  if (!isfinite(cur)) return 2;
+  const long long ss = iround(cur - g.one_us);
+  if (ss < 0 || ss >= n - 1) return 2;
+  *search_start = ss;
+  return 0;
+}
+
+// Right-edge candidate of TryRefineFromRightHSync / TryConfirmRightHSync without
+// the final distance test against the left result (TbcFieldDecodePipeline.cs:4807-4868,
+// 4950-4987).  Independent of the carried porch level.
+__device__ bool right_prelim(const WaveCtx& w, const HsyncParams& p, const HsyncGeom& g, long long search_start,
+                             double* cand, double* rporch) {
   const long long n = w.n;
   auto ire_hz = [&](double ire) { return p.ire0 + p.hz_ire * ire; };
-  for (int l = w.lane; l < lines; l += 64) {
-    ro[l] = loc[l];
-    fo[l] = fin[l];
-  }
-  __syncthreads();
-  double prev_porch = -1.0;
-  for (int line = 0; line < lines; line++) {
-    const bool vsync_line = (line >= 3 && line <= 6) || (p.is_pal && line >= 1 && line <= 2);
-    if (vsync_line) {
-      if (w.lane == 0) fo[line] = 1;
-      continue;
-    }
-    const double cur = loc[line];
-    if (!isfinite(cur)) continue;
-    const long long search_start = iround(cur - one_us);
-    if (search_start < 0 || search_start >= n - 1) continue;
-    double crossing = 0.0;
-    const int cnt = (int)(search_count < n - search_start - 1 ? search_count : n - search_start - 1);
-    const bool has = zero_crossing(w, search_start, p.threshold, 0, cnt, &crossing);
-    const double original = cur;
-    double right_ref = original;
-    double out_loc = original;
-    unsigned char out_filled = fin[line];
-    if (has && !fin[line]) {
-      double refined_loc = crossing, porch = 0.0, sync_level = 0.0;
-      bool preserve = false, left_ok = false;
-      const bool valid = area_valid(w, crossing, one_us, 3.5, ire_hz(-65.0), ire_hz(110.0)) &&
-                         !area_has_nan(w, crossing, one_us, 3.5);
-      if (valid && mean_range(w, crossing + one_us, crossing + 2.5 * one_us, &sync_level)) {
-        preserve = true;
-        bool porch_ok = true;
-        if (prev_porch > 0.0)
-          porch = prev_porch;
-        else if (!mean_range(w, crossing - one_us, crossing - 0.5 * one_us, &porch))
-          porch_ok = false;
-        if (porch_ok) {
-          const double mid = (porch + sync_level) / 2.0;
-          const int c4 = (int)(400 < n - search_start - 1 ? 400 : n - search_start - 1);
-          double rc = 0.0;
-          bool good = zero_crossing(w, search_start, mid, 0, c4, &rc) && fabs(rc - crossing) < one_us / 2.0;
-          if (!good && prev_porch > 0.0) {
-            good = zero_crossing(w, search_start, (prev_porch + sync_level) / 2.0, 0, c4, &rc) &&
-                   fabs(rc - crossing) < one_us / 2.0;
-            if (good) porch = prev_porch;
-          }
-          if (good) {
-            refined_loc = rc;
-            left_ok = true;
-          }
+  const long long rs = search_start + g.normal_len - g.one_us;
+  if (rs < 0 || rs >= n - 1) return false;
+  if (w.at(rs) > p.threshold) return false;
+  const int rcount = g.normal_len * 2 > 1 ? g.normal_len * 2 : 1;
+  const int rc_n = (int)(rcount < n - rs - 1 ? rcount : n - rs - 1);
+  double rcross = 0.0;
+  if (!zero_crossing(w, rs, p.threshold, 1, rc_n, &rcross)) return false;
+  const double rds = rcross - g.normal_len;
+  double sync_level = 0.0;
+  if (!(area_valid(w, rds, g.one_us, 8.0, ire_hz(-65.0), ire_hz(30.0)) && !area_has_nan(w, rds, g.one_us, 8.0)))
+    return false;
+  if (!mean_range(w, rds + g.one_us, rds + 2.5 * g.one_us, &sync_level)) return false;
+  if (!mean_range(w, rds + g.normal_len + g.one_us, rds + g.normal_len + 2.0 * g.one_us, rporch)) return false;
+  const double mid = (*rporch + sync_level) / 2.0;
+  const int c4 = (int)(400 < n - rs - 1 ? 400 : n - rs - 1);
+  double rrc = 0.0;
+  if (!zero_crossing(w, rs, mid, 0, c4, &rrc)) return false;
+  if (!(fabs(rrc - rcross) < g.one_us / 2.0)) return false;
+  const double sr = (double)(float)p.fs_mhz;
+  *cand = rcross - g.normal_len + (2.25 * (sr / 40.0));
+  return true;
+}
+
+// One refinable line given the porch level carried in (`p_in`), as in the loop body
+// of RefineLineLocationsFromHSync with TryRefineLeftHSync (:4870-4948).  Writes the
+// refined location, filled flag and the porch level carried out.
+__device__ void refine_line(const WaveCtx& w, const HsyncParams& p, const HsyncGeom& g, long long search_start,
+                            double cur, unsigned char fin, double p_in, bool right_ok, double rcand, double rporch,
+                            double* out_loc, unsigned char* out_filled, double* p_out) {
+  const long long n = w.n;
+  auto ire_hz = [&](double ire) { return p.ire0 + p.hz_ire * ire; };
+  double prev_porch = p_in;
+  double crossing = 0.0;
+  const int cnt = (int)(g.search_count < n - search_start - 1 ? g.search_count : n - search_start - 1);
+  const bool has = zero_crossing(w, search_start, p.threshold, 0, cnt, &crossing);
+  const double original = cur;
+  double right_ref = original;
+  double loc = original;
+  unsigned char fil = fin;
+  if (has && !fin) {
+    double refined_loc = crossing, porch = 0.0, sync_level = 0.0;
+    bool preserve = false, left_ok = false;
+    const bool valid = area_valid(w, crossing, g.one_us, 3.5, ire_hz(-65.0), ire_hz(110.0)) &&
+                       !area_has_nan(w, crossing, g.one_us, 3.5);
+    if (valid && mean_range(w, crossing + g.one_us, crossing + 2.5 * g.one_us, &sync_level)) {
+      preserve = true;
+      bool porch_ok = true;
+      if (prev_porch > 0.0)
+        porch = prev_porch;
+      else if (!mean_range(w, crossing - g.one_us, crossing - 0.5 * g.one_us, &porch))
+        porch_ok = false;
+      if (porch_ok) {
+        const double mid = (porch + sync_level) / 2.0;
+        const int c4 = (int)(400 < n - search_start - 1 ? 400 : n - search_start - 1);
+        double rc = 0.0;
+        bool good = zero_crossing(w, search_start, mid, 0, c4, &rc) && fabs(rc - crossing) < g.one_us / 2.0;
+        if (!good && prev_porch > 0.0) {
+          good = zero_crossing(w, search_start, (prev_porch + sync_level) / 2.0, 0, c4, &rc) &&
+                 fabs(rc - crossing) < g.one_us / 2.0;
+          if (good) porch = prev_porch;
+        }
+        if (good) {
+          refined_loc = rc;
+          left_ok = true;
         }
       }
-      if (left_ok) {
-        out_loc = refined_loc;
-        right_ref = refined_loc;
-        out_filled = fin[line];
-        if (porch > 0.0) prev_porch = porch;
-      } else {
-        out_filled = 1;
-        out_loc = original;
-        if (preserve) right_ref = refined_loc;
-      }
+    }
+    if (left_ok) {
+      loc = refined_loc;
+      right_ref = refined_loc;
+      fil = fin;
+      if (porch > 0.0) prev_porch = porch;
     } else {
-      out_filled = 1;
-      out_loc = original;
+      fil = 1;
+      loc = original;
+      if (preserve) right_ref = refined_loc;
     }
-    if (p.use_right) {
-      do {
-        const long long rs = search_start + normal_len - one_us;
-        if (rs < 0 || rs >= n - 1) break;
-        if ((double)w.s[rs] > p.threshold) break;
-        const int rcount = normal_len * 2 > 1 ? normal_len * 2 : 1;
-        const int rc_n = (int)(rcount < n - rs - 1 ? rcount : n - rs - 1);
-        double rcross = 0.0;
-        if (!zero_crossing(w, rs, p.threshold, 1, rc_n, &rcross)) break;
-        const double rds = rcross - normal_len;
-        double sync_level = 0.0, rporch = 0.0;
-        if (!(area_valid(w, rds, one_us, 8.0, ire_hz(-65.0), ire_hz(30.0)) && !area_has_nan(w, rds, one_us, 8.0)))
-          break;
-        if (!mean_range(w, rds + one_us, rds + 2.5 * one_us, &sync_level)) break;
-        if (!mean_range(w, rds + normal_len + one_us, rds + normal_len + 2.0 * one_us, &rporch)) break;
-        const double mid = (rporch + sync_level) / 2.0;
-        const int c4 = (int)(400 < n - rs - 1 ? 400 : n - rs - 1);
-        double rrc = 0.0;
-        if (!zero_crossing(w, rs, mid, 0, c4, &rrc)) break;
-        if (!(fabs(rrc - rcross) < one_us / 2.0)) break;
-        const double sr = (double)(float)p.fs_mhz;
-        const double cand = rcross - normal_len + (2.25 * (sr / 40.0));
-        if (fabs(cand - right_ref) >= one_us * 2.0) break;
-        out_loc = cand;
-        out_filled = 0;
-        if (rporch > 0.0) prev_porch = rporch;
-      } while (false);
-    }
+  } else {
+    fil = 1;
+    loc = original;
+  }
+  if (p.use_right && right_ok && !(fabs(rcand - right_ref) >= g.one_us * 2.0)) {
+    loc = rcand;
+    fil = 0;
+    if (rporch > 0.0) prev_porch = rporch;
+  }
+  *out_loc = loc;
+  *out_filled = fil;
+  *p_out = prev_porch;
+}
+
+// Per-(field, line) refinement state kept between the passes.
+struct HsyncWork {
+  unsigned char* kind;   // line_kind
+  unsigned char* rt_ok;  // right prelim succeeded
+  double* rt_cand;
+  double* rt_porch;
+  double* p_used;        // porch level the speculative pass assumed
+  double* p_out;         // porch level the speculative pass carried out
+  int* fb_line;          // per field: first line to recompute sequentially (== lines: none)
+  double* fb_porch;      // per field: porch level entering that line
+  int stride;            // entries per field (max lines)
+};
+
+// Pass 1: line kinds and right-edge candidates, one wavefront per line.
+__global__ void __launch_bounds__(64) hsync_prelim_kernel(const float* const* spans, const long long* span_len,
+                                                          const double* const* locs, const int* line_counts,
+                                                          HsyncParams p, HsyncWork wk, double bias) {
+  const int f = blockIdx.y;
+  const int line = blockIdx.x;
+  if (line >= line_counts[f]) return;
+  WaveCtx w{spans[f], span_len[f], (int)threadIdx.x, bias};
+  const HsyncGeom g = hsync_geom(p);
+  long long ss = 0;
+  const int k = line_kind(p, g, w.n, line, locs[f][line], &ss);
+  double cand = 0.0, rp = 0.0;
+  const bool ok = (k == 0 && p.use_right) ? right_prelim(w, p, g, ss, &cand, &rp) : false;
+  if (w.lane == 0) {
+    const int i = f * wk.stride + line;
+    wk.kind[i] = (unsigned char)k;
+    wk.rt_ok[i] = ok ? 1 : 0;
+    wk.rt_cand[i] = cand;
+    wk.rt_porch[i] = rp;
+  }
+}
+
+// Pass 2: every line refined in parallel, assuming the porch level carried in is the
+// right-edge porch of the nearest earlier line with a right candidate (-1 if none).
+__global__ void __launch_bounds__(64) hsync_spec_kernel(const float* const* spans, const long long* span_len,
+                                                        const double* const* locs, const unsigned char* const* filled,
+                                                        const int* line_counts, HsyncParams p, HsyncWork wk,
+                                                        double* const* refined_out, unsigned char* const* filled_out,
+                                                        double bias) {
+  const int f = blockIdx.y;
+  const int line = blockIdx.x;
+  if (line >= line_counts[f]) return;
+  WaveCtx w{spans[f], span_len[f], (int)threadIdx.x, bias};
+  const HsyncGeom g = hsync_geom(p);
+  const int base = f * wk.stride;
+  const int k = wk.kind[base + line];
+  const double cur = locs[f][line];
+  if (k != 0) {
     if (w.lane == 0) {
-      ro[line] = out_loc;
-      fo[line] = out_filled;
+      refined_out[f][line] = cur;
+      filled_out[f][line] = k == 1 ? 1 : filled[f][line];
+    }
+    return;
+  }
+  double p_in = -1.0;
+  for (int l = line - 1; l >= 0; l--) {
+    const int i = base + l;
+    if (wk.kind[i] == 0 && wk.rt_ok[i] && wk.rt_porch[i] > 0.0) {
+      p_in = wk.rt_porch[i];
+      break;
+    }
+  }
+  long long ss = iround(cur - g.one_us);
+  double loc = 0.0, pout = 0.0;
+  unsigned char fil = 0;
+  refine_line(w, p, g, ss, cur, filled[f][line], p_in, wk.rt_ok[base + line] != 0, wk.rt_cand[base + line],
+              wk.rt_porch[base + line], &loc, &fil, &pout);
+  if (w.lane == 0) {
+    refined_out[f][line] = loc;
+    filled_out[f][line] = fil;
+    wk.p_used[base + line] = p_in;
+    wk.p_out[base + line] = pout;
+  }
+}
+
+// Pass 3: walk the carried porch level sequentially (one thread per field) and find
+// the first line whose speculation does not hold.
+__global__ void hsync_verify_kernel(const int* line_counts, int fields, HsyncWork wk) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= fields) return;
+  const int lines = line_counts[f];
+  const int base = f * wk.stride;
+  double pc = -1.0;
+  int fb = lines;
+  for (int l = 0; l < lines; l++) {
+    if (wk.kind[base + l] != 0) continue;
+    const double pu = wk.p_used[base + l];
+    if (__double_as_longlong(pu) != __double_as_longlong(pc)) {
+      fb = l;
+      break;
+    }
+    pc = wk.p_out[base + l];
+  }
+  wk.fb_line[f] = fb;
+  wk.fb_porch[f] = pc;
+}
+
+// Pass 4: sequential refinement from the first mis-speculated line (no-op for
+// fields whose speculation held everywhere).
+__global__ void __launch_bounds__(64) hsync_refine_kernel(const float* const* spans, const long long* span_len,
+                                                          const double* const* locs, const unsigned char* const* filled,
+                                                          const int* line_counts, HsyncParams p, HsyncWork wk,
+                                                          double* const* refined_out,
+                                                          unsigned char* const* filled_out, double bias) {
+  const int f = blockIdx.x;
+  const int lines = line_counts[f];
+  const int start = wk.fb_line[f];
+  if (start >= lines) return;
+  WaveCtx w{spans[f], span_len[f], (int)threadIdx.x, bias};
+  const HsyncGeom g = hsync_geom(p);
+  const int base = f * wk.stride;
+  double prev_porch = wk.fb_porch[f];
+  for (int line = start; line < lines; line++) {
+    const int k = wk.kind[base + line];
+    if (k != 0) continue;  // skipped lines were already written by the parallel pass
+    const double cur = locs[f][line];
+    long long ss = iround(cur - g.one_us);
+    double loc = 0.0, pout = 0.0;
+    unsigned char fil = 0;
+    refine_line(w, p, g, ss, cur, filled[f][line], prev_porch, wk.rt_ok[base + line] != 0,
+                wk.rt_cand[base + line], wk.rt_porch[base + line], &loc, &fil, &pout);
+    prev_porch = pout;
+    if (w.lane == 0) {
+      refined_out[f][line] = loc;
+      filled_out[f][line] = fil;
     }
   }
 }
@@ -379,7 +522,9 @@ __global__ void __launch_bounds__(64) hsync_refine_kernel(const float* const* sp
 // TbcLineResampler linear positions + SampleSinc + VideoOutputConverter
 // (Tbc/TbcLineResampler.cs:732-804, 1167-1234, 1495-1525;
 // Tbc/VideoOutputConverter.cs:74-90).  One thread per output sample.
-__device__ __forceinline__ double sample_sinc(const float* src, long long len, double position,
+// Source samples are plane offsets; the reference reads (float)video with video in
+// FP64 Hz, reproduced here as (float)((double)offset + bias).
+__device__ __forceinline__ double sample_sinc(const float* src, long long len, double bias, double position,
                                               const float* __restrict__ lut) {
   if (!isfinite(position)) return 0.0;
   const int taps = 16, phases = 65536;
@@ -411,14 +556,15 @@ __device__ __forceinline__ double sample_sinc(const float* src, long long len, d
     float prod;
     {
 #pragma clang fp contract(off)
-      prod = src[si] * wgt;
+      prod = (float)((double)src[si] + bias) * wgt;
     }
     result = (double)prod + result;
   }
   return result;
 }
 
-__global__ void __launch_bounds__(256) tbc_resample_kernel(const float* src, long long src_len, const double* locs,
+__global__ void __launch_bounds__(256) tbc_resample_kernel(const float* src, long long src_len, double bias,
+                                                           const double* locs,
                                                            const double* level_adjust, int level_per_line,
                                                            TbcParams p, const float* __restrict__ lut,
                                                            unsigned short* out_u16, double* out_f64) {
@@ -443,7 +589,7 @@ __global__ void __launch_bounds__(256) tbc_resample_kernel(const float* src, lon
   double r;
   {
 #pragma clang fp contract(off)
-    r = (double)(float)(sample_sinc(src, src_len, pos, lut) * adj);
+    r = (double)(float)(sample_sinc(src, src_len, bias, pos, lut) * adj);
   }
   if (out_f64) out_f64[d] = r;
   if (p.has_conv && out_u16) {
@@ -566,7 +712,8 @@ struct FieldTbcParams {
   double conv_offset;
 };
 
-__global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long long src_len, const double* locs,
+__global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long long src_len, double bias,
+                                                         const long long* origins, const double* locs,
                                                          const double* line_x, const double* line_y0,
                                                          const double* pow_table, FieldTbcParams p,
                                                          const float* __restrict__ lut, unsigned short* out) {
@@ -585,15 +732,17 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
   frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
   double pos, adj;
   const double x = line_x[(long long)f * p.lines + left];
+  const long long org = origins ? origins[f] : 0;
   {
 #pragma clang fp contract(off)
     pos = ll + (delta * frac);
+    pos -= (double)org;
     adj = p.smooth ? x + (line_y0[(long long)f * p.lines + left] - x) * pow_table[smp] : x;
   }
   double r;
   {
 #pragma clang fp contract(off)
-    r = (double)(float)(sample_sinc(src, src_len, pos, lut) * adj);
+    r = (double)(float)(sample_sinc(src + org, src_len - org, bias, pos, lut) * adj);
   }
   double sd;
   {
@@ -615,16 +764,17 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
 
 extern "C" {
 
-hipError_t vhsd_launch_pulses(const float* s, long long n, double high, int min_len, int max_len, int* tile_counts,
+hipError_t vhsd_launch_pulses(const float* s, long long n, double bias, double high, int min_len, int max_len,
+                              int* tile_counts,
                               int n_tiles_cap, long long* edge_pos, unsigned char* edge_kind, int edge_cap,
                               int* counters, int* starts, int* lengths, int capacity, hipStream_t stream) {
   const int tiles = (int)((n + vhsd::kPulseTile - 1) / vhsd::kPulseTile);
   if (tiles > n_tiles_cap) return hipErrorInvalidValue;
   if (tiles == 0) return hipMemsetAsync(counters, 0, 2 * sizeof(int), stream);
-  hipLaunchKernelGGL(vhsd::pulse_count_kernel, dim3(tiles), dim3(vhsd::kPulseThreads), 0, stream, s, n, high,
+  hipLaunchKernelGGL(vhsd::pulse_count_kernel, dim3(tiles), dim3(vhsd::kPulseThreads), 0, stream, s, n, bias, high,
                      tile_counts);
   hipLaunchKernelGGL(vhsd::exclusive_scan_kernel, dim3(1), dim3(1024), 0, stream, tile_counts, tiles, counters);
-  hipLaunchKernelGGL(vhsd::pulse_write_kernel, dim3(tiles), dim3(vhsd::kPulseThreads), 0, stream, s, n, high,
+  hipLaunchKernelGGL(vhsd::pulse_write_kernel, dim3(tiles), dim3(vhsd::kPulseThreads), 0, stream, s, n, bias, high,
                      tile_counts, edge_pos, edge_kind, edge_cap);
   hipLaunchKernelGGL(vhsd::pulse_pair_kernel, dim3(1), dim3(1024), 0, stream, edge_pos, edge_kind, counters,
                      edge_cap, min_len, max_len, starts, lengths, capacity, counters + 1);
@@ -633,14 +783,49 @@ hipError_t vhsd_launch_pulses(const float* s, long long n, double high, int min_
 
 hipError_t vhsd_launch_hsync(const float* const* spans, const long long* span_len, const double* const* locs,
                              const unsigned char* const* filled, const int* line_counts, const vhsd::HsyncParams* p,
-                             double* const* refined_out, unsigned char* const* filled_out, int fields,
-                             hipStream_t stream) {
+                             double* const* refined_out, unsigned char* const* filled_out, int fields, int max_lines,
+                             double bias, void* work, hipStream_t stream) {
+  if (fields <= 0 || max_lines <= 0) return hipSuccess;
+  // work layout: kind, rt_ok [fields*max_lines bytes each], rt_cand, rt_porch, p_used, p_out
+  // [fields*max_lines doubles each], fb_line [fields ints], fb_porch [fields doubles]
+  const size_t nl = (size_t)fields * max_lines;
+  unsigned char* base = reinterpret_cast<unsigned char*>(work);
+  vhsd::HsyncWork wk;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* q = base + off;
+    off += (bytes + 255) & ~(size_t)255;
+    return q;
+  };
+  wk.kind = take(nl);
+  wk.rt_ok = take(nl);
+  wk.rt_cand = reinterpret_cast<double*>(take(nl * 8));
+  wk.rt_porch = reinterpret_cast<double*>(take(nl * 8));
+  wk.p_used = reinterpret_cast<double*>(take(nl * 8));
+  wk.p_out = reinterpret_cast<double*>(take(nl * 8));
+  wk.fb_line = reinterpret_cast<int*>(take((size_t)fields * 4));
+  wk.fb_porch = reinterpret_cast<double*>(take((size_t)fields * 8));
+  wk.stride = max_lines;
+  const dim3 lg(max_lines, fields);
+  hipLaunchKernelGGL(vhsd::hsync_prelim_kernel, lg, dim3(64), 0, stream, spans, span_len, locs, line_counts, *p, wk,
+                     bias);
+  hipLaunchKernelGGL(vhsd::hsync_spec_kernel, lg, dim3(64), 0, stream, spans, span_len, locs, filled, line_counts, *p,
+                     wk, refined_out, filled_out, bias);
+  hipLaunchKernelGGL(vhsd::hsync_verify_kernel, dim3((fields + 63) / 64), dim3(64), 0, stream, line_counts, fields,
+                     wk);
   hipLaunchKernelGGL(vhsd::hsync_refine_kernel, dim3(fields), dim3(64), 0, stream, spans, span_len, locs, filled,
-                     line_counts, *p, refined_out, filled_out);
+                     line_counts, *p, wk, refined_out, filled_out, bias);
   return hipGetLastError();
 }
 
-hipError_t vhsd_launch_tbc_fields(const float* src, long long src_len, const double* refined, int nloc, int lines,
+size_t vhsd_hsync_work_bytes(int fields, int max_lines) {
+  const size_t nl = (size_t)fields * max_lines;
+  auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  return 2 * r(nl) + 4 * r(nl * 8) + r((size_t)fields * 4) + r((size_t)fields * 8);
+}
+
+hipError_t vhsd_launch_tbc_fields(const float* src, long long src_len, double bias, const long long* origins,
+                                  const double* refined, int nloc, int lines,
                                   int fields, double nominal, double line_decay, const double* pow_table,
                                   double* line_x, double* line_y0, int out_len, int smooth, double ire0,
                                   double out_scale, double inv_hz_ire, double conv_offset, const float* lut,
@@ -652,17 +837,18 @@ hipError_t vhsd_launch_tbc_fields(const float* src, long long src_len, const dou
   vhsd::FieldTbcParams p{out_len, lines, nloc, nominal, smooth, ire0, out_scale, inv_hz_ire, conv_offset};
   const long long dest = (long long)out_len * lines;
   hipLaunchKernelGGL(vhsd::tbc_fields_kernel, dim3((unsigned)((dest + 255) / 256), fields), dim3(256), 0, stream, src,
-                     src_len, refined, line_x, line_y0, pow_table, p, lut, out);
+                     src_len, bias, origins, refined, line_x, line_y0, pow_table, p, lut, out);
   return hipGetLastError();
 }
 
-hipError_t vhsd_launch_tbc(const float* src, long long src_len, const double* locs, const double* level_adjust,
+hipError_t vhsd_launch_tbc(const float* src, long long src_len, double bias, const double* locs, const double* level_adjust,
                            int level_per_line, const vhsd::TbcParams* p, const float* lut, unsigned short* out_u16,
                            double* out_f64, hipStream_t stream) {
   const long long dest = (long long)p->out_len * p->line_count;
   if (dest <= 0) return hipSuccess;
   const int blocks = (int)((dest + 255) / 256);
-  hipLaunchKernelGGL(vhsd::tbc_resample_kernel, dim3(blocks), dim3(256), 0, stream, src, src_len, locs, level_adjust,
+  hipLaunchKernelGGL(vhsd::tbc_resample_kernel, dim3(blocks), dim3(256), 0, stream, src, src_len, bias, locs,
+                     level_adjust,
                      level_per_line, *p, lut, out_u16, out_f64);
   return hipGetLastError();
 }

@@ -20,6 +20,9 @@ struct DemodArgs {
   int vlp_offset;                // sync reference roll (samples)
   float fs;                      // sample rate as FP32 (the discriminator's frequency)
   double diff_max;               // spike-repair threshold; <= 0 disables
+  float bias;                    // plane bias (Hz), exactly representable in FP32
+  float video_corr;              // bias * (Re Video[0] - 1): restores the DC the biased FFT removed
+  float vlp_corr;                // bias * (Re VideoLowPass05[0] - 1)
   SosScanArgs env;               // envelope filter
   float* out_video;
   float* out_vlp;
@@ -74,19 +77,22 @@ hipError_t vhsd_launch_block_demod(const vhsd::DemodArgs* a, int nblocks, hipStr
 hipError_t vhsd_launch_block_chroma(const vhsd::ChromaArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_repair(const vhsd::RepairArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const float2* oct, hipStream_t stream);
-hipError_t vhsd_launch_pulses(const float* s, long long n, double high, int min_len, int max_len, int* tile_counts,
+hipError_t vhsd_launch_pulses(const float* s, long long n, double bias, double high, int min_len, int max_len,
+                              int* tile_counts,
                               int n_tiles_cap, long long* edge_pos, unsigned char* edge_kind, int edge_cap,
                               int* counters, int* starts, int* lengths, int capacity, hipStream_t stream);
 hipError_t vhsd_launch_hsync(const float* const* spans, const long long* span_len, const double* const* locs,
                              const unsigned char* const* filled, const int* line_counts, const vhsd::HsyncParams* p,
-                             double* const* refined_out, unsigned char* const* filled_out, int fields,
-                             hipStream_t stream);
-hipError_t vhsd_launch_tbc_fields(const float* src, long long src_len, const double* refined, int nloc, int lines,
+                             double* const* refined_out, unsigned char* const* filled_out, int fields, int max_lines,
+                             double bias, void* work, hipStream_t stream);
+size_t vhsd_hsync_work_bytes(int fields, int max_lines);
+hipError_t vhsd_launch_tbc_fields(const float* src, long long src_len, double bias, const long long* origins,
+                                  const double* refined, int nloc, int lines,
                                   int fields, double nominal, double line_decay, const double* pow_table,
                                   double* line_x, double* line_y0, int out_len, int smooth, double ire0,
                                   double out_scale, double inv_hz_ire, double conv_offset, const float* lut,
                                   unsigned short* out, hipStream_t stream);
-hipError_t vhsd_launch_tbc(const float* src, long long src_len, const double* locs, const double* level_adjust,
+hipError_t vhsd_launch_tbc(const float* src, long long src_len, double bias, const double* locs, const double* level_adjust,
                            int level_per_line, const vhsd::TbcParams* p, const float* lut, unsigned short* out_u16,
                            double* out_f64, hipStream_t stream);
 }

@@ -211,6 +211,8 @@ struct vhsdecode_hip_demod_context {
   double fs = 0.0;
   int vlp_offset = 0, chroma_offset = 0;
   double diff_max = 0.0;
+  double bias = 0.0;
+  float video_corr = 0.0f, vlp_corr = 0.0f;
   bool has_chroma = false;
   vhsd::SosScanArgs env{}, chroma{};
   DevBuf<float2> rfv, vid, vlp, oct;
@@ -227,6 +229,7 @@ struct vhsdecode_hip_demod_context {
   DevBuf<double> h_loc, h_ref;
   DevBuf<unsigned char> h_fil, h_fo;
   DevBuf<unsigned char> h_ptrs;
+  DevBuf<unsigned char> h_work;
   // tbc
   DevBuf<float> lut;
   bool lut_ready = false;
@@ -262,7 +265,7 @@ struct vhsdecode_hip_demod_context {
     env_trans.release(); chroma_trans.release(); block_start.release(); out_offset.release(); flags.release();
     rep_a.release(); rep_dm.release(); tile_counts.release(); counters.release(); p_starts.release();
     p_lens.release(); edge_pos.release(); edge_kind.release(); h_loc.release(); h_ref.release();
-    h_fil.release(); h_fo.release(); h_ptrs.release(); lut.release(); t_loc.release(); t_adj.release();
+    h_fil.release(); h_fo.release(); h_ptrs.release(); h_work.release(); lut.release(); t_loc.release(); t_adj.release();
     t_f64.release(); t_u16.release(); s_rf.release(); s_planes.release(); s_flags.release();
     f_loc.release(); f_ref.release(); f_x.release(); f_y0.release(); f_pow.release(); f_fil.release();
     f_fo.release(); f_tab.release();
@@ -353,6 +356,9 @@ int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int
   a.vlp_offset = c->vlp_offset;
   a.fs = (float)c->fs;
   a.diff_max = c->diff_max;
+  a.bias = (float)c->bias;
+  a.video_corr = c->video_corr;
+  a.vlp_corr = c->vlp_corr;
   a.env = c->env;
   a.env.trans = c->env_trans.p;
   a.out_video = out->video;
@@ -460,6 +466,12 @@ int32_t vhsdecode_hip_demod_create(const vhsdecode_hip_demod_config_v1* cfg, vhs
     c->vlp_offset = f->video_lowpass05_offset;
     c->chroma_offset = f->chroma_offset;
     c->diff_max = f->diff_demod_max;
+    c->bias = f->plane_bias_hz;
+    if (!std::isfinite(c->bias) || (double)(float)c->bias != c->bias)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "plane_bias_hz must be exactly representable in FP32");
+    // DC restored after the biased transforms: bias * (Re H[0] - 1)
+    c->video_corr = (float)(c->bias * (f->video[0] - 1.0));
+    c->vlp_corr = (float)(c->bias * (f->video_lowpass05[0] - 1.0));
     auto bail = [&](int32_t code) {
       delete c;
       return code;
@@ -583,7 +595,7 @@ int32_t vhsdecode_hip_find_pulses(vhsdecode_hip_demod_context* c, const float* s
     VHSD_HIP(c->p_lens.ensure(std::max(capacity, 1)));
     VHSD_HIP(hipMemsetAsync(c->counters.p, 0, 2 * sizeof(int), st));
     ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_PULSES, length);
-    VHSD_HIP(vhsd_launch_pulses(sync_ref, length, threshold, min_length, max_length, c->tile_counts.p, tiles,
+    VHSD_HIP(vhsd_launch_pulses(sync_ref, length, c->bias, threshold, min_length, max_length, c->tile_counts.p, tiles,
                                 c->edge_pos.p, c->edge_kind.p, (int)edge_cap, c->counters.p, c->p_starts.p,
                                 c->p_lens.p, capacity, st));
     int host_counters[2] = {0, 0};
@@ -638,7 +650,9 @@ int32_t vhsdecode_hip_refine_hsync(vhsdecode_hip_demod_context* c, const float* 
                         params->hz_ire, params->is_pal, params->use_right_hsync};
     {
       ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_HSYNC, 1);
-      VHSD_HIP(vhsd_launch_hsync(&dt->span, &dt->len, &dt->loc, &dt->fil, &dt->lines, &p, &dt->ref, &dt->fo, 1, st));
+      VHSD_HIP(c->h_work.ensure(vhsd_hsync_work_bytes(1, lines)));
+      VHSD_HIP(vhsd_launch_hsync(&dt->span, &dt->len, &dt->loc, &dt->fil, &dt->lines, &p, &dt->ref, &dt->fo, 1, lines,
+                                 c->bias, c->h_work.p, st));
     }
     VHSD_HIP(hipMemcpyAsync(refined, c->h_ref.p, sizeof(double) * lines, hipMemcpyDeviceToHost, st));
     VHSD_HIP(hipMemcpyAsync(filled_out, c->h_fo.p, lines, hipMemcpyDeviceToHost, st));
@@ -742,7 +756,7 @@ int32_t vhsdecode_hip_tbc_resample(vhsdecode_hip_demod_context* c, const float* 
       VHSD_HIP(hipMemcpyAsync(c->t_adj.p, adj.data(), sizeof(double) * adj.size(), hipMemcpyHostToDevice, st));
     {
       ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_TBC, (long long)dest);
-      VHSD_HIP(vhsd_launch_tbc(source, source_length, c->t_loc.p, c->t_adj.p, per_sample ? 0 : 1, &t, c->lut.p,
+      VHSD_HIP(vhsd_launch_tbc(source, source_length, c->bias, c->t_loc.p, c->t_adj.p, per_sample ? 0 : 1, &t, c->lut.p,
                                out_u16 ? c->t_u16.p : nullptr, out_f64 ? c->t_f64.p : nullptr, st));
     }
     if (out_u16 && dest)
@@ -775,7 +789,7 @@ int32_t vhsdecode_hip_tbc_resample_device(vhsdecode_hip_demod_context* c, const 
     if ((rc = tbc_params(params, params->nominal_line_length, &t)) != 0) return rc;
     hipStream_t st = pick_stream(c, stream);
     ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_TBC, (long long)params->output_line_length * params->line_count);
-    VHSD_HIP(vhsd_launch_tbc(source, source_length, locations, level_adjust, per_sample ? 0 : 1, &t, c->lut.p,
+    VHSD_HIP(vhsd_launch_tbc(source, source_length, c->bias, locations, level_adjust, per_sample ? 0 : 1, &t, c->lut.p,
                              out_u16, out_f64, st));
     return VHSDECODE_HIP_OK;
   } catch (const std::exception& e) {
@@ -785,7 +799,8 @@ int32_t vhsdecode_hip_tbc_resample_device(vhsdecode_hip_demod_context* c, const 
 
 int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod_context* c, const float* sync_ref, const float* video,
                                            int64_t span_length, const double* line_locations, const uint8_t* filled,
-                                           int32_t nfields, const vhsdecode_hip_field_params_v1* p, uint16_t* out,
+                                           const int64_t* origins, int32_t nfields,
+                                           const vhsdecode_hip_field_params_v1* p, uint16_t* out,
                                            double* refined_out, void* stream) {
   try {
     if (!c || !sync_ref || !video || !line_locations || !filled || !p || !out || nfields < 0 || span_length <= 0)
@@ -800,6 +815,10 @@ int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod_context* c, const
     for (size_t i = 0; i < nl; i++)
       if (!std::isfinite(line_locations[i]))
         return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "line locations must be finite");
+    if (origins)
+      for (int f = 0; f < nfields; f++)
+        if (origins[f] < 0 || origins[f] >= span_length)
+          return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "field origins must lie inside the span");
     std::lock_guard<std::mutex> lock(c->mu);
     VHSD_HIP(hipSetDevice(c->device));
     int32_t rc = ensure_lut(c);
@@ -812,7 +831,7 @@ int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod_context* c, const
     VHSD_HIP(c->f_x.ensure((size_t)nfields * p->field_lines));
     VHSD_HIP(c->f_y0.ensure((size_t)nfields * p->field_lines));
     // per-field pointer tables for the batched hsync kernel
-    const size_t tab_bytes = (size_t)nfields * 52;
+    const size_t tab_bytes = (size_t)nfields * 60;
     VHSD_HIP(c->f_tab.ensure(tab_bytes));
     // smoothing powers table: pow[s] = (1 - alpha)^(s + 1), s < output_line_length
     const bool smooth = p->wow_level_smoothing > 0.0;
@@ -844,7 +863,8 @@ int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod_context* c, const
     const size_t tab_off = (size_t)(tab - stg);
     // table layout (byte offsets, 8-byte slots): spans, lens, locs, fil, ref, fo, then int line counts
     const size_t o_sp = 0, o_ln = 8 * (size_t)nfields, o_lc = 16 * (size_t)nfields, o_fi = 24 * (size_t)nfields,
-                 o_rf = 32 * (size_t)nfields, o_fo = 40 * (size_t)nfields, o_cnt = 48 * (size_t)nfields;
+                 o_rf = 32 * (size_t)nfields, o_fo = 40 * (size_t)nfields, o_org = 48 * (size_t)nfields,
+                 o_cnt = 56 * (size_t)nfields;
     const size_t used_tab = o_cnt + 4 * (size_t)nfields;
     if (tab_off + used_tab > bytes) return fail(VHSDECODE_HIP_E_INTERNAL, "staging overflow");
     for (int f = 0; f < nfields; f++) {
@@ -857,6 +877,8 @@ int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod_context* c, const
       std::memcpy(tab + o_fi + 8 * (size_t)f, &vp[3], 8);
       std::memcpy(tab + o_rf + 8 * (size_t)f, &vp[4], 8);
       std::memcpy(tab + o_fo + 8 * (size_t)f, &vp[5], 8);
+      const long long org = origins ? (long long)origins[f] : 0;
+      std::memcpy(tab + o_org + 8 * (size_t)f, &org, 8);
       std::memcpy(tab + o_cnt + 4 * (size_t)f, &nloc, 4);
     }
     VHSD_HIP(hipMemcpyAsync(c->f_loc.p, stg, nl * sizeof(double), hipMemcpyHostToDevice, st));
@@ -871,17 +893,20 @@ int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod_context* c, const
     double* const* d_refs = reinterpret_cast<double* const*>(dtab + o_rf);
     unsigned char* const* d_fos = reinterpret_cast<unsigned char* const*>(dtab + o_fo);
     const int* d_lcount = reinterpret_cast<const int*>(dtab + o_cnt);
+    const long long* d_org = reinterpret_cast<const long long*>(dtab + o_org);
     vhsd::HsyncParams hp{p->threshold, p->sample_rate_mhz, p->hsync_pulse_us, p->ire0, p->hz_ire, p->is_pal,
                          p->use_right_hsync};
     {
       ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_HSYNC, nfields);
-      VHSD_HIP(vhsd_launch_hsync(d_spans, d_lens, d_locs, d_fils, d_lcount, &hp, d_refs, d_fos, nfields, st));
+      VHSD_HIP(c->h_work.ensure(vhsd_hsync_work_bytes(nfields, nloc)));
+      VHSD_HIP(vhsd_launch_hsync(d_spans, d_lens, d_locs, d_fils, d_lcount, &hp, d_refs, d_fos, nfields, nloc,
+                                 c->bias, c->h_work.p, st));
     }
     const double out_scale = p->output_scale;
     const double conv_off = (p->output_zero + 0.5) - (p->vsync_ire * out_scale);
     {
       ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_TBC, (long long)nfields * p->field_lines * p->output_line_length);
-      VHSD_HIP(vhsd_launch_tbc_fields(video, span_length, c->f_ref.p, nloc, p->field_lines, nfields,
+      VHSD_HIP(vhsd_launch_tbc_fields(video, span_length, c->bias, d_org, c->f_ref.p, nloc, p->field_lines, nfields,
                                       p->nominal_line_length, line_decay, smooth ? c->f_pow.p : nullptr, c->f_x.p,
                                       c->f_y0.p, p->output_line_length, smooth ? 1 : 0, p->ire0, out_scale,
                                       1.0 / p->hz_ire, conv_off, c->lut.p, out, st));

@@ -57,7 +57,8 @@ class FiltersV1(ctypes.Structure):
                 ("sample_rate_hz", ctypes.c_double), ("rf_video", _dp), ("video", _dp),
                 ("video_lowpass05", _dp), ("video_lowpass05_offset", ctypes.c_int32), ("envelope_sos", _dp),
                 ("envelope_sections", ctypes.c_int32), ("chroma_sos", _dp), ("chroma_sections", ctypes.c_int32),
-                ("chroma_offset", ctypes.c_int32), ("diff_demod_max", ctypes.c_double)]
+                ("chroma_offset", ctypes.c_int32), ("diff_demod_max", ctypes.c_double),
+                ("plane_bias_hz", ctypes.c_double)]
 
 
 class DemodConfigV1(ctypes.Structure):
@@ -144,7 +145,8 @@ def load():
                                                     ctypes.POINTER(TbcParamsV1), ctypes.c_void_p, ctypes.c_void_p,
                                                     ctypes.c_void_p]
     L.vhsdecode_hip_decode_fields_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                                     ctypes.c_int64, _dp, _u8p, ctypes.c_int32,
+                                                     ctypes.c_int64, _dp, _u8p, ctypes.POINTER(ctypes.c_int64),
+                                                     ctypes.c_int32,
                                                      ctypes.POINTER(FieldParamsV1), ctypes.c_void_p, _dp,
                                                      ctypes.c_void_p]
     L.vhsdecode_hip_set_profiling.argtypes = [ctypes.c_void_p, ctypes.c_int32]

@@ -30,11 +30,20 @@ def _ptr(a: np.ndarray, t):
 class DecodedSpan:
     """Stitched demodulated planes of consecutive blocks (RfDecodedSpan)."""
 
-    video: np.ndarray
-    video_lowpass: np.ndarray
+    video: np.ndarray             # FP32 offsets from `bias` (Hz)
+    video_lowpass: np.ndarray     # FP32 offsets from `bias` (Hz)
     envelope: np.ndarray
     chroma: Optional[np.ndarray]
     block_flags: np.ndarray
+    bias: float = 0.0
+
+    @property
+    def video_hz(self) -> np.ndarray:
+        return self.video.astype(np.float64) + self.bias
+
+    @property
+    def video_lowpass_hz(self) -> np.ndarray:
+        return self.video_lowpass.astype(np.float64) + self.bias
 
     @property
     def weak_rf(self) -> bool:
@@ -72,6 +81,8 @@ class HipBlockDemodulator:
         f.chroma_sections = 0 if self._chr is None else self._chr.shape[0]
         f.chroma_offset = int(filters.chroma_offset)
         f.diff_demod_max = float(filters.diff_demod_max) if filters.diff_demod_max is not None else -1.0
+        f.plane_bias_hz = float(filters.plane_bias_hz)
+        self.bias = float(filters.plane_bias_hz)
         cfg = _lib.DemodConfigV1()
         cfg.struct_size = ctypes.sizeof(_lib.DemodConfigV1)
         cfg.device_id = device
@@ -125,7 +136,7 @@ class HipBlockDemodulator:
         span.block_flags = _ptr(flags, _lib._u32p)
         _lib.check(self._L.vhsdecode_hip_demod_span_host(self._h, _ptr(rf, _lib._i16p), rf.size, first_sample,
                                                          nblocks, ctypes.byref(span)))
-        return DecodedSpan(out["video"], out["vlp"], out["env"], ch, flags)
+        return DecodedSpan(out["video"], out["vlp"], out["env"], ch, flags, self.bias)
 
     def demod_span_device(self, rf_ptr: int, first_sample: int, nblocks: int, video_ptr: int, vlp_ptr: int,
                           env_ptr: int, chroma_ptr: int = 0, flags_ptr: int = 0, stream: int = 0):
@@ -192,7 +203,8 @@ class HipBlockDemodulator:
         return u16, f64
 
     def decode_fields_device(self, sync_ref_ptr: int, video_ptr: int, span_length: int, locations: np.ndarray,
-                             filled: np.ndarray, geom, out_ptr: int, want_refined: bool = False, stream: int = 0):
+                             filled: np.ndarray, geom, out_ptr: int, origins: Optional[np.ndarray] = None,
+                             want_refined: bool = False, stream: int = 0):
         """Batched hsync refinement + level adjusts + TBC for fields [nfields, lines + 1] of line locations;
         uint16 luma to device memory at out_ptr ([nfields, lines * out_len])."""
         loc = np.ascontiguousarray(locations, dtype=np.float64)
@@ -203,9 +215,11 @@ class HipBlockDemodulator:
                                geom.hsync / geom.fs_mhz, ire0, hz_ire, 1 if geom.is_pal else 0, 1, geom.out_len,
                                geom.field_lines, geom.nominal, geom.smoothing, ozero, vsync, oscale)
         ref = np.empty_like(loc) if want_refined else None
+        org = None if origins is None else np.ascontiguousarray(origins, dtype=np.int64)
         _lib.check(self._L.vhsdecode_hip_decode_fields_device(
             self._h, ctypes.c_void_p(sync_ref_ptr), ctypes.c_void_p(video_ptr), span_length, _ptr(loc, _lib._dp),
-            _ptr(fil, _lib._u8p), nfields, ctypes.byref(p), ctypes.c_void_p(out_ptr),
+            _ptr(fil, _lib._u8p), _ptr(org, ctypes.POINTER(ctypes.c_int64)) if org is not None else None, nfields,
+            ctypes.byref(p), ctypes.c_void_p(out_ptr),
             _ptr(ref, _lib._dp) if ref is not None else None, ctypes.c_void_p(stream)))
         return ref
 

@@ -116,6 +116,19 @@ def line_locations_from_pulses(starts: np.ndarray, lens: np.ndarray, g: SyncGeom
     return out
 
 
+def field_origin(line0: float, g: SyncGeometry, block_length: int = 32768, block_cut: int = 1024,
+                 stride: int = 30720, lead_lines: float = 8.0) -> int:
+    """Start of a field's read window in span coordinates, placed the way
+    DecodeReadWindowPlanner.Resolve places it (Decode/DecodeReadWindowPlanner.cs:37-63):
+    the intended start (here `lead_lines` before line 0) less the block cut, aligned
+    down to a block, expressed as a multiple of the block stride.  TBC source
+    coordinates count from it, so their FP32 resolution matches the reference's."""
+    intended = max(0.0, line0 - lead_lines * g.line_len)
+    read = max(0, int(intended) - block_cut)
+    aligned = (read // block_length) * block_length
+    return (aligned // stride) * stride
+
+
 class HipFieldDecoder:
     """Decodes every complete field of a device-resident demodulated span:
     pulses (device) -> line locations (host) -> batched hsync + TBC (device)."""
@@ -138,6 +151,7 @@ class HipFieldDecoder:
             return fields, None
         locs = np.stack([f.locations for f in fields])
         filled = np.stack([f.filled for f in fields]).astype(np.uint8)
+        origins = np.array([field_origin(f.line0, self.geom) for f in fields], dtype=np.int64)
         ref = self.demod.decode_fields_device(span_vlp_ptr, span_video_ptr, span_len, locs, filled, self.geom,
-                                              out_ptr, want_refined=want_refined)
+                                              out_ptr, origins=origins, want_refined=want_refined)
         return fields, ref

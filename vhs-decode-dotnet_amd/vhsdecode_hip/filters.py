@@ -44,6 +44,7 @@ class TapeFilterSet:
     chroma_offset: int
     video_lowpass05_offset: int
     diff_demod_max: Optional[float]
+    plane_bias_hz: float = 0.0    # mid-level carrier; FP32 planes are stored as offsets from it
 
 
 # --------------------------------------------------------------------------- responses
@@ -329,5 +330,7 @@ def build_tape_filters(fmt: FormatParameters, sample_rate_hz: float = 40e6, bloc
         chroma_offset = int(float(rf.get("chroma_offset") or 5.0) * (fs / 40_000_000.0))
     sp = fmt.sysparams
     diff_max = (float(sp["ire0"]) + float(sp["hz_ire"]) * 100.0) * 2.0
+    # the 50 IRE carrier, rounded to a whole Hz so it is exact in FP32
+    bias = float(np.round(float(sp["ire0"]) + 50.0 * float(sp["hz_ire"])))
     return TapeFilterSet(n, fs, rf_video, video, vlp, vlp05, env, chroma, chroma_offset,
-                         VIDEO_LOWPASS05_OFFSET, diff_max)
+                         VIDEO_LOWPASS05_OFFSET, diff_max, bias)
