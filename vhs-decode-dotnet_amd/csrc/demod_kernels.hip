@@ -30,10 +30,17 @@ namespace vhsd {
 // LDS layout shared by the block kernels (bytes): 128 KiB exchange buffer,
 // 16 KiB twiddle octant, small scratch.
 struct BlockLds {
-  c32 buf[kFftM];
+  c32 buf[kFftBufEntries];
   float2 oct[kOctantEntries];
   float scratch[kScratchFloats];
 };
+
+// Strided element k = j + 1024 m of a global array through a uniform row pointer, so
+// the address is an SGPR base plus the lane offset instead of a 64-bit VGPR per m.
+template <typename T>
+__device__ __forceinline__ T& row_at(T* base, int m) {
+  return (base + (long long)m * kFftThreads)[threadIdx.x];
+}
 
 __device__ __forceinline__ void load_octant(BlockLds& s, const float2* __restrict__ oct_g) {
   for (int u = threadIdx.x; u < kOctantEntries; u += blockDim.x) s.oct[u] = oct_g[u];
@@ -41,17 +48,26 @@ __device__ __forceinline__ void load_octant(BlockLds& s, const float2* __restric
 
 // rfft post-processing: Z = FFT16384(x[2m] + i x[2m+1]) held in strided
 // distribution -> X[k] for k = j + 1024 m (and X[16384] returned via xm on thread 0).
+// Strided slot m of thread j in the padded exchange buffer, and the slot holding
+// entry (M - k) mod M for k = j + 1024 m: thread 1024 - j, slot 15 - m (thread 0:
+// slot (16 - m) mod 16 of thread 0).
+constexpr int kPadRow = kFftThreads + kFftThreads / 32;
+__device__ __forceinline__ int mirror_slot(int j, int m) {
+  return j > 0 ? lds_pad(kFftThreads - j) + (15 - m) * kPadRow : ((16 - m) & 15) * kPadRow;
+}
+
 __device__ __forceinline__ void rfft_post(c32* v, BlockLds& s, c32& xm) {
   const int j = threadIdx.x;
+  const int pj = lds_pad(j);
 #pragma unroll
-  for (int m = 0; m < 16; m++) s.buf[j + m * kFftThreads] = v[m];
+  for (int m = 0; m < 16; m++) s.buf[pj + m * kPadRow] = v[m];
   __syncthreads();
   c32 z0 = s.buf[0];
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     const int k = j + m * kFftThreads;
     const c32 zk = v[m];
-    const c32 zc = conjc(s.buf[(kFftM - k) & (kFftM - 1)]);
+    const c32 zc = conjc(s.buf[mirror_slot(j, m)]);
     const c32 e = add(zk, zc);
     const c32 o = sub(zk, zc);
     // X[k] = 0.5*(e - i*W^k*o)
@@ -67,8 +83,9 @@ __device__ __forceinline__ void rfft_post(c32* v, BlockLds& s, c32& xm) {
 // numpy.fft.irfft semantics: imaginary parts of V[0] and V[M] are ignored.
 __device__ __forceinline__ void irfft_pre(c32* v, c32 vm, BlockLds& s) {
   const int j = threadIdx.x;
+  const int pj = lds_pad(j);
 #pragma unroll
-  for (int m = 0; m < 16; m++) s.buf[j + m * kFftThreads] = v[m];
+  for (int m = 0; m < 16; m++) s.buf[pj + m * kPadRow] = v[m];
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < 16; m++) {
@@ -79,7 +96,7 @@ __device__ __forceinline__ void irfft_pre(c32* v, c32 vm, BlockLds& s) {
       y = c32{a + b, a - b};
     } else {
       const c32 vk = v[m];
-      const c32 vc = conjc(s.buf[kFftM - k]);
+      const c32 vc = conjc(s.buf[mirror_slot(j, m)]);
       const c32 e = add(vk, vc);
       const c32 o = mul(sub(vk, vc), conjc(tw32k(s.oct, k)));
       y = add(e, mul_pi(o));
@@ -89,42 +106,54 @@ __device__ __forceinline__ void irfft_pre(c32* v, c32 vm, BlockLds& s) {
   __syncthreads();
 }
 
-// Stage B+: analytic signal from the filtered half spectrum F (strided, F[M] in fm),
-// FM discriminator and real part.  Writes demod (2 per m) and the real part into
-// dem[32] / re[32] laid out as [2m] = even sample 2*(j+1024m), [2m+1] = odd sample.
-__device__ __forceinline__ void analytic_demod(c32* F, c32 fm, BlockLds& s, float fs, float* dem, float* re,
-                                               float* theta_e) {
+// Stage B+: analytic signal from the filtered half spectrum F (strided, F[M] in fm)
+// and the FM discriminator.  Writes demod into dem[32] laid out as [2m] = even sample
+// 2*(j+1024m), [2m+1] = odd sample, and leaves the analytic real parts (the envelope
+// source) in the scratch area as float2(even, odd) per k = j + 1024 m.  State that
+// must outlive a transform goes through the block's scratch area (`stash`, written
+// and read back by this workgroup only) instead of registers.
+__device__ __forceinline__ void analytic_demod(c32* F, c32 fm, BlockLds& s, float fs, float* dem,
+                                               float2* __restrict__ stash) {
   const int j = threadIdx.x;
   const float inv_n = 1.0f / 32768.0f;
-  c32 v[16];
+  float2* st_go = stash;            // odd-pass input spectrum
+  float2* st_th = stash + kFftM;    // (angle, real part) of the even samples, then real parts
+  // odd output samples' input, parked until the even pass is done
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    const c32 g = (k == 0) ? sub(F[m], fm) : mul(scale(F[m], 2.0f), conjc(tw32k(s.oct, k)));
+    row_at(st_go, m) = make_float2(g.x, g.y);
+  }
   // even output samples
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     const int k = j + m * kFftThreads;
-    v[m] = (k == 0) ? add(F[m], fm) : scale(F[m], 2.0f);
+    F[m] = (k == 0) ? add(F[m], fm) : scale(F[m], 2.0f);
   }
-  fft16k_inverse(v, s.buf, s.oct);
+  fft16k_inverse(F, s.buf, s.oct);
 #pragma unroll
   for (int m = 0; m < 16; m++) {
-    const c32 a = scale(v[m], inv_n);
-    re[2 * m] = a.x;
-    theta_e[m] = vhs_atan2(a.y, a.x);
+    const c32 a = scale(F[m], inv_n);
+    row_at(st_th, m) = make_float2(vhs_atan2(a.y, a.x), a.x);
   }
-  // odd output samples
 #pragma unroll
   for (int m = 0; m < 16; m++) {
-    const int k = j + m * kFftThreads;
-    v[m] = (k == 0) ? sub(F[m], fm) : mul(scale(F[m], 2.0f), conjc(tw32k(s.oct, k)));
+    const float2 g = row_at(st_go, m);
+    F[m] = c32{g.x, g.y};
   }
-  fft16k_inverse(v, s.buf, s.oct);
+  fft16k_inverse(F, s.buf, s.oct);
   float* th_o = reinterpret_cast<float*>(s.buf);
+  float theta_e[16];
 #pragma unroll
   for (int m = 0; m < 16; m++) {
-    const c32 a = scale(v[m], inv_n);
-    re[2 * m + 1] = a.x;
+    const float2 te = row_at(st_th, m);
+    theta_e[m] = te.x;
+    const c32 a = scale(F[m], inv_n);
     const float t = vhs_atan2(a.y, a.x);
-    dem[2 * m + 1] = vhs_freq_diff(t, theta_e[m], fs);
+    dem[2 * m + 1] = vhs_freq_diff(t, te.x, fs);
     th_o[j + m * kFftThreads] = t;
+    row_at(st_th, m) = make_float2(te.y, a.x);
   }
   __syncthreads();
 #pragma unroll
@@ -142,7 +171,8 @@ __device__ __forceinline__ void analytic_demod(c32* F, c32 fm, BlockLds& s, floa
 
 // Video and sync-reference planes from the demodulated signal held as
 // dem[2m], dem[2m+1] (samples 2k, 2k+1, k = j + 1024 m).
-__device__ __forceinline__ void video_stage(const float* dem, BlockLds& s, const DemodArgs& a, long long out_base) {
+__device__ __forceinline__ void video_stage(const float* dem, BlockLds& s, const DemodArgs& a, long long out_base,
+                                            float2* __restrict__ stash) {
   const int j = threadIdx.x;
   // The ~4 MHz carrier level would dominate the FP32 rounding of the transforms; remove
   // the plane bias first (exact: both are near the same binade) and restore the DC the
@@ -154,6 +184,9 @@ __device__ __forceinline__ void video_stage(const float* dem, BlockLds& s, const
   c32 dm;
   rfft_post(D, s, dm);
   const float inv_n = 1.0f / 32768.0f;
+  // the spectrum feeds two inverse transforms; park it rather than hold it in registers
+#pragma unroll
+  for (int m = 0; m < 16; m++) row_at(stash, m) = make_float2(D[m].x, D[m].y);
 #pragma unroll 1
   for (int pass = 0; pass < 2; pass++) {
     const float2* filt = pass == 0 ? a.video : a.vlp05;
@@ -163,23 +196,25 @@ __device__ __forceinline__ void video_stage(const float* dem, BlockLds& s, const
     c32 v[16];
 #pragma unroll
     for (int m = 0; m < 16; m++) {
-      const float2 f = filt[j + m * kFftThreads];
-      v[m] = mul(D[m], c32{f.x, f.y});
+      const float2 f = row_at(filt, m);
+      const float2 d = row_at(stash, m);
+      v[m] = mul(c32{d.x, d.y}, c32{f.x, f.y});
     }
     const float2 fM = filt[kFftM];
     const c32 vm = mul(dm, c32{fM.x, fM.y});
     irfft_pre(v, vm, s);
     fft16k_inverse(v, s.buf, s.oct);
     // v[m] = N * (y[2k] + i y[2k+1]); output sample n = 2k lands at n - shift (roll by -shift)
+    // position o = n - out_begin (even: out_begin, shift and out_base are even)
+    float2* orow = reinterpret_cast<float2*>(out + out_base - shift - a.out_begin);
 #pragma unroll
     for (int m = 0; m < 16; m++) {
       const int k = j + m * kFftThreads;
-      const int n = 2 * k - shift;  // even
-      const int o = n - a.out_begin;  // even: out_begin, shift and out_base are even
+      const int o = 2 * k - shift - a.out_begin;
       if (o >= 0 && o < a.out_count) {
         const float2 val = make_float2(v[m].x * inv_n + corr, v[m].y * inv_n + corr);
         if (o + 1 < a.out_count)
-          *reinterpret_cast<float2*>(out + out_base + o) = val;
+          row_at(orow, m) = val;
         else
           out[out_base + o] = val.x;
       }
@@ -200,8 +235,8 @@ __global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) {
   const int16_t* x = a.rf + start;
 #pragma unroll
   for (int m = 0; m < 16; m++) {
-    const int k = j + m * kFftThreads;
-    v[m] = c32{(float)x[2 * k], (float)x[2 * k + 1]};
+    const int16_t* xr = x + 2 * m * kFftThreads;
+    v[m] = c32{(float)xr[2 * j], (float)xr[2 * j + 1]};
   }
   __syncthreads();
   fft16k_forward(v, s.buf, s.oct);
@@ -210,7 +245,7 @@ __global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) {
   // ---- filter by RfVideo (RfMtf is all ones for tape)
 #pragma unroll
   for (int m = 0; m < 16; m++) {
-    const float2 f = a.rf_video[j + m * kFftThreads];
+    const float2 f = row_at(a.rf_video, m);
     v[m] = mul(v[m], c32{f.x, f.y});
   }
   const float2 fM = a.rf_video[kFftM];
@@ -225,8 +260,9 @@ __global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) {
   fm_b = c32{s.scratch[0], s.scratch[1]};
   __syncthreads();
   // ---- stage B: analytic signal + discriminator
-  float dem[32], re[32], th[16];
-  analytic_demod(v, fm_b, s, a.fs, dem, re, th);
+  float2* stash = a.stash + (long long)blk * kStashEntries;
+  float dem[32];
+  analytic_demod(v, fm_b, s, a.fs, dem, stash);
   // spike check over demod[20 .. N-20)
   bool spike = false;
   if (a.diff_max > 0.0f) {
@@ -239,22 +275,26 @@ __global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) {
   }
   // ---- envelope: raw[(n + 4) mod N] = |float(re[n])|, FP32 sosfiltfilt
   float* raw = reinterpret_cast<float*>(s.buf);
+  const float2* st_re = stash + kFftM;
 #pragma unroll
   for (int m = 0; m < 16; m++) {
-    const int n0 = 2 * (j + m * kFftThreads);
-    raw[(n0 + 4) & 32767] = fabsf(re[2 * m]);
-    raw[(n0 + 5) & 32767] = fabsf(re[2 * m + 1]);
+    const int k = j + m * kFftThreads;
+    const int n0 = 2 * k;
+    const float2 r = row_at(st_re, m);
+    raw[(n0 + 4) & 32767] = fabsf(r.x);
+    raw[(n0 + 5) & 32767] = fabsf(r.y);
   }
   __syncthreads();
-  const bool weak = sos_filtfilt_block(a.env, raw, 32768, s.scratch, a.out_env + out_base, a.out_begin,
-                                       a.out_count, /*roll=*/0);
+  // the VHS envelope filter is first order (DecodeFilterSet.BuildVhsEnvelopeSos): one section
+  const bool weak = sos_filtfilt_block_t<1>(a.env, raw, 32768, s.scratch, a.out_env + out_base, a.out_begin,
+                                            a.out_count, /*roll=*/0, false);
   // ---- flags
   const unsigned long long any_spike = __ballot(spike);
   if ((j & 63) == 0 && any_spike) atomicOr(&a.flags[blk], 2u);
   if (j == 0 && weak) atomicOr(&a.flags[blk], 1u);
   __syncthreads();
   // ---- stage C: video and sync reference
-  video_stage(dem, s, a, out_base);
+  video_stage(dem, s, a, out_base, stash);
 }
 
 // ------------------------------------------------------------------ chroma
@@ -392,7 +432,7 @@ __global__ void __launch_bounds__(1024) vhs_block_repair_kernel(RepairArgs r) {
     dem[2 * m + 1] = dm[2 * k + 1];
   }
   __syncthreads();
-  video_stage(dem, s, a, out_base);
+  video_stage(dem, s, a, out_base, r.d.stash + (long long)blk * kStashEntries);
 }
 
 // ------------------------------------------------------------------ diagnostics

@@ -22,6 +22,11 @@ constexpr int kFftM = 16384;          // complex points per transform
 constexpr int kFftE = 16;             // complex values per thread
 constexpr int kOctant = 2048;         // octant table covers u in [0, 2048] of the 16384-th roots
 constexpr int kOctantEntries = kOctant + 1;
+// The exchange buffer is padded by one entry every 32 so the radix-16 passes'
+// strided writes spread over the LDS banks.
+constexpr int kFftBufEntries = kFftM + kFftM / 32;
+
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
 
 struct c32 {
   float x, y;
@@ -113,7 +118,7 @@ __device__ __forceinline__ void dft16(c32* v) {
 }
 
 // Forward FFT of 16384 points held in strided distribution.  `buf` is the
-// 16384-entry LDS exchange buffer; the caller guarantees it is free and that
+// padded LDS exchange buffer (kFftBufEntries, indexed through lds_pad); the caller guarantees it is free and that
 // all threads reach this call.  Output in the same distribution.
 __device__ __forceinline__ void fft16k_forward(c32* v, c32* __restrict__ buf, const float2* __restrict__ oct) {
   const int j = threadIdx.x;
@@ -129,12 +134,16 @@ __device__ __forceinline__ void fft16k_forward(c32* v, c32* __restrict__ buf, co
       for (int r = 1; r < 16; r++) v[r] = mul(v[r], tw16k(oct, k * r * stride));
     }
     dft16(v);
-    const int base = (j >> (4 * pass)) * ns * 16 + k;
+    // lds_pad(base + c*ns) == lds_pad(base) + c*ns + ((c*ns) >> 5) in every pass (the low
+    // five bits of base and c*ns never carry into each other), so the per-c offsets
+    // are compile-time constants; likewise for the strided reads.
+    const int base = lds_pad((j >> (4 * pass)) * ns * 16 + k);
 #pragma unroll
-    for (int c = 0; c < 16; c++) buf[base + c * ns] = v[c];
+    for (int c = 0; c < 16; c++) buf[base + c * ns + ((c * ns) >> 5)] = v[c];
     __syncthreads();
+    const int rbase = lds_pad(j);
 #pragma unroll
-    for (int m = 0; m < 16; m++) v[m] = buf[j + m * kFftThreads];
+    for (int m = 0; m < 16; m++) v[m] = buf[rbase + m * (kFftThreads + kFftThreads / 32)];
     __syncthreads();
   }
   // radix-4 pass, Ns = 4096: butterfly b = j + 1024 q on elements v[q + 4 r]

@@ -244,6 +244,16 @@ __device__ bool area_has_nan(const WaveCtx& w, double crossing, int one_us, doub
   return __ballot(nan) != 0;
 }
 
+// Broadcast lane k's double to the whole wave (k uniform).
+__device__ __forceinline__ double wave_read(double v, int k) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), k);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), k);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// TryMeanRange: the sum runs left to right in FP64 like the reference; lanes load a
+// chunk in parallel and every lane accumulates the broadcast values in order.
 __device__ bool mean_range(const WaveCtx& w, double start, double end, double* mean) {
   long long a = iround(start);
   a = a < 0 ? 0 : (a > w.n ? w.n : a);
@@ -254,9 +264,12 @@ __device__ bool mean_range(const WaveCtx& w, double start, double end, double* m
     return false;
   }
   double sum = 0.0;
-  if (w.lane == 0)
-    for (long long i = a; i < b; i++) sum += w.at(i);
-  sum = __shfl(sum, 0);
+  for (long long base = a; base < b; base += 64) {
+    const long long i = base + w.lane;
+    const double v = i < b ? w.at(i) : 0.0;
+    const int cnt = (int)(b - base < 64 ? b - base : 64);
+    for (int k = 0; k < cnt; k++) sum += wave_read(v, k);
+  }
   *mean = sum / (double)(b - a);
   return true;
 }
@@ -394,8 +407,8 @@ struct HsyncWork {
   double* rt_porch;
   double* p_used;        // porch level the speculative pass assumed
   double* p_out;         // porch level the speculative pass carried out
-  int* fb_line;          // per field: first line to recompute sequentially (== lines: none)
-  double* fb_porch;      // per field: porch level entering that line
+  int* fb_line;          // per field: number of lines the ordered walk refined again
+  double* fb_porch;      // reserved
   int stride;            // entries per field (max lines)
 };
 
@@ -464,58 +477,42 @@ __global__ void __launch_bounds__(64) hsync_spec_kernel(const float* const* span
   }
 }
 
-// Pass 3: walk the carried porch level sequentially (one thread per field) and find
-// the first line whose speculation does not hold.
-__global__ void hsync_verify_kernel(const int* line_counts, int fields, HsyncWork wk) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= fields) return;
-  const int lines = line_counts[f];
-  const int base = f * wk.stride;
-  double pc = -1.0;
-  int fb = lines;
-  for (int l = 0; l < lines; l++) {
-    if (wk.kind[base + l] != 0) continue;
-    const double pu = wk.p_used[base + l];
-    if (__double_as_longlong(pu) != __double_as_longlong(pc)) {
-      fb = l;
-      break;
-    }
-    pc = wk.p_out[base + l];
-  }
-  wk.fb_line[f] = fb;
-  wk.fb_porch[f] = pc;
-}
-
-// Pass 4: sequential refinement from the first mis-speculated line (no-op for
-// fields whose speculation held everywhere).
-__global__ void __launch_bounds__(64) hsync_refine_kernel(const float* const* spans, const long long* span_len,
+// Pass 3: one wavefront per field walks the carried porch level in line order.  A
+// line whose speculative porch matches the true one keeps its parallel result; any
+// other line is refined again here with the true porch, exactly as the sequential
+// reference would, and the walk continues from its result.
+__global__ void __launch_bounds__(64) hsync_verify_kernel(const float* const* spans, const long long* span_len,
                                                           const double* const* locs, const unsigned char* const* filled,
                                                           const int* line_counts, HsyncParams p, HsyncWork wk,
                                                           double* const* refined_out,
                                                           unsigned char* const* filled_out, double bias) {
   const int f = blockIdx.x;
   const int lines = line_counts[f];
-  const int start = wk.fb_line[f];
-  if (start >= lines) return;
   WaveCtx w{spans[f], span_len[f], (int)threadIdx.x, bias};
   const HsyncGeom g = hsync_geom(p);
   const int base = f * wk.stride;
-  double prev_porch = wk.fb_porch[f];
-  for (int line = start; line < lines; line++) {
-    const int k = wk.kind[base + line];
-    if (k != 0) continue;  // skipped lines were already written by the parallel pass
+  double pc = -1.0;
+  int redone = 0;
+  for (int line = 0; line < lines; line++) {
+    if (wk.kind[base + line] != 0) continue;
+    if (__double_as_longlong(wk.p_used[base + line]) == __double_as_longlong(pc)) {
+      pc = wk.p_out[base + line];
+      continue;
+    }
     const double cur = locs[f][line];
-    long long ss = iround(cur - g.one_us);
+    const long long ss = iround(cur - g.one_us);
     double loc = 0.0, pout = 0.0;
     unsigned char fil = 0;
-    refine_line(w, p, g, ss, cur, filled[f][line], prev_porch, wk.rt_ok[base + line] != 0,
-                wk.rt_cand[base + line], wk.rt_porch[base + line], &loc, &fil, &pout);
-    prev_porch = pout;
+    refine_line(w, p, g, ss, cur, filled[f][line], pc, wk.rt_ok[base + line] != 0, wk.rt_cand[base + line],
+                wk.rt_porch[base + line], &loc, &fil, &pout);
+    pc = pout;
+    redone++;
     if (w.lane == 0) {
       refined_out[f][line] = loc;
       filled_out[f][line] = fil;
     }
   }
+  if (w.lane == 0) wk.fb_line[f] = redone;
 }
 
 // ------------------------------------------------------------------ TBC resample
@@ -811,9 +808,7 @@ hipError_t vhsd_launch_hsync(const float* const* spans, const long long* span_le
                      bias);
   hipLaunchKernelGGL(vhsd::hsync_spec_kernel, lg, dim3(64), 0, stream, spans, span_len, locs, filled, line_counts, *p,
                      wk, refined_out, filled_out, bias);
-  hipLaunchKernelGGL(vhsd::hsync_verify_kernel, dim3((fields + 63) / 64), dim3(64), 0, stream, line_counts, fields,
-                     wk);
-  hipLaunchKernelGGL(vhsd::hsync_refine_kernel, dim3(fields), dim3(64), 0, stream, spans, span_len, locs, filled,
+  hipLaunchKernelGGL(vhsd::hsync_verify_kernel, dim3(fields), dim3(64), 0, stream, spans, span_len, locs, filled,
                      line_counts, *p, wk, refined_out, filled_out, bias);
   return hipGetLastError();
 }

@@ -7,6 +7,9 @@
 
 namespace vhsd {
 
+// Per-block scratch of the demod kernels: two spectra of 16384 complex FP32 values.
+constexpr long long kStashEntries = 2 * 16384 + 64;
+
 struct DemodArgs {
   const int16_t* rf;             // RF samples (device)
   const long long* block_start;  // input sample offset of each block
@@ -28,6 +31,7 @@ struct DemodArgs {
   float* out_vlp;
   float* out_env;
   unsigned* flags;               // per block: bit0 weak RF, bit1 spike repair needed
+  float2* stash;                 // [nblocks][kStashEntries] scratch, private to each block's workgroup
 };
 
 struct ChromaArgs {

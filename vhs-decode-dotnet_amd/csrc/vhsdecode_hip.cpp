@@ -220,6 +220,7 @@ struct vhsdecode_hip_demod_context {
   DevBuf<long long> block_start, out_offset;
   DevBuf<unsigned> flags;
   DevBuf<float2> rep_a;
+  DevBuf<float2> stash;
   DevBuf<float> rep_dm;
   // pulse scratch
   DevBuf<int> tile_counts, counters, p_starts, p_lens;
@@ -263,7 +264,7 @@ struct vhsdecode_hip_demod_context {
     (void)hipSetDevice(device);
     rfv.release(); vid.release(); vlp.release(); oct.release();
     env_trans.release(); chroma_trans.release(); block_start.release(); out_offset.release(); flags.release();
-    rep_a.release(); rep_dm.release(); tile_counts.release(); counters.release(); p_starts.release();
+    rep_a.release(); rep_dm.release(); stash.release(); tile_counts.release(); counters.release(); p_starts.release();
     p_lens.release(); edge_pos.release(); edge_kind.release(); h_loc.release(); h_ref.release();
     h_fil.release(); h_fo.release(); h_ptrs.release(); h_work.release(); lut.release(); t_loc.release(); t_adj.release();
     t_f64.release(); t_u16.release(); s_rf.release(); s_planes.release(); s_flags.release();
@@ -365,6 +366,8 @@ int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int
   a.out_vlp = out->video_lowpass;
   a.out_env = out->envelope;
   a.flags = flags;
+  VHSD_HIP(c->stash.ensure((size_t)nblocks * vhsd::kStashEntries));
+  a.stash = c->stash.p;
   {
     ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_BLOCK_DEMOD, nblocks);
     VHSD_HIP(vhsd_launch_block_demod(&a, nblocks, st));
@@ -440,6 +443,8 @@ int32_t vhsdecode_hip_demod_create(const vhsdecode_hip_demod_config_v1* cfg, vhs
       return fail(VHSDECODE_HIP_E_UNSUPPORTED, "this build demodulates 32768-sample blocks");
     if (!f->rf_video || !f->video || !f->video_lowpass05 || !f->envelope_sos || f->envelope_sections <= 0)
       return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "rf_video, video, video_lowpass05 and envelope_sos are required");
+    if (f->envelope_sections != 1)
+      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "the envelope filter must be a single section (first-order low-pass)");
     if (!(f->sample_rate_hz > 0.0)) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "sample_rate_hz must be positive");
     if (cfg->block_cut < 0 || cfg->block_cut_end < 0 || cfg->block_cut + cfg->block_cut_end >= f->block_length ||
         (cfg->block_cut & 1) || (cfg->block_cut_end & 1))

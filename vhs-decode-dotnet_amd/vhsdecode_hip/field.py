@@ -13,8 +13,8 @@ of sequential heuristics for damaged tape.  `line_locations_from_pulses` is this
 backend's own compact equivalent for well-formed sync: pulses are classified by
 length, each vertical-sync group of broad pulses anchors a field, its parity comes
 from the half-line phase of the broad group against the preceding hsync, and every
-line start is snapped to the pulse that opens it (or carried forward one nominal
-line when none does, marked `filled`).  Its agreement with the reference's sync
+line start is snapped to the pulse that opens it (lines no pulse opens are
+interpolated from their measured neighbours and marked `filled`).  Its agreement with the reference's sync
 analysis on irregular signals is unpinned; on the synthetic tapes it is checked
 against the generator's closed-form line starts.
 """
@@ -80,39 +80,33 @@ def line_locations_from_pulses(starts: np.ndarray, lens: np.ndarray, g: SyncGeom
                                tol: float = 0.12) -> List[FieldLines]:
     starts = np.asarray(starts, dtype=np.float64)
     lens = np.asarray(lens, dtype=np.float64)
+    out: List[FieldLines] = []
+    if starts.size == 0:
+        return out
     broad = lens > 0.5 * (g.hsync + g.vsync)
     hsync = (lens > 0.5 * (g.eq + g.hsync)) & ~broad
-    out: List[FieldLines] = []
+    hs_idx = np.flatnonzero(hsync)
     nl = g.field_lines
+    n = np.arange(nl + 1, dtype=np.float64)
     for i in np.flatnonzero(broad & np.concatenate([[True], ~broad[:-1]])):
-        before = np.flatnonzero(hsync[:i])
-        if before.size == 0:
+        k = np.searchsorted(hs_idx, i) - 1
+        if k < 0:
             continue
-        d = (starts[i] - starts[before[-1]]) / g.line_len
+        d = (starts[i] - starts[hs_idx[k]]) / g.line_len
         mid_line = abs(d - np.round(d)) > 0.25
         line0 = starts[i] - (0.5 * g.line_len if mid_line else 0.0)
         if line0 + (nl + 1) * g.line_len >= span_len:
             continue
-        locs = np.empty(nl + 1)
-        filled = np.zeros(nl + 1, dtype=bool)
-        prev = line0
-        k = i
-        for n in range(nl + 1):
-            pred = line0 if n == 0 else prev + g.line_len
-            # nearest pulse start to the prediction
-            while k + 1 < starts.size and starts[k + 1] <= pred:
-                k += 1
-            cand = [c for c in (k, k + 1) if 0 <= c < starts.size]
-            best = min(cand, key=lambda c: abs(starts[c] - pred)) if cand else None
-            if best is not None and abs(starts[best] - pred) <= tol * g.line_len:
-                locs[n] = starts[best]
-            else:
-                locs[n] = pred
-                filled[n] = True
-            prev = locs[n]
-        # the broad group of the first field starts on a line boundary in 625/50;
-        # in 525/60 the first field's vsync starts on a line boundary as well
-        out.append(FieldLines(line0, locs, filled, first_field=not mid_line))
+        pred = line0 + n * g.line_len
+        j = np.clip(np.searchsorted(starts, pred), 1, starts.size - 1)
+        left, right = starts[j - 1], starts[j]
+        best = np.where(np.abs(left - pred) <= np.abs(right - pred), left, right)
+        hit = np.abs(best - pred) <= tol * g.line_len
+        if not hit.any():
+            continue
+        # lines no pulse opened: interpolate between the measured neighbours
+        locs = np.where(hit, best, np.interp(n, n[hit], best[hit] - n[hit] * g.line_len) + n * g.line_len)
+        out.append(FieldLines(line0, locs, ~hit, first_field=not mid_line))
     return out
 
 
