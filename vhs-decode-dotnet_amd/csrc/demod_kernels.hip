@@ -230,6 +230,7 @@ __global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) {
   const long long start = a.block_start[blk];
   const long long out_base = a.out_offset[blk];
   load_octant(s, a.oct);
+  load_trans<1>(a.env, s.scratch + kEnvTransOffset);
   // ---- stage A: rfft of the int16 block
   c32 v[16];
   const int16_t* x = a.rf + start;
@@ -274,7 +275,7 @@ __global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) {
     }
   }
   // ---- envelope: raw[(n + 4) mod N] = |float(re[n])|, FP32 sosfiltfilt
-  float* raw = reinterpret_cast<float*>(s.buf);
+  float* raw = reinterpret_cast<float*>(s.buf) + kSosMargin + a.env.pad;
   const float2* st_re = stash + kFftM;
 #pragma unroll
   for (int m = 0; m < 16; m++) {
@@ -286,7 +287,8 @@ __global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) {
   }
   __syncthreads();
   // the VHS envelope filter is first order (DecodeFilterSet.BuildVhsEnvelopeSos): one section
-  const bool weak = sos_filtfilt_block_t<1>(a.env, raw, 32768, s.scratch, a.out_env + out_base, a.out_begin,
+  const bool weak = sos_filtfilt_block_t<1>(a.env, reinterpret_cast<float*>(s.buf) + kSosMargin, 32768, s.scratch,
+                                            s.scratch + kEnvTransOffset, a.out_env + out_base, a.out_begin,
                                             a.out_count, /*roll=*/0, false);
   // ---- flags
   const unsigned long long any_spike = __ballot(spike);
@@ -298,17 +300,24 @@ __global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) {
 }
 
 // ------------------------------------------------------------------ chroma
+template <int S>
 __global__ void __launch_bounds__(1024) vhs_block_chroma_kernel(ChromaArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
-  const int j = threadIdx.x;
   const int blk = blockIdx.x;
   const int16_t* x = a.rf + a.block_start[blk];
-  float* raw = reinterpret_cast<float*>(s.buf);
-  for (int n = j; n < 32768; n += kFftThreads) raw[n] = (float)x[n];
+  float* ext = reinterpret_cast<float*>(s.buf) + kSosMargin;
+  float* raw = ext + a.sos.pad;
+  float* trans = reinterpret_cast<float*>(s.oct);  // the twiddle octant is unused here
+  load_trans<S>(a.sos, trans);
+  int16_t xv[32];
+#pragma unroll
+  for (int m = 0; m < 32; m++) xv[m] = row_at(x, m);
+#pragma unroll
+  for (int m = 0; m < 32; m++) row_at(raw, m) = (float)xv[m];
   __syncthreads();
-  sos_filtfilt_block(a.sos, raw, 32768, s.scratch, a.out_chroma + a.out_offset[blk], a.out_begin, a.out_count,
-                     a.chroma_offset, /*subtract_mean=*/true);
+  sos_filtfilt_block_t<S>(a.sos, ext, 32768, s.scratch, trans, a.out_chroma + a.out_offset[blk], a.out_begin,
+                          a.out_count, a.chroma_offset, /*subtract_mean=*/true);
 }
 
 // ------------------------------------------------------------------ spike repair
@@ -460,6 +469,16 @@ __global__ void __launch_bounds__(1024) debug_fft_kernel(float2* data, int inver
 }  // namespace vhsd
 
 // ------------------------------------------------------------------ launchers
+template <int S>
+static hipError_t launch_chroma_s(const vhsd::ChromaArgs* a, int nblocks, hipStream_t stream) {
+  const size_t lds = sizeof(vhsd::BlockLds);
+  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_chroma_kernel<S>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(vhsd::vhs_block_chroma_kernel<S>, dim3(nblocks), dim3(1024), lds, stream, *a);
+  return hipGetLastError();
+}
+
 extern "C" {
 
 size_t vhsd_block_lds_bytes() { return sizeof(vhsd::BlockLds); }
@@ -474,12 +493,13 @@ hipError_t vhsd_launch_block_demod(const vhsd::DemodArgs* a, int nblocks, hipStr
 }
 
 hipError_t vhsd_launch_block_chroma(const vhsd::ChromaArgs* a, int nblocks, hipStream_t stream) {
-  const size_t lds = sizeof(vhsd::BlockLds);
-  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_chroma_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(vhsd::vhs_block_chroma_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
-  return hipGetLastError();
+  switch (a->sos.sections) {
+    case 1: return launch_chroma_s<1>(a, nblocks, stream);
+    case 2: return launch_chroma_s<2>(a, nblocks, stream);
+    case 3: return launch_chroma_s<3>(a, nblocks, stream);
+    case 4: return launch_chroma_s<4>(a, nblocks, stream);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const float2* oct, hipStream_t stream) {

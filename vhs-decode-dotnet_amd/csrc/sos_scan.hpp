@@ -4,12 +4,20 @@
 // 584-833, 924-966, 1034-1063): odd extension of `pad` samples at both ends,
 // steady-state initial conditions scaled by the first (forward) or last
 // (backward) value, direct-form-II-transposed biquads evaluated with the same
-// FP32 operation order.  The sequential recurrence is split into 1024 chunks,
-// one per thread: each chunk is run from a zero state, the chunk-entry states
-// come from a Hillis-Steele scan over the affine state maps (transition matrix
-// (A^C)^(2^d) per level, precomputed in FP64 on the host), and each chunk is then
-// re-run from its true entry state.  Only the injected entry states differ in
-// rounding from the sequential reference (relative error ~1e-7).
+// FP32 operation order.
+//
+// The extended sequence (length L = n + 2 pad) lives in LDS and is cut into
+// chunks of C samples, one per thread: forward chunks start at index 0, backward
+// chunks at index L-1, so in both directions thread 0 holds the chunk processed
+// first and only the last non-empty chunk can be partial.  A partial chunk runs
+// over LDS margin cells past the end of the sequence in its processing order, so
+// no sample loop carries a per-sample condition and every LDS access is one base
+// address plus a compile-time offset.  Each chunk is run from a zero state, the
+// chunk-entry states come from a Hillis-Steele scan over the affine state maps
+// (transition matrix (A^C)^(2^d) per level, precomputed in FP64 on the host),
+// and each chunk is then re-run from its true entry state.  Only the injected
+// entry states differ in rounding from the sequential reference (relative error
+// ~1e-7).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -21,6 +29,10 @@ constexpr int kMaxSections = 4;
 constexpr int kScanThreads = 1024;
 constexpr int kScanLevels = 10;  // log2(kScanThreads)
 constexpr int kChunk = 33;       // supports blocks of up to 1024*33 - 2*pad samples
+// LDS layout: the sequence starts kSosMargin floats into the buffer, which must hold
+// kSosMargin + L + kChunk floats (partial chunks overrun by less than kChunk on
+// either side).
+constexpr int kSosMargin = 64;
 
 struct SosScanArgs {
   int sections;                          // 1..kMaxSections
@@ -43,29 +55,39 @@ __device__ __forceinline__ void sos_step(const float (&c)[kMaxSections][5], floa
   }
 }
 
-// Exclusive scan of chunk end-states: returns in `st` the state entering this
-// thread's chunk, given `st` = end state of the chunk when started from zero
-// (thread 0: from the true initial state).  `ex` is LDS space of 1024*2S floats.
+// Copies the transition-matrix table to LDS (kScanLevels * (2S)^2 floats); the
+// caller's next barrier publishes it.
 template <int S>
-__device__ __forceinline__ void scan_states(float* st, float* ex, const float* __restrict__ trans) {
-  const int j = threadIdx.x;
+__device__ __forceinline__ void load_trans(const SosScanArgs& p, float* dst) {
+  constexpr int T = kScanLevels * 4 * S * S;
+  for (int u = threadIdx.x; u < T; u += kScanThreads) dst[u] = p.trans[u];
+}
+
+// Exclusive scan of chunk end-states in thread order: on entry `st` is the end
+// state of this thread's chunk when started from zero (thread 0: from the true
+// initial state); on return it is the state entering the chunk.  `ex` is LDS space
+// of 1024*2S floats, laid out component-major (conflict-free); `trans` the LDS copy
+// of the level matrices.
+template <int S>
+__device__ __forceinline__ void scan_states(float* st, float* ex, const float* trans) {
   constexpr int W = 2 * S;
+  const int q = threadIdx.x;
 #pragma unroll 1
   for (int d = 0; d < kScanLevels; d++) {
 #pragma unroll
-    for (int q = 0; q < W; q++) ex[j * W + q] = st[q];
+    for (int c = 0; c < W; c++) ex[c * kScanThreads + q] = st[c];
     __syncthreads();
     const int off = 1 << d;
-    if (j >= off) {
+    if (q >= off) {
       float p[W];
 #pragma unroll
-      for (int q = 0; q < W; q++) p[q] = ex[(j - off) * W + q];
+      for (int c = 0; c < W; c++) p[c] = ex[c * kScanThreads + q - off];
       const float* m = trans + d * W * W;
 #pragma unroll
       for (int r = 0; r < W; r++) {
         float acc = 0.0f;
 #pragma unroll
-        for (int q = 0; q < W; q++) acc += m[r * W + q] * p[q];
+        for (int c = 0; c < W; c++) acc += m[r * W + c] * p[c];
         st[r] += acc;
       }
     }
@@ -73,132 +95,107 @@ __device__ __forceinline__ void scan_states(float* st, float* ex, const float* _
   }
   // inclusive -> exclusive
 #pragma unroll
-  for (int q = 0; q < W; q++) ex[j * W + q] = st[q];
+  for (int c = 0; c < W; c++) ex[c * kScanThreads + q] = st[c];
   __syncthreads();
-  if (j > 0) {
+  if (q > 0) {
 #pragma unroll
-    for (int q = 0; q < W; q++) st[q] = ex[(j - 1) * W + q];
+    for (int c = 0; c < W; c++) st[c] = ex[c * kScanThreads + q - 1];
   }
   __syncthreads();
 }
 
-// One direction over this thread's chunk values `val[0..len)` (in processing
-// order).  Thread 0 starts from zi * first value of the whole sequence.
+// One direction over this thread's chunk val[0..kChunk) in processing order; the
+// chunk of thread 0 starts from zi * first_value.
 template <int S>
-__device__ __forceinline__ void run_direction(const SosScanArgs& p, float* val, int len, float first_value,
-                                              float* ex) {
-  const int j = threadIdx.x;
+__device__ __forceinline__ void run_direction(const SosScanArgs& p, float (&val)[kChunk], float first_value,
+                                              float* ex, const float* trans) {
+  const bool lead = threadIdx.x == 0;
   float z[2 * S];
   float init[2 * S];
 #pragma unroll
-  for (int q = 0; q < 2 * S; q++) {
-    init[q] = p.zi[q] * first_value;
-    z[q] = (j == 0) ? init[q] : 0.0f;
+  for (int c = 0; c < 2 * S; c++) {
+    init[c] = p.zi[c] * first_value;
+    z[c] = lead ? init[c] : 0.0f;
   }
 #pragma unroll
   for (int t = 0; t < kChunk; t++) {
-    if (t < len) {
-      float v = val[t];
-      sos_step<S>(p.coef, z, v);
-    }
+    float v = val[t];
+    sos_step<S>(p.coef, z, v);
   }
-  scan_states<S>(z, ex, p.trans);
-  if (j == 0) {
+  scan_states<S>(z, ex, trans);
+  if (lead) {
 #pragma unroll
-    for (int q = 0; q < 2 * S; q++) z[q] = init[q];
+    for (int c = 0; c < 2 * S; c++) z[c] = init[c];
   }
 #pragma unroll
   for (int t = 0; t < kChunk; t++) {
-    if (t < len) {
-      float v = val[t];
-      sos_step<S>(p.coef, z, v);
-      val[t] = v;
-    }
+    float v = val[t];
+    sos_step<S>(p.coef, z, v);
+    val[t] = v;
   }
 }
 
-// Filter N samples held in LDS `raw` (FP32).  Writes the kept window of the
-// result, rolled by `roll` (numpy.roll semantics), optionally mean-subtracted,
-// to out[0 .. out_count) = result[out_begin ..).  Returns true (on every
-// thread) when any output sample is exactly zero.
+// Filters the n samples the caller stored at ext[pad .. pad + n) (LDS, FP32; ext
+// starts kSosMargin floats into a buffer of at least kSosMargin + n + 2 pad + kChunk
+// floats, all of which are overwritten).  Writes the kept window of the result,
+// rolled by `roll` (numpy.roll semantics), optionally mean-subtracted, to
+// out[0 .. out_count) = result[out_begin ..).  Returns true (on every thread) when
+// any of the n output samples is exactly zero.  `red` is 64 floats of LDS, `trans`
+// the LDS copy made by load_trans.  The caller's stores to ext must be followed by
+// a barrier before the call.
 template <int S>
-__device__ bool sos_filtfilt_block_t(const SosScanArgs& p, float* raw, int n, float* scratch, float* out,
-                                     int out_begin, int out_count, int roll, bool subtract_mean) {
+__device__ __forceinline__ bool sos_filtfilt_block_t(const SosScanArgs& p, float* ext, int n, float* red,
+                                                     const float* trans, float* out, int out_begin, int out_count,
+                                                     int roll, bool subtract_mean) {
   const int j = threadIdx.x;
   const int pad = p.pad;
   const int L = n + 2 * pad;
   const int C = p.chunk;
-  float* head = scratch;          // [pad] extension values, then results
-  float* tail = scratch + 64;     // [pad]
-  float* red = scratch + 128;     // reductions
+  float* ex = ext - kSosMargin;  // scan exchange: the whole buffer, free while chunks sit in registers
+  // ---- odd extension: ext[i] = 2 x[0] - x[pad - i], ext[pad + n + k] = 2 x[n-1] - x[n - 2 - k]
+  {
+    const float first = ext[pad];
+    const float last = ext[pad + n - 1];
+    float e = 0.0f;
+    if (j < pad) e = (2.0f * first) - ext[2 * pad - j];
+    else if (j < 2 * pad) e = (2.0f * last) - ext[pad + n - 2 - (j - pad)];
+    __syncthreads();
+    if (j < pad) ext[j] = e;
+    else if (j < 2 * pad) ext[pad + n + (j - pad)] = e;
+    __syncthreads();
+  }
+  const bool active = j * C < L;  // chunk holds at least one sample
   float val[kChunk];
-  // ---- forward: chunk j covers ext indices [j*C, j*C + len)
-  const int fbeg = j * C;
-  int len = L - fbeg;
-  len = len < 0 ? 0 : (len > C ? C : len);
-  const float first = raw[0];
-  const float last = raw[n - 1];
+  // ---- forward: thread j owns ext[jC .. jC + C)
+  float* fwd = ext + j * C;
 #pragma unroll
-  for (int t = 0; t < kChunk; t++) {
-    const int i = fbeg + t;
-    float e = 0.0f;
-    if (t < len) {
-      if (i < pad)
-        e = (2.0f * first) - raw[pad - i];
-      else if (i < pad + n)
-        e = raw[i - pad];
-      else
-        e = (2.0f * last) - raw[n - 2 - (i - pad - n)];
-    }
-    val[t] = e;
-  }
-  const float ext0 = (2.0f * first) - raw[pad];
-  const float ext_first = pad > 0 ? ext0 : first;
-  __syncthreads();  // raw no longer needed: its space becomes the scan exchange
-  run_direction<S>(p, val, len, ext_first, raw);
-  // ---- exchange forward results into LDS (interior) and scratch (extensions)
+  for (int t = 0; t < kChunk; t++) val[t] = active ? fwd[t] : 0.0f;
+  const float ext_first = ext[0];
+  __syncthreads();
+  run_direction<S>(p, val, ext_first, ex, trans);
+  if (active) {
 #pragma unroll
-  for (int t = 0; t < kChunk; t++) {
-    if (t < len) {
-      const int i = fbeg + t;
-      if (i < pad)
-        head[i] = val[t];
-      else if (i < pad + n)
-        raw[i - pad] = val[t];
-      else
-        tail[i - pad - n] = val[t];
-    }
+    for (int t = 0; t < kChunk; t++) fwd[t] = val[t];
   }
   __syncthreads();
-  // ---- backward: reversed chunk j covers ext indices i = L-1-r, r in [j*C, j*C + len)
-  int blen = L - j * C;
-  blen = blen < 0 ? 0 : (blen > C ? C : blen);
-  const float y_last = pad > 0 ? tail[pad - 1] : raw[n - 1];
+  // ---- backward: thread j owns ext[L-1-jC-t], t = 0 .. C-1 (descending)
+  float* bwd = ext + (L - 1 - j * C);
 #pragma unroll
-  for (int t = 0; t < kChunk; t++) {
-    const int i = L - 1 - (j * C + t);
-    float e = 0.0f;
-    if (t < blen) {
-      if (i < pad)
-        e = head[i];
-      else if (i < pad + n)
-        e = raw[i - pad];
-      else
-        e = tail[i - pad - n];
-    }
-    val[t] = e;
-  }
+  for (int t = 0; t < kChunk; t++) val[t] = active ? bwd[-t] : 0.0f;
+  const float y_last = ext[L - 1];
   __syncthreads();
-  run_direction<S>(p, val, blen, y_last, raw);
-  // ---- zero test and mean over the N interior outputs
+  run_direction<S>(p, val, y_last, ex, trans);
+  // ---- zero test and sum over the n interior outputs; results back to LDS
   bool zero = false;
   float sum = 0.0f;
+  if (active) {
 #pragma unroll
-  for (int t = 0; t < kChunk; t++) {
-    const int i = L - 1 - (j * C + t);
-    if (t < blen && i >= pad && i < pad + n) {
-      zero |= (val[t] == 0.0f);
-      sum += val[t];
+    for (int t = 0; t < kChunk; t++) {
+      bwd[-t] = val[t];
+      const int i = L - 1 - j * C - t;
+      const bool in = (i >= pad) && (i < pad + n);
+      zero |= in && (val[t] == 0.0f);
+      sum += in ? val[t] : 0.0f;
     }
   }
   const int wave = j >> 6;
@@ -217,32 +214,17 @@ __device__ bool sos_filtfilt_block_t(const SosScanArgs& p, float* raw, int n, fl
     total += red[w];
     any_zero |= red[32 + w] != 0.0f;
   }
-  const float mean = total / (float)n;
-  // ---- write the kept window
-#pragma unroll
-  for (int t = 0; t < kChunk; t++) {
-    const int i = L - 1 - (j * C + t);
-    if (t < blen && i >= pad && i < pad + n) {
-      int s = i - pad + roll;
-      s %= n;
-      if (s < 0) s += n;
-      const int o = s - out_begin;
-      if (o >= 0 && o < out_count) out[o] = subtract_mean ? (val[t] - mean) : val[t];
-    }
+  const float mean = subtract_mean ? total / (float)n : 0.0f;
+  // ---- write the kept window, coalesced: out[o] = y[(out_begin + o - roll) mod n]
+  const float* y = ext + pad;
+  for (int o = j; o < out_count; o += kScanThreads) {
+    int i = out_begin + o - roll;
+    if (i < 0) i += n;
+    if (i >= n) i -= n;
+    out[o] = y[i] - mean;
   }
   __syncthreads();
   return any_zero;
-}
-
-__device__ __forceinline__ bool sos_filtfilt_block(const SosScanArgs& p, float* raw, int n, float* scratch,
-                                                   float* out, int out_begin, int out_count, int roll,
-                                                   bool subtract_mean = false) {
-  switch (p.sections) {
-    case 1: return sos_filtfilt_block_t<1>(p, raw, n, scratch, out, out_begin, out_count, roll, subtract_mean);
-    case 2: return sos_filtfilt_block_t<2>(p, raw, n, scratch, out, out_begin, out_count, roll, subtract_mean);
-    case 3: return sos_filtfilt_block_t<3>(p, raw, n, scratch, out, out_begin, out_count, roll, subtract_mean);
-    default: return sos_filtfilt_block_t<4>(p, raw, n, scratch, out, out_begin, out_count, roll, subtract_mean);
-  }
 }
 
 }  // namespace vhsd

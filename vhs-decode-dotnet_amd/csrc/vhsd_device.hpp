@@ -5,6 +5,8 @@
 namespace vhsd {
 
 constexpr int kScratchFloats = 512;
+// LDS scratch offset of the envelope filter's transition matrices (kScanLevels * 4 floats)
+constexpr int kEnvTransOffset = 256;
 
 // PortedMath.VhsRustAtan2Approximation (PortedMath.cs:519-544): FP32 polynomial
 // atan2, evaluated with separately rounded operations (no contraction) so that

@@ -93,8 +93,12 @@ int32_t prepare_sos(const double* sos, int sections, int n, vhsd::SosScanArgs* a
   a->sections = sections;
   a->pad = default_pad_length(sos, sections);
   const int L = n + 2 * a->pad;
-  a->chunk = (L + vhsd::kScanThreads - 1) / vhsd::kScanThreads;
-  if (a->chunk > vhsd::kChunk || a->pad > 64) return fail(VHSDECODE_HIP_E_UNSUPPORTED, "SOS padding too long");
+  // every thread runs exactly kChunk samples; the extended block plus its LDS margins
+  // must fit the 33792-float block buffer
+  a->chunk = vhsd::kChunk;
+  if (a->pad > 64 || L > vhsd::kScanThreads * vhsd::kChunk ||
+      vhsd::kSosMargin + L + vhsd::kChunk > 2 * vhsd::kFftBufEntries)
+    return fail(VHSDECODE_HIP_E_UNSUPPORTED, "SOS padding too long");
   float c32[vhsd::kMaxSections][6];
   for (int s = 0; s < sections; s++) {
     for (int q = 0; q < 6; q++) c32[s][q] = (float)sos[s * 6 + q];
