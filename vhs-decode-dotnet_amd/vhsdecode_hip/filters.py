@@ -304,6 +304,10 @@ def build_tape_filters(fmt: FormatParameters, sample_rate_hz: float = 40e6, bloc
         raise ValueError("Block length must be a positive even value.")
     if fmt.rfparams.get("video_custom_luma_filters"):
         raise NotImplementedError("custom luma filter files are not supported by this backend")
+    if fmt.rfparams.get("use_sub_deemphasis"):
+        raise NotImplementedError("sub-deemphasis (RfDemodulator.cs:2897-2999) is not implemented by this backend")
+    if fmt.rfparams.get("boost_bpf_mult") or fmt.tape_format in ("BETAMAX", "BETAMAX_HIFI"):
+        raise NotImplementedError("the RF high boost and Betamax fsc notch are not implemented by this backend")
     n = block_length
     fs = float(sample_rate_hz)
     nyq = fs / 2.0
