@@ -81,6 +81,23 @@ VHSDECODE_HIP_API const char* vhsdecode_hip_get_last_error(void);
  * The filter tables are the caller's DecodeFilterSet (Dsp/DecodeFilterSet.cs:9-82):
  * full-circle complex128 spectra of block_length bins, interleaved (re, im); SOS
  * sections as [sections][6] = b0 b1 b2 a0 a1 a2. */
+/* Sub-deemphasis of the video plane (RfDemodulator.ApplySubDeemphasisRealCore,
+ * src/VHSDecode.Core/Dsp/RfDemodulator.cs:2897-2999; options of
+ * Decode/DecodeSession.cs:1250-1278), enabled by use_sub_deemphasis for VHS LP/EP,
+ * Video8 and others.  Absent options carry neutral values. */
+typedef struct vhsdecode_hip_subdeemph_v1 {
+  uint32_t struct_size;
+  const double* high_pass;         /* BuildNonlinearHighPassFilter: whole-circle response, N complex (re, im) */
+  const double* amplitude_sos;     /* [1][6]: ButterworthLowPass(1, amplitude low-pass / Nyquist) */
+  double deviation;                /* SubDeemphasisOptions.Deviation (> 0) */
+  double exponential_scaling;
+  double scaling1;                 /* 1 when absent */
+  double scaling2;                 /* 1 when absent */
+  double logistic_mid;             /* 0 when absent */
+  double logistic_rate;            /* <= 0: no logistic term */
+  double static_factor;            /* 0 when absent */
+} vhsdecode_hip_subdeemph_v1;
+
 typedef struct vhsdecode_hip_filters_v1 {
   uint32_t struct_size;
   int32_t block_length;            /* N; this build supports 32768 */
@@ -98,6 +115,7 @@ typedef struct vhsdecode_hip_filters_v1 {
   double plane_bias_hz;            /* video and video_lowpass planes hold FP32 (value - plane_bias_hz); an
                                       FP32 plane at the carrier (~4 MHz) would keep only 1/4-1/2 Hz, offsets
                                       near a mid-level carrier keep 1/16 Hz.  0 writes absolute Hz. */
+  const vhsdecode_hip_subdeemph_v1* sub_deemphasis;  /* NULL: the format does not use it */
 } vhsdecode_hip_filters_v1;
 
 typedef struct vhsdecode_hip_demod_config_v1 {
@@ -258,7 +276,8 @@ enum {
   VHSDECODE_HIP_KERNEL_PULSES = 3,
   VHSDECODE_HIP_KERNEL_HSYNC = 4,
   VHSDECODE_HIP_KERNEL_TBC = 5,
-  VHSDECODE_HIP_KERNEL_COUNT = 6
+  VHSDECODE_HIP_KERNEL_SUB_DEEMPHASIS = 6,
+  VHSDECODE_HIP_KERNEL_COUNT = 7
 };
 
 typedef struct vhsdecode_hip_profile_v1 {

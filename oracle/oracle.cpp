@@ -255,6 +255,104 @@ void sosfiltfilt_f32(const double* sos, int sections, const double* input, int n
   for (int i = 0; i < n; i++) output[i] = ext[pad + i];
 }
 
+// SosFilter.ApplyForwardBackward in FP64 (Dsp/SosFilter.cs:178-298, 540-581, 972-1001):
+// odd extension, steady-state zi scaled by the first sample of each pass, sections
+// applied one at a time over the whole sequence (the 1-, 3-section path; 2 and 4
+// sections cascade per sample, which is the same arithmetic per section).
+void sosfiltfilt_f64(const double* sos, int sections, const double* input, int n, int pad, double* output) {
+  int m = n + 2 * pad;
+  std::vector<double> ext(m);
+  if (pad == 0) {
+    for (int i = 0; i < n; i++) ext[i] = input[i];
+  } else {
+    double first = input[0];
+    for (int i = 0; i < pad; i++) ext[i] = (2.0 * first) - input[pad - i];
+    for (int i = 0; i < n; i++) ext[pad + i] = input[i];
+    double last = input[n - 1];
+    for (int i = 0; i < pad; i++) ext[pad + n + i] = (2.0 * last) - input[n - 2 - i];
+  }
+  std::vector<double> c(6 * sections), zi(2 * sections);
+  double scale = 1.0;
+  for (int k = 0; k < sections; k++) {
+    const double* q = sos + 6 * k;
+    double a0 = q[3];
+    for (int t = 0; t < 6; t++) c[6 * k + t] = a0 != 1.0 ? q[t] / a0 : q[t];
+    const double* z = &c[6 * k];
+    double t1 = z[1] - (z[4] * z[0]);
+    double t2 = z[2] - (z[5] * z[0]);
+    double z0 = ((0.0 + t1) + t2) / ((1.0 + z[4]) + z[5]);
+    double z1 = ((1.0 + z[4]) * z0) - t1;
+    zi[2 * k] = scale * z0;
+    zi[2 * k + 1] = scale * z1;
+    scale *= (((0.0 + z[0]) + z[1]) + z[2]) / (((0.0 + z[3]) + z[4]) + z[5]);
+  }
+  auto pass = [&]() {
+    double x0 = ext[0];
+    for (int k = 0; k < sections; k++) {
+      const double* z = &c[6 * k];
+      double s1 = zi[2 * k] * x0, s2 = zi[2 * k + 1] * x0;
+      for (int i = 0; i < m; i++) {
+        double x = ext[i];
+        double y = (z[0] * x) + s1;
+        s1 = ((z[1] * x) - (z[4] * y)) + s2;
+        s2 = (z[2] * x) - (z[5] * y);
+        ext[i] = y;
+      }
+    }
+  };
+  pass();
+  std::reverse(ext.begin(), ext.end());
+  pass();
+  std::reverse(ext.begin(), ext.end());
+  for (int i = 0; i < n; i++) output[i] = ext[pad + i];
+}
+
+// numpy-style |z| (RfDemodulator.NumpyComplexMagnitude, RfDemodulator.cs:3216-3242).
+double numpy_abs(cd v) {
+  double re = std::fabs(v.real()), im = std::fabs(v.imag());
+  if (std::isinf(re) || std::isinf(im)) return INFINITY;
+  if (std::isnan(re) || std::isnan(im)) return NAN;
+  double larger = std::max(re, im), smaller = std::min(re, im);
+  if (larger == 0.0) return 0.0;
+  double r = smaller / larger;
+  return std::sqrt(std::fma(r, r, 1.0)) * larger;
+}
+
+// Sub-deemphasis on the VHS real path (RfDemodulator.ApplySubDeemphasisRealCore,
+// RfDemodulator.cs:2897-2999; BuildAnalyticMagnitude :3151-3179).  `vs` is the video
+// half spectrum (demod spectrum x Video), `hp` the whole-circle high-pass response,
+// p = {deviation, exponential_scaling, scaling1, scaling2, logistic_mid,
+// logistic_rate, static_factor} with neutral values (1, 1, 0, 0, 0) for absent options.
+void sub_deemphasis(double* video, const cd* vs, const cd* hp, const double* amp_sos, const double* p, int n) {
+  const int half = n / 2 + 1;
+  std::vector<cd> hs(half);
+  for (int k = 0; k < half; k++) hs[k] = cmul_fma(vs[k], hp[k]);
+  std::vector<double> high(n);
+  irfft(hs.data(), n, high.data());
+  std::vector<cd> spec(n);
+  for (int i = 0; i < n; i++) spec[i] = cd(high[i], 0.0);
+  fft(spec.data(), n, false);
+  const int nyq = n / 2;
+  for (int i = 1; i < nyq; i++) spec[i] = cd(spec[i].real() * 2.0, spec[i].imag() * 2.0);
+  for (int i = nyq + 1; i < n; i++) spec[i] = cd(0.0, 0.0);
+  fft(spec.data(), n, true);
+  std::vector<double> amp(n);
+  const double deviation = p[0] / 2.0;
+  for (int i = 0; i < n; i++) amp[i] = numpy_abs(spec[i]) / deviation;
+  int pad = default_pad_length(amp_sos, 1);
+  std::vector<double> a(n);
+  sosfiltfilt_f64(amp_sos, 1, amp.data(), n, n > pad ? pad : 0, a.data());
+  for (int i = 0; i < n; i++) {
+    double v = std::max(0.0, a[i]);
+    v *= p[2];
+    v = std::pow(v, p[1]);
+    v *= p[3];
+    if (p[5] > 0.0) v *= 1.0 / (1.0 + std::exp(-p[5] * (v - p[4])));
+    double stat = high[i] * p[6];
+    video[i] -= (high[i] * (1.0 - v)) + stat;
+  }
+}
+
 // numpy.roll semantics (FrequencyDomainFilter.RollInPlaceCore, FrequencyDomainFilter.cs:145-161).
 template <typename T>
 void roll(T* v, int n, int shift) {
@@ -318,7 +416,8 @@ int oracle_demod_block(const double* input, int n, double fs, const double* rf_v
                        const double* env_sos, int env_sections, const double* chroma_sos,
                        int chroma_sections, int chroma_offset, double diff_demod_max,
                        double* video_out, double* vlp_out, double* env_out, float* chroma_out,
-                       double* demod_out) {
+                       double* demod_out, const double* sd_high_pass, const double* sd_amp_sos,
+                       const double* sd_params) {
   const cd* rf_video = reinterpret_cast<const cd*>(rf_video_c);
   const cd* video_f = reinterpret_cast<const cd*>(video_c);
   const cd* vlp05 = reinterpret_cast<const cd*>(vlp05_c);
@@ -379,6 +478,8 @@ int oracle_demod_block(const double* input, int n, double fs, const double* rf_v
   rfft(demod.data(), n, dspec.data());
   for (int k = 0; k < half; k++) vs[k] = cmul_fma(dspec[k], video_f[k]);
   irfft(vs.data(), n, video_out);
+  if (sd_high_pass)
+    sub_deemphasis(video_out, vs.data(), reinterpret_cast<const cd*>(sd_high_pass), sd_amp_sos, sd_params, n);
   for (int k = 0; k < half; k++) vs[k] = cmul_fma(dspec[k], vlp05[k]);
   irfft(vs.data(), n, vlp_out);
   if (vlp_offset != 0) roll(vlp_out, n, -vlp_offset);

@@ -42,7 +42,7 @@ def lib():
         L.oracle_sosfiltfilt_f32.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _fp]
         L.oracle_demod_block.argtypes = [_dp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, ctypes.c_int,
                                          _dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                         _dp, _dp, _dp, _fp, _dp]
+                                         _dp, _dp, _dp, _fp, _dp, _dp, _dp, _dp]
         L.oracle_demod_block.restype = ctypes.c_int
         L.oracle_find_pulses.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                          _ip, _ip, ctypes.c_int]
@@ -118,11 +118,19 @@ def demod_block(x: np.ndarray, filters, fs: Optional[float] = None) -> DemodResu
     chroma = np.empty(n, dtype=np.float32)
     demod = np.empty(n)
     dmax = filters.diff_demod_max if filters.diff_demod_max is not None else -1.0
+    sd = getattr(filters, "sub_deemphasis", None)
+    if sd is not None:
+        sd_hp = np.ascontiguousarray(sd.high_pass, dtype=np.complex128)
+        sd_sos = np.ascontiguousarray(sd.amplitude_sos, dtype=np.float64)
+        sd_par = sd.params()
+        sd_args = (_p(sd_hp.view(np.float64), _dp), _p(sd_sos, _dp), _p(sd_par, _dp))
+    else:
+        sd_args = (None, None, None)
     weak = lib().oracle_demod_block(
         _p(xi, _dp), n, fs, _p(rv.view(np.float64), _dp), _p(vf.view(np.float64), _dp),
         _p(lf.view(np.float64), _dp), int(filters.video_lowpass05_offset), _p(env, _dp), env.shape[0],
         _p(ch, _dp), ch.shape[0], int(filters.chroma_offset), float(dmax), _p(video, _dp), _p(vlp, _dp),
-        _p(envelope, _dp), _p(chroma, _fp), _p(demod, _dp))
+        _p(envelope, _dp), _p(chroma, _fp), _p(demod, _dp), *sd_args)
     return DemodResult(video, vlp, envelope, chroma if chs is not None else None, demod, bool(weak))
 
 

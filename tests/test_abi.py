@@ -51,6 +51,7 @@ def _c_sizeof(struct):
     ("vhsdecode_hip_tbc_params_v1", _lib.TbcParamsV1),
     ("vhsdecode_hip_profile_v1", _lib.ProfileV1),
     ("vhsdecode_hip_field_params_v1", _lib.FieldParamsV1),
+    ("vhsdecode_hip_subdeemph_v1", _lib.SubDeemphV1),
 ])
 def test_struct_layouts_match_header(name, cls):
     assert _c_sizeof(name) == ctypes.sizeof(cls)

@@ -1,0 +1,62 @@
+"""Sub-deemphasis on the HIP path (vhs_block_subdeemph_kernel, through the C ABI)
+against the oracle's restatement of RfDemodulator.ApplySubDeemphasisRealCore
+(Dsp/RfDemodulator.cs:2897-2999), which the upstream fixture hashes pin
+(tests/test_oracle.py).  Runs on an MI355X only."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vhsdecode_hip import filters, formats, synth
+from vhsdecode_hip.demod import HipBlockDemodulator
+
+pytestmark = pytest.mark.gpu
+
+N = 32768
+CUT = 1024
+STRIDE = N - 2 * CUT
+VIDEO_RMS_HZ = 2.0
+VIDEO_MAX_HZ = 40.0
+
+
+@pytest.fixture(scope="module", params=[("PAL", "lp", 3), ("NTSC", "ep", 4)], ids=["pal_vhs_lp", "ntsc_vhs_ep"])
+def case(request):
+    system, speed, seed = request.param
+    fmt = formats.get_format(system, "VHS", speed)
+    fl = filters.build_tape_filters(fmt, 40e6, N)
+    assert fl.sub_deemphasis is not None
+    tape = synth.generate(fmt, fields=1, seed=seed, snr_db=30.0)
+    rf = tape.rf[: N + 11 * STRIDE]
+    nb = 12
+    with HipBlockDemodulator(fl, max_blocks=nb) as dem:
+        gpu = dem.demod_span_host(rf, 0, nb)
+    on, off = [], []
+    for b in range(nb):
+        blk = rf[b * STRIDE:b * STRIDE + N].astype(np.float64)
+        on.append(O.demod_block(blk, fl).video[CUT:CUT + STRIDE])
+    sd = fl.sub_deemphasis
+    fl.sub_deemphasis = None
+    for b in range(nb):
+        blk = rf[b * STRIDE:b * STRIDE + N].astype(np.float64)
+        off.append(O.demod_block(blk, fl).video[CUT:CUT + STRIDE])
+    fl.sub_deemphasis = sd
+    print(f"\n[{system} VHS {speed}] blocks={nb}", flush=True)
+    return dict(gpu=gpu, on=np.concatenate(on), off=np.concatenate(off))
+
+
+def test_video_plane_with_sub_deemphasis(case):
+    err = case["gpu"].video_hz - case["on"]
+    rms = float(np.sqrt(np.mean(err ** 2)))
+    print(f"video RMS {rms:.3f} Hz, max {np.max(np.abs(err)):.2f} Hz")
+    assert rms <= VIDEO_RMS_HZ
+    assert np.max(np.abs(err)) <= VIDEO_MAX_HZ
+
+
+def test_sub_deemphasis_is_applied(case):
+    """The correction is far larger than the FP32 error bar: the GPU plane is
+    close to the sub-deemphasised oracle and far from the plain one."""
+    corr = case["on"] - case["off"]
+    err_off = case["gpu"].video_hz - case["off"]
+    rms_corr = float(np.sqrt(np.mean(corr ** 2)))
+    print(f"sub-deemphasis correction RMS {rms_corr:.1f} Hz")
+    assert rms_corr > 10 * VIDEO_RMS_HZ
+    assert float(np.sqrt(np.mean(err_off ** 2))) > 5 * VIDEO_RMS_HZ

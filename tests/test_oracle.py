@@ -130,6 +130,38 @@ def test_pal_vhs_block_matches_upstream_channel_hashes(family):
     assert _sha_f32(r.envelope) == c["envelope"]
 
 
+@pytest.mark.parametrize("system,fmt,speed", [("PAL", "VHS", "lp"), ("PAL", "VHS", "ep"), ("PAL", "VHS", "vp"),
+                                              ("PAL", "VIDEO8", "sp"), ("PAL", "VIDEO8", "lp"),
+                                              ("NTSC", "VIDEO8", "sp")])
+def test_sub_deemphasis_matches_upstream_video_hash(family, system, fmt, speed):
+    """Formats with use_sub_deemphasis (RfDemodulator.ApplySubDeemphasisRealCore,
+    RfDemodulator.cs:2897-2999): the sub-deemphasised Video channel, the sync
+    reference and the envelope match the fixture bit-exactly through the float32
+    cast.  (Video8's order-10 chroma band-pass is designed differently; its
+    chroma hash is not matched and not asserted.)"""
+    x = _lcg_u8_block().astype(np.float64)
+    fl = filters.build_tape_filters(formats.get_format(system, fmt, speed), 40e6, N)
+    assert fl.sub_deemphasis is not None
+    r = O.demod_block(x, fl)
+    c = _case(family, system, fmt, speed)
+    assert _sha_f32(r.video) == c["demod"]
+    assert _sha_f32(r.video_lowpass) == c["demod_05"]
+    assert _sha_f32(r.envelope) == c["envelope"]
+
+
+def test_sub_deemphasis_changes_only_the_video_channel():
+    """Switching sub-deemphasis off changes Video and leaves the other channels
+    untouched (the correction is applied after the video inverse transform)."""
+    x = _lcg_u8_block().astype(np.float64)
+    fl = filters.build_tape_filters(formats.get_format("PAL", "VHS", "lp"), 40e6, N)
+    on = O.demod_block(x, fl)
+    fl.sub_deemphasis = None
+    off = O.demod_block(x, fl)
+    assert not np.array_equal(on.video, off.video)
+    np.testing.assert_array_equal(on.video_lowpass, off.video_lowpass)
+    np.testing.assert_array_equal(on.envelope, off.envelope)
+
+
 def test_ntsc_vhs_chroma_matches_upstream_hash(family):
     """NTSC VHS: the chroma channel (input-only filter) matches bit-exactly.  The
     RF-side channels depend on the order-16 band-pass transfer-function design,

@@ -184,9 +184,11 @@ __device__ __forceinline__ void video_stage(const float* dem, BlockLds& s, const
   c32 dm;
   rfft_post(D, s, dm);
   const float inv_n = 1.0f / 32768.0f;
-  // the spectrum feeds two inverse transforms; park it rather than hold it in registers
+  // the spectrum feeds two inverse transforms (and the sub-deemphasis kernel); park it
+  // rather than hold it in registers
 #pragma unroll
   for (int m = 0; m < 16; m++) row_at(stash, m) = make_float2(D[m].x, D[m].y);
+  if (j == 0) stash[2 * kFftM] = make_float2(dm.x, dm.y);
 #pragma unroll 1
   for (int pass = 0; pass < 2; pass++) {
     const float2* filt = pass == 0 ? a.video : a.vlp05;
@@ -318,6 +320,95 @@ __global__ void __launch_bounds__(1024) vhs_block_chroma_kernel(ChromaArgs a) {
   __syncthreads();
   sos_filtfilt_block_t<S>(a.sos, ext, 32768, s.scratch, trans, a.out_chroma + a.out_offset[blk], a.out_begin,
                           a.out_count, a.chroma_offset, /*subtract_mean=*/true);
+}
+
+// ------------------------------------------------------------------ sub-deemphasis
+// RfDemodulator.ApplySubDeemphasisRealCore (Dsp/RfDemodulator.cs:2897-2999) on one block:
+// H = D x Video x HighPass; the analytic signal of irfft(H) (one-sided H, inverse as
+// even/odd 16384-point passes) gives both the high-passed video (its real part) and
+// its magnitude; the magnitude / (deviation / 2) is low-pass filtered forward-backward,
+// mapped through max(0, .) * scaling1, pow, * scaling2 and the optional logistic, and
+// video -= high * (1 - gain) + high * static_factor over the kept window.
+// |z| the way RfDemodulator.NumpyComplexMagnitude forms it (RfDemodulator.cs:3216-3242):
+// larger * sqrt(1 + (smaller / larger)^2).
+__device__ __forceinline__ float numpy_abs(c32 z) {
+  const float re = fabsf(z.x), im = fabsf(z.y);
+  const float lg = fmaxf(re, im), sm = fminf(re, im);
+  if (lg == 0.0f) return 0.0f;
+  const float r = sm / lg;
+  return sqrtf(fmaf(r, r, 1.0f)) * lg;
+}
+
+__global__ void __launch_bounds__(1024) vhs_block_subdeemph_kernel(SubDeemphArgs a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
+  const int j = threadIdx.x;
+  const int blk = blockIdx.x;
+  float2* stash = a.stash + (long long)blk * kStashEntries;
+  load_octant(s, a.oct);
+  load_trans<1>(a.amp, s.scratch + kEnvTransOffset);
+  c32 F[16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const float2 d = row_at(stash, m);
+    const float2 f = row_at(a.vh, m);
+    F[m] = mul(c32{d.x, d.y}, c32{f.x, f.y});
+  }
+  const float2 dM = stash[2 * kFftM];
+  const float2 fM = a.vh[kFftM];
+  const c32 fm = mul(c32{dM.x, dM.y}, c32{fM.x, fM.y});
+  __syncthreads();  // twiddles and transition matrices in LDS; D read by every thread
+  const float inv_n = 1.0f / 32768.0f;
+  float2* st_go = stash;          // odd-pass input
+  float2* st_ev = stash + kFftM;  // even samples (real part, magnitude), then real parts (even, odd)
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    const c32 g = (k == 0) ? sub(F[m], fm) : mul(scale(F[m], 2.0f), conjc(tw32k(s.oct, k)));
+    row_at(st_go, m) = make_float2(g.x, g.y);
+  }
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    F[m] = (k == 0) ? add(F[m], fm) : scale(F[m], 2.0f);
+  }
+  fft16k_inverse(F, s.buf, s.oct);
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const c32 e = scale(F[m], inv_n);
+    row_at(st_ev, m) = make_float2(e.x, numpy_abs(e));
+  }
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const float2 g = row_at(st_go, m);
+    F[m] = c32{g.x, g.y};
+  }
+  fft16k_inverse(F, s.buf, s.oct);
+  float* ext = reinterpret_cast<float*>(s.buf) + kSosMargin;
+  float* amp = ext + a.amp.pad;
+  const float hd = a.half_deviation;
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    const c32 o = scale(F[m], inv_n);
+    const float2 ev = row_at(st_ev, m);
+    amp[2 * k] = ev.y / hd;
+    amp[2 * k + 1] = numpy_abs(o) / hd;
+    row_at(st_ev, m) = make_float2(ev.x, o.x);
+  }
+  __syncthreads();
+  sos_filtfilt_block_t<1>(a.amp, ext, 32768, s.scratch, s.scratch + kEnvTransOffset, nullptr, 0, 0, 0, false);
+  const float* y = ext + a.amp.pad;
+  float* vout = a.out_video + a.out_offset[blk];
+  for (int o = j; o < a.out_count; o += kFftThreads) {
+    const int i = a.out_begin + o;
+    float v = fmaxf(0.0f, y[i]) * a.scaling1;
+    v = powf(v, a.exp_scaling) * a.scaling2;
+    if (a.logistic_rate > 0.0f) v *= 1.0f / (1.0f + expf(-a.logistic_rate * (v - a.logistic_mid)));
+    const float2 r = st_ev[i >> 1];
+    const float hp = (i & 1) ? r.y : r.x;
+    vout[o] -= (hp * (1.0f - v)) + (hp * a.static_factor);
+  }
 }
 
 // ------------------------------------------------------------------ spike repair
@@ -509,6 +600,15 @@ hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const floa
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(vhsd::debug_fft_kernel, dim3(count), dim3(1024), lds, stream, reinterpret_cast<float2*>(data),
                      inverse, oct);
+  return hipGetLastError();
+}
+
+hipError_t vhsd_launch_block_subdeemph(const vhsd::SubDeemphArgs* a, int nblocks, hipStream_t stream) {
+  const size_t lds = sizeof(vhsd::BlockLds);
+  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_subdeemph_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(vhsd::vhs_block_subdeemph_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
   return hipGetLastError();
 }
 

@@ -7,7 +7,8 @@
 
 namespace vhsd {
 
-// Per-block scratch of the demod kernels: two spectra of 16384 complex FP32 values.
+// Per-block scratch of the demod kernels: two spectra of 16384 complex FP32 values,
+// then the demod spectrum's Nyquist bin at index 2 * 16384.
 constexpr long long kStashEntries = 2 * 16384 + 64;
 
 struct DemodArgs {
@@ -45,6 +46,26 @@ struct ChromaArgs {
   float* out_chroma;
 };
 
+// Sub-deemphasis of the kept video window of each block, from the demod spectrum the
+// block demod (or repair) kernel leaves in the block's stash: D at [0, M), D[M] at [2M].
+struct SubDeemphArgs {
+  const long long* out_offset;
+  int out_begin;
+  int out_count;
+  const float2* vh;        // [M+1] Video x HighPass half spectrum
+  const float2* oct;
+  float2* stash;
+  SosScanArgs amp;         // amplitude low-pass (one section)
+  float half_deviation;    // deviation / 2
+  float exp_scaling;
+  float scaling1;
+  float scaling2;
+  float logistic_mid;
+  float logistic_rate;     // <= 0: off
+  float static_factor;
+  float* out_video;
+};
+
 struct RepairArgs {
   DemodArgs d;
   float2* scratch_analytic;  // [nblocks][32768] complex
@@ -80,6 +101,7 @@ size_t vhsd_block_lds_bytes();
 hipError_t vhsd_launch_block_demod(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_chroma(const vhsd::ChromaArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_repair(const vhsd::RepairArgs* a, int nblocks, hipStream_t stream);
+hipError_t vhsd_launch_block_subdeemph(const vhsd::SubDeemphArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const float2* oct, hipStream_t stream);
 hipError_t vhsd_launch_pulses(const float* s, long long n, double bias, double high, int min_len, int max_len,
                               int* tile_counts,

@@ -221,6 +221,12 @@ struct vhsdecode_hip_demod_context {
   vhsd::SosScanArgs env{}, chroma{};
   DevBuf<float2> rfv, vid, vlp, oct;
   DevBuf<float> env_trans, chroma_trans;
+  // sub-deemphasis
+  bool has_subdeemph = false;
+  vhsd::SosScanArgs sd_amp{};
+  DevBuf<float2> sd_vh;
+  DevBuf<float> sd_trans;
+  float sd_params[7] = {};
   DevBuf<long long> block_start, out_offset;
   DevBuf<unsigned> flags;
   DevBuf<float2> rep_a;
@@ -266,7 +272,7 @@ struct vhsdecode_hip_demod_context {
 
   ~vhsdecode_hip_demod_context() {
     (void)hipSetDevice(device);
-    rfv.release(); vid.release(); vlp.release(); oct.release();
+    rfv.release(); vid.release(); vlp.release(); oct.release(); sd_vh.release(); sd_trans.release();
     env_trans.release(); chroma_trans.release(); block_start.release(); out_offset.release(); flags.release();
     rep_a.release(); rep_dm.release(); stash.release(); tile_counts.release(); counters.release(); p_starts.release();
     p_lens.release(); edge_pos.release(); edge_kind.release(); h_loc.release(); h_ref.release();
@@ -382,6 +388,27 @@ int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int
     vhsd::RepairArgs r{a, c->rep_a.p, c->rep_dm.p};
     ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_SPIKE_REPAIR, nblocks);
     VHSD_HIP(vhsd_launch_block_repair(&r, nblocks, st));
+  }
+  if (c->has_subdeemph) {
+    vhsd::SubDeemphArgs sd{};
+    sd.out_offset = c->out_offset.p;
+    sd.out_begin = c->cut;
+    sd.out_count = c->stride;
+    sd.vh = c->sd_vh.p;
+    sd.oct = c->oct.p;
+    sd.stash = c->stash.p;
+    sd.amp = c->sd_amp;
+    sd.amp.trans = c->sd_trans.p;
+    sd.half_deviation = c->sd_params[0];
+    sd.exp_scaling = c->sd_params[1];
+    sd.scaling1 = c->sd_params[2];
+    sd.scaling2 = c->sd_params[3];
+    sd.logistic_mid = c->sd_params[4];
+    sd.logistic_rate = c->sd_params[5];
+    sd.static_factor = c->sd_params[6];
+    sd.out_video = out->video;
+    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_SUB_DEEMPHASIS, nblocks);
+    VHSD_HIP(vhsd_launch_block_subdeemph(&sd, nblocks, st));
   }
   if (c->has_chroma && out->chroma) {
     vhsd::ChromaArgs ch{};
@@ -506,6 +533,34 @@ int32_t vhsdecode_hip_demod_create(const vhsdecode_hip_demod_config_v1* cfg, vhs
           hipMemcpy(c->chroma_trans.p, tr.data(), sizeof(float) * tr.size(), hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(VHSDECODE_HIP_E_HIP, "chroma table upload failed"));
       c->has_chroma = true;
+    }
+    if (const vhsdecode_hip_subdeemph_v1* sd = f->sub_deemphasis) {
+      if (sd->struct_size != sizeof(vhsdecode_hip_subdeemph_v1))
+        return bail(fail(VHSDECODE_HIP_E_STRUCT_SIZE, "sub_deemphasis struct_size mismatch"));
+      if (!sd->high_pass || !sd->amplitude_sos || !(sd->deviation > 0.0))
+        return bail(fail(VHSDECODE_HIP_E_INVALID_ARGUMENT,
+                         "sub_deemphasis needs high_pass, amplitude_sos and a positive deviation"));
+      // Video x HighPass in FP64, stored as an FP32 half spectrum
+      std::vector<double> vh((size_t)2 * c->n);
+      for (int k = 0; k <= c->n / 2; k++) {
+        const double ar = f->video[2 * k], ai = f->video[2 * k + 1];
+        const double br = sd->high_pass[2 * k], bi = sd->high_pass[2 * k + 1];
+        vh[2 * k] = std::fma(ar, br, -(ai * bi));
+        vh[2 * k + 1] = std::fma(ar, bi, ai * br);
+      }
+      if ((rc = upload_half(c->sd_vh, vh.data(), c->n)) != 0) return bail(rc);
+      if ((rc = prepare_sos(sd->amplitude_sos, 1, c->n, &c->sd_amp, &tr)) != 0) return bail(rc);
+      if (c->sd_trans.ensure(tr.size()) != hipSuccess ||
+          hipMemcpy(c->sd_trans.p, tr.data(), sizeof(float) * tr.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(VHSDECODE_HIP_E_HIP, "sub-deemphasis table upload failed"));
+      c->sd_params[0] = (float)(sd->deviation / 2.0);
+      c->sd_params[1] = (float)sd->exponential_scaling;
+      c->sd_params[2] = (float)sd->scaling1;
+      c->sd_params[3] = (float)sd->scaling2;
+      c->sd_params[4] = (float)sd->logistic_mid;
+      c->sd_params[5] = (float)(sd->logistic_rate > 0.0 ? sd->logistic_rate : 0.0);
+      c->sd_params[6] = (float)sd->static_factor;
+      c->has_subdeemph = true;
     }
     if (c->block_start.ensure(c->max_blocks) != hipSuccess || c->out_offset.ensure(c->max_blocks) != hipSuccess ||
         c->flags.ensure(c->max_blocks) != hipSuccess)

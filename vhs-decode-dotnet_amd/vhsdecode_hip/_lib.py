@@ -34,7 +34,7 @@ REQUIRED_EXPORTS = (
     "vhsdecode_hip_debug_fft16k",
 )
 
-KERNEL_NAMES = ("block_demod", "spike_repair", "chroma", "pulses", "hsync", "tbc")
+KERNEL_NAMES = ("block_demod", "spike_repair", "chroma", "pulses", "hsync", "tbc", "sub_deemphasis")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -52,13 +52,20 @@ class RuntimeInfoV1(ctypes.Structure):
                 ("gcn_arch", ctypes.c_char * 64)]
 
 
+class SubDeemphV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("high_pass", _dp), ("amplitude_sos", _dp),
+                ("deviation", ctypes.c_double), ("exponential_scaling", ctypes.c_double),
+                ("scaling1", ctypes.c_double), ("scaling2", ctypes.c_double), ("logistic_mid", ctypes.c_double),
+                ("logistic_rate", ctypes.c_double), ("static_factor", ctypes.c_double)]
+
+
 class FiltersV1(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("block_length", ctypes.c_int32),
                 ("sample_rate_hz", ctypes.c_double), ("rf_video", _dp), ("video", _dp),
                 ("video_lowpass05", _dp), ("video_lowpass05_offset", ctypes.c_int32), ("envelope_sos", _dp),
                 ("envelope_sections", ctypes.c_int32), ("chroma_sos", _dp), ("chroma_sections", ctypes.c_int32),
                 ("chroma_offset", ctypes.c_int32), ("diff_demod_max", ctypes.c_double),
-                ("plane_bias_hz", ctypes.c_double)]
+                ("plane_bias_hz", ctypes.c_double), ("sub_deemphasis", ctypes.POINTER(SubDeemphV1))]
 
 
 class DemodConfigV1(ctypes.Structure):
@@ -98,8 +105,8 @@ class FieldParamsV1(ctypes.Structure):
 
 
 class ProfileV1(ctypes.Structure):
-    _fields_ = [("struct_size", ctypes.c_uint32), ("launches", ctypes.c_int64 * 6), ("total_ms", ctypes.c_double * 6),
-                ("units", ctypes.c_int64 * 6)]
+    _fields_ = [("struct_size", ctypes.c_uint32), ("launches", ctypes.c_int64 * len(KERNEL_NAMES)),
+                ("total_ms", ctypes.c_double * len(KERNEL_NAMES)), ("units", ctypes.c_int64 * len(KERNEL_NAMES))]
 
 
 _lib = None

@@ -82,6 +82,18 @@ class HipBlockDemodulator:
         f.chroma_offset = int(filters.chroma_offset)
         f.diff_demod_max = float(filters.diff_demod_max) if filters.diff_demod_max is not None else -1.0
         f.plane_bias_hz = float(filters.plane_bias_hz)
+        sd = getattr(filters, "sub_deemphasis", None)
+        if sd is not None:
+            self._sd_hp = np.ascontiguousarray(sd.high_pass, dtype=np.complex128).view(np.float64)
+            self._sd_sos = np.ascontiguousarray(sd.amplitude_sos, dtype=np.float64)
+            d = _lib.SubDeemphV1()
+            d.struct_size = ctypes.sizeof(_lib.SubDeemphV1)
+            d.high_pass = _ptr(self._sd_hp, _lib._dp)
+            d.amplitude_sos = _ptr(self._sd_sos, _lib._dp)
+            (d.deviation, d.exponential_scaling, d.scaling1, d.scaling2, d.logistic_mid, d.logistic_rate,
+             d.static_factor) = (float(v) for v in sd.params())
+            self._sd = d
+            f.sub_deemphasis = ctypes.pointer(d)
         self.bias = float(filters.plane_bias_hz)
         cfg = _lib.DemodConfigV1()
         cfg.struct_size = ctypes.sizeof(_lib.DemodConfigV1)

@@ -45,6 +45,29 @@ class TapeFilterSet:
     video_lowpass05_offset: int
     diff_demod_max: Optional[float]
     plane_bias_hz: float = 0.0    # mid-level carrier; FP32 planes are stored as offsets from it
+    sub_deemphasis: Optional["SubDeemphasis"] = None
+
+
+@dataclass
+class SubDeemphasis:
+    """SubDeemphasisOptions (DecodeSession.cs:1250-1278) with its high-pass response
+    (RfDemodulator.BuildNonlinearHighPassFilter, RfDemodulator.cs:3094-3142) and the
+    amplitude low-pass (IirFilterDesign.ButterworthLowPass order 1).  Absent options
+    carry their neutral values: scaling 1, logistic rate 0, static factor 0."""
+
+    high_pass: np.ndarray         # complex128[N], whole circle
+    amplitude_sos: np.ndarray     # float64[1, 6]
+    deviation: float
+    exponential_scaling: float
+    scaling1: float = 1.0
+    scaling2: float = 1.0
+    logistic_mid: float = 0.0
+    logistic_rate: float = 0.0
+    static_factor: float = 0.0
+
+    def params(self) -> np.ndarray:
+        return np.array([self.deviation, self.exponential_scaling, self.scaling1, self.scaling2,
+                         self.logistic_mid, self.logistic_rate, self.static_factor], dtype=np.float64)
 
 
 # --------------------------------------------------------------------------- responses
@@ -204,6 +227,22 @@ def _lowpass_first_order_sos(fnorm: float) -> np.ndarray:
     return np.array([[k * norm, k * norm, 0.0, 1.0, (k - 1.0) * norm, 0.0]])
 
 
+def _butter_highpass_sos(order: int, fnorm: float) -> np.ndarray:
+    """IirFilterDesign.ButterworthHighPass (IirFilterDesign.cs:108-111, 1078-1143):
+    bilinear biquads with k = tan(pi f / 2), then a first-order section for odd orders."""
+    k = math.tan(math.pi * fnorm / 2.0)
+    sections = []
+    for i in range(order // 2):
+        q = 1.0 / (2.0 * math.sin(((2.0 * i) + 1.0) * math.pi / (2.0 * order)))
+        k2 = k * k
+        norm = 1.0 / (1.0 + (k / q) + k2)
+        sections.append([norm, -2.0 * norm, norm, 1.0, 2.0 * (k2 - 1.0) * norm, (1.0 - (k / q) + k2) * norm])
+    if order % 2:
+        norm = 1.0 / (1.0 + k)
+        sections.append([norm, -norm, 0.0, 1.0, (k - 1.0) * norm, 0.0])
+    return np.array(sections, dtype=np.float64)
+
+
 def _bandpass_sos(order: int, lo: float, hi: float) -> np.ndarray:
     """IirFilterDesign.ButterworthBandPassSos (IirFilterDesign.cs:594-668)."""
     if order == 1:
@@ -304,8 +343,6 @@ def build_tape_filters(fmt: FormatParameters, sample_rate_hz: float = 40e6, bloc
         raise ValueError("Block length must be a positive even value.")
     if fmt.rfparams.get("video_custom_luma_filters"):
         raise NotImplementedError("custom luma filter files are not supported by this backend")
-    if fmt.rfparams.get("use_sub_deemphasis"):
-        raise NotImplementedError("sub-deemphasis (RfDemodulator.cs:2897-2999) is not implemented by this backend")
     if fmt.rfparams.get("boost_bpf_mult") or fmt.tape_format in ("BETAMAX", "BETAMAX_HIFI"):
         raise NotImplementedError("the RF high boost and Betamax fsc notch are not implemented by this backend")
     n = block_length
@@ -337,4 +374,37 @@ def build_tape_filters(fmt: FormatParameters, sample_rate_hz: float = 40e6, bloc
     # the 50 IRE carrier, rounded to a whole Hz so it is exact in FP32
     bias = float(np.round(float(sp["ire0"]) + 50.0 * float(sp["hz_ire"])))
     return TapeFilterSet(n, fs, rf_video, video, vlp, vlp05, env, chroma, chroma_offset,
-                         VIDEO_LOWPASS05_OFFSET, diff_max, bias)
+                         VIDEO_LOWPASS05_OFFSET, diff_max, bias, _sub_deemphasis(fmt, fs, n))
+
+
+def _sub_deemphasis(fmt: FormatParameters, fs: float, n: int) -> Optional[SubDeemphasis]:
+    """BuildSubDeemphasisOptions (DecodeSession.cs:1250-1278), enabled by the format's
+    use_sub_deemphasis."""
+    rf, sp = fmt.rfparams, fmt.sysparams
+    if not rf.get("use_sub_deemphasis"):
+        return None
+    nyq = fs / 2.0
+
+    def opt(key, default=None):
+        v = rf.get(key)
+        return default if v is None else float(v)
+
+    hpf = float(rf["nonlinear_highpass_freq"])
+    upper = opt("nonlinear_bandpass_upper")
+    order = int(rf.get("nonlinear_bandpass_order") or 1)
+    if not 0.0 < hpf < nyq:
+        raise ValueError("Nonlinear high-pass frequency must be between 0 Hz and Nyquist.")
+    if upper is None:
+        high_pass = _freqz_sos(_butter_highpass_sos(order, hpf / nyq), n)
+    else:
+        raise NotImplementedError("band-pass sub-deemphasis (Betamax) is not implemented by this backend")
+    amp_lpf = opt("nonlinear_amp_lpf_freq", 700_000.0)
+    deviation = sp.get("nonlinear_deviation")
+    deviation = float(deviation) if deviation is not None else float(sp["hz_ire"]) * (100.0 + -float(sp["vsync_ire"]))
+    if deviation <= 0.0 or not 0.0 < amp_lpf < nyq:
+        raise ValueError("invalid sub-deemphasis options")
+    rate = opt("nonlinear_logistic_rate", 0.0)
+    return SubDeemphasis(high_pass, _lowpass_first_order_sos(amp_lpf / nyq), deviation,
+                         opt("nonlinear_exp_scaling", 0.25), opt("nonlinear_scaling_1", 1.0),
+                         opt("nonlinear_scaling_2", 1.0), opt("nonlinear_logistic_mid", 0.0),
+                         rate if rate > 0.0 else 0.0, opt("nonlinear_static_factor", 0.0))
