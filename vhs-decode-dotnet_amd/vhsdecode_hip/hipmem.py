@@ -54,6 +54,11 @@ def synchronize():
     _check(hip().hipDeviceSynchronize())
 
 
+def copy_device(dst: int, src: int, nbytes: int):
+    """Device-to-device copy (the runtime's blit kernel); synchronous."""
+    _check(hip().hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), int(nbytes), 3))
+
+
 class DeviceBuffer:
     """Raw HBM allocation (hipMalloc)."""
 
