@@ -277,7 +277,8 @@ enum {
   VHSDECODE_HIP_KERNEL_HSYNC = 4,
   VHSDECODE_HIP_KERNEL_TBC = 5,
   VHSDECODE_HIP_KERNEL_SUB_DEEMPHASIS = 6,
-  VHSDECODE_HIP_KERNEL_COUNT = 7
+  VHSDECODE_HIP_KERNEL_BLOCK_VIDEO = 7,
+  VHSDECODE_HIP_KERNEL_COUNT = 8
 };
 
 typedef struct vhsdecode_hip_profile_v1 {

@@ -82,7 +82,7 @@ __device__ __forceinline__ void rfft_post(c32* v, BlockLds& s, c32& xm) {
 // Y[k] = (V[k] + conj(V[M-k])) + i * (V[k] - conj(V[M-k])) * exp(+2 pi i k / N).
 // numpy.fft.irfft semantics: imaginary parts of V[0] and V[M] are ignored.
 __device__ __forceinline__ void irfft_pre(c32* v, c32 vm, BlockLds& s) {
-  const int j = threadIdx.x;
+  const int j = opaque_tid();
   const int pj = lds_pad(j);
 #pragma unroll
   for (int m = 0; m < 16; m++) s.buf[pj + m * kPadRow] = v[m];
@@ -107,12 +107,13 @@ __device__ __forceinline__ void irfft_pre(c32* v, c32 vm, BlockLds& s) {
 }
 
 // Stage B+: analytic signal from the filtered half spectrum F (strided, F[M] in fm)
-// and the FM discriminator.  Writes demod into dem[32] laid out as [2m] = even sample
-// 2*(j+1024m), [2m+1] = odd sample, and leaves the analytic real parts (the envelope
-// source) in the scratch area as float2(even, odd) per k = j + 1024 m.  State that
-// must outlive a transform goes through the block's scratch area (`stash`, written
-// and read back by this workgroup only) instead of registers.
-__device__ __forceinline__ void analytic_demod(c32* F, c32 fm, BlockLds& s, float fs, float* dem,
+// and the FM discriminator.  Writes the demod to stash[k] = float2(sample 2k,
+// sample 2k+1) for k = j + 1024 m and leaves the analytic real parts (the envelope
+// source) at stash[M + k] in the same form.  State that must outlive a transform
+// goes through the block's stash (written and read back by this workgroup only)
+// instead of registers.  Returns whether this thread saw a demod sample in
+// [20, N-20) above diff_max.
+__device__ __forceinline__ bool analytic_demod(c32* F, c32 fm, BlockLds& s, float fs, double diff_max,
                                                float2* __restrict__ stash) {
   const int j = threadIdx.x;
   const float inv_n = 1.0f / 32768.0f;
@@ -145,17 +146,19 @@ __device__ __forceinline__ void analytic_demod(c32* F, c32 fm, BlockLds& s, floa
   fft16k_inverse(F, s.buf, s.oct);
   float* th_o = reinterpret_cast<float*>(s.buf);
   float theta_e[16];
+  float dem_o[16];
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     const float2 te = row_at(st_th, m);
     theta_e[m] = te.x;
     const c32 a = scale(F[m], inv_n);
     const float t = vhs_atan2(a.y, a.x);
-    dem[2 * m + 1] = vhs_freq_diff(t, te.x, fs);
+    dem_o[m] = vhs_freq_diff(t, te.x, fs);
     th_o[j + m * kFftThreads] = t;
     row_at(st_th, m) = make_float2(te.y, a.x);
   }
   __syncthreads();
+  bool spike = false;
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     // previous odd sample of even sample 2*(j + 1024 m) is odd index (j-1) + 1024 m
@@ -164,14 +167,21 @@ __device__ __forceinline__ void analytic_demod(c32* F, c32 fm, BlockLds& s, floa
       prev = th_o[j - 1 + m * kFftThreads];
     else
       prev = (m > 0) ? th_o[kFftThreads - 1 + (m - 1) * kFftThreads] : 0.0f;
-    dem[2 * m] = (j == 0 && m == 0) ? 0.0f : vhs_freq_diff(theta_e[m], prev, fs);
+    const float de = (j == 0 && m == 0) ? 0.0f : vhs_freq_diff(theta_e[m], prev, fs);
+    row_at(st_go, m) = make_float2(de, dem_o[m]);
+    if (diff_max > 0.0) {
+      const int n0 = 2 * (j + m * kFftThreads);
+      spike |= (n0 >= 20 && n0 < 32768 - 20 && (double)de > diff_max);
+      spike |= (n0 + 1 >= 20 && n0 + 1 < 32768 - 20 && (double)dem_o[m] > diff_max);
+    }
   }
   __syncthreads();
+  return spike;
 }
 
-// Video and sync-reference planes from the demodulated signal held as
-// dem[2m], dem[2m+1] (samples 2k, 2k+1, k = j + 1024 m).
-__device__ __forceinline__ void video_stage(const float* dem, BlockLds& s, const DemodArgs& a, long long out_base,
+// Video and sync-reference planes from the demodulated signal held in the stash as
+// stash[k] = float2(sample 2k, sample 2k+1), k = j + 1024 m.
+__device__ __forceinline__ void video_stage(BlockLds& s, const DemodArgs& a, long long out_base,
                                             float2* __restrict__ stash) {
   const int j = threadIdx.x;
   // The ~4 MHz carrier level would dominate the FP32 rounding of the transforms; remove
@@ -179,7 +189,10 @@ __device__ __forceinline__ void video_stage(const float* dem, BlockLds& s, const
   // filters pass as a constant correction, leaving offsets from the bias in the planes.
   c32 D[16];
 #pragma unroll
-  for (int m = 0; m < 16; m++) D[m] = c32{dem[2 * m] - a.bias, dem[2 * m + 1] - a.bias};
+  for (int m = 0; m < 16; m++) {
+    const float2 d = row_at(stash, m);
+    D[m] = c32{d.x - a.bias, d.y - a.bias};
+  }
   fft16k_forward(D, s.buf, s.oct);
   c32 dm;
   rfft_post(D, s, dm);
@@ -264,18 +277,8 @@ __global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) {
   __syncthreads();
   // ---- stage B: analytic signal + discriminator
   float2* stash = a.stash + (long long)blk * kStashEntries;
-  float dem[32];
-  analytic_demod(v, fm_b, s, a.fs, dem, stash);
-  // spike check over demod[20 .. N-20)
-  bool spike = false;
-  if (a.diff_max > 0.0f) {
-#pragma unroll
-    for (int m = 0; m < 16; m++) {
-      const int n0 = 2 * (j + m * kFftThreads);
-      spike |= (n0 >= 20 && n0 < 32768 - 20 && (double)dem[2 * m] > a.diff_max);
-      spike |= (n0 + 1 >= 20 && n0 + 1 < 32768 - 20 && (double)dem[2 * m + 1] > a.diff_max);
-    }
-  }
+  // demod to the stash; spike check over demod[20 .. N-20)
+  const bool spike = analytic_demod(v, fm_b, s, a.fs, a.diff_max, stash);
   // ---- envelope: raw[(n + 4) mod N] = |float(re[n])|, FP32 sosfiltfilt
   float* raw = reinterpret_cast<float*>(s.buf) + kSosMargin + a.env.pad;
   const float2* st_re = stash + kFftM;
@@ -296,9 +299,17 @@ __global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) {
   const unsigned long long any_spike = __ballot(spike);
   if ((j & 63) == 0 && any_spike) atomicOr(&a.flags[blk], 2u);
   if (j == 0 && weak) atomicOr(&a.flags[blk], 1u);
+}
+
+// Stage C: video and sync-reference planes from the demod the block demod (or spike
+// repair) kernel left in each block's stash.
+__global__ void __launch_bounds__(1024) vhs_block_video_kernel(DemodArgs a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
+  const int blk = blockIdx.x;
+  load_octant(s, a.oct);
   __syncthreads();
-  // ---- stage C: video and sync reference
-  video_stage(dem, s, a, out_base, stash);
+  video_stage(s, a, a.out_offset[blk], a.stash + (long long)blk * kStashEntries);
 }
 
 // ------------------------------------------------------------------ chroma
@@ -420,7 +431,6 @@ __global__ void __launch_bounds__(1024) vhs_block_repair_kernel(RepairArgs r) {
   const int blk = blockIdx.x;
   if ((a.flags[blk] & 2u) == 0) return;  // uniform across the workgroup
   const long long start = a.block_start[blk];
-  const long long out_base = a.out_offset[blk];
   float2* A = r.scratch_analytic + (long long)blk * 32768;
   float* dm = r.scratch_demod + (long long)blk * 65536;
   float* dd = dm + 32768;
@@ -524,15 +534,13 @@ __global__ void __launch_bounds__(1024) vhs_block_repair_kernel(RepairArgs r) {
   }
   __threadfence_block();
   __syncthreads();
-  float dem[32];
+  // the repaired demod replaces the block's stashed demod; the video kernel runs next
+  float2* stash = r.d.stash + (long long)blk * kStashEntries;
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     const int k = j + m * kFftThreads;
-    dem[2 * m] = dm[2 * k];
-    dem[2 * m + 1] = dm[2 * k + 1];
+    row_at(stash, m) = make_float2(dm[2 * k], dm[2 * k + 1]);
   }
-  __syncthreads();
-  video_stage(dem, s, a, out_base, r.d.stash + (long long)blk * kStashEntries);
 }
 
 // ------------------------------------------------------------------ diagnostics
@@ -600,6 +608,15 @@ hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const floa
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(vhsd::debug_fft_kernel, dim3(count), dim3(1024), lds, stream, reinterpret_cast<float2*>(data),
                      inverse, oct);
+  return hipGetLastError();
+}
+
+hipError_t vhsd_launch_block_video(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream) {
+  const size_t lds = sizeof(vhsd::BlockLds);
+  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_video_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(vhsd::vhs_block_video_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
   return hipGetLastError();
 }
 

@@ -15,6 +15,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "vhsd_device.hpp"
+
 namespace vhsd {
 
 constexpr int kFftThreads = 1024;
@@ -27,6 +29,7 @@ constexpr int kOctantEntries = kOctant + 1;
 constexpr int kFftBufEntries = kFftM + kFftM / 32;
 
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
+
 
 struct c32 {
   float x, y;
@@ -42,7 +45,9 @@ __device__ __forceinline__ c32 scale(c32 a, float s) { return c32{a.x * s, a.y *
 __device__ __forceinline__ c32 mul_mi(c32 a) { return c32{a.y, -a.x}; }
 __device__ __forceinline__ c32 mul_pi(c32 a) { return c32{-a.y, a.x}; }
 
-// exp(-2*pi*i*t/16384) for any t (taken mod 16384), from the octant table.
+// exp(-2*pi*i*t/16384) for any t (taken mod 16384), from the octant table, without
+// branches: the octant reflection is a select, the quadrant rotation by (-i)^q a
+// swap select plus sign flips.
 __device__ __forceinline__ c32 tw16k(const float2* __restrict__ oct, int t) {
   t &= 16383;
   const int q = t >> 12;  // quadrant of 4096 steps
@@ -51,15 +56,13 @@ __device__ __forceinline__ c32 tw16k(const float2* __restrict__ oct, int t) {
   const float2 e = oct[lo ? r : 4096 - r];
   const float c = lo ? e.x : e.y;
   const float s = lo ? e.y : e.x;
-  // base exp(-i*theta) = (c, -s); rotate by (-i)^q
-  float ox, oy;
-  switch (q) {
-    case 0: ox = c; oy = -s; break;
-    case 1: ox = -s; oy = -c; break;
-    case 2: ox = -c; oy = s; break;
-    default: ox = s; oy = c; break;
-  }
-  return c32{ox, oy};
+  // exp(-i*theta) (-i)^q: q=0 (c,-s), q=1 (-s,-c), q=2 (-c,s), q=3 (s,c)
+  const bool odd = q & 1;
+  const float a = odd ? s : c;
+  const float b = odd ? c : s;
+  const unsigned fx = ((unsigned)((q + 1) >> 1) & 1u) << 31;  // negate x for q = 1, 2
+  const unsigned fy = (unsigned)(q < 2) << 31;                 // negate y for q = 0, 1
+  return c32{__uint_as_float(__float_as_uint(a) ^ fx), __uint_as_float(__float_as_uint(b) ^ fy)};
 }
 
 // exp(-2*pi*i*t/32768) for t in [0, 32768)
@@ -121,7 +124,7 @@ __device__ __forceinline__ void dft16(c32* v) {
 // padded LDS exchange buffer (kFftBufEntries, indexed through lds_pad); the caller guarantees it is free and that
 // all threads reach this call.  Output in the same distribution.
 __device__ __forceinline__ void fft16k_forward(c32* v, c32* __restrict__ buf, const float2* __restrict__ oct) {
-  const int j = threadIdx.x;
+  const int j = opaque_tid();
   // radix-16 passes with Ns = 1, 16, 256
 #pragma unroll
   for (int pass = 0; pass < 3; pass++) {

@@ -4,6 +4,18 @@
 
 namespace vhsd {
 
+// threadIdx.x through an opaque register copy.  A kernel inlines several transforms;
+// without this the compiler computes each pass's per-thread LDS addresses and twiddle
+// indices once and keeps them live across every transform (and the code between
+// them), which exhausts the 128-VGPR budget of a 1024-thread workgroup.  Recomputing
+// them per transform costs a few integer ops.
+__device__ __forceinline__ int opaque_tid() {
+  int j = threadIdx.x;
+  asm volatile("v_mov_b32 %0, %0" : "+v"(j));
+  return j;
+}
+
+
 constexpr int kScratchFloats = 512;
 // LDS scratch offset of the envelope filter's transition matrices (kScanLevels * 4 floats)
 constexpr int kEnvTransOffset = 256;

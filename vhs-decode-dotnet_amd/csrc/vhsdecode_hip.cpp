@@ -389,6 +389,10 @@ int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int
     ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_SPIKE_REPAIR, nblocks);
     VHSD_HIP(vhsd_launch_block_repair(&r, nblocks, st));
   }
+  {
+    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_BLOCK_VIDEO, nblocks);
+    VHSD_HIP(vhsd_launch_block_video(&a, nblocks, st));
+  }
   if (c->has_subdeemph) {
     vhsd::SubDeemphArgs sd{};
     sd.out_offset = c->out_offset.p;

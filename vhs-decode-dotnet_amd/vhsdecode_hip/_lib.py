@@ -34,7 +34,7 @@ REQUIRED_EXPORTS = (
     "vhsdecode_hip_debug_fft16k",
 )
 
-KERNEL_NAMES = ("block_demod", "spike_repair", "chroma", "pulses", "hsync", "tbc", "sub_deemphasis")
+KERNEL_NAMES = ("block_demod", "spike_repair", "chroma", "pulses", "hsync", "tbc", "sub_deemphasis", "block_video")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
