@@ -264,6 +264,39 @@ VHSDECODE_HIP_API int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod
                                                              const vhsdecode_hip_field_params_v1* params,
                                                              uint16_t* out, double* refined_out, void* stream);
 
+/* ---------------------------------------------------------------- whole-span field decode
+ * Pulses, line locations, hsync refinement and TBC of every complete field of a
+ * demodulated span without a host round trip: the per-field tail of
+ * TbcFieldDecodePipeline.DecodePrepared (Decode/TbcFieldDecodePipeline.cs:1071-1805)
+ * for a run of fields.  Line locations come from this backend's compact locator
+ * (each vertical-sync group anchors a field, parity from its half-line phase, line
+ * starts snapped to the pulses that open them), the stand-in for SyncAnalyzer's
+ * sequential pulse classification and line-0 resolution (Dsp/SyncAnalysis.cs:131-,
+ * TbcFieldDecodePipeline.cs:1176-1295). */
+typedef struct vhsdecode_hip_locator_v1 {
+  uint32_t struct_size;
+  int32_t min_pulse_length;        /* FindRawPulses bounds (samples), TbcFieldDecodePipeline.cs:2375-2384 */
+  int32_t max_pulse_length;
+  double line_length;              /* nominal input line length (samples, exact) */
+  double broad_min_length;         /* pulses longer than this are broad (vertical sync) */
+  double hsync_min_length;         /* longer than this and not broad: hsync */
+  double snap_tolerance;           /* lines */
+  double lead_lines;               /* field read-window lead before line 0 (lines) */
+  int32_t block_length;            /* read-window alignment, DecodeReadWindowPlanner.cs:37-63 */
+  int32_t block_cut;
+  int32_t block_stride;
+} vhsdecode_hip_locator_v1;
+
+/* sync_ref, video: device span planes; out: device [max_fields][field_lines * output_line_length];
+ * field_count: device int32 receiving the number of fields decoded (<= max_fields; only those
+ * fields of `out` are written); field_line0 (device double[max_fields]) and field_first
+ * (device int32[max_fields], 1 for the first field of a frame) are optional.  Asynchronous
+ * on `stream`. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_decode_span_fields_device(
+    vhsdecode_hip_demod_context* context, const float* sync_ref, const float* video, int64_t span_length,
+    const vhsdecode_hip_field_params_v1* params, const vhsdecode_hip_locator_v1* locator, int32_t max_fields,
+    uint16_t* out, int32_t* field_count, double* field_line0, int32_t* field_first, void* stream);
+
 /* ---------------------------------------------------------------- observability
  * Per-kernel device time, the counterpart of vhsdecode_cuda_fast_result_v1.elapsed_seconds
  * (vhsdecode_cuda_fast.h:100-106) at kernel granularity.  When enabled, the context
@@ -278,7 +311,8 @@ enum {
   VHSDECODE_HIP_KERNEL_TBC = 5,
   VHSDECODE_HIP_KERNEL_SUB_DEEMPHASIS = 6,
   VHSDECODE_HIP_KERNEL_BLOCK_VIDEO = 7,
-  VHSDECODE_HIP_KERNEL_COUNT = 8
+  VHSDECODE_HIP_KERNEL_LOCATE = 8,
+  VHSDECODE_HIP_KERNEL_COUNT = 9
 };
 
 typedef struct vhsdecode_hip_profile_v1 {

@@ -72,6 +72,32 @@ struct RepairArgs {
   float* scratch_demod;      // [nblocks][65536]: demod, diffed demod
 };
 
+// Line locator geometry (samples unless stated).
+struct LocateParams {
+  double line_len;      // nominal input line length
+  double broad_min;     // pulses longer than this are broad (vertical sync)
+  double hsync_min;     // pulses longer than this and not broad are hsync
+  double tol;           // snap tolerance in lines
+  int lines;            // output lines per field; lines + 1 locations
+  long long span_len;
+  double lead_lines;    // read-window lead before line 0, in lines
+  long long block_length, block_cut, stride;
+};
+
+// Per-field device tables the locator writes for the hsync and TBC kernels.
+struct FieldTables {
+  const float** spans;
+  long long* span_len;
+  const double** locs;
+  const unsigned char** filled;
+  double** refined;
+  unsigned char** filled_out;
+  int* line_counts;
+  long long* origins;
+  double* info_line0;
+  int* info_first;
+};
+
 struct HsyncParams {
   double threshold;
   double fs_mhz;       // sample rate in MHz (SyncAnalyzer.SampleRateMHz)
@@ -101,6 +127,11 @@ size_t vhsd_block_lds_bytes();
 hipError_t vhsd_launch_block_demod(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_chroma(const vhsd::ChromaArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_repair(const vhsd::RepairArgs* a, int nblocks, hipStream_t stream);
+hipError_t vhsd_launch_locate(const float* sync_ref, long long span_len, const int* starts, const int* lens,
+                              const int* pulse_count, int capacity, const vhsd::LocateParams* lp, int max_fields,
+                              int cand_cap, double* cand_line0, int* cand_first, int* cand_count, double* cand_locs,
+                              unsigned char* cand_filled, int* cand_hit, double* ref_base, unsigned char* fo_base,
+                              const vhsd::FieldTables* t, int* field_count, hipStream_t stream);
 hipError_t vhsd_launch_block_video(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_subdeemph(const vhsd::SubDeemphArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const float2* oct, hipStream_t stream);
@@ -118,7 +149,7 @@ hipError_t vhsd_launch_tbc_fields(const float* src, long long src_len, double bi
                                   int fields, double nominal, double line_decay, const double* pow_table,
                                   double* line_x, double* line_y0, int out_len, int smooth, double ire0,
                                   double out_scale, double inv_hz_ire, double conv_offset, const float* lut,
-                                  unsigned short* out, hipStream_t stream);
+                                  unsigned short* out, const int* field_count, hipStream_t stream);
 hipError_t vhsd_launch_tbc(const float* src, long long src_len, double bias, const double* locs, const double* level_adjust,
                            int level_per_line, const vhsd::TbcParams* p, const float* lut, unsigned short* out_u16,
                            double* out_f64, hipStream_t stream);

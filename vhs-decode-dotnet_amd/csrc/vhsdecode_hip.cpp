@@ -248,6 +248,10 @@ struct vhsdecode_hip_demod_context {
   DevBuf<unsigned short> t_u16;
   // batched fields
   DevBuf<double> f_loc, f_ref, f_x, f_y0, f_pow;
+  // device line locator
+  DevBuf<double> l_line0, l_locs;
+  DevBuf<int> l_first, l_hit, l_count;
+  DevBuf<unsigned char> l_fil, l_tab;
   DevBuf<unsigned char> f_fil, f_fo, f_tab;
   double f_pow_alpha = -1.0;
   int f_pow_len = 0;
@@ -278,6 +282,8 @@ struct vhsdecode_hip_demod_context {
     p_lens.release(); edge_pos.release(); edge_kind.release(); h_loc.release(); h_ref.release();
     h_fil.release(); h_fo.release(); h_ptrs.release(); h_work.release(); lut.release(); t_loc.release(); t_adj.release();
     t_f64.release(); t_u16.release(); s_rf.release(); s_planes.release(); s_flags.release();
+    l_line0.release(); l_locs.release(); l_first.release(); l_hit.release(); l_count.release(); l_fil.release();
+    l_tab.release();
     f_loc.release(); f_ref.release(); f_x.release(); f_y0.release(); f_pow.release(); f_fil.release();
     f_fo.release(); f_tab.release();
     if (f_stage) (void)hipHostFree(f_stage);
@@ -977,11 +983,130 @@ int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod_context* c, const
       VHSD_HIP(vhsd_launch_tbc_fields(video, span_length, c->bias, d_org, c->f_ref.p, nloc, p->field_lines, nfields,
                                       p->nominal_line_length, line_decay, smooth ? c->f_pow.p : nullptr, c->f_x.p,
                                       c->f_y0.p, p->output_line_length, smooth ? 1 : 0, p->ire0, out_scale,
-                                      1.0 / p->hz_ire, conv_off, c->lut.p, out, st));
+                                      1.0 / p->hz_ire, conv_off, c->lut.p, out, nullptr, st));
     }
     if (refined_out) {
       VHSD_HIP(hipMemcpyAsync(refined_out, c->f_ref.p, nl * sizeof(double), hipMemcpyDeviceToHost, st));
       VHSD_HIP(hipStreamSynchronize(st));
+    }
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_decode_span_fields_device(vhsdecode_hip_demod_context* c, const float* sync_ref,
+                                                const float* video, int64_t span_length,
+                                                const vhsdecode_hip_field_params_v1* p,
+                                                const vhsdecode_hip_locator_v1* lz, int32_t max_fields, uint16_t* out,
+                                                int32_t* field_count, double* field_line0, int32_t* field_first,
+                                                void* stream) {
+  try {
+    if (!c || !sync_ref || !video || !p || !lz || !out || !field_count || max_fields <= 0 || span_length <= 0)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid span field decode request");
+    if (p->struct_size != sizeof(vhsdecode_hip_field_params_v1) || lz->struct_size != sizeof(vhsdecode_hip_locator_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "field_params or locator struct_size mismatch");
+    if (p->field_lines <= 0 || p->field_lines + 1 > 512 || p->output_line_length <= 0 ||
+        !(p->nominal_line_length > 0.0) || !(lz->line_length > 0.0) || lz->block_length <= 0 ||
+        lz->block_stride <= 0 || lz->block_cut < 0)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid field or locator geometry");
+    if (span_length > INT32_MAX)
+      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "spans are limited to 2^31 - 1 samples (int32 pulse positions)");
+    std::lock_guard<std::mutex> lock(c->mu);
+    VHSD_HIP(hipSetDevice(c->device));
+    int32_t rc = ensure_lut(c);
+    if (rc) return rc;
+    hipStream_t st = pick_stream(c, stream);
+    const int nloc = p->field_lines + 1;
+    const size_t nl = (size_t)max_fields * nloc;
+    // pulses: at most one per 16 samples is kept (sync pulses are >= 1 us apart)
+    const int capacity = (int)std::min<int64_t>(span_length / 16 + 1024, INT32_MAX / 2);
+    const int tiles = (int)((span_length + 4095) / 4096);
+    const size_t edge_cap = (size_t)(span_length / 2 + 2);
+    VHSD_HIP(c->tile_counts.ensure(std::max(tiles, 1)));
+    VHSD_HIP(c->counters.ensure(2));
+    VHSD_HIP(c->edge_pos.ensure(edge_cap));
+    VHSD_HIP(c->edge_kind.ensure(edge_cap));
+    VHSD_HIP(c->p_starts.ensure(capacity));
+    VHSD_HIP(c->p_lens.ensure(capacity));
+    // candidates: every vertical-sync group is one; leave room for rejected ones
+    const int cand_cap = 2 * max_fields + 16;
+    VHSD_HIP(c->l_line0.ensure(cand_cap));
+    VHSD_HIP(c->l_first.ensure(cand_cap));
+    VHSD_HIP(c->l_hit.ensure(cand_cap));
+    VHSD_HIP(c->l_count.ensure(1));
+    VHSD_HIP(c->l_locs.ensure((size_t)cand_cap * nloc));
+    VHSD_HIP(c->l_fil.ensure((size_t)cand_cap * nloc));
+    VHSD_HIP(c->f_ref.ensure(nl));
+    VHSD_HIP(c->f_fo.ensure(nl));
+    VHSD_HIP(c->f_x.ensure((size_t)max_fields * p->field_lines));
+    VHSD_HIP(c->f_y0.ensure((size_t)max_fields * p->field_lines));
+    // field tables: 8 pointer-sized arrays, then line counts and first flags (int)
+    const size_t mf = (size_t)max_fields;
+    VHSD_HIP(c->l_tab.ensure(mf * (8 * 8 + 4 + 4) + 64));
+    unsigned char* tb = c->l_tab.p;
+    vhsd::FieldTables t;
+    t.spans = reinterpret_cast<const float**>(tb);
+    t.span_len = reinterpret_cast<long long*>(tb + 8 * mf);
+    t.locs = reinterpret_cast<const double**>(tb + 16 * mf);
+    t.filled = reinterpret_cast<const unsigned char**>(tb + 24 * mf);
+    t.refined = reinterpret_cast<double**>(tb + 32 * mf);
+    t.filled_out = reinterpret_cast<unsigned char**>(tb + 40 * mf);
+    t.origins = reinterpret_cast<long long*>(tb + 48 * mf);
+    t.info_line0 = field_line0 ? field_line0 : reinterpret_cast<double*>(tb + 56 * mf);
+    t.line_counts = reinterpret_cast<int*>(tb + 64 * mf);
+    t.info_first = field_first ? field_first : reinterpret_cast<int*>(tb + 68 * mf);
+    const bool smooth = p->wow_level_smoothing > 0.0;
+    const double alpha = smooth ? 1.0 / (p->wow_level_smoothing * p->output_line_length) : 0.0;
+    if (smooth && (c->f_pow_alpha != alpha || c->f_pow_len != p->output_line_length)) {
+      std::vector<double> pw(p->output_line_length);
+      for (int s2 = 0; s2 < p->output_line_length; s2++) pw[s2] = std::pow(1.0 - alpha, (double)(s2 + 1));
+      VHSD_HIP(c->f_pow.ensure(pw.size()));
+      VHSD_HIP(hipMemcpy(c->f_pow.p, pw.data(), sizeof(double) * pw.size(), hipMemcpyHostToDevice));
+      c->f_pow_alpha = alpha;
+      c->f_pow_len = p->output_line_length;
+    }
+    const double line_decay = smooth ? std::pow(1.0 - alpha, (double)p->output_line_length) : 0.0;
+    {
+      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_PULSES, span_length);
+      VHSD_HIP(hipMemsetAsync(c->counters.p, 0, 2 * sizeof(int), st));
+      VHSD_HIP(vhsd_launch_pulses(sync_ref, span_length, c->bias, p->threshold, lz->min_pulse_length,
+                                  lz->max_pulse_length, c->tile_counts.p, tiles, c->edge_pos.p, c->edge_kind.p,
+                                  (int)edge_cap, c->counters.p, c->p_starts.p, c->p_lens.p, capacity, st));
+    }
+    vhsd::LocateParams lp{};
+    lp.line_len = lz->line_length;
+    lp.broad_min = lz->broad_min_length;
+    lp.hsync_min = lz->hsync_min_length;
+    lp.tol = lz->snap_tolerance;
+    lp.lines = p->field_lines;
+    lp.span_len = span_length;
+    lp.lead_lines = lz->lead_lines;
+    lp.block_length = lz->block_length;
+    lp.block_cut = lz->block_cut;
+    lp.stride = lz->block_stride;
+    {
+      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_LOCATE, max_fields);
+      VHSD_HIP(vhsd_launch_locate(sync_ref, span_length, c->p_starts.p, c->p_lens.p, c->counters.p + 1, capacity, &lp,
+                                  max_fields, cand_cap, c->l_line0.p, c->l_first.p, c->l_count.p, c->l_locs.p,
+                                  c->l_fil.p, c->l_hit.p, c->f_ref.p, c->f_fo.p, &t, field_count, st));
+    }
+    vhsd::HsyncParams hp{p->threshold, p->sample_rate_mhz, p->hsync_pulse_us, p->ire0, p->hz_ire, p->is_pal,
+                         p->use_right_hsync};
+    {
+      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_HSYNC, max_fields);
+      VHSD_HIP(c->h_work.ensure(vhsd_hsync_work_bytes(max_fields, nloc)));
+      VHSD_HIP(vhsd_launch_hsync(t.spans, t.span_len, t.locs, t.filled, t.line_counts, &hp, t.refined, t.filled_out,
+                                 max_fields, nloc, c->bias, c->h_work.p, st));
+    }
+    const double out_scale = p->output_scale;
+    const double conv_off = (p->output_zero + 0.5) - (p->vsync_ire * out_scale);
+    {
+      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_TBC, (long long)max_fields * p->field_lines * p->output_line_length);
+      VHSD_HIP(vhsd_launch_tbc_fields(video, span_length, c->bias, t.origins, c->f_ref.p, nloc, p->field_lines,
+                                      max_fields, p->nominal_line_length, line_decay, smooth ? c->f_pow.p : nullptr,
+                                      c->f_x.p, c->f_y0.p, p->output_line_length, smooth ? 1 : 0, p->ire0, out_scale,
+                                      1.0 / p->hz_ire, conv_off, c->lut.p, out, field_count, st));
     }
     return VHSDECODE_HIP_OK;
   } catch (const std::exception& e) {

@@ -29,12 +29,14 @@ REQUIRED_EXPORTS = (
     "vhsdecode_hip_tbc_resample",
     "vhsdecode_hip_tbc_resample_device",
     "vhsdecode_hip_decode_fields_device",
+    "vhsdecode_hip_decode_span_fields_device",
     "vhsdecode_hip_set_profiling",
     "vhsdecode_hip_get_profile",
     "vhsdecode_hip_debug_fft16k",
 )
 
-KERNEL_NAMES = ("block_demod", "spike_repair", "chroma", "pulses", "hsync", "tbc", "sub_deemphasis", "block_video")
+KERNEL_NAMES = ("block_demod", "spike_repair", "chroma", "pulses", "hsync", "tbc", "sub_deemphasis", "block_video",
+                "locate")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -104,6 +106,14 @@ class FieldParamsV1(ctypes.Structure):
                 ("vsync_ire", ctypes.c_double), ("output_scale", ctypes.c_double)]
 
 
+class LocatorV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("min_pulse_length", ctypes.c_int32),
+                ("max_pulse_length", ctypes.c_int32), ("line_length", ctypes.c_double),
+                ("broad_min_length", ctypes.c_double), ("hsync_min_length", ctypes.c_double),
+                ("snap_tolerance", ctypes.c_double), ("lead_lines", ctypes.c_double),
+                ("block_length", ctypes.c_int32), ("block_cut", ctypes.c_int32), ("block_stride", ctypes.c_int32)]
+
+
 class ProfileV1(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("launches", ctypes.c_int64 * len(KERNEL_NAMES)),
                 ("total_ms", ctypes.c_double * len(KERNEL_NAMES)), ("units", ctypes.c_int64 * len(KERNEL_NAMES))]
@@ -151,6 +161,10 @@ def load():
                                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                                     ctypes.POINTER(TbcParamsV1), ctypes.c_void_p, ctypes.c_void_p,
                                                     ctypes.c_void_p]
+    L.vhsdecode_hip_decode_span_fields_device.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(FieldParamsV1),
+        ctypes.POINTER(LocatorV1), ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.c_void_p]
     L.vhsdecode_hip_decode_fields_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                      ctypes.c_int64, _dp, _u8p, ctypes.POINTER(ctypes.c_int64),
                                                      ctypes.c_int32,

@@ -214,6 +214,23 @@ class HipBlockDemodulator:
                                                       _ptr(f64, _lib._dp) if f64 is not None else None))
         return u16, f64
 
+    def field_params(self, geom):
+        ire0, hz_ire, ozero, vsync, oscale = geom.conv
+        return _lib.FieldParamsV1(ctypes.sizeof(_lib.FieldParamsV1), geom.threshold, geom.fs_mhz,
+                                  geom.hsync / geom.fs_mhz, ire0, hz_ire, 1 if geom.is_pal else 0, 1, geom.out_len,
+                                  geom.field_lines, geom.nominal, geom.smoothing, ozero, vsync, oscale)
+
+    def decode_span_fields_device(self, sync_ref_ptr: int, video_ptr: int, span_length: int, geom, locator,
+                                  max_fields: int, out_ptr: int, count_ptr: int, line0_ptr: int = 0,
+                                  first_ptr: int = 0, stream: int = 0):
+        """Pulses -> line locations -> hsync -> TBC of up to max_fields fields, all on the device and
+        asynchronous; the field count lands in device int32 at count_ptr."""
+        p = self.field_params(geom)
+        _lib.check(self._L.vhsdecode_hip_decode_span_fields_device(
+            self._h, ctypes.c_void_p(sync_ref_ptr), ctypes.c_void_p(video_ptr), span_length, ctypes.byref(p),
+            ctypes.byref(locator), max_fields, ctypes.c_void_p(out_ptr), ctypes.c_void_p(count_ptr),
+            ctypes.c_void_p(line0_ptr or None), ctypes.c_void_p(first_ptr or None), ctypes.c_void_p(stream)))
+
     def decode_fields_device(self, sync_ref_ptr: int, video_ptr: int, span_length: int, locations: np.ndarray,
                              filled: np.ndarray, geom, out_ptr: int, origins: Optional[np.ndarray] = None,
                              want_refined: bool = False, stream: int = 0):
@@ -222,10 +239,7 @@ class HipBlockDemodulator:
         loc = np.ascontiguousarray(locations, dtype=np.float64)
         fil = np.ascontiguousarray(filled, dtype=np.uint8)
         nfields = loc.shape[0]
-        ire0, hz_ire, ozero, vsync, oscale = geom.conv
-        p = _lib.FieldParamsV1(ctypes.sizeof(_lib.FieldParamsV1), geom.threshold, geom.fs_mhz,
-                               geom.hsync / geom.fs_mhz, ire0, hz_ire, 1 if geom.is_pal else 0, 1, geom.out_len,
-                               geom.field_lines, geom.nominal, geom.smoothing, ozero, vsync, oscale)
+        p = self.field_params(geom)
         ref = np.empty_like(loc) if want_refined else None
         org = None if origins is None else np.ascontiguousarray(origins, dtype=np.int64)
         _lib.check(self._L.vhsdecode_hip_decode_fields_device(

@@ -28,6 +28,9 @@ import numpy as np
 
 from .formats import FormatParameters
 
+SNAP_TOLERANCE = 0.12   # lines
+LEAD_LINES = 8.0        # read-window lead before line 0
+
 
 @dataclass
 class FieldLines:
@@ -77,7 +80,7 @@ def sync_geometry(fmt: FormatParameters, sample_rate_hz: float = 40e6) -> SyncGe
 
 
 def line_locations_from_pulses(starts: np.ndarray, lens: np.ndarray, g: SyncGeometry, span_len: int,
-                               tol: float = 0.12) -> List[FieldLines]:
+                               tol: float = SNAP_TOLERANCE) -> List[FieldLines]:
     starts = np.asarray(starts, dtype=np.float64)
     lens = np.asarray(lens, dtype=np.float64)
     out: List[FieldLines] = []
@@ -111,7 +114,7 @@ def line_locations_from_pulses(starts: np.ndarray, lens: np.ndarray, g: SyncGeom
 
 
 def field_origin(line0: float, g: SyncGeometry, block_length: int = 32768, block_cut: int = 1024,
-                 stride: int = 30720, lead_lines: float = 8.0) -> int:
+                 stride: int = 30720, lead_lines: float = LEAD_LINES) -> int:
     """Start of a field's read window in span coordinates, placed the way
     DecodeReadWindowPlanner.Resolve places it (Decode/DecodeReadWindowPlanner.cs:37-63):
     the intended start (here `lead_lines` before line 0) less the block cut, aligned
@@ -121,6 +124,16 @@ def field_origin(line0: float, g: SyncGeometry, block_length: int = 32768, block
     read = max(0, int(intended) - block_cut)
     aligned = (read // block_length) * block_length
     return (aligned // stride) * stride
+
+
+def locator(g: SyncGeometry, block_length: int = 32768, block_cut: int = 1024, stride: int = 30720):
+    """The device line locator's parameters (vhsdecode_hip_locator_v1) for geometry g;
+    the same thresholds line_locations_from_pulses applies."""
+    from . import _lib
+    import ctypes
+    return _lib.LocatorV1(ctypes.sizeof(_lib.LocatorV1), g.min_pulse, g.max_pulse, g.line_len,
+                          0.5 * (g.hsync + g.vsync), 0.5 * (g.eq + g.hsync), SNAP_TOLERANCE, LEAD_LINES,
+                          block_length, block_cut, stride)
 
 
 class HipFieldDecoder:
@@ -137,6 +150,13 @@ class HipFieldDecoder:
         st, ln = self.demod.find_pulses(span_vlp_ptr, span_len, g.threshold, g.min_pulse, g.max_pulse)
         self.last_pulses = (st, ln)
         return line_locations_from_pulses(st, ln, g, span_len)
+
+    def decode_async(self, span_video_ptr: int, span_vlp_ptr: int, span_len: int, out_ptr: int, max_fields: int,
+                     count_ptr: int, line0_ptr: int = 0, first_ptr: int = 0, stream: int = 0):
+        """Every step on the device with no host round trip; the field count lands in
+        device int32 at count_ptr (and line 0 / first-field flags, when given)."""
+        self.demod.decode_span_fields_device(span_vlp_ptr, span_video_ptr, span_len, self.geom, locator(self.geom),
+                                             max_fields, out_ptr, count_ptr, line0_ptr, first_ptr, stream)
 
     def decode(self, span_video_ptr: int, span_vlp_ptr: int, span_len: int, out_ptr: int, max_fields: int,
                want_refined: bool = False):
