@@ -477,10 +477,13 @@ __global__ void __launch_bounds__(64) hsync_spec_kernel(const float* const* span
   }
 }
 
-// Pass 3: one wavefront per field walks the carried porch level in line order.  A
-// line whose speculative porch matches the true one keeps its parallel result; any
-// other line is refined again here with the true porch, exactly as the sequential
-// reference would, and the walk continues from its result.
+// Pass 3: one wavefront per field checks the carried porch level in line order.  A
+// line whose speculative porch matches the true one keeps its parallel result.  The
+// check runs 64 lines at a time: the true porch carried into a line is the porch
+// carried out of the previous refined line, so every line before the first mismatch
+// is verified at once; from the first mismatch on the wave walks the lines in order,
+// refining again (with the true porch, exactly as the sequential reference would)
+// every line whose speculation was wrong.
 __global__ void __launch_bounds__(64) hsync_verify_kernel(const float* const* spans, const long long* span_len,
                                                           const double* const* locs, const unsigned char* const* filled,
                                                           const int* line_counts, HsyncParams p, HsyncWork wk,
@@ -491,26 +494,49 @@ __global__ void __launch_bounds__(64) hsync_verify_kernel(const float* const* sp
   WaveCtx w{spans[f], span_len[f], (int)threadIdx.x, bias};
   const HsyncGeom g = hsync_geom(p);
   const int base = f * wk.stride;
-  double pc = -1.0;
+  // Alternate a parallel check (64 lines at a time: the first line at or after `from`
+  // whose speculative porch differs from the carried one) with an in-order refinement
+  // of that line, which fixes the porch carried on.
+  double carry = -1.0;  // porch carried into line `from`
+  int from = 0;
   int redone = 0;
-  for (int line = 0; line < lines; line++) {
-    if (wk.kind[base + line] != 0) continue;
-    if (__double_as_longlong(wk.p_used[base + line]) == __double_as_longlong(pc)) {
-      pc = wk.p_out[base + line];
-      continue;
+  while (from < lines) {
+    int bad_line = lines;
+    double bad_pc = 0.0;
+    for (int c0 = from; c0 < lines; c0 += 64) {
+      const int l = c0 + w.lane;
+      const bool valid = l < lines;
+      const bool is0 = valid && wk.kind[base + l] == 0;
+      const double pu = is0 ? wk.p_used[base + l] : 0.0;
+      const double po = is0 ? wk.p_out[base + l] : 0.0;
+      const unsigned long long mask = __ballot(is0);
+      const unsigned long long below = mask & ((1ull << w.lane) - 1ull);
+      const double prev_out = __shfl(po, below ? 63 - __clzll((long long)below) : 0);
+      const double expected = below ? prev_out : carry;
+      const bool bad = is0 && __double_as_longlong(pu) != __double_as_longlong(expected);
+      const unsigned long long bm = __ballot(bad);
+      if (bm) {
+        const int first = __ffsll((long long)bm) - 1;
+        bad_line = c0 + first;
+        bad_pc = __shfl(expected, first);
+        break;
+      }
+      if (mask) carry = __shfl(po, 63 - __clzll((long long)mask));
     }
-    const double cur = locs[f][line];
+    if (bad_line >= lines) break;
+    const double cur = locs[f][bad_line];
     const long long ss = iround(cur - g.one_us);
     double loc = 0.0, pout = 0.0;
     unsigned char fil = 0;
-    refine_line(w, p, g, ss, cur, filled[f][line], pc, wk.rt_ok[base + line] != 0, wk.rt_cand[base + line],
-                wk.rt_porch[base + line], &loc, &fil, &pout);
-    pc = pout;
+    refine_line(w, p, g, ss, cur, filled[f][bad_line], bad_pc, wk.rt_ok[base + bad_line] != 0,
+                wk.rt_cand[base + bad_line], wk.rt_porch[base + bad_line], &loc, &fil, &pout);
     redone++;
     if (w.lane == 0) {
-      refined_out[f][line] = loc;
-      filled_out[f][line] = fil;
+      refined_out[f][bad_line] = loc;
+      filled_out[f][bad_line] = fil;
     }
+    carry = pout;
+    from = bad_line + 1;
   }
   if (w.lane == 0) wk.fb_line[f] = redone;
 }
@@ -536,8 +562,16 @@ __device__ __forceinline__ double sample_sinc(const float* src, long long len, d
   a0 = a0 < 0 ? 0 : (a0 > phases ? phases : a0);
   int a1 = pe < 0 ? pe + phases + 1 : pe;
   a1 = a1 < 0 ? 0 : (a1 > phases ? phases : a1);
-  const float* ws = lut + a0 * taps;
-  const float* we = lut + a1 * taps;
+  // table rows are 64-byte aligned: fetch each as four 16-byte loads
+  const float4* ws4 = reinterpret_cast<const float4*>(lut + a0 * taps);
+  const float4* we4 = reinterpret_cast<const float4*>(lut + a1 * taps);
+  float ws[16], we[16];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const float4 a = ws4[q], b = we4[q];
+    ws[4 * q] = a.x; ws[4 * q + 1] = a.y; ws[4 * q + 2] = a.z; ws[4 * q + 3] = a.w;
+    we[4 * q] = b.x; we[4 * q + 1] = b.y; we[4 * q + 2] = b.z; we[4 * q + 3] = b.w;
+  }
   const long long s0 = (long long)ci - (taps / 2 - 1);
   const bool interior = len >= taps && s0 >= 0 && s0 <= len - taps;
   double result = 0.0;
@@ -718,12 +752,12 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
                                                          const int* field_count) {
   const int f = blockIdx.y;
   if (field_count && f >= *field_count) return;
-  const long long d = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long dest_len = (long long)p.out_len * p.lines;
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;  // a field has < 2^31 output samples
+  const int dest_len = p.out_len * p.lines;
   if (d >= dest_len) return;
   const double* loc = locs + (long long)f * p.nloc;
-  const int left = (int)(d / p.out_len);
-  const int smp = (int)(d - (long long)left * p.out_len);
+  const int left = d / p.out_len;
+  const int smp = d - left * p.out_len;
   const double ll = loc[left];
   const double delta = loc[left + 1] - ll;
   const double scale = p.nominal / p.out_len;
