@@ -45,6 +45,7 @@ enum {
   VHSDECODE_HIP_E_OUT_OF_MEMORY = -30005,
   VHSDECODE_HIP_E_UNSUPPORTED = -30006,
   VHSDECODE_HIP_E_CAPACITY = -30007,
+  VHSDECODE_HIP_E_TIMEOUT = -30008,
   VHSDECODE_HIP_E_INTERNAL = -30099
 };
 
@@ -296,6 +297,80 @@ VHSDECODE_HIP_API int32_t vhsdecode_hip_decode_span_fields_device(
     vhsdecode_hip_demod_context* context, const float* sync_ref, const float* video, int64_t span_length,
     const vhsdecode_hip_field_params_v1* params, const vhsdecode_hip_locator_v1* locator, int32_t max_fields,
     uint16_t* out, int32_t* field_count, double* field_line0, int32_t* field_first, void* stream);
+
+/* ---------------------------------------------------------------- served windows
+ * A device-resident request loop for seekable decode windows from many concurrent
+ * streams, the served counterpart of the reference's preview windows
+ * (src/VHSDecode.Preview: preview_create / decode_window / destroy,
+ * vhsdecode_cuda_fast.h:55-180) made asynchronous.  Requests queue in arrival order
+ * and are admitted as device slots free up, within an HBM budget; each admitted
+ * window runs the whole chain (RF upload, block demod, field decode, download of
+ * the uint16 fields) on its slot's own HIP stream, so windows overlap on the
+ * device.  Filter tables and the sinc table stay resident per slot and format.
+ * A worker thread owned by the server issues all device work; submit and wait may be
+ * called from any thread.  A window's result does not depend on what else is in
+ * flight. */
+typedef struct vhsdecode_hip_serve vhsdecode_hip_serve;
+
+typedef struct vhsdecode_hip_serve_config_v1 {
+  uint32_t struct_size;
+  int32_t device_id;
+  int32_t max_slots;               /* windows in flight at once */
+  uint64_t hbm_budget_bytes;       /* admission budget for the slots' device memory */
+  int64_t max_window_samples;      /* largest window a request may ask for */
+  int32_t max_fields;              /* largest field count a request may ask for */
+} vhsdecode_hip_serve_config_v1;
+
+/* One decode format: the block filters and cuts, the field geometry and the line
+ * locator.  Everything is copied; the caller's arrays may be freed after the call. */
+typedef struct vhsdecode_hip_serve_format_v1 {
+  uint32_t struct_size;
+  const vhsdecode_hip_filters_v1* filters;
+  int32_t block_cut;
+  int32_t block_cut_end;
+  const vhsdecode_hip_field_params_v1* field;
+  const vhsdecode_hip_locator_v1* locator;
+  const float* sinc_table;         /* 65537 x 16 FP32, or NULL for the Kaiser fallback */
+  int64_t sinc_count;
+} vhsdecode_hip_serve_format_v1;
+
+typedef struct vhsdecode_hip_window_v1 {
+  uint32_t struct_size;
+  int32_t stream_id;
+  int64_t first_sample;            /* even */
+  int64_t sample_count;            /* >= block_length, <= max_window_samples */
+  int32_t max_fields;              /* <= the server's max_fields */
+  uint16_t* fields_out;            /* host [max_fields][field_lines * output_line_length]; pinned
+                                      memory (hipHostMalloc) is written by DMA, other memory
+                                      through a blocking copy */
+} vhsdecode_hip_window_v1;
+
+typedef struct vhsdecode_hip_window_result_v1 {
+  uint32_t struct_size;
+  int32_t status;                  /* 0, or the error the window failed with */
+  int32_t field_count;             /* fields written to fields_out */
+  double queued_ms;                /* submit -> admitted to a slot */
+  double latency_ms;               /* submit -> fields in host memory */
+} vhsdecode_hip_window_result_v1;
+
+VHSDECODE_HIP_API int32_t vhsdecode_hip_serve_create(const vhsdecode_hip_serve_config_v1* config,
+                                                     vhsdecode_hip_serve** server);
+VHSDECODE_HIP_API int32_t vhsdecode_hip_serve_add_format(vhsdecode_hip_serve* server,
+                                                         const vhsdecode_hip_serve_format_v1* format,
+                                                         int32_t* format_id);
+/* A stream reads `samples` (host, int16, caller-owned for the server's lifetime);
+ * with loop != 0 positions wrap modulo `length` (an endless recording). */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_serve_add_stream(vhsdecode_hip_serve* server, int32_t format_id,
+                                                         const int16_t* samples, int64_t length, int32_t loop,
+                                                         int32_t* stream_id);
+VHSDECODE_HIP_API int32_t vhsdecode_hip_serve_submit(vhsdecode_hip_serve* server,
+                                                     const vhsdecode_hip_window_v1* window, uint64_t* ticket);
+/* Waits up to timeout_ms (< 0: forever) for a ticket; VHSDECODE_HIP_E_TIMEOUT if it is
+ * still in flight.  A ticket's result is returned once. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_serve_wait(vhsdecode_hip_serve* server, uint64_t ticket, int32_t timeout_ms,
+                                                   vhsdecode_hip_window_result_v1* result);
+/* Finishes the windows in flight, drops queued ones, frees every device resource. */
+VHSDECODE_HIP_API void vhsdecode_hip_serve_destroy(vhsdecode_hip_serve* server);
 
 /* ---------------------------------------------------------------- observability
  * Per-kernel device time, the counterpart of vhsdecode_cuda_fast_result_v1.elapsed_seconds

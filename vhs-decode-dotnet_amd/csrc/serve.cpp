@@ -1,0 +1,494 @@
+// Served decode windows: the request loop behind vhsdecode_hip_serve_*.
+//
+// One worker thread owns every device call.  It admits queued requests in arrival
+// order into slots (a HIP stream, device buffers sized for the largest window, and
+// one demod context per decode format, so filter tables stay resident), issues the
+// whole chain of an admitted window asynchronously on the slot's stream, and polls
+// the slots' completion events.  Slots are created on demand while the HBM budget
+// allows, and reused.  Windows in different slots overlap on the device; a window's
+// kernels and inputs do not depend on the other windows in flight, so its result is
+// the same alone and in the mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <exception>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "host_common.hpp"
+#include "vhsdecode_hip.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+using vhsd_host::set_error;
+
+constexpr int kBlock = 32768;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+bool is_pinned(const void* p) {
+  hipPointerAttribute_t attr{};
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();  // plain pageable memory reports an error; clear it
+    return false;
+  }
+  return attr.type == hipMemoryTypeHost;
+}
+
+// Owned copy of one decode format.
+struct Format {
+  std::vector<double> rf_video, video, vlp05, env_sos, chroma_sos, hp, amp_sos;
+  std::vector<float> sinc;
+  vhsdecode_hip_filters_v1 filters{};
+  vhsdecode_hip_subdeemph_v1 sd{};
+  int cut = 0, cut_end = 0, stride = 0;
+  vhsdecode_hip_field_params_v1 field{};
+  vhsdecode_hip_locator_v1 locator{};
+  size_t per_field = 0;  // output samples per field
+};
+
+struct Source {
+  int format;
+  const int16_t* samples;
+  int64_t length;
+  bool loop;
+};
+
+struct Request {
+  vhsdecode_hip_window_v1 w{};
+  Clock::time_point t_submit, t_admit;
+  int32_t status = 0;
+  int32_t field_count = 0;
+  double queued_ms = 0.0, latency_ms = 0.0;
+  bool done = false;
+  bool pinned_out = false;
+};
+
+struct Slot {
+  int64_t max_blocks = 0;  // blocks of the largest window at the smallest stride known when created
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  int16_t* rf = nullptr;
+  float* planes = nullptr;
+  unsigned* flags = nullptr;
+  uint16_t* fields = nullptr;
+  size_t fields_cap = 0;  // uint16 entries
+  int32_t* meta = nullptr;
+  int32_t* meta_host = nullptr;  // pinned
+  uint16_t* stage = nullptr;     // pinned staging for pageable outputs
+  size_t stage_cap = 0;
+  std::map<int, vhsdecode_hip_demod_context*> ctx;
+  std::shared_ptr<Request> req;
+};
+
+}  // namespace
+
+struct vhsdecode_hip_serve {
+  vhsdecode_hip_serve_config_v1 cfg{};
+  int min_stride = kBlock;  // smallest block stride of the registered formats
+  size_t slot_bytes() const;
+  std::mutex mu;
+  std::condition_variable cv;       // worker wake-ups
+  std::condition_variable done_cv;  // waiters
+  std::vector<std::unique_ptr<Format>> formats;
+  std::vector<Source> sources;
+  std::deque<std::shared_ptr<Request>> queue;
+  std::map<uint64_t, std::shared_ptr<Request>> tickets;
+  std::vector<std::unique_ptr<Slot>> slots;
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  std::thread worker;
+
+  ~vhsdecode_hip_serve();
+  void run();
+  int32_t make_slot(Slot** out);
+  void release_slot(Slot* s);
+  int32_t launch(Slot* s, const std::shared_ptr<Request>& r);
+  void finish(Slot* s, int32_t status);
+};
+
+namespace {
+
+#define SV_HIP(call)                                                                                      \
+  do {                                                                                                    \
+    hipError_t e_ = (call);                                                                               \
+    if (e_ != hipSuccess) return set_error(VHSDECODE_HIP_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace
+
+// Device bytes of one slot: RF, the four planes, and the demod context's per-block
+// scratch (spectrum stash, repair scratch) and per-sample pulse edges.
+size_t vhsdecode_hip_serve::slot_bytes() const {
+  const int64_t blocks = (cfg.max_window_samples - kBlock) / min_stride + 1;
+  const int64_t span = blocks * min_stride;
+  return (size_t)cfg.max_window_samples * 2 + (size_t)span * 16 +
+         (size_t)blocks * (8 * (2 * 16384 + 64) + 8 * 32768 + 4 * 65536) + (size_t)span * 10 +
+         (size_t)cfg.max_fields * 1135 * 313 * 2;
+}
+
+int32_t vhsdecode_hip_serve::make_slot(Slot** out) {
+  auto s = std::make_unique<Slot>();
+  s->max_blocks = (cfg.max_window_samples - kBlock) / min_stride + 1;
+  const int64_t span = s->max_blocks * min_stride;
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&s->done, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->rf, sizeof(int16_t) * (size_t)cfg.max_window_samples);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->planes, sizeof(float) * 4 * (size_t)span);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->flags, sizeof(unsigned) * (size_t)s->max_blocks);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->meta, 64);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&s->meta_host, 64, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    release_slot(s.get());
+    return set_error(VHSDECODE_HIP_E_OUT_OF_MEMORY, std::string("slot allocation failed: ") + hipGetErrorString(e));
+  }
+  *out = s.get();
+  slots.push_back(std::move(s));
+  return VHSDECODE_HIP_OK;
+}
+
+void vhsdecode_hip_serve::release_slot(Slot* s) {
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (auto& kv : s->ctx) vhsdecode_hip_demod_destroy(kv.second);
+  s->ctx.clear();
+  if (s->rf) (void)hipFree(s->rf);
+  if (s->planes) (void)hipFree(s->planes);
+  if (s->flags) (void)hipFree(s->flags);
+  if (s->fields) (void)hipFree(s->fields);
+  if (s->meta) (void)hipFree(s->meta);
+  if (s->meta_host) (void)hipHostFree(s->meta_host);
+  if (s->stage) (void)hipHostFree(s->stage);
+  if (s->done) (void)hipEventDestroy(s->done);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  *s = Slot{};
+}
+
+int32_t vhsdecode_hip_serve::launch(Slot* s, const std::shared_ptr<Request>& r) {
+  const vhsdecode_hip_window_v1& w = r->w;
+  const Source& src = sources[w.stream_id];
+  Format& f = *formats[src.format];
+  auto it = s->ctx.find(src.format);
+  vhsdecode_hip_demod_context* ctx = nullptr;
+  if (it == s->ctx.end()) {
+    vhsdecode_hip_demod_config_v1 dc{};
+    dc.struct_size = sizeof(dc);
+    dc.device_id = cfg.device_id;
+    dc.block_cut = f.cut;
+    dc.block_cut_end = f.cut_end;
+    dc.max_blocks = (int32_t)s->max_blocks;
+    dc.filters = &f.filters;
+    int32_t rc = vhsdecode_hip_demod_create(&dc, &ctx);
+    if (rc) return rc;
+    if (!f.sinc.empty() && (rc = vhsdecode_hip_set_sinc_table(ctx, f.sinc.data(), (int64_t)f.sinc.size())) != 0) {
+      vhsdecode_hip_demod_destroy(ctx);
+      return rc;
+    }
+    s->ctx[src.format] = ctx;
+  } else {
+    ctx = it->second;
+  }
+  const size_t need = (size_t)w.max_fields * f.per_field;
+  if (need > s->fields_cap) {
+    if (s->fields) (void)hipFree(s->fields);
+    s->fields = nullptr;
+    s->fields_cap = 0;
+    SV_HIP(hipMalloc((void**)&s->fields, sizeof(uint16_t) * need));
+    s->fields_cap = need;
+  }
+  // RF window: one copy, or one per wrap of a looped recording
+  int64_t pos = w.first_sample, done = 0;
+  while (done < w.sample_count) {
+    const int64_t p = src.loop ? pos % src.length : pos;
+    const int64_t n = std::min<int64_t>(w.sample_count - done, src.length - p);
+    SV_HIP(hipMemcpyAsync(s->rf + done, src.samples + p, sizeof(int16_t) * (size_t)n, hipMemcpyHostToDevice,
+                          s->stream));
+    done += n;
+    pos += n;
+  }
+  const int stride = f.stride;
+  const int32_t nb = (int32_t)((w.sample_count - kBlock) / stride + 1);
+  if (nb > s->max_blocks)
+    return set_error(VHSDECODE_HIP_E_CAPACITY, "window needs more blocks than the slot was sized for");
+  const int64_t n = (int64_t)nb * stride;
+  vhsdecode_hip_span_v1 sp{};
+  sp.struct_size = sizeof(sp);
+  sp.video = s->planes;
+  sp.video_lowpass = s->planes + n;
+  sp.envelope = s->planes + 2 * n;
+  sp.chroma = s->planes + 3 * n;
+  sp.block_flags = s->flags;
+  int32_t rc = vhsdecode_hip_demod_span_device(ctx, s->rf, 0, nb, &sp, s->stream);
+  if (rc) return rc;
+  rc = vhsdecode_hip_decode_span_fields_device(ctx, sp.video_lowpass, sp.video, n, &f.field, &f.locator,
+                                               w.max_fields, s->fields, s->meta, nullptr, nullptr, s->stream);
+  if (rc) return rc;
+  SV_HIP(hipMemcpyAsync(s->meta_host, s->meta, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+  uint16_t* dst = w.fields_out;
+  if (!r->pinned_out) {
+    if (need > s->stage_cap) {
+      if (s->stage) (void)hipHostFree(s->stage);
+      s->stage = nullptr;
+      s->stage_cap = 0;
+      SV_HIP(hipHostMalloc((void**)&s->stage, sizeof(uint16_t) * need, hipHostMallocDefault));
+      s->stage_cap = need;
+    }
+    dst = s->stage;
+  }
+  SV_HIP(hipMemcpyAsync(dst, s->fields, sizeof(uint16_t) * need, hipMemcpyDeviceToHost, s->stream));
+  SV_HIP(hipEventRecord(s->done, s->stream));
+  return VHSDECODE_HIP_OK;
+}
+
+void vhsdecode_hip_serve::finish(Slot* s, int32_t status) {
+  const std::shared_ptr<Request> r = s->req;
+  s->req.reset();
+  r->status = status;
+  if (status == 0) {
+    r->field_count = s->meta_host[0];
+    if (!r->pinned_out) {
+      const Format& f = *formats[sources[r->w.stream_id].format];
+      std::memcpy(r->w.fields_out, s->stage, sizeof(uint16_t) * (size_t)r->field_count * f.per_field);
+    }
+  }
+  r->latency_ms = ms_between(r->t_submit, Clock::now());
+  r->done = true;
+}
+
+void vhsdecode_hip_serve::run() {
+  (void)hipSetDevice(cfg.device_id);
+  std::unique_lock<std::mutex> lock(mu);
+  for (;;) {
+    bool finished = false;
+    for (auto& s : slots) {
+      if (!s->req) continue;
+      const hipError_t q = hipEventQuery(s->done);
+      if (q == hipErrorNotReady) {
+        (void)hipGetLastError();  // not an error: keep it out of the launchers' error checks
+        continue;
+      }
+      finish(s.get(), q == hipSuccess ? 0 : VHSDECODE_HIP_E_HIP);
+      finished = true;
+    }
+    if (finished) done_cv.notify_all();
+    while (!stop && !queue.empty()) {
+      Slot* slot = nullptr;
+      for (auto& s : slots)
+        if (!s->req) {
+          slot = s.get();
+          break;
+        }
+      if (!slot && (int)slots.size() < cfg.max_slots &&
+          (slots.size() + 1) * slot_bytes() <= cfg.hbm_budget_bytes) {
+        if (make_slot(&slot) != 0) slot = nullptr;
+      }
+      if (!slot) break;
+      std::shared_ptr<Request> r = queue.front();
+      queue.pop_front();
+      r->t_admit = Clock::now();
+      r->queued_ms = ms_between(r->t_submit, r->t_admit);
+      slot->req = r;
+      const int32_t rc = launch(slot, r);
+      if (rc) {
+        (void)hipStreamSynchronize(slot->stream);
+        finish(slot, rc);
+        done_cv.notify_all();
+      }
+    }
+    bool busy = false;
+    for (auto& s : slots) busy |= (bool)s->req;
+    if (stop && !busy) break;
+    if (busy)
+      cv.wait_for(lock, std::chrono::microseconds(100));
+    else
+      cv.wait(lock, [&] { return stop || !queue.empty(); });
+  }
+  for (auto& r : queue) {
+    r->status = VHSDECODE_HIP_E_INVALID_ARGUMENT;
+    r->done = true;
+  }
+  queue.clear();
+  done_cv.notify_all();
+}
+
+vhsdecode_hip_serve::~vhsdecode_hip_serve() {
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    stop = true;
+  }
+  cv.notify_all();
+  if (worker.joinable()) worker.join();
+  (void)hipSetDevice(cfg.device_id);
+  for (auto& s : slots) release_slot(s.get());
+}
+
+extern "C" {
+
+int32_t vhsdecode_hip_serve_create(const vhsdecode_hip_serve_config_v1* cfg, vhsdecode_hip_serve** out) {
+  try {
+    if (!cfg || !out) return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "config and server must not be NULL");
+    *out = nullptr;
+    if (cfg->struct_size != sizeof(vhsdecode_hip_serve_config_v1))
+      return set_error(VHSDECODE_HIP_E_STRUCT_SIZE, "serve_config struct_size mismatch");
+    if (cfg->max_slots <= 0 || cfg->max_window_samples < kBlock || cfg->max_fields <= 0 ||
+        cfg->max_window_samples > INT32_MAX)
+      return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid serve limits");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+      return set_error(VHSDECODE_HIP_E_NO_DEVICE, "no HIP device is available");
+    if (cfg->device_id < 0 || cfg->device_id >= count)
+      return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "device_id out of range");
+    auto* s = new (std::nothrow) vhsdecode_hip_serve();
+    if (!s) return set_error(VHSDECODE_HIP_E_OUT_OF_MEMORY, "server allocation failed");
+    s->cfg = *cfg;
+    s->worker = std::thread([s] { s->run(); });
+    *out = s;
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return set_error(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_serve_add_format(vhsdecode_hip_serve* s, const vhsdecode_hip_serve_format_v1* fm,
+                                       int32_t* format_id) {
+  try {
+    if (!s || !fm || !format_id || !fm->filters || !fm->field || !fm->locator)
+      return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid format");
+    if (fm->struct_size != sizeof(vhsdecode_hip_serve_format_v1) ||
+        fm->filters->struct_size != sizeof(vhsdecode_hip_filters_v1) ||
+        fm->field->struct_size != sizeof(vhsdecode_hip_field_params_v1) ||
+        fm->locator->struct_size != sizeof(vhsdecode_hip_locator_v1))
+      return set_error(VHSDECODE_HIP_E_STRUCT_SIZE, "format struct_size mismatch");
+    const vhsdecode_hip_filters_v1& src = *fm->filters;
+    if (src.block_length != kBlock || !src.rf_video || !src.video || !src.video_lowpass05 || !src.envelope_sos)
+      return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "format filters are incomplete");
+    const int stride = kBlock - fm->block_cut - fm->block_cut_end;
+    if (stride < kBlock / 2) return set_error(VHSDECODE_HIP_E_UNSUPPORTED, "block cuts leave less than half a block");
+    auto f = std::make_unique<Format>();
+    const size_t spec = 2 * (size_t)kBlock;
+    f->rf_video.assign(src.rf_video, src.rf_video + spec);
+    f->video.assign(src.video, src.video + spec);
+    f->vlp05.assign(src.video_lowpass05, src.video_lowpass05 + spec);
+    f->env_sos.assign(src.envelope_sos, src.envelope_sos + 6 * (size_t)src.envelope_sections);
+    if (src.chroma_sos && src.chroma_sections > 0)
+      f->chroma_sos.assign(src.chroma_sos, src.chroma_sos + 6 * (size_t)src.chroma_sections);
+    f->filters = src;
+    f->filters.rf_video = f->rf_video.data();
+    f->filters.video = f->video.data();
+    f->filters.video_lowpass05 = f->vlp05.data();
+    f->filters.envelope_sos = f->env_sos.data();
+    f->filters.chroma_sos = f->chroma_sos.empty() ? nullptr : f->chroma_sos.data();
+    f->filters.chroma_sections = f->chroma_sos.empty() ? 0 : src.chroma_sections;
+    f->filters.sub_deemphasis = nullptr;
+    if (src.sub_deemphasis) {
+      const vhsdecode_hip_subdeemph_v1& sd = *src.sub_deemphasis;
+      if (sd.struct_size != sizeof(vhsdecode_hip_subdeemph_v1) || !sd.high_pass || !sd.amplitude_sos)
+        return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid sub_deemphasis");
+      f->hp.assign(sd.high_pass, sd.high_pass + spec);
+      f->amp_sos.assign(sd.amplitude_sos, sd.amplitude_sos + 6);
+      f->sd = sd;
+      f->sd.high_pass = f->hp.data();
+      f->sd.amplitude_sos = f->amp_sos.data();
+      f->filters.sub_deemphasis = &f->sd;
+    }
+    if (fm->sinc_table && fm->sinc_count > 0) f->sinc.assign(fm->sinc_table, fm->sinc_table + fm->sinc_count);
+    f->cut = fm->block_cut;
+    f->cut_end = fm->block_cut_end;
+    f->stride = stride;
+    f->field = *fm->field;
+    f->locator = *fm->locator;
+    f->per_field = (size_t)f->field.field_lines * f->field.output_line_length;
+    std::lock_guard<std::mutex> lock(s->mu);
+    const int min_stride = std::min(s->min_stride, stride);
+    if (s->slots.empty()) {
+      const int keep = s->min_stride;
+      s->min_stride = min_stride;
+      if (s->slot_bytes() > s->cfg.hbm_budget_bytes) {
+        s->min_stride = keep;
+        return set_error(VHSDECODE_HIP_E_CAPACITY, "the HBM budget does not hold one slot of the largest window");
+      }
+    } else if (min_stride < s->min_stride) {
+      return set_error(VHSDECODE_HIP_E_UNSUPPORTED, "formats with a smaller block stride must be added before serving");
+    }
+    *format_id = (int32_t)s->formats.size();
+    s->formats.push_back(std::move(f));
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return set_error(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_serve_add_stream(vhsdecode_hip_serve* s, int32_t format_id, const int16_t* samples,
+                                       int64_t length, int32_t loop, int32_t* stream_id) {
+  if (!s || !samples || !stream_id || length < kBlock)
+    return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid stream");
+  std::lock_guard<std::mutex> lock(s->mu);
+  if (format_id < 0 || format_id >= (int32_t)s->formats.size())
+    return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "unknown format");
+  *stream_id = (int32_t)s->sources.size();
+  s->sources.push_back(Source{format_id, samples, length, loop != 0});
+  return VHSDECODE_HIP_OK;
+}
+
+int32_t vhsdecode_hip_serve_submit(vhsdecode_hip_serve* s, const vhsdecode_hip_window_v1* w, uint64_t* ticket) {
+  try {
+    if (!s || !w || !ticket || !w->fields_out) return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid window");
+    if (w->struct_size != sizeof(vhsdecode_hip_window_v1))
+      return set_error(VHSDECODE_HIP_E_STRUCT_SIZE, "window struct_size mismatch");
+    auto r = std::make_shared<Request>();
+    r->w = *w;
+    r->t_submit = Clock::now();
+    r->pinned_out = is_pinned(w->fields_out);
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (w->stream_id < 0 || w->stream_id >= (int32_t)s->sources.size())
+      return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "unknown stream");
+    const Source& src = s->sources[w->stream_id];
+    if (w->sample_count < kBlock || w->sample_count > s->cfg.max_window_samples || w->first_sample < 0 ||
+        (w->first_sample & 1) || w->max_fields <= 0 || w->max_fields > s->cfg.max_fields ||
+        (!src.loop && w->first_sample + w->sample_count > src.length))
+      return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "window outside the stream or the server's limits");
+    *ticket = s->next_ticket++;
+    s->tickets[*ticket] = r;
+    s->queue.push_back(r);
+    s->cv.notify_all();
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return set_error(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_serve_wait(vhsdecode_hip_serve* s, uint64_t ticket, int32_t timeout_ms,
+                                 vhsdecode_hip_window_result_v1* res) {
+  if (!s || !res) return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "server and result are required");
+  if (res->struct_size != sizeof(vhsdecode_hip_window_result_v1))
+    return set_error(VHSDECODE_HIP_E_STRUCT_SIZE, "window_result struct_size mismatch");
+  std::unique_lock<std::mutex> lock(s->mu);
+  auto it = s->tickets.find(ticket);
+  if (it == s->tickets.end()) return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "unknown ticket");
+  const std::shared_ptr<Request> r = it->second;
+  auto ready = [&] { return r->done; };
+  if (timeout_ms < 0)
+    s->done_cv.wait(lock, ready);
+  else if (!s->done_cv.wait_for(lock, std::chrono::milliseconds(timeout_ms), ready))
+    return set_error(VHSDECODE_HIP_E_TIMEOUT, "window still in flight");
+  s->tickets.erase(ticket);
+  res->status = r->status;
+  res->field_count = r->field_count;
+  res->queued_ms = r->queued_ms;
+  res->latency_ms = r->latency_ms;
+  return VHSDECODE_HIP_OK;
+}
+
+void vhsdecode_hip_serve_destroy(vhsdecode_hip_serve* s) { delete s; }
+
+}  // extern "C"

@@ -18,6 +18,8 @@
 
 #include "vhsd_args.hpp"
 
+#include "host_common.hpp"
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -206,6 +208,8 @@ double median(std::vector<double> v) {
 }
 
 }  // namespace
+
+int32_t vhsd_host::set_error(int32_t code, const std::string& msg) { return fail(code, msg); }
 
 struct vhsdecode_hip_demod_context {
   std::mutex mu;

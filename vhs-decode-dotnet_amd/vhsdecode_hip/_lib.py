@@ -30,6 +30,12 @@ REQUIRED_EXPORTS = (
     "vhsdecode_hip_tbc_resample_device",
     "vhsdecode_hip_decode_fields_device",
     "vhsdecode_hip_decode_span_fields_device",
+    "vhsdecode_hip_serve_create",
+    "vhsdecode_hip_serve_add_format",
+    "vhsdecode_hip_serve_add_stream",
+    "vhsdecode_hip_serve_submit",
+    "vhsdecode_hip_serve_wait",
+    "vhsdecode_hip_serve_destroy",
     "vhsdecode_hip_set_profiling",
     "vhsdecode_hip_get_profile",
     "vhsdecode_hip_debug_fft16k",
@@ -114,6 +120,29 @@ class LocatorV1(ctypes.Structure):
                 ("block_length", ctypes.c_int32), ("block_cut", ctypes.c_int32), ("block_stride", ctypes.c_int32)]
 
 
+class ServeConfigV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("device_id", ctypes.c_int32), ("max_slots", ctypes.c_int32),
+                ("hbm_budget_bytes", ctypes.c_uint64), ("max_window_samples", ctypes.c_int64),
+                ("max_fields", ctypes.c_int32)]
+
+
+class ServeFormatV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("filters", ctypes.POINTER(FiltersV1)),
+                ("block_cut", ctypes.c_int32), ("block_cut_end", ctypes.c_int32),
+                ("field", ctypes.POINTER(FieldParamsV1)), ("locator", ctypes.POINTER(LocatorV1)),
+                ("sinc_table", _fp), ("sinc_count", ctypes.c_int64)]
+
+
+class WindowV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("stream_id", ctypes.c_int32), ("first_sample", ctypes.c_int64),
+                ("sample_count", ctypes.c_int64), ("max_fields", ctypes.c_int32), ("fields_out", ctypes.c_void_p)]
+
+
+class WindowResultV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("status", ctypes.c_int32), ("field_count", ctypes.c_int32),
+                ("queued_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
+
+
 class ProfileV1(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("launches", ctypes.c_int64 * len(KERNEL_NAMES)),
                 ("total_ms", ctypes.c_double * len(KERNEL_NAMES)), ("units", ctypes.c_int64 * len(KERNEL_NAMES))]
@@ -161,6 +190,17 @@ def load():
                                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                                     ctypes.POINTER(TbcParamsV1), ctypes.c_void_p, ctypes.c_void_p,
                                                     ctypes.c_void_p]
+    L.vhsdecode_hip_serve_create.argtypes = [ctypes.POINTER(ServeConfigV1), ctypes.POINTER(ctypes.c_void_p)]
+    L.vhsdecode_hip_serve_add_format.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServeFormatV1),
+                                                 ctypes.POINTER(ctypes.c_int32)]
+    L.vhsdecode_hip_serve_add_stream.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64,
+                                                 ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+    L.vhsdecode_hip_serve_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(WindowV1),
+                                             ctypes.POINTER(ctypes.c_uint64)]
+    L.vhsdecode_hip_serve_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32,
+                                           ctypes.POINTER(WindowResultV1)]
+    L.vhsdecode_hip_serve_destroy.argtypes = [ctypes.c_void_p]
+    L.vhsdecode_hip_serve_destroy.restype = None
     L.vhsdecode_hip_decode_span_fields_device.argtypes = [
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(FieldParamsV1),
         ctypes.POINTER(LocatorV1), ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,

@@ -50,18 +50,11 @@ class DecodedSpan:
         return bool((self.block_flags & 1).any())
 
 
-class HipBlockDemodulator:
-    def __init__(self, filters: TapeFilterSet, block_cut: int = BLOCK_CUT, block_cut_end: int = BLOCK_CUT_END,
-                 max_blocks: int = 64, device: int = 0, sinc_table: Optional[np.ndarray] = None):
-        L = _lib.load()
-        self.filters = filters
-        self.block_length = filters.block_length
-        self.block_cut = block_cut
-        self.block_cut_end = block_cut_end
-        self.stride = self.block_length - block_cut - block_cut_end
-        self.max_blocks = max_blocks
-        self.device = device
-        # keep FP64 tables alive for the duration of the create call
+class FilterTables:
+    """A TapeFilterSet as the C ABI's vhsdecode_hip_filters_v1 (plus its sub-deemphasis
+    block), with the arrays it points into kept alive alongside."""
+
+    def __init__(self, filters: TapeFilterSet):
         self._rfv = np.ascontiguousarray(filters.rf_video, dtype=np.complex128).view(np.float64)
         self._vid = np.ascontiguousarray(filters.video, dtype=np.complex128).view(np.float64)
         self._vlp = np.ascontiguousarray(filters.video_lowpass05, dtype=np.complex128).view(np.float64)
@@ -69,7 +62,7 @@ class HipBlockDemodulator:
         self._chr = None if filters.chroma_sos is None else np.ascontiguousarray(filters.chroma_sos, dtype=np.float64)
         f = _lib.FiltersV1()
         f.struct_size = ctypes.sizeof(_lib.FiltersV1)
-        f.block_length = self.block_length
+        f.block_length = int(filters.block_length)
         f.sample_rate_hz = float(filters.sample_rate_hz)
         f.rf_video = _ptr(self._rfv, _lib._dp)
         f.video = _ptr(self._vid, _lib._dp)
@@ -94,6 +87,23 @@ class HipBlockDemodulator:
              d.static_factor) = (float(v) for v in sd.params())
             self._sd = d
             f.sub_deemphasis = ctypes.pointer(d)
+        self.struct = f
+
+
+class HipBlockDemodulator:
+    def __init__(self, filters: TapeFilterSet, block_cut: int = BLOCK_CUT, block_cut_end: int = BLOCK_CUT_END,
+                 max_blocks: int = 64, device: int = 0, sinc_table: Optional[np.ndarray] = None):
+        L = _lib.load()
+        self.filters = filters
+        self.block_length = filters.block_length
+        self.block_cut = block_cut
+        self.block_cut_end = block_cut_end
+        self.stride = self.block_length - block_cut - block_cut_end
+        self.max_blocks = max_blocks
+        self.device = device
+        # keep FP64 tables alive for the duration of the create call
+        self._tables = FilterTables(filters)
+        f = self._tables.struct
         self.bias = float(filters.plane_bias_hz)
         cfg = _lib.DemodConfigV1()
         cfg.struct_size = ctypes.sizeof(_lib.DemodConfigV1)
@@ -137,7 +147,7 @@ class HipBlockDemodulator:
             nblocks = self.blocks_for(rf.size, first_sample)
         plane = nblocks * self.stride
         out = {k: np.empty(plane, dtype=np.float32) for k in ("video", "vlp", "env")}
-        ch = np.empty(plane, dtype=np.float32) if (chroma and self._chr is not None) else None
+        ch = np.empty(plane, dtype=np.float32) if (chroma and self.filters.chroma_sos is not None) else None
         flags = np.zeros(nblocks, dtype=np.uint32)
         span = _lib.SpanV1()
         span.struct_size = ctypes.sizeof(_lib.SpanV1)

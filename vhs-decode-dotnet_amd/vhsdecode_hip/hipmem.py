@@ -36,6 +36,8 @@ def hip():
         _hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
         _hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
         _hip.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
+        _hip.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+        _hip.hipHostFree.argtypes = [ctypes.c_void_p]
     return _hip
 
 
@@ -57,6 +59,32 @@ def synchronize():
 def copy_device(dst: int, src: int, nbytes: int):
     """Device-to-device copy (the runtime's blit kernel); synchronous."""
     _check(hip().hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), int(nbytes), 3))
+
+
+class PinnedArray:
+    """Page-locked host memory (hipHostMalloc) viewed as a numpy array; DMA reads and
+    writes it directly."""
+
+    def __init__(self, count: int, dtype=np.int16):
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(count) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(hip().hipHostMalloc(ctypes.byref(p), max(self.nbytes, 1), 0))
+        self.ptr = p.value
+        buf = (ctypes.c_char * max(self.nbytes, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=self.dtype, count=int(count))
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            self.array = None
+            _check(hip().hipHostFree(ctypes.c_void_p(self.ptr)))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class DeviceBuffer:
