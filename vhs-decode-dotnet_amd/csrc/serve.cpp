@@ -97,7 +97,9 @@ struct Slot {
 struct vhsdecode_hip_serve {
   vhsdecode_hip_serve_config_v1 cfg{};
   int min_stride = kBlock;  // smallest block stride of the registered formats
+  size_t used_bytes = 0;    // device bytes of the slots and their contexts
   size_t slot_bytes() const;
+  size_t context_bytes() const;
   std::mutex mu;
   std::condition_variable cv;       // worker wake-ups
   std::condition_variable done_cv;  // waiters
@@ -128,14 +130,19 @@ namespace {
 
 }  // namespace
 
-// Device bytes of one slot: RF, the four planes, and the demod context's per-block
-// scratch (spectrum stash, repair scratch) and per-sample pulse edges.
+// Device bytes of one slot: RF, the four planes and the field output.
 size_t vhsdecode_hip_serve::slot_bytes() const {
   const int64_t blocks = (cfg.max_window_samples - kBlock) / min_stride + 1;
   const int64_t span = blocks * min_stride;
-  return (size_t)cfg.max_window_samples * 2 + (size_t)span * 16 +
-         (size_t)blocks * (8 * (2 * 16384 + 64) + 8 * 32768 + 4 * 65536) + (size_t)span * 10 +
-         (size_t)cfg.max_fields * 1135 * 313 * 2;
+  return (size_t)cfg.max_window_samples * 2 + (size_t)span * 16 + (size_t)cfg.max_fields * 1135 * 313 * 2;
+}
+
+// Device bytes of one demod context at the slot's size: per-block spectrum stash and
+// repair scratch, per-sample pulse edges, filter and sinc tables.
+size_t vhsdecode_hip_serve::context_bytes() const {
+  const int64_t blocks = (cfg.max_window_samples - kBlock) / min_stride + 1;
+  const int64_t span = blocks * min_stride;
+  return (size_t)blocks * (8 * (2 * 16384 + 64) + 8 * 32768 + 4 * 65536) + (size_t)span * 10 + ((size_t)8 << 20);
 }
 
 int32_t vhsdecode_hip_serve::make_slot(Slot** out) {
@@ -153,6 +160,7 @@ int32_t vhsdecode_hip_serve::make_slot(Slot** out) {
     release_slot(s.get());
     return set_error(VHSDECODE_HIP_E_OUT_OF_MEMORY, std::string("slot allocation failed: ") + hipGetErrorString(e));
   }
+  used_bytes += slot_bytes();
   *out = s.get();
   slots.push_back(std::move(s));
   return VHSDECODE_HIP_OK;
@@ -181,6 +189,12 @@ int32_t vhsdecode_hip_serve::launch(Slot* s, const std::shared_ptr<Request>& r) 
   auto it = s->ctx.find(src.format);
   vhsdecode_hip_demod_context* ctx = nullptr;
   if (it == s->ctx.end()) {
+    // keep within the budget: drop this idle slot's contexts of other formats first
+    while (!s->ctx.empty() && used_bytes + context_bytes() > cfg.hbm_budget_bytes) {
+      vhsdecode_hip_demod_destroy(s->ctx.begin()->second);
+      s->ctx.erase(s->ctx.begin());
+      used_bytes -= context_bytes();
+    }
     vhsdecode_hip_demod_config_v1 dc{};
     dc.struct_size = sizeof(dc);
     dc.device_id = cfg.device_id;
@@ -195,6 +209,7 @@ int32_t vhsdecode_hip_serve::launch(Slot* s, const std::shared_ptr<Request>& r) 
       return rc;
     }
     s->ctx[src.format] = ctx;
+    used_bytes += context_bytes();
   } else {
     ctx = it->second;
   }
@@ -282,16 +297,22 @@ void vhsdecode_hip_serve::run() {
     }
     if (finished) done_cv.notify_all();
     while (!stop && !queue.empty()) {
+      const int want = formats.empty() ? -1 : sources[queue.front()->w.stream_id].format;
       Slot* slot = nullptr;
-      for (auto& s : slots)
-        if (!s->req) {
+      Slot* any_free = nullptr;
+      for (auto& s : slots) {
+        if (s->req) continue;
+        if (!any_free) any_free = s.get();
+        if (s->ctx.count(want)) {
           slot = s.get();
           break;
         }
+      }
       if (!slot && (int)slots.size() < cfg.max_slots &&
-          (slots.size() + 1) * slot_bytes() <= cfg.hbm_budget_bytes) {
+          used_bytes + slot_bytes() + context_bytes() <= cfg.hbm_budget_bytes) {
         if (make_slot(&slot) != 0) slot = nullptr;
       }
+      if (!slot) slot = any_free;
       if (!slot) break;
       std::shared_ptr<Request> r = queue.front();
       queue.pop_front();
@@ -413,7 +434,7 @@ int32_t vhsdecode_hip_serve_add_format(vhsdecode_hip_serve* s, const vhsdecode_h
     if (s->slots.empty()) {
       const int keep = s->min_stride;
       s->min_stride = min_stride;
-      if (s->slot_bytes() > s->cfg.hbm_budget_bytes) {
+      if (s->slot_bytes() + s->context_bytes() > s->cfg.hbm_budget_bytes) {
         s->min_stride = keep;
         return set_error(VHSDECODE_HIP_E_CAPACITY, "the HBM budget does not hold one slot of the largest window");
       }
