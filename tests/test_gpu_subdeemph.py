@@ -18,10 +18,11 @@ VIDEO_RMS_HZ = 2.0
 VIDEO_MAX_HZ = 40.0
 
 
-@pytest.fixture(scope="module", params=[("PAL", "lp", 3), ("NTSC", "ep", 4)], ids=["pal_vhs_lp", "ntsc_vhs_ep"])
+@pytest.fixture(scope="module", params=[("PAL", "VHS", "lp", 3), ("NTSC", "VHS", "ep", 4), ("PAL", "SVHS", "sp", 5)],
+                ids=["pal_vhs_lp", "ntsc_vhs_ep", "pal_svhs_sp"])
 def case(request):
-    system, speed, seed = request.param
-    fmt = formats.get_format(system, "VHS", speed)
+    system, tape_format, speed, seed = request.param
+    fmt = formats.get_format(system, tape_format, speed)
     fl = filters.build_tape_filters(fmt, 40e6, N)
     assert fl.sub_deemphasis is not None
     tape = synth.generate(fmt, fields=1, seed=seed, snr_db=30.0)
@@ -39,7 +40,7 @@ def case(request):
         blk = rf[b * STRIDE:b * STRIDE + N].astype(np.float64)
         off.append(O.demod_block(blk, fl).video[CUT:CUT + STRIDE])
     fl.sub_deemphasis = sd
-    print(f"\n[{system} VHS {speed}] blocks={nb}", flush=True)
+    print(f"\n[{system} {tape_format} {speed}] blocks={nb}", flush=True)
     return dict(gpu=gpu, on=np.concatenate(on), off=np.concatenate(off))
 
 

@@ -149,6 +149,22 @@ def test_sub_deemphasis_matches_upstream_video_hash(family, system, fmt, speed):
     assert _sha_f32(r.envelope) == c["envelope"]
 
 
+@pytest.mark.parametrize("system", ["PAL", "NTSC"])
+@pytest.mark.parametrize("speed", ["sp", "lp", "ep", "vp"])
+def test_svhs_matches_upstream_channel_hashes(family, system, speed):
+    """S-VHS: custom luma filters (the stored linear sub-deemphasis response and a high
+    shelf, DecodeFilterSet.cs:1560-1700) on Video, sub-deemphasis, and all four
+    channels bit-exact."""
+    x = _lcg_u8_block().astype(np.float64)
+    fl = filters.build_tape_filters(formats.get_format(system, "SVHS", speed), 40e6, N)
+    r = O.demod_block(x, fl)
+    c = _case(family, system, "SVHS", speed)
+    assert _sha_f32(r.video) == c["demod"]
+    assert _sha_f32(r.video_lowpass) == c["demod_05"]
+    assert _sha_f32(r.chroma) == c["demod_burst"]
+    assert _sha_f32(r.envelope) == c["envelope"]
+
+
 def test_sub_deemphasis_changes_only_the_video_channel():
     """Switching sub-deemphasis off changes Video and leaves the other channels
     untouched (the correction is applied after the video inverse transform)."""

@@ -336,13 +336,56 @@ def _deemphasis(fmt: FormatParameters, fs: float, n: int, configured_strength: b
     return None
 
 
+_CUSTOM_FILTERS = None
+
+
+def _custom_filter_file(name: str, fs: float, n: int) -> Optional[np.ndarray]:
+    """DecodeFilterSetBuilder.TryLoadCustomFilter (DecodeFilterSet.cs:1658-1700): the
+    response stored for `name` at this sample rate (the reference's
+    Formats/custom/<name>-<rate>.txt, carried as data), a half spectrum mirrored to the
+    whole circle; None when no file exists for the rate or its length does not fit."""
+    global _CUSTOM_FILTERS
+    if _CUSTOM_FILTERS is None:
+        import os
+        path = os.path.join(os.path.dirname(__file__), "data", "custom_luma_filters.npz")
+        with np.load(path) as z:
+            _CUSTOM_FILTERS = {k: z[k] for k in z.files}
+    half = _CUSTOM_FILTERS.get(f"{name}-{int(fs)}")
+    if half is None:
+        return None
+    if half.size == n:
+        return half.copy()
+    if half.size != n // 2 + 1:
+        return None
+    out = np.empty(n, dtype=np.complex128)
+    out[:half.size] = half
+    i = np.arange(1, half.size - 1)
+    out[n - i] = np.conj(half[i])
+    return out
+
+
+def _apply_custom_video_filters(video: np.ndarray, fmt: FormatParameters, fs: float, n: int) -> None:
+    """ApplyCustomVideoFilters (DecodeFilterSet.cs:1560-1605): file responses and
+    high/low shelves (designed with half the sample rate as the rate argument, as the
+    reference calls Shelf) multiply the Video filter in order."""
+    for flt in fmt.rfparams.get("video_custom_luma_filters") or []:
+        kind = flt.get("type")
+        if kind == "file":
+            resp = _custom_filter_file(flt.get("filename") or "", fs, n)
+        elif kind in ("highshelf", "lowshelf"):
+            b, a = _shelf(float(flt["midfreq"]), float(flt["gain"]), kind == "highshelf", fs / 2.0, float(flt["q"]))
+            resp = _freqz_ba(b, a, n)
+        else:
+            resp = None
+        if resp is not None and resp.size == video.size:
+            video *= resp
+
+
 def build_tape_filters(fmt: FormatParameters, sample_rate_hz: float = 40e6, block_length: int = 32768) -> TapeFilterSet:
     """`DecodeFilterSetBuilder.BuildBasic` for a colour-under tape format, reduced to the
     tables the VHS block demodulator consumes (DecodeFilterSet.cs:140-256)."""
     if block_length <= 0 or block_length % 2:
         raise ValueError("Block length must be a positive even value.")
-    if fmt.rfparams.get("video_custom_luma_filters"):
-        raise NotImplementedError("custom luma filter files are not supported by this backend")
     if fmt.rfparams.get("boost_bpf_mult") or fmt.tape_format in ("BETAMAX", "BETAMAX_HIFI"):
         raise NotImplementedError("the RF high boost and Betamax fsc notch are not implemented by this backend")
     n = block_length
@@ -352,6 +395,7 @@ def build_tape_filters(fmt: FormatParameters, sample_rate_hz: float = 40e6, bloc
     vlp = _video_lowpass(fmt, nyq, n)
     deemph = _deemphasis(fmt, fs, n, configured_strength=True)
     video = vlp * deemph if deemph is not None else vlp.copy()
+    _apply_custom_video_filters(video, fmt, fs, n)
     ref_deemph = _deemphasis(fmt, fs, n, configured_strength=False)
     reference = vlp * ref_deemph if ref_deemph is not None else vlp.copy()
     if 500_000.0 >= nyq:
