@@ -18,8 +18,9 @@ VIDEO_RMS_HZ = 2.0
 VIDEO_MAX_HZ = 40.0
 
 
-@pytest.fixture(scope="module", params=[("PAL", "VHS", "lp", 3), ("NTSC", "VHS", "ep", 4), ("PAL", "SVHS", "sp", 5)],
-                ids=["pal_vhs_lp", "ntsc_vhs_ep", "pal_svhs_sp"])
+@pytest.fixture(scope="module", params=[("PAL", "VHS", "lp", 3), ("NTSC", "VHS", "ep", 4), ("PAL", "SVHS", "sp", 5),
+                                        ("NTSC", "VIDEO8", "sp", 6)],
+                ids=["pal_vhs_lp", "ntsc_vhs_ep", "pal_svhs_sp", "ntsc_video8_sp"])
 def case(request):
     system, tape_format, speed, seed = request.param
     fmt = formats.get_format(system, tape_format, speed)
@@ -30,10 +31,12 @@ def case(request):
     nb = 12
     with HipBlockDemodulator(fl, max_blocks=nb) as dem:
         gpu = dem.demod_span_host(rf, 0, nb)
-    on, off = [], []
+    on, off, chroma = [], [], []
     for b in range(nb):
         blk = rf[b * STRIDE:b * STRIDE + N].astype(np.float64)
-        on.append(O.demod_block(blk, fl).video[CUT:CUT + STRIDE])
+        r = O.demod_block(blk, fl)
+        on.append(r.video[CUT:CUT + STRIDE])
+        chroma.append(r.chroma[CUT:CUT + STRIDE])
     sd = fl.sub_deemphasis
     fl.sub_deemphasis = None
     for b in range(nb):
@@ -41,7 +44,8 @@ def case(request):
         off.append(O.demod_block(blk, fl).video[CUT:CUT + STRIDE])
     fl.sub_deemphasis = sd
     print(f"\n[{system} {tape_format} {speed}] blocks={nb}", flush=True)
-    return dict(gpu=gpu, on=np.concatenate(on), off=np.concatenate(off))
+    return dict(gpu=gpu, on=np.concatenate(on), off=np.concatenate(off), chroma=np.concatenate(chroma),
+                sections=fl.chroma_sos.shape[0])
 
 
 def test_video_plane_with_sub_deemphasis(case):
@@ -61,3 +65,13 @@ def test_sub_deemphasis_is_applied(case):
     print(f"sub-deemphasis correction RMS {rms_corr:.1f} Hz")
     assert rms_corr > 10 * VIDEO_RMS_HZ
     assert float(np.sqrt(np.mean(err_off ** 2))) > 5 * VIDEO_RMS_HZ
+
+
+def test_chroma_plane(case):
+    """Chroma band-passes of 4 sections and of Video8's 10 sections (scan filter with
+    20-state affine maps)."""
+    ref = case["chroma"].astype(np.float64)
+    err = np.abs(case["gpu"].chroma.astype(np.float64) - ref)
+    rel = float(err.max() / np.abs(ref).max())
+    print(f"chroma ({case['sections']} sections) max rel {rel:.3e}")
+    assert rel <= 2e-3

@@ -597,6 +597,7 @@ hipError_t vhsd_launch_block_chroma(const vhsd::ChromaArgs* a, int nblocks, hipS
     case 2: return launch_chroma_s<2>(a, nblocks, stream);
     case 3: return launch_chroma_s<3>(a, nblocks, stream);
     case 4: return launch_chroma_s<4>(a, nblocks, stream);
+    case 10: return launch_chroma_s<10>(a, nblocks, stream);
     default: return hipErrorInvalidValue;
   }
 }

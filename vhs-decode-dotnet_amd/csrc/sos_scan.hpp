@@ -25,7 +25,7 @@
 
 namespace vhsd {
 
-constexpr int kMaxSections = 4;
+constexpr int kMaxSections = 10;  // chroma band-passes up to order 10 (Video8)
 constexpr int kScanThreads = 1024;
 constexpr int kScanLevels = 10;  // log2(kScanThreads)
 constexpr int kChunk = 33;       // supports blocks of up to 1024*33 - 2*pad samples
@@ -111,12 +111,8 @@ __device__ __forceinline__ void run_direction(const SosScanArgs& p, float (&val)
                                               float* ex, const float* trans) {
   const bool lead = threadIdx.x == 0;
   float z[2 * S];
-  float init[2 * S];
 #pragma unroll
-  for (int c = 0; c < 2 * S; c++) {
-    init[c] = p.zi[c] * first_value;
-    z[c] = lead ? init[c] : 0.0f;
-  }
+  for (int c = 0; c < 2 * S; c++) z[c] = lead ? p.zi[c] * first_value : 0.0f;
 #pragma unroll
   for (int t = 0; t < kChunk; t++) {
     float v = val[t];
@@ -125,7 +121,7 @@ __device__ __forceinline__ void run_direction(const SosScanArgs& p, float (&val)
   scan_states<S>(z, ex, trans);
   if (lead) {
 #pragma unroll
-    for (int c = 0; c < 2 * S; c++) z[c] = init[c];
+    for (int c = 0; c < 2 * S; c++) z[c] = p.zi[c] * first_value;  // recomputed: not kept live across the scan
   }
 #pragma unroll
   for (int t = 0; t < kChunk; t++) {

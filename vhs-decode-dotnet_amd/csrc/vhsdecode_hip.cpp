@@ -89,8 +89,8 @@ void steady_state_f32(const float (*c)[6], int sections, float* zi) {
 // Build SosScanArgs for an N-sample block: FP32 coefficients, zi, and the
 // per-level state-transition matrices A^(C 2^d) (FP64 products, stored FP32).
 int32_t prepare_sos(const double* sos, int sections, int n, vhsd::SosScanArgs* a, std::vector<float>* trans) {
-  if (sections < 1 || sections > vhsd::kMaxSections)
-    return fail(VHSDECODE_HIP_E_UNSUPPORTED, "SOS filters of 1 to 4 sections are supported");
+  if (sections < 1 || (sections > 4 && sections != vhsd::kMaxSections))
+    return fail(VHSDECODE_HIP_E_UNSUPPORTED, "SOS filters of 1 to 4 or 10 sections are supported");
   std::memset(a, 0, sizeof(*a));
   a->sections = sections;
   a->pad = default_pad_length(sos, sections);
