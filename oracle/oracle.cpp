@@ -353,6 +353,78 @@ void sub_deemphasis(double* video, const cd* vs, const cd* hp, const double* amp
   }
 }
 
+// numpy's pairwise float32 sum of FP64 values cast to float (NumpyReduction.PairwiseSumFloat32,
+// Dsp/NumpyReduction.cs:253-339): blocks of at most 128 summed with eight accumulators,
+// longer runs split at half (rounded down to a multiple of 8) and added.
+float pairwise_sum_f32(const double* v, int n) {
+  if (n < 8) {
+    float s = -0.0f;
+    for (int i = 0; i < n; i++) s += (float)v[i];
+    return s;
+  }
+  if (n > 128) {
+    int split = n / 2;
+    split -= split % 8;
+    return pairwise_sum_f32(v, split) + pairwise_sum_f32(v + split, n - split);
+  }
+  float r[8];
+  for (int k = 0; k < 8; k++) r[k] = (float)v[k];
+  int i = 8;
+  const int end = n - (n % 8);
+  for (; i < end; i += 8)
+    for (int k = 0; k < 8; k++) r[k] += (float)v[i + k];
+  float s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  for (; i < n; i++) s += (float)v[i];
+  return s;
+}
+
+// IirFilter.ApplyForwardBackwardInPlace for a transfer function b/a with a[0] == 1
+// (Dsp/IirFilter.cs:78-115, 361-373, 375-420, 476-498): odd extension of 3 * max(len)
+// samples, closed-form steady state, direct-form-II-transposed recurrence in the
+// reference's operation order.
+void tf_filtfilt(const double* b, const double* a, int order, double* x, int n) {
+  const int m = order + 1;
+  int edge = 3 * m;
+  edge = std::min(edge, n - 1);
+  double nsum = -0.0, dsum = -0.0;
+  for (int i = 0; i < m; i++) {
+    nsum += b[i];
+    dsum += a[i];
+  }
+  const double steady = nsum / dsum;
+  std::vector<double> zi(order);
+  double cum = 0.0;
+  for (int i = order; i >= 1; i--) {
+    cum += b[i] - (steady * a[i]);
+    zi[i - 1] = cum;
+  }
+  const int L = n + 2 * edge;
+  std::vector<double> e(L);
+  if (edge > 0) {
+    for (int i = 0; i < edge; i++) e[i] = (2.0 * x[0]) - x[edge - i];
+    for (int i = 0; i < n; i++) e[edge + i] = x[i];
+    for (int i = 0; i < edge; i++) e[edge + n + i] = (2.0 * x[n - 1]) - x[n - 2 - i];
+  } else {
+    for (int i = 0; i < n; i++) e[i] = x[i];
+  }
+  auto pass = [&]() {
+    std::vector<double> s(order);
+    for (int i = 0; i < order; i++) s[i] = zi[i] * e[0];
+    for (int t = 0; t < L; t++) {
+      const double xv = e[t];
+      const double y = (b[0] * xv) + (order > 0 ? s[0] : 0.0);
+      for (int i = 1; i < order; i++) s[i - 1] = ((b[i] * xv) + s[i]) - (a[i] * y);
+      if (order > 0) s[order - 1] = (b[order] * xv) - (a[order] * y);
+      e[t] = y;
+    }
+  };
+  pass();
+  std::reverse(e.begin(), e.end());
+  pass();
+  std::reverse(e.begin(), e.end());
+  for (int i = 0; i < n; i++) x[i] = e[edge + i];
+}
+
 // numpy.roll semantics (FrequencyDomainFilter.RollInPlaceCore, FrequencyDomainFilter.cs:145-161).
 template <typename T>
 void roll(T* v, int n, int shift) {
@@ -417,7 +489,7 @@ int oracle_demod_block(const double* input, int n, double fs, const double* rf_v
                        int chroma_sections, int chroma_offset, double diff_demod_max,
                        double* video_out, double* vlp_out, double* env_out, float* chroma_out,
                        double* demod_out, const double* sd_high_pass, const double* sd_amp_sos,
-                       const double* sd_params) {
+                       const double* sd_params, const double* boost, const double* notch) {
   const cd* rf_video = reinterpret_cast<const cd*>(rf_video_c);
   const cd* video_f = reinterpret_cast<const cd*>(video_c);
   const cd* vlp05 = reinterpret_cast<const cd*>(vlp05_c);
@@ -440,6 +512,22 @@ int oracle_demod_block(const double* input, int n, double fs, const double* rf_v
   }
   fft(analytic.data(), n, true);
 
+  // RF high boost (boost = {multiplier, RfTop SOS[6]}) takes the complex path
+  // (RfDemodulator.cs:356-392): the envelope source is Re(analytic) and the filtered
+  // real RF is Re(ifft(full filtered spectrum)).
+  const bool boosting = boost && boost[0] != 0.0;
+  std::vector<cd> rfs;
+  if (boosting) {
+    rfs.assign(n, cd(0.0, 0.0));
+    for (int i = 0; i < n; i++) rfs[i] = cd(input[i], 0.0);
+    fft(rfs.data(), n, false);
+    for (int k = 0; k < n; k++) rfs[k] = cmul_fma(rfs[k], rf_video[k]);
+    std::vector<cd> t(rfs);
+    fft(t.data(), n, true);
+    for (int i = 0; i < n; i++) rf_real[i] = analytic[i].real();
+    for (int i = 0; i < n; i++) analytic[i] = cd(t[i].real(), 0.0);  // keep Re(ifft(rfs)) for the boost
+  }
+
   // envelope: |float(x)| rolled by +4, FP32 sosfiltfilt (RfDemodulator.cs:1755-1839)
   std::vector<double> raw(n);
   int shift = 4 % n;
@@ -452,6 +540,30 @@ int oracle_demod_block(const double* input, int n, double fs, const double* rf_v
   for (int i = 0; i < n; i++) {
     env_out[i] = (double)env[i];
     if (env[i] == 0.0f) weak = 1;
+  }
+
+  if (boosting) {
+    // TryBuildVhsRfHighPart + ApplyVhsComplexRfHighBoostIfPresent (RfDemodulator.cs:2374-2449)
+    std::vector<double> filtered_real(n);
+    for (int i = 0; i < n; i++) filtered_real[i] = analytic[i].real();
+    if (!weak) {
+      const float mean = pairwise_sum_f32(env_out, n) / (float)n;
+      const float numer = mean * 0.9f;
+      std::vector<double> high(n);
+      sosfiltfilt_f64(boost + 1, 1, filtered_real.data(), n, default_pad_length(boost + 1, 1), high.data());
+      std::vector<cd> hs(n);
+      for (int i = 0; i < n; i++) {
+        const float sc = numer / (float)env_out[i];
+        hs[i] = cd((high[i] * (double)sc) * boost[0], 0.0);
+      }
+      fft(hs.data(), n, false);
+      for (int k = 0; k < n; k++) rfs[k] = cd(rfs[k].real() + hs[k].real(), rfs[k].imag() + hs[k].imag());
+    }
+    for (int k = 0; k < n; k++) {
+      double h = (k == 0 || k == n / 2) ? 1.0 : (k < n / 2 ? 2.0 : 0.0);
+      analytic[k] = cd(rfs[k].real() * h, rfs[k].imag() * h);
+    }
+    fft(analytic.data(), n, true);
   }
 
   std::vector<double> demod(n);
@@ -480,6 +592,9 @@ int oracle_demod_block(const double* input, int n, double fs, const double* rf_v
   irfft(vs.data(), n, video_out);
   if (sd_high_pass)
     sub_deemphasis(video_out, vs.data(), reinterpret_cast<const cd*>(sd_high_pass), sd_amp_sos, sd_params, n);
+  // Betamax fsc notch (RfDemodulator.ApplyBetamaxFscNotchInPlace, RfDemodulator.cs:1018-1033):
+  // notch = {b0, b1, b2, a0, a1, a2}
+  if (notch) tf_filtfilt(notch, notch + 3, 2, video_out, n);
   for (int k = 0; k < half; k++) vs[k] = cmul_fma(dspec[k], vlp05[k]);
   irfft(vs.data(), n, vlp_out);
   if (vlp_offset != 0) roll(vlp_out, n, -vlp_offset);

@@ -42,7 +42,7 @@ def lib():
         L.oracle_sosfiltfilt_f32.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _fp]
         L.oracle_demod_block.argtypes = [_dp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, ctypes.c_int,
                                          _dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                         _dp, _dp, _dp, _fp, _dp, _dp, _dp, _dp]
+                                         _dp, _dp, _dp, _fp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.oracle_demod_block.restype = ctypes.c_int
         L.oracle_find_pulses.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                          _ip, _ip, ctypes.c_int]
@@ -126,6 +126,12 @@ def demod_block(x: np.ndarray, filters, fs: Optional[float] = None) -> DemodResu
         sd_args = (_p(sd_hp.view(np.float64), _dp), _p(sd_sos, _dp), _p(sd_par, _dp))
     else:
         sd_args = (None, None, None)
+    bo = getattr(filters, "rf_high_boost", None)
+    boost = None if bo is None else np.ascontiguousarray(np.concatenate([[bo.multiplier], np.ravel(bo.rf_top_sos)]),
+                                                         dtype=np.float64)
+    nt = getattr(filters, "fsc_notch", None)
+    notch = None if nt is None else np.ascontiguousarray(np.concatenate([nt[0], nt[1]]), dtype=np.float64)
+    sd_args = sd_args + (_p(boost, _dp) if boost is not None else None, _p(notch, _dp) if notch is not None else None)
     weak = lib().oracle_demod_block(
         _p(xi, _dp), n, fs, _p(rv.view(np.float64), _dp), _p(vf.view(np.float64), _dp),
         _p(lf.view(np.float64), _dp), int(filters.video_lowpass05_offset), _p(env, _dp), env.shape[0],

@@ -165,6 +165,37 @@ def test_svhs_matches_upstream_channel_hashes(family, system, speed):
     assert _sha_f32(r.envelope) == c["envelope"]
 
 
+@pytest.mark.parametrize("system", ["PAL", "NTSC"])
+@pytest.mark.parametrize("speed", ["sp", "lp", "ep", "vp"])
+def test_betamax_matches_upstream_hashes(family, system, speed):
+    """Betamax: RF high boost on the complex path (RfDemodulator.cs:356-392, 2374-2449,
+    with numpy's pairwise float32 envelope mean), band-pass sub-deemphasis and the fsc
+    notch (RfDemodulator.cs:1018-1051): Video, VideoLowPass and Envelope bit-exact.  The
+    PAL chroma band-pass pairs its sections differently (the reference ties them with
+    the platform's csinh rounding), so only NTSC chroma is asserted."""
+    x = _lcg_u8_block().astype(np.float64)
+    fl = filters.build_tape_filters(formats.get_format(system, "BETAMAX", speed), 40e6, N)
+    assert fl.rf_high_boost is not None and fl.fsc_notch is not None and fl.sub_deemphasis is not None
+    r = O.demod_block(x, fl)
+    c = _case(family, system, "BETAMAX", speed)
+    assert _sha_f32(r.video) == c["demod"]
+    assert _sha_f32(r.video_lowpass) == c["demod_05"]
+    assert _sha_f32(r.envelope) == c["envelope"]
+    if system == "NTSC":
+        assert _sha_f32(r.chroma) == c["demod_burst"]
+
+
+def test_rf_high_boost_changes_the_demod():
+    """The boost moves the demod (so Video and VideoLowPass) and leaves the envelope."""
+    x = _lcg_u8_block().astype(np.float64)
+    fl = filters.build_tape_filters(formats.get_format("PAL", "BETAMAX", "sp"), 40e6, N)
+    on = O.demod_block(x, fl)
+    fl.rf_high_boost = None
+    off = O.demod_block(x, fl)
+    assert np.max(np.abs(on.video_lowpass - off.video_lowpass)) > 1.0
+    np.testing.assert_array_equal(on.envelope, off.envelope)
+
+
 def test_sub_deemphasis_changes_only_the_video_channel():
     """Switching sub-deemphasis off changes Video and leaves the other channels
     untouched (the correction is applied after the video inverse transform)."""

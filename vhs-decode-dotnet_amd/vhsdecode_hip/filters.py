@@ -46,6 +46,19 @@ class TapeFilterSet:
     diff_demod_max: Optional[float]
     plane_bias_hz: float = 0.0    # mid-level carrier; FP32 planes are stored as offsets from it
     sub_deemphasis: Optional["SubDeemphasis"] = None
+    rf_high_boost: Optional["RfHighBoost"] = None
+    fsc_notch: Optional[tuple] = None   # (b[3], a[3]) of the Betamax fsc notch on Video
+
+
+@dataclass
+class RfHighBoost:
+    """RfHighBoostOptions (DecodeFilterSet.cs:90; DecodeSession.cs BuildRfHighBoostOptions)
+    with its band-pass (BuildVhsRfTopSos, DecodeFilterSet.cs:622-641): the RF top band,
+    scaled by 0.9 * mean(envelope) / envelope and `multiplier`, is added to the filtered
+    RF before demodulation (RfDemodulator.cs:2374-2449)."""
+
+    multiplier: float
+    rf_top_sos: np.ndarray        # float64[1, 6]
 
 
 @dataclass
@@ -386,8 +399,6 @@ def build_tape_filters(fmt: FormatParameters, sample_rate_hz: float = 40e6, bloc
     tables the VHS block demodulator consumes (DecodeFilterSet.cs:140-256)."""
     if block_length <= 0 or block_length % 2:
         raise ValueError("Block length must be a positive even value.")
-    if fmt.rfparams.get("boost_bpf_mult") or fmt.tape_format in ("BETAMAX", "BETAMAX_HIFI"):
-        raise NotImplementedError("the RF high boost and Betamax fsc notch are not implemented by this backend")
     n = block_length
     fs = float(sample_rate_hz)
     nyq = fs / 2.0
@@ -417,8 +428,28 @@ def build_tape_filters(fmt: FormatParameters, sample_rate_hz: float = 40e6, bloc
     diff_max = (float(sp["ire0"]) + float(sp["hz_ire"]) * 100.0) * 2.0
     # the 50 IRE carrier, rounded to a whole Hz so it is exact in FP32
     bias = float(np.round(float(sp["ire0"]) + 50.0 * float(sp["hz_ire"])))
+    boost = None
+    if rf.get("boost_bpf_mult") is not None and rf.get("boost_bpf_low") is not None \
+            and rf.get("boost_bpf_high") is not None and float(rf["boost_bpf_mult"]) != 0.0:
+        lo, hi = float(rf["boost_bpf_low"]), float(rf["boost_bpf_high"])
+        if lo <= 0.0 or hi <= lo or hi >= nyq:
+            raise ValueError("VHS RF boost band-pass frequencies must be inside the Nyquist range.")
+        boost = RfHighBoost(float(rf["boost_bpf_mult"]), _bandpass_sos(1, lo / nyq, hi / nyq))
+    notch = None
+    if fmt.tape_format in ("BETAMAX", "BETAMAX_HIFI"):
+        notch = _notch(float(sp["fsc_mhz"]) * 1e6 / nyq, 2.0)
     return TapeFilterSet(n, fs, rf_video, video, vlp, vlp05, env, chroma, chroma_offset,
-                         VIDEO_LOWPASS05_OFFSET, diff_max, bias, _sub_deemphasis(fmt, fs, n))
+                         VIDEO_LOWPASS05_OFFSET, diff_max, bias, _sub_deemphasis(fmt, fs, n), boost, notch)
+
+
+def _notch(fnorm: float, q: float):
+    """IirFilterDesign.Notch (IirFilterDesign.cs:836-852)."""
+    bandwidth = (fnorm / q) * math.pi
+    omega = fnorm * math.pi
+    beta = math.tan(bandwidth / 2.0)
+    gain = 1.0 / (1.0 + beta)
+    cos = math.cos(omega)
+    return (np.array([gain, (-2.0 * cos) * gain, gain]), np.array([1.0, (-2.0 * gain) * cos, (2.0 * gain) - 1.0]))
 
 
 def _sub_deemphasis(fmt: FormatParameters, fs: float, n: int) -> Optional[SubDeemphasis]:
@@ -441,7 +472,11 @@ def _sub_deemphasis(fmt: FormatParameters, fs: float, n: int) -> Optional[SubDee
     if upper is None:
         high_pass = _freqz_sos(_butter_highpass_sos(order, hpf / nyq), n)
     else:
-        raise NotImplementedError("band-pass sub-deemphasis (Betamax) is not implemented by this backend")
+        # IirFilterDesign.ButterworthBandPass transfer function (IirFilterDesign.cs:472-487)
+        if not hpf < upper < nyq:
+            raise ValueError("Nonlinear band-pass upper frequency must be above the high-pass frequency and below Nyquist.")
+        b, a = signal.butter(order, [hpf / nyq, upper / nyq], btype="bandpass", output="ba")
+        high_pass = _freqz_ba(b, a, n)
     amp_lpf = opt("nonlinear_amp_lpf_freq", 700_000.0)
     deviation = sp.get("nonlinear_deviation")
     deviation = float(deviation) if deviation is not None else float(sp["hz_ire"]) * (100.0 + -float(sp["vsync_ire"]))
