@@ -117,6 +117,12 @@ typedef struct vhsdecode_hip_filters_v1 {
                                       FP32 plane at the carrier (~4 MHz) would keep only 1/4-1/2 Hz, offsets
                                       near a mid-level carrier keep 1/16 Hz.  0 writes absolute Hz. */
   const vhsdecode_hip_subdeemph_v1* sub_deemphasis;  /* NULL: the format does not use it */
+  const double* rf_boost_sos;      /* RF high boost band-pass [1][6] (BuildVhsRfTopSos,
+                                      Dsp/DecodeFilterSet.cs:622-641); NULL: no boost */
+  double rf_boost_multiplier;      /* RfHighBoostOptions.Multiplier */
+  const double* video_notch;       /* Betamax fsc notch transfer function b0 b1 b2 a0 a1 a2
+                                      (RfDemodulator.cs:1018-1051); NULL: none.  Needs
+                                      sub_deemphasis, which Betamax formats enable */
 } vhsdecode_hip_filters_v1;
 
 typedef struct vhsdecode_hip_demod_config_v1 {

@@ -19,8 +19,9 @@ VIDEO_MAX_HZ = 40.0
 
 
 @pytest.fixture(scope="module", params=[("PAL", "VHS", "lp", 3), ("NTSC", "VHS", "ep", 4), ("PAL", "SVHS", "sp", 5),
-                                        ("NTSC", "VIDEO8", "sp", 6)],
-                ids=["pal_vhs_lp", "ntsc_vhs_ep", "pal_svhs_sp", "ntsc_video8_sp"])
+                                        ("NTSC", "VIDEO8", "sp", 6), ("PAL", "BETAMAX", "sp", 7),
+                                        ("NTSC", "BETAMAX", "lp", 8)],
+                ids=["pal_vhs_lp", "ntsc_vhs_ep", "pal_svhs_sp", "ntsc_video8_sp", "pal_betamax_sp", "ntsc_betamax_lp"])
 def case(request):
     system, tape_format, speed, seed = request.param
     fmt = formats.get_format(system, tape_format, speed)
@@ -31,12 +32,13 @@ def case(request):
     nb = 12
     with HipBlockDemodulator(fl, max_blocks=nb) as dem:
         gpu = dem.demod_span_host(rf, 0, nb)
-    on, off, chroma = [], [], []
+    on, off, chroma, vlp = [], [], [], []
     for b in range(nb):
         blk = rf[b * STRIDE:b * STRIDE + N].astype(np.float64)
         r = O.demod_block(blk, fl)
         on.append(r.video[CUT:CUT + STRIDE])
         chroma.append(r.chroma[CUT:CUT + STRIDE])
+        vlp.append(r.video_lowpass[CUT:CUT + STRIDE])
     sd = fl.sub_deemphasis
     fl.sub_deemphasis = None
     for b in range(nb):
@@ -45,7 +47,7 @@ def case(request):
     fl.sub_deemphasis = sd
     print(f"\n[{system} {tape_format} {speed}] blocks={nb}", flush=True)
     return dict(gpu=gpu, on=np.concatenate(on), off=np.concatenate(off), chroma=np.concatenate(chroma),
-                sections=fl.chroma_sos.shape[0])
+                vlp=np.concatenate(vlp), sections=fl.chroma_sos.shape[0], boost=fl.rf_high_boost is not None)
 
 
 def test_video_plane_with_sub_deemphasis(case):
@@ -75,3 +77,10 @@ def test_chroma_plane(case):
     rel = float(err.max() / np.abs(ref).max())
     print(f"chroma ({case['sections']} sections) max rel {rel:.3e}")
     assert rel <= 2e-3
+
+
+def test_sync_reference_plane(case):
+    """The sync reference follows the demod (RF-boosted for Betamax)."""
+    err = np.abs(case["gpu"].video_lowpass_hz - case["vlp"])
+    print(f"sync reference max {err.max():.3f} Hz (boost {case['boost']})")
+    assert err.max() <= 20.0

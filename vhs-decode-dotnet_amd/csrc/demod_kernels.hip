@@ -33,6 +33,7 @@ struct BlockLds {
   c32 buf[kFftBufEntries];
   float2 oct[kOctantEntries];
   float scratch[kScratchFloats];
+  float tree[256];  // pairwise-sum levels
 };
 
 // Strided element k = j + 1024 m of a global array through a uniform row pointer, so
@@ -301,6 +302,156 @@ __global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) {
   if (j == 0 && weak) atomicOr(&a.flags[blk], 1u);
 }
 
+// numpy's pairwise float32 sum of the 32768 values at y (NumpyReduction.PairwiseSumFloat32,
+// Dsp/NumpyReduction.cs:253-339) in its exact order: 256 runs of 128 summed with eight
+// accumulators each (accumulator k adds elements k, k+8, ... in order; then
+// ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7))), and the run sums added pairwise up a binary
+// tree.  Every thread returns the total.
+__device__ float pairwise_sum_32768(const float* y, float* tree) {
+  const int t = threadIdx.x;
+  float run[2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int c = t + h * kFftThreads;  // chain: run c >> 3, accumulator c & 7
+    const float* base = y + ((c >> 3) << 7) + (c & 7);
+    float r = base[0];
+#pragma unroll
+    for (int i = 1; i < 16; i++) r += base[8 * i];
+    // accumulators of a run sit in 8 consecutive lanes
+    r = r + __shfl_xor(r, 1);
+    r = r + __shfl_xor(r, 2);
+    r = r + __shfl_xor(r, 4);
+    run[h] = r;
+  }
+  if ((t & 7) == 0) {
+    tree[t >> 3] = run[0];
+    tree[128 + (t >> 3)] = run[1];
+  }
+  __syncthreads();
+  for (int width = 128; width >= 1; width >>= 1) {
+    float v = 0.0f;
+    if (t < width) v = tree[2 * t] + tree[2 * t + 1];
+    __syncthreads();
+    if (t < width) tree[t] = v;
+    __syncthreads();
+  }
+  const float total = tree[0];
+  __syncthreads();
+  return total;
+}
+
+// Block demod with the RF high boost (Betamax; RfDemodulator.cs:356-392, 635-663,
+// 2374-2449): rfft -> x RfVideo (kept in aux) -> filtered RF by irfft -> envelope ->
+// unless weak, high = FP32 forward-backward RF top band-pass of the filtered RF, scaled
+// by 0.9 mean(envelope) / envelope and the multiplier, whose spectrum is added to the
+// filtered spectrum -> analytic signal and discriminator as in the plain kernel.
+__global__ void __launch_bounds__(1024) vhs_block_demod_boost_kernel(DemodArgs a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
+  const int j = threadIdx.x;
+  const int blk = blockIdx.x;
+  const long long start = a.block_start[blk];
+  const long long out_base = a.out_offset[blk];
+  float2* stash = a.stash + (long long)blk * kStashEntries;
+  float2* aux = a.aux + (long long)blk * kAuxEntries;
+  load_octant(s, a.oct);
+  load_trans<1>(a.env, s.scratch + kEnvTransOffset);
+  load_trans<1>(a.boost, s.scratch + kAuxTransOffset);
+  c32 v[16];
+  const int16_t* x = a.rf + start;
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int16_t* xr = x + 2 * m * kFftThreads;
+    v[m] = c32{(float)xr[2 * j], (float)xr[2 * j + 1]};
+  }
+  __syncthreads();
+  fft16k_forward(v, s.buf, s.oct);
+  c32 xm;
+  rfft_post(v, s, xm);
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const float2 f = row_at(a.rf_video, m);
+    v[m] = mul(v[m], c32{f.x, f.y});
+    row_at(aux, m) = make_float2(v[m].x, v[m].y);
+  }
+  const float2 fM = a.rf_video[kFftM];
+  const c32 fm = mul(xm, c32{fM.x, fM.y});  // every thread computed xm from the same LDS words
+  if (j == 0) aux[kFftM] = make_float2(fm.x, fm.y);
+  // filtered real RF (the envelope source and the boost input), as sample pairs
+  const float inv_n = 1.0f / 32768.0f;
+  irfft_pre(v, fm, s);
+  fft16k_inverse(v, s.buf, s.oct);
+  float2* st_x = stash + kFftM;
+#pragma unroll
+  for (int m = 0; m < 16; m++) row_at(st_x, m) = make_float2(v[m].x * inv_n, v[m].y * inv_n);
+  // envelope: raw[(n + 4) mod N] = |float(x[n])|
+  float* ext = reinterpret_cast<float*>(s.buf) + kSosMargin;
+  {
+    float* raw = ext + a.env.pad;
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const int n0 = 2 * (j + m * kFftThreads);
+      raw[(n0 + 4) & 32767] = fabsf(v[m].x * inv_n);
+      raw[(n0 + 5) & 32767] = fabsf(v[m].y * inv_n);
+    }
+  }
+  __syncthreads();
+  const bool weak = sos_filtfilt_block_t<1>(a.env, ext, 32768, s.scratch, s.scratch + kEnvTransOffset,
+                                            a.out_env + out_base, a.out_begin, a.out_count, /*roll=*/0, false);
+  c32 fb;
+  if (!weak) {  // uniform: every thread holds the same flag
+    const float* env = ext + a.env.pad;
+    const float mean = pairwise_sum_32768(env, s.tree) / 32768.0f;
+    const float numer = mean * 0.9f;
+    // per-sample scale, parked in the stash while the RF top filter uses the LDS
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const int k = j + m * kFftThreads;
+      row_at(stash, m) = make_float2(numer / env[2 * k], numer / env[2 * k + 1]);
+    }
+    __syncthreads();
+    float* hx = ext + a.boost.pad;
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const int k = j + m * kFftThreads;
+      const float2 xv = row_at(st_x, m);
+      hx[2 * k] = xv.x;
+      hx[2 * k + 1] = xv.y;
+    }
+    __syncthreads();
+    sos_filtfilt_block_t<1>(a.boost, ext, 32768, s.scratch, s.scratch + kAuxTransOffset, nullptr, 0, 0, 0, false);
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const int k = j + m * kFftThreads;
+      const float2 sc = row_at(stash, m);
+      v[m] = c32{(hx[2 * k] * sc.x) * a.boost_mult, (hx[2 * k + 1] * sc.y) * a.boost_mult};
+    }
+    __syncthreads();
+    fft16k_forward(v, s.buf, s.oct);
+    c32 hm;
+    rfft_post(v, s, hm);
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const float2 f = row_at(aux, m);
+      v[m] = c32{f.x + v[m].x, f.y + v[m].y};
+    }
+    const float2 f0 = aux[kFftM];
+    fb = c32{f0.x + hm.x, f0.y + hm.y};
+  } else {
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const float2 f = row_at(aux, m);
+      v[m] = c32{f.x, f.y};
+    }
+    const float2 f0 = aux[kFftM];
+    fb = c32{f0.x, f0.y};
+    if (j == 0) atomicOr(&a.flags[blk], 1u);
+  }
+  const bool spike = analytic_demod(v, fb, s, a.fs, a.diff_max, stash);
+  const unsigned long long any_spike = __ballot(spike);
+  if ((j & 63) == 0 && any_spike) atomicOr(&a.flags[blk], 2u);
+}
+
 // Stage C: video and sync-reference planes from the demod the block demod (or spike
 // repair) kernel left in each block's stash.
 __global__ void __launch_bounds__(1024) vhs_block_video_kernel(DemodArgs a) {
@@ -356,8 +507,30 @@ __global__ void __launch_bounds__(1024) vhs_block_subdeemph_kernel(SubDeemphArgs
   const int j = threadIdx.x;
   const int blk = blockIdx.x;
   float2* stash = a.stash + (long long)blk * kStashEntries;
+  float2* aux = a.aux + (long long)blk * kAuxEntries;
+  const float inv_n = 1.0f / 32768.0f;
   load_octant(s, a.oct);
   load_trans<1>(a.amp, s.scratch + kEnvTransOffset);
+  if (a.notch) {
+    load_trans<1>(a.notch_sos, s.scratch + kAuxTransOffset);
+    // the whole block's video, as the video kernel formed it: irfft(D x Video) / N + corr
+    c32 w[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const float2 d = row_at(stash, m);
+      const float2 f = row_at(a.video, m);
+      w[m] = mul(c32{d.x, d.y}, c32{f.x, f.y});
+    }
+    const float2 dM0 = stash[2 * kFftM];
+    const float2 fV = a.video[kFftM];
+    const c32 vm = mul(c32{dM0.x, dM0.y}, c32{fV.x, fV.y});
+    __syncthreads();  // twiddles in LDS
+    irfft_pre(w, vm, s);
+    fft16k_inverse(w, s.buf, s.oct);
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      row_at(aux, m) = make_float2(w[m].x * inv_n + a.video_corr, w[m].y * inv_n + a.video_corr);
+  }
   c32 F[16];
 #pragma unroll
   for (int m = 0; m < 16; m++) {
@@ -369,7 +542,6 @@ __global__ void __launch_bounds__(1024) vhs_block_subdeemph_kernel(SubDeemphArgs
   const float2 fM = a.vh[kFftM];
   const c32 fm = mul(c32{dM.x, dM.y}, c32{fM.x, fM.y});
   __syncthreads();  // twiddles and transition matrices in LDS; D read by every thread
-  const float inv_n = 1.0f / 32768.0f;
   float2* st_go = stash;          // odd-pass input
   float2* st_ev = stash + kFftM;  // even samples (real part, magnitude), then real parts (even, odd)
 #pragma unroll
@@ -411,15 +583,45 @@ __global__ void __launch_bounds__(1024) vhs_block_subdeemph_kernel(SubDeemphArgs
   sos_filtfilt_block_t<1>(a.amp, ext, 32768, s.scratch, s.scratch + kEnvTransOffset, nullptr, 0, 0, 0, false);
   const float* y = ext + a.amp.pad;
   float* vout = a.out_video + a.out_offset[blk];
-  for (int o = j; o < a.out_count; o += kFftThreads) {
-    const int i = a.out_begin + o;
-    float v = fmaxf(0.0f, y[i]) * a.scaling1;
+  auto gain = [&](float amp) {
+    float v = fmaxf(0.0f, amp) * a.scaling1;
     v = powf(v, a.exp_scaling) * a.scaling2;
     if (a.logistic_rate > 0.0f) v *= 1.0f / (1.0f + expf(-a.logistic_rate * (v - a.logistic_mid)));
-    const float2 r = st_ev[i >> 1];
-    const float hp = (i & 1) ? r.y : r.x;
-    vout[o] -= (hp * (1.0f - v)) + (hp * a.static_factor);
+    return v;
+  };
+  if (!a.notch) {
+    for (int o = j; o < a.out_count; o += kFftThreads) {
+      const int i = a.out_begin + o;
+      const float v = gain(y[i]);
+      const float2 r = st_ev[i >> 1];
+      const float hp = (i & 1) ? r.y : r.x;
+      vout[o] -= (hp * (1.0f - v)) + (hp * a.static_factor);
+    }
+    return;
   }
+  // fsc notch (RfDemodulator.ApplyBetamaxFscNotchInPlace, RfDemodulator.cs:1018-1033) over the
+  // whole corrected block, then the kept window
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    const float g0 = gain(y[2 * k]), g1 = gain(y[2 * k + 1]);
+    const float2 r = row_at(st_ev, m);
+    const float2 vv = row_at(aux, m);
+    row_at(aux, m) = make_float2(vv.x - ((r.x * (1.0f - g0)) + (r.x * a.static_factor)),
+                                 vv.y - ((r.y * (1.0f - g1)) + (r.y * a.static_factor)));
+  }
+  __syncthreads();
+  float* nx = ext + a.notch_sos.pad;
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    const float2 c = row_at(aux, m);
+    nx[2 * k] = c.x;
+    nx[2 * k + 1] = c.y;
+  }
+  __syncthreads();
+  sos_filtfilt_block_t<1>(a.notch_sos, ext, 32768, s.scratch, s.scratch + kAuxTransOffset, vout, a.out_begin,
+                          a.out_count, /*roll=*/0, false);
 }
 
 // ------------------------------------------------------------------ spike repair
@@ -609,6 +811,15 @@ hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const floa
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(vhsd::debug_fft_kernel, dim3(count), dim3(1024), lds, stream, reinterpret_cast<float2*>(data),
                      inverse, oct);
+  return hipGetLastError();
+}
+
+hipError_t vhsd_launch_block_demod_boost(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream) {
+  const size_t lds = sizeof(vhsd::BlockLds);
+  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_demod_boost_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(vhsd::vhs_block_demod_boost_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
   return hipGetLastError();
 }
 

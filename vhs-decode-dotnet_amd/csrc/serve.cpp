@@ -48,7 +48,7 @@ bool is_pinned(const void* p) {
 
 // Owned copy of one decode format.
 struct Format {
-  std::vector<double> rf_video, video, vlp05, env_sos, chroma_sos, hp, amp_sos;
+  std::vector<double> rf_video, video, vlp05, env_sos, chroma_sos, hp, amp_sos, boost_sos, notch;
   std::vector<float> sinc;
   vhsdecode_hip_filters_v1 filters{};
   vhsdecode_hip_subdeemph_v1 sd{};
@@ -421,6 +421,16 @@ int32_t vhsdecode_hip_serve_add_format(vhsdecode_hip_serve* s, const vhsdecode_h
       f->sd.high_pass = f->hp.data();
       f->sd.amplitude_sos = f->amp_sos.data();
       f->filters.sub_deemphasis = &f->sd;
+    }
+    f->filters.rf_boost_sos = nullptr;
+    if (src.rf_boost_sos) {
+      f->boost_sos.assign(src.rf_boost_sos, src.rf_boost_sos + 6);
+      f->filters.rf_boost_sos = f->boost_sos.data();
+    }
+    f->filters.video_notch = nullptr;
+    if (src.video_notch) {
+      f->notch.assign(src.video_notch, src.video_notch + 6);
+      f->filters.video_notch = f->notch.data();
     }
     if (fm->sinc_table && fm->sinc_count > 0) f->sinc.assign(fm->sinc_table, fm->sinc_table + fm->sinc_count);
     f->cut = fm->block_cut;

@@ -10,6 +10,9 @@ namespace vhsd {
 // Per-block scratch of the demod kernels: two spectra of 16384 complex FP32 values,
 // then the demod spectrum's Nyquist bin at index 2 * 16384.
 constexpr long long kStashEntries = 2 * 16384 + 64;
+// Per-block auxiliary scratch of the RF-boost and notch paths: one half spectrum or one
+// block of sample pairs.
+constexpr long long kAuxEntries = 16384 + 64;
 
 struct DemodArgs {
   const int16_t* rf;             // RF samples (device)
@@ -33,6 +36,11 @@ struct DemodArgs {
   float* out_env;
   unsigned* flags;               // per block: bit0 weak RF, bit1 spike repair needed
   float2* stash;                 // [nblocks][kStashEntries] scratch, private to each block's workgroup
+  // RF high boost (Betamax): the RF top band, scaled by 0.9 mean(envelope) / envelope and
+  // boost_mult, is added to the filtered RF before the discriminator
+  SosScanArgs boost;             // RF top band-pass (one section)
+  float boost_mult;
+  float2* aux;                   // [nblocks][kAuxEntries]
 };
 
 struct ChromaArgs {
@@ -64,6 +72,13 @@ struct SubDeemphArgs {
   float logistic_rate;     // <= 0: off
   float static_factor;
   float* out_video;
+  // fsc notch (Betamax): applied to the whole sub-deemphasised block before the kept
+  // window is written; the block's video is rebuilt from the stashed demod spectrum
+  int notch;
+  SosScanArgs notch_sos;         // the notch transfer function as one section
+  const float2* video;           // [M+1] Video half spectrum
+  float video_corr;
+  float2* aux;                   // [nblocks][kAuxEntries]
 };
 
 struct RepairArgs {
@@ -132,6 +147,7 @@ hipError_t vhsd_launch_locate(const float* sync_ref, long long span_len, const i
                               int cand_cap, double* cand_line0, int* cand_first, int* cand_count, double* cand_locs,
                               unsigned char* cand_filled, int* cand_hit, double* ref_base, unsigned char* fo_base,
                               const vhsd::FieldTables* t, int* field_count, hipStream_t stream);
+hipError_t vhsd_launch_block_demod_boost(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_video(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_subdeemph(const vhsd::SubDeemphArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const float2* oct, hipStream_t stream);

@@ -225,6 +225,12 @@ struct vhsdecode_hip_demod_context {
   vhsd::SosScanArgs env{}, chroma{};
   DevBuf<float2> rfv, vid, vlp, oct;
   DevBuf<float> env_trans, chroma_trans;
+  // RF high boost and fsc notch (Betamax)
+  bool has_boost = false, has_notch = false;
+  vhsd::SosScanArgs boost{}, notch{};
+  float boost_mult = 0.0f;
+  DevBuf<float> boost_trans, notch_trans;
+  DevBuf<float2> aux;
   // sub-deemphasis
   bool has_subdeemph = false;
   vhsd::SosScanArgs sd_amp{};
@@ -281,6 +287,7 @@ struct vhsdecode_hip_demod_context {
   ~vhsdecode_hip_demod_context() {
     (void)hipSetDevice(device);
     rfv.release(); vid.release(); vlp.release(); oct.release(); sd_vh.release(); sd_trans.release();
+    boost_trans.release(); notch_trans.release(); aux.release();
     env_trans.release(); chroma_trans.release(); block_start.release(); out_offset.release(); flags.release();
     rep_a.release(); rep_dm.release(); stash.release(); tile_counts.release(); counters.release(); p_starts.release();
     p_lens.release(); edge_pos.release(); edge_kind.release(); h_loc.release(); h_ref.release();
@@ -388,9 +395,19 @@ int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int
   a.flags = flags;
   VHSD_HIP(c->stash.ensure((size_t)nblocks * vhsd::kStashEntries));
   a.stash = c->stash.p;
+  if (c->has_boost || c->has_notch) VHSD_HIP(c->aux.ensure((size_t)nblocks * vhsd::kAuxEntries));
+  a.aux = c->aux.p;
+  if (c->has_boost) {
+    a.boost = c->boost;
+    a.boost.trans = c->boost_trans.p;
+    a.boost_mult = c->boost_mult;
+  }
   {
     ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_BLOCK_DEMOD, nblocks);
-    VHSD_HIP(vhsd_launch_block_demod(&a, nblocks, st));
+    if (c->has_boost)
+      VHSD_HIP(vhsd_launch_block_demod_boost(&a, nblocks, st));
+    else
+      VHSD_HIP(vhsd_launch_block_demod(&a, nblocks, st));
   }
   if (c->diff_max > 0.0) {
     VHSD_HIP(c->rep_a.ensure((size_t)nblocks * 32768));
@@ -421,6 +438,14 @@ int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int
     sd.logistic_rate = c->sd_params[5];
     sd.static_factor = c->sd_params[6];
     sd.out_video = out->video;
+    sd.notch = c->has_notch ? 1 : 0;
+    if (c->has_notch) {
+      sd.notch_sos = c->notch;
+      sd.notch_sos.trans = c->notch_trans.p;
+    }
+    sd.video = c->vid.p;
+    sd.video_corr = c->video_corr;
+    sd.aux = c->aux.p;
     ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_SUB_DEEMPHASIS, nblocks);
     VHSD_HIP(vhsd_launch_block_subdeemph(&sd, nblocks, st));
   }
@@ -575,6 +600,23 @@ int32_t vhsdecode_hip_demod_create(const vhsdecode_hip_demod_config_v1* cfg, vhs
       c->sd_params[5] = (float)(sd->logistic_rate > 0.0 ? sd->logistic_rate : 0.0);
       c->sd_params[6] = (float)sd->static_factor;
       c->has_subdeemph = true;
+    }
+    if (f->rf_boost_sos && f->rf_boost_multiplier != 0.0) {
+      if ((rc = prepare_sos(f->rf_boost_sos, 1, c->n, &c->boost, &tr)) != 0) return bail(rc);
+      if (c->boost_trans.ensure(tr.size()) != hipSuccess ||
+          hipMemcpy(c->boost_trans.p, tr.data(), sizeof(float) * tr.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(VHSDECODE_HIP_E_HIP, "RF boost table upload failed"));
+      c->boost_mult = (float)f->rf_boost_multiplier;
+      c->has_boost = true;
+    }
+    if (f->video_notch) {
+      if (!c->has_subdeemph)
+        return bail(fail(VHSDECODE_HIP_E_UNSUPPORTED, "the fsc notch is supported together with sub-deemphasis"));
+      if ((rc = prepare_sos(f->video_notch, 1, c->n, &c->notch, &tr)) != 0) return bail(rc);
+      if (c->notch_trans.ensure(tr.size()) != hipSuccess ||
+          hipMemcpy(c->notch_trans.p, tr.data(), sizeof(float) * tr.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(VHSDECODE_HIP_E_HIP, "notch table upload failed"));
+      c->has_notch = true;
     }
     if (c->block_start.ensure(c->max_blocks) != hipSuccess || c->out_offset.ensure(c->max_blocks) != hipSuccess ||
         c->flags.ensure(c->max_blocks) != hipSuccess)

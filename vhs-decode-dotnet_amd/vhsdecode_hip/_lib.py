@@ -73,7 +73,8 @@ class FiltersV1(ctypes.Structure):
                 ("video_lowpass05", _dp), ("video_lowpass05_offset", ctypes.c_int32), ("envelope_sos", _dp),
                 ("envelope_sections", ctypes.c_int32), ("chroma_sos", _dp), ("chroma_sections", ctypes.c_int32),
                 ("chroma_offset", ctypes.c_int32), ("diff_demod_max", ctypes.c_double),
-                ("plane_bias_hz", ctypes.c_double), ("sub_deemphasis", ctypes.POINTER(SubDeemphV1))]
+                ("plane_bias_hz", ctypes.c_double), ("sub_deemphasis", ctypes.POINTER(SubDeemphV1)),
+                ("rf_boost_sos", _dp), ("rf_boost_multiplier", ctypes.c_double), ("video_notch", _dp)]
 
 
 class DemodConfigV1(ctypes.Structure):

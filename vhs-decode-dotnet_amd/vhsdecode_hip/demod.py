@@ -87,6 +87,15 @@ class FilterTables:
              d.static_factor) = (float(v) for v in sd.params())
             self._sd = d
             f.sub_deemphasis = ctypes.pointer(d)
+        bo = getattr(filters, "rf_high_boost", None)
+        if bo is not None:
+            self._boost = np.ascontiguousarray(bo.rf_top_sos, dtype=np.float64).reshape(-1)
+            f.rf_boost_sos = _ptr(self._boost, _lib._dp)
+            f.rf_boost_multiplier = float(bo.multiplier)
+        nt = getattr(filters, "fsc_notch", None)
+        if nt is not None:
+            self._notch = np.ascontiguousarray(np.concatenate([nt[0], nt[1]]), dtype=np.float64)
+            f.video_notch = _ptr(self._notch, _lib._dp)
         self.struct = f
 
 
