@@ -99,6 +99,35 @@ typedef struct vhsdecode_hip_subdeemph_v1 {
   double static_factor;            /* 0 when absent */
 } vhsdecode_hip_subdeemph_v1;
 
+/* LaserDisc analog audio (DecodeFilterSetBuilder.BuildLdAnalogAudio,
+ * src/VHSDecode.Core/Dsp/DecodeFilterSet.cs:527-562, 913-1017; decoded per block by
+ * RfBlockDecodePipeline.DecodeAnalogAudioChannel, Decode/RfBlockDecodePipeline.cs:732-752,
+ * and stitched by RfBlockStreamDecoder.CopyAudioTrimmedWindow, Decode/RfBlockStreamDecoder.cs:1690-1709). */
+typedef struct vhsdecode_hip_ld_audio_v1 {
+  uint32_t struct_size;
+  int32_t bin_count;               /* slice length: a power of two <= 2048 dividing block_length, block_cut
+                                      and the stride by the same decimation block_length / bin_count */
+  int32_t low_bin[2];              /* left, right: first positive bin of each slice (DetermineFftSlice) */
+  const double* stage1[2];         /* bin_count complex (re, im) each: the slice of the FIR band-pass
+                                      response times the slice's Hilbert multiplier */
+  double slice_rate_hz;            /* sample_rate_hz * bin_count / block_length */
+  double low_frequency_hz[2];      /* sample_rate_hz * low_bin / block_length */
+} vhsdecode_hip_ld_audio_v1;
+
+/* The LaserDisc complex path (RfDemodulator.DemodulateCore with the conjugate-product
+ * discriminator the ld command selects, src/VHSDecode.Core/Decode/DecodeSession.cs:1121-1123;
+ * RfDemodulator.cs:564-605, 722-743, 769-775, 824-907). */
+typedef struct vhsdecode_hip_laserdisc_v1 {
+  uint32_t struct_size;
+  const double* rf_mtf;            /* DecodeFilterSet.RfMtf (BuildLdMtf), NULL = all ones */
+  double clip_low_hz;              /* ClipDemodForVideo bounds (1.5 MHz and 0.75 fs); clip_high_hz <= clip_low_hz: none */
+  double clip_high_hz;
+  const double* video_burst;       /* DecodeFilterSet.LdVideoBurst, NULL: none; written to span.chroma (absolute Hz) */
+  int32_t video_burst_offset;      /* DecodeFilterSet.LdVideoBurstOffset (40): even, within the tail cut */
+  const double* video_pilot;       /* DecodeFilterSet.LdVideoPilot (PAL), NULL: none; written to span.pilot */
+  const vhsdecode_hip_ld_audio_v1* analog_audio;  /* NULL: no analog audio */
+} vhsdecode_hip_laserdisc_v1;
+
 typedef struct vhsdecode_hip_filters_v1 {
   uint32_t struct_size;
   int32_t block_length;            /* N; this build supports 32768 */
@@ -123,6 +152,9 @@ typedef struct vhsdecode_hip_filters_v1 {
   const double* video_notch;       /* Betamax fsc notch transfer function b0 b1 b2 a0 a1 a2
                                       (RfDemodulator.cs:1018-1051); NULL: none.  Needs
                                       sub_deemphasis, which Betamax formats enable */
+  const vhsdecode_hip_laserdisc_v1* laserdisc;  /* non-NULL: LaserDisc complex path.  envelope_sos, chroma_sos,
+                                      sub_deemphasis, rf_boost_sos and video_notch are then ignored, the
+                                      envelope plane holds |analytic| and diff_demod_max must be <= 0 */
 } vhsdecode_hip_filters_v1;
 
 typedef struct vhsdecode_hip_demod_config_v1 {
@@ -145,6 +177,9 @@ typedef struct vhsdecode_hip_span_v1 {
   float* envelope;
   float* chroma;
   uint32_t* block_flags;           /* [nblocks]: bit0 weak RF (an envelope sample == 0), bit1 spike repaired */
+  float* pilot;                    /* LaserDisc PAL pilot reference (absolute Hz), NULL: not written */
+  float* audio_left;               /* LaserDisc analog audio, nblocks * stride / decimation samples each (Hz), */
+  float* audio_right;              /* NULL: not written */
 } vhsdecode_hip_span_v1;
 
 VHSDECODE_HIP_API int32_t vhsdecode_hip_demod_create(const vhsdecode_hip_demod_config_v1* config,
@@ -161,6 +196,36 @@ VHSDECODE_HIP_API int32_t vhsdecode_hip_demod_span_device(vhsdecode_hip_demod_co
 VHSDECODE_HIP_API int32_t vhsdecode_hip_demod_span_host(vhsdecode_hip_demod_context* context, const int16_t* rf,
                                                         int64_t rf_count, int64_t first_sample, int32_t nblocks,
                                                         const vhsdecode_hip_span_v1* out);
+
+/* ---------------------------------------------------------------- HiFi Hilbert discriminator
+ * HiFiHilbertDiscriminator (src/VHSDecode.Core/HiFi/HiFiHilbertDiscriminator.cs:7-156): the
+ * phase of each block's analytic signal by a circular FFT of the whole block
+ * (ComputeAnalyticPhase :48-86), then one output per phase step (DemodulatePhase :88-156).
+ * Blocks are demodulated independently, as HiFiBlockDecoder calls Demodulate per block
+ * (HiFi/HiFiBlockDecoder.cs:186-214); the reference's Hilbert IF block is 2^22 samples
+ * (2^23 Hz, HiFiConstants.cs:55-57). */
+typedef struct vhsdecode_hip_hifi vhsdecode_hip_hifi;
+
+typedef struct vhsdecode_hip_hifi_config_v1 {
+  uint32_t struct_size;
+  int32_t device_id;
+  int64_t block_length;            /* 16384 * 2^k, k = 0 .. 8 (2^14 .. 2^22 samples) */
+  int32_t sample_rate_hz;          /* IF rate (> 0) */
+  double carrier_hz;               /* rounded half-to-even to whole Hz, as the reference's constructor */
+  double deviation_hz;             /* truncated to whole Hz */
+} vhsdecode_hip_hifi_config_v1;
+
+VHSDECODE_HIP_API int32_t vhsdecode_hip_hifi_create(const vhsdecode_hip_hifi_config_v1* config,
+                                                    vhsdecode_hip_hifi** hifi);
+VHSDECODE_HIP_API void vhsdecode_hip_hifi_destroy(vhsdecode_hip_hifi* hifi);
+/* input and output: nblocks * block_length floats of device memory; asynchronous on
+ * `stream` (NULL: the context's stream).  The last output sample of each block is not
+ * written, as in the reference. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_hifi_demodulate_device(vhsdecode_hip_hifi* hifi, const float* input,
+                                                               float* output, int32_t nblocks, void* stream);
+/* Host variant (synchronous). */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_hifi_demodulate_host(vhsdecode_hip_hifi* hifi, const float* input,
+                                                             float* output, int32_t nblocks);
 
 /* ---------------------------------------------------------------- sync pulses
  * PulseDetection.FindPulses (src/VHSDecode.Core/Dsp/PulseDetection.cs:45-222) with the

@@ -9,7 +9,7 @@
 // rounded operations, and explicit std::fma is used exactly where the reference
 // calls Math.FusedMultiplyAdd / MathF.FusedMultiplyAdd.
 //
-// Pinning (see tests/test_oracle.py):
+// Pinning (see tests/test_oracle.py and tests/test_laserdisc_oracle.py):
 //  - fft: against numpy.fft (pocketfft, the library the reference ports).
 //  - demod_block: SHA-256 of the float32 channels of the reference's tape-family
 //    fixture (tests/VHSDecode.Tests/Fixtures/tape-family-v0.4.0.json) and
@@ -18,9 +18,15 @@
 //  - tbc: the three SHA-256 KATs of TbcParallelResamplerTests.cs:76-183.
 //  - sosfiltfilt_f32: scipy.signal.sosfiltfilt in float64 (tolerance) and the
 //    fixture hashes above (bit-exact).
+//  - demod_block_ld: SHA-256 of the float32 Video, DemodRaw, VideoLowPass and
+//    VideoPilot of LaserDiscVideoFilterCompatibilityTests.cs:94-134 (VideoBurst is
+//    unmatched); conj_unwrap: the vector of CompatibilityTests.cs:4748-4751.
+//  - ld_audio_channel: closed-form FM tones (after the band-pass's linear-phase delay).
+//  - hifi_*: the SHA-256 float-bit vectors of HiFiCompatibilityTests.cs:576-670.
 //  - hsync_refine: generator ground truth (closed-form edges) only; the branch
 //    semantics on irregular signals are parity unpinned.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -453,6 +459,63 @@ float mean_f32_fastmath(const float* v, int n) {
   return lanes[0] / (float)n;
 }
 
+
+// ---------------------------------------------------------------- LaserDisc / HiFi
+// System.Numerics.Complex.Abs: |larger| * sqrt(1 + (smaller/larger)^2).
+double dotnet_abs(cd v) {
+  double a = std::fabs(v.real()), b = std::fabs(v.imag());
+  double small = a < b ? a : b, large = a < b ? b : a;
+  if (small == 0.0) return large;
+  if (std::isinf(large) && !std::isnan(small)) return INFINITY;
+  double r = small / large;
+  return large * std::sqrt(1.0 + r * r);
+}
+
+// PortedMath.UnwrapHilbertConjugateProduct (Dsp/PortedMath.cs:39-68): atan2 of the
+// conjugate product of consecutive samples, wrapped to [0, 2 pi), times fs / 2 pi.
+void conj_unwrap(const cd* a, int n, double fs, double* out) {
+  const double tau = 6.283185307179586;
+  const double scale = fs / tau;
+  out[0] = 0.0;
+  for (int i = 1; i < n; i++) {
+    const cd p = a[i - 1], c = a[i];
+    const double re = (c.real() * p.real()) + (c.imag() * p.imag());
+    const double im = (c.imag() * p.real()) - (c.real() * p.imag());
+    double d = std::atan2(im, re);
+    if (d < 0.0) d += tau;
+    out[i] = d * scale;
+  }
+}
+
+// Any-length DFT in long double (the reference's FastFourierTransform.ForwardAnyLength
+// for lengths that are not powers of two; an exact DFT within FP64 rounding).
+void dft_any(cd* x, int n, bool inverse) {
+  std::vector<cd> out(n);
+  const long double two_pi = 6.283185307179586476925286766559L;
+  for (int k = 0; k < n; k++) {
+    long double re = 0.0L, im = 0.0L;
+    for (int t = 0; t < n; t++) {
+      const long double ang = (inverse ? two_pi : -two_pi) * (long double)(((long long)k * t) % n) / n;
+      const long double c = cosl(ang), s = sinl(ang);
+      re += (long double)x[t].real() * c - (long double)x[t].imag() * s;
+      im += (long double)x[t].real() * s + (long double)x[t].imag() * c;
+    }
+    if (inverse) {
+      re /= n;
+      im /= n;
+    }
+    out[k] = cd((double)re, (double)im);
+  }
+  for (int k = 0; k < n; k++) x[k] = out[k];
+}
+
+void fft_any(cd* x, int n, bool inverse) {
+  if ((n & (n - 1)) == 0)
+    fft(x, n, inverse);
+  else
+    dft_any(x, n, inverse);
+}
+
 }  // namespace
 
 extern "C" {
@@ -607,6 +670,124 @@ int oracle_demod_block(const double* input, int n, double fs, const double* rf_v
     for (int i = 0; i < n; i++) chroma_out[i] = (float)(chroma_out[i] - mean);
   }
   return weak;
+}
+
+// RfDemodulator.DemodulateCore, LaserDisc complex path (RfDemodulator.cs:564-605,
+// :621-633 envelope, :722-743 conjugate-product demod, :769-775 ClipDemodForVideo,
+// :824-843 video, :869-907 sync reference / burst / pilot) followed by
+// RfBlockDecodePipeline.QuantizeLaserDiscVideoChannels (RfBlockDecodePipeline.cs:775-793).
+// Filters are whole-circle complex128[n] interleaved; burst_c / pilot_c may be null.
+// Outputs are n doubles each holding float32 values (envelope unquantised, as the
+// reference leaves it).
+void oracle_demod_block_ld(const double* input, int n, double fs, const double* rf_video_c, const double* mtf_c,
+                           const double* video_c, const double* vlp05_c, int vlp_offset, const double* burst_c,
+                           int burst_offset, const double* pilot_c, double clip_low, double clip_high,
+                           double* video_out, double* demod_out, double* vlp_out, double* env_out,
+                           double* burst_out, double* pilot_out) {
+  const cd* rf_video = reinterpret_cast<const cd*>(rf_video_c);
+  const cd* mtf = reinterpret_cast<const cd*>(mtf_c);
+  std::vector<cd> spec(n);
+  for (int i = 0; i < n; i++) spec[i] = cd(input[i], 0.0);
+  fft(spec.data(), n, false);
+  std::vector<cd> a(n);
+  for (int k = 0; k < n; k++) {
+    cd v = cmul_fma(spec[k], rf_video[k]);
+    if (mtf) v = cmul_fma(v, mtf[k]);
+    const double h = (k == 0 || k == n / 2) ? 1.0 : (k < n / 2 ? 2.0 : 0.0);
+    a[k] = cd(v.real() * h, v.imag() * h);
+  }
+  fft(a.data(), n, true);
+  for (int i = 0; i < n; i++) env_out[i] = dotnet_abs(a[i]);
+  std::vector<double> demod(n);
+  conj_unwrap(a.data(), n, fs, demod.data());
+  std::vector<cd> ds(n);
+  for (int i = 0; i < n; i++) {
+    double d = demod[i];
+    if (clip_high > clip_low) d = std::min(std::max(d, clip_low), clip_high);
+    ds[i] = cd(d, 0.0);
+  }
+  fft(ds.data(), n, false);
+  auto filtered_real = [&](const double* filt_c, double* out, int offset) {
+    const cd* f = reinterpret_cast<const cd*>(filt_c);
+    std::vector<cd> t(n);
+    for (int k = 0; k < n; k++) t[k] = cmul_fma(ds[k], f[k]);
+    fft(t.data(), n, true);
+    for (int i = 0; i < n; i++) out[i] = t[i].real();
+    if (offset != 0) roll(out, n, -offset);
+    for (int i = 0; i < n; i++) out[i] = (double)(float)out[i];
+  };
+  filtered_real(video_c, video_out, 0);
+  filtered_real(vlp05_c, vlp_out, vlp_offset);
+  if (burst_c && burst_out) filtered_real(burst_c, burst_out, burst_offset);
+  if (pilot_c && pilot_out) filtered_real(pilot_c, pilot_out, 0);
+  for (int i = 0; i < n; i++) demod_out[i] = (double)(float)demod[i];
+}
+
+void oracle_conj_unwrap(const double* analytic_interleaved, int n, double fs, double* out) {
+  conj_unwrap(reinterpret_cast<const cd*>(analytic_interleaved), n, fs, out);
+}
+
+// RfBlockDecodePipeline.DecodeAnalogAudioChannel (RfBlockDecodePipeline.cs:732-752):
+// the power-of-two slice of the block's full spectrum (DecodeFilterSet.SliceSpectrum,
+// DecodeFilterSet.cs:991-1017) times the stage-1 response, a complex inverse FFT, the
+// conjugate-product discriminator at the slice rate, plus the slice's low frequency,
+// rounded to float32.  `spectrum` is the block's complex128[n] forward FFT.
+void oracle_ld_audio_channel(const double* spectrum_c, int n, int low_bin, int bin_count, const double* stage1_c,
+                             double slice_rate, double low_freq, double* out) {
+  const cd* spec = reinterpret_cast<const cd*>(spectrum_c);
+  const cd* st = reinterpret_cast<const cd*>(stage1_c);
+  const int half = bin_count / 2;
+  std::vector<cd> s(bin_count);
+  for (int i = 0; i < half; i++) {
+    s[i] = spec[low_bin + i];
+    s[half + i] = spec[n - low_bin - half + i];
+  }
+  for (int i = 0; i < bin_count; i++) {
+    const cd x = s[i], y = st[i];
+    s[i] = cd(x.real() * y.real() - x.imag() * y.imag(), x.imag() * y.real() + x.real() * y.imag());
+  }
+  fft(s.data(), bin_count, true);
+  conj_unwrap(s.data(), bin_count, slice_rate, out);
+  for (int i = 0; i < bin_count; i++) out[i] = (double)(float)(out[i] + low_freq);
+}
+
+// HiFiHilbertDiscriminator.ComputeAnalyticPhase (HiFi/HiFiHilbertDiscriminator.cs:48-86):
+// phase[i] = float(atan2 of the analytic signal) in place.
+void oracle_hifi_analytic_phase(float* x, int n) {
+  std::vector<cd> s(n);
+  for (int i = 0; i < n; i++) s[i] = cd((double)x[i], 0.0);
+  fft_any(s.data(), n, false);
+  const int mid = n / 2;
+  for (int i = 0; i < n; i++) {
+    const double m = i == 0 ? 1.0 : (i < mid ? 2.0 : (i == mid ? 1.0 : 0.0));
+    s[i] = cd(s[i].real() * m, s[i].imag() * m);
+  }
+  fft_any(s.data(), n, true);
+  for (int i = 0; i < n; i++) x[i] = (float)std::atan2(s[i].imag(), s[i].real());
+}
+
+// HiFiHilbertDiscriminator.DemodulatePhase (HiFi/HiFiHilbertDiscriminator.cs:88-156):
+// writes output[0 .. n-2]; output[n-1] is left as it was.
+void oracle_hifi_demodulate_phase(const float* phase, int n, float* output, int fs, int carrier, int deviation) {
+  const double pi = 3.141592653589793, tau = 6.283185307179586;
+  double prev_phase = 0.0, prev_corr = 0.0;
+  const double carrier_scale = (double)carrier / deviation;
+  const float rate = (float)fs;
+  for (int i = 0; i < n - 1; i++) {
+    const double cur = (double)phase[i];
+    const double diff = cur - prev_phase;
+    prev_phase = cur;
+    double m = std::fmod(diff + pi, tau);
+    if (m < 0.0) m += tau;
+    double wrapped = m - pi;
+    if (wrapped == -pi && diff > 0.0) wrapped = pi;
+    double corr = wrapped - diff;
+    if (std::fabs(diff) < pi) corr = 0.0;
+    corr = prev_corr + corr;
+    const double value = (corr - prev_corr) / tau * (double)rate / deviation;
+    output[i] = (float)std::min(std::max(value - carrier_scale, (double)-FLT_MAX), (double)FLT_MAX);
+    prev_corr = corr;
+  }
 }
 
 // PulseDetection.FindPulsesScalar (PulseDetection.cs:195-222).  Writes up to cap

@@ -44,6 +44,14 @@ def lib():
                                          _dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          _dp, _dp, _dp, _fp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.oracle_demod_block.restype = ctypes.c_int
+        L.oracle_demod_block_ld.argtypes = [_dp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                            _dp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
+                                            _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_conj_unwrap.argtypes = [_dp, ctypes.c_int, ctypes.c_double, _dp]
+        L.oracle_ld_audio_channel.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
+                                              ctypes.c_double, ctypes.c_double, _dp]
+        L.oracle_hifi_analytic_phase.argtypes = [_fp, ctypes.c_int]
+        L.oracle_hifi_demodulate_phase.argtypes = [_fp, ctypes.c_int, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.oracle_find_pulses.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                          _ip, _ip, ctypes.c_int]
         L.oracle_find_pulses.restype = ctypes.c_int
@@ -185,3 +193,76 @@ def tbc_resample(src, locations, first_line, line_count, out_len, lut, nominal=N
     if rc != 0:
         raise ValueError("invalid line range for TBC resampling")
     return (out, u16) if conv is not None else out
+
+
+class LdBlock:
+    def __init__(self, n: int):
+        self.video = np.zeros(n)
+        self.demod = np.zeros(n)
+        self.video_lowpass = np.zeros(n)
+        self.envelope = np.zeros(n)
+        self.video_burst = np.zeros(n)
+        self.video_pilot = np.zeros(n)
+
+
+def _cplx(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def demod_block_ld(block: np.ndarray, filters) -> LdBlock:
+    """LaserDisc block demod (oracle_demod_block_ld) with a LaserDiscFilterSet."""
+    x = np.ascontiguousarray(block, dtype=np.float64)
+    n = x.size
+    out = LdBlock(n)
+    rv, mtf, vid, vlp = (_cplx(a) for a in (filters.rf_video, filters.rf_mtf, filters.video, filters.video_lowpass05))
+    burst, pilot = _cplx(filters.video_burst), _cplx(filters.video_pilot)
+    lib().oracle_demod_block_ld(
+        _p(x, _dp), n, filters.sample_rate_hz, _p(rv.view(np.float64), _dp), _p(mtf.view(np.float64), _dp),
+        _p(vid.view(np.float64), _dp), _p(vlp.view(np.float64), _dp), filters.video_lowpass05_offset,
+        _p(burst.view(np.float64), _dp) if burst is not None else None, filters.video_burst_offset,
+        _p(pilot.view(np.float64), _dp) if pilot is not None else None, filters.clip_low_hz, filters.clip_high_hz,
+        _p(out.video, _dp), _p(out.demod, _dp), _p(out.video_lowpass, _dp), _p(out.envelope, _dp),
+        _p(out.video_burst, _dp), _p(out.video_pilot, _dp))
+    if burst is None:
+        out.video_burst = None
+    if pilot is None:
+        out.video_pilot = None
+    return out
+
+
+def conj_unwrap(analytic: np.ndarray, fs: float) -> np.ndarray:
+    a = np.ascontiguousarray(analytic, dtype=np.complex128)
+    out = np.zeros(a.size)
+    lib().oracle_conj_unwrap(_p(a.view(np.float64), _dp), a.size, fs, _p(out, _dp))
+    return out
+
+
+def ld_audio_block(block: np.ndarray, filters):
+    """LaserDisc analog audio of one block: (left, right) at the slice rate."""
+    spec = fft(np.asarray(block, dtype=np.float64).astype(np.complex128))
+    res = []
+    for ch in (filters.audio_left, filters.audio_right):
+        st = _cplx(ch.stage1)
+        out = np.zeros(ch.bin_count)
+        lib().oracle_ld_audio_channel(_p(spec.view(np.float64), _dp), spec.size, ch.low_bin, ch.bin_count,
+                                      _p(st.view(np.float64), _dp), ch.slice_rate_hz, ch.low_frequency_hz,
+                                      _p(out, _dp))
+        res.append(out)
+    return res[0], res[1]
+
+
+def hifi_analytic_phase(x: np.ndarray) -> np.ndarray:
+    buf = np.ascontiguousarray(x, dtype=np.float32).copy()
+    lib().oracle_hifi_analytic_phase(_p(buf, _fp), buf.size)
+    return buf
+
+
+def hifi_demodulate_phase(phase: np.ndarray, output: np.ndarray, fs: int, carrier: int, deviation: int) -> None:
+    ph = np.ascontiguousarray(phase, dtype=np.float32)
+    assert output.dtype == np.float32 and output.flags["C_CONTIGUOUS"] and output.size >= ph.size
+    lib().oracle_hifi_demodulate_phase(_p(ph, _fp), ph.size, _p(output, _fp), fs, carrier, deviation)
+
+
+def hifi_demodulate(x: np.ndarray, output: np.ndarray, fs: int, carrier: int, deviation: int) -> None:
+    """HiFiHilbertDiscriminator.Demodulate (HiFi/HiFiHilbertDiscriminator.cs:42-46)."""
+    hifi_demodulate_phase(hifi_analytic_phase(x), output, fs, carrier, deviation)

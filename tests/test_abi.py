@@ -57,6 +57,9 @@ def _c_sizeof(struct):
     ("vhsdecode_hip_serve_format_v1", _lib.ServeFormatV1),
     ("vhsdecode_hip_window_v1", _lib.WindowV1),
     ("vhsdecode_hip_window_result_v1", _lib.WindowResultV1),
+    ("vhsdecode_hip_laserdisc_v1", _lib.LaserDiscV1),
+    ("vhsdecode_hip_ld_audio_v1", _lib.LdAudioV1),
+    ("vhsdecode_hip_hifi_config_v1", _lib.HifiConfigV1),
 ])
 def test_struct_layouts_match_header(name, cls):
     assert _c_sizeof(name) == ctypes.sizeof(cls)

@@ -189,10 +189,16 @@ __device__ __forceinline__ void video_stage(BlockLds& s, const DemodArgs& a, lon
   // the plane bias first (exact: both are near the same binade) and restore the DC the
   // filters pass as a constant correction, leaving offsets from the bias in the planes.
   c32 D[16];
+  const bool clip = a.clip_hi > a.clip_lo;
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     const float2 d = row_at(stash, m);
-    D[m] = c32{d.x - a.bias, d.y - a.bias};
+    float x0 = d.x, x1 = d.y;
+    if (clip) {
+      x0 = fminf(fmaxf(x0, a.clip_lo), a.clip_hi);
+      x1 = fminf(fmaxf(x1, a.clip_lo), a.clip_hi);
+    }
+    D[m] = c32{x0 - a.stash_sub, x1 - a.stash_sub};
   }
   fft16k_forward(D, s.buf, s.oct);
   c32 dm;
@@ -203,12 +209,14 @@ __device__ __forceinline__ void video_stage(BlockLds& s, const DemodArgs& a, lon
 #pragma unroll
   for (int m = 0; m < 16; m++) row_at(stash, m) = make_float2(D[m].x, D[m].y);
   if (j == 0) stash[2 * kFftM] = make_float2(dm.x, dm.y);
+  // passes: video, sync reference, then up to two reference planes (LaserDisc burst, pilot)
 #pragma unroll 1
-  for (int pass = 0; pass < 2; pass++) {
-    const float2* filt = pass == 0 ? a.video : a.vlp05;
-    float* out = pass == 0 ? a.out_video : a.out_vlp;
-    const int shift = pass == 0 ? 0 : a.vlp_offset;
-    const float corr = pass == 0 ? a.video_corr : a.vlp_corr;
+  for (int pass = 0; pass < 2 + a.nref; pass++) {
+    const bool ref1 = pass == 3;
+    const float2* filt = pass == 0 ? a.video : pass == 1 ? a.vlp05 : (ref1 ? a.ref_filt[1] : a.ref_filt[0]);
+    float* out = pass == 0 ? a.out_video : pass == 1 ? a.out_vlp : (ref1 ? a.ref_out[1] : a.ref_out[0]);
+    const int shift = pass == 0 ? 0 : pass == 1 ? a.vlp_offset : (ref1 ? a.ref_shift[1] : a.ref_shift[0]);
+    const float corr = pass == 0 ? a.video_corr : pass == 1 ? a.vlp_corr : (ref1 ? a.ref_corr[1] : a.ref_corr[0]);
     c32 v[16];
 #pragma unroll
     for (int m = 0; m < 16; m++) {
@@ -450,6 +458,191 @@ __global__ void __launch_bounds__(1024) vhs_block_demod_boost_kernel(DemodArgs a
   const bool spike = analytic_demod(v, fb, s, a.fs, a.diff_max, stash);
   const unsigned long long any_spike = __ballot(spike);
   if ((j & 63) == 0 && any_spike) atomicOr(&a.flags[blk], 2u);
+}
+
+// ------------------------------------------------------------------ LaserDisc
+// PortedMath.UnwrapHilbertConjugateProduct (Dsp/PortedMath.cs:39-68) for one pair of
+// FP32 analytic samples: the products are exact in FP64, then atan2 wrapped to
+// [0, 2 pi), times fs / 2 pi.
+__device__ __forceinline__ double conj_disc(float2 c, float2 p, double scale) {
+  const double re = (double)c.x * p.x + (double)c.y * p.y;
+  const double im = (double)c.y * p.x - (double)c.x * p.y;
+  double d = atan2(im, re);
+  if (d < 0.0) d += 6.283185307179586;
+  return d * scale;
+}
+
+__device__ __forceinline__ float mag32(float2 v) { return sqrtf(v.x * v.x + v.y * v.y); }
+
+// Analytic signal and conjugate-product discriminator of the LaserDisc complex path
+// (RfDemodulator.cs:564-605, 722-743).  As analytic_demod, with the analytic samples
+// themselves kept: even samples in the stash, odd samples in LDS.  The demod goes to
+// stash[k] = (sample 2k, sample 2k+1) as offsets from the plane bias; the envelope
+// |analytic| (BuildAnalyticMagnitudeEnvelope, RfDemodulator.cs:1709-1727) is written
+// straight to the kept window of the envelope plane.
+__device__ __forceinline__ void analytic_demod_ld(c32* F, c32 fm, BlockLds& s, const DemodArgs& a,
+                                                  long long out_base, float2* __restrict__ stash) {
+  const int j = threadIdx.x;
+  const float inv_n = 1.0f / 32768.0f;
+  float2* st_go = stash;
+  float2* st_ev = stash + kFftM;
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    const c32 g = (k == 0) ? sub(F[m], fm) : mul(scale(F[m], 2.0f), conjc(tw32k(s.oct, k)));
+    row_at(st_go, m) = make_float2(g.x, g.y);
+  }
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    F[m] = (k == 0) ? add(F[m], fm) : scale(F[m], 2.0f);
+  }
+  fft16k_inverse(F, s.buf, s.oct);
+#pragma unroll
+  for (int m = 0; m < 16; m++) row_at(st_ev, m) = make_float2(F[m].x * inv_n, F[m].y * inv_n);
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const float2 g = row_at(st_go, m);
+    F[m] = c32{g.x, g.y};
+  }
+  fft16k_inverse(F, s.buf, s.oct);
+  float2* odd = reinterpret_cast<float2*>(s.buf);
+#pragma unroll
+  for (int m = 0; m < 16; m++) odd[j + m * kFftThreads] = make_float2(F[m].x * inv_n, F[m].y * inv_n);
+  __syncthreads();
+  const double scale_hz = a.fs_d / 6.283185307179586;
+  const double bias = (double)a.bias;
+  float* env = a.out_env + out_base - a.out_begin;
+#pragma unroll 1
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    const float2 ae = row_at(st_ev, m);
+    const float2 ao = odd[k];
+    const double de = k == 0 ? 0.0 : conj_disc(ae, odd[k - 1], scale_hz);
+    const double dd = conj_disc(ao, ae, scale_hz);
+    row_at(st_go, m) = make_float2((float)(de - bias), (float)(dd - bias));
+    const int o = 2 * k - a.out_begin;
+    if (o >= 0 && o < a.out_count) env[2 * k] = mag32(ae);
+    if (o + 1 >= 0 && o + 1 < a.out_count) env[2 * k + 1] = mag32(ao);
+  }
+  __syncthreads();
+}
+
+// One 2^log2n-point inverse complex FFT per channel, both channels at once, radix-2
+// Stockham between two LDS buffers; 1024 threads, one butterfly per thread per channel
+// slot.  Returns the buffer holding the (unnormalised) result.
+__device__ __forceinline__ float2* audio_ifft(float2* x, float2* y, int n, int log2n, const float2* __restrict__ oct) {
+  const int half = n >> 1;
+  for (int p = 0; p < log2n; p++) {
+    const int ns = 1 << p;
+    for (int t = threadIdx.x; t < 2 * half; t += blockDim.x) {
+      const int ch = t / half;
+      const int b = t - ch * half;
+      const float2* xi = x + ch * n;
+      float2* yo = y + ch * n;
+      const int k = b & (ns - 1);
+      const c32 u = c32{xi[b].x, xi[b].y};
+      c32 v = c32{xi[b + half].x, xi[b + half].y};
+      // exp(+2 pi i k / (2 ns)) = conj(tw16k(k * 16384 / (2 ns)))
+      v = mul(v, conjc(tw16k(oct, (k * (kFftM >> p)) >> 1)));
+      const int idx = ((b - k) << 1) + k;
+      const c32 s0 = add(u, v), s1 = sub(u, v);
+      yo[idx] = make_float2(s0.x, s0.y);
+      yo[idx + ns] = make_float2(s1.x, s1.y);
+    }
+    __syncthreads();
+    float2* t = x;
+    x = y;
+    y = t;
+  }
+  return x;
+}
+
+// LaserDisc block demod (RfDemodulator.DemodulateCore complex path, RfDemodulator.cs:564-605,
+// 621-633, 722-743): rfft -> x (RfVideo . RfMtf) -> analytic -> conjugate-product
+// discriminator, |analytic| envelope.  With analog audio the two carrier slices of the
+// input spectrum (SliceSpectrum, DecodeFilterSet.cs:991-1017) are multiplied by their
+// stage-1 responses, inverse transformed and demodulated at the slice rate
+// (RfBlockDecodePipeline.DecodeAnalogAudioChannel, RfBlockDecodePipeline.cs:732-752);
+// each block writes its kept audio window (RfBlockStreamDecoder.cs:1690-1709).
+__global__ void __launch_bounds__(1024) vhs_block_demod_ld_kernel(DemodArgs a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
+  const int j = threadIdx.x;
+  const int blk = blockIdx.x;
+  const long long start = a.block_start[blk];
+  const long long out_base = a.out_offset[blk];
+  load_octant(s, a.oct);
+  c32 v[16];
+  const int16_t* x = a.rf + start;
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int16_t* xr = x + 2 * m * kFftThreads;
+    v[m] = c32{(float)xr[2 * j], (float)xr[2 * j + 1]};
+  }
+  __syncthreads();
+  fft16k_forward(v, s.buf, s.oct);
+  c32 xm;
+  rfft_post(v, s, xm);
+  float2* slices = a.aux + (long long)blk * kAuxEntries;
+  if (a.audio) {
+    // slice[i] = X[low + i] for i < half; slice[half + i] = conj(X[low + half - i])
+    const int n = a.audio_bins, half = n >> 1;
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const int k = j + m * kFftThreads;
+#pragma unroll
+      for (int ch = 0; ch < 2; ch++) {
+        const int low = a.audio_low[ch];
+        const float2* st1 = ch ? a.audio_stage1[1] : a.audio_stage1[0];
+        float2* sl = slices + ch * n;
+        const int i = k - low;
+        if (i >= 0 && i < half) {
+          const c32 y = mul(v[m], c32{st1[i].x, st1[i].y});
+          sl[i] = make_float2(y.x, y.y);
+        }
+        const int r = low + half - k;
+        if (r >= 0 && r < half) {
+          const c32 y = mul(conjc(v[m]), c32{st1[half + r].x, st1[half + r].y});
+          sl[half + r] = make_float2(y.x, y.y);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const float2 f = row_at(a.rf_video, m);
+    v[m] = mul(v[m], c32{f.x, f.y});
+  }
+  const float2 fM = a.rf_video[kFftM];
+  const c32 fm = mul(xm, c32{fM.x, fM.y});
+  if (j == 0) {
+    s.scratch[0] = fm.x;
+    s.scratch[1] = fm.y;
+  }
+  __syncthreads();
+  const c32 fm_b = c32{s.scratch[0], s.scratch[1]};
+  __syncthreads();
+  float2* stash = a.stash + (long long)blk * kStashEntries;
+  analytic_demod_ld(v, fm_b, s, a, out_base, stash);
+  if (!a.audio) return;
+  // ---- analog audio: both slices through LDS, inverse FFT, discriminator, kept window
+  const int n = a.audio_bins;
+  float2* b0 = reinterpret_cast<float2*>(s.buf);
+  float2* b1 = b0 + 2 * n;
+  for (int t = j; t < 2 * n; t += kFftThreads) b0[t] = slices[t];
+  __syncthreads();
+  const float2* res = audio_ifft(b0, b1, n, a.audio_log2, s.oct);
+  const long long ob = out_base / a.audio_decimation;
+  for (int t = j; t < 2 * n; t += kFftThreads) {
+    const int ch = t >= n;
+    const int i = t - ch * n;
+    const float2* r = res + ch * n;
+    const double d = i == 0 ? 0.0 : conj_disc(r[i], r[i - 1], a.audio_scale);
+    const float val = (float)(d + (ch ? a.audio_low_freq[1] : a.audio_low_freq[0]));
+    const int o = i - a.audio_cut;
+    if (o >= 0 && o < a.audio_stride) (ch ? a.out_audio[1] : a.out_audio[0])[ob + o] = val;
+  }
 }
 
 // Stage C: video and sync-reference planes from the demod the block demod (or spike
@@ -820,6 +1013,15 @@ hipError_t vhsd_launch_block_demod_boost(const vhsd::DemodArgs* a, int nblocks, 
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(vhsd::vhs_block_demod_boost_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
+  return hipGetLastError();
+}
+
+hipError_t vhsd_launch_block_demod_ld(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream) {
+  const size_t lds = sizeof(vhsd::BlockLds);
+  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_demod_ld_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(vhsd::vhs_block_demod_ld_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
   return hipGetLastError();
 }
 

@@ -391,6 +391,8 @@ int32_t vhsdecode_hip_serve_add_format(vhsdecode_hip_serve* s, const vhsdecode_h
         fm->locator->struct_size != sizeof(vhsdecode_hip_locator_v1))
       return set_error(VHSDECODE_HIP_E_STRUCT_SIZE, "format struct_size mismatch");
     const vhsdecode_hip_filters_v1& src = *fm->filters;
+    if (src.laserdisc)
+      return set_error(VHSDECODE_HIP_E_UNSUPPORTED, "served windows decode tape formats; LaserDisc is block and span only");
     if (src.block_length != kBlock || !src.rf_video || !src.video || !src.video_lowpass05 || !src.envelope_sos)
       return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "format filters are incomplete");
     const int stride = kBlock - fm->block_cut - fm->block_cut_end;

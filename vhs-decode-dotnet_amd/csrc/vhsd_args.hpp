@@ -41,6 +41,30 @@ struct DemodArgs {
   SosScanArgs boost;             // RF top band-pass (one section)
   float boost_mult;
   float2* aux;                   // [nblocks][kAuxEntries]
+  // The video kernel reads the stash as clamp(x, clip_lo, clip_hi) - stash_sub: the tape
+  // demod kernels stash absolute Hz (stash_sub = bias, no clip); the LaserDisc kernel
+  // stashes offsets from the bias (stash_sub = 0) and clips for video (clip_hi > clip_lo).
+  float stash_sub;
+  float clip_lo, clip_hi;
+  // reference planes after video and sync reference (LaserDisc burst and pilot)
+  int nref;
+  const float2* ref_filt[2];     // [M+1] half spectra
+  float* ref_out[2];             // absolute Hz planes
+  int ref_shift[2];              // roll (samples)
+  float ref_corr[2];             // bias * Re H[0]
+  // LaserDisc conjugate-product discriminator and analog audio
+  double fs_d;                   // sample rate in FP64
+  int audio;                     // 1: two audio slices per block
+  int audio_low[2];              // first positive bin of each slice
+  int audio_bins;                // slice length (power of two, <= 2048)
+  int audio_log2;
+  const float2* audio_stage1[2]; // [audio_bins] stage-1 responses
+  double audio_scale;            // slice rate / 2 pi
+  double audio_low_freq[2];      // Hz added to each demodulated slice
+  int audio_cut;                 // kept window of each block's slice: [cut, cut + stride)
+  int audio_stride;
+  int audio_decimation;          // block_length / audio_bins
+  float* out_audio[2];
 };
 
 struct ChromaArgs {
@@ -148,6 +172,7 @@ hipError_t vhsd_launch_locate(const float* sync_ref, long long span_len, const i
                               unsigned char* cand_filled, int* cand_hit, double* ref_base, unsigned char* fo_base,
                               const vhsd::FieldTables* t, int* field_count, hipStream_t stream);
 hipError_t vhsd_launch_block_demod_boost(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
+hipError_t vhsd_launch_block_demod_ld(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_video(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_subdeemph(const vhsd::SubDeemphArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const float2* oct, hipStream_t stream);

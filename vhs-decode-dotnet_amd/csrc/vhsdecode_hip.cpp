@@ -231,6 +231,18 @@ struct vhsdecode_hip_demod_context {
   float boost_mult = 0.0f;
   DevBuf<float> boost_trans, notch_trans;
   DevBuf<float2> aux;
+  // LaserDisc complex path
+  bool ld = false;
+  float clip_lo = 0.0f, clip_hi = 0.0f;   // offsets from the bias
+  int nref = 0;                           // burst then pilot
+  bool ref_is_pilot[2] = {false, false};
+  DevBuf<float2> ref_filt[2];
+  float ref_corr[2] = {0.0f, 0.0f};
+  int ref_shift[2] = {0, 0};
+  bool has_audio = false;
+  int audio_bins = 0, audio_log2 = 0, audio_low[2] = {0, 0}, audio_dec = 0;
+  DevBuf<float2> audio_st[2];
+  double audio_rate = 0.0, audio_low_freq[2] = {0.0, 0.0};
   // sub-deemphasis
   bool has_subdeemph = false;
   vhsd::SosScanArgs sd_amp{};
@@ -287,6 +299,7 @@ struct vhsdecode_hip_demod_context {
   ~vhsdecode_hip_demod_context() {
     (void)hipSetDevice(device);
     rfv.release(); vid.release(); vlp.release(); oct.release(); sd_vh.release(); sd_trans.release();
+    ref_filt[0].release(); ref_filt[1].release(); audio_st[0].release(); audio_st[1].release();
     boost_trans.release(); notch_trans.release(); aux.release();
     env_trans.release(); chroma_trans.release(); block_start.release(); out_offset.release(); flags.release();
     rep_a.release(); rep_dm.release(); stash.release(); tile_counts.release(); counters.release(); p_starts.release();
@@ -344,12 +357,73 @@ int32_t upload_half(DevBuf<float2>& dst, const double* full, int n) {
   return VHSDECODE_HIP_OK;
 }
 
+// The half spectrum of the Hermitian part (H[k] + conj(H[N-k])) / 2 of a whole-circle
+// response: Re(ifft(X H)) of a real signal's spectrum X is irfft of X times it, which is
+// what the LaserDisc path's complex inverse followed by the real part computes.
+int32_t upload_half_hermitian(DevBuf<float2>& dst, const double* full, int n) {
+  const int m = n / 2 + 1;
+  std::vector<double> h((size_t)2 * n, 0.0);
+  for (int k = 0; k < m; k++) {
+    const int q = (n - k) % n;
+    h[2 * k] = 0.5 * (full[2 * k] + full[2 * q]);
+    h[2 * k + 1] = 0.5 * (full[2 * k + 1] - full[2 * q + 1]);
+  }
+  return upload_half(dst, h.data(), n);
+}
+
 int32_t ensure_lut(vhsdecode_hip_demod_context* c) {
   if (c->lut_ready) return VHSDECODE_HIP_OK;
   std::vector<float> t = kaiser_sinc_table();
   VHSD_HIP(c->lut.ensure(t.size()));
   VHSD_HIP(hipMemcpy(c->lut.p, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice));
   c->lut_ready = true;
+  return VHSDECODE_HIP_OK;
+}
+
+// LaserDisc: complex-path demod kernel, then video, sync reference, burst and pilot
+// planes from the stashed demod (clipped for video).
+int32_t demod_span_ld(vhsdecode_hip_demod_context* c, vhsd::DemodArgs& a, int32_t nblocks,
+                      const vhsdecode_hip_span_v1* out, hipStream_t st) {
+  a.stash_sub = 0.0f;
+  a.clip_lo = c->clip_lo;
+  a.clip_hi = c->clip_hi;
+  a.fs_d = c->fs;
+  a.diff_max = 0.0;
+  int nref = 0;
+  for (int r = 0; r < c->nref; r++) {
+    float* dst = c->ref_is_pilot[r] ? out->pilot : out->chroma;
+    if (!dst) continue;
+    a.ref_filt[nref] = c->ref_filt[r].p;
+    a.ref_out[nref] = dst;
+    a.ref_shift[nref] = c->ref_shift[r];
+    a.ref_corr[nref] = c->ref_corr[r];
+    nref++;
+  }
+  a.nref = nref;
+  if (c->has_audio && out->audio_left && out->audio_right) {
+    a.audio = 1;
+    a.audio_bins = c->audio_bins;
+    a.audio_log2 = c->audio_log2;
+    for (int ch = 0; ch < 2; ch++) {
+      a.audio_low[ch] = c->audio_low[ch];
+      a.audio_stage1[ch] = c->audio_st[ch].p;
+      a.audio_low_freq[ch] = c->audio_low_freq[ch];
+    }
+    a.audio_scale = c->audio_rate / 6.283185307179586;
+    a.audio_cut = c->cut / c->audio_dec;
+    a.audio_stride = c->stride / c->audio_dec;
+    a.audio_decimation = c->audio_dec;
+    a.out_audio[0] = out->audio_left;
+    a.out_audio[1] = out->audio_right;
+  }
+  {
+    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_BLOCK_DEMOD, nblocks);
+    VHSD_HIP(vhsd_launch_block_demod_ld(&a, nblocks, st));
+  }
+  {
+    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_BLOCK_VIDEO, nblocks);
+    VHSD_HIP(vhsd_launch_block_video(&a, nblocks, st));
+  }
   return VHSDECODE_HIP_OK;
 }
 
@@ -395,8 +469,10 @@ int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int
   a.flags = flags;
   VHSD_HIP(c->stash.ensure((size_t)nblocks * vhsd::kStashEntries));
   a.stash = c->stash.p;
-  if (c->has_boost || c->has_notch) VHSD_HIP(c->aux.ensure((size_t)nblocks * vhsd::kAuxEntries));
+  if (c->has_boost || c->has_notch || c->has_audio) VHSD_HIP(c->aux.ensure((size_t)nblocks * vhsd::kAuxEntries));
   a.aux = c->aux.p;
+  a.stash_sub = a.bias;
+  if (c->ld) return demod_span_ld(c, a, nblocks, out, st);
   if (c->has_boost) {
     a.boost = c->boost;
     a.boost.trans = c->boost_trans.p;
@@ -500,6 +576,82 @@ int32_t vhsdecode_hip_get_runtime_info(int32_t device_id, vhsdecode_hip_runtime_
   }
 }
 
+namespace {
+
+// LaserDisc tables: RfVideo x RfMtf (FMA complex product, as the reference's
+// NumpyComplexMultiply) for the analytic signal, Hermitian parts of Video,
+// VideoLowPass05, burst and pilot for the real-output passes, and the audio slices.
+int32_t setup_laserdisc(vhsdecode_hip_demod_context* c, const vhsdecode_hip_filters_v1* f,
+                        const vhsdecode_hip_laserdisc_v1* ld) {
+  const int n = c->n;
+  c->ld = true;
+  std::vector<double> fr((size_t)2 * n);
+  for (int k = 0; k < n; k++) {
+    double ar = f->rf_video[2 * k], ai = f->rf_video[2 * k + 1];
+    if (ld->rf_mtf) {
+      const double br = ld->rf_mtf[2 * k], bi = ld->rf_mtf[2 * k + 1];
+      const double r = std::fma(ar, br, -(ai * bi));
+      const double i = std::fma(ar, bi, ai * br);
+      ar = r;
+      ai = i;
+    }
+    fr[2 * k] = ar;
+    fr[2 * k + 1] = ai;
+  }
+  int32_t rc;
+  if ((rc = upload_half(c->rfv, fr.data(), n)) != 0) return rc;
+  if ((rc = upload_half_hermitian(c->vid, f->video, n)) != 0) return rc;
+  if ((rc = upload_half_hermitian(c->vlp, f->video_lowpass05, n)) != 0) return rc;
+  if (ld->clip_high_hz > ld->clip_low_hz) {
+    c->clip_lo = (float)(ld->clip_low_hz - c->bias);
+    c->clip_hi = (float)(ld->clip_high_hz - c->bias);
+  }
+  const double* refs[2] = {ld->video_burst, ld->video_pilot};
+  for (int r = 0; r < 2; r++) {
+    if (!refs[r]) continue;
+    const int shift = r == 0 ? ld->video_burst_offset : 0;
+    if ((shift & 1) || shift < 0 || shift > c->cut_end)
+      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "video_burst_offset must be even and within the tail cut");
+    const int i = c->nref++;
+    if ((rc = upload_half_hermitian(c->ref_filt[i], refs[r], n)) != 0) return rc;
+    c->ref_is_pilot[i] = r == 1;
+    c->ref_shift[i] = shift;
+    c->ref_corr[i] = (float)(c->bias * refs[r][0]);
+  }
+  if (const vhsdecode_hip_ld_audio_v1* au = ld->analog_audio) {
+    if (au->struct_size != sizeof(vhsdecode_hip_ld_audio_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "analog_audio struct_size mismatch");
+    const int bins = au->bin_count;
+    int lg = 0;
+    while ((1 << lg) < bins) lg++;
+    if (bins < 2 || bins > 2048 || (1 << lg) != bins || n % bins)
+      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "analog audio slices must be powers of two from 2 to 2048 bins");
+    const int dec = n / bins;
+    if (c->cut % dec || c->stride % dec)
+      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "block cut and stride must be multiples of the audio decimation");
+    if (!au->stage1[0] || !au->stage1[1] || !(au->slice_rate_hz > 0.0))
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "analog audio needs both stage-1 responses and a slice rate");
+    for (int ch = 0; ch < 2; ch++) {
+      if (au->low_bin[ch] < 0 || au->low_bin[ch] + bins / 2 >= n / 2)
+        return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "analog audio slice is outside the positive half spectrum");
+      std::vector<float2> h(bins);
+      for (int i = 0; i < bins; i++) h[i] = make_float2((float)au->stage1[ch][2 * i], (float)au->stage1[ch][2 * i + 1]);
+      VHSD_HIP(c->audio_st[ch].ensure(bins));
+      VHSD_HIP(hipMemcpy(c->audio_st[ch].p, h.data(), sizeof(float2) * bins, hipMemcpyHostToDevice));
+      c->audio_low[ch] = au->low_bin[ch];
+      c->audio_low_freq[ch] = au->low_frequency_hz[ch];
+    }
+    c->audio_bins = bins;
+    c->audio_log2 = lg;
+    c->audio_dec = dec;
+    c->audio_rate = au->slice_rate_hz;
+    c->has_audio = true;
+  }
+  return VHSDECODE_HIP_OK;
+}
+
+}  // namespace
+
 int32_t vhsdecode_hip_demod_create(const vhsdecode_hip_demod_config_v1* cfg, vhsdecode_hip_demod_context** out) {
   try {
     if (!cfg || !out) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "config and context must not be NULL");
@@ -511,10 +663,15 @@ int32_t vhsdecode_hip_demod_create(const vhsdecode_hip_demod_config_v1* cfg, vhs
       return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "filters missing or struct_size mismatch");
     if (f->block_length != 32768)
       return fail(VHSDECODE_HIP_E_UNSUPPORTED, "this build demodulates 32768-sample blocks");
-    if (!f->rf_video || !f->video || !f->video_lowpass05 || !f->envelope_sos || f->envelope_sections <= 0)
-      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "rf_video, video, video_lowpass05 and envelope_sos are required");
-    if (f->envelope_sections != 1)
+    const vhsdecode_hip_laserdisc_v1* ld = f->laserdisc;
+    if (ld && ld->struct_size != sizeof(vhsdecode_hip_laserdisc_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "laserdisc struct_size mismatch");
+    if (!f->rf_video || !f->video || !f->video_lowpass05)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "rf_video, video and video_lowpass05 are required");
+    if (!ld && (!f->envelope_sos || f->envelope_sections != 1))
       return fail(VHSDECODE_HIP_E_UNSUPPORTED, "the envelope filter must be a single section (first-order low-pass)");
+    if (ld && f->diff_demod_max > 0.0)
+      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "the LaserDisc path has no diff-demod repair");
     if (!(f->sample_rate_hz > 0.0)) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "sample_rate_hz must be positive");
     if (cfg->block_cut < 0 || cfg->block_cut_end < 0 || cfg->block_cut + cfg->block_cut_end >= f->block_length ||
         (cfg->block_cut & 1) || (cfg->block_cut_end & 1))
@@ -554,14 +711,25 @@ int32_t vhsdecode_hip_demod_create(const vhsdecode_hip_demod_config_v1* cfg, vhs
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
       return bail(fail(VHSDECODE_HIP_E_HIP, "hipStreamCreate failed"));
     int32_t rc;
-    if ((rc = upload_half(c->rfv, f->rf_video, c->n)) != 0) return bail(rc);
-    if ((rc = upload_half(c->vid, f->video, c->n)) != 0) return bail(rc);
-    if ((rc = upload_half(c->vlp, f->video_lowpass05, c->n)) != 0) return bail(rc);
+    if (ld) {
+      if ((rc = setup_laserdisc(c, f, ld)) != 0) return bail(rc);
+    } else {
+      if ((rc = upload_half(c->rfv, f->rf_video, c->n)) != 0) return bail(rc);
+      if ((rc = upload_half(c->vid, f->video, c->n)) != 0) return bail(rc);
+      if ((rc = upload_half(c->vlp, f->video_lowpass05, c->n)) != 0) return bail(rc);
+    }
     std::vector<float2> oct = octant_table();
     if (c->oct.ensure(oct.size()) != hipSuccess ||
         hipMemcpy(c->oct.p, oct.data(), sizeof(float2) * oct.size(), hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(VHSDECODE_HIP_E_HIP, "twiddle upload failed"));
     std::vector<float> tr;
+    if (ld) {
+      if (c->block_start.ensure(c->max_blocks) != hipSuccess || c->out_offset.ensure(c->max_blocks) != hipSuccess ||
+          c->flags.ensure(c->max_blocks) != hipSuccess)
+        return bail(fail(VHSDECODE_HIP_E_OUT_OF_MEMORY, "descriptor allocation failed"));
+      *out = c;
+      return VHSDECODE_HIP_OK;
+    }
     if ((rc = prepare_sos(f->envelope_sos, f->envelope_sections, c->n, &c->env, &tr)) != 0) return bail(rc);
     if (c->env_trans.ensure(tr.size()) != hipSuccess ||
         hipMemcpy(c->env_trans.p, tr.data(), sizeof(float) * tr.size(), hipMemcpyHostToDevice) != hipSuccess)
@@ -666,7 +834,8 @@ int32_t vhsdecode_hip_demod_span_host(vhsdecode_hip_demod_context* c, const int1
     const int64_t in_count = need - first_sample;
     const size_t plane = (size_t)nblocks * c->stride;
     VHSD_HIP(c->s_rf.ensure((size_t)in_count));
-    VHSD_HIP(c->s_planes.ensure(plane * 4));
+    const size_t audio_plane = c->has_audio ? plane / c->audio_dec : 0;
+    VHSD_HIP(c->s_planes.ensure(plane * 5 + 2 * audio_plane));
     VHSD_HIP(c->s_flags.ensure((size_t)nblocks));
     hipStream_t st = c->stream;
     VHSD_HIP(hipMemcpyAsync(c->s_rf.p, rf + first_sample, sizeof(int16_t) * in_count, hipMemcpyHostToDevice, st));
@@ -677,6 +846,11 @@ int32_t vhsdecode_hip_demod_span_host(vhsdecode_hip_demod_context* c, const int1
     dev.envelope = c->s_planes.p + 2 * plane;
     dev.chroma = out->chroma ? c->s_planes.p + 3 * plane : nullptr;
     dev.block_flags = c->s_flags.p;
+    dev.pilot = out->pilot ? c->s_planes.p + 4 * plane : nullptr;
+    if (c->has_audio && out->audio_left && out->audio_right) {
+      dev.audio_left = c->s_planes.p + 5 * plane;
+      dev.audio_right = dev.audio_left + audio_plane;
+    }
     int32_t rc = demod_span_locked(c, c->s_rf.p, 0, nblocks, &dev, st);
     if (rc != 0) return rc;
     if (out->video)
@@ -689,6 +863,12 @@ int32_t vhsdecode_hip_demod_span_host(vhsdecode_hip_demod_context* c, const int1
       VHSD_HIP(hipMemcpyAsync(out->chroma, dev.chroma, sizeof(float) * plane, hipMemcpyDeviceToHost, st));
     if (out->block_flags)
       VHSD_HIP(hipMemcpyAsync(out->block_flags, dev.block_flags, sizeof(unsigned) * nblocks, hipMemcpyDeviceToHost, st));
+    if (out->pilot && dev.pilot)
+      VHSD_HIP(hipMemcpyAsync(out->pilot, dev.pilot, sizeof(float) * plane, hipMemcpyDeviceToHost, st));
+    if (dev.audio_left) {
+      VHSD_HIP(hipMemcpyAsync(out->audio_left, dev.audio_left, sizeof(float) * audio_plane, hipMemcpyDeviceToHost, st));
+      VHSD_HIP(hipMemcpyAsync(out->audio_right, dev.audio_right, sizeof(float) * audio_plane, hipMemcpyDeviceToHost, st));
+    }
     VHSD_HIP(hipStreamSynchronize(st));
     return VHSDECODE_HIP_OK;
   } catch (const std::exception& e) {

@@ -39,6 +39,10 @@ REQUIRED_EXPORTS = (
     "vhsdecode_hip_set_profiling",
     "vhsdecode_hip_get_profile",
     "vhsdecode_hip_debug_fft16k",
+    "vhsdecode_hip_hifi_create",
+    "vhsdecode_hip_hifi_destroy",
+    "vhsdecode_hip_hifi_demodulate_device",
+    "vhsdecode_hip_hifi_demodulate_host",
 )
 
 KERNEL_NAMES = ("block_demod", "spike_repair", "chroma", "pulses", "hsync", "tbc", "sub_deemphasis", "block_video",
@@ -67,6 +71,22 @@ class SubDeemphV1(ctypes.Structure):
                 ("logistic_rate", ctypes.c_double), ("static_factor", ctypes.c_double)]
 
 
+class HifiConfigV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("device_id", ctypes.c_int32), ("block_length", ctypes.c_int64),
+                ("sample_rate_hz", ctypes.c_int32), ("carrier_hz", ctypes.c_double), ("deviation_hz", ctypes.c_double)]
+
+
+class LdAudioV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("bin_count", ctypes.c_int32), ("low_bin", ctypes.c_int32 * 2),
+                ("stage1", _dp * 2), ("slice_rate_hz", ctypes.c_double), ("low_frequency_hz", ctypes.c_double * 2)]
+
+
+class LaserDiscV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("rf_mtf", _dp), ("clip_low_hz", ctypes.c_double),
+                ("clip_high_hz", ctypes.c_double), ("video_burst", _dp), ("video_burst_offset", ctypes.c_int32),
+                ("video_pilot", _dp), ("analog_audio", ctypes.POINTER(LdAudioV1))]
+
+
 class FiltersV1(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("block_length", ctypes.c_int32),
                 ("sample_rate_hz", ctypes.c_double), ("rf_video", _dp), ("video", _dp),
@@ -74,7 +94,8 @@ class FiltersV1(ctypes.Structure):
                 ("envelope_sections", ctypes.c_int32), ("chroma_sos", _dp), ("chroma_sections", ctypes.c_int32),
                 ("chroma_offset", ctypes.c_int32), ("diff_demod_max", ctypes.c_double),
                 ("plane_bias_hz", ctypes.c_double), ("sub_deemphasis", ctypes.POINTER(SubDeemphV1)),
-                ("rf_boost_sos", _dp), ("rf_boost_multiplier", ctypes.c_double), ("video_notch", _dp)]
+                ("rf_boost_sos", _dp), ("rf_boost_multiplier", ctypes.c_double), ("video_notch", _dp),
+                ("laserdisc", ctypes.POINTER(LaserDiscV1))]
 
 
 class DemodConfigV1(ctypes.Structure):
@@ -85,7 +106,8 @@ class DemodConfigV1(ctypes.Structure):
 
 class SpanV1(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("video", _fp), ("video_lowpass", _fp), ("envelope", _fp),
-                ("chroma", _fp), ("block_flags", _u32p)]
+                ("chroma", _fp), ("block_flags", _u32p), ("pilot", _fp), ("audio_left", _fp),
+                ("audio_right", _fp)]
 
 
 class HsyncParamsV1(ctypes.Structure):
@@ -215,6 +237,12 @@ def load():
     L.vhsdecode_hip_get_profile.argtypes = [ctypes.c_void_p, ctypes.POINTER(ProfileV1)]
     L.vhsdecode_hip_debug_fft16k.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                              ctypes.c_void_p]
+    L.vhsdecode_hip_hifi_create.argtypes = [ctypes.POINTER(HifiConfigV1), ctypes.POINTER(ctypes.c_void_p)]
+    L.vhsdecode_hip_hifi_destroy.argtypes = [ctypes.c_void_p]
+    L.vhsdecode_hip_hifi_destroy.restype = None
+    L.vhsdecode_hip_hifi_demodulate_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                       ctypes.c_int32, ctypes.c_void_p]
+    L.vhsdecode_hip_hifi_demodulate_host.argtypes = [ctypes.c_void_p, _fp, _fp, ctypes.c_int32]
     for name in REQUIRED_EXPORTS:
         fn = getattr(L, name)
         if name not in ("vhsdecode_hip_get_last_error", "vhsdecode_hip_demod_destroy"):

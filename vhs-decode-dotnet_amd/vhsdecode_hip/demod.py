@@ -33,9 +33,12 @@ class DecodedSpan:
     video: np.ndarray             # FP32 offsets from `bias` (Hz)
     video_lowpass: np.ndarray     # FP32 offsets from `bias` (Hz)
     envelope: np.ndarray
-    chroma: Optional[np.ndarray]
+    chroma: Optional[np.ndarray]  # LaserDisc: the colour-burst reference (absolute Hz)
     block_flags: np.ndarray
     bias: float = 0.0
+    pilot: Optional[np.ndarray] = None        # LaserDisc PAL pilot reference (absolute Hz)
+    audio_left: Optional[np.ndarray] = None   # LaserDisc analog audio (Hz at the slice rate)
+    audio_right: Optional[np.ndarray] = None
 
     @property
     def video_hz(self) -> np.ndarray:
@@ -54,7 +57,10 @@ class FilterTables:
     """A TapeFilterSet as the C ABI's vhsdecode_hip_filters_v1 (plus its sub-deemphasis
     block), with the arrays it points into kept alive alongside."""
 
-    def __init__(self, filters: TapeFilterSet):
+    def __init__(self, filters):
+        if hasattr(filters, "rf_mtf"):
+            self._init_laserdisc(filters)
+            return
         self._rfv = np.ascontiguousarray(filters.rf_video, dtype=np.complex128).view(np.float64)
         self._vid = np.ascontiguousarray(filters.video, dtype=np.complex128).view(np.float64)
         self._vlp = np.ascontiguousarray(filters.video_lowpass05, dtype=np.complex128).view(np.float64)
@@ -96,6 +102,52 @@ class FilterTables:
         if nt is not None:
             self._notch = np.ascontiguousarray(np.concatenate([nt[0], nt[1]]), dtype=np.float64)
             f.video_notch = _ptr(self._notch, _lib._dp)
+        self.struct = f
+
+    def _init_laserdisc(self, filters) -> None:
+        """A LaserDiscFilterSet (ld_filters.build_laserdisc_filters) as filters_v1 with its
+        laserdisc_v1 and ld_audio_v1 blocks."""
+        def c(a):
+            return None if a is None else np.ascontiguousarray(a, dtype=np.complex128).view(np.float64)
+
+        self._rfv, self._vid, self._vlp = c(filters.rf_video), c(filters.video), c(filters.video_lowpass05)
+        self._mtf, self._burst, self._pilot = c(filters.rf_mtf), c(filters.video_burst), c(filters.video_pilot)
+        f = _lib.FiltersV1()
+        f.struct_size = ctypes.sizeof(_lib.FiltersV1)
+        f.block_length = int(filters.block_length)
+        f.sample_rate_hz = float(filters.sample_rate_hz)
+        f.rf_video = _ptr(self._rfv, _lib._dp)
+        f.video = _ptr(self._vid, _lib._dp)
+        f.video_lowpass05 = _ptr(self._vlp, _lib._dp)
+        f.video_lowpass05_offset = int(filters.video_lowpass05_offset)
+        f.diff_demod_max = -1.0
+        f.plane_bias_hz = float(filters.plane_bias_hz)
+        ld = _lib.LaserDiscV1()
+        ld.struct_size = ctypes.sizeof(_lib.LaserDiscV1)
+        ld.rf_mtf = _ptr(self._mtf, _lib._dp)
+        ld.clip_low_hz = float(filters.clip_low_hz)
+        ld.clip_high_hz = float(filters.clip_high_hz)
+        ld.video_burst = _ptr(self._burst, _lib._dp) if self._burst is not None else None
+        ld.video_burst_offset = int(filters.video_burst_offset)
+        ld.video_pilot = _ptr(self._pilot, _lib._dp) if self._pilot is not None else None
+        if filters.audio_left is not None:
+            self._st1 = [c(filters.audio_left.stage1), c(filters.audio_right.stage1)]
+            au = _lib.LdAudioV1()
+            au.struct_size = ctypes.sizeof(_lib.LdAudioV1)
+            au.bin_count = int(filters.audio_left.bin_count)
+            if filters.audio_right.bin_count != au.bin_count:
+                raise ValueError("LD analog audio channel lengths did not match.")
+            au.low_bin[0] = int(filters.audio_left.low_bin)
+            au.low_bin[1] = int(filters.audio_right.low_bin)
+            au.stage1[0] = _ptr(self._st1[0], _lib._dp)
+            au.stage1[1] = _ptr(self._st1[1], _lib._dp)
+            au.slice_rate_hz = float(filters.audio_left.slice_rate_hz)
+            au.low_frequency_hz[0] = float(filters.audio_left.low_frequency_hz)
+            au.low_frequency_hz[1] = float(filters.audio_right.low_frequency_hz)
+            self._audio = au
+            ld.analog_audio = ctypes.pointer(au)
+        self._ld = ld
+        f.laserdisc = ctypes.pointer(ld)
         self.struct = f
 
 
@@ -156,7 +208,13 @@ class HipBlockDemodulator:
             nblocks = self.blocks_for(rf.size, first_sample)
         plane = nblocks * self.stride
         out = {k: np.empty(plane, dtype=np.float32) for k in ("video", "vlp", "env")}
-        ch = np.empty(plane, dtype=np.float32) if (chroma and self.filters.chroma_sos is not None) else None
+        ld = hasattr(self.filters, "rf_mtf")
+        has_chroma = (self.filters.video_burst is not None) if ld else (self.filters.chroma_sos is not None)
+        ch = np.empty(plane, dtype=np.float32) if (chroma and has_chroma) else None
+        pilot = np.empty(plane, dtype=np.float32) if ld and self.filters.video_pilot is not None else None
+        aud = None
+        if ld and self.filters.audio_left is not None:
+            aud = [np.empty(plane // self.filters.audio_decimation, dtype=np.float32) for _ in range(2)]
         flags = np.zeros(nblocks, dtype=np.uint32)
         span = _lib.SpanV1()
         span.struct_size = ctypes.sizeof(_lib.SpanV1)
@@ -165,9 +223,14 @@ class HipBlockDemodulator:
         span.envelope = _ptr(out["env"], _lib._fp)
         span.chroma = _ptr(ch, _lib._fp) if ch is not None else None
         span.block_flags = _ptr(flags, _lib._u32p)
+        span.pilot = _ptr(pilot, _lib._fp) if pilot is not None else None
+        if aud is not None:
+            span.audio_left = _ptr(aud[0], _lib._fp)
+            span.audio_right = _ptr(aud[1], _lib._fp)
         _lib.check(self._L.vhsdecode_hip_demod_span_host(self._h, _ptr(rf, _lib._i16p), rf.size, first_sample,
                                                          nblocks, ctypes.byref(span)))
-        return DecodedSpan(out["video"], out["vlp"], out["env"], ch, flags, self.bias)
+        return DecodedSpan(out["video"], out["vlp"], out["env"], ch, flags, self.bias, pilot,
+                           aud[0] if aud else None, aud[1] if aud else None)
 
     def demod_span_device(self, rf_ptr: int, first_sample: int, nblocks: int, video_ptr: int, vlp_ptr: int,
                           env_ptr: int, chroma_ptr: int = 0, flags_ptr: int = 0, stream: int = 0):

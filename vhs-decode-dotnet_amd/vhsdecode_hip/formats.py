@@ -56,3 +56,23 @@ def get_format(system: str = "PAL", tape_format: str = "VHS", tape_speed: str = 
         if (case["system"], case["tape_format"], case["tape_speed"]) == (system, tape_format, tape_speed):
             return FormatParameters(system, tape_format, tape_speed, case["sysparams"], case["rfparams"])
     raise KeyError(f"no format parameters for {system}/{tape_format}/{tape_speed}")
+
+
+def get_laserdisc_format(system: str = "NTSC", lowband: bool = False, deemp_low_mhz: float = 0.0,
+                         deemp_high_mhz: float = 0.0, deemp_strength: float = 1.0) -> FormatParameters:
+    """FormatCatalog.GetLaserDiscParameters (src/VHSDecode.Core/Formats/FormatCatalog.cs:95-112)
+    with the ld command's de-emphasis overrides (Decode/DecodeSession.cs:1350-1372):
+    --deemp_low / --deemp_high replace the time constants, --deemp_strength is always set."""
+    for case in _catalogue()["ld_cases"]:
+        if case["system"] == system and bool(case["lowband"]) == bool(lowband):
+            rf = dict(case["rfparams"])
+            if deemp_low_mhz > 0 or deemp_high_mhz > 0:
+                high_tc, low_tc = rf["video_deemp"][0], rf["video_deemp"][1]
+                if deemp_low_mhz > 0:
+                    low_tc = 1.0 / (deemp_low_mhz * 1_000_000.0)
+                if deemp_high_mhz > 0:
+                    high_tc = 1.0 / (deemp_high_mhz * 1_000_000.0)
+                rf["video_deemp"] = [high_tc, low_tc]
+            rf["video_deemp_strength"] = deemp_strength
+            return FormatParameters(system, "LD", "", dict(case["sysparams"]), rf)
+    raise KeyError(f"no LaserDisc parameters for {system} (lowband={lowband})")

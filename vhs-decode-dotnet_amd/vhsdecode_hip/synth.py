@@ -31,6 +31,12 @@ class SyntheticTape:
     lines_per_frame: int
     samples_per_line: float
     fmt: FormatParameters
+    audio_carriers: tuple = ()     # (carrier Hz, deviation Hz, tone Hz, level) of each FM audio carrier
+
+    def audio_frequency(self, index: int, sample_index: np.ndarray) -> np.ndarray:
+        """Closed-form instantaneous frequency (Hz) of audio carrier `index` at the given samples."""
+        fc, dev, tone, _ = self.audio_carriers[index]
+        return fc + dev * np.sin(2.0 * np.pi * tone * np.asarray(sample_index, dtype=np.float64) / self.sample_rate_hz)
 
 
 def _line_ire(kind: str, x_us: np.ndarray, line_us: float, p: dict, line_no: int, is_pal: bool) -> np.ndarray:
@@ -112,7 +118,11 @@ def _frame_plan(is_pal: bool) -> List[str]:
 
 def generate(fmt: FormatParameters, fields: int, sample_rate_hz: float = 40e6, snr_db: float = 30.0,
              seed: int = 1, amplitude: float = 8000.0, jitter_ns: float = 0.0, lead_in_lines: int = 20,
-             lead_out_lines: int = 20, chroma_under_level: float = 0.12) -> SyntheticTape:
+             lead_out_lines: int = 20, chroma_under_level: float = 0.12,
+             audio_carriers: tuple = ()) -> SyntheticTape:
+    """`audio_carriers`: (carrier Hz, deviation Hz, tone Hz, level relative to the video
+    carrier) per FM audio carrier, e.g. the LaserDisc analog sound carriers near 2.3 and
+    2.8 MHz, added with their phase accumulated in FP64."""
     p = dict(fmt.sysparams)
     is_pal = fmt.parent_system == "PAL"
     line_us = float(p["line_period"])
@@ -149,6 +159,9 @@ def generate(fmt: FormatParameters, fields: int, sample_rate_hz: float = 40e6, s
     cu = fmt.rfparams.get("color_under_carrier")
     if cu and chroma_under_level > 0:
         rf += chroma_under_level * amplitude * np.sin(2.0 * np.pi * float(cu) * sample_idx / sample_rate_hz)
+    for fc, dev, tone, level in audio_carriers:
+        inst = fc + dev * np.sin(2.0 * np.pi * tone * sample_idx / sample_rate_hz)
+        rf += level * amplitude * np.cos(2.0 * np.pi * np.cumsum(inst) / sample_rate_hz)
     sigma = (amplitude / np.sqrt(2.0)) / (10.0 ** (snr_db / 20.0))
     rf += rng.normal(0.0, sigma, n)
     rf = np.clip(np.rint(rf), -32768, 32767).astype(np.int16)
@@ -158,4 +171,5 @@ def generate(fmt: FormatParameters, fields: int, sample_rate_hz: float = 40e6, s
     for f in range(fields):
         field_starts.append(acc)
         acc += field_lines[f % 2]
-    return SyntheticTape(rf, sample_rate_hz, starts, np.array(field_starts), frame_lines, spl, fmt)
+    return SyntheticTape(rf, sample_rate_hz, starts, np.array(field_starts), frame_lines, spl, fmt,
+                         tuple(tuple(float(v) for v in c) for c in audio_carriers))
