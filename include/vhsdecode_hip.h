@@ -227,6 +227,46 @@ VHSDECODE_HIP_API int32_t vhsdecode_hip_hifi_demodulate_device(vhsdecode_hip_hif
 VHSDECODE_HIP_API int32_t vhsdecode_hip_hifi_demodulate_host(vhsdecode_hip_hifi* hifi, const float* input,
                                                              float* output, int32_t nblocks);
 
+/* ---------------------------------------------------------------- batched segments
+ * Many RF segments of mixed formats and sample rates (one demod context per format
+ * and rate) decoded by one call.  Each segment is read as RfBlockStreamDecoder reads a
+ * capture (Decode/RfBlockStreamDecoder.cs:419-700): whole blocks at the context's
+ * stride from rf_offset while a whole block fits in `length` (a segment shorter than a
+ * block yields nothing), each block's kept window stitched at out_offset + b * stride
+ * of the shared span planes.  The blocks of a context form one launch of the block
+ * chain; the launches of different contexts run concurrently, and with use_graph the
+ * whole batch is captured once into a HIP graph and replayed while the rf and span
+ * pointers stay the same.  The batch keeps its own descriptors and per-block scratch;
+ * it must not outlive its contexts. */
+typedef struct vhsdecode_hip_batch vhsdecode_hip_batch;
+
+typedef struct vhsdecode_hip_batch_segment_v1 {
+  uint32_t struct_size;
+  vhsdecode_hip_demod_context* context;  /* decode format and sample rate */
+  int64_t rf_offset;               /* first sample of the segment in the packed RF buffer (even) */
+  int64_t length;                  /* samples in the segment */
+  int64_t out_offset;              /* first kept sample of the segment in the span planes; output ranges must
+                                      not overlap (LaserDisc audio: a multiple of the decimation) */
+} vhsdecode_hip_batch_segment_v1;
+
+typedef struct vhsdecode_hip_batch_config_v1 {
+  uint32_t struct_size;
+  const vhsdecode_hip_batch_segment_v1* segments;
+  int32_t count;
+  int32_t use_graph;               /* 1: capture the batch into a graph on the first run and replay it */
+} vhsdecode_hip_batch_config_v1;
+
+VHSDECODE_HIP_API int32_t vhsdecode_hip_batch_create(const vhsdecode_hip_batch_config_v1* config,
+                                                     vhsdecode_hip_batch** batch);
+/* Blocks decoded per run, the span plane length the segments need and the RF samples they read. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_batch_info(const vhsdecode_hip_batch* batch, int64_t* blocks,
+                                                   int64_t* output_samples, int64_t* rf_samples);
+/* rf and every span plane are device memory; asynchronous on `stream` (NULL: the batch's stream). */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_batch_run_device(vhsdecode_hip_batch* batch, const int16_t* rf,
+                                                         int64_t rf_count, const vhsdecode_hip_span_v1* out,
+                                                         void* stream);
+VHSDECODE_HIP_API void vhsdecode_hip_batch_destroy(vhsdecode_hip_batch* batch);
+
 /* ---------------------------------------------------------------- sync pulses
  * PulseDetection.FindPulses (src/VHSDecode.Core/Dsp/PulseDetection.cs:45-222) with the
  * VHS bounds of TbcFieldDecodePipeline.FindRawPulses (Decode/TbcFieldDecodePipeline.cs:2375-2384).

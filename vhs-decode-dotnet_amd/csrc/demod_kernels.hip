@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "fft16k.hpp"
 #include "sos_scan.hpp"
 #include "vhsd_args.hpp"
@@ -938,6 +940,13 @@ __global__ void __launch_bounds__(1024) vhs_block_repair_kernel(RepairArgs r) {
   }
 }
 
+// Fills n words (a kernel rather than hipMemsetAsync, so the chain can be captured
+// into a graph and replayed).
+__global__ void __launch_bounds__(256) fill_u32_kernel(unsigned* p, long long n, unsigned value) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
 // ------------------------------------------------------------------ diagnostics
 __global__ void __launch_bounds__(1024) debug_fft_kernel(float2* data, int inverse, const float2* oct_g) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -1013,6 +1022,13 @@ hipError_t vhsd_launch_block_demod_boost(const vhsd::DemodArgs* a, int nblocks, 
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(vhsd::vhs_block_demod_boost_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
+  return hipGetLastError();
+}
+
+hipError_t vhsd_launch_fill_u32(unsigned* p, long long n, unsigned value, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  const int blocks = (int)std::min<long long>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(vhsd::fill_u32_kernel, dim3(blocks), dim3(256), 0, stream, p, n, value);
   return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -427,28 +428,20 @@ int32_t demod_span_ld(vhsdecode_hip_demod_context* c, vhsd::DemodArgs& a, int32_
   return VHSDECODE_HIP_OK;
 }
 
-int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int64_t first_sample, int32_t nblocks,
-                          const vhsdecode_hip_span_v1* out, hipStream_t st) {
-  if (nblocks <= 0) return VHSDECODE_HIP_OK;
-  if (nblocks > c->max_blocks)
-    return fail(VHSDECODE_HIP_E_CAPACITY, "nblocks exceeds the context's max_blocks");
-  if (!rf || !out || !out->video || !out->video_lowpass || !out->envelope)
-    return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "rf and the video, video_lowpass and envelope planes are required");
-  if (first_sample < 0 || (first_sample & 1))
-    return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "first_sample must be even and non-negative");
-  std::vector<long long> starts(nblocks), offs(nblocks);
-  for (int b = 0; b < nblocks; b++) {
-    starts[b] = first_sample + (long long)b * c->stride;
-    offs[b] = (long long)b * c->stride;
-  }
-  VHSD_HIP(hipMemcpyAsync(c->block_start.p, starts.data(), sizeof(long long) * nblocks, hipMemcpyHostToDevice, st));
-  VHSD_HIP(hipMemcpyAsync(c->out_offset.p, offs.data(), sizeof(long long) * nblocks, hipMemcpyHostToDevice, st));
-  unsigned* flags = out->block_flags ? out->block_flags : c->flags.p;
-  VHSD_HIP(hipMemsetAsync(flags, 0, sizeof(unsigned) * nblocks, st));
+// The block chain of one context over nblocks blocks whose input offsets (into rf) and
+// stitched output offsets (into the span planes) are the device arrays starts/offs.
+// Only kernel launches are enqueued (no copies or memsets), so a caller may capture
+// the call into a graph.  stash (nblocks * kStashEntries) and, when the context needs
+// it, aux (nblocks * kAuxEntries) are the per-block scratch.
+int32_t demod_blocks_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, const long long* starts,
+                            const long long* offs, int32_t nblocks, const vhsdecode_hip_span_v1* out,
+                            unsigned* flags, float2* stash, float2* aux, float2* rep_a, float* rep_dm,
+                            hipStream_t st) {
+  VHSD_HIP(vhsd_launch_fill_u32(flags, nblocks, 0u, st));
   vhsd::DemodArgs a{};
   a.rf = rf;
-  a.block_start = c->block_start.p;
-  a.out_offset = c->out_offset.p;
+  a.block_start = starts;
+  a.out_offset = offs;
   a.out_begin = c->cut;
   a.out_count = c->stride;
   a.rf_video = c->rfv.p;
@@ -467,10 +460,8 @@ int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int
   a.out_vlp = out->video_lowpass;
   a.out_env = out->envelope;
   a.flags = flags;
-  VHSD_HIP(c->stash.ensure((size_t)nblocks * vhsd::kStashEntries));
-  a.stash = c->stash.p;
-  if (c->has_boost || c->has_notch || c->has_audio) VHSD_HIP(c->aux.ensure((size_t)nblocks * vhsd::kAuxEntries));
-  a.aux = c->aux.p;
+  a.stash = stash;
+  a.aux = aux;
   a.stash_sub = a.bias;
   if (c->ld) return demod_span_ld(c, a, nblocks, out, st);
   if (c->has_boost) {
@@ -486,9 +477,7 @@ int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int
       VHSD_HIP(vhsd_launch_block_demod(&a, nblocks, st));
   }
   if (c->diff_max > 0.0) {
-    VHSD_HIP(c->rep_a.ensure((size_t)nblocks * 32768));
-    VHSD_HIP(c->rep_dm.ensure((size_t)nblocks * 65536));
-    vhsd::RepairArgs r{a, c->rep_a.p, c->rep_dm.p};
+    vhsd::RepairArgs r{a, rep_a, rep_dm};
     ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_SPIKE_REPAIR, nblocks);
     VHSD_HIP(vhsd_launch_block_repair(&r, nblocks, st));
   }
@@ -498,12 +487,12 @@ int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int
   }
   if (c->has_subdeemph) {
     vhsd::SubDeemphArgs sd{};
-    sd.out_offset = c->out_offset.p;
+    sd.out_offset = offs;
     sd.out_begin = c->cut;
     sd.out_count = c->stride;
     sd.vh = c->sd_vh.p;
     sd.oct = c->oct.p;
-    sd.stash = c->stash.p;
+    sd.stash = stash;
     sd.amp = c->sd_amp;
     sd.amp.trans = c->sd_trans.p;
     sd.half_deviation = c->sd_params[0];
@@ -521,15 +510,15 @@ int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int
     }
     sd.video = c->vid.p;
     sd.video_corr = c->video_corr;
-    sd.aux = c->aux.p;
+    sd.aux = aux;
     ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_SUB_DEEMPHASIS, nblocks);
     VHSD_HIP(vhsd_launch_block_subdeemph(&sd, nblocks, st));
   }
   if (c->has_chroma && out->chroma) {
     vhsd::ChromaArgs ch{};
     ch.rf = rf;
-    ch.block_start = c->block_start.p;
-    ch.out_offset = c->out_offset.p;
+    ch.block_start = starts;
+    ch.out_offset = offs;
     ch.out_begin = c->cut;
     ch.out_count = c->stride;
     ch.chroma_offset = c->chroma_offset;
@@ -540,6 +529,40 @@ int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int
     VHSD_HIP(vhsd_launch_block_chroma(&ch, nblocks, st));
   }
   return VHSDECODE_HIP_OK;
+}
+
+// Per-block scratch of the chain for nblocks blocks (allocates outside any capture).
+int32_t reserve_block_scratch(vhsdecode_hip_demod_context* c, int32_t nblocks) {
+  VHSD_HIP(c->stash.ensure((size_t)nblocks * vhsd::kStashEntries));
+  if (c->has_boost || c->has_notch || c->has_audio) VHSD_HIP(c->aux.ensure((size_t)nblocks * vhsd::kAuxEntries));
+  if (c->diff_max > 0.0) {
+    VHSD_HIP(c->rep_a.ensure((size_t)nblocks * 32768));
+    VHSD_HIP(c->rep_dm.ensure((size_t)nblocks * 65536));
+  }
+  return VHSDECODE_HIP_OK;
+}
+
+int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int64_t first_sample, int32_t nblocks,
+                          const vhsdecode_hip_span_v1* out, hipStream_t st) {
+  if (nblocks <= 0) return VHSDECODE_HIP_OK;
+  if (nblocks > c->max_blocks)
+    return fail(VHSDECODE_HIP_E_CAPACITY, "nblocks exceeds the context's max_blocks");
+  if (!rf || !out || !out->video || !out->video_lowpass || !out->envelope)
+    return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "rf and the video, video_lowpass and envelope planes are required");
+  if (first_sample < 0 || (first_sample & 1))
+    return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "first_sample must be even and non-negative");
+  std::vector<long long> starts(nblocks), offs(nblocks);
+  for (int b = 0; b < nblocks; b++) {
+    starts[b] = first_sample + (long long)b * c->stride;
+    offs[b] = (long long)b * c->stride;
+  }
+  int32_t rc = reserve_block_scratch(c, nblocks);
+  if (rc != VHSDECODE_HIP_OK) return rc;
+  VHSD_HIP(hipMemcpyAsync(c->block_start.p, starts.data(), sizeof(long long) * nblocks, hipMemcpyHostToDevice, st));
+  VHSD_HIP(hipMemcpyAsync(c->out_offset.p, offs.data(), sizeof(long long) * nblocks, hipMemcpyHostToDevice, st));
+  unsigned* flags = out->block_flags ? out->block_flags : c->flags.p;
+  return demod_blocks_locked(c, rf, c->block_start.p, c->out_offset.p, nblocks, out, flags, c->stash.p, c->aux.p,
+                             c->rep_a.p, c->rep_dm.p, st);
 }
 
 }  // namespace
@@ -818,6 +841,217 @@ int32_t vhsdecode_hip_demod_span_device(vhsdecode_hip_demod_context* c, const in
   } catch (const std::exception& e) {
     return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
   }
+}
+
+// ---------------------------------------------------------------- batched segments
+// A batch groups the whole blocks of many segments by context (decode format and
+// sample rate); each group is one launch of the block chain over its blocks, whose
+// input and output offsets come from the group's device descriptor arrays.  The
+// groups run on branch streams forked from the caller's stream, and the whole fork
+// is captured once into a graph and replayed while the RF and plane pointers stay
+// the same.
+struct vhsdecode_hip_batch {
+  struct Group {
+    vhsdecode_hip_demod_context* c = nullptr;
+    int nblocks = 0;
+    std::vector<long long> h_starts, h_offs;
+    DevBuf<long long> starts, offs;
+    DevBuf<unsigned> flags;
+    DevBuf<float2> stash, aux, rep_a;
+    DevBuf<float> rep_dm;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+  };
+  std::mutex mu;
+  int device = 0;
+  std::vector<Group> groups;
+  long long blocks = 0, out_end = 0, rf_end = 0, audio_div = 1;
+  bool use_graph = false;
+  hipStream_t capture = nullptr;
+  hipEvent_t fork = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void* key_rf = nullptr;
+  vhsdecode_hip_span_v1 key_out{};
+
+  ~vhsdecode_hip_batch() {
+    (void)hipSetDevice(device);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto& g : groups) {
+      g.starts.release(); g.offs.release(); g.flags.release(); g.stash.release(); g.aux.release();
+      g.rep_a.release(); g.rep_dm.release();
+      if (g.done) (void)hipEventDestroy(g.done);
+      if (g.stream) (void)hipStreamDestroy(g.stream);
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    if (capture) (void)hipStreamDestroy(capture);
+  }
+};
+
+namespace {
+
+// Enqueue every group on its branch stream, forked from and joined into `st`.
+int32_t batch_enqueue(vhsdecode_hip_batch* b, const int16_t* rf, const vhsdecode_hip_span_v1* out, hipStream_t st) {
+  VHSD_HIP(hipEventRecord(b->fork, st));
+  for (auto& g : b->groups) {
+    if (g.nblocks == 0) continue;
+    VHSD_HIP(hipStreamWaitEvent(g.stream, b->fork, 0));
+    const bool prof = g.c->profiling;
+    g.c->profiling = false;
+    const int32_t rc = demod_blocks_locked(g.c, rf, g.starts.p, g.offs.p, g.nblocks, out, g.flags.p, g.stash.p,
+                                           g.aux.p, g.rep_a.p, g.rep_dm.p, g.stream);
+    g.c->profiling = prof;
+    if (rc != VHSDECODE_HIP_OK) return rc;
+    VHSD_HIP(hipEventRecord(g.done, g.stream));
+    VHSD_HIP(hipStreamWaitEvent(st, g.done, 0));
+  }
+  return VHSDECODE_HIP_OK;
+}
+
+bool same_span(const vhsdecode_hip_span_v1& a, const vhsdecode_hip_span_v1& b) {
+  return a.video == b.video && a.video_lowpass == b.video_lowpass && a.envelope == b.envelope &&
+         a.chroma == b.chroma && a.pilot == b.pilot && a.audio_left == b.audio_left && a.audio_right == b.audio_right;
+}
+
+}  // namespace
+
+int32_t vhsdecode_hip_batch_create(const vhsdecode_hip_batch_config_v1* cfg, vhsdecode_hip_batch** out) {
+  try {
+    if (!cfg || !out) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "config and batch must not be NULL");
+    *out = nullptr;
+    if (cfg->struct_size != sizeof(vhsdecode_hip_batch_config_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "batch_config struct_size mismatch");
+    if (cfg->count < 0 || (cfg->count > 0 && !cfg->segments))
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "segments missing");
+    auto b = std::make_unique<vhsdecode_hip_batch>();
+    b->use_graph = cfg->use_graph != 0;
+    std::vector<std::pair<long long, long long>> ranges;
+    for (int i = 0; i < cfg->count; i++) {
+      const vhsdecode_hip_batch_segment_v1& sg = cfg->segments[i];
+      if (sg.struct_size != sizeof(vhsdecode_hip_batch_segment_v1))
+        return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "batch_segment struct_size mismatch");
+      vhsdecode_hip_demod_context* c = sg.context;
+      if (!c || sg.rf_offset < 0 || (sg.rf_offset & 1) || sg.length < 0 || sg.out_offset < 0)
+        return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "segment needs a context, an even rf_offset and non-negative sizes");
+      if (i > 0 && c->device != b->device) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "contexts on different devices");
+      b->device = c->device;
+      // whole blocks only, as RfBlockStreamDecoder reads them (a short read ends the stream)
+      const long long nb = sg.length >= c->n ? (sg.length - c->n) / c->stride + 1 : 0;
+      if (nb == 0) continue;
+      if (c->has_audio && (sg.out_offset % c->audio_dec))
+        return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "LaserDisc audio needs out_offset divisible by the decimation");
+      size_t gi = 0;
+      while (gi < b->groups.size() && b->groups[gi].c != c) gi++;
+      if (gi == b->groups.size()) {
+        b->groups.emplace_back();
+        b->groups.back().c = c;
+      }
+      auto& g = b->groups[gi];
+      for (long long k = 0; k < nb; k++) {
+        g.h_starts.push_back(sg.rf_offset + k * c->stride);
+        g.h_offs.push_back(sg.out_offset + k * c->stride);
+      }
+      ranges.emplace_back(sg.out_offset, sg.out_offset + nb * c->stride);
+      b->out_end = std::max(b->out_end, sg.out_offset + nb * c->stride);
+      b->rf_end = std::max(b->rf_end, sg.rf_offset + (nb - 1) * c->stride + c->n);
+      b->blocks += nb;
+    }
+    std::sort(ranges.begin(), ranges.end());
+    for (size_t i = 1; i < ranges.size(); i++)
+      if (ranges[i].first < ranges[i - 1].second)
+        return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "segment output ranges overlap");
+    VHSD_HIP(hipSetDevice(b->device));
+    VHSD_HIP(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
+    VHSD_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    for (auto& g : b->groups) {
+      g.nblocks = (int)g.h_starts.size();
+      const size_t nb = (size_t)g.nblocks;
+      VHSD_HIP(g.starts.ensure(nb));
+      VHSD_HIP(g.offs.ensure(nb));
+      VHSD_HIP(g.flags.ensure(nb));
+      VHSD_HIP(g.stash.ensure(nb * vhsd::kStashEntries));
+      if (g.c->has_boost || g.c->has_notch || g.c->has_audio) VHSD_HIP(g.aux.ensure(nb * vhsd::kAuxEntries));
+      if (g.c->diff_max > 0.0) {
+        VHSD_HIP(g.rep_a.ensure(nb * 32768));
+        VHSD_HIP(g.rep_dm.ensure(nb * 65536));
+      }
+      VHSD_HIP(hipMemcpy(g.starts.p, g.h_starts.data(), sizeof(long long) * nb, hipMemcpyHostToDevice));
+      VHSD_HIP(hipMemcpy(g.offs.p, g.h_offs.data(), sizeof(long long) * nb, hipMemcpyHostToDevice));
+      VHSD_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+      VHSD_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+    }
+    *out = b.release();
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_batch_info(const vhsdecode_hip_batch* b, int64_t* blocks, int64_t* output_samples,
+                                 int64_t* rf_samples) {
+  if (!b) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "batch is NULL");
+  if (blocks) *blocks = b->blocks;
+  if (output_samples) *output_samples = b->out_end;
+  if (rf_samples) *rf_samples = b->rf_end;
+  return VHSDECODE_HIP_OK;
+}
+
+int32_t vhsdecode_hip_batch_run_device(vhsdecode_hip_batch* b, const int16_t* rf, int64_t rf_count,
+                                       const vhsdecode_hip_span_v1* out, void* stream) {
+  try {
+    if (!b || !rf || !out) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "batch, rf and span are required");
+    if (out->struct_size != sizeof(vhsdecode_hip_span_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "span struct_size mismatch");
+    if (!out->video || !out->video_lowpass || !out->envelope)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "the video, video_lowpass and envelope planes are required");
+    if (rf_count < b->rf_end) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "rf buffer is shorter than the segments");
+    std::lock_guard<std::mutex> lock(b->mu);
+    VHSD_HIP(hipSetDevice(b->device));
+    // the contexts' own calls are serialized against the enqueue
+    std::vector<vhsdecode_hip_demod_context*> ctxs;
+    for (auto& g : b->groups) ctxs.push_back(g.c);
+    std::sort(ctxs.begin(), ctxs.end());
+    std::vector<std::unique_lock<std::mutex>> locks;
+    for (auto* c : ctxs) locks.emplace_back(c->mu);
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : b->capture;
+    if (!b->use_graph) return batch_enqueue(b, rf, out, st);
+    if (!b->exec || b->key_rf != rf || !same_span(b->key_out, *out)) {
+      if (b->exec) (void)hipGraphExecDestroy(b->exec);
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->exec = nullptr;
+      b->graph = nullptr;
+      VHSD_HIP(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
+      const int32_t rc = batch_enqueue(b, rf, out, b->capture);
+      hipGraph_t graph = nullptr;
+      const hipError_t ce = hipStreamEndCapture(b->capture, &graph);
+      if (rc != VHSDECODE_HIP_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      VHSD_HIP(ce);
+      b->graph = graph;
+      VHSD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      b->key_rf = rf;
+      b->key_out = *out;
+    }
+    VHSD_HIP(hipGraphLaunch(b->exec, st));
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+void vhsdecode_hip_batch_destroy(vhsdecode_hip_batch* b) {
+  if (!b) return;
+  {
+    std::lock_guard<std::mutex> lock(b->mu);
+    (void)hipSetDevice(b->device);
+    for (auto& g : b->groups)
+      if (g.stream) (void)hipStreamSynchronize(g.stream);
+    if (b->capture) (void)hipStreamSynchronize(b->capture);
+  }
+  delete b;
 }
 
 int32_t vhsdecode_hip_demod_span_host(vhsdecode_hip_demod_context* c, const int16_t* rf, int64_t rf_count,

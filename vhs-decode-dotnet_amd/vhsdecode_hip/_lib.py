@@ -39,6 +39,10 @@ REQUIRED_EXPORTS = (
     "vhsdecode_hip_set_profiling",
     "vhsdecode_hip_get_profile",
     "vhsdecode_hip_debug_fft16k",
+    "vhsdecode_hip_batch_create",
+    "vhsdecode_hip_batch_info",
+    "vhsdecode_hip_batch_run_device",
+    "vhsdecode_hip_batch_destroy",
     "vhsdecode_hip_hifi_create",
     "vhsdecode_hip_hifi_destroy",
     "vhsdecode_hip_hifi_demodulate_device",
@@ -69,6 +73,16 @@ class SubDeemphV1(ctypes.Structure):
                 ("deviation", ctypes.c_double), ("exponential_scaling", ctypes.c_double),
                 ("scaling1", ctypes.c_double), ("scaling2", ctypes.c_double), ("logistic_mid", ctypes.c_double),
                 ("logistic_rate", ctypes.c_double), ("static_factor", ctypes.c_double)]
+
+
+class BatchSegmentV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("context", ctypes.c_void_p), ("rf_offset", ctypes.c_int64),
+                ("length", ctypes.c_int64), ("out_offset", ctypes.c_int64)]
+
+
+class BatchConfigV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("segments", ctypes.POINTER(BatchSegmentV1)),
+                ("count", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
 
 
 class HifiConfigV1(ctypes.Structure):
@@ -237,6 +251,13 @@ def load():
     L.vhsdecode_hip_get_profile.argtypes = [ctypes.c_void_p, ctypes.POINTER(ProfileV1)]
     L.vhsdecode_hip_debug_fft16k.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                              ctypes.c_void_p]
+    L.vhsdecode_hip_batch_create.argtypes = [ctypes.POINTER(BatchConfigV1), ctypes.POINTER(ctypes.c_void_p)]
+    L.vhsdecode_hip_batch_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+    L.vhsdecode_hip_batch_run_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                 ctypes.POINTER(SpanV1), ctypes.c_void_p]
+    L.vhsdecode_hip_batch_destroy.argtypes = [ctypes.c_void_p]
+    L.vhsdecode_hip_batch_destroy.restype = None
     L.vhsdecode_hip_hifi_create.argtypes = [ctypes.POINTER(HifiConfigV1), ctypes.POINTER(ctypes.c_void_p)]
     L.vhsdecode_hip_hifi_destroy.argtypes = [ctypes.c_void_p]
     L.vhsdecode_hip_hifi_destroy.restype = None

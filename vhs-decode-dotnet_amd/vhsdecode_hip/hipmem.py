@@ -115,6 +115,11 @@ class DeviceBuffer:
                                D2H))
         return out
 
+    def fill(self, byte_value: int):
+        """Set every byte (synchronous, so work on other streams sees it)."""
+        _check(hip().hipMemsetAsync(ctypes.c_void_p(self.ptr), int(byte_value), self.nbytes, ctypes.c_void_p(0)))
+        synchronize()
+
     def zero(self, stream=None):
         _check(hip().hipMemsetAsync(ctypes.c_void_p(self.ptr), 0, self.nbytes, ctypes.c_void_p(stream or 0)))
 
