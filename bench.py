@@ -212,6 +212,150 @@ def served_run(seconds: float, rate: float, streams: int, window_s: float, seed:
     }
 
 
+BATCH_MIX = [("PAL", "VHS", "sp"), ("NTSC", "VHS", "sp"), ("PAL", "SVHS", "sp"), ("PAL", "BETAMAX", "sp"),
+             ("PAL", "VIDEO8", "sp")]
+
+
+def batched_run(seed: int, segments: int, steps: int):
+    """Uneven segments (seeded lengths 16384..262144) of PAL/NTSC VHS, PAL S-VHS, PAL
+    Betamax and PAL Video8 at 20 and 40 MSPS, 30 dB: the graph-captured batch
+    (vhsdecode_hip_batch_*) against the plain path, one demod_span_device call per
+    segment on one stream.  Both decode exactly the segments' whole blocks."""
+    from vhsdecode_hip import filters, formats, hipmem, synth
+    from vhsdecode_hip.batch import HipBatch, pack_segments
+    from vhsdecode_hip.demod import HipBlockDemodulator
+    rng = np.random.default_rng(seed)
+    lengths = rng.integers(16384, 262145, size=segments)
+    kinds = [(m, fs) for m in BATCH_MIX for fs in (20e6, 40e6)]
+    dems, tapes = [], []
+    for i, ((system, name, speed), fs) in enumerate(kinds):
+        fmt = formats.get_format(system, name, speed)
+        dems.append(HipBlockDemodulator(filters.build_tape_filters(fmt, fs, N), max_blocks=16))
+        tapes.append(synth.generate(fmt, fields=1, sample_rate_hz=fs, seed=seed + i, snr_db=30.0).rf)
+    choice = rng.integers(0, len(kinds), size=segments)
+    segs = pack_segments(lengths, [dems[c] for c in choice])
+    rf_host = np.zeros(segs[-1].rf_offset + int(lengths[-1]) + 2, dtype=np.int16)
+    for s, c in zip(segs, choice):
+        t = tapes[c]
+        start = 2 * int(rng.integers(0, (t.size - s.length) // 2))
+        rf_host[s.rf_offset:s.rf_offset + s.length] = t[start:start + s.length]
+    rf = hipmem.DeviceBuffer.from_array(rf_host)
+    batch = HipBatch(segs, use_graph=True)
+    n = batch.output_samples
+    planes = hipmem.DeviceBuffer(4 * n * 4)
+    flags = hipmem.DeviceBuffer(16 * 4)
+    p = [planes.ptr + k * n * 4 for k in range(4)]
+    live = [s for s in segs if s.blocks() > 0]
+
+    def run_batch():
+        batch.run_device(rf.ptr, rf_host.size, *p)
+
+    def run_plain():
+        for s in live:
+            o = s.out_offset * 4
+            s.demodulator.demod_span_device(rf.ptr, s.rf_offset, s.blocks(), p[0] + o, p[1] + o, p[2] + o,
+                                            p[3] + o, flags.ptr)
+
+    out = {}
+    for name, fn in (("batched", run_batch), ("plain", run_plain)):
+        for _ in range(2):
+            fn()
+        hipmem.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        hipmem.synchronize()
+        out[name] = (time.perf_counter() - t0) / steps
+    samples = sum(s.blocks() * s.demodulator.stride for s in segs)
+    batch.close()
+    for d in dems:
+        d.close()
+    return {"segments": segments, "contexts": len(kinds), "blocks": int(batch.blocks),
+            "short_segments": sum(1 for s in segs if s.blocks() == 0), "decoded_samples": int(samples),
+            "segment_samples": int(lengths.sum()),
+            "batched_ms": round(out["batched"] * 1e3, 4), "plain_ms": round(out["plain"] * 1e3, 4),
+            "batched_msps": round(samples / out["batched"] / 1e6, 1), "plain_msps": round(samples / out["plain"] / 1e6, 1),
+            "speedup": round(out["plain"] / out["batched"], 2),
+            "formats": [f"{m[0]} {m[1]} {m[2].upper()} {fs / 1e6:.0f} MSPS" for m, fs in kinds],
+            "scope": "block demod chain (demod, repair, video, sub-deemphasis, chroma); no TBC (segments are not fields)"}
+
+
+def laserdisc_run(seed: int, fields: int, steps: int, lut):
+    """Synthetic NTSC LaserDisc at 40 MSPS int16 (7.6 -> 9.3 MHz video FM, analog sound
+    carriers at 2.30/2.81 MHz with +-100 kHz deviation and 1 kHz / 3 kHz tones, 30 dB):
+    LaserDisc block demod with both audio channels in the same launch, then pulses,
+    line locations, hsync and TBC of every field with the tape TBC kernels.  Plus a HiFi
+    stream: two channels (1.3 and 1.7 MHz carriers, 440 / 880 Hz tones, +-150 kHz) of
+    one second at the 2^23 Hz Hilbert IF rate through the HiFi discriminator."""
+    from vhsdecode_hip import field, formats, hipmem, ld_filters, synth
+    from vhsdecode_hip.demod import HipBlockDemodulator
+    from vhsdecode_hip.hifi import HipHiFiHilbertDiscriminator
+    fmt = formats.get_laserdisc_format("NTSC")
+    fl = ld_filters.build_laserdisc_filters(fmt, 40e6, N)
+    sp = fmt.sysparams
+    car = ((float(sp["audio_lfreq"]), 100e3, 1000.0, 0.1), (float(sp["audio_rfreq"]), 100e3, 3000.0, 0.1))
+    tape = synth.generate(fmt, fields=fields, seed=seed, snr_db=30.0, audio_carriers=car)
+    nb = (tape.rf.size - N) // STRIDE + 1
+    dem = HipBlockDemodulator(fl, max_blocks=nb)
+    dem.set_sinc_table(lut)
+    dec = field.HipFieldDecoder(dem, fmt)
+    g = dec.geom
+    span = nb * STRIDE
+    rf = hipmem.DeviceBuffer.from_array(tape.rf)
+    planes = hipmem.DeviceBuffer(4 * span * 4)
+    aud = hipmem.DeviceBuffer(2 * (span // fl.audio_decimation) * 4)
+    flags = hipmem.DeviceBuffer(nb * 4)
+    video, vlp, env, burst = (planes.ptr + k * span * 4 for k in range(4))
+    max_fields = fields + 1
+    out = hipmem.DeviceBuffer(max_fields * g.field_lines * g.out_len * 2)
+    meta = hipmem.DeviceBuffer(64)
+
+    def step():
+        dem.demod_span_device(rf.ptr, 0, nb, video, vlp, env, burst, flags.ptr,
+                              audio_left_ptr=aud.ptr, audio_right_ptr=aud.ptr + (span // fl.audio_decimation) * 4)
+        dec.decode_async(video, vlp, span, out.ptr, max_fields, meta.ptr)
+
+    step()
+    hipmem.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    hipmem.synchronize()
+    ld_s = (time.perf_counter() - t0) / steps
+    nfields = int(meta.download(np.int32, 1)[0])
+    rf_samples = (nb - 1) * STRIDE + N
+    dem.close()
+    if_rate, block = 1 << 23, 1 << 22
+    t = np.arange(2 * block, dtype=np.float64) / if_rate
+    hifi = []
+    for carrier, tone in ((1_300_000.0, 440.0), (1_700_000.0, 880.0)):
+        x = np.cos(2 * np.pi * np.cumsum(carrier + 150e3 * np.sin(2 * np.pi * tone * t)) / if_rate).astype(np.float32)
+        hifi.append((HipHiFiHilbertDiscriminator(if_rate, carrier, 150e3, block),
+                     hipmem.DeviceBuffer.from_array(x), hipmem.DeviceBuffer(x.nbytes)))
+
+    def hifi_step():
+        for d, xi, xo in hifi:
+            d.demodulate_device(xi.ptr, xo.ptr, 2)
+
+    hifi_step()
+    hipmem.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hifi_step()
+    hipmem.synchronize()
+    hifi_s = (time.perf_counter() - t0) / steps
+    for d, _, _ in hifi:
+        d.close()
+    return {"workload": f"NTSC LaserDisc 40 MSPS int16, {fields} fields, 30 dB, analog audio 2.30/2.81 MHz",
+            "rf_samples": int(rf_samples), "blocks": int(nb), "fields": nfields,
+            "ms": round(ld_s * 1e3, 4), "msps": round(rf_samples / ld_s / 1e6, 1),
+            "audio_samples_per_channel": int(span // fl.audio_decimation),
+            "hifi": {"channels": 2, "if_rate_hz": if_rate, "block": block, "seconds": 1.0,
+                     "ms": round(hifi_s * 1e3, 4),
+                     "if_msps": round(2 * 2 * block / hifi_s / 1e6, 1)},
+            "tbc": "tape TBC kernels on the LaserDisc planes (LaserDisc-specific field logic not on the device)"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,6 +370,8 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=16)
     ap.add_argument("--serve-window", type=float, default=2.0, help="window length (s of RF)")
     ap.add_argument("--serve-burst", type=int, default=32)
+    ap.add_argument("--batch-segments", type=int, default=64, help="batched-segment run (0: skip)")
+    ap.add_argument("--ld-fields", type=int, default=20, help="LaserDisc + HiFi run (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -326,6 +472,16 @@ def main():
                      "block_chain": {"kernels": sorted(avg_ms), "ms": round(chain_ms, 5),
                                      "achieved": round(chain_bytes / (chain_ms * 1e-3) / 1e9, 2)}},
     }
+    if args.batch_segments > 0:
+        bt = batched_run(SEED, args.batch_segments, steps=10)
+        result["batched"] = bt
+        print(f"[bench] batched: {bt['segments']} segments / {bt['blocks']} blocks in {bt['batched_ms']} ms "
+              f"({bt['batched_msps']} MSPS) vs plain {bt['plain_ms']} ms ({bt['plain_msps']} MSPS)", file=sys.stderr)
+    if args.ld_fields > 0:
+        ldr = laserdisc_run(SEED, args.ld_fields, steps=5, lut=lut)
+        result["laserdisc"] = ldr
+        print(f"[bench] laserdisc: {ldr['fields']} fields {ldr['msps']} MSPS; hifi {ldr['hifi']['if_msps']} IF MSPS",
+              file=sys.stderr)
     if args.serve_seconds > 0:
         dem.close()
         sv = served_run(args.serve_seconds, args.serve_rate, args.serve_streams, args.serve_window, SEED, lut,

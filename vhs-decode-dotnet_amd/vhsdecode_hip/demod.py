@@ -233,8 +233,11 @@ class HipBlockDemodulator:
                            aud[0] if aud else None, aud[1] if aud else None)
 
     def demod_span_device(self, rf_ptr: int, first_sample: int, nblocks: int, video_ptr: int, vlp_ptr: int,
-                          env_ptr: int, chroma_ptr: int = 0, flags_ptr: int = 0, stream: int = 0):
-        """Device-pointer entry (HBM addresses, e.g. hipmem.DeviceBuffer.ptr); asynchronous on `stream`."""
+                          env_ptr: int, chroma_ptr: int = 0, flags_ptr: int = 0, stream: int = 0,
+                          pilot_ptr: int = 0, audio_left_ptr: int = 0, audio_right_ptr: int = 0):
+        """Device-pointer entry (HBM addresses, e.g. hipmem.DeviceBuffer.ptr); asynchronous on `stream`.
+        LaserDisc: chroma_ptr receives the burst reference, pilot_ptr the PAL pilot and the audio
+        pointers nblocks * stride / decimation samples each."""
         span = _lib.SpanV1()
         span.struct_size = ctypes.sizeof(_lib.SpanV1)
         span.video = ctypes.cast(ctypes.c_void_p(video_ptr), _lib._fp)
@@ -242,6 +245,10 @@ class HipBlockDemodulator:
         span.envelope = ctypes.cast(ctypes.c_void_p(env_ptr), _lib._fp)
         span.chroma = ctypes.cast(ctypes.c_void_p(chroma_ptr), _lib._fp) if chroma_ptr else None
         span.block_flags = ctypes.cast(ctypes.c_void_p(flags_ptr), _lib._u32p) if flags_ptr else None
+        span.pilot = ctypes.cast(ctypes.c_void_p(pilot_ptr), _lib._fp) if pilot_ptr else None
+        if audio_left_ptr and audio_right_ptr:
+            span.audio_left = ctypes.cast(ctypes.c_void_p(audio_left_ptr), _lib._fp)
+            span.audio_right = ctypes.cast(ctypes.c_void_p(audio_right_ptr), _lib._fp)
         _lib.check(self._L.vhsdecode_hip_demod_span_device(self._h, ctypes.c_void_p(rf_ptr), first_sample, nblocks,
                                                            ctypes.byref(span), ctypes.c_void_p(stream)))
 
