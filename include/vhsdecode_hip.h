@@ -518,6 +518,15 @@ VHSDECODE_HIP_API int32_t vhsdecode_hip_get_profile(vhsdecode_hip_demod_context*
 VHSDECODE_HIP_API int32_t vhsdecode_hip_debug_fft16k(vhsdecode_hip_demod_context* context, float* data,
                                                      int32_t count, int32_t inverse, void* stream);
 
+/* Diagnostic: the block repair kernel's ReplaceSpikes step (RfDemodulator.ReplaceSpikes,
+ * src/VHSDecode.Core/Dsp/RfDemodulator.cs:966-1001) on caller-supplied 32768-sample
+ * demod and diffed-demod arrays (device FP32); writes the repaired demod to `out`.
+ * Exposed so the sequential window logic can be checked against the reference's
+ * semantics on chosen inputs. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_debug_replace_spikes(vhsdecode_hip_demod_context* context, const float* demod,
+                                                             const float* diffed, double max_value, float* out,
+                                                             void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -723,6 +723,10 @@ void oracle_demod_block_ld(const double* input, int n, double fs, const double* 
   for (int i = 0; i < n; i++) demod_out[i] = (double)(float)demod[i];
 }
 
+void oracle_replace_spikes(double* demod, const double* diffed, int n, double max_value) {
+  replace_spikes(demod, diffed, n, max_value);
+}
+
 void oracle_conj_unwrap(const double* analytic_interleaved, int n, double fs, double* out) {
   conj_unwrap(reinterpret_cast<const cd*>(analytic_interleaved), n, fs, out);
 }

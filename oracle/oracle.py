@@ -48,6 +48,7 @@ def lib():
                                             _dp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
                                             _dp, _dp, _dp, _dp, _dp, _dp]
         L.oracle_conj_unwrap.argtypes = [_dp, ctypes.c_int, ctypes.c_double, _dp]
+        L.oracle_replace_spikes.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_double]
         L.oracle_ld_audio_channel.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
                                               ctypes.c_double, ctypes.c_double, _dp]
         L.oracle_hifi_analytic_phase.argtypes = [_fp, ctypes.c_int]
@@ -228,6 +229,14 @@ def demod_block_ld(block: np.ndarray, filters) -> LdBlock:
     if pilot is None:
         out.video_pilot = None
     return out
+
+
+def replace_spikes(demod: np.ndarray, diffed: np.ndarray, max_value: float) -> np.ndarray:
+    """RfDemodulator.ReplaceSpikes on copies of the arrays; returns the repaired demod."""
+    d = np.ascontiguousarray(demod, dtype=np.float64).copy()
+    dd = np.ascontiguousarray(diffed, dtype=np.float64)
+    lib().oracle_replace_spikes(_p(d, _dp), _p(dd, _dp), d.size, float(max_value))
+    return d
 
 
 def conj_unwrap(analytic: np.ndarray, fs: float) -> np.ndarray:

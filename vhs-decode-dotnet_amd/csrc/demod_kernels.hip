@@ -248,11 +248,10 @@ __device__ __forceinline__ void video_stage(BlockLds& s, const DemodArgs& a, lon
   }
 }
 
-__global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) {
+__device__ __forceinline__ void vhs_block_demod_body(const DemodArgs& a, const int blk) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
   const int j = threadIdx.x;
-  const int blk = blockIdx.x;
   const long long start = a.block_start[blk];
   const long long out_base = a.out_offset[blk];
   load_octant(s, a.oct);
@@ -312,6 +311,14 @@ __global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) {
   if (j == 0 && weak) atomicOr(&a.flags[blk], 1u);
 }
 
+__global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) { vhs_block_demod_body(a, blockIdx.x); }
+
+// Table mode: workgroup i decodes global block t.block_ids[i] with its format's record.
+__global__ void __launch_bounds__(1024) vhs_block_demod_table_kernel(const DemodArgs* __restrict__ recs, BlockTable t) {
+  const int blk = t.block_ids[blockIdx.x];
+  vhs_block_demod_body(recs[t.block_rec[blk]], blk);
+}
+
 // numpy's pairwise float32 sum of the 32768 values at y (NumpyReduction.PairwiseSumFloat32,
 // Dsp/NumpyReduction.cs:253-339) in its exact order: 256 runs of 128 summed with eight
 // accumulators each (accumulator k adds elements k, k+8, ... in order; then
@@ -355,11 +362,10 @@ __device__ float pairwise_sum_32768(const float* y, float* tree) {
 // unless weak, high = FP32 forward-backward RF top band-pass of the filtered RF, scaled
 // by 0.9 mean(envelope) / envelope and the multiplier, whose spectrum is added to the
 // filtered spectrum -> analytic signal and discriminator as in the plain kernel.
-__global__ void __launch_bounds__(1024) vhs_block_demod_boost_kernel(DemodArgs a) {
+__device__ __forceinline__ void vhs_block_demod_boost_body(const DemodArgs& a, const int blk) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
   const int j = threadIdx.x;
-  const int blk = blockIdx.x;
   const long long start = a.block_start[blk];
   const long long out_base = a.out_offset[blk];
   float2* stash = a.stash + (long long)blk * kStashEntries;
@@ -460,6 +466,14 @@ __global__ void __launch_bounds__(1024) vhs_block_demod_boost_kernel(DemodArgs a
   const bool spike = analytic_demod(v, fb, s, a.fs, a.diff_max, stash);
   const unsigned long long any_spike = __ballot(spike);
   if ((j & 63) == 0 && any_spike) atomicOr(&a.flags[blk], 2u);
+}
+
+__global__ void __launch_bounds__(1024) vhs_block_demod_boost_kernel(DemodArgs a) { vhs_block_demod_boost_body(a, blockIdx.x); }
+
+// Table mode: workgroup i decodes global block t.block_ids[i] with its format's record.
+__global__ void __launch_bounds__(1024) vhs_block_demod_boost_table_kernel(const DemodArgs* __restrict__ recs, BlockTable t) {
+  const int blk = t.block_ids[blockIdx.x];
+  vhs_block_demod_boost_body(recs[t.block_rec[blk]], blk);
 }
 
 // ------------------------------------------------------------------ LaserDisc
@@ -567,11 +581,10 @@ __device__ __forceinline__ float2* audio_ifft(float2* x, float2* y, int n, int l
 // stage-1 responses, inverse transformed and demodulated at the slice rate
 // (RfBlockDecodePipeline.DecodeAnalogAudioChannel, RfBlockDecodePipeline.cs:732-752);
 // each block writes its kept audio window (RfBlockStreamDecoder.cs:1690-1709).
-__global__ void __launch_bounds__(1024) vhs_block_demod_ld_kernel(DemodArgs a) {
+__device__ __forceinline__ void vhs_block_demod_ld_body(const DemodArgs& a, const int blk) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
   const int j = threadIdx.x;
-  const int blk = blockIdx.x;
   const long long start = a.block_start[blk];
   const long long out_base = a.out_offset[blk];
   load_octant(s, a.oct);
@@ -647,23 +660,37 @@ __global__ void __launch_bounds__(1024) vhs_block_demod_ld_kernel(DemodArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(1024) vhs_block_demod_ld_kernel(DemodArgs a) { vhs_block_demod_ld_body(a, blockIdx.x); }
+
+// Table mode: workgroup i decodes global block t.block_ids[i] with its format's record.
+__global__ void __launch_bounds__(1024) vhs_block_demod_ld_table_kernel(const DemodArgs* __restrict__ recs, BlockTable t) {
+  const int blk = t.block_ids[blockIdx.x];
+  vhs_block_demod_ld_body(recs[t.block_rec[blk]], blk);
+}
+
 // Stage C: video and sync-reference planes from the demod the block demod (or spike
 // repair) kernel left in each block's stash.
-__global__ void __launch_bounds__(1024) vhs_block_video_kernel(DemodArgs a) {
+__device__ __forceinline__ void vhs_block_video_body(const DemodArgs& a, const int blk) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
-  const int blk = blockIdx.x;
   load_octant(s, a.oct);
   __syncthreads();
   video_stage(s, a, a.out_offset[blk], a.stash + (long long)blk * kStashEntries);
 }
 
+__global__ void __launch_bounds__(1024) vhs_block_video_kernel(DemodArgs a) { vhs_block_video_body(a, blockIdx.x); }
+
+// Table mode: workgroup i decodes global block t.block_ids[i] with its format's record.
+__global__ void __launch_bounds__(1024) vhs_block_video_table_kernel(const DemodArgs* __restrict__ recs, BlockTable t) {
+  const int blk = t.block_ids[blockIdx.x];
+  vhs_block_video_body(recs[t.block_rec[blk]], blk);
+}
+
 // ------------------------------------------------------------------ chroma
 template <int S>
-__global__ void __launch_bounds__(1024) vhs_block_chroma_kernel(ChromaArgs a) {
+__device__ __forceinline__ void vhs_block_chroma_body(const ChromaArgs& a, const int blk) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
-  const int blk = blockIdx.x;
   const int16_t* x = a.rf + a.block_start[blk];
   float* ext = reinterpret_cast<float*>(s.buf) + kSosMargin;
   float* raw = ext + a.sos.pad;
@@ -677,6 +704,16 @@ __global__ void __launch_bounds__(1024) vhs_block_chroma_kernel(ChromaArgs a) {
   __syncthreads();
   sos_filtfilt_block_t<S>(a.sos, ext, 32768, s.scratch, trans, a.out_chroma + a.out_offset[blk], a.out_begin,
                           a.out_count, a.chroma_offset, /*subtract_mean=*/true);
+}
+
+template <int S>
+__global__ void __launch_bounds__(1024) vhs_block_chroma_kernel(ChromaArgs a) { vhs_block_chroma_body<S>(a, blockIdx.x); }
+
+// Table mode: workgroup i decodes global block t.block_ids[i] with its format's record.
+template <int S>
+__global__ void __launch_bounds__(1024) vhs_block_chroma_table_kernel(const ChromaArgs* __restrict__ recs, BlockTable t) {
+  const int blk = t.block_ids[blockIdx.x];
+  vhs_block_chroma_body<S>(recs[t.block_rec[blk]], blk);
 }
 
 // ------------------------------------------------------------------ sub-deemphasis
@@ -696,11 +733,10 @@ __device__ __forceinline__ float numpy_abs(c32 z) {
   return sqrtf(fmaf(r, r, 1.0f)) * lg;
 }
 
-__global__ void __launch_bounds__(1024) vhs_block_subdeemph_kernel(SubDeemphArgs a) {
+__device__ __forceinline__ void vhs_block_subdeemph_body(const SubDeemphArgs& a, const int blk) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
   const int j = threadIdx.x;
-  const int blk = blockIdx.x;
   float2* stash = a.stash + (long long)blk * kStashEntries;
   float2* aux = a.aux + (long long)blk * kAuxEntries;
   const float inv_n = 1.0f / 32768.0f;
@@ -819,13 +855,85 @@ __global__ void __launch_bounds__(1024) vhs_block_subdeemph_kernel(SubDeemphArgs
                           a.out_count, /*roll=*/0, false);
 }
 
+__global__ void __launch_bounds__(1024) vhs_block_subdeemph_kernel(SubDeemphArgs a) { vhs_block_subdeemph_body(a, blockIdx.x); }
+
+// Table mode: workgroup i decodes global block t.block_ids[i] with its format's record.
+__global__ void __launch_bounds__(1024) vhs_block_subdeemph_table_kernel(const SubDeemphArgs* __restrict__ recs, BlockTable t) {
+  const int blk = t.block_ids[blockIdx.x];
+  vhs_block_subdeemph_body(recs[t.block_rec[blk]], blk);
+}
+
 // ------------------------------------------------------------------ spike repair
-__global__ void __launch_bounds__(1024) vhs_block_repair_kernel(RepairArgs r) {
+// ReplaceSpikes (RfDemodulator.cs:966-1001) for one 32768-sample block: spike indices
+// come from the unmodified demod dm (global), then windows [i-8, min(i+30, N-1)) are
+// processed in ascending order against the progressively repaired demod.  The order
+// is inherently sequential; one wave walks the spike list with a lane per window
+// sample (a window is at most 38 samples) on the demod held in LDS.  `list` is
+// 32768 ints of global scratch.  On return the repaired demod is in s.buf (as floats)
+// and every thread has passed a barrier.
+__device__ __forceinline__ void replace_spikes_block(const float* __restrict__ dm, const float* __restrict__ dd,
+                                                     int* __restrict__ list, double mx, BlockLds& s) {
+  const int j = threadIdx.x;
+  // ascending spike list: thread j owns samples [32 j, 32 j + 32)
+  unsigned mask = 0;
+#pragma unroll 4
+  for (int q = 0; q < 32; q++) mask |= ((double)dm[32 * j + q] > mx ? 1u : 0u) << q;
+  const int cnt = __popc(mask);
+  int incl = cnt;
+  const int lane = j & 63, wave = j >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o);
+    if (lane >= o) incl += u;
+  }
+  int* wsum = reinterpret_cast<int*>(s.scratch);
+  if (lane == 63) wsum[wave] = incl;
+  float* dms = reinterpret_cast<float*>(s.buf);  // 32768 floats
+  for (int n = j; n < 32768; n += kFftThreads) dms[n] = dm[n];
+  __syncthreads();
+  int base = incl - cnt, total = 0;
+  for (int w = 0; w < kFftThreads / 64; w++) {
+    const int v = wsum[w];
+    if (w < wave) base += v;
+    total += v;
+  }
+  while (mask) {
+    const int q = __ffs(mask) - 1;
+    mask &= mask - 1;
+    list[base++] = 32 * j + q;
+  }
+  __threadfence_block();
+  __syncthreads();
+  if (j < 64) {
+    for (int q0 = 0; q0 < total; q0 += 64) {
+      const int mine = q0 + lane < total ? list[q0 + lane] : 0;
+      const int nq = total - q0 < 64 ? total - q0 : 64;
+      for (int q = 0; q < nq; q++) {
+        const int i = __shfl(mine, q);
+        const int st = i - 8 > 0 ? i - 8 : 0;
+        const int en = i + 30 < 32767 ? i + 30 : 32767;
+        if (st >= en) continue;
+        const int k = st + lane;
+        const bool valid = k < en;
+        const float d = valid ? dd[k] : -INFINITY;
+        float md = d, mo = valid ? dms[k] : -INFINITY;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+          md = fmaxf(md, __shfl_xor(md, o));
+          mo = fmaxf(mo, __shfl_xor(mo, o));
+        }
+        if (md < mo && valid) dms[k] = d;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void vhs_block_repair_body(const RepairArgs& r, const int blk) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
   const DemodArgs& a = r.d;
   const int j = threadIdx.x;
-  const int blk = blockIdx.x;
   if ((a.flags[blk] & 2u) == 0) return;  // uniform across the workgroup
   const long long start = a.block_start[blk];
   float2* A = r.scratch_analytic + (long long)blk * 32768;
@@ -904,40 +1012,28 @@ __global__ void __launch_bounds__(1024) vhs_block_repair_kernel(RepairArgs r) {
   }
   __threadfence_block();
   __syncthreads();
-  // ReplaceSpikes (RfDemodulator.cs:966-1001): spike indices are collected from
-  // the unmodified demod, then windows [i-8, min(i+30, N-1)) are processed in
-  // ascending order against the progressively repaired demod.  Sequential by
-  // definition; this path only runs for blocks that contain a spike.
-  // sequential repair with an explicit spike list held in LDS scratch space
-  if (j == 0) {
-    const double mx = a.diff_max;
-    int* list = reinterpret_cast<int*>(s.buf);  // up to 32768 indices
-    int count = 0;
-    for (int i = 0; i < 32768; i++)
-      if ((double)dm[i] > mx) list[count++] = i;
-    for (int q = 0; q < count; q++) {
-      const int i = list[q];
-      const int st = i - 8 > 0 ? i - 8 : 0;
-      const int en = i + 30 < 32767 ? i + 30 : 32767;
-      if (st >= en) continue;
-      float md = -INFINITY, mo = -INFINITY;
-      for (int k = st; k < en; k++) {
-        md = fmaxf(md, dd[k]);
-        mo = fmaxf(mo, dm[k]);
-      }
-      if (md < mo)
-        for (int k = st; k < en; k++) dm[k] = dd[k];
-    }
-  }
-  __threadfence_block();
-  __syncthreads();
+  // ReplaceSpikes (RfDemodulator.cs:966-1001): spike indices are collected from the
+  // unmodified demod, then windows [i-8, min(i+30, N-1)) are processed in ascending
+  // order against the progressively repaired demod.  The order is inherently
+  // sequential; one wave walks the spike list with a lane per window sample (the
+  // window is at most 38 samples), the demod held in LDS.
+  replace_spikes_block(dm, dd, reinterpret_cast<int*>(A), a.diff_max, s);  // the analytic scratch is free now
+  const float* dms = reinterpret_cast<const float*>(s.buf);
   // the repaired demod replaces the block's stashed demod; the video kernel runs next
   float2* stash = r.d.stash + (long long)blk * kStashEntries;
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     const int k = j + m * kFftThreads;
-    row_at(stash, m) = make_float2(dm[2 * k], dm[2 * k + 1]);
+    row_at(stash, m) = make_float2(dms[2 * k], dms[2 * k + 1]);
   }
+}
+
+__global__ void __launch_bounds__(1024) vhs_block_repair_kernel(RepairArgs r) { vhs_block_repair_body(r, blockIdx.x); }
+
+// Table mode: workgroup i decodes global block t.block_ids[i] with its format's record.
+__global__ void __launch_bounds__(1024) vhs_block_repair_table_kernel(const RepairArgs* __restrict__ recs, BlockTable t) {
+  const int blk = t.block_ids[blockIdx.x];
+  vhs_block_repair_body(recs[t.block_rec[blk]], blk);
 }
 
 // Fills n words (a kernel rather than hipMemsetAsync, so the chain can be captured
@@ -948,6 +1044,16 @@ __global__ void __launch_bounds__(256) fill_u32_kernel(unsigned* p, long long n,
 }
 
 // ------------------------------------------------------------------ diagnostics
+// One block's ReplaceSpikes on caller-supplied demod and diffed demod.
+__global__ void __launch_bounds__(1024) debug_replace_spikes_kernel(const float* dm, const float* dd, int* list,
+                                                                    double mx, float* out) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
+  replace_spikes_block(dm, dd, list, mx, s);
+  const float* dms = reinterpret_cast<const float*>(s.buf);
+  for (int n = threadIdx.x; n < 32768; n += kFftThreads) out[n] = dms[n];
+}
+
 __global__ void __launch_bounds__(1024) debug_fft_kernel(float2* data, int inverse, const float2* oct_g) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
@@ -972,6 +1078,21 @@ __global__ void __launch_bounds__(1024) debug_fft_kernel(float2* data, int inver
 }  // namespace vhsd
 
 // ------------------------------------------------------------------ launchers
+template <typename K, typename... Args>
+static hipError_t launch_block_kernel(K kernel, int count, hipStream_t stream, Args... args) {
+  const size_t lds = sizeof(vhsd::BlockLds);
+  hipError_t e = hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kernel, dim3(count), dim3(1024), lds, stream, args...);
+  return hipGetLastError();
+}
+
+template <int S>
+static hipError_t launch_chroma_table_s(const vhsd::ChromaArgs* recs, vhsd::BlockTable t, int count,
+                                        hipStream_t stream) {
+  return launch_block_kernel(vhsd::vhs_block_chroma_table_kernel<S>, count, stream, recs, t);
+}
+
 template <int S>
 static hipError_t launch_chroma_s(const vhsd::ChromaArgs* a, int nblocks, hipStream_t stream) {
   const size_t lds = sizeof(vhsd::BlockLds);
@@ -1006,6 +1127,11 @@ hipError_t vhsd_launch_block_chroma(const vhsd::ChromaArgs* a, int nblocks, hipS
   }
 }
 
+hipError_t vhsd_launch_debug_replace_spikes(const float* dm, const float* dd, int* list, double mx, float* out,
+                                           hipStream_t stream) {
+  return launch_block_kernel(vhsd::debug_replace_spikes_kernel, 1, stream, dm, dd, list, mx, out);
+}
+
 hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const float2* oct, hipStream_t stream) {
   const size_t lds = sizeof(vhsd::BlockLds);
   hipError_t e = hipFuncSetAttribute((const void*)vhsd::debug_fft_kernel,
@@ -1023,6 +1149,37 @@ hipError_t vhsd_launch_block_demod_boost(const vhsd::DemodArgs* a, int nblocks, 
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(vhsd::vhs_block_demod_boost_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
   return hipGetLastError();
+}
+
+hipError_t vhsd_launch_block_table(int kind, int sections, const void* recs, vhsd::BlockTable t, int count,
+                                   hipStream_t stream) {
+  using namespace vhsd;
+  if (count <= 0) return hipSuccess;
+  switch (kind) {
+    case kTableDemod:
+      return launch_block_kernel(vhs_block_demod_table_kernel, count, stream, (const DemodArgs*)recs, t);
+    case kTableDemodBoost:
+      return launch_block_kernel(vhs_block_demod_boost_table_kernel, count, stream, (const DemodArgs*)recs, t);
+    case kTableDemodLd:
+      return launch_block_kernel(vhs_block_demod_ld_table_kernel, count, stream, (const DemodArgs*)recs, t);
+    case kTableVideo:
+      return launch_block_kernel(vhs_block_video_table_kernel, count, stream, (const DemodArgs*)recs, t);
+    case kTableSubDeemph:
+      return launch_block_kernel(vhs_block_subdeemph_table_kernel, count, stream, (const SubDeemphArgs*)recs, t);
+    case kTableRepair:
+      return launch_block_kernel(vhs_block_repair_table_kernel, count, stream, (const RepairArgs*)recs, t);
+    case kTableChroma:
+      switch (sections) {
+        case 1: return launch_chroma_table_s<1>((const ChromaArgs*)recs, t, count, stream);
+        case 2: return launch_chroma_table_s<2>((const ChromaArgs*)recs, t, count, stream);
+        case 3: return launch_chroma_table_s<3>((const ChromaArgs*)recs, t, count, stream);
+        case 4: return launch_chroma_table_s<4>((const ChromaArgs*)recs, t, count, stream);
+        case 10: return launch_chroma_table_s<10>((const ChromaArgs*)recs, t, count, stream);
+        default: return hipErrorInvalidValue;
+      }
+    default:
+      return hipErrorInvalidValue;
+  }
 }
 
 hipError_t vhsd_launch_fill_u32(unsigned* p, long long n, unsigned value, hipStream_t stream) {

@@ -105,6 +105,25 @@ struct SubDeemphArgs {
   float2* aux;                   // [nblocks][kAuxEntries]
 };
 
+// Table mode of the block kernels: one launch decodes blocks of several formats;
+// workgroup i takes global block block_ids[i] and that block's record
+// recs[block_rec[block]] (per-block arrays of the records are indexed by the global
+// block).
+struct BlockTable {
+  const int* block_ids;
+  const int* block_rec;
+};
+
+enum BlockTableKernel {
+  kTableDemod = 0,
+  kTableDemodBoost = 1,
+  kTableDemodLd = 2,
+  kTableVideo = 3,
+  kTableSubDeemph = 4,
+  kTableRepair = 5,
+  kTableChroma = 6,
+};
+
 struct RepairArgs {
   DemodArgs d;
   float2* scratch_analytic;  // [nblocks][32768] complex
@@ -174,9 +193,15 @@ hipError_t vhsd_launch_locate(const float* sync_ref, long long span_len, const i
 hipError_t vhsd_launch_block_demod_boost(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_demod_ld(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_fill_u32(unsigned* p, long long n, unsigned value, hipStream_t stream);
+// kind: vhsd::BlockTableKernel; sections: chroma SOS sections (kTableChroma only); recs: device
+// array of the kind's argument records (DemodArgs, SubDeemphArgs, RepairArgs or ChromaArgs).
+hipError_t vhsd_launch_block_table(int kind, int sections, const void* recs, vhsd::BlockTable t, int count,
+                                   hipStream_t stream);
 hipError_t vhsd_launch_block_video(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_subdeemph(const vhsd::SubDeemphArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const float2* oct, hipStream_t stream);
+hipError_t vhsd_launch_debug_replace_spikes(const float* dm, const float* dd, int* list, double mx, float* out,
+                                           hipStream_t stream);
 hipError_t vhsd_launch_pulses(const float* s, long long n, double bias, double high, int min_len, int max_len,
                               int* tile_counts,
                               int n_tiles_cap, long long* edge_pos, unsigned char* edge_kind, int edge_cap,

@@ -253,6 +253,7 @@ struct vhsdecode_hip_demod_context {
   DevBuf<long long> block_start, out_offset;
   DevBuf<unsigned> flags;
   DevBuf<float2> rep_a;
+  DevBuf<int> dbg_list;
   DevBuf<float2> stash;
   DevBuf<float> rep_dm;
   // pulse scratch
@@ -303,7 +304,7 @@ struct vhsdecode_hip_demod_context {
     ref_filt[0].release(); ref_filt[1].release(); audio_st[0].release(); audio_st[1].release();
     boost_trans.release(); notch_trans.release(); aux.release();
     env_trans.release(); chroma_trans.release(); block_start.release(); out_offset.release(); flags.release();
-    rep_a.release(); rep_dm.release(); stash.release(); tile_counts.release(); counters.release(); p_starts.release();
+    rep_a.release(); rep_dm.release(); stash.release(); dbg_list.release(); tile_counts.release(); counters.release(); p_starts.release();
     p_lens.release(); edge_pos.release(); edge_kind.release(); h_loc.release(); h_ref.release();
     h_fil.release(); h_fo.release(); h_ptrs.release(); h_work.release(); lut.release(); t_loc.release(); t_adj.release();
     t_f64.release(); t_u16.release(); s_rf.release(); s_planes.release(); s_flags.release();
@@ -381,64 +382,25 @@ int32_t ensure_lut(vhsdecode_hip_demod_context* c) {
   return VHSDECODE_HIP_OK;
 }
 
-// LaserDisc: complex-path demod kernel, then video, sync reference, burst and pilot
-// planes from the stashed demod (clipped for video).
-int32_t demod_span_ld(vhsdecode_hip_demod_context* c, vhsd::DemodArgs& a, int32_t nblocks,
-                      const vhsdecode_hip_span_v1* out, hipStream_t st) {
-  a.stash_sub = 0.0f;
-  a.clip_lo = c->clip_lo;
-  a.clip_hi = c->clip_hi;
-  a.fs_d = c->fs;
-  a.diff_max = 0.0;
-  int nref = 0;
-  for (int r = 0; r < c->nref; r++) {
-    float* dst = c->ref_is_pilot[r] ? out->pilot : out->chroma;
-    if (!dst) continue;
-    a.ref_filt[nref] = c->ref_filt[r].p;
-    a.ref_out[nref] = dst;
-    a.ref_shift[nref] = c->ref_shift[r];
-    a.ref_corr[nref] = c->ref_corr[r];
-    nref++;
-  }
-  a.nref = nref;
-  if (c->has_audio && out->audio_left && out->audio_right) {
-    a.audio = 1;
-    a.audio_bins = c->audio_bins;
-    a.audio_log2 = c->audio_log2;
-    for (int ch = 0; ch < 2; ch++) {
-      a.audio_low[ch] = c->audio_low[ch];
-      a.audio_stage1[ch] = c->audio_st[ch].p;
-      a.audio_low_freq[ch] = c->audio_low_freq[ch];
-    }
-    a.audio_scale = c->audio_rate / 6.283185307179586;
-    a.audio_cut = c->cut / c->audio_dec;
-    a.audio_stride = c->stride / c->audio_dec;
-    a.audio_decimation = c->audio_dec;
-    a.out_audio[0] = out->audio_left;
-    a.out_audio[1] = out->audio_right;
-  }
-  {
-    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_BLOCK_DEMOD, nblocks);
-    VHSD_HIP(vhsd_launch_block_demod_ld(&a, nblocks, st));
-  }
-  {
-    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_BLOCK_VIDEO, nblocks);
-    VHSD_HIP(vhsd_launch_block_video(&a, nblocks, st));
-  }
-  return VHSDECODE_HIP_OK;
-}
+// Argument records of one context's block chain, and which kernels it runs.
+struct ChainArgs {
+  vhsd::DemodArgs d{};
+  vhsd::RepairArgs r{};
+  vhsd::SubDeemphArgs sd{};
+  vhsd::ChromaArgs ch{};
+  int demod_kind = vhsd::kTableDemod;  // plain, RF boost or LaserDisc
+  bool repair = false, subdeemph = false, chroma = false;
+  int chroma_sections = 0;
+};
 
-// The block chain of one context over nblocks blocks whose input offsets (into rf) and
-// stitched output offsets (into the span planes) are the device arrays starts/offs.
-// Only kernel launches are enqueued (no copies or memsets), so a caller may capture
-// the call into a graph.  stash (nblocks * kStashEntries) and, when the context needs
-// it, aux (nblocks * kAuxEntries) are the per-block scratch.
-int32_t demod_blocks_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, const long long* starts,
-                            const long long* offs, int32_t nblocks, const vhsdecode_hip_span_v1* out,
-                            unsigned* flags, float2* stash, float2* aux, float2* rep_a, float* rep_dm,
-                            hipStream_t st) {
-  VHSD_HIP(vhsd_launch_fill_u32(flags, nblocks, 0u, st));
-  vhsd::DemodArgs a{};
+// Fill the records for blocks whose input offsets (into rf) and stitched output
+// offsets (into the span planes) are the device arrays starts/offs, with per-block
+// flags and scratch indexed by the same block index.
+void build_chain(vhsdecode_hip_demod_context* c, const int16_t* rf, const long long* starts, const long long* offs,
+                 const vhsdecode_hip_span_v1* out, unsigned* flags, float2* stash, float2* aux, float2* rep_a,
+                 float* rep_dm, ChainArgs* ca) {
+  vhsd::DemodArgs& a = ca->d;
+  a = vhsd::DemodArgs{};
   a.rf = rf;
   a.block_start = starts;
   a.out_offset = offs;
@@ -463,30 +425,55 @@ int32_t demod_blocks_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, c
   a.stash = stash;
   a.aux = aux;
   a.stash_sub = a.bias;
-  if (c->ld) return demod_span_ld(c, a, nblocks, out, st);
   if (c->has_boost) {
     a.boost = c->boost;
     a.boost.trans = c->boost_trans.p;
     a.boost_mult = c->boost_mult;
   }
-  {
-    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_BLOCK_DEMOD, nblocks);
-    if (c->has_boost)
-      VHSD_HIP(vhsd_launch_block_demod_boost(&a, nblocks, st));
-    else
-      VHSD_HIP(vhsd_launch_block_demod(&a, nblocks, st));
+  ca->demod_kind = c->ld ? vhsd::kTableDemodLd : (c->has_boost ? vhsd::kTableDemodBoost : vhsd::kTableDemod);
+  if (c->ld) {
+    // LaserDisc: the demod is stashed as offsets from the bias and clipped for video;
+    // burst and pilot are extra passes of the video kernel
+    a.stash_sub = 0.0f;
+    a.clip_lo = c->clip_lo;
+    a.clip_hi = c->clip_hi;
+    a.fs_d = c->fs;
+    a.diff_max = 0.0;
+    int nref = 0;
+    for (int r = 0; r < c->nref; r++) {
+      float* dst = c->ref_is_pilot[r] ? out->pilot : out->chroma;
+      if (!dst) continue;
+      a.ref_filt[nref] = c->ref_filt[r].p;
+      a.ref_out[nref] = dst;
+      a.ref_shift[nref] = c->ref_shift[r];
+      a.ref_corr[nref] = c->ref_corr[r];
+      nref++;
+    }
+    a.nref = nref;
+    if (c->has_audio && out->audio_left && out->audio_right) {
+      a.audio = 1;
+      a.audio_bins = c->audio_bins;
+      a.audio_log2 = c->audio_log2;
+      for (int ch = 0; ch < 2; ch++) {
+        a.audio_low[ch] = c->audio_low[ch];
+        a.audio_stage1[ch] = c->audio_st[ch].p;
+        a.audio_low_freq[ch] = c->audio_low_freq[ch];
+      }
+      a.audio_scale = c->audio_rate / 6.283185307179586;
+      a.audio_cut = c->cut / c->audio_dec;
+      a.audio_stride = c->stride / c->audio_dec;
+      a.audio_decimation = c->audio_dec;
+      a.out_audio[0] = out->audio_left;
+      a.out_audio[1] = out->audio_right;
+    }
+    return;
   }
-  if (c->diff_max > 0.0) {
-    vhsd::RepairArgs r{a, rep_a, rep_dm};
-    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_SPIKE_REPAIR, nblocks);
-    VHSD_HIP(vhsd_launch_block_repair(&r, nblocks, st));
-  }
-  {
-    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_BLOCK_VIDEO, nblocks);
-    VHSD_HIP(vhsd_launch_block_video(&a, nblocks, st));
-  }
+  ca->repair = c->diff_max > 0.0;
+  ca->r = vhsd::RepairArgs{a, rep_a, rep_dm};
+  ca->subdeemph = c->has_subdeemph;
   if (c->has_subdeemph) {
-    vhsd::SubDeemphArgs sd{};
+    vhsd::SubDeemphArgs& sd = ca->sd;
+    sd = vhsd::SubDeemphArgs{};
     sd.out_offset = offs;
     sd.out_begin = c->cut;
     sd.out_count = c->stride;
@@ -511,11 +498,11 @@ int32_t demod_blocks_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, c
     sd.video = c->vid.p;
     sd.video_corr = c->video_corr;
     sd.aux = aux;
-    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_SUB_DEEMPHASIS, nblocks);
-    VHSD_HIP(vhsd_launch_block_subdeemph(&sd, nblocks, st));
   }
-  if (c->has_chroma && out->chroma) {
-    vhsd::ChromaArgs ch{};
+  ca->chroma = c->has_chroma && out->chroma;
+  if (ca->chroma) {
+    vhsd::ChromaArgs& ch = ca->ch;
+    ch = vhsd::ChromaArgs{};
     ch.rf = rf;
     ch.block_start = starts;
     ch.out_offset = offs;
@@ -525,8 +512,43 @@ int32_t demod_blocks_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, c
     ch.sos = c->chroma;
     ch.sos.trans = c->chroma_trans.p;
     ch.out_chroma = out->chroma;
+    ca->chroma_sections = c->chroma.sections;
+  }
+}
+
+// The block chain of one context over nblocks blocks (kernel launches only, so a caller
+// may capture the call into a graph).
+int32_t demod_blocks_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, const long long* starts,
+                            const long long* offs, int32_t nblocks, const vhsdecode_hip_span_v1* out,
+                            unsigned* flags, float2* stash, float2* aux, float2* rep_a, float* rep_dm,
+                            hipStream_t st) {
+  ChainArgs ca;
+  build_chain(c, rf, starts, offs, out, flags, stash, aux, rep_a, rep_dm, &ca);
+  VHSD_HIP(vhsd_launch_fill_u32(flags, nblocks, 0u, st));
+  {
+    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_BLOCK_DEMOD, nblocks);
+    if (ca.demod_kind == vhsd::kTableDemodLd)
+      VHSD_HIP(vhsd_launch_block_demod_ld(&ca.d, nblocks, st));
+    else if (ca.demod_kind == vhsd::kTableDemodBoost)
+      VHSD_HIP(vhsd_launch_block_demod_boost(&ca.d, nblocks, st));
+    else
+      VHSD_HIP(vhsd_launch_block_demod(&ca.d, nblocks, st));
+  }
+  if (ca.repair) {
+    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_SPIKE_REPAIR, nblocks);
+    VHSD_HIP(vhsd_launch_block_repair(&ca.r, nblocks, st));
+  }
+  {
+    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_BLOCK_VIDEO, nblocks);
+    VHSD_HIP(vhsd_launch_block_video(&ca.d, nblocks, st));
+  }
+  if (ca.subdeemph) {
+    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_SUB_DEEMPHASIS, nblocks);
+    VHSD_HIP(vhsd_launch_block_subdeemph(&ca.sd, nblocks, st));
+  }
+  if (ca.chroma) {
     ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_CHROMA, nblocks);
-    VHSD_HIP(vhsd_launch_block_chroma(&ch, nblocks, st));
+    VHSD_HIP(vhsd_launch_block_chroma(&ca.ch, nblocks, st));
   }
   return VHSDECODE_HIP_OK;
 }
@@ -844,74 +866,122 @@ int32_t vhsdecode_hip_demod_span_device(vhsdecode_hip_demod_context* c, const in
 }
 
 // ---------------------------------------------------------------- batched segments
-// A batch groups the whole blocks of many segments by context (decode format and
-// sample rate); each group is one launch of the block chain over its blocks, whose
-// input and output offsets come from the group's device descriptor arrays.  The
-// groups run on branch streams forked from the caller's stream, and the whole fork
-// is captured once into a graph and replayed while the RF and plane pointers stay
-// the same.
+// A batch decodes the whole blocks of many segments of mixed formats and sample rates.
+// Every block gets a global index; per-block arrays (input and output offsets, flags,
+// stash and scratch) are indexed by it, and each context contributes one argument
+// record per kernel kind pointing at those shared arrays.  Each kernel kind is then
+// ONE launch in table mode over all blocks that need it, whatever their format
+// (workgroup i -> global block ids[i] -> record rec[block]): demod (plain, RF boost,
+// LaserDisc), repair, video, sub-deemphasis and chroma (per section count).  The
+// launch sequence is captured once into a HIP graph and replayed while the RF and
+// plane pointers stay the same; the records are rebuilt (outside any capture) when
+// they change.
 struct vhsdecode_hip_batch {
-  struct Group {
-    vhsdecode_hip_demod_context* c = nullptr;
-    int nblocks = 0;
-    std::vector<long long> h_starts, h_offs;
-    DevBuf<long long> starts, offs;
-    DevBuf<unsigned> flags;
-    DevBuf<float2> stash, aux, rep_a;
-    DevBuf<float> rep_dm;
-    hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+  struct Kind {
+    int kernel = 0, sections = 0;
+    std::vector<int> ids;
+    DevBuf<int> d_ids;
   };
   std::mutex mu;
   int device = 0;
-  std::vector<Group> groups;
-  long long blocks = 0, out_end = 0, rf_end = 0, audio_div = 1;
+  std::vector<vhsdecode_hip_demod_context*> groups;  // one per context
+  std::vector<int> h_rec;                            // block -> group
+  std::vector<long long> h_starts, h_offs;
+  DevBuf<int> rec;
+  DevBuf<long long> starts, offs;
+  DevBuf<unsigned> flags;
+  DevBuf<float2> stash, aux, rep_a;
+  DevBuf<float> rep_dm;
+  DevBuf<vhsd::DemodArgs> r_demod;
+  DevBuf<vhsd::RepairArgs> r_repair;
+  DevBuf<vhsd::SubDeemphArgs> r_sd;
+  DevBuf<vhsd::ChromaArgs> r_chroma;
+  std::vector<Kind> kinds;
+  long long blocks = 0, out_end = 0, rf_end = 0;
   bool use_graph = false;
-  hipStream_t capture = nullptr;
-  hipEvent_t fork = nullptr;
+  hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   const void* key_rf = nullptr;
   vhsdecode_hip_span_v1 key_out{};
+  bool have_key = false;
 
   ~vhsdecode_hip_batch() {
     (void)hipSetDevice(device);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (auto& g : groups) {
-      g.starts.release(); g.offs.release(); g.flags.release(); g.stash.release(); g.aux.release();
-      g.rep_a.release(); g.rep_dm.release();
-      if (g.done) (void)hipEventDestroy(g.done);
-      if (g.stream) (void)hipStreamDestroy(g.stream);
-    }
-    if (fork) (void)hipEventDestroy(fork);
-    if (capture) (void)hipStreamDestroy(capture);
+    rec.release(); starts.release(); offs.release(); flags.release(); stash.release(); aux.release();
+    rep_a.release(); rep_dm.release(); r_demod.release(); r_repair.release(); r_sd.release(); r_chroma.release();
+    for (auto& k : kinds) k.d_ids.release();
+    if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
 namespace {
 
-// Enqueue every group on its branch stream, forked from and joined into `st`.
-int32_t batch_enqueue(vhsdecode_hip_batch* b, const int16_t* rf, const vhsdecode_hip_span_v1* out, hipStream_t st) {
-  VHSD_HIP(hipEventRecord(b->fork, st));
-  for (auto& g : b->groups) {
-    if (g.nblocks == 0) continue;
-    VHSD_HIP(hipStreamWaitEvent(g.stream, b->fork, 0));
-    const bool prof = g.c->profiling;
-    g.c->profiling = false;
-    const int32_t rc = demod_blocks_locked(g.c, rf, g.starts.p, g.offs.p, g.nblocks, out, g.flags.p, g.stash.p,
-                                           g.aux.p, g.rep_a.p, g.rep_dm.p, g.stream);
-    g.c->profiling = prof;
-    if (rc != VHSDECODE_HIP_OK) return rc;
-    VHSD_HIP(hipEventRecord(g.done, g.stream));
-    VHSD_HIP(hipStreamWaitEvent(st, g.done, 0));
+bool same_span(const vhsdecode_hip_span_v1& a, const vhsdecode_hip_span_v1& b) {
+  return a.video == b.video && a.video_lowpass == b.video_lowpass && a.envelope == b.envelope &&
+         a.chroma == b.chroma && a.pilot == b.pilot && a.audio_left == b.audio_left && a.audio_right == b.audio_right;
+}
+
+// Rebuild the records for these pointers and the per-kind block lists (synchronous).
+int32_t batch_prepare(vhsdecode_hip_batch* b, const int16_t* rf, const vhsdecode_hip_span_v1* out) {
+  const size_t g = b->groups.size();
+  std::vector<ChainArgs> ca(g);
+  std::vector<vhsd::DemodArgs> d(g);
+  std::vector<vhsd::RepairArgs> r(g);
+  std::vector<vhsd::SubDeemphArgs> sd(g);
+  std::vector<vhsd::ChromaArgs> ch(g);
+  for (size_t i = 0; i < g; i++) {
+    build_chain(b->groups[i], rf, b->starts.p, b->offs.p, out, b->flags.p, b->stash.p, b->aux.p, b->rep_a.p,
+                b->rep_dm.p, &ca[i]);
+    d[i] = ca[i].d;
+    r[i] = ca[i].r;
+    sd[i] = ca[i].sd;
+    ch[i] = ca[i].ch;
+  }
+  VHSD_HIP(hipMemcpy(b->r_demod.p, d.data(), sizeof(vhsd::DemodArgs) * g, hipMemcpyHostToDevice));
+  VHSD_HIP(hipMemcpy(b->r_repair.p, r.data(), sizeof(vhsd::RepairArgs) * g, hipMemcpyHostToDevice));
+  VHSD_HIP(hipMemcpy(b->r_sd.p, sd.data(), sizeof(vhsd::SubDeemphArgs) * g, hipMemcpyHostToDevice));
+  VHSD_HIP(hipMemcpy(b->r_chroma.p, ch.data(), sizeof(vhsd::ChromaArgs) * g, hipMemcpyHostToDevice));
+  // launch order: demod kinds, repair, video, sub-deemphasis, chroma by section count
+  std::vector<vhsdecode_hip_batch::Kind> kinds;
+  auto add = [&](int kernel, int sections, auto pred) {
+    vhsdecode_hip_batch::Kind k;
+    k.kernel = kernel;
+    k.sections = sections;
+    for (long long blk = 0; blk < b->blocks; blk++)
+      if (pred(ca[b->h_rec[blk]])) k.ids.push_back((int)blk);
+    if (!k.ids.empty()) kinds.push_back(std::move(k));
+  };
+  for (int kind : {vhsd::kTableDemod, vhsd::kTableDemodBoost, vhsd::kTableDemodLd})
+    add(kind, 0, [kind](const ChainArgs& x) { return x.demod_kind == kind; });
+  add(vhsd::kTableRepair, 0, [](const ChainArgs& x) { return x.repair; });
+  add(vhsd::kTableVideo, 0, [](const ChainArgs&) { return true; });
+  add(vhsd::kTableSubDeemph, 0, [](const ChainArgs& x) { return x.subdeemph; });
+  for (int sec : {1, 2, 3, 4, 10})
+    add(vhsd::kTableChroma, sec, [sec](const ChainArgs& x) { return x.chroma && x.chroma_sections == sec; });
+  for (auto& k : b->kinds) k.d_ids.release();
+  b->kinds = std::move(kinds);
+  for (auto& k : b->kinds) {
+    VHSD_HIP(k.d_ids.ensure(k.ids.size()));
+    VHSD_HIP(hipMemcpy(k.d_ids.p, k.ids.data(), sizeof(int) * k.ids.size(), hipMemcpyHostToDevice));
   }
   return VHSDECODE_HIP_OK;
 }
 
-bool same_span(const vhsdecode_hip_span_v1& a, const vhsdecode_hip_span_v1& b) {
-  return a.video == b.video && a.video_lowpass == b.video_lowpass && a.envelope == b.envelope &&
-         a.chroma == b.chroma && a.pilot == b.pilot && a.audio_left == b.audio_left && a.audio_right == b.audio_right;
+// Kernel launches only: safe to capture.
+int32_t batch_enqueue(vhsdecode_hip_batch* b, hipStream_t st) {
+  VHSD_HIP(vhsd_launch_fill_u32(b->flags.p, b->blocks, 0u, st));
+  for (const auto& k : b->kinds) {
+    const void* recs = k.kernel == vhsd::kTableRepair      ? (const void*)b->r_repair.p
+                       : k.kernel == vhsd::kTableSubDeemph ? (const void*)b->r_sd.p
+                       : k.kernel == vhsd::kTableChroma    ? (const void*)b->r_chroma.p
+                                                           : (const void*)b->r_demod.p;
+    VHSD_HIP(vhsd_launch_block_table(k.kernel, k.sections, recs, vhsd::BlockTable{k.d_ids.p, b->rec.p},
+                                     (int)k.ids.size(), st));
+  }
+  return VHSDECODE_HIP_OK;
 }
 
 }  // namespace
@@ -927,6 +997,7 @@ int32_t vhsdecode_hip_batch_create(const vhsdecode_hip_batch_config_v1* cfg, vhs
     auto b = std::make_unique<vhsdecode_hip_batch>();
     b->use_graph = cfg->use_graph != 0;
     std::vector<std::pair<long long, long long>> ranges;
+    bool need_aux = false, need_rep = false;
     for (int i = 0; i < cfg->count; i++) {
       const vhsdecode_hip_batch_segment_v1& sg = cfg->segments[i];
       if (sg.struct_size != sizeof(vhsdecode_hip_batch_segment_v1))
@@ -934,52 +1005,55 @@ int32_t vhsdecode_hip_batch_create(const vhsdecode_hip_batch_config_v1* cfg, vhs
       vhsdecode_hip_demod_context* c = sg.context;
       if (!c || sg.rf_offset < 0 || (sg.rf_offset & 1) || sg.length < 0 || sg.out_offset < 0)
         return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "segment needs a context, an even rf_offset and non-negative sizes");
-      if (i > 0 && c->device != b->device) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "contexts on different devices");
-      b->device = c->device;
+      if (!b->groups.empty() && c->device != b->device)
+        return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "contexts on different devices");
       // whole blocks only, as RfBlockStreamDecoder reads them (a short read ends the stream)
       const long long nb = sg.length >= c->n ? (sg.length - c->n) / c->stride + 1 : 0;
       if (nb == 0) continue;
       if (c->has_audio && (sg.out_offset % c->audio_dec))
         return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "LaserDisc audio needs out_offset divisible by the decimation");
+      b->device = c->device;
       size_t gi = 0;
-      while (gi < b->groups.size() && b->groups[gi].c != c) gi++;
-      if (gi == b->groups.size()) {
-        b->groups.emplace_back();
-        b->groups.back().c = c;
-      }
-      auto& g = b->groups[gi];
+      while (gi < b->groups.size() && b->groups[gi] != c) gi++;
+      if (gi == b->groups.size()) b->groups.push_back(c);
+      need_aux |= c->has_boost || c->has_notch || c->has_audio;
+      need_rep |= !c->ld && c->diff_max > 0.0;
       for (long long k = 0; k < nb; k++) {
-        g.h_starts.push_back(sg.rf_offset + k * c->stride);
-        g.h_offs.push_back(sg.out_offset + k * c->stride);
+        b->h_starts.push_back(sg.rf_offset + k * c->stride);
+        b->h_offs.push_back(sg.out_offset + k * c->stride);
+        b->h_rec.push_back((int)gi);
       }
       ranges.emplace_back(sg.out_offset, sg.out_offset + nb * c->stride);
       b->out_end = std::max(b->out_end, sg.out_offset + nb * c->stride);
       b->rf_end = std::max(b->rf_end, sg.rf_offset + (nb - 1) * c->stride + c->n);
-      b->blocks += nb;
     }
+    b->blocks = (long long)b->h_starts.size();
+    if (b->blocks > (1LL << 30)) return fail(VHSDECODE_HIP_E_CAPACITY, "too many blocks in one batch");
     std::sort(ranges.begin(), ranges.end());
     for (size_t i = 1; i < ranges.size(); i++)
       if (ranges[i].first < ranges[i - 1].second)
         return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "segment output ranges overlap");
     VHSD_HIP(hipSetDevice(b->device));
-    VHSD_HIP(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
-    VHSD_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
-    for (auto& g : b->groups) {
-      g.nblocks = (int)g.h_starts.size();
-      const size_t nb = (size_t)g.nblocks;
-      VHSD_HIP(g.starts.ensure(nb));
-      VHSD_HIP(g.offs.ensure(nb));
-      VHSD_HIP(g.flags.ensure(nb));
-      VHSD_HIP(g.stash.ensure(nb * vhsd::kStashEntries));
-      if (g.c->has_boost || g.c->has_notch || g.c->has_audio) VHSD_HIP(g.aux.ensure(nb * vhsd::kAuxEntries));
-      if (g.c->diff_max > 0.0) {
-        VHSD_HIP(g.rep_a.ensure(nb * 32768));
-        VHSD_HIP(g.rep_dm.ensure(nb * 65536));
-      }
-      VHSD_HIP(hipMemcpy(g.starts.p, g.h_starts.data(), sizeof(long long) * nb, hipMemcpyHostToDevice));
-      VHSD_HIP(hipMemcpy(g.offs.p, g.h_offs.data(), sizeof(long long) * nb, hipMemcpyHostToDevice));
-      VHSD_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
-      VHSD_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+    VHSD_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    const size_t nb = (size_t)std::max<long long>(b->blocks, 1), ng = std::max<size_t>(b->groups.size(), 1);
+    VHSD_HIP(b->starts.ensure(nb));
+    VHSD_HIP(b->offs.ensure(nb));
+    VHSD_HIP(b->rec.ensure(nb));
+    VHSD_HIP(b->flags.ensure(nb));
+    VHSD_HIP(b->stash.ensure(nb * vhsd::kStashEntries));
+    if (need_aux) VHSD_HIP(b->aux.ensure(nb * vhsd::kAuxEntries));
+    if (need_rep) {
+      VHSD_HIP(b->rep_a.ensure(nb * 32768));
+      VHSD_HIP(b->rep_dm.ensure(nb * 65536));
+    }
+    VHSD_HIP(b->r_demod.ensure(ng));
+    VHSD_HIP(b->r_repair.ensure(ng));
+    VHSD_HIP(b->r_sd.ensure(ng));
+    VHSD_HIP(b->r_chroma.ensure(ng));
+    if (b->blocks > 0) {
+      VHSD_HIP(hipMemcpy(b->starts.p, b->h_starts.data(), sizeof(long long) * b->blocks, hipMemcpyHostToDevice));
+      VHSD_HIP(hipMemcpy(b->offs.p, b->h_offs.data(), sizeof(long long) * b->blocks, hipMemcpyHostToDevice));
+      VHSD_HIP(hipMemcpy(b->rec.p, b->h_rec.data(), sizeof(int) * b->blocks, hipMemcpyHostToDevice));
     }
     *out = b.release();
     return VHSDECODE_HIP_OK;
@@ -1007,36 +1081,41 @@ int32_t vhsdecode_hip_batch_run_device(vhsdecode_hip_batch* b, const int16_t* rf
       return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "the video, video_lowpass and envelope planes are required");
     if (rf_count < b->rf_end) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "rf buffer is shorter than the segments");
     std::lock_guard<std::mutex> lock(b->mu);
+    if (b->blocks == 0) return VHSDECODE_HIP_OK;
     VHSD_HIP(hipSetDevice(b->device));
-    // the contexts' own calls are serialized against the enqueue
-    std::vector<vhsdecode_hip_demod_context*> ctxs;
-    for (auto& g : b->groups) ctxs.push_back(g.c);
-    std::sort(ctxs.begin(), ctxs.end());
-    std::vector<std::unique_lock<std::mutex>> locks;
-    for (auto* c : ctxs) locks.emplace_back(c->mu);
-    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : b->capture;
-    if (!b->use_graph) return batch_enqueue(b, rf, out, st);
-    if (!b->exec || b->key_rf != rf || !same_span(b->key_out, *out)) {
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : b->stream;
+    if (!b->have_key || b->key_rf != rf || !same_span(b->key_out, *out)) {
+      // the previous run may still read the records: let it finish before rewriting them
+      VHSD_HIP(hipDeviceSynchronize());
+      const int32_t rc = batch_prepare(b, rf, out);
+      if (rc != VHSDECODE_HIP_OK) return rc;
       if (b->exec) (void)hipGraphExecDestroy(b->exec);
       if (b->graph) (void)hipGraphDestroy(b->graph);
       b->exec = nullptr;
       b->graph = nullptr;
-      VHSD_HIP(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
-      const int32_t rc = batch_enqueue(b, rf, out, b->capture);
-      hipGraph_t graph = nullptr;
-      const hipError_t ce = hipStreamEndCapture(b->capture, &graph);
-      if (rc != VHSDECODE_HIP_OK) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return rc;
-      }
-      VHSD_HIP(ce);
-      b->graph = graph;
-      VHSD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
       b->key_rf = rf;
       b->key_out = *out;
+      b->have_key = true;
+      if (b->use_graph) {
+        VHSD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int32_t erc = batch_enqueue(b, b->stream);
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+        if (erc != VHSDECODE_HIP_OK) {
+          if (graph) (void)hipGraphDestroy(graph);
+          b->have_key = false;
+          return erc;
+        }
+        VHSD_HIP(ce);
+        b->graph = graph;
+        VHSD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      }
     }
-    VHSD_HIP(hipGraphLaunch(b->exec, st));
-    return VHSDECODE_HIP_OK;
+    if (b->use_graph) {
+      VHSD_HIP(hipGraphLaunch(b->exec, st));
+      return VHSDECODE_HIP_OK;
+    }
+    return batch_enqueue(b, st);
   } catch (const std::exception& e) {
     return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
   }
@@ -1047,9 +1126,8 @@ void vhsdecode_hip_batch_destroy(vhsdecode_hip_batch* b) {
   {
     std::lock_guard<std::mutex> lock(b->mu);
     (void)hipSetDevice(b->device);
-    for (auto& g : b->groups)
-      if (g.stream) (void)hipStreamSynchronize(g.stream);
-    if (b->capture) (void)hipStreamSynchronize(b->capture);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    (void)hipDeviceSynchronize();
   }
   delete b;
 }
@@ -1607,6 +1685,20 @@ int32_t vhsdecode_hip_get_profile(vhsdecode_hip_demod_context* c, vhsdecode_hip_
       c->prof_ms[k] = 0.0;
       c->prof_units[k] = 0;
     }
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_debug_replace_spikes(vhsdecode_hip_demod_context* c, const float* demod, const float* diffed,
+                                           double max_value, float* out, void* stream) {
+  try {
+    if (!c || !demod || !diffed || !out) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid spike request");
+    std::lock_guard<std::mutex> lock(c->mu);
+    VHSD_HIP(hipSetDevice(c->device));
+    VHSD_HIP(c->dbg_list.ensure(32768));
+    VHSD_HIP(vhsd_launch_debug_replace_spikes(demod, diffed, c->dbg_list.p, max_value, out, pick_stream(c, stream)));
     return VHSDECODE_HIP_OK;
   } catch (const std::exception& e) {
     return fail(VHSDECODE_HIP_E_INTERNAL, e.what());

@@ -39,6 +39,7 @@ REQUIRED_EXPORTS = (
     "vhsdecode_hip_set_profiling",
     "vhsdecode_hip_get_profile",
     "vhsdecode_hip_debug_fft16k",
+    "vhsdecode_hip_debug_replace_spikes",
     "vhsdecode_hip_batch_create",
     "vhsdecode_hip_batch_info",
     "vhsdecode_hip_batch_run_device",
@@ -258,6 +259,8 @@ def load():
                                                  ctypes.POINTER(SpanV1), ctypes.c_void_p]
     L.vhsdecode_hip_batch_destroy.argtypes = [ctypes.c_void_p]
     L.vhsdecode_hip_batch_destroy.restype = None
+    L.vhsdecode_hip_debug_replace_spikes.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                                     ctypes.c_void_p, ctypes.c_void_p]
     L.vhsdecode_hip_hifi_create.argtypes = [ctypes.POINTER(HifiConfigV1), ctypes.POINTER(ctypes.c_void_p)]
     L.vhsdecode_hip_hifi_destroy.argtypes = [ctypes.c_void_p]
     L.vhsdecode_hip_hifi_destroy.restype = None
