@@ -370,6 +370,8 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=16)
     ap.add_argument("--serve-window", type=float, default=2.0, help="window length (s of RF)")
     ap.add_argument("--serve-burst", type=int, default=32)
+    ap.add_argument("--pipeline", type=int, default=3,
+                    help="spans in flight: consecutive steps alternate between this many pipelines")
     ap.add_argument("--batch-segments", type=int, default=64, help="batched-segment run (0: skip)")
     ap.add_argument("--ld-fields", type=int, default=20, help="LaserDisc + HiFi run (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -386,17 +388,36 @@ def main():
     fmt, tape, fl, nb = build_workload(args.fields)
     lut = np.load(os.path.join(ROOT, "vhs-decode-dotnet_amd", "vhsdecode_hip", "data",
                                "sinc_lut.npz"))["downscale_sinc_lut"]
-    dem = HipBlockDemodulator(fl, max_blocks=nb)
-    dem.set_sinc_table(lut)
-    dec = field.HipFieldDecoder(dem, fmt)
-    g = dec.geom
     span = nb * STRIDE
     rf = hipmem.DeviceBuffer.from_array(tape.rf)
-    planes = hipmem.DeviceBuffer(4 * span * 4)
-    flags = hipmem.DeviceBuffer(nb * 4)
-    video, vlp, env, chroma = (planes.ptr + k * span * 4 for k in range(4))
     max_fields = args.fields + 1
-    out = hipmem.DeviceBuffer(max_fields * g.field_lines * g.out_len * 2)
+
+    class Pipe:
+        """One decode pipeline: its own demod context (stash, descriptors, field scratch),
+        span planes, field outputs and stream."""
+
+        def __init__(self, own_stream: bool):
+            self.dem = HipBlockDemodulator(fl, max_blocks=nb)
+            self.dem.set_sinc_table(lut)
+            self.dec = field.HipFieldDecoder(self.dem, fmt)
+            g = self.dec.geom
+            self.planes = hipmem.DeviceBuffer(4 * span * 4)
+            self.flags = hipmem.DeviceBuffer(nb * 4)
+            self.out = hipmem.DeviceBuffer(max_fields * g.field_lines * g.out_len * 2)
+            self.meta = hipmem.DeviceBuffer(64)
+            self.stream = hipmem.Stream() if own_stream else None
+            self.p = [self.planes.ptr + k * span * 4 for k in range(4)]
+
+        def step(self):
+            # every stage on the device, no host round trip inside the step
+            st = self.stream.handle if self.stream else 0
+            self.dem.demod_span_device(rf.ptr, 0, nb, *self.p, self.flags.ptr, stream=st)
+            self.dec.decode_async(self.p[0], self.p[1], span, self.out.ptr, max_fields, self.meta.ptr, stream=st)
+
+    # Consecutive spans alternate between pipelines on their own streams, so one span's
+    # field decode (small latency-bound kernels) overlaps the next span's block kernels.
+    pipes = [Pipe(args.pipeline > 1) for _ in range(max(1, args.pipeline))]
+    dem = pipes[0].dem
 
     if args.copy_mib > 0:
         a = hipmem.DeviceBuffer(args.copy_mib << 20)
@@ -404,25 +425,18 @@ def main():
         hipmem.copy_device(b.ptr, a.ptr, args.copy_mib << 20)
         del a, b
 
-    meta = hipmem.DeviceBuffer(64)
-
-    def step():
-        # every stage on the device, no host round trip inside the step
-        dem.demod_span_device(rf.ptr, 0, nb, video, vlp, env, chroma, flags.ptr)
-        dec.decode_async(video, vlp, span, out.ptr, max_fields, meta.ptr)
-
-    for _ in range(args.warmup):
-        step()
+    for i in range(args.warmup):
+        pipes[i % len(pipes)].step()
     hipmem.synchronize()
     dem.set_profiling(True)
     dem.take_profile()
     hipmem.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
+    for i in range(args.steps):
+        pipes[i % len(pipes)].step()
     hipmem.synchronize()
     t1 = time.perf_counter()
-    nfields = int(meta.download(np.int32, 1)[0])
+    nfields = int(pipes[0].meta.download(np.int32, 1)[0])
     prof = dem.take_profile()
     dem.set_profiling(False)
 
@@ -465,7 +479,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"NTSC VHS 40 MSPS int16, {args.fields} fields, 28 dB SNR, 20 ns 1/f line jitter",
                    "rf_samples_per_step": rf_samples, "blocks_per_step": nb, "fields_per_step": nfields,
-                   "block_length": N, "parallelism": "single GPU"},
+                   "block_length": N, "parallelism": "single GPU",
+                   "spans_in_flight": len(pipes)},
         "roofline": {"bound": "hbm", "kernel": kernel_name, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "bytes_per_launch": launch_bytes, "avg_launch_ms": round(demod_ms, 5),
@@ -483,7 +498,8 @@ def main():
         print(f"[bench] laserdisc: {ldr['fields']} fields {ldr['msps']} MSPS; hifi {ldr['hifi']['if_msps']} IF MSPS",
               file=sys.stderr)
     if args.serve_seconds > 0:
-        dem.close()
+        for p in pipes:
+            p.dem.close()
         sv = served_run(args.serve_seconds, args.serve_rate, args.serve_streams, args.serve_window, SEED, lut,
                         args.serve_burst)
         result["serving"] = sv

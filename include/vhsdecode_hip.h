@@ -180,6 +180,10 @@ typedef struct vhsdecode_hip_span_v1 {
   float* pilot;                    /* LaserDisc PAL pilot reference (absolute Hz), NULL: not written */
   float* audio_left;               /* LaserDisc analog audio, nblocks * stride / decimation samples each (Hz), */
   float* audio_right;              /* NULL: not written */
+  void* chroma_stream;             /* device entry only: non-NULL computes the chroma plane on this stream,
+                                      after the work already enqueued on the call's stream, so it overlaps the
+                                      rest of the chain and later work; the caller orders its own uses of the
+                                      plane.  The context's next span call waits for it. NULL: the call's stream */
 } vhsdecode_hip_span_v1;
 
 VHSDECODE_HIP_API int32_t vhsdecode_hip_demod_create(const vhsdecode_hip_demod_config_v1* config,

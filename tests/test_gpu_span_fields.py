@@ -74,3 +74,56 @@ def test_device_locator_respects_max_fields(span):
     fields, count, line0, first, a, b = _run_both(span, 1)
     assert count == 1 == len(fields)
     np.testing.assert_array_equal(a[:1], b[:1])
+
+
+def test_chroma_on_its_own_stream_is_identical(span):
+    """span_v1.chroma_stream: the chroma plane computed on another stream (overlapping
+    the rest of the chain) equals the one computed in order; the context's next call
+    waits for it before reusing its descriptors."""
+    dem, n = span["dem"], span["n"]
+    rf, planes, flags = span["keep"]
+    nb = n // STRIDE
+    base = planes.download(np.float32, 4 * n).reshape(4, n)
+    side = hipmem.Stream()
+    other = hipmem.DeviceBuffer(4 * n * 4)
+    other.fill(0xFF)
+    p = [other.ptr + k * n * 4 for k in range(4)]
+    for _ in range(2):
+        dem.demod_span_device(rf.ptr, 0, nb, p[0], p[1], p[2], p[3], flags.ptr, chroma_stream=side.handle)
+    hipmem.synchronize()
+    got = other.download(np.float32, 4 * n).reshape(4, n)
+    np.testing.assert_array_equal(got, base)
+
+
+def test_pipelines_on_separate_streams_match(span):
+    """Spans in flight: two contexts decoding on their own streams, their kernels
+    interleaving on the device, give the same fields as one decode in order."""
+    fmt, n = span["fmt"], span["n"]
+    rf, _, _ = span["keep"]
+    nb = n // STRIDE
+    ref_dec = field.HipFieldDecoder(span["dem"], fmt)
+    g = ref_dec.geom
+    per = g.field_lines * g.out_len
+    pipes = []
+    for _ in range(2):
+        d = HipBlockDemodulator(span["dem"].filters, max_blocks=nb)
+        d.set_sinc_table(np.load(_lut_path())["downscale_sinc_lut"])
+        planes = hipmem.DeviceBuffer(4 * n * 4)
+        pipes.append(dict(dem=d, dec=field.HipFieldDecoder(d, fmt), planes=planes,
+                          p=[planes.ptr + k * n * 4 for k in range(4)], flags=hipmem.DeviceBuffer(nb * 4),
+                          out=hipmem.DeviceBuffer(4 * per * 2), meta=hipmem.DeviceBuffer(64), st=hipmem.Stream()))
+    for rep in range(3):
+        for q in pipes:
+            q["dem"].demod_span_device(rf.ptr, 0, nb, *q["p"], q["flags"].ptr, stream=q["st"].handle)
+            q["dec"].decode_async(q["p"][0], q["p"][1], n, q["out"].ptr, 4, q["meta"].ptr, stream=q["st"].handle)
+    hipmem.synchronize()
+    outs = [q["out"].download(np.uint16, 4 * per) for q in pipes]
+    counts = [int(q["meta"].download(np.int32, 1)[0]) for q in pipes]
+    assert counts[0] == counts[1] >= 1
+    np.testing.assert_array_equal(outs[0][:counts[0] * per], outs[1][:counts[0] * per])
+    # and equal to the in-order decode of the fixture's planes
+    ref = hipmem.DeviceBuffer(4 * per * 2)
+    meta = hipmem.DeviceBuffer(64)
+    ref_dec.decode_async(span["video"], span["vlp"], n, ref.ptr, 4, meta.ptr)
+    hipmem.synchronize()
+    np.testing.assert_array_equal(ref.download(np.uint16, 4 * per)[:counts[0] * per], outs[0][:counts[0] * per])

@@ -282,6 +282,9 @@ struct vhsdecode_hip_demod_context {
   void* f_stage = nullptr;   // pinned host staging for the location upload
   size_t f_stage_bytes = 0;
   hipEvent_t f_stage_done = nullptr;
+  // chroma on a caller stream (span_v1.chroma_stream): fork point and completion
+  hipEvent_t chroma_fork = nullptr, chroma_done = nullptr;
+  bool chroma_pending = false;
   // host-variant staging
   DevBuf<int16_t> s_rf;
   DevBuf<float> s_planes;
@@ -314,6 +317,8 @@ struct vhsdecode_hip_demod_context {
     f_fo.release(); f_tab.release();
     if (f_stage) (void)hipHostFree(f_stage);
     if (f_stage_done) (void)hipEventDestroy(f_stage_done);
+    if (chroma_fork) (void)hipEventDestroy(chroma_fork);
+    if (chroma_done) (void)hipEventDestroy(chroma_done);
     for (auto& r : recs) {
       (void)hipEventDestroy(r.a);
       (void)hipEventDestroy(r.b);
@@ -524,6 +529,19 @@ int32_t demod_blocks_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, c
                             hipStream_t st) {
   ChainArgs ca;
   build_chain(c, rf, starts, offs, out, flags, stash, aux, rep_a, rep_dm, &ca);
+  // chroma only reads the RF: on a caller-supplied stream it overlaps the rest of the
+  // chain and whatever the caller enqueues next (the field decode)
+  hipStream_t cs = out->chroma_stream ? reinterpret_cast<hipStream_t>(out->chroma_stream) : nullptr;
+  if (ca.chroma && cs) {
+    if (!c->chroma_fork) VHSD_HIP(hipEventCreateWithFlags(&c->chroma_fork, hipEventDisableTiming));
+    if (!c->chroma_done) VHSD_HIP(hipEventCreateWithFlags(&c->chroma_done, hipEventDisableTiming));
+    VHSD_HIP(hipEventRecord(c->chroma_fork, st));
+    VHSD_HIP(hipStreamWaitEvent(cs, c->chroma_fork, 0));
+    ProfScope ps(c, cs, VHSDECODE_HIP_KERNEL_CHROMA, nblocks);
+    VHSD_HIP(vhsd_launch_block_chroma(&ca.ch, nblocks, cs));
+    VHSD_HIP(hipEventRecord(c->chroma_done, cs));
+    c->chroma_pending = true;
+  }
   VHSD_HIP(vhsd_launch_fill_u32(flags, nblocks, 0u, st));
   {
     ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_BLOCK_DEMOD, nblocks);
@@ -546,7 +564,7 @@ int32_t demod_blocks_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, c
     ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_SUB_DEEMPHASIS, nblocks);
     VHSD_HIP(vhsd_launch_block_subdeemph(&ca.sd, nblocks, st));
   }
-  if (ca.chroma) {
+  if (ca.chroma && !cs) {
     ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_CHROMA, nblocks);
     VHSD_HIP(vhsd_launch_block_chroma(&ca.ch, nblocks, st));
   }
@@ -580,6 +598,11 @@ int32_t demod_span_locked(vhsdecode_hip_demod_context* c, const int16_t* rf, int
   }
   int32_t rc = reserve_block_scratch(c, nblocks);
   if (rc != VHSDECODE_HIP_OK) return rc;
+  if (c->chroma_pending) {
+    // a chroma launched on another stream may still read the descriptor arrays
+    VHSD_HIP(hipStreamWaitEvent(st, c->chroma_done, 0));
+    c->chroma_pending = false;
+  }
   VHSD_HIP(hipMemcpyAsync(c->block_start.p, starts.data(), sizeof(long long) * nblocks, hipMemcpyHostToDevice, st));
   VHSD_HIP(hipMemcpyAsync(c->out_offset.p, offs.data(), sizeof(long long) * nblocks, hipMemcpyHostToDevice, st));
   unsigned* flags = out->block_flags ? out->block_flags : c->flags.p;

@@ -122,7 +122,7 @@ class DemodConfigV1(ctypes.Structure):
 class SpanV1(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("video", _fp), ("video_lowpass", _fp), ("envelope", _fp),
                 ("chroma", _fp), ("block_flags", _u32p), ("pilot", _fp), ("audio_left", _fp),
-                ("audio_right", _fp)]
+                ("audio_right", _fp), ("chroma_stream", ctypes.c_void_p)]
 
 
 class HsyncParamsV1(ctypes.Structure):

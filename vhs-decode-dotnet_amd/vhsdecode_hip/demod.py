@@ -234,7 +234,8 @@ class HipBlockDemodulator:
 
     def demod_span_device(self, rf_ptr: int, first_sample: int, nblocks: int, video_ptr: int, vlp_ptr: int,
                           env_ptr: int, chroma_ptr: int = 0, flags_ptr: int = 0, stream: int = 0,
-                          pilot_ptr: int = 0, audio_left_ptr: int = 0, audio_right_ptr: int = 0):
+                          pilot_ptr: int = 0, audio_left_ptr: int = 0, audio_right_ptr: int = 0,
+                          chroma_stream: int = 0):
         """Device-pointer entry (HBM addresses, e.g. hipmem.DeviceBuffer.ptr); asynchronous on `stream`.
         LaserDisc: chroma_ptr receives the burst reference, pilot_ptr the PAL pilot and the audio
         pointers nblocks * stride / decimation samples each."""
@@ -249,6 +250,7 @@ class HipBlockDemodulator:
         if audio_left_ptr and audio_right_ptr:
             span.audio_left = ctypes.cast(ctypes.c_void_p(audio_left_ptr), _lib._fp)
             span.audio_right = ctypes.cast(ctypes.c_void_p(audio_right_ptr), _lib._fp)
+        span.chroma_stream = chroma_stream or None
         _lib.check(self._L.vhsdecode_hip_demod_span_device(self._h, ctypes.c_void_p(rf_ptr), first_sample, nblocks,
                                                            ctypes.byref(span), ctypes.c_void_p(stream)))
 
