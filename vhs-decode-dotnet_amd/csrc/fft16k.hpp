@@ -11,7 +11,9 @@
 // Twiddles come from an LDS table of the first octant of the 16384-th roots of
 // unity (oct[u] = (cos, sin)(2*pi*u/16384), u = 0..2048), expanded by octant
 // symmetry, so every 16384-th root is a table value up to exact sign/swap
-// operations.  Odd 32768-th roots take one extra multiply by exp(-i*pi/16384).
+// operations.  A radix-16 pass looks up w, w^2, w^4 and w^8 and forms the other
+// eleven powers as products of at most three table values; the radix-4 pass looks
+// up w and w^2.  Odd 32768-th roots take one extra multiply by exp(-i*pi/16384).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -131,10 +133,31 @@ __device__ __forceinline__ void fft16k_forward(c32* v, c32* __restrict__ buf, co
     const int ns = 1 << (4 * pass);
     const int k = j & (ns - 1);
     if (pass > 0) {
-      // twiddle W_{16 ns}^{k r} = exp(-2 pi i k r / (16 ns)) = tw16k(k r * 16384 / (16 ns))
-      const int stride = kFftM / (16 * ns);
-#pragma unroll
-      for (int r = 1; r < 16; r++) v[r] = mul(v[r], tw16k(oct, k * r * stride));
+      // twiddle W_{16 ns}^{k r} = exp(-2 pi i k r / (16 ns)) = tw16k(k r * 16384 / (16 ns)):
+      // table values for r = 1, 2, 4, 8, the others as products of at most three of them
+      const int t1 = k * (kFftM / (16 * ns));
+      const c32 w1 = tw16k(oct, t1), w2 = tw16k(oct, 2 * t1);
+      const c32 w3 = mul(w2, w1);
+      v[1] = mul(v[1], w1);
+      v[2] = mul(v[2], w2);
+      v[3] = mul(v[3], w3);
+      {
+        const c32 w4 = tw16k(oct, 4 * t1);
+        v[4] = mul(v[4], w4);
+        v[5] = mul(v[5], mul(w4, w1));
+        v[6] = mul(v[6], mul(w4, w2));
+        v[7] = mul(v[7], mul(w4, w3));
+        const c32 w8 = tw16k(oct, 8 * t1);
+        v[8] = mul(v[8], w8);
+        v[9] = mul(v[9], mul(w8, w1));
+        v[10] = mul(v[10], mul(w8, w2));
+        v[11] = mul(v[11], mul(w8, w3));
+        const c32 w12 = mul(w8, w4);
+        v[12] = mul(v[12], w12);
+        v[13] = mul(v[13], mul(w12, w1));
+        v[14] = mul(v[14], mul(w12, w2));
+        v[15] = mul(v[15], mul(w12, w3));
+      }
     }
     dft16(v);
     // lds_pad(base + c*ns) == lds_pad(base) + c*ns + ((c*ns) >> 5) in every pass (the low
@@ -153,8 +176,8 @@ __device__ __forceinline__ void fft16k_forward(c32* v, c32* __restrict__ buf, co
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     const int b = j + kFftThreads * q;
-    c32 a0 = v[q], a1 = mul(v[q + 4], tw16k(oct, b)), a2 = mul(v[q + 8], tw16k(oct, 2 * b)),
-        a3 = mul(v[q + 12], tw16k(oct, 3 * b));
+    const c32 w1 = tw16k(oct, b), w2 = tw16k(oct, 2 * b);
+    c32 a0 = v[q], a1 = mul(v[q + 4], w1), a2 = mul(v[q + 8], w2), a3 = mul(v[q + 12], mul(w2, w1));
     dft4(a0, a1, a2, a3);
     v[q] = a0;
     v[q + 4] = a1;
