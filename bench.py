@@ -468,6 +468,9 @@ def main():
             print(f"[bench] {name:13s} launches={p['launches']:4d} avg={p['ms'] / p['launches'] * 1e3:9.1f} us",
                   file=sys.stderr)
     print(f"[bench] blocks={nb} fields/step={nfields} step={elapsed / args.steps * 1e3:.3f} ms", file=sys.stderr)
+    print("[bench] one span in flight: " + ", ".join(
+        f"{name} {p['ms'] / p['launches'] * 1e3:.1f} us" for name, p in prof_iso.items() if p["launches"]),
+        file=sys.stderr)
 
     iso_ms = prof_iso[dominant]["ms"] / max(1, prof_iso[dominant]["launches"])
     iso_achieved = launch_bytes / (iso_ms * 1e-3) / 1e9

@@ -47,13 +47,28 @@ __device__ __forceinline__ float vhs_atan2(float y, float x) {
   return r;
 }
 
-// PortedMath.VhsRustFrequencyDifference (PortedMath.cs:546-551).
+// x / (2 pi) in FP32, correctly rounded: a multiply by the correctly rounded
+// reciprocal and one FMA correction step give the IEEE quotient for every FP32 x with
+// 2^-100 <= |x| <= 2^120 (checked exhaustively over all such inputs); zeros, tiny,
+// huge and non-finite inputs take the IEEE division.
+__device__ __forceinline__ float div_tau(float x) {
+#pragma clang fp contract(off)
+  const float tau = 6.28318530717958647692f;
+  const float rcp = 1.0f / tau;
+  const float ax = fabsf(x);
+  if (!(ax >= 0x1p-100f && ax <= 0x1p+120f)) return x / tau;
+  const float q = x * rcp;
+  return __builtin_fmaf(__builtin_fmaf(-q, tau, x), rcp, q);
+}
+
+// PortedMath.VhsRustFrequencyDifference (PortedMath.cs:546-551): the two divisions by
+// 2 pi are the IEEE quotients (div_tau).
 __device__ __forceinline__ float vhs_freq_diff(float cur, float prev, float freq) {
 #pragma clang fp contract(off)
   const float tau = 6.28318530717958647692f;
   float d = cur - prev;
-  d -= floorf(d / tau) * tau;
-  return (d * freq) / tau;
+  d -= floorf(div_tau(d)) * tau;
+  return div_tau(d * freq);
 }
 
 }  // namespace vhsd
