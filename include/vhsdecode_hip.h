@@ -413,6 +413,56 @@ VHSDECODE_HIP_API int32_t vhsdecode_hip_decode_span_fields_device(
     const vhsdecode_hip_field_params_v1* params, const vhsdecode_hip_locator_v1* locator, int32_t max_fields,
     uint16_t* out, int32_t* field_count, double* field_line0, int32_t* field_first, void* stream);
 
+/* The refined line locations of the last field decode on this context
+ * (vhsdecode_hip_decode_fields_device or vhsdecode_hip_decode_span_fields_device):
+ * device memory owned by the context, [fields][location_count] in span coordinates,
+ * valid until the next field decode call on it. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_field_locations_device(vhsdecode_hip_demod_context* context,
+                                                               const double** locations, int32_t* location_count);
+
+/* ---------------------------------------------------------------- dropout detection
+ * TbcFieldDecodePipeline.DetectTapeDropouts (Decode/TbcFieldDecodePipeline.cs:6593-6642)
+ * for a run of fields, on the device: the float32 pairwise mean of the RF envelope
+ * over each field's read window (NumpyReduction.MeanFloat32; the window as
+ * DecodeReadWindowPlanner.Resolve places it, Decode/DecodeReadWindowPlanner.cs:37-63)
+ * gives the threshold, RfDropoutDetector.FindDropouts (Decode/TbcDropoutMapping.cs:52-107)
+ * finds the hysteresis ranges between the field's first and last detection line, and
+ * TbcDropoutMapper.MapTapeRfToTbc (:288-366) turns them into .tbc.json rows
+ * (fieldLine, startx, endx).  Output line k of a field starts at its location k, which
+ * is the reference's first rendered location, so a row's field line is k + 1. */
+typedef struct vhsdecode_hip_dropout_params_v1 {
+  uint32_t struct_size;
+  double threshold_fraction;       /* TbcDropoutDetectionOptions.ThresholdFraction: 0.18 (DDD), 0.35 (CxADC) */
+  double absolute_threshold;       /* AbsoluteThreshold when >= 0; negative: fraction x envelope mean */
+  double hysteresis;               /* 1.25 */
+  int32_t merge_threshold;         /* RfDropoutDetector.DefaultMergeThreshold (30) */
+  int32_t minimum_length;          /* RfDropoutDetector.DefaultMinimumLength (10) */
+  int32_t output_line_length;      /* TbcFrameSpec output line length */
+  int32_t first_field_lines;       /* CurrentFieldLineCount of a first field (NTSC 263, PAL 312) */
+  int32_t second_field_lines;      /* ... of a second field (NTSC 262, PAL 313) */
+  int32_t capacity;                /* rows (and ranges) stored per field */
+  int64_t read_samples;            /* DecodeReadWindowPlanner.EstimateReadSampleCount */
+  double line_length;              /* nominal input line length (samples, exact) */
+  double lead_lines;               /* read-window lead before line 0 (lines), as the locator */
+  int32_t block_length;
+  int32_t block_cut;
+  int32_t block_stride;
+} vhsdecode_hip_dropout_params_v1;
+
+/* envelope: device span plane; locations: device [max_fields][location_count] (span
+ * coordinates, e.g. vhsdecode_hip_field_locations_device); field_line0 (double),
+ * field_first (int32) and field_count (int32, fields to process, <= max_fields): device,
+ * as vhsdecode_hip_decode_span_fields_device writes them.  Outputs, device: row_count
+ * [max_fields]; field_line, start_x, end_x [max_fields][capacity]; range_count,
+ * range_start, range_end (sample positions in the span) likewise, or all three NULL.
+ * A count above capacity means only the first `capacity` entries were stored; -1 means
+ * the field's detection range exceeded 8388608 samples.  Asynchronous on `stream`. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_detect_dropouts_device(
+    vhsdecode_hip_demod_context* context, const float* envelope, int64_t span_length, const double* locations,
+    int32_t location_count, const double* field_line0, const int32_t* field_first, const int32_t* field_count,
+    int32_t max_fields, const vhsdecode_hip_dropout_params_v1* params, int32_t* row_count, int32_t* field_line,
+    int32_t* start_x, int32_t* end_x, int32_t* range_count, int32_t* range_start, int32_t* range_end, void* stream);
+
 /* ---------------------------------------------------------------- served windows
  * A device-resident request loop for seekable decode windows from many concurrent
  * streams, the served counterpart of the reference's preview windows
@@ -502,7 +552,8 @@ enum {
   VHSDECODE_HIP_KERNEL_SUB_DEEMPHASIS = 6,
   VHSDECODE_HIP_KERNEL_BLOCK_VIDEO = 7,
   VHSDECODE_HIP_KERNEL_LOCATE = 8,
-  VHSDECODE_HIP_KERNEL_COUNT = 9
+  VHSDECODE_HIP_KERNEL_DROPOUTS = 9,
+  VHSDECODE_HIP_KERNEL_COUNT = 10
 };
 
 typedef struct vhsdecode_hip_profile_v1 {

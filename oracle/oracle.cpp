@@ -359,24 +359,28 @@ void sub_deemphasis(double* video, const cd* vs, const cd* hp, const double* amp
   }
 }
 
-// numpy's pairwise float32 sum of FP64 values cast to float (NumpyReduction.PairwiseSumFloat32,
-// Dsp/NumpyReduction.cs:253-339): blocks of at most 128 summed with eight accumulators,
-// longer runs split at half (rounded down to a multiple of 8) and added.
-float pairwise_sum_f32(const double* v, int n) {
+// numpy's pairwise float32 sum of values cast to float (NumpyReduction.PairwiseSumFloat32,
+// Dsp/NumpyReduction.cs:253-339): spans longer than 128 split at half their length
+// rounded down to a multiple of 8 and the halves added; 8..128 values use eight
+// accumulators (accumulator k adds elements k, k+8, ... in order), combined as
+// ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)), then the remainder in order; fewer than 8 are
+// added in order starting from -0.0f.
+template <class T>
+float pairwise_sum_f32(const T* v, int64_t n) {
   if (n < 8) {
     float s = -0.0f;
-    for (int i = 0; i < n; i++) s += (float)v[i];
+    for (int64_t i = 0; i < n; i++) s += (float)v[i];
     return s;
   }
   if (n > 128) {
-    int split = n / 2;
+    int64_t split = n / 2;
     split -= split % 8;
     return pairwise_sum_f32(v, split) + pairwise_sum_f32(v + split, n - split);
   }
   float r[8];
   for (int k = 0; k < 8; k++) r[k] = (float)v[k];
-  int i = 8;
-  const int end = n - (n % 8);
+  int64_t i = 8;
+  const int64_t end = n - (n % 8);
   for (; i < end; i += 8)
     for (int k = 0; k < 8; k++) r[k] += (float)v[i + k];
   float s = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
@@ -1115,6 +1119,184 @@ int oracle_tbc_resample(const double* src, int src_len, const double* locations,
     }
   }
   return 0;
+}
+
+
+// ---------------------------------------------------------------- dropout detection
+// Pinned at 32768 values by the Betamax envelope mean of the fixture hashes and at up to
+// 8192 values by numpy.sum (one pairwise call per 8192-value buffer).
+float oracle_pairwise_sum_f32(const float* v, int64_t n) { return pairwise_sum_f32(v, n); }
+
+// RfDropoutDetector.FindDropouts (Decode/TbcDropoutMapping.cs:52-107): hysteresis
+// ranges of env[start, end).  A value <= threshold opens a range unless one is open or
+// the last closed no more than merge samples before; a value >= threshold * hysteresis
+// closes the open range there; an open range closes at `end`; ranges no longer than
+// min_len are dropped.  Returns the number of ranges (all of them; at most cap stored).
+int oracle_find_dropouts(const double* env, int start, int end, double threshold, double hysteresis, int merge,
+                         int min_len, int* out_start, int* out_end, int cap) {
+  const double down = threshold, up = threshold * hysteresis;
+  std::vector<std::pair<int, int>> raw;
+  for (int i = start; i < end; i++) {
+    const double value = env[i];
+    if (value <= down) {
+      const bool ended = !raw.empty() && raw.back().second != -1 && i - raw.back().second > merge;
+      if (raw.empty() || ended) raw.push_back({i, -1});
+    } else if (value >= up && !raw.empty() && raw.back().second == -1) {
+      raw.back().second = i;
+    }
+  }
+  if (!raw.empty() && raw.back().second == -1) raw.back().second = end;
+  int n = 0;
+  for (auto& r : raw) {
+    if (r.second - r.first > min_len) {
+      if (n < cap) {
+        out_start[n] = r.first;
+        out_end[n] = r.second;
+      }
+      n++;
+    }
+  }
+  return n;
+}
+
+// TbcDropoutMapper.MapTapeRfToTbc (Decode/TbcDropoutMapping.cs:288-366): each range's
+// start line is searched forward from the previous range's end line; the range is
+// written line by line (start pixel floor, end pixel ceil, clamped to the line) with
+// field line = location index - line_offset.  A range whose start lies past end_line
+// ends the mapping.  Returns the entry count (at most cap stored).
+int oracle_map_tape_rf_to_tbc(const double* rs, const double* re, int nr, const double* loc, int nloc, int out_len,
+                              int start_line, int end_line, int line_offset, int* field_line, int* start_x,
+                              int* end_x, int cap) {
+  if (out_len <= 0 || start_line < 0 || end_line <= start_line || end_line >= nloc) return -1;
+  int n = 0;
+  auto put = [&](int fl, int sx, int ex) {
+    if (n < cap) {
+      field_line[n] = fl;
+      start_x[n] = sx;
+      end_x[n] = ex;
+    }
+    n++;
+  };
+  // the reference appends starts and ends to separate lists; an entry's end is filled
+  // when its line's end is known
+  std::vector<int> fls, sxs, exs;
+  int li = start_line;
+  double ls = loc[li], le = loc[li + 1];
+  for (int k = 0; k < nr; k++) {
+    const double s = rs[k], e = re[k];
+    bool found = false;
+    while (li < end_line) {
+      const double len = le - ls;
+      if ((s >= ls || li == start_line) && s < le && len > 0.0) {
+        const int px = (int)std::floor(((s - ls) / len) * out_len);
+        fls.push_back(li - line_offset);
+        sxs.push_back(std::max(0, px));
+        found = true;
+        break;
+      }
+      li++;
+      if (li < end_line) {
+        ls = loc[li];
+        le = loc[li + 1];
+      }
+    }
+    if (!found) break;
+    while (li < end_line) {
+      const double len = le - ls;
+      if (e < le && len > 0.0) {
+        const int px = (int)std::ceil(((e - ls) / len) * out_len);
+        exs.push_back(std::min(out_len, px));
+        break;
+      }
+      exs.push_back(out_len);
+      li++;
+      if (li < end_line) {
+        ls = loc[li];
+        le = loc[li + 1];
+        sxs.push_back(0);
+        fls.push_back(li - line_offset);
+      }
+    }
+  }
+  // every start gets its end: a new line's start is added only while lines remain,
+  // and the next pass of the end loop adds its end
+  if (fls.size() != sxs.size() || fls.size() != exs.size()) return -2;
+  for (size_t i = 0; i < fls.size(); i++) put(fls[i], sxs[i], exs[i]);
+  return n;
+}
+
+// TbcFieldDecodePipeline.DetectTapeDropouts (Decode/TbcFieldDecodePipeline.cs:6593-6642)
+// for one field of a span, in span coordinates.  loc holds the field's nloc line
+// locations with output line k at loc[k] (the reference's location index
+// outputFirstLine + k, TbcFieldDecodePipeline.cs:2475-2483, which equals its
+// startLine = LaserDiscLineOffset + 1, :6087-6095), so the detection runs over
+// loc[0] .. loc[min(nloc - 1, line_count + 1)] and field line = k + 1.  The threshold
+// is the float32 pairwise mean of env over the field's read window [w0, w1)
+// (DecodeReadWindowPlanner.Resolve, Decode/DecodeReadWindowPlanner.cs:37-63) times
+// (float)fraction, or `absolute` when it is >= 0.  Ranges (span coordinates) and the
+// map are written up to cap entries; returns the map entry count, ranges in *nranges.
+int oracle_tape_dropouts(const float* env, int64_t span_len, const double* loc, int nloc, int line_count,
+                         int64_t w0, int64_t w1, double fraction, double absolute, double hysteresis, int merge,
+                         int min_len, int out_len, int* range_start, int* range_end, int* nranges, int* field_line,
+                         int* start_x, int* end_x, int cap, float* mean_out) {
+  *nranges = 0;
+  const int end_line = std::min(nloc - 1, line_count + 1);
+  if (end_line <= 0) return 0;
+  w0 = std::max<int64_t>(0, w0);
+  w1 = std::min<int64_t>(span_len, w1);
+  const float mean = w1 > w0 ? pairwise_sum_f32(env + w0, w1 - w0) / (float)(w1 - w0) : NAN;
+  if (mean_out) *mean_out = mean;
+  const double thr = absolute >= 0.0 ? absolute : (double)(mean * (float)fraction);
+  // the reference clamps to its window's envelope, [0, w1 - w0) in window coordinates
+  const int64_t a = std::min(std::max<int64_t>((int64_t)std::floor(loc[0]), w0), w1);
+  const int64_t b = std::min(std::max<int64_t>((int64_t)std::ceil(loc[end_line]), a), w1);
+  if (b <= a) return 0;
+  std::vector<double> ev((size_t)(b - a));
+  for (int64_t i = a; i < b; i++) ev[(size_t)(i - a)] = (double)env[i];
+  std::vector<int> rs(1 << 16), re(1 << 16);
+  int nr = oracle_find_dropouts(ev.data(), 0, (int)(b - a), thr, hysteresis, merge, min_len, rs.data(), re.data(),
+                                (int)rs.size());
+  nr = std::min<int>(nr, (int)rs.size());
+  std::vector<double> drs(nr), dre(nr);
+  for (int k = 0; k < nr; k++) {
+    drs[k] = (double)(rs[k] + a);
+    dre[k] = (double)(re[k] + a);
+    if (k < cap) {
+      range_start[k] = (int)(rs[k] + a);
+      range_end[k] = (int)(re[k] + a);
+    }
+  }
+  *nranges = nr;
+  if (nr == 0) return 0;
+  return oracle_map_tape_rf_to_tbc(drs.data(), dre.data(), nr, loc, nloc, out_len, 0, end_line, -1, field_line,
+                                   start_x, end_x, cap);
+}
+
+
+// DecodeReadWindowPlanner.EstimateReadSampleCount (Decode/DecodeReadWindowPlanner.cs:15-35):
+// 350 lines (NTSC, aligned down to 16384) or 400 lines (PAL) of the line length
+// rounded half to even, in whole blocks plus two.
+int64_t oracle_estimate_read_samples(double line_period_us, double fs_hz, int is_pal, int64_t block_length) {
+  const int64_t line = (int64_t)std::nearbyint(line_period_us * (fs_hz / 1000000.0));
+  const int64_t base = is_pal ? line * 400 : ((line * 350) / 16384) * 16384;
+  return (base / block_length + 2) * block_length;
+}
+
+// DecodeReadWindowPlanner.Resolve (Decode/DecodeReadWindowPlanner.cs:37-63): the span of
+// whole block strides that holds the block-aligned request.
+void oracle_resolve_read_window(int64_t intended, int64_t requested, int64_t block_length, int64_t block_cut,
+                                int64_t stride, int64_t* start, int64_t* length) {
+  if (requested == 0) {
+    *start = intended;
+    *length = 0;
+    return;
+  }
+  const int64_t read = std::max<int64_t>(0, intended - block_cut);
+  const int64_t aligned = (read / block_length) * block_length;
+  const int64_t end = aligned + requested;
+  const int64_t first = aligned / stride, last = end / stride;
+  *start = first * stride;
+  *length = (last - first + 1) * stride;
 }
 
 }  // extern "C"

@@ -53,6 +53,22 @@ def lib():
                                               ctypes.c_double, ctypes.c_double, _dp]
         L.oracle_hifi_analytic_phase.argtypes = [_fp, ctypes.c_int]
         L.oracle_hifi_demodulate_phase.argtypes = [_fp, ctypes.c_int, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.oracle_pairwise_sum_f32.argtypes = [_fp, ctypes.c_int64]
+        L.oracle_pairwise_sum_f32.restype = ctypes.c_float
+        L.oracle_find_dropouts.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int, ctypes.c_int, _ip, _ip, ctypes.c_int]
+        L.oracle_find_dropouts.restype = ctypes.c_int
+        L.oracle_map_tape_rf_to_tbc.argtypes = [_dp, _dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, ctypes.c_int]
+        L.oracle_map_tape_rf_to_tbc.restype = ctypes.c_int
+        L.oracle_tape_dropouts.argtypes = [_fp, ctypes.c_int64, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                           ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _ip, _ip, _ip,
+                                           ctypes.c_int, _fp]
+        L.oracle_tape_dropouts.restype = ctypes.c_int
+        L.oracle_estimate_read_samples.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int64]
+        L.oracle_estimate_read_samples.restype = ctypes.c_int64
+        L.oracle_resolve_read_window.argtypes = [ctypes.c_int64] * 5 + [ctypes.POINTER(ctypes.c_int64)] * 2
         L.oracle_find_pulses.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                          _ip, _ip, ctypes.c_int]
         L.oracle_find_pulses.restype = ctypes.c_int
@@ -275,3 +291,79 @@ def hifi_demodulate_phase(phase: np.ndarray, output: np.ndarray, fs: int, carrie
 def hifi_demodulate(x: np.ndarray, output: np.ndarray, fs: int, carrier: int, deviation: int) -> None:
     """HiFiHilbertDiscriminator.Demodulate (HiFi/HiFiHilbertDiscriminator.cs:42-46)."""
     hifi_demodulate_phase(hifi_analytic_phase(x), output, fs, carrier, deviation)
+
+
+def pairwise_sum_f32(values: np.ndarray) -> np.float32:
+    """NumpyReduction.PairwiseSumFloat32 of float32 values."""
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    return np.float32(lib().oracle_pairwise_sum_f32(_p(v, _fp), v.size))
+
+
+def find_dropouts(envelope, start, end, threshold, hysteresis, merge=30, min_len=10, cap=1 << 16):
+    """RfDropoutDetector.FindDropouts: list of (start, end) sample ranges."""
+    e = np.ascontiguousarray(envelope, dtype=np.float64)
+    rs = np.zeros(cap, dtype=np.int32)
+    re = np.zeros(cap, dtype=np.int32)
+    n = lib().oracle_find_dropouts(_p(e, _dp), int(start), int(end), float(threshold), float(hysteresis), int(merge),
+                                   int(min_len), _p(rs, _ip), _p(re, _ip), cap)
+    n = min(n, cap)
+    return [(int(rs[k]), int(re[k])) for k in range(n)]
+
+
+def map_tape_rf_to_tbc(ranges, locations, out_len, start_line, end_line, line_offset, cap=1 << 16):
+    """TbcDropoutMapper.MapTapeRfToTbc: (field_line, start_x, end_x) int arrays."""
+    rs = np.ascontiguousarray([r[0] for r in ranges], dtype=np.float64)
+    re = np.ascontiguousarray([r[1] for r in ranges], dtype=np.float64)
+    loc = np.ascontiguousarray(locations, dtype=np.float64)
+    fl, sx, ex = (np.zeros(cap, dtype=np.int32) for _ in range(3))
+    n = lib().oracle_map_tape_rf_to_tbc(_p(rs, _dp), _p(re, _dp), rs.size, _p(loc, _dp), loc.size, int(out_len),
+                                        int(start_line), int(end_line), int(line_offset), _p(fl, _ip), _p(sx, _ip),
+                                        _p(ex, _ip), cap)
+    if n < 0:
+        raise ValueError(f"map_tape_rf_to_tbc failed ({n})")
+    n = min(n, cap)
+    return fl[:n].copy(), sx[:n].copy(), ex[:n].copy()
+
+
+def tape_dropouts(envelope, locations, line_count, window, fraction=0.18, absolute=-1.0, hysteresis=1.25, merge=30,
+                  min_len=10, out_len=910, cap=1 << 14):
+    """DetectTapeDropouts for one field (span coordinates, output line k at
+    locations[k]): dict with the threshold mean, ranges and the TBC map."""
+    env = np.ascontiguousarray(envelope, dtype=np.float32)
+    loc = np.ascontiguousarray(locations, dtype=np.float64)
+    rs, re, fl, sx, ex = (np.zeros(cap, dtype=np.int32) for _ in range(5))
+    nr = ctypes.c_int(0)
+    mean = np.zeros(1, dtype=np.float32)
+    n = lib().oracle_tape_dropouts(_p(env, _fp), env.size, _p(loc, _dp), loc.size, int(line_count), int(window[0]),
+                                   int(window[1]), float(fraction), float(absolute), float(hysteresis), int(merge),
+                                   int(min_len), int(out_len), _p(rs, _ip), _p(re, _ip), ctypes.byref(nr),
+                                   _p(fl, _ip), _p(sx, _ip), _p(ex, _ip), cap, _p(mean, _fp))
+    if n < 0:
+        raise ValueError(f"tape_dropouts failed ({n})")
+    k = min(nr.value, cap)
+    n = min(n, cap)
+    return dict(mean=mean[0], ranges=list(zip(rs[:k].tolist(), re[:k].tolist())), field_line=fl[:n].copy(),
+                start_x=sx[:n].copy(), end_x=ex[:n].copy())
+
+
+def estimate_read_samples(line_period_us: float, fs_hz: float, is_pal: bool, block_length: int = 32768) -> int:
+    """DecodeReadWindowPlanner.EstimateReadSampleCount."""
+    return int(lib().oracle_estimate_read_samples(line_period_us, fs_hz, 1 if is_pal else 0, block_length))
+
+
+def resolve_read_window(intended: int, requested: int, block_length: int = 32768, block_cut: int = 1024,
+                        stride: int = 30720):
+    """DecodeReadWindowPlanner.Resolve: (start, length)."""
+    a, b = ctypes.c_int64(), ctypes.c_int64()
+    lib().oracle_resolve_read_window(int(intended), int(requested), block_length, block_cut, stride,
+                                     ctypes.byref(a), ctypes.byref(b))
+    return int(a.value), int(b.value)
+
+
+def field_read_window(line0: float, lead_lines: float, line_len: float, requested: int, span_len: int,
+                      block_length: int = 32768, block_cut: int = 1024, stride: int = 30720):
+    """A located field's read window in span coordinates: Resolve of the intended start
+    lead_lines before line 0 (truncated to a sample), clipped to the span."""
+    intended = int(max(0.0, line0 - lead_lines * line_len))
+    start, length = resolve_read_window(intended, requested, block_length, block_cut, stride)
+    return min(start, span_len), min(start + length, span_len)

@@ -178,6 +178,44 @@ struct TbcParams {
   double conv_offset;  // (outputZero + 0.5) - vsync_ire * outputScale
 };
 
+// RF-envelope dropout detection of a run of fields (dropout_kernels.hip).
+constexpr int kDropMaxLocations = 512;
+constexpr int kDropSummaryWords = 4096;  // summary bits: 131072 mask words per field
+
+struct DropoutArgs {
+  const float* envelope;           // span plane
+  long long span_len;
+  const double* locations;         // [fields][nloc], span coordinates
+  int nloc;
+  const double* line0;             // [fields]
+  const int* field_first;          // [fields]
+  const int* field_count;          // device count of fields to process
+  double fraction;
+  double absolute_threshold;       // >= 0: used instead of fraction x mean
+  double hysteresis;
+  int merge;
+  int min_len;
+  int out_len;
+  int first_lines, second_lines;
+  int capacity;                    // rows and ranges stored per field
+  // read window (DecodeReadWindowPlanner)
+  long long read_samples;
+  double line_len, lead_lines;
+  long long block_length, block_cut, stride;
+  // scratch
+  float* partials;                 // [fields][64]
+  unsigned long long* masks;       // [fields][2][words_cap]
+  int words_cap;
+  // outputs
+  int* row_count;
+  int* field_line;
+  int* start_x;
+  int* end_x;
+  int* range_count;                // optional (NULL)
+  int* range_start;
+  int* range_end;
+};
+
 }  // namespace vhsd
 
 extern "C" {
@@ -185,6 +223,7 @@ size_t vhsd_block_lds_bytes();
 hipError_t vhsd_launch_block_demod(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_chroma(const vhsd::ChromaArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_repair(const vhsd::RepairArgs* a, int nblocks, hipStream_t stream);
+hipError_t vhsd_launch_dropouts(const vhsd::DropoutArgs* a, int max_fields, hipStream_t stream);
 hipError_t vhsd_launch_locate(const float* sync_ref, long long span_len, const int* starts, const int* lens,
                               const int* pulse_count, int capacity, const vhsd::LocateParams* lp, int max_fields,
                               int cand_cap, double* cand_line0, int* cand_first, int* cand_count, double* cand_locs,

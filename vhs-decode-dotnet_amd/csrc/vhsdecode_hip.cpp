@@ -272,6 +272,9 @@ struct vhsdecode_hip_demod_context {
   DevBuf<unsigned short> t_u16;
   // batched fields
   DevBuf<double> f_loc, f_ref, f_x, f_y0, f_pow;
+  int f_ref_nloc = 0;  // locations per field of the last field decode
+  DevBuf<float> d_part;               // dropout mean partial sums
+  DevBuf<unsigned long long> d_mask;  // dropout low/high bit masks
   // device line locator
   DevBuf<double> l_line0, l_locs;
   DevBuf<int> l_first, l_hit, l_count;
@@ -1461,6 +1464,7 @@ int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod_context* c, const
     hipStream_t st = pick_stream(c, stream);
     VHSD_HIP(c->f_loc.ensure(nl));
     VHSD_HIP(c->f_ref.ensure(nl));
+    c->f_ref_nloc = nloc;
     VHSD_HIP(c->f_fil.ensure(nl));
     VHSD_HIP(c->f_fo.ensure(nl));
     VHSD_HIP(c->f_x.ensure((size_t)nfields * p->field_lines));
@@ -1601,6 +1605,7 @@ int32_t vhsdecode_hip_decode_span_fields_device(vhsdecode_hip_demod_context* c, 
     VHSD_HIP(c->f_ref.ensure(nl));
     VHSD_HIP(c->f_fo.ensure(nl));
     VHSD_HIP(c->f_x.ensure((size_t)max_fields * p->field_lines));
+    c->f_ref_nloc = nloc;
     VHSD_HIP(c->f_y0.ensure((size_t)max_fields * p->field_lines));
     // field tables: 8 pointer-sized arrays, then line counts and first flags (int)
     const size_t mf = (size_t)max_fields;
@@ -1669,6 +1674,95 @@ int32_t vhsdecode_hip_decode_span_fields_device(vhsdecode_hip_demod_context* c, 
                                       c->f_x.p, c->f_y0.p, p->output_line_length, smooth ? 1 : 0, p->ire0, out_scale,
                                       1.0 / p->hz_ire, conv_off, c->lut.p, out, field_count, st));
     }
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_field_locations_device(vhsdecode_hip_demod_context* c, const double** locations,
+                                             int32_t* location_count) {
+  if (!c || !locations || !location_count) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid arguments");
+  std::lock_guard<std::mutex> lock(c->mu);
+  if (!c->f_ref.p || c->f_ref_nloc <= 0) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "no field decode yet");
+  *locations = c->f_ref.p;
+  *location_count = c->f_ref_nloc;
+  return VHSDECODE_HIP_OK;
+}
+
+int32_t vhsdecode_hip_detect_dropouts_device(vhsdecode_hip_demod_context* c, const float* envelope,
+                                             int64_t span_length, const double* locations, int32_t location_count,
+                                             const double* field_line0, const int32_t* field_first,
+                                             const int32_t* field_count, int32_t max_fields,
+                                             const vhsdecode_hip_dropout_params_v1* p, int32_t* row_count,
+                                             int32_t* field_line, int32_t* start_x, int32_t* end_x,
+                                             int32_t* range_count, int32_t* range_start, int32_t* range_end,
+                                             void* stream) {
+  try {
+    if (!c || !envelope || !locations || !field_line0 || !field_first || !field_count || !p || !row_count ||
+        !field_line || !start_x || !end_x || max_fields < 0 || span_length <= 0)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid dropout detection request");
+    if (p->struct_size != sizeof(vhsdecode_hip_dropout_params_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "dropout_params struct_size mismatch");
+    const bool ranges = range_count || range_start || range_end;
+    if (ranges && !(range_count && range_start && range_end))
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "range outputs are given all three or none");
+    if (location_count < 2 || location_count > vhsd::kDropMaxLocations)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "location_count must be 2..512");
+    if (p->capacity <= 0 || p->output_line_length <= 0 || p->merge_threshold < 0 || p->minimum_length < 0 ||
+        !(p->hysteresis > 0.0) || !(p->threshold_fraction >= 0.0) || p->first_field_lines <= 0 ||
+        p->second_field_lines <= 0 || p->read_samples < 0 || !(p->line_length > 0.0) || !(p->lead_lines >= 0.0) ||
+        p->block_length <= 0 || p->block_cut < 0 || p->block_stride <= 0)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid dropout parameters");
+    if (span_length > INT32_MAX)
+      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "spans are limited to 2^31 - 1 samples (int32 range positions)");
+    // every node of the window's pairwise tree at depth 15 must be a leaf (<= 128 values):
+    // a node at depth d holds fewer than n / 2^d + 16 values
+    const int64_t window_max = p->read_samples + p->block_length + 2 * (int64_t)p->block_stride;
+    if (window_max > 3600000)
+      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "read windows are limited to 3.6 M samples");
+    if (max_fields == 0) return VHSDECODE_HIP_OK;
+    std::lock_guard<std::mutex> lock(c->mu);
+    VHSD_HIP(hipSetDevice(c->device));
+    hipStream_t st = pick_stream(c, stream);
+    const int words_cap = (int)std::min<int64_t>(span_length / 64 + 2, (int64_t)vhsd::kDropSummaryWords * 32);
+    VHSD_HIP(c->d_part.ensure((size_t)max_fields * 64));
+    VHSD_HIP(c->d_mask.ensure((size_t)max_fields * 2 * words_cap));
+    vhsd::DropoutArgs a{};
+    a.envelope = envelope;
+    a.span_len = span_length;
+    a.locations = locations;
+    a.nloc = location_count;
+    a.line0 = field_line0;
+    a.field_first = field_first;
+    a.field_count = field_count;
+    a.fraction = p->threshold_fraction;
+    a.absolute_threshold = p->absolute_threshold;
+    a.hysteresis = p->hysteresis;
+    a.merge = p->merge_threshold;
+    a.min_len = p->minimum_length;
+    a.out_len = p->output_line_length;
+    a.first_lines = p->first_field_lines;
+    a.second_lines = p->second_field_lines;
+    a.capacity = p->capacity;
+    a.read_samples = p->read_samples;
+    a.line_len = p->line_length;
+    a.lead_lines = p->lead_lines;
+    a.block_length = p->block_length;
+    a.block_cut = p->block_cut;
+    a.stride = p->block_stride;
+    a.partials = c->d_part.p;
+    a.masks = c->d_mask.p;
+    a.words_cap = words_cap;
+    a.row_count = row_count;
+    a.field_line = field_line;
+    a.start_x = start_x;
+    a.end_x = end_x;
+    a.range_count = range_count;
+    a.range_start = range_start;
+    a.range_end = range_end;
+    ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_DROPOUTS, max_fields);
+    VHSD_HIP(vhsd_launch_dropouts(&a, max_fields, st));
     return VHSDECODE_HIP_OK;
   } catch (const std::exception& e) {
     return fail(VHSDECODE_HIP_E_INTERNAL, e.what());

@@ -48,10 +48,12 @@ REQUIRED_EXPORTS = (
     "vhsdecode_hip_hifi_destroy",
     "vhsdecode_hip_hifi_demodulate_device",
     "vhsdecode_hip_hifi_demodulate_host",
+    "vhsdecode_hip_field_locations_device",
+    "vhsdecode_hip_detect_dropouts_device",
 )
 
 KERNEL_NAMES = ("block_demod", "spike_repair", "chroma", "pulses", "hsync", "tbc", "sub_deemphasis", "block_video",
-                "locate")
+                "locate", "dropouts")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -158,6 +160,16 @@ class LocatorV1(ctypes.Structure):
                 ("block_length", ctypes.c_int32), ("block_cut", ctypes.c_int32), ("block_stride", ctypes.c_int32)]
 
 
+class DropoutParamsV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("threshold_fraction", ctypes.c_double),
+                ("absolute_threshold", ctypes.c_double), ("hysteresis", ctypes.c_double),
+                ("merge_threshold", ctypes.c_int32), ("minimum_length", ctypes.c_int32),
+                ("output_line_length", ctypes.c_int32), ("first_field_lines", ctypes.c_int32),
+                ("second_field_lines", ctypes.c_int32), ("capacity", ctypes.c_int32),
+                ("read_samples", ctypes.c_int64), ("line_length", ctypes.c_double), ("lead_lines", ctypes.c_double),
+                ("block_length", ctypes.c_int32), ("block_cut", ctypes.c_int32), ("block_stride", ctypes.c_int32)]
+
+
 class ServeConfigV1(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("device_id", ctypes.c_int32), ("max_slots", ctypes.c_int32),
                 ("hbm_budget_bytes", ctypes.c_uint64), ("max_window_samples", ctypes.c_int64),
@@ -243,6 +255,11 @@ def load():
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(FieldParamsV1),
         ctypes.POINTER(LocatorV1), ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
         ctypes.c_void_p]
+    L.vhsdecode_hip_field_locations_device.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                                       ctypes.POINTER(ctypes.c_int32)]
+    L.vhsdecode_hip_detect_dropouts_device.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(DropoutParamsV1)] + [ctypes.c_void_p] * 8
     L.vhsdecode_hip_decode_fields_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                      ctypes.c_int64, _dp, _u8p, ctypes.POINTER(ctypes.c_int64),
                                                      ctypes.c_int32,

@@ -340,6 +340,28 @@ class HipBlockDemodulator:
             _ptr(ref, _lib._dp) if ref is not None else None, ctypes.c_void_p(stream)))
         return ref
 
+    def field_locations_device(self):
+        """(device pointer, locations per field) of the refined line locations of the last
+        field decode on this context; valid until the next field decode."""
+        ptr = ctypes.c_void_p()
+        cnt = ctypes.c_int32()
+        _lib.check(self._L.vhsdecode_hip_field_locations_device(self._h, ctypes.byref(ptr), ctypes.byref(cnt)))
+        return int(ptr.value), int(cnt.value)
+
+    def detect_dropouts_device(self, envelope_ptr: int, span_length: int, locations_ptr: int, location_count: int,
+                               line0_ptr: int, first_ptr: int, count_ptr: int, max_fields: int, params,
+                               row_count_ptr: int, field_line_ptr: int, start_x_ptr: int, end_x_ptr: int,
+                               range_count_ptr: int = 0, range_start_ptr: int = 0, range_end_ptr: int = 0,
+                               stream: int = 0):
+        """Tape RF-envelope dropout detection of up to max_fields fields (every pointer device
+        memory; asynchronous on `stream`): .tbc.json rows per field at [field][capacity]."""
+        v = ctypes.c_void_p
+        _lib.check(self._L.vhsdecode_hip_detect_dropouts_device(
+            self._h, v(envelope_ptr), span_length, v(locations_ptr), location_count, v(line0_ptr), v(first_ptr),
+            v(count_ptr), max_fields, ctypes.byref(params), v(row_count_ptr), v(field_line_ptr), v(start_x_ptr),
+            v(end_x_ptr), v(range_count_ptr or None), v(range_start_ptr or None), v(range_end_ptr or None),
+            v(stream)))
+
     def set_profiling(self, enable: bool = True):
         _lib.check(self._L.vhsdecode_hip_set_profiling(self._h, 1 if enable else 0))
 

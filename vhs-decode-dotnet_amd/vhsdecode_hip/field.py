@@ -136,6 +136,35 @@ def locator(g: SyncGeometry, block_length: int = 32768, block_cut: int = 1024, s
                           block_length, block_cut, stride)
 
 
+def read_sample_count(fmt: FormatParameters, sample_rate_hz: float = 40e6, block_length: int = 32768) -> int:
+    """DecodeReadWindowPlanner.EstimateReadSampleCount (Decode/DecodeReadWindowPlanner.cs:15-35):
+    350 lines (NTSC, aligned down to 16384) or 400 lines (PAL) of whole-sample line
+    length, in whole blocks plus two."""
+    line = int(np.round(float(fmt.sysparams["line_period"]) * (sample_rate_hz / 1e6)))
+    if fmt.parent_system == "PAL":
+        base = line * 400
+    else:
+        base = (line * 350 // 16384) * 16384
+    return (base // block_length + 2) * block_length
+
+
+def dropout_params(fmt: FormatParameters, sample_rate_hz: float = 40e6, fraction: float = 0.18,
+                   absolute: float = -1.0, hysteresis: float = 1.25, capacity: int = 1024, merge: int = 30,
+                   min_len: int = 10, block_length: int = 32768, block_cut: int = 1024, stride: int = 30720):
+    """vhsdecode_hip_dropout_params_v1 for a tape format: TbcDropoutDetectionOptions.DefaultDdd
+    (fraction 0.18, hysteresis 1.25; DefaultCxAdc uses 0.35), the field line counts of
+    FieldLinesForParityDetection (TbcFieldDecodePipeline.cs:728-744) and the read window of
+    DecodeReadWindowPlanner with this backend's line-0 lead."""
+    from . import _lib
+    g = sync_geometry(fmt, sample_rate_hz)
+    lines = g.field_lines
+    first, second = (lines - 1, lines) if g.is_pal else (lines, lines - 1)
+    return _lib.DropoutParamsV1(ctypes.sizeof(_lib.DropoutParamsV1), fraction, absolute, hysteresis, merge, min_len,
+                                g.out_len, first, second, capacity,
+                                read_sample_count(fmt, sample_rate_hz, block_length), g.line_len, LEAD_LINES,
+                                block_length, block_cut, stride)
+
+
 class HipFieldDecoder:
     """Decodes every complete field of a device-resident demodulated span:
     pulses (device) -> line locations (host) -> batched hsync + TBC (device)."""

@@ -61,6 +61,14 @@ def copy_device(dst: int, src: int, nbytes: int):
     _check(hip().hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), int(nbytes), 3))
 
 
+def download_ptr(ptr: int, dtype, count: int) -> np.ndarray:
+    """Copy `count` elements from device address `ptr` (memory another owner holds) to a
+    new numpy array; synchronous."""
+    out = np.empty(count, dtype=dtype)
+    _check(hip().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptr), out.nbytes, D2H))
+    return out
+
+
 class PinnedArray:
     """Page-locked host memory (hipHostMalloc) viewed as a numpy array; DMA reads and
     writes it directly."""
