@@ -35,6 +35,7 @@ N = 32768
 CUT = 1024
 STRIDE = N - 2 * CUT
 HBM_PEAK_GBS = 8000.0
+DROPOUT_ROWS = 1024      # .tbc.json dropout rows kept per field
 SEED = 20240501
 
 
@@ -370,6 +371,7 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=16)
     ap.add_argument("--serve-window", type=float, default=2.0, help="window length (s of RF)")
     ap.add_argument("--serve-burst", type=int, default=32)
+    ap.add_argument("--no-dropouts", action="store_true", help="skip the per-field dropout detection")
     ap.add_argument("--pipeline", type=int, default=3,
                     help="spans in flight: consecutive steps alternate between this many pipelines")
     ap.add_argument("--batch-segments", type=int, default=64, help="batched-segment run (0: skip)")
@@ -392,6 +394,10 @@ def main():
     rf = hipmem.DeviceBuffer.from_array(tape.rf)
     max_fields = args.fields + 1
 
+    # RF-envelope dropout detection of every field, on by default as in the reference's
+    # VHS decode (TbcDropoutDetectionOptions.DefaultDdd)
+    dropouts = None if args.no_dropouts else field.dropout_params(fmt, capacity=DROPOUT_ROWS)
+
     class Pipe:
         """One decode pipeline: its own demod context (stash, descriptors, field scratch),
         span planes, field outputs and stream."""
@@ -404,7 +410,11 @@ def main():
             self.planes = hipmem.DeviceBuffer(4 * span * 4)
             self.flags = hipmem.DeviceBuffer(nb * 4)
             self.out = hipmem.DeviceBuffer(max_fields * g.field_lines * g.out_len * 2)
-            self.meta = hipmem.DeviceBuffer(64)
+            # field count, line 0 and parity of each field (the dropout detection reads them)
+            self.meta = hipmem.DeviceBuffer(8 + 12 * max_fields)
+            self.line0_ptr = self.meta.ptr + 8
+            self.first_ptr = self.line0_ptr + 8 * max_fields
+            self.drop = hipmem.DeviceBuffer(4 * max_fields * (1 + 3 * DROPOUT_ROWS))
             self.stream = hipmem.Stream() if own_stream else None
             self.p = [self.planes.ptr + k * span * 4 for k in range(4)]
 
@@ -412,7 +422,16 @@ def main():
             # every stage on the device, no host round trip inside the step
             st = self.stream.handle if self.stream else 0
             self.dem.demod_span_device(rf.ptr, 0, nb, *self.p, self.flags.ptr, stream=st)
-            self.dec.decode_async(self.p[0], self.p[1], span, self.out.ptr, max_fields, self.meta.ptr, stream=st)
+            self.dec.decode_async(self.p[0], self.p[1], span, self.out.ptr, max_fields, self.meta.ptr,
+                                  self.line0_ptr, self.first_ptr, stream=st)
+            if dropouts is not None:
+                loc_ptr, nloc = self.dem.field_locations_device()
+                d = self.drop.ptr
+                rows = d + 4 * max_fields
+                self.dem.detect_dropouts_device(self.p[2], span, loc_ptr, nloc, self.line0_ptr, self.first_ptr,
+                                                self.meta.ptr, max_fields, dropouts, d, rows,
+                                                rows + 4 * max_fields * DROPOUT_ROWS,
+                                                rows + 8 * max_fields * DROPOUT_ROWS, stream=st)
 
     # Consecutive spans alternate between pipelines on their own streams, so one span's
     # field decode (small latency-bound kernels) overlaps the next span's block kernels.
@@ -491,7 +510,7 @@ def main():
         "config": {"workload": f"NTSC VHS 40 MSPS int16, {args.fields} fields, 28 dB SNR, 20 ns 1/f line jitter",
                    "rf_samples_per_step": rf_samples, "blocks_per_step": nb, "fields_per_step": nfields,
                    "block_length": N, "parallelism": "single GPU",
-                   "spans_in_flight": len(pipes)},
+                   "spans_in_flight": len(pipes), "dropout_detection": dropouts is not None},
         "roofline": {"bound": "hbm", "kernel": kernel_name, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "bytes_per_launch": launch_bytes, "avg_launch_ms": round(demod_ms, 5),

@@ -141,6 +141,30 @@ def test_capacity_overflow_reports_full_count(synthetic):
     assert over == 3  # the fourth field has no dips
 
 
+@pytest.mark.parametrize("span_len", [5000, 12000, 70000, 400_000])
+def test_short_spans_clip_the_window(synthetic, span_len):
+    """Spans shorter than a read window: the window and the detection range are clipped
+    to the span, and the pairwise tree of a short window has its leaves at or above the
+    depth the mean kernel splits workgroups at."""
+    s = synthetic
+    env = s["env"][1_000_000:1_000_000 + span_len].copy()
+    rng = np.random.default_rng(span_len)
+    for _ in range(6):
+        a = int(rng.integers(0, span_len - 100))
+        env[a:a + int(rng.integers(5, 90))] = 10.0
+    g = field.sync_geometry(s["fmt"])
+    line0 = np.array([120.5, 50.25])
+    locs = np.stack([l0 + g.line_len * np.arange(s["nloc"]) for l0 in line0])
+    env_d = hipmem.DeviceBuffer.from_array(env)
+    loc_d = hipmem.DeviceBuffer.from_array(locs)
+    p = field.dropout_params(s["fmt"], capacity=256)
+    got = _run(s["dem"], env_d.ptr, span_len, loc_d.ptr, s["nloc"], line0, np.array([1, 0], np.int32), 2, 2, p)
+    for f in range(2):
+        want = _oracle(env, locs[f], f == 0, line0[f], p, span_len)
+        _compare(got, want, f, p.capacity)
+    assert got[1][0] >= 1
+
+
 def test_rows_without_range_outputs(synthetic):
     s = synthetic
     p = field.dropout_params(s["fmt"], capacity=256)

@@ -8,18 +8,21 @@
 //                between the field's first and last detection line
 //   map rows   = TbcDropoutMapper.MapTapeRfToTbc (Decode/TbcDropoutMapping.cs:288-366)
 //
-// Two launches:
-//  * dropout_mean_slots_kernel, 64 workgroups per field: the pairwise tree of the
-//    window is cut at depth 15; every depth-15 slot (or the leftmost slot under a
-//    shallower leaf) sums its leaf in the reference's order (eight accumulators), and
-//    each workgroup combines its 512 slots up to depth 6 in LDS.
-//  * dropout_fields_kernel, one workgroup per field: finishes the tree (depth 6 to
-//    the root), classifies every sample of the detection range as low (<= threshold)
-//    or high (>= threshold * hysteresis) into two bit masks (one wave ballot per 64
-//    samples, plus one summary bit per mask word in LDS), then one wave walks the
-//    hysteresis chain (range start = first low at or after the merge point, range
-//    end = first high after it) with next-set-bit queries on the masks and maps
-//    each kept range to TBC rows against the field's line locations in LDS.
+// Four launches:
+//  * dropout_mean_kernel, 128 workgroups per field: workgroup g stages the window
+//    values under node g of depth 7 of the pairwise tree in LDS (16-byte loads); its
+//    threads are the depth-15 slots below, and the leftmost slot under each leaf
+//    sums the leaf in the reference's order (eight accumulators); the workgroup
+//    combines its slots up to depth 7.  dropout_threshold_kernel (one workgroup per
+//    field) completes the tree to the root and writes the threshold and the
+//    detection range.
+//  * dropout_masks_kernel, 32 workgroups per field: every sample of the detection
+//    range is classified low (<= threshold) or high (>= threshold * hysteresis) into
+//    two bit masks, one wave ballot per 64 samples, with one summary bit per mask word.
+//  * dropout_walk_kernel, one workgroup per field: the summaries and line locations
+//    go to LDS and one wave walks the hysteresis chain (range start = first low at
+//    or after the merge point, range end = first high after it) with next-set-bit
+//    queries on the masks, mapping each kept range to TBC rows as it goes.
 //
 // Line convention: output line k of a field starts at its location k (the reference's
 // location index outputFirstLine + k, TbcFieldDecodePipeline.cs:2475-2483, which is its
@@ -32,23 +35,29 @@
 namespace vhsd {
 namespace {
 
-constexpr int kMeanDepth = 15;     // slots per field: 2^15
-constexpr int kMeanGroupDepth = 6; // workgroups per field: 2^6
-constexpr int kMeanThreads = 1 << (kMeanDepth - kMeanGroupDepth);
-constexpr int kDropThreads = 1024;
+constexpr int kSlotDepth = 15;        // leaves are at depth <= 15 (windows <= 3.6 M samples)
+constexpr int kGroupDepth = 7;        // mean workgroups per field: 2^7
+constexpr int kGroups = 1 << kGroupDepth;
+constexpr int kSlotThreads = 1 << (kSlotDepth - kGroupDepth);
+constexpr int kMaskGroups = 32;       // mask workgroups per field
+constexpr int kMaskThreads = 1024;
+constexpr int kWalkThreads = 256;
+static_assert(kGroups == 128, "the threshold kernel reduces the group minima over two waves");
 
 // Field f's read window [w0, w1) in span coordinates: DecodeReadWindowPlanner.Resolve
 // (Decode/DecodeReadWindowPlanner.cs:37-63) of the intended start lead_lines before
 // line 0, with the requested read length; the same arithmetic places the TBC origin.
 __device__ inline void read_window(const DropoutArgs& a, double line0, long long& w0, long long& w1) {
+  // every position is below 2^31 (checked on the host), so 32-bit unsigned division
   double intended = line0 - a.lead_lines * a.line_len;
   intended = intended > 0.0 ? intended : 0.0;
   long long read = (long long)intended - a.block_cut;
   read = read > 0 ? read : 0;
-  const long long aligned = (read / a.block_length) * a.block_length;
-  const long long end = aligned + a.read_samples;
-  w0 = (aligned / a.stride) * a.stride;
-  w1 = (end / a.stride + 1) * a.stride;
+  const unsigned bl = (unsigned)a.block_length, st = (unsigned)a.stride;
+  const unsigned aligned = ((unsigned)read / bl) * bl;
+  const unsigned end = aligned + (unsigned)a.read_samples;
+  w0 = (long long)((aligned / st) * st);
+  w1 = (long long)(end / st + 1) * st;
   w0 = w0 < a.span_len ? w0 : a.span_len;
   w1 = w1 < a.span_len ? w1 : a.span_len;
 }
@@ -56,7 +65,7 @@ __device__ inline void read_window(const DropoutArgs& a, double line0, long long
 // Node of the pairwise tree of n values at `level` along path bits `path` (most
 // significant first): a node splits at half its length rounded down to a multiple of
 // 8 while it is longer than 128.  `exists` is false below a leaf.
-__device__ inline void pw_node(long long n, int level, unsigned path, long long& off, long long& len, bool& exists) {
+__device__ inline void pw_node(int n, int level, unsigned path, int& off, int& len, bool& exists) {
   off = 0;
   len = n;
   exists = true;
@@ -65,8 +74,28 @@ __device__ inline void pw_node(long long n, int level, unsigned path, long long&
       exists = false;
       return;
     }
-    const long long half = len / 2;
-    const long long split = half - half % 8;
+    const int split = (len >> 1) & ~7;
+    if ((path >> (level - 1 - l)) & 1u) {
+      off += split;
+      len -= split;
+    } else {
+      len = split;
+    }
+  }
+}
+
+// Like pw_node, but a path under a shallower leaf ends at that leaf; `owner` is true
+// when the path is the leftmost one under it (the one that sums it).
+__device__ inline void pw_descend(int n, int level, unsigned path, int& off, int& len, bool& owner) {
+  off = 0;
+  len = n;
+  owner = true;
+  for (int l = 0; l < level; l++) {
+    if (len <= 128) {
+      owner = (path & ((1u << (level - l)) - 1)) == 0;
+      return;
+    }
+    const int split = (len >> 1) & ~7;
     if ((path >> (level - 1 - l)) & 1u) {
       off += split;
       len -= split;
@@ -101,52 +130,243 @@ __device__ float pw_leaf(const float* v, int len) {
   return s;
 }
 
-__global__ void __launch_bounds__(kMeanThreads) dropout_mean_slots_kernel(DropoutArgs a) {
+__global__ void __launch_bounds__(kSlotThreads) dropout_mean_kernel(DropoutArgs a) {
 #pragma clang fp contract(off)
+  extern __shared__ float stage[];
+  __shared__ float v[kSlotThreads];
+  __shared__ float wmin[kSlotThreads / 64];
   const int f = blockIdx.y;
   if (f >= *a.field_count) return;
-  __shared__ float v[kMeanThreads];
+  const int g = blockIdx.x, t = threadIdx.x;
+  long long w0, w1;
+  read_window(a, a.line0[f], w0, w1);
+  const int n = (int)(w1 - w0);
+  int goff, glen;
+  bool gown;
+  pw_descend(n, kGroupDepth, (unsigned)g, goff, glen, gown);
+  // groups under a leaf another group owns contribute nothing (their value is never read)
+  if (!(gown && n > 0 && glen <= a.stage_cap)) {
+    if (t == 0) {
+      a.partials[(size_t)f * kGroups + g] = 0.0f;
+      a.group_min[(size_t)f * kGroups + g] = __builtin_inff();
+    }
+    return;
+  }
+  const float* src = a.envelope + w0 + goff;
+  if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+    // 16-byte loads, up to eight in flight per thread
+    const int n4 = glen >> 2;
+    const float4* s4 = reinterpret_cast<const float4*>(src);
+    float4* d4 = reinterpret_cast<float4*>(stage);
+    int i = t;
+    for (; i + 7 * kSlotThreads < n4; i += 8 * kSlotThreads) {
+      float4 x[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) x[k] = s4[i + k * kSlotThreads];
+#pragma unroll
+      for (int k = 0; k < 8; k++) d4[i + k * kSlotThreads] = x[k];
+    }
+    for (; i < n4; i += kSlotThreads) d4[i] = s4[i];
+    for (int j = (n4 << 2) + t; j < glen; j += kSlotThreads) stage[j] = src[j];
+  } else {
+    for (int j = t; j < glen; j += kSlotThreads) stage[j] = src[j];
+  }
+  __syncthreads();
+  {
+    // the smallest staged value (NaN never compares low, and fminf skips it)
+    float m = __builtin_inff();
+    for (int j = t; j < glen; j += kSlotThreads) m = fminf(m, stage[j]);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) m = fminf(m, __shfl_xor(m, d));
+    if ((t & 63) == 0) wmin[t >> 6] = m;
+    __syncthreads();
+    if (t == 0) {
+      float gm = wmin[0];
+      for (int w = 1; w < kSlotThreads / 64; w++) gm = fminf(gm, wmin[w]);
+      a.group_min[(size_t)f * kGroups + g] = gm;
+    }
+  }
+  float gval;
+  if (glen <= 128) {
+    // the group's node is itself a leaf (at depth 7 or above)
+    if (t != 0) return;
+    gval = pw_leaf(stage, glen);
+  } else {
+    // leaves: eight lanes per leaf, lane k running accumulator k (elements k, k+8, ...);
+    // the three xor-shuffles combine them as ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7))
+    const int oct = t >> 3, k8 = t & 7;
+    for (int q = 0; q < 8; q++) {
+      const int slot = oct * 8 + q;
+      int off, len;
+      bool own;
+      pw_descend(glen, kSlotDepth - kGroupDepth, (unsigned)slot, off, len, own);
+      const bool vec = own && len >= 8 && len <= 128;
+      const int vend = len - len % 8;
+      float r = 0.0f;
+      if (vec) {
+        r = stage[off + k8];
+        for (int i = off + 8 + k8; i < off + vend; i += 8) r += stage[i];
+      }
+      r = r + __shfl_xor(r, 1);
+      r = r + __shfl_xor(r, 2);
+      r = r + __shfl_xor(r, 4);
+      if (k8 == 0) {
+        float sum = 0.0f;
+        if (own && len <= 128) {
+          if (len < 8) {
+            sum = -0.0f;
+            for (int i = 0; i < len; i++) sum += stage[off + i];
+          } else {
+            sum = r;
+            for (int i = off + vend; i < off + len; i++) sum += stage[i];
+          }
+        }
+        v[slot] = sum;
+      }
+    }
+    __syncthreads();
+    for (int level = kSlotDepth - 1; level >= kGroupDepth; level--) {
+      const int cnt = 1 << (level - kGroupDepth);
+      if (t < cnt) {
+        int o2, l2;
+        bool ex;
+        pw_node(glen, level - kGroupDepth, (unsigned)t, o2, l2, ex);
+        if (ex && l2 > 128) {
+          const int left = t << (kSlotDepth - level);
+          const int right = (2 * t + 1) << (kSlotDepth - 1 - level);
+          v[left] = v[left] + v[right];
+        }
+      }
+      __syncthreads();
+    }
+    if (t != 0) return;
+    gval = v[0];
+  }
+  a.partials[(size_t)f * kGroups + g] = gval;
+}
+
+// One workgroup per field: the pairwise tree from depth 7 to the root, then the
+// threshold and the detection range.
+__global__ void __launch_bounds__(kGroups) dropout_threshold_kernel(DropoutArgs a) {
+#pragma clang fp contract(off)
+  __shared__ float v[kGroups];
+  __shared__ float wm[kGroups / 64];
+  const int f = blockIdx.x;
+  if (f >= *a.field_count) return;
+  const int t = threadIdx.x;
   long long w0, w1;
   read_window(a, a.line0[f], w0, w1);
   const long long n = w1 - w0;
-  const int t = threadIdx.x;
-  const unsigned slot = (blockIdx.x << (kMeanDepth - kMeanGroupDepth)) | t;
-  // descend to this slot's leaf; only the leftmost slot under a leaf sums it
-  long long off = 0, len = n;
-  bool mine = true;
-  for (int l = 0; l < kMeanDepth; l++) {
-    if (len <= 128) {
-      mine = (slot & ((1u << (kMeanDepth - l)) - 1)) == 0;
-      break;
-    }
-    const long long half = len / 2;
-    const long long split = half - half % 8;
-    if ((slot >> (kMeanDepth - 1 - l)) & 1u) {
-      off += split;
-      len -= split;
-    } else {
-      len = split;
-    }
-  }
-  // windows are limited on the host so that every depth-15 node is a leaf (len <= 128)
-  v[t] = (mine && n > 0 && len <= 128) ? pw_leaf(a.envelope + w0 + off, (int)len) : 0.0f;
+  v[t] = a.partials[(size_t)f * kGroups + t];
+  float m = a.group_min[(size_t)f * kGroups + t];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) m = fminf(m, __shfl_xor(m, d));
+  if ((t & 63) == 0) wm[t >> 6] = m;
   __syncthreads();
-  for (int level = kMeanDepth - 1; level >= kMeanGroupDepth; level--) {
-    const int cnt = 1 << (level - kMeanGroupDepth);
+  for (int level = kGroupDepth - 1; level >= 0; level--) {
+    const int cnt = 1 << level;
     if (t < cnt) {
-      const unsigned j = (blockIdx.x << (level - kMeanGroupDepth)) | t;
-      long long o2, l2;
+      int o2, l2;
       bool ex;
-      pw_node(n, level, j, o2, l2, ex);
+      pw_node((int)n, level, (unsigned)t, o2, l2, ex);
       if (ex && l2 > 128) {
-        const int left = t << (kMeanDepth - level);
-        const int right = (2 * t + 1) << (kMeanDepth - 1 - level);
+        const int left = t << (kGroupDepth - level);
+        const int right = (2 * t + 1) << (kGroupDepth - 1 - level);
         v[left] = v[left] + v[right];
       }
     }
     __syncthreads();
   }
-  if (t == 0) a.partials[(size_t)f * (1 << kMeanGroupDepth) + blockIdx.x] = v[0];
+  if (t != 0) return;
+  // NumpyReduction.MeanFloat32: float sum / count (NaN for an empty window)
+  const float mean = n > 0 ? v[0] / (float)n : __builtin_nanf("");
+  const double thr = a.absolute_threshold >= 0.0 ? a.absolute_threshold : (double)(mean * (float)a.fraction);
+  const int nloc = a.nloc;
+  const double* loc = a.locations + (size_t)f * nloc;
+  const int line_count = a.field_first[f] ? a.first_lines : a.second_lines;
+  const int end_line = nloc - 1 < line_count + 1 ? nloc - 1 : line_count + 1;
+  long long a0 = 0, b0 = 0;
+  if (end_line > 0) {
+    a0 = (long long)floor(loc[0]);
+    a0 = a0 < w0 ? w0 : (a0 > w1 ? w1 : a0);
+    b0 = (long long)ceil(loc[end_line]);
+    b0 = b0 < a0 ? a0 : (b0 > w1 ? w1 : b0);
+  }
+  const long long span = b0 - a0;
+  const int nwords = (int)((span + 63) >> 6);
+  DropoutField inf;
+  inf.threshold = thr;
+  inf.up = thr * a.hysteresis;
+  inf.a0 = a0;
+  inf.b0 = b0;
+  inf.nwords = nwords;
+  inf.end_line = end_line;
+  inf.status = span <= 0 ? 0 : ((nwords > a.words_cap || nwords > kDropSummaryWords * 32) ? -1 : 1);
+  // no window sample at or below the threshold: no range can open, the masks and the
+  // walk have nothing to do
+  const float window_min = fminf(wm[0], wm[1]);
+  if (inf.status == 1 && (double)window_min > thr) inf.status = 0;
+  a.info[f] = inf;
+}
+
+__global__ void __launch_bounds__(kMaskThreads) dropout_masks_kernel(DropoutArgs a) {
+#pragma clang fp contract(off)
+  __shared__ unsigned lsum[kDropSummaryWords / kMaskGroups];
+  __shared__ unsigned hsum[kDropSummaryWords / kMaskGroups];
+  const int f = blockIdx.y;
+  if (f >= *a.field_count) return;
+  const int g = blockIdx.x, t = threadIdx.x;
+  const DropoutField inf = a.info[f];
+  if (inf.status != 1) return;
+  const double thr = inf.threshold, up = inf.up;
+  const long long a0 = inf.a0, b0 = inf.b0;
+  const int nwords = inf.nwords;
+  const int nsw = (nwords + 31) >> 5;
+  const int per = (nsw + kMaskGroups - 1) / kMaskGroups;  // summary words per workgroup (<= 128)
+  const int sw0 = g * per;
+  const int sw1 = sw0 + per < nsw ? sw0 + per : nsw;
+  if (sw0 >= sw1) return;
+  if (t < per) {
+    lsum[t] = 0u;
+    hsum[t] = 0u;
+  }
+  __syncthreads();
+  unsigned long long* lowm = a.masks + (size_t)f * 2 * a.words_cap;
+  unsigned long long* highm = lowm + a.words_cap;
+  const int lane = t & 63, wave = t >> 6;
+  const int wbeg = sw0 << 5;
+  const int wend = (sw1 << 5) < nwords ? (sw1 << 5) : nwords;
+  // each wave takes eight words at a time, the eight loads in flight together
+  constexpr int kWaves = kMaskThreads / 64;
+  for (int w8 = wbeg + 8 * wave; w8 < wend; w8 += 8 * kWaves) {
+    float x[8];
+    bool in[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const long long i = a0 + ((long long)(w8 + k) << 6) + lane;
+      in[k] = w8 + k < wend && i < b0;
+      x[k] = in[k] ? a.envelope[i] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int w = w8 + k;
+      const double xd = (double)x[k];
+      const unsigned long long lo = __ballot(in[k] && xd <= thr);
+      const unsigned long long hi = __ballot(in[k] && xd >= up);
+      if (lane == 0 && w < wend) {
+        lowm[w] = lo;
+        highm[w] = hi;
+        if (lo) atomicOr(&lsum[(w >> 5) - sw0], 1u << (w & 31));
+        if (hi) atomicOr(&hsum[(w >> 5) - sw0], 1u << (w & 31));
+      }
+    }
+  }
+  __syncthreads();
+  unsigned* gs = a.summaries + (size_t)f * 2 * kDropSummaryWords;
+  if (t < sw1 - sw0) {
+    gs[sw0 + t] = lsum[t];
+    gs[kDropSummaryWords + sw0 + t] = hsum[t];
+  }
 }
 
 // First set bit at relative position >= rel in a bit mask of nwords 64-bit words with
@@ -177,85 +397,38 @@ __device__ long long next_set(const unsigned long long* mask, const unsigned* su
   return -1;
 }
 
-__global__ void __launch_bounds__(kDropThreads) dropout_fields_kernel(DropoutArgs a) {
+__global__ void __launch_bounds__(kWalkThreads) dropout_walk_kernel(DropoutArgs a) {
 #pragma clang fp contract(off)
-  const int f = blockIdx.x;
-  if (f >= *a.field_count) return;
   __shared__ double loc[kDropMaxLocations];
-  __shared__ float pv[1 << kMeanGroupDepth];
   __shared__ unsigned lsum[kDropSummaryWords];
   __shared__ unsigned hsum[kDropSummaryWords];
+  const int f = blockIdx.x;
+  if (f >= *a.field_count) return;
   const int t = threadIdx.x;
-  const int nloc = a.nloc;
-  for (int i = t; i < nloc; i += kDropThreads) loc[i] = a.locations[(size_t)f * nloc + i];
-  if (t < (1 << kMeanGroupDepth)) pv[t] = a.partials[(size_t)f * (1 << kMeanGroupDepth) + t];
-  long long w0, w1;
-  read_window(a, a.line0[f], w0, w1);
-  const long long n = w1 - w0;
-  __syncthreads();
-  for (int level = kMeanGroupDepth - 1; level >= 0; level--) {
-    const int cnt = 1 << level;
-    if (t < cnt) {
-      long long o2, l2;
-      bool ex;
-      pw_node(n, level, (unsigned)t, o2, l2, ex);
-      if (ex && l2 > 128) {
-        const int left = t << (kMeanGroupDepth - level);
-        const int right = (2 * t + 1) << (kMeanGroupDepth - 1 - level);
-        pv[left] = pv[left] + pv[right];
-      }
-    }
-    __syncthreads();
-  }
-  // NumpyReduction.MeanFloat32: float sum / count (NaN for an empty window)
-  const float mean = n > 0 ? pv[0] / (float)n : __builtin_nanf("");
-  const double thr = a.absolute_threshold >= 0.0 ? a.absolute_threshold : (double)(mean * (float)a.fraction);
-  const double up = thr * a.hysteresis;
-  const int line_count = a.field_first[f] ? a.first_lines : a.second_lines;
-  const int end_line = nloc - 1 < line_count + 1 ? nloc - 1 : line_count + 1;
-  long long a0 = 0, b0 = 0;
-  if (end_line > 0) {
-    a0 = (long long)floor(loc[0]);
-    a0 = a0 < w0 ? w0 : (a0 > w1 ? w1 : a0);
-    b0 = (long long)ceil(loc[end_line]);
-    b0 = b0 < a0 ? a0 : (b0 > w1 ? w1 : b0);
-  }
-  const long long span = b0 - a0;
-  const int nwords = (int)((span + 63) >> 6);
-  if (span <= 0 || nwords > a.words_cap || nwords > kDropSummaryWords * 32) {
+  const DropoutField inf = a.info[f];
+  if (inf.status != 1) {
     if (t == 0) {
-      // a detection range longer than the mask capacity is reported as -1
-      const int rc = span <= 0 ? 0 : -1;
-      a.row_count[f] = rc;
-      if (a.range_count) a.range_count[f] = rc;
+      a.row_count[f] = inf.status;
+      if (a.range_count) a.range_count[f] = inf.status;
     }
     return;
   }
+  const int nloc = a.nloc;
+  for (int i = t; i < nloc; i += kWalkThreads) loc[i] = a.locations[(size_t)f * nloc + i];
+  const int nwords = inf.nwords;
   const int nsw = (nwords + 31) >> 5;
-  for (int i = t; i < nsw; i += kDropThreads) {
-    lsum[i] = 0u;
-    hsum[i] = 0u;
+  const unsigned* gs = a.summaries + (size_t)f * 2 * kDropSummaryWords;
+  for (int i = t; i < nsw; i += kWalkThreads) {
+    lsum[i] = gs[i];
+    hsum[i] = gs[kDropSummaryWords + i];
   }
   __syncthreads();
-  unsigned long long* lowm = a.masks + (size_t)f * 2 * a.words_cap;
-  unsigned long long* highm = lowm + a.words_cap;
-  const int lane = t & 63, wave = t >> 6;
-  for (int w = wave; w < nwords; w += kDropThreads / 64) {
-    const long long i = a0 + ((long long)w << 6) + lane;
-    const bool in = i < b0;
-    const double x = in ? (double)a.envelope[i] : 0.0;
-    const unsigned long long lo = __ballot(in && x <= thr);
-    const unsigned long long hi = __ballot(in && x >= up);
-    if (lane == 0) {
-      lowm[w] = lo;
-      highm[w] = hi;
-      if (lo) atomicOr(&lsum[w >> 5], 1u << (w & 31));
-      if (hi) atomicOr(&hsum[w >> 5], 1u << (w & 31));
-    }
-  }
-  __syncthreads();
-  if (wave != 0) return;
-  // ---- one wave: hysteresis chain, then the TBC rows of each kept range
+  if (t >= 64) return;
+  const int lane = t;
+  const unsigned long long* lowm = a.masks + (size_t)f * 2 * a.words_cap;
+  const unsigned long long* highm = lowm + a.words_cap;
+  const long long a0 = inf.a0, span = inf.b0 - inf.a0;
+  const int end_line = inf.end_line;
   const int cap = a.capacity;
   const int out_len = a.out_len;
   const size_t rbase = (size_t)f * cap;
@@ -286,6 +459,8 @@ __global__ void __launch_bounds__(kDropThreads) dropout_fields_kernel(DropoutArg
       }
       nr++;
       if (mapping) {
+        // TbcDropoutMapper.MapTapeRfToTbc: the start line is searched forward from the
+        // previous range's end line; rows run line by line to the line holding the end
         const double S = (double)(a0 + s), E = (double)(a0 + e);
         bool found = false;
         while (li < end_line) {
@@ -337,12 +512,30 @@ __global__ void __launch_bounds__(kDropThreads) dropout_fields_kernel(DropoutArg
 }  // namespace
 }  // namespace vhsd
 
+int vhsd_dropout_stage_floats(long long max_window) {
+  // the longest node at depth 7 of a window of max_window values (right children are
+  // the longer ones)
+  long long len = max_window;
+  for (int l = 0; l < vhsd::kGroupDepth && len > 128; l++) len -= (len / 2) - (len / 2) % 8;
+  return (int)len;
+}
+
 hipError_t vhsd_launch_dropouts(const vhsd::DropoutArgs* a, int max_fields, hipStream_t stream) {
   if (max_fields <= 0) return hipSuccess;
-  if (a->nloc < 2 || a->nloc > vhsd::kDropMaxLocations || a->capacity <= 0 || a->words_cap <= 0)
+  if (a->nloc < 2 || a->nloc > vhsd::kDropMaxLocations || a->capacity <= 0 || a->words_cap <= 0 ||
+      a->stage_cap <= 0 || a->stage_cap > 36864)
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(vhsd::dropout_mean_slots_kernel, dim3(1 << vhsd::kMeanGroupDepth, max_fields),
-                     dim3(vhsd::kMeanThreads), 0, stream, *a);
-  hipLaunchKernelGGL(vhsd::dropout_fields_kernel, dim3(max_fields), dim3(vhsd::kDropThreads), 0, stream, *a);
+  const size_t stage_bytes = (size_t)a->stage_cap * sizeof(float);
+  if (stage_bytes > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void*)vhsd::dropout_mean_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)stage_bytes);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(vhsd::dropout_mean_kernel, dim3(vhsd::kGroups, max_fields), dim3(vhsd::kSlotThreads),
+                     stage_bytes, stream, *a);
+  hipLaunchKernelGGL(vhsd::dropout_threshold_kernel, dim3(max_fields), dim3(vhsd::kGroups), 0, stream, *a);
+  hipLaunchKernelGGL(vhsd::dropout_masks_kernel, dim3(vhsd::kMaskGroups, max_fields), dim3(vhsd::kMaskThreads), 0,
+                     stream, *a);
+  hipLaunchKernelGGL(vhsd::dropout_walk_kernel, dim3(max_fields), dim3(vhsd::kWalkThreads), 0, stream, *a);
   return hipGetLastError();
 }

@@ -182,6 +182,12 @@ struct TbcParams {
 constexpr int kDropMaxLocations = 512;
 constexpr int kDropSummaryWords = 4096;  // summary bits: 131072 mask words per field
 
+struct DropoutField {               // per-field state the mask pass hands to the walk
+  double threshold, up;
+  long long a0, b0;                 // detection range [a0, b0) in span samples
+  int nwords, end_line, status;     // status: 1 walk, 0 empty range, -1 range over capacity
+};
+
 struct DropoutArgs {
   const float* envelope;           // span plane
   long long span_len;
@@ -203,8 +209,12 @@ struct DropoutArgs {
   double line_len, lead_lines;
   long long block_length, block_cut, stride;
   // scratch
-  float* partials;                 // [fields][64]
+  float* partials;                 // [fields][128]
+  float* group_min;                // [fields][128] smallest envelope value under each group
+  int stage_cap;                   // LDS floats for one depth-7 node of the window tree
+  DropoutField* info;              // [fields]
   unsigned long long* masks;       // [fields][2][words_cap]
+  unsigned* summaries;             // [fields][2][kDropSummaryWords]
   int words_cap;
   // outputs
   int* row_count;
@@ -223,6 +233,7 @@ size_t vhsd_block_lds_bytes();
 hipError_t vhsd_launch_block_demod(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_chroma(const vhsd::ChromaArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_repair(const vhsd::RepairArgs* a, int nblocks, hipStream_t stream);
+int vhsd_dropout_stage_floats(long long max_window);
 hipError_t vhsd_launch_dropouts(const vhsd::DropoutArgs* a, int max_fields, hipStream_t stream);
 hipError_t vhsd_launch_locate(const float* sync_ref, long long span_len, const int* starts, const int* lens,
                               const int* pulse_count, int capacity, const vhsd::LocateParams* lp, int max_fields,

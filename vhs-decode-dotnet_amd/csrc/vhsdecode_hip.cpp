@@ -275,6 +275,8 @@ struct vhsdecode_hip_demod_context {
   int f_ref_nloc = 0;  // locations per field of the last field decode
   DevBuf<float> d_part;               // dropout mean partial sums
   DevBuf<unsigned long long> d_mask;  // dropout low/high bit masks
+  DevBuf<unsigned> d_sum;             // dropout mask summaries
+  DevBuf<vhsd::DropoutField> d_info;  // dropout per-field state
   // device line locator
   DevBuf<double> l_line0, l_locs;
   DevBuf<int> l_first, l_hit, l_count;
@@ -1726,8 +1728,10 @@ int32_t vhsdecode_hip_detect_dropouts_device(vhsdecode_hip_demod_context* c, con
     VHSD_HIP(hipSetDevice(c->device));
     hipStream_t st = pick_stream(c, stream);
     const int words_cap = (int)std::min<int64_t>(span_length / 64 + 2, (int64_t)vhsd::kDropSummaryWords * 32);
-    VHSD_HIP(c->d_part.ensure((size_t)max_fields * 64));
+    VHSD_HIP(c->d_part.ensure((size_t)max_fields * 2 * 128));
     VHSD_HIP(c->d_mask.ensure((size_t)max_fields * 2 * words_cap));
+    VHSD_HIP(c->d_sum.ensure((size_t)max_fields * 2 * vhsd::kDropSummaryWords));
+    VHSD_HIP(c->d_info.ensure((size_t)max_fields));
     vhsd::DropoutArgs a{};
     a.envelope = envelope;
     a.span_len = span_length;
@@ -1752,7 +1756,11 @@ int32_t vhsdecode_hip_detect_dropouts_device(vhsdecode_hip_demod_context* c, con
     a.block_cut = p->block_cut;
     a.stride = p->block_stride;
     a.partials = c->d_part.p;
+    a.group_min = c->d_part.p + (size_t)max_fields * 128;
+    a.stage_cap = vhsd_dropout_stage_floats(window_max);
+    a.info = c->d_info.p;
     a.masks = c->d_mask.p;
+    a.summaries = c->d_sum.p;
     a.words_cap = words_cap;
     a.row_count = row_count;
     a.field_line = field_line;
