@@ -31,7 +31,10 @@ __device__ __forceinline__ float vhs_atan2(float y, float x) {
   const float min_normal = 1.17549435E-38f;
   x += copysignf(min_normal, x);
   const bool swap = fabsf(x) < fabsf(y);
-  const float input = swap ? x / y : y / x;
+  // one division of selected operands (the compiler otherwise emits both quotients
+  // under divergent branches)
+  const float num = swap ? x : y, den = swap ? y : x;
+  const float input = num / den;
   const float sq = input * input;
   float r = input * (0.99997726f +
                      (sq * (-0.33262347f +
