@@ -255,7 +255,6 @@ __device__ __forceinline__ void vhs_block_demod_body(const DemodArgs& a, const i
   const long long start = a.block_start[blk];
   const long long out_base = a.out_offset[blk];
   load_octant(s, a.oct);
-  load_trans<1>(a.env, s.scratch + kEnvTransOffset);
   // ---- stage A: rfft of the int16 block
   c32 v[16];
   const int16_t* x = a.rf + start;
@@ -303,7 +302,7 @@ __device__ __forceinline__ void vhs_block_demod_body(const DemodArgs& a, const i
   __syncthreads();
   // the VHS envelope filter is first order (DecodeFilterSet.BuildVhsEnvelopeSos): one section
   const bool weak = sos_filtfilt_block_t<1>(a.env, reinterpret_cast<float*>(s.buf) + kSosMargin, 32768, s.scratch,
-                                            s.scratch + kEnvTransOffset, a.out_env + out_base, a.out_begin,
+                                            a.out_env + out_base, a.out_begin,
                                             a.out_count, /*roll=*/0, false);
   // ---- flags
   const unsigned long long any_spike = __ballot(spike);
@@ -371,8 +370,6 @@ __device__ __forceinline__ void vhs_block_demod_boost_body(const DemodArgs& a, c
   float2* stash = a.stash + (long long)blk * kStashEntries;
   float2* aux = a.aux + (long long)blk * kAuxEntries;
   load_octant(s, a.oct);
-  load_trans<1>(a.env, s.scratch + kEnvTransOffset);
-  load_trans<1>(a.boost, s.scratch + kAuxTransOffset);
   c32 v[16];
   const int16_t* x = a.rf + start;
 #pragma unroll
@@ -412,7 +409,7 @@ __device__ __forceinline__ void vhs_block_demod_boost_body(const DemodArgs& a, c
     }
   }
   __syncthreads();
-  const bool weak = sos_filtfilt_block_t<1>(a.env, ext, 32768, s.scratch, s.scratch + kEnvTransOffset,
+  const bool weak = sos_filtfilt_block_t<1>(a.env, ext, 32768, s.scratch,
                                             a.out_env + out_base, a.out_begin, a.out_count, /*roll=*/0, false);
   c32 fb;
   if (!weak) {  // uniform: every thread holds the same flag
@@ -435,7 +432,7 @@ __device__ __forceinline__ void vhs_block_demod_boost_body(const DemodArgs& a, c
       hx[2 * k + 1] = xv.y;
     }
     __syncthreads();
-    sos_filtfilt_block_t<1>(a.boost, ext, 32768, s.scratch, s.scratch + kAuxTransOffset, nullptr, 0, 0, 0, false);
+    sos_filtfilt_block_t<1>(a.boost, ext, 32768, s.scratch, nullptr, 0, 0, 0, false);
 #pragma unroll
     for (int m = 0; m < 16; m++) {
       const int k = j + m * kFftThreads;
@@ -694,15 +691,13 @@ __device__ __forceinline__ void vhs_block_chroma_body(const ChromaArgs& a, const
   const int16_t* x = a.rf + a.block_start[blk];
   float* ext = reinterpret_cast<float*>(s.buf) + kSosMargin;
   float* raw = ext + a.sos.pad;
-  float* trans = reinterpret_cast<float*>(s.oct);  // the twiddle octant is unused here
-  load_trans<S>(a.sos, trans);
   int16_t xv[32];
 #pragma unroll
   for (int m = 0; m < 32; m++) xv[m] = row_at(x, m);
 #pragma unroll
   for (int m = 0; m < 32; m++) row_at(raw, m) = (float)xv[m];
   __syncthreads();
-  sos_filtfilt_block_t<S>(a.sos, ext, 32768, s.scratch, trans, a.out_chroma + a.out_offset[blk], a.out_begin,
+  sos_filtfilt_block_t<S>(a.sos, ext, 32768, s.scratch, a.out_chroma + a.out_offset[blk], a.out_begin,
                           a.out_count, a.chroma_offset, /*subtract_mean=*/true);
 }
 
@@ -741,9 +736,7 @@ __device__ __forceinline__ void vhs_block_subdeemph_body(const SubDeemphArgs& a,
   float2* aux = a.aux + (long long)blk * kAuxEntries;
   const float inv_n = 1.0f / 32768.0f;
   load_octant(s, a.oct);
-  load_trans<1>(a.amp, s.scratch + kEnvTransOffset);
   if (a.notch) {
-    load_trans<1>(a.notch_sos, s.scratch + kAuxTransOffset);
     // the whole block's video, as the video kernel formed it: irfft(D x Video) / N + corr
     c32 w[16];
 #pragma unroll
@@ -811,7 +804,7 @@ __device__ __forceinline__ void vhs_block_subdeemph_body(const SubDeemphArgs& a,
     row_at(st_ev, m) = make_float2(ev.x, o.x);
   }
   __syncthreads();
-  sos_filtfilt_block_t<1>(a.amp, ext, 32768, s.scratch, s.scratch + kEnvTransOffset, nullptr, 0, 0, 0, false);
+  sos_filtfilt_block_t<1>(a.amp, ext, 32768, s.scratch, nullptr, 0, 0, 0, false);
   const float* y = ext + a.amp.pad;
   float* vout = a.out_video + a.out_offset[blk];
   auto gain = [&](float amp) {
@@ -851,7 +844,7 @@ __device__ __forceinline__ void vhs_block_subdeemph_body(const SubDeemphArgs& a,
     nx[2 * k + 1] = c.y;
   }
   __syncthreads();
-  sos_filtfilt_block_t<1>(a.notch_sos, ext, 32768, s.scratch, s.scratch + kAuxTransOffset, vout, a.out_begin,
+  sos_filtfilt_block_t<1>(a.notch_sos, ext, 32768, s.scratch, vout, a.out_begin,
                           a.out_count, /*roll=*/0, false);
 }
 

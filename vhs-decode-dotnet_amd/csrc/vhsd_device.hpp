@@ -17,11 +17,6 @@ __device__ __forceinline__ int opaque_tid() {
 
 
 constexpr int kScratchFloats = 512;
-// LDS scratch offset of the envelope filter's transition matrices (kScanLevels * 4 floats)
-constexpr int kEnvTransOffset = 256;
-// LDS scratch offset of a second one-section filter's matrices (RF boost band-pass,
-// fsc notch)
-constexpr int kAuxTransOffset = 320;
 
 // PortedMath.VhsRustAtan2Approximation (PortedMath.cs:519-544): FP32 polynomial
 // atan2, evaluated with separately rounded operations (no contraction) so that

@@ -100,7 +100,7 @@ int32_t prepare_sos(const double* sos, int sections, int n, vhsd::SosScanArgs* a
   // must fit the 33792-float block buffer
   a->chunk = vhsd::kChunk;
   if (a->pad > 64 || L > vhsd::kScanThreads * vhsd::kChunk ||
-      vhsd::kSosMargin + L + vhsd::kChunk > 2 * vhsd::kFftBufEntries)
+      vhsd::kSosMargin + L + vhsd::kChunk + 32 * 2 * sections > 2 * vhsd::kFftBufEntries)
     return fail(VHSDECODE_HIP_E_UNSUPPORTED, "SOS padding too long");
   float c32[vhsd::kMaxSections][6];
   for (int s = 0; s < sections; s++) {
@@ -146,11 +146,30 @@ int32_t prepare_sos(const double* sos, int sections, int n, vhsd::SosScanArgs* a
     base = matmul(base, base);
     e >>= 1;
   }
-  trans->assign((size_t)vhsd::kScanLevels * W * W, 0.0f);
+  trans->assign((size_t)vhsd::kScanLevels * W * W + (size_t)vhsd::kChunk * W, 0.0f);
   std::vector<double> Q = P;
   for (int d = 0; d < vhsd::kScanLevels; d++) {
     for (int i = 0; i < W * W; i++) (*trans)[(size_t)d * W * W + i] = (float)Q[i];
     Q = matmul(Q, Q);
+  }
+  // zero-input response of a chunk: output t for each unit entry state, in FP64
+  float* resp = trans->data() + (size_t)vhsd::kScanLevels * W * W;
+  for (int col = 0; col < W; col++) {
+    std::vector<double> z(W, 0.0);
+    z[col] = 1.0;
+    for (int t = 0; t < vhsd::kChunk; t++) {
+      double v = 0.0;
+      for (int k = 0; k < sections; k++) {
+        const double b0 = c32[k][0], b1 = c32[k][1], b2 = c32[k][2], a1 = c32[k][4], a2 = c32[k][5];
+        const double f = b0 * v + z[2 * k];
+        const double z1 = b1 * v - a1 * f + z[2 * k + 1];
+        const double z2 = b2 * v - a2 * f;
+        z[2 * k] = z1;
+        z[2 * k + 1] = z2;
+        v = f;
+      }
+      resp[t * W + col] = (float)v;
+    }
   }
   return VHSDECODE_HIP_OK;
 }
