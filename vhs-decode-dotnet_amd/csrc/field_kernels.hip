@@ -547,38 +547,41 @@ __global__ void __launch_bounds__(64) hsync_verify_kernel(const float* const* sp
 // Tbc/VideoOutputConverter.cs:74-90).  One thread per output sample.
 // Source samples are plane offsets; the reference reads (float)video with video in
 // FP64 Hz, reproduced here as (float)((double)offset + bias).
-__device__ __forceinline__ double sample_sinc(const float* src, long long len, double bias, double position,
-                                              const float* __restrict__ lut) {
-  if (!isfinite(position)) return 0.0;
-  const int taps = 16, phases = 65536;
+// Table rows and interpolation weight of SampleSinc for a finite position.
+struct SincPhase {
+  int ci;       // integer part of the FP32 coordinate
+  float alpha;  // weight of the second row
+  int a0, a1;   // table rows (each 16 floats, 64-byte aligned)
+};
+
+__device__ __forceinline__ SincPhase sinc_phase(double position) {
+  const int phases = 65536;
+  SincPhase ph;
   const float coord = (float)position;
-  const int ci = (int)coord;
-  const float frac = coord - (float)ci;
+  ph.ci = (int)coord;
+  const float frac = coord - (float)ph.ci;
   const float pp = frac * (float)phases;
   const int ps = (int)pp;
   const int pe = ps + 1;
-  const float alpha = pp - (float)ps;
+  ph.alpha = pp - (float)ps;
   int a0 = ps < 0 ? ps + phases + 1 : ps;
-  a0 = a0 < 0 ? 0 : (a0 > phases ? phases : a0);
+  ph.a0 = a0 < 0 ? 0 : (a0 > phases ? phases : a0);
   int a1 = pe < 0 ? pe + phases + 1 : pe;
-  a1 = a1 < 0 ? 0 : (a1 > phases ? phases : a1);
-  // table rows are 64-byte aligned: fetch each as four 16-byte loads
-  const float4* ws4 = reinterpret_cast<const float4*>(lut + a0 * taps);
-  const float4* we4 = reinterpret_cast<const float4*>(lut + a1 * taps);
-  float ws[16], we[16];
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const float4 a = ws4[q], b = we4[q];
-    ws[4 * q] = a.x; ws[4 * q + 1] = a.y; ws[4 * q + 2] = a.z; ws[4 * q + 3] = a.w;
-    we[4 * q] = b.x; we[4 * q + 1] = b.y; we[4 * q + 2] = b.z; we[4 * q + 3] = b.w;
-  }
-  const long long s0 = (long long)ci - (taps / 2 - 1);
+  ph.a1 = a1 < 0 ? 0 : (a1 > phases ? phases : a1);
+  return ph;
+}
+
+// The 16-tap sum of SampleSinc given the two table rows.
+__device__ __forceinline__ double sinc_sum(const float* src, long long len, double bias, const SincPhase& ph,
+                                           const float (&ws)[16], const float (&we)[16]) {
+  const int taps = 16;
+  const long long s0 = (long long)ph.ci - (taps / 2 - 1);
   const bool interior = len >= taps && s0 >= 0 && s0 <= len - taps;
   double result = 0.0;
 #pragma unroll
   for (int t = 0; t < taps; t++) {
     const float sw = ws[t];
-    const float wgt = __builtin_fmaf(alpha, we[t] - sw, sw);
+    const float wgt = __builtin_fmaf(ph.alpha, we[t] - sw, sw);
     long long si = s0 + t;
     if (!interior) {
       if (si < 0) si += len;
@@ -592,6 +595,23 @@ __device__ __forceinline__ double sample_sinc(const float* src, long long len, d
     result = (double)prod + result;
   }
   return result;
+}
+
+__device__ __forceinline__ double sample_sinc(const float* src, long long len, double bias, double position,
+                                              const float* __restrict__ lut) {
+  if (!isfinite(position)) return 0.0;
+  const SincPhase ph = sinc_phase(position);
+  // table rows are 64-byte aligned: fetch each as four 16-byte loads
+  const float4* ws4 = reinterpret_cast<const float4*>(lut + ph.a0 * 16);
+  const float4* we4 = reinterpret_cast<const float4*>(lut + ph.a1 * 16);
+  float ws[16], we[16];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const float4 a = ws4[q], b = we4[q];
+    ws[4 * q] = a.x; ws[4 * q + 1] = a.y; ws[4 * q + 2] = a.z; ws[4 * q + 3] = a.w;
+    we[4 * q] = b.x; we[4 * q + 1] = b.y; we[4 * q + 2] = b.z; we[4 * q + 3] = b.w;
+  }
+  return sinc_sum(src, len, bias, ph, ws, we);
 }
 
 __global__ void __launch_bounds__(256) tbc_resample_kernel(const float* src, long long src_len, double bias,
@@ -744,17 +764,24 @@ struct FieldTbcParams {
   double conv_offset;
 };
 
+// Stage for the cooperative table-row fetch: per wave, 64 pixels x 8 float4 (two
+// rows), padded to 9 float4 per pixel against LDS bank conflicts.
+constexpr int kTbcRowStride = 9;
+
 __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long long src_len, double bias,
                                                          const long long* origins, const double* locs,
                                                          const double* line_x, const double* line_y0,
                                                          const double* pow_table, FieldTbcParams p,
                                                          const float* __restrict__ lut, unsigned short* out,
                                                          const int* field_count) {
+  __shared__ float4 rows[256 / 64][64 * kTbcRowStride];
   const int f = blockIdx.y;
   if (field_count && f >= *field_count) return;
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;  // a field has < 2^31 output samples
   const int dest_len = p.out_len * p.lines;
-  if (d >= dest_len) return;
+  const int d0 = blockIdx.x * blockDim.x + threadIdx.x;  // a field has < 2^31 output samples
+  // every lane takes part in the row fetch; lanes past the field repeat its last sample
+  const bool valid = d0 < dest_len;
+  const int d = valid ? d0 : dest_len - 1;
   const double* loc = locs + (long long)f * p.nloc;
   const int left = d / p.out_len;
   const int smp = d - left * p.out_len;
@@ -773,10 +800,40 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
     pos -= (double)org;
     adj = p.smooth ? x + (line_y0[(long long)f * p.lines + left] - x) * pow_table[smp] : x;
   }
+  const bool finite = isfinite(pos);
+  const SincPhase ph = finite ? sinc_phase(pos) : SincPhase{0, 0.0f, 0, 0};
+  // SampleSinc's two table rows, fetched cooperatively: in each load, four lanes read
+  // the four 16-byte quarters of one pixel's row, so a wave load touches 16 rows
+  // instead of 64; the rows then reach their pixels' lanes through LDS.
+  const int lane = threadIdx.x & 63;
+  float4* stage = rows[threadIdx.x >> 6];
+  const float4* lut4 = reinterpret_cast<const float4*>(lut);
+  {
+    const int quarter = lane & 3, group = lane >> 2;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int pix = group + 16 * k;
+      const int r0 = __shfl(ph.a0, pix), r1 = __shfl(ph.a1, pix);
+      const float4 c0 = lut4[r0 * 4 + quarter];
+      const float4 c1 = lut4[r1 * 4 + quarter];
+      stage[pix * kTbcRowStride + quarter] = c0;
+      stage[pix * kTbcRowStride + 4 + quarter] = c1;
+    }
+  }
+  __syncthreads();
+  float ws[16], we[16];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const float4 a = stage[lane * kTbcRowStride + q], b = stage[lane * kTbcRowStride + 4 + q];
+    ws[4 * q] = a.x; ws[4 * q + 1] = a.y; ws[4 * q + 2] = a.z; ws[4 * q + 3] = a.w;
+    we[4 * q] = b.x; we[4 * q + 1] = b.y; we[4 * q + 2] = b.z; we[4 * q + 3] = b.w;
+  }
+  if (!valid) return;
   double r;
   {
 #pragma clang fp contract(off)
-    r = (double)(float)(sample_sinc(src + org, src_len - org, bias, pos, lut) * adj);
+    const double sv = finite ? sinc_sum(src + org, src_len - org, bias, ph, ws, we) : 0.0;
+    r = (double)(float)(sv * adj);
   }
   double sd;
   {
