@@ -9,10 +9,11 @@ namespace vhsd {
 
 // ------------------------------------------------------------------ pulses
 // PulseDetection.FindPulsesScalar (Dsp/PulseDetection.cs:195-222): a two-state
-// machine over `value <= high`, NaN samples keep the state.  Each sample is
-// classified independently from the state after the previous sample (the last
-// non-NaN sample at or before it), edges are compacted in order, then pulses
-// pair each end edge with the preceding start edge.
+// machine over `value <= high`, NaN samples keep the state.  Runs of samples are
+// processed independently from the state after the sample before the run (the last
+// non-NaN sample at or before it), edges are compacted in order (per-tile counts,
+// then each tile writes at the sum of the earlier tiles' counts), then pulses pair
+// each end edge with the preceding start edge.
 
 constexpr int kPulseTile = 4096;   // samples per workgroup
 constexpr int kPulseThreads = 256;
@@ -26,27 +27,55 @@ __device__ __forceinline__ int in_pulse_at(const float* s, long long i, double b
   return 0;  // all NaN from the start: the initial state `s[0] <= high` is false
 }
 
-// edge code at sample i >= 1: 0 none, 1 start, 2 end
-__device__ __forceinline__ int edge_at(const float* s, long long i, double bias, double high) {
-  const float v = s[i];
-  if (v != v) return 0;
-  const int cur = ((double)v + bias) <= high ? 1 : 0;
-  const float p = s[i - 1];
-  const int prev = (p == p) ? (((double)p + bias) <= high ? 1 : 0) : in_pulse_at(s, i - 1, bias, high);
-  if (!prev && cur) return 1;
-  if (prev && !cur) return 2;
-  return 0;
+// Each thread walks kPulseRun consecutive samples carrying the state after its
+// previous sample (the state after sample i0 - 1 comes from in_pulse_at), so a
+// sample's edge needs one load; NaN samples keep the state, sample 0 sets it
+// without an edge, and samples past the span count as NaN.
+constexpr int kPulseRun = kPulseTile / kPulseThreads;  // 16
+static_assert(kPulseRun == 16, "the run loads are four float4");
+
+__device__ __forceinline__ void load_run(const float* s, long long i0, long long n, bool vec, float (&x)[kPulseRun]) {
+  if (vec && i0 + kPulseRun <= n) {
+    const float4* p4 = reinterpret_cast<const float4*>(s + i0);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const float4 v = p4[q];
+      x[4 * q] = v.x; x[4 * q + 1] = v.y; x[4 * q + 2] = v.z; x[4 * q + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kPulseRun; k++) x[k] = i0 + k < n ? s[i0 + k] : __builtin_nanf("");
+  }
+}
+
+// Edge codes of the run (0 none, 1 start, 2 end) as two bits per sample.
+__device__ __forceinline__ unsigned run_edges(const float* s, long long i0, long long n, double bias, double high,
+                                              bool vec) {
+  float x[kPulseRun];
+  load_run(s, i0, n, vec, x);
+  int st = i0 > 0 && i0 <= n ? in_pulse_at(s, i0 - 1, bias, high) : 0;
+  unsigned codes = 0;
+#pragma unroll
+  for (int k = 0; k < kPulseRun; k++) {
+    const float v = x[k];
+    if (v == v) {
+      const int cur = ((double)v + bias) <= high ? 1 : 0;
+      if (i0 + k >= 1) {
+        const unsigned e = (!st && cur) ? 1u : ((st && !cur) ? 2u : 0u);
+        codes |= e << (2 * k);
+      }
+      st = cur;
+    }
+  }
+  return codes;
 }
 
 __global__ void __launch_bounds__(kPulseThreads) pulse_count_kernel(const float* s, long long n, double bias,
                                                                     double high, int* tile_counts) {
   __shared__ int wave_counts[kPulseThreads / 64];
-  const long long base = (long long)blockIdx.x * kPulseTile;
-  int c = 0;
-  for (int t = threadIdx.x; t < kPulseTile; t += kPulseThreads) {
-    const long long i = base + t;
-    if (i >= 1 && i < n) c += edge_at(s, i, bias, high) != 0;
-  }
+  const bool vec = (reinterpret_cast<uintptr_t>(s) & 15) == 0;
+  const long long i0 = (long long)blockIdx.x * kPulseTile + (long long)threadIdx.x * kPulseRun;
+  int c = __popc(run_edges(s, i0, n, bias, high, vec));  // a sample's two bits are never both set
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
   if ((threadIdx.x & 63) == 0) wave_counts[threadIdx.x >> 6] = c;
   __syncthreads();
@@ -57,79 +86,68 @@ __global__ void __launch_bounds__(kPulseThreads) pulse_count_kernel(const float*
   }
 }
 
-// single workgroup exclusive scan of tile counts (in place); total in *total
-__global__ void __launch_bounds__(1024) exclusive_scan_kernel(int* counts, int n, int* total) {
-  __shared__ int part[1024];
-  __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
+// Sum of counts[0, m) by the whole workgroup (every thread gets it).
+__device__ __forceinline__ int block_total(const int* counts, int m, int* red) {
+  int v = 0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) v += counts[i];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int base = 0; base < n; base += 1024) {
-    const int i = base + threadIdx.x;
-    const int v = i < n ? counts[i] : 0;
-    part[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-      const int add = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
-      __syncthreads();
-      part[threadIdx.x] += add;
-      __syncthreads();
-    }
-    const int incl = part[threadIdx.x];
-    if (i < n) counts[i] = carry + incl - v;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry += incl;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *total = carry;
+  int tot = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) tot += red[w];
+  __syncthreads();
+  return tot;
 }
 
-// ordered compaction of edges: edge_pos[k], edge_kind[k]
+// Ordered compaction of edges: edge_pos[k], edge_kind[k].  The tile's offset is the
+// sum of the earlier tiles' counts.
 __global__ void __launch_bounds__(kPulseThreads) pulse_write_kernel(const float* s, long long n, double bias,
-                                                                    double high,
-                                                                    const int* tile_offsets, long long* edge_pos,
-                                                                    unsigned char* edge_kind, int edge_cap) {
-  __shared__ int wave_counts[kPulseThreads / 64];
-  __shared__ int running;
-  const long long base = (long long)blockIdx.x * kPulseTile;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  if (threadIdx.x == 0) running = tile_offsets[blockIdx.x];
+                                                                    double high, const int* tile_counts,
+                                                                    long long* edge_pos, unsigned char* edge_kind,
+                                                                    int edge_cap) {
+  __shared__ int red[kPulseThreads / 64];
+  __shared__ int wave_tot[kPulseThreads / 64];
+  const bool vec = (reinterpret_cast<uintptr_t>(s) & 15) == 0;
+  const int tile_off = block_total(tile_counts, blockIdx.x, red);
+  const long long i0 = (long long)blockIdx.x * kPulseTile + (long long)threadIdx.x * kPulseRun;
+  const unsigned codes = run_edges(s, i0, n, bias, high, vec);
+  const int c = __popc(codes);
+  // exclusive scan of the per-thread counts
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int incl = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) wave_tot[wave] = incl;
   __syncthreads();
-  for (int t0 = 0; t0 < kPulseTile; t0 += kPulseThreads) {
-    const long long i = base + t0 + threadIdx.x;
-    const int e = (i >= 1 && i < n) ? edge_at(s, i, bias, high) : 0;
-    const unsigned long long m = __ballot(e != 0);
-    if (lane == 0) wave_counts[wave] = __popcll(m);
-    __syncthreads();
-    int before = running;
-    for (int w = 0; w < wave; w++) before += wave_counts[w];
-    if (e != 0) {
-      const int idx = before + __popcll(m & ((1ull << lane) - 1ull));
-      if (idx < edge_cap) {
-        edge_pos[idx] = i;
-        edge_kind[idx] = (unsigned char)e;
-      }
+  int idx = tile_off + incl - c;
+  for (int w = 0; w < wave; w++) idx += wave_tot[w];
+  unsigned rest = codes;
+  while (rest) {
+    const int k = (__ffs(rest) - 1) >> 1;  // sample k of the run
+    if (idx < edge_cap) {
+      edge_pos[idx] = i0 + k;
+      edge_kind[idx] = (unsigned char)((rest >> (2 * k)) & 3u);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int tot = 0;
-      for (int w = 0; w < kPulseThreads / 64; w++) tot += wave_counts[w];
-      running += tot;
-    }
-    __syncthreads();
+    idx++;
+    rest &= ~(3u << (2 * k));
   }
 }
 
 // Pair end edges with their start edge, filter by length and start != 0, compact
-// in order.  Single workgroup.  out_count receives the number of pulses found;
-// only the first `capacity` are stored.
+// in order.  Single workgroup.  counters[0] receives the number of edges,
+// counters[1] the number of pulses found; only the first `capacity` are stored.
 __global__ void __launch_bounds__(1024) pulse_pair_kernel(const long long* edge_pos, const unsigned char* edge_kind,
-                                                          const int* n_edges, int edge_cap, int min_len, int max_len,
-                                                          int* starts, int* lengths, int capacity,
-                                                          int* out_count) {
+                                                          const int* tile_counts, int tiles, int edge_cap, int min_len,
+                                                          int max_len, int* starts, int* lengths, int capacity,
+                                                          int* counters) {
   __shared__ int wave_counts[16];
   __shared__ int running;
-  const int ne = *n_edges < edge_cap ? *n_edges : edge_cap;
+  __shared__ int red[16];
+  const int total = block_total(tile_counts, tiles, red);
+  const int ne = total < edge_cap ? total : edge_cap;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   if (threadIdx.x == 0) running = 0;
@@ -164,7 +182,10 @@ __global__ void __launch_bounds__(1024) pulse_pair_kernel(const long long* edge_
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out_count = running;
+  if (threadIdx.x == 0) {
+    counters[0] = total;
+    counters[1] = running;
+  }
 }
 
 // ------------------------------------------------------------------ hsync refine
@@ -1068,11 +1089,10 @@ hipError_t vhsd_launch_pulses(const float* s, long long n, double bias, double h
   if (tiles == 0) return hipMemsetAsync(counters, 0, 2 * sizeof(int), stream);
   hipLaunchKernelGGL(vhsd::pulse_count_kernel, dim3(tiles), dim3(vhsd::kPulseThreads), 0, stream, s, n, bias, high,
                      tile_counts);
-  hipLaunchKernelGGL(vhsd::exclusive_scan_kernel, dim3(1), dim3(1024), 0, stream, tile_counts, tiles, counters);
   hipLaunchKernelGGL(vhsd::pulse_write_kernel, dim3(tiles), dim3(vhsd::kPulseThreads), 0, stream, s, n, bias, high,
                      tile_counts, edge_pos, edge_kind, edge_cap);
-  hipLaunchKernelGGL(vhsd::pulse_pair_kernel, dim3(1), dim3(1024), 0, stream, edge_pos, edge_kind, counters,
-                     edge_cap, min_len, max_len, starts, lengths, capacity, counters + 1);
+  hipLaunchKernelGGL(vhsd::pulse_pair_kernel, dim3(1), dim3(1024), 0, stream, edge_pos, edge_kind, tile_counts, tiles,
+                     edge_cap, min_len, max_len, starts, lengths, capacity, counters);
   return hipGetLastError();
 }
 

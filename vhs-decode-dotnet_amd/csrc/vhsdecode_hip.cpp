@@ -1252,7 +1252,6 @@ int32_t vhsdecode_hip_find_pulses(vhsdecode_hip_demod_context* c, const float* s
     VHSD_HIP(c->edge_kind.ensure(edge_cap));
     VHSD_HIP(c->p_starts.ensure(std::max(capacity, 1)));
     VHSD_HIP(c->p_lens.ensure(std::max(capacity, 1)));
-    VHSD_HIP(hipMemsetAsync(c->counters.p, 0, 2 * sizeof(int), st));
     ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_PULSES, length);
     VHSD_HIP(vhsd_launch_pulses(sync_ref, length, c->bias, threshold, min_length, max_length, c->tile_counts.p, tiles,
                                 c->edge_pos.p, c->edge_kind.p, (int)edge_cap, c->counters.p, c->p_starts.p,
@@ -1656,8 +1655,7 @@ int32_t vhsdecode_hip_decode_span_fields_device(vhsdecode_hip_demod_context* c, 
     const double line_decay = smooth ? std::pow(1.0 - alpha, (double)p->output_line_length) : 0.0;
     {
       ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_PULSES, span_length);
-      VHSD_HIP(hipMemsetAsync(c->counters.p, 0, 2 * sizeof(int), st));
-      VHSD_HIP(vhsd_launch_pulses(sync_ref, span_length, c->bias, p->threshold, lz->min_pulse_length,
+        VHSD_HIP(vhsd_launch_pulses(sync_ref, span_length, c->bias, p->threshold, lz->min_pulse_length,
                                   lz->max_pulse_length, c->tile_counts.p, tiles, c->edge_pos.p, c->edge_kind.p,
                                   (int)edge_cap, c->counters.p, c->p_starts.p, c->p_lens.p, capacity, st));
     }
