@@ -9,11 +9,11 @@
 //   map rows   = TbcDropoutMapper.MapTapeRfToTbc (Decode/TbcDropoutMapping.cs:288-366)
 //
 // Four launches:
-//  * dropout_mean_kernel, 128 workgroups per field: workgroup g stages the window
-//    values under node g of depth 7 of the pairwise tree in LDS (16-byte loads); its
-//    threads are the depth-15 slots below, and the leftmost slot under each leaf
-//    sums the leaf in the reference's order (eight accumulators); the workgroup
-//    combines its slots up to depth 7.  dropout_threshold_kernel (one workgroup per
+//  * dropout_mean_kernel, 128 workgroups per field: workgroup g covers node g of
+//    depth 7 of the pairwise tree; its threads are the depth-15 slots below, and the
+//    leftmost slot under each leaf sums the leaf in the reference's order (eight
+//    lanes, one accumulator each, loads issued up front); the workgroup combines its
+//    slots up to depth 7 and records the smallest value under the node.  dropout_threshold_kernel (one workgroup per
 //    field) completes the tree to the root and writes the threshold and the
 //    detection range.
 //  * dropout_masks_kernel, 32 workgroups per field: every sample of the detection
@@ -132,7 +132,6 @@ __device__ float pw_leaf(const float* v, int len) {
 
 __global__ void __launch_bounds__(kSlotThreads) dropout_mean_kernel(DropoutArgs a) {
 #pragma clang fp contract(off)
-  extern __shared__ float stage[];
   __shared__ float v[kSlotThreads];
   __shared__ float wmin[kSlotThreads / 64];
   const int f = blockIdx.y;
@@ -145,7 +144,7 @@ __global__ void __launch_bounds__(kSlotThreads) dropout_mean_kernel(DropoutArgs 
   bool gown;
   pw_descend(n, kGroupDepth, (unsigned)g, goff, glen, gown);
   // groups under a leaf another group owns contribute nothing (their value is never read)
-  if (!(gown && n > 0 && glen <= a.stage_cap)) {
+  if (!(gown && n > 0)) {
     if (t == 0) {
       a.partials[(size_t)f * kGroups + g] = 0.0f;
       a.group_min[(size_t)f * kGroups + g] = __builtin_inff();
@@ -153,47 +152,16 @@ __global__ void __launch_bounds__(kSlotThreads) dropout_mean_kernel(DropoutArgs 
     return;
   }
   const float* src = a.envelope + w0 + goff;
-  if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
-    // 16-byte loads, up to eight in flight per thread
-    const int n4 = glen >> 2;
-    const float4* s4 = reinterpret_cast<const float4*>(src);
-    float4* d4 = reinterpret_cast<float4*>(stage);
-    int i = t;
-    for (; i + 7 * kSlotThreads < n4; i += 8 * kSlotThreads) {
-      float4 x[8];
-#pragma unroll
-      for (int k = 0; k < 8; k++) x[k] = s4[i + k * kSlotThreads];
-#pragma unroll
-      for (int k = 0; k < 8; k++) d4[i + k * kSlotThreads] = x[k];
-    }
-    for (; i < n4; i += kSlotThreads) d4[i] = s4[i];
-    for (int j = (n4 << 2) + t; j < glen; j += kSlotThreads) stage[j] = src[j];
-  } else {
-    for (int j = t; j < glen; j += kSlotThreads) stage[j] = src[j];
-  }
-  __syncthreads();
-  {
-    // the smallest staged value (NaN never compares low, and fminf skips it)
-    float m = __builtin_inff();
-    for (int j = t; j < glen; j += kSlotThreads) m = fminf(m, stage[j]);
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) m = fminf(m, __shfl_xor(m, d));
-    if ((t & 63) == 0) wmin[t >> 6] = m;
-    __syncthreads();
-    if (t == 0) {
-      float gm = wmin[0];
-      for (int w = 1; w < kSlotThreads / 64; w++) gm = fminf(gm, wmin[w]);
-      a.group_min[(size_t)f * kGroups + g] = gm;
-    }
-  }
-  float gval;
+  float m = __builtin_inff();  // smallest value this thread loaded (fminf skips NaN)
+  float gval = 0.0f;
   if (glen <= 128) {
     // the group's node is itself a leaf (at depth 7 or above)
-    if (t != 0) return;
-    gval = pw_leaf(stage, glen);
+    for (int j = t; j < glen; j += kSlotThreads) m = fminf(m, src[j]);
+    if (t == 0) gval = pw_leaf(src, glen);
   } else {
-    // leaves: eight lanes per leaf, lane k running accumulator k (elements k, k+8, ...);
-    // the three xor-shuffles combine them as ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7))
+    // leaves: eight lanes per leaf, lane k running accumulator k (elements k, k+8, ...)
+    // with all of its loads issued first; the three xor-shuffles combine the
+    // accumulators as ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7))
     const int oct = t >> 3, k8 = t & 7;
     for (int q = 0; q < 8; q++) {
       const int slot = oct * 8 + q;
@@ -202,10 +170,22 @@ __global__ void __launch_bounds__(kSlotThreads) dropout_mean_kernel(DropoutArgs 
       pw_descend(glen, kSlotDepth - kGroupDepth, (unsigned)slot, off, len, own);
       const bool vec = own && len >= 8 && len <= 128;
       const int vend = len - len % 8;
-      float r = 0.0f;
+      float x[16];
+#pragma unroll
+      for (int it = 0; it < 16; it++) {
+        const int i = k8 + 8 * it;
+        x[it] = (vec && i < vend) ? src[off + i] : 0.0f;
+      }
+      float r = x[0];
+#pragma unroll
+      for (int it = 1; it < 16; it++) {
+        if (k8 + 8 * it < vend) r += x[it];
+      }
       if (vec) {
-        r = stage[off + k8];
-        for (int i = off + 8 + k8; i < off + vend; i += 8) r += stage[i];
+#pragma unroll
+        for (int it = 0; it < 16; it++) {
+          if (k8 + 8 * it < vend) m = fminf(m, x[it]);
+        }
       }
       r = r + __shfl_xor(r, 1);
       r = r + __shfl_xor(r, 2);
@@ -215,10 +195,18 @@ __global__ void __launch_bounds__(kSlotThreads) dropout_mean_kernel(DropoutArgs 
         if (own && len <= 128) {
           if (len < 8) {
             sum = -0.0f;
-            for (int i = 0; i < len; i++) sum += stage[off + i];
+            for (int i = 0; i < len; i++) {
+              const float e = src[off + i];
+              sum += e;
+              m = fminf(m, e);
+            }
           } else {
             sum = r;
-            for (int i = off + vend; i < off + len; i++) sum += stage[i];
+            for (int i = vend; i < len; i++) {
+              const float e = src[off + i];
+              sum += e;
+              m = fminf(m, e);
+            }
           }
         }
         v[slot] = sum;
@@ -239,10 +227,18 @@ __global__ void __launch_bounds__(kSlotThreads) dropout_mean_kernel(DropoutArgs 
       }
       __syncthreads();
     }
-    if (t != 0) return;
     gval = v[0];
   }
-  a.partials[(size_t)f * kGroups + g] = gval;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) m = fminf(m, __shfl_xor(m, d));
+  if ((t & 63) == 0) wmin[t >> 6] = m;
+  __syncthreads();
+  if (t == 0) {
+    float gm = wmin[0];
+    for (int w = 1; w < kSlotThreads / 64; w++) gm = fminf(gm, wmin[w]);
+    a.group_min[(size_t)f * kGroups + g] = gm;
+    a.partials[(size_t)f * kGroups + g] = gval;
+  }
 }
 
 // One workgroup per field: the pairwise tree from depth 7 to the root, then the
@@ -512,27 +508,12 @@ __global__ void __launch_bounds__(kWalkThreads) dropout_walk_kernel(DropoutArgs 
 }  // namespace
 }  // namespace vhsd
 
-int vhsd_dropout_stage_floats(long long max_window) {
-  // the longest node at depth 7 of a window of max_window values (right children are
-  // the longer ones)
-  long long len = max_window;
-  for (int l = 0; l < vhsd::kGroupDepth && len > 128; l++) len -= (len / 2) - (len / 2) % 8;
-  return (int)len;
-}
-
 hipError_t vhsd_launch_dropouts(const vhsd::DropoutArgs* a, int max_fields, hipStream_t stream) {
   if (max_fields <= 0) return hipSuccess;
-  if (a->nloc < 2 || a->nloc > vhsd::kDropMaxLocations || a->capacity <= 0 || a->words_cap <= 0 ||
-      a->stage_cap <= 0 || a->stage_cap > 36864)
+  if (a->nloc < 2 || a->nloc > vhsd::kDropMaxLocations || a->capacity <= 0 || a->words_cap <= 0)
     return hipErrorInvalidValue;
-  const size_t stage_bytes = (size_t)a->stage_cap * sizeof(float);
-  if (stage_bytes > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void*)vhsd::dropout_mean_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)stage_bytes);
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL(vhsd::dropout_mean_kernel, dim3(vhsd::kGroups, max_fields), dim3(vhsd::kSlotThreads),
-                     stage_bytes, stream, *a);
+  hipLaunchKernelGGL(vhsd::dropout_mean_kernel, dim3(vhsd::kGroups, max_fields), dim3(vhsd::kSlotThreads), 0,
+                     stream, *a);
   hipLaunchKernelGGL(vhsd::dropout_threshold_kernel, dim3(max_fields), dim3(vhsd::kGroups), 0, stream, *a);
   hipLaunchKernelGGL(vhsd::dropout_masks_kernel, dim3(vhsd::kMaskGroups, max_fields), dim3(vhsd::kMaskThreads), 0,
                      stream, *a);

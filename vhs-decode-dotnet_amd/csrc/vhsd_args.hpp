@@ -211,7 +211,6 @@ struct DropoutArgs {
   // scratch
   float* partials;                 // [fields][128]
   float* group_min;                // [fields][128] smallest envelope value under each group
-  int stage_cap;                   // LDS floats for one depth-7 node of the window tree
   DropoutField* info;              // [fields]
   unsigned long long* masks;       // [fields][2][words_cap]
   unsigned* summaries;             // [fields][2][kDropSummaryWords]
@@ -233,7 +232,6 @@ size_t vhsd_block_lds_bytes();
 hipError_t vhsd_launch_block_demod(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_chroma(const vhsd::ChromaArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_repair(const vhsd::RepairArgs* a, int nblocks, hipStream_t stream);
-int vhsd_dropout_stage_floats(long long max_window);
 hipError_t vhsd_launch_dropouts(const vhsd::DropoutArgs* a, int max_fields, hipStream_t stream);
 hipError_t vhsd_launch_locate(const float* sync_ref, long long span_len, const int* starts, const int* lens,
                               const int* pulse_count, int capacity, const vhsd::LocateParams* lp, int max_fields,

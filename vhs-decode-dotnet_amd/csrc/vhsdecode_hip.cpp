@@ -1774,7 +1774,6 @@ int32_t vhsdecode_hip_detect_dropouts_device(vhsdecode_hip_demod_context* c, con
     a.stride = p->block_stride;
     a.partials = c->d_part.p;
     a.group_min = c->d_part.p + (size_t)max_fields * 128;
-    a.stage_cap = vhsd_dropout_stage_floats(window_max);
     a.info = c->d_info.p;
     a.masks = c->d_mask.p;
     a.summaries = c->d_sum.p;
