@@ -50,22 +50,27 @@ def build_workload(fields: int):
 
 def cpu_baseline(fmt, tape, fl, nb, lut, seconds: float, threads: int):
     """FP64 oracle on the same tape: block demod (threads over blocks), pulses,
-    line locations, hsync refinement and TBC per field; repeated for ~`seconds`."""
+    line locations, then per field hsync refinement, TBC and the envelope dropout
+    detection; repeated for ~`seconds`, then one pass on one thread."""
     from oracle import oracle as O
     from vhsdecode_hip import field
     g = field.sync_geometry(fmt)
+    dp = field.dropout_params(fmt)
+    first_lines, second_lines = dp.first_field_lines, dp.second_field_lines
     rf = tape.rf
 
     def one_block(b):
         r = O.demod_block(rf[b * STRIDE:b * STRIDE + N].astype(np.float64), fl)
-        return b, r.video[CUT:CUT + STRIDE], r.video_lowpass[CUT:CUT + STRIDE]
+        return b, r.video[CUT:CUT + STRIDE], r.video_lowpass[CUT:CUT + STRIDE], r.envelope[CUT:CUT + STRIDE]
 
     def one_pass(pool):
         video = np.empty(nb * STRIDE)
         vlp = np.empty(nb * STRIDE)
-        for b, v, l in pool.map(one_block, range(nb)):
+        env = np.empty(nb * STRIDE, dtype=np.float32)
+        for b, v, l, e in pool.map(one_block, range(nb)):
             video[b * STRIDE:(b + 1) * STRIDE] = v
             vlp[b * STRIDE:(b + 1) * STRIDE] = l
+            env[b * STRIDE:(b + 1) * STRIDE] = e
         st, ln = O.find_pulses(vlp, g.threshold, g.min_pulse, g.max_pulse)
         fields = field.line_locations_from_pulses(st, ln, g, vlp.size)
 
@@ -73,8 +78,11 @@ def cpu_baseline(fmt, tape, fl, nb, lut, seconds: float, threads: int):
             ref, _ = O.refine_hsync(vlp, f.locations, f.filled.astype(np.uint8), g.threshold, g.fs_mhz,
                                     g.hsync / g.fs_mhz, g.conv[0], g.conv[1], g.is_pal, True)
             og = field.field_origin(f.line0, g)
-            return O.tbc_resample(video[og:], ref - og, 0, g.field_lines, g.out_len, lut, nominal=g.nominal,
-                                  smoothing=g.smoothing, conv=g.conv)[1]
+            out = O.tbc_resample(video[og:], ref - og, 0, g.field_lines, g.out_len, lut, nominal=g.nominal,
+                                 smoothing=g.smoothing, conv=g.conv)[1]
+            win = O.field_read_window(f.line0, dp.lead_lines, dp.line_length, dp.read_samples, env.size)
+            O.tape_dropouts(env, ref, first_lines if f.first_field else second_lines, win, out_len=g.out_len)
+            return out
 
         return len(list(pool.map(one_field, fields)))
 
@@ -88,7 +96,11 @@ def cpu_baseline(fmt, tape, fl, nb, lut, seconds: float, threads: int):
             if el >= seconds:
                 break
     samples = ((nb - 1) * STRIDE + N) * reps
-    return samples / el / 1e6, reps, el
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        t1 = time.perf_counter()
+        one_pass(pool)
+        el1 = time.perf_counter() - t1
+    return samples / el / 1e6, reps, el, ((nb - 1) * STRIDE + N) / el1 / 1e6
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_block_kernels_ntsc_vhs_10fields.md")
@@ -541,11 +553,13 @@ def main():
               f"burst capacity {sv['burst_capacity_msps']} MSPS", file=sys.stderr)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        msps, reps, el = cpu_baseline(fmt, tape, fl, nb, lut, args.cpu_seconds, threads)
+        msps, reps, el, msps1 = cpu_baseline(fmt, tape, fl, nb, lut, args.cpu_seconds, threads)
         result["cpu_baseline"] = {"value": round(msps, 3), "unit": "MSPS", "cores": threads, "kind": "port",
                                   "sample": f"the full {args.fields}-field tape x {reps} passes "
-                                            f"({el:.1f} s): FP64 oracle block demod and per-field hsync refine + TBC "
-                                            f"on {threads} threads; pulses and line locations serial"}
+                                            f"({el:.1f} s): FP64 oracle block demod and per-field hsync refine, TBC "
+                                            f"and dropout detection on {threads} threads; pulses and line locations "
+                                            f"serial",
+                                  "single_thread_value": round(msps1, 3)}
     print(json.dumps(result), flush=True)
 
 
