@@ -111,17 +111,20 @@ __device__ __forceinline__ void irfft_pre(c32* v, c32 vm, BlockLds& s) {
 
 // Stage B+: analytic signal from the filtered half spectrum F (strided, F[M] in fm)
 // and the FM discriminator.  Writes the demod to stash[k] = float2(sample 2k,
-// sample 2k+1) for k = j + 1024 m and leaves the analytic real parts (the envelope
-// source) at stash[M + k] in the same form.  State that must outlive a transform
-// goes through the block's stash (written and read back by this workgroup only)
-// instead of registers.  Returns whether this thread saw a demod sample in
+// sample 2k+1) for k = j + 1024 m.  With kEnvRaw the envelope source
+// raw[(n + 4) mod N] = |float(Re analytic[n])| goes straight to LDS at `raw` (inside
+// the transform buffer, free after the last transform).  State that must outlive a
+// transform goes through the block's stash (written and read back by this workgroup
+// only) instead of registers.  Returns whether this thread saw a demod sample in
 // [20, N-20) above diff_max.
+template <bool kEnvRaw>
 __device__ __forceinline__ bool analytic_demod(c32* F, c32 fm, BlockLds& s, float fs, double diff_max,
-                                               float2* __restrict__ stash) {
+                                               float2* __restrict__ stash, float* raw) {
   const int j = threadIdx.x;
+  const int lane = j & 63, wave = j >> 6;
   const float inv_n = 1.0f / 32768.0f;
   float2* st_go = stash;            // odd-pass input spectrum
-  float2* st_th = stash + kFftM;    // (angle, real part) of the even samples, then real parts
+  float2* st_th = stash + kFftM;    // (angle, real part) of the even samples
   // odd output samples' input, parked until the even pass is done
 #pragma unroll
   for (int m = 0; m < 16; m++) {
@@ -147,8 +150,12 @@ __device__ __forceinline__ bool analytic_demod(c32* F, c32 fm, BlockLds& s, floa
     F[m] = c32{g.x, g.y};
   }
   fft16k_inverse(F, s.buf, s.oct);
-  float* th_o = reinterpret_cast<float*>(s.buf);
+  // odd sample 2k+1 against even sample 2k in the same thread; even sample 2k against
+  // odd sample 2k-1, which is the previous lane's (shuffle) or, across waves, the
+  // last lane of the previous wave (through `edge`, 16 angles per wave)
+  float* edge = s.scratch;
   float theta_e[16];
+  float theta_o[16];
   float dem_o[16];
 #pragma unroll
   for (int m = 0; m < 16; m++) {
@@ -156,20 +163,28 @@ __device__ __forceinline__ bool analytic_demod(c32* F, c32 fm, BlockLds& s, floa
     theta_e[m] = te.x;
     const c32 a = scale(F[m], inv_n);
     const float t = vhs_atan2(a.y, a.x);
+    theta_o[m] = t;
     dem_o[m] = vhs_freq_diff(t, te.x, fs);
-    th_o[j + m * kFftThreads] = t;
-    row_at(st_th, m) = make_float2(te.y, a.x);
+    if (lane == 63) edge[wave * 16 + m] = t;
+    if constexpr (kEnvRaw) {
+      const int n0 = 2 * (j + m * kFftThreads);
+      raw[(n0 + 4) & 32767] = fabsf(te.y);
+      raw[(n0 + 5) & 32767] = fabsf(a.x);
+    }
   }
   __syncthreads();
   bool spike = false;
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     // previous odd sample of even sample 2*(j + 1024 m) is odd index (j-1) + 1024 m
+    const float up = __shfl_up(theta_o[m], 1u);
     float prev;
-    if (j > 0)
-      prev = th_o[j - 1 + m * kFftThreads];
+    if (lane > 0)
+      prev = up;
+    else if (wave > 0)
+      prev = edge[(wave - 1) * 16 + m];
     else
-      prev = (m > 0) ? th_o[kFftThreads - 1 + (m - 1) * kFftThreads] : 0.0f;
+      prev = (m > 0) ? edge[15 * 16 + (m - 1)] : 0.0f;
     const float de = (j == 0 && m == 0) ? 0.0f : vhs_freq_diff(theta_e[m], prev, fs);
     row_at(st_go, m) = make_float2(de, dem_o[m]);
     if (diff_max > 0.0) {
@@ -286,20 +301,10 @@ __device__ __forceinline__ void vhs_block_demod_body(const DemodArgs& a, const i
   __syncthreads();
   // ---- stage B: analytic signal + discriminator
   float2* stash = a.stash + (long long)blk * kStashEntries;
-  // demod to the stash; spike check over demod[20 .. N-20)
-  const bool spike = analytic_demod(v, fm_b, s, a.fs, a.diff_max, stash);
-  // ---- envelope: raw[(n + 4) mod N] = |float(re[n])|, FP32 sosfiltfilt
+  // demod to the stash; spike check over demod[20 .. N-20); the envelope source
+  // raw[(n + 4) mod N] = |float(re[n])| to LDS for the FP32 sosfiltfilt
   float* raw = reinterpret_cast<float*>(s.buf) + kSosMargin + a.env.pad;
-  const float2* st_re = stash + kFftM;
-#pragma unroll
-  for (int m = 0; m < 16; m++) {
-    const int k = j + m * kFftThreads;
-    const int n0 = 2 * k;
-    const float2 r = row_at(st_re, m);
-    raw[(n0 + 4) & 32767] = fabsf(r.x);
-    raw[(n0 + 5) & 32767] = fabsf(r.y);
-  }
-  __syncthreads();
+  const bool spike = analytic_demod<true>(v, fm_b, s, a.fs, a.diff_max, stash, raw);
   // the VHS envelope filter is first order (DecodeFilterSet.BuildVhsEnvelopeSos): one section
   const bool weak = sos_filtfilt_block_t<1>(a.env, reinterpret_cast<float*>(s.buf) + kSosMargin, 32768, s.scratch,
                                             a.out_env + out_base, a.out_begin,
@@ -460,7 +465,7 @@ __device__ __forceinline__ void vhs_block_demod_boost_body(const DemodArgs& a, c
     fb = c32{f0.x, f0.y};
     if (j == 0) atomicOr(&a.flags[blk], 1u);
   }
-  const bool spike = analytic_demod(v, fb, s, a.fs, a.diff_max, stash);
+  const bool spike = analytic_demod<false>(v, fb, s, a.fs, a.diff_max, stash, nullptr);
   const unsigned long long any_spike = __ballot(spike);
   if ((j & 63) == 0 && any_spike) atomicOr(&a.flags[blk], 2u);
 }
