@@ -235,11 +235,20 @@ __device__ __forceinline__ void video_stage(BlockLds& s, const DemodArgs& a, lon
     const int shift = pass == 0 ? 0 : pass == 1 ? a.vlp_offset : (ref1 ? a.ref_shift[1] : a.ref_shift[0]);
     const float corr = pass == 0 ? a.video_corr : pass == 1 ? a.vlp_corr : (ref1 ? a.ref_corr[1] : a.ref_corr[0]);
     c32 v[16];
+    if (pass == 0) {
+      // the first pass takes the spectrum from registers; later passes read it back
 #pragma unroll
-    for (int m = 0; m < 16; m++) {
-      const float2 f = row_at(filt, m);
-      const float2 d = row_at(stash, m);
-      v[m] = mul(c32{d.x, d.y}, c32{f.x, f.y});
+      for (int m = 0; m < 16; m++) {
+        const float2 f = row_at(filt, m);
+        v[m] = mul(D[m], c32{f.x, f.y});
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        const float2 f = row_at(filt, m);
+        const float2 d = row_at(stash, m);
+        v[m] = mul(c32{d.x, d.y}, c32{f.x, f.y});
+      }
     }
     const float2 fM = filt[kFftM];
     const c32 vm = mul(dm, c32{fM.x, fM.y});
