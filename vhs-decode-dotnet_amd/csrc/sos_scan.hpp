@@ -12,14 +12,19 @@
 // first and only the last non-empty chunk can be partial.  A partial chunk runs
 // over LDS margin cells past the end of the sequence in its processing order, so
 // no sample loop carries a per-sample condition and every LDS access is one base
-// address plus a compile-time offset.  Each chunk is run once from a zero state
-// (thread 0 from the true initial state); the chunk-entry states come from a scan
-// over the affine state maps (transition matrix (A^C)^(2^d) per level, precomputed
-// in FP64 on the host): a shuffle scan inside each wave, a scan of the 16 wave
-// totals, and a second shuffle scan with each wave's entry state injected.  The
-// zero-input response of the chunk to its entry state (one row of C output
-// weights per state component, also precomputed) is then added to the chunk's
-// outputs.  Relative to the sequential reference the outputs differ by rounding
+// address plus a compile-time offset.
+//
+// A cascade processed sample by sample gives the same values as its sections
+// applied one after another over the whole sequence (each section's step reads
+// only its own state and its input), so each direction runs the sections in turn
+// on the chunk held in registers.  Per section, each chunk runs once from a zero
+// state (thread 0 from the section's true initial state); the chunk-entry states
+// come from a scan over the section's affine state maps (2 x 2 matrices
+// (A^C)^(2^d), precomputed in FP64 on the host and read as uniform scalars): a
+// shuffle scan inside each wave, a scan of the 16 wave totals, and a second
+// shuffle scan with each wave's entry state injected.  The zero-input response of
+// the section to its entry state (a precomputed C x 2 table) is then added to the
+// chunk.  Relative to the sequential reference the outputs differ by rounding
 // (relative error ~1e-6).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -33,8 +38,8 @@ constexpr int kScanThreads = 1024;
 constexpr int kScanLevels = 10;  // log2(kScanThreads)
 constexpr int kChunk = 33;       // supports blocks of up to 1024*33 - 2*pad samples
 // LDS layout: the sequence starts kSosMargin floats into the buffer, which must hold
-// kSosMargin + L + kChunk + 32 * 2S floats (partial chunks overrun by less than
-// kChunk on either side; the scan exchange follows).
+// kSosMargin + L + kChunk + 64 floats (partial chunks overrun by less than kChunk on
+// either side; the scan exchange follows).
 constexpr int kSosMargin = 64;
 
 struct SosScanArgs {
@@ -43,157 +48,113 @@ struct SosScanArgs {
   int chunk;                             // samples per thread chunk, <= kChunk
   float coef[kMaxSections][5];           // b0, b1, b2, a1, a2 (FP32 cast of the FP64 design)
   float zi[2 * kMaxSections];            // steady-state initial conditions (FP32)
-  const float* trans;                    // [kScanLevels][2S][2S] row-major, A^(chunk * 2^d), then
-                                         // [kChunk][2S]: output t of a chunk per unit entry state
+  const float* trans;                    // per section: [kScanLevels][2][2] row-major, A^(chunk * 2^d);
+                                         // then per section [kChunk][2]: output t of a chunk per
+                                         // unit entry state
 };
 
-// One sample through the cascade (direct form II transposed), with fused
-// multiply-adds: the chunked evaluation matches the sequential FP32 reference to
-// rounding, not bit for bit, so the per-sample operation order is free.
-template <int S>
-__device__ __forceinline__ void sos_step(const float (&c)[kMaxSections][5], float* z, float& v) {
-#pragma unroll
-  for (int k = 0; k < S; k++) {
-    const float f = fmaf(c[k][0], v, z[2 * k]);
-    z[2 * k] = fmaf(c[k][1], v, fmaf(-c[k][3], f, z[2 * k + 1]));
-    z[2 * k + 1] = fmaf(c[k][2], v, -(c[k][4] * f));
-    v = f;
-  }
-}
-
-// y += M x for a state transition of the cascade: section k's state depends only on
-// sections 0..k, so every power of the one-step matrix is block lower triangular
-// (2 x 2 blocks) and the upper blocks are skipped.
-template <int S>
-__device__ __forceinline__ void add_matvec(float* y, const float* __restrict__ m, const float* x) {
-  constexpr int W = 2 * S;
-#pragma unroll
-  for (int r = 0; r < W; r++) {
-    float acc = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 2 * (r / 2 + 1); c++) acc = fmaf(m[r * W + c], x[c], acc);
-    y[r] += acc;
-  }
+// y += M x for a 2 x 2 section transition.
+__device__ __forceinline__ void add_matvec2(float* y, const float* __restrict__ m, const float* x) {
+  const float y0 = fmaf(m[0], x[0], m[1] * x[1]);
+  const float y1 = fmaf(m[2], x[0], m[3] * x[1]);
+  y[0] += y0;
+  y[1] += y1;
 }
 
 // Inclusive scan of affine chunk maps across the lanes of one wave with a zero state
 // entering lane 0: x := x + M_d x[lane - 2^d] for d = 0..5.
-template <int S>
-__device__ __forceinline__ void wave_scan(float* x, const float* __restrict__ trans, int lane) {
-  constexpr int W = 2 * S;
+__device__ __forceinline__ void wave_scan2(float* x, const float* __restrict__ trans, int lane) {
 #pragma unroll
   for (int d = 0; d < 6; d++) {
-    float pv[W];
-#pragma unroll
-    for (int c = 0; c < W; c++) pv[c] = __shfl_up(x[c], 1u << d);
-    if (lane >= (1 << d)) add_matvec<S>(x, trans + d * W * W, pv);
+    float pv[2];
+    pv[0] = __shfl_up(x[0], 1u << d);
+    pv[1] = __shfl_up(x[1], 1u << d);
+    if (lane >= (1 << d)) add_matvec2(x, trans + d * 4, pv);
   }
 }
 
-// Exclusive scan of chunk end-states in thread order: on entry `st` is the end
-// state of this thread's chunk when started from zero (thread 0: from the true
-// initial state); on return it is the state entering the chunk.  `ex` is LDS space
-// of at least 32 * 2S floats; `trans` the level matrices (global memory).
-template <int S>
-__device__ __forceinline__ void scan_states(float* st, float* ex, const float* __restrict__ trans) {
-  constexpr int W = 2 * S;
+// Exclusive scan of one section's chunk end-states in thread order: on entry `st` is
+// the end state of this thread's chunk when started from zero (thread 0: from the
+// true initial state); on return it is the state entering the chunk.  `ex` is LDS
+// space of at least 64 floats; `trans` the section's level matrices (global memory).
+__device__ __forceinline__ void scan_states2(float* st, float* ex, const float* __restrict__ trans) {
   constexpr int kWaves = kScanThreads / 64;
   static_assert(kWaves == 16, "the wave totals are scanned by 16 lanes in four levels");
   const int q = threadIdx.x, lane = q & 63, wave = q >> 6;
-  float x[W];
-#pragma unroll
-  for (int c = 0; c < W; c++) x[c] = st[c];
-  wave_scan<S>(x, trans, lane);
+  float x[2] = {st[0], st[1]};
+  wave_scan2(x, trans, lane);
   if (lane == 63) {
-#pragma unroll
-    for (int c = 0; c < W; c++) ex[wave * W + c] = x[c];
+    ex[wave * 2] = x[0];
+    ex[wave * 2 + 1] = x[1];
   }
   __syncthreads();
   if (wave == 0) {
     // wave totals: one wave step spans 64 chunks, levels 6..9
-    float tot[W];
-#pragma unroll
-    for (int c = 0; c < W; c++) tot[c] = lane < kWaves ? ex[lane * W + c] : 0.0f;
+    float tot[2];
+    tot[0] = lane < kWaves ? ex[lane * 2] : 0.0f;
+    tot[1] = lane < kWaves ? ex[lane * 2 + 1] : 0.0f;
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-      float pv[W];
-#pragma unroll
-      for (int c = 0; c < W; c++) pv[c] = __shfl_up(tot[c], 1u << e);
-      if (lane >= (1 << e)) add_matvec<S>(tot, trans + (6 + e) * W * W, pv);
+      float pv[2];
+      pv[0] = __shfl_up(tot[0], 1u << e);
+      pv[1] = __shfl_up(tot[1], 1u << e);
+      if (lane >= (1 << e)) add_matvec2(tot, trans + (6 + e) * 4, pv);
     }
-#pragma unroll
-    for (int c = 0; c < W; c++) {
-      const float pv = __shfl_up(tot[c], 1u);
-      if (lane < kWaves) ex[kWaves * W + lane * W + c] = lane == 0 ? 0.0f : pv;
+    const float p0 = __shfl_up(tot[0], 1u), p1 = __shfl_up(tot[1], 1u);
+    if (lane < kWaves) {
+      ex[2 * kWaves + lane * 2] = lane == 0 ? 0.0f : p0;
+      ex[2 * kWaves + lane * 2 + 1] = lane == 0 ? 0.0f : p1;
     }
   }
   __syncthreads();
   // the wave's entry state, injected into its lane 0 (end state = zero-start end + A^C entry)
-  float e_in[W];
-#pragma unroll
-  for (int c = 0; c < W; c++) e_in[c] = ex[kWaves * W + wave * W + c];
-#pragma unroll
-  for (int c = 0; c < W; c++) x[c] = st[c];
-  if (lane == 0) add_matvec<S>(x, trans, e_in);
-  wave_scan<S>(x, trans, lane);
-#pragma unroll
-  for (int c = 0; c < W; c++) {
-    const float pv = __shfl_up(x[c], 1u);
-    st[c] = lane == 0 ? e_in[c] : pv;
-  }
+  const float e_in[2] = {ex[2 * kWaves + wave * 2], ex[2 * kWaves + wave * 2 + 1]};
+  x[0] = st[0];
+  x[1] = st[1];
+  if (lane == 0) add_matvec2(x, trans, e_in);
+  wave_scan2(x, trans, lane);
+  const float p0 = __shfl_up(x[0], 1u), p1 = __shfl_up(x[1], 1u);
+  st[0] = lane == 0 ? e_in[0] : p0;
+  st[1] = lane == 0 ? e_in[1] : p1;
   __syncthreads();
 }
 
-// One direction over this thread's chunk val[0..kChunk) in processing order; the
-// chunk of thread 0 starts from zi * first_value, the others from zero and then
-// receive the response to their entry state.
-// Long cascades park the chunk in LDS (`park`, this thread's cells, stride `step`;
-// NULL for a thread whose chunk lies past the sequence) during the scan to free
-// registers.
+// One direction over this thread's chunk val[0..kChunk) in processing order, one
+// section after another (direct form II transposed, fused multiply-adds).  The chunk
+// of thread 0 starts each section from zi * first_value, the others from zero and
+// then receive the section's response to their entry state.
 template <int S>
 __device__ __forceinline__ void run_direction(const SosScanArgs& p, float (&val)[kChunk], float first_value,
-                                              float* ex, float* park, int step) {
-  constexpr int W = 2 * S;
+                                              float* ex) {
   const bool lead = threadIdx.x == 0;
-  float z[W];
-#pragma unroll
-  for (int c = 0; c < W; c++) z[c] = lead ? p.zi[c] * first_value : 0.0f;
-#pragma unroll
-  for (int t = 0; t < kChunk; t++) {
-    float v = val[t];
-    sos_step<S>(p.coef, z, v);
-    val[t] = v;
-  }
-  if constexpr (S > 4) {
-    if (park) {
-#pragma unroll
-      for (int t = 0; t < kChunk; t++) park[t * step] = val[t];
-    }
-  }
-  // the level matrices are read from global memory with uniform addresses (scalar
-  // loads into SGPRs, which the FMAs take directly)
-  scan_states<S>(z, ex, p.trans);
-  if constexpr (S > 4) {
-    if (park) {
-#pragma unroll
-      for (int t = 0; t < kChunk; t++) val[t] = park[t * step];
-    }
-  }
-  if (!lead) {
-    const float* __restrict__ resp = p.trans + kScanLevels * W * W;
+#pragma unroll 1
+  for (int k = 0; k < S; k++) {
+    const float b0 = p.coef[k][0], b1 = p.coef[k][1], b2 = p.coef[k][2], a1 = p.coef[k][3], a2 = p.coef[k][4];
+    float z[2];
+    z[0] = lead ? p.zi[2 * k] * first_value : 0.0f;
+    z[1] = lead ? p.zi[2 * k + 1] * first_value : 0.0f;
 #pragma unroll
     for (int t = 0; t < kChunk; t++) {
-      float acc = 0.0f;
+      const float v = val[t];
+      const float f = fmaf(b0, v, z[0]);
+      z[0] = fmaf(b1, v, fmaf(-a1, f, z[1]));
+      z[1] = fmaf(b2, v, -(a2 * f));
+      val[t] = f;
+    }
+    // the level matrices are read from global memory with uniform addresses (scalar
+    // loads into SGPRs, which the FMAs take directly)
+    scan_states2(z, ex, p.trans + k * kScanLevels * 4);
+    if (!lead) {
+      const float* __restrict__ resp = p.trans + S * kScanLevels * 4 + k * kChunk * 2;
 #pragma unroll
-      for (int c = 0; c < W; c++) acc = fmaf(resp[t * W + c], z[c], acc);
-      val[t] += acc;
+      for (int t = 0; t < kChunk; t++) val[t] = fmaf(resp[2 * t], z[0], fmaf(resp[2 * t + 1], z[1], val[t]));
     }
   }
 }
 
 // Filters the n samples the caller stored at ext[pad .. pad + n) (LDS, FP32; ext
 // starts kSosMargin floats into a buffer of at least kSosMargin + n + 2 pad + kChunk
-// + 64 S floats, all of which are overwritten).  Writes the kept window of the result,
+// + 64 floats, all of which are overwritten).  Writes the kept window of the result,
 // rolled by `roll` (numpy.roll semantics), optionally mean-subtracted, to
 // out[0 .. out_count) = result[out_begin ..).  Returns true (on every thread) when
 // any of the n output samples is exactly zero.  `red` is 64 floats of LDS.  The
@@ -205,7 +166,7 @@ __device__ __forceinline__ bool sos_filtfilt_block_t(const SosScanArgs& p, float
   const int pad = p.pad;
   const int L = n + 2 * pad;
   const int C = p.chunk;
-  float* ex = ext + L + kChunk;  // scan exchange (32 * 2S floats) past the sequence and its overrun
+  float* ex = ext + L + kChunk;  // scan exchange (64 floats) past the sequence and its overrun
   // ---- odd extension: ext[i] = 2 x[0] - x[pad - i], ext[pad + n + k] = 2 x[n-1] - x[n - 2 - k]
   {
     const float first = ext[pad];
@@ -226,7 +187,7 @@ __device__ __forceinline__ bool sos_filtfilt_block_t(const SosScanArgs& p, float
   for (int t = 0; t < kChunk; t++) val[t] = active ? fwd[t] : 0.0f;
   const float ext_first = ext[0];
   __syncthreads();
-  run_direction<S>(p, val, ext_first, ex, active ? fwd : nullptr, 1);
+  run_direction<S>(p, val, ext_first, ex);
   if (active) {
 #pragma unroll
     for (int t = 0; t < kChunk; t++) fwd[t] = val[t];
@@ -238,7 +199,7 @@ __device__ __forceinline__ bool sos_filtfilt_block_t(const SosScanArgs& p, float
   for (int t = 0; t < kChunk; t++) val[t] = active ? bwd[-t] : 0.0f;
   const float y_last = ext[L - 1];
   __syncthreads();
-  run_direction<S>(p, val, y_last, ex, active ? bwd : nullptr, -1);
+  run_direction<S>(p, val, y_last, ex);
   // ---- zero test and sum over the n interior outputs; results back to LDS
   bool zero = false;
   float sum = 0.0f;

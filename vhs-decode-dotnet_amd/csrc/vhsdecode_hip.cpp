@@ -87,8 +87,8 @@ void steady_state_f32(const float (*c)[6], int sections, float* zi) {
   }
 }
 
-// Build SosScanArgs for an N-sample block: FP32 coefficients, zi, and the
-// per-level state-transition matrices A^(C 2^d) (FP64 products, stored FP32).
+// Build SosScanArgs for an N-sample block: FP32 coefficients, zi, and per section
+// the state-transition matrices A^(C 2^d) and the chunk response (FP64, stored FP32).
 int32_t prepare_sos(const double* sos, int sections, int n, vhsd::SosScanArgs* a, std::vector<float>* trans) {
   if (sections < 1 || (sections > 4 && sections != vhsd::kMaxSections))
     return fail(VHSDECODE_HIP_E_UNSUPPORTED, "SOS filters of 1 to 4 or 10 sections are supported");
@@ -100,7 +100,7 @@ int32_t prepare_sos(const double* sos, int sections, int n, vhsd::SosScanArgs* a
   // must fit the 33792-float block buffer
   a->chunk = vhsd::kChunk;
   if (a->pad > 64 || L > vhsd::kScanThreads * vhsd::kChunk ||
-      vhsd::kSosMargin + L + vhsd::kChunk + 32 * 2 * sections > 2 * vhsd::kFftBufEntries)
+      vhsd::kSosMargin + L + vhsd::kChunk + 64 > 2 * vhsd::kFftBufEntries)
     return fail(VHSDECODE_HIP_E_UNSUPPORTED, "SOS padding too long");
   float c32[vhsd::kMaxSections][6];
   for (int s = 0; s < sections; s++) {
@@ -112,63 +112,41 @@ int32_t prepare_sos(const double* sos, int sections, int n, vhsd::SosScanArgs* a
     a->coef[s][4] = c32[s][5];
   }
   steady_state_f32(c32, sections, a->zi);
-  const int W = 2 * sections;
-  // one-step transition with zero input, evaluated in FP64 on the FP32 coefficients
-  std::vector<double> A(W * W, 0.0);
-  for (int col = 0; col < W; col++) {
-    std::vector<double> z(W, 0.0);
-    z[col] = 1.0;
-    double v = 0.0;
-    for (int k = 0; k < sections; k++) {
-      const double b0 = c32[k][0], b1 = c32[k][1], b2 = c32[k][2], a1 = c32[k][4], a2 = c32[k][5];
-      const double f = b0 * v + z[2 * k];
-      const double z1 = b1 * v - a1 * f + z[2 * k + 1];
-      const double z2 = b2 * v - a2 * f;
-      z[2 * k] = z1;
-      z[2 * k + 1] = z2;
-      v = f;
-    }
-    for (int r = 0; r < W; r++) A[r * W + col] = z[r];
-  }
-  auto matmul = [W](const std::vector<double>& x, const std::vector<double>& y) {
-    std::vector<double> o(W * W, 0.0);
-    for (int r = 0; r < W; r++)
-      for (int k = 0; k < W; k++)
-        for (int c = 0; c < W; c++) o[r * W + c] += x[r * W + k] * y[k * W + c];
-    return o;
+  // per section: the one-step transition with zero input (state (z0, z1): output
+  // f = z0, z0' = z1 - a1 f, z1' = -a2 f), its powers A^(C 2^d) in FP64, then the
+  // chunk's zero-input response (output t per unit entry state)
+  const int L2 = vhsd::kScanLevels * 4;
+  trans->assign((size_t)sections * L2 + (size_t)sections * vhsd::kChunk * 2, 0.0f);
+  auto mul2 = [](const double* x, const double* y, double* o) {
+    const double r0 = x[0] * y[0] + x[1] * y[2], r1 = x[0] * y[1] + x[1] * y[3];
+    const double r2 = x[2] * y[0] + x[3] * y[2], r3 = x[2] * y[1] + x[3] * y[3];
+    o[0] = r0;
+    o[1] = r1;
+    o[2] = r2;
+    o[3] = r3;
   };
-  std::vector<double> P(W * W, 0.0);
-  for (int r = 0; r < W; r++) P[r * W + r] = 1.0;
-  std::vector<double> base = A;
-  int e = a->chunk;
-  while (e > 0) {
-    if (e & 1) P = matmul(P, base);
-    base = matmul(base, base);
-    e >>= 1;
-  }
-  trans->assign((size_t)vhsd::kScanLevels * W * W + (size_t)vhsd::kChunk * W, 0.0f);
-  std::vector<double> Q = P;
-  for (int d = 0; d < vhsd::kScanLevels; d++) {
-    for (int i = 0; i < W * W; i++) (*trans)[(size_t)d * W * W + i] = (float)Q[i];
-    Q = matmul(Q, Q);
-  }
-  // zero-input response of a chunk: output t for each unit entry state, in FP64
-  float* resp = trans->data() + (size_t)vhsd::kScanLevels * W * W;
-  for (int col = 0; col < W; col++) {
-    std::vector<double> z(W, 0.0);
-    z[col] = 1.0;
-    for (int t = 0; t < vhsd::kChunk; t++) {
-      double v = 0.0;
-      for (int k = 0; k < sections; k++) {
-        const double b0 = c32[k][0], b1 = c32[k][1], b2 = c32[k][2], a1 = c32[k][4], a2 = c32[k][5];
-        const double f = b0 * v + z[2 * k];
-        const double z1 = b1 * v - a1 * f + z[2 * k + 1];
-        const double z2 = b2 * v - a2 * f;
-        z[2 * k] = z1;
-        z[2 * k + 1] = z2;
-        v = f;
+  for (int k = 0; k < sections; k++) {
+    const double a1 = c32[k][4], a2 = c32[k][5];
+    const double A[4] = {-a1, 1.0, -a2, 0.0};
+    double P[4] = {1.0, 0.0, 0.0, 1.0}, base[4] = {A[0], A[1], A[2], A[3]};
+    for (int e = a->chunk; e > 0; e >>= 1) {
+      if (e & 1) mul2(P, base, P);
+      mul2(base, base, base);
+    }
+    float* lev = trans->data() + (size_t)k * L2;
+    for (int d = 0; d < vhsd::kScanLevels; d++) {
+      for (int i = 0; i < 4; i++) lev[d * 4 + i] = (float)P[i];
+      mul2(P, P, P);
+    }
+    float* resp = trans->data() + (size_t)sections * L2 + (size_t)k * vhsd::kChunk * 2;
+    for (int col = 0; col < 2; col++) {
+      double z0 = col == 0 ? 1.0 : 0.0, z1 = col == 1 ? 1.0 : 0.0;
+      for (int t = 0; t < vhsd::kChunk; t++) {
+        const double f = z0;
+        resp[t * 2 + col] = (float)f;
+        z0 = z1 - a1 * f;
+        z1 = -a2 * f;
       }
-      resp[t * W + col] = (float)v;
     }
   }
   return VHSDECODE_HIP_OK;
