@@ -1189,6 +1189,42 @@ hipError_t vhsd_launch_block_table(int kind, int sections, const void* recs, vhs
   }
 }
 
+namespace vhsd {
+// The first stage of a batch: every demod kind and every chroma section count in one
+// launch, so blocks of all formats run together (the kind is uniform per workgroup).
+__global__ void __launch_bounds__(1024) vhs_block_stage1_table_kernel(const DemodArgs* __restrict__ drecs,
+                                                                      const ChromaArgs* __restrict__ crecs,
+                                                                      StageTable t) {
+  const int i = blockIdx.x;
+  if (i < t.n_demod) {
+    const int blk = t.demod_ids[i];
+    const int g = t.block_rec[blk];
+    switch (t.rec_kind[g]) {
+      case kTableDemodBoost: vhs_block_demod_boost_body(drecs[g], blk); break;
+      case kTableDemodLd: vhs_block_demod_ld_body(drecs[g], blk); break;
+      default: vhs_block_demod_body(drecs[g], blk); break;
+    }
+  } else {
+    const int blk = t.chroma_ids[i - t.n_demod];
+    const int g = t.block_rec[blk];
+    switch (t.rec_sections[g]) {
+      case 1: vhs_block_chroma_body<1>(crecs[g], blk); break;
+      case 2: vhs_block_chroma_body<2>(crecs[g], blk); break;
+      case 3: vhs_block_chroma_body<3>(crecs[g], blk); break;
+      case 4: vhs_block_chroma_body<4>(crecs[g], blk); break;
+      default: vhs_block_chroma_body<10>(crecs[g], blk); break;
+    }
+  }
+}
+}  // namespace vhsd
+
+hipError_t vhsd_launch_stage1_table(const vhsd::DemodArgs* drecs, const vhsd::ChromaArgs* crecs,
+                                    vhsd::StageTable t, int n_chroma, hipStream_t stream) {
+  const int count = t.n_demod + n_chroma;
+  if (count <= 0) return hipSuccess;
+  return launch_block_kernel(vhsd::vhs_block_stage1_table_kernel, count, stream, drecs, crecs, t);
+}
+
 hipError_t vhsd_launch_fill_u32(unsigned* p, long long n, unsigned value, hipStream_t stream) {
   if (n <= 0) return hipSuccess;
   const int blocks = (int)std::min<long long>((n + 255) / 256, 1024);

@@ -109,6 +109,18 @@ struct SubDeemphArgs {
 // workgroup i takes global block block_ids[i] and that block's record
 // recs[block_rec[block]] (per-block arrays of the records are indexed by the global
 // block).
+// A batch's first stage in one launch: workgroups [0, n_demod) run the demod stage of
+// block demod_ids[i] (plain, RF boost or LaserDisc by its record's kind), the rest the
+// chroma filter of block chroma_ids[i - n_demod] (by its record's section count).
+struct StageTable {
+  const int* demod_ids;
+  int n_demod;
+  const int* chroma_ids;
+  const int* block_rec;     // block -> record
+  const int* rec_kind;      // record -> kTableDemod / kTableDemodBoost / kTableDemodLd
+  const int* rec_sections;  // record -> chroma sections
+};
+
 struct BlockTable {
   const int* block_ids;
   const int* block_rec;
@@ -240,6 +252,8 @@ hipError_t vhsd_launch_locate(const float* sync_ref, long long span_len, const i
                               const vhsd::FieldTables* t, int* field_count, hipStream_t stream);
 hipError_t vhsd_launch_block_demod_boost(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_demod_ld(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
+hipError_t vhsd_launch_stage1_table(const vhsd::DemodArgs* drecs, const vhsd::ChromaArgs* crecs,
+                                    vhsd::StageTable t, int n_chroma, hipStream_t stream);
 hipError_t vhsd_launch_fill_u32(unsigned* p, long long n, unsigned value, hipStream_t stream);
 // kind: vhsd::BlockTableKernel; sections: chroma SOS sections (kTableChroma only); recs: device
 // array of the kind's argument records (DemodArgs, SubDeemphArgs, RepairArgs or ChromaArgs).

@@ -921,7 +921,10 @@ struct vhsdecode_hip_batch {
   DevBuf<vhsd::RepairArgs> r_repair;
   DevBuf<vhsd::SubDeemphArgs> r_sd;
   DevBuf<vhsd::ChromaArgs> r_chroma;
-  std::vector<Kind> kinds;
+  std::vector<Kind> kinds;  // launches after the first stage
+  // first stage: every demod kind and chroma section count in one launch
+  DevBuf<int> s1_demod_ids, s1_chroma_ids, rec_kind, rec_sections;
+  int n_demod = 0, n_chroma = 0;
   long long blocks = 0, out_end = 0, rf_end = 0;
   bool use_graph = false;
   hipStream_t stream = nullptr;
@@ -937,6 +940,7 @@ struct vhsdecode_hip_batch {
     if (graph) (void)hipGraphDestroy(graph);
     rec.release(); starts.release(); offs.release(); flags.release(); stash.release(); aux.release();
     rep_a.release(); rep_dm.release(); r_demod.release(); r_repair.release(); r_sd.release(); r_chroma.release();
+    s1_demod_ids.release(); s1_chroma_ids.release(); rec_kind.release(); rec_sections.release();
     for (auto& k : kinds) k.d_ids.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -969,7 +973,30 @@ int32_t batch_prepare(vhsdecode_hip_batch* b, const int16_t* rf, const vhsdecode
   VHSD_HIP(hipMemcpy(b->r_repair.p, r.data(), sizeof(vhsd::RepairArgs) * g, hipMemcpyHostToDevice));
   VHSD_HIP(hipMemcpy(b->r_sd.p, sd.data(), sizeof(vhsd::SubDeemphArgs) * g, hipMemcpyHostToDevice));
   VHSD_HIP(hipMemcpy(b->r_chroma.p, ch.data(), sizeof(vhsd::ChromaArgs) * g, hipMemcpyHostToDevice));
-  // launch order: demod kinds, repair, video, sub-deemphasis, chroma by section count
+  // first stage: the demod of every block and the chroma of every block that has it
+  std::vector<int> s1_demod, s1_chroma, rkind(g), rsec(g);
+  for (size_t i = 0; i < g; i++) {
+    rkind[i] = ca[i].demod_kind;
+    rsec[i] = ca[i].chroma_sections;
+  }
+  for (long long blk = 0; blk < b->blocks; blk++) {
+    s1_demod.push_back((int)blk);
+    if (ca[b->h_rec[blk]].chroma) s1_chroma.push_back((int)blk);
+  }
+  b->n_demod = (int)s1_demod.size();
+  b->n_chroma = (int)s1_chroma.size();
+  VHSD_HIP(b->s1_demod_ids.ensure(std::max<size_t>(s1_demod.size(), 1)));
+  VHSD_HIP(b->s1_chroma_ids.ensure(std::max<size_t>(s1_chroma.size(), 1)));
+  VHSD_HIP(b->rec_kind.ensure(g));
+  VHSD_HIP(b->rec_sections.ensure(g));
+  if (!s1_demod.empty())
+    VHSD_HIP(hipMemcpy(b->s1_demod_ids.p, s1_demod.data(), sizeof(int) * s1_demod.size(), hipMemcpyHostToDevice));
+  if (!s1_chroma.empty())
+    VHSD_HIP(hipMemcpy(b->s1_chroma_ids.p, s1_chroma.data(), sizeof(int) * s1_chroma.size(),
+                       hipMemcpyHostToDevice));
+  VHSD_HIP(hipMemcpy(b->rec_kind.p, rkind.data(), sizeof(int) * g, hipMemcpyHostToDevice));
+  VHSD_HIP(hipMemcpy(b->rec_sections.p, rsec.data(), sizeof(int) * g, hipMemcpyHostToDevice));
+  // then: repair, video, sub-deemphasis
   std::vector<vhsdecode_hip_batch::Kind> kinds;
   auto add = [&](int kernel, int sections, auto pred) {
     vhsdecode_hip_batch::Kind k;
@@ -979,13 +1006,9 @@ int32_t batch_prepare(vhsdecode_hip_batch* b, const int16_t* rf, const vhsdecode
       if (pred(ca[b->h_rec[blk]])) k.ids.push_back((int)blk);
     if (!k.ids.empty()) kinds.push_back(std::move(k));
   };
-  for (int kind : {vhsd::kTableDemod, vhsd::kTableDemodBoost, vhsd::kTableDemodLd})
-    add(kind, 0, [kind](const ChainArgs& x) { return x.demod_kind == kind; });
   add(vhsd::kTableRepair, 0, [](const ChainArgs& x) { return x.repair; });
   add(vhsd::kTableVideo, 0, [](const ChainArgs&) { return true; });
   add(vhsd::kTableSubDeemph, 0, [](const ChainArgs& x) { return x.subdeemph; });
-  for (int sec : {1, 2, 3, 4, 10})
-    add(vhsd::kTableChroma, sec, [sec](const ChainArgs& x) { return x.chroma && x.chroma_sections == sec; });
   for (auto& k : b->kinds) k.d_ids.release();
   b->kinds = std::move(kinds);
   for (auto& k : b->kinds) {
@@ -998,6 +1021,9 @@ int32_t batch_prepare(vhsdecode_hip_batch* b, const int16_t* rf, const vhsdecode
 // Kernel launches only: safe to capture.
 int32_t batch_enqueue(vhsdecode_hip_batch* b, hipStream_t st) {
   VHSD_HIP(vhsd_launch_fill_u32(b->flags.p, b->blocks, 0u, st));
+  const vhsd::StageTable t1{b->s1_demod_ids.p, b->n_demod, b->s1_chroma_ids.p, b->rec.p, b->rec_kind.p,
+                            b->rec_sections.p};
+  VHSD_HIP(vhsd_launch_stage1_table(b->r_demod.p, b->r_chroma.p, t1, b->n_chroma, st));
   for (const auto& k : b->kinds) {
     const void* recs = k.kernel == vhsd::kTableRepair      ? (const void*)b->r_repair.p
                        : k.kernel == vhsd::kTableSubDeemph ? (const void*)b->r_sd.p
