@@ -326,11 +326,6 @@ __device__ __forceinline__ void vhs_block_demod_body(const DemodArgs& a, const i
 
 __global__ void __launch_bounds__(1024) vhs_block_demod_kernel(DemodArgs a) { vhs_block_demod_body(a, blockIdx.x); }
 
-// Table mode: workgroup i decodes global block t.block_ids[i] with its format's record.
-__global__ void __launch_bounds__(1024) vhs_block_demod_table_kernel(const DemodArgs* __restrict__ recs, BlockTable t) {
-  const int blk = t.block_ids[blockIdx.x];
-  vhs_block_demod_body(recs[t.block_rec[blk]], blk);
-}
 
 // numpy's pairwise float32 sum of the 32768 values at y (NumpyReduction.PairwiseSumFloat32,
 // Dsp/NumpyReduction.cs:253-339) in its exact order: 256 runs of 128 summed with eight
@@ -481,11 +476,6 @@ __device__ __forceinline__ void vhs_block_demod_boost_body(const DemodArgs& a, c
 
 __global__ void __launch_bounds__(1024) vhs_block_demod_boost_kernel(DemodArgs a) { vhs_block_demod_boost_body(a, blockIdx.x); }
 
-// Table mode: workgroup i decodes global block t.block_ids[i] with its format's record.
-__global__ void __launch_bounds__(1024) vhs_block_demod_boost_table_kernel(const DemodArgs* __restrict__ recs, BlockTable t) {
-  const int blk = t.block_ids[blockIdx.x];
-  vhs_block_demod_boost_body(recs[t.block_rec[blk]], blk);
-}
 
 // ------------------------------------------------------------------ LaserDisc
 // PortedMath.UnwrapHilbertConjugateProduct (Dsp/PortedMath.cs:39-68) for one pair of
@@ -673,11 +663,6 @@ __device__ __forceinline__ void vhs_block_demod_ld_body(const DemodArgs& a, cons
 
 __global__ void __launch_bounds__(1024) vhs_block_demod_ld_kernel(DemodArgs a) { vhs_block_demod_ld_body(a, blockIdx.x); }
 
-// Table mode: workgroup i decodes global block t.block_ids[i] with its format's record.
-__global__ void __launch_bounds__(1024) vhs_block_demod_ld_table_kernel(const DemodArgs* __restrict__ recs, BlockTable t) {
-  const int blk = t.block_ids[blockIdx.x];
-  vhs_block_demod_ld_body(recs[t.block_rec[blk]], blk);
-}
 
 // Stage C: video and sync-reference planes from the demod the block demod (or spike
 // repair) kernel left in each block's stash.
@@ -691,11 +676,6 @@ __device__ __forceinline__ void vhs_block_video_body(const DemodArgs& a, const i
 
 __global__ void __launch_bounds__(1024) vhs_block_video_kernel(DemodArgs a) { vhs_block_video_body(a, blockIdx.x); }
 
-// Table mode: workgroup i decodes global block t.block_ids[i] with its format's record.
-__global__ void __launch_bounds__(1024) vhs_block_video_table_kernel(const DemodArgs* __restrict__ recs, BlockTable t) {
-  const int blk = t.block_ids[blockIdx.x];
-  vhs_block_video_body(recs[t.block_rec[blk]], blk);
-}
 
 // ------------------------------------------------------------------ chroma
 template <int S>
@@ -718,12 +698,6 @@ __device__ __forceinline__ void vhs_block_chroma_body(const ChromaArgs& a, const
 template <int S>
 __global__ void __launch_bounds__(1024) vhs_block_chroma_kernel(ChromaArgs a) { vhs_block_chroma_body<S>(a, blockIdx.x); }
 
-// Table mode: workgroup i decodes global block t.block_ids[i] with its format's record.
-template <int S>
-__global__ void __launch_bounds__(1024) vhs_block_chroma_table_kernel(const ChromaArgs* __restrict__ recs, BlockTable t) {
-  const int blk = t.block_ids[blockIdx.x];
-  vhs_block_chroma_body<S>(recs[t.block_rec[blk]], blk);
-}
 
 // ------------------------------------------------------------------ sub-deemphasis
 // RfDemodulator.ApplySubDeemphasisRealCore (Dsp/RfDemodulator.cs:2897-2999) on one block:
@@ -864,11 +838,6 @@ __device__ __forceinline__ void vhs_block_subdeemph_body(const SubDeemphArgs& a,
 
 __global__ void __launch_bounds__(1024) vhs_block_subdeemph_kernel(SubDeemphArgs a) { vhs_block_subdeemph_body(a, blockIdx.x); }
 
-// Table mode: workgroup i decodes global block t.block_ids[i] with its format's record.
-__global__ void __launch_bounds__(1024) vhs_block_subdeemph_table_kernel(const SubDeemphArgs* __restrict__ recs, BlockTable t) {
-  const int blk = t.block_ids[blockIdx.x];
-  vhs_block_subdeemph_body(recs[t.block_rec[blk]], blk);
-}
 
 // ------------------------------------------------------------------ spike repair
 // ReplaceSpikes (RfDemodulator.cs:966-1001) for one 32768-sample block: spike indices
@@ -1037,11 +1006,6 @@ __device__ __forceinline__ void vhs_block_repair_body(const RepairArgs& r, const
 
 __global__ void __launch_bounds__(1024) vhs_block_repair_kernel(RepairArgs r) { vhs_block_repair_body(r, blockIdx.x); }
 
-// Table mode: workgroup i decodes global block t.block_ids[i] with its format's record.
-__global__ void __launch_bounds__(1024) vhs_block_repair_table_kernel(const RepairArgs* __restrict__ recs, BlockTable t) {
-  const int blk = t.block_ids[blockIdx.x];
-  vhs_block_repair_body(recs[t.block_rec[blk]], blk);
-}
 
 // Fills n words (a kernel rather than hipMemsetAsync, so the chain can be captured
 // into a graph and replayed).
@@ -1094,11 +1058,6 @@ static hipError_t launch_block_kernel(K kernel, int count, hipStream_t stream, A
   return hipGetLastError();
 }
 
-template <int S>
-static hipError_t launch_chroma_table_s(const vhsd::ChromaArgs* recs, vhsd::BlockTable t, int count,
-                                        hipStream_t stream) {
-  return launch_block_kernel(vhsd::vhs_block_chroma_table_kernel<S>, count, stream, recs, t);
-}
 
 template <int S>
 static hipError_t launch_chroma_s(const vhsd::ChromaArgs* a, int nblocks, hipStream_t stream) {
@@ -1158,36 +1117,6 @@ hipError_t vhsd_launch_block_demod_boost(const vhsd::DemodArgs* a, int nblocks, 
   return hipGetLastError();
 }
 
-hipError_t vhsd_launch_block_table(int kind, int sections, const void* recs, vhsd::BlockTable t, int count,
-                                   hipStream_t stream) {
-  using namespace vhsd;
-  if (count <= 0) return hipSuccess;
-  switch (kind) {
-    case kTableDemod:
-      return launch_block_kernel(vhs_block_demod_table_kernel, count, stream, (const DemodArgs*)recs, t);
-    case kTableDemodBoost:
-      return launch_block_kernel(vhs_block_demod_boost_table_kernel, count, stream, (const DemodArgs*)recs, t);
-    case kTableDemodLd:
-      return launch_block_kernel(vhs_block_demod_ld_table_kernel, count, stream, (const DemodArgs*)recs, t);
-    case kTableVideo:
-      return launch_block_kernel(vhs_block_video_table_kernel, count, stream, (const DemodArgs*)recs, t);
-    case kTableSubDeemph:
-      return launch_block_kernel(vhs_block_subdeemph_table_kernel, count, stream, (const SubDeemphArgs*)recs, t);
-    case kTableRepair:
-      return launch_block_kernel(vhs_block_repair_table_kernel, count, stream, (const RepairArgs*)recs, t);
-    case kTableChroma:
-      switch (sections) {
-        case 1: return launch_chroma_table_s<1>((const ChromaArgs*)recs, t, count, stream);
-        case 2: return launch_chroma_table_s<2>((const ChromaArgs*)recs, t, count, stream);
-        case 3: return launch_chroma_table_s<3>((const ChromaArgs*)recs, t, count, stream);
-        case 4: return launch_chroma_table_s<4>((const ChromaArgs*)recs, t, count, stream);
-        case 10: return launch_chroma_table_s<10>((const ChromaArgs*)recs, t, count, stream);
-        default: return hipErrorInvalidValue;
-      }
-    default:
-      return hipErrorInvalidValue;
-  }
-}
 
 namespace vhsd {
 // The first stage of a batch: every demod kind and every chroma section count in one
@@ -1216,7 +1145,35 @@ __global__ void __launch_bounds__(1024) vhs_block_stage1_table_kernel(const Demo
     }
   }
 }
+
+// The second stage of a batch: repair, video and sub-deemphasis of one block in one
+// workgroup, in that order (each body starts from the block's stash and planes the
+// previous one wrote; the barrier between them orders the workgroup's global and LDS
+// accesses).
+__global__ void __launch_bounds__(1024) vhs_block_stage2_table_kernel(const RepairArgs* __restrict__ rrecs,
+                                                                      const DemodArgs* __restrict__ drecs,
+                                                                      const SubDeemphArgs* __restrict__ srecs,
+                                                                      StageTable2 t) {
+  const int blk = blockIdx.x;
+  const int g = t.block_rec[blk];
+  if (t.rec_repair[g]) {
+    vhs_block_repair_body(rrecs[g], blk);
+    __syncthreads();
+  }
+  vhs_block_video_body(drecs[g], blk);
+  if (t.rec_subdeemph[g]) {
+    __syncthreads();
+    vhs_block_subdeemph_body(srecs[g], blk);
+  }
+}
 }  // namespace vhsd
+
+hipError_t vhsd_launch_stage2_table(const vhsd::RepairArgs* rrecs, const vhsd::DemodArgs* drecs,
+                                    const vhsd::SubDeemphArgs* srecs, vhsd::StageTable2 t, int blocks,
+                                    hipStream_t stream) {
+  if (blocks <= 0) return hipSuccess;
+  return launch_block_kernel(vhsd::vhs_block_stage2_table_kernel, blocks, stream, rrecs, drecs, srecs, t);
+}
 
 hipError_t vhsd_launch_stage1_table(const vhsd::DemodArgs* drecs, const vhsd::ChromaArgs* crecs,
                                     vhsd::StageTable t, int n_chroma, hipStream_t stream) {

@@ -105,10 +105,9 @@ struct SubDeemphArgs {
   float2* aux;                   // [nblocks][kAuxEntries]
 };
 
-// Table mode of the block kernels: one launch decodes blocks of several formats;
-// workgroup i takes global block block_ids[i] and that block's record
-// recs[block_rec[block]] (per-block arrays of the records are indexed by the global
-// block).
+// Batched block kernels: one launch decodes blocks of several formats; a workgroup
+// takes a global block and that block's record recs[block_rec[block]] (the per-block
+// arrays of the records are indexed by the global block).
 // A batch's first stage in one launch: workgroups [0, n_demod) run the demod stage of
 // block demod_ids[i] (plain, RF boost or LaserDisc by its record's kind), the rest the
 // chroma filter of block chroma_ids[i - n_demod] (by its record's section count).
@@ -121,19 +120,20 @@ struct StageTable {
   const int* rec_sections;  // record -> chroma sections
 };
 
-struct BlockTable {
-  const int* block_ids;
-  const int* block_rec;
+// A batch's second stage in one launch, per block: the spike repair (when its record
+// has it and the block is flagged), the video stage, then the sub-deemphasis (when
+// its record has it).
+struct StageTable2 {
+  const int* block_rec;       // block -> record
+  const int* rec_repair;      // record -> 1 when the repair may run
+  const int* rec_subdeemph;   // record -> 1 when the sub-deemphasis runs
 };
 
-enum BlockTableKernel {
+// Demod-stage kinds of a batch record (StageTable::rec_kind).
+enum BlockDemodKind {
   kTableDemod = 0,
   kTableDemodBoost = 1,
   kTableDemodLd = 2,
-  kTableVideo = 3,
-  kTableSubDeemph = 4,
-  kTableRepair = 5,
-  kTableChroma = 6,
 };
 
 struct RepairArgs {
@@ -254,11 +254,10 @@ hipError_t vhsd_launch_block_demod_boost(const vhsd::DemodArgs* a, int nblocks, 
 hipError_t vhsd_launch_block_demod_ld(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_stage1_table(const vhsd::DemodArgs* drecs, const vhsd::ChromaArgs* crecs,
                                     vhsd::StageTable t, int n_chroma, hipStream_t stream);
+hipError_t vhsd_launch_stage2_table(const vhsd::RepairArgs* rrecs, const vhsd::DemodArgs* drecs,
+                                    const vhsd::SubDeemphArgs* srecs, vhsd::StageTable2 t, int blocks,
+                                    hipStream_t stream);
 hipError_t vhsd_launch_fill_u32(unsigned* p, long long n, unsigned value, hipStream_t stream);
-// kind: vhsd::BlockTableKernel; sections: chroma SOS sections (kTableChroma only); recs: device
-// array of the kind's argument records (DemodArgs, SubDeemphArgs, RepairArgs or ChromaArgs).
-hipError_t vhsd_launch_block_table(int kind, int sections, const void* recs, vhsd::BlockTable t, int count,
-                                   hipStream_t stream);
 hipError_t vhsd_launch_block_video(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_subdeemph(const vhsd::SubDeemphArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const float2* oct, hipStream_t stream);

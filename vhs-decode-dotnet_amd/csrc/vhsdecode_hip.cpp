@@ -894,19 +894,15 @@ int32_t vhsdecode_hip_demod_span_device(vhsdecode_hip_demod_context* c, const in
 // A batch decodes the whole blocks of many segments of mixed formats and sample rates.
 // Every block gets a global index; per-block arrays (input and output offsets, flags,
 // stash and scratch) are indexed by it, and each context contributes one argument
-// record per kernel kind pointing at those shared arrays.  Each kernel kind is then
-// ONE launch in table mode over all blocks that need it, whatever their format
-// (workgroup i -> global block ids[i] -> record rec[block]): demod (plain, RF boost,
-// LaserDisc), repair, video, sub-deemphasis and chroma (per section count).  The
-// launch sequence is captured once into a HIP graph and replayed while the RF and
-// plane pointers stay the same; the records are rebuilt (outside any capture) when
-// they change.
+// record per kernel kind pointing at those shared arrays.  The whole batch is two
+// launches whatever the mix of formats: the first stage runs the demod (plain, RF
+// boost or LaserDisc by the record) of every block and the chroma filter (by the
+// record's section count) of every block that has one; the second runs, per block,
+// the repair (flagged blocks), the video stage and the sub-deemphasis.  The launch
+// sequence is captured once into a HIP graph and replayed while the RF and plane
+// pointers stay the same; the records are rebuilt (outside any capture) when they
+// change.
 struct vhsdecode_hip_batch {
-  struct Kind {
-    int kernel = 0, sections = 0;
-    std::vector<int> ids;
-    DevBuf<int> d_ids;
-  };
   std::mutex mu;
   int device = 0;
   std::vector<vhsdecode_hip_demod_context*> groups;  // one per context
@@ -921,9 +917,10 @@ struct vhsdecode_hip_batch {
   DevBuf<vhsd::RepairArgs> r_repair;
   DevBuf<vhsd::SubDeemphArgs> r_sd;
   DevBuf<vhsd::ChromaArgs> r_chroma;
-  std::vector<Kind> kinds;  // launches after the first stage
   // first stage: every demod kind and chroma section count in one launch
   DevBuf<int> s1_demod_ids, s1_chroma_ids, rec_kind, rec_sections;
+  // second stage: repair, video and sub-deemphasis of every block in one launch
+  DevBuf<int> rec_repair, rec_subdeemph;
   int n_demod = 0, n_chroma = 0;
   long long blocks = 0, out_end = 0, rf_end = 0;
   bool use_graph = false;
@@ -941,7 +938,7 @@ struct vhsdecode_hip_batch {
     rec.release(); starts.release(); offs.release(); flags.release(); stash.release(); aux.release();
     rep_a.release(); rep_dm.release(); r_demod.release(); r_repair.release(); r_sd.release(); r_chroma.release();
     s1_demod_ids.release(); s1_chroma_ids.release(); rec_kind.release(); rec_sections.release();
-    for (auto& k : kinds) k.d_ids.release();
+    rec_repair.release(); rec_subdeemph.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -996,25 +993,16 @@ int32_t batch_prepare(vhsdecode_hip_batch* b, const int16_t* rf, const vhsdecode
                        hipMemcpyHostToDevice));
   VHSD_HIP(hipMemcpy(b->rec_kind.p, rkind.data(), sizeof(int) * g, hipMemcpyHostToDevice));
   VHSD_HIP(hipMemcpy(b->rec_sections.p, rsec.data(), sizeof(int) * g, hipMemcpyHostToDevice));
-  // then: repair, video, sub-deemphasis
-  std::vector<vhsdecode_hip_batch::Kind> kinds;
-  auto add = [&](int kernel, int sections, auto pred) {
-    vhsdecode_hip_batch::Kind k;
-    k.kernel = kernel;
-    k.sections = sections;
-    for (long long blk = 0; blk < b->blocks; blk++)
-      if (pred(ca[b->h_rec[blk]])) k.ids.push_back((int)blk);
-    if (!k.ids.empty()) kinds.push_back(std::move(k));
-  };
-  add(vhsd::kTableRepair, 0, [](const ChainArgs& x) { return x.repair; });
-  add(vhsd::kTableVideo, 0, [](const ChainArgs&) { return true; });
-  add(vhsd::kTableSubDeemph, 0, [](const ChainArgs& x) { return x.subdeemph; });
-  for (auto& k : b->kinds) k.d_ids.release();
-  b->kinds = std::move(kinds);
-  for (auto& k : b->kinds) {
-    VHSD_HIP(k.d_ids.ensure(k.ids.size()));
-    VHSD_HIP(hipMemcpy(k.d_ids.p, k.ids.data(), sizeof(int) * k.ids.size(), hipMemcpyHostToDevice));
+  // second stage: repair (flagged blocks of records that have it), video, sub-deemphasis
+  std::vector<int> rrep(g), rsd(g);
+  for (size_t i = 0; i < g; i++) {
+    rrep[i] = ca[i].repair ? 1 : 0;
+    rsd[i] = ca[i].subdeemph ? 1 : 0;
   }
+  VHSD_HIP(b->rec_repair.ensure(g));
+  VHSD_HIP(b->rec_subdeemph.ensure(g));
+  VHSD_HIP(hipMemcpy(b->rec_repair.p, rrep.data(), sizeof(int) * g, hipMemcpyHostToDevice));
+  VHSD_HIP(hipMemcpy(b->rec_subdeemph.p, rsd.data(), sizeof(int) * g, hipMemcpyHostToDevice));
   return VHSDECODE_HIP_OK;
 }
 
@@ -1024,14 +1012,8 @@ int32_t batch_enqueue(vhsdecode_hip_batch* b, hipStream_t st) {
   const vhsd::StageTable t1{b->s1_demod_ids.p, b->n_demod, b->s1_chroma_ids.p, b->rec.p, b->rec_kind.p,
                             b->rec_sections.p};
   VHSD_HIP(vhsd_launch_stage1_table(b->r_demod.p, b->r_chroma.p, t1, b->n_chroma, st));
-  for (const auto& k : b->kinds) {
-    const void* recs = k.kernel == vhsd::kTableRepair      ? (const void*)b->r_repair.p
-                       : k.kernel == vhsd::kTableSubDeemph ? (const void*)b->r_sd.p
-                       : k.kernel == vhsd::kTableChroma    ? (const void*)b->r_chroma.p
-                                                           : (const void*)b->r_demod.p;
-    VHSD_HIP(vhsd_launch_block_table(k.kernel, k.sections, recs, vhsd::BlockTable{k.d_ids.p, b->rec.p},
-                                     (int)k.ids.size(), st));
-  }
+  const vhsd::StageTable2 t2{b->rec.p, b->rec_repair.p, b->rec_subdeemph.p};
+  VHSD_HIP(vhsd_launch_stage2_table(b->r_repair.p, b->r_demod.p, b->r_sd.p, t2, (int)b->blocks, st));
   return VHSDECODE_HIP_OK;
 }
 
