@@ -42,3 +42,38 @@ def write_fields(destination: BinaryIO, fields: np.ndarray, count: int, system: 
     for f in range(count):
         write_field(destination, flat[f * per:(f + 1) * per], system)
     return 2 * per * count
+
+
+def dropouts_object(field_line, start_x, end_x, enabled: bool = True):
+    """The field's `"dropOuts"` object of the `.tbc.json` metadata, as
+    TbcOutputMetadataWriter.AddDropouts writes it (Decode/TbcOutputMetadataWriter.cs:1091-1107):
+    three parallel integer arrays keyed fieldLine / startx / endx, or None (the key is left
+    out) when dropout detection is off or the field has no rows."""
+    rows = [np.asarray(a).reshape(-1) for a in (field_line, start_x, end_x)]
+    if not (rows[0].size == rows[1].size == rows[2].size):
+        raise ValueError("dropout row arrays differ in length")
+    if not enabled or rows[0].size == 0:
+        return None
+    return {key: [int(v) for v in a] for key, a in zip(("fieldLine", "startx", "endx"), rows)}
+
+
+def field_dropout_objects(row_count, field_line, start_x, end_x, capacity: int, fields: int,
+                          enabled: bool = True) -> list:
+    """Splits the per-field rows of vhsdecode_hip_detect_dropouts_device ([field][capacity]
+    arrays, row_count[field] rows each, copied to the host) into one `"dropOuts"` object (or
+    None) per field.  A field whose count exceeds `capacity` or is -1 (detection range over
+    the device limit) has lost rows: that raises rather than writing a partial list."""
+    counts = np.asarray(row_count).reshape(-1)
+    if fields < 0 or fields > counts.size:
+        raise ValueError("field count exceeds the row counts")
+    planes = [np.asarray(a).reshape(-1) for a in (field_line, start_x, end_x)]
+    if any(p.size < fields * capacity for p in planes):
+        raise ValueError("row arrays are smaller than fields x capacity")
+    out = []
+    for f in range(fields):
+        n = int(counts[f])
+        if n < 0 or n > capacity:
+            raise ValueError(f"field {f}: {n} dropout rows do not fit the capacity of {capacity}")
+        base = f * capacity
+        out.append(dropouts_object(*(p[base:base + n] for p in planes), enabled=enabled))
+    return out
