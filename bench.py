@@ -465,6 +465,7 @@ def main():
     t0 = time.perf_counter()
     for i in range(args.steps):
         pipes[i % len(pipes)].step()
+    t_enq = time.perf_counter()
     hipmem.synchronize()
     t1 = time.perf_counter()
     nfields = int(pipes[0].meta.download(np.int32, 1)[0])
@@ -500,7 +501,8 @@ def main():
         if p["launches"]:
             print(f"[bench] {name:13s} launches={p['launches']:4d} avg={p['ms'] / p['launches'] * 1e3:9.1f} us",
                   file=sys.stderr)
-    print(f"[bench] blocks={nb} fields/step={nfields} step={elapsed / args.steps * 1e3:.3f} ms", file=sys.stderr)
+    print(f"[bench] blocks={nb} fields/step={nfields} step={elapsed / args.steps * 1e3:.3f} ms "
+          f"(host enqueue {(t_enq - t0) / args.steps * 1e3:.3f} ms per step)", file=sys.stderr)
     print("[bench] one span in flight: " + ", ".join(
         f"{name} {p['ms'] / p['launches'] * 1e3:.1f} us" for name, p in prof_iso.items() if p["launches"]),
         file=sys.stderr)
