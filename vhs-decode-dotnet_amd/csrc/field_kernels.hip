@@ -618,6 +618,29 @@ __device__ __forceinline__ double sinc_sum(const float* src, long long len, doub
   return result;
 }
 
+// sinc_sum for taps read from a wave's source window staged in LDS: window sample w
+// sits at float (w >> 2) * stride4 + 32 + (w & 3) (the pad slot of pixel w >> 2 in the
+// TBC row stage); rel = first tap - window start.  Same operations in the same order.
+__device__ __forceinline__ double sinc_sum_window(const float* win, int stride4, int rel, double bias,
+                                                  const SincPhase& ph, const float (&ws)[16],
+                                                  const float (&we)[16]) {
+  double result = 0.0;
+#pragma unroll
+  for (int t = 0; t < 16; t++) {
+    const float sw = ws[t];
+    const float wgt = __builtin_fmaf(ph.alpha, we[t] - sw, sw);
+    const int w = rel + t;
+    const float v = win[(w >> 2) * stride4 + 32 + (w & 3)];
+    float prod;
+    {
+#pragma clang fp contract(off)
+      prod = (float)((double)v + bias) * wgt;
+    }
+    result = (double)prod + result;
+  }
+  return result;
+}
+
 __device__ __forceinline__ double sample_sinc(const float* src, long long len, double bias, double position,
                                               const float* __restrict__ lut) {
   if (!isfinite(position)) return 0.0;
@@ -786,8 +809,10 @@ struct FieldTbcParams {
 };
 
 // Stage for the cooperative table-row fetch: per wave, 64 pixels x 8 float4 (two
-// rows), padded to 9 float4 per pixel against LDS bank conflicts.
+// rows), padded to 9 float4 per pixel against LDS bank conflicts.  The 64 pad slots
+// (256 floats) hold the wave's source window.
 constexpr int kTbcRowStride = 9;
+constexpr int kTbcWindow = 256;
 
 __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long long src_len, double bias,
                                                          const long long* origins, const double* locs,
@@ -841,6 +866,21 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
       stage[pix * kTbcRowStride + 4 + quarter] = c1;
     }
   }
+  // The wave's 64 pixels take their 16 taps from a few hundred consecutive source
+  // samples: one coalesced load of the 256 samples from the first lane's first tap
+  // replaces most of the per-lane tap loads.  A lane whose taps leave that window (span
+  // edges, a jump between line locations) reads the plane directly.
+  const float* sp_src = src + org;
+  const long long sp_len = src_len - org;
+  const long long s0 = (long long)ph.ci - 7;
+  const long long base = __shfl(s0, 0);
+  float* win = reinterpret_cast<float*>(stage);
+#pragma unroll
+  for (int i = 0; i < kTbcWindow / 64; i++) {
+    const int w = lane + 64 * i;
+    const long long g = base + w;
+    win[(w >> 2) * (4 * kTbcRowStride) + 32 + (w & 3)] = (g >= 0 && g < sp_len) ? sp_src[g] : 0.0f;
+  }
   __syncthreads();
   float ws[16], we[16];
 #pragma unroll
@@ -853,7 +893,11 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
   double r;
   {
 #pragma clang fp contract(off)
-    const double sv = finite ? sinc_sum(src + org, src_len - org, bias, ph, ws, we) : 0.0;
+    const long long rel = s0 - base;
+    const bool in_window = sp_len >= 16 && s0 >= 0 && s0 <= sp_len - 16 && rel >= 0 && rel <= kTbcWindow - 16;
+    const double sv = !finite ? 0.0
+                      : in_window ? sinc_sum_window(win, 4 * kTbcRowStride, (int)rel, bias, ph, ws, we)
+                                  : sinc_sum(sp_src, sp_len, bias, ph, ws, we);
     r = (double)(float)(sv * adj);
   }
   double sd;
