@@ -618,19 +618,16 @@ __device__ __forceinline__ double sinc_sum(const float* src, long long len, doub
   return result;
 }
 
-// sinc_sum for taps read from a wave's source window staged in LDS: window sample w
-// sits at float (w >> 2) * stride4 + 32 + (w & 3) (the pad slot of pixel w >> 2 in the
-// TBC row stage); rel = first tap - window start.  Same operations in the same order.
-__device__ __forceinline__ double sinc_sum_window(const float* win, int stride4, int rel, double bias,
-                                                  const SincPhase& ph, const float (&ws)[16],
-                                                  const float (&we)[16]) {
+// sinc_sum for taps read from a wave's source window staged in LDS (taps[t], t = 0..15,
+// the lane's first tap onwards).  Same operations in the same order.
+__device__ __forceinline__ double sinc_sum_window(const float* taps, double bias, const SincPhase& ph,
+                                                  const float (&ws)[16], const float (&we)[16]) {
   double result = 0.0;
 #pragma unroll
   for (int t = 0; t < 16; t++) {
     const float sw = ws[t];
     const float wgt = __builtin_fmaf(ph.alpha, we[t] - sw, sw);
-    const int w = rel + t;
-    const float v = win[(w >> 2) * stride4 + 32 + (w & 3)];
+    const float v = taps[t];
     float prod;
     {
 #pragma clang fp contract(off)
@@ -806,13 +803,15 @@ struct FieldTbcParams {
   double out_scale;
   double inv_hz_ire;
   double conv_offset;
+  float inv_out_len;  // 1 / out_len (FP32), the estimate of each sample's output line
 };
 
-// Stage for the cooperative table-row fetch: per wave, 64 pixels x 8 float4 (two
-// rows), padded to 9 float4 per pixel against LDS bank conflicts.  The 64 pad slots
-// (256 floats) hold the wave's source window.
-constexpr int kTbcRowStride = 9;
+// Stage for the cooperative table-row fetch: per wave, 64 pixels x 8 float4 (two rows),
+// slot s of pixel p at p * 8 + (s ^ (p & 7)) so that the 16-byte accesses of eight
+// consecutive pixels fall on distinct banks; then the wave's source window (256 floats).
 constexpr int kTbcWindow = 256;
+constexpr int kTbcStage = 64 * 8 + kTbcWindow / 4;
+__device__ __forceinline__ int tbc_slot(int pix, int s) { return pix * 8 + (s ^ (pix & 7)); }
 
 __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long long src_len, double bias,
                                                          const long long* origins, const double* locs,
@@ -820,7 +819,7 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
                                                          const double* pow_table, FieldTbcParams p,
                                                          const float* __restrict__ lut, unsigned short* out,
                                                          const int* field_count) {
-  __shared__ float4 rows[256 / 64][64 * kTbcRowStride];
+  __shared__ float4 rows[256 / 64][kTbcStage];
   const int f = blockIdx.y;
   if (field_count && f >= *field_count) return;
   const int dest_len = p.out_len * p.lines;
@@ -829,8 +828,17 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
   const bool valid = d0 < dest_len;
   const int d = valid ? d0 : dest_len - 1;
   const double* loc = locs + (long long)f * p.nloc;
-  const int left = d / p.out_len;
-  const int smp = d - left * p.out_len;
+  // output line of sample d: an FP32 estimate (d < 2^24 is exact in FP32, so it is off
+  // by at most one) and one correction
+  int left = (int)((float)d * p.inv_out_len);
+  int smp = d - left * p.out_len;
+  if (smp < 0) {
+    left -= 1;
+    smp += p.out_len;
+  } else if (smp >= p.out_len) {
+    left += 1;
+    smp -= p.out_len;
+  }
   const double ll = loc[left];
   const double delta = loc[left + 1] - ll;
   const double scale = p.nominal / p.out_len;
@@ -862,8 +870,8 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
       const int r0 = __shfl(ph.a0, pix), r1 = __shfl(ph.a1, pix);
       const float4 c0 = lut4[r0 * 4 + quarter];
       const float4 c1 = lut4[r1 * 4 + quarter];
-      stage[pix * kTbcRowStride + quarter] = c0;
-      stage[pix * kTbcRowStride + 4 + quarter] = c1;
+      stage[tbc_slot(pix, quarter)] = c0;
+      stage[tbc_slot(pix, 4 + quarter)] = c1;
     }
   }
   // The wave's 64 pixels take their 16 taps from a few hundred consecutive source
@@ -874,18 +882,18 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
   const long long sp_len = src_len - org;
   const long long s0 = (long long)ph.ci - 7;
   const long long base = __shfl(s0, 0);
-  float* win = reinterpret_cast<float*>(stage);
+  float* win = reinterpret_cast<float*>(stage + 64 * 8);
 #pragma unroll
   for (int i = 0; i < kTbcWindow / 64; i++) {
     const int w = lane + 64 * i;
     const long long g = base + w;
-    win[(w >> 2) * (4 * kTbcRowStride) + 32 + (w & 3)] = (g >= 0 && g < sp_len) ? sp_src[g] : 0.0f;
+    win[w] = (g >= 0 && g < sp_len) ? sp_src[g] : 0.0f;
   }
   __syncthreads();
   float ws[16], we[16];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    const float4 a = stage[lane * kTbcRowStride + q], b = stage[lane * kTbcRowStride + 4 + q];
+    const float4 a = stage[tbc_slot(lane, q)], b = stage[tbc_slot(lane, 4 + q)];
     ws[4 * q] = a.x; ws[4 * q + 1] = a.y; ws[4 * q + 2] = a.z; ws[4 * q + 3] = a.w;
     we[4 * q] = b.x; we[4 * q + 1] = b.y; we[4 * q + 2] = b.z; we[4 * q + 3] = b.w;
   }
@@ -896,7 +904,7 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
     const long long rel = s0 - base;
     const bool in_window = sp_len >= 16 && s0 >= 0 && s0 <= sp_len - 16 && rel >= 0 && rel <= kTbcWindow - 16;
     const double sv = !finite ? 0.0
-                      : in_window ? sinc_sum_window(win, 4 * kTbcRowStride, (int)rel, bias, ph, ws, we)
+                      : in_window ? sinc_sum_window(win + (int)rel, bias, ph, ws, we)
                                   : sinc_sum(sp_src, sp_len, bias, ph, ws, we);
     r = (double)(float)(sv * adj);
   }
@@ -1191,7 +1199,9 @@ hipError_t vhsd_launch_tbc_fields(const float* src, long long src_len, double bi
   if (lines > vhsd::kLevelThreads || nloc < 2 || lines > nloc - 1) return hipErrorInvalidValue;
   hipLaunchKernelGGL(vhsd::tbc_levels_kernel, dim3(fields), dim3(vhsd::kLevelThreads), 0, stream, refined, nloc,
                      lines, nominal, smooth ? line_decay : 0.0, line_x, line_y0, field_count);
-  vhsd::FieldTbcParams p{out_len, lines, nloc, nominal, smooth, ire0, out_scale, inv_hz_ire, conv_offset};
+  vhsd::FieldTbcParams p{out_len,    lines,      nloc,        nominal,
+                         smooth,     ire0,       out_scale,   inv_hz_ire,
+                         conv_offset, 1.0f / (float)out_len};
   const long long dest = (long long)out_len * lines;
   hipLaunchKernelGGL(vhsd::tbc_fields_kernel, dim3((unsigned)((dest + 255) / 256), fields), dim3(256), 0, stream, src,
                      src_len, bias, origins, refined, line_x, line_y0, pow_table, p, lut, out, field_count);
