@@ -46,6 +46,8 @@ enum {
   VHSDECODE_HIP_E_UNSUPPORTED = -30006,
   VHSDECODE_HIP_E_CAPACITY = -30007,
   VHSDECODE_HIP_E_TIMEOUT = -30008,
+  VHSDECODE_HIP_E_CANCELLED = -30009,      /* the run's cancel callback asked to stop */
+  VHSDECODE_HIP_E_IO = -30010,             /* an output file could not be created or written */
   VHSDECODE_HIP_E_INTERNAL = -30099
 };
 
@@ -536,6 +538,57 @@ VHSDECODE_HIP_API int32_t vhsdecode_hip_serve_wait(vhsdecode_hip_serve* server, 
                                                    vhsdecode_hip_window_result_v1* result);
 /* Finishes the windows in flight, drops queued ones, frees every device resource. */
 VHSDECODE_HIP_API void vhsdecode_hip_serve_destroy(vhsdecode_hip_serve* server);
+
+/* ---------------------------------------------------------------- whole run
+ * Replaces vhsdecode_cuda_fast_run (src/VHSDecode.CudaFast.Native/include/vhsdecode_cuda_fast.h:55-67,
+ * 83-106, 153-156; src/VHSDecode.CudaFast.Native/src/vhsdecode_cuda_fast.cpp:238-370): RF read
+ * through the caller's callback in spans, each span decoded on the device (block demod,
+ * pulses, line locations, hsync, TBC and, with `dropouts`, RF-envelope dropout detection),
+ * and the fields written in order to <output_base>.tbc with <output_base>.tbc.json
+ * (videoParameters, then per field isFirstField, seqNo, fileLoc and dropOuts, the layout of
+ * the reference's TBCWriter::write_json, src/VHSDecode.CudaFast.Native/src/tbc_writer.cpp:209-312).
+ * Colour-under chroma is not decoded: no _chroma.tbc is written.
+ * Spans start at multiples of lcm(block length, block stride) (491520 samples for
+ * 1024/1024 cuts), so the blocks two spans share give the same planes and every field's
+ * read-window origin, hence every output sample, is what one span from the start of the
+ * input gives; a field found again in the next span is written once. */
+enum { VHSDECODE_HIP_INPUT_FLOAT32 = 0, VHSDECODE_HIP_INPUT_INT16 = 1 };
+
+/* Reads sample_count samples from sample_offset into destination (int16 or float, as the
+ * run's input_sample_format says); a short count means the end of the input (or a cancel). */
+typedef size_t (*vhsdecode_hip_read_callback)(void* user_data, void* destination, uint64_t sample_offset,
+                                              size_t sample_count);
+/* Nonzero asks the run to stop (checked before every span). */
+typedef int32_t (*vhsdecode_hip_cancel_callback)(void* user_data);
+
+typedef struct vhsdecode_hip_run_config_v1 {
+  uint32_t struct_size;
+  int32_t device_id;
+  const vhsdecode_hip_serve_format_v1* format;  /* filters, block cuts, field and locator params, sinc table */
+  const vhsdecode_hip_dropout_params_v1* dropouts; /* NULL: no dropout detection (the reference's --noDOD) */
+  uint64_t total_samples;
+  uint32_t input_sample_format;    /* VHSDECODE_HIP_INPUT_*; FP32 samples are rounded to int16 */
+  uint32_t maximum_output_fields;  /* 0: every field */
+  int64_t span_samples;            /* RF samples per decode span; 0: twelve fields' worth */
+  const char* output_base_utf8;
+  int32_t overwrite;               /* 0: fail when an output file exists */
+  vhsdecode_hip_read_callback read_callback;
+  vhsdecode_hip_cancel_callback cancel_callback; /* may be NULL */
+  void* user_data;
+} vhsdecode_hip_run_config_v1;
+
+typedef struct vhsdecode_hip_run_result_v1 {
+  uint32_t struct_size;
+  uint32_t fields_written;
+  uint32_t output_line_length;
+  uint32_t output_field_lines;
+  double elapsed_seconds;
+  uint32_t spans;                  /* decode spans the run took */
+  uint32_t dropout_rows;           /* .tbc.json dropout rows written over all fields */
+} vhsdecode_hip_run_result_v1;
+
+VHSDECODE_HIP_API int32_t vhsdecode_hip_run(const vhsdecode_hip_run_config_v1* config,
+                                            vhsdecode_hip_run_result_v1* result);
 
 /* ---------------------------------------------------------------- observability
  * Per-kernel device time, the counterpart of vhsdecode_cuda_fast_result_v1.elapsed_seconds

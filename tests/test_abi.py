@@ -63,6 +63,8 @@ def _c_sizeof(struct):
     ("vhsdecode_hip_batch_segment_v1", _lib.BatchSegmentV1),
     ("vhsdecode_hip_batch_config_v1", _lib.BatchConfigV1),
     ("vhsdecode_hip_dropout_params_v1", _lib.DropoutParamsV1),
+    ("vhsdecode_hip_run_config_v1", _lib.RunConfigV1),
+    ("vhsdecode_hip_run_result_v1", _lib.RunResultV1),
 ])
 def test_struct_layouts_match_header(name, cls):
     assert _c_sizeof(name) == ctypes.sizeof(cls)

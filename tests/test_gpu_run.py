@@ -1,0 +1,105 @@
+"""The whole run (vhsdecode_hip_run) against one decode of the whole tape as a single span
+(demod_span_device + decode_span_fields_device): the run reads the tape through its
+callback in spans of four fields on one block grid, and must write the same fields, in
+order, with the same parity and line-0 positions, and the .tbc.json that describes them.
+Runs on an MI355X only."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from vhsdecode_hip import field, filters, formats, hipmem, synth
+from vhsdecode_hip.demod import HipBlockDemodulator
+from vhsdecode_hip.run import decode_run
+
+pytestmark = pytest.mark.gpu
+
+N = 32768
+CUT = 1024
+STRIDE = N - 2 * CUT
+
+
+def _lut():
+    import vhsdecode_hip
+    return np.load(os.path.join(os.path.dirname(vhsdecode_hip.__file__), "data", "sinc_lut.npz"))["downscale_sinc_lut"]
+
+
+@pytest.fixture(scope="module", params=[("NTSC", 9, 31), ("PAL", 7, 32)], ids=["ntsc_vhs", "pal_vhs"])
+def tape(request):
+    system, nfields, seed = request.param
+    fmt = formats.get_format(system, "VHS", "sp")
+    fl = filters.build_tape_filters(fmt, 40e6, N)
+    t = synth.generate(fmt, fields=nfields, seed=seed, snr_db=30.0, jitter_ns=20.0)
+    lut = _lut()
+    dem = HipBlockDemodulator(fl, max_blocks=256)
+    dem.set_sinc_table(lut)
+    nb = dem.blocks_for(t.rf.size)
+    n = nb * STRIDE
+    rf = hipmem.DeviceBuffer.from_array(t.rf)
+    planes = hipmem.DeviceBuffer(4 * n * 4)
+    flags = hipmem.DeviceBuffer(nb * 4)
+    video, vlp, env, chroma = (planes.ptr + k * n * 4 for k in range(4))
+    dem.demod_span_device(rf.ptr, 0, nb, video, vlp, env, chroma, flags.ptr)
+    dec = field.HipFieldDecoder(dem, fmt)
+    g = dec.geom
+    mf = nfields + 2
+    per = g.field_lines * g.out_len
+    out = hipmem.DeviceBuffer(mf * per * 2)
+    meta = hipmem.DeviceBuffer(8 + 12 * mf)
+    dec.decode_async(video, vlp, n, out.ptr, mf, meta.ptr, meta.ptr + 8, meta.ptr + 8 + 8 * mf)
+    hipmem.synchronize()
+    count = int(meta.download(np.int32, 1)[0])
+    line0 = meta.download(np.float64, mf, offset=8)[:count]
+    first = meta.download(np.int32, mf, offset=8 + 8 * mf)[:count]
+    fields = out.download(np.uint16, mf * per).reshape(mf, per)[:count]
+    line_len = field.locator(g).line_length
+    field_len = line_len * (g.field_lines - 0.5)
+    print(f"\n[{system}] tape {t.rf.size} samples, whole-span fields {count}", flush=True)
+    return dict(system=system, fmt=fmt, fl=fl, rf=t.rf, lut=lut, count=count, line0=line0, first=first,
+                fields=fields, per=per, span=int(4 * field_len) + N)
+
+
+def test_run_writes_the_whole_span_fields(tape, tmp_path):
+    base = str(tmp_path / "capture")
+    res = decode_run(tape["rf"], tape["fmt"], tape["fl"], base, sinc_table=tape["lut"], span_samples=tape["span"])
+    data = np.fromfile(base + ".tbc", dtype="<u2")
+    doc = json.load(open(base + ".tbc.json"))
+    print(f"[{tape['system']}] run: {res.fields_written} fields in {res.spans} spans, {res.elapsed_seconds:.3f} s",
+          flush=True)
+    assert res.spans >= 2
+    assert res.fields_written == tape["count"] == data.size // tape["per"]
+    assert data.size == res.fields_written * tape["per"]
+    vp = doc["videoParameters"]
+    assert vp["numberOfSequentialFields"] == res.fields_written
+    assert vp["fieldWidth"] * vp["fieldHeight"] == tape["per"]
+    assert vp["system"] == tape["system"] and vp["isSourcePal"] == (tape["system"] == "PAL")
+    recs = doc["fields"]
+    assert [r["seqNo"] for r in recs] == list(range(1, res.fields_written + 1))
+    assert [r["isFirstField"] for r in recs] == [bool(x) for x in tape["first"]]
+    np.testing.assert_array_equal([r["fileLoc"] for r in recs], np.floor(CUT + tape["line0"]).astype(np.int64))
+    for r in recs:
+        assert set(r["dropOuts"]) == {"fieldLine", "startx", "endx"}
+    got = data.reshape(-1, tape["per"])
+    diff = np.abs(got.astype(np.int32) - tape["fields"].astype(np.int32))
+    print(f"[{tape['system']}] samples differing {np.count_nonzero(diff)} of {diff.size}, max {diff.max()}",
+          flush=True)
+    lines = diff.reshape(diff.shape[0], -1, vp["fieldWidth"]).max(axis=2)
+    for f in range(diff.shape[0]):
+        bad = np.nonzero(lines[f])[0]
+        print(f"  field {f} line0 {tape['line0'][f]:.3f}: {bad.size} lines differ"
+              + (f" ({bad[:6].tolist()} .. {bad[-3:].tolist()}), max {lines[f].max()}" if bad.size else ""), flush=True)
+    # the spans share the whole-span decode's block grid, so every plane sample is the same
+    np.testing.assert_array_equal(got, tape["fields"])
+
+
+def test_run_float32_input_and_field_limit(tape, tmp_path):
+    a, b = str(tmp_path / "i16"), str(tmp_path / "f32")
+    decode_run(tape["rf"], tape["fmt"], tape["fl"], a, sinc_table=tape["lut"], span_samples=tape["span"])
+    res = decode_run(tape["rf"].astype(np.float32), tape["fmt"], tape["fl"], b, sinc_table=tape["lut"],
+                     span_samples=tape["span"], maximum_output_fields=2)
+    assert res.fields_written == 2
+    ia = np.fromfile(a + ".tbc", dtype="<u2")
+    ib = np.fromfile(b + ".tbc", dtype="<u2")
+    np.testing.assert_array_equal(ib, ia[:ib.size])
+    assert json.load(open(b + ".tbc.json"))["videoParameters"]["numberOfSequentialFields"] == 2

@@ -48,6 +48,7 @@ REQUIRED_EXPORTS = (
     "vhsdecode_hip_hifi_destroy",
     "vhsdecode_hip_hifi_demodulate_device",
     "vhsdecode_hip_hifi_demodulate_host",
+    "vhsdecode_hip_run",
     "vhsdecode_hip_field_locations_device",
     "vhsdecode_hip_detect_dropouts_device",
 )
@@ -183,6 +184,27 @@ class ServeFormatV1(ctypes.Structure):
                 ("sinc_table", _fp), ("sinc_count", ctypes.c_int64)]
 
 
+READ_CALLBACK = ctypes.CFUNCTYPE(ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_size_t)
+CANCEL_CALLBACK = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p)
+INPUT_FLOAT32, INPUT_INT16 = 0, 1
+
+
+class RunConfigV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("device_id", ctypes.c_int32),
+                ("format", ctypes.POINTER(ServeFormatV1)), ("dropouts", ctypes.POINTER(DropoutParamsV1)),
+                ("total_samples", ctypes.c_uint64), ("input_sample_format", ctypes.c_uint32),
+                ("maximum_output_fields", ctypes.c_uint32), ("span_samples", ctypes.c_int64),
+                ("output_base_utf8", ctypes.c_char_p), ("overwrite", ctypes.c_int32),
+                ("read_callback", READ_CALLBACK), ("cancel_callback", CANCEL_CALLBACK),
+                ("user_data", ctypes.c_void_p)]
+
+
+class RunResultV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("fields_written", ctypes.c_uint32),
+                ("output_line_length", ctypes.c_uint32), ("output_field_lines", ctypes.c_uint32),
+                ("elapsed_seconds", ctypes.c_double), ("spans", ctypes.c_uint32), ("dropout_rows", ctypes.c_uint32)]
+
+
 class WindowV1(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("stream_id", ctypes.c_int32), ("first_sample", ctypes.c_int64),
                 ("sample_count", ctypes.c_int64), ("max_fields", ctypes.c_int32), ("fields_out", ctypes.c_void_p)]
@@ -284,6 +306,7 @@ def load():
     L.vhsdecode_hip_hifi_demodulate_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                        ctypes.c_int32, ctypes.c_void_p]
     L.vhsdecode_hip_hifi_demodulate_host.argtypes = [ctypes.c_void_p, _fp, _fp, ctypes.c_int32]
+    L.vhsdecode_hip_run.argtypes = [ctypes.POINTER(RunConfigV1), ctypes.POINTER(RunResultV1)]
     for name in REQUIRED_EXPORTS:
         fn = getattr(L, name)
         if name not in ("vhsdecode_hip_get_last_error", "vhsdecode_hip_demod_destroy"):
