@@ -1,7 +1,8 @@
 """The whole run (vhsdecode_hip_run) against one decode of the whole tape as a single span
-(demod_span_device + decode_span_fields_device): the run reads the tape through its
-callback in spans of four fields on one block grid, and must write the same fields, in
-order, with the same parity and line-0 positions, and the .tbc.json that describes them.
+(demod_span_device + decode_span_fields_device + detect_dropouts_device) on tapes with RF
+dropouts: the run reads the tape through its callback in spans of five fields, and must
+write the same fields, in order, with the same parity, line-0 positions and dropout rows,
+and the .tbc.json that describes them.
 Runs on an MI355X only."""
 import json
 import os
@@ -31,12 +32,18 @@ def tape(request):
     fmt = formats.get_format(system, "VHS", "sp")
     fl = filters.build_tape_filters(fmt, 40e6, N)
     t = synth.generate(fmt, fields=nfields, seed=seed, snr_db=30.0, jitter_ns=20.0)
+    rf_host = t.rf.copy()
+    rng = np.random.default_rng(seed)
+    for _ in range(12):  # RF dropouts: stretches of low-level noise
+        at = int(rng.integers(400_000, rf_host.size - 400_000))
+        ln = int(rng.integers(40, 3000))
+        rf_host[at:at + ln] = np.rint(rng.normal(0.0, 40.0, ln)).astype(np.int16)
     lut = _lut()
     dem = HipBlockDemodulator(fl, max_blocks=256)
     dem.set_sinc_table(lut)
-    nb = dem.blocks_for(t.rf.size)
+    nb = dem.blocks_for(rf_host.size)
     n = nb * STRIDE
-    rf = hipmem.DeviceBuffer.from_array(t.rf)
+    rf = hipmem.DeviceBuffer.from_array(rf_host)
     planes = hipmem.DeviceBuffer(4 * n * 4)
     flags = hipmem.DeviceBuffer(nb * 4)
     video, vlp, env, chroma = (planes.ptr + k * n * 4 for k in range(4))
@@ -53,11 +60,25 @@ def tape(request):
     line0 = meta.download(np.float64, mf, offset=8)[:count]
     first = meta.download(np.int32, mf, offset=8 + 8 * mf)[:count]
     fields = out.download(np.uint16, mf * per).reshape(mf, per)[:count]
+    # the whole span's dropout rows, the parameters decode_run uses
+    cap = 1024
+    dp = field.dropout_params(fmt, 40e6, capacity=cap)
+    loc_ptr, nloc = dem.field_locations_device()
+    drop = hipmem.DeviceBuffer(4 * mf * (1 + 3 * cap))
+    rows = drop.ptr + 4 * mf
+    dem.detect_dropouts_device(env, n, loc_ptr, nloc, meta.ptr + 8, meta.ptr + 8 + 8 * mf, meta.ptr, mf, dp, drop.ptr,
+                               rows, rows + 4 * mf * cap, rows + 8 * mf * cap)
+    hipmem.synchronize()
+    allr = drop.download(np.int32, mf * (1 + 3 * cap))
+    nrow = allr[:mf]
+    planes3 = allr[mf:].reshape(3, mf, cap)
+    drops = [tuple(planes3[k, f, :nrow[f]].tolist() for k in range(3)) for f in range(count)]
     line_len = field.locator(g).line_length
     field_len = line_len * (g.field_lines - 0.5)
     print(f"\n[{system}] tape {t.rf.size} samples, whole-span fields {count}", flush=True)
-    return dict(system=system, fmt=fmt, fl=fl, rf=t.rf, lut=lut, count=count, line0=line0, first=first,
-                fields=fields, per=per, span=int(4 * field_len) + N)
+    print(f"[{system}] whole-span dropout rows per field {nrow[:count].tolist()}", flush=True)
+    return dict(system=system, fmt=fmt, fl=fl, rf=rf_host, lut=lut, count=count, line0=line0, first=first,
+                fields=fields, per=per, drops=drops, span=int(5 * field_len) + N)
 
 
 def test_run_writes_the_whole_span_fields(tape, tmp_path):
@@ -78,8 +99,10 @@ def test_run_writes_the_whole_span_fields(tape, tmp_path):
     assert [r["seqNo"] for r in recs] == list(range(1, res.fields_written + 1))
     assert [r["isFirstField"] for r in recs] == [bool(x) for x in tape["first"]]
     np.testing.assert_array_equal([r["fileLoc"] for r in recs], np.floor(CUT + tape["line0"]).astype(np.int64))
-    for r in recs:
-        assert set(r["dropOuts"]) == {"fieldLine", "startx", "endx"}
+    for r, (fl_, sx, ex) in zip(recs, tape["drops"]):
+        d = r["dropOuts"]
+        assert (d["fieldLine"], d["startx"], d["endx"]) == (fl_, sx, ex)
+    assert res.dropout_rows == sum(len(x[0]) for x in tape["drops"]) > 0
     got = data.reshape(-1, tape["per"])
     diff = np.abs(got.astype(np.int32) - tape["fields"].astype(np.int32))
     print(f"[{tape['system']}] samples differing {np.count_nonzero(diff)} of {diff.size}, max {diff.max()}",

@@ -133,6 +133,17 @@ bool write_json(const std::string& path, const vhsdecode_hip_field_params_v1& fp
   return std::rename(tmp.c_str(), path.c_str()) == 0;
 }
 
+// End of a field's dropout read window in span coordinates before the span clamps it: the
+// window the dropout kernels place (DecodeReadWindowPlanner.Resolve,
+// Decode/DecodeReadWindowPlanner.cs:37-63), read_samples from the block-aligned start,
+// rounded up past a whole stride.
+int64_t dropout_window_end(const vhsdecode_hip_dropout_params_v1& d, double line0) {
+  const double intended = std::max(0.0, line0 - d.lead_lines * d.line_length);
+  const int64_t read = std::max<int64_t>(0, (int64_t)intended - d.block_cut);
+  const int64_t aligned = read / d.block_length * d.block_length;
+  return ((aligned + d.read_samples) / d.block_stride + 1) * d.block_stride;
+}
+
 int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_result_v1& res) {
   const vhsdecode_hip_serve_format_v1& fm = *cfg.format;
   const vhsdecode_hip_field_params_v1& fp = *fm.field;
@@ -148,8 +159,12 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
   const int64_t align = lz.block_length > 0 ? lz.block_length : kBlock;
   const int64_t grid = align / std::gcd(align, (int64_t)stride) * stride;
   const int64_t step = grid <= (int64_t)field_len ? grid : stride;
-  const int64_t min_span = (int64_t)std::ceil(3.0 * field_len) + step + kBlock;
-  const int64_t span_req = cfg.span_samples > 0 ? cfg.span_samples : (int64_t)std::ceil(12.0 * field_len) + step + kBlock;
+  // with dropout detection a field is written from a span that holds its whole read
+  // window (the envelope mean runs over it), so a span also holds a window and two fields
+  const int64_t window = cfg.dropouts ? cfg.dropouts->read_samples + 2 * (int64_t)stride : 0;
+  const int64_t min_span =
+      std::max((int64_t)std::ceil(3.0 * field_len), window + (int64_t)std::ceil(2.0 * field_len)) + step + kBlock;
+  const int64_t span_req = cfg.span_samples > 0 ? cfg.span_samples : std::max(min_span, (int64_t)std::ceil(12.0 * field_len) + step + kBlock);
   if (span_req < min_span)
     return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT,
                      "span_samples must hold three fields and one step of the span grid");
@@ -161,7 +176,9 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
   const int32_t max_fields = (int32_t)std::ceil((double)plane_max / field_len) + 2;
   const size_t per_field = (size_t)fp.field_lines * fp.output_line_length;
   const int32_t cap = cfg.dropouts ? cfg.dropouts->capacity : 0;
-  if (cfg.dropouts && cap <= 0) return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "dropout capacity must be positive");
+  if (cfg.dropouts && (cap <= 0 || cfg.dropouts->read_samples <= 0 || cfg.dropouts->block_length <= 0 ||
+                       cfg.dropouts->block_stride <= 0))
+    return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "dropout capacity, read window and block geometry must be positive");
 
   const std::string base(cfg.output_base_utf8);
   const std::string tbc_path = base + ".tbc", json_path = base + ".tbc.json";
@@ -297,11 +314,15 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
       RUN_HIP(hipMemcpy(h_fields.p, d_fields.p, sizeof(uint16_t) * per_field * (size_t)count,
                         hipMemcpyDeviceToHost));
     spans++;
-    double last_found = -1.0;
+    double resume = -1.0;  // line 0 (RF position) of a field left for the next span
+    bool wrote = false;
     for (int32_t f = 0; f < count; f++) {
       const double abs0 = (double)pos + fm.block_cut + h_line0[f];
-      last_found = abs0;
       if (last_abs >= 0.0 && abs0 < last_abs + 0.5 * field_len) continue;  // found again
+      if (cfg.dropouts && !last_span && dropout_window_end(*cfg.dropouts, h_line0[f]) > plane) {
+        resume = abs0;  // its read window leaves the span: the next span writes it
+        break;
+      }
       FieldRecord r;
       r.first = h_first[f] != 0;
       r.file_loc = (uint64_t)std::max(0.0, std::floor(abs0));
@@ -320,16 +341,19 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
         return set_error(VHSDECODE_HIP_E_IO, "cannot write " + tbc_path);
       written.push_back(std::move(r));
       last_abs = abs0;
+      wrote = true;
       if (cfg.maximum_output_fields && written.size() >= cfg.maximum_output_fields) {
         done = true;
         break;
       }
     }
     if (last_span) break;
-    // next span: on the span grid, half a field plus the lead after the last field found
-    // (or, with none found, keeping three fields of overlap)
-    const double target = last_found >= 0.0 ? last_found + 0.5 * field_len - lead - fm.block_cut
-                                            : (double)(pos + plane) - 3.0 * field_len;
+    // next span: on the span grid, half a field plus the lead before a field left for it,
+    // else half a field plus the lead after the last field written, else (none found)
+    // keeping three fields of overlap
+    const double target = resume >= 0.0 ? resume - 0.5 * field_len - lead - fm.block_cut
+                          : wrote       ? last_abs + 0.5 * field_len - lead - fm.block_cut
+                                        : (double)(pos + plane) - 3.0 * field_len;
     int64_t next = (int64_t)std::floor(std::max(0.0, target) / (double)step) * step;
     if (next <= pos) next = pos + step;
     pos = next;
