@@ -293,6 +293,31 @@ def batched_run(seed: int, segments: int, steps: int):
             "scope": "block demod chain (demod, repair, video, sub-deemphasis, chroma); no TBC (segments are not fields)"}
 
 
+def whole_run(fmt, tape, fl, lut, copies: int, dropouts: bool):
+    """The whole-run entry (vhsdecode_hip_run) over `copies` back-to-back copies of the
+    tape held in host RAM: callback reads, PCIe upload, device decode of each span, .tbc and
+    .tbc.json written to a temporary directory.  End to end, so never the headline value."""
+    import shutil
+    import tempfile
+    from vhsdecode_hip.run import decode_run
+    rf = np.tile(tape.rf, copies)
+    tmp = tempfile.mkdtemp(prefix="vhsd_run_")
+    try:
+        base = os.path.join(tmp, "capture")
+        decode_run(rf[: 4 * (rf.size // copies)], fmt, fl, base, sinc_table=lut, dropouts=dropouts)  # warm-up
+        t0 = time.perf_counter()
+        res = decode_run(rf, fmt, fl, base, sinc_table=lut, dropouts=dropouts)
+        wall = time.perf_counter() - t0
+        tbc_bytes = os.path.getsize(base + ".tbc")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return {"samples": int(rf.size), "fields_written": res.fields_written, "spans": res.spans,
+            "dropout_rows": res.dropout_rows, "seconds": round(wall, 4),
+            "msps": round(rf.size / wall / 1e6, 1), "tbc_bytes": tbc_bytes,
+            "path": "read callback from host RAM -> pinned staging -> PCIe -> device decode per span -> "
+                    ".tbc/.tbc.json on local disk; one span at a time"}
+
+
 def laserdisc_run(seed: int, fields: int, steps: int, lut):
     """Synthetic NTSC LaserDisc at 40 MSPS int16 (7.6 -> 9.3 MHz video FM, analog sound
     carriers at 2.30/2.81 MHz with +-100 kHz deviation and 1 kHz / 3 kHz tones, 30 dB):
@@ -388,6 +413,8 @@ def main():
                     help="spans in flight: consecutive steps alternate between this many pipelines")
     ap.add_argument("--batch-segments", type=int, default=64, help="batched-segment run (0: skip)")
     ap.add_argument("--ld-fields", type=int, default=20, help="LaserDisc + HiFi run (0: skip)")
+    ap.add_argument("--run-copies", type=int, default=6,
+                    help="whole-run entry over this many copies of the tape from host RAM (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -540,6 +567,11 @@ def main():
         result["batched"] = bt
         print(f"[bench] batched: {bt['segments']} segments / {bt['blocks']} blocks in {bt['batched_ms']} ms "
               f"({bt['batched_msps']} MSPS) vs plain {bt['plain_ms']} ms ({bt['plain_msps']} MSPS)", file=sys.stderr)
+    if args.run_copies > 0:
+        wr = whole_run(fmt, tape, fl, lut, args.run_copies, dropouts is not None)
+        result["whole_run"] = wr
+        print(f"[bench] whole run: {wr['fields_written']} fields in {wr['spans']} spans, {wr['seconds']} s "
+              f"({wr['msps']} MSPS end to end)", file=sys.stderr)
     if args.ld_fields > 0:
         ldr = laserdisc_run(SEED, args.ld_fields, steps=5, lut=lut)
         result["laserdisc"] = ldr
