@@ -548,10 +548,13 @@ VHSDECODE_HIP_API void vhsdecode_hip_serve_destroy(vhsdecode_hip_serve* server);
  * (videoParameters, then per field isFirstField, seqNo, fileLoc and dropOuts, the layout of
  * the reference's TBCWriter::write_json, src/VHSDecode.CudaFast.Native/src/tbc_writer.cpp:209-312).
  * Colour-under chroma is not decoded: no _chroma.tbc is written.
- * Spans start at multiples of lcm(block length, block stride) (491520 samples for
- * 1024/1024 cuts), so the blocks two spans share give the same planes and every field's
- * read-window origin, hence every output sample, is what one span from the start of the
- * input gives; a field found again in the next span is written once. */
+ * Spans sit at fixed places on multiples of lcm(block length, block stride) (491520
+ * samples for 1024/1024 cuts) and overlap by a field and a half plus the read-window lead
+ * (plus a dropout read window with detection on), so the blocks two spans share give the
+ * same planes and every field's read-window origin, hence every output sample, is what
+ * one span from the start of the input gives; a field found again is written once.  A
+ * prefetch thread reads the next span through the callback while the device decodes the
+ * current one: the callbacks must be callable from a thread other than the caller's. */
 enum { VHSDECODE_HIP_INPUT_FLOAT32 = 0, VHSDECODE_HIP_INPUT_INT16 = 1 };
 
 /* Reads sample_count samples from sample_offset into destination (int16 or float, as the
@@ -569,7 +572,8 @@ typedef struct vhsdecode_hip_run_config_v1 {
   uint64_t total_samples;
   uint32_t input_sample_format;    /* VHSDECODE_HIP_INPUT_*; FP32 samples are rounded to int16 */
   uint32_t maximum_output_fields;  /* 0: every field */
-  int64_t span_samples;            /* RF samples per decode span; 0: twelve fields' worth */
+  int64_t span_samples;            /* RF samples per decode span (at least the overlap and one grid step);
+                                      0: twelve fields' worth */
   const char* output_base_utf8;
   int32_t overwrite;               /* 0: fail when an output file exists */
   vhsdecode_hip_read_callback read_callback;

@@ -5,17 +5,18 @@
 // and, when asked, vhsdecode_hip_detect_dropouts_device) on one stream, and the fields are
 // appended in order to <base>.tbc; <base>.tbc.json is written at the end.
 //
-// Span placement: every span starts at a multiple of lcm(block length, block stride)
-// (491520 samples for 1024/1024 cuts).  Plane sample i of a span starting at RF sample p
-// is RF sample p + cut + i, so the blocks two spans share are the same blocks, and each
-// field's read-window origin (aligned down to a block length, then to a stride, in span
-// coordinates) lands where it lands in one span from the start of the input: the TBC's
-// FP32 source coordinates, which count from that origin, are then the same, and so is
-// every output sample.  (A format whose lcm exceeds a field steps by the stride instead.)
-// The next span starts half a field (plus the read-window lead) after the line 0 of the
-// last field found, rounded down to that grid, so the field after it is complete in the
-// next span; a field whose line 0 lies within half a field of the last one written is the
-// same field found again and is skipped.
+// Span placement: span k starts at RF sample k * advance, a multiple of lcm(block length,
+// block stride) (491520 samples for 1024/1024 cuts), and consecutive spans overlap by a
+// field and a half plus the read-window lead (plus a dropout read window when detection is
+// on).  Plane sample i of a span starting at RF sample p is RF sample p + cut + i, so the
+// blocks two spans share are the same blocks, and each field's read-window origin (aligned
+// down to a block length, then to a stride, in span coordinates) lands where it lands in
+// one span from the start of the input: the TBC's FP32 source coordinates, which count from
+// that origin, are then the same, and so is every output sample.  (A format whose lcm
+// exceeds a field steps by the stride instead.)  A field whose line 0 lies within half a
+// field of the last one written is the same field found again and is skipped.  Because the
+// places are fixed, a prefetch thread reads span k + 1 through the callback while the
+// device decodes span k (the callbacks run on that thread for every span but the first).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,6 +24,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <future>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -159,21 +161,28 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
   const int64_t align = lz.block_length > 0 ? lz.block_length : kBlock;
   const int64_t grid = align / std::gcd(align, (int64_t)stride) * stride;
   const int64_t step = grid <= (int64_t)field_len ? grid : stride;
-  // with dropout detection a field is written from a span that holds its whole read
-  // window (the envelope mean runs over it), so a span also holds a window and two fields
+  // Spans sit at fixed places, span k at k * advance, so the next span's RF can be read
+  // while the device decodes the current one.  Consecutive spans overlap by a field and a
+  // half plus the read-window lead (and, with dropout detection, a whole read window: the
+  // envelope mean runs over it) plus a step of the grid, so every field lies whole, with
+  // its window, inside one span.
+  const double lead = lz.lead_lines * lz.line_length;
   const int64_t window = cfg.dropouts ? cfg.dropouts->read_samples + 2 * (int64_t)stride : 0;
-  const int64_t min_span =
-      std::max((int64_t)std::ceil(3.0 * field_len), window + (int64_t)std::ceil(2.0 * field_len)) + step + kBlock;
-  const int64_t span_req = cfg.span_samples > 0 ? cfg.span_samples : std::max(min_span, (int64_t)std::ceil(12.0 * field_len) + step + kBlock);
+  const int64_t overlap = (int64_t)std::ceil(1.5 * field_len + lead) + fm.block_cut + window + step;
+  const int64_t min_span = overlap + step + stride + kBlock;
+  const int64_t span_req =
+      cfg.span_samples > 0 ? cfg.span_samples : std::max(min_span, (int64_t)std::ceil(12.0 * field_len) + kBlock);
   if (span_req < min_span)
     return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT,
-                     "span_samples must hold three fields and one step of the span grid");
+                     "span_samples must hold the span overlap (a field and a half, the lead and any dropout window) "
+                     "and one step of the span grid");
   if (span_req > (int64_t)INT32_MAX)
     return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "span_samples above 2^31 - 1");
   const int32_t nb_max = (int32_t)((span_req - kBlock) / stride + 1);
   const int64_t read_max = (int64_t)(nb_max - 1) * stride + kBlock;
   const int64_t plane_max = (int64_t)nb_max * stride;
   const int32_t max_fields = (int32_t)std::ceil((double)plane_max / field_len) + 2;
+  const int64_t advance = (plane_max - overlap) / step * step;
   const size_t per_field = (size_t)fp.field_lines * fp.output_line_length;
   const int32_t cap = cfg.dropouts ? cfg.dropouts->capacity : 0;
   if (cfg.dropouts && (cap <= 0 || cfg.dropouts->read_samples <= 0 || cfg.dropouts->block_length <= 0 ||
@@ -218,13 +227,14 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
   // dropouts: row counts, then field line / start x / end x, [max_fields][cap] each
   const size_t drop_bytes = sizeof(int32_t) * (size_t)max_fields * (1 + 3 * (size_t)cap);
   if (cfg.dropouts) RUN_HIP(hipMalloc(&d_drop.p, drop_bytes));
-  PinnedBuf h_rf, h_fields, h_meta, h_drop;
-  RUN_HIP(hipHostMalloc(&h_rf.p, sizeof(int16_t) * (size_t)read_max, hipHostMallocDefault));
+  PinnedBuf h_rf[2], h_fields, h_meta, h_drop;
+  for (PinnedBuf& b : h_rf) RUN_HIP(hipHostMalloc(&b.p, sizeof(int16_t) * (size_t)read_max, hipHostMallocDefault));
   RUN_HIP(hipHostMalloc(&h_fields.p, sizeof(uint16_t) * per_field * (size_t)max_fields, hipHostMallocDefault));
   RUN_HIP(hipHostMalloc(&h_meta.p, meta_bytes, hipHostMallocDefault));
   if (cfg.dropouts) RUN_HIP(hipHostMalloc(&h_drop.p, drop_bytes, hipHostMallocDefault));
-  std::vector<float> staging;
-  if (cfg.input_sample_format == VHSDECODE_HIP_INPUT_FLOAT32) staging.resize((size_t)read_max);
+  std::vector<float> staging[2];
+  if (cfg.input_sample_format == VHSDECODE_HIP_INPUT_FLOAT32)
+    for (std::vector<float>& v : staging) v.resize((size_t)read_max);
 
   OwnedFile tbc;
   tbc.f = std::fopen(tbc_path.c_str(), "wb");
@@ -245,43 +255,66 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
   std::vector<FieldRecord> written;
   double last_abs = -1.0;  // RF position of the last written field's line 0
   const uint64_t total = cfg.total_samples;
-  const double lead = lz.lead_lines * lz.line_length;
-  int64_t pos = 0;
   uint32_t spans = 0, drop_rows = 0;
   bool done = false;
-  const auto t0 = std::chrono::steady_clock::now();
-  while (!done) {
-    if (cfg.cancel_callback && cfg.cancel_callback(cfg.user_data))
-      return set_error(VHSDECODE_HIP_E_CANCELLED, "the run was cancelled");
-    if ((uint64_t)pos + kBlock > total) break;
-    int64_t avail = (int64_t)std::min<uint64_t>(total - (uint64_t)pos, (uint64_t)read_max);
+
+  // One span's RF into a pinned buffer (FP32 input rounded to int16); runs on the prefetch
+  // thread for every span but the first.  `last`: no whole block of input after this span.
+  struct SpanRead {
+    int64_t pos = 0;
+    int32_t nb = 0;
+    int64_t n_read = 0;
+    bool last = true;
+    bool ok = false;
+  };
+  auto read_span = [&](int64_t p, int slot) -> SpanRead {
+    SpanRead r;
+    r.pos = p;
+    if ((uint64_t)p + kBlock > total) return r;
+    const int64_t avail = (int64_t)std::min<uint64_t>(total - (uint64_t)p, (uint64_t)read_max);
     int32_t nb = (int32_t)((avail - kBlock) / stride + 1);
-    int64_t n_read = (int64_t)(nb - 1) * stride + kBlock;
-    bool last_span = (uint64_t)pos + (uint64_t)n_read + stride > total;
+    int64_t n = (int64_t)(nb - 1) * stride + kBlock;
+    bool last = (uint64_t)p + (uint64_t)n + stride > total;
+    int16_t* dst = static_cast<int16_t*>(h_rf[slot].p);
     size_t got;
     if (cfg.input_sample_format == VHSDECODE_HIP_INPUT_INT16) {
-      got = cfg.read_callback(cfg.user_data, h_rf.p, (uint64_t)pos, (size_t)n_read);
+      got = cfg.read_callback(cfg.user_data, dst, (uint64_t)p, (size_t)n);
     } else {
-      got = cfg.read_callback(cfg.user_data, staging.data(), (uint64_t)pos, (size_t)n_read);
-      int16_t* dst = static_cast<int16_t*>(h_rf.p);
-      const size_t m = std::min(got, (size_t)n_read);
-      for (size_t i = 0; i < m; i++) {
-        const float v = staging[i];
-        const float c = std::isfinite(v) ? std::min(32767.0f, std::max(-32768.0f, std::nearbyint(v))) : 0.0f;
-        dst[i] = (int16_t)c;
+      std::vector<float>& stage = staging[slot];
+      got = cfg.read_callback(cfg.user_data, stage.data(), (uint64_t)p, (size_t)n);
+      const size_t m = std::min(got, (size_t)n);
+      for (size_t q = 0; q < m; q++) {
+        const float v = stage[q];
+        dst[q] = (int16_t)(std::isfinite(v) ? std::min(32767.0f, std::max(-32768.0f, std::nearbyint(v))) : 0.0f);
       }
     }
-    if (got < (size_t)n_read) {
-      // a short read is the end of the input (or a cancel, checked above and next round)
-      if (cfg.cancel_callback && cfg.cancel_callback(cfg.user_data))
-        return set_error(VHSDECODE_HIP_E_CANCELLED, "the run was cancelled");
-      if (got < (size_t)kBlock) break;
+    if (got < (size_t)n) {  // a short read is the end of the input (or a cancel)
+      if (got < (size_t)kBlock) return r;
       nb = (int32_t)(((int64_t)got - kBlock) / stride + 1);
-      n_read = (int64_t)(nb - 1) * stride + kBlock;
-      last_span = true;
+      n = (int64_t)(nb - 1) * stride + kBlock;
+      last = true;
     }
+    r.nb = nb;
+    r.n_read = n;
+    r.last = last;
+    r.ok = true;
+    return r;
+  };
+
+  const auto t0 = std::chrono::steady_clock::now();
+  SpanRead cur = read_span(0, 0);
+  for (int k = 0; cur.ok && !done; k++) {
+    if (cfg.cancel_callback && cfg.cancel_callback(cfg.user_data))
+      return set_error(VHSDECODE_HIP_E_CANCELLED, "the run was cancelled");
+    // the next span's read overlaps this span's device work (the future's destructor
+    // waits for it on every exit path, before the buffers go)
+    std::future<SpanRead> next;
+    if (!cur.last) next = std::async(std::launch::async, read_span, cur.pos + advance, (k + 1) & 1);
+    const int64_t pos = cur.pos;
+    const int32_t nb = cur.nb;
     const int64_t plane = (int64_t)nb * stride;
-    RUN_HIP(hipMemcpyAsync(d_rf.p, h_rf.p, sizeof(int16_t) * (size_t)n_read, hipMemcpyHostToDevice, st.s));
+    RUN_HIP(hipMemcpyAsync(d_rf.p, h_rf[k & 1].p, sizeof(int16_t) * (size_t)cur.n_read, hipMemcpyHostToDevice,
+                           st.s));
     float* planes = static_cast<float*>(d_planes.p);
     vhsdecode_hip_span_v1 sp{};
     sp.struct_size = sizeof(sp);
@@ -311,18 +344,16 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
     RUN_HIP(hipStreamSynchronize(st.s));
     const int32_t count = std::min(*h_count, max_fields);
     if (count > 0)
-      RUN_HIP(hipMemcpy(h_fields.p, d_fields.p, sizeof(uint16_t) * per_field * (size_t)count,
-                        hipMemcpyDeviceToHost));
+      RUN_HIP(hipMemcpyAsync(h_fields.p, d_fields.p, sizeof(uint16_t) * per_field * (size_t)count,
+                             hipMemcpyDeviceToHost, st.s));
+    RUN_HIP(hipStreamSynchronize(st.s));
     spans++;
-    double resume = -1.0;  // line 0 (RF position) of a field left for the next span
-    bool wrote = false;
     for (int32_t f = 0; f < count; f++) {
       const double abs0 = (double)pos + fm.block_cut + h_line0[f];
-      if (last_abs >= 0.0 && abs0 < last_abs + 0.5 * field_len) continue;  // found again
-      if (cfg.dropouts && !last_span && dropout_window_end(*cfg.dropouts, h_line0[f]) > plane) {
-        resume = abs0;  // its read window leaves the span: the next span writes it
-        break;
-      }
+      if (last_abs >= 0.0 && abs0 < last_abs + 0.5 * field_len) continue;  // written from the last span
+      // a field whose read window leaves the span is written from the next span, which
+      // holds it whole; so are the fields after it
+      if (cfg.dropouts && !cur.last && dropout_window_end(*cfg.dropouts, h_line0[f]) > plane) break;
       FieldRecord r;
       r.first = h_first[f] != 0;
       r.file_loc = (uint64_t)std::max(0.0, std::floor(abs0));
@@ -341,22 +372,13 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
         return set_error(VHSDECODE_HIP_E_IO, "cannot write " + tbc_path);
       written.push_back(std::move(r));
       last_abs = abs0;
-      wrote = true;
       if (cfg.maximum_output_fields && written.size() >= cfg.maximum_output_fields) {
         done = true;
         break;
       }
     }
-    if (last_span) break;
-    // next span: on the span grid, half a field plus the lead before a field left for it,
-    // else half a field plus the lead after the last field written, else (none found)
-    // keeping three fields of overlap
-    const double target = resume >= 0.0 ? resume - 0.5 * field_len - lead - fm.block_cut
-                          : wrote       ? last_abs + 0.5 * field_len - lead - fm.block_cut
-                                        : (double)(pos + plane) - 3.0 * field_len;
-    int64_t next = (int64_t)std::floor(std::max(0.0, target) / (double)step) * step;
-    if (next <= pos) next = pos + step;
-    pos = next;
+    if (cur.last || done) break;
+    cur = next.get();
   }
   const auto t1 = std::chrono::steady_clock::now();
   if (std::fflush(tbc.f) != 0) return set_error(VHSDECODE_HIP_E_IO, "cannot write " + tbc_path);
