@@ -313,7 +313,9 @@ def whole_run(fmt, tape, fl, lut, copies: int, dropouts: bool):
         shutil.rmtree(tmp, ignore_errors=True)
     return {"samples": int(rf.size), "fields_written": res.fields_written, "spans": res.spans,
             "dropout_rows": res.dropout_rows, "seconds": round(wall, 4),
-            "msps": round(rf.size / wall / 1e6, 1), "tbc_bytes": tbc_bytes,
+            "msps": round(rf.size / wall / 1e6, 1),
+            "loop_seconds": round(res.elapsed_seconds, 4),
+            "loop_msps": round(rf.size / res.elapsed_seconds / 1e6, 1), "tbc_bytes": tbc_bytes,
             "path": "read callback from host RAM -> pinned staging -> PCIe -> device decode per span -> "
                     ".tbc/.tbc.json on local disk; one span at a time"}
 
@@ -413,7 +415,7 @@ def main():
                     help="spans in flight: consecutive steps alternate between this many pipelines")
     ap.add_argument("--batch-segments", type=int, default=64, help="batched-segment run (0: skip)")
     ap.add_argument("--ld-fields", type=int, default=20, help="LaserDisc + HiFi run (0: skip)")
-    ap.add_argument("--run-copies", type=int, default=6,
+    ap.add_argument("--run-copies", type=int, default=18,
                     help="whole-run entry over this many copies of the tape from host RAM (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (counter calibration)")
@@ -571,7 +573,7 @@ def main():
         wr = whole_run(fmt, tape, fl, lut, args.run_copies, dropouts is not None)
         result["whole_run"] = wr
         print(f"[bench] whole run: {wr['fields_written']} fields in {wr['spans']} spans, {wr['seconds']} s "
-              f"({wr['msps']} MSPS end to end)", file=sys.stderr)
+              f"({wr['msps']} MSPS end to end with set-up; span loop {wr['loop_msps']} MSPS)", file=sys.stderr)
     if args.ld_fields > 0:
         ldr = laserdisc_run(SEED, args.ld_fields, steps=5, lut=lut)
         result["laserdisc"] = ldr

@@ -573,7 +573,7 @@ typedef struct vhsdecode_hip_run_config_v1 {
   uint32_t input_sample_format;    /* VHSDECODE_HIP_INPUT_*; FP32 samples are rounded to int16 */
   uint32_t maximum_output_fields;  /* 0: every field */
   int64_t span_samples;            /* RF samples per decode span (at least the overlap and one grid step);
-                                      0: twelve fields' worth */
+                                      0: twenty-four fields' worth */
   const char* output_base_utf8;
   int32_t overwrite;               /* 0: fail when an output file exists */
   vhsdecode_hip_read_callback read_callback;

@@ -171,7 +171,7 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
   const int64_t overlap = (int64_t)std::ceil(1.5 * field_len + lead) + fm.block_cut + window + step;
   const int64_t min_span = overlap + step + stride + kBlock;
   const int64_t span_req =
-      cfg.span_samples > 0 ? cfg.span_samples : std::max(min_span, (int64_t)std::ceil(12.0 * field_len) + kBlock);
+      cfg.span_samples > 0 ? cfg.span_samples : std::max(min_span, (int64_t)std::ceil(24.0 * field_len) + kBlock);
   if (span_req < min_span)
     return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT,
                      "span_samples must hold the span overlap (a field and a half, the lead and any dropout window) "
