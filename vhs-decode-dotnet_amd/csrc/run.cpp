@@ -227,9 +227,10 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
   // dropouts: row counts, then field line / start x / end x, [max_fields][cap] each
   const size_t drop_bytes = sizeof(int32_t) * (size_t)max_fields * (1 + 3 * (size_t)cap);
   if (cfg.dropouts) RUN_HIP(hipMalloc(&d_drop.p, drop_bytes));
-  PinnedBuf h_rf[2], h_fields, h_meta, h_drop;
+  PinnedBuf h_rf[2], h_fields[2], h_meta, h_drop;
   for (PinnedBuf& b : h_rf) RUN_HIP(hipHostMalloc(&b.p, sizeof(int16_t) * (size_t)read_max, hipHostMallocDefault));
-  RUN_HIP(hipHostMalloc(&h_fields.p, sizeof(uint16_t) * per_field * (size_t)max_fields, hipHostMallocDefault));
+  for (PinnedBuf& b : h_fields)
+    RUN_HIP(hipHostMalloc(&b.p, sizeof(uint16_t) * per_field * (size_t)max_fields, hipHostMallocDefault));
   RUN_HIP(hipHostMalloc(&h_meta.p, meta_bytes, hipHostMallocDefault));
   if (cfg.dropouts) RUN_HIP(hipHostMalloc(&h_drop.p, drop_bytes, hipHostMallocDefault));
   std::vector<float> staging[2];
@@ -301,6 +302,17 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
     return r;
   };
 
+  // .tbc writes of one span's fields on a writer thread, one job in flight, so the file
+  // keeps the field order; a job writes fields [first, first + n) of a staging slot.
+  std::future<bool> write_job;
+  auto write_fields = [&](int slot, std::vector<int32_t> which) -> bool {
+    const uint16_t* base_p = static_cast<const uint16_t*>(h_fields[slot].p);
+    for (const int32_t f : which)
+      if (std::fwrite(base_p + (size_t)f * per_field, sizeof(uint16_t), per_field, tbc.f) != per_field) return false;
+    return true;
+  };
+  auto finish_writes = [&]() -> bool { return !write_job.valid() || write_job.get(); };
+
   const auto t0 = std::chrono::steady_clock::now();
   SpanRead cur = read_span(0, 0);
   for (int k = 0; cur.ok && !done; k++) {
@@ -343,11 +355,13 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
     RUN_HIP(hipMemcpyAsync(h_meta.p, d_meta.p, meta_bytes, hipMemcpyDeviceToHost, st.s));
     RUN_HIP(hipStreamSynchronize(st.s));
     const int32_t count = std::min(*h_count, max_fields);
+    // slot k & 1 was last written from two spans ago; that job has finished (one in flight)
     if (count > 0)
-      RUN_HIP(hipMemcpyAsync(h_fields.p, d_fields.p, sizeof(uint16_t) * per_field * (size_t)count,
+      RUN_HIP(hipMemcpyAsync(h_fields[k & 1].p, d_fields.p, sizeof(uint16_t) * per_field * (size_t)count,
                              hipMemcpyDeviceToHost, st.s));
     RUN_HIP(hipStreamSynchronize(st.s));
     spans++;
+    std::vector<int32_t> to_write;
     for (int32_t f = 0; f < count; f++) {
       const double abs0 = (double)pos + fm.block_cut + h_line0[f];
       if (last_abs >= 0.0 && abs0 < last_abs + 0.5 * field_len) continue;  // written from the last span
@@ -367,9 +381,7 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
         r.end.assign(fl + 2 * rows, fl + 2 * rows + n);
         drop_rows += (uint32_t)n;
       }
-      const uint16_t* src = static_cast<const uint16_t*>(h_fields.p) + (size_t)f * per_field;
-      if (std::fwrite(src, sizeof(uint16_t), per_field, tbc.f) != per_field)
-        return set_error(VHSDECODE_HIP_E_IO, "cannot write " + tbc_path);
+      to_write.push_back(f);
       written.push_back(std::move(r));
       last_abs = abs0;
       if (cfg.maximum_output_fields && written.size() >= cfg.maximum_output_fields) {
@@ -377,9 +389,12 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
         break;
       }
     }
+    if (!finish_writes()) return set_error(VHSDECODE_HIP_E_IO, "cannot write " + tbc_path);
+    if (!to_write.empty()) write_job = std::async(std::launch::async, write_fields, k & 1, std::move(to_write));
     if (cur.last || done) break;
     cur = next.get();
   }
+  if (!finish_writes()) return set_error(VHSDECODE_HIP_E_IO, "cannot write " + tbc_path);
   const auto t1 = std::chrono::steady_clock::now();
   if (std::fflush(tbc.f) != 0) return set_error(VHSDECODE_HIP_E_IO, "cannot write " + tbc_path);
   const double out_rate = fp.output_line_length * (fp.sample_rate_mhz * 1e6 / lz.line_length);
