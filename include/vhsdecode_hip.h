@@ -283,6 +283,26 @@ VHSDECODE_HIP_API int32_t vhsdecode_hip_find_pulses(vhsdecode_hip_demod_context*
                                                     int32_t max_length, int32_t* starts, int32_t* lengths,
                                                     int32_t capacity, int32_t* count);
 
+/* The same pulse search with no allocation and no context.  This entry point has no
+ * counterpart in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  The kernels keep their tile
+ * counts and edge list in a file-scope device array of the library (one per device,
+ * ~4 MiB, held until the process ends); sync_ref (a span plane holding offsets from
+ * plane_bias_hz), starts, lengths and counts are device memory; asynchronous on `stream`.
+ * counts[0] receives the number of pulse edges, counts[1] the number of pulses (only
+ * min(counts[1], capacity) are stored).  If counts[0] exceeds
+ * VHSDECODE_HIP_NOALLOC_MAX_EDGES the edge list was cut there and the pulse list is
+ * incomplete.  Lengths above VHSDECODE_HIP_NOALLOC_MAX_SAMPLES are rejected with
+ * VHSDECODE_HIP_E_CAPACITY.  Every call on a device shares that device's array: calls
+ * that overlap on one device (different streams or host threads) race. */
+#define VHSDECODE_HIP_NOALLOC_MAX_SAMPLES (16LL * 1024 * 1024)
+#define VHSDECODE_HIP_NOALLOC_MAX_EDGES (448 * 1024)
+VHSDECODE_HIP_API int32_t vhsdecode_hip_find_pulses_noalloc(const float* sync_ref, int64_t length,
+                                                            double plane_bias_hz, double threshold,
+                                                            int32_t min_length, int32_t max_length, int32_t* starts,
+                                                            int32_t* lengths, int32_t capacity, int32_t* counts,
+                                                            void* stream);
+
 /* ---------------------------------------------------------------- hsync refinement
  * TbcFieldDecodePipeline.RefineLineLocationsFromHSync, vhs decode type
  * (Decode/TbcFieldDecodePipeline.cs:4570-4684 with :4807-5052), FP64 on the device. */

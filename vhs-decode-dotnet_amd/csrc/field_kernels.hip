@@ -1237,3 +1237,82 @@ hipError_t vhsd_launch_tbc(const float* src, long long src_len, double bias, con
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ pulses without allocation
+// vhsdecode_hip_find_pulses_noalloc: the pulse kernels above with their tile counts and
+// edge list in a file-scope device array instead of context memory, for callers that
+// cannot allocate (code under graph capture, tight serving loops).  The array is one
+// per device; its device address comes from hipGetSymbolAddress once per device.
+#include <mutex>
+
+#include "host_common.hpp"
+#include "vhsdecode_hip.h"
+
+namespace {
+constexpr size_t kNoallocTileBytes = (size_t)(VHSDECODE_HIP_NOALLOC_MAX_SAMPLES / vhsd::kPulseTile) * sizeof(int);
+constexpr size_t kNoallocEdgePosBytes = (size_t)VHSDECODE_HIP_NOALLOC_MAX_EDGES * sizeof(long long);
+constexpr size_t kNoallocEdgeKindBytes = (size_t)VHSDECODE_HIP_NOALLOC_MAX_EDGES;
+constexpr size_t kNoallocScratchBytes = kNoallocTileBytes + kNoallocEdgePosBytes + kNoallocEdgeKindBytes;
+static_assert(VHSDECODE_HIP_NOALLOC_MAX_SAMPLES % vhsd::kPulseTile == 0, "whole tiles");
+static_assert(kNoallocTileBytes % 256 == 0 && kNoallocEdgePosBytes % 256 == 0, "aligned sub-buffers");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoallocMaxDevices] = {};
+
+// The device address of `scratch` on the current device (looked up once per device).
+hipError_t noalloc_scratch(int device, unsigned char** out) {
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_noalloc_ptr[device] = p;
+  }
+  *out = static_cast<unsigned char*>(g_noalloc_ptr[device]);
+  return hipSuccess;
+}
+}  // namespace
+
+extern "C" int32_t vhsdecode_hip_find_pulses_noalloc(const float* sync_ref, int64_t length, double plane_bias_hz,
+                                                     double threshold, int32_t min_length, int32_t max_length,
+                                                     int32_t* starts, int32_t* lengths, int32_t capacity,
+                                                     int32_t* counts, void* stream) {
+  if (length < 0 || length > VHSDECODE_HIP_NOALLOC_MAX_SAMPLES)
+    return vhsd_host::set_error(VHSDECODE_HIP_E_CAPACITY,
+                                "find_pulses_noalloc: length " + std::to_string(length) + " exceeds the maximum of " +
+                                    std::to_string((long long)VHSDECODE_HIP_NOALLOC_MAX_SAMPLES) + " samples");
+  if ((!sync_ref && length > 0) || !counts || capacity < 0 || ((!starts || !lengths) && capacity > 0))
+    return vhsd_host::set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "find_pulses_noalloc: invalid argument");
+  int device = 0;
+  hipError_t e = hipGetDevice(&device);
+  if (e == hipSuccess && (device < 0 || device >= kNoallocMaxDevices)) e = hipErrorInvalidDevice;
+  unsigned char* base = nullptr;
+  if (e == hipSuccess) e = noalloc_scratch(device, &base);
+  if (e != hipSuccess)
+    return vhsd_host::set_error(VHSDECODE_HIP_E_HIP, std::string("find_pulses_noalloc: ") + hipGetErrorString(e));
+  int* tile_counts = reinterpret_cast<int*>(base);
+  long long* edge_pos = reinterpret_cast<long long*>(base + kNoallocTileBytes);
+  unsigned char* edge_kind = base + kNoallocTileBytes + kNoallocEdgePosBytes;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int tiles = (int)((length + vhsd::kPulseTile - 1) / vhsd::kPulseTile);
+  // Every launch rewrites what the next one reads (tile counts, edges, both counters), so
+  // nothing is cleared between calls.  With no tiles the pair kernel alone writes zeros.
+  if (tiles > 0) {
+    hipLaunchKernelGGL(vhsd::pulse_count_kernel, dim3(tiles), dim3(vhsd::kPulseThreads), 0, st, sync_ref, length,
+                       plane_bias_hz, threshold, tile_counts);
+    hipLaunchKernelGGL(vhsd::pulse_write_kernel, dim3(tiles), dim3(vhsd::kPulseThreads), 0, st, sync_ref, length,
+                       plane_bias_hz, threshold, tile_counts, edge_pos, edge_kind,
+                       (int)VHSDECODE_HIP_NOALLOC_MAX_EDGES);
+  }
+  hipLaunchKernelGGL(vhsd::pulse_pair_kernel, dim3(1), dim3(1024), 0, st, edge_pos, edge_kind, tile_counts, tiles,
+                     (int)VHSDECODE_HIP_NOALLOC_MAX_EDGES, min_length, max_length, starts, lengths, capacity, counts);
+  e = hipGetLastError();
+  if (e != hipSuccess)
+    return vhsd_host::set_error(VHSDECODE_HIP_E_HIP, std::string("find_pulses_noalloc: ") + hipGetErrorString(e));
+  return VHSDECODE_HIP_OK;
+}

@@ -14,6 +14,8 @@ _PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.environ.get("VHSDECODE_HIP_LIBRARY", os.path.join(_PKG_ROOT, "lib", "libvhsdecode_hip.so"))
 
 ABI_VERSION = 1
+NOALLOC_MAX_SAMPLES = 16 * 1024 * 1024
+NOALLOC_MAX_EDGES = 448 * 1024
 
 REQUIRED_EXPORTS = (
     "vhsdecode_hip_get_abi_version",
@@ -51,6 +53,7 @@ REQUIRED_EXPORTS = (
     "vhsdecode_hip_run",
     "vhsdecode_hip_field_locations_device",
     "vhsdecode_hip_detect_dropouts_device",
+    "vhsdecode_hip_find_pulses_noalloc",
 )
 
 KERNEL_NAMES = ("block_demod", "spike_repair", "chroma", "pulses", "hsync", "tbc", "sub_deemphasis", "block_video",
@@ -253,6 +256,10 @@ def load():
                                                 ctypes.c_int32, ctypes.POINTER(SpanV1)]
     L.vhsdecode_hip_find_pulses.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_double,
                                             ctypes.c_int32, ctypes.c_int32, _i32p, _i32p, ctypes.c_int32, _i32p]
+    L.vhsdecode_hip_find_pulses_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+                                                    ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                                    ctypes.c_void_p]
     L.vhsdecode_hip_refine_hsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, _dp, _u8p,
                                              ctypes.c_int32, ctypes.POINTER(HsyncParamsV1), _dp, _u8p]
     L.vhsdecode_hip_set_sinc_table.argtypes = [ctypes.c_void_p, _fp, ctypes.c_int64]
