@@ -367,3 +367,327 @@ def field_read_window(line0: float, lead_lines: float, line_len: float, requeste
     intended = int(max(0.0, line0 - lead_lines * line_len))
     start, length = resolve_read_window(intended, requested, block_length, block_cut, stride)
     return min(start, span_len), min(start + length, span_len)
+
+
+# ---------------------------------------------------------------- VHS field sync sequence (sync.cpp)
+class SyncConfig(ctypes.Structure):
+    _fields_ = [("sample_rate_hz", ctypes.c_double), ("is_pal", ctypes.c_int32),
+                ("line_period_us", ctypes.c_double), ("hsync_us", ctypes.c_double), ("eq_us", ctypes.c_double),
+                ("vsync_us", ctypes.c_double), ("num_pulses", ctypes.c_int32),
+                ("hsync_tolerance_us", ctypes.c_double), ("eq_tolerance_us", ctypes.c_double),
+                ("ire0", ctypes.c_double), ("hz_ire", ctypes.c_double), ("vsync_ire", ctypes.c_double),
+                ("fps", ctypes.c_double), ("frame_lines", ctypes.c_double), ("output_lines", ctypes.c_int32),
+                ("block_length", ctypes.c_int64), ("block_cut", ctypes.c_int64), ("block_stride", ctypes.c_int64),
+                ("read_samples", ctypes.c_int64), ("level_divisor", ctypes.c_int32),
+                ("serration_mode", ctypes.c_int32), ("tf_order", ctypes.c_int32 * 4),
+                ("tf_b", (ctypes.c_double * 21) * 4), ("tf_a", (ctypes.c_double * 21) * 4),
+                ("sos_sections", ctypes.c_int32 * 4), ("sos", ((ctypes.c_double * 6) * 10) * 4)]
+
+
+class SyncField(ctypes.Structure):
+    _fields_ = [("begin", ctypes.c_int64), ("window_start", ctypes.c_int64), ("window_length", ctypes.c_int64),
+                ("status", ctypes.c_int32), ("level_source", ctypes.c_int32), ("sync_level", ctypes.c_double),
+                ("blank_level", ctypes.c_double), ("dc_offset", ctypes.c_double), ("threshold", ctypes.c_double),
+                ("serration_candidates", ctypes.c_int32), ("serration_measurements", ctypes.c_int32),
+                ("raw_pulses", ctypes.c_int32), ("refined_pulses", ctypes.c_int32),
+                ("hsync_median", ctypes.c_double), ("mean_line_length", ctypes.c_double),
+                ("is_first_field", ctypes.c_int32), ("parity_confidence", ctypes.c_int32),
+                ("line0", ctypes.c_double), ("first_hsync", ctypes.c_double), ("first_hsync_line", ctypes.c_double),
+                ("processed_lines", ctypes.c_int32), ("next_offset", ctypes.c_double), ("next_begin", ctypes.c_int64)]
+
+
+def _sync_lib():
+    L = lib()
+    if not getattr(L, "_sync_bound", False):
+        L.oracle_sync_config_size.restype = ctypes.c_int
+        L.oracle_sync_field_size.restype = ctypes.c_int
+        assert L.oracle_sync_config_size() == ctypes.sizeof(SyncConfig)
+        assert L.oracle_sync_field_size() == ctypes.sizeof(SyncField)
+        L.oracle_vhs_field_sequence.argtypes = [_dp, ctypes.c_int64, ctypes.POINTER(SyncConfig), ctypes.c_int64,
+                                                ctypes.c_int, ctypes.c_int, ctypes.POINTER(SyncField), _dp, _dp,
+                                                ctypes.c_int64]
+        L.oracle_vhs_field_sequence.restype = ctypes.c_int
+        L.oracle_serration_levels.argtypes = [_dp, ctypes.c_int64, _dp, _dp]
+        L.oracle_serration_levels.restype = ctypes.c_int
+        for name in ("oracle_mean_f64", "oracle_mean_f64_fastmath", "oracle_median_f64"):
+            getattr(L, name).argtypes = [_dp, ctypes.c_int64]
+            getattr(L, name).restype = ctypes.c_double
+        L.oracle_filtfilt_tf.argtypes = [_dp, _dp, ctypes.c_int, _dp, ctypes.c_int64]
+        L.oracle_filtfilt_sos.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int64]
+        L.oracle_refine_serration_levels.argtypes = [_dp, ctypes.c_int64, ctypes.POINTER(SyncConfig),
+                                                     ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                     ctypes.c_double, _dp, _dp, _ip, _ip]
+        L.oracle_refine_serration_levels.restype = ctypes.c_int
+        L.oracle_serration_analyze.argtypes = [_dp, ctypes.c_int64, ctypes.POINTER(SyncConfig), _ip, _ip, _ip,
+                                               ctypes.c_int, _ip, _dp]
+        L.oracle_serration_measure.argtypes = [_dp, ctypes.c_int64, ctypes.POINTER(SyncConfig), ctypes.c_int, _dp]
+        L.oracle_serration_measure.restype = ctypes.c_int
+        L.oracle_sync_timing.argtypes = [_dp, _ip, _ip, ctypes.c_int, ctypes.c_int, _dp]
+        L.oracle_sync_classify.argtypes = [_dp, _ip, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
+                                           ctypes.c_int64, ctypes.c_double, _ip, _ip, _ip, _ip, ctypes.c_int]
+        L.oracle_sync_classify.restype = ctypes.c_int
+        L.oracle_sync_mean_line_length.argtypes = [_dp, _ip, _ip, _ip, _ip, ctypes.c_int]
+        L.oracle_sync_mean_line_length.restype = ctypes.c_double
+        L.oracle_sync_upstream_lines.argtypes = [_ip, _ip, _ip, _ip, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                                 ctypes.c_double, ctypes.c_int, _dp]
+        L.oracle_sync_resolve_cadence.argtypes = [_ip, _ip, _ip, _ip, ctypes.c_int, ctypes.c_double] + \
+            [ctypes.c_int] * 9 + [_ip]
+        L.oracle_sync_resolve_cadence.restype = ctypes.c_int
+        L.oracle_sync_line0_transitions.argtypes = [_ip, _ip, _ip, _ip, ctypes.c_int, ctypes.c_double] + \
+            [ctypes.c_int] * 5 + [_dp]
+        L.oracle_sync_next_field.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double] * 3 + [ctypes.c_int]
+        L.oracle_sync_next_field.restype = ctypes.c_double
+        L._sync_bound = True
+    return L
+
+
+def butterworth_serration_filter(fs: float, cutoff: float, transition: float, highpass: bool):
+    """VsyncSerrationDetector.DesignFilter (Dsp/VsyncSerrationDetector.cs:582-603) with the
+    order rule of IirFilterDesign.ButterworthLowPassOrder (Dsp/IirFilterDesign.cs:441-469):
+    (order, b, a, sos).  The transfer function comes from scipy.signal.butter, whose
+    zpk -> bilinear -> polynomial steps the reference ports."""
+    import math
+
+    import scipy.signal as ss
+    nyq = fs / 2.0
+    wp = cutoff / nyq
+    ws = (cutoff + transition) / nyq
+    warped_pass = math.tan(math.pi * wp / 2.0)
+    warped_stop = math.tan(math.pi * ws / 2.0)
+    ratio = warped_stop / warped_pass
+    gpass = math.pow(10.0, 0.1 * 3.0) - 1.0
+    gstop = math.pow(10.0, 0.1 * 30.0) - 1.0
+    order = int(math.ceil(math.log10(gstop / gpass) / (2.0 * math.log10(ratio))))
+    natural = math.pow(gpass, -1.0 / (2.0 * order)) * warped_pass
+    wn = math.atan(natural) * 2.0 / math.pi
+    order = min(order, 20)
+    btype = "highpass" if highpass else "lowpass"
+    b, a = ss.butter(order, wn, btype=btype, output="ba")
+    sos = ss.butter(order, wn, btype=btype, output="sos")
+    return order, np.asarray(b, dtype=np.float64), np.asarray(a, dtype=np.float64), np.asarray(sos, dtype=np.float64)
+
+
+SERRATION_REFERENCE, SERRATION_SOS, SERRATION_NONE = 0, 1, 2
+
+
+def sync_config(fmt, sample_rate_hz: float = 40e6, block_length: int = 32768, block_cut: int = 1024,
+                stride: int = 30720, serration_mode: int = SERRATION_REFERENCE, level_divisor: int = 1) -> SyncConfig:
+    """The vhs command's sync configuration of a tape format (SyncAnalyzer.FromParameters with the
+    vhs tolerances 0.7/0.9 us, DecodeSession.cs:420-428; VsyncSerrationDetector of
+    TbcFieldDecodePipeline.BuildVsyncSerrationDetector, :788-806; VideoOutputConverter levels)."""
+    sp = fmt.sysparams
+    is_pal = fmt.parent_system == "PAL"
+    c = SyncConfig()
+    c.sample_rate_hz = sample_rate_hz
+    c.is_pal = 1 if is_pal else 0
+    c.line_period_us = float(sp["line_period"])
+    c.hsync_us, c.eq_us, c.vsync_us = float(sp["hsyncPulseUS"]), float(sp["eqPulseUS"]), float(sp["vsyncPulseUS"])
+    c.num_pulses = int(sp["numPulses"])
+    c.hsync_tolerance_us, c.eq_tolerance_us = 0.7, 0.9
+    c.ire0 = float(fmt.rfparams["ire0"]) if "ire0" in fmt.rfparams else float(sp["ire0"])
+    c.hz_ire, c.vsync_ire = float(sp["hz_ire"]), float(sp["vsync_ire"])
+    c.fps, c.frame_lines = float(sp["FPS"]), float(sp["frame_lines"])
+    c.output_lines = 313 if is_pal else 263
+    c.block_length, c.block_cut, c.block_stride = block_length, block_cut, stride
+    c.read_samples = estimate_read_samples(c.line_period_us, sample_rate_hz, is_pal, block_length)
+    c.level_divisor = level_divisor
+    c.serration_mode = serration_mode
+    fsr = sample_rate_hz / level_divisor
+    vf, hf = c.fps * 2.0, c.fps * c.frame_lines
+    specs = [(vf * 5.0, 1000.0, False), (hf, hf, True), (hf, hf, False), (hf / 3.0, hf / 2.0, False)]
+    for k, (cut, tw, hp) in enumerate(specs):
+        order, b, a, sos = butterworth_serration_filter(fsr, cut, tw, hp)
+        c.tf_order[k] = order
+        for i in range(order + 1):
+            c.tf_b[k][i] = b[i]
+            c.tf_a[k][i] = a[i]
+        c.sos_sections[k] = sos.shape[0]
+        for s_ in range(sos.shape[0]):
+            for j in range(6):
+                c.sos[k][s_][j] = sos[s_, j]
+    return c
+
+
+def vhs_field_sequence(vlp: np.ndarray, cfg: SyncConfig, first_begin: int = 0, max_fields: int = 64,
+                       want_shifted: bool = False):
+    """The reference's VHS field sequence over a span of FP64 sync reference (global sample 0 on the
+    stride grid): per field the read window, levels, threshold, parity, line 0 and the line
+    locations (window-relative).  Returns (fields, locations[nfields][max_lines], shifted or None)."""
+    L = _sync_lib()
+    x = np.ascontiguousarray(vlp, dtype=np.float64)
+    max_lines = int(cfg.output_lines) + 10
+    fields = (SyncField * max_fields)()
+    locs = np.zeros((max_fields, max_lines), dtype=np.float64)
+    shifted = None
+    stride_s = 0
+    if want_shifted:
+        stride_s = int(cfg.read_samples) + 2 * int(cfg.block_stride)
+        shifted = np.zeros((max_fields, stride_s), dtype=np.float64)
+    n = L.oracle_vhs_field_sequence(_p(x, _dp), x.size, ctypes.byref(cfg), int(first_begin), max_fields, max_lines,
+                                    fields, _p(locs, _dp), _p(shifted, _dp) if shifted is not None else None,
+                                    stride_s)
+    return list(fields[:n]), locs[:n], (shifted[:n] if shifted is not None else None)
+
+
+def serration_levels(serration: np.ndarray):
+    L = _sync_lib()
+    x = np.ascontiguousarray(serration, dtype=np.float64)
+    s, b = ctypes.c_double(), ctypes.c_double()
+    ok = L.oracle_serration_levels(_p(x, _dp), x.size, ctypes.byref(s), ctypes.byref(b))
+    return (s.value, b.value) if ok else None
+
+
+def refine_serration_levels(data: np.ndarray, cfg: SyncConfig, initial_sync: float, initial_blank: float,
+                            reference_sync: float, hz_ire: float):
+    L = _sync_lib()
+    x = np.ascontiguousarray(data, dtype=np.float64)
+    s, b = ctypes.c_double(), ctypes.c_double()
+    vc, pc = ctypes.c_int(), ctypes.c_int()
+    rc = L.oracle_refine_serration_levels(_p(x, _dp), x.size, ctypes.byref(cfg), initial_sync, initial_blank,
+                                          reference_sync, hz_ire, ctypes.byref(s), ctypes.byref(b), ctypes.byref(vc),
+                                          ctypes.byref(pc))
+    return (rc, s.value, b.value, vc.value, pc.value)
+
+
+# ---- SyncAnalyzer / FieldParityDetector / VBlankSyncResolver pieces (sync.cpp)
+HSYNC, EQUALIZING, VSYNC, EQUALIZING_SECOND = 0, 1, 2, 3
+
+
+def analyzer(sample_rate_hz, line_period_us, hsync_us, eq_us, vsync_us, num_pulses=6, hsync_tol=0.5, eq_tol=0.5):
+    """The arguments of the SyncAnalyzer constructor (Dsp/SyncAnalysis.cs:35-66)."""
+    return np.array([sample_rate_hz, line_period_us, hsync_us, eq_us, vsync_us, num_pulses, hsync_tol, eq_tol],
+                    dtype=np.float64)
+
+
+def _pulse_arrays(pulses):
+    """pulses: [(kind, start, length, in_order)] -> four int32 arrays."""
+    a = np.array([[p[0], p[1], p[2], 1 if p[3] else 0] for p in pulses], dtype=np.int32).reshape(-1, 4)
+    return [np.ascontiguousarray(a[:, k]) for k in range(4)]
+
+
+def sync_timing(an, pulses, preserve_units=False):
+    """EstimateTiming over raw (start, length) pulses: dict of the SyncTiming fields."""
+    L = _sync_lib()
+    st = np.ascontiguousarray([p[0] for p in pulses], dtype=np.int32)
+    ln = np.ascontiguousarray([p[1] for p in pulses], dtype=np.int32)
+    out = np.zeros(8)
+    L.oracle_sync_timing(_p(an, _dp), _p(st, _ip), _p(ln, _ip), st.size, 1 if preserve_units else 0, _p(out, _dp))
+    keys = ("hsync_median", "hsync_lo", "hsync_hi", "eq_lo", "eq_hi", "vsync_lo", "vsync_hi", "nominal")
+    return dict(zip(keys, out.tolist()))
+
+
+def sync_classify(an, pulses, refine=False, preserve_units=False, reference=None, rescue_step=0.0):
+    """ClassifyPulses / RefinePulses: [(kind, start, length, in_order)]."""
+    L = _sync_lib()
+    st = np.ascontiguousarray([p[0] for p in pulses], dtype=np.int32)
+    ln = np.ascontiguousarray([p[1] for p in pulses], dtype=np.int32)
+    cap = 4 * max(1, st.size) + 64
+    k, s, l, o = (np.zeros(cap, dtype=np.int32) for _ in range(4))
+    ref = None if reference is None else np.ascontiguousarray(reference, dtype=np.float64)
+    n = L.oracle_sync_classify(_p(an, _dp), _p(st, _ip), _p(ln, _ip), st.size, 1 if preserve_units else 0,
+                               1 if refine else 0, _p(ref, _dp) if ref is not None else None,
+                               0 if ref is None else ref.size, rescue_step, _p(k, _ip), _p(s, _ip), _p(l, _ip),
+                               _p(o, _ip), cap)
+    return [(int(k[i]), int(s[i]), int(l[i]), bool(o[i])) for i in range(min(n, cap))]
+
+
+def sync_mean_line_length(an, pulses):
+    a = _pulse_arrays(pulses)
+    return float(_sync_lib().oracle_sync_mean_line_length(_p(an, _dp), *[_p(x, _ip) for x in a], a[0].size))
+
+
+def sync_upstream_lines(pulses, reference_pulse, reference_line, mean_line_length, lines):
+    a = _pulse_arrays(pulses)
+    out = np.zeros(lines)
+    _sync_lib().oracle_sync_upstream_lines(*[_p(x, _ip) for x in a], a[0].size, reference_pulse, reference_line,
+                                           mean_line_length, lines, _p(out, _dp))
+    return out
+
+
+def _tri(v):
+    return -1 if v is None else (1 if v else 0)
+
+
+def sync_resolve_cadence(pulses, mean_line_length, is_pal, field_lines, previous_first=None, minimum_confidence=100,
+                         has_previous_hsync=False, has_fallback_line=False, fallback_first=None,
+                         fallback_confidence=-1):
+    a = _pulse_arrays(pulses)
+    conf = ctypes.c_int()
+    first = _sync_lib().oracle_sync_resolve_cadence(
+        *[_p(x, _ip) for x in a], a[0].size, mean_line_length, 1 if is_pal else 0, field_lines[0], field_lines[1],
+        _tri(previous_first), minimum_confidence, 1 if has_previous_hsync else 0, 1 if has_fallback_line else 0,
+        _tri(fallback_first), fallback_confidence, ctypes.byref(conf))
+    return bool(first), conf.value
+
+
+def sync_line0_transitions(pulses, mean_line_length, is_pal, num_pulses, first_field, current_lines, first_lines):
+    """EstimateLine0FromTransitions: (first, last, combined), each None or
+    dict(line0, first_hsync, first_hsync_line, valid_count)."""
+    a = _pulse_arrays(pulses)
+    out = np.zeros(15)
+    _sync_lib().oracle_sync_line0_transitions(*[_p(x, _ip) for x in a], a[0].size, mean_line_length,
+                                              1 if is_pal else 0, num_pulses, 1 if first_field else 0, current_lines,
+                                              first_lines, _p(out, _dp))
+    res = []
+    for k in range(3):
+        r = out[5 * k:5 * k + 5]
+        res.append(None if r[0] == 0 else dict(line0=r[1], first_hsync=r[2], first_hsync_line=r[3],
+                                                valid_count=int(r[4])))
+    return tuple(res)
+
+
+def sync_next_field(is_pal, num_pulses, first_field, current_lines, mean_line_length, first_hsync, nominal=0.0,
+                    cut=0):
+    return float(_sync_lib().oracle_sync_next_field(1 if is_pal else 0, num_pulses, 1 if first_field else 0,
+                                                    current_lines, mean_line_length, first_hsync, nominal, cut))
+
+
+def serration_analyze(data: np.ndarray, cfg: SyncConfig, cap: int = 1 << 16):
+    """VsyncSerrationDetector.Analyze on a fresh detector: dict of the minima lists,
+    candidates, measurement count and first measured levels."""
+    L = _sync_lib()
+    x = np.ascontiguousarray(data, dtype=np.float64)
+    e, h, c = (np.zeros(cap, dtype=np.int32) for _ in range(3))
+    counts = np.zeros(4, dtype=np.int32)
+    lv = np.zeros(2)
+    L.oracle_serration_analyze(_p(x, _dp), x.size, ctypes.byref(cfg), _p(e, _ip), _p(h, _ip), _p(c, _ip), cap,
+                               _p(counts, _ip), _p(lv, _dp))
+    return dict(envelope_minima=e[:counts[0]].copy(), harmonic_minima=h[:counts[1]].copy(),
+                candidates=c[:counts[2]].copy(), measurements=int(counts[3]), levels=tuple(lv))
+
+
+def serration_measure(data: np.ndarray, cfg: SyncConfig, position: int):
+    x = np.ascontiguousarray(data, dtype=np.float64)
+    lv = np.zeros(2)
+    ok = _sync_lib().oracle_serration_measure(_p(x, _dp), x.size, ctypes.byref(cfg), position, _p(lv, _dp))
+    return tuple(lv) if ok else None
+
+
+def serration_config(sample_rate_hz: float, fps: float, frame_lines: float, eq_us: float,
+                     serration_mode: int = SERRATION_REFERENCE) -> SyncConfig:
+    """A SyncConfig carrying only the VsyncSerrationDetector geometry and filters
+    (new VsyncSerrationDetector(sampleRateHz, framesPerSecond, frameLines, equalizingPulseUs))."""
+    c = SyncConfig()
+    c.sample_rate_hz = sample_rate_hz
+    c.fps, c.frame_lines, c.eq_us = fps, frame_lines, eq_us
+    c.line_period_us = 1e6 / (fps * frame_lines)
+    c.hsync_us, c.vsync_us, c.num_pulses = 4.7, 27.3, 5
+    c.hsync_tolerance_us = c.eq_tolerance_us = 0.5
+    c.hz_ire = 1.0
+    c.output_lines = 313
+    c.block_length, c.block_cut, c.block_stride, c.read_samples = 32768, 1024, 30720, 32768
+    c.level_divisor = 1
+    c.serration_mode = serration_mode
+    vf, hf = fps * 2.0, fps * frame_lines
+    specs = [(vf * 5.0, 1000.0, False), (hf, hf, True), (hf, hf, False), (hf / 3.0, hf / 2.0, False)]
+    for k, (cut, tw, hp) in enumerate(specs):
+        order, b, a, sos = butterworth_serration_filter(sample_rate_hz, cut, tw, hp)
+        c.tf_order[k] = order
+        for i in range(order + 1):
+            c.tf_b[k][i] = b[i]
+            c.tf_a[k][i] = a[i]
+        c.sos_sections[k] = sos.shape[0]
+        for s_ in range(sos.shape[0]):
+            for j in range(6):
+                c.sos[k][s_][j] = sos[s_, j]
+    return c
