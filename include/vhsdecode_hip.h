@@ -435,6 +435,93 @@ VHSDECODE_HIP_API int32_t vhsdecode_hip_decode_span_fields_device(
     const vhsdecode_hip_field_params_v1* params, const vhsdecode_hip_locator_v1* locator, int32_t max_fields,
     uint16_t* out, int32_t* field_count, double* field_line0, int32_t* field_first, void* stream);
 
+/* ---------------------------------------------------------------- VHS field sequence
+ * The field decode of the vhs command with its default (v0.4.0) sync path, for the fields
+ * of a demodulated span read in sequence: TbcFieldDecodePipeline.DecodePrepared
+ * (Decode/TbcFieldDecodePipeline.cs:1071-1420) from PrepareSyncSpan's level detection
+ * (:2776-3234: the fallback serration-level search and refinement, the moving averages of
+ * VsyncSerrationDetector and VhsFieldLevelState, the DC shift of the sync reference) to
+ * FindRawPulses, SyncAnalyzer.EstimateTiming/RefinePulses/ComputeMeanLineLength
+ * (Dsp/SyncAnalysis.cs:131-538), FieldParityDetector.ResolveCadence, ResolveLine0Location
+ * (:5452-5651) with VBlankSyncResolver.EstimateLine0FromTransitions, BuildUpstreamLineLocations
+ * (SyncAnalysis.cs:585-657), RefineLineLocationsFromHSync on the shifted reference at the
+ * field's threshold, the NTSC fsc phase shift (:4022-4038) and the TBC render from the
+ * output's first line (OutputFirstLine, :2475-2483).  Each field is read from the window
+ * DecodeReadWindowPlanner.Resolve places at the previous field's next-field offset
+ * (TbcFieldSequenceDecodeEngine.cs:161-171); the first from `first_begin`.
+ * VsyncSerrationDetector's candidate search is not run (no serration measurement). */
+typedef struct vhsdecode_hip_vhs_sync_v1 {
+  uint32_t struct_size;
+  int32_t is_pal;
+  double line_period_us;           /* sysparams line_period */
+  double hsync_pulse_us;           /* hsyncPulseUS */
+  double eq_pulse_us;              /* eqPulseUS */
+  double vsync_pulse_us;           /* vsyncPulseUS */
+  int32_t num_pulses;              /* numPulses */
+  double hsync_tolerance_us;       /* 0.7 for vhs (DecodeSession.cs:420-428) */
+  double eq_tolerance_us;          /* 0.9 for vhs */
+  double frames_per_second;        /* FPS */
+  double frame_lines;              /* frame_lines */
+  double output_rate_mhz;          /* outfreq (TbcFrameSpec.OutputSampleRateHz / 1e6) */
+  int64_t read_samples;            /* DecodeReadWindowPlanner.EstimateReadSampleCount */
+  int32_t block_length;            /* read-window alignment */
+  int32_t block_cut;
+  int32_t block_stride;
+} vhsdecode_hip_vhs_sync_v1;
+
+/* One field of the sequence (device output record; struct_size is written). */
+typedef struct vhsdecode_hip_vhs_field_v1 {
+  uint32_t struct_size;
+  int32_t status;                  /* 0 decoded; 1 no pulses, 2 no first hsync, 3 insufficient data
+                                      (the reference's recovery kinds); 4 window not the predicted one;
+                                      5 pulse capacity exceeded */
+  int64_t begin;                   /* intended read start (span coordinates) */
+  int64_t window_start;            /* read window (span coordinates) */
+  int64_t window_length;
+  int32_t level_source;            /* 0 default threshold, 1 serration levels, 2 refined levels,
+                                      3 saved levels, 4 reversed defaults */
+  int32_t raw_pulses;
+  int32_t refined_pulses;
+  int32_t is_first_field;
+  int32_t parity_confidence;
+  int32_t reserved;
+  double sync_level, blank_level;  /* Hz */
+  double dc_offset;                /* added to the sync reference */
+  double threshold;                /* sync threshold on the shifted reference */
+  double hsync_median;
+  double mean_line_length;
+  double line0;                    /* window coordinates */
+  double first_hsync;
+  double first_hsync_line;
+  double next_offset;              /* ComputeNextFieldOffsetSamples (window coordinates) */
+  int64_t next_begin;              /* span coordinates */
+} vhsdecode_hip_vhs_field_v1;
+
+/* sync_ref, video: device span planes whose sample 0 lies on the block-stride grid;
+ * first_begin: the first field's intended read start (span coordinates); reset_state: 1
+ * starts a new sequence (no level history, parity or sync history), 0 continues the one
+ * this context's last call left (its span coordinates shifted by `state_shift` samples,
+ * the distance from that call's span start to this one's).  out: device
+ * [max_fields][field_lines * output_line_length]; fields: optional device records
+ * [max_fields]; field_count: device int32 (fields decoded, in order, each with status 0);
+ * next_begin: optional device int64 receiving where the sequence continues (span
+ * coordinates).  params->threshold is the nominal sync threshold used only to place the
+ * fields' read windows.  Asynchronous on `stream`. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_decode_vhs_fields_device(
+    vhsdecode_hip_demod_context* context, const float* sync_ref, const float* video, int64_t span_length,
+    int64_t first_begin, int32_t reset_state, int64_t state_shift, const vhsdecode_hip_vhs_sync_v1* sync,
+    const vhsdecode_hip_field_params_v1* params, int32_t max_fields, uint16_t* out,
+    vhsdecode_hip_vhs_field_v1* fields, int32_t* field_count, int64_t* next_begin, void* stream);
+
+/* The line tables of the last vhsdecode_hip_decode_vhs_fields_device call on this context:
+ * which 0: BuildUpstreamLineLocations (TbcFieldDecodePipeline.cs:1244-1295), 1: after
+ * RefineLineLocationsFromHSync (:1344-1348), 2: the TBC render locations (less the NTSC
+ * fsc phase shift, :1401-1407).  Device memory owned by the context, [fields][stride]
+ * in window coordinates (the field record's window_start is the origin), valid until
+ * the next vhs field decode on it. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_vhs_field_lines_device(vhsdecode_hip_demod_context* context, int32_t which,
+                                                               const double** table, int32_t* stride);
+
 /* The refined line locations of the last field decode on this context
  * (vhsdecode_hip_decode_fields_device or vhsdecode_hip_decode_span_fields_device):
  * device memory owned by the context, [fields][location_count] in span coordinates,

@@ -1647,8 +1647,9 @@ int oracle_sync_field_size() { return (int)sizeof(OracleSyncField); }
 // offset (TbcFieldSequenceDecodeEngine.cs:161-171, 1936-1941), recovery offsets added to
 // the intended start as the sequence engine does (:846).  Stops when a window would run
 // past `n` or after `max_fields` attempts.  `locs` receives [max_fields][max_lines]
-// window-relative line locations of decoded fields; `shifted` (optional,
-// [max_fields][window]) is not written.  Returns the number of records written.
+// window-relative line locations of decoded fields; `shifted_out` (optional,
+// [max_fields][shifted_stride]) receives each decoded field's DC-shifted sync reference
+// (the data its hsync refinement reads).  Returns the number of records written.
 int oracle_vhs_field_sequence(const double* vlp, int64_t n, const OracleSyncConfig* cfg, int64_t first_begin,
                               int max_fields, int max_lines, OracleSyncField* fields, double* locs,
                               double* shifted_out, int64_t shifted_stride) {

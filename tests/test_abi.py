@@ -65,6 +65,8 @@ def _c_sizeof(struct):
     ("vhsdecode_hip_dropout_params_v1", _lib.DropoutParamsV1),
     ("vhsdecode_hip_run_config_v1", _lib.RunConfigV1),
     ("vhsdecode_hip_run_result_v1", _lib.RunResultV1),
+    ("vhsdecode_hip_vhs_sync_v1", _lib.VhsSyncV1),
+    ("vhsdecode_hip_vhs_field_v1", _lib.VhsFieldV1),
 ])
 def test_struct_layouts_match_header(name, cls):
     assert _c_sizeof(name) == ctypes.sizeof(cls)

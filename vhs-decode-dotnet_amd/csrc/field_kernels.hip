@@ -200,7 +200,11 @@ struct WaveCtx {
   long long n;
   int lane;
   double bias;
-  __device__ __forceinline__ double at(long long i) const { return (double)s[i] + bias; }
+  double dc;  // DC offset added to the sync reference (PrepareSyncSpanFromLevels); 0 adds nothing
+  __device__ __forceinline__ double at(long long i) const {
+#pragma clang fp contract(off)
+    return ((double)s[i] + bias) + dc;
+  }
 };
 
 __device__ __forceinline__ int iround(double v) { return (int)round(v); }
@@ -431,7 +435,14 @@ struct HsyncWork {
   int* fb_line;          // per field: number of lines the ordered walk refined again
   double* fb_porch;      // reserved
   int stride;            // entries per field (max lines)
+  const double* field_threshold;  // per field sync threshold, or NULL for HsyncParams.threshold
+  const double* field_dc;         // per field DC offset of the sync reference, or NULL for none
 };
+
+__device__ __forceinline__ HsyncParams field_params(HsyncParams p, const HsyncWork& wk, int f) {
+  if (wk.field_threshold) p.threshold = wk.field_threshold[f];
+  return p;
+}
 
 // Pass 1: line kinds and right-edge candidates, one wavefront per line.
 __global__ void __launch_bounds__(64) hsync_prelim_kernel(const float* const* spans, const long long* span_len,
@@ -440,7 +451,8 @@ __global__ void __launch_bounds__(64) hsync_prelim_kernel(const float* const* sp
   const int f = blockIdx.y;
   const int line = blockIdx.x;
   if (line >= line_counts[f]) return;
-  WaveCtx w{spans[f], span_len[f], (int)threadIdx.x, bias};
+  WaveCtx w{spans[f], span_len[f], (int)threadIdx.x, bias, wk.field_dc ? wk.field_dc[f] : 0.0};
+  p = field_params(p, wk, f);
   const HsyncGeom g = hsync_geom(p);
   long long ss = 0;
   const int k = line_kind(p, g, w.n, line, locs[f][line], &ss);
@@ -465,7 +477,8 @@ __global__ void __launch_bounds__(64) hsync_spec_kernel(const float* const* span
   const int f = blockIdx.y;
   const int line = blockIdx.x;
   if (line >= line_counts[f]) return;
-  WaveCtx w{spans[f], span_len[f], (int)threadIdx.x, bias};
+  WaveCtx w{spans[f], span_len[f], (int)threadIdx.x, bias, wk.field_dc ? wk.field_dc[f] : 0.0};
+  p = field_params(p, wk, f);
   const HsyncGeom g = hsync_geom(p);
   const int base = f * wk.stride;
   const int k = wk.kind[base + line];
@@ -512,7 +525,8 @@ __global__ void __launch_bounds__(64) hsync_verify_kernel(const float* const* sp
                                                           unsigned char* const* filled_out, double bias) {
   const int f = blockIdx.x;
   const int lines = line_counts[f];
-  WaveCtx w{spans[f], span_len[f], (int)threadIdx.x, bias};
+  WaveCtx w{spans[f], span_len[f], (int)threadIdx.x, bias, wk.field_dc ? wk.field_dc[f] : 0.0};
+  p = field_params(p, wk, f);
   const HsyncGeom g = hsync_geom(p);
   const int base = f * wk.stride;
   // Alternate a parallel check (64 lines at a time: the first line at or after `from`
@@ -739,7 +753,11 @@ __device__ double block_median_512(double* buf, int n, double pad) {
 }
 
 // line_decay = (1 - alpha)^out_len, the smoothing carry over one whole line; 0 disables smoothing.
-__global__ void __launch_bounds__(kLevelThreads) tbc_levels_kernel(const double* locs, int nloc, int lines,
+// Lines: the output lines plus, with first_lines, the field's lines before its first
+// output line (TbcLineResampler's level adjusts run from line 0 of the locations,
+// Tbc/TbcLineResampler.cs:818-878).  line_x / line_y0 hold line_stride entries per field.
+__global__ void __launch_bounds__(kLevelThreads) tbc_levels_kernel(const double* locs, int nloc, int out_lines,
+                                                                   const int* first_lines, int line_stride,
                                                                    double nominal, double line_decay, double* line_x,
                                                                    double* line_y0, const int* field_count) {
   __shared__ double buf[kLevelThreads];
@@ -747,6 +765,7 @@ __global__ void __launch_bounds__(kLevelThreads) tbc_levels_kernel(const double*
   __shared__ int nan_flag;
   const int f = blockIdx.x;
   if (field_count && f >= *field_count) return;  // uniform across the workgroup
+  const int lines = out_lines + (first_lines ? first_lines[f] : 0);
   const int t = threadIdx.x;
   const double* loc = locs + (long long)f * nloc;
   const double inv = 1.0 / nominal;
@@ -774,12 +793,12 @@ __global__ void __launch_bounds__(kLevelThreads) tbc_levels_kernel(const double*
   const double thr = mad0 > 0.0 ? 15.0 * mad0 : 0.001;
   if (t < lines) {
     const double x = fabs(fac[t] - m0) > thr ? m0 : fac[t];
-    line_x[(long long)f * lines + t] = x;
+    line_x[(long long)f * line_stride + t] = x;
     fac[t] = x;
   }
   __syncthreads();
   if (t == 0) {
-    double* y0 = line_y0 + (long long)f * lines;
+    double* y0 = line_y0 + (long long)f * line_stride;
     if (line_decay > 0.0) {
       // entry value of each line; line 0's first sample is the unsmoothed factor
       double prev = fac[0];
@@ -795,8 +814,10 @@ __global__ void __launch_bounds__(kLevelThreads) tbc_levels_kernel(const double*
 
 struct FieldTbcParams {
   int out_len;
-  int lines;        // output lines per field (first_line = 0)
-  int nloc;         // line locations per field (lines + 1)
+  int lines;        // output lines per field
+  int nloc;         // line locations per field (at least first line + lines + 1)
+  int line_stride;  // level entries per field
+  int relative;     // 1: locations count from the field's origin (no origin subtraction)
   double nominal;
   int smooth;       // 1: apply the smoothing powers table
   double ire0;
@@ -815,10 +836,10 @@ __device__ __forceinline__ int tbc_slot(int pix, int s) { return pix * 8 + (s ^ 
 
 __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long long src_len, double bias,
                                                          const long long* origins, const double* locs,
-                                                         const double* line_x, const double* line_y0,
-                                                         const double* pow_table, FieldTbcParams p,
-                                                         const float* __restrict__ lut, unsigned short* out,
-                                                         const int* field_count) {
+                                                         const int* first_lines, const double* line_x,
+                                                         const double* line_y0, const double* pow_table,
+                                                         FieldTbcParams p, const float* __restrict__ lut,
+                                                         unsigned short* out, const int* field_count) {
   __shared__ float4 rows[256 / 64][kTbcStage];
   const int f = blockIdx.y;
   if (field_count && f >= *field_count) return;
@@ -826,10 +847,13 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
   const int d0 = blockIdx.x * blockDim.x + threadIdx.x;  // a field has < 2^31 output samples
   // every lane takes part in the row fetch; lanes past the field repeat its last sample
   const bool valid = d0 < dest_len;
-  const int d = valid ? d0 : dest_len - 1;
+  const int dd = valid ? d0 : dest_len - 1;
+  // output sample dd is resampled sample d of the field, counted from its line 0
+  // (LinearLineLocationInterpolator.FillOutputPositions with the output's first line)
+  const int d = dd + (first_lines ? first_lines[f] * p.out_len : 0);
   const double* loc = locs + (long long)f * p.nloc;
-  // output line of sample d: an FP32 estimate (d < 2^24 is exact in FP32, so it is off
-  // by at most one) and one correction
+  // line of sample d: an FP32 estimate (d < 2^24 is exact in FP32, so it is off by at
+  // most one) and one correction
   int left = (int)((float)d * p.inv_out_len);
   int smp = d - left * p.out_len;
   if (smp < 0) {
@@ -846,13 +870,13 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
   double frac = (sp - (left * p.nominal)) / p.nominal;
   frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
   double pos, adj;
-  const double x = line_x[(long long)f * p.lines + left];
+  const double x = line_x[(long long)f * p.line_stride + left];
   const long long org = origins ? origins[f] : 0;
   {
 #pragma clang fp contract(off)
     pos = ll + (delta * frac);
-    pos -= (double)org;
-    adj = p.smooth ? x + (line_y0[(long long)f * p.lines + left] - x) * pow_table[smp] : x;
+    if (!p.relative) pos -= (double)org;
+    adj = p.smooth ? x + (line_y0[(long long)f * p.line_stride + left] - x) * pow_table[smp] : x;
   }
   const bool finite = isfinite(pos);
   const SincPhase ph = finite ? sinc_phase(pos) : SincPhase{0, 0.0f, 0, 0};
@@ -921,7 +945,7 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
     o = 65535;
   else
     o = (unsigned short)v;
-  out[(long long)f * dest_len + d] = o;
+  out[(long long)f * dest_len + dd] = o;
 }
 
 // ------------------------------------------------------------------ line locations
@@ -1151,7 +1175,8 @@ hipError_t vhsd_launch_pulses(const float* s, long long n, double bias, double h
 hipError_t vhsd_launch_hsync(const float* const* spans, const long long* span_len, const double* const* locs,
                              const unsigned char* const* filled, const int* line_counts, const vhsd::HsyncParams* p,
                              double* const* refined_out, unsigned char* const* filled_out, int fields, int max_lines,
-                             double bias, void* work, hipStream_t stream) {
+                             double bias, void* work, const double* field_threshold, const double* field_dc,
+                             hipStream_t stream) {
   if (fields <= 0 || max_lines <= 0) return hipSuccess;
   // work layout: kind, rt_ok [fields*max_lines bytes each], rt_cand, rt_porch, p_used, p_out
   // [fields*max_lines doubles each], fb_line [fields ints], fb_porch [fields doubles]
@@ -1173,6 +1198,8 @@ hipError_t vhsd_launch_hsync(const float* const* spans, const long long* span_le
   wk.fb_line = reinterpret_cast<int*>(take((size_t)fields * 4));
   wk.fb_porch = reinterpret_cast<double*>(take((size_t)fields * 8));
   wk.stride = max_lines;
+  wk.field_threshold = field_threshold;
+  wk.field_dc = field_dc;
   const dim3 lg(max_lines, fields);
   hipLaunchKernelGGL(vhsd::hsync_prelim_kernel, lg, dim3(64), 0, stream, spans, span_len, locs, line_counts, *p, wk,
                      bias);
@@ -1190,21 +1217,25 @@ size_t vhsd_hsync_work_bytes(int fields, int max_lines) {
 }
 
 hipError_t vhsd_launch_tbc_fields(const float* src, long long src_len, double bias, const long long* origins,
-                                  const double* refined, int nloc, int lines,
+                                  const double* refined, int nloc, int lines, const int* first_lines,
+                                  int max_first_line, int relative,
                                   int fields, double nominal, double line_decay, const double* pow_table,
                                   double* line_x, double* line_y0, int out_len, int smooth, double ire0,
                                   double out_scale, double inv_hz_ire, double conv_offset, const float* lut,
                                   unsigned short* out, const int* field_count, hipStream_t stream) {
   if (fields <= 0) return hipSuccess;
-  if (lines > vhsd::kLevelThreads || nloc < 2 || lines > nloc - 1) return hipErrorInvalidValue;
+  const int line_stride = lines + (first_lines ? max_first_line : 0);
+  if (line_stride > vhsd::kLevelThreads || nloc < 2 || line_stride > nloc - 1 || max_first_line < 0)
+    return hipErrorInvalidValue;
   hipLaunchKernelGGL(vhsd::tbc_levels_kernel, dim3(fields), dim3(vhsd::kLevelThreads), 0, stream, refined, nloc,
-                     lines, nominal, smooth ? line_decay : 0.0, line_x, line_y0, field_count);
-  vhsd::FieldTbcParams p{out_len,    lines,      nloc,        nominal,
-                         smooth,     ire0,       out_scale,   inv_hz_ire,
-                         conv_offset, 1.0f / (float)out_len};
+                     lines, first_lines, line_stride, nominal, smooth ? line_decay : 0.0, line_x, line_y0,
+                     field_count);
+  vhsd::FieldTbcParams p{out_len,    lines,      nloc,        line_stride, relative,  nominal,
+                         smooth,     ire0,       out_scale,   inv_hz_ire,  conv_offset, 1.0f / (float)out_len};
   const long long dest = (long long)out_len * lines;
   hipLaunchKernelGGL(vhsd::tbc_fields_kernel, dim3((unsigned)((dest + 255) / 256), fields), dim3(256), 0, stream, src,
-                     src_len, bias, origins, refined, line_x, line_y0, pow_table, p, lut, out, field_count);
+                     src_len, bias, origins, refined, first_lines, line_x, line_y0, pow_table, p, lut, out,
+                     field_count);
   return hipGetLastError();
 }
 

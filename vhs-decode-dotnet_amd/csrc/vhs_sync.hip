@@ -1,0 +1,1725 @@
+// The reference's VHS field sync path (v0.4.0 profile) for the fields of a demodulated
+// span, on the device: per-field level detection (the fallback serration-level search and
+// its refinement), the levels' moving-average chain across fields, sync pulses on the
+// DC-shifted sync reference, pulse classification and refinement, field parity, line 0,
+// the read window of the next field and the upstream line locations.
+//
+// Reference: src/VHSDecode.Core/Decode/TbcFieldDecodePipeline.cs DecodePrepared
+// :1071-1290, PrepareSyncSpan :2776-3190, PrepareSyncSpanFromLevels :3197-3234,
+// ResolveLine0Location :5452-5651, level selection :5679-5782, sync history :6119-6375;
+// Dsp/SyncAnalysis.cs :131-657; Dsp/LevelDetection.cs :98-407, 445-524;
+// Dsp/VsyncSerrationDetector.cs :475-536 (CheckLevels), :727-754 (moving average);
+// Dsp/VhsFieldLevelState.cs; Dsp/FieldParityDetector.cs :16-268;
+// Dsp/VBlankSyncResolver.cs :317-747; Decode/DecodeReadWindowPlanner.cs :37-63.
+//
+// Fields run in parallel on read windows speculated from the vertical-sync groups of the
+// span (a window is the reference's DecodeReadWindowPlanner.Resolve of the previous
+// field's next-field offset, aligned down to a 32768-sample block, so a prediction within
+// a few samples of the true offset gives the same window); a one-thread chain kernel then
+// walks the fields in order, carries every cross-field state (level moving averages,
+// parity cadence, sync history) exactly as the reference does and checks each window
+// against the one the previous field implies.  The first field whose window was
+// mispredicted ends the span's field list; the caller resumes there.
+//
+// The VsyncSerrationDetector candidate search is not run: it filters each ~1M-sample
+// window with four transfer-function IIR recurrences whose candidates are set by the
+// recurrences' own rounding (an accurate evaluation of the same filters picks other
+// candidates), so no parallel evaluation reproduces it.  Its measurements are taken as
+// absent, which is what the reference measures on a 0.5 MHz sync reference whenever the
+// equalizing pulses widen past 1.25 eqPulseUS at the serration level.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vhs_sync.hpp"
+
+namespace vhsd {
+namespace vsync {
+
+constexpr int kT = 1024;   // threads of the per-field workgroups
+constexpr int kRun = 16;   // samples per thread per pulse tile
+constexpr int kTile = kT * kRun;
+constexpr int kMaxMeans = 256;    // pulse means per level refinement
+constexpr int kMaxLeaves = 4096;  // pairwise-sum leaves per level refinement
+
+__device__ __forceinline__ double round_away(double x) { return round(x); }
+__device__ __forceinline__ double round_even(double x) { return rint(x); }
+__device__ __forceinline__ bool in_range(double v, double lo, double hi) { return v >= lo && v <= hi; }
+
+struct Plane {
+  const float* p;
+  long long n;
+  double bias;
+  double dc;
+  __device__ __forceinline__ double at(long long i) const {
+#pragma clang fp contract(off)
+    return ((double)p[i] + bias) + dc;
+  }
+};
+
+// ------------------------------------------------------------------ workgroup helpers
+__device__ __forceinline__ int wave_scan_incl(int v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+// Exclusive prefix of v over the workgroup; *total receives the sum.  red: LDS int[16 + 1].
+__device__ int wg_exclusive(int v, int* red, int* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int incl = wave_scan_incl(v, lane);
+  if (lane == 63) red[wave] = incl;
+  __syncthreads();
+  int before = 0, tot = 0;
+  for (int w = 0; w < kT / 64; w++) {
+    const int x = red[w];
+    if (w < wave) before += x;
+    tot += x;
+  }
+  __syncthreads();
+  *total = tot;
+  return before + incl - v;
+}
+
+__device__ double wg_min(double v, double* dred) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  if (lane == 0) dred[wave] = v;
+  __syncthreads();
+  double m = dred[0];
+  for (int w = 1; w < kT / 64; w++) m = fmin(m, dred[w]);
+  __syncthreads();
+  return m;
+}
+
+__device__ long long wg_sum_ll(long long v, long long* lred) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (lane == 0) lred[wave] = v;
+  __syncthreads();
+  long long s = 0;
+  for (int w = 0; w < kT / 64; w++) s += lred[w];
+  __syncthreads();
+  return s;
+}
+
+// ------------------------------------------------------------------ pulses
+// PulseDetection.FindPulses (Dsp/PulseDetection.cs:45-222) over the plane: a two-state
+// machine over `value <= high` (NaN keeps the state); runs of kRun samples per thread,
+// edges compacted in order into epos/ekind, then each end edge paired with the edge
+// before it.  Pulses starting at 0 are dropped.  Returns the pulse count (every thread);
+// stores the first `cap` into starts/lens; *vsync receives the number of pulses longer
+// than vs_min.  *edges_overflow is set when the edge list was cut at ecap.
+struct PulseScratch {
+  int red[kT / 64 + 4];
+  int offset;
+};
+
+__device__ __forceinline__ int state_before(const Plane& x, long long i, double high) {
+  for (long long k = i; k >= 0; k--) {
+    const float v = x.p[k];
+    if (v == v) return x.at(k) <= high ? 1 : 0;
+  }
+  return 0;
+}
+
+__device__ int wg_find_pulses(const Plane& x, double high, int min_len, int max_len, double vs_min, int* starts,
+                              int* lens, int cap, int* epos, unsigned char* ekind, int ecap, PulseScratch* ps,
+                              int* vsync, int* edges_overflow) {
+  const int tid = threadIdx.x;
+  if (tid == 0) ps->offset = 0;
+  __syncthreads();
+  for (long long base = 0; base < x.n; base += kTile) {
+    const long long i0 = base + (long long)tid * kRun;
+    unsigned codes = 0;
+    if (i0 < x.n) {
+      int st = i0 > 0 ? state_before(x, i0 - 1, high) : 0;
+#pragma unroll
+      for (int k = 0; k < kRun; k++) {
+        const long long i = i0 + k;
+        if (i < x.n) {
+          const float v = x.p[i];
+          if (v == v) {
+            const int cur = x.at(i) <= high ? 1 : 0;
+            if (i >= 1) codes |= ((!st && cur) ? 1u : ((st && !cur) ? 2u : 0u)) << (2 * k);
+            st = cur;
+          }
+        }
+      }
+    }
+    const int c = __popc(codes);
+    int tot;
+    int idx = wg_exclusive(c, ps->red, &tot) + ps->offset;
+    unsigned rest = codes;
+    while (rest) {
+      const int k = (__ffs(rest) - 1) >> 1;
+      if (idx < ecap) {
+        epos[idx] = (int)(i0 + k);
+        ekind[idx] = (unsigned char)((rest >> (2 * k)) & 3u);
+      }
+      idx++;
+      rest &= ~(3u << (2 * k));
+    }
+    __syncthreads();
+    if (tid == 0) ps->offset += tot;
+    __syncthreads();
+  }
+  const int total_edges = ps->offset;
+  if (tid == 0 && edges_overflow && total_edges > ecap) *edges_overflow = 1;
+  const int ne = total_edges < ecap ? total_edges : ecap;
+  __syncthreads();
+  if (tid == 0) ps->offset = 0;
+  __syncthreads();
+  int vs_count = 0;
+  for (int b0 = 0; b0 < ne; b0 += kT) {
+    const int k = b0 + tid;
+    bool ok = false;
+    int st = 0, len = 0;
+    if (k < ne && ekind[k] == 2) {
+      st = k > 0 ? epos[k - 1] : 0;
+      len = epos[k] - st;
+      ok = in_range((double)len, min_len, max_len) && st != 0;
+    }
+    if (ok && (double)len > vs_min) vs_count++;
+    int tot;
+    const int idx = wg_exclusive(ok ? 1 : 0, ps->red, &tot) + ps->offset;
+    if (ok && idx < cap) {
+      starts[idx] = st;
+      lens[idx] = len;
+    }
+    __syncthreads();
+    if (tid == 0) ps->offset += tot;
+    __syncthreads();
+  }
+  const int count = ps->offset;
+  int vt;
+  (void)wg_exclusive(vs_count, ps->red, &vt);
+  if (vsync) *vsync = vt;
+  __syncthreads();
+  return count;
+}
+
+// ------------------------------------------------------------------ pairwise means
+// NumpyReduction.MeanFloat64 (Dsp/NumpyReduction.cs:226-227, 400-451) of several ranges:
+// the recursion's leaves (<= 128 values) are summed in parallel, then each range's tree is
+// combined in the recursion's order by one thread.
+struct MeanScratch {
+  int leaf_start[kMaxLeaves];
+  short leaf_len[kMaxLeaves];
+  double leaf_sum[kMaxLeaves];
+  int n_leaves;
+};
+
+__device__ void collect_leaves(int s, int n, MeanScratch* m) {
+  // iterative depth-first split, left before right
+  int stack_s[32], stack_n[32], sp = 0;
+  stack_s[sp] = s;
+  stack_n[sp] = n;
+  sp++;
+  while (sp > 0) {
+    sp--;
+    const int a = stack_s[sp], len = stack_n[sp];
+    if (len > 128) {
+      int split = len / 2;
+      split -= split % 8;
+      stack_s[sp] = a + split;
+      stack_n[sp] = len - split;
+      sp++;
+      stack_s[sp] = a;
+      stack_n[sp] = split;
+      sp++;
+    } else if (m->n_leaves < kMaxLeaves) {
+      m->leaf_start[m->n_leaves] = a;
+      m->leaf_len[m->n_leaves] = (short)len;
+      m->n_leaves++;
+    }
+  }
+}
+
+__device__ double leaf_sum(const Plane& x, int s, int n) {
+#pragma clang fp contract(off)
+  if (n < 8) {
+    double r = -0.0;
+    for (int i = 0; i < n; i++) r += x.at(s + i);
+    return r;
+  }
+  double r[8];
+  for (int k = 0; k < 8; k++) r[k] = x.at(s + k);
+  int i = 8;
+  const int end = n - (n % 8);
+  for (; i < end; i += 8)
+    for (int k = 0; k < 8; k++) r[k] += x.at(s + i + k);
+  double t = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  for (; i < n; i++) t += x.at(s + i);
+  return t;
+}
+
+// combine the leaves of one range (they are consecutive from *cursor, in depth-first order)
+__device__ double combine(int n, const MeanScratch* m, int* cursor) {
+#pragma clang fp contract(off)
+  if (n > 128) {
+    int split = n / 2;
+    split -= split % 8;
+    const double l = combine(split, m, cursor);
+    const double r = combine(n - split, m, cursor);
+    return l + r;
+  }
+  return m->leaf_sum[(*cursor)++];
+}
+
+// means[k] = mean of x[rs[k], re[k]) for k < count (all threads call; count <= kMaxMeans)
+__device__ void wg_means(const Plane& x, const int* rs, const int* re, int count, double* means, MeanScratch* m) {
+  if (threadIdx.x == 0) {
+    m->n_leaves = 0;
+    for (int k = 0; k < count; k++) collect_leaves(rs[k], re[k] - rs[k], m);
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < m->n_leaves; l += kT) m->leaf_sum[l] = leaf_sum(x, m->leaf_start[l], m->leaf_len[l]);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int cursor = 0;
+    for (int k = 0; k < count; k++) {
+      const int n = re[k] - rs[k];
+      const double s = combine(n, m, &cursor);
+      means[k] = s / (double)n;
+    }
+  }
+  __syncthreads();
+}
+
+// NumpyReduction.MedianFloat64 of v[0, n) (n small; one thread; sorts in place)
+__device__ double median_small(double* v, int n) {
+  if (n <= 0) return __builtin_nan("");
+  for (int i = 0; i < n; i++)
+    if (v[i] != v[i]) return v[i];
+  for (int i = 1; i < n; i++) {
+    const double x = v[i];
+    int j = i - 1;
+    while (j >= 0 && v[j] > x) {
+      v[j + 1] = v[j];
+      j--;
+    }
+    v[j + 1] = x;
+  }
+  const int mid = n / 2;
+  return (n & 1) ? v[mid] : (v[mid - 1] + v[mid]) / 2.0;
+}
+
+// ------------------------------------------------------------------ level refinement
+struct RefineScratch {
+  PulseScratch ps;
+  MeanScratch ms;
+  int rs[kMaxMeans], re[kMaxMeans];
+  double means[kMaxMeans];
+  double black[kMaxMeans];
+  int nv, nb;
+  int loc_first, loc_last, n_locs;
+  int idx_count;
+  long long lred[kT / 64];
+  int flag;
+};
+
+// VsyncSerrationDetector.CheckLevels (Dsp/VsyncSerrationDetector.cs:475-536), the window
+// counts taken by the workgroup when the early returns do not decide.
+__device__ bool wg_check_levels(const Plane& x, double old_sync, double new_sync, double new_blank, double ref_sync,
+                                double hz_ire, RefineScratch* r) {
+#pragma clang fp contract(off)
+  if (x.n <= 0 || hz_ire == 0.0) return false;
+  const double diff = (new_blank - new_sync) / hz_ire;
+  if ((ref_sync - new_sync) > (hz_ire * 15.0) || diff > 47.0) return false;
+  if (new_sync - old_sync < (hz_ire * 5.0)) return true;
+  long long bs = 0, bb = 0;
+  for (long long i = threadIdx.x; i < x.n; i += kT) {
+    const double v = x.at(i);
+    if (v < new_sync) bs++;
+    if (v < new_blank) bb++;
+  }
+  bs = wg_sum_ll(bs, r->lred);
+  bb = wg_sum_ll(bb, r->lred);
+  const double a = (double)bs / x.n;
+  const double b = (double)bb / x.n;
+  return !(a > 0.07 || b < 0.005);
+}
+
+enum Fail { kNone = 0, kMissing = 1, kNonFinite = 2, kCheckFailed = 3 };
+
+struct RefineOut {
+  int ok;
+  double sync, blank;
+  int fail;
+  int has_measured;
+  double measured;
+};
+
+// LevelDetection.RefineSerrationLevelsFromPulses (Dsp/LevelDetection.cs:172-235) with
+// FallbackVsyncLocationMeans (:22-53) and PulsesBlackLevelMeans (:445-524) on a pulse
+// list in order (starts strictly increasing).
+__device__ RefineOut wg_refine_from_pulses(const Plane& x, const int* starts, const int* lens, int npulses,
+                                           const SyncCfg& c, double ref_sync, RefineScratch* r) {
+#pragma clang fp contract(off)
+  RefineOut o{0, 0.0, 0.0, kMissing, 0, 0.0};
+  const double vl = c.vsync_us * c.fs_mhz;
+  const double lo = vl * 0.8, hi = vl * 1.2;
+  const int off = (int)c.fs_mhz;
+  if (threadIdx.x == 0) r->nv = 0;
+  __syncthreads();
+  // vsync-length pulses, in order: ordered compaction
+  for (int b0 = 0; b0 < npulses; b0 += kT) {
+    const int k = b0 + threadIdx.x;
+    bool ok = false;
+    int s = 0, e = 0;
+    if (k < npulses) {
+      const int len = lens[k];
+      if (!(len <= lo || len >= hi)) {
+        s = starts[k] + off;
+        e = starts[k] + len - off;
+        ok = !(s < 0 || e > x.n || e <= s);
+      }
+    }
+    int tot;
+    const int idx = wg_exclusive(ok ? 1 : 0, r->ps.red, &tot) + r->nv;
+    if (ok && idx < kMaxMeans) {
+      r->rs[idx] = s;
+      r->re[idx] = e;
+      if (idx == 0) r->loc_first = k;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) r->nv += tot;
+    __syncthreads();
+  }
+  // the last vsync pulse: the largest k written (recompute exactly)
+  if (threadIdx.x == 0) r->flag = -1;
+  __syncthreads();
+  for (int k = threadIdx.x; k < npulses; k += kT) {
+    const int len = lens[k];
+    if (!(len <= lo || len >= hi)) {
+      const int s = starts[k] + off, e = starts[k] + len - off;
+      if (!(s < 0 || e > x.n || e <= s)) atomicMax(&r->flag, k);
+    }
+  }
+  __syncthreads();
+  const int nv = r->nv < kMaxMeans ? r->nv : kMaxMeans;
+  if (nv == 0) return o;
+  const int loc_last = r->flag;
+  wg_means(x, r->rs, r->re, nv, r->means, &r->ms);
+  double sync = 0.0;
+  if (threadIdx.x == 0) r->black[0] = median_small(r->means, nv);
+  __syncthreads();
+  sync = r->black[0];
+  o.has_measured = 1;
+  o.measured = sync;
+  // PulsesBlackLevelMeans: indices around the vsync pulses
+  const int last_index = npulses - 1;
+  int before = r->loc_first, after = loc_last;
+  if (nv != 12) {
+    while (before > 1 && starts[before] - starts[before - 1] < 600) before--;
+    // pulse starts increase, so starts[after] - starts[after + 1] < 600 holds to the end
+    after = last_index;
+  }
+  if (threadIdx.x == 0) r->nb = 0;
+  __syncthreads();
+  const double fs = c.fs_mhz;
+  // candidates: [max(before - 5, 1), before) if before > 1, then [after + 1, max(after + 6, last_index))
+  const int a0 = before > 1 ? (before - 5 > 1 ? before - 5 : 1) : 0;
+  const int a1 = before > 1 ? before : 0;
+  const int b0i = after < last_index - 1 ? after + 1 : 0;
+  const int b1i = after < last_index - 1 ? (after + 6 > last_index ? after + 6 : last_index) : 0;
+  const int na = a1 - a0, nb = b1i - b0i;
+  for (int base = 0; base < na + nb; base += kT) {
+    const int q = base + threadIdx.x;
+    bool ok = false;
+    int s = 0, e = 0;
+    if (q < na + nb) {
+      const int i = q < na ? a0 + q : b0i + (q - na);
+      if (i >= 0 && i < npulses && in_range((double)lens[i], fs * 0.75, fs * 3.0)) {
+        long long ss = (long long)(starts[i] + (fs * 5.0));
+        ss = ss < 0 ? 0 : (ss > x.n ? x.n : ss);
+        long long ee = (long long)(starts[i] + (fs * 20.0));
+        ee = ee < ss ? ss : (ee > x.n ? x.n : ee);
+        ok = ee > ss;
+        s = (int)ss;
+        e = (int)ee;
+      }
+    }
+    int tot;
+    const int idx = wg_exclusive(ok ? 1 : 0, r->ps.red, &tot) + r->nb;
+    if (ok && idx < kMaxMeans) {
+      r->rs[idx] = s;
+      r->re[idx] = e;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) r->nb += tot;
+    __syncthreads();
+  }
+  const int nbm = r->nb < kMaxMeans ? r->nb : kMaxMeans;
+  if (nbm == 0) {
+    o.fail = kNonFinite;
+    return o;
+  }
+  wg_means(x, r->rs, r->re, nbm, r->black, &r->ms);
+  if (threadIdx.x == 0) r->means[0] = median_small(r->black, nbm);
+  __syncthreads();
+  const double blank = r->means[0];
+  __syncthreads();
+  if (!isfinite(sync) || !isfinite(blank) || blank < sync) {
+    o.fail = kNonFinite;
+    return o;
+  }
+  if (nv <= 3 || !wg_check_levels(x, ref_sync, sync, blank, ref_sync, c.hz_ire, r)) {
+    o.fail = kCheckFailed;
+    return o;
+  }
+  o.ok = 1;
+  o.fail = kNone;
+  o.sync = sync;
+  o.blank = blank;
+  return o;
+}
+
+// LevelDetection.RefineSerrationLevels (:98-170): pulses at the initial levels' midpoint
+__device__ RefineOut wg_refine_levels(const Plane& x, double s0, double b0, const SyncCfg& c, double ref_sync,
+                                      int* pstart, int* plen, int pcap, int* epos, unsigned char* ekind, int ecap,
+                                      RefineScratch* r, int* overflow) {
+#pragma clang fp contract(off)
+  if (x.n <= 0) return RefineOut{0, 0.0, 0.0, kMissing, 0, 0.0};
+  const double thr = (s0 + b0) / 2.0;
+  const double eql = (double)c.ser_eq_len, ll = (double)c.ser_line_len;
+  const int mn = (int)ceil(eql / 8.0) > 0 ? (int)ceil(eql / 8.0) : 0;
+  const int mx = (int)floor(ll * 5.0) > 1 ? (int)floor(ll * 5.0) : 1;
+  const int np = wg_find_pulses(x, thr, mn, mx, 1e300, pstart, plen, pcap, epos, ekind, ecap, &r->ps, nullptr,
+                                overflow);
+  if (np > pcap && threadIdx.x == 0) *overflow = 1;
+  return wg_refine_from_pulses(x, pstart, plen, np < pcap ? np : pcap, c, ref_sync, r);
+}
+
+// ------------------------------------------------------------------ S1: per-field level search
+// LevelDetection.SearchFallbackSerrationLevels (:265-396, divisor 1, no long-pulse check)
+// then RefineSerrationLevels at the found levels: both depend only on the window.
+__global__ void __launch_bounds__(kT) level_search_kernel(SyncCfg c, SyncWork w) {
+#pragma clang fp contract(off)
+  __shared__ RefineScratch r;
+  __shared__ double dred[kT / 64];
+  const int f = blockIdx.x;
+  if (f >= *w.spec_count) return;
+  const Plane x{w.sync_ref + w.win_start[f], w.win_len[f], c.bias, 0.0};
+  int* pst = w.pulse_start + (long long)f * c.pulse_cap;
+  int* pln = w.pulse_len + (long long)f * c.pulse_cap;
+  int* epos = w.edge_pos + (long long)f * c.edge_cap;
+  unsigned char* ek = w.edge_kind + (long long)f * c.edge_cap;
+  int* ovf = w.overflow + f;
+  LevelSearch out{};
+  // window minimum (Math.Min keeps a NaN)
+  double mn = INFINITY;
+  bool nan = false;
+  for (long long i = threadIdx.x; i < x.n; i += kT) {
+    const double v = x.at(i);
+    if (v != v) nan = true;
+    mn = fmin(mn, v);
+  }
+  mn = wg_min(mn, dred);
+  if (__syncthreads_or(nan ? 1 : 0)) mn = __builtin_nan("");
+  const double blank_level = c.ire0;  // IreToHz(0)
+  const double ref_sync = c.ire0 + (c.hz_ire * c.vsync_ire);
+  const double eql = (double)c.ser_eq_len, ll = (double)c.ser_line_len;
+  const double min_vsync = (c.vsync_us * c.fs_mhz) * 0.8;
+  const int pmn = (int)ceil(eql / 8.0) > 0 ? (int)ceil(eql / 8.0) : 0;
+  const int pmx = (int)floor(ll * 5.0) > 1 ? (int)floor(ll * 5.0) : 1;
+  double min_sync = mn, prev_min_sync = mn;
+  int prev_vsync = 0;
+  bool found = false, check_next = true;
+  double list_thr = 0.0;  // threshold of the pulse list the refinement reads
+  for (int retry = 0; retry < 30; retry++) {
+    const double thr = (min_sync + blank_level) / 2.0;
+    int vc = 0;
+    const int np = wg_find_pulses(x, thr, pmn, pmx, min_vsync, pst, pln, 0, epos, ek, c.edge_cap, &r.ps, &vc, ovf);
+    list_thr = thr;
+    bool stop = false;
+    if (np > 200) {
+      if (vc > 4) {
+        if (vc == 12 && !check_next) {
+          stop = true;
+        } else if (!found || vc > prev_vsync) {
+          found = true;
+          prev_vsync = vc;
+          prev_min_sync = min_sync;
+          check_next = true;
+        } else if (vc < prev_vsync || !check_next) {
+          min_sync = prev_min_sync;
+          list_thr = (min_sync + blank_level) / 2.0;
+          stop = true;
+        } else {
+          check_next = false;
+        }
+      }
+    }
+    if (stop) break;
+    // AdvanceFallbackSyncSearchLevel (:398-407)
+    const double sync_ire = (min_sync - blank_level) / c.hz_ire;
+    const double next_ire = sync_ire + 5.0;
+    const double scaled = c.hz_ire * next_ire;
+    min_sync = blank_level + scaled;
+  }
+  const int np = wg_find_pulses(x, list_thr, pmn, pmx, 1e300, pst, pln, c.pulse_cap, epos, ek, c.edge_cap, &r.ps,
+                                nullptr, ovf);
+  if (np > c.pulse_cap && threadIdx.x == 0) *ovf = 1;
+  const RefineOut first = wg_refine_from_pulses(x, pst, pln, np < c.pulse_cap ? np : c.pulse_cap, c, ref_sync, &r);
+  out.first_ok = first.ok;
+  out.first_sync = first.sync;
+  out.first_blank = first.blank;
+  out.first_fail = first.fail;
+  out.first_has_measured = first.has_measured;
+  out.first_measured = first.measured;
+  if (first.ok) {
+    const RefineOut sec = wg_refine_levels(x, first.sync, first.blank, c, ref_sync, pst, pln, c.pulse_cap, epos, ek,
+                                           c.edge_cap, &r, ovf);
+    out.ref_ok = sec.ok;
+    out.ref_sync = sec.sync;
+    out.ref_blank = sec.blank;
+    out.ref_fail = sec.fail;
+    out.ref_has_measured = sec.has_measured;
+    out.ref_measured = sec.measured;
+  }
+  if (threadIdx.x == 0) w.search[f] = out;
+}
+
+// ------------------------------------------------------------------ S2: level chain
+// The moving averages of VsyncSerrationDetector (window 2, watermark 1, unbounded pushes
+// trimmed on pull) and VhsFieldLevelState (window 10, bounded pushes), kept by every
+// thread of the one workgroup (identical inputs, identical branches) so the rare
+// refinement from field-state levels can run on the whole workgroup.
+struct Avg {
+  double v[kAvgCap];
+  int n;
+  int window;
+  int watermark;
+  bool bounded;
+  __device__ bool has_values() const { return n > watermark; }
+  __device__ void push(double x) {
+    if (bounded && n == window) {
+      for (int i = 1; i < n; i++) v[i - 1] = v[i];
+      n--;
+    }
+    if (n == kAvgCap) {  // unbounded lists never exceed a few values between pulls
+      for (int i = 1; i < n; i++) v[i - 1] = v[i];
+      n--;
+    }
+    v[n++] = x;
+  }
+  __device__ bool pull(double* out) {
+#pragma clang fp contract(off)
+    if (n == 0) return false;
+    if (n >= window) {
+      const int drop = n - window;
+      for (int i = 0; i < window; i++) v[i] = v[i + drop];
+      n = window;
+    }
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += v[i];
+    *out = s / (double)n;
+    return true;
+  }
+};
+
+struct LevelChain {
+  Avg ser_s, ser_b, fs_s, fs_b;
+  int has_last;
+  double last_s, last_b;
+
+  __device__ void load(const LevelStateDev& s) {
+    ser_s.n = s.ser_n;
+    ser_b.n = s.ser_n;
+    for (int i = 0; i < s.ser_n; i++) {
+      ser_s.v[i] = s.ser_sync[i];
+      ser_b.v[i] = s.ser_blank[i];
+    }
+    fs_s.n = s.fs_sync_n;
+    fs_b.n = s.fs_blank_n;
+    for (int i = 0; i < s.fs_sync_n; i++) fs_s.v[i] = s.fs_sync[i];
+    for (int i = 0; i < s.fs_blank_n; i++) fs_b.v[i] = s.fs_blank[i];
+    has_last = s.has_last;
+    last_s = s.last_sync;
+    last_b = s.last_blank;
+  }
+  __device__ void store(LevelStateDev* s) const {
+    s->ser_n = ser_s.n;
+    for (int i = 0; i < ser_s.n; i++) {
+      s->ser_sync[i] = ser_s.v[i];
+      s->ser_blank[i] = ser_b.v[i];
+    }
+    s->fs_sync_n = fs_s.n;
+    s->fs_blank_n = fs_b.n;
+    for (int i = 0; i < fs_s.n; i++) s->fs_sync[i] = fs_s.v[i];
+    for (int i = 0; i < fs_b.n; i++) s->fs_blank[i] = fs_b.v[i];
+    s->has_last = has_last;
+    s->last_sync = last_s;
+    s->last_blank = last_b;
+  }
+  __device__ bool ser_pull(double* s, double* b) {
+    double a = 0.0, c2 = 0.0;
+    const bool hs = ser_s.pull(&a);
+    const bool hb = ser_b.pull(&c2);
+    *s = a;
+    *b = c2;
+    return hs && hb;
+  }
+  __device__ void ser_push(double s, double b) {
+    ser_s.push(s);
+    ser_b.push(b);
+  }
+  __device__ bool ser_has() const { return ser_s.has_values() && ser_b.has_values(); }
+  __device__ bool fs_has() const { return fs_s.has_values() && fs_b.has_values(); }
+  __device__ void fs_push(double s, double b) {
+    fs_b.push(b);
+    fs_s.push(s);
+  }
+  __device__ bool fs_pull(double* s, double* b) {
+    double bb;
+    if (!fs_b.pull(&bb)) return false;
+    double ss;
+    if (!fs_s.pull(&ss)) return false;
+    *s = ss;
+    *b = bb;
+    return true;
+  }
+  // TryResolveVhsSerrationRefinementFallback (TbcFieldDecodePipeline.cs:5679-5704)
+  __device__ bool fieldstate_fallback(int fail, bool* has, double* s, double* b) {
+    *has = false;
+    if (fail != kNonFinite && fail != kMissing) return false;
+    double tmp;
+    bool has_sync = false;
+    if (fail == kMissing) has_sync = fs_s.pull(&tmp);
+    *has = fs_pull(s, b);
+    if (fail == kMissing && !has_sync) return false;
+    return true;
+  }
+};
+
+__global__ void __launch_bounds__(kT) level_chain_kernel(SyncCfg c, SyncWork w) {
+#pragma clang fp contract(off)
+  __shared__ RefineScratch r;
+  LevelChain ch;
+  ch.ser_s = Avg{{}, 0, 2, 1, false};
+  ch.ser_b = Avg{{}, 0, 2, 1, false};
+  ch.fs_s = Avg{{}, 0, c.fs_window, 0, true};
+  ch.fs_b = Avg{{}, 0, c.fs_window, 0, true};
+  ch.load(*w.level_state_in);
+  const int nf = *w.spec_count;
+  const double ref_sync = c.ire0 + (c.hz_ire * c.vsync_ire);
+  for (int f = 0; f < nf; f++) {
+    LevelStateDev snap;
+    ch.store(&snap);
+    if (threadIdx.x == 0) w.level_before[f] = snap;
+    const Plane x{w.sync_ref + w.win_start[f], w.win_len[f], c.bias, 0.0};
+    const LevelSearch sr = w.search[f];
+    // VsyncSerrationDetector.Analyze with no measurement: pull (trims), HasLevels
+    double ps = 0.0, pb = 0.0;
+    const bool s_ok = ch.ser_pull(&ps, &pb);
+    const bool has_levels = ch.ser_has();
+    const bool has_pending = has_levels && s_ok;
+    // fallback search (first pass from the level-search kernel)
+    bool has_first = sr.first_ok != 0;
+    double fs_ = sr.first_sync, fb = sr.first_blank;
+    if (sr.first_has_measured) ch.fs_s.push(sr.first_measured);
+    bool from_search = has_first;
+    if (!has_first) {
+      bool hl;
+      double ls, lb;
+      if (ch.fieldstate_fallback(sr.first_fail, &hl, &ls, &lb)) {
+        has_first = hl;
+        fs_ = ls;
+        fb = lb;
+      }
+    }
+    bool pushed = false;
+    if (has_first) {
+      RefineOut rr;
+      if (from_search) {
+        rr = RefineOut{sr.ref_ok, sr.ref_sync, sr.ref_blank, sr.ref_fail, sr.ref_has_measured, sr.ref_measured};
+      } else {
+        int* pst = w.pulse_start + (long long)f * c.pulse_cap;
+        int* pln = w.pulse_len + (long long)f * c.pulse_cap;
+        rr = wg_refine_levels(x, fs_, fb, c, ref_sync, pst, pln, c.pulse_cap, w.edge_pos + (long long)f * c.edge_cap,
+                              w.edge_kind + (long long)f * c.edge_cap, c.edge_cap, &r, w.overflow + f);
+      }
+      if (rr.has_measured) ch.fs_s.push(rr.measured);
+      if (rr.ok) {
+        ch.fs_push(rr.sync, rr.blank);
+        ch.ser_push(rr.sync, rr.blank);
+        pushed = true;
+      } else {
+        bool hl;
+        double ls, lb;
+        if (ch.fieldstate_fallback(rr.fail, &hl, &ls, &lb) && hl) {
+          ch.ser_push(ls, lb);
+          pushed = true;
+        }
+      }
+    }
+    int source = 0;
+    double ss = 0.0, sb = 0.0;
+    bool done = false;
+    if (pushed) {
+      bool ok;
+      if (ch.ser_has()) {
+        double a, b;
+        const bool hp = ch.ser_pull(&a, &b);
+        ok = false;
+        if (hp) {
+          if (wg_check_levels(x, ref_sync, a, b, ref_sync, c.hz_ire, &r)) {
+            ss = a;
+            sb = b;
+            ok = true;
+          } else {
+            ok = ch.fs_pull(&ss, &sb);
+          }
+        }
+      } else {
+        ok = ch.fs_pull(&ss, &sb);
+      }
+      if (ok) {
+        source = 2;
+        done = true;
+      }
+    }
+    if (!done && has_pending) {
+      if (wg_check_levels(x, ref_sync, ps, pb, ref_sync, c.hz_ire, &r)) {
+        ss = ps;
+        sb = pb;
+        source = 1;
+        done = true;
+      } else if (ch.fs_pull(&ss, &sb)) {
+        source = 1;
+        done = true;
+      } else if (ch.ser_has() && !ch.fs_has()) {
+        ss = c.ire0;
+        sb = ref_sync;
+        source = 4;
+        done = true;
+      }
+    }
+    if (!done) {
+      double ts, tb;
+      bool has_term = false;
+      if (ch.fs_has() && ch.fs_pull(&ts, &tb)) has_term = true;
+      else if (ch.has_last) {
+        ts = ch.last_s;
+        tb = ch.last_b;
+        has_term = true;
+      }
+      if (has_term) {
+        ss = ts;
+        sb = tb;
+        source = 3;
+        done = true;
+      }
+    }
+    FieldLevels lv{};
+    if (done) {
+      if (source != 4) {
+        ch.has_last = 1;
+        ch.last_s = ss;
+        ch.last_b = sb;
+      }
+      lv.source = source;
+      lv.sync = ss;
+      lv.blank = sb;
+      const double dc = c.ire0 - sb;
+      if (!isfinite(dc) || dc == 0.0) {
+        lv.dc = 0.0;
+        lv.threshold = (ss + sb) / 2.0;
+      } else {
+        lv.dc = dc;
+        lv.threshold = ((ss + dc) + (sb + dc)) / 2.0;
+      }
+    } else {
+      lv.source = 0;
+      lv.dc = 0.0;
+      lv.threshold = c.ire0 + (c.hz_ire * (c.vsync_ire / 2.0));
+    }
+    if (threadIdx.x == 0) w.levels[f] = lv;
+  }
+  LevelStateDev fin;
+  ch.store(&fin);
+  if (threadIdx.x == 0) *w.level_state_after_all = fin;
+}
+
+// ------------------------------------------------------------------ S3: pulses
+__global__ void __launch_bounds__(kT) field_pulses_kernel(SyncCfg c, SyncWork w) {
+  __shared__ PulseScratch ps;
+  const int f = blockIdx.x;
+  if (f >= *w.spec_count) return;
+  const FieldLevels lv = w.levels[f];
+  const Plane x{w.sync_ref + w.win_start[f], w.win_len[f], c.bias, lv.dc};
+  int* pst = w.pulse_start + (long long)f * c.pulse_cap;
+  int* pln = w.pulse_len + (long long)f * c.pulse_cap;
+  const int np = wg_find_pulses(x, lv.threshold, c.pulse_min, c.pulse_max, 1e300, pst, pln, c.pulse_cap,
+                                w.edge_pos + (long long)f * c.edge_cap, w.edge_kind + (long long)f * c.edge_cap,
+                                c.edge_cap, &ps, nullptr, w.overflow + f);
+  if (threadIdx.x == 0) {
+    if (np > c.pulse_cap) w.overflow[f] = 1;
+    w.raw_count[f] = np < c.pulse_cap ? np : c.pulse_cap;
+  }
+}
+
+// ------------------------------------------------------------------ S4: per-field classification
+enum Kind { kHSync = 0, kEq = 1, kVSync = 2, kEq2 = 3 };
+
+struct Timing {
+  double hsync_lo, hsync_hi, eq_lo, eq_hi, vs_lo, vs_hi, median;
+};
+
+struct LogicScratch {
+  int rs[kPulseCap], rl[kPulseCap];            // raw pulses (rescue rewrites them)
+  int cs[kPulseCap], cl[kPulseCap];            // refined pulses
+  unsigned char ck[kPulseCap], co[kPulseCap];  // kind, in order
+  int hist[kHistBins];
+  int nref;
+};
+
+__device__ __forceinline__ int classify(int len, const Timing& t) {
+  if (in_range(len, t.hsync_lo, t.hsync_hi)) return kHSync;
+  if (in_range(len, t.eq_lo, t.eq_hi)) return kEq;
+  return in_range(len, t.vs_lo, t.vs_hi) ? kVSync : -1;
+}
+
+__device__ bool quality(int prev_kind, int prev_start, int kind, int start, double nominal) {
+#pragma clang fp contract(off)
+  double lo, hi;
+  if (prev_kind != kHSync && kind != kHSync) {
+    lo = 0.4;
+    hi = 0.6;
+  } else if (prev_kind == kHSync && kind == kHSync) {
+    lo = 0.9;
+    hi = 1.1;
+  } else {
+    lo = 0.4;
+    hi = 1.1;
+  }
+  return in_range((start - prev_start) / nominal, lo, hi);
+}
+
+__device__ __forceinline__ double round_half_line(double v) {
+#pragma clang fp contract(off)
+  const double h = round_away(v / 0.5) * 0.5;
+  return round_away(h * 10.0) / 10.0;
+}
+
+struct Bounds {
+  double first_hsync, first_eq2, last_hsync, last_eq2;
+};
+
+__device__ Bounds field_bounds(int pal, bool first) {
+  if (!pal) return first ? Bounds{1.0, 0.5, 0.5, 1.0} : Bounds{0.5, 1.0, 1.0, 0.5};
+  return first ? Bounds{0.5, 0.5, 1.0, 1.0} : Bounds{1.0, 1.0, 0.5, 0.5};
+}
+
+__device__ double first_hsync_line(int pal, int num_pulses, bool first) {
+#pragma clang fp contract(off)
+  const Bounds b = field_bounds(pal, first);
+  const double section = num_pulses / 2.0;
+  return b.first_hsync + (section * 3.0) - 0.5 + b.first_eq2;
+}
+
+struct Acc {
+  double loc, off;
+  int n;
+};
+
+__device__ Acc acc_pair(double fs_, double fl, double ss, double sl, double mll, double fhl) {
+#pragma clang fp contract(off)
+  const double actual = (fs_ - ss) / mll;
+  const double expected = fl - sl;
+  if (actual <= expected - 0.5 || actual >= expected + 0.5) return Acc{0.0, 0.0, 0};
+  return Acc{ss + (mll * (fhl - sl)), actual - expected, 1};
+}
+
+__device__ Acc acc_add(Acc a, Acc b) {
+#pragma clang fp contract(off)
+  return Acc{a.loc + b.loc, a.off + b.off, a.n + b.n};
+}
+
+// FinalizeEstimate (Dsp/VBlankSyncResolver.cs:666-718) over the refined pulses in LDS
+__device__ Estimate finalize(const LogicScratch* L, Acc a, double mll, double fhl, int cur_lines) {
+#pragma clang fp contract(off)
+  Estimate e{};
+  if (a.n == 0) return e;
+  double fh = round_away((a.loc + a.off) / a.n);
+  const double unaligned = fh;
+  double off = 0.0;
+  int cnt = 0;
+  for (int i = 0; i < L->nref; i++) {
+    if (L->ck[i] != kHSync || !L->co[i]) continue;
+    const double line = ((L->cs[i] - fh) / mll) + fhl;
+    const int rl = (int)round_away(line);
+    if (rl > cur_lines) break;
+    if (rl >= fhl) {
+      off += fh + (mll * (rl - fhl)) - L->cs[i];
+      cnt++;
+    }
+  }
+  if (cnt > 0) fh -= off / cnt;
+  e.ok = 1;
+  e.line0 = fh - (mll * fhl);
+  e.first_hsync = fh;
+  e.unaligned = unaligned;
+  e.valid_count = a.n;
+  return e;
+}
+
+__device__ void partial_markers(const double* smp, const int* has, double eq1_line, double section, double* ms,
+                                double* ml, int* nm) {
+#pragma clang fp contract(off)
+  const double lines[4] = {eq1_line, eq1_line + section, eq1_line + (section * 2.0),
+                           eq1_line + (section * 3.0) - 0.5};
+  int n = 0;
+  for (int i = 0; i < 4; i++)
+    if (has[i]) {
+      ms[n] = smp[i];
+      ml[n] = lines[i];
+      n++;
+    }
+  *nm = n;
+}
+
+__global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) {
+#pragma clang fp contract(off)
+  __shared__ LogicScratch L;
+  const int f = blockIdx.x;
+  if (f >= *w.spec_count) return;
+  const int np = w.raw_count[f];
+  const int* gst = w.pulse_start + (long long)f * c.pulse_cap;
+  const int* gln = w.pulse_len + (long long)f * c.pulse_cap;
+  for (int i = threadIdx.x; i < np; i += kT) {
+    L.rs[i] = gst[i];
+    L.rl[i] = gln[i];
+  }
+  for (int i = threadIdx.x; i < kHistBins; i += kT) L.hist[i] = 0;
+  __syncthreads();
+  // EstimateTiming (:131-166): median of the hsync-like lengths by a histogram
+  const double hs_lo = (c.hsync_us - 1.75) * c.fs_mhz, hs_hi = (c.hsync_us + 2.0) * c.fs_mhz;
+  const int hbase = (int)ceil(hs_lo);
+  for (int i = threadIdx.x; i < np; i += kT) {
+    const int len = L.rl[i];
+    if (in_range(len, hs_lo, hs_hi)) {
+      const int b = len - hbase;
+      if (b >= 0 && b < kHistBins) atomicAdd(&L.hist[b], 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  FieldLogic out{};
+  int nh = 0;
+  for (int b = 0; b < kHistBins; b++) nh += L.hist[b];
+  double med;
+  if (nh > 0) {
+    auto kth = [&](int k) {
+      int acc = 0;
+      for (int b = 0; b < kHistBins; b++) {
+        acc += L.hist[b];
+        if (k < acc) return (double)(b + hbase);
+      }
+      return 0.0;
+    };
+    const int mid = nh / 2;
+    med = (nh & 1) ? kth(mid) : (kth(mid - 1) + kth(mid)) / 2.0;
+  } else {
+    med = c.hsync_us;  // v0.4.0 keeps the empty-list fallback in microseconds
+  }
+  const double typical = c.hsync_us * c.fs_mhz;
+  const double toff = med - typical;
+  Timing t;
+  t.median = med;
+  t.hsync_lo = med - (c.hsync_tol * c.fs_mhz);
+  t.hsync_hi = med + (c.hsync_tol * c.fs_mhz);
+  t.eq_lo = ((c.eq_us - c.eq_tol) * c.fs_mhz) + toff;
+  t.eq_hi = ((c.eq_us + c.eq_tol) * c.fs_mhz) + toff;
+  t.vs_lo = ((c.vsync_us * 0.5) * c.fs_mhz) + toff;
+  t.vs_hi = ((c.vsync_us + 1.0) * c.fs_mhz) + toff;
+  out.hsync_median = med;
+  // RefinePulses (:243-314) with the long-hsync rescue on the DC-shifted reference
+  const FieldLevels lv = w.levels[f];
+  const Plane x{w.sync_ref + w.win_start[f], w.win_len[f], c.bias, lv.dc};
+  const double nominal = c.nominal;
+  const double step = 10.0 * c.hz_ire;
+  int nref = 0;
+  auto add = [&](int kind, int st, int ln) {
+    const bool io = nref > 0 && quality(L.ck[nref - 1], L.cs[nref - 1], kind, st, nominal);
+    if (nref < kPulseCap) {
+      L.ck[nref] = (unsigned char)kind;
+      L.cs[nref] = st;
+      L.cl[nref] = ln;
+      L.co[nref] = io ? 1 : 0;
+      nref++;
+    }
+  };
+  int index = 0;
+  while (index < np) {
+    const int ps_ = L.rs[index], pl_ = L.rl[index];
+    if (in_range(pl_, t.hsync_lo, t.hsync_hi)) {
+      add(kHSync, ps_, pl_);
+      index++;
+      continue;
+    }
+    if (x.n > 0 && step > 0.0 && in_range(pl_, t.hsync_hi, t.hsync_hi * 3.0)) {
+      // TryRescueLongHSync (:316-338): first pulse below (first sample - step) inside the pulse
+      long long s = ps_ < 0 ? 0 : (ps_ > x.n ? x.n : ps_);
+      long long e = (long long)ps_ + pl_;
+      e = e < s ? s : (e > x.n ? x.n : e);
+      bool rescued = false;
+      int rs_ = 0, rl_ = 0;
+      if (s < e) {
+        const double thr = x.at(s) - step;
+        const int mn = (int)ceil((c.eq_us * c.fs_mhz) / 8.0) > 0 ? (int)ceil((c.eq_us * c.fs_mhz) / 8.0) : 0;
+        const int mx = (int)(nominal * 5.0) > 1 ? (int)(nominal * 5.0) : 1;
+        bool inp = x.at(s) <= thr;
+        long long cur = 0;
+        for (long long p = 0; p < e - s && !rescued; p++) {
+          const double v = x.at(s + p);
+          if (inp) {
+            if (v > thr) {
+              const long long len = p - cur;
+              if (in_range((double)len, mn, mx) && cur != 0) {
+                rescued = true;
+                rs_ = (int)(s + cur);
+                rl_ = (int)len;
+              }
+              inp = false;
+            }
+          } else if (v <= thr) {
+            cur = p;
+            inp = true;
+          }
+        }
+      }
+      if (rescued && (rs_ != ps_ || rl_ != pl_)) {
+        L.rs[index] = rs_;
+        L.rl[index] = rl_;
+        continue;
+      }
+      index++;
+      continue;
+    }
+    const bool can_start = index > 2 && in_range(pl_, t.eq_lo, t.eq_hi) && nref > 0 && L.ck[nref - 1] == kHSync;
+    if (!can_start) {
+      index++;
+      continue;
+    }
+    // TryReadVBlank (:340-471)
+    const int start = index - 2 > 0 ? index - 2 : 0;
+    const int end_ex = np < index + 24 ? np : index + 24;
+    int vk[32], vs[32], vl[32];
+    unsigned char vo[32];
+    int nv = 0, state = -1;
+    double state_end = 0.0, state_len = 0.0;
+    bool has_len = false, done = false;
+    for (int i = start; i < end_ex && !done; i++) {
+      const int rk = classify(L.rl[i], t);
+      if (rk < 0) continue;
+      const int pst = L.rs[i];
+      int acc = -1;
+      bool fin = false;
+      switch (state) {
+        case -1:
+          if (rk == kHSync) acc = kHSync;
+          break;
+        case kHSync:
+          if (rk == kHSync) acc = kHSync;
+          else if (rk == kEq) {
+            acc = kEq;
+            has_len = true;
+            state_len = c.num_pulses / 2.0;
+          } else if (rk == kVSync)
+            acc = kVSync;
+          break;
+        case kEq:
+          if (rk == kEq) acc = kEq;
+          else if (rk == kVSync) {
+            acc = kVSync;
+            has_len = true;
+            state_len = c.num_pulses / 2.0;
+          } else if (rk == kHSync)
+            acc = kHSync;
+          break;
+        case kVSync:
+          if (rk == kEq) {
+            acc = kEq2;
+            has_len = true;
+            state_len = c.num_pulses / 2.0;
+          } else if (rk == kVSync)
+            acc = kVSync;
+          else if (rk == kHSync && pst > state_end)
+            acc = kHSync;
+          break;
+        case kEq2:
+          if (rk == kEq) acc = kEq2;
+          else if (rk == kHSync) {
+            acc = kHSync;
+            fin = true;
+          }
+          break;
+      }
+      if (acc < 0) continue;
+      if (acc != state) {
+        if (pst < state_end) acc = -1;
+        else if (has_len) {
+          state_end = pst + ((state_len - 0.1) * nominal);
+          has_len = false;
+        }
+      }
+      if (acc >= 0) {
+        const bool io = nv > 0 && quality(vk[nv - 1], vs[nv - 1], acc, pst, nominal);
+        if (nv < 32) {
+          vk[nv] = acc;
+          vs[nv] = pst;
+          vl[nv] = L.rl[i];
+          vo[nv] = io ? 1 : 0;
+          nv++;
+        }
+        state = acc;
+      }
+      if (fin) done = true;
+    }
+    if (!done) {
+      index++;
+      continue;
+    }
+    for (int k = 2; k < nv; k++) {
+      if (nref < kPulseCap) {
+        L.ck[nref] = (unsigned char)vk[k];
+        L.cs[nref] = vs[k];
+        L.cl[nref] = vl[k];
+        L.co[nref] = vo[k];
+        nref++;
+      }
+    }
+    index += (nv - 2 > 1 ? nv - 2 : 1);
+  }
+  L.nref = nref;
+  out.refined = nref;
+  // ComputeMeanLineLength (:490-538)
+  {
+    int run_start = -1, run_len = -1, cur_start = -1, cur_len = 0;
+    for (int i = 0; i < nref; i++) {
+      if (L.ck[i] != kHSync) {
+        if (cur_start >= 0 && cur_len > run_len) {
+          run_start = cur_start;
+          run_len = cur_len;
+        }
+        cur_start = -1;
+        cur_len = 0;
+      } else if (cur_start < 0) {
+        cur_start = i;
+        cur_len = 0;
+      } else {
+        cur_len++;
+      }
+    }
+    if (cur_start >= 0 && cur_len > run_len) {
+      run_start = cur_start;
+      run_len = cur_len;
+    }
+    double sum = 0.0;
+    int cnt = 0;
+    for (int i = run_start + 1; i < run_start + run_len; i++) {
+      const double ll = (double)(L.cs[i] - L.cs[i - 1]);
+      if (in_range(ll / nominal, 0.95, 1.05)) {
+        sum += ll;
+        cnt++;
+      }
+    }
+    out.mll = cnt > 0 ? sum / (double)cnt : nominal;
+  }
+  const double mll = out.mll;
+  // MeasureVBlankBoundaryConsensus (FieldParityDetector.cs:172-253)
+  {
+    double lens[4] = {-1.0, -1.0, -1.0, -1.0};
+    int group = 0, last = -1;
+    if (nref > 0) {
+      const double second = L.cs[0] + (c.field_lines0 * mll);
+      for (int i = 0; i < nref; i++) {
+        if (last >= 0 && L.co[i]) {
+          if (group == 0 && L.cs[i] > second) group = 2;
+          if (L.ck[last] == kHSync && L.ck[i] != kHSync) lens[group] = round_half_line((L.cs[i] - L.cs[last]) / mll);
+          else if (L.ck[last] != kHSync && L.ck[i] == kHSync)
+            lens[group + 1] = round_half_line((L.cs[i] - L.cs[last]) / mll);
+        }
+        last = i;
+      }
+    }
+    const double ffn[4] = {1.0, 0.5, 0.5, 1.0}, sfn[4] = {0.5, 1.0, 1.0, 0.5};
+    const double ffp[4] = {0.5, 0.5, 1.0, 1.0}, sfp[4] = {1.0, 1.0, 0.5, 0.5};
+    const double* ff = c.is_pal ? ffp : ffn;
+    const double* sf = c.is_pal ? sfp : sfn;
+    int cons = 0, det = 0;
+    for (int i = 0; i < 4; i++) {
+      if (lens[i] < 0.0) continue;
+      if (fabs(lens[i] - ff[i]) < 1e-9) {
+        cons++;
+        det++;
+      } else if (fabs(lens[i] - sf[i]) < 1e-9) {
+        det++;
+      }
+    }
+    out.consensus = cons;
+    out.detected = det;
+    if (det > 0) {
+      const double wgt = det / 4.0;
+      out.first_conf = (int)round_away((cons / (double)det) * wgt * 100.0);
+      out.second_conf = (int)round_away(((det - cons) / (double)det) * wgt * 100.0);
+    }
+  }
+  // EstimateLine0FromTransitions (VBlankSyncResolver.cs:317-445) for both parities
+  {
+    double fsm[4] = {0, 0, 0, 0}, lsm[4] = {0, 0, 0, 0};
+    int fh[4] = {0, 0, 0, 0}, lh[4] = {0, 0, 0, 0};
+    double* as = fsm;
+    int* ah = fh;
+    int last = -1;
+    for (int i = 0; i < nref; i++) {
+      if (last >= 0 && L.co[i]) {
+        if (as == fsm && L.cs[i] > L.cs[0] + (c.field_lines0 * mll)) {
+          as = lsm;
+          ah = lh;
+        }
+        const int pk = L.ck[last], ck = L.ck[i];
+        if (pk == kHSync && ck != kHSync) {
+          as[0] = L.cs[i];
+          ah[0] = 1;
+        } else if (pk == kEq && ck == kVSync) {
+          as[1] = L.cs[i];
+          ah[1] = 1;
+        } else if (pk == kVSync && ck == kEq2) {
+          as[2] = L.cs[i];
+          ah[2] = 1;
+        } else if (pk != kHSync && ck == kHSync) {
+          as[3] = L.cs[last];
+          ah[3] = 1;
+        }
+      }
+      last = i;
+    }
+    for (int par = 0; par < 2; par++) {
+      const bool first = par == 0;
+      const int cur_lines = first ? c.field_lines0 : c.field_lines1;
+      const Bounds b = field_bounds(c.is_pal, first);
+      const double section = c.num_pulses / 2.0;
+      const double fhl = b.first_hsync + (section * 3.0) - 0.5 + b.first_eq2;
+      double fms[4], fml[4], lms[4], lml[4];
+      int nfm, nlm;
+      partial_markers(fsm, fh, b.first_hsync, section, fms, fml, &nfm);
+      partial_markers(lsm, lh, cur_lines + b.last_hsync, section, lms, lml, &nlm);
+      Acc fa{0.0, 0.0, 0}, la{0.0, 0.0, 0};
+      for (int i = 0; i < nfm; i++)
+        for (int j = i + 1; j < nfm; j++) fa = acc_add(fa, acc_pair(fms[i], fml[i], fms[j], fml[j], mll, fhl));
+      for (int i = 0; i < nlm; i++)
+        for (int j = i + 1; j < nlm; j++) la = acc_add(la, acc_pair(lms[i], lml[i], lms[j], lml[j], mll, fhl));
+      Transitions& tr = out.tr[par];
+      tr.first = finalize(&L, fa, mll, fhl, cur_lines);
+      tr.last = finalize(&L, la, mll, fhl, cur_lines);
+      tr.combined = Estimate{};
+      if (fa.n > 0 && la.n > 0) {
+        const double fr = fa.loc / fa.n, lr = la.loc / la.n;
+        if (fabs(fr - lr) < 0.5 * mll) {
+          Acc cr{0.0, 0.0, 0};
+          for (int i = 0; i < nfm; i++)
+            for (int j = 0; j < nlm; j++) cr = acc_add(cr, acc_pair(fms[i], fml[i], lms[j], lml[j], mll, fhl));
+          tr.combined = finalize(&L, acc_add(acc_add(fa, la), cr), mll, fhl, cur_lines);
+        }
+      }
+    }
+  }
+  // the refined pulses for the line locations and the rare previous-field path
+  int* ks = w.ref_kind + (long long)f * c.pulse_cap;
+  int* ss = w.ref_start + (long long)f * c.pulse_cap;
+  for (int i = 0; i < nref; i++) {
+    ks[i] = L.ck[i] | (L.co[i] << 8);
+    ss[i] = L.cs[i];
+  }
+  out.raw = np;
+  w.logic[f] = out;
+}
+
+// ------------------------------------------------------------------ S5: field chain
+// AlignFirstHSyncToValidPulses (TbcFieldDecodePipeline.cs:6207-6242) over the stored refined pulses
+__device__ double align_first_hsync(const int* ks, const int* ss, int n, double fh, double mll, double fhl,
+                                    int cur_lines) {
+#pragma clang fp contract(off)
+  double off = 0.0;
+  int cnt = 0;
+  for (int i = 0; i < n; i++) {
+    if ((ks[i] & 0xff) != kHSync || !(ks[i] >> 8)) continue;
+    const double line = ((ss[i] - fh) / mll) + fhl;
+    const int rl = (int)round_away(line);
+    if (rl > cur_lines) break;
+    if (rl >= fhl) {
+      off += fh + (mll * (rl - fhl)) - ss[i];
+      cnt++;
+    }
+  }
+  return cnt > 0 ? fh - (off / cnt) : fh;
+}
+
+__device__ void resolve_window(const SyncCfg& c, long long begin, long long* ws, long long* wl) {
+  long long read = begin - c.block_cut;
+  read = read > 0 ? read : 0;
+  const long long a0 = (read / c.block_length) * c.block_length;
+  const long long a1 = a0 + c.read_samples;
+  const long long fb = a0 / c.stride, lb = a1 / c.stride;
+  *ws = fb * c.stride;
+  *wl = (lb - fb + 1) * c.stride;
+}
+
+__global__ void field_chain_kernel(SyncCfg c, SyncWork w) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0) return;
+  SyncHistoryDev h = *w.history_in;
+  h.prev_first_hsync_read -= w.state_shift;
+  const int nf = *w.spec_count;
+  long long begin = w.first_begin;
+  int out_count = 0;
+  int stop_status = 0;
+  for (int f = 0; f < nf; f++) {
+    FieldRecord& rec = w.records[f];
+    rec = FieldRecord{};
+    long long ws, wl;
+    resolve_window(c, begin, &ws, &wl);
+    rec.begin = begin;
+    rec.window_start = ws;
+    rec.window_length = wl;
+    if (ws != w.win_start[f] || wl != w.win_len[f]) {
+      rec.status = kStatusMispredicted;
+      stop_status = rec.status;
+      break;
+    }
+    const FieldLevels lv = w.levels[f];
+    const FieldLogic lg = w.logic[f];
+    rec.level_source = lv.source;
+    rec.sync_level = lv.sync;
+    rec.blank_level = lv.blank;
+    rec.dc_offset = lv.dc;
+    rec.threshold = lv.threshold;
+    rec.raw_pulses = lg.raw;
+    rec.refined_pulses = lg.refined;
+    rec.hsync_median = lg.hsync_median;
+    rec.mean_line_length = lg.mll;
+    if (w.overflow[f]) {
+      rec.status = kStatusCapacity;
+      stop_status = rec.status;
+      break;
+    }
+    if (lg.raw == 0) {
+      rec.status = h.pulse_search_initialized ? 2 : 1;
+      stop_status = rec.status;
+      break;
+    }
+    h.pulse_search_initialized = 1;
+    const double mll = lg.mll;
+    // FieldParityDetector.ResolveCadence, no fallback line
+    bool first;
+    int conf = 0;
+    if (h.prev_first >= 0) first = !h.prev_first;
+    else first = lg.detected == 0 || round_away(lg.consensus / (double)lg.detected) == 1.0;
+    int eff = 100;
+    if (lg.detected > 0 && !h.has_prev_first_hsync && eff > 50) eff = 50;
+    if (lg.first_conf >= eff && lg.first_conf > lg.second_conf) {
+      first = true;
+      conf = lg.first_conf;
+    } else if (lg.second_conf >= eff && lg.first_conf < lg.second_conf) {
+      first = false;
+      conf = lg.second_conf;
+    }
+    rec.is_first_field = first ? 1 : 0;
+    rec.parity_confidence = conf;
+    h.prev_first = first ? 1 : 0;
+    const int cur_lines = first ? c.field_lines0 : c.field_lines1;
+    const double fhl = first_hsync_line(c.is_pal, c.num_pulses, first);
+    // EstimateFirstHSyncFromPrevious (:6146-6173)
+    bool has_est = false;
+    double est = 0.0;
+    if (h.has_prev_first_hsync && h.prev_first_hsync > 0.0 && mll > 0.0) {
+      const double cur = first ? c.field_lines0 : c.field_lines1;
+      const double prev = first ? c.field_lines1 : c.field_lines0;
+      const double est_lines = c.is_pal ? cur : prev;
+      const double last_off = (h.prev_first_hsync_read - ws) / mll;
+      est = round_away((last_off + est_lines + (h.prev_first_hsync / mll)) * mll);
+      has_est = true;
+    }
+    // ResolveLine0Location, vhs (:5452-5651)
+    const Transitions& tr = lg.tr[first ? 0 : 1];
+    bool ok = false, used_prev = false;
+    double line0 = 0.0, fh = 0.0, unaligned = 0.0;
+    if (tr.combined.ok) {
+      ok = true;
+      line0 = tr.combined.line0;
+      fh = tr.combined.first_hsync;
+      unaligned = tr.combined.unaligned;
+    } else {
+      const int fc = tr.first.ok ? tr.first.valid_count : 0;
+      const int lc = tr.last.ok ? tr.last.valid_count : 0;
+      const bool has_prev = h.has_prev_first_hsync && h.prev_first_hsync > 0.0;
+      const Estimate* single = nullptr;
+      if (fc == 6 || (!has_prev && fc > lc)) single = tr.first.ok ? &tr.first : nullptr;
+      else if (lc == 6 || (!has_prev && lc > fc)) single = tr.last.ok ? &tr.last : nullptr;
+      if (single) {
+        ok = true;
+        line0 = single->line0;
+        fh = single->first_hsync;
+        unaligned = single->unaligned;
+      } else if (has_est) {
+        double e = est;
+        if (h.prev_hsync_difference >= -0.5 && h.prev_hsync_difference <= 0.5) e += mll * h.prev_hsync_difference;
+        const int* ks = w.ref_kind + (long long)f * c.pulse_cap;
+        const int* ss = w.ref_start + (long long)f * c.pulse_cap;
+        if (e <= 0.0) e = lg.refined > 0 ? ss[0] : 0.0;
+        unaligned = round_away(e);
+        fh = align_first_hsync(ks, ss, lg.refined, unaligned, mll, fhl, cur_lines);
+        line0 = fh - (fhl * mll);
+        ok = true;
+        used_prev = true;
+      }
+    }
+    if (!ok) {
+      rec.status = 2;
+      stop_status = rec.status;
+      break;
+    }
+    // UpdateSyncHistory (:6119-6144)
+    if (!used_prev && has_est && isfinite(unaligned)) h.prev_hsync_difference = (unaligned - est) / mll;
+    if (isfinite(fh)) {
+      h.prev_first_hsync_read = ws;
+      h.prev_first_hsync = fh;
+      h.has_prev_first_hsync = 1;
+    }
+    rec.line0 = line0;
+    rec.first_hsync = fh;
+    rec.first_hsync_line = fhl;
+    const int processed = c.output_lines + 10;
+    const double last_line = (((double)wl - line0) / mll) - 1.0;
+    if (last_line < processed) {
+      rec.status = 3;
+      stop_status = rec.status;
+      break;
+    }
+    // ComputeVhsNextFieldOffsetSamples (:2299-2325)
+    const Bounds b = field_bounds(c.is_pal, first);
+    const double next_loc = fh + (mll * ((cur_lines + b.last_hsync) - fhl));
+    double next_off = next_loc - (8.0 * c.nominal) + (double)c.block_cut;
+    if (!(isfinite(next_off) && next_off > 0.0)) {
+      const int fbl = c.output_lines - 7 > 0 ? c.output_lines - 7 : 0;
+      next_off = line0 + (fbl * mll) + (double)c.block_cut;  // line-location fallback, projected
+      next_off = next_off > 1.0 ? next_off : 1.0;
+    }
+    rec.next_offset = next_off;
+    const long long adv = (long long)round_away(next_off);
+    rec.next_begin = ws + (adv > 1 ? adv : 1);
+    rec.status = 0;
+    begin = rec.next_begin;
+    out_count++;
+    // the level state after this field
+    w.level_state_out[0] = f + 1 < nf ? w.level_before[f + 1] : *w.level_state_after_all;
+    *w.history_out = h;
+  }
+  if (out_count == 0) {
+    w.level_state_out[0] = *w.level_state_in;
+    SyncHistoryDev h0 = *w.history_in;
+    h0.prev_first_hsync_read -= w.state_shift;
+    *w.history_out = h0;
+  }
+  *w.field_count = out_count;
+  *w.next_begin_out = begin;
+  *w.stop_status = stop_status;
+}
+
+// ------------------------------------------------------------------ S6: line locations
+// BuildUpstreamLineLocations (SyncAnalysis.cs:585-657) from trunc(first hsync) at line
+// FirstHSyncLine, written window-relative for every processed line.
+__global__ void __launch_bounds__(kT) field_lines_kernel(SyncCfg c, SyncWork w) {
+#pragma clang fp contract(off)
+  __shared__ int srt[kPulseCap];
+  const int f = blockIdx.x;
+  if (f >= *w.field_count) return;
+  const FieldRecord rec = w.records[f];
+  const int n = rec.refined_pulses;
+  const int* ss = w.ref_start + (long long)f * c.pulse_cap;
+  for (int i = threadIdx.x; i < n; i += kT) srt[i] = ss[i];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int i = 1; i < n; i++) {  // nearly sorted: insertion sort (OrderBy is stable; equal keys are equal values)
+    const int v = srt[i];
+    int j = i - 1;
+    while (j >= 0 && srt[j] > v) {
+      srt[j + 1] = srt[j];
+      j--;
+    }
+    srt[j + 1] = v;
+  }
+  const double mll = rec.mean_line_length;
+  const double ref_pulse = trunc(rec.first_hsync);
+  const int ref_line = (int)rec.first_hsync_line;
+  const int lines = c.output_lines + 10;
+  double* out = w.lines + (long long)f * c.line_stride;
+  int cur = 0;
+  const double maxd = mll / 1.5;
+  for (int line = 0; line < lines; line++) {
+    const double expected = ref_pulse + (mll * (line - ref_line));
+    out[line] = expected;
+    if (cur >= n) continue;
+    double cd = fabs(srt[cur] - expected);
+    double smallest = maxd;
+    double sel = -1.0;
+    int si = cur;
+    while (si < n - 1) {
+      if (cd <= smallest) {
+        smallest = cd;
+        cur = si;
+        sel = srt[si];
+      }
+      const double nd = fabs(srt[si + 1] - expected);
+      if (nd > cd) break;
+      cd = nd;
+      si++;
+    }
+    if (sel >= 0.0) {
+      out[line] = sel;
+      cur++;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ S0: speculated windows
+// The read windows of the span's fields: the first from the caller's start, each next one
+// three-and-a-bit lines (the equalizing section) before the next vertical-sync group of
+// broad pulses, less eight lines, plus the block cut, as the next-field offset places it.
+__global__ void __launch_bounds__(kT) spec_windows_kernel(SyncCfg c, SyncWork w, const int* starts, const int* lens,
+                                                          const int* pulse_count, int capacity, long long span_len,
+                                                          int max_fields) {
+#pragma clang fp contract(off)
+  __shared__ int groups[kMaxGroups];
+  __shared__ int ng;
+  __shared__ int red[kT / 64 + 4];
+  const int pc = *pulse_count < capacity ? *pulse_count : capacity;
+  const double broad = 0.5 * ((c.hsync_us + c.vsync_us) * c.fs_mhz);
+  if (threadIdx.x == 0) ng = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < pc; b0 += kT) {
+    const int k = b0 + threadIdx.x;
+    const bool g = k < pc && lens[k] > broad && (k == 0 || lens[k - 1] <= broad);
+    int tot;
+    const int idx = wg_exclusive(g ? 1 : 0, red, &tot) + ng;
+    if (g && idx < kMaxGroups) groups[idx] = starts[k];
+    __syncthreads();
+    if (threadIdx.x == 0) ng += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const int n_groups = ng < kMaxGroups ? ng : kMaxGroups;
+  const double line = c.nominal;
+  const double eq_lines = c.num_pulses / 2.0;
+  long long begin = w.first_begin;
+  int nf = 0;
+  int g = 0;
+  for (int f = 0; f < max_fields; f++) {
+    long long ws, wl;
+    resolve_window(c, begin, &ws, &wl);
+    if (ws + wl > span_len) break;
+    w.win_start[f] = ws;
+    w.win_len[f] = wl;
+    nf++;
+    // this field's vertical sync: the first group at least two lines into its window
+    while (g < n_groups && groups[g] < ws + 2.0 * line) g++;
+    if (g + 1 >= n_groups) break;
+    const double next_eq1 = groups[g + 1] - eq_lines * line;
+    begin = (long long)round_away(next_eq1 - 8.0 * line + (double)c.block_cut);
+    g++;
+  }
+  *w.spec_count = nf;
+}
+
+// ------------------------------------------------------------------ sequence state and tails
+__global__ void reset_kernel(LevelStateDev* level, SyncHistoryDev* hist) {
+  if (threadIdx.x != 0) return;
+  *level = LevelStateDev{};
+  SyncHistoryDev h{};
+  h.prev_first = -1;
+  h.prev_hsync_difference = -1.0;
+  *hist = h;
+}
+
+__global__ void tail_tables_kernel(SyncCfg c, SyncWork w, FieldTail t, int max_fields) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= max_fields) return;
+  const int n = *w.field_count;
+  const int lines = c.output_lines + 10;
+  if (f >= n) {
+    t.line_counts[f] = 0;
+    return;
+  }
+  const FieldRecord rec = w.records[f];
+  t.spans[f] = w.sync_ref + rec.window_start;
+  t.span_len[f] = rec.window_length;
+  t.locs[f] = w.lines + (long long)f * c.line_stride;
+  t.filled[f] = t.zeros;
+  t.refined[f] = t.refined_base + (long long)f * c.line_stride;
+  t.filled_out[f] = t.filled_out_base + (long long)f * c.line_stride;
+  t.line_counts[f] = lines;
+  t.origins[f] = rec.window_start;
+  t.first_lines[f] = t.out_first_line[rec.is_first_field ? 0 : 1];
+  t.threshold[f] = rec.threshold;
+  t.dc[f] = rec.dc_offset;
+}
+
+// ApplyNtscFscPhaseShiftCore (TbcFieldDecodePipeline.cs:4022-4038): every location less the shift
+__global__ void render_kernel(SyncCfg c, SyncWork w, FieldTail t, int max_fields) {
+#pragma clang fp contract(off)
+  const int f = blockIdx.y;
+  if (f >= *w.field_count) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lines = c.output_lines + 10;
+  if (i >= lines) return;
+  const double v = t.refined_base[(long long)f * c.line_stride + i];
+  t.render[(long long)f * c.line_stride + i] = t.fsc_shift != 0.0 ? v - t.fsc_shift : v;
+}
+
+}  // namespace vsync
+}  // namespace vhsd
+
+extern "C" hipError_t vhsd_launch_vhs_sync_reset(vhsd::vsync::LevelStateDev* level, vhsd::vsync::SyncHistoryDev* hist,
+                                                 hipStream_t stream) {
+  hipLaunchKernelGGL(vhsd::vsync::reset_kernel, dim3(1), dim3(64), 0, stream, level, hist);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t vhsd_launch_vhs_tail_tables(const vhsd::vsync::SyncCfg* c, const vhsd::vsync::SyncWork* w,
+                                                  const vhsd::vsync::FieldTail* t, int max_fields,
+                                                  hipStream_t stream) {
+  hipLaunchKernelGGL(vhsd::vsync::tail_tables_kernel, dim3((max_fields + 63) / 64), dim3(64), 0, stream, *c, *w, *t,
+                     max_fields);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t vhsd_launch_vhs_render(const vhsd::vsync::SyncCfg* c, const vhsd::vsync::SyncWork* w,
+                                             const vhsd::vsync::FieldTail* t, int max_fields, hipStream_t stream) {
+  const int lines = c->output_lines + 10;
+  hipLaunchKernelGGL(vhsd::vsync::render_kernel, dim3((lines + 255) / 256, max_fields), dim3(256), 0, stream, *c, *w,
+                     *t, max_fields);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t vhsd_launch_vhs_sync(const vhsd::vsync::SyncCfg* c, const vhsd::vsync::SyncWork* w,
+                                           const int* starts, const int* lens, const int* pulse_count, int capacity,
+                                           long long span_len, int max_fields, hipStream_t stream) {
+  using namespace vhsd::vsync;
+  if (max_fields <= 0) return hipSuccess;
+  if (c->pulse_cap > kPulseCap || c->output_lines + 10 > c->line_stride) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(spec_windows_kernel, dim3(1), dim3(kT), 0, stream, *c, *w, starts, lens, pulse_count, capacity,
+                     span_len, max_fields);
+  hipLaunchKernelGGL(level_search_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
+  hipLaunchKernelGGL(level_chain_kernel, dim3(1), dim3(kT), 0, stream, *c, *w);
+  hipLaunchKernelGGL(field_pulses_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
+  hipLaunchKernelGGL(field_logic_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
+  hipLaunchKernelGGL(field_chain_kernel, dim3(1), dim3(64), 0, stream, *c, *w);
+  hipLaunchKernelGGL(field_lines_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
+  return hipGetLastError();
+}

@@ -1,0 +1,158 @@
+// Device structures of the VHS field sync path (vhs_sync.hip) shared with the host.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vhsdecode_hip.h"
+
+namespace vhsd {
+namespace vsync {
+
+constexpr int kPulseCap = 4096;   // pulses kept per field window
+constexpr int kHistBins = 512;    // hsync-length histogram (3.75 us of lengths)
+constexpr int kAvgCap = 16;       // moving-average values kept
+constexpr int kMaxGroups = 256;   // vertical-sync groups per span
+constexpr int kStatusMispredicted = 4;
+constexpr int kStatusCapacity = 5;
+
+struct SyncCfg {
+  double fs_mhz;
+  double hsync_us, eq_us, vsync_us;
+  double hsync_tol, eq_tol;          // microseconds
+  int num_pulses;
+  int is_pal;
+  double ire0, hz_ire, vsync_ire;
+  double nominal;                    // SyncAnalyzer.NominalLineLength
+  int field_lines0, field_lines1;    // FieldLinesForParityDetection
+  int output_lines;                  // TbcFrameSpec.OutputLineCount
+  long long block_length, block_cut, stride, read_samples;
+  int ser_eq_len, ser_line_len;      // VsyncSerrationDetector original-rate lengths
+  int pulse_min, pulse_max;          // FindRawPulses bounds for vhs
+  int fs_window;                     // VhsFieldLevelState window
+  double bias;                       // plane bias (Hz)
+  int pulse_cap, edge_cap;           // per-field list capacities
+  int line_stride;                   // line locations per field in the output table
+};
+
+struct LevelStateDev {
+  double ser_sync[kAvgCap], ser_blank[kAvgCap];
+  int ser_n;
+  double fs_sync[kAvgCap], fs_blank[kAvgCap];
+  int fs_sync_n, fs_blank_n;
+  int has_last;
+  double last_sync, last_blank;
+};
+
+struct SyncHistoryDev {
+  int prev_first;                    // -1: none
+  int has_prev_first_hsync;
+  double prev_first_hsync;
+  long long prev_first_hsync_read;   // window start of that field (span coordinates)
+  double prev_hsync_difference;
+  int pulse_search_initialized;
+};
+
+struct LevelSearch {
+  int first_ok;
+  double first_sync, first_blank;
+  int first_fail, first_has_measured;
+  double first_measured;
+  int ref_ok;
+  double ref_sync, ref_blank;
+  int ref_fail, ref_has_measured;
+  double ref_measured;
+};
+
+struct FieldLevels {
+  int source;
+  double sync, blank, dc, threshold;
+};
+
+struct Estimate {
+  int ok;
+  double line0, first_hsync, unaligned;
+  int valid_count;
+};
+
+struct Transitions {
+  Estimate first, last, combined;
+};
+
+struct FieldLogic {
+  int raw, refined;
+  double hsync_median, mll;
+  int consensus, detected, first_conf, second_conf;
+  Transitions tr[2];  // first field, second field
+};
+
+typedef vhsdecode_hip_vhs_field_v1 FieldRecord;
+
+struct SyncWork {
+  const float* sync_ref;
+  long long first_begin;
+  long long state_shift;   // span coordinates of the incoming history less this span's
+  int* spec_count;
+  long long* win_start;
+  long long* win_len;
+  int* pulse_start;
+  int* pulse_len;
+  int* edge_pos;
+  unsigned char* edge_kind;
+  int* overflow;
+  LevelSearch* search;
+  FieldLevels* levels;
+  LevelStateDev* level_before;
+  const LevelStateDev* level_state_in;
+  LevelStateDev* level_state_out;
+  LevelStateDev* level_state_after_all;
+  int* raw_count;
+  FieldLogic* logic;
+  int* ref_kind;
+  int* ref_start;
+  FieldRecord* records;
+  int* field_count;
+  long long* next_begin_out;
+  int* stop_status;
+  const SyncHistoryDev* history_in;
+  SyncHistoryDev* history_out;
+  double* lines;
+};
+
+}  // namespace vsync
+}  // namespace vhsd
+
+namespace vhsd {
+namespace vsync {
+// Tables for the hsync refinement and the TBC render of the decoded fields.
+struct FieldTail {
+  const float** spans;
+  long long* span_len;
+  const double** locs;
+  const unsigned char** filled;
+  double** refined;
+  unsigned char** filled_out;
+  int* line_counts;
+  long long* origins;
+  int* first_lines;
+  double* threshold;
+  double* dc;
+  const unsigned char* zeros;     // line_stride zero bytes (no line was filled by interpolation)
+  double* refined_base;           // [max_fields][line_stride]
+  unsigned char* filled_out_base;
+  double* render;                 // [max_fields][line_stride]
+  double fsc_shift;               // NTSC fsc phase shift (samples), 0 for PAL
+  int out_first_line[2];          // OutputFirstLine: first field, second field
+};
+}  // namespace vsync
+}  // namespace vhsd
+
+extern "C" hipError_t vhsd_launch_vhs_sync_reset(vhsd::vsync::LevelStateDev* level, vhsd::vsync::SyncHistoryDev* hist,
+                                                 hipStream_t stream);
+extern "C" hipError_t vhsd_launch_vhs_tail_tables(const vhsd::vsync::SyncCfg* c, const vhsd::vsync::SyncWork* w,
+                                                  const vhsd::vsync::FieldTail* t, int max_fields,
+                                                  hipStream_t stream);
+extern "C" hipError_t vhsd_launch_vhs_render(const vhsd::vsync::SyncCfg* c, const vhsd::vsync::SyncWork* w,
+                                             const vhsd::vsync::FieldTail* t, int max_fields, hipStream_t stream);
+extern "C" hipError_t vhsd_launch_vhs_sync(const vhsd::vsync::SyncCfg* c, const vhsd::vsync::SyncWork* w,
+                                           const int* starts, const int* lens, const int* pulse_count, int capacity,
+                                           long long span_len, int max_fields, hipStream_t stream);

@@ -270,10 +270,12 @@ hipError_t vhsd_launch_pulses(const float* s, long long n, double bias, double h
 hipError_t vhsd_launch_hsync(const float* const* spans, const long long* span_len, const double* const* locs,
                              const unsigned char* const* filled, const int* line_counts, const vhsd::HsyncParams* p,
                              double* const* refined_out, unsigned char* const* filled_out, int fields, int max_lines,
-                             double bias, void* work, hipStream_t stream);
+                             double bias, void* work, const double* field_threshold, const double* field_dc,
+                             hipStream_t stream);
 size_t vhsd_hsync_work_bytes(int fields, int max_lines);
 hipError_t vhsd_launch_tbc_fields(const float* src, long long src_len, double bias, const long long* origins,
-                                  const double* refined, int nloc, int lines,
+                                  const double* refined, int nloc, int lines, const int* first_lines,
+                                  int max_first_line, int relative,
                                   int fields, double nominal, double line_decay, const double* pow_table,
                                   double* line_x, double* line_y0, int out_len, int smooth, double ire0,
                                   double out_scale, double inv_hz_ire, double conv_offset, const float* lut,

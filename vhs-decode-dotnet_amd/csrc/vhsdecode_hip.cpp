@@ -18,6 +18,7 @@
 #include "sos_scan.hpp"
 
 #include "vhsd_args.hpp"
+#include "vhs_sync.hpp"
 
 #include "host_common.hpp"
 
@@ -279,6 +280,17 @@ struct vhsdecode_hip_demod_context {
   DevBuf<int> l_first, l_hit, l_count;
   DevBuf<unsigned char> l_fil, l_tab;
   DevBuf<unsigned char> f_fil, f_fo, f_tab;
+  // VHS field sequence (vhs_sync.hip): one allocation carved into the per-field lists,
+  // records and tables; the sequence state (level moving averages and sync history) lives
+  // at its start and carries from call to call
+  DevBuf<unsigned char> vs_mem, vs_state;
+  bool vs_state_ready = false;
+  const double* vs_lines = nullptr;    // line locations, hsync-refined and render locations of the
+  const double* vs_refined = nullptr;  // last vhs field decode ([fields][stride], window coordinates)
+  const double* vs_render = nullptr;
+  const long long* vs_origins = nullptr;
+  const int* vs_first_lines = nullptr;
+  int vs_line_stride = 0;
   double f_pow_alpha = -1.0;
   int f_pow_len = 0;
   void* f_stage = nullptr;   // pinned host staging for the location upload
@@ -315,6 +327,7 @@ struct vhsdecode_hip_demod_context {
     t_f64.release(); t_u16.release(); s_rf.release(); s_planes.release(); s_flags.release();
     l_line0.release(); l_locs.release(); l_first.release(); l_hit.release(); l_count.release(); l_fil.release();
     l_tab.release();
+    vs_mem.release(); vs_state.release();
     f_loc.release(); f_ref.release(); f_x.release(); f_y0.release(); f_pow.release(); f_fil.release();
     f_fo.release(); f_tab.release();
     if (f_stage) (void)hipHostFree(f_stage);
@@ -1296,7 +1309,7 @@ int32_t vhsdecode_hip_refine_hsync(vhsdecode_hip_demod_context* c, const float* 
       ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_HSYNC, 1);
       VHSD_HIP(c->h_work.ensure(vhsd_hsync_work_bytes(1, lines)));
       VHSD_HIP(vhsd_launch_hsync(&dt->span, &dt->len, &dt->loc, &dt->fil, &dt->lines, &p, &dt->ref, &dt->fo, 1, lines,
-                                 c->bias, c->h_work.p, st));
+                                 c->bias, c->h_work.p, nullptr, nullptr, st));
     }
     VHSD_HIP(hipMemcpyAsync(refined, c->h_ref.p, sizeof(double) * lines, hipMemcpyDeviceToHost, st));
     VHSD_HIP(hipMemcpyAsync(filled_out, c->h_fo.p, lines, hipMemcpyDeviceToHost, st));
@@ -1545,13 +1558,14 @@ int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod_context* c, const
       ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_HSYNC, nfields);
       VHSD_HIP(c->h_work.ensure(vhsd_hsync_work_bytes(nfields, nloc)));
       VHSD_HIP(vhsd_launch_hsync(d_spans, d_lens, d_locs, d_fils, d_lcount, &hp, d_refs, d_fos, nfields, nloc,
-                                 c->bias, c->h_work.p, st));
+                                 c->bias, c->h_work.p, nullptr, nullptr, st));
     }
     const double out_scale = p->output_scale;
     const double conv_off = (p->output_zero + 0.5) - (p->vsync_ire * out_scale);
     {
       ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_TBC, (long long)nfields * p->field_lines * p->output_line_length);
-      VHSD_HIP(vhsd_launch_tbc_fields(video, span_length, c->bias, d_org, c->f_ref.p, nloc, p->field_lines, nfields,
+      VHSD_HIP(vhsd_launch_tbc_fields(video, span_length, c->bias, d_org, c->f_ref.p, nloc, p->field_lines, nullptr, 0, 0,
+                                      nfields,
                                       p->nominal_line_length, line_decay, smooth ? c->f_pow.p : nullptr, c->f_x.p,
                                       c->f_y0.p, p->output_line_length, smooth ? 1 : 0, p->ire0, out_scale,
                                       1.0 / p->hz_ire, conv_off, c->lut.p, out, nullptr, st));
@@ -1668,13 +1682,13 @@ int32_t vhsdecode_hip_decode_span_fields_device(vhsdecode_hip_demod_context* c, 
       ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_HSYNC, max_fields);
       VHSD_HIP(c->h_work.ensure(vhsd_hsync_work_bytes(max_fields, nloc)));
       VHSD_HIP(vhsd_launch_hsync(t.spans, t.span_len, t.locs, t.filled, t.line_counts, &hp, t.refined, t.filled_out,
-                                 max_fields, nloc, c->bias, c->h_work.p, st));
+                                 max_fields, nloc, c->bias, c->h_work.p, nullptr, nullptr, st));
     }
     const double out_scale = p->output_scale;
     const double conv_off = (p->output_zero + 0.5) - (p->vsync_ire * out_scale);
     {
       ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_TBC, (long long)max_fields * p->field_lines * p->output_line_length);
-      VHSD_HIP(vhsd_launch_tbc_fields(video, span_length, c->bias, t.origins, c->f_ref.p, nloc, p->field_lines,
+      VHSD_HIP(vhsd_launch_tbc_fields(video, span_length, c->bias, t.origins, c->f_ref.p, nloc, p->field_lines, nullptr, 0, 0,
                                       max_fields, p->nominal_line_length, line_decay, smooth ? c->f_pow.p : nullptr,
                                       c->f_x.p, c->f_y0.p, p->output_line_length, smooth ? 1 : 0, p->ire0, out_scale,
                                       1.0 / p->hz_ire, conv_off, c->lut.p, out, field_count, st));
@@ -1683,6 +1697,242 @@ int32_t vhsdecode_hip_decode_span_fields_device(vhsdecode_hip_demod_context* c, 
   } catch (const std::exception& e) {
     return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
   }
+}
+
+int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, const float* sync_ref,
+                                               const float* video, int64_t span_length, int64_t first_begin,
+                                               int32_t reset_state, int64_t state_shift,
+                                               const vhsdecode_hip_vhs_sync_v1* sy,
+                                               const vhsdecode_hip_field_params_v1* p, int32_t max_fields,
+                                               uint16_t* out, vhsdecode_hip_vhs_field_v1* fields,
+                                               int32_t* field_count, int64_t* next_begin, void* stream) {
+  using namespace vhsd::vsync;
+  try {
+    if (!c || !sync_ref || !video || !sy || !p || !out || !field_count || max_fields <= 0 || span_length <= 0 ||
+        first_begin < 0)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid vhs field decode request");
+    if (sy->struct_size != sizeof(vhsdecode_hip_vhs_sync_v1) || p->struct_size != sizeof(vhsdecode_hip_field_params_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "vhs_sync or field_params struct_size mismatch");
+    if (span_length > INT32_MAX)
+      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "spans are limited to 2^31 - 1 samples (int32 pulse positions)");
+    const double fs_mhz = c->fs / 1000000.0;
+    const int out_lines = p->field_lines;
+    const int line_stride = out_lines + 10;
+    const int max_first = 4;
+    if (out_lines <= 0 || out_lines + max_first + 1 > line_stride || line_stride > 511 ||
+        p->output_line_length <= 0 || !(sy->line_period_us > 0.0) || sy->num_pulses <= 0 ||
+        sy->block_length <= 0 || sy->block_stride <= 0 || sy->block_cut < 0 || sy->read_samples <= 0 ||
+        !(sy->frames_per_second > 0.0) || !(sy->frame_lines > 0.0) || !(p->hz_ire != 0.0))
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid vhs sync or field geometry");
+    if ((3.75 * fs_mhz) + 2.0 > kHistBins)
+      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "sample rate too high for the hsync-length histogram");
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (!reset_state && !c->vs_state_ready)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "no vhs field sequence to continue on this context");
+    VHSD_HIP(hipSetDevice(c->device));
+    int32_t rc = ensure_lut(c);
+    if (rc) return rc;
+    hipStream_t st = pick_stream(c, stream);
+    // nominal-threshold pulses of the span: their vertical-sync groups place the read windows
+    const int capacity = (int)std::min<int64_t>(span_length / 16 + 1024, INT32_MAX / 2);
+    const int tiles = (int)((span_length + 4095) / 4096);
+    const size_t edge_cap_span = (size_t)(span_length / 2 + 2);
+    VHSD_HIP(c->tile_counts.ensure(std::max(tiles, 1)));
+    VHSD_HIP(c->counters.ensure(2));
+    VHSD_HIP(c->edge_pos.ensure(edge_cap_span));
+    VHSD_HIP(c->edge_kind.ensure(edge_cap_span));
+    VHSD_HIP(c->p_starts.ensure(capacity));
+    VHSD_HIP(c->p_lens.ensure(capacity));
+    // SyncAnalyzer.FromParameters (Dsp/SyncAnalysis.cs:88-114) and the vhs detector geometry
+    SyncCfg cfg{};
+    cfg.fs_mhz = fs_mhz;
+    double lp_us = sy->line_period_us;
+    if (fs_mhz > 0.0) lp_us = std::nearbyint(lp_us * fs_mhz) / fs_mhz;
+    cfg.nominal = lp_us * fs_mhz;
+    cfg.hsync_us = sy->hsync_pulse_us;
+    cfg.eq_us = sy->eq_pulse_us;
+    cfg.vsync_us = sy->vsync_pulse_us;
+    cfg.hsync_tol = sy->hsync_tolerance_us;
+    cfg.eq_tol = sy->eq_tolerance_us;
+    cfg.num_pulses = sy->num_pulses;
+    cfg.is_pal = sy->is_pal ? 1 : 0;
+    cfg.ire0 = p->ire0;
+    cfg.hz_ire = p->hz_ire;
+    cfg.vsync_ire = p->vsync_ire;
+    cfg.field_lines0 = cfg.is_pal ? out_lines - 1 : out_lines;
+    cfg.field_lines1 = cfg.is_pal ? out_lines : out_lines - 1;
+    cfg.output_lines = out_lines;
+    cfg.block_length = sy->block_length;
+    cfg.block_cut = sy->block_cut;
+    cfg.stride = sy->block_stride;
+    cfg.read_samples = sy->read_samples;
+    const double vf = sy->frames_per_second * 2.0, hf = sy->frames_per_second * sy->frame_lines;
+    cfg.ser_eq_len = (int)std::nearbyint(c->fs * sy->eq_pulse_us * 1e-6);
+    cfg.ser_line_len = (int)std::nearbyint(c->fs / hf);
+    (void)vf;
+    cfg.pulse_min = std::max(0, (int)std::ceil((sy->eq_pulse_us * fs_mhz) / 8.0));
+    cfg.pulse_max = std::max(1, (int)(cfg.nominal * 5.0));
+    const double field_rate = sy->frames_per_second * 2.0;
+    cfg.fs_window = (int)std::nearbyint(field_rate < 60.0 ? field_rate / 5.0 : field_rate / 6.0);
+    if (cfg.fs_window < 1 || cfg.fs_window > kAvgCap)
+      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "field-state moving-average window out of range");
+    cfg.bias = c->bias;
+    cfg.pulse_cap = kPulseCap;
+    cfg.edge_cap = 65536;
+    cfg.line_stride = line_stride;
+    // one allocation for every list, record and table
+    const size_t F = (size_t)max_fields;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+      const size_t o = off;
+      off += (bytes + 255) & ~(size_t)255;
+      return o;
+    };
+    const size_t o_spec = take(4), o_ws = take(8 * F), o_wl = take(8 * F);
+    const size_t o_pst = take(4 * F * kPulseCap), o_pln = take(4 * F * kPulseCap);
+    const size_t o_epos = take(4 * F * cfg.edge_cap), o_ek = take(F * cfg.edge_cap), o_ovf = take(4 * F);
+    const size_t o_search = take(sizeof(LevelSearch) * F), o_levels = take(sizeof(FieldLevels) * F);
+    const size_t o_before = take(sizeof(LevelStateDev) * F), o_after = take(sizeof(LevelStateDev));
+    const size_t o_raw = take(4 * F), o_logic = take(sizeof(FieldLogic) * F);
+    const size_t o_rk = take(4 * F * kPulseCap), o_rs = take(4 * F * kPulseCap);
+    const size_t o_rec = take(sizeof(FieldRecord) * F), o_cnt = take(4), o_nb = take(8), o_stop = take(4);
+    const size_t o_lines = take(8 * F * line_stride);
+    const size_t o_tp = take(8 * F * 6), o_lc = take(4 * F), o_org = take(8 * F), o_fl = take(4 * F);
+    const size_t o_thr = take(8 * F), o_dc = take(8 * F), o_zero = take(line_stride);
+    const size_t o_refb = take(8 * F * line_stride), o_fob = take(F * line_stride), o_rend = take(8 * F * line_stride);
+    VHSD_HIP(c->vs_mem.ensure(off));
+    unsigned char* m = c->vs_mem.p;
+    const size_t state_hist = (sizeof(LevelStateDev) + 255) & ~(size_t)255;
+    if (!c->vs_state.p) VHSD_HIP(c->vs_state.ensure(state_hist + sizeof(SyncHistoryDev)));
+    VHSD_HIP(c->f_x.ensure(F * line_stride));
+    VHSD_HIP(c->f_y0.ensure(F * line_stride));
+    LevelStateDev* level_state = reinterpret_cast<LevelStateDev*>(c->vs_state.p);
+    SyncHistoryDev* history = reinterpret_cast<SyncHistoryDev*>(c->vs_state.p + state_hist);
+    if (reset_state) {
+      VHSD_HIP(vhsd_launch_vhs_sync_reset(level_state, history, st));
+      c->vs_state_ready = true;
+    }
+    // the zero `filled` row (BuildUpstreamLineLocations fills no line); written by a fill kernel
+    VHSD_HIP(vhsd_launch_fill_u32(reinterpret_cast<unsigned*>(m + o_zero), (line_stride + 3) / 4, 0u, st));
+    SyncWork w{};
+    w.sync_ref = sync_ref;
+    w.first_begin = first_begin;
+    w.state_shift = reset_state ? 0 : state_shift;
+    w.spec_count = reinterpret_cast<int*>(m + o_spec);
+    w.win_start = reinterpret_cast<long long*>(m + o_ws);
+    w.win_len = reinterpret_cast<long long*>(m + o_wl);
+    w.pulse_start = reinterpret_cast<int*>(m + o_pst);
+    w.pulse_len = reinterpret_cast<int*>(m + o_pln);
+    w.edge_pos = reinterpret_cast<int*>(m + o_epos);
+    w.edge_kind = m + o_ek;
+    w.overflow = reinterpret_cast<int*>(m + o_ovf);
+    w.search = reinterpret_cast<LevelSearch*>(m + o_search);
+    w.levels = reinterpret_cast<FieldLevels*>(m + o_levels);
+    w.level_before = reinterpret_cast<LevelStateDev*>(m + o_before);
+    w.level_state_in = level_state;
+    w.level_state_out = level_state;
+    w.level_state_after_all = reinterpret_cast<LevelStateDev*>(m + o_after);
+    w.raw_count = reinterpret_cast<int*>(m + o_raw);
+    w.logic = reinterpret_cast<FieldLogic*>(m + o_logic);
+    w.ref_kind = reinterpret_cast<int*>(m + o_rk);
+    w.ref_start = reinterpret_cast<int*>(m + o_rs);
+    w.records = reinterpret_cast<FieldRecord*>(m + o_rec);
+    w.field_count = reinterpret_cast<int*>(m + o_cnt);
+    w.next_begin_out = reinterpret_cast<long long*>(m + o_nb);
+    w.stop_status = reinterpret_cast<int*>(m + o_stop);
+    w.history_in = history;
+    w.history_out = history;
+    w.lines = reinterpret_cast<double*>(m + o_lines);
+    VHSD_HIP(vhsd_launch_fill_u32(reinterpret_cast<unsigned*>(w.overflow), (long long)F, 0u, st));
+    {
+      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_PULSES, span_length);
+      VHSD_HIP(vhsd_launch_pulses(sync_ref, span_length, c->bias, p->threshold, cfg.pulse_min, cfg.pulse_max,
+                                  c->tile_counts.p, tiles, c->edge_pos.p, c->edge_kind.p, (int)edge_cap_span,
+                                  c->counters.p, c->p_starts.p, c->p_lens.p, capacity, st));
+    }
+    {
+      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_LOCATE, max_fields);
+      VHSD_HIP(vhsd_launch_vhs_sync(&cfg, &w, c->p_starts.p, c->p_lens.p, c->counters.p + 1, capacity, span_length,
+                                    max_fields, st));
+    }
+    FieldTail t{};
+    t.spans = reinterpret_cast<const float**>(m + o_tp);
+    t.span_len = reinterpret_cast<long long*>(m + o_tp + 8 * F);
+    t.locs = reinterpret_cast<const double**>(m + o_tp + 16 * F);
+    t.filled = reinterpret_cast<const unsigned char**>(m + o_tp + 24 * F);
+    t.refined = reinterpret_cast<double**>(m + o_tp + 32 * F);
+    t.filled_out = reinterpret_cast<unsigned char**>(m + o_tp + 40 * F);
+    t.line_counts = reinterpret_cast<int*>(m + o_lc);
+    t.origins = reinterpret_cast<long long*>(m + o_org);
+    t.first_lines = reinterpret_cast<int*>(m + o_fl);
+    t.threshold = reinterpret_cast<double*>(m + o_thr);
+    t.dc = reinterpret_cast<double*>(m + o_dc);
+    t.zeros = m + o_zero;
+    t.refined_base = reinterpret_cast<double*>(m + o_refb);
+    t.filled_out_base = m + o_fob;
+    t.render = reinterpret_cast<double*>(m + o_rend);
+    // ApplyNtscFscPhaseShiftCore with fsc = output rate / 4 (TbcFieldDecodePipeline.cs:1401-1407, 4022-4038)
+    const double fsc_mhz = (sy->output_rate_mhz * 1000000.0) / 4000000.0;
+    t.fsc_shift = (cfg.is_pal || !(fsc_mhz > 0.0)) ? 0.0 : ((117.25 / 360.0) / fsc_mhz) * fs_mhz;
+    // OutputFirstLine (TbcFieldDecodePipeline.cs:2475-2483)
+    t.out_first_line[0] = cfg.is_pal ? 3 : 1;
+    t.out_first_line[1] = cfg.is_pal ? 4 : 1;
+    VHSD_HIP(vhsd_launch_vhs_tail_tables(&cfg, &w, &t, max_fields, st));
+    vhsd::HsyncParams hp{p->threshold, fs_mhz, sy->hsync_pulse_us, p->ire0, p->hz_ire, cfg.is_pal,
+                         p->use_right_hsync};
+    {
+      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_HSYNC, max_fields);
+      VHSD_HIP(c->h_work.ensure(vhsd_hsync_work_bytes(max_fields, line_stride)));
+      VHSD_HIP(vhsd_launch_hsync(t.spans, t.span_len, t.locs, t.filled, t.line_counts, &hp, t.refined, t.filled_out,
+                                 max_fields, line_stride, c->bias, c->h_work.p, t.threshold, t.dc, st));
+    }
+    VHSD_HIP(vhsd_launch_vhs_render(&cfg, &w, &t, max_fields, st));
+    c->f_ref_nloc = 0;
+    const bool smooth = p->wow_level_smoothing > 0.0;
+    const double alpha = smooth ? 1.0 / (p->wow_level_smoothing * p->output_line_length) : 0.0;
+    if (smooth && (c->f_pow_alpha != alpha || c->f_pow_len != p->output_line_length)) {
+      std::vector<double> pw(p->output_line_length);
+      for (int s2 = 0; s2 < p->output_line_length; s2++) pw[s2] = std::pow(1.0 - alpha, (double)(s2 + 1));
+      VHSD_HIP(c->f_pow.ensure(pw.size()));
+      VHSD_HIP(hipMemcpy(c->f_pow.p, pw.data(), sizeof(double) * pw.size(), hipMemcpyHostToDevice));
+      c->f_pow_alpha = alpha;
+      c->f_pow_len = p->output_line_length;
+    }
+    const double line_decay = smooth ? std::pow(1.0 - alpha, (double)p->output_line_length) : 0.0;
+    const double out_scale = p->output_scale;
+    const double conv_off = (p->output_zero + 0.5) - (p->vsync_ire * out_scale);
+    {
+      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_TBC, (long long)max_fields * out_lines * p->output_line_length);
+      VHSD_HIP(vhsd_launch_tbc_fields(video, span_length, c->bias, t.origins, t.render, line_stride, out_lines,
+                                      t.first_lines, max_first, 1, max_fields, p->nominal_line_length, line_decay,
+                                      smooth ? c->f_pow.p : nullptr, c->f_x.p, c->f_y0.p, p->output_line_length,
+                                      smooth ? 1 : 0, p->ire0, out_scale, 1.0 / p->hz_ire, conv_off, c->lut.p, out,
+                                      w.field_count, st));
+    }
+    if (fields) VHSD_HIP(hipMemcpyAsync(fields, w.records, sizeof(FieldRecord) * F, hipMemcpyDeviceToDevice, st));
+    VHSD_HIP(hipMemcpyAsync(field_count, w.field_count, 4, hipMemcpyDeviceToDevice, st));
+    if (next_begin) VHSD_HIP(hipMemcpyAsync(next_begin, w.next_begin_out, 8, hipMemcpyDeviceToDevice, st));
+    c->vs_lines = w.lines;
+    c->vs_refined = t.refined_base;
+    c->vs_render = t.render;
+    c->vs_origins = t.origins;
+    c->vs_first_lines = t.first_lines;
+    c->vs_line_stride = line_stride;
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_vhs_field_lines_device(vhsdecode_hip_demod_context* c, int32_t which, const double** table,
+                                             int32_t* stride) {
+  if (!c || !table || !stride || which < 0 || which > 2)
+    return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid arguments");
+  std::lock_guard<std::mutex> lock(c->mu);
+  if (!c->vs_render) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "no vhs field decode yet");
+  *table = which == 0 ? c->vs_lines : which == 1 ? c->vs_refined : c->vs_render;
+  *stride = c->vs_line_stride;
+  return VHSDECODE_HIP_OK;
 }
 
 int32_t vhsdecode_hip_field_locations_device(vhsdecode_hip_demod_context* c, const double** locations,

@@ -54,6 +54,8 @@ REQUIRED_EXPORTS = (
     "vhsdecode_hip_field_locations_device",
     "vhsdecode_hip_detect_dropouts_device",
     "vhsdecode_hip_find_pulses_noalloc",
+    "vhsdecode_hip_decode_vhs_fields_device",
+    "vhsdecode_hip_vhs_field_lines_device",
 )
 
 KERNEL_NAMES = ("block_demod", "spike_repair", "chroma", "pulses", "hsync", "tbc", "sub_deemphasis", "block_video",
@@ -154,6 +156,28 @@ class FieldParamsV1(ctypes.Structure):
                 ("field_lines", ctypes.c_int32), ("nominal_line_length", ctypes.c_double),
                 ("wow_level_smoothing", ctypes.c_double), ("output_zero", ctypes.c_int32),
                 ("vsync_ire", ctypes.c_double), ("output_scale", ctypes.c_double)]
+
+
+class VhsSyncV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("is_pal", ctypes.c_int32), ("line_period_us", ctypes.c_double),
+                ("hsync_pulse_us", ctypes.c_double), ("eq_pulse_us", ctypes.c_double),
+                ("vsync_pulse_us", ctypes.c_double), ("num_pulses", ctypes.c_int32),
+                ("hsync_tolerance_us", ctypes.c_double), ("eq_tolerance_us", ctypes.c_double),
+                ("frames_per_second", ctypes.c_double), ("frame_lines", ctypes.c_double),
+                ("output_rate_mhz", ctypes.c_double), ("read_samples", ctypes.c_int64), ("block_length", ctypes.c_int32), ("block_cut", ctypes.c_int32),
+                ("block_stride", ctypes.c_int32)]
+
+
+class VhsFieldV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("status", ctypes.c_int32), ("begin", ctypes.c_int64),
+                ("window_start", ctypes.c_int64), ("window_length", ctypes.c_int64),
+                ("level_source", ctypes.c_int32), ("raw_pulses", ctypes.c_int32), ("refined_pulses", ctypes.c_int32),
+                ("is_first_field", ctypes.c_int32), ("parity_confidence", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("sync_level", ctypes.c_double), ("blank_level", ctypes.c_double),
+                ("dc_offset", ctypes.c_double), ("threshold", ctypes.c_double), ("hsync_median", ctypes.c_double),
+                ("mean_line_length", ctypes.c_double), ("line0", ctypes.c_double), ("first_hsync", ctypes.c_double),
+                ("first_hsync_line", ctypes.c_double), ("next_offset", ctypes.c_double),
+                ("next_begin", ctypes.c_int64)]
 
 
 class LocatorV1(ctypes.Structure):
@@ -314,6 +338,13 @@ def load():
                                                        ctypes.c_int32, ctypes.c_void_p]
     L.vhsdecode_hip_hifi_demodulate_host.argtypes = [ctypes.c_void_p, _fp, _fp, ctypes.c_int32]
     L.vhsdecode_hip_run.argtypes = [ctypes.POINTER(RunConfigV1), ctypes.POINTER(RunResultV1)]
+    L.vhsdecode_hip_decode_vhs_fields_device.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
+        ctypes.c_int64, ctypes.POINTER(VhsSyncV1), ctypes.POINTER(FieldParamsV1), ctypes.c_int32, ctypes.c_void_p,
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.vhsdecode_hip_vhs_field_lines_device.argtypes = [ctypes.c_void_p, ctypes.c_int32,
+                                                       ctypes.POINTER(ctypes.c_void_p),
+                                                       ctypes.POINTER(ctypes.c_int32)]
     for name in REQUIRED_EXPORTS:
         fn = getattr(L, name)
         if name not in ("vhsdecode_hip_get_last_error", "vhsdecode_hip_demod_destroy"):

@@ -340,6 +340,29 @@ class HipBlockDemodulator:
             _ptr(ref, _lib._dp) if ref is not None else None, ctypes.c_void_p(stream)))
         return ref
 
+    def decode_vhs_fields_device(self, sync_ref_ptr: int, video_ptr: int, span_length: int, first_begin: int,
+                                 sync, geom, max_fields: int, out_ptr: int, count_ptr: int, fields_ptr: int = 0,
+                                 next_begin_ptr: int = 0, reset_state: bool = True, state_shift: int = 0,
+                                 stream: int = 0):
+        """The vhs field sequence from `first_begin` on: read windows, sync levels, pulses,
+        parity, line 0, line locations, hsync refinement and TBC, all on the device and
+        asynchronous.  `sync` is a VhsSyncV1; records (VhsFieldV1) land at fields_ptr."""
+        p = self.field_params(geom)
+        v = ctypes.c_void_p
+        _lib.check(self._L.vhsdecode_hip_decode_vhs_fields_device(
+            self._h, v(sync_ref_ptr), v(video_ptr), span_length, first_begin, 1 if reset_state else 0, state_shift,
+            ctypes.byref(sync), ctypes.byref(p), max_fields, v(out_ptr), v(fields_ptr or None), v(count_ptr),
+            v(next_begin_ptr or None), v(stream)))
+
+    def vhs_field_lines_device(self, which: int):
+        """(device pointer, stride) of a line table of the last vhs field decode: 0 line
+        locations, 1 hsync-refined, 2 render locations; window coordinates."""
+        ptr = ctypes.c_void_p()
+        stride = ctypes.c_int32()
+        _lib.check(self._L.vhsdecode_hip_vhs_field_lines_device(self._h, which, ctypes.byref(ptr),
+                                                                ctypes.byref(stride)))
+        return int(ptr.value), int(stride.value)
+
     def field_locations_device(self):
         """(device pointer, locations per field) of the refined line locations of the last
         field decode on this context; valid until the next field decode."""

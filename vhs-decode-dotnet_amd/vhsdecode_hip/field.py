@@ -136,6 +136,21 @@ def locator(g: SyncGeometry, block_length: int = 32768, block_cut: int = 1024, s
                           block_length, block_cut, stride)
 
 
+def vhs_sync_params(fmt: FormatParameters, sample_rate_hz: float = 40e6, block_length: int = 32768,
+                    block_cut: int = 1024, stride: int = 30720):
+    """vhsdecode_hip_vhs_sync_v1 of a tape format: SyncAnalyzer.FromParameters with the vhs
+    command's pulse tolerances (0.7 / 0.9 us, DecodeSession.cs:420-428) and the read-window
+    geometry of DecodeReadWindowPlanner (Decode/DecodeReadWindowPlanner.cs:15-63)."""
+    from . import _lib
+    sp = fmt.sysparams
+    return _lib.VhsSyncV1(ctypes.sizeof(_lib.VhsSyncV1), 1 if fmt.parent_system == "PAL" else 0,
+                          float(sp["line_period"]), float(sp["hsyncPulseUS"]), float(sp["eqPulseUS"]),
+                          float(sp["vsyncPulseUS"]), int(sp["numPulses"]), 0.7, 0.9, float(sp["FPS"]),
+                          float(sp["frame_lines"]), float(sp["outfreq"]),
+                          read_sample_count(fmt, sample_rate_hz, block_length),
+                          block_length, block_cut, stride)
+
+
 def read_sample_count(fmt: FormatParameters, sample_rate_hz: float = 40e6, block_length: int = 32768) -> int:
     """DecodeReadWindowPlanner.EstimateReadSampleCount (Decode/DecodeReadWindowPlanner.cs:15-35):
     350 lines (NTSC, aligned down to 16384) or 400 lines (PAL) of whole-sample line
