@@ -402,39 +402,6 @@ VHSDECODE_HIP_API int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod
                                                              const vhsdecode_hip_field_params_v1* params,
                                                              uint16_t* out, double* refined_out, void* stream);
 
-/* ---------------------------------------------------------------- whole-span field decode
- * Pulses, line locations, hsync refinement and TBC of every complete field of a
- * demodulated span without a host round trip: the per-field tail of
- * TbcFieldDecodePipeline.DecodePrepared (Decode/TbcFieldDecodePipeline.cs:1071-1805)
- * for a run of fields.  Line locations come from this backend's compact locator
- * (each vertical-sync group anchors a field, parity from its half-line phase, line
- * starts snapped to the pulses that open them), the stand-in for SyncAnalyzer's
- * sequential pulse classification and line-0 resolution (Dsp/SyncAnalysis.cs:131-,
- * TbcFieldDecodePipeline.cs:1176-1295). */
-typedef struct vhsdecode_hip_locator_v1 {
-  uint32_t struct_size;
-  int32_t min_pulse_length;        /* FindRawPulses bounds (samples), TbcFieldDecodePipeline.cs:2375-2384 */
-  int32_t max_pulse_length;
-  double line_length;              /* nominal input line length (samples, exact) */
-  double broad_min_length;         /* pulses longer than this are broad (vertical sync) */
-  double hsync_min_length;         /* longer than this and not broad: hsync */
-  double snap_tolerance;           /* lines */
-  double lead_lines;               /* field read-window lead before line 0 (lines) */
-  int32_t block_length;            /* read-window alignment, DecodeReadWindowPlanner.cs:37-63 */
-  int32_t block_cut;
-  int32_t block_stride;
-} vhsdecode_hip_locator_v1;
-
-/* sync_ref, video: device span planes; out: device [max_fields][field_lines * output_line_length];
- * field_count: device int32 receiving the number of fields decoded (<= max_fields; only those
- * fields of `out` are written); field_line0 (device double[max_fields]) and field_first
- * (device int32[max_fields], 1 for the first field of a frame) are optional.  Asynchronous
- * on `stream`. */
-VHSDECODE_HIP_API int32_t vhsdecode_hip_decode_span_fields_device(
-    vhsdecode_hip_demod_context* context, const float* sync_ref, const float* video, int64_t span_length,
-    const vhsdecode_hip_field_params_v1* params, const vhsdecode_hip_locator_v1* locator, int32_t max_fields,
-    uint16_t* out, int32_t* field_count, double* field_line0, int32_t* field_first, void* stream);
-
 /* ---------------------------------------------------------------- VHS field sequence
  * The field decode of the vhs command with its default (v0.4.0) sync path, for the fields
  * of a demodulated span read in sequence: TbcFieldDecodePipeline.DecodePrepared
@@ -472,9 +439,12 @@ typedef struct vhsdecode_hip_vhs_sync_v1 {
 /* One field of the sequence (device output record; struct_size is written). */
 typedef struct vhsdecode_hip_vhs_field_v1 {
   uint32_t struct_size;
-  int32_t status;                  /* 0 decoded; 1 no pulses, 2 no first hsync, 3 insufficient data
-                                      (the reference's recovery kinds); 4 window not the predicted one;
-                                      5 pulse capacity exceeded */
+  int32_t status;                  /* 0 decoded; 1 no sync pulses, 2 no first hsync, 3 insufficient
+                                      data (the reference's recovery kinds: next_begin is the intended
+                                      start plus the recovery offset); 4 window not the speculated one
+                                      and 6 window not read in this call (next_begin = begin: call again
+                                      from there, on this span if the window fits in it); 5 a field
+                                      window holds more sync edges or pulses than the device lists */
   int64_t begin;                   /* intended read start (span coordinates) */
   int64_t window_start;            /* read window (span coordinates) */
   int64_t window_length;
@@ -501,29 +471,36 @@ typedef struct vhsdecode_hip_vhs_field_v1 {
  * first_begin: the first field's intended read start (span coordinates); reset_state: 1
  * starts a new sequence (no level history, parity or sync history), 0 continues the one
  * this context's last call left (its span coordinates shifted by `state_shift` samples,
- * the distance from that call's span start to this one's).  out: device
- * [max_fields][field_lines * output_line_length]; fields: optional device records
- * [max_fields]; field_count: device int32 (fields decoded, in order, each with status 0);
- * next_begin: optional device int64 receiving where the sequence continues (span
- * coordinates).  params->threshold is the nominal sync threshold used only to place the
- * fields' read windows.  Asynchronous on `stream`. */
+ * the distance from that call's span start to this one's).  Up to `rounds` rounds run
+ * on the device, each speculating the read windows of the fields ahead from the span's
+ * vertical-sync pulses, decoding them in parallel and verifying them in sequence order;
+ * a round ends at a mispredicted window or a failed field and the next one carries on
+ * from there.  out: device [max_fields][field_lines * output_line_length], field k's
+ * output at slot k; fields: optional device records [max_fields]; field_count: device
+ * int32, the records written (0 .. count - 1, in sequence order: status 0 decoded, its
+ * field in out; status 1-3 failed, with the reference's recovery applied, its out slot
+ * untouched); record `count`, when count < max_fields, says why the call stopped (its
+ * status) and where the sequence continues.  next_begin: optional device int64 receiving
+ * where the sequence continues (span coordinates).  params->threshold is the nominal
+ * sync threshold used only to speculate the read windows.  Asynchronous on `stream`. */
 VHSDECODE_HIP_API int32_t vhsdecode_hip_decode_vhs_fields_device(
     vhsdecode_hip_demod_context* context, const float* sync_ref, const float* video, int64_t span_length,
     int64_t first_begin, int32_t reset_state, int64_t state_shift, const vhsdecode_hip_vhs_sync_v1* sync,
-    const vhsdecode_hip_field_params_v1* params, int32_t max_fields, uint16_t* out,
+    const vhsdecode_hip_field_params_v1* params, int32_t max_fields, int32_t rounds, uint16_t* out,
     vhsdecode_hip_vhs_field_v1* fields, int32_t* field_count, int64_t* next_begin, void* stream);
 
-/* The line tables of the last vhsdecode_hip_decode_vhs_fields_device call on this context:
- * which 0: BuildUpstreamLineLocations (TbcFieldDecodePipeline.cs:1244-1295), 1: after
- * RefineLineLocationsFromHSync (:1344-1348), 2: the TBC render locations (less the NTSC
- * fsc phase shift, :1401-1407).  Device memory owned by the context, [fields][stride]
- * in window coordinates (the field record's window_start is the origin), valid until
- * the next vhs field decode on it. */
+/* The line tables of the last vhsdecode_hip_decode_vhs_fields_device call on this context,
+ * [records][stride] (rows of decoded records only): which 0, BuildUpstreamLineLocations
+ * (TbcFieldDecodePipeline.cs:1244-1295); 1, after RefineLineLocationsFromHSync
+ * (:1344-1348); 2, the TBC render locations (less the NTSC fsc phase shift, :1394-1400),
+ * all three in window coordinates (the record's window_start is the origin); 3, the
+ * render locations from the field's output first line (:2475-2483) in span coordinates,
+ * the locations vhsdecode_hip_detect_dropouts_device takes.  Device memory owned by the
+ * context, valid until the next vhs field decode on it. */
 VHSDECODE_HIP_API int32_t vhsdecode_hip_vhs_field_lines_device(vhsdecode_hip_demod_context* context, int32_t which,
                                                                const double** table, int32_t* stride);
 
-/* The refined line locations of the last field decode on this context
- * (vhsdecode_hip_decode_fields_device or vhsdecode_hip_decode_span_fields_device):
+/* The refined line locations of the last vhsdecode_hip_decode_fields_device call on this context:
  * device memory owned by the context, [fields][location_count] in span coordinates,
  * valid until the next field decode call on it. */
 VHSDECODE_HIP_API int32_t vhsdecode_hip_field_locations_device(vhsdecode_hip_demod_context* context,
@@ -532,13 +509,13 @@ VHSDECODE_HIP_API int32_t vhsdecode_hip_field_locations_device(vhsdecode_hip_dem
 /* ---------------------------------------------------------------- dropout detection
  * TbcFieldDecodePipeline.DetectTapeDropouts (Decode/TbcFieldDecodePipeline.cs:6593-6642)
  * for a run of fields, on the device: the float32 pairwise mean of the RF envelope
- * over each field's read window (NumpyReduction.MeanFloat32; the window as
- * DecodeReadWindowPlanner.Resolve places it, Decode/DecodeReadWindowPlanner.cs:37-63)
- * gives the threshold, RfDropoutDetector.FindDropouts (Decode/TbcDropoutMapping.cs:52-107)
- * finds the hysteresis ranges between the field's first and last detection line, and
- * TbcDropoutMapper.MapTapeRfToTbc (:288-366) turns them into .tbc.json rows
- * (fieldLine, startx, endx).  Output line k of a field starts at its location k, which
- * is the reference's first rendered location, so a row's field line is k + 1. */
+ * over each field's read window (NumpyReduction.MeanFloat32; the window of the field's
+ * sequence record) gives the threshold, RfDropoutDetector.FindDropouts
+ * (Decode/TbcDropoutMapping.cs:52-107) finds the hysteresis ranges between the field's
+ * first and last detection line, and TbcDropoutMapper.MapTapeRfToTbc (:288-366) turns
+ * them into .tbc.json rows (fieldLine, startx, endx).  Output line k of a field starts
+ * at its location k, which is the reference's first rendered location, so a row's field
+ * line is k + 1. */
 typedef struct vhsdecode_hip_dropout_params_v1 {
   uint32_t struct_size;
   double threshold_fraction;       /* TbcDropoutDetectionOptions.ThresholdFraction: 0.18 (DDD), 0.35 (CxADC) */
@@ -550,25 +527,21 @@ typedef struct vhsdecode_hip_dropout_params_v1 {
   int32_t first_field_lines;       /* CurrentFieldLineCount of a first field (NTSC 263, PAL 312) */
   int32_t second_field_lines;      /* ... of a second field (NTSC 262, PAL 313) */
   int32_t capacity;                /* rows (and ranges) stored per field */
-  int64_t read_samples;            /* DecodeReadWindowPlanner.EstimateReadSampleCount */
-  double line_length;              /* nominal input line length (samples, exact) */
-  double lead_lines;               /* read-window lead before line 0 (lines), as the locator */
-  int32_t block_length;
-  int32_t block_cut;
-  int32_t block_stride;
 } vhsdecode_hip_dropout_params_v1;
 
-/* envelope: device span plane; locations: device [max_fields][location_count] (span
- * coordinates, e.g. vhsdecode_hip_field_locations_device); field_line0 (double),
- * field_first (int32) and field_count (int32, fields to process, <= max_fields): device,
- * as vhsdecode_hip_decode_span_fields_device writes them.  Outputs, device: row_count
- * [max_fields]; field_line, start_x, end_x [max_fields][capacity]; range_count,
- * range_start, range_end (sample positions in the span) likewise, or all three NULL.
- * A count above capacity means only the first `capacity` entries were stored; -1 means
- * the field's detection range exceeded 8388608 samples.  Asynchronous on `stream`. */
+/* envelope: device span plane; locations: device [max_fields][location_count], each
+ * field's locations from its output first line in span coordinates (table 3 of
+ * vhsdecode_hip_vhs_field_lines_device); fields (read window, parity, status) and
+ * field_count (int32, records to process, <= max_fields): device, as
+ * vhsdecode_hip_decode_vhs_fields_device writes them.  Records whose status is not 0 get
+ * no rows.  Outputs, device: row_count [max_fields]; field_line, start_x, end_x
+ * [max_fields][capacity]; range_count, range_start, range_end (sample positions in the
+ * span) likewise, or all three NULL.  A count above capacity means only the first
+ * `capacity` entries were stored; -1 means the field's read window exceeded 3.6 M
+ * samples or its detection range 8388608.  Asynchronous on `stream`. */
 VHSDECODE_HIP_API int32_t vhsdecode_hip_detect_dropouts_device(
     vhsdecode_hip_demod_context* context, const float* envelope, int64_t span_length, const double* locations,
-    int32_t location_count, const double* field_line0, const int32_t* field_first, const int32_t* field_count,
+    int32_t location_count, const vhsdecode_hip_vhs_field_v1* fields, const int32_t* field_count,
     int32_t max_fields, const vhsdecode_hip_dropout_params_v1* params, int32_t* row_count, int32_t* field_line,
     int32_t* start_x, int32_t* end_x, int32_t* range_count, int32_t* range_start, int32_t* range_end, void* stream);
 
@@ -595,15 +568,15 @@ typedef struct vhsdecode_hip_serve_config_v1 {
   int32_t max_fields;              /* largest field count a request may ask for */
 } vhsdecode_hip_serve_config_v1;
 
-/* One decode format: the block filters and cuts, the field geometry and the line
- * locator.  Everything is copied; the caller's arrays may be freed after the call. */
+/* One decode format: the block filters and cuts, the field geometry and the vhs sync
+ * parameters.  Everything is copied; the caller's arrays may be freed after the call. */
 typedef struct vhsdecode_hip_serve_format_v1 {
   uint32_t struct_size;
   const vhsdecode_hip_filters_v1* filters;
   int32_t block_cut;
   int32_t block_cut_end;
   const vhsdecode_hip_field_params_v1* field;
-  const vhsdecode_hip_locator_v1* locator;
+  const vhsdecode_hip_vhs_sync_v1* sync;
   const float* sinc_table;         /* 65537 x 16 FP32, or NULL for the Kaiser fallback */
   int64_t sinc_count;
 } vhsdecode_hip_serve_format_v1;
@@ -655,13 +628,15 @@ VHSDECODE_HIP_API void vhsdecode_hip_serve_destroy(vhsdecode_hip_serve* server);
  * (videoParameters, then per field isFirstField, seqNo, fileLoc and dropOuts, the layout of
  * the reference's TBCWriter::write_json, src/VHSDecode.CudaFast.Native/src/tbc_writer.cpp:209-312).
  * Colour-under chroma is not decoded: no _chroma.tbc is written.
- * Spans sit at fixed places on multiples of lcm(block length, block stride) (491520
- * samples for 1024/1024 cuts) and overlap by a field and a half plus the read-window lead
- * (plus a dropout read window with detection on), so the blocks two spans share give the
- * same planes and every field's read-window origin, hence every output sample, is what
- * one span from the start of the input gives; a field found again is written once.  A
- * prefetch thread reads the next span through the callback while the device decodes the
- * current one: the callbacks must be callable from a thread other than the caller's. */
+ * The fields are the vhs field sequence (vhsdecode_hip_decode_vhs_fields_device) from the
+ * start of the input, carried from span to span.  Spans sit at fixed places on multiples
+ * of lcm(block length, block stride) (491520 samples for 1024/1024 cuts) and overlap by
+ * the longest read window and a grid step, so the blocks two spans share give the same
+ * planes and every field's read window, hence every output sample, is what one span from
+ * the start of the input gives.  A prefetch thread reads the next span through the
+ * callback while the device decodes the current one: the callbacks must be callable from
+ * a thread other than the caller's.  A cancel (the cancel callback, polled between spans
+ * and after a short read) ends the run with VHSDECODE_HIP_E_CANCELLED and no .tbc.json. */
 enum { VHSDECODE_HIP_INPUT_FLOAT32 = 0, VHSDECODE_HIP_INPUT_INT16 = 1 };
 
 /* Reads sample_count samples from sample_offset into destination (int16 or float, as the
@@ -674,7 +649,7 @@ typedef int32_t (*vhsdecode_hip_cancel_callback)(void* user_data);
 typedef struct vhsdecode_hip_run_config_v1 {
   uint32_t struct_size;
   int32_t device_id;
-  const vhsdecode_hip_serve_format_v1* format;  /* filters, block cuts, field and locator params, sinc table */
+  const vhsdecode_hip_serve_format_v1* format;  /* filters, block cuts, field and sync params, sinc table */
   const vhsdecode_hip_dropout_params_v1* dropouts; /* NULL: no dropout detection (the reference's --noDOD) */
   uint64_t total_samples;
   uint32_t input_sample_format;    /* VHSDECODE_HIP_INPUT_*; FP32 samples are rounded to int16 */

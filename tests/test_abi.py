@@ -52,7 +52,6 @@ def _c_sizeof(struct):
     ("vhsdecode_hip_profile_v1", _lib.ProfileV1),
     ("vhsdecode_hip_field_params_v1", _lib.FieldParamsV1),
     ("vhsdecode_hip_subdeemph_v1", _lib.SubDeemphV1),
-    ("vhsdecode_hip_locator_v1", _lib.LocatorV1),
     ("vhsdecode_hip_serve_config_v1", _lib.ServeConfigV1),
     ("vhsdecode_hip_serve_format_v1", _lib.ServeFormatV1),
     ("vhsdecode_hip_window_v1", _lib.WindowV1),

@@ -1,9 +1,9 @@
 """The whole run (vhsdecode_hip_run) against one decode of the whole tape as a single span
-(demod_span_device + decode_span_fields_device + detect_dropouts_device) on tapes with RF
-dropouts: the run reads the tape through its callback in spans of five fields, and must
-write the same fields, in order, with the same parity, line-0 positions and dropout rows,
-and the .tbc.json that describes them.
-Runs on an MI355X only."""
+(demod_span_device + decode_vhs_fields_device + detect_dropouts_device) on tapes with RF
+dropouts: the run reads the tape through its callback in spans of five fields, carrying
+the field sequence from span to span, and must write the same fields, in order, with the
+same parity, read-window positions and dropout rows, and the .tbc.json that describes
+them.  Runs on an MI355X only."""
 import json
 import os
 
@@ -48,36 +48,47 @@ def tape(request):
     flags = hipmem.DeviceBuffer(nb * 4)
     video, vlp, env, chroma = (planes.ptr + k * n * 4 for k in range(4))
     dem.demod_span_device(rf.ptr, 0, nb, video, vlp, env, chroma, flags.ptr)
-    dec = field.HipFieldDecoder(dem, fmt)
+    dec = field.HipVhsFieldDecoder(dem, fmt)
     g = dec.geom
-    mf = nfields + 2
+    mf = nfields + 4
     per = g.field_lines * g.out_len
+    import ctypes
+    from vhsdecode_hip import _lib
+    size = ctypes.sizeof(_lib.VhsFieldV1)
     out = hipmem.DeviceBuffer(mf * per * 2)
-    meta = hipmem.DeviceBuffer(8 + 12 * mf)
-    dec.decode_async(video, vlp, n, out.ptr, mf, meta.ptr, meta.ptr + 8, meta.ptr + 8 + 8 * mf)
+    recs = hipmem.DeviceBuffer(mf * size)
+    meta = hipmem.DeviceBuffer(16)
+    dec.decode_async(video, vlp, n, out.ptr, mf, meta.ptr, recs.ptr, meta.ptr + 8, rounds=16)
     hipmem.synchronize()
     count = int(meta.download(np.int32, 1)[0])
-    line0 = meta.download(np.float64, mf, offset=8)[:count]
-    first = meta.download(np.int32, mf, offset=8 + 8 * mf)[:count]
-    fields = out.download(np.uint16, mf * per).reshape(mf, per)[:count]
+    raw = recs.download(np.uint8, mf * size)
+    records = [_lib.VhsFieldV1.from_buffer_copy(raw[k * size:(k + 1) * size].tobytes()) for k in range(mf)]
+    # one call read the whole span: it stopped at a window past the span's end
+    assert count < mf and records[count].status == field.NOT_READ
+    assert records[count].window_start + records[count].window_length > n
+    fields = out.download(np.uint16, mf * per).reshape(mf, per)
     # the whole span's dropout rows, the parameters decode_run uses
     cap = 1024
     dp = field.dropout_params(fmt, 40e6, capacity=cap)
-    loc_ptr, nloc = dem.field_locations_device()
+    loc_ptr, nloc = dem.vhs_field_lines_device(3)
     drop = hipmem.DeviceBuffer(4 * mf * (1 + 3 * cap))
     rows = drop.ptr + 4 * mf
-    dem.detect_dropouts_device(env, n, loc_ptr, nloc, meta.ptr + 8, meta.ptr + 8 + 8 * mf, meta.ptr, mf, dp, drop.ptr,
-                               rows, rows + 4 * mf * cap, rows + 8 * mf * cap)
+    dem.detect_dropouts_device(env, n, loc_ptr, nloc, recs.ptr, meta.ptr, mf, dp, drop.ptr, rows,
+                               rows + 4 * mf * cap, rows + 8 * mf * cap)
     hipmem.synchronize()
     allr = drop.download(np.int32, mf * (1 + 3 * cap))
     nrow = allr[:mf]
     planes3 = allr[mf:].reshape(3, mf, cap)
-    drops = [tuple(planes3[k, f, :nrow[f]].tolist() for k in range(3)) for f in range(count)]
-    line_len = field.locator(g).line_length
-    field_len = line_len * (g.field_lines - 0.5)
+    ok = [k for k in range(count) if records[k].status == field.DECODED]
+    drops = [tuple(planes3[k, f, :nrow[f]].tolist() for k in range(3)) for f in ok]
+    first = np.array([records[k].is_first_field for k in ok], dtype=np.int32)
+    window = np.array([records[k].window_start for k in ok], dtype=np.int64)
+    fields = fields[ok]
+    count = len(ok)
+    field_len = g.line_len * (g.field_lines - 0.5)
     print(f"\n[{system}] tape {t.rf.size} samples, whole-span fields {count}", flush=True)
-    print(f"[{system}] whole-span dropout rows per field {nrow[:count].tolist()}", flush=True)
-    return dict(system=system, fmt=fmt, fl=fl, rf=rf_host, lut=lut, count=count, line0=line0, first=first,
+    print(f"[{system}] whole-span dropout rows per field {[len(d[0]) for d in drops]}", flush=True)
+    return dict(system=system, fmt=fmt, fl=fl, rf=rf_host, lut=lut, count=count, window=window, first=first,
                 fields=fields, per=per, drops=drops, span=int(5 * field_len) + N)
 
 
@@ -98,7 +109,8 @@ def test_run_writes_the_whole_span_fields(tape, tmp_path):
     recs = doc["fields"]
     assert [r["seqNo"] for r in recs] == list(range(1, res.fields_written + 1))
     assert [r["isFirstField"] for r in recs] == [bool(x) for x in tape["first"]]
-    np.testing.assert_array_equal([r["fileLoc"] for r in recs], np.floor(CUT + tape["line0"]).astype(np.int64))
+    # fileLoc: the RF sample where the field's read window starts (TbcDecodedField.StartSample)
+    np.testing.assert_array_equal([r["fileLoc"] for r in recs], tape["window"])
     for r, (fl_, sx, ex) in zip(recs, tape["drops"]):
         d = r["dropOuts"]
         assert (d["fieldLine"], d["startx"], d["endx"]) == (fl_, sx, ex)
@@ -110,9 +122,9 @@ def test_run_writes_the_whole_span_fields(tape, tmp_path):
     lines = diff.reshape(diff.shape[0], -1, vp["fieldWidth"]).max(axis=2)
     for f in range(diff.shape[0]):
         bad = np.nonzero(lines[f])[0]
-        print(f"  field {f} line0 {tape['line0'][f]:.3f}: {bad.size} lines differ"
+        print(f"  field {f} window {tape['window'][f]}: {bad.size} lines differ"
               + (f" ({bad[:6].tolist()} .. {bad[-3:].tolist()}), max {lines[f].max()}" if bad.size else ""), flush=True)
-    # the spans share the whole-span decode's block grid, so every plane sample is the same
+    # the spans share the whole-span decode's block grid and read windows, so every sample is the same
     np.testing.assert_array_equal(got, tape["fields"])
 
 
@@ -126,3 +138,24 @@ def test_run_float32_input_and_field_limit(tape, tmp_path):
     ib = np.fromfile(b + ".tbc", dtype="<u2")
     np.testing.assert_array_equal(ib, ia[:ib.size])
     assert json.load(open(b + ".tbc.json"))["videoParameters"]["numberOfSequentialFields"] == 2
+
+
+def test_run_cancel_through_a_short_read(tape, tmp_path):
+    """A cancel reported by a short read of a later span (the read callback runs on the
+    prefetch thread): the run stops with VHSDECODE_HIP_E_CANCELLED and writes no .tbc.json."""
+    base = str(tmp_path / "cancelled")
+    state = {"cancel": False}
+
+    def deliver(offset, count):
+        if offset > 0:
+            state["cancel"] = True
+            return 0
+        return count
+
+    from vhsdecode_hip import _lib
+    with pytest.raises(_lib.HipError) as e:
+        decode_run(tape["rf"], tape["fmt"], tape["fl"], base, sinc_table=tape["lut"], span_samples=tape["span"],
+                   cancel=lambda: state["cancel"], deliver=deliver)
+    assert e.value.status == -30009  # VHSDECODE_HIP_E_CANCELLED
+    assert state["cancel"]
+    assert not os.path.exists(base + ".tbc.json")

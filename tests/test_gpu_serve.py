@@ -1,6 +1,7 @@
 """Served windows (vhsdecode_hip_serve_*): many windows from several looped streams in
 flight at once, each result identical to the same window decoded alone, and to the
-plain device path on the same RF.  Runs on an MI355X only."""
+plain device path on the same RF (a fresh vhs field sequence from 0.12 s before the
+window, then PreviewFrameAssembler's frame rule, restated here).  Runs on an MI355X only."""
 import numpy as np
 import pytest
 
@@ -13,6 +14,7 @@ pytestmark = pytest.mark.gpu
 N = 32768
 STRIDE = N - 2 * 1024
 WINDOW = N + 100 * STRIDE  # 101 blocks, about 3.1 M samples (3-4 fields)
+PREROLL = 4_800_000        # 0.12 s at 40 MSPS (DecodePreviewSegmentProvider.PrerollSeconds)
 MAX_FIELDS = 4
 
 
@@ -41,7 +43,8 @@ def setup():
 
 
 def _server(setup, slots):
-    srv = HipServer(max_slots=slots, hbm_budget_bytes=32 << 30, max_window_samples=WINDOW, max_fields=MAX_FIELDS)
+    srv = HipServer(max_slots=slots, hbm_budget_bytes=32 << 30, max_window_samples=WINDOW + PREROLL + 2,
+                    max_fields=MAX_FIELDS)
     fids = [srv.add_format(fmt, fl, setup["lut"]) for fmt, fl, _ in setup["bases"]]
     for s in range(6):
         fmt, fl, pin = setup["bases"][s // 3]
@@ -73,31 +76,60 @@ def test_windows_in_the_mix_equal_windows_alone(setup):
             np.testing.assert_array_equal(alone.array[:n], outs[i].array[:n])
 
 
-def test_served_window_equals_plain_device_path(setup):
-    s, first = setup["reqs"][3]
+def _kept(records, origin, target, half_frame, max_out):
+    """PreviewFrameAssembler.Accept (src/VHSDecode.Preview/PreviewFrameAssembler.cs:55-95):
+    record indices of the fields of the kept frames."""
+    out, first, first_start = [], None, 0
+    for k, r in enumerate(records):
+        if len(out) + 2 > max_out:
+            break
+        if r.status != field.DECODED:
+            continue
+        start = origin + r.window_start
+        if r.is_first_field:
+            first, first_start = k, start
+            continue
+        if first is None:
+            continue
+        mid = max(first_start, start) + half_frame
+        a, first = first, None
+        if mid >= target:
+            out += [a, k]
+    return out
+
+
+@pytest.mark.parametrize("req", [3, 8])
+def test_served_window_equals_plain_device_path(setup, req):
+    s, first = setup["reqs"][req]
     fmt, fl, pin = setup["bases"][s // 3]
     base = pin.array
-    idx = (first + np.arange(WINDOW)) % base.size
+    start = max(0, first - PREROLL) & ~1
+    count = first - start + WINDOW
+    idx = (start + np.arange(count)) % base.size
     rf = np.ascontiguousarray(base[idx])
     per = _per(setup, s)
     out = hipmem.PinnedArray(MAX_FIELDS * per, np.uint16)
     with _server(setup, 2) as srv:
         r = srv.wait(srv.submit(s, first, WINDOW, MAX_FIELDS, out.ptr))
-    dem = HipBlockDemodulator(fl, max_blocks=128, sinc_table=setup["lut"])
+    dem = HipBlockDemodulator(fl, max_blocks=300, sinc_table=setup["lut"])
     nb = dem.blocks_for(rf.size)
     n = nb * STRIDE
     d_rf = hipmem.DeviceBuffer.from_array(rf)
     planes = hipmem.DeviceBuffer(4 * n * 4)
     flags = hipmem.DeviceBuffer(nb * 4)
     video, vlp, env, chroma = (planes.ptr + k * n * 4 for k in range(4))
-    dout = hipmem.DeviceBuffer(MAX_FIELDS * per * 2)
-    meta = hipmem.DeviceBuffer(64)
     dem.demod_span_device(d_rf.ptr, 0, nb, video, vlp, env, chroma, flags.ptr)
-    field.HipFieldDecoder(dem, fmt).decode_async(video, vlp, n, dout.ptr, MAX_FIELDS, meta.ptr)
-    hipmem.synchronize()
-    count = int(meta.download(np.int32, 1)[0])
-    assert r.status == 0 and r.field_count == count >= 1
-    np.testing.assert_array_equal(out.array[:count * per], dout.download(np.uint16, count * per))
+    dec = field.HipVhsFieldDecoder(dem, fmt)
+    # the server's field budget: output fields, the preroll's fields and 8
+    fps = float(fmt.sysparams["FPS"])
+    budget = MAX_FIELDS + int(np.ceil(0.12 * fps * 2.0)) + 8
+    res = dec.decode_span(video, vlp, n, max_fields=budget, rounds=4)
+    half_frame = int(np.floor(40e6 / (fps * 2.0) + 0.5))
+    kept = _kept(res.records[:budget], start + 1024, first, half_frame, MAX_FIELDS)
+    print(f"[serve {req}] records {len(res.records)} kept {kept} served {r.field_count}", flush=True)
+    assert r.status == 0 and r.field_count == len(kept) >= 2
+    for j, k in enumerate(kept):
+        np.testing.assert_array_equal(out.array[j * per:(j + 1) * per], res.outputs[k])
 
 
 def test_pageable_output_matches_pinned(setup):

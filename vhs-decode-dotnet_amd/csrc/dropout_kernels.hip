@@ -28,6 +28,7 @@
 // location index outputFirstLine + k, TbcFieldDecodePipeline.cs:2475-2483, which is its
 // detection start line LaserDiscLineOffset + 1, :6087-6095), so the detection runs over
 // locations 0 .. min(count - 1, line_count + 1) and the rows carry field line k + 1.
+// The read window and parity of each field come from its sequence record.
 #include <hip/hip_runtime.h>
 
 #include "vhsd_args.hpp"
@@ -44,22 +45,17 @@ constexpr int kMaskThreads = 1024;
 constexpr int kWalkThreads = 256;
 static_assert(kGroups == 128, "the threshold kernel reduces the group minima over two waves");
 
-// Field f's read window [w0, w1) in span coordinates: DecodeReadWindowPlanner.Resolve
-// (Decode/DecodeReadWindowPlanner.cs:37-63) of the intended start lead_lines before
-// line 0, with the requested read length; the same arithmetic places the TBC origin.
-__device__ inline void read_window(const DropoutArgs& a, double line0, long long& w0, long long& w1) {
-  // every position is below 2^31 (checked on the host), so 32-bit unsigned division
-  double intended = line0 - a.lead_lines * a.line_len;
-  intended = intended > 0.0 ? intended : 0.0;
-  long long read = (long long)intended - a.block_cut;
-  read = read > 0 ? read : 0;
-  const unsigned bl = (unsigned)a.block_length, st = (unsigned)a.stride;
-  const unsigned aligned = ((unsigned)read / bl) * bl;
-  const unsigned end = aligned + (unsigned)a.read_samples;
-  w0 = (long long)((aligned / st) * st);
-  w1 = (long long)(end / st + 1) * st;
-  w0 = w0 < a.span_len ? w0 : a.span_len;
-  w1 = w1 < a.span_len ? w1 : a.span_len;
+constexpr long long kMaxWindow = 3600000;  // every depth-15 node of a window's pairwise tree is a leaf
+
+// Field f's read window [w0, w1) in span coordinates, as its record states it (the
+// window DecodeReadWindowPlanner.Resolve placed, Decode/DecodeReadWindowPlanner.cs:37-63);
+// false when the window is longer than the pairwise tree the kernels walk.
+__device__ inline bool read_window(const DropoutArgs& a, int f, long long& w0, long long& w1) {
+  const long long s = a.fields[f].window_start, l = a.fields[f].window_length;
+  w0 = s < 0 ? 0 : (s > a.span_len ? a.span_len : s);
+  const long long e = s + (l > 0 ? l : 0);
+  w1 = e < w0 ? w0 : (e > a.span_len ? a.span_len : e);
+  return w1 - w0 <= kMaxWindow;
 }
 
 // Node of the pairwise tree of n values at `level` along path bits `path` (most
@@ -138,8 +134,8 @@ __global__ void __launch_bounds__(kSlotThreads) dropout_mean_kernel(DropoutArgs 
   if (f >= *a.field_count) return;
   const int g = blockIdx.x, t = threadIdx.x;
   long long w0, w1;
-  read_window(a, a.line0[f], w0, w1);
-  const int n = (int)(w1 - w0);
+  const bool fits = read_window(a, f, w0, w1);
+  const int n = fits && a.fields[f].status == 0 ? (int)(w1 - w0) : 0;
   int goff, glen;
   bool gown;
   pw_descend(n, kGroupDepth, (unsigned)g, goff, glen, gown);
@@ -251,8 +247,8 @@ __global__ void __launch_bounds__(kGroups) dropout_threshold_kernel(DropoutArgs 
   if (f >= *a.field_count) return;
   const int t = threadIdx.x;
   long long w0, w1;
-  read_window(a, a.line0[f], w0, w1);
-  const long long n = w1 - w0;
+  const bool fits = read_window(a, f, w0, w1);
+  const long long n = fits ? w1 - w0 : 0;
   v[t] = a.partials[(size_t)f * kGroups + t];
   float m = a.group_min[(size_t)f * kGroups + t];
 #pragma unroll
@@ -279,7 +275,7 @@ __global__ void __launch_bounds__(kGroups) dropout_threshold_kernel(DropoutArgs 
   const double thr = a.absolute_threshold >= 0.0 ? a.absolute_threshold : (double)(mean * (float)a.fraction);
   const int nloc = a.nloc;
   const double* loc = a.locations + (size_t)f * nloc;
-  const int line_count = a.field_first[f] ? a.first_lines : a.second_lines;
+  const int line_count = a.fields[f].is_first_field ? a.first_lines : a.second_lines;
   const int end_line = nloc - 1 < line_count + 1 ? nloc - 1 : line_count + 1;
   long long a0 = 0, b0 = 0;
   if (end_line > 0) {
@@ -297,7 +293,11 @@ __global__ void __launch_bounds__(kGroups) dropout_threshold_kernel(DropoutArgs 
   inf.b0 = b0;
   inf.nwords = nwords;
   inf.end_line = end_line;
-  inf.status = span <= 0 ? 0 : ((nwords > a.words_cap || nwords > kDropSummaryWords * 32) ? -1 : 1);
+  // a failed record (no field decoded) has no rows
+  inf.status = a.fields[f].status != 0 ? 0
+               : !fits                 ? -1
+               : span <= 0             ? 0
+               : ((nwords > a.words_cap || nwords > kDropSummaryWords * 32) ? -1 : 1);
   // no window sample at or below the threshold: no range can open, the masks and the
   // walk have nothing to do
   const float window_min = fminf(wm[0], wm[1]);

@@ -839,7 +839,8 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
                                                          const int* first_lines, const double* line_x,
                                                          const double* line_y0, const double* pow_table,
                                                          FieldTbcParams p, const float* __restrict__ lut,
-                                                         unsigned short* out, const int* field_count) {
+                                                         unsigned short* out, const int* out_index,
+                                                         const int* field_count) {
   __shared__ float4 rows[256 / 64][kTbcStage];
   const int f = blockIdx.y;
   if (field_count && f >= *field_count) return;
@@ -945,211 +946,7 @@ __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long 
     o = 65535;
   else
     o = (unsigned short)v;
-  out[(long long)f * dest_len + dd] = o;
-}
-
-// ------------------------------------------------------------------ line locations
-// Device form of this backend's compact line locator (vhsdecode_hip/field.py,
-// line_locations_from_pulses, the stand-in for SyncAnalyzer.EstimateTiming /
-// ClassifyPulses / RefinePulses, ResolveLine0Location and BuildUpstreamLineLocations,
-// Dsp/SyncAnalysis.cs:131-, Decode/TbcFieldDecodePipeline.cs:1176-1295): the same
-// FP64 arithmetic in the same order, so its results equal the host form exactly.
-//
-// Pass 1 (one workgroup): pulses are classified by length; the first broad pulse of
-// each vertical-sync group anchors a field candidate; its parity comes from the
-// half-line phase of the broad group against the last hsync pulse before it.
-// Pass 2 (one workgroup per candidate): each line start snaps to the nearest pulse
-// within `tol` lines of its prediction; lines no pulse opens are interpolated from
-// their measured neighbours (numpy.interp semantics, clamped at the ends).
-// Pass 3 (one thread): candidates with at least one measured line become fields,
-// in order, up to max_fields; the per-field tables for the hsync and TBC kernels
-// are written on the device.
-
-constexpr int kLocThreads = 512;  // lines + 1 <= kLocThreads
-
-__device__ __forceinline__ int wave_incl_max(int v, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int u = __shfl_up(v, o);
-    if (lane >= o) v = v > u ? v : u;
-  }
-  return v;
-}
-
-__device__ __forceinline__ int wave_incl_min_rev(int v, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int u = __shfl_down(v, o);
-    if (lane + o < 64) v = v < u ? v : u;
-  }
-  return v;
-}
-
-__global__ void __launch_bounds__(1024) field_candidates_kernel(const int* starts, const int* lens,
-                                                                const int* pulse_count, int capacity, LocateParams lp,
-                                                                int cand_cap, double* cand_line0, int* cand_first,
-                                                                int* cand_count) {
-#pragma clang fp contract(off)
-  __shared__ int wave_max[16];
-  __shared__ int wave_broad_last[16];
-  __shared__ int wave_cnt[16];
-  __shared__ int carry_hs, carry_broad, ncand;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int pc = *pulse_count;
-  const int P = pc < capacity ? pc : capacity;
-  if (tid == 0) {
-    carry_hs = -1;
-    carry_broad = 0;
-    ncand = 0;
-  }
-  __syncthreads();
-  for (int t0 = 0; t0 < P; t0 += 1024) {
-    const int i = t0 + tid;
-    const bool valid = i < P;
-    const double len = valid ? (double)lens[i] : 0.0;
-    const bool broad = valid && len > lp.broad_min;
-    const bool hs = valid && len > lp.hsync_min && !broad;
-    const int wm = wave_incl_max(hs ? i : -1, lane);
-    const int pb = __shfl_up(broad ? 1 : 0, 1);
-    if (lane == 63) {
-      wave_max[wave] = wm;
-      wave_broad_last[wave] = broad ? 1 : 0;
-    }
-    __syncthreads();
-    int last_hs = carry_hs;
-    for (int w = 0; w < wave; w++) last_hs = last_hs > wave_max[w] ? last_hs : wave_max[w];
-    last_hs = last_hs > wm ? last_hs : wm;
-    const int prev_broad = lane > 0 ? pb : (wave > 0 ? wave_broad_last[wave - 1] : carry_broad);
-    bool ok = false;
-    double line0 = 0.0;
-    bool first = false;
-    if (broad && !(i > 0 && prev_broad) && last_hs >= 0) {
-      const double d = ((double)starts[i] - (double)starts[last_hs]) / lp.line_len;
-      const bool mid = fabs(d - rint(d)) > 0.25;
-      line0 = (double)starts[i] - (mid ? 0.5 * lp.line_len : 0.0);
-      first = !mid;
-      ok = !(line0 + (double)(lp.lines + 1) * lp.line_len >= (double)lp.span_len);
-    }
-    const unsigned long long m = __ballot(ok);
-    if (lane == 0) wave_cnt[wave] = __popcll(m);
-    __syncthreads();
-    int before = ncand;
-    for (int w = 0; w < wave; w++) before += wave_cnt[w];
-    if (ok) {
-      const int idx = before + __popcll(m & ((1ull << lane) - 1ull));
-      if (idx < cand_cap) {
-        cand_line0[idx] = line0;
-        cand_first[idx] = first ? 1 : 0;
-      }
-    }
-    __syncthreads();
-    if (tid == 0) {
-      int tot = 0, mx = carry_hs;
-      for (int w = 0; w < 16; w++) {
-        tot += wave_cnt[w];
-        mx = mx > wave_max[w] ? mx : wave_max[w];
-      }
-      ncand += tot;
-      carry_hs = mx;
-      carry_broad = wave_broad_last[15];
-    }
-    __syncthreads();
-  }
-  if (tid == 0) *cand_count = ncand < cand_cap ? ncand : cand_cap;
-}
-
-__global__ void __launch_bounds__(kLocThreads) field_lines_kernel(const int* starts, const int* pulse_count,
-                                                                  int capacity, LocateParams lp,
-                                                                  const double* cand_line0, const int* cand_count,
-                                                                  double* cand_locs, unsigned char* cand_filled,
-                                                                  int* cand_hit) {
-#pragma clang fp contract(off)
-  __shared__ double off_s[kLocThreads];
-  __shared__ int wmax[kLocThreads / 64], wmin[kLocThreads / 64];
-  const int c = blockIdx.x;
-  if (c >= *cand_count) return;  // uniform across the workgroup
-  const int n = threadIdx.x, lane = n & 63, wave = n >> 6;
-  const int pc = *pulse_count;
-  const int P = pc < capacity ? pc : capacity;
-  const int nloc = lp.lines + 1;
-  const bool valid = n < nloc;
-  const double line0 = cand_line0[c];
-  const double nl = (double)n * lp.line_len;
-  const double pred = line0 + nl;
-  bool hit = false;
-  double best = 0.0;
-  if (valid && P >= 2) {
-    int lo = 0, hi = P;  // first index with starts[idx] >= pred
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if ((double)starts[mid] < pred) lo = mid + 1;
-      else hi = mid;
-    }
-    const int j = lo < 1 ? 1 : (lo > P - 1 ? P - 1 : lo);
-    const double left = (double)starts[j - 1], right = (double)starts[j];
-    best = fabs(left - pred) <= fabs(right - pred) ? left : right;
-    hit = fabs(best - pred) <= lp.tol * lp.line_len;
-  }
-  const double off = best - nl;
-  off_s[n] = off;
-  const int pv = wave_incl_max(hit ? n : -1, lane);
-  const int nv = wave_incl_min_rev(hit ? n : 0x7fffffff, lane);
-  if (lane == 63) wmax[wave] = pv;
-  if (lane == 0) wmin[wave] = nv;
-  const int any_hit = __syncthreads_or(hit ? 1 : 0);
-  int prev = pv, next = nv;
-  for (int w = 0; w < wave; w++) prev = prev > wmax[w] ? prev : wmax[w];
-  for (int w = wave + 1; w < kLocThreads / 64; w++) next = next < wmin[w] ? next : wmin[w];
-  if (!valid) return;
-  double loc;
-  if (hit) {
-    loc = best;
-  } else {
-    double v;
-    if (prev < 0 && next == 0x7fffffff) v = 0.0;  // no measured line: the candidate is dropped
-    else if (prev < 0) v = off_s[next];
-    else if (next == 0x7fffffff) v = off_s[prev];
-    else {
-      const double slope = (off_s[next] - off_s[prev]) / (double)(next - prev);
-      v = slope * (double)(n - prev) + off_s[prev];
-    }
-    loc = v + nl;
-  }
-  cand_locs[(long long)c * nloc + n] = loc;
-  cand_filled[(long long)c * nloc + n] = hit ? 0 : 1;
-  if (n == 0) cand_hit[c] = any_hit;
-}
-
-__global__ void field_select_kernel(const float* sync_ref, long long span_len, LocateParams lp, int max_fields,
-                                    const double* cand_line0, const int* cand_first, const int* cand_count,
-                                    const int* cand_hit, const double* cand_locs, const unsigned char* cand_filled,
-                                    double* ref_base, unsigned char* fo_base, FieldTables t, int* field_count) {
-  if (threadIdx.x != 0) return;
-  const int nloc = lp.lines + 1;
-  const int nc = *cand_count;
-  int f = 0;
-  for (int c = 0; c < nc && f < max_fields; c++) {
-    if (!cand_hit[c]) continue;
-    const double line0 = cand_line0[c];
-    double intended = line0 - lp.lead_lines * lp.line_len;
-    intended = intended > 0.0 ? intended : 0.0;
-    long long read = (long long)intended - lp.block_cut;
-    read = read > 0 ? read : 0;
-    const long long aligned = (read / lp.block_length) * lp.block_length;
-    t.origins[f] = (aligned / lp.stride) * lp.stride;
-    t.spans[f] = sync_ref;
-    t.span_len[f] = span_len;
-    t.locs[f] = cand_locs + (long long)c * nloc;
-    t.filled[f] = cand_filled + (long long)c * nloc;
-    t.refined[f] = ref_base + (long long)f * nloc;
-    t.filled_out[f] = fo_base + (long long)f * nloc;
-    t.line_counts[f] = nloc;
-    t.info_line0[f] = line0;
-    t.info_first[f] = cand_first[c];
-    f++;
-  }
-  *field_count = f;
-  for (int g = f; g < max_fields; g++) t.line_counts[g] = 0;
+  out[(long long)(out_index ? out_index[f] : f) * dest_len + dd] = o;
 }
 
 }  // namespace vhsd
@@ -1222,7 +1019,8 @@ hipError_t vhsd_launch_tbc_fields(const float* src, long long src_len, double bi
                                   int fields, double nominal, double line_decay, const double* pow_table,
                                   double* line_x, double* line_y0, int out_len, int smooth, double ire0,
                                   double out_scale, double inv_hz_ire, double conv_offset, const float* lut,
-                                  unsigned short* out, const int* field_count, hipStream_t stream) {
+                                  unsigned short* out, const int* out_index, const int* field_count,
+                                  hipStream_t stream) {
   if (fields <= 0) return hipSuccess;
   const int line_stride = lines + (first_lines ? max_first_line : 0);
   if (line_stride > vhsd::kLevelThreads || nloc < 2 || line_stride > nloc - 1 || max_first_line < 0)
@@ -1235,23 +1033,7 @@ hipError_t vhsd_launch_tbc_fields(const float* src, long long src_len, double bi
   const long long dest = (long long)out_len * lines;
   hipLaunchKernelGGL(vhsd::tbc_fields_kernel, dim3((unsigned)((dest + 255) / 256), fields), dim3(256), 0, stream, src,
                      src_len, bias, origins, refined, first_lines, line_x, line_y0, pow_table, p, lut, out,
-                     field_count);
-  return hipGetLastError();
-}
-
-hipError_t vhsd_launch_locate(const float* sync_ref, long long span_len, const int* starts, const int* lens,
-                              const int* pulse_count, int capacity, const vhsd::LocateParams* lp, int max_fields,
-                              int cand_cap, double* cand_line0, int* cand_first, int* cand_count, double* cand_locs,
-                              unsigned char* cand_filled, int* cand_hit, double* ref_base, unsigned char* fo_base,
-                              const vhsd::FieldTables* t, int* field_count, hipStream_t stream) {
-  if (lp->lines + 1 > vhsd::kLocThreads || max_fields <= 0 || cand_cap <= 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(vhsd::field_candidates_kernel, dim3(1), dim3(1024), 0, stream, starts, lens, pulse_count,
-                     capacity, *lp, cand_cap, cand_line0, cand_first, cand_count);
-  hipLaunchKernelGGL(vhsd::field_lines_kernel, dim3(cand_cap), dim3(vhsd::kLocThreads), 0, stream, starts,
-                     pulse_count, capacity, *lp, cand_line0, cand_count, cand_locs, cand_filled, cand_hit);
-  hipLaunchKernelGGL(vhsd::field_select_kernel, dim3(1), dim3(64), 0, stream, sync_ref, span_len, *lp, max_fields,
-                     cand_line0, cand_first, cand_count, cand_hit, cand_locs, cand_filled, ref_base, fo_base, *t,
-                     field_count);
+                     out_index, field_count);
   return hipGetLastError();
 }
 

@@ -1,25 +1,28 @@
 // Whole-run decode behind vhsdecode_hip_run: the counterpart of vhsdecode_cuda_fast_run
-// (src/VHSDecode.CudaFast.Native/src/vhsdecode_cuda_fast.cpp:238-370).  The RF is read
-// through the caller's callback one span at a time into pinned host memory, each span runs
-// the device chain (vhsdecode_hip_demod_span_device, vhsdecode_hip_decode_span_fields_device
-// and, when asked, vhsdecode_hip_detect_dropouts_device) on one stream, and the fields are
-// appended in order to <base>.tbc; <base>.tbc.json is written at the end.
+// (src/VHSDecode.CudaFast.Native/src/vhsdecode_cuda_fast.cpp:238-370) for the vhs field
+// sequence of TbcFieldSequenceDecodeEngine (Decode/TbcFieldSequenceDecodeEngine.cs:161-171,
+// 804-846).  The RF is read through the caller's callback one span at a time into pinned
+// host memory; each span runs the device chain (vhsdecode_hip_demod_span_device,
+// vhsdecode_hip_decode_vhs_fields_device and, when asked,
+// vhsdecode_hip_detect_dropouts_device) on one stream, and the decoded fields are
+// appended in sequence order to <base>.tbc; <base>.tbc.json is written at the end.
 //
 // Span placement: span k starts at RF sample k * advance, a multiple of lcm(block length,
-// block stride) (491520 samples for 1024/1024 cuts), and consecutive spans overlap by a
-// field and a half plus the read-window lead (plus a dropout read window when detection is
-// on).  Plane sample i of a span starting at RF sample p is RF sample p + cut + i, so the
-// blocks two spans share are the same blocks, and each field's read-window origin (aligned
-// down to a block length, then to a stride, in span coordinates) lands where it lands in
-// one span from the start of the input: the TBC's FP32 source coordinates, which count from
-// that origin, are then the same, and so is every output sample.  (A format whose lcm
-// exceeds a field steps by the stride instead.)  A field whose line 0 lies within half a
-// field of the last one written is the same field found again and is skipped.  Because the
-// places are fixed, a prefetch thread reads span k + 1 through the callback while the
-// device decodes span k (the callbacks run on that thread for every span but the first).
+// block stride) (491520 samples for 1024/1024 cuts).  Plane sample i of a span starting at
+// RF sample p is global plane sample p + i, so the blocks two spans share are the same
+// blocks and a field's read window (aligned down to a block length, then to a stride) is
+// the same window in every span that holds it.  One field sequence runs through all the
+// spans: a span's calls decode fields until the next read window leaves the span, and
+// the next span (overlapping by the longest read window and a grid step, so it holds that
+// window whole) continues from there with the carried level, parity and sync state.
+// Because the places are fixed, a prefetch thread reads span k + 1 through the callback
+// while the device decodes span k (the callbacks run on that thread for every span but
+// the first).  A failed field (the reference's recovery kinds) writes nothing and the
+// sequence continues at its recovery offset.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -36,6 +39,7 @@ namespace {
 
 using vhsd_host::set_error;
 constexpr int kBlock = 32768;
+constexpr int kRounds = 4;  // field sequence rounds per call
 
 #define RUN_HIP(expr)                                                                         \
   do {                                                                                        \
@@ -76,7 +80,7 @@ struct OwnedStream {
 
 struct FieldRecord {
   bool first = false;
-  uint64_t file_loc = 0;  // RF sample of the field's line 0
+  uint64_t file_loc = 0;  // RF sample where the field's read window starts (TbcDecodedField.StartSample)
   std::vector<int32_t> line, start, end;
 };
 
@@ -135,67 +139,58 @@ bool write_json(const std::string& path, const vhsdecode_hip_field_params_v1& fp
   return std::rename(tmp.c_str(), path.c_str()) == 0;
 }
 
-// End of a field's dropout read window in span coordinates before the span clamps it: the
-// window the dropout kernels place (DecodeReadWindowPlanner.Resolve,
-// Decode/DecodeReadWindowPlanner.cs:37-63), read_samples from the block-aligned start,
-// rounded up past a whole stride.
-int64_t dropout_window_end(const vhsdecode_hip_dropout_params_v1& d, double line0) {
-  const double intended = std::max(0.0, line0 - d.lead_lines * d.line_length);
-  const int64_t read = std::max<int64_t>(0, (int64_t)intended - d.block_cut);
-  const int64_t aligned = read / d.block_length * d.block_length;
-  return ((aligned + d.read_samples) / d.block_stride + 1) * d.block_stride;
-}
-
 int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_result_v1& res) {
   const vhsdecode_hip_serve_format_v1& fm = *cfg.format;
   const vhsdecode_hip_field_params_v1& fp = *fm.field;
-  const vhsdecode_hip_locator_v1& lz = *fm.locator;
+  const vhsdecode_hip_vhs_sync_v1& sy = *fm.sync;
   const int stride = kBlock - fm.block_cut - fm.block_cut_end;
   if (fm.block_cut < 0 || fm.block_cut_end < 0 || stride <= 0)
     return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "block cuts leave no stride");
-  if (!(lz.line_length > 0.0) || fp.field_lines <= 0 || fp.output_line_length <= 0)
+  if (sy.block_stride != stride || sy.block_cut != fm.block_cut || sy.block_length != kBlock)
+    return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "sync block geometry differs from the format's block cuts");
+  if (!(sy.line_period_us > 0.0) || fp.field_lines <= 0 || fp.output_line_length <= 0 || sy.read_samples <= 0 ||
+      !(fp.sample_rate_mhz > 0.0))
     return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "field geometry must be positive");
-  // one field of input, and the span: at least three fields, so each span holds a whole
-  // field beyond the half field of overlap with the previous one
-  const double field_len = lz.line_length * (fp.field_lines - 0.5);
-  const int64_t align = lz.block_length > 0 ? lz.block_length : kBlock;
-  const int64_t grid = align / std::gcd(align, (int64_t)stride) * stride;
-  const int64_t step = grid <= (int64_t)field_len ? grid : stride;
-  // Spans sit at fixed places, span k at k * advance, so the next span's RF can be read
-  // while the device decodes the current one.  Consecutive spans overlap by a field and a
-  // half plus the read-window lead (and, with dropout detection, a whole read window: the
-  // envelope mean runs over it) plus a step of the grid, so every field lies whole, with
-  // its window, inside one span.
-  const double lead = lz.lead_lines * lz.line_length;
-  const int64_t window = cfg.dropouts ? cfg.dropouts->read_samples + 2 * (int64_t)stride : 0;
-  const int64_t overlap = (int64_t)std::ceil(1.5 * field_len + lead) + fm.block_cut + window + step;
-  const int64_t min_span = overlap + step + stride + kBlock;
+  // Spans sit on multiples of the grid lcm(block length, stride), so a field's read
+  // window (aligned down to a block, then to a stride) is the same window in every span
+  // that holds it; consecutive spans overlap by the longest read window plus a grid step,
+  // so the field the sequence reaches at the end of one span lies whole in the next.
+  const int64_t grid = (int64_t)kBlock / std::gcd((int64_t)kBlock, (int64_t)stride) * stride;
+  const double line_len = sy.line_period_us * fp.sample_rate_mhz;
+  const double field_len = line_len * (fp.field_lines - 0.5);
+  const int64_t window_max = sy.read_samples + 2 * (int64_t)stride;
+  const int64_t overlap = window_max + grid;
+  const int64_t min_span = overlap + grid + stride + kBlock;
   const int64_t span_req =
       cfg.span_samples > 0 ? cfg.span_samples : std::max(min_span, (int64_t)std::ceil(24.0 * field_len) + kBlock);
   if (span_req < min_span)
     return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT,
-                     "span_samples must hold the span overlap (a field and a half, the lead and any dropout window) "
-                     "and one step of the span grid");
+                     "span_samples must hold the span overlap (the longest read window and a grid step) and one "
+                     "step of the span grid");
   if (span_req > (int64_t)INT32_MAX)
     return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "span_samples above 2^31 - 1");
   const int32_t nb_max = (int32_t)((span_req - kBlock) / stride + 1);
   const int64_t read_max = (int64_t)(nb_max - 1) * stride + kBlock;
   const int64_t plane_max = (int64_t)nb_max * stride;
-  const int32_t max_fields = (int32_t)std::ceil((double)plane_max / field_len) + 2;
-  const int64_t advance = (plane_max - overlap) / step * step;
+  // records per call: every field of a span, its failures and the stop record
+  const int32_t max_fields = (int32_t)std::ceil((double)plane_max / field_len) + 8;
+  const int64_t advance = (plane_max - overlap) / grid * grid;
   const size_t per_field = (size_t)fp.field_lines * fp.output_line_length;
   const int32_t cap = cfg.dropouts ? cfg.dropouts->capacity : 0;
-  if (cfg.dropouts && (cap <= 0 || cfg.dropouts->read_samples <= 0 || cfg.dropouts->block_length <= 0 ||
-                       cfg.dropouts->block_stride <= 0))
-    return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "dropout capacity, read window and block geometry must be positive");
+  if (cfg.dropouts && cap <= 0) return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "dropout capacity must be positive");
 
   const std::string base(cfg.output_base_utf8);
   const std::string tbc_path = base + ".tbc", json_path = base + ".tbc.json";
   if (!cfg.overwrite && (file_exists(tbc_path) || file_exists(json_path)))
     return set_error(VHSDECODE_HIP_E_IO, "an output file exists and overwrite is off: " + tbc_path);
 
-  if (cfg.cancel_callback && cfg.cancel_callback(cfg.user_data))
-    return set_error(VHSDECODE_HIP_E_CANCELLED, "the run was cancelled before it started");
+  // the cancel callback runs on this thread; the read wrapper latches a cancel it sees
+  std::atomic<bool> cancelled{false};
+  auto poll_cancel = [&]() -> bool {
+    if (!cancelled.load() && cfg.cancel_callback && cfg.cancel_callback(cfg.user_data)) cancelled.store(true);
+    return cancelled.load();
+  };
+  if (poll_cancel()) return set_error(VHSDECODE_HIP_E_CANCELLED, "the run was cancelled before it started");
   RUN_HIP(hipSetDevice(cfg.device_id));
   OwnedContext ctx;
   {
@@ -215,22 +210,24 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
   }
   OwnedStream st;
   RUN_HIP(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
-  // device: RF, four planes, block flags, fields, field metadata, dropout rows
-  DeviceBuf d_rf, d_planes, d_flags, d_fields, d_meta, d_drop;
+  // device: RF, four planes, block flags, fields, records, count and next begin, dropout rows
+  DeviceBuf d_rf, d_planes, d_flags, d_fields, d_records, d_meta, d_drop;
   RUN_HIP(hipMalloc(&d_rf.p, sizeof(int16_t) * (size_t)read_max));
   RUN_HIP(hipMalloc(&d_planes.p, sizeof(float) * 4 * (size_t)plane_max));
   RUN_HIP(hipMalloc(&d_flags.p, sizeof(uint32_t) * (size_t)nb_max));
   RUN_HIP(hipMalloc(&d_fields.p, sizeof(uint16_t) * per_field * (size_t)max_fields));
-  // meta: count (int32, padded to 8), line 0 (double[max_fields]), first (int32[max_fields])
-  const size_t meta_bytes = 8 + (size_t)max_fields * 12;
+  const size_t rec_bytes = sizeof(vhsdecode_hip_vhs_field_v1) * (size_t)max_fields;
+  RUN_HIP(hipMalloc(&d_records.p, rec_bytes));
+  const size_t meta_bytes = 16;  // record count (int32, padded to 8), next begin (int64)
   RUN_HIP(hipMalloc(&d_meta.p, meta_bytes));
   // dropouts: row counts, then field line / start x / end x, [max_fields][cap] each
   const size_t drop_bytes = sizeof(int32_t) * (size_t)max_fields * (1 + 3 * (size_t)cap);
   if (cfg.dropouts) RUN_HIP(hipMalloc(&d_drop.p, drop_bytes));
-  PinnedBuf h_rf[2], h_fields[2], h_meta, h_drop;
+  PinnedBuf h_rf[2], h_fields[2], h_records, h_meta, h_drop;
   for (PinnedBuf& b : h_rf) RUN_HIP(hipHostMalloc(&b.p, sizeof(int16_t) * (size_t)read_max, hipHostMallocDefault));
   for (PinnedBuf& b : h_fields)
     RUN_HIP(hipHostMalloc(&b.p, sizeof(uint16_t) * per_field * (size_t)max_fields, hipHostMallocDefault));
+  RUN_HIP(hipHostMalloc(&h_records.p, rec_bytes, hipHostMallocDefault));
   RUN_HIP(hipHostMalloc(&h_meta.p, meta_bytes, hipHostMallocDefault));
   if (cfg.dropouts) RUN_HIP(hipHostMalloc(&h_drop.p, drop_bytes, hipHostMallocDefault));
   std::vector<float> staging[2];
@@ -241,20 +238,16 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
   tbc.f = std::fopen(tbc_path.c_str(), "wb");
   if (!tbc.f) return set_error(VHSDECODE_HIP_E_IO, "cannot create " + tbc_path);
 
-  unsigned char* meta = static_cast<unsigned char*>(h_meta.p);
-  const int32_t* h_count = reinterpret_cast<const int32_t*>(meta);
-  const double* h_line0 = reinterpret_cast<const double*>(meta + 8);
-  const int32_t* h_first = reinterpret_cast<const int32_t*>(meta + 8 + 8 * (size_t)max_fields);
-  unsigned char* dm = static_cast<unsigned char*>(d_meta.p);
-  int32_t* d_count = reinterpret_cast<int32_t*>(dm);
-  double* d_line0 = reinterpret_cast<double*>(dm + 8);
-  int32_t* d_first = reinterpret_cast<int32_t*>(dm + 8 + 8 * (size_t)max_fields);
+  const int32_t* h_count = static_cast<const int32_t*>(h_meta.p);
+  const int64_t* h_next = reinterpret_cast<const int64_t*>(static_cast<unsigned char*>(h_meta.p) + 8);
+  int32_t* d_count = static_cast<int32_t*>(d_meta.p);
+  int64_t* d_next = reinterpret_cast<int64_t*>(static_cast<unsigned char*>(d_meta.p) + 8);
+  const vhsdecode_hip_vhs_field_v1* recs = static_cast<const vhsdecode_hip_vhs_field_v1*>(h_records.p);
   int32_t* dd = static_cast<int32_t*>(d_drop.p);
   const int32_t* hd = static_cast<const int32_t*>(h_drop.p);
   const size_t rows = (size_t)max_fields * (size_t)cap;
 
   std::vector<FieldRecord> written;
-  double last_abs = -1.0;  // RF position of the last written field's line 0
   const uint64_t total = cfg.total_samples;
   uint32_t spans = 0, drop_rows = 0;
   bool done = false;
@@ -268,6 +261,12 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
     bool last = true;
     bool ok = false;
   };
+  // The read callback, latching a cancel it reports through a short read.
+  auto read_input = [&](void* dst, uint64_t p, size_t n) -> size_t {
+    const size_t got = cfg.read_callback(cfg.user_data, dst, p, n);
+    if (got < n && cfg.cancel_callback && cfg.cancel_callback(cfg.user_data)) cancelled.store(true);
+    return got;
+  };
   auto read_span = [&](int64_t p, int slot) -> SpanRead {
     SpanRead r;
     r.pos = p;
@@ -279,10 +278,10 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
     int16_t* dst = static_cast<int16_t*>(h_rf[slot].p);
     size_t got;
     if (cfg.input_sample_format == VHSDECODE_HIP_INPUT_INT16) {
-      got = cfg.read_callback(cfg.user_data, dst, (uint64_t)p, (size_t)n);
+      got = read_input(dst, (uint64_t)p, (size_t)n);
     } else {
       std::vector<float>& stage = staging[slot];
-      got = cfg.read_callback(cfg.user_data, stage.data(), (uint64_t)p, (size_t)n);
+      got = read_input(stage.data(), (uint64_t)p, (size_t)n);
       const size_t m = std::min(got, (size_t)n);
       for (size_t q = 0; q < m; q++) {
         const float v = stage[q];
@@ -302,8 +301,8 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
     return r;
   };
 
-  // .tbc writes of one span's fields on a writer thread, one job in flight, so the file
-  // keeps the field order; a job writes fields [first, first + n) of a staging slot.
+  // .tbc writes on a writer thread, one job in flight, so the file keeps the field order;
+  // a job writes the listed fields of a staging slot.
   std::future<bool> write_job;
   auto write_fields = [&](int slot, std::vector<int32_t> which) -> bool {
     const uint16_t* base_p = static_cast<const uint16_t*>(h_fields[slot].p);
@@ -315,9 +314,12 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
 
   const auto t0 = std::chrono::steady_clock::now();
   SpanRead cur = read_span(0, 0);
+  int64_t next_global = 0;  // where the field sequence continues (global plane coordinates)
+  int64_t prev_pos = 0;
+  bool fresh = true;
+  int call = 0;
   for (int k = 0; cur.ok && !done; k++) {
-    if (cfg.cancel_callback && cfg.cancel_callback(cfg.user_data))
-      return set_error(VHSDECODE_HIP_E_CANCELLED, "the run was cancelled");
+    if (poll_cancel()) break;
     // the next span's read overlaps this span's device work (the future's destructor
     // waits for it on every exit path, before the buffers go)
     std::future<SpanRead> next;
@@ -337,67 +339,91 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
     sp.block_flags = static_cast<uint32_t*>(d_flags.p);
     int32_t rc = vhsdecode_hip_demod_span_device(ctx.c, static_cast<const int16_t*>(d_rf.p), 0, nb, &sp, st.s);
     if (rc) return rc;
-    rc = vhsdecode_hip_decode_span_fields_device(ctx.c, sp.video_lowpass, sp.video, plane, &fp, &lz, max_fields,
-                                                 static_cast<uint16_t*>(d_fields.p), d_count, d_line0, d_first,
-                                                 st.s);
-    if (rc) return rc;
-    if (cfg.dropouts) {
-      const double* locs = nullptr;
-      int32_t nloc = 0;
-      rc = vhsdecode_hip_field_locations_device(ctx.c, &locs, &nloc);
-      if (rc) return rc;
-      rc = vhsdecode_hip_detect_dropouts_device(ctx.c, sp.envelope, plane, locs, nloc, d_line0, d_first, d_count,
-                                                max_fields, cfg.dropouts, dd, dd + max_fields, dd + max_fields + rows,
-                                                dd + max_fields + 2 * rows, nullptr, nullptr, nullptr, st.s);
-      if (rc) return rc;
-      RUN_HIP(hipMemcpyAsync(h_drop.p, d_drop.p, drop_bytes, hipMemcpyDeviceToHost, st.s));
-    }
-    RUN_HIP(hipMemcpyAsync(h_meta.p, d_meta.p, meta_bytes, hipMemcpyDeviceToHost, st.s));
-    RUN_HIP(hipStreamSynchronize(st.s));
-    const int32_t count = std::min(*h_count, max_fields);
-    // slot k & 1 was last written from two spans ago; that job has finished (one in flight)
-    if (count > 0)
-      RUN_HIP(hipMemcpyAsync(h_fields[k & 1].p, d_fields.p, sizeof(uint16_t) * per_field * (size_t)count,
-                             hipMemcpyDeviceToHost, st.s));
-    RUN_HIP(hipStreamSynchronize(st.s));
     spans++;
-    std::vector<int32_t> to_write;
-    for (int32_t f = 0; f < count; f++) {
-      const double abs0 = (double)pos + fm.block_cut + h_line0[f];
-      if (last_abs >= 0.0 && abs0 < last_abs + 0.5 * field_len) continue;  // written from the last span
-      // a field whose read window leaves the span is written from the next span, which
-      // holds it whole; so are the fields after it
-      if (cfg.dropouts && !cur.last && dropout_window_end(*cfg.dropouts, h_line0[f]) > plane) break;
-      FieldRecord r;
-      r.first = h_first[f] != 0;
-      r.file_loc = (uint64_t)std::max(0.0, std::floor(abs0));
+    if (next_global < pos) return set_error(VHSDECODE_HIP_E_INTERNAL, "the field sequence fell behind the span");
+    int64_t begin = next_global - pos;
+    int64_t shift = pos - prev_pos;
+    prev_pos = pos;
+    // calls on this span until the sequence's next read window leaves it
+    for (int stall = 0;;) {
+      rc = vhsdecode_hip_decode_vhs_fields_device(ctx.c, sp.video_lowpass, sp.video, plane, begin, fresh ? 1 : 0,
+                                                  shift, &sy, &fp, max_fields, kRounds,
+                                                  static_cast<uint16_t*>(d_fields.p),
+                                                  static_cast<vhsdecode_hip_vhs_field_v1*>(d_records.p), d_count,
+                                                  d_next, st.s);
+      if (rc) return rc;
+      fresh = false;
+      shift = 0;
+      call++;
       if (cfg.dropouts) {
-        const int32_t n = hd[f];
-        if (n < 0 || n > cap)
-          return set_error(VHSDECODE_HIP_E_CAPACITY, "a field has more dropout rows than the dropout capacity");
-        const int32_t* fl = hd + max_fields + (size_t)f * cap;
-        r.line.assign(fl, fl + n);
-        r.start.assign(fl + rows, fl + rows + n);
-        r.end.assign(fl + 2 * rows, fl + 2 * rows + n);
-        drop_rows += (uint32_t)n;
+        const double* span_locs = nullptr;
+        int32_t loc_stride = 0;
+        rc = vhsdecode_hip_vhs_field_lines_device(ctx.c, 3, &span_locs, &loc_stride);
+        if (rc) return rc;
+        rc = vhsdecode_hip_detect_dropouts_device(
+            ctx.c, sp.envelope, plane, span_locs, loc_stride,
+            static_cast<const vhsdecode_hip_vhs_field_v1*>(d_records.p), d_count, max_fields, cfg.dropouts, dd,
+            dd + max_fields, dd + max_fields + rows, dd + max_fields + 2 * rows, nullptr, nullptr, nullptr, st.s);
+        if (rc) return rc;
+        RUN_HIP(hipMemcpyAsync(h_drop.p, d_drop.p, drop_bytes, hipMemcpyDeviceToHost, st.s));
       }
-      to_write.push_back(f);
-      written.push_back(std::move(r));
-      last_abs = abs0;
-      if (cfg.maximum_output_fields && written.size() >= cfg.maximum_output_fields) {
-        done = true;
-        break;
+      RUN_HIP(hipMemcpyAsync(h_meta.p, d_meta.p, meta_bytes, hipMemcpyDeviceToHost, st.s));
+      RUN_HIP(hipMemcpyAsync(h_records.p, d_records.p, rec_bytes, hipMemcpyDeviceToHost, st.s));
+      RUN_HIP(hipStreamSynchronize(st.s));
+      const int32_t count = std::min(*h_count, max_fields);
+      // staging slot `call & 1` was last written two calls ago; that job has finished
+      const int slot = call & 1;
+      if (count > 0)
+        RUN_HIP(hipMemcpyAsync(h_fields[slot].p, d_fields.p, sizeof(uint16_t) * per_field * (size_t)count,
+                               hipMemcpyDeviceToHost, st.s));
+      RUN_HIP(hipStreamSynchronize(st.s));
+      std::vector<int32_t> to_write;
+      for (int32_t f = 0; f < count && !done; f++) {
+        if (recs[f].status != 0) continue;  // a failed field: the reference writes nothing for it
+        FieldRecord r;
+        r.first = recs[f].is_first_field != 0;
+        r.file_loc = (uint64_t)(pos + recs[f].window_start);
+        if (cfg.dropouts) {
+          const int32_t n = hd[f];
+          if (n < 0 || n > cap)
+            return set_error(VHSDECODE_HIP_E_CAPACITY, "a field has more dropout rows than the dropout capacity");
+          const int32_t* fl = hd + max_fields + (size_t)f * cap;
+          r.line.assign(fl, fl + n);
+          r.start.assign(fl + rows, fl + rows + n);
+          r.end.assign(fl + 2 * rows, fl + 2 * rows + n);
+          drop_rows += (uint32_t)n;
+        }
+        to_write.push_back(f);
+        written.push_back(std::move(r));
+        if (cfg.maximum_output_fields && written.size() >= cfg.maximum_output_fields) done = true;
       }
+      if (!finish_writes()) return set_error(VHSDECODE_HIP_E_IO, "cannot write " + tbc_path);
+      if (!to_write.empty()) write_job = std::async(std::launch::async, write_fields, slot, std::move(to_write));
+      const int64_t next_begin = *h_next;
+      next_global = pos + next_begin;
+      if (done) break;
+      if (count >= max_fields) {
+        begin = next_begin;
+        continue;
+      }
+      const vhsdecode_hip_vhs_field_v1& stop = recs[count];
+      if (stop.status == 5)
+        return set_error(VHSDECODE_HIP_E_CAPACITY, "a read window holds more sync edges or pulses than the device lists");
+      const bool fits = stop.window_start + stop.window_length <= plane;
+      if (stop.status == 6 && !fits) break;  // the next window is the next span's
+      if (count == 0 && next_begin == begin && ++stall > 2)
+        return set_error(VHSDECODE_HIP_E_INTERNAL, "the field sequence made no progress");
+      begin = next_begin;
     }
-    if (!finish_writes()) return set_error(VHSDECODE_HIP_E_IO, "cannot write " + tbc_path);
-    if (!to_write.empty()) write_job = std::async(std::launch::async, write_fields, k & 1, std::move(to_write));
     if (cur.last || done) break;
+    if (poll_cancel()) break;
     cur = next.get();
   }
   if (!finish_writes()) return set_error(VHSDECODE_HIP_E_IO, "cannot write " + tbc_path);
+  if (poll_cancel()) return set_error(VHSDECODE_HIP_E_CANCELLED, "the run was cancelled");
   const auto t1 = std::chrono::steady_clock::now();
   if (std::fflush(tbc.f) != 0) return set_error(VHSDECODE_HIP_E_IO, "cannot write " + tbc_path);
-  const double out_rate = fp.output_line_length * (fp.sample_rate_mhz * 1e6 / lz.line_length);
+  const double out_rate = sy.output_rate_mhz * 1e6;
   if (!write_json(json_path, fp, out_rate, written)) return set_error(VHSDECODE_HIP_E_IO, "cannot write " + json_path);
   res.fields_written = (uint32_t)written.size();
   res.output_line_length = (uint32_t)fp.output_line_length;
@@ -416,13 +442,12 @@ extern "C" int32_t vhsdecode_hip_run(const vhsdecode_hip_run_config_v1* config, 
       result->struct_size != sizeof(vhsdecode_hip_run_result_v1))
     return set_error(VHSDECODE_HIP_E_STRUCT_SIZE, "run config or result struct_size mismatch");
   const vhsdecode_hip_serve_format_v1* fm = config->format;
-  if (!fm || fm->struct_size != sizeof(vhsdecode_hip_serve_format_v1) || !fm->filters || !fm->field ||
-      !fm->locator)
-    return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "run format with filters, field and locator params required");
+  if (!fm || fm->struct_size != sizeof(vhsdecode_hip_serve_format_v1) || !fm->filters || !fm->field || !fm->sync)
+    return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "run format with filters, field and sync params required");
   if (fm->field->struct_size != sizeof(vhsdecode_hip_field_params_v1) ||
-      fm->locator->struct_size != sizeof(vhsdecode_hip_locator_v1) ||
+      fm->sync->struct_size != sizeof(vhsdecode_hip_vhs_sync_v1) ||
       (config->dropouts && config->dropouts->struct_size != sizeof(vhsdecode_hip_dropout_params_v1)))
-    return set_error(VHSDECODE_HIP_E_STRUCT_SIZE, "field, locator or dropout params struct_size mismatch");
+    return set_error(VHSDECODE_HIP_E_STRUCT_SIZE, "field, sync or dropout params struct_size mismatch");
   if (fm->filters->laserdisc)
     return set_error(VHSDECODE_HIP_E_UNSUPPORTED, "the whole run decodes tape formats; LaserDisc is block and span only");
   if (!config->read_callback || !config->output_base_utf8 || !config->output_base_utf8[0] ||

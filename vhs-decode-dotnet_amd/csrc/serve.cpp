@@ -8,6 +8,14 @@
 // allows, and reused.  Windows in different slots overlap on the device; a window's
 // kernels and inputs do not depend on the other windows in flight, so its result is
 // the same alone and in the mix.
+//
+// A window decodes as the reference's preview windows do
+// (src/VHSDecode.Preview/DecodePreviewSegmentProvider.cs:300-360): a fresh field sequence
+// from 0.12 s before the requested sample, whose fields then pass the frame rule of
+// PreviewFrameAssembler.Accept (src/VHSDecode.Preview/PreviewFrameAssembler.cs:55-95):
+// a second field completes the frame of the first field before it, and the frame is kept
+// when the later of the two fields' start samples plus one field duration reaches the
+// requested sample.  The kept fields, in order, are the window's result.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,6 +40,8 @@ using Clock = std::chrono::steady_clock;
 using vhsd_host::set_error;
 
 constexpr int kBlock = 32768;
+constexpr int kRounds = 4;               // sequence rounds per window
+constexpr double kPrerollSeconds = 0.12;  // DecodePreviewSegmentProvider.PrerollSeconds
 
 double ms_between(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
@@ -54,8 +64,11 @@ struct Format {
   vhsdecode_hip_subdeemph_v1 sd{};
   int cut = 0, cut_end = 0, stride = 0;
   vhsdecode_hip_field_params_v1 field{};
-  vhsdecode_hip_locator_v1 locator{};
-  size_t per_field = 0;  // output samples per field
+  vhsdecode_hip_vhs_sync_v1 sync{};
+  int64_t preroll = 0;     // samples decoded before the requested first sample
+  int64_t half_frame = 0;  // samples of one field (PreviewFrameAssembler._halfFrameSamples)
+  int preroll_fields = 0;  // fields the preroll spans, rounded up
+  size_t per_field = 0;    // output samples per field
 };
 
 struct Source {
@@ -82,9 +95,11 @@ struct Slot {
   int16_t* rf = nullptr;
   float* planes = nullptr;
   unsigned* flags = nullptr;
-  uint16_t* fields = nullptr;
-  size_t fields_cap = 0;  // uint16 entries
-  int32_t* meta = nullptr;
+  uint16_t* fields = nullptr;  // the sequence's fields, by record
+  uint16_t* kept = nullptr;    // the window's fields
+  size_t fields_cap = 0;       // uint16 entries of each
+  vhsdecode_hip_vhs_field_v1* records = nullptr;
+  int32_t* meta = nullptr;     // record count, next begin (int64 at 8), kept count (16), kept records (32)
   int32_t* meta_host = nullptr;  // pinned
   uint16_t* stage = nullptr;     // pinned staging for pageable outputs
   size_t stage_cap = 0;
@@ -100,6 +115,8 @@ struct vhsdecode_hip_serve {
   size_t used_bytes = 0;    // device bytes of the slots and their contexts
   size_t slot_bytes() const;
   size_t context_bytes() const;
+  int records_cap() const { return cfg.max_fields + 16; }
+  size_t meta_bytes() const { return 32 + 4 * (size_t)records_cap(); }
   std::mutex mu;
   std::condition_variable cv;       // worker wake-ups
   std::condition_variable done_cv;  // waiters
@@ -128,21 +145,64 @@ namespace {
     if (e_ != hipSuccess) return set_error(VHSDECODE_HIP_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// PreviewFrameAssembler.Accept over the window's sequence records: field indices of the
+// kept frames, in order (one thread; a window holds a few hundred records at most).
+__global__ void window_select_kernel(const vhsdecode_hip_vhs_field_v1* rec, const int32_t* count, long long origin,
+                                     long long target, long long half_frame, int max_out, int32_t* kept_count,
+                                     int32_t* kept) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int n = *count;
+  int out = 0, first = -1;
+  long long first_start = 0;
+  for (int k = 0; k < n && out + 2 <= max_out; k++) {
+    if (rec[k].status != 0) continue;  // a failed field renders nothing
+    const long long start = origin + rec[k].window_start;
+    if (rec[k].is_first_field) {
+      first = k;
+      first_start = start;
+      continue;
+    }
+    if (first < 0) continue;
+    const long long mid = (first_start > start ? first_start : start) + half_frame;
+    const int a = first;
+    first = -1;
+    if (mid < target) continue;
+    kept[out++] = a;
+    kept[out++] = k;
+  }
+  *kept_count = out;
+}
+
+// The kept fields, gathered in order: one workgroup row per kept field.
+__global__ void window_gather_kernel(const uint16_t* fields, size_t per_field, const int32_t* kept_count,
+                                     const int32_t* kept, uint16_t* out) {
+  const int f = blockIdx.y;
+  if (f >= *kept_count) return;
+  const uint16_t* src = fields + (size_t)kept[f] * per_field;
+  uint16_t* dst = out + (size_t)f * per_field;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < per_field; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace
 
-// Device bytes of one slot: RF, the four planes and the field output.
+// Device bytes of one slot: RF, the four planes, the sequence's and the window's fields
+// and the records.
 size_t vhsdecode_hip_serve::slot_bytes() const {
   const int64_t blocks = (cfg.max_window_samples - kBlock) / min_stride + 1;
   const int64_t span = blocks * min_stride;
-  return (size_t)cfg.max_window_samples * 2 + (size_t)span * 16 + (size_t)cfg.max_fields * 1135 * 313 * 2;
+  return (size_t)cfg.max_window_samples * 2 + (size_t)span * 16 + (size_t)records_cap() * 1135 * 313 * 2 * 2 +
+         (size_t)records_cap() * sizeof(vhsdecode_hip_vhs_field_v1) + meta_bytes();
 }
 
 // Device bytes of one demod context at the slot's size: per-block spectrum stash and
-// repair scratch, per-sample pulse edges, filter and sinc tables.
+// repair scratch, per-sample pulse edges, filter and sinc tables, and the field
+// sequence's per-field lists (sync edges and pulses of each read window, line tables).
 size_t vhsdecode_hip_serve::context_bytes() const {
   const int64_t blocks = (cfg.max_window_samples - kBlock) / min_stride + 1;
   const int64_t span = blocks * min_stride;
-  return (size_t)blocks * (8 * (2 * 16384 + 64) + 8 * 32768 + 4 * 65536) + (size_t)span * 10 + ((size_t)8 << 20);
+  return (size_t)blocks * (8 * (2 * 16384 + 64) + 8 * 32768 + 4 * 65536) + (size_t)span * 10 + ((size_t)8 << 20) +
+         (size_t)records_cap() * (5 * 65536 + 4 * 4 * 4096 + 24 * 1024 + 32 * 8 * 330);
 }
 
 int32_t vhsdecode_hip_serve::make_slot(Slot** out) {
@@ -154,8 +214,10 @@ int32_t vhsdecode_hip_serve::make_slot(Slot** out) {
   if (e == hipSuccess) e = hipMalloc((void**)&s->rf, sizeof(int16_t) * (size_t)cfg.max_window_samples);
   if (e == hipSuccess) e = hipMalloc((void**)&s->planes, sizeof(float) * 4 * (size_t)span);
   if (e == hipSuccess) e = hipMalloc((void**)&s->flags, sizeof(unsigned) * (size_t)s->max_blocks);
-  if (e == hipSuccess) e = hipMalloc((void**)&s->meta, 64);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&s->meta_host, 64, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->meta, meta_bytes());
+  if (e == hipSuccess) e = hipHostMalloc((void**)&s->meta_host, meta_bytes(), hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipMalloc((void**)&s->records, sizeof(vhsdecode_hip_vhs_field_v1) * (size_t)records_cap());
   if (e != hipSuccess) {
     release_slot(s.get());
     return set_error(VHSDECODE_HIP_E_OUT_OF_MEMORY, std::string("slot allocation failed: ") + hipGetErrorString(e));
@@ -174,6 +236,8 @@ void vhsdecode_hip_serve::release_slot(Slot* s) {
   if (s->planes) (void)hipFree(s->planes);
   if (s->flags) (void)hipFree(s->flags);
   if (s->fields) (void)hipFree(s->fields);
+  if (s->kept) (void)hipFree(s->kept);
+  if (s->records) (void)hipFree(s->records);
   if (s->meta) (void)hipFree(s->meta);
   if (s->meta_host) (void)hipHostFree(s->meta_host);
   if (s->stage) (void)hipHostFree(s->stage);
@@ -213,26 +277,30 @@ int32_t vhsdecode_hip_serve::launch(Slot* s, const std::shared_ptr<Request>& r) 
   } else {
     ctx = it->second;
   }
-  const size_t need = (size_t)w.max_fields * f.per_field;
+  const size_t need = (size_t)records_cap() * f.per_field;
   if (need > s->fields_cap) {
     if (s->fields) (void)hipFree(s->fields);
-    s->fields = nullptr;
+    if (s->kept) (void)hipFree(s->kept);
+    s->fields = s->kept = nullptr;
     s->fields_cap = 0;
     SV_HIP(hipMalloc((void**)&s->fields, sizeof(uint16_t) * need));
+    SV_HIP(hipMalloc((void**)&s->kept, sizeof(uint16_t) * need));
     s->fields_cap = need;
   }
-  // RF window: one copy, or one per wrap of a looped recording
-  int64_t pos = w.first_sample, done = 0;
-  while (done < w.sample_count) {
+  // RF from the preroll start: one copy, or one per wrap of a looped recording
+  const int64_t start = std::max<int64_t>(0, w.first_sample - f.preroll) & ~(int64_t)1;
+  const int64_t count = w.first_sample - start + w.sample_count;
+  int64_t pos = start, done = 0;
+  while (done < count) {
     const int64_t p = src.loop ? pos % src.length : pos;
-    const int64_t n = std::min<int64_t>(w.sample_count - done, src.length - p);
+    const int64_t n = std::min<int64_t>(count - done, src.length - p);
     SV_HIP(hipMemcpyAsync(s->rf + done, src.samples + p, sizeof(int16_t) * (size_t)n, hipMemcpyHostToDevice,
                           s->stream));
     done += n;
     pos += n;
   }
   const int stride = f.stride;
-  const int32_t nb = (int32_t)((w.sample_count - kBlock) / stride + 1);
+  const int32_t nb = (int32_t)((count - kBlock) / stride + 1);
   if (nb > s->max_blocks)
     return set_error(VHSDECODE_HIP_E_CAPACITY, "window needs more blocks than the slot was sized for");
   const int64_t n = (int64_t)nb * stride;
@@ -245,22 +313,35 @@ int32_t vhsdecode_hip_serve::launch(Slot* s, const std::shared_ptr<Request>& r) 
   sp.block_flags = s->flags;
   int32_t rc = vhsdecode_hip_demod_span_device(ctx, s->rf, 0, nb, &sp, s->stream);
   if (rc) return rc;
-  rc = vhsdecode_hip_decode_span_fields_device(ctx, sp.video_lowpass, sp.video, n, &f.field, &f.locator,
-                                               w.max_fields, s->fields, s->meta, nullptr, nullptr, s->stream);
+  // the sequence: DecodePreviewSegmentProvider's field budget (output fields, preroll and 8)
+  const int32_t records = std::min(records_cap(), w.max_fields + f.preroll_fields + 8);
+  unsigned char* meta = reinterpret_cast<unsigned char*>(s->meta);
+  rc = vhsdecode_hip_decode_vhs_fields_device(ctx, sp.video_lowpass, sp.video, n, 0, 1, 0, &f.sync, &f.field,
+                                              records, kRounds, s->fields, s->records, s->meta,
+                                              reinterpret_cast<int64_t*>(meta + 8), s->stream);
   if (rc) return rc;
-  SV_HIP(hipMemcpyAsync(s->meta_host, s->meta, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+  // span sample i is RF sample start + cut + i
+  hipLaunchKernelGGL(window_select_kernel, dim3(1), dim3(64), 0, s->stream, s->records, s->meta,
+                     (long long)(start + f.cut), (long long)w.first_sample, (long long)f.half_frame, w.max_fields,
+                     reinterpret_cast<int32_t*>(meta + 16), reinterpret_cast<int32_t*>(meta + 32));
+  hipLaunchKernelGGL(window_gather_kernel, dim3(64, w.max_fields), dim3(256), 0, s->stream, s->fields, f.per_field,
+                     reinterpret_cast<const int32_t*>(meta + 16), reinterpret_cast<const int32_t*>(meta + 32),
+                     s->kept);
+  SV_HIP(hipGetLastError());
+  SV_HIP(hipMemcpyAsync(s->meta_host, s->meta, 32, hipMemcpyDeviceToHost, s->stream));
+  const size_t out_n = (size_t)w.max_fields * f.per_field;
   uint16_t* dst = w.fields_out;
   if (!r->pinned_out) {
-    if (need > s->stage_cap) {
+    if (out_n > s->stage_cap) {
       if (s->stage) (void)hipHostFree(s->stage);
       s->stage = nullptr;
       s->stage_cap = 0;
-      SV_HIP(hipHostMalloc((void**)&s->stage, sizeof(uint16_t) * need, hipHostMallocDefault));
-      s->stage_cap = need;
+      SV_HIP(hipHostMalloc((void**)&s->stage, sizeof(uint16_t) * out_n, hipHostMallocDefault));
+      s->stage_cap = out_n;
     }
     dst = s->stage;
   }
-  SV_HIP(hipMemcpyAsync(dst, s->fields, sizeof(uint16_t) * need, hipMemcpyDeviceToHost, s->stream));
+  SV_HIP(hipMemcpyAsync(dst, s->kept, sizeof(uint16_t) * out_n, hipMemcpyDeviceToHost, s->stream));
   SV_HIP(hipEventRecord(s->done, s->stream));
   return VHSDECODE_HIP_OK;
 }
@@ -270,7 +351,7 @@ void vhsdecode_hip_serve::finish(Slot* s, int32_t status) {
   s->req.reset();
   r->status = status;
   if (status == 0) {
-    r->field_count = s->meta_host[0];
+    r->field_count = s->meta_host[4];
     if (!r->pinned_out) {
       const Format& f = *formats[sources[r->w.stream_id].format];
       std::memcpy(r->w.fields_out, s->stage, sizeof(uint16_t) * (size_t)r->field_count * f.per_field);
@@ -383,12 +464,12 @@ int32_t vhsdecode_hip_serve_create(const vhsdecode_hip_serve_config_v1* cfg, vhs
 int32_t vhsdecode_hip_serve_add_format(vhsdecode_hip_serve* s, const vhsdecode_hip_serve_format_v1* fm,
                                        int32_t* format_id) {
   try {
-    if (!s || !fm || !format_id || !fm->filters || !fm->field || !fm->locator)
+    if (!s || !fm || !format_id || !fm->filters || !fm->field || !fm->sync)
       return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid format");
     if (fm->struct_size != sizeof(vhsdecode_hip_serve_format_v1) ||
         fm->filters->struct_size != sizeof(vhsdecode_hip_filters_v1) ||
         fm->field->struct_size != sizeof(vhsdecode_hip_field_params_v1) ||
-        fm->locator->struct_size != sizeof(vhsdecode_hip_locator_v1))
+        fm->sync->struct_size != sizeof(vhsdecode_hip_vhs_sync_v1))
       return set_error(VHSDECODE_HIP_E_STRUCT_SIZE, "format struct_size mismatch");
     const vhsdecode_hip_filters_v1& src = *fm->filters;
     if (src.laserdisc)
@@ -439,8 +520,14 @@ int32_t vhsdecode_hip_serve_add_format(vhsdecode_hip_serve* s, const vhsdecode_h
     f->cut_end = fm->block_cut_end;
     f->stride = stride;
     f->field = *fm->field;
-    f->locator = *fm->locator;
+    f->sync = *fm->sync;
     f->per_field = (size_t)f->field.field_lines * f->field.output_line_length;
+    const double fs = f->field.sample_rate_mhz * 1e6, field_rate = f->sync.frames_per_second * 2.0;
+    if (!(fs > 0.0) || !(field_rate > 0.0))
+      return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "format sample rate and frame rate must be positive");
+    f->preroll = (int64_t)std::llround(kPrerollSeconds * fs);
+    f->half_frame = std::max<int64_t>(1, (int64_t)std::llround(fs / field_rate));
+    f->preroll_fields = (int)std::ceil(kPrerollSeconds * field_rate);
     std::lock_guard<std::mutex> lock(s->mu);
     const int min_stride = std::min(s->min_stride, stride);
     if (s->slots.empty()) {
@@ -486,10 +573,11 @@ int32_t vhsdecode_hip_serve_submit(vhsdecode_hip_serve* s, const vhsdecode_hip_w
     if (w->stream_id < 0 || w->stream_id >= (int32_t)s->sources.size())
       return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "unknown stream");
     const Source& src = s->sources[w->stream_id];
-    if (w->sample_count < kBlock || w->sample_count > s->cfg.max_window_samples || w->first_sample < 0 ||
-        (w->first_sample & 1) || w->max_fields <= 0 || w->max_fields > s->cfg.max_fields ||
+    const int64_t preroll = std::min<int64_t>(w->first_sample, s->formats[src.format]->preroll);
+    if (w->sample_count < kBlock || w->sample_count + preroll + 1 > s->cfg.max_window_samples ||
+        w->first_sample < 0 || (w->first_sample & 1) || w->max_fields <= 0 || w->max_fields > s->cfg.max_fields ||
         (!src.loop && w->first_sample + w->sample_count > src.length))
-      return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "window outside the stream or the server's limits");
+      return set_error(VHSDECODE_HIP_E_INVALID_ARGUMENT, "window (with its preroll) outside the stream or the server's limits");
     *ticket = s->next_ticket++;
     s->tickets[*ticket] = r;
     s->queue.push_back(r);

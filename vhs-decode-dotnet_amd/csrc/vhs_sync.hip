@@ -107,14 +107,19 @@ __device__ long long wg_sum_ll(long long v, long long* lred) {
 
 // ------------------------------------------------------------------ pulses
 // PulseDetection.FindPulses (Dsp/PulseDetection.cs:45-222) over the plane: a two-state
-// machine over `value <= high` (NaN keeps the state); runs of kRun samples per thread,
-// edges compacted in order into epos/ekind, then each end edge paired with the edge
-// before it.  Pulses starting at 0 are dropped.  Returns the pulse count (every thread);
-// stores the first `cap` into starts/lens; *vsync receives the number of pulses longer
-// than vs_min.  *edges_overflow is set when the edge list was cut at ecap.
+// machine over `value <= high` (NaN keeps the state), in tiles of kRun samples per
+// thread.  The transitions alternate (pulse start, pulse end), so a pulse is an end
+// transition paired with the transition before it: within a thread's run that is a local
+// transition, and before the run's first one it is the last transition of the earlier
+// threads (a workgroup max-scan of each run's last transition) or of the earlier tiles.
+// Pulses starting at sample 0 (or with no start transition) are dropped.  Pulses are
+// compacted in order; returns the pulse count (every thread) and stores the first `cap`
+// into starts/lens; *vsync receives the number of pulses longer than vs_min.
 struct PulseScratch {
   int red[kT / 64 + 4];
+  int wlast[kT / 64];
   int offset;
+  int carry;  // position of the last transition before this tile, -1 for none
 };
 
 __device__ __forceinline__ int state_before(const Plane& x, long long i, double high) {
@@ -125,12 +130,33 @@ __device__ __forceinline__ int state_before(const Plane& x, long long i, double 
   return 0;
 }
 
-__device__ int wg_find_pulses(const Plane& x, double high, int min_len, int max_len, double vs_min, int* starts,
-                              int* lens, int cap, int* epos, unsigned char* ekind, int ecap, PulseScratch* ps,
-                              int* vsync, int* edges_overflow) {
-  const int tid = threadIdx.x;
-  if (tid == 0) ps->offset = 0;
+// Largest v over the workgroup's threads before this one (exclusive), -1 for none.
+__device__ int wg_exclusive_max(int v, PulseScratch* ps) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o);
+    if (lane >= o) incl = incl > u ? incl : u;
+  }
+  if (lane == 63) ps->wlast[wave] = incl;
   __syncthreads();
+  int before = -1;
+  for (int w = 0; w < wave; w++) before = before > ps->wlast[w] ? before : ps->wlast[w];
+  const int up = __shfl_up(incl, 1);
+  const int excl = lane > 0 ? (before > up ? before : up) : before;
+  return excl;
+}
+
+__device__ int wg_find_pulses(const Plane& x, double high, int min_len, int max_len, double vs_min, int* starts,
+                              int* lens, int cap, PulseScratch* ps, int* vsync) {
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    ps->offset = 0;
+    ps->carry = -1;
+  }
+  __syncthreads();
+  int vs_count = 0;
   for (long long base = 0; base < x.n; base += kTile) {
     const long long i0 = base + (long long)tid * kRun;
     unsigned codes = 0;
@@ -149,48 +175,67 @@ __device__ int wg_find_pulses(const Plane& x, double high, int min_len, int max_
         }
       }
     }
-    const int c = __popc(codes);
-    int tot;
-    int idx = wg_exclusive(c, ps->red, &tot) + ps->offset;
+    // this run's last transition, and the last one before the run
+    const int mine = codes ? (int)(i0 + ((31 - __clz(codes)) >> 1)) : -1;
+    int prev = wg_exclusive_max(mine, ps);
+    if (prev < 0) prev = ps->carry;
+    // pair each end transition with the transition before it
+    int cnt = 0;
     unsigned rest = codes;
+    int pv = prev;
     while (rest) {
       const int k = (__ffs(rest) - 1) >> 1;
-      if (idx < ecap) {
-        epos[idx] = (int)(i0 + k);
-        ekind[idx] = (unsigned char)((rest >> (2 * k)) & 3u);
+      const int pos = (int)(i0 + k);
+      if (((rest >> (2 * k)) & 3u) == 2u && pv > 0) {
+        const int len = pos - pv;
+        if (in_range((double)len, min_len, max_len)) {
+          cnt++;
+          if ((double)len > vs_min) vs_count++;
+        }
       }
-      idx++;
+      pv = pos;
       rest &= ~(3u << (2 * k));
     }
-    __syncthreads();
-    if (tid == 0) ps->offset += tot;
-    __syncthreads();
-  }
-  const int total_edges = ps->offset;
-  if (tid == 0 && edges_overflow && total_edges > ecap) *edges_overflow = 1;
-  const int ne = total_edges < ecap ? total_edges : ecap;
-  __syncthreads();
-  if (tid == 0) ps->offset = 0;
-  __syncthreads();
-  int vs_count = 0;
-  for (int b0 = 0; b0 < ne; b0 += kT) {
-    const int k = b0 + tid;
-    bool ok = false;
-    int st = 0, len = 0;
-    if (k < ne && ekind[k] == 2) {
-      st = k > 0 ? epos[k - 1] : 0;
-      len = epos[k] - st;
-      ok = in_range((double)len, min_len, max_len) && st != 0;
-    }
-    if (ok && (double)len > vs_min) vs_count++;
     int tot;
-    const int idx = wg_exclusive(ok ? 1 : 0, ps->red, &tot) + ps->offset;
-    if (ok && idx < cap) {
-      starts[idx] = st;
-      lens[idx] = len;
+    int idx = wg_exclusive(cnt, ps->red, &tot) + ps->offset;
+    rest = codes;
+    pv = prev;
+    while (rest && cnt) {
+      const int k = (__ffs(rest) - 1) >> 1;
+      const int pos = (int)(i0 + k);
+      if (((rest >> (2 * k)) & 3u) == 2u && pv > 0) {
+        const int len = pos - pv;
+        if (in_range((double)len, min_len, max_len)) {
+          if (idx < cap) {
+            starts[idx] = pv;
+            lens[idx] = len;
+          }
+          idx++;
+        }
+      }
+      pv = pos;
+      rest &= ~(3u << (2 * k));
+    }
+    // the tile's last transition carries into the next tile
+    int last_all;
+    {
+      const int lane = tid & 63;
+      int m = mine;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const int u = __shfl_xor(m, o);
+        m = m > u ? m : u;
+      }
+      if (lane == 0) ps->wlast[tid >> 6] = m;
+      __syncthreads();
+      last_all = -1;
+      for (int w = 0; w < kT / 64; w++) last_all = last_all > ps->wlast[w] ? last_all : ps->wlast[w];
     }
     __syncthreads();
-    if (tid == 0) ps->offset += tot;
+    if (tid == 0) {
+      ps->offset += tot;
+      if (last_all >= 0) ps->carry = last_all;
+    }
     __syncthreads();
   }
   const int count = ps->offset;
@@ -480,16 +525,14 @@ __device__ RefineOut wg_refine_from_pulses(const Plane& x, const int* starts, co
 
 // LevelDetection.RefineSerrationLevels (:98-170): pulses at the initial levels' midpoint
 __device__ RefineOut wg_refine_levels(const Plane& x, double s0, double b0, const SyncCfg& c, double ref_sync,
-                                      int* pstart, int* plen, int pcap, int* epos, unsigned char* ekind, int ecap,
-                                      RefineScratch* r, int* overflow) {
+                                      int* pstart, int* plen, int pcap, RefineScratch* r, int* overflow) {
 #pragma clang fp contract(off)
   if (x.n <= 0) return RefineOut{0, 0.0, 0.0, kMissing, 0, 0.0};
   const double thr = (s0 + b0) / 2.0;
   const double eql = (double)c.ser_eq_len, ll = (double)c.ser_line_len;
   const int mn = (int)ceil(eql / 8.0) > 0 ? (int)ceil(eql / 8.0) : 0;
   const int mx = (int)floor(ll * 5.0) > 1 ? (int)floor(ll * 5.0) : 1;
-  const int np = wg_find_pulses(x, thr, mn, mx, 1e300, pstart, plen, pcap, epos, ekind, ecap, &r->ps, nullptr,
-                                overflow);
+  const int np = wg_find_pulses(x, thr, mn, mx, 1e300, pstart, plen, pcap, &r->ps, nullptr);
   if (np > pcap && threadIdx.x == 0) *overflow = 1;
   return wg_refine_from_pulses(x, pstart, plen, np < pcap ? np : pcap, c, ref_sync, r);
 }
@@ -506,8 +549,6 @@ __global__ void __launch_bounds__(kT) level_search_kernel(SyncCfg c, SyncWork w)
   const Plane x{w.sync_ref + w.win_start[f], w.win_len[f], c.bias, 0.0};
   int* pst = w.pulse_start + (long long)f * c.pulse_cap;
   int* pln = w.pulse_len + (long long)f * c.pulse_cap;
-  int* epos = w.edge_pos + (long long)f * c.edge_cap;
-  unsigned char* ek = w.edge_kind + (long long)f * c.edge_cap;
   int* ovf = w.overflow + f;
   LevelSearch out{};
   // window minimum (Math.Min keeps a NaN)
@@ -533,7 +574,7 @@ __global__ void __launch_bounds__(kT) level_search_kernel(SyncCfg c, SyncWork w)
   for (int retry = 0; retry < 30; retry++) {
     const double thr = (min_sync + blank_level) / 2.0;
     int vc = 0;
-    const int np = wg_find_pulses(x, thr, pmn, pmx, min_vsync, pst, pln, 0, epos, ek, c.edge_cap, &r.ps, &vc, ovf);
+    const int np = wg_find_pulses(x, thr, pmn, pmx, min_vsync, pst, pln, 0, &r.ps, &vc);
     list_thr = thr;
     bool stop = false;
     if (np > 200) {
@@ -561,8 +602,7 @@ __global__ void __launch_bounds__(kT) level_search_kernel(SyncCfg c, SyncWork w)
     const double scaled = c.hz_ire * next_ire;
     min_sync = blank_level + scaled;
   }
-  const int np = wg_find_pulses(x, list_thr, pmn, pmx, 1e300, pst, pln, c.pulse_cap, epos, ek, c.edge_cap, &r.ps,
-                                nullptr, ovf);
+  const int np = wg_find_pulses(x, list_thr, pmn, pmx, 1e300, pst, pln, c.pulse_cap, &r.ps, nullptr);
   if (np > c.pulse_cap && threadIdx.x == 0) *ovf = 1;
   const RefineOut first = wg_refine_from_pulses(x, pst, pln, np < c.pulse_cap ? np : c.pulse_cap, c, ref_sync, &r);
   out.first_ok = first.ok;
@@ -572,8 +612,7 @@ __global__ void __launch_bounds__(kT) level_search_kernel(SyncCfg c, SyncWork w)
   out.first_has_measured = first.has_measured;
   out.first_measured = first.measured;
   if (first.ok) {
-    const RefineOut sec = wg_refine_levels(x, first.sync, first.blank, c, ref_sync, pst, pln, c.pulse_cap, epos, ek,
-                                           c.edge_cap, &r, ovf);
+    const RefineOut sec = wg_refine_levels(x, first.sync, first.blank, c, ref_sync, pst, pln, c.pulse_cap, &r, ovf);
     out.ref_ok = sec.ok;
     out.ref_sync = sec.sync;
     out.ref_blank = sec.blank;
@@ -740,8 +779,7 @@ __global__ void __launch_bounds__(kT) level_chain_kernel(SyncCfg c, SyncWork w) 
       } else {
         int* pst = w.pulse_start + (long long)f * c.pulse_cap;
         int* pln = w.pulse_len + (long long)f * c.pulse_cap;
-        rr = wg_refine_levels(x, fs_, fb, c, ref_sync, pst, pln, c.pulse_cap, w.edge_pos + (long long)f * c.edge_cap,
-                              w.edge_kind + (long long)f * c.edge_cap, c.edge_cap, &r, w.overflow + f);
+        rr = wg_refine_levels(x, fs_, fb, c, ref_sync, pst, pln, c.pulse_cap, &r, w.overflow + f);
       }
       if (rr.has_measured) ch.fs_s.push(rr.measured);
       if (rr.ok) {
@@ -854,9 +892,8 @@ __global__ void __launch_bounds__(kT) field_pulses_kernel(SyncCfg c, SyncWork w)
   const Plane x{w.sync_ref + w.win_start[f], w.win_len[f], c.bias, lv.dc};
   int* pst = w.pulse_start + (long long)f * c.pulse_cap;
   int* pln = w.pulse_len + (long long)f * c.pulse_cap;
-  const int np = wg_find_pulses(x, lv.threshold, c.pulse_min, c.pulse_max, 1e300, pst, pln, c.pulse_cap,
-                                w.edge_pos + (long long)f * c.edge_cap, w.edge_kind + (long long)f * c.edge_cap,
-                                c.edge_cap, &ps, nullptr, w.overflow + f);
+  const int np = wg_find_pulses(x, lv.threshold, c.pulse_min, c.pulse_max, 1e300, pst, pln, c.pulse_cap, &ps,
+                                nullptr);
   if (threadIdx.x == 0) {
     if (np > c.pulse_cap) w.overflow[f] = 1;
     w.raw_count[f] = np < c.pulse_cap ? np : c.pulse_cap;
@@ -870,10 +907,17 @@ struct Timing {
   double hsync_lo, hsync_hi, eq_lo, eq_hi, vs_lo, vs_hi, median;
 };
 
+// The pulse lists of one field: in LDS up to kPulseLds pulses, else in the field's
+// global lists (a window full of noise).
 struct LogicScratch {
-  int rs[kPulseCap], rl[kPulseCap];            // raw pulses (rescue rewrites them)
-  int cs[kPulseCap], cl[kPulseCap];            // refined pulses
-  unsigned char ck[kPulseCap], co[kPulseCap];  // kind, in order
+  int* rs;            // raw pulses (rescue rewrites them)
+  int* rl;
+  int* cs;            // refined pulses
+  int* cl;
+  unsigned char* ck;  // kind
+  unsigned char* co;  // in order
+  int lrs[kPulseLds], lrl[kPulseLds], lcs[kPulseLds], lcl[kPulseLds];
+  unsigned char lck[kPulseLds], lco[kPulseLds];
   int hist[kHistBins];
   int nref;
 };
@@ -989,12 +1033,23 @@ __global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) 
   const int f = blockIdx.x;
   if (f >= *w.spec_count) return;
   const int np = w.raw_count[f];
-  const int* gst = w.pulse_start + (long long)f * c.pulse_cap;
-  const int* gln = w.pulse_len + (long long)f * c.pulse_cap;
-  for (int i = threadIdx.x; i < np; i += kT) {
-    L.rs[i] = gst[i];
-    L.rl[i] = gln[i];
+  int* gst = w.pulse_start + (long long)f * c.pulse_cap;
+  int* gln = w.pulse_len + (long long)f * c.pulse_cap;
+  const bool lds = np <= kPulseLds;
+  if (threadIdx.x == 0) {
+    unsigned char* aux = w.logic_aux + (long long)f * 6 * c.pulse_cap;
+    L.rs = lds ? L.lrs : gst;
+    L.rl = lds ? L.lrl : gln;
+    L.cs = lds ? L.lcs : w.ref_start + (long long)f * c.pulse_cap;
+    L.cl = lds ? L.lcl : reinterpret_cast<int*>(aux);
+    L.ck = lds ? L.lck : aux + 4LL * c.pulse_cap;
+    L.co = lds ? L.lco : aux + 5LL * c.pulse_cap;
   }
+  if (lds)
+    for (int i = threadIdx.x; i < np; i += kT) {
+      L.lrs[i] = gst[i];
+      L.lrl[i] = gln[i];
+    }
   for (int i = threadIdx.x; i < kHistBins; i += kT) L.hist[i] = 0;
   __syncthreads();
   // EstimateTiming (:131-166): median of the hsync-like lengths by a histogram
@@ -1046,7 +1101,7 @@ __global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) 
   int nref = 0;
   auto add = [&](int kind, int st, int ln) {
     const bool io = nref > 0 && quality(L.ck[nref - 1], L.cs[nref - 1], kind, st, nominal);
-    if (nref < kPulseCap) {
+    if (nref < c.pulse_cap) {
       L.ck[nref] = (unsigned char)kind;
       L.cs[nref] = st;
       L.cl[nref] = ln;
@@ -1186,7 +1241,7 @@ __global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) 
       continue;
     }
     for (int k = 2; k < nv; k++) {
-      if (nref < kPulseCap) {
+      if (nref < c.pulse_cap) {
         L.ck[nref] = (unsigned char)vk[k];
         L.cs[nref] = vs[k];
         L.cl[nref] = vl[k];
@@ -1374,20 +1429,43 @@ __device__ void resolve_window(const SyncCfg& c, long long begin, long long* ws,
 __global__ void field_chain_kernel(SyncCfg c, SyncWork w) {
 #pragma clang fp contract(off)
   if (threadIdx.x != 0) return;
+  SeqCtl& ctl = *w.ctl;
+  if (ctl.halted) return;
+  const int base = ctl.base;
   SyncHistoryDev h = *w.history_in;
   h.prev_first_hsync_read -= w.state_shift;
+  // the state the next call continues from: as the last field (decoded or failed) left it
+  *w.history_out = h;
+  w.level_state_out[0] = *w.level_state_in;
   const int nf = *w.spec_count;
-  long long begin = w.first_begin;
+  long long begin = ctl.begin;
   int out_count = 0;
-  int stop_status = 0;
-  for (int f = 0; f < nf; f++) {
-    FieldRecord& rec = w.records[f];
+  int stop_status = kStatusNotRead;
+  int f = 0;
+  // A field the reference fails (no sync pulses, no first hsync, too little data) is
+  // recorded with its recovery offset (BuildRecoveryException, TbcFieldDecodePipeline.cs
+  // :5936-5959; tape offsets, never a stop) and the sequence state it leaves; the call
+  // ends there, since later windows were speculated from this field decoding.
+  auto recover = [&](FieldRecord& rec, int status, double offset) {
+    rec.status = status;
+    rec.next_offset = offset;
+    const long long nb = begin + (long long)offset;
+    rec.next_begin = nb > 0 ? nb : 0;
+    begin = rec.next_begin;
+    stop_status = status;
+    *w.history_out = h;
+    w.level_state_out[0] = f + 1 < nf ? w.level_before[f + 1] : *w.level_state_after_all;
+  };
+  for (; f < nf; f++) {
+    FieldRecord& rec = w.records[base + f];
     rec = FieldRecord{};
+    rec.struct_size = sizeof(FieldRecord);
     long long ws, wl;
     resolve_window(c, begin, &ws, &wl);
     rec.begin = begin;
     rec.window_start = ws;
     rec.window_length = wl;
+    rec.next_begin = begin;
     if (ws != w.win_start[f] || wl != w.win_len[f]) {
       rec.status = kStatusMispredicted;
       stop_status = rec.status;
@@ -1410,8 +1488,8 @@ __global__ void field_chain_kernel(SyncCfg c, SyncWork w) {
       break;
     }
     if (lg.raw == 0) {
-      rec.status = h.pulse_search_initialized ? 2 : 1;
-      stop_status = rec.status;
+      const bool missing_start = h.pulse_search_initialized != 0;
+      recover(rec, missing_start ? 2 : 1, missing_start ? c.nominal * 100.0 : (c.fs_mhz * 1000000.0) / 10.0);
       break;
     }
     h.pulse_search_initialized = 1;
@@ -1481,8 +1559,7 @@ __global__ void field_chain_kernel(SyncCfg c, SyncWork w) {
       }
     }
     if (!ok) {
-      rec.status = 2;
-      stop_status = rec.status;
+      recover(rec, 2, c.nominal * 100.0);
       break;
     }
     // UpdateSyncHistory (:6119-6144)
@@ -1498,8 +1575,9 @@ __global__ void field_chain_kernel(SyncCfg c, SyncWork w) {
     const int processed = c.output_lines + 10;
     const double last_line = (((double)wl - line0) / mll) - 1.0;
     if (last_line < processed) {
-      rec.status = 3;
-      stop_status = rec.status;
+      // ThrowIfInsufficientFieldData (:5961-6008), tape
+      const double suggested = line0 - (mll * 20.0);
+      recover(rec, 3, suggested > c.nominal ? suggested : c.nominal);
       break;
     }
     // ComputeVhsNextFieldOffsetSamples (:2299-2325)
@@ -1521,15 +1599,29 @@ __global__ void field_chain_kernel(SyncCfg c, SyncWork w) {
     w.level_state_out[0] = f + 1 < nf ? w.level_before[f + 1] : *w.level_state_after_all;
     *w.history_out = h;
   }
-  if (out_count == 0) {
-    w.level_state_out[0] = *w.level_state_in;
-    SyncHistoryDev h0 = *w.history_in;
-    h0.prev_first_hsync_read -= w.state_shift;
-    *w.history_out = h0;
+  // a failed field takes a record; the stop record after the decoded fields does not
+  const bool failed = stop_status >= 1 && stop_status <= 3;
+  const int used = base + out_count + (failed ? 1 : 0);
+  long long nws, nwl;
+  resolve_window(c, begin, &nws, &nwl);
+  if (f == nf && base + f < c.max_records) {
+    // the next field's window was not speculated (it runs past the span, or no vertical
+    // sync in the span places it)
+    FieldRecord& rec = w.records[base + f];
+    rec = FieldRecord{};
+    rec.struct_size = sizeof(FieldRecord);
+    rec.status = kStatusNotRead;
+    rec.begin = begin;
+    rec.window_start = nws;
+    rec.window_length = nwl;
+    rec.next_begin = begin;
   }
-  *w.field_count = out_count;
-  *w.next_begin_out = begin;
-  *w.stop_status = stop_status;
+  ctl.begin = begin;
+  ctl.records = used;
+  ctl.decoded = out_count;
+  if (stop_status == kStatusCapacity || used >= c.max_records ||
+      (stop_status == kStatusNotRead && nws + nwl > w.span_len))
+    ctl.halted = 1;
 }
 
 // ------------------------------------------------------------------ S6: line locations
@@ -1537,13 +1629,16 @@ __global__ void field_chain_kernel(SyncCfg c, SyncWork w) {
 // FirstHSyncLine, written window-relative for every processed line.
 __global__ void __launch_bounds__(kT) field_lines_kernel(SyncCfg c, SyncWork w) {
 #pragma clang fp contract(off)
-  __shared__ int srt[kPulseCap];
+  __shared__ int lsrt[kPulseLds];
   const int f = blockIdx.x;
-  if (f >= *w.field_count) return;
-  const FieldRecord rec = w.records[f];
+  if (f >= w.ctl->decoded) return;
+  const FieldRecord rec = w.records[w.ctl->base + f];
   const int n = rec.refined_pulses;
-  const int* ss = w.ref_start + (long long)f * c.pulse_cap;
-  for (int i = threadIdx.x; i < n; i += kT) srt[i] = ss[i];
+  int* ss = w.ref_start + (long long)f * c.pulse_cap;
+  // sorted in LDS, or in place (the field chain has read the list) when it is longer
+  int* srt = n <= kPulseLds ? lsrt : ss;
+  if (n <= kPulseLds)
+    for (int i = threadIdx.x; i < n; i += kT) lsrt[i] = ss[i];
   __syncthreads();
   if (threadIdx.x != 0) return;
   for (int i = 1; i < n; i++) {  // nearly sorted: insertion sort (OrderBy is stable; equal keys are equal values)
@@ -1599,6 +1694,17 @@ __global__ void __launch_bounds__(kT) spec_windows_kernel(SyncCfg c, SyncWork w,
   __shared__ int groups[kMaxGroups];
   __shared__ int ng;
   __shared__ int red[kT / 64 + 4];
+  SeqCtl& ctl = *w.ctl;
+  if (ctl.halted) {
+    // a halted call: this round decodes nothing
+    if (threadIdx.x == 0) {
+      *w.spec_count = 0;
+      ctl.decoded = 0;
+    }
+    return;
+  }
+  // this round's list-capacity flags start clear
+  for (int i = threadIdx.x; i < max_fields; i += kT) w.overflow[i] = 0;
   const int pc = *pulse_count < capacity ? *pulse_count : capacity;
   const double broad = 0.5 * ((c.hsync_us + c.vsync_us) * c.fs_mhz);
   if (threadIdx.x == 0) ng = 0;
@@ -1617,10 +1723,14 @@ __global__ void __launch_bounds__(kT) spec_windows_kernel(SyncCfg c, SyncWork w,
   const int n_groups = ng < kMaxGroups ? ng : kMaxGroups;
   const double line = c.nominal;
   const double eq_lines = c.num_pulses / 2.0;
-  long long begin = w.first_begin;
+  long long begin = ctl.begin;
+  ctl.base = ctl.records;
+  ctl.decoded = 0;
+  const int room = c.max_records - ctl.records;
+  const int lim = room < max_fields ? room : max_fields;
   int nf = 0;
   int g = 0;
-  for (int f = 0; f < max_fields; f++) {
+  for (int f = 0; f < lim; f++) {
     long long ws, wl;
     resolve_window(c, begin, &ws, &wl);
     if (ws + wl > span_len) break;
@@ -1650,13 +1760,13 @@ __global__ void reset_kernel(LevelStateDev* level, SyncHistoryDev* hist) {
 __global__ void tail_tables_kernel(SyncCfg c, SyncWork w, FieldTail t, int max_fields) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= max_fields) return;
-  const int n = *w.field_count;
+  const int n = w.ctl->decoded;
   const int lines = c.output_lines + 10;
   if (f >= n) {
     t.line_counts[f] = 0;
     return;
   }
-  const FieldRecord rec = w.records[f];
+  const FieldRecord rec = w.records[w.ctl->base + f];
   t.spans[f] = w.sync_ref + rec.window_start;
   t.span_len[f] = rec.window_length;
   t.locs[f] = w.lines + (long long)f * c.line_stride;
@@ -1668,13 +1778,14 @@ __global__ void tail_tables_kernel(SyncCfg c, SyncWork w, FieldTail t, int max_f
   t.first_lines[f] = t.out_first_line[rec.is_first_field ? 0 : 1];
   t.threshold[f] = rec.threshold;
   t.dc[f] = rec.dc_offset;
+  t.out_index[f] = w.ctl->base + f;
 }
 
 // ApplyNtscFscPhaseShiftCore (TbcFieldDecodePipeline.cs:4022-4038): every location less the shift
 __global__ void render_kernel(SyncCfg c, SyncWork w, FieldTail t, int max_fields) {
 #pragma clang fp contract(off)
   const int f = blockIdx.y;
-  if (f >= *w.field_count) return;
+  if (f >= w.ctl->decoded) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lines = c.output_lines + 10;
   if (i >= lines) return;
@@ -1682,8 +1793,58 @@ __global__ void render_kernel(SyncCfg c, SyncWork w, FieldTail t, int max_fields
   t.render[(long long)f * c.line_stride + i] = t.fsc_shift != 0.0 ? v - t.fsc_shift : v;
 }
 
+// The round's tables into the call's, by record: line locations, refined and render
+// locations (window coordinates), and the render locations from the field's output first
+// line in span coordinates (entries past the last location are NaN).
+__global__ void export_kernel(SyncCfg c, SyncWork w, FieldTail t, int max_fields) {
+#pragma clang fp contract(off)
+  const int f = blockIdx.y;
+  if (f >= w.ctl->decoded) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lines = c.output_lines + 10;
+  if (i >= lines) return;
+  const long long src = (long long)f * c.line_stride + i;
+  const long long dst = (long long)(w.ctl->base + f) * c.line_stride + i;
+  t.acc_lines[dst] = w.lines[src];
+  t.acc_refined[dst] = t.refined_base[src];
+  t.acc_render[dst] = t.render[src];
+  const int first = t.first_lines[f];
+  t.acc_span[dst] = i + first < lines ? t.render[src + first] + (double)t.origins[f] : __builtin_nan("");
+}
+
+__global__ void results_kernel(const SeqCtl* ctl, int* record_count, long long* next_begin) {
+  if (threadIdx.x != 0) return;
+  *record_count = ctl->records;
+  if (next_begin) *next_begin = ctl->begin;
+}
+
+__global__ void ctl_init_kernel(SeqCtl* ctl, long long first_begin) {
+  if (threadIdx.x != 0) return;
+  *ctl = SeqCtl{first_begin, 0, 0, 0, 0};
+}
+
 }  // namespace vsync
 }  // namespace vhsd
+
+extern "C" hipError_t vhsd_launch_vhs_ctl_init(vhsd::vsync::SeqCtl* ctl, long long first_begin, hipStream_t stream) {
+  hipLaunchKernelGGL(vhsd::vsync::ctl_init_kernel, dim3(1), dim3(64), 0, stream, ctl, first_begin);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t vhsd_launch_vhs_results(const vhsd::vsync::SeqCtl* ctl, int* record_count, int64_t* next_begin,
+                                              hipStream_t stream) {
+  hipLaunchKernelGGL(vhsd::vsync::results_kernel, dim3(1), dim3(64), 0, stream, ctl, record_count,
+                     reinterpret_cast<long long*>(next_begin));
+  return hipGetLastError();
+}
+
+extern "C" hipError_t vhsd_launch_vhs_export(const vhsd::vsync::SyncCfg* c, const vhsd::vsync::SyncWork* w,
+                                             const vhsd::vsync::FieldTail* t, int max_fields, hipStream_t stream) {
+  const int lines = c->output_lines + 10;
+  hipLaunchKernelGGL(vhsd::vsync::export_kernel, dim3((lines + 255) / 256, max_fields), dim3(256), 0, stream, *c, *w,
+                     *t, max_fields);
+  return hipGetLastError();
+}
 
 extern "C" hipError_t vhsd_launch_vhs_sync_reset(vhsd::vsync::LevelStateDev* level, vhsd::vsync::SyncHistoryDev* hist,
                                                  hipStream_t stream) {
@@ -1712,7 +1873,7 @@ extern "C" hipError_t vhsd_launch_vhs_sync(const vhsd::vsync::SyncCfg* c, const 
                                            long long span_len, int max_fields, hipStream_t stream) {
   using namespace vhsd::vsync;
   if (max_fields <= 0) return hipSuccess;
-  if (c->pulse_cap > kPulseCap || c->output_lines + 10 > c->line_stride) return hipErrorInvalidValue;
+  if (c->pulse_cap < 1 || c->output_lines + 10 > c->line_stride) return hipErrorInvalidValue;
   hipLaunchKernelGGL(spec_windows_kernel, dim3(1), dim3(kT), 0, stream, *c, *w, starts, lens, pulse_count, capacity,
                      span_len, max_fields);
   hipLaunchKernelGGL(level_search_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);

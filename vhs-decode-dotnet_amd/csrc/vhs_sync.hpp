@@ -8,12 +8,13 @@
 namespace vhsd {
 namespace vsync {
 
-constexpr int kPulseCap = 4096;   // pulses kept per field window
+constexpr int kPulseLds = 4096;   // pulse lists of a field staged in LDS up to this length
 constexpr int kHistBins = 512;    // hsync-length histogram (3.75 us of lengths)
 constexpr int kAvgCap = 16;       // moving-average values kept
 constexpr int kMaxGroups = 256;   // vertical-sync groups per span
 constexpr int kStatusMispredicted = 4;
 constexpr int kStatusCapacity = 5;
+constexpr int kStatusNotRead = 6;
 
 struct SyncCfg {
   double fs_mhz;
@@ -30,8 +31,9 @@ struct SyncCfg {
   int pulse_min, pulse_max;          // FindRawPulses bounds for vhs
   int fs_window;                     // VhsFieldLevelState window
   double bias;                       // plane bias (Hz)
-  int pulse_cap, edge_cap;           // per-field list capacities
+  int pulse_cap;                     // pulses kept per field window (the most a window can hold)
   int line_stride;                   // line locations per field in the output table
+  int max_records;                   // field records the caller holds
 };
 
 struct LevelStateDev {
@@ -87,17 +89,28 @@ struct FieldLogic {
 
 typedef vhsdecode_hip_vhs_field_v1 FieldRecord;
 
+// The sequence position across the rounds of one call: each round speculates windows from
+// `begin`, decodes them into records [base, base + decoded) and moves `begin` and `records`
+// on; `halted` ends the call (the next window runs past the span, a list overflowed, or
+// the records are full).
+struct SeqCtl {
+  long long begin;
+  int records;
+  int base;
+  int decoded;
+  int halted;
+};
+
 struct SyncWork {
   const float* sync_ref;
-  long long first_begin;
+  long long span_len;
+  SeqCtl* ctl;
   long long state_shift;   // span coordinates of the incoming history less this span's
   int* spec_count;
   long long* win_start;
   long long* win_len;
   int* pulse_start;
   int* pulse_len;
-  int* edge_pos;
-  unsigned char* edge_kind;
   int* overflow;
   LevelSearch* search;
   FieldLevels* levels;
@@ -109,13 +122,11 @@ struct SyncWork {
   FieldLogic* logic;
   int* ref_kind;
   int* ref_start;
-  FieldRecord* records;
-  int* field_count;
-  long long* next_begin_out;
-  int* stop_status;
+  unsigned char* logic_aux;  // [max_fields][6 * pulse_cap]: refined lengths, kinds and order flags
+  FieldRecord* records;      // [max_records], the whole call
   const SyncHistoryDev* history_in;
   SyncHistoryDev* history_out;
-  double* lines;
+  double* lines;             // [max_fields][line_stride], this round
 };
 
 }  // namespace vsync
@@ -140,12 +151,24 @@ struct FieldTail {
   double* refined_base;           // [max_fields][line_stride]
   unsigned char* filled_out_base;
   double* render;                 // [max_fields][line_stride]
+  int* out_index;                 // [max_fields]: the field's record (and output) index
+  // the call's tables, by record: line locations, refined, render (window coordinates)
+  // and the render locations from the output first line in span coordinates
+  double* acc_lines;
+  double* acc_refined;
+  double* acc_render;
+  double* acc_span;
   double fsc_shift;               // NTSC fsc phase shift (samples), 0 for PAL
   int out_first_line[2];          // OutputFirstLine: first field, second field
 };
 }  // namespace vsync
 }  // namespace vhsd
 
+extern "C" hipError_t vhsd_launch_vhs_results(const vhsd::vsync::SeqCtl* ctl, int* record_count, int64_t* next_begin,
+                                              hipStream_t stream);
+extern "C" hipError_t vhsd_launch_vhs_ctl_init(vhsd::vsync::SeqCtl* ctl, long long first_begin, hipStream_t stream);
+extern "C" hipError_t vhsd_launch_vhs_export(const vhsd::vsync::SyncCfg* c, const vhsd::vsync::SyncWork* w,
+                                             const vhsd::vsync::FieldTail* t, int max_fields, hipStream_t stream);
 extern "C" hipError_t vhsd_launch_vhs_sync_reset(vhsd::vsync::LevelStateDev* level, vhsd::vsync::SyncHistoryDev* hist,
                                                  hipStream_t stream);
 extern "C" hipError_t vhsd_launch_vhs_tail_tables(const vhsd::vsync::SyncCfg* c, const vhsd::vsync::SyncWork* w,

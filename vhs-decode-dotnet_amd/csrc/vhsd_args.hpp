@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "sos_scan.hpp"
+#include "vhsdecode_hip.h"
 
 namespace vhsd {
 
@@ -142,32 +143,6 @@ struct RepairArgs {
   float* scratch_demod;      // [nblocks][65536]: demod, diffed demod
 };
 
-// Line locator geometry (samples unless stated).
-struct LocateParams {
-  double line_len;      // nominal input line length
-  double broad_min;     // pulses longer than this are broad (vertical sync)
-  double hsync_min;     // pulses longer than this and not broad are hsync
-  double tol;           // snap tolerance in lines
-  int lines;            // output lines per field; lines + 1 locations
-  long long span_len;
-  double lead_lines;    // read-window lead before line 0, in lines
-  long long block_length, block_cut, stride;
-};
-
-// Per-field device tables the locator writes for the hsync and TBC kernels.
-struct FieldTables {
-  const float** spans;
-  long long* span_len;
-  const double** locs;
-  const unsigned char** filled;
-  double** refined;
-  unsigned char** filled_out;
-  int* line_counts;
-  long long* origins;
-  double* info_line0;
-  int* info_first;
-};
-
 struct HsyncParams {
   double threshold;
   double fs_mhz;       // sample rate in MHz (SyncAnalyzer.SampleRateMHz)
@@ -203,10 +178,9 @@ struct DropoutField {               // per-field state the mask pass hands to th
 struct DropoutArgs {
   const float* envelope;           // span plane
   long long span_len;
-  const double* locations;         // [fields][nloc], span coordinates
+  const double* locations;         // [fields][nloc], span coordinates, from the output first line
   int nloc;
-  const double* line0;             // [fields]
-  const int* field_first;          // [fields]
+  const vhsdecode_hip_vhs_field_v1* fields;  // [fields]: read window and parity
   const int* field_count;          // device count of fields to process
   double fraction;
   double absolute_threshold;       // >= 0: used instead of fraction x mean
@@ -216,10 +190,6 @@ struct DropoutArgs {
   int out_len;
   int first_lines, second_lines;
   int capacity;                    // rows and ranges stored per field
-  // read window (DecodeReadWindowPlanner)
-  long long read_samples;
-  double line_len, lead_lines;
-  long long block_length, block_cut, stride;
   // scratch
   float* partials;                 // [fields][128]
   float* group_min;                // [fields][128] smallest envelope value under each group
@@ -245,11 +215,6 @@ hipError_t vhsd_launch_block_demod(const vhsd::DemodArgs* a, int nblocks, hipStr
 hipError_t vhsd_launch_block_chroma(const vhsd::ChromaArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_repair(const vhsd::RepairArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_dropouts(const vhsd::DropoutArgs* a, int max_fields, hipStream_t stream);
-hipError_t vhsd_launch_locate(const float* sync_ref, long long span_len, const int* starts, const int* lens,
-                              const int* pulse_count, int capacity, const vhsd::LocateParams* lp, int max_fields,
-                              int cand_cap, double* cand_line0, int* cand_first, int* cand_count, double* cand_locs,
-                              unsigned char* cand_filled, int* cand_hit, double* ref_base, unsigned char* fo_base,
-                              const vhsd::FieldTables* t, int* field_count, hipStream_t stream);
 hipError_t vhsd_launch_block_demod_boost(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_demod_ld(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_stage1_table(const vhsd::DemodArgs* drecs, const vhsd::ChromaArgs* crecs,
@@ -279,7 +244,8 @@ hipError_t vhsd_launch_tbc_fields(const float* src, long long src_len, double bi
                                   int fields, double nominal, double line_decay, const double* pow_table,
                                   double* line_x, double* line_y0, int out_len, int smooth, double ire0,
                                   double out_scale, double inv_hz_ire, double conv_offset, const float* lut,
-                                  unsigned short* out, const int* field_count, hipStream_t stream);
+                                  unsigned short* out, const int* out_index, const int* field_count,
+                                  hipStream_t stream);
 hipError_t vhsd_launch_tbc(const float* src, long long src_len, double bias, const double* locs, const double* level_adjust,
                            int level_per_line, const vhsd::TbcParams* p, const float* lut, unsigned short* out_u16,
                            double* out_f64, hipStream_t stream);

@@ -275,10 +275,6 @@ struct vhsdecode_hip_demod_context {
   DevBuf<unsigned long long> d_mask;  // dropout low/high bit masks
   DevBuf<unsigned> d_sum;             // dropout mask summaries
   DevBuf<vhsd::DropoutField> d_info;  // dropout per-field state
-  // device line locator
-  DevBuf<double> l_line0, l_locs;
-  DevBuf<int> l_first, l_hit, l_count;
-  DevBuf<unsigned char> l_fil, l_tab;
   DevBuf<unsigned char> f_fil, f_fo, f_tab;
   // VHS field sequence (vhs_sync.hip): one allocation carved into the per-field lists,
   // records and tables; the sequence state (level moving averages and sync history) lives
@@ -288,6 +284,7 @@ struct vhsdecode_hip_demod_context {
   const double* vs_lines = nullptr;    // line locations, hsync-refined and render locations of the
   const double* vs_refined = nullptr;  // last vhs field decode ([fields][stride], window coordinates)
   const double* vs_render = nullptr;
+  const double* vs_span = nullptr;     // render locations from the output first line, span coordinates
   const long long* vs_origins = nullptr;
   const int* vs_first_lines = nullptr;
   int vs_line_stride = 0;
@@ -325,8 +322,6 @@ struct vhsdecode_hip_demod_context {
     p_lens.release(); edge_pos.release(); edge_kind.release(); h_loc.release(); h_ref.release();
     h_fil.release(); h_fo.release(); h_ptrs.release(); h_work.release(); lut.release(); t_loc.release(); t_adj.release();
     t_f64.release(); t_u16.release(); s_rf.release(); s_planes.release(); s_flags.release();
-    l_line0.release(); l_locs.release(); l_first.release(); l_hit.release(); l_count.release(); l_fil.release();
-    l_tab.release();
     vs_mem.release(); vs_state.release();
     f_loc.release(); f_ref.release(); f_x.release(); f_y0.release(); f_pow.release(); f_fil.release();
     f_fo.release(); f_tab.release();
@@ -1568,130 +1563,11 @@ int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod_context* c, const
                                       nfields,
                                       p->nominal_line_length, line_decay, smooth ? c->f_pow.p : nullptr, c->f_x.p,
                                       c->f_y0.p, p->output_line_length, smooth ? 1 : 0, p->ire0, out_scale,
-                                      1.0 / p->hz_ire, conv_off, c->lut.p, out, nullptr, st));
+                                      1.0 / p->hz_ire, conv_off, c->lut.p, out, nullptr, nullptr, st));
     }
     if (refined_out) {
       VHSD_HIP(hipMemcpyAsync(refined_out, c->f_ref.p, nl * sizeof(double), hipMemcpyDeviceToHost, st));
       VHSD_HIP(hipStreamSynchronize(st));
-    }
-    return VHSDECODE_HIP_OK;
-  } catch (const std::exception& e) {
-    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
-  }
-}
-
-int32_t vhsdecode_hip_decode_span_fields_device(vhsdecode_hip_demod_context* c, const float* sync_ref,
-                                                const float* video, int64_t span_length,
-                                                const vhsdecode_hip_field_params_v1* p,
-                                                const vhsdecode_hip_locator_v1* lz, int32_t max_fields, uint16_t* out,
-                                                int32_t* field_count, double* field_line0, int32_t* field_first,
-                                                void* stream) {
-  try {
-    if (!c || !sync_ref || !video || !p || !lz || !out || !field_count || max_fields <= 0 || span_length <= 0)
-      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid span field decode request");
-    if (p->struct_size != sizeof(vhsdecode_hip_field_params_v1) || lz->struct_size != sizeof(vhsdecode_hip_locator_v1))
-      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "field_params or locator struct_size mismatch");
-    if (p->field_lines <= 0 || p->field_lines + 1 > 512 || p->output_line_length <= 0 ||
-        !(p->nominal_line_length > 0.0) || !(lz->line_length > 0.0) || lz->block_length <= 0 ||
-        lz->block_stride <= 0 || lz->block_cut < 0)
-      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid field or locator geometry");
-    if (span_length > INT32_MAX)
-      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "spans are limited to 2^31 - 1 samples (int32 pulse positions)");
-    std::lock_guard<std::mutex> lock(c->mu);
-    VHSD_HIP(hipSetDevice(c->device));
-    int32_t rc = ensure_lut(c);
-    if (rc) return rc;
-    hipStream_t st = pick_stream(c, stream);
-    const int nloc = p->field_lines + 1;
-    const size_t nl = (size_t)max_fields * nloc;
-    // pulses: at most one per 16 samples is kept (sync pulses are >= 1 us apart)
-    const int capacity = (int)std::min<int64_t>(span_length / 16 + 1024, INT32_MAX / 2);
-    const int tiles = (int)((span_length + 4095) / 4096);
-    const size_t edge_cap = (size_t)(span_length / 2 + 2);
-    VHSD_HIP(c->tile_counts.ensure(std::max(tiles, 1)));
-    VHSD_HIP(c->counters.ensure(2));
-    VHSD_HIP(c->edge_pos.ensure(edge_cap));
-    VHSD_HIP(c->edge_kind.ensure(edge_cap));
-    VHSD_HIP(c->p_starts.ensure(capacity));
-    VHSD_HIP(c->p_lens.ensure(capacity));
-    // candidates: every vertical-sync group is one; leave room for rejected ones
-    const int cand_cap = 2 * max_fields + 16;
-    VHSD_HIP(c->l_line0.ensure(cand_cap));
-    VHSD_HIP(c->l_first.ensure(cand_cap));
-    VHSD_HIP(c->l_hit.ensure(cand_cap));
-    VHSD_HIP(c->l_count.ensure(1));
-    VHSD_HIP(c->l_locs.ensure((size_t)cand_cap * nloc));
-    VHSD_HIP(c->l_fil.ensure((size_t)cand_cap * nloc));
-    VHSD_HIP(c->f_ref.ensure(nl));
-    VHSD_HIP(c->f_fo.ensure(nl));
-    VHSD_HIP(c->f_x.ensure((size_t)max_fields * p->field_lines));
-    c->f_ref_nloc = nloc;
-    VHSD_HIP(c->f_y0.ensure((size_t)max_fields * p->field_lines));
-    // field tables: 8 pointer-sized arrays, then line counts and first flags (int)
-    const size_t mf = (size_t)max_fields;
-    VHSD_HIP(c->l_tab.ensure(mf * (8 * 8 + 4 + 4) + 64));
-    unsigned char* tb = c->l_tab.p;
-    vhsd::FieldTables t;
-    t.spans = reinterpret_cast<const float**>(tb);
-    t.span_len = reinterpret_cast<long long*>(tb + 8 * mf);
-    t.locs = reinterpret_cast<const double**>(tb + 16 * mf);
-    t.filled = reinterpret_cast<const unsigned char**>(tb + 24 * mf);
-    t.refined = reinterpret_cast<double**>(tb + 32 * mf);
-    t.filled_out = reinterpret_cast<unsigned char**>(tb + 40 * mf);
-    t.origins = reinterpret_cast<long long*>(tb + 48 * mf);
-    t.info_line0 = field_line0 ? field_line0 : reinterpret_cast<double*>(tb + 56 * mf);
-    t.line_counts = reinterpret_cast<int*>(tb + 64 * mf);
-    t.info_first = field_first ? field_first : reinterpret_cast<int*>(tb + 68 * mf);
-    const bool smooth = p->wow_level_smoothing > 0.0;
-    const double alpha = smooth ? 1.0 / (p->wow_level_smoothing * p->output_line_length) : 0.0;
-    if (smooth && (c->f_pow_alpha != alpha || c->f_pow_len != p->output_line_length)) {
-      std::vector<double> pw(p->output_line_length);
-      for (int s2 = 0; s2 < p->output_line_length; s2++) pw[s2] = std::pow(1.0 - alpha, (double)(s2 + 1));
-      VHSD_HIP(c->f_pow.ensure(pw.size()));
-      VHSD_HIP(hipMemcpy(c->f_pow.p, pw.data(), sizeof(double) * pw.size(), hipMemcpyHostToDevice));
-      c->f_pow_alpha = alpha;
-      c->f_pow_len = p->output_line_length;
-    }
-    const double line_decay = smooth ? std::pow(1.0 - alpha, (double)p->output_line_length) : 0.0;
-    {
-      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_PULSES, span_length);
-        VHSD_HIP(vhsd_launch_pulses(sync_ref, span_length, c->bias, p->threshold, lz->min_pulse_length,
-                                  lz->max_pulse_length, c->tile_counts.p, tiles, c->edge_pos.p, c->edge_kind.p,
-                                  (int)edge_cap, c->counters.p, c->p_starts.p, c->p_lens.p, capacity, st));
-    }
-    vhsd::LocateParams lp{};
-    lp.line_len = lz->line_length;
-    lp.broad_min = lz->broad_min_length;
-    lp.hsync_min = lz->hsync_min_length;
-    lp.tol = lz->snap_tolerance;
-    lp.lines = p->field_lines;
-    lp.span_len = span_length;
-    lp.lead_lines = lz->lead_lines;
-    lp.block_length = lz->block_length;
-    lp.block_cut = lz->block_cut;
-    lp.stride = lz->block_stride;
-    {
-      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_LOCATE, max_fields);
-      VHSD_HIP(vhsd_launch_locate(sync_ref, span_length, c->p_starts.p, c->p_lens.p, c->counters.p + 1, capacity, &lp,
-                                  max_fields, cand_cap, c->l_line0.p, c->l_first.p, c->l_count.p, c->l_locs.p,
-                                  c->l_fil.p, c->l_hit.p, c->f_ref.p, c->f_fo.p, &t, field_count, st));
-    }
-    vhsd::HsyncParams hp{p->threshold, p->sample_rate_mhz, p->hsync_pulse_us, p->ire0, p->hz_ire, p->is_pal,
-                         p->use_right_hsync};
-    {
-      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_HSYNC, max_fields);
-      VHSD_HIP(c->h_work.ensure(vhsd_hsync_work_bytes(max_fields, nloc)));
-      VHSD_HIP(vhsd_launch_hsync(t.spans, t.span_len, t.locs, t.filled, t.line_counts, &hp, t.refined, t.filled_out,
-                                 max_fields, nloc, c->bias, c->h_work.p, nullptr, nullptr, st));
-    }
-    const double out_scale = p->output_scale;
-    const double conv_off = (p->output_zero + 0.5) - (p->vsync_ire * out_scale);
-    {
-      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_TBC, (long long)max_fields * p->field_lines * p->output_line_length);
-      VHSD_HIP(vhsd_launch_tbc_fields(video, span_length, c->bias, t.origins, c->f_ref.p, nloc, p->field_lines, nullptr, 0, 0,
-                                      max_fields, p->nominal_line_length, line_decay, smooth ? c->f_pow.p : nullptr,
-                                      c->f_x.p, c->f_y0.p, p->output_line_length, smooth ? 1 : 0, p->ire0, out_scale,
-                                      1.0 / p->hz_ire, conv_off, c->lut.p, out, field_count, st));
     }
     return VHSDECODE_HIP_OK;
   } catch (const std::exception& e) {
@@ -1704,12 +1580,12 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
                                                int32_t reset_state, int64_t state_shift,
                                                const vhsdecode_hip_vhs_sync_v1* sy,
                                                const vhsdecode_hip_field_params_v1* p, int32_t max_fields,
-                                               uint16_t* out, vhsdecode_hip_vhs_field_v1* fields,
+                                               int32_t rounds, uint16_t* out, vhsdecode_hip_vhs_field_v1* fields,
                                                int32_t* field_count, int64_t* next_begin, void* stream) {
   using namespace vhsd::vsync;
   try {
     if (!c || !sync_ref || !video || !sy || !p || !out || !field_count || max_fields <= 0 || span_length <= 0 ||
-        first_begin < 0)
+        first_begin < 0 || rounds < 1 || rounds > 64)
       return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid vhs field decode request");
     if (sy->struct_size != sizeof(vhsdecode_hip_vhs_sync_v1) || p->struct_size != sizeof(vhsdecode_hip_field_params_v1))
       return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "vhs_sync or field_params struct_size mismatch");
@@ -1777,9 +1653,15 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
     if (cfg.fs_window < 1 || cfg.fs_window > kAvgCap)
       return fail(VHSDECODE_HIP_E_UNSUPPORTED, "field-state moving-average window out of range");
     cfg.bias = c->bias;
-    cfg.pulse_cap = kPulseCap;
-    cfg.edge_cap = 65536;
+    // the most pulses a read window can hold: each is at least the shortest length the
+    // pulse searches keep, and is followed by a sample above the threshold
+    const int64_t window_max = sy->read_samples + 2 * (int64_t)sy->block_stride;
+    const int min_kept = std::max(1, std::min(cfg.pulse_min, (int)std::ceil(cfg.ser_eq_len / 8.0)));
+    const int64_t pulse_cap = window_max / (min_kept + 1) + 16;
+    if (pulse_cap > INT32_MAX / 8) return fail(VHSDECODE_HIP_E_UNSUPPORTED, "read windows too long");
+    cfg.pulse_cap = (int)pulse_cap;
     cfg.line_stride = line_stride;
+    cfg.max_records = max_fields;
     // one allocation for every list, record and table
     const size_t F = (size_t)max_fields;
     size_t off = 0;
@@ -1788,18 +1670,19 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
       off += (bytes + 255) & ~(size_t)255;
       return o;
     };
-    const size_t o_spec = take(4), o_ws = take(8 * F), o_wl = take(8 * F);
-    const size_t o_pst = take(4 * F * kPulseCap), o_pln = take(4 * F * kPulseCap);
-    const size_t o_epos = take(4 * F * cfg.edge_cap), o_ek = take(F * cfg.edge_cap), o_ovf = take(4 * F);
+    const size_t o_ctl = take(sizeof(SeqCtl)), o_spec = take(4), o_ws = take(8 * F), o_wl = take(8 * F);
+    const size_t PC = (size_t)cfg.pulse_cap;
+    const size_t o_pst = take(4 * F * PC), o_pln = take(4 * F * PC), o_ovf = take(4 * F);
     const size_t o_search = take(sizeof(LevelSearch) * F), o_levels = take(sizeof(FieldLevels) * F);
     const size_t o_before = take(sizeof(LevelStateDev) * F), o_after = take(sizeof(LevelStateDev));
     const size_t o_raw = take(4 * F), o_logic = take(sizeof(FieldLogic) * F);
-    const size_t o_rk = take(4 * F * kPulseCap), o_rs = take(4 * F * kPulseCap);
-    const size_t o_rec = take(sizeof(FieldRecord) * F), o_cnt = take(4), o_nb = take(8), o_stop = take(4);
+    const size_t o_rk = take(4 * F * PC), o_rs = take(4 * F * PC), o_aux = take(6 * F * PC);
+    const size_t o_rec = take(sizeof(FieldRecord) * F);
     const size_t o_lines = take(8 * F * line_stride);
     const size_t o_tp = take(8 * F * 6), o_lc = take(4 * F), o_org = take(8 * F), o_fl = take(4 * F);
-    const size_t o_thr = take(8 * F), o_dc = take(8 * F), o_zero = take(line_stride);
+    const size_t o_thr = take(8 * F), o_dc = take(8 * F), o_zero = take(line_stride), o_oi = take(4 * F);
     const size_t o_refb = take(8 * F * line_stride), o_fob = take(F * line_stride), o_rend = take(8 * F * line_stride);
+    const size_t o_acc = take(4 * 8 * F * line_stride);
     VHSD_HIP(c->vs_mem.ensure(off));
     unsigned char* m = c->vs_mem.p;
     const size_t state_hist = (sizeof(LevelStateDev) + 255) & ~(size_t)255;
@@ -1812,19 +1695,20 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
       VHSD_HIP(vhsd_launch_vhs_sync_reset(level_state, history, st));
       c->vs_state_ready = true;
     }
+    SeqCtl* ctl = reinterpret_cast<SeqCtl*>(m + o_ctl);
+    VHSD_HIP(vhsd_launch_vhs_ctl_init(ctl, first_begin, st));
     // the zero `filled` row (BuildUpstreamLineLocations fills no line); written by a fill kernel
     VHSD_HIP(vhsd_launch_fill_u32(reinterpret_cast<unsigned*>(m + o_zero), (line_stride + 3) / 4, 0u, st));
     SyncWork w{};
     w.sync_ref = sync_ref;
-    w.first_begin = first_begin;
+    w.span_len = span_length;
+    w.ctl = ctl;
     w.state_shift = reset_state ? 0 : state_shift;
     w.spec_count = reinterpret_cast<int*>(m + o_spec);
     w.win_start = reinterpret_cast<long long*>(m + o_ws);
     w.win_len = reinterpret_cast<long long*>(m + o_wl);
     w.pulse_start = reinterpret_cast<int*>(m + o_pst);
     w.pulse_len = reinterpret_cast<int*>(m + o_pln);
-    w.edge_pos = reinterpret_cast<int*>(m + o_epos);
-    w.edge_kind = m + o_ek;
     w.overflow = reinterpret_cast<int*>(m + o_ovf);
     w.search = reinterpret_cast<LevelSearch*>(m + o_search);
     w.levels = reinterpret_cast<FieldLevels*>(m + o_levels);
@@ -1836,24 +1720,16 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
     w.logic = reinterpret_cast<FieldLogic*>(m + o_logic);
     w.ref_kind = reinterpret_cast<int*>(m + o_rk);
     w.ref_start = reinterpret_cast<int*>(m + o_rs);
-    w.records = reinterpret_cast<FieldRecord*>(m + o_rec);
-    w.field_count = reinterpret_cast<int*>(m + o_cnt);
-    w.next_begin_out = reinterpret_cast<long long*>(m + o_nb);
-    w.stop_status = reinterpret_cast<int*>(m + o_stop);
+    w.logic_aux = m + o_aux;
+    w.records = fields ? reinterpret_cast<FieldRecord*>(fields) : reinterpret_cast<FieldRecord*>(m + o_rec);
     w.history_in = history;
     w.history_out = history;
     w.lines = reinterpret_cast<double*>(m + o_lines);
-    VHSD_HIP(vhsd_launch_fill_u32(reinterpret_cast<unsigned*>(w.overflow), (long long)F, 0u, st));
     {
       ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_PULSES, span_length);
       VHSD_HIP(vhsd_launch_pulses(sync_ref, span_length, c->bias, p->threshold, cfg.pulse_min, cfg.pulse_max,
                                   c->tile_counts.p, tiles, c->edge_pos.p, c->edge_kind.p, (int)edge_cap_span,
                                   c->counters.p, c->p_starts.p, c->p_lens.p, capacity, st));
-    }
-    {
-      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_LOCATE, max_fields);
-      VHSD_HIP(vhsd_launch_vhs_sync(&cfg, &w, c->p_starts.p, c->p_lens.p, c->counters.p + 1, capacity, span_length,
-                                    max_fields, st));
     }
     FieldTail t{};
     t.spans = reinterpret_cast<const float**>(m + o_tp);
@@ -1871,23 +1747,22 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
     t.refined_base = reinterpret_cast<double*>(m + o_refb);
     t.filled_out_base = m + o_fob;
     t.render = reinterpret_cast<double*>(m + o_rend);
-    // ApplyNtscFscPhaseShiftCore with fsc = output rate / 4 (TbcFieldDecodePipeline.cs:1401-1407, 4022-4038)
+    t.out_index = reinterpret_cast<int*>(m + o_oi);
+    const size_t acc_n = F * line_stride;
+    t.acc_lines = reinterpret_cast<double*>(m + o_acc);
+    t.acc_refined = t.acc_lines + acc_n;
+    t.acc_render = t.acc_refined + acc_n;
+    t.acc_span = t.acc_render + acc_n;
+    // ApplyNtscFscPhaseShiftCore with fsc = TbcFrameSpec.OutputSampleRateHz / 4
+    // (TbcFieldDecodePipeline.cs:1394-1400, 4022-4038)
     const double fsc_mhz = (sy->output_rate_mhz * 1000000.0) / 4000000.0;
     t.fsc_shift = (cfg.is_pal || !(fsc_mhz > 0.0)) ? 0.0 : ((117.25 / 360.0) / fsc_mhz) * fs_mhz;
     // OutputFirstLine (TbcFieldDecodePipeline.cs:2475-2483)
     t.out_first_line[0] = cfg.is_pal ? 3 : 1;
     t.out_first_line[1] = cfg.is_pal ? 4 : 1;
-    VHSD_HIP(vhsd_launch_vhs_tail_tables(&cfg, &w, &t, max_fields, st));
     vhsd::HsyncParams hp{p->threshold, fs_mhz, sy->hsync_pulse_us, p->ire0, p->hz_ire, cfg.is_pal,
                          p->use_right_hsync};
-    {
-      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_HSYNC, max_fields);
-      VHSD_HIP(c->h_work.ensure(vhsd_hsync_work_bytes(max_fields, line_stride)));
-      VHSD_HIP(vhsd_launch_hsync(t.spans, t.span_len, t.locs, t.filled, t.line_counts, &hp, t.refined, t.filled_out,
-                                 max_fields, line_stride, c->bias, c->h_work.p, t.threshold, t.dc, st));
-    }
-    VHSD_HIP(vhsd_launch_vhs_render(&cfg, &w, &t, max_fields, st));
-    c->f_ref_nloc = 0;
+    VHSD_HIP(c->h_work.ensure(vhsd_hsync_work_bytes(max_fields, line_stride)));
     const bool smooth = p->wow_level_smoothing > 0.0;
     const double alpha = smooth ? 1.0 / (p->wow_level_smoothing * p->output_line_length) : 0.0;
     if (smooth && (c->f_pow_alpha != alpha || c->f_pow_len != p->output_line_length)) {
@@ -1901,22 +1776,41 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
     const double line_decay = smooth ? std::pow(1.0 - alpha, (double)p->output_line_length) : 0.0;
     const double out_scale = p->output_scale;
     const double conv_off = (p->output_zero + 0.5) - (p->vsync_ire * out_scale);
-    {
-      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_TBC, (long long)max_fields * out_lines * p->output_line_length);
-      VHSD_HIP(vhsd_launch_tbc_fields(video, span_length, c->bias, t.origins, t.render, line_stride, out_lines,
-                                      t.first_lines, max_first, 1, max_fields, p->nominal_line_length, line_decay,
-                                      smooth ? c->f_pow.p : nullptr, c->f_x.p, c->f_y0.p, p->output_line_length,
-                                      smooth ? 1 : 0, p->ire0, out_scale, 1.0 / p->hz_ire, conv_off, c->lut.p, out,
-                                      w.field_count, st));
+    // Each round speculates the read windows from where the sequence stands, decodes
+    // them in parallel and verifies them in order; a round after a mispredicted window
+    // or a failed field carries on from there, and a halted call's rounds do nothing.
+    for (int r = 0; r < rounds; r++) {
+      SyncWork wr = w;
+      if (r > 0) wr.state_shift = 0;
+      {
+        ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_LOCATE, max_fields);
+        VHSD_HIP(vhsd_launch_vhs_sync(&cfg, &wr, c->p_starts.p, c->p_lens.p, c->counters.p + 1, capacity,
+                                      span_length, max_fields, st));
+      }
+      VHSD_HIP(vhsd_launch_vhs_tail_tables(&cfg, &wr, &t, max_fields, st));
+      {
+        ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_HSYNC, max_fields);
+        VHSD_HIP(vhsd_launch_hsync(t.spans, t.span_len, t.locs, t.filled, t.line_counts, &hp, t.refined,
+                                   t.filled_out, max_fields, line_stride, c->bias, c->h_work.p, t.threshold, t.dc,
+                                   st));
+      }
+      VHSD_HIP(vhsd_launch_vhs_render(&cfg, &wr, &t, max_fields, st));
+      VHSD_HIP(vhsd_launch_vhs_export(&cfg, &wr, &t, max_fields, st));
+      {
+        ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_TBC, (long long)max_fields * out_lines * p->output_line_length);
+        VHSD_HIP(vhsd_launch_tbc_fields(video, span_length, c->bias, t.origins, t.render, line_stride, out_lines,
+                                        t.first_lines, max_first, 1, max_fields, p->nominal_line_length,
+                                        line_decay, smooth ? c->f_pow.p : nullptr, c->f_x.p, c->f_y0.p,
+                                        p->output_line_length, smooth ? 1 : 0, p->ire0, out_scale, 1.0 / p->hz_ire,
+                                        conv_off, c->lut.p, out, t.out_index, &ctl->decoded, st));
+      }
     }
-    if (fields) VHSD_HIP(hipMemcpyAsync(fields, w.records, sizeof(FieldRecord) * F, hipMemcpyDeviceToDevice, st));
-    VHSD_HIP(hipMemcpyAsync(field_count, w.field_count, 4, hipMemcpyDeviceToDevice, st));
-    if (next_begin) VHSD_HIP(hipMemcpyAsync(next_begin, w.next_begin_out, 8, hipMemcpyDeviceToDevice, st));
-    c->vs_lines = w.lines;
-    c->vs_refined = t.refined_base;
-    c->vs_render = t.render;
-    c->vs_origins = t.origins;
-    c->vs_first_lines = t.first_lines;
+    VHSD_HIP(vhsd_launch_vhs_results(ctl, field_count, next_begin, st));
+    c->f_ref_nloc = 0;
+    c->vs_lines = t.acc_lines;
+    c->vs_refined = t.acc_refined;
+    c->vs_render = t.acc_render;
+    c->vs_span = t.acc_span;
     c->vs_line_stride = line_stride;
     return VHSDECODE_HIP_OK;
   } catch (const std::exception& e) {
@@ -1926,11 +1820,11 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
 
 int32_t vhsdecode_hip_vhs_field_lines_device(vhsdecode_hip_demod_context* c, int32_t which, const double** table,
                                              int32_t* stride) {
-  if (!c || !table || !stride || which < 0 || which > 2)
+  if (!c || !table || !stride || which < 0 || which > 3)
     return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid arguments");
   std::lock_guard<std::mutex> lock(c->mu);
   if (!c->vs_render) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "no vhs field decode yet");
-  *table = which == 0 ? c->vs_lines : which == 1 ? c->vs_refined : c->vs_render;
+  *table = which == 0 ? c->vs_lines : which == 1 ? c->vs_refined : which == 2 ? c->vs_render : c->vs_span;
   *stride = c->vs_line_stride;
   return VHSDECODE_HIP_OK;
 }
@@ -1947,14 +1841,14 @@ int32_t vhsdecode_hip_field_locations_device(vhsdecode_hip_demod_context* c, con
 
 int32_t vhsdecode_hip_detect_dropouts_device(vhsdecode_hip_demod_context* c, const float* envelope,
                                              int64_t span_length, const double* locations, int32_t location_count,
-                                             const double* field_line0, const int32_t* field_first,
-                                             const int32_t* field_count, int32_t max_fields,
+                                             const vhsdecode_hip_vhs_field_v1* fields, const int32_t* field_count,
+                                             int32_t max_fields,
                                              const vhsdecode_hip_dropout_params_v1* p, int32_t* row_count,
                                              int32_t* field_line, int32_t* start_x, int32_t* end_x,
                                              int32_t* range_count, int32_t* range_start, int32_t* range_end,
                                              void* stream) {
   try {
-    if (!c || !envelope || !locations || !field_line0 || !field_first || !field_count || !p || !row_count ||
+    if (!c || !envelope || !locations || !fields || !field_count || !p || !row_count ||
         !field_line || !start_x || !end_x || max_fields < 0 || span_length <= 0)
       return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid dropout detection request");
     if (p->struct_size != sizeof(vhsdecode_hip_dropout_params_v1))
@@ -1966,16 +1860,10 @@ int32_t vhsdecode_hip_detect_dropouts_device(vhsdecode_hip_demod_context* c, con
       return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "location_count must be 2..512");
     if (p->capacity <= 0 || p->output_line_length <= 0 || p->merge_threshold < 0 || p->minimum_length < 0 ||
         !(p->hysteresis > 0.0) || !(p->threshold_fraction >= 0.0) || p->first_field_lines <= 0 ||
-        p->second_field_lines <= 0 || p->read_samples < 0 || !(p->line_length > 0.0) || !(p->lead_lines >= 0.0) ||
-        p->block_length <= 0 || p->block_cut < 0 || p->block_stride <= 0)
+        p->second_field_lines <= 0)
       return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid dropout parameters");
     if (span_length > INT32_MAX)
       return fail(VHSDECODE_HIP_E_UNSUPPORTED, "spans are limited to 2^31 - 1 samples (int32 range positions)");
-    // every node of the window's pairwise tree at depth 15 must be a leaf (<= 128 values):
-    // a node at depth d holds fewer than n / 2^d + 16 values
-    const int64_t window_max = p->read_samples + p->block_length + 2 * (int64_t)p->block_stride;
-    if (window_max > 3600000)
-      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "read windows are limited to 3.6 M samples");
     if (max_fields == 0) return VHSDECODE_HIP_OK;
     std::lock_guard<std::mutex> lock(c->mu);
     VHSD_HIP(hipSetDevice(c->device));
@@ -1990,8 +1878,7 @@ int32_t vhsdecode_hip_detect_dropouts_device(vhsdecode_hip_demod_context* c, con
     a.span_len = span_length;
     a.locations = locations;
     a.nloc = location_count;
-    a.line0 = field_line0;
-    a.field_first = field_first;
+    a.fields = fields;
     a.field_count = field_count;
     a.fraction = p->threshold_fraction;
     a.absolute_threshold = p->absolute_threshold;
@@ -2002,12 +1889,6 @@ int32_t vhsdecode_hip_detect_dropouts_device(vhsdecode_hip_demod_context* c, con
     a.first_lines = p->first_field_lines;
     a.second_lines = p->second_field_lines;
     a.capacity = p->capacity;
-    a.read_samples = p->read_samples;
-    a.line_len = p->line_length;
-    a.lead_lines = p->lead_lines;
-    a.block_length = p->block_length;
-    a.block_cut = p->block_cut;
-    a.stride = p->block_stride;
     a.partials = c->d_part.p;
     a.group_min = c->d_part.p + (size_t)max_fields * 128;
     a.info = c->d_info.p;

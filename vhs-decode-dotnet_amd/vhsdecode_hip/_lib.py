@@ -31,7 +31,6 @@ REQUIRED_EXPORTS = (
     "vhsdecode_hip_tbc_resample",
     "vhsdecode_hip_tbc_resample_device",
     "vhsdecode_hip_decode_fields_device",
-    "vhsdecode_hip_decode_span_fields_device",
     "vhsdecode_hip_serve_create",
     "vhsdecode_hip_serve_add_format",
     "vhsdecode_hip_serve_add_stream",
@@ -180,22 +179,12 @@ class VhsFieldV1(ctypes.Structure):
                 ("next_begin", ctypes.c_int64)]
 
 
-class LocatorV1(ctypes.Structure):
-    _fields_ = [("struct_size", ctypes.c_uint32), ("min_pulse_length", ctypes.c_int32),
-                ("max_pulse_length", ctypes.c_int32), ("line_length", ctypes.c_double),
-                ("broad_min_length", ctypes.c_double), ("hsync_min_length", ctypes.c_double),
-                ("snap_tolerance", ctypes.c_double), ("lead_lines", ctypes.c_double),
-                ("block_length", ctypes.c_int32), ("block_cut", ctypes.c_int32), ("block_stride", ctypes.c_int32)]
-
-
 class DropoutParamsV1(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("threshold_fraction", ctypes.c_double),
                 ("absolute_threshold", ctypes.c_double), ("hysteresis", ctypes.c_double),
                 ("merge_threshold", ctypes.c_int32), ("minimum_length", ctypes.c_int32),
                 ("output_line_length", ctypes.c_int32), ("first_field_lines", ctypes.c_int32),
-                ("second_field_lines", ctypes.c_int32), ("capacity", ctypes.c_int32),
-                ("read_samples", ctypes.c_int64), ("line_length", ctypes.c_double), ("lead_lines", ctypes.c_double),
-                ("block_length", ctypes.c_int32), ("block_cut", ctypes.c_int32), ("block_stride", ctypes.c_int32)]
+                ("second_field_lines", ctypes.c_int32), ("capacity", ctypes.c_int32)]
 
 
 class ServeConfigV1(ctypes.Structure):
@@ -207,7 +196,7 @@ class ServeConfigV1(ctypes.Structure):
 class ServeFormatV1(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("filters", ctypes.POINTER(FiltersV1)),
                 ("block_cut", ctypes.c_int32), ("block_cut_end", ctypes.c_int32),
-                ("field", ctypes.POINTER(FieldParamsV1)), ("locator", ctypes.POINTER(LocatorV1)),
+                ("field", ctypes.POINTER(FieldParamsV1)), ("sync", ctypes.POINTER(VhsSyncV1)),
                 ("sinc_table", _fp), ("sinc_count", ctypes.c_int64)]
 
 
@@ -304,15 +293,11 @@ def load():
                                            ctypes.POINTER(WindowResultV1)]
     L.vhsdecode_hip_serve_destroy.argtypes = [ctypes.c_void_p]
     L.vhsdecode_hip_serve_destroy.restype = None
-    L.vhsdecode_hip_decode_span_fields_device.argtypes = [
-        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(FieldParamsV1),
-        ctypes.POINTER(LocatorV1), ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-        ctypes.c_void_p]
     L.vhsdecode_hip_field_locations_device.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
                                                        ctypes.POINTER(ctypes.c_int32)]
     L.vhsdecode_hip_detect_dropouts_device.argtypes = [
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
-        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(DropoutParamsV1)] + [ctypes.c_void_p] * 8
+        ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(DropoutParamsV1)] + [ctypes.c_void_p] * 8
     L.vhsdecode_hip_decode_fields_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                      ctypes.c_int64, _dp, _u8p, ctypes.POINTER(ctypes.c_int64),
                                                      ctypes.c_int32,
@@ -340,8 +325,8 @@ def load():
     L.vhsdecode_hip_run.argtypes = [ctypes.POINTER(RunConfigV1), ctypes.POINTER(RunResultV1)]
     L.vhsdecode_hip_decode_vhs_fields_device.argtypes = [
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
-        ctypes.c_int64, ctypes.POINTER(VhsSyncV1), ctypes.POINTER(FieldParamsV1), ctypes.c_int32, ctypes.c_void_p,
-        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        ctypes.c_int64, ctypes.POINTER(VhsSyncV1), ctypes.POINTER(FieldParamsV1), ctypes.c_int32, ctypes.c_int32,
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     L.vhsdecode_hip_vhs_field_lines_device.argtypes = [ctypes.c_void_p, ctypes.c_int32,
                                                        ctypes.POINTER(ctypes.c_void_p),
                                                        ctypes.POINTER(ctypes.c_int32)]

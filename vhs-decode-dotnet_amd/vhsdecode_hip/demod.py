@@ -311,17 +311,6 @@ class HipBlockDemodulator:
                                   geom.hsync / geom.fs_mhz, ire0, hz_ire, 1 if geom.is_pal else 0, 1, geom.out_len,
                                   geom.field_lines, geom.nominal, geom.smoothing, ozero, vsync, oscale)
 
-    def decode_span_fields_device(self, sync_ref_ptr: int, video_ptr: int, span_length: int, geom, locator,
-                                  max_fields: int, out_ptr: int, count_ptr: int, line0_ptr: int = 0,
-                                  first_ptr: int = 0, stream: int = 0):
-        """Pulses -> line locations -> hsync -> TBC of up to max_fields fields, all on the device and
-        asynchronous; the field count lands in device int32 at count_ptr."""
-        p = self.field_params(geom)
-        _lib.check(self._L.vhsdecode_hip_decode_span_fields_device(
-            self._h, ctypes.c_void_p(sync_ref_ptr), ctypes.c_void_p(video_ptr), span_length, ctypes.byref(p),
-            ctypes.byref(locator), max_fields, ctypes.c_void_p(out_ptr), ctypes.c_void_p(count_ptr),
-            ctypes.c_void_p(line0_ptr or None), ctypes.c_void_p(first_ptr or None), ctypes.c_void_p(stream)))
-
     def decode_fields_device(self, sync_ref_ptr: int, video_ptr: int, span_length: int, locations: np.ndarray,
                              filled: np.ndarray, geom, out_ptr: int, origins: Optional[np.ndarray] = None,
                              want_refined: bool = False, stream: int = 0):
@@ -343,20 +332,22 @@ class HipBlockDemodulator:
     def decode_vhs_fields_device(self, sync_ref_ptr: int, video_ptr: int, span_length: int, first_begin: int,
                                  sync, geom, max_fields: int, out_ptr: int, count_ptr: int, fields_ptr: int = 0,
                                  next_begin_ptr: int = 0, reset_state: bool = True, state_shift: int = 0,
-                                 stream: int = 0):
+                                 rounds: int = 4, stream: int = 0):
         """The vhs field sequence from `first_begin` on: read windows, sync levels, pulses,
         parity, line 0, line locations, hsync refinement and TBC, all on the device and
-        asynchronous.  `sync` is a VhsSyncV1; records (VhsFieldV1) land at fields_ptr."""
+        asynchronous.  `sync` is a VhsSyncV1; records (VhsFieldV1) land at fields_ptr and
+        their count at count_ptr."""
         p = self.field_params(geom)
         v = ctypes.c_void_p
         _lib.check(self._L.vhsdecode_hip_decode_vhs_fields_device(
             self._h, v(sync_ref_ptr), v(video_ptr), span_length, first_begin, 1 if reset_state else 0, state_shift,
-            ctypes.byref(sync), ctypes.byref(p), max_fields, v(out_ptr), v(fields_ptr or None), v(count_ptr),
+            ctypes.byref(sync), ctypes.byref(p), max_fields, rounds, v(out_ptr), v(fields_ptr or None), v(count_ptr),
             v(next_begin_ptr or None), v(stream)))
 
     def vhs_field_lines_device(self, which: int):
-        """(device pointer, stride) of a line table of the last vhs field decode: 0 line
-        locations, 1 hsync-refined, 2 render locations; window coordinates."""
+        """(device pointer, stride) of a line table of the last vhs field decode, by record:
+        0 line locations, 1 hsync-refined, 2 render locations (window coordinates), 3 render
+        locations from the output first line (span coordinates)."""
         ptr = ctypes.c_void_p()
         stride = ctypes.c_int32()
         _lib.check(self._L.vhsdecode_hip_vhs_field_lines_device(self._h, which, ctypes.byref(ptr),
@@ -372,18 +363,17 @@ class HipBlockDemodulator:
         return int(ptr.value), int(cnt.value)
 
     def detect_dropouts_device(self, envelope_ptr: int, span_length: int, locations_ptr: int, location_count: int,
-                               line0_ptr: int, first_ptr: int, count_ptr: int, max_fields: int, params,
-                               row_count_ptr: int, field_line_ptr: int, start_x_ptr: int, end_x_ptr: int,
-                               range_count_ptr: int = 0, range_start_ptr: int = 0, range_end_ptr: int = 0,
-                               stream: int = 0):
-        """Tape RF-envelope dropout detection of up to max_fields fields (every pointer device
-        memory; asynchronous on `stream`): .tbc.json rows per field at [field][capacity]."""
+                               fields_ptr: int, count_ptr: int, max_fields: int, params, row_count_ptr: int,
+                               field_line_ptr: int, start_x_ptr: int, end_x_ptr: int, range_count_ptr: int = 0,
+                               range_start_ptr: int = 0, range_end_ptr: int = 0, stream: int = 0):
+        """Tape RF-envelope dropout detection of the records (VhsFieldV1) of a vhs field
+        decode (every pointer device memory; asynchronous on `stream`): .tbc.json rows per
+        record at [record][capacity]."""
         v = ctypes.c_void_p
         _lib.check(self._L.vhsdecode_hip_detect_dropouts_device(
-            self._h, v(envelope_ptr), span_length, v(locations_ptr), location_count, v(line0_ptr), v(first_ptr),
-            v(count_ptr), max_fields, ctypes.byref(params), v(row_count_ptr), v(field_line_ptr), v(start_x_ptr),
-            v(end_x_ptr), v(range_count_ptr or None), v(range_start_ptr or None), v(range_end_ptr or None),
-            v(stream)))
+            self._h, v(envelope_ptr), span_length, v(locations_ptr), location_count, v(fields_ptr), v(count_ptr),
+            max_fields, ctypes.byref(params), v(row_count_ptr), v(field_line_ptr), v(start_x_ptr), v(end_x_ptr),
+            v(range_count_ptr or None), v(range_start_ptr or None), v(range_end_ptr or None), v(stream)))
 
     def set_profiling(self, enable: bool = True):
         _lib.check(self._L.vhsdecode_hip_set_profiling(self._h, 1 if enable else 0))

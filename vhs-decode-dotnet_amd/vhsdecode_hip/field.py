@@ -1,22 +1,18 @@
-"""Field decode over the HIP backend: span demod -> sync pulses -> line locations ->
-hsync refinement -> TBC resampling to 16-bit luma fields.
+"""Field decode over the HIP backend: the vhs field sequence of a demodulated span
+(read windows, sync levels, pulses, parity, line 0, line locations, hsync refinement
+and TBC resampling to 16-bit luma fields), all on the device.
 
-Mirrors the order of TbcFieldDecodePipeline.DecodePrepared
-(src/VHSDecode.Core/Decode/TbcFieldDecodePipeline.cs:1071-1805): FindRawPulses
-(:2375-2384), line locations, RefineLineLocationsFromHSync (:1344-1348) and the
-TBC render (Tbc/TbcFieldRenderer.cs:289-340, TbcLineResampler.ResamplePreparedToUInt16).
-
-Line locations: the reference derives them with SyncAnalyzer.EstimateTiming /
-ClassifyPulses / RefinePulses, ResolveLine0Location and BuildUpstreamLineLocations
-(Dsp/SyncAnalysis.cs:131-, TbcFieldDecodePipeline.cs:1176-1295), thousands of lines
-of sequential heuristics for damaged tape.  `line_locations_from_pulses` is this
-backend's own compact equivalent for well-formed sync: pulses are classified by
-length, each vertical-sync group of broad pulses anchors a field, its parity comes
-from the half-line phase of the broad group against the preceding hsync, and every
-line start is snapped to the pulse that opens it (lines no pulse opens are
-interpolated from their measured neighbours and marked `filled`).  Its agreement with the reference's sync
-analysis on irregular signals is unpinned; on the synthetic tapes it is checked
-against the generator's closed-form line starts.
+Mirrors TbcFieldDecodePipeline.DecodePrepared
+(src/VHSDecode.Core/Decode/TbcFieldDecodePipeline.cs:1071-1420) field after field as
+TbcFieldSequenceDecodeEngine reads them (Decode/TbcFieldSequenceDecodeEngine.cs:161-171,
+804-846): PrepareSyncSpan's level detection (:2776-3234), FindRawPulses,
+SyncAnalyzer.EstimateTiming / RefinePulses / ComputeMeanLineLength
+(Dsp/SyncAnalysis.cs:131-538), FieldParityDetector.ResolveCadence, ResolveLine0Location
+(:5452-5651), BuildUpstreamLineLocations (SyncAnalysis.cs:585-657),
+RefineLineLocationsFromHSync (:1344-1348) and the TBC render
+(Tbc/TbcFieldRenderer.cs:289-340, TbcLineResampler.ResamplePreparedToUInt16).  The
+serration candidate search of VsyncSerrationDetector is not run (its measurements are
+taken as absent); the rest is the reference's arithmetic.
 """
 from __future__ import annotations
 
@@ -27,17 +23,6 @@ from typing import List, Optional
 import numpy as np
 
 from .formats import FormatParameters
-
-SNAP_TOLERANCE = 0.12   # lines
-LEAD_LINES = 8.0        # read-window lead before line 0
-
-
-@dataclass
-class FieldLines:
-    line0: float                 # span position of line 0's sync leading edge (pulse start)
-    locations: np.ndarray        # FP64 [lines + 1]
-    filled: np.ndarray           # bool [lines + 1], True where no pulse opened the line
-    first_field: bool
 
 
 @dataclass
@@ -79,63 +64,6 @@ def sync_geometry(fmt: FormatParameters, sample_rate_hz: float = 40e6) -> SyncGe
         conv=(ire0, hz_ire, int(sp["outputZero"]), vsync_ire, scale))
 
 
-def line_locations_from_pulses(starts: np.ndarray, lens: np.ndarray, g: SyncGeometry, span_len: int,
-                               tol: float = SNAP_TOLERANCE) -> List[FieldLines]:
-    starts = np.asarray(starts, dtype=np.float64)
-    lens = np.asarray(lens, dtype=np.float64)
-    out: List[FieldLines] = []
-    if starts.size == 0:
-        return out
-    broad = lens > 0.5 * (g.hsync + g.vsync)
-    hsync = (lens > 0.5 * (g.eq + g.hsync)) & ~broad
-    hs_idx = np.flatnonzero(hsync)
-    nl = g.field_lines
-    n = np.arange(nl + 1, dtype=np.float64)
-    for i in np.flatnonzero(broad & np.concatenate([[True], ~broad[:-1]])):
-        k = np.searchsorted(hs_idx, i) - 1
-        if k < 0:
-            continue
-        d = (starts[i] - starts[hs_idx[k]]) / g.line_len
-        mid_line = abs(d - np.round(d)) > 0.25
-        line0 = starts[i] - (0.5 * g.line_len if mid_line else 0.0)
-        if line0 + (nl + 1) * g.line_len >= span_len:
-            continue
-        pred = line0 + n * g.line_len
-        j = np.clip(np.searchsorted(starts, pred), 1, starts.size - 1)
-        left, right = starts[j - 1], starts[j]
-        best = np.where(np.abs(left - pred) <= np.abs(right - pred), left, right)
-        hit = np.abs(best - pred) <= tol * g.line_len
-        if not hit.any():
-            continue
-        # lines no pulse opened: interpolate between the measured neighbours
-        locs = np.where(hit, best, np.interp(n, n[hit], best[hit] - n[hit] * g.line_len) + n * g.line_len)
-        out.append(FieldLines(line0, locs, ~hit, first_field=not mid_line))
-    return out
-
-
-def field_origin(line0: float, g: SyncGeometry, block_length: int = 32768, block_cut: int = 1024,
-                 stride: int = 30720, lead_lines: float = LEAD_LINES) -> int:
-    """Start of a field's read window in span coordinates, placed the way
-    DecodeReadWindowPlanner.Resolve places it (Decode/DecodeReadWindowPlanner.cs:37-63):
-    the intended start (here `lead_lines` before line 0) less the block cut, aligned
-    down to a block, expressed as a multiple of the block stride.  TBC source
-    coordinates count from it, so their FP32 resolution matches the reference's."""
-    intended = max(0.0, line0 - lead_lines * g.line_len)
-    read = max(0, int(intended) - block_cut)
-    aligned = (read // block_length) * block_length
-    return (aligned // stride) * stride
-
-
-def locator(g: SyncGeometry, block_length: int = 32768, block_cut: int = 1024, stride: int = 30720):
-    """The device line locator's parameters (vhsdecode_hip_locator_v1) for geometry g;
-    the same thresholds line_locations_from_pulses applies."""
-    from . import _lib
-    import ctypes
-    return _lib.LocatorV1(ctypes.sizeof(_lib.LocatorV1), g.min_pulse, g.max_pulse, g.line_len,
-                          0.5 * (g.hsync + g.vsync), 0.5 * (g.eq + g.hsync), SNAP_TOLERANCE, LEAD_LINES,
-                          block_length, block_cut, stride)
-
-
 def vhs_sync_params(fmt: FormatParameters, sample_rate_hz: float = 40e6, block_length: int = 32768,
                     block_cut: int = 1024, stride: int = 30720):
     """vhsdecode_hip_vhs_sync_v1 of a tape format: SyncAnalyzer.FromParameters with the vhs
@@ -165,51 +93,96 @@ def read_sample_count(fmt: FormatParameters, sample_rate_hz: float = 40e6, block
 
 def dropout_params(fmt: FormatParameters, sample_rate_hz: float = 40e6, fraction: float = 0.18,
                    absolute: float = -1.0, hysteresis: float = 1.25, capacity: int = 1024, merge: int = 30,
-                   min_len: int = 10, block_length: int = 32768, block_cut: int = 1024, stride: int = 30720):
+                   min_len: int = 10):
     """vhsdecode_hip_dropout_params_v1 for a tape format: TbcDropoutDetectionOptions.DefaultDdd
-    (fraction 0.18, hysteresis 1.25; DefaultCxAdc uses 0.35), the field line counts of
-    FieldLinesForParityDetection (TbcFieldDecodePipeline.cs:728-744) and the read window of
-    DecodeReadWindowPlanner with this backend's line-0 lead."""
+    (fraction 0.18, hysteresis 1.25; DefaultCxAdc uses 0.35) and the field line counts of
+    FieldLinesForParityDetection (TbcFieldDecodePipeline.cs:728-744)."""
     from . import _lib
     g = sync_geometry(fmt, sample_rate_hz)
     lines = g.field_lines
     first, second = (lines - 1, lines) if g.is_pal else (lines, lines - 1)
     return _lib.DropoutParamsV1(ctypes.sizeof(_lib.DropoutParamsV1), fraction, absolute, hysteresis, merge, min_len,
-                                g.out_len, first, second, capacity,
-                                read_sample_count(fmt, sample_rate_hz, block_length), g.line_len, LEAD_LINES,
-                                block_length, block_cut, stride)
+                                g.out_len, first, second, capacity)
 
 
-class HipFieldDecoder:
-    """Decodes every complete field of a device-resident demodulated span:
-    pulses (device) -> line locations (host) -> batched hsync + TBC (device)."""
+# record statuses (vhsdecode_hip_vhs_field_v1.status)
+DECODED, NO_PULSES, NO_FIRST_HSYNC, INSUFFICIENT_DATA, MISPREDICTED, CAPACITY, NOT_READ = range(7)
 
-    def __init__(self, demod, fmt: FormatParameters, sample_rate_hz: float = 40e6):
+
+@dataclass
+class SpanDecode:
+    records: list                # VhsFieldV1 of every record, in sequence order
+    outputs: dict                # record index -> uint16 field (decoded records only)
+    next_begin: int              # where the sequence continues (span coordinates)
+    tables: Optional[list]       # per call: the four line tables [records][stride] (want_tables)
+
+
+class HipVhsFieldDecoder:
+    """The vhs field sequence over device-resident demodulated spans.  `decode_async`
+    enqueues one call (every step on the device, no host round trip); `decode_span`
+    drives calls until the sequence's next read window leaves the span, carrying the
+    sequence state between calls (and, with `state_shift`, between spans)."""
+
+    def __init__(self, demod, fmt: FormatParameters, sample_rate_hz: float = 40e6, block_length: int = 32768,
+                 block_cut: int = 1024, stride: int = 30720):
         self.demod = demod
         self.fmt = fmt
         self.geom = sync_geometry(fmt, sample_rate_hz)
-
-    def locate(self, span_vlp_ptr: int, span_len: int) -> List[FieldLines]:
-        g = self.geom
-        st, ln = self.demod.find_pulses(span_vlp_ptr, span_len, g.threshold, g.min_pulse, g.max_pulse)
-        self.last_pulses = (st, ln)
-        return line_locations_from_pulses(st, ln, g, span_len)
+        self.sync = vhs_sync_params(fmt, sample_rate_hz, block_length, block_cut, stride)
+        self.started = False
 
     def decode_async(self, span_video_ptr: int, span_vlp_ptr: int, span_len: int, out_ptr: int, max_fields: int,
-                     count_ptr: int, line0_ptr: int = 0, first_ptr: int = 0, stream: int = 0):
-        """Every step on the device with no host round trip; the field count lands in
-        device int32 at count_ptr (and line 0 / first-field flags, when given)."""
-        self.demod.decode_span_fields_device(span_vlp_ptr, span_video_ptr, span_len, self.geom, locator(self.geom),
-                                             max_fields, out_ptr, count_ptr, line0_ptr, first_ptr, stream)
+                     count_ptr: int, fields_ptr: int = 0, next_begin_ptr: int = 0, first_begin: int = 0,
+                     reset: bool = True, state_shift: int = 0, rounds: int = 4, stream: int = 0):
+        self.demod.decode_vhs_fields_device(span_vlp_ptr, span_video_ptr, span_len, first_begin, self.sync,
+                                            self.geom, max_fields, out_ptr, count_ptr, fields_ptr, next_begin_ptr,
+                                            reset, state_shift, rounds, stream)
+        self.started = True
 
-    def decode(self, span_video_ptr: int, span_vlp_ptr: int, span_len: int, out_ptr: int, max_fields: int,
-               want_refined: bool = False):
-        fields = self.locate(span_vlp_ptr, span_len)[:max_fields]
-        if not fields:
-            return fields, None
-        locs = np.stack([f.locations for f in fields])
-        filled = np.stack([f.filled for f in fields]).astype(np.uint8)
-        origins = np.array([field_origin(f.line0, self.geom) for f in fields], dtype=np.int64)
-        ref = self.demod.decode_fields_device(span_vlp_ptr, span_video_ptr, span_len, locs, filled, self.geom,
-                                              out_ptr, origins=origins, want_refined=want_refined)
-        return fields, ref
+    def decode_span(self, span_video_ptr: int, span_vlp_ptr: int, span_len: int, max_fields: int = 64,
+                    first_begin: int = 0, reset: bool = True, state_shift: int = 0, rounds: int = 4,
+                    want_tables: bool = False) -> SpanDecode:
+        from . import _lib, hipmem
+        g = self.geom
+        per = g.field_lines * g.out_len
+        size = ctypes.sizeof(_lib.VhsFieldV1)
+        out = hipmem.DeviceBuffer(max_fields * per * 2)
+        recs = hipmem.DeviceBuffer(max_fields * size)
+        meta = hipmem.DeviceBuffer(16)
+        records, outputs, tables = [], {}, ([] if want_tables else None)
+        begin = first_begin
+        stall = 0
+        while True:
+            self.decode_async(span_video_ptr, span_vlp_ptr, span_len, out.ptr, max_fields, meta.ptr, recs.ptr,
+                              meta.ptr + 8, begin, reset, state_shift, rounds)
+            hipmem.synchronize()
+            reset, state_shift = False, 0
+            count = int(meta.download(np.int32, 1)[0])
+            next_begin = int(meta.download(np.int64, 1, offset=8)[0])
+            n = min(count + 1, max_fields)
+            raw = recs.download(np.uint8, n * size)
+            got = [_lib.VhsFieldV1.from_buffer_copy(raw[k * size:(k + 1) * size].tobytes()) for k in range(n)]
+            fields = out.download(np.uint16, max(count, 1) * per).reshape(-1, per)
+            for k in range(count):
+                if got[k].status == DECODED:
+                    outputs[len(records) + k] = fields[k].copy()
+            if want_tables:
+                tabs = []
+                for which in range(4):
+                    ptr, stride = self.demod.vhs_field_lines_device(which)
+                    rows = hipmem.download_ptr(ptr, np.float64, max(count, 1) * stride).reshape(-1, stride)
+                    tabs.append(rows[:count])
+                tables.append(tabs)
+            records += got[:count]
+            if count >= max_fields:
+                begin = next_begin
+                continue
+            stop = got[count]
+            if stop.status == CAPACITY:
+                raise RuntimeError("a read window holds more sync edges or pulses than the device lists")
+            if stop.status == NOT_READ and stop.window_start + stop.window_length > span_len:
+                return SpanDecode(records, outputs, next_begin, tables)
+            stall = stall + 1 if count == 0 and next_begin == begin else 0
+            if stall > 2:
+                raise RuntimeError("the field sequence made no progress")
+            begin = next_begin

@@ -31,9 +31,11 @@ def decode_run(rf: np.ndarray, fmt: FormatParameters, filters: TapeFilterSet, ou
                sinc_table: Optional[np.ndarray] = None, dropouts: bool = True, dropout_capacity: int = 1024,
                span_samples: int = 0, maximum_output_fields: int = 0, overwrite: bool = True,
                cancel: Optional[Callable[[], bool]] = None, block_cut: int = 1024, block_cut_end: int = 1024,
-               device: int = 0) -> RunResult:
+               device: int = 0, deliver: Optional[Callable[[int, int], int]] = None) -> RunResult:
     """Decodes the whole of `rf` (int16, or float32 samples the library rounds to int16)
-    read through the library's callback; returns the run's result record."""
+    read through the library's callback; returns the run's result record.  `deliver`
+    (offset, count) -> samples, when given, bounds what each read returns (a short read
+    ends the input, or reports a cancel)."""
     L = _lib.load()
     src = np.ascontiguousarray(rf)
     if src.dtype == np.int16:
@@ -47,6 +49,8 @@ def decode_run(rf: np.ndarray, fmt: FormatParameters, filters: TapeFilterSet, ou
 
     def read(_user, dst, offset, count):
         n = max(0, min(int(count), src.size - int(offset)))
+        if deliver is not None:
+            n = max(0, min(n, int(deliver(int(offset), int(count)))))
         if n:
             ctypes.memmove(dst, base_addr + int(offset) * item, n * item)
         return n
@@ -63,13 +67,12 @@ def decode_run(rf: np.ndarray, fmt: FormatParameters, filters: TapeFilterSet, ou
                             hz_ire, 1 if g.is_pal else 0, 1, g.out_len, g.field_lines, g.nominal, g.smoothing,
                             ozero, vsync, oscale)
     stride = filters.block_length - block_cut - block_cut_end
-    loc = field.locator(g, filters.block_length, block_cut, stride)
+    sync = field.vhs_sync_params(fmt, filters.sample_rate_hz, filters.block_length, block_cut, stride)
     lut = None if sinc_table is None else np.ascontiguousarray(sinc_table, dtype=np.float32).reshape(-1)
     fm = _lib.ServeFormatV1(ctypes.sizeof(_lib.ServeFormatV1), ctypes.pointer(tables.struct), block_cut,
-                            block_cut_end, ctypes.pointer(fp), ctypes.pointer(loc),
+                            block_cut_end, ctypes.pointer(fp), ctypes.pointer(sync),
                             _ptr(lut, _lib._fp) if lut is not None else None, 0 if lut is None else lut.size)
-    dp = field.dropout_params(fmt, filters.sample_rate_hz, capacity=dropout_capacity, block_length=filters.block_length,
-                              block_cut=block_cut, stride=stride) if dropouts else None
+    dp = field.dropout_params(fmt, filters.sample_rate_hz, capacity=dropout_capacity) if dropouts else None
     cfg = _lib.RunConfigV1(ctypes.sizeof(_lib.RunConfigV1), device, ctypes.pointer(fm),
                            ctypes.pointer(dp) if dp is not None else None, src.size, fmt_code,
                            maximum_output_fields, span_samples, output_base.encode("utf-8"), 1 if overwrite else 0,

@@ -46,10 +46,10 @@ class HipServer:
                                 hz_ire, 1 if g.is_pal else 0, 1, g.out_len, g.field_lines, g.nominal, g.smoothing,
                                 ozero, vsync, oscale)
         stride = filters.block_length - block_cut - block_cut_end
-        loc = field.locator(g, filters.block_length, block_cut, stride)
+        sync = field.vhs_sync_params(fmt, filters.sample_rate_hz, filters.block_length, block_cut, stride)
         lut = None if sinc_table is None else np.ascontiguousarray(sinc_table, dtype=np.float32).reshape(-1)
         f = _lib.ServeFormatV1(ctypes.sizeof(_lib.ServeFormatV1), ctypes.pointer(tables.struct), block_cut,
-                               block_cut_end, ctypes.pointer(fp), ctypes.pointer(loc),
+                               block_cut_end, ctypes.pointer(fp), ctypes.pointer(sync),
                                _ptr(lut, _lib._fp) if lut is not None else None, 0 if lut is None else lut.size)
         fid = ctypes.c_int32()
         _lib.check(self._L.vhsdecode_hip_serve_add_format(self._h, ctypes.byref(f), ctypes.byref(fid)))

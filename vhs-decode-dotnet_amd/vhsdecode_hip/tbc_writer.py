@@ -1,7 +1,7 @@
 """Raw `.tbc` output: little-endian uint16 fields, one after another, the layout
 TbcOutputWriter writes (src/VHSDecode.Core/Tbc/TbcOutputWriter.cs:28-90) for the frame
 shapes of TbcFrameSpec (Tbc/TbcFrameSpec.cs:20-36).  The device path already produces
-fields in this layout (vhsdecode_hip_decode_span_fields_device, served windows); this
+fields in this layout (vhsdecode_hip_decode_vhs_fields_device, served windows); this
 writes them."""
 from __future__ import annotations
 
