@@ -132,16 +132,20 @@ def measured_traffic(kernel: str):
     return None
 
 
-def served_run(seconds: float, rate: float, streams: int, window_s: float, seed: int, lut, burst: int):
-    """Served windows: `streams` looped synthetic tapes (NTSC VHS SP, PAL VHS SP, PAL VHS LP
-    in turn) in pinned host memory, Poisson arrivals at `rate` requests/s of `window_s`-second
-    windows at random positions for `seconds`, then a burst of `burst` windows submitted at
-    once (served capacity).  Latency is submit -> uint16 fields in host memory."""
+def served_run(requests: int, rate: float, streams: int, window_s: float, seed: int, lut, sweep, sweep_requests: int):
+    """Served windows, open loop: `streams` looped synthetic tapes (NTSC VHS SP, PAL VHS SP,
+    PAL VHS LP in turn) in pinned host memory; `requests` seek requests for `window_s`-second
+    windows at random positions arrive as a Poisson process of `rate` per second, each
+    submitted at its scheduled time by one thread while another collects the results.  A
+    request's latency runs from its scheduled arrival to its fields in host memory (the
+    server's submit -> done time plus any lag of the submission behind the schedule).
+    Then the same at each rate of `sweep` (`sweep_requests` requests each)."""
     import threading
     from vhsdecode_hip import field, filters, formats, hipmem, synth
     from vhsdecode_hip.serve import HipServer
     fs = 40e6
     window = int(window_s * fs) // 2 * 2
+    preroll = int(0.12 * fs) + 2
     specs = [("NTSC", "VHS", "sp"), ("PAL", "VHS", "sp"), ("PAL", "VHS", "lp")]
     bases, fmts, fls = [], [], []
     for k, (system, tf, speed) in enumerate(specs):
@@ -154,85 +158,78 @@ def served_run(seconds: float, rate: float, streams: int, window_s: float, seed:
         fmts.append(fmt)
         fls.append(fl)
     max_fields = int(window_s * 60) + 2
-    srv = HipServer(max_slots=16, hbm_budget_bytes=64 << 30, max_window_samples=window, max_fields=max_fields)
+    srv = HipServer(max_slots=16, hbm_budget_bytes=64 << 30, max_window_samples=window + preroll,
+                    max_fields=max_fields)
     fids = [srv.add_format(fmts[k], fls[k], lut) for k in range(len(specs))]
     sids = [srv.add_stream(fids[i % 3], bases[i % 3].ptr, bases[i % 3].array.size, loop=True) for i in range(streams)]
     per = max(field.sync_geometry(f).field_lines * field.sync_geometry(f).out_len for f in fmts)
-    pool = [hipmem.PinnedArray(max_fields * per, np.uint16) for _ in range(16)]
-    free = list(range(len(pool)))
-    lock = threading.Condition()
+    pool = [hipmem.PinnedArray(max_fields * per, np.uint16) for _ in range(24)]
     rng = np.random.default_rng(seed)
 
-    def take():
-        with lock:
-            while not free:
-                lock.wait()
-            return free.pop()
+    def run_point(n_req, r_rate):
+        free = list(range(len(pool)))
+        cv = threading.Condition()
+        tickets = []  # (ticket, buffer, lag seconds)
+        done_t = threading.Event()
 
-    def give(b):
-        with lock:
-            free.append(b)
-            lock.notify()
+        def submitter():
+            arrivals = np.cumsum(rng.exponential(1.0 / r_rate, size=n_req))
+            sids_r = rng.integers(0, streams, size=n_req)
+            firsts = 2 * rng.integers(preroll // 2, 10**9, size=n_req)
+            t0 = time.perf_counter()
+            for i in range(n_req):
+                now = time.perf_counter() - t0
+                if arrivals[i] > now:
+                    time.sleep(arrivals[i] - now)
+                with cv:
+                    while not free:
+                        cv.wait()
+                    b = free.pop()
+                lag = (time.perf_counter() - t0) - arrivals[i]
+                t = srv.submit(sids[int(sids_r[i])], int(firsts[i]), window, max_fields, pool[b].ptr)
+                with cv:
+                    tickets.append((t, b, lag))
+                    cv.notify_all()
+            done_t.set()
+
+        th = threading.Thread(target=submitter)
+        t_start = time.perf_counter()
+        th.start()
+        lat, fields, ok, k = [], 0, True, 0
+        while k < n_req:
+            with cv:
+                while k >= len(tickets):
+                    cv.wait()
+                t, b, lag = tickets[k]
+            r = srv.wait(t)
+            ok &= r.status == 0
+            fields += r.field_count
+            lat.append(r.latency_ms + lag * 1e3)
+            with cv:
+                free.append(b)
+                cv.notify_all()
+            k += 1
+        th.join()
+        wall = time.perf_counter() - t_start
+        lat = np.array(lat)
+        return {"rate_per_s": r_rate, "requests": n_req, "all_ok": bool(ok),
+                "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                "max_ms": round(float(lat.max()), 2), "fields": int(fields),
+                "offered_msps": round(r_rate * window / 1e6, 1),
+                "aggregate_msps": round(n_req * window / wall / 1e6, 1), "seconds": round(wall, 2)}
 
     # warm every slot and format once (slot buffers, contexts and tables are created on first use)
-    warm = []
-    for k in range(16):
-        b = take()
-        warm.append((srv.submit(sids[k % len(sids)], 0, window, max_fields, pool[b].ptr), b))
-    for t, b in warm:
+    warm = [srv.submit(sids[k % len(sids)], preroll, window, max_fields, pool[k].ptr) for k in range(16)]
+    for t in warm:
         srv.wait(t)
-        give(b)
-
-    results = []
-
-    def collector(items):
-        for t, b in items:
-            r = srv.wait(t)
-            results.append(r)
-            give(b)
-
-    # Poisson arrivals
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(seconds * rate * 2) + 16))
-    arrivals = arrivals[arrivals < seconds]
-    pending = []
-    t0 = time.perf_counter()
-    for a in arrivals:
-        now = time.perf_counter() - t0
-        if a > now:
-            time.sleep(a - now)
-        sid = int(rng.integers(0, streams))
-        first = 2 * int(rng.integers(0, 10**9))
-        b = take()
-        pending.append((srv.submit(sids[sid], first, window, max_fields, pool[b].ptr), b))
-        if len(pending) >= 4:
-            collector(pending)
-            pending = []
-    collector(pending)
-    t_end = time.perf_counter()
-    lat = np.array([r.latency_ms for r in results])
-    ok = all(r.status == 0 for r in results)
-    # burst: capacity with every slot busy
-    items = []
-    tb = time.perf_counter()
-    for i in range(burst):
-        b = take() if i < len(pool) else None
-        if b is None:
-            collector(items)
-            items = []
-            b = take()
-        items.append((srv.submit(sids[i % streams], 2 * int(rng.integers(0, 10**9)), window, max_fields,
-                                 pool[b].ptr), b))
-    collector(items)
-    burst_s = time.perf_counter() - tb
+    head = run_point(requests, rate)
+    points = [run_point(sweep_requests, r) for r in sweep]
     srv.close()
     return {
-        "streams": streams, "rate_per_s": rate, "window_s": window_s, "window_samples": window,
-        "requests": int(lat.size), "all_ok": bool(ok),
-        "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
-        "offered_msps": round(rate * window / 1e6, 1),
-        "aggregate_msps": round(lat.size * window / (t_end - t0) / 1e6, 1),
-        "burst_windows": burst, "burst_capacity_msps": round(burst * window / burst_s / 1e6, 1),
+        "streams": streams, "window_s": window_s, "window_samples": window, "preroll_samples": preroll,
+        "max_fields": max_fields, **head, "sweep": points,
         "formats": ["NTSC VHS SP", "PAL VHS SP", "PAL VHS LP"], "hbm_budget_gb": 64,
+        "latency": "scheduled arrival -> uint16 fields in host memory, open-loop Poisson arrivals",
         "source": "pinned host RAM, looped 8-field synthetic tapes (30 dB, 20 ns jitter)",
     }
 
@@ -412,11 +409,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serve-seconds", type=float, default=12.0, help="served-window run length (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=1000, help="served-window requests (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=8.0)
     ap.add_argument("--serve-streams", type=int, default=16)
     ap.add_argument("--serve-window", type=float, default=2.0, help="window length (s of RF)")
-    ap.add_argument("--serve-burst", type=int, default=32)
+    ap.add_argument("--serve-sweep", type=str, default="16,32,64", help="further arrival rates (req/s)")
+    ap.add_argument("--serve-sweep-requests", type=int, default=200)
     ap.add_argument("--no-dropouts", action="store_true", help="skip the per-field dropout detection")
     ap.add_argument("--pipeline", type=int, default=3,
                     help="spans in flight: consecutive steps alternate between this many pipelines")
@@ -606,14 +604,17 @@ def main():
         result["laserdisc"] = ldr
         print(f"[bench] laserdisc: {ldr['msps']} MSPS; hifi {ldr['hifi']['if_msps']} IF MSPS",
               file=sys.stderr)
-    if args.serve_seconds > 0:
+    if args.serve_requests > 0:
         for p in pipes:
             p.dem.close()
-        sv = served_run(args.serve_seconds, args.serve_rate, args.serve_streams, args.serve_window, SEED, lut,
-                        args.serve_burst)
+        sweep = [float(x) for x in args.serve_sweep.split(",") if x.strip()]
+        sv = served_run(args.serve_requests, args.serve_rate, args.serve_streams, args.serve_window, SEED, lut,
+                        sweep, args.serve_sweep_requests)
         result["serving"] = sv
-        print(f"[bench] served: {sv['requests']} windows p50 {sv['p50_ms']} ms p99 {sv['p99_ms']} ms, "
-              f"burst capacity {sv['burst_capacity_msps']} MSPS", file=sys.stderr)
+        print(f"[bench] served: {sv['requests']} windows at {sv['rate_per_s']}/s p50 {sv['p50_ms']} ms "
+              f"p99 {sv['p99_ms']} ms; sweep " + ", ".join(
+                  f"{p['rate_per_s']}/s p99 {p['p99_ms']} ms {p['aggregate_msps']} MSPS" for p in sv["sweep"]),
+              file=sys.stderr)
     if not args.no_cpu_baseline:
         nproc = os.cpu_count() or 1
         usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
