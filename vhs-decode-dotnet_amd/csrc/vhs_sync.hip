@@ -30,6 +30,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <climits>
+#include <cmath>
+
 #include "vhs_sync.hpp"
 
 namespace vhsd {
@@ -39,7 +42,6 @@ constexpr int kT = 1024;   // threads of the per-field workgroups
 constexpr int kRun = 16;   // samples per thread per pulse tile
 constexpr int kTile = kT * kRun;
 constexpr int kMaxMeans = 256;    // pulse means per level refinement
-constexpr int kMaxLeaves = 4096;  // pairwise-sum leaves per level refinement
 
 __device__ __forceinline__ double round_away(double x) { return round(x); }
 __device__ __forceinline__ double round_even(double x) { return rint(x); }
@@ -73,7 +75,7 @@ __device__ int wg_exclusive(int v, int* red, int* total) {
   if (lane == 63) red[wave] = incl;
   __syncthreads();
   int before = 0, tot = 0;
-  for (int w = 0; w < kT / 64; w++) {
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
     const int x = red[w];
     if (w < wave) before += x;
     tot += x;
@@ -89,7 +91,7 @@ __device__ double wg_min(double v, double* dred) {
   if (lane == 0) dred[wave] = v;
   __syncthreads();
   double m = dred[0];
-  for (int w = 1; w < kT / 64; w++) m = fmin(m, dred[w]);
+  for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmin(m, dred[w]);
   __syncthreads();
   return m;
 }
@@ -100,7 +102,7 @@ __device__ long long wg_sum_ll(long long v, long long* lred) {
   if (lane == 0) lred[wave] = v;
   __syncthreads();
   long long s = 0;
-  for (int w = 0; w < kT / 64; w++) s += lred[w];
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += lred[w];
   __syncthreads();
   return s;
 }
@@ -157,7 +159,7 @@ __device__ int wg_find_pulses(const Plane& x, double high, int min_len, int max_
   }
   __syncthreads();
   int vs_count = 0;
-  for (long long base = 0; base < x.n; base += kTile) {
+  for (long long base = 0; base < x.n; base += (long long)blockDim.x * kRun) {
     const long long i0 = base + (long long)tid * kRun;
     unsigned codes = 0;
     if (i0 < x.n) {
@@ -229,7 +231,7 @@ __device__ int wg_find_pulses(const Plane& x, double high, int min_len, int max_
       if (lane == 0) ps->wlast[tid >> 6] = m;
       __syncthreads();
       last_all = -1;
-      for (int w = 0; w < kT / 64; w++) last_all = last_all > ps->wlast[w] ? last_all : ps->wlast[w];
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) last_all = last_all > ps->wlast[w] ? last_all : ps->wlast[w];
     }
     __syncthreads();
     if (tid == 0) {
@@ -248,37 +250,37 @@ __device__ int wg_find_pulses(const Plane& x, double high, int min_len, int max_
 
 // ------------------------------------------------------------------ pairwise means
 // NumpyReduction.MeanFloat64 (Dsp/NumpyReduction.cs:226-227, 400-451) of several ranges:
-// the recursion's leaves (<= 128 values) are summed in parallel, then each range's tree is
-// combined in the recursion's order by one thread.
+// numpy's pairwise sum splits a block longer than 128 at half its length rounded down to
+// a multiple of 8, sums a leaf with eight accumulators, and adds the halves in order.  One
+// wave per range: lane l sums the leaf at path l of the tree cut at depth kMeanDepth (the
+// leftmost path under a shallower leaf sums it), then the lanes combine the tree level by
+// level in the recursion's order.  Ranges whose tree is deeper than kMeanDepth (longer
+// than ~7000 samples) are summed by one lane in recursion order.
+constexpr int kMeanDepth = 6;
 struct MeanScratch {
-  int leaf_start[kMaxLeaves];
-  short leaf_len[kMaxLeaves];
-  double leaf_sum[kMaxLeaves];
-  int n_leaves;
+  int unused;
 };
 
-__device__ void collect_leaves(int s, int n, MeanScratch* m) {
-  // iterative depth-first split, left before right
-  int stack_s[32], stack_n[32], sp = 0;
-  stack_s[sp] = s;
-  stack_n[sp] = n;
-  sp++;
-  while (sp > 0) {
-    sp--;
-    const int a = stack_s[sp], len = stack_n[sp];
-    if (len > 128) {
-      int split = len / 2;
-      split -= split % 8;
-      stack_s[sp] = a + split;
-      stack_n[sp] = len - split;
-      sp++;
-      stack_s[sp] = a;
-      stack_n[sp] = split;
-      sp++;
-    } else if (m->n_leaves < kMaxLeaves) {
-      m->leaf_start[m->n_leaves] = a;
-      m->leaf_len[m->n_leaves] = (short)len;
-      m->n_leaves++;
+// Node of the pairwise tree of n values along path bits `path` at `level`; `owner` is true
+// for the leftmost path under a leaf at or above that level (the one that sums it).
+__device__ __forceinline__ void pw_path(int n, int level, unsigned path, int& off, int& len, bool& owner,
+                                        bool& exists) {
+  off = 0;
+  len = n;
+  owner = true;
+  exists = true;
+  for (int l = 0; l < level; l++) {
+    if (len <= 128) {
+      owner = (path & ((1u << (level - l)) - 1)) == 0;
+      exists = false;
+      return;
+    }
+    const int split = (len >> 1) & ~7;
+    if ((path >> (level - 1 - l)) & 1u) {
+      off += split;
+      len -= split;
+    } else {
+      len = split;
     }
   }
 }
@@ -301,35 +303,70 @@ __device__ double leaf_sum(const Plane& x, int s, int n) {
   return t;
 }
 
-// combine the leaves of one range (they are consecutive from *cursor, in depth-first order)
-__device__ double combine(int n, const MeanScratch* m, int* cursor) {
+// the recursion itself, for one lane (deep trees only)
+__device__ double pairwise_serial(const Plane& x, int s, int n) {
 #pragma clang fp contract(off)
-  if (n > 128) {
-    int split = n / 2;
-    split -= split % 8;
-    const double l = combine(split, m, cursor);
-    const double r = combine(n - split, m, cursor);
-    return l + r;
+  // explicit stack of pending right halves; partial sums combine as the recursion does
+  int st_s[40], st_n[40], st_state[40];
+  double st_left[40];
+  int sp = 0;
+  int cs = s, cn = n;
+  double result = 0.0;
+  for (;;) {
+    while (cn > 128) {
+      int split = cn / 2;
+      split -= split % 8;
+      st_s[sp] = cs + split;
+      st_n[sp] = cn - split;
+      st_state[sp] = 0;
+      sp++;
+      cn = split;
+    }
+    result = leaf_sum(x, cs, cn);
+    // unwind: a pending right half gets summed next; a finished one adds to its left
+    while (sp > 0 && st_state[sp - 1] == 1) {
+      sp--;
+      result = st_left[sp] + result;
+    }
+    if (sp == 0) return result;
+    st_state[sp - 1] = 1;
+    st_left[sp - 1] = result;
+    cs = st_s[sp - 1];
+    cn = st_n[sp - 1];
   }
-  return m->leaf_sum[(*cursor)++];
 }
 
-// means[k] = mean of x[rs[k], re[k]) for k < count (all threads call; count <= kMaxMeans)
+// means[k] = mean of x[rs[k], re[k]) for k < count (all threads call)
 __device__ void wg_means(const Plane& x, const int* rs, const int* re, int count, double* means, MeanScratch* m) {
-  if (threadIdx.x == 0) {
-    m->n_leaves = 0;
-    for (int k = 0; k < count; k++) collect_leaves(rs[k], re[k] - rs[k], m);
-  }
-  __syncthreads();
-  for (int l = threadIdx.x; l < m->n_leaves; l += kT) m->leaf_sum[l] = leaf_sum(x, m->leaf_start[l], m->leaf_len[l]);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int cursor = 0;
-    for (int k = 0; k < count; k++) {
-      const int n = re[k] - rs[k];
-      const double s = combine(n, m, &cursor);
-      means[k] = s / (double)n;
+#pragma clang fp contract(off)
+  (void)m;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int k = wave; k < count; k += (int)(blockDim.x >> 6)) {
+    const int s = rs[k], n = re[k] - rs[k];
+    double total;
+    if (n > 7000) {
+      total = lane == 0 ? pairwise_serial(x, s, n) : 0.0;
+      total = __shfl(total, 0);
+    } else {
+      int off, len;
+      bool owner, exists;
+      pw_path(n, kMeanDepth, (unsigned)lane, off, len, owner, exists);
+      // lane l holds the value of the tree node whose slot it is (slot of a node at
+      // level L along path t: t << (kMeanDepth - L))
+      double val = owner ? leaf_sum(x, s + off, len) : 0.0;
+      for (int level = kMeanDepth - 1; level >= 0; level--) {
+        const int sh = kMeanDepth - level;
+        const double right = __shfl(val, (lane + (1 << (sh - 1))) & 63);
+        if ((lane & ((1 << sh) - 1)) == 0) {
+          int o2, l2;
+          bool ow2, ex2;
+          pw_path(n, level, (unsigned)(lane >> sh), o2, l2, ow2, ex2);
+          if (ex2 && l2 > 128) val = val + right;
+        }
+      }
+      total = __shfl(val, 0);
     }
+    if (lane == 0) means[k] = total / (double)n;
   }
   __syncthreads();
 }
@@ -376,7 +413,7 @@ __device__ bool wg_check_levels(const Plane& x, double old_sync, double new_sync
   if ((ref_sync - new_sync) > (hz_ire * 15.0) || diff > 47.0) return false;
   if (new_sync - old_sync < (hz_ire * 5.0)) return true;
   long long bs = 0, bb = 0;
-  for (long long i = threadIdx.x; i < x.n; i += kT) {
+  for (long long i = threadIdx.x; i < x.n; i += blockDim.x) {
     const double v = x.at(i);
     if (v < new_sync) bs++;
     if (v < new_blank) bb++;
@@ -411,7 +448,7 @@ __device__ RefineOut wg_refine_from_pulses(const Plane& x, const int* starts, co
   if (threadIdx.x == 0) r->nv = 0;
   __syncthreads();
   // vsync-length pulses, in order: ordered compaction
-  for (int b0 = 0; b0 < npulses; b0 += kT) {
+  for (int b0 = 0; b0 < npulses; b0 += blockDim.x) {
     const int k = b0 + threadIdx.x;
     bool ok = false;
     int s = 0, e = 0;
@@ -437,7 +474,7 @@ __device__ RefineOut wg_refine_from_pulses(const Plane& x, const int* starts, co
   // the last vsync pulse: the largest k written (recompute exactly)
   if (threadIdx.x == 0) r->flag = -1;
   __syncthreads();
-  for (int k = threadIdx.x; k < npulses; k += kT) {
+  for (int k = threadIdx.x; k < npulses; k += blockDim.x) {
     const int len = lens[k];
     if (!(len <= lo || len >= hi)) {
       const int s = starts[k] + off, e = starts[k] + len - off;
@@ -472,7 +509,7 @@ __device__ RefineOut wg_refine_from_pulses(const Plane& x, const int* starts, co
   const int b0i = after < last_index - 1 ? after + 1 : 0;
   const int b1i = after < last_index - 1 ? (after + 6 > last_index ? after + 6 : last_index) : 0;
   const int na = a1 - a0, nb = b1i - b0i;
-  for (int base = 0; base < na + nb; base += kT) {
+  for (int base = 0; base < na + nb; base += blockDim.x) {
     const int q = base + threadIdx.x;
     bool ok = false;
     int s = 0, e = 0;
@@ -537,15 +574,274 @@ __device__ RefineOut wg_refine_levels(const Plane& x, double s0, double b0, cons
   return wg_refine_from_pulses(x, pstart, plen, np < pcap ? np : pcap, c, ref_sync, r);
 }
 
+// ------------------------------------------------------------------ segmented pulse search
+// The pulse searches over whole read windows split into segments of kSegLen samples, one
+// 256-thread workgroup each, so the fields' windows keep every CU busy.  A segment's
+// summary holds its first transition (and whether it ends a pulse), its last transition
+// and the pulses (in range) whose start and end both lie in it; combining the summaries
+// in order adds the pulses that cross a segment boundary (a segment's first transition
+// ending a pulse whose start is the last transition before the segment) and gives each
+// segment its output offset and the transition carried into it, with which a second
+// pass writes the pulses in order.  Counts, lists and pairings are those of
+// wg_find_pulses over the whole window.
+constexpr int kSegT = 256;
+constexpr int kSegTiles = 4;
+static_assert(kSegLen == kSegT * kRun * kSegTiles, "segment length");
+
+struct SegScratch {
+  int red[kSegT / 64 + 4];
+  int wlast[kSegT / 64];
+  int wfirst[kSegT / 64];
+  int offset;
+  int carry;
+  int first;
+};
+
+template <int NT>
+__device__ int wgt_exclusive(int v, int* red, int* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int incl = wave_scan_incl(v, lane);
+  if (lane == 63) red[wave] = incl;
+  __syncthreads();
+  int before = 0, tot = 0;
+  for (int w = 0; w < NT / 64; w++) {
+    const int x = red[w];
+    if (w < wave) before += x;
+    tot += x;
+  }
+  __syncthreads();
+  *total = tot;
+  return before + incl - v;
+}
+
+// Segment [s0, s1) of the window at threshold `high`: its summary (every thread), and
+// with `starts` the pulses paired from `carry_in` on, written from `out_base` (below cap).
+__device__ SegSummary seg_scan(const Plane& x, long long s0, long long s1, double high, int min_len, int max_len,
+                               double vs_min, int carry_in, int* starts, int* lens, int cap, int out_base,
+                               SegScratch* ps) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) {
+    ps->offset = 0;
+    ps->carry = carry_in;
+    ps->first = -1;
+  }
+  __syncthreads();
+  int n_in = 0, n_vs = 0;
+  int first_end = 0;
+  for (long long base = s0; base < s1; base += kSegT * kRun) {
+    const long long i0 = base + (long long)tid * kRun;
+    unsigned codes = 0;
+    if (i0 < s1) {
+      int st = i0 > 0 ? state_before(x, i0 - 1, high) : 0;
+#pragma unroll
+      for (int k = 0; k < kRun; k++) {
+        const long long i = i0 + k;
+        if (i < s1) {
+          const float v = x.p[i];
+          if (v == v) {
+            const int cur = x.at(i) <= high ? 1 : 0;
+            if (i >= 1) codes |= ((!st && cur) ? 1u : ((st && !cur) ? 2u : 0u)) << (2 * k);
+            st = cur;
+          }
+        }
+      }
+    }
+    const int mine = codes ? (int)(i0 + ((31 - __clz(codes)) >> 1)) : -1;
+    const int myfirst = codes ? (int)(i0 + ((__ffs(codes) - 1) >> 1)) : INT_MAX;
+    // the transition before this run: the latest of the earlier threads', or the carry
+    int incl = mine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(incl, o);
+      if (lane >= o) incl = incl > u ? incl : u;
+    }
+    int fmin = myfirst;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const int u = __shfl_xor(fmin, o);
+      fmin = fmin < u ? fmin : u;
+    }
+    if (lane == 63) ps->wlast[wave] = incl;
+    if (lane == 0) ps->wfirst[wave] = fmin;
+    __syncthreads();
+    int before = ps->carry;
+    for (int w = 0; w < wave; w++) before = before > ps->wlast[w] ? before : ps->wlast[w];
+    const int up = __shfl_up(incl, 1);
+    const int prev = lane > 0 ? (before > up ? before : up) : before;
+    int tile_last = ps->carry, tile_first = INT_MAX;
+    for (int w = 0; w < kSegT / 64; w++) {
+      tile_last = tile_last > ps->wlast[w] ? tile_last : ps->wlast[w];
+      tile_first = tile_first < ps->wfirst[w] ? tile_first : ps->wfirst[w];
+    }
+    int cnt = 0;
+    unsigned rest = codes;
+    int pv = prev;
+    while (rest) {
+      const int k = (__ffs(rest) - 1) >> 1;
+      const int pos = (int)(i0 + k);
+      const bool end = ((rest >> (2 * k)) & 3u) == 2u;
+      if (pos == tile_first && ps->first < 0) first_end = end ? 1 : 0;
+      if (end && pv > 0) {
+        const int len = pos - pv;
+        if (in_range((double)len, min_len, max_len)) {
+          cnt++;
+          if (!starts && (double)len > vs_min) n_vs++;
+        }
+      }
+      pv = pos;
+      rest &= ~(3u << (2 * k));
+    }
+    n_in += cnt;
+    int tot = 0;
+    if (starts) {
+      int idx = wgt_exclusive<kSegT>(cnt, ps->red, &tot) + ps->offset + out_base;
+      rest = codes;
+      pv = prev;
+      while (rest && cnt) {
+        const int k = (__ffs(rest) - 1) >> 1;
+        const int pos = (int)(i0 + k);
+        if (((rest >> (2 * k)) & 3u) == 2u && pv > 0) {
+          const int len = pos - pv;
+          if (in_range((double)len, min_len, max_len)) {
+            if (idx < cap) {
+              starts[idx] = pv;
+              lens[idx] = len;
+            }
+            idx++;
+          }
+        }
+        pv = pos;
+        rest &= ~(3u << (2 * k));
+      }
+    }
+    // the segment's first transition, the offset and the carry into the next tile
+    const bool set_first = tile_first != INT_MAX && ps->first < 0;
+    __syncthreads();
+    if (tid == 0) {
+      if (set_first) ps->first = tile_first;
+      ps->offset += tot;
+      ps->carry = tile_last;
+    }
+    __syncthreads();
+  }
+  // reduce the per-thread counts (count mode) and the first transition's kind
+  int tot_in, tot_vs, tot_fe;
+  (void)wgt_exclusive<kSegT>(n_in, ps->red, &tot_in);
+  (void)wgt_exclusive<kSegT>(n_vs, ps->red, &tot_vs);
+  (void)wgt_exclusive<kSegT>(first_end, ps->red, &tot_fe);
+  SegSummary sm;
+  sm.first_pos = ps->first;
+  sm.first_end = tot_fe > 0 ? 1 : 0;
+  sm.last_pos = ps->first < 0 ? -1 : ps->carry;
+  sm.n_in = tot_in;
+  sm.n_vs = tot_vs;
+  __syncthreads();
+  return sm;
+}
+
+// Pulse search jobs over the fields' windows: thresholds [field][R], the DC added to the
+// plane, length bounds and summaries [field][R][segments].
+struct SegJob {
+  const double* thr;
+  int R;
+  const double* dc;        // per field, or null
+  int min_len, max_len;
+  double vs_min;
+  const int* active;       // per field: run the job (null: every field)
+  SegSummary* sum;
+  int segs;                // segments per window
+  const int* choice;       // per field: the threshold index the list is written at
+  int* seg_carry;          // [field][segments]
+  int* seg_off;
+  int* count;              // per field: pulses of the list
+  int* starts;             // [field][pulse_cap]
+  int* lens;
+};
+
+__device__ __forceinline__ Plane job_plane(const SyncCfg& c, const SyncWork& w, const SegJob& j, int f) {
+  return Plane{w.sync_ref + w.win_start[f], w.win_len[f], c.bias, j.dc ? j.dc[f] : 0.0};
+}
+
+__global__ void __launch_bounds__(kSegT) seg_count_kernel(SyncCfg c, SyncWork w, SegJob j) {
+  __shared__ SegScratch ps;
+  const int sg = blockIdx.x, r = blockIdx.y, f = blockIdx.z;
+  if (f >= *w.spec_count || (j.active && !j.active[f])) return;
+  const Plane x = job_plane(c, w, j, f);
+  const long long s0 = (long long)sg * kSegLen;
+  const long long s1 = s0 + kSegLen < x.n ? s0 + kSegLen : x.n;
+  SegSummary sm{-1, 0, -1, 0, 0};
+  if (s0 < x.n)
+    sm = seg_scan(x, s0, s1, j.thr[(long long)f * j.R + r], j.min_len, j.max_len, j.vs_min, -1, nullptr, nullptr,
+                  0, 0, &ps);
+  if (threadIdx.x == 0) j.sum[((long long)f * j.R + r) * j.segs + sg] = sm;
+}
+
+// Segment summaries of one threshold, in order: the pulse count, the long-pulse count
+// and (with seg_off) each segment's offset and the transition carried into it.
+__device__ void seg_combine(const SegJob& j, int f, int r, int* np, int* nvs, int* seg_carry, int* seg_off) {
+  int last = -1, count = 0, vs = 0;
+  const SegSummary* sm = j.sum + ((long long)f * j.R + r) * j.segs;
+  for (int sg = 0; sg < j.segs; sg++) {
+    if (seg_off) {
+      seg_carry[sg] = last;
+      seg_off[sg] = count;
+    }
+    const SegSummary a = sm[sg];
+    if (a.first_pos < 0) continue;
+    if (a.first_end && last > 0) {
+      const int len = a.first_pos - last;
+      if (in_range((double)len, j.min_len, j.max_len)) {
+        count++;
+        if ((double)len > j.vs_min) vs++;
+      }
+    }
+    count += a.n_in;
+    vs += a.n_vs;
+    last = a.last_pos;
+  }
+  *np = count;
+  if (nvs) *nvs = vs;
+}
+
+// One threshold per field (j.choice, or 0): the list's segment offsets and its count.
+__global__ void seg_plan_kernel(SyncCfg c, SyncWork w, SegJob j, int max_fields) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= max_fields || f >= *w.spec_count || (j.active && !j.active[f])) return;
+  const int r = j.choice ? j.choice[f] : 0;
+  if (r < 0) return;
+  int np;
+  seg_combine(j, f, r, &np, nullptr, j.seg_carry + (long long)f * j.segs, j.seg_off + (long long)f * j.segs);
+  j.count[f] = np;
+}
+
+__global__ void __launch_bounds__(kSegT) seg_write_kernel(SyncCfg c, SyncWork w, SegJob j) {
+  __shared__ SegScratch ps;
+  const int sg = blockIdx.x, f = blockIdx.y;
+  if (f >= *w.spec_count || (j.active && !j.active[f])) return;
+  const int r = j.choice ? j.choice[f] : 0;
+  if (r < 0) return;
+  const Plane x = job_plane(c, w, j, f);
+  const long long s0 = (long long)sg * kSegLen;
+  if (s0 >= x.n) return;
+  const long long s1 = s0 + kSegLen < x.n ? s0 + kSegLen : x.n;
+  const long long fo = (long long)f * j.segs + sg;
+  (void)seg_scan(x, s0, s1, j.thr[(long long)f * j.R + r], j.min_len, j.max_len, 1e300, j.seg_carry[fo],
+                 j.starts + (long long)f * c.pulse_cap, j.lens + (long long)f * c.pulse_cap, c.pulse_cap,
+                 j.seg_off[fo], &ps);
+}
+
 // ------------------------------------------------------------------ S1: per-field level search
 // LevelDetection.SearchFallbackSerrationLevels (:265-396, divisor 1, no long-pulse check)
 // then RefineSerrationLevels at the found levels: both depend only on the window.
+// The whole search of a field on one workgroup: the fields whose threshold sequence runs
+// past the kSpecR thresholds evaluated at once (it runs to 30 on windows without usable
+// sync).
 __global__ void __launch_bounds__(kT) level_search_kernel(SyncCfg c, SyncWork w) {
 #pragma clang fp contract(off)
   __shared__ RefineScratch r;
   __shared__ double dred[kT / 64];
   const int f = blockIdx.x;
-  if (f >= *w.spec_count) return;
+  if (f >= *w.spec_count || !w.seg.fallback[f]) return;
   const Plane x{w.sync_ref + w.win_start[f], w.win_len[f], c.bias, 0.0};
   int* pst = w.pulse_start + (long long)f * c.pulse_cap;
   int* pln = w.pulse_len + (long long)f * c.pulse_cap;
@@ -623,39 +919,201 @@ __global__ void __launch_bounds__(kT) level_search_kernel(SyncCfg c, SyncWork w)
   if (threadIdx.x == 0) w.search[f] = out;
 }
 
+// The level search of every field through segment workgroups: the window minimum, the
+// counts at the first kSpecR thresholds of the search sequence at once, the sequence's
+// decisions (a field whose sequence needs more thresholds goes to level_search_kernel),
+// the list at the chosen threshold, the first refinement, the list at its midpoint and
+// the second refinement.
+__global__ void __launch_bounds__(kSegT) seg_min_kernel(SyncCfg c, SyncWork w) {
+  __shared__ double dred[kSegT / 64];
+  const int sg = blockIdx.x, f = blockIdx.y;
+  if (f >= *w.spec_count) return;
+  const Plane x{w.sync_ref + w.win_start[f], w.win_len[f], c.bias, 0.0};
+  const long long s0 = (long long)sg * kSegLen;
+  const long long s1 = s0 + kSegLen < x.n ? s0 + kSegLen : x.n;
+  double mn = INFINITY;
+  bool nan = false;
+  for (long long i = s0 + threadIdx.x; i < s1; i += kSegT) {
+    const double v = x.at(i);
+    if (v != v) nan = true;
+    mn = fmin(mn, v);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) mn = fmin(mn, __shfl_xor(mn, o));
+  if (lane == 0) dred[wave] = mn;
+  const int any_nan = __syncthreads_or(nan ? 1 : 0);
+  if (threadIdx.x == 0) {
+    double m = dred[0];
+    for (int k = 1; k < kSegT / 64; k++) m = fmin(m, dred[k]);
+    w.seg.seg_min[(long long)f * w.seg.segs + sg] = m;
+    w.seg.seg_nan[(long long)f * w.seg.segs + sg] = any_nan;
+  }
+}
+
+__device__ __forceinline__ double next_min_sync(const SyncCfg& c, double min_sync) {
+#pragma clang fp contract(off)
+  // AdvanceFallbackSyncSearchLevel (LevelDetection.cs:398-407)
+  const double blank = c.ire0;
+  const double sync_ire = (min_sync - blank) / c.hz_ire;
+  const double next_ire = sync_ire + 5.0;
+  const double scaled = c.hz_ire * next_ire;
+  return blank + scaled;
+}
+
+__global__ void search_thresholds_kernel(SyncCfg c, SyncWork w, int max_fields) {
+#pragma clang fp contract(off)
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= max_fields || f >= *w.spec_count) return;
+  const long long n = w.win_len[f];
+  const int segs = (int)((n + kSegLen - 1) / kSegLen);
+  double mn = INFINITY;
+  bool nan = false;
+  for (int sg = 0; sg < segs; sg++) {
+    mn = fmin(mn, w.seg.seg_min[(long long)f * w.seg.segs + sg]);
+    nan |= w.seg.seg_nan[(long long)f * w.seg.segs + sg] != 0;
+  }
+  if (nan) mn = __builtin_nan("");
+  double min_sync = mn;
+  for (int r = 0; r < kSpecR; r++) {
+    w.seg.thr[(long long)f * kSpecR + r] = (min_sync + c.ire0) / 2.0;
+    min_sync = next_min_sync(c, min_sync);
+  }
+  w.seg.fallback[f] = 0;
+}
+
+// SearchFallbackSerrationLevels' retry decisions (LevelDetection.cs:265-396) over the
+// counts of the first kSpecR thresholds.
+__global__ void search_decide_kernel(SyncCfg c, SyncWork w, SegJob j, int max_fields) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= max_fields || f >= *w.spec_count) return;
+  int prev_vsync = 0, prev_r = 0;
+  bool found = false, check_next = true;
+  int choice = -1;
+  for (int r = 0; r < kSpecR && choice < 0; r++) {
+    int np, vc;
+    seg_combine(j, f, r, &np, &vc, nullptr, nullptr);
+    if (np > 200 && vc > 4) {
+      if (vc == 12 && !check_next) {
+        choice = r;
+      } else if (!found || vc > prev_vsync) {
+        found = true;
+        prev_vsync = vc;
+        prev_r = r;
+        check_next = true;
+      } else if (vc < prev_vsync || !check_next) {
+        choice = prev_r;
+      } else {
+        check_next = false;
+      }
+    }
+  }
+  w.seg.choice[f] = choice;
+  w.seg.fallback[f] = choice < 0 ? 1 : 0;
+  w.seg.active[f] = choice < 0 ? 0 : 1;
+  w.seg.active2[f] = 0;
+}
+
+// The first refinement on the chosen list, then the second refinement's threshold.
+__global__ void __launch_bounds__(kT) refine_first_kernel(SyncCfg c, SyncWork w) {
+#pragma clang fp contract(off)
+  __shared__ RefineScratch r;
+  const int f = blockIdx.x;
+  if (f >= *w.spec_count || !w.seg.active[f]) return;
+  const Plane x{w.sync_ref + w.win_start[f], w.win_len[f], c.bias, 0.0};
+  const int* pst = w.pulse_start + (long long)f * c.pulse_cap;
+  const int* pln = w.pulse_len + (long long)f * c.pulse_cap;
+  const int np = w.seg.count[f] < c.pulse_cap ? w.seg.count[f] : c.pulse_cap;
+  const double ref_sync = c.ire0 + (c.hz_ire * c.vsync_ire);
+  const RefineOut first = wg_refine_from_pulses(x, pst, pln, np, c, ref_sync, &r);
+  if (threadIdx.x == 0) {
+    LevelSearch out{};
+    out.first_ok = first.ok;
+    out.first_sync = first.sync;
+    out.first_blank = first.blank;
+    out.first_fail = first.fail;
+    out.first_has_measured = first.has_measured;
+    out.first_measured = first.measured;
+    w.search[f] = out;
+    // RefineSerrationLevels (:98-170): pulses at the first levels' midpoint
+    w.seg.thr2[f] = (first.sync + first.blank) / 2.0;
+    w.seg.active2[f] = first.ok ? 1 : 0;
+  }
+}
+
+__global__ void __launch_bounds__(kT) refine_second_kernel(SyncCfg c, SyncWork w) {
+#pragma clang fp contract(off)
+  __shared__ RefineScratch r;
+  const int f = blockIdx.x;
+  if (f >= *w.spec_count || !w.seg.active2[f]) return;
+  const Plane x{w.sync_ref + w.win_start[f], w.win_len[f], c.bias, 0.0};
+  const int* pst = w.pulse_start + (long long)f * c.pulse_cap;
+  const int* pln = w.pulse_len + (long long)f * c.pulse_cap;
+  const int np = w.seg.count2[f] < c.pulse_cap ? w.seg.count2[f] : c.pulse_cap;
+  const double ref_sync = c.ire0 + (c.hz_ire * c.vsync_ire);
+  const RefineOut sec = wg_refine_from_pulses(x, pst, pln, np, c, ref_sync, &r);
+  if (threadIdx.x == 0) {
+    LevelSearch out = w.search[f];
+    out.ref_ok = sec.ok;
+    out.ref_sync = sec.sync;
+    out.ref_blank = sec.blank;
+    out.ref_fail = sec.fail;
+    out.ref_has_measured = sec.has_measured;
+    out.ref_measured = sec.measured;
+    w.search[f] = out;
+  }
+}
+
+// The field thresholds and DC offsets the pulse search of S3 runs at.
+__global__ void field_thresholds_kernel(SyncCfg c, SyncWork w, int max_fields) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= max_fields || f >= *w.spec_count) return;
+  const FieldLevels lv = w.levels[f];
+  w.seg.thr3[f] = lv.threshold;
+  w.seg.dc3[f] = lv.dc;
+}
+
+__global__ void field_raw_count_kernel(SyncCfg c, SyncWork w, int max_fields) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= max_fields || f >= *w.spec_count) return;
+  const int np = w.seg.count3[f];
+  if (np > c.pulse_cap) w.overflow[f] = 1;
+  w.raw_count[f] = np < c.pulse_cap ? np : c.pulse_cap;
+}
+
 // ------------------------------------------------------------------ S2: level chain
 // The moving averages of VsyncSerrationDetector (window 2, watermark 1, unbounded pushes
 // trimmed on pull) and VhsFieldLevelState (window 10, bounded pushes), kept by every
 // thread of the one workgroup (identical inputs, identical branches) so the rare
 // refinement from field-state levels can run on the whole workgroup.
 struct Avg {
+  // values in registers: every loop runs over the whole capacity with compile-time indices
   double v[kAvgCap];
   int n;
   int window;
   int watermark;
   bool bounded;
   __device__ bool has_values() const { return n > watermark; }
+  __device__ void shift_out() {  // drop the oldest value
+#pragma unroll
+    for (int i = 0; i + 1 < kAvgCap; i++) v[i] = v[i + 1];
+    n--;
+  }
   __device__ void push(double x) {
-    if (bounded && n == window) {
-      for (int i = 1; i < n; i++) v[i - 1] = v[i];
-      n--;
-    }
-    if (n == kAvgCap) {  // unbounded lists never exceed a few values between pulls
-      for (int i = 1; i < n; i++) v[i - 1] = v[i];
-      n--;
-    }
-    v[n++] = x;
+    if (bounded && n == window) shift_out();
+    if (n == kAvgCap) shift_out();  // unbounded lists never exceed a few values between pulls
+#pragma unroll
+    for (int i = 0; i < kAvgCap; i++)
+      if (i == n) v[i] = x;
+    n++;
   }
   __device__ bool pull(double* out) {
 #pragma clang fp contract(off)
     if (n == 0) return false;
-    if (n >= window) {
-      const int drop = n - window;
-      for (int i = 0; i < window; i++) v[i] = v[i + drop];
-      n = window;
-    }
+    while (n > window) shift_out();
     double s = 0.0;
-    for (int i = 0; i < n; i++) s += v[i];
+#pragma unroll
+    for (int i = 0; i < kAvgCap; i++)
+      if (i < n) s += v[i];
     *out = s / (double)n;
     return true;
   }
@@ -669,28 +1127,30 @@ struct LevelChain {
   __device__ void load(const LevelStateDev& s) {
     ser_s.n = s.ser_n;
     ser_b.n = s.ser_n;
-    for (int i = 0; i < s.ser_n; i++) {
-      ser_s.v[i] = s.ser_sync[i];
-      ser_b.v[i] = s.ser_blank[i];
-    }
     fs_s.n = s.fs_sync_n;
     fs_b.n = s.fs_blank_n;
-    for (int i = 0; i < s.fs_sync_n; i++) fs_s.v[i] = s.fs_sync[i];
-    for (int i = 0; i < s.fs_blank_n; i++) fs_b.v[i] = s.fs_blank[i];
+#pragma unroll
+    for (int i = 0; i < kAvgCap; i++) {
+      ser_s.v[i] = s.ser_sync[i];
+      ser_b.v[i] = s.ser_blank[i];
+      fs_s.v[i] = s.fs_sync[i];
+      fs_b.v[i] = s.fs_blank[i];
+    }
     has_last = s.has_last;
     last_s = s.last_sync;
     last_b = s.last_blank;
   }
   __device__ void store(LevelStateDev* s) const {
     s->ser_n = ser_s.n;
-    for (int i = 0; i < ser_s.n; i++) {
-      s->ser_sync[i] = ser_s.v[i];
-      s->ser_blank[i] = ser_b.v[i];
-    }
     s->fs_sync_n = fs_s.n;
     s->fs_blank_n = fs_b.n;
-    for (int i = 0; i < fs_s.n; i++) s->fs_sync[i] = fs_s.v[i];
-    for (int i = 0; i < fs_b.n; i++) s->fs_blank[i] = fs_b.v[i];
+#pragma unroll
+    for (int i = 0; i < kAvgCap; i++) {
+      s->ser_sync[i] = ser_s.v[i];
+      s->ser_blank[i] = ser_b.v[i];
+      s->fs_sync[i] = fs_s.v[i];
+      s->fs_blank[i] = fs_b.v[i];
+    }
     s->has_last = has_last;
     s->last_sync = last_s;
     s->last_blank = last_b;
@@ -735,7 +1195,10 @@ struct LevelChain {
   }
 };
 
-__global__ void __launch_bounds__(kT) level_chain_kernel(SyncCfg c, SyncWork w) {
+// 256 threads: the moving averages stay in registers (every thread keeps them); the
+// workgroup helpers run at any workgroup size.
+constexpr int kChainT = 256;
+__global__ void __launch_bounds__(kChainT) level_chain_kernel(SyncCfg c, SyncWork w) {
 #pragma clang fp contract(off)
   __shared__ RefineScratch r;
   LevelChain ch;
@@ -747,9 +1210,7 @@ __global__ void __launch_bounds__(kT) level_chain_kernel(SyncCfg c, SyncWork w) 
   const int nf = *w.spec_count;
   const double ref_sync = c.ire0 + (c.hz_ire * c.vsync_ire);
   for (int f = 0; f < nf; f++) {
-    LevelStateDev snap;
-    ch.store(&snap);
-    if (threadIdx.x == 0) w.level_before[f] = snap;
+    if (threadIdx.x == 0) ch.store(&w.level_before[f]);
     const Plane x{w.sync_ref + w.win_start[f], w.win_len[f], c.bias, 0.0};
     const LevelSearch sr = w.search[f];
     // VsyncSerrationDetector.Analyze with no measurement: pull (trims), HasLevels
@@ -878,26 +1339,7 @@ __global__ void __launch_bounds__(kT) level_chain_kernel(SyncCfg c, SyncWork w) 
     }
     if (threadIdx.x == 0) w.levels[f] = lv;
   }
-  LevelStateDev fin;
-  ch.store(&fin);
-  if (threadIdx.x == 0) *w.level_state_after_all = fin;
-}
-
-// ------------------------------------------------------------------ S3: pulses
-__global__ void __launch_bounds__(kT) field_pulses_kernel(SyncCfg c, SyncWork w) {
-  __shared__ PulseScratch ps;
-  const int f = blockIdx.x;
-  if (f >= *w.spec_count) return;
-  const FieldLevels lv = w.levels[f];
-  const Plane x{w.sync_ref + w.win_start[f], w.win_len[f], c.bias, lv.dc};
-  int* pst = w.pulse_start + (long long)f * c.pulse_cap;
-  int* pln = w.pulse_len + (long long)f * c.pulse_cap;
-  const int np = wg_find_pulses(x, lv.threshold, c.pulse_min, c.pulse_max, 1e300, pst, pln, c.pulse_cap, &ps,
-                                nullptr);
-  if (threadIdx.x == 0) {
-    if (np > c.pulse_cap) w.overflow[f] = 1;
-    w.raw_count[f] = np < c.pulse_cap ? np : c.pulse_cap;
-  }
+  if (threadIdx.x == 0) ch.store(w.level_state_after_all);
 }
 
 // ------------------------------------------------------------------ S4: per-field classification
@@ -1876,9 +2318,41 @@ extern "C" hipError_t vhsd_launch_vhs_sync(const vhsd::vsync::SyncCfg* c, const 
   if (c->pulse_cap < 1 || c->output_lines + 10 > c->line_stride) return hipErrorInvalidValue;
   hipLaunchKernelGGL(spec_windows_kernel, dim3(1), dim3(kT), 0, stream, *c, *w, starts, lens, pulse_count, capacity,
                      span_len, max_fields);
+  const SegBuffers& b = w->seg;
+  const int segs = b.segs;
+  const int small = (max_fields + 63) / 64;
+  // S1: level search
+  const double eql = (double)c->ser_eq_len, ll = (double)c->ser_line_len;
+  const int pmn = (int)std::ceil(eql / 8.0) > 0 ? (int)std::ceil(eql / 8.0) : 0;
+  const int pmx = (int)std::floor(ll * 5.0) > 1 ? (int)std::floor(ll * 5.0) : 1;
+  const double min_vsync = (c->vsync_us * c->fs_mhz) * 0.8;
+  hipLaunchKernelGGL(seg_min_kernel, dim3(segs, max_fields), dim3(kSegT), 0, stream, *c, *w);
+  hipLaunchKernelGGL(search_thresholds_kernel, dim3(small), dim3(64), 0, stream, *c, *w, max_fields);
+  SegJob j1{b.thr, kSpecR, nullptr, pmn, pmx, min_vsync, nullptr, b.sum, segs, b.choice, b.seg_carry, b.seg_off,
+            b.count, w->pulse_start, w->pulse_len};
+  hipLaunchKernelGGL(seg_count_kernel, dim3(segs, kSpecR, max_fields), dim3(kSegT), 0, stream, *c, *w, j1);
+  hipLaunchKernelGGL(search_decide_kernel, dim3(small), dim3(64), 0, stream, *c, *w, j1, max_fields);
   hipLaunchKernelGGL(level_search_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
-  hipLaunchKernelGGL(level_chain_kernel, dim3(1), dim3(kT), 0, stream, *c, *w);
-  hipLaunchKernelGGL(field_pulses_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
+  j1.active = b.active;
+  hipLaunchKernelGGL(seg_plan_kernel, dim3(small), dim3(64), 0, stream, *c, *w, j1, max_fields);
+  hipLaunchKernelGGL(seg_write_kernel, dim3(segs, max_fields), dim3(kSegT), 0, stream, *c, *w, j1);
+  hipLaunchKernelGGL(refine_first_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
+  SegJob j2{b.thr2, 1, nullptr, pmn, pmx, 1e300, b.active2, b.sum, segs, nullptr, b.seg_carry, b.seg_off, b.count2,
+            w->pulse_start, w->pulse_len};
+  hipLaunchKernelGGL(seg_count_kernel, dim3(segs, 1, max_fields), dim3(kSegT), 0, stream, *c, *w, j2);
+  hipLaunchKernelGGL(seg_plan_kernel, dim3(small), dim3(64), 0, stream, *c, *w, j2, max_fields);
+  hipLaunchKernelGGL(seg_write_kernel, dim3(segs, max_fields), dim3(kSegT), 0, stream, *c, *w, j2);
+  hipLaunchKernelGGL(refine_second_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
+  // S2: the level chain
+  hipLaunchKernelGGL(level_chain_kernel, dim3(1), dim3(kChainT), 0, stream, *c, *w);
+  // S3: the fields' pulses at their thresholds on the DC-shifted reference
+  hipLaunchKernelGGL(field_thresholds_kernel, dim3(small), dim3(64), 0, stream, *c, *w, max_fields);
+  SegJob j3{b.thr3, 1, b.dc3, c->pulse_min, c->pulse_max, 1e300, nullptr, b.sum, segs, nullptr, b.seg_carry,
+            b.seg_off, b.count3, w->pulse_start, w->pulse_len};
+  hipLaunchKernelGGL(seg_count_kernel, dim3(segs, 1, max_fields), dim3(kSegT), 0, stream, *c, *w, j3);
+  hipLaunchKernelGGL(seg_plan_kernel, dim3(small), dim3(64), 0, stream, *c, *w, j3, max_fields);
+  hipLaunchKernelGGL(seg_write_kernel, dim3(segs, max_fields), dim3(kSegT), 0, stream, *c, *w, j3);
+  hipLaunchKernelGGL(field_raw_count_kernel, dim3(small), dim3(64), 0, stream, *c, *w, max_fields);
   hipLaunchKernelGGL(field_logic_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
   hipLaunchKernelGGL(field_chain_kernel, dim3(1), dim3(64), 0, stream, *c, *w);
   hipLaunchKernelGGL(field_lines_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);

@@ -89,6 +89,39 @@ struct FieldLogic {
 
 typedef vhsdecode_hip_vhs_field_v1 FieldRecord;
 
+constexpr int kSegLen = 16384;  // samples per segment of the segmented pulse searches
+constexpr int kSpecR = 8;       // level-search thresholds evaluated at once
+
+// A segment's pulse-search summary (vhs_sync.hip, seg_scan).
+struct SegSummary {
+  int first_pos;  // first transition, -1 for none
+  int first_end;  // 1 when it ends a pulse
+  int last_pos;
+  int n_in;       // pulses in range starting and ending in the segment
+  int n_vs;       // of those, longer than vs_min
+};
+
+// Scratch of the segmented pulse searches ([field] unless stated).
+struct SegBuffers {
+  int segs;             // segments per window (the longest)
+  double* seg_min;      // [field][segs]
+  int* seg_nan;         // [field][segs]
+  double* thr;          // [field][kSpecR]: the level search's threshold sequence
+  int* choice;          // its list threshold (index), -1: the whole search on one workgroup
+  int* fallback;
+  int* active;
+  int* count;           // pulses of the search's list
+  double* thr2;         // the second refinement's threshold
+  int* active2;
+  int* count2;
+  double* thr3;         // the field threshold and DC offset
+  double* dc3;
+  int* count3;
+  SegSummary* sum;      // [field][kSpecR][segs]
+  int* seg_carry;       // [field][segs]
+  int* seg_off;         // [field][segs]
+};
+
 // The sequence position across the rounds of one call: each round speculates windows from
 // `begin`, decodes them into records [base, base + decoded) and moves `begin` and `records`
 // on; `halted` ends the call (the next window runs past the span, a list overflowed, or
@@ -127,6 +160,7 @@ struct SyncWork {
   const SyncHistoryDev* history_in;
   SyncHistoryDev* history_out;
   double* lines;             // [max_fields][line_stride], this round
+  SegBuffers seg;
 };
 
 }  // namespace vsync

@@ -1683,6 +1683,12 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
     const size_t o_thr = take(8 * F), o_dc = take(8 * F), o_zero = take(line_stride), o_oi = take(4 * F);
     const size_t o_refb = take(8 * F * line_stride), o_fob = take(F * line_stride), o_rend = take(8 * F * line_stride);
     const size_t o_acc = take(4 * 8 * F * line_stride);
+    // segmented pulse searches: segments of the longest read window
+    const int segs = (int)((window_max + kSegLen - 1) / kSegLen);
+    const size_t FS = F * (size_t)segs;
+    const size_t o_smin = take(8 * FS), o_snan = take(4 * FS), o_sthr = take(8 * F * kSpecR);
+    const size_t o_sint = take(4 * F * 9), o_sdbl = take(8 * F * 3);
+    const size_t o_ssum = take(sizeof(SegSummary) * FS * kSpecR), o_scar = take(4 * FS), o_soff = take(4 * FS);
     VHSD_HIP(c->vs_mem.ensure(off));
     unsigned char* m = c->vs_mem.p;
     const size_t state_hist = (sizeof(LevelStateDev) + 255) & ~(size_t)255;
@@ -1725,6 +1731,28 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
     w.history_in = history;
     w.history_out = history;
     w.lines = reinterpret_cast<double*>(m + o_lines);
+    {
+      SegBuffers& b = w.seg;
+      b.segs = segs;
+      b.seg_min = reinterpret_cast<double*>(m + o_smin);
+      b.seg_nan = reinterpret_cast<int*>(m + o_snan);
+      b.thr = reinterpret_cast<double*>(m + o_sthr);
+      int* ints = reinterpret_cast<int*>(m + o_sint);
+      b.choice = ints;
+      b.fallback = ints + F;
+      b.active = ints + 2 * F;
+      b.count = ints + 3 * F;
+      b.active2 = ints + 4 * F;
+      b.count2 = ints + 5 * F;
+      b.count3 = ints + 6 * F;
+      double* dbls = reinterpret_cast<double*>(m + o_sdbl);
+      b.thr2 = dbls;
+      b.thr3 = dbls + F;
+      b.dc3 = dbls + 2 * F;
+      b.sum = reinterpret_cast<SegSummary*>(m + o_ssum);
+      b.seg_carry = reinterpret_cast<int*>(m + o_scar);
+      b.seg_off = reinterpret_cast<int*>(m + o_soff);
+    }
     {
       ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_PULSES, span_length);
       VHSD_HIP(vhsd_launch_pulses(sync_ref, span_length, c->bias, p->threshold, cfg.pulse_min, cfg.pulse_max,
