@@ -1349,27 +1349,6 @@ struct Timing {
   double hsync_lo, hsync_hi, eq_lo, eq_hi, vs_lo, vs_hi, median;
 };
 
-// The pulse lists of one field: in LDS up to kPulseLds pulses, else in the field's
-// global lists (a window full of noise).
-struct LogicScratch {
-  int* rs;            // raw pulses (rescue rewrites them)
-  int* rl;
-  int* cs;            // refined pulses
-  int* cl;
-  unsigned char* ck;  // kind
-  unsigned char* co;  // in order
-  int lrs[kPulseLds], lrl[kPulseLds], lcs[kPulseLds], lcl[kPulseLds];
-  unsigned char lck[kPulseLds], lco[kPulseLds];
-  int hist[kHistBins];
-  int nref;
-};
-
-__device__ __forceinline__ int classify(int len, const Timing& t) {
-  if (in_range(len, t.hsync_lo, t.hsync_hi)) return kHSync;
-  if (in_range(len, t.eq_lo, t.eq_hi)) return kEq;
-  return in_range(len, t.vs_lo, t.vs_hi) ? kVSync : -1;
-}
-
 __device__ bool quality(int prev_kind, int prev_start, int kind, int start, double nominal) {
 #pragma clang fp contract(off)
   double lo, hi;
@@ -1426,34 +1405,6 @@ __device__ Acc acc_add(Acc a, Acc b) {
   return Acc{a.loc + b.loc, a.off + b.off, a.n + b.n};
 }
 
-// FinalizeEstimate (Dsp/VBlankSyncResolver.cs:666-718) over the refined pulses in LDS
-__device__ Estimate finalize(const LogicScratch* L, Acc a, double mll, double fhl, int cur_lines) {
-#pragma clang fp contract(off)
-  Estimate e{};
-  if (a.n == 0) return e;
-  double fh = round_away((a.loc + a.off) / a.n);
-  const double unaligned = fh;
-  double off = 0.0;
-  int cnt = 0;
-  for (int i = 0; i < L->nref; i++) {
-    if (L->ck[i] != kHSync || !L->co[i]) continue;
-    const double line = ((L->cs[i] - fh) / mll) + fhl;
-    const int rl = (int)round_away(line);
-    if (rl > cur_lines) break;
-    if (rl >= fhl) {
-      off += fh + (mll * (rl - fhl)) - L->cs[i];
-      cnt++;
-    }
-  }
-  if (cnt > 0) fh -= off / cnt;
-  e.ok = 1;
-  e.line0 = fh - (mll * fhl);
-  e.first_hsync = fh;
-  e.unaligned = unaligned;
-  e.valid_count = a.n;
-  return e;
-}
-
 __device__ void partial_markers(const double* smp, const int* has, double eq1_line, double section, double* ms,
                                 double* ml, int* nm) {
 #pragma clang fp contract(off)
@@ -1469,35 +1420,131 @@ __device__ void partial_markers(const double* smp, const int* has, double eq1_li
   *nm = n;
 }
 
+// The pulse lists of one field: in LDS up to kPulseLds pulses, else in the field's
+// global lists (a window full of noise).  eff_*: a raw pulse after TryRescueLongHSync as
+// often as it applies; cls_*: ClassifyPulses' kind of the raw and the rescued length;
+// visited: the RefinePulses walk has reached the pulse (so it reads the rescued one).
+struct LogicScratch {
+  int* rs;            // raw pulses
+  int* rl;
+  int* es;            // rescued pulses
+  int* el;
+  signed char* craw;  // classify(raw length)
+  signed char* ceff;  // classify(rescued length)
+  unsigned char* eflag;    // bit 0: hsync length, bit 1: rescue length (rescue failed), bit 2: eq length
+  unsigned char* visited;
+  int* cs;            // refined pulses
+  int* cl;
+  unsigned char* ck;  // kind
+  unsigned char* co;  // in order
+  int lrs[kPulseLds], lrl[kPulseLds], les[kPulseLds], lel[kPulseLds], lcs[kPulseLds], lcl[kPulseLds];
+  signed char lcraw[kPulseLds], lceff[kPulseLds];
+  unsigned char leflag[kPulseLds], lvisited[kPulseLds], lck[kPulseLds], lco[kPulseLds];
+  int hist[kHistBins];
+  int nref;
+  Timing t;
+  double mll;
+  // parallel reductions
+  int red[kT / 64];
+  long long lred[kT / 64];
+  int g0;                   // first refined pulse of the second group
+  int slot[4];              // consensus transitions: last refined index per (group, kind)
+  int mark[8];              // line-0 markers: last refined index per (group, marker)
+  unsigned long long run;   // (size << 32) | (~start): the longest hsync run, first on ties
+  // FinalizeEstimate jobs: parity x (first, last, combined)
+  Acc acc[6];
+  double fhl[2];
+  Estimate est[6];
+};
+
+__device__ __forceinline__ int classify(int len, const Timing& t) {
+  if (in_range(len, t.hsync_lo, t.hsync_hi)) return kHSync;
+  if (in_range(len, t.eq_lo, t.eq_hi)) return kEq;
+  return in_range(len, t.vs_lo, t.vs_hi) ? kVSync : -1;
+}
+
+
+// TryRescueLongHSync (:316-338) on one wave: the first run below (first sample - step)
+// inside [s, e) that ends inside it, has a length the search keeps and does not start at
+// s.  The samples come 64 at a time; the run boundaries are walked on the ballots.
+__device__ bool wave_rescue(const Plane& x, long long s, long long e, double step, int mn, int mx, int* rs,
+                            int* rl) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const double thr = x.at(s) - step;
+  bool inp = x.at(s) <= thr;
+  long long cur = 0;
+  for (long long base = 0; base < e - s; base += 64) {
+    const long long p = base + lane;
+    const bool valid = p < e - s;
+    const double v = valid ? x.at(s + p) : 0.0;
+    const unsigned long long below = __ballot(valid && v <= thr);
+    const unsigned long long above = __ballot(valid && v > thr);
+    int pos = 0;
+    for (;;) {
+      unsigned long long m = inp ? above : below;
+      m = pos >= 64 ? 0ull : (m & (~0ull << pos));
+      if (!m) break;
+      const int b = __ffsll((long long)m) - 1;
+      if (inp) {
+        const long long len = base + b - cur;
+        if (in_range((double)len, mn, mx) && cur != 0) {
+          *rs = (int)(s + cur);
+          *rl = (int)len;
+          return true;
+        }
+        inp = false;
+      } else {
+        cur = base + b;
+        inp = true;
+      }
+      pos = b + 1;
+    }
+  }
+  return false;
+}
+
 __global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) {
 #pragma clang fp contract(off)
   __shared__ LogicScratch L;
   const int f = blockIdx.x;
   if (f >= *w.spec_count) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int np = w.raw_count[f];
   int* gst = w.pulse_start + (long long)f * c.pulse_cap;
   int* gln = w.pulse_len + (long long)f * c.pulse_cap;
   const bool lds = np <= kPulseLds;
-  if (threadIdx.x == 0) {
-    unsigned char* aux = w.logic_aux + (long long)f * 6 * c.pulse_cap;
+  if (tid == 0) {
+    unsigned char* aux = w.logic_aux + (long long)f * kLogicAux * c.pulse_cap;
+    const long long pc = c.pulse_cap;
     L.rs = lds ? L.lrs : gst;
     L.rl = lds ? L.lrl : gln;
+    L.es = lds ? L.les : reinterpret_cast<int*>(aux);
+    L.el = lds ? L.lel : reinterpret_cast<int*>(aux + 4 * pc);
+    L.cl = lds ? L.lcl : reinterpret_cast<int*>(aux + 8 * pc);
+    L.craw = lds ? L.lcraw : reinterpret_cast<signed char*>(aux + 12 * pc);
+    L.ceff = lds ? L.lceff : reinterpret_cast<signed char*>(aux + 13 * pc);
+    L.eflag = lds ? L.leflag : aux + 14 * pc;
+    L.visited = lds ? L.lvisited : aux + 15 * pc;
+    L.ck = lds ? L.lck : aux + 16 * pc;
+    L.co = lds ? L.lco : aux + 17 * pc;
     L.cs = lds ? L.lcs : w.ref_start + (long long)f * c.pulse_cap;
-    L.cl = lds ? L.lcl : reinterpret_cast<int*>(aux);
-    L.ck = lds ? L.lck : aux + 4LL * c.pulse_cap;
-    L.co = lds ? L.lco : aux + 5LL * c.pulse_cap;
+    L.g0 = INT_MAX;
+    L.run = 0;
+    for (int k = 0; k < 4; k++) L.slot[k] = -1;
+    for (int k = 0; k < 8; k++) L.mark[k] = -1;
   }
   if (lds)
-    for (int i = threadIdx.x; i < np; i += kT) {
+    for (int i = tid; i < np; i += kT) {
       L.lrs[i] = gst[i];
       L.lrl[i] = gln[i];
     }
-  for (int i = threadIdx.x; i < kHistBins; i += kT) L.hist[i] = 0;
+  for (int i = tid; i < kHistBins; i += kT) L.hist[i] = 0;
   __syncthreads();
   // EstimateTiming (:131-166): median of the hsync-like lengths by a histogram
   const double hs_lo = (c.hsync_us - 1.75) * c.fs_mhz, hs_hi = (c.hsync_us + 2.0) * c.fs_mhz;
   const int hbase = (int)ceil(hs_lo);
-  for (int i = threadIdx.x; i < np; i += kT) {
+  for (int i = tid; i < np; i += kT) {
     const int len = L.rl[i];
     if (in_range(len, hs_lo, hs_hi)) {
       const int b = len - hbase;
@@ -1505,245 +1552,317 @@ __global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) 
     }
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  FieldLogic out{};
-  int nh = 0;
-  for (int b = 0; b < kHistBins; b++) nh += L.hist[b];
-  double med;
-  if (nh > 0) {
+  if (wave == 0) {
+    constexpr int kPer = kHistBins / 64;
+    int loc[kPer];
+    int lsum = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      loc[k] = L.hist[lane * kPer + k];
+      lsum += loc[k];
+    }
+    const int incl = wave_scan_incl(lsum, lane);
+    const int nh = __shfl(incl, 63);
+    const int before = incl - lsum;
+    // the bin holding sorted position k (k < nh)
     auto kth = [&](int k) {
-      int acc = 0;
-      for (int b = 0; b < kHistBins; b++) {
-        acc += L.hist[b];
-        if (k < acc) return (double)(b + hbase);
+      int bin = 0;
+      if (k >= before && k < incl) {
+        int acc = before;
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+          acc += loc[q];
+          if (k < acc) {
+            bin = lane * kPer + q;
+            break;
+          }
+        }
       }
-      return 0.0;
+      const unsigned long long owner = __ballot(k >= before && k < incl);
+      return (double)(__shfl(bin, __ffsll((long long)owner) - 1) + hbase);
     };
-    const int mid = nh / 2;
-    med = (nh & 1) ? kth(mid) : (kth(mid - 1) + kth(mid)) / 2.0;
-  } else {
-    med = c.hsync_us;  // v0.4.0 keeps the empty-list fallback in microseconds
+    double med;
+    if (nh > 0) {
+      const int mid = nh / 2;
+      const double a = kth(mid);
+      med = (nh & 1) ? a : (kth(mid - 1) + a) / 2.0;
+    } else {
+      med = c.hsync_us;  // v0.4.0 keeps the empty-list fallback in microseconds
+    }
+    if (lane == 0) {
+      const double typical = c.hsync_us * c.fs_mhz;
+      const double toff = med - typical;
+      Timing t;
+      t.median = med;
+      t.hsync_lo = med - (c.hsync_tol * c.fs_mhz);
+      t.hsync_hi = med + (c.hsync_tol * c.fs_mhz);
+      t.eq_lo = ((c.eq_us - c.eq_tol) * c.fs_mhz) + toff;
+      t.eq_hi = ((c.eq_us + c.eq_tol) * c.fs_mhz) + toff;
+      t.vs_lo = ((c.vsync_us * 0.5) * c.fs_mhz) + toff;
+      t.vs_hi = ((c.vsync_us + 1.0) * c.fs_mhz) + toff;
+      L.t = t;
+    }
   }
-  const double typical = c.hsync_us * c.fs_mhz;
-  const double toff = med - typical;
-  Timing t;
-  t.median = med;
-  t.hsync_lo = med - (c.hsync_tol * c.fs_mhz);
-  t.hsync_hi = med + (c.hsync_tol * c.fs_mhz);
-  t.eq_lo = ((c.eq_us - c.eq_tol) * c.fs_mhz) + toff;
-  t.eq_hi = ((c.eq_us + c.eq_tol) * c.fs_mhz) + toff;
-  t.vs_lo = ((c.vsync_us * 0.5) * c.fs_mhz) + toff;
-  t.vs_hi = ((c.vsync_us + 1.0) * c.fs_mhz) + toff;
-  out.hsync_median = med;
+  __syncthreads();
+  const Timing t = L.t;
   // RefinePulses (:243-314) with the long-hsync rescue on the DC-shifted reference
   const FieldLevels lv = w.levels[f];
   const Plane x{w.sync_ref + w.win_start[f], w.win_len[f], c.bias, lv.dc};
   const double nominal = c.nominal;
   const double step = 10.0 * c.hz_ire;
-  int nref = 0;
-  auto add = [&](int kind, int st, int ln) {
-    const bool io = nref > 0 && quality(L.ck[nref - 1], L.cs[nref - 1], kind, st, nominal);
-    if (nref < c.pulse_cap) {
-      L.ck[nref] = (unsigned char)kind;
-      L.cs[nref] = st;
-      L.cl[nref] = ln;
-      L.co[nref] = io ? 1 : 0;
-      nref++;
-    }
-  };
-  int index = 0;
-  while (index < np) {
-    const int ps_ = L.rs[index], pl_ = L.rl[index];
-    if (in_range(pl_, t.hsync_lo, t.hsync_hi)) {
-      add(kHSync, ps_, pl_);
-      index++;
-      continue;
-    }
-    if (x.n > 0 && step > 0.0 && in_range(pl_, t.hsync_hi, t.hsync_hi * 3.0)) {
-      // TryRescueLongHSync (:316-338): first pulse below (first sample - step) inside the pulse
-      long long s = ps_ < 0 ? 0 : (ps_ > x.n ? x.n : ps_);
-      long long e = (long long)ps_ + pl_;
-      e = e < s ? s : (e > x.n ? x.n : e);
-      bool rescued = false;
-      int rs_ = 0, rl_ = 0;
-      if (s < e) {
-        const double thr = x.at(s) - step;
-        const int mn = (int)ceil((c.eq_us * c.fs_mhz) / 8.0) > 0 ? (int)ceil((c.eq_us * c.fs_mhz) / 8.0) : 0;
-        const int mx = (int)(nominal * 5.0) > 1 ? (int)(nominal * 5.0) : 1;
-        bool inp = x.at(s) <= thr;
-        long long cur = 0;
-        for (long long p = 0; p < e - s && !rescued; p++) {
-          const double v = x.at(s + p);
-          if (inp) {
-            if (v > thr) {
-              const long long len = p - cur;
-              if (in_range((double)len, mn, mx) && cur != 0) {
-                rescued = true;
-                rs_ = (int)(s + cur);
-                rl_ = (int)len;
-              }
-              inp = false;
-            }
-          } else if (v <= thr) {
-            cur = p;
-            inp = true;
-          }
-        }
+  const bool can_rescue = x.n > 0 && step > 0.0;
+  auto is_h = [&](int len) { return in_range(len, t.hsync_lo, t.hsync_hi); };
+  auto is_r = [&](int len) { return can_rescue && in_range(len, t.hsync_hi, t.hsync_hi * 3.0); };
+  for (int i = tid; i < np; i += kT) {
+    const int len = L.rl[i];
+    L.craw[i] = (signed char)classify(len, t);
+    L.visited[i] = 0;
+    L.es[i] = L.rs[i];
+    L.el[i] = len;
+  }
+  __syncthreads();
+  // each rescue-length pulse on one wave, rescued as often as the walk would rescue it
+  // (the walk examines a rewritten pulse again)
+  {
+    const int mn = (int)ceil((c.eq_us * c.fs_mhz) / 8.0) > 0 ? (int)ceil((c.eq_us * c.fs_mhz) / 8.0) : 0;
+    const int mx = (int)(nominal * 5.0) > 1 ? (int)(nominal * 5.0) : 1;
+    for (int i = wave; i < np; i += kT / 64) {
+      int ps = L.rs[i], pl = L.rl[i];
+      if (is_h(pl) || !is_r(pl)) continue;
+      for (;;) {
+        long long s = ps < 0 ? 0 : (ps > x.n ? x.n : ps);
+        long long e = (long long)ps + pl;
+        e = e < s ? s : (e > x.n ? x.n : e);
+        int r_s = 0, r_l = 0;
+        const bool ok = s < e && wave_rescue(x, s, e, step, mn, mx, &r_s, &r_l);
+        if (!ok || (r_s == ps && r_l == pl)) break;
+        ps = r_s;
+        pl = r_l;
+        if (is_h(pl) || !is_r(pl)) break;
       }
-      if (rescued && (rs_ != ps_ || rl_ != pl_)) {
-        L.rs[index] = rs_;
-        L.rl[index] = rl_;
+      if (lane == 0) {
+        L.es[i] = ps;
+        L.el[i] = pl;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < np; i += kT) {
+    const int len = L.el[i];
+    L.ceff[i] = (signed char)classify(len, t);
+    L.eflag[i] = (unsigned char)((is_h(len) ? 1 : 0) | (!is_h(len) && is_r(len) ? 2 : 0) |
+                                 (in_range(len, t.eq_lo, t.eq_hi) ? 4 : 0));
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int nref = 0, pk = -1, pstart = 0;
+    auto add = [&](int kind, int st, int ln) {
+      const bool io = nref > 0 && quality(pk, pstart, kind, st, nominal);
+      if (nref < c.pulse_cap) {
+        L.ck[nref] = (unsigned char)kind;
+        L.cs[nref] = st;
+        L.cl[nref] = ln;
+        L.co[nref] = io ? 1 : 0;
+        nref++;
+        pk = kind;
+        pstart = st;
+      }
+    };
+    int index = 0;
+    while (index < np) {
+      L.visited[index] = 1;
+      const int fl = L.eflag[index];
+      if (fl & 1) {
+        add(kHSync, L.es[index], L.el[index]);
+        index++;
         continue;
       }
-      index++;
-      continue;
-    }
-    const bool can_start = index > 2 && in_range(pl_, t.eq_lo, t.eq_hi) && nref > 0 && L.ck[nref - 1] == kHSync;
-    if (!can_start) {
-      index++;
-      continue;
-    }
-    // TryReadVBlank (:340-471)
-    const int start = index - 2 > 0 ? index - 2 : 0;
-    const int end_ex = np < index + 24 ? np : index + 24;
-    int vk[32], vs[32], vl[32];
-    unsigned char vo[32];
-    int nv = 0, state = -1;
-    double state_end = 0.0, state_len = 0.0;
-    bool has_len = false, done = false;
-    for (int i = start; i < end_ex && !done; i++) {
-      const int rk = classify(L.rl[i], t);
-      if (rk < 0) continue;
-      const int pst = L.rs[i];
-      int acc = -1;
-      bool fin = false;
-      switch (state) {
-        case -1:
-          if (rk == kHSync) acc = kHSync;
-          break;
-        case kHSync:
-          if (rk == kHSync) acc = kHSync;
-          else if (rk == kEq) {
-            acc = kEq;
-            has_len = true;
-            state_len = c.num_pulses / 2.0;
-          } else if (rk == kVSync)
-            acc = kVSync;
-          break;
-        case kEq:
-          if (rk == kEq) acc = kEq;
-          else if (rk == kVSync) {
-            acc = kVSync;
-            has_len = true;
-            state_len = c.num_pulses / 2.0;
-          } else if (rk == kHSync)
-            acc = kHSync;
-          break;
-        case kVSync:
-          if (rk == kEq) {
-            acc = kEq2;
-            has_len = true;
-            state_len = c.num_pulses / 2.0;
-          } else if (rk == kVSync)
-            acc = kVSync;
-          else if (rk == kHSync && pst > state_end)
-            acc = kHSync;
-          break;
-        case kEq2:
-          if (rk == kEq) acc = kEq2;
-          else if (rk == kHSync) {
-            acc = kHSync;
-            fin = true;
+      // a rescue-length pulse the rescue does not shorten is passed over
+      if ((fl & 2) || !(index > 2 && (fl & 4) && nref > 0 && pk == kHSync)) {
+        index++;
+        continue;
+      }
+      // TryReadVBlank (:340-471): pulses the walk has reached are read rescued
+      const int start = index - 2 > 0 ? index - 2 : 0;
+      const int end_ex = np < index + 24 ? np : index + 24;
+      int vk[32], vs[32], vl[32];
+      unsigned char vo[32];
+      int nv = 0, state = -1;
+      double state_end = 0.0, state_len = 0.0;
+      bool has_len = false, done = false;
+      for (int i = start; i < end_ex && !done; i++) {
+        const bool seen = L.visited[i] != 0;
+        const int rk = seen ? L.ceff[i] : L.craw[i];
+        if (rk < 0) continue;
+        const int pst = seen ? L.es[i] : L.rs[i];
+        int acc = -1;
+        bool fin = false;
+        switch (state) {
+          case -1:
+            if (rk == kHSync) acc = kHSync;
+            break;
+          case kHSync:
+            if (rk == kHSync) acc = kHSync;
+            else if (rk == kEq) {
+              acc = kEq;
+              has_len = true;
+              state_len = c.num_pulses / 2.0;
+            } else if (rk == kVSync)
+              acc = kVSync;
+            break;
+          case kEq:
+            if (rk == kEq) acc = kEq;
+            else if (rk == kVSync) {
+              acc = kVSync;
+              has_len = true;
+              state_len = c.num_pulses / 2.0;
+            } else if (rk == kHSync)
+              acc = kHSync;
+            break;
+          case kVSync:
+            if (rk == kEq) {
+              acc = kEq2;
+              has_len = true;
+              state_len = c.num_pulses / 2.0;
+            } else if (rk == kVSync)
+              acc = kVSync;
+            else if (rk == kHSync && pst > state_end)
+              acc = kHSync;
+            break;
+          case kEq2:
+            if (rk == kEq) acc = kEq2;
+            else if (rk == kHSync) {
+              acc = kHSync;
+              fin = true;
+            }
+            break;
+        }
+        if (acc < 0) continue;
+        if (acc != state) {
+          if (pst < state_end) acc = -1;
+          else if (has_len) {
+            state_end = pst + ((state_len - 0.1) * nominal);
+            has_len = false;
           }
-          break;
+        }
+        if (acc >= 0) {
+          const bool io = nv > 0 && quality(vk[nv - 1], vs[nv - 1], acc, pst, nominal);
+          if (nv < 32) {
+            vk[nv] = acc;
+            vs[nv] = pst;
+            vl[nv] = seen ? L.el[i] : L.rl[i];
+            vo[nv] = io ? 1 : 0;
+            nv++;
+          }
+          state = acc;
+        }
+        if (fin) done = true;
       }
-      if (acc < 0) continue;
-      if (acc != state) {
-        if (pst < state_end) acc = -1;
-        else if (has_len) {
-          state_end = pst + ((state_len - 0.1) * nominal);
-          has_len = false;
+      if (!done) {
+        index++;
+        continue;
+      }
+      for (int k = 2; k < nv; k++) {
+        if (nref < c.pulse_cap) {
+          L.ck[nref] = (unsigned char)vk[k];
+          L.cs[nref] = vs[k];
+          L.cl[nref] = vl[k];
+          L.co[nref] = vo[k];
+          nref++;
+          pk = vk[k];
+          pstart = vs[k];
         }
       }
-      if (acc >= 0) {
-        const bool io = nv > 0 && quality(vk[nv - 1], vs[nv - 1], acc, pst, nominal);
-        if (nv < 32) {
-          vk[nv] = acc;
-          vs[nv] = pst;
-          vl[nv] = L.rl[i];
-          vo[nv] = io ? 1 : 0;
-          nv++;
-        }
-        state = acc;
-      }
-      if (fin) done = true;
+      index += (nv - 2 > 1 ? nv - 2 : 1);
     }
-    if (!done) {
-      index++;
-      continue;
-    }
-    for (int k = 2; k < nv; k++) {
-      if (nref < c.pulse_cap) {
-        L.ck[nref] = (unsigned char)vk[k];
-        L.cs[nref] = vs[k];
-        L.cl[nref] = vl[k];
-        L.co[nref] = vo[k];
-        nref++;
-      }
-    }
-    index += (nv - 2 > 1 ? nv - 2 : 1);
+    L.nref = nref;
   }
-  L.nref = nref;
-  out.refined = nref;
-  // ComputeMeanLineLength (:490-538)
+  __syncthreads();
+  const int nref = L.nref;
+  // ComputeMeanLineLength (:490-538): the first longest run of hsync pulses
   {
-    int run_start = -1, run_len = -1, cur_start = -1, cur_len = 0;
-    for (int i = 0; i < nref; i++) {
-      if (L.ck[i] != kHSync) {
-        if (cur_start >= 0 && cur_len > run_len) {
-          run_start = cur_start;
-          run_len = cur_len;
-        }
-        cur_start = -1;
-        cur_len = 0;
-      } else if (cur_start < 0) {
-        cur_start = i;
-        cur_len = 0;
-      } else {
-        cur_len++;
+    int carry = -1;  // last non-hsync index before the block
+    for (int b0 = 0; b0 < nref; b0 += kT) {
+      const int i = b0 + tid;
+      const bool in = i < nref;
+      const bool h = in && L.ck[i] == kHSync;
+      const int brk = in && !h ? i : -1;
+      // inclusive max-scan of the break positions
+      int incl = brk;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(incl, o);
+        if (lane >= o) incl = incl > u ? incl : u;
       }
+      if (lane == 63) L.red[wave] = incl;
+      __syncthreads();
+      int before = carry;
+      for (int ww = 0; ww < wave; ww++) before = before > L.red[ww] ? before : L.red[ww];
+      const int last_brk = incl > before ? incl : before;
+      if (h && (i + 1 == nref || L.ck[i + 1] != kHSync)) {
+        const int st = last_brk + 1;
+        const unsigned long long key = ((unsigned long long)(i - st + 1) << 32) | (0xFFFFFFFFull - (unsigned)st);
+        atomicMax(&L.run, key);
+      }
+      int blk = carry;
+      for (int ww = 0; ww < kT / 64; ww++) blk = blk > L.red[ww] ? blk : L.red[ww];
+      __syncthreads();
+      carry = blk;
     }
-    if (cur_start >= 0 && cur_len > run_len) {
-      run_start = cur_start;
-      run_len = cur_len;
-    }
-    double sum = 0.0;
+    __syncthreads();
+    const unsigned long long best = L.run;
+    const int run_start = best ? (int)(0xFFFFFFFFull - (best & 0xFFFFFFFFull)) : -1;
+    const int run_len = best ? (int)(best >> 32) - 1 : -1;
+    long long sum = 0;
     int cnt = 0;
-    for (int i = run_start + 1; i < run_start + run_len; i++) {
-      const double ll = (double)(L.cs[i] - L.cs[i - 1]);
-      if (in_range(ll / nominal, 0.95, 1.05)) {
-        sum += ll;
+    for (int i = run_start + 1 + tid; i < run_start + run_len; i += kT) {
+      const int d = L.cs[i] - L.cs[i - 1];
+      if (in_range((double)d / nominal, 0.95, 1.05)) {
+        sum += d;
         cnt++;
       }
     }
-    out.mll = cnt > 0 ? sum / (double)cnt : nominal;
+    sum = wg_sum_ll(sum, L.lred);
+    const long long c2 = wg_sum_ll(cnt, L.lred);
+    // the sum of whole sample counts is exact in FP64, so its order does not matter
+    if (tid == 0) L.mll = c2 > 0 ? (double)sum / (double)c2 : nominal;
+    __syncthreads();
   }
-  const double mll = out.mll;
-  // MeasureVBlankBoundaryConsensus (FieldParityDetector.cs:172-253)
+  const double mll = L.mll;
+  // MeasureVBlankBoundaryConsensus (FieldParityDetector.cs:172-253) and the transition
+  // markers of EstimateLine0FromTransitions (VBlankSyncResolver.cs:317-445): the refined
+  // pulses after the first in-order one past the first field's lines count to the second
+  // group; the last transition of each kind in a group is the one kept
+  const double second = nref > 0 ? L.cs[0] + (c.field_lines0 * mll) : 0.0;
+  for (int i = 1 + tid; i < nref; i += kT)
+    if (L.co[i] && L.cs[i] > second) atomicMin(&L.g0, i);
+  __syncthreads();
   {
-    double lens[4] = {-1.0, -1.0, -1.0, -1.0};
-    int group = 0, last = -1;
-    if (nref > 0) {
-      const double second = L.cs[0] + (c.field_lines0 * mll);
-      for (int i = 0; i < nref; i++) {
-        if (last >= 0 && L.co[i]) {
-          if (group == 0 && L.cs[i] > second) group = 2;
-          if (L.ck[last] == kHSync && L.ck[i] != kHSync) lens[group] = round_half_line((L.cs[i] - L.cs[last]) / mll);
-          else if (L.ck[last] != kHSync && L.ck[i] == kHSync)
-            lens[group + 1] = round_half_line((L.cs[i] - L.cs[last]) / mll);
-        }
-        last = i;
+    const int g0 = L.g0;
+    for (int i = 1 + tid; i < nref; i += kT) {
+      if (!L.co[i]) continue;
+      const int grp = i >= g0 ? 1 : 0;
+      const int pk = L.ck[i - 1], k = L.ck[i];
+      if (pk == kHSync && k != kHSync) {
+        atomicMax(&L.slot[grp * 2], i);
+        atomicMax(&L.mark[grp * 4], i);
+      } else if (pk != kHSync && k == kHSync) {
+        atomicMax(&L.slot[grp * 2 + 1], i);
+        atomicMax(&L.mark[grp * 4 + 3], i);
+      } else if (pk == kEq && k == kVSync) {
+        atomicMax(&L.mark[grp * 4 + 1], i);
+      } else if (pk == kVSync && k == kEq2) {
+        atomicMax(&L.mark[grp * 4 + 2], i);
       }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    FieldLogic& out = w.logic[f];
+    double lens[4];
+    for (int k = 0; k < 4; k++) {
+      const int i = L.slot[k];
+      lens[k] = i < 0 ? -1.0 : round_half_line((L.cs[i] - L.cs[i - 1]) / mll);
     }
     const double ffn[4] = {1.0, 0.5, 0.5, 1.0}, sfn[4] = {0.5, 1.0, 1.0, 0.5};
     const double ffp[4] = {0.5, 0.5, 1.0, 1.0}, sfp[4] = {1.0, 1.0, 0.5, 0.5};
@@ -1761,41 +1880,21 @@ __global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) 
     }
     out.consensus = cons;
     out.detected = det;
+    out.first_conf = 0;
+    out.second_conf = 0;
     if (det > 0) {
       const double wgt = det / 4.0;
       out.first_conf = (int)round_away((cons / (double)det) * wgt * 100.0);
       out.second_conf = (int)round_away(((det - cons) / (double)det) * wgt * 100.0);
     }
-  }
-  // EstimateLine0FromTransitions (VBlankSyncResolver.cs:317-445) for both parities
-  {
-    double fsm[4] = {0, 0, 0, 0}, lsm[4] = {0, 0, 0, 0};
-    int fh[4] = {0, 0, 0, 0}, lh[4] = {0, 0, 0, 0};
-    double* as = fsm;
-    int* ah = fh;
-    int last = -1;
-    for (int i = 0; i < nref; i++) {
-      if (last >= 0 && L.co[i]) {
-        if (as == fsm && L.cs[i] > L.cs[0] + (c.field_lines0 * mll)) {
-          as = lsm;
-          ah = lh;
-        }
-        const int pk = L.ck[last], ck = L.ck[i];
-        if (pk == kHSync && ck != kHSync) {
-          as[0] = L.cs[i];
-          ah[0] = 1;
-        } else if (pk == kEq && ck == kVSync) {
-          as[1] = L.cs[i];
-          ah[1] = 1;
-        } else if (pk == kVSync && ck == kEq2) {
-          as[2] = L.cs[i];
-          ah[2] = 1;
-        } else if (pk != kHSync && ck == kHSync) {
-          as[3] = L.cs[last];
-          ah[3] = 1;
-        }
-      }
-      last = i;
+    double fsm[4], lsm[4];
+    int fh[4], lh[4];
+    for (int k = 0; k < 4; k++) {
+      const int a = L.mark[k], b = L.mark[4 + k];
+      fh[k] = a >= 0;
+      lh[k] = b >= 0;
+      fsm[k] = a < 0 ? 0.0 : (double)(k == 3 ? L.cs[a - 1] : L.cs[a]);
+      lsm[k] = b < 0 ? 0.0 : (double)(k == 3 ? L.cs[b - 1] : L.cs[b]);
     }
     for (int par = 0; par < 2; par++) {
       const bool first = par == 0;
@@ -1812,30 +1911,99 @@ __global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) 
         for (int j = i + 1; j < nfm; j++) fa = acc_add(fa, acc_pair(fms[i], fml[i], fms[j], fml[j], mll, fhl));
       for (int i = 0; i < nlm; i++)
         for (int j = i + 1; j < nlm; j++) la = acc_add(la, acc_pair(lms[i], lml[i], lms[j], lml[j], mll, fhl));
-      Transitions& tr = out.tr[par];
-      tr.first = finalize(&L, fa, mll, fhl, cur_lines);
-      tr.last = finalize(&L, la, mll, fhl, cur_lines);
-      tr.combined = Estimate{};
+      Acc comb{0.0, 0.0, 0};
       if (fa.n > 0 && la.n > 0) {
         const double fr = fa.loc / fa.n, lr = la.loc / la.n;
         if (fabs(fr - lr) < 0.5 * mll) {
           Acc cr{0.0, 0.0, 0};
           for (int i = 0; i < nfm; i++)
             for (int j = 0; j < nlm; j++) cr = acc_add(cr, acc_pair(fms[i], fml[i], lms[j], lml[j], mll, fhl));
-          tr.combined = finalize(&L, acc_add(acc_add(fa, la), cr), mll, fhl, cur_lines);
+          comb = acc_add(acc_add(fa, la), cr);
         }
       }
+      L.acc[par * 3] = fa;
+      L.acc[par * 3 + 1] = la;
+      L.acc[par * 3 + 2] = comb;
+      L.fhl[par] = fhl;
+    }
+  }
+  __syncthreads();
+  // FinalizeEstimate (Dsp/VBlankSyncResolver.cs:666-718): one wave per estimate; the
+  // alignment offsets add up in pulse order
+  if (wave < 6) {
+    const Acc a = L.acc[wave];
+    const int par = wave / 3;
+    const double fhl = L.fhl[par];
+    const int cur_lines = par == 0 ? c.field_lines0 : c.field_lines1;
+    Estimate e{};
+    if (a.n > 0) {
+      double fh = round_away((a.loc + a.off) / a.n);
+      const double unaligned = fh;
+      int brk = nref;
+      for (int b0 = 0; b0 < nref; b0 += 64) {
+        const int i = b0 + lane;
+        bool bad = false;
+        if (i < nref && L.ck[i] == kHSync && L.co[i]) {
+          const int rl = (int)round_away(((L.cs[i] - fh) / mll) + fhl);
+          bad = rl > cur_lines;
+        }
+        const unsigned long long m = __ballot(bad);
+        if (m) {
+          brk = b0 + __ffsll((long long)m) - 1;
+          break;
+        }
+      }
+      double off = 0.0;
+      int cnt = 0;
+      for (int b0 = 0; b0 < brk; b0 += 64) {
+        const int i = b0 + lane;
+        bool use = false;
+        double term = 0.0;
+        if (i < brk && L.ck[i] == kHSync && L.co[i]) {
+          const int rl = (int)round_away(((L.cs[i] - fh) / mll) + fhl);
+          if (rl >= fhl) {
+            use = true;
+            term = fh + (mll * (rl - fhl)) - L.cs[i];
+          }
+        }
+        const unsigned long long um = __ballot(use);
+        for (int k = 0; k < 64; k++) {
+          const double tk = __shfl(term, k);
+          if ((um >> k) & 1ull) {
+            off += tk;
+            cnt++;
+          }
+        }
+      }
+      if (cnt > 0) fh -= off / cnt;
+      e.ok = 1;
+      e.line0 = fh - (mll * fhl);
+      e.first_hsync = fh;
+      e.unaligned = unaligned;
+      e.valid_count = a.n;
+    }
+    if (lane == 0) L.est[wave] = e;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    FieldLogic& out = w.logic[f];
+    out.raw = np;
+    out.refined = nref;
+    out.hsync_median = t.median;
+    out.mll = mll;
+    for (int par = 0; par < 2; par++) {
+      out.tr[par].first = L.est[par * 3];
+      out.tr[par].last = L.est[par * 3 + 1];
+      out.tr[par].combined = L.est[par * 3 + 2];
     }
   }
   // the refined pulses for the line locations and the rare previous-field path
   int* ks = w.ref_kind + (long long)f * c.pulse_cap;
   int* ss = w.ref_start + (long long)f * c.pulse_cap;
-  for (int i = 0; i < nref; i++) {
+  for (int i = tid; i < nref; i += kT) {
     ks[i] = L.ck[i] | (L.co[i] << 8);
     ss[i] = L.cs[i];
   }
-  out.raw = np;
-  w.logic[f] = out;
 }
 
 // ------------------------------------------------------------------ S5: field chain

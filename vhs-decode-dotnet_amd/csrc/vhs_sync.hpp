@@ -10,6 +10,7 @@ namespace vsync {
 
 constexpr int kPulseLds = 4096;   // pulse lists of a field staged in LDS up to this length
 constexpr int kHistBins = 512;    // hsync-length histogram (3.75 us of lengths)
+constexpr int kLogicAux = 18;     // bytes per pulse of a field's classification lists in global memory
 constexpr int kAvgCap = 16;       // moving-average values kept
 constexpr int kMaxGroups = 256;   // vertical-sync groups per span
 constexpr int kStatusMispredicted = 4;
@@ -155,7 +156,7 @@ struct SyncWork {
   FieldLogic* logic;
   int* ref_kind;
   int* ref_start;
-  unsigned char* logic_aux;  // [max_fields][6 * pulse_cap]: refined lengths, kinds and order flags
+  unsigned char* logic_aux;  // [max_fields][kLogicAux * pulse_cap]: a field's classification lists past kPulseLds pulses
   FieldRecord* records;      // [max_records], the whole call
   const SyncHistoryDev* history_in;
   SyncHistoryDev* history_out;

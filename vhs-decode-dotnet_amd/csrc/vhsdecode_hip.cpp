@@ -1676,7 +1676,7 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
     const size_t o_search = take(sizeof(LevelSearch) * F), o_levels = take(sizeof(FieldLevels) * F);
     const size_t o_before = take(sizeof(LevelStateDev) * F), o_after = take(sizeof(LevelStateDev));
     const size_t o_raw = take(4 * F), o_logic = take(sizeof(FieldLogic) * F);
-    const size_t o_rk = take(4 * F * PC), o_rs = take(4 * F * PC), o_aux = take(6 * F * PC);
+    const size_t o_rk = take(4 * F * PC), o_rs = take(4 * F * PC), o_aux = take((size_t)kLogicAux * F * PC);
     const size_t o_rec = take(sizeof(FieldRecord) * F);
     const size_t o_lines = take(8 * F * line_stride);
     const size_t o_tp = take(8 * F * 6), o_lc = take(4 * F), o_org = take(8 * F), o_fl = take(4 * F);
