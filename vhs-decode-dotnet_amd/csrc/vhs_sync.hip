@@ -1349,20 +1349,51 @@ struct Timing {
   double hsync_lo, hsync_hi, eq_lo, eq_hi, vs_lo, vs_hi, median;
 };
 
-__device__ bool quality(int prev_kind, int prev_start, int kind, int start, double nominal) {
+// The order-quality windows of RefinePulses (SyncAnalysis.cs:286-300) as whole sample
+// distances: the quotient distance / line length rounds monotonically in the distance, so
+// each bound is the first (last) distance whose quotient passes it.  Windows: 0 neither
+// pulse hsync, 1 both hsync, 2 one hsync.
+struct QualityWindows {
+  int lo[3], hi[3];
+};
+
+__device__ long long first_distance_at_least(double lo, double nominal) {
 #pragma clang fp contract(off)
-  double lo, hi;
-  if (prev_kind != kHSync && kind != kHSync) {
-    lo = 0.4;
-    hi = 0.6;
-  } else if (prev_kind == kHSync && kind == kHSync) {
-    lo = 0.9;
-    hi = 1.1;
-  } else {
-    lo = 0.4;
-    hi = 1.1;
+  long long d = (long long)floor(lo * nominal);
+  while ((double)(d - 1) / nominal >= lo) d--;
+  while (!((double)d / nominal >= lo)) d++;
+  return d;
+}
+
+__device__ long long last_distance_at_most(double hi, double nominal) {
+#pragma clang fp contract(off)
+  long long d = (long long)ceil(hi * nominal);
+  while (!((double)d / nominal <= hi)) d--;
+  while ((double)(d + 1) / nominal <= hi) d++;
+  return d;
+}
+
+__device__ QualityWindows quality_windows(double nominal) {
+  QualityWindows q;
+  const double lo[3] = {0.4, 0.9, 0.4}, hi[3] = {0.6, 1.1, 1.1};
+  const bool finite = nominal > 0.0 && nominal < 1e9;
+  for (int k = 0; k < 3; k++) {
+    // the start distances are whole ints, so the bounds clamp to the int range
+    const long long a = finite ? first_distance_at_least(lo[k], nominal) : LLONG_MAX;
+    const long long b = finite ? last_distance_at_most(hi[k], nominal) : LLONG_MIN;
+    q.lo[k] = a > INT_MAX ? INT_MAX : (a < INT_MIN ? INT_MIN : (int)a);
+    q.hi[k] = b > INT_MAX ? INT_MAX : (b < INT_MIN ? INT_MIN : (int)b);
   }
-  return in_range((start - prev_start) / nominal, lo, hi);
+  return q;
+}
+
+__device__ __forceinline__ bool quality_int(const QualityWindows& q, int prev_kind, int prev_start, int kind,
+                                            int start) {
+  const bool ph = prev_kind == kHSync, h = kind == kHSync;
+  const int lo = (!ph && !h) ? q.lo[0] : ((ph && h) ? q.lo[1] : q.lo[2]);
+  const int hi = (!ph && !h) ? q.hi[0] : ((ph && h) ? q.hi[1] : q.hi[2]);
+  const int d = start - prev_start;
+  return d >= lo && d <= hi;
 }
 
 __device__ __forceinline__ double round_half_line(double v) {
@@ -1405,25 +1436,68 @@ __device__ Acc acc_add(Acc a, Acc b) {
   return Acc{a.loc + b.loc, a.off + b.off, a.n + b.n};
 }
 
-__device__ void partial_markers(const double* smp, const int* has, double eq1_line, double section, double* ms,
-                                double* ml, int* nm) {
+// The markers found, in marker order, with their lines (partial marker list of
+// EstimateLine0FromTransitions).  Held in registers: every access names its slot.
+struct MarkerList {
+  double s0, s1, s2, s3, l0, l1, l2, l3;
+  int n;
+  __device__ __forceinline__ void push(double smp, double line) {
+    switch (n) {
+      case 0: s0 = smp; l0 = line; break;
+      case 1: s1 = smp; l1 = line; break;
+      case 2: s2 = smp; l2 = line; break;
+      default: s3 = smp; l3 = line; break;
+    }
+    n++;
+  }
+  __device__ __forceinline__ double smp(int i) const { return i == 0 ? s0 : (i == 1 ? s1 : (i == 2 ? s2 : s3)); }
+  __device__ __forceinline__ double line(int i) const { return i == 0 ? l0 : (i == 1 ? l1 : (i == 2 ? l2 : l3)); }
+};
+
+__device__ __forceinline__ MarkerList partial_markers(const double* smp, const int* has, double eq1_line,
+                                                      double section) {
 #pragma clang fp contract(off)
   const double lines[4] = {eq1_line, eq1_line + section, eq1_line + (section * 2.0),
                            eq1_line + (section * 3.0) - 0.5};
-  int n = 0;
+  MarkerList m{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
+#pragma unroll
   for (int i = 0; i < 4; i++)
-    if (has[i]) {
-      ms[n] = smp[i];
-      ml[n] = lines[i];
-      n++;
-    }
-  *nm = n;
+    if (has[i]) m.push(smp[i], lines[i]);
+  return m;
+}
+
+// The k-th pair (i < j) of n items in the order of the nested loops over i, then j.
+__device__ __forceinline__ void pair_ij(int k, int n, int* i, int* j) {
+  int a = 0;
+  while (k >= n - 1 - a) {
+    k -= n - 1 - a;
+    a++;
+  }
+  *i = a;
+  *j = a + 1 + k;
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+// acc_add over lanes [from, to) of the wave, in lane order, from zero
+__device__ __forceinline__ Acc acc_lanes(const Acc& p, int from, int to) {
+#pragma clang fp contract(off)
+  Acc a{0.0, 0.0, 0};
+  for (int l = from; l < to; l++) {
+    a.loc = a.loc + readlane_d(p.loc, l);
+    a.off = a.off + readlane_d(p.off, l);
+    a.n = a.n + __builtin_amdgcn_readlane(p.n, l);
+  }
+  return a;
 }
 
 // The pulse lists of one field: in LDS up to kPulseLds pulses, else in the field's
 // global lists (a window full of noise).  eff_*: a raw pulse after TryRescueLongHSync as
-// often as it applies; cls_*: ClassifyPulses' kind of the raw and the rescued length;
-// visited: the RefinePulses walk has reached the pulse (so it reads the rescued one).
+// often as it applies; craw/ceff: ClassifyPulses' kind of the raw and the rescued length.
 struct LogicScratch {
   int* rs;            // raw pulses
   int* rl;
@@ -1432,14 +1506,13 @@ struct LogicScratch {
   signed char* craw;  // classify(raw length)
   signed char* ceff;  // classify(rescued length)
   unsigned char* eflag;    // bit 0: hsync length, bit 1: rescue length (rescue failed), bit 2: eq length
-  unsigned char* visited;
   int* cs;            // refined pulses
   int* cl;
   unsigned char* ck;  // kind
   unsigned char* co;  // in order
   int lrs[kPulseLds], lrl[kPulseLds], les[kPulseLds], lel[kPulseLds], lcs[kPulseLds], lcl[kPulseLds];
   signed char lcraw[kPulseLds], lceff[kPulseLds];
-  unsigned char leflag[kPulseLds], lvisited[kPulseLds], lck[kPulseLds], lco[kPulseLds];
+  unsigned char leflag[kPulseLds], lck[kPulseLds], lco[kPulseLds];
   int hist[kHistBins];
   int nref;
   Timing t;
@@ -1504,6 +1577,204 @@ __device__ bool wave_rescue(const Plane& x, long long s, long long e, double ste
   return false;
 }
 
+// RefinePulses (SyncAnalysis.cs:243-314) on one wave: its control flow is uniform, 64
+// pulses' flags and rescued pulses sit in the lanes and are read by lane index, lane 0
+// writes the refined pulses.
+template <bool kLds>
+__device__ __forceinline__ void refine_walk(LogicScratch& L, const SyncCfg& c, int np, double nominal) {
+#pragma clang fp contract(off)
+  {
+    const int lane = threadIdx.x & 63;
+    const QualityWindows qw = quality_windows(nominal);
+    // the lists in LDS are named directly, so their accesses are LDS accesses
+    const unsigned char* eflag = kLds ? L.leflag : L.eflag;
+    const int* es = kLds ? L.les : L.es;
+    const int* el = kLds ? L.lel : L.el;
+    const int* rs = kLds ? L.lrs : L.rs;
+    const int* rl = kLds ? L.lrl : L.rl;
+    const signed char* craw = kLds ? L.lcraw : L.craw;
+    const signed char* ceff = kLds ? L.lceff : L.ceff;
+    int* cs = kLds ? L.lcs : L.cs;
+    int* cl = kLds ? L.lcl : L.cl;
+    unsigned char* ck = kLds ? L.lck : L.ck;
+    unsigned char* co = kLds ? L.lco : L.co;
+    const int cap = c.pulse_cap;
+    int nref = 0, pk = -1, pstart = 0;
+    int pv1 = -1, pv2 = -1;  // the last two pulses the walk reached before this one
+    int cb = -64;            // the first of the 64 pulses held in the lanes
+    int cfl = 0, ces = 0, cel = 0;
+    unsigned long long hmask = 0;  // the lanes' hsync-length flags
+    int index = 0;
+    while (index < np) {
+      // the walk's state is the same on every lane
+      index = __builtin_amdgcn_readfirstlane(index);
+      nref = __builtin_amdgcn_readfirstlane(nref);
+      pk = __builtin_amdgcn_readfirstlane(pk);
+      pstart = __builtin_amdgcn_readfirstlane(pstart);
+      pv1 = __builtin_amdgcn_readfirstlane(pv1);
+      pv2 = __builtin_amdgcn_readfirstlane(pv2);
+      cb = __builtin_amdgcn_readfirstlane(cb);
+      if (index >= cb + 64) {
+        cb = index;
+        const int i = cb + lane;
+        cfl = i < np ? eflag[i] : 0;
+        ces = i < np ? es[i] : 0;
+        cel = i < np ? el[i] : 0;
+        // the loads land here, so the walk's later steps wait for nothing
+        __builtin_amdgcn_s_waitcnt(0);
+        hmask = __ballot((cfl & 1) != 0);
+      }
+      const int k = index - cb;
+      const int fl = __builtin_amdgcn_readlane(cfl, k);
+      if (fl & 1) {
+        // a run of hsync-length pulses is appended as it stands, one lane a pulse: each is
+        // judged in order against the pulse before it (the refined list never outruns
+        // the walk's index, which stays below the list capacity)
+        const unsigned long long rest = ~(hmask >> k);
+        const int run = rest == 0 ? 64 - k : (__builtin_ctzll(rest) < 64 - k ? __builtin_ctzll(rest) : 64 - k);
+        const int j = lane - k;
+        const int prev_s = __shfl_up(ces, 1);
+        if (j >= 0 && j < run) {
+          const bool io = j == 0 ? (nref > 0 && quality_int(qw, pk, pstart, kHSync, ces))
+                                 : quality_int(qw, kHSync, prev_s, kHSync, ces);
+          const int q = nref + j;
+          if (q < cap) {
+            ck[q] = (unsigned char)kHSync;
+            cs[q] = ces;
+            cl[q] = cel;
+            co[q] = io ? 1 : 0;
+          }
+        }
+        nref += run;
+        pk = kHSync;
+        pstart = __builtin_amdgcn_readlane(ces, k + run - 1);
+        pv2 = run >= 2 ? index + run - 2 : pv1;
+        pv1 = index + run - 1;
+        index += run;
+        continue;
+      }
+      // a rescue-length pulse the rescue does not shorten is passed over
+      if ((fl & 2) || !(index > 2 && (fl & 4) && nref > 0 && pk == kHSync)) {
+        pv2 = pv1;
+        pv1 = index;
+        index++;
+        continue;
+      }
+      // TryReadVBlank (:340-471): pulses the walk has reached are read rescued.  The
+      // accepted pulses after the first two go straight behind the refined list and
+      // count when the read completes (the list never outruns the walk's index, so they
+      // stay inside it).
+      const int start = index - 2 > 0 ? index - 2 : 0;
+      const int end_ex = np < index + 24 ? np : index + 24;
+      int nv = 0, state = -1, lastk = -1, lasts = 0, written = 0, wk = -1, ws = 0;
+      double state_end = 0.0, state_len = 0.0;
+      bool has_len = false, done = false;
+      // the read's pulses, one a lane (at most 26)
+      int vrk = -1, vst = 0, vln = 0;
+      {
+        const int i = start + lane;
+        if (i < end_ex) {
+          const bool seen = i == index || i == pv1 || i == pv2;
+          vrk = seen ? ceff[i] : craw[i];
+          vst = seen ? es[i] : rs[i];
+          vln = seen ? el[i] : rl[i];
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+      }
+      for (int i = start; i < end_ex && !done; i++) {
+        const int rk = __builtin_amdgcn_readlane(vrk, i - start);
+        if (rk < 0) continue;
+        const int pst = __builtin_amdgcn_readlane(vst, i - start);
+        int acc = -1;
+        bool fin = false;
+        switch (state) {
+          case -1:
+            if (rk == kHSync) acc = kHSync;
+            break;
+          case kHSync:
+            if (rk == kHSync) acc = kHSync;
+            else if (rk == kEq) {
+              acc = kEq;
+              has_len = true;
+              state_len = c.num_pulses / 2.0;
+            } else if (rk == kVSync)
+              acc = kVSync;
+            break;
+          case kEq:
+            if (rk == kEq) acc = kEq;
+            else if (rk == kVSync) {
+              acc = kVSync;
+              has_len = true;
+              state_len = c.num_pulses / 2.0;
+            } else if (rk == kHSync)
+              acc = kHSync;
+            break;
+          case kVSync:
+            if (rk == kEq) {
+              acc = kEq2;
+              has_len = true;
+              state_len = c.num_pulses / 2.0;
+            } else if (rk == kVSync)
+              acc = kVSync;
+            else if (rk == kHSync && pst > state_end)
+              acc = kHSync;
+            break;
+          case kEq2:
+            if (rk == kEq) acc = kEq2;
+            else if (rk == kHSync) {
+              acc = kHSync;
+              fin = true;
+            }
+            break;
+        }
+        if (acc < 0) continue;
+        if (acc != state) {
+          if (pst < state_end) acc = -1;
+          else if (has_len) {
+            state_end = pst + ((state_len - 0.1) * nominal);
+            has_len = false;
+          }
+        }
+        if (acc >= 0) {
+          const bool io = nv > 0 && quality_int(qw, lastk, lasts, acc, pst);
+          if (nv < 32) {
+            if (nv >= 2 && nref + (nv - 2) < cap) {
+              if (lane == 0) {
+                const int q = nref + (nv - 2);
+                ck[q] = (unsigned char)acc;
+                cs[q] = pst;
+                cl[q] = __builtin_amdgcn_readlane(vln, i - start);
+                co[q] = io ? 1 : 0;
+              }
+              written++;
+              wk = acc;
+              ws = pst;
+            }
+            lastk = acc;
+            lasts = pst;
+            nv++;
+          }
+          state = acc;
+        }
+        if (fin) done = true;
+      }
+      pv2 = pv1;
+      pv1 = index;
+      if (!done) {
+        index++;
+        continue;
+      }
+      if (written > 0) {
+        nref += written;
+        pk = wk;
+        pstart = ws;
+      }
+      index += (nv - 2 > 1 ? nv - 2 : 1);
+    }
+    if (lane == 0) L.nref = nref;
+  }
+}
+
 __global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) {
 #pragma clang fp contract(off)
   __shared__ LogicScratch L;
@@ -1525,9 +1796,8 @@ __global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) 
     L.craw = lds ? L.lcraw : reinterpret_cast<signed char*>(aux + 12 * pc);
     L.ceff = lds ? L.lceff : reinterpret_cast<signed char*>(aux + 13 * pc);
     L.eflag = lds ? L.leflag : aux + 14 * pc;
-    L.visited = lds ? L.lvisited : aux + 15 * pc;
-    L.ck = lds ? L.lck : aux + 16 * pc;
-    L.co = lds ? L.lco : aux + 17 * pc;
+    L.ck = lds ? L.lck : aux + 15 * pc;
+    L.co = lds ? L.lco : aux + 16 * pc;
     L.cs = lds ? L.lcs : w.ref_start + (long long)f * c.pulse_cap;
     L.g0 = INT_MAX;
     L.run = 0;
@@ -1616,7 +1886,6 @@ __global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) 
   for (int i = tid; i < np; i += kT) {
     const int len = L.rl[i];
     L.craw[i] = (signed char)classify(len, t);
-    L.visited[i] = 0;
     L.es[i] = L.rs[i];
     L.el[i] = len;
   }
@@ -1654,128 +1923,10 @@ __global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) 
                                  (in_range(len, t.eq_lo, t.eq_hi) ? 4 : 0));
   }
   __syncthreads();
-  if (tid == 0) {
-    int nref = 0, pk = -1, pstart = 0;
-    auto add = [&](int kind, int st, int ln) {
-      const bool io = nref > 0 && quality(pk, pstart, kind, st, nominal);
-      if (nref < c.pulse_cap) {
-        L.ck[nref] = (unsigned char)kind;
-        L.cs[nref] = st;
-        L.cl[nref] = ln;
-        L.co[nref] = io ? 1 : 0;
-        nref++;
-        pk = kind;
-        pstart = st;
-      }
-    };
-    int index = 0;
-    while (index < np) {
-      L.visited[index] = 1;
-      const int fl = L.eflag[index];
-      if (fl & 1) {
-        add(kHSync, L.es[index], L.el[index]);
-        index++;
-        continue;
-      }
-      // a rescue-length pulse the rescue does not shorten is passed over
-      if ((fl & 2) || !(index > 2 && (fl & 4) && nref > 0 && pk == kHSync)) {
-        index++;
-        continue;
-      }
-      // TryReadVBlank (:340-471): pulses the walk has reached are read rescued
-      const int start = index - 2 > 0 ? index - 2 : 0;
-      const int end_ex = np < index + 24 ? np : index + 24;
-      int vk[32], vs[32], vl[32];
-      unsigned char vo[32];
-      int nv = 0, state = -1;
-      double state_end = 0.0, state_len = 0.0;
-      bool has_len = false, done = false;
-      for (int i = start; i < end_ex && !done; i++) {
-        const bool seen = L.visited[i] != 0;
-        const int rk = seen ? L.ceff[i] : L.craw[i];
-        if (rk < 0) continue;
-        const int pst = seen ? L.es[i] : L.rs[i];
-        int acc = -1;
-        bool fin = false;
-        switch (state) {
-          case -1:
-            if (rk == kHSync) acc = kHSync;
-            break;
-          case kHSync:
-            if (rk == kHSync) acc = kHSync;
-            else if (rk == kEq) {
-              acc = kEq;
-              has_len = true;
-              state_len = c.num_pulses / 2.0;
-            } else if (rk == kVSync)
-              acc = kVSync;
-            break;
-          case kEq:
-            if (rk == kEq) acc = kEq;
-            else if (rk == kVSync) {
-              acc = kVSync;
-              has_len = true;
-              state_len = c.num_pulses / 2.0;
-            } else if (rk == kHSync)
-              acc = kHSync;
-            break;
-          case kVSync:
-            if (rk == kEq) {
-              acc = kEq2;
-              has_len = true;
-              state_len = c.num_pulses / 2.0;
-            } else if (rk == kVSync)
-              acc = kVSync;
-            else if (rk == kHSync && pst > state_end)
-              acc = kHSync;
-            break;
-          case kEq2:
-            if (rk == kEq) acc = kEq2;
-            else if (rk == kHSync) {
-              acc = kHSync;
-              fin = true;
-            }
-            break;
-        }
-        if (acc < 0) continue;
-        if (acc != state) {
-          if (pst < state_end) acc = -1;
-          else if (has_len) {
-            state_end = pst + ((state_len - 0.1) * nominal);
-            has_len = false;
-          }
-        }
-        if (acc >= 0) {
-          const bool io = nv > 0 && quality(vk[nv - 1], vs[nv - 1], acc, pst, nominal);
-          if (nv < 32) {
-            vk[nv] = acc;
-            vs[nv] = pst;
-            vl[nv] = seen ? L.el[i] : L.rl[i];
-            vo[nv] = io ? 1 : 0;
-            nv++;
-          }
-          state = acc;
-        }
-        if (fin) done = true;
-      }
-      if (!done) {
-        index++;
-        continue;
-      }
-      for (int k = 2; k < nv; k++) {
-        if (nref < c.pulse_cap) {
-          L.ck[nref] = (unsigned char)vk[k];
-          L.cs[nref] = vs[k];
-          L.cl[nref] = vl[k];
-          L.co[nref] = vo[k];
-          nref++;
-          pk = vk[k];
-          pstart = vs[k];
-        }
-      }
-      index += (nv - 2 > 1 ? nv - 2 : 1);
-    }
-    L.nref = nref;
+  // the walk on wave 0
+  if (__builtin_amdgcn_readfirstlane(wave) == 0) {
+    if (lds) refine_walk<true>(L, c, np, nominal);
+    else refine_walk<false>(L, c, np, nominal);
   }
   __syncthreads();
   const int nref = L.nref;
@@ -1857,8 +2008,9 @@ __global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) 
     }
   }
   __syncthreads();
-  if (tid == 0) {
-    FieldLogic& out = w.logic[f];
+  // consensus and the line-0 marker pairs on wave 0: one lane a pair (32 lanes a parity),
+  // the pairs added up in the nested loops' order
+  if (__builtin_amdgcn_readfirstlane(wave) == 0) {
     double lens[4];
     for (int k = 0; k < 4; k++) {
       const int i = L.slot[k];
@@ -1866,26 +2018,27 @@ __global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) 
     }
     const double ffn[4] = {1.0, 0.5, 0.5, 1.0}, sfn[4] = {0.5, 1.0, 1.0, 0.5};
     const double ffp[4] = {0.5, 0.5, 1.0, 1.0}, sfp[4] = {1.0, 1.0, 0.5, 0.5};
-    const double* ff = c.is_pal ? ffp : ffn;
-    const double* sf = c.is_pal ? sfp : sfn;
     int cons = 0, det = 0;
     for (int i = 0; i < 4; i++) {
       if (lens[i] < 0.0) continue;
-      if (fabs(lens[i] - ff[i]) < 1e-9) {
+      if (fabs(lens[i] - (c.is_pal ? ffp[i] : ffn[i])) < 1e-9) {
         cons++;
         det++;
-      } else if (fabs(lens[i] - sf[i]) < 1e-9) {
+      } else if (fabs(lens[i] - (c.is_pal ? sfp[i] : sfn[i])) < 1e-9) {
         det++;
       }
     }
-    out.consensus = cons;
-    out.detected = det;
-    out.first_conf = 0;
-    out.second_conf = 0;
-    if (det > 0) {
-      const double wgt = det / 4.0;
-      out.first_conf = (int)round_away((cons / (double)det) * wgt * 100.0);
-      out.second_conf = (int)round_away(((det - cons) / (double)det) * wgt * 100.0);
+    if (lane == 0) {
+      FieldLogic& out = w.logic[f];
+      out.consensus = cons;
+      out.detected = det;
+      out.first_conf = 0;
+      out.second_conf = 0;
+      if (det > 0) {
+        const double wgt = det / 4.0;
+        out.first_conf = (int)round_away((cons / (double)det) * wgt * 100.0);
+        out.second_conf = (int)round_away(((det - cons) / (double)det) * wgt * 100.0);
+      }
     }
     double fsm[4], lsm[4];
     int fh[4], lh[4];
@@ -1896,41 +2049,53 @@ __global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) 
       fsm[k] = a < 0 ? 0.0 : (double)(k == 3 ? L.cs[a - 1] : L.cs[a]);
       lsm[k] = b < 0 ? 0.0 : (double)(k == 3 ? L.cs[b - 1] : L.cs[b]);
     }
-    for (int par = 0; par < 2; par++) {
-      const bool first = par == 0;
-      const int cur_lines = first ? c.field_lines0 : c.field_lines1;
-      const Bounds b = field_bounds(c.is_pal, first);
-      const double section = c.num_pulses / 2.0;
-      const double fhl = b.first_hsync + (section * 3.0) - 0.5 + b.first_eq2;
-      double fms[4], fml[4], lms[4], lml[4];
-      int nfm, nlm;
-      partial_markers(fsm, fh, b.first_hsync, section, fms, fml, &nfm);
-      partial_markers(lsm, lh, cur_lines + b.last_hsync, section, lms, lml, &nlm);
-      Acc fa{0.0, 0.0, 0}, la{0.0, 0.0, 0};
-      for (int i = 0; i < nfm; i++)
-        for (int j = i + 1; j < nfm; j++) fa = acc_add(fa, acc_pair(fms[i], fml[i], fms[j], fml[j], mll, fhl));
-      for (int i = 0; i < nlm; i++)
-        for (int j = i + 1; j < nlm; j++) la = acc_add(la, acc_pair(lms[i], lml[i], lms[j], lml[j], mll, fhl));
+    const int par = lane >> 5, idx = lane & 31;
+    const bool first = par == 0;
+    const int cur_lines = first ? c.field_lines0 : c.field_lines1;
+    const Bounds b = field_bounds(c.is_pal, first);
+    const double section = c.num_pulses / 2.0;
+    const double fhl = b.first_hsync + (section * 3.0) - 0.5 + b.first_eq2;
+    const MarkerList fm = partial_markers(fsm, fh, b.first_hsync, section);
+    const MarkerList lm = partial_markers(lsm, lh, cur_lines + b.last_hsync, section);
+    // which markers were found does not depend on the parity
+    const int nf2 = fm.n * (fm.n - 1) / 2, nl2 = lm.n * (lm.n - 1) / 2, nx = fm.n * lm.n;
+    Acc p{0.0, 0.0, 0};
+    if (idx < nf2 + nl2 + nx) {
+      int i, j;
+      if (idx < nf2) {
+        pair_ij(idx, fm.n, &i, &j);
+        p = acc_pair(fm.smp(i), fm.line(i), fm.smp(j), fm.line(j), mll, fhl);
+      } else if (idx < nf2 + nl2) {
+        pair_ij(idx - nf2, lm.n, &i, &j);
+        p = acc_pair(lm.smp(i), lm.line(i), lm.smp(j), lm.line(j), mll, fhl);
+      } else {
+        const int q = idx - nf2 - nl2;
+        i = q / lm.n;
+        j = q - i * lm.n;
+        p = acc_pair(fm.smp(i), fm.line(i), lm.smp(j), lm.line(j), mll, fhl);
+      }
+    }
+    for (int pr = 0; pr < 2; pr++) {
+      const int base = pr * 32;
+      const Acc fa = acc_lanes(p, base, base + nf2);
+      const Acc la = acc_lanes(p, base + nf2, base + nf2 + nl2);
       Acc comb{0.0, 0.0, 0};
       if (fa.n > 0 && la.n > 0) {
         const double fr = fa.loc / fa.n, lr = la.loc / la.n;
-        if (fabs(fr - lr) < 0.5 * mll) {
-          Acc cr{0.0, 0.0, 0};
-          for (int i = 0; i < nfm; i++)
-            for (int j = 0; j < nlm; j++) cr = acc_add(cr, acc_pair(fms[i], fml[i], lms[j], lml[j], mll, fhl));
-          comb = acc_add(acc_add(fa, la), cr);
-        }
+        if (fabs(fr - lr) < 0.5 * mll) comb = acc_add(acc_add(fa, la), acc_lanes(p, base + nf2 + nl2, base + nf2 + nl2 + nx));
       }
-      L.acc[par * 3] = fa;
-      L.acc[par * 3 + 1] = la;
-      L.acc[par * 3 + 2] = comb;
-      L.fhl[par] = fhl;
+      if (lane == 0) {
+        L.acc[pr * 3] = fa;
+        L.acc[pr * 3 + 1] = la;
+        L.acc[pr * 3 + 2] = comb;
+      }
     }
+    if (idx == 0) L.fhl[par] = fhl;
   }
   __syncthreads();
   // FinalizeEstimate (Dsp/VBlankSyncResolver.cs:666-718): one wave per estimate; the
   // alignment offsets add up in pulse order
-  if (wave < 6) {
+  if (__builtin_amdgcn_readfirstlane(wave) < 6) {
     const Acc a = L.acc[wave];
     const int par = wave / 3;
     const double fhl = L.fhl[par];
@@ -1966,13 +2131,12 @@ __global__ void __launch_bounds__(kT) field_logic_kernel(SyncCfg c, SyncWork w) 
             term = fh + (mll * (rl - fhl)) - L.cs[i];
           }
         }
-        const unsigned long long um = __ballot(use);
-        for (int k = 0; k < 64; k++) {
-          const double tk = __shfl(term, k);
-          if ((um >> k) & 1ull) {
-            off += tk;
-            cnt++;
-          }
+        unsigned long long um = __ballot(use);
+        while (um) {
+          const int k = __builtin_ctzll(um);
+          um &= um - 1;
+          off += readlane_d(term, k);
+          cnt++;
         }
       }
       if (cnt > 0) fh -= off / cnt;
@@ -2235,61 +2399,159 @@ __global__ void field_chain_kernel(SyncCfg c, SyncWork w) {
 }
 
 // ------------------------------------------------------------------ S6: line locations
+constexpr int kLineLds = 512;  // processed lines per field (the line stride is at most 511)
+
 // BuildUpstreamLineLocations (SyncAnalysis.cs:585-657) from trunc(first hsync) at line
-// FirstHSyncLine, written window-relative for every processed line.
+// FirstHSyncLine, written window-relative for every processed line.  A line's search walks
+// the sorted starts from the current pulse while the next start is not farther from the
+// expected position, and takes where it stops when that is within mll / 1.5.  Over sorted
+// starts the distance falls up to the expected position, so a walk that begins at or
+// before J (the first index from one below the lower bound whose successor is strictly
+// farther, at most n - 2) stops at J: J and its distance are found for every line at
+// once, and the pass over the lines that carries the current pulse walks only where the
+// current pulse has passed J.
 __global__ void __launch_bounds__(kT) field_lines_kernel(SyncCfg c, SyncWork w) {
 #pragma clang fp contract(off)
+  __shared__ int lsrc[kPulseLds];
   __shared__ int lsrt[kPulseLds];
+  __shared__ int lj[kLineLds];
+  __shared__ int lsel[kLineLds];
+  __shared__ unsigned char lok[kLineLds];
+  __shared__ int unsorted;
   const int f = blockIdx.x;
+  const int tid = threadIdx.x;
   if (f >= w.ctl->decoded) return;
   const FieldRecord rec = w.records[w.ctl->base + f];
   const int n = rec.refined_pulses;
   int* ss = w.ref_start + (long long)f * c.pulse_cap;
-  // sorted in LDS, or in place (the field chain has read the list) when it is longer
-  int* srt = n <= kPulseLds ? lsrt : ss;
-  if (n <= kPulseLds)
-    for (int i = threadIdx.x; i < n; i += kT) lsrt[i] = ss[i];
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  for (int i = 1; i < n; i++) {  // nearly sorted: insertion sort (OrderBy is stable; equal keys are equal values)
-    const int v = srt[i];
-    int j = i - 1;
-    while (j >= 0 && srt[j] > v) {
-      srt[j + 1] = srt[j];
-      j--;
-    }
-    srt[j + 1] = v;
-  }
   const double mll = rec.mean_line_length;
   const double ref_pulse = trunc(rec.first_hsync);
   const int ref_line = (int)rec.first_hsync_line;
   const int lines = c.output_lines + 10;
   double* out = w.lines + (long long)f * c.line_stride;
-  int cur = 0;
   const double maxd = mll / 1.5;
-  for (int line = 0; line < lines; line++) {
-    const double expected = ref_pulse + (mll * (line - ref_line));
-    out[line] = expected;
-    if (cur >= n) continue;
-    double cd = fabs(srt[cur] - expected);
-    double smallest = maxd;
-    double sel = -1.0;
-    int si = cur;
-    while (si < n - 1) {
-      if (cd <= smallest) {
-        smallest = cd;
-        cur = si;
-        sel = srt[si];
+  if (n > kPulseLds) {
+    // a window full of noise: sorted in place (the field chain has read the list) and
+    // walked line by line
+    if (tid != 0) return;
+    for (int i = 1; i < n; i++) {  // nearly sorted: insertion sort (equal keys are equal values)
+      const int v = ss[i];
+      int j = i - 1;
+      while (j >= 0 && ss[j] > v) {
+        ss[j + 1] = ss[j];
+        j--;
       }
-      const double nd = fabs(srt[si + 1] - expected);
-      if (nd > cd) break;
-      cd = nd;
-      si++;
+      ss[j + 1] = v;
     }
-    if (sel >= 0.0) {
-      out[line] = sel;
-      cur++;
+    int cur = 0;
+    for (int line = 0; line < lines; line++) {
+      const double expected = ref_pulse + (mll * (line - ref_line));
+      out[line] = expected;
+      if (cur >= n) continue;
+      double cd = fabs(ss[cur] - expected);
+      double smallest = maxd;
+      double sel = -1.0;
+      int si = cur;
+      while (si < n - 1) {
+        if (cd <= smallest) {
+          smallest = cd;
+          cur = si;
+          sel = ss[si];
+        }
+        const double nd = fabs(ss[si + 1] - expected);
+        if (nd > cd) break;
+        cd = nd;
+        si++;
+      }
+      if (sel >= 0.0) {
+        out[line] = sel;
+        cur++;
+      }
     }
+    return;
+  }
+  if (tid == 0) unsorted = 0;
+  for (int i = tid; i < n; i += kT) lsrc[i] = ss[i];
+  __syncthreads();
+  for (int i = tid; i + 1 < n; i += kT)
+    if (lsrc[i] > lsrc[i + 1]) unsorted = 1;
+  __syncthreads();
+  if (unsorted) {
+    // rank sort (OrderBy is stable, and equal keys are equal values)
+    for (int i = tid; i < n; i += kT) {
+      const int v = lsrc[i];
+      int r = 0;
+      for (int j = 0; j < n; j++) {
+        const int u = lsrc[j];
+        r += (u < v || (u == v && j < i)) ? 1 : 0;
+      }
+      lsrt[r] = v;
+    }
+  } else {
+    for (int i = tid; i < n; i += kT) lsrt[i] = lsrc[i];
+  }
+  __syncthreads();
+  const int* srt = lsrt;
+  // J and the acceptance at J for every line
+  for (int line = tid; line < lines; line += kT) {
+    if (n < 2) break;
+    const double e = ref_pulse + (mll * (line - ref_line));
+    int lo = 0, hi = n;  // lower bound: the first start >= e
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if ((double)srt[mid] < e) lo = mid + 1;
+      else hi = mid;
+    }
+    int j = lo - 1 > 0 ? lo - 1 : 0;
+    while (j < n - 2 && !(fabs(srt[j + 1] - e) > fabs(srt[j] - e))) j++;
+    if (j > n - 2) j = n - 2;
+    lj[line] = j;
+    lok[line] = (unsigned char)((fabs(srt[j] - e) <= maxd ? 1 : 0) | ((double)srt[j] >= 0.0 ? 2 : 0));
+  }
+  __syncthreads();
+  // the pass that carries the current pulse, on wave 0: 64 lines' J and acceptance sit
+  // in the lanes and are read by lane index
+  if (tid < 64) {
+    const int lane = tid;
+    int cur = 0;
+    for (int l0 = 0; l0 < lines; l0 += 64) {
+      const int l = l0 + lane;
+      const int myj = l < lines ? lj[l] : 0;
+      const int myok = l < lines ? lok[l] : 0;
+      const int cnt = lines - l0 < 64 ? lines - l0 : 64;
+      int mysel = -1;
+      for (int k = 0; k < cnt; k++) {
+        const int jl = __builtin_amdgcn_readlane(myj, k), okl = __builtin_amdgcn_readlane(myok, k);
+        int sel = -1;
+        if (cur < n - 1) {
+          int t, ok;
+          if (cur <= jl) {
+            t = jl;
+            ok = okl;
+          } else {
+            const double e = ref_pulse + (mll * ((l0 + k) - ref_line));
+            t = cur;
+            while (t < n - 2 && !(fabs(srt[t + 1] - e) > fabs(srt[t] - e))) t++;
+            ok = (fabs(srt[t] - e) <= maxd ? 1 : 0) | ((double)srt[t] >= 0.0 ? 2 : 0);
+          }
+          // the walk's pick moves the current pulse; a non-negative pick is taken and passed
+          if (ok & 1) {
+            cur = t;
+            if (ok & 2) {
+              sel = t;
+              cur = t + 1;
+            }
+          }
+        }
+        if (lane == k) mysel = sel;
+      }
+      if (l < lines) lsel[l] = mysel;
+    }
+  }
+  __syncthreads();
+  for (int line = tid; line < lines; line += kT) {
+    const int sel = lsel[line];
+    out[line] = sel >= 0 ? (double)srt[sel] : ref_pulse + (mll * (line - ref_line));
   }
 }
 

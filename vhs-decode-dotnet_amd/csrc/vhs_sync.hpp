@@ -10,7 +10,7 @@ namespace vsync {
 
 constexpr int kPulseLds = 4096;   // pulse lists of a field staged in LDS up to this length
 constexpr int kHistBins = 512;    // hsync-length histogram (3.75 us of lengths)
-constexpr int kLogicAux = 18;     // bytes per pulse of a field's classification lists in global memory
+constexpr int kLogicAux = 17;     // bytes per pulse of a field's classification lists in global memory
 constexpr int kAvgCap = 16;       // moving-average values kept
 constexpr int kMaxGroups = 256;   // vertical-sync groups per span
 constexpr int kStatusMispredicted = 4;
