@@ -40,7 +40,6 @@ namespace vsync {
 
 constexpr int kT = 1024;   // threads of the per-field workgroups
 constexpr int kRun = 16;   // samples per thread per pulse tile
-constexpr int kTile = kT * kRun;
 constexpr int kMaxMeans = 256;    // pulse means per level refinement
 
 __device__ __forceinline__ double round_away(double x) { return round(x); }
@@ -576,7 +575,8 @@ __device__ RefineOut wg_refine_levels(const Plane& x, double s0, double b0, cons
 
 // ------------------------------------------------------------------ segmented pulse search
 // The pulse searches over whole read windows split into segments of kSegLen samples, one
-// 256-thread workgroup each, so the fields' windows keep every CU busy.  A segment's
+// 256-thread workgroup each (one tile of kRun samples a thread), so the fields' windows
+// keep every CU busy and a workgroup waits for one round of loads.  A segment's
 // summary holds its first transition (and whether it ends a pulse), its last transition
 // and the pulses (in range) whose start and end both lie in it; combining the summaries
 // in order adds the pulses that cross a segment boundary (a segment's first transition
@@ -585,7 +585,7 @@ __device__ RefineOut wg_refine_levels(const Plane& x, double s0, double b0, cons
 // pass writes the pulses in order.  Counts, lists and pairings are those of
 // wg_find_pulses over the whole window.
 constexpr int kSegT = 256;
-constexpr int kSegTiles = 4;
+constexpr int kSegTiles = 1;
 static_assert(kSegLen == kSegT * kRun * kSegTiles, "segment length");
 
 struct SegScratch {
@@ -744,6 +744,7 @@ __device__ SegSummary seg_scan(const Plane& x, long long s0, long long s1, doubl
 struct SegJob {
   const double* thr;
   int R;
+  int r0;                  // first threshold of a count launch (grid y counts from it)
   const double* dc;        // per field, or null
   int min_len, max_len;
   double vs_min;
@@ -764,7 +765,7 @@ __device__ __forceinline__ Plane job_plane(const SyncCfg& c, const SyncWork& w, 
 
 __global__ void __launch_bounds__(kSegT) seg_count_kernel(SyncCfg c, SyncWork w, SegJob j) {
   __shared__ SegScratch ps;
-  const int sg = blockIdx.x, r = blockIdx.y, f = blockIdx.z;
+  const int sg = blockIdx.x, r = blockIdx.y + j.r0, f = blockIdx.z;
   if (f >= *w.spec_count || (j.active && !j.active[f])) return;
   const Plane x = job_plane(c, w, j, f);
   const long long s0 = (long long)sg * kSegLen;
@@ -776,42 +777,57 @@ __global__ void __launch_bounds__(kSegT) seg_count_kernel(SyncCfg c, SyncWork w,
   if (threadIdx.x == 0) j.sum[((long long)f * j.R + r) * j.segs + sg] = sm;
 }
 
-// Segment summaries of one threshold, in order: the pulse count, the long-pulse count
-// and (with seg_off) each segment's offset and the transition carried into it.
-__device__ void seg_combine(const SegJob& j, int f, int r, int* np, int* nvs, int* seg_carry, int* seg_off) {
+// Segment summaries of one threshold, in order, on one wave: the pulse count, the
+// long-pulse count and (with seg_off) each segment's offset and the transition carried
+// into it.  The lanes hold 64 segments at a time; the running state steps through them by
+// lane index.
+__device__ void wave_seg_combine(const SegJob& j, int f, int r, int* np, int* nvs, int* seg_carry, int* seg_off) {
+  const int lane = threadIdx.x & 63;
   int last = -1, count = 0, vs = 0;
   const SegSummary* sm = j.sum + ((long long)f * j.R + r) * j.segs;
-  for (int sg = 0; sg < j.segs; sg++) {
-    if (seg_off) {
-      seg_carry[sg] = last;
-      seg_off[sg] = count;
-    }
-    const SegSummary a = sm[sg];
-    if (a.first_pos < 0) continue;
-    if (a.first_end && last > 0) {
-      const int len = a.first_pos - last;
-      if (in_range((double)len, j.min_len, j.max_len)) {
-        count++;
-        if ((double)len > j.vs_min) vs++;
+  for (int b0 = 0; b0 < j.segs; b0 += 64) {
+    const int sg = b0 + lane;
+    SegSummary a{-1, 0, -1, 0, 0};
+    if (sg < j.segs) a = sm[sg];
+    const int n = j.segs - b0 < 64 ? j.segs - b0 : 64;
+    int my_carry = -1, my_off = 0;
+    for (int k = 0; k < n; k++) {
+      if (lane == k) {
+        my_carry = last;
+        my_off = count;
       }
+      const int fp = __builtin_amdgcn_readlane(a.first_pos, k);
+      if (fp < 0) continue;
+      if (__builtin_amdgcn_readlane(a.first_end, k) && last > 0) {
+        const int len = fp - last;
+        if (in_range((double)len, j.min_len, j.max_len)) {
+          count++;
+          if ((double)len > j.vs_min) vs++;
+        }
+      }
+      count += __builtin_amdgcn_readlane(a.n_in, k);
+      vs += __builtin_amdgcn_readlane(a.n_vs, k);
+      last = __builtin_amdgcn_readlane(a.last_pos, k);
     }
-    count += a.n_in;
-    vs += a.n_vs;
-    last = a.last_pos;
+    if (seg_off && sg < j.segs) {
+      seg_carry[sg] = my_carry;
+      seg_off[sg] = my_off;
+    }
   }
   *np = count;
   if (nvs) *nvs = vs;
 }
 
-// One threshold per field (j.choice, or 0): the list's segment offsets and its count.
-__global__ void seg_plan_kernel(SyncCfg c, SyncWork w, SegJob j, int max_fields) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+// One threshold per field (j.choice, or 0), one wave a field: the list's segment offsets
+// and its count.
+__global__ void __launch_bounds__(64) seg_plan_kernel(SyncCfg c, SyncWork w, SegJob j, int max_fields) {
+  const int f = blockIdx.x;
   if (f >= max_fields || f >= *w.spec_count || (j.active && !j.active[f])) return;
   const int r = j.choice ? j.choice[f] : 0;
   if (r < 0) return;
   int np;
-  seg_combine(j, f, r, &np, nullptr, j.seg_carry + (long long)f * j.segs, j.seg_off + (long long)f * j.segs);
-  j.count[f] = np;
+  wave_seg_combine(j, f, r, &np, nullptr, j.seg_carry + (long long)f * j.segs, j.seg_off + (long long)f * j.segs);
+  if (threadIdx.x == 0) j.count[f] = np;
 }
 
 __global__ void __launch_bounds__(kSegT) seg_write_kernel(SyncCfg c, SyncWork w, SegJob j) {
@@ -960,18 +976,24 @@ __device__ __forceinline__ double next_min_sync(const SyncCfg& c, double min_syn
   return blank + scaled;
 }
 
-__global__ void search_thresholds_kernel(SyncCfg c, SyncWork w, int max_fields) {
+// The window minimum from its segments' (one wave a field; the minimum of finite values
+// does not depend on the order, and a NaN anywhere makes it NaN as Math.Min does), then
+// the level search's threshold sequence.
+__global__ void __launch_bounds__(64) search_thresholds_kernel(SyncCfg c, SyncWork w, int max_fields) {
 #pragma clang fp contract(off)
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.x;
   if (f >= max_fields || f >= *w.spec_count) return;
   const long long n = w.win_len[f];
   const int segs = (int)((n + kSegLen - 1) / kSegLen);
   double mn = INFINITY;
   bool nan = false;
-  for (int sg = 0; sg < segs; sg++) {
+  for (int sg = threadIdx.x; sg < segs; sg += 64) {
     mn = fmin(mn, w.seg.seg_min[(long long)f * w.seg.segs + sg]);
     nan |= w.seg.seg_nan[(long long)f * w.seg.segs + sg] != 0;
   }
+  for (int o = 32; o > 0; o >>= 1) mn = fmin(mn, __shfl_xor(mn, o));
+  nan = __ballot(nan) != 0;
+  if (threadIdx.x != 0) return;
   if (nan) mn = __builtin_nan("");
   double min_sync = mn;
   for (int r = 0; r < kSpecR; r++) {
@@ -979,26 +1001,42 @@ __global__ void search_thresholds_kernel(SyncCfg c, SyncWork w, int max_fields) 
     min_sync = next_min_sync(c, min_sync);
   }
   w.seg.fallback[f] = 0;
+  w.seg.pending[f] = 0;
 }
 
 // SearchFallbackSerrationLevels' retry decisions (LevelDetection.cs:265-396) over the
-// counts of the first kSpecR thresholds.
-__global__ void search_decide_kernel(SyncCfg c, SyncWork w, SegJob j, int max_fields) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= max_fields || f >= *w.spec_count) return;
+// counts of the first `avail` thresholds, one wave a threshold's counts.  A search that
+// has not decided within them is pending (its next thresholds are counted and it is
+// decided again over all kSpecR); one that has not decided within kSpecR falls back to
+// the whole search on one workgroup.
+__global__ void __launch_bounds__(64 * kSpecR) search_decide_kernel(SyncCfg c, SyncWork w, SegJob j, int avail,
+                                                                     int only_pending) {
+  __shared__ int cnt[kSpecR], vcs[kSpecR];
+  const int f = blockIdx.x;
+  if (f >= *w.spec_count || (only_pending && !w.seg.pending[f])) return;
+  const int r = threadIdx.x >> 6;
+  if (r < avail) {
+    int np, vc;
+    wave_seg_combine(j, f, r, &np, &vc, nullptr, nullptr);
+    if ((threadIdx.x & 63) == 0) {
+      cnt[r] = np;
+      vcs[r] = vc;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   int prev_vsync = 0, prev_r = 0;
   bool found = false, check_next = true;
   int choice = -1;
-  for (int r = 0; r < kSpecR && choice < 0; r++) {
-    int np, vc;
-    seg_combine(j, f, r, &np, &vc, nullptr, nullptr);
+  for (int k = 0; k < avail && choice < 0; k++) {
+    const int np = cnt[k], vc = vcs[k];
     if (np > 200 && vc > 4) {
       if (vc == 12 && !check_next) {
-        choice = r;
+        choice = k;
       } else if (!found || vc > prev_vsync) {
         found = true;
         prev_vsync = vc;
-        prev_r = r;
+        prev_r = k;
         check_next = true;
       } else if (vc < prev_vsync || !check_next) {
         choice = prev_r;
@@ -1007,8 +1045,10 @@ __global__ void search_decide_kernel(SyncCfg c, SyncWork w, SegJob j, int max_fi
       }
     }
   }
+  const bool more = choice < 0 && avail < kSpecR;
+  w.seg.pending[f] = more ? 1 : 0;
   w.seg.choice[f] = choice;
-  w.seg.fallback[f] = choice < 0 ? 1 : 0;
+  w.seg.fallback[f] = (choice < 0 && !more) ? 1 : 0;
   w.seg.active[f] = choice < 0 ? 0 : 1;
   w.seg.active2[f] = 0;
 }
@@ -2757,30 +2797,36 @@ extern "C" hipError_t vhsd_launch_vhs_sync(const vhsd::vsync::SyncCfg* c, const 
   const int pmx = (int)std::floor(ll * 5.0) > 1 ? (int)std::floor(ll * 5.0) : 1;
   const double min_vsync = (c->vsync_us * c->fs_mhz) * 0.8;
   hipLaunchKernelGGL(seg_min_kernel, dim3(segs, max_fields), dim3(kSegT), 0, stream, *c, *w);
-  hipLaunchKernelGGL(search_thresholds_kernel, dim3(small), dim3(64), 0, stream, *c, *w, max_fields);
-  SegJob j1{b.thr, kSpecR, nullptr, pmn, pmx, min_vsync, nullptr, b.sum, segs, b.choice, b.seg_carry, b.seg_off,
+  hipLaunchKernelGGL(search_thresholds_kernel, dim3(max_fields), dim3(64), 0, stream, *c, *w, max_fields);
+  SegJob j1{b.thr, kSpecR, 0, nullptr, pmn, pmx, min_vsync, nullptr, b.sum, segs, b.choice, b.seg_carry, b.seg_off,
             b.count, w->pulse_start, w->pulse_len};
-  hipLaunchKernelGGL(seg_count_kernel, dim3(segs, kSpecR, max_fields), dim3(kSegT), 0, stream, *c, *w, j1);
-  hipLaunchKernelGGL(search_decide_kernel, dim3(small), dim3(64), 0, stream, *c, *w, j1, max_fields);
+  hipLaunchKernelGGL(seg_count_kernel, dim3(segs, kSpecHalf, max_fields), dim3(kSegT), 0, stream, *c, *w, j1);
+  hipLaunchKernelGGL(search_decide_kernel, dim3(max_fields), dim3(64 * kSpecR), 0, stream, *c, *w, j1, kSpecHalf, 0);
+  SegJob j1b = j1;
+  j1b.r0 = kSpecHalf;
+  j1b.active = b.pending;
+  hipLaunchKernelGGL(seg_count_kernel, dim3(segs, kSpecR - kSpecHalf, max_fields), dim3(kSegT), 0, stream, *c, *w,
+                     j1b);
+  hipLaunchKernelGGL(search_decide_kernel, dim3(max_fields), dim3(64 * kSpecR), 0, stream, *c, *w, j1, kSpecR, 1);
   hipLaunchKernelGGL(level_search_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
   j1.active = b.active;
-  hipLaunchKernelGGL(seg_plan_kernel, dim3(small), dim3(64), 0, stream, *c, *w, j1, max_fields);
+  hipLaunchKernelGGL(seg_plan_kernel, dim3(max_fields), dim3(64), 0, stream, *c, *w, j1, max_fields);
   hipLaunchKernelGGL(seg_write_kernel, dim3(segs, max_fields), dim3(kSegT), 0, stream, *c, *w, j1);
   hipLaunchKernelGGL(refine_first_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
-  SegJob j2{b.thr2, 1, nullptr, pmn, pmx, 1e300, b.active2, b.sum, segs, nullptr, b.seg_carry, b.seg_off, b.count2,
-            w->pulse_start, w->pulse_len};
+  SegJob j2{b.thr2, 1, 0, nullptr, pmn, pmx, 1e300, b.active2, b.sum, segs, nullptr, b.seg_carry, b.seg_off,
+            b.count2, w->pulse_start, w->pulse_len};
   hipLaunchKernelGGL(seg_count_kernel, dim3(segs, 1, max_fields), dim3(kSegT), 0, stream, *c, *w, j2);
-  hipLaunchKernelGGL(seg_plan_kernel, dim3(small), dim3(64), 0, stream, *c, *w, j2, max_fields);
+  hipLaunchKernelGGL(seg_plan_kernel, dim3(max_fields), dim3(64), 0, stream, *c, *w, j2, max_fields);
   hipLaunchKernelGGL(seg_write_kernel, dim3(segs, max_fields), dim3(kSegT), 0, stream, *c, *w, j2);
   hipLaunchKernelGGL(refine_second_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
   // S2: the level chain
   hipLaunchKernelGGL(level_chain_kernel, dim3(1), dim3(kChainT), 0, stream, *c, *w);
   // S3: the fields' pulses at their thresholds on the DC-shifted reference
   hipLaunchKernelGGL(field_thresholds_kernel, dim3(small), dim3(64), 0, stream, *c, *w, max_fields);
-  SegJob j3{b.thr3, 1, b.dc3, c->pulse_min, c->pulse_max, 1e300, nullptr, b.sum, segs, nullptr, b.seg_carry,
+  SegJob j3{b.thr3, 1, 0, b.dc3, c->pulse_min, c->pulse_max, 1e300, nullptr, b.sum, segs, nullptr, b.seg_carry,
             b.seg_off, b.count3, w->pulse_start, w->pulse_len};
   hipLaunchKernelGGL(seg_count_kernel, dim3(segs, 1, max_fields), dim3(kSegT), 0, stream, *c, *w, j3);
-  hipLaunchKernelGGL(seg_plan_kernel, dim3(small), dim3(64), 0, stream, *c, *w, j3, max_fields);
+  hipLaunchKernelGGL(seg_plan_kernel, dim3(max_fields), dim3(64), 0, stream, *c, *w, j3, max_fields);
   hipLaunchKernelGGL(seg_write_kernel, dim3(segs, max_fields), dim3(kSegT), 0, stream, *c, *w, j3);
   hipLaunchKernelGGL(field_raw_count_kernel, dim3(small), dim3(64), 0, stream, *c, *w, max_fields);
   hipLaunchKernelGGL(field_logic_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);

@@ -90,8 +90,9 @@ struct FieldLogic {
 
 typedef vhsdecode_hip_vhs_field_v1 FieldRecord;
 
-constexpr int kSegLen = 16384;  // samples per segment of the segmented pulse searches
-constexpr int kSpecR = 8;       // level-search thresholds evaluated at once
+constexpr int kSegLen = 4096;   // samples per segment of the segmented pulse searches
+constexpr int kSpecR = 8;       // level-search thresholds evaluated on segment workgroups
+constexpr int kSpecHalf = 4;    // of which the first pass counts these (most searches decide in them)
 
 // A segment's pulse-search summary (vhs_sync.hip, seg_scan).
 struct SegSummary {
@@ -111,6 +112,7 @@ struct SegBuffers {
   int* choice;          // its list threshold (index), -1: the whole search on one workgroup
   int* fallback;
   int* active;
+  int* pending;         // the search needs the second half of the threshold sequence
   int* count;           // pulses of the search's list
   double* thr2;         // the second refinement's threshold
   int* active2;

@@ -1745,6 +1745,7 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
       b.active2 = ints + 4 * F;
       b.count2 = ints + 5 * F;
       b.count3 = ints + 6 * F;
+      b.pending = ints + 7 * F;
       double* dbls = reinterpret_cast<double*>(m + o_sdbl);
       b.thr2 = dbls;
       b.thr3 = dbls + F;
