@@ -661,6 +661,12 @@ typedef struct vhsdecode_hip_run_config_v1 {
   vhsdecode_hip_read_callback read_callback;
   vhsdecode_hip_cancel_callback cancel_callback; /* may be NULL */
   void* user_data;
+  /* .tbc.json videoParameters (TBCWriter::write_json, tbc_writer.cpp:219-255):
+   * system: NULL writes "PAL" or "NTSC" by the field params, else this name ("PAL-M");
+   * colourBurstStart/End = (int)(us * 1e-6 * output rate + 0.5) (sysparams colorBurstUS). */
+  const char* system_name;
+  double colour_burst_start_us;
+  double colour_burst_end_us;
 } vhsdecode_hip_run_config_v1;
 
 typedef struct vhsdecode_hip_run_result_v1 {

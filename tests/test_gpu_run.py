@@ -106,6 +106,11 @@ def test_run_writes_the_whole_span_fields(tape, tmp_path):
     assert vp["numberOfSequentialFields"] == res.fields_written
     assert vp["fieldWidth"] * vp["fieldHeight"] == tape["per"]
     assert vp["system"] == tape["system"] and vp["isSourcePal"] == (tape["system"] == "PAL")
+    # the burst window at the output rate, rounded as TBCWriter::write_json rounds it
+    b0, b1 = tape["fmt"].sysparams["colorBurstUS"]
+    assert (vp["colourBurstStart"], vp["colourBurstEnd"]) == (int(b0 * 1e-6 * vp["sampleRate"] + 0.5),
+                                                              int(b1 * 1e-6 * vp["sampleRate"] + 0.5))
+    assert list(vp).index("colourBurstStart") == list(vp).index("fieldHeight") + 1
     recs = doc["fields"]
     assert [r["seqNo"] for r in recs] == list(range(1, res.fields_written + 1))
     assert [r["isFirstField"] for r in recs] == [bool(x) for x in tape["first"]]

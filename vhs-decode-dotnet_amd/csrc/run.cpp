@@ -97,9 +97,10 @@ void write_ints(std::FILE* f, const std::vector<int32_t>& v) {
 
 // The .tbc.json of the run.  Level and geometry keys follow the reference's writer:
 // black/white 16-bit IRE from the output scale with a 10 % margin, the output sample rate,
-// the field shape and the active-video span of the system (NTSC 134..894, PAL 185..1107).
+// the field shape, the colour-burst window at the output rate and the active-video span of
+// the system (NTSC 134..894, PAL 185..1107).
 bool write_json(const std::string& path, const vhsdecode_hip_field_params_v1& fp, double output_rate_hz,
-                const std::vector<FieldRecord>& fields) {
+                const vhsdecode_hip_run_config_v1& cfg, const std::vector<FieldRecord>& fields) {
   const std::string tmp = path + ".tmp";
   std::FILE* f = std::fopen(tmp.c_str(), "wb");
   if (!f) return false;
@@ -107,8 +108,10 @@ bool write_json(const std::string& path, const vhsdecode_hip_field_params_v1& fp
   const double margin = 0.1;
   const double black = ((0.0 - fp.vsync_ire) * fp.output_scale + fp.output_zero) * (1.0 - margin);
   const double white = ((100.0 - fp.vsync_ire) * fp.output_scale + fp.output_zero) * (1.0 + margin);
+  const int burst_start = (int)(cfg.colour_burst_start_us * 1e-6 * output_rate_hz + 0.5);
+  const int burst_end = (int)(cfg.colour_burst_end_us * 1e-6 * output_rate_hz + 0.5);
   std::fprintf(f, "{\n  \"videoParameters\": {\n");
-  std::fprintf(f, "    \"system\": \"%s\",\n", pal ? "PAL" : "NTSC");
+  std::fprintf(f, "    \"system\": \"%s\",\n", cfg.system_name ? cfg.system_name : (pal ? "PAL" : "NTSC"));
   std::fprintf(f, "    \"isSubcarrierLocked\": false,\n");
   std::fprintf(f, "    \"isSourcePal\": %s,\n", pal ? "true" : "false");
   std::fprintf(f, "    \"numberOfSequentialFields\": %zu,\n", fields.size());
@@ -117,6 +120,8 @@ bool write_json(const std::string& path, const vhsdecode_hip_field_params_v1& fp
   std::fprintf(f, "    \"sampleRate\": %.0f,\n", output_rate_hz);
   std::fprintf(f, "    \"fieldWidth\": %d,\n", fp.output_line_length);
   std::fprintf(f, "    \"fieldHeight\": %d,\n", fp.field_lines);
+  std::fprintf(f, "    \"colourBurstStart\": %d,\n", burst_start);
+  std::fprintf(f, "    \"colourBurstEnd\": %d,\n", burst_end);
   std::fprintf(f, "    \"activeVideoStart\": %d,\n", pal ? 185 : 134);
   std::fprintf(f, "    \"activeVideoEnd\": %d,\n", pal ? 1107 : 894);
   std::fprintf(f, "    \"tapeFormat\": \"VHS\",\n");
@@ -424,7 +429,7 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
   const auto t1 = std::chrono::steady_clock::now();
   if (std::fflush(tbc.f) != 0) return set_error(VHSDECODE_HIP_E_IO, "cannot write " + tbc_path);
   const double out_rate = sy.output_rate_mhz * 1e6;
-  if (!write_json(json_path, fp, out_rate, written)) return set_error(VHSDECODE_HIP_E_IO, "cannot write " + json_path);
+  if (!write_json(json_path, fp, out_rate, cfg, written)) return set_error(VHSDECODE_HIP_E_IO, "cannot write " + json_path);
   res.fields_written = (uint32_t)written.size();
   res.output_line_length = (uint32_t)fp.output_line_length;
   res.output_field_lines = (uint32_t)fp.field_lines;

@@ -212,7 +212,8 @@ class RunConfigV1(ctypes.Structure):
                 ("maximum_output_fields", ctypes.c_uint32), ("span_samples", ctypes.c_int64),
                 ("output_base_utf8", ctypes.c_char_p), ("overwrite", ctypes.c_int32),
                 ("read_callback", READ_CALLBACK), ("cancel_callback", CANCEL_CALLBACK),
-                ("user_data", ctypes.c_void_p)]
+                ("user_data", ctypes.c_void_p), ("system_name", ctypes.c_char_p),
+                ("colour_burst_start_us", ctypes.c_double), ("colour_burst_end_us", ctypes.c_double)]
 
 
 class RunResultV1(ctypes.Structure):

@@ -27,6 +27,17 @@ class RunResult:
     dropout_rows: int
 
 
+def _system_name(fmt: FormatParameters) -> bytes:
+    """videoParameters.system as TBCWriter::write_json names it (tbc_writer.cpp:219-225)."""
+    return b"PAL-M" if fmt.system in ("MPAL", "PAL-M") else (b"PAL" if fmt.parent_system == "PAL" else b"NTSC")
+
+
+def _burst_us(fmt: FormatParameters) -> tuple:
+    """The colour-burst window in microseconds (sysparams colorBurstUS)."""
+    burst = fmt.sysparams.get("colorBurstUS", (0.0, 0.0))
+    return float(burst[0]), float(burst[1])
+
+
 def decode_run(rf: np.ndarray, fmt: FormatParameters, filters: TapeFilterSet, output_base: str,
                sinc_table: Optional[np.ndarray] = None, dropouts: bool = True, dropout_capacity: int = 1024,
                span_samples: int = 0, maximum_output_fields: int = 0, overwrite: bool = True,
@@ -76,7 +87,7 @@ def decode_run(rf: np.ndarray, fmt: FormatParameters, filters: TapeFilterSet, ou
     cfg = _lib.RunConfigV1(ctypes.sizeof(_lib.RunConfigV1), device, ctypes.pointer(fm),
                            ctypes.pointer(dp) if dp is not None else None, src.size, fmt_code,
                            maximum_output_fields, span_samples, output_base.encode("utf-8"), 1 if overwrite else 0,
-                           read_cb, cancel_cb, None)
+                           read_cb, cancel_cb, None, _system_name(fmt), *_burst_us(fmt))
     res = _lib.RunResultV1()
     res.struct_size = ctypes.sizeof(_lib.RunResultV1)
     _lib.check(L.vhsdecode_hip_run(ctypes.byref(cfg), ctypes.byref(res)))
