@@ -592,9 +592,6 @@ struct SegScratch {
   int red[kSegT / 64 + 4];
   int wlast[kSegT / 64];
   int wfirst[kSegT / 64];
-  int offset;
-  int carry;
-  int first;
 };
 
 template <int NT>
@@ -614,115 +611,120 @@ __device__ int wgt_exclusive(int v, int* red, int* total) {
   return before + incl - v;
 }
 
-// Segment [s0, s1) of the window at threshold `high`: its summary (every thread), and
-// with `starts` the pulses paired from `carry_in` on, written from `out_base` (below cap).
-__device__ SegSummary seg_scan(const Plane& x, long long s0, long long s1, double high, int min_len, int max_len,
-                               double vs_min, int carry_in, int* starts, int* lens, int cap, int out_base,
-                               SegScratch* ps) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (tid == 0) {
-    ps->offset = 0;
-    ps->carry = carry_in;
-    ps->first = -1;
+// One thread's kRun samples of a segment and the last sample before them that is not
+// NaN (NaN when there is none): loaded once for every threshold the segment is searched at.
+struct SegRun {
+  float v[kRun];
+  float before;
+};
+
+__device__ __forceinline__ void seg_load(const Plane& x, long long i0, long long s1, SegRun* run) {
+  run->before = __builtin_nanf("");
+  if (i0 >= s1) return;
+#pragma unroll
+  for (int k = 0; k < kRun; k++) run->v[k] = i0 + k < s1 ? x.p[i0 + k] : __builtin_nanf("");
+  for (long long k = i0 - 1; k >= 0; k--) {
+    const float v = x.p[k];
+    if (v == v) {
+      run->before = v;
+      break;
+    }
   }
-  __syncthreads();
+}
+
+// Segment [s0, s1) (one tile: s1 - s0 <= kSegT * kRun) of the window at threshold
+// `high`: its summary (every thread), and with `starts` the pulses paired from
+// `carry_in` on, written from `out_base` (below cap).  The samples come from seg_load.
+__device__ SegSummary seg_scan(const Plane& x, const SegRun& run, long long s0, long long s1, double high,
+                               int min_len, int max_len, double vs_min, int carry_in, int* starts, int* lens, int cap,
+                               int out_base, SegScratch* ps) {
+#pragma clang fp contract(off)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int n_in = 0, n_vs = 0;
   int first_end = 0;
-  for (long long base = s0; base < s1; base += kSegT * kRun) {
-    const long long i0 = base + (long long)tid * kRun;
-    unsigned codes = 0;
-    if (i0 < s1) {
-      int st = i0 > 0 ? state_before(x, i0 - 1, high) : 0;
+  const long long i0 = s0 + (long long)tid * kRun;
+  unsigned codes = 0;
+  if (i0 < s1) {
+    const float bv = run.before;
+    int st = (i0 > 0 && bv == bv) ? ((((double)bv + x.bias) + x.dc) <= high ? 1 : 0) : 0;
 #pragma unroll
-      for (int k = 0; k < kRun; k++) {
-        const long long i = i0 + k;
-        if (i < s1) {
-          const float v = x.p[i];
-          if (v == v) {
-            const int cur = x.at(i) <= high ? 1 : 0;
-            if (i >= 1) codes |= ((!st && cur) ? 1u : ((st && !cur) ? 2u : 0u)) << (2 * k);
-            st = cur;
-          }
-        }
+    for (int k = 0; k < kRun; k++) {
+      const long long i = i0 + k;
+      const float v = run.v[k];
+      if (i < s1 && v == v) {
+        const int cur = (((double)v + x.bias) + x.dc) <= high ? 1 : 0;
+        if (i >= 1) codes |= ((!st && cur) ? 1u : ((st && !cur) ? 2u : 0u)) << (2 * k);
+        st = cur;
       }
     }
-    const int mine = codes ? (int)(i0 + ((31 - __clz(codes)) >> 1)) : -1;
-    const int myfirst = codes ? (int)(i0 + ((__ffs(codes) - 1) >> 1)) : INT_MAX;
-    // the transition before this run: the latest of the earlier threads', or the carry
-    int incl = mine;
+  }
+  const int mine = codes ? (int)(i0 + ((31 - __clz(codes)) >> 1)) : -1;
+  const int myfirst = codes ? (int)(i0 + ((__ffs(codes) - 1) >> 1)) : INT_MAX;
+  // the transition before this run: the latest of the earlier threads', or the carry
+  int incl = mine;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int u = __shfl_up(incl, o);
-      if (lane >= o) incl = incl > u ? incl : u;
-    }
-    int fmin = myfirst;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o);
+    if (lane >= o) incl = incl > u ? incl : u;
+  }
+  int fmin = myfirst;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const int u = __shfl_xor(fmin, o);
-      fmin = fmin < u ? fmin : u;
+  for (int o = 32; o > 0; o >>= 1) {
+    const int u = __shfl_xor(fmin, o);
+    fmin = fmin < u ? fmin : u;
+  }
+  if (lane == 63) ps->wlast[wave] = incl;
+  if (lane == 0) ps->wfirst[wave] = fmin;
+  __syncthreads();
+  int before = carry_in;
+  for (int w = 0; w < wave; w++) before = before > ps->wlast[w] ? before : ps->wlast[w];
+  const int up = __shfl_up(incl, 1);
+  const int prev = lane > 0 ? (before > up ? before : up) : before;
+  int tile_last = carry_in, tile_first = INT_MAX;
+  for (int w = 0; w < kSegT / 64; w++) {
+    tile_last = tile_last > ps->wlast[w] ? tile_last : ps->wlast[w];
+    tile_first = tile_first < ps->wfirst[w] ? tile_first : ps->wfirst[w];
+  }
+  int cnt = 0;
+  unsigned rest = codes;
+  int pv = prev;
+  while (rest) {
+    const int k = (__ffs(rest) - 1) >> 1;
+    const int pos = (int)(i0 + k);
+    const bool end = ((rest >> (2 * k)) & 3u) == 2u;
+    if (pos == tile_first) first_end = end ? 1 : 0;
+    if (end && pv > 0) {
+      const int len = pos - pv;
+      if (in_range((double)len, min_len, max_len)) {
+        cnt++;
+        if (!starts && (double)len > vs_min) n_vs++;
+      }
     }
-    if (lane == 63) ps->wlast[wave] = incl;
-    if (lane == 0) ps->wfirst[wave] = fmin;
-    __syncthreads();
-    int before = ps->carry;
-    for (int w = 0; w < wave; w++) before = before > ps->wlast[w] ? before : ps->wlast[w];
-    const int up = __shfl_up(incl, 1);
-    const int prev = lane > 0 ? (before > up ? before : up) : before;
-    int tile_last = ps->carry, tile_first = INT_MAX;
-    for (int w = 0; w < kSegT / 64; w++) {
-      tile_last = tile_last > ps->wlast[w] ? tile_last : ps->wlast[w];
-      tile_first = tile_first < ps->wfirst[w] ? tile_first : ps->wfirst[w];
-    }
-    int cnt = 0;
-    unsigned rest = codes;
-    int pv = prev;
-    while (rest) {
+    pv = pos;
+    rest &= ~(3u << (2 * k));
+  }
+  n_in += cnt;
+  if (starts) {
+    int tot = 0;
+    int idx = wgt_exclusive<kSegT>(cnt, ps->red, &tot) + out_base;
+    rest = codes;
+    pv = prev;
+    while (rest && cnt) {
       const int k = (__ffs(rest) - 1) >> 1;
       const int pos = (int)(i0 + k);
-      const bool end = ((rest >> (2 * k)) & 3u) == 2u;
-      if (pos == tile_first && ps->first < 0) first_end = end ? 1 : 0;
-      if (end && pv > 0) {
+      if (((rest >> (2 * k)) & 3u) == 2u && pv > 0) {
         const int len = pos - pv;
         if (in_range((double)len, min_len, max_len)) {
-          cnt++;
-          if (!starts && (double)len > vs_min) n_vs++;
+          if (idx < cap) {
+            starts[idx] = pv;
+            lens[idx] = len;
+          }
+          idx++;
         }
       }
       pv = pos;
       rest &= ~(3u << (2 * k));
     }
-    n_in += cnt;
-    int tot = 0;
-    if (starts) {
-      int idx = wgt_exclusive<kSegT>(cnt, ps->red, &tot) + ps->offset + out_base;
-      rest = codes;
-      pv = prev;
-      while (rest && cnt) {
-        const int k = (__ffs(rest) - 1) >> 1;
-        const int pos = (int)(i0 + k);
-        if (((rest >> (2 * k)) & 3u) == 2u && pv > 0) {
-          const int len = pos - pv;
-          if (in_range((double)len, min_len, max_len)) {
-            if (idx < cap) {
-              starts[idx] = pv;
-              lens[idx] = len;
-            }
-            idx++;
-          }
-        }
-        pv = pos;
-        rest &= ~(3u << (2 * k));
-      }
-    }
-    // the segment's first transition, the offset and the carry into the next tile
-    const bool set_first = tile_first != INT_MAX && ps->first < 0;
-    __syncthreads();
-    if (tid == 0) {
-      if (set_first) ps->first = tile_first;
-      ps->offset += tot;
-      ps->carry = tile_last;
-    }
-    __syncthreads();
   }
   // reduce the per-thread counts (count mode) and the first transition's kind
   int tot_in, tot_vs, tot_fe;
@@ -730,9 +732,9 @@ __device__ SegSummary seg_scan(const Plane& x, long long s0, long long s1, doubl
   (void)wgt_exclusive<kSegT>(n_vs, ps->red, &tot_vs);
   (void)wgt_exclusive<kSegT>(first_end, ps->red, &tot_fe);
   SegSummary sm;
-  sm.first_pos = ps->first;
+  sm.first_pos = tile_first == INT_MAX ? -1 : tile_first;
   sm.first_end = tot_fe > 0 ? 1 : 0;
-  sm.last_pos = ps->first < 0 ? -1 : ps->carry;
+  sm.last_pos = tile_first == INT_MAX ? -1 : tile_last;
   sm.n_in = tot_in;
   sm.n_vs = tot_vs;
   __syncthreads();
@@ -744,7 +746,7 @@ __device__ SegSummary seg_scan(const Plane& x, long long s0, long long s1, doubl
 struct SegJob {
   const double* thr;
   int R;
-  int r0;                  // first threshold of a count launch (grid y counts from it)
+  int r0, nr;              // thresholds [r0, r0 + nr) of a count launch
   const double* dc;        // per field, or null
   int min_len, max_len;
   double vs_min;
@@ -763,56 +765,71 @@ __device__ __forceinline__ Plane job_plane(const SyncCfg& c, const SyncWork& w, 
   return Plane{w.sync_ref + w.win_start[f], w.win_len[f], c.bias, j.dc ? j.dc[f] : 0.0};
 }
 
+// A segment's summaries at thresholds [j.r0, j.r0 + j.nr), its samples loaded once.
 __global__ void __launch_bounds__(kSegT) seg_count_kernel(SyncCfg c, SyncWork w, SegJob j) {
   __shared__ SegScratch ps;
-  const int sg = blockIdx.x, r = blockIdx.y + j.r0, f = blockIdx.z;
+  const int sg = blockIdx.x, f = blockIdx.z;
   if (f >= *w.spec_count || (j.active && !j.active[f])) return;
   const Plane x = job_plane(c, w, j, f);
   const long long s0 = (long long)sg * kSegLen;
   const long long s1 = s0 + kSegLen < x.n ? s0 + kSegLen : x.n;
-  SegSummary sm{-1, 0, -1, 0, 0};
-  if (s0 < x.n)
-    sm = seg_scan(x, s0, s1, j.thr[(long long)f * j.R + r], j.min_len, j.max_len, j.vs_min, -1, nullptr, nullptr,
-                  0, 0, &ps);
-  if (threadIdx.x == 0) j.sum[((long long)f * j.R + r) * j.segs + sg] = sm;
+  SegRun run;
+  if (s0 < x.n) seg_load(x, s0 + (long long)threadIdx.x * kRun, s1, &run);
+  for (int r = j.r0; r < j.r0 + j.nr; r++) {
+    SegSummary sm{-1, 0, -1, 0, 0};
+    if (s0 < x.n)
+      sm = seg_scan(x, run, s0, s1, j.thr[(long long)f * j.R + r], j.min_len, j.max_len, j.vs_min, -1, nullptr,
+                    nullptr, 0, 0, &ps);
+    if (threadIdx.x == 0) j.sum[((long long)f * j.R + r) * j.segs + sg] = sm;
+  }
 }
 
 // Segment summaries of one threshold, in order, on one wave: the pulse count, the
 // long-pulse count and (with seg_off) each segment's offset and the transition carried
-// into it.  The lanes hold 64 segments at a time; the running state steps through them by
-// lane index.
+// into it.  A segment's boundary pulse depends only on the last transition of the
+// nearest earlier segment that has one, so that transition comes from a max-scan over
+// (segment, last transition) and the counts from a prefix sum, 64 segments at a time.
 __device__ void wave_seg_combine(const SegJob& j, int f, int r, int* np, int* nvs, int* seg_carry, int* seg_off) {
   const int lane = threadIdx.x & 63;
-  int last = -1, count = 0, vs = 0;
+  long long carry_key = -1;  // (segment << 32) | last transition of the latest segment with one
+  int count = 0, vs = 0;
   const SegSummary* sm = j.sum + ((long long)f * j.R + r) * j.segs;
   for (int b0 = 0; b0 < j.segs; b0 += 64) {
     const int sg = b0 + lane;
     SegSummary a{-1, 0, -1, 0, 0};
     if (sg < j.segs) a = sm[sg];
-    const int n = j.segs - b0 < 64 ? j.segs - b0 : 64;
-    int my_carry = -1, my_off = 0;
-    for (int k = 0; k < n; k++) {
-      if (lane == k) {
-        my_carry = last;
-        my_off = count;
-      }
-      const int fp = __builtin_amdgcn_readlane(a.first_pos, k);
-      if (fp < 0) continue;
-      if (__builtin_amdgcn_readlane(a.first_end, k) && last > 0) {
-        const int len = fp - last;
+    const bool has = a.first_pos >= 0;
+    long long incl = has ? (((long long)sg << 32) | (unsigned)a.last_pos) : -1;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long u = __shfl_up(incl, o);
+      if (lane >= o) incl = incl > u ? incl : u;
+    }
+    long long excl = __shfl_up(incl, 1);
+    if (lane == 0) excl = -1;
+    excl = excl > carry_key ? excl : carry_key;
+    const int last = excl < 0 ? -1 : (int)(excl & 0xffffffffll);
+    int c = 0, v = 0;
+    if (has) {
+      if (a.first_end && last > 0) {
+        const int len = a.first_pos - last;
         if (in_range((double)len, j.min_len, j.max_len)) {
-          count++;
-          if ((double)len > j.vs_min) vs++;
+          c = 1;
+          if ((double)len > j.vs_min) v = 1;
         }
       }
-      count += __builtin_amdgcn_readlane(a.n_in, k);
-      vs += __builtin_amdgcn_readlane(a.n_vs, k);
-      last = __builtin_amdgcn_readlane(a.last_pos, k);
+      c += a.n_in;
+      v += a.n_vs;
     }
+    const int ci = wave_scan_incl(c, lane), vi = wave_scan_incl(v, lane);
     if (seg_off && sg < j.segs) {
-      seg_carry[sg] = my_carry;
-      seg_off[sg] = my_off;
+      seg_carry[sg] = last;
+      seg_off[sg] = count + ci - c;
     }
+    count += __shfl(ci, 63);
+    vs += __shfl(vi, 63);
+    const long long top = __shfl(incl, 63);
+    carry_key = top > carry_key ? top : carry_key;
   }
   *np = count;
   if (nvs) *nvs = vs;
@@ -841,7 +858,9 @@ __global__ void __launch_bounds__(kSegT) seg_write_kernel(SyncCfg c, SyncWork w,
   if (s0 >= x.n) return;
   const long long s1 = s0 + kSegLen < x.n ? s0 + kSegLen : x.n;
   const long long fo = (long long)f * j.segs + sg;
-  (void)seg_scan(x, s0, s1, j.thr[(long long)f * j.R + r], j.min_len, j.max_len, 1e300, j.seg_carry[fo],
+  SegRun run;
+  seg_load(x, s0 + (long long)threadIdx.x * kRun, s1, &run);
+  (void)seg_scan(x, run, s0, s1, j.thr[(long long)f * j.R + r], j.min_len, j.max_len, 1e300, j.seg_carry[fo],
                  j.starts + (long long)f * c.pulse_cap, j.lens + (long long)f * c.pulse_cap, c.pulse_cap,
                  j.seg_off[fo], &ps);
 }
@@ -2798,22 +2817,22 @@ extern "C" hipError_t vhsd_launch_vhs_sync(const vhsd::vsync::SyncCfg* c, const 
   const double min_vsync = (c->vsync_us * c->fs_mhz) * 0.8;
   hipLaunchKernelGGL(seg_min_kernel, dim3(segs, max_fields), dim3(kSegT), 0, stream, *c, *w);
   hipLaunchKernelGGL(search_thresholds_kernel, dim3(max_fields), dim3(64), 0, stream, *c, *w, max_fields);
-  SegJob j1{b.thr, kSpecR, 0, nullptr, pmn, pmx, min_vsync, nullptr, b.sum, segs, b.choice, b.seg_carry, b.seg_off,
+  SegJob j1{b.thr, kSpecR, 0, kSpecHalf, nullptr, pmn, pmx, min_vsync, nullptr, b.sum, segs, b.choice, b.seg_carry, b.seg_off,
             b.count, w->pulse_start, w->pulse_len};
-  hipLaunchKernelGGL(seg_count_kernel, dim3(segs, kSpecHalf, max_fields), dim3(kSegT), 0, stream, *c, *w, j1);
+  hipLaunchKernelGGL(seg_count_kernel, dim3(segs, 1, max_fields), dim3(kSegT), 0, stream, *c, *w, j1);
   hipLaunchKernelGGL(search_decide_kernel, dim3(max_fields), dim3(64 * kSpecR), 0, stream, *c, *w, j1, kSpecHalf, 0);
   SegJob j1b = j1;
   j1b.r0 = kSpecHalf;
+  j1b.nr = kSpecR - kSpecHalf;
   j1b.active = b.pending;
-  hipLaunchKernelGGL(seg_count_kernel, dim3(segs, kSpecR - kSpecHalf, max_fields), dim3(kSegT), 0, stream, *c, *w,
-                     j1b);
+  hipLaunchKernelGGL(seg_count_kernel, dim3(segs, 1, max_fields), dim3(kSegT), 0, stream, *c, *w, j1b);
   hipLaunchKernelGGL(search_decide_kernel, dim3(max_fields), dim3(64 * kSpecR), 0, stream, *c, *w, j1, kSpecR, 1);
   hipLaunchKernelGGL(level_search_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
   j1.active = b.active;
   hipLaunchKernelGGL(seg_plan_kernel, dim3(max_fields), dim3(64), 0, stream, *c, *w, j1, max_fields);
   hipLaunchKernelGGL(seg_write_kernel, dim3(segs, max_fields), dim3(kSegT), 0, stream, *c, *w, j1);
   hipLaunchKernelGGL(refine_first_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
-  SegJob j2{b.thr2, 1, 0, nullptr, pmn, pmx, 1e300, b.active2, b.sum, segs, nullptr, b.seg_carry, b.seg_off,
+  SegJob j2{b.thr2, 1, 0, 1, nullptr, pmn, pmx, 1e300, b.active2, b.sum, segs, nullptr, b.seg_carry, b.seg_off,
             b.count2, w->pulse_start, w->pulse_len};
   hipLaunchKernelGGL(seg_count_kernel, dim3(segs, 1, max_fields), dim3(kSegT), 0, stream, *c, *w, j2);
   hipLaunchKernelGGL(seg_plan_kernel, dim3(max_fields), dim3(64), 0, stream, *c, *w, j2, max_fields);
@@ -2823,7 +2842,7 @@ extern "C" hipError_t vhsd_launch_vhs_sync(const vhsd::vsync::SyncCfg* c, const 
   hipLaunchKernelGGL(level_chain_kernel, dim3(1), dim3(kChainT), 0, stream, *c, *w);
   // S3: the fields' pulses at their thresholds on the DC-shifted reference
   hipLaunchKernelGGL(field_thresholds_kernel, dim3(small), dim3(64), 0, stream, *c, *w, max_fields);
-  SegJob j3{b.thr3, 1, 0, b.dc3, c->pulse_min, c->pulse_max, 1e300, nullptr, b.sum, segs, nullptr, b.seg_carry,
+  SegJob j3{b.thr3, 1, 0, 1, b.dc3, c->pulse_min, c->pulse_max, 1e300, nullptr, b.sum, segs, nullptr, b.seg_carry,
             b.seg_off, b.count3, w->pulse_start, w->pulse_len};
   hipLaunchKernelGGL(seg_count_kernel, dim3(segs, 1, max_fields), dim3(kSegT), 0, stream, *c, *w, j3);
   hipLaunchKernelGGL(seg_plan_kernel, dim3(max_fields), dim3(64), 0, stream, *c, *w, j3, max_fields);
