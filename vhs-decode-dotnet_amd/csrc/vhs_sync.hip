@@ -2259,18 +2259,42 @@ __device__ void resolve_window(const SyncCfg& c, long long begin, long long* ws,
   *wl = (lb - fb + 1) * c.stride;
 }
 
-__global__ void field_chain_kernel(SyncCfg c, SyncWork w) {
+// The sequence walk over the round's fields on one thread, their inputs first staged in
+// LDS by the wave (so the walk waits on no global load) and the records, the sync
+// history and the level state written as it goes (stores only; the level state and the
+// history once, as the last field processed leaves them).
+constexpr int kChainLds = 64;
+
+__global__ void __launch_bounds__(64) field_chain_kernel(SyncCfg c, SyncWork w) {
 #pragma clang fp contract(off)
-  if (threadIdx.x != 0) return;
+  __shared__ FieldLogic s_logic[kChainLds];
+  __shared__ FieldLevels s_levels[kChainLds];
+  __shared__ long long s_ws[kChainLds], s_wl[kChainLds];
+  __shared__ int s_ovf[kChainLds];
   SeqCtl& ctl = *w.ctl;
   if (ctl.halted) return;
+  const int nf = *w.spec_count;
+  const bool staged = nf <= kChainLds;
+  if (staged)
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+      s_logic[i] = w.logic[i];
+      s_levels[i] = w.levels[i];
+      s_ws[i] = w.win_start[i];
+      s_wl[i] = w.win_len[i];
+      s_ovf[i] = w.overflow[i];
+    }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const FieldLogic* in_logic = staged ? s_logic : w.logic;
+  const FieldLevels* in_levels = staged ? s_levels : w.levels;
+  const long long* in_ws = staged ? s_ws : w.win_start;
+  const long long* in_wl = staged ? s_wl : w.win_len;
+  const int* in_ovf = staged ? s_ovf : w.overflow;
   const int base = ctl.base;
   SyncHistoryDev h = *w.history_in;
   h.prev_first_hsync_read -= w.state_shift;
   // the state the next call continues from: as the last field (decoded or failed) left it
-  *w.history_out = h;
-  w.level_state_out[0] = *w.level_state_in;
-  const int nf = *w.spec_count;
+  int last_done = -1;
   long long begin = ctl.begin;
   int out_count = 0;
   int stop_status = kStatusNotRead;
@@ -2286,152 +2310,161 @@ __global__ void field_chain_kernel(SyncCfg c, SyncWork w) {
     rec.next_begin = nb > 0 ? nb : 0;
     begin = rec.next_begin;
     stop_status = status;
-    *w.history_out = h;
-    w.level_state_out[0] = f + 1 < nf ? w.level_before[f + 1] : *w.level_state_after_all;
+    last_done = f;
   };
   for (; f < nf; f++) {
-    FieldRecord& rec = w.records[base + f];
-    rec = FieldRecord{};
+    FieldRecord rec{};
     rec.struct_size = sizeof(FieldRecord);
-    long long ws, wl;
-    resolve_window(c, begin, &ws, &wl);
-    rec.begin = begin;
-    rec.window_start = ws;
-    rec.window_length = wl;
-    rec.next_begin = begin;
-    if (ws != w.win_start[f] || wl != w.win_len[f]) {
-      rec.status = kStatusMispredicted;
-      stop_status = rec.status;
-      break;
-    }
-    const FieldLevels lv = w.levels[f];
-    const FieldLogic lg = w.logic[f];
-    rec.level_source = lv.source;
-    rec.sync_level = lv.sync;
-    rec.blank_level = lv.blank;
-    rec.dc_offset = lv.dc;
-    rec.threshold = lv.threshold;
-    rec.raw_pulses = lg.raw;
-    rec.refined_pulses = lg.refined;
-    rec.hsync_median = lg.hsync_median;
-    rec.mean_line_length = lg.mll;
-    if (w.overflow[f]) {
-      rec.status = kStatusCapacity;
-      stop_status = rec.status;
-      break;
-    }
-    if (lg.raw == 0) {
-      const bool missing_start = h.pulse_search_initialized != 0;
-      recover(rec, missing_start ? 2 : 1, missing_start ? c.nominal * 100.0 : (c.fs_mhz * 1000000.0) / 10.0);
-      break;
-    }
-    h.pulse_search_initialized = 1;
-    const double mll = lg.mll;
-    // FieldParityDetector.ResolveCadence, no fallback line
-    bool first;
-    int conf = 0;
-    if (h.prev_first >= 0) first = !h.prev_first;
-    else first = lg.detected == 0 || round_away(lg.consensus / (double)lg.detected) == 1.0;
-    int eff = 100;
-    if (lg.detected > 0 && !h.has_prev_first_hsync && eff > 50) eff = 50;
-    if (lg.first_conf >= eff && lg.first_conf > lg.second_conf) {
-      first = true;
-      conf = lg.first_conf;
-    } else if (lg.second_conf >= eff && lg.first_conf < lg.second_conf) {
-      first = false;
-      conf = lg.second_conf;
-    }
-    rec.is_first_field = first ? 1 : 0;
-    rec.parity_confidence = conf;
-    h.prev_first = first ? 1 : 0;
-    const int cur_lines = first ? c.field_lines0 : c.field_lines1;
-    const double fhl = first_hsync_line(c.is_pal, c.num_pulses, first);
-    // EstimateFirstHSyncFromPrevious (:6146-6173)
-    bool has_est = false;
-    double est = 0.0;
-    if (h.has_prev_first_hsync && h.prev_first_hsync > 0.0 && mll > 0.0) {
-      const double cur = first ? c.field_lines0 : c.field_lines1;
-      const double prev = first ? c.field_lines1 : c.field_lines0;
-      const double est_lines = c.is_pal ? cur : prev;
-      const double last_off = (h.prev_first_hsync_read - ws) / mll;
-      est = round_away((last_off + est_lines + (h.prev_first_hsync / mll)) * mll);
-      has_est = true;
-    }
-    // ResolveLine0Location, vhs (:5452-5651)
-    const Transitions& tr = lg.tr[first ? 0 : 1];
-    bool ok = false, used_prev = false;
-    double line0 = 0.0, fh = 0.0, unaligned = 0.0;
-    if (tr.combined.ok) {
-      ok = true;
-      line0 = tr.combined.line0;
-      fh = tr.combined.first_hsync;
-      unaligned = tr.combined.unaligned;
-    } else {
-      const int fc = tr.first.ok ? tr.first.valid_count : 0;
-      const int lc = tr.last.ok ? tr.last.valid_count : 0;
-      const bool has_prev = h.has_prev_first_hsync && h.prev_first_hsync > 0.0;
-      const Estimate* single = nullptr;
-      if (fc == 6 || (!has_prev && fc > lc)) single = tr.first.ok ? &tr.first : nullptr;
-      else if (lc == 6 || (!has_prev && lc > fc)) single = tr.last.ok ? &tr.last : nullptr;
-      if (single) {
-        ok = true;
-        line0 = single->line0;
-        fh = single->first_hsync;
-        unaligned = single->unaligned;
-      } else if (has_est) {
-        double e = est;
-        if (h.prev_hsync_difference >= -0.5 && h.prev_hsync_difference <= 0.5) e += mll * h.prev_hsync_difference;
-        const int* ks = w.ref_kind + (long long)f * c.pulse_cap;
-        const int* ss = w.ref_start + (long long)f * c.pulse_cap;
-        if (e <= 0.0) e = lg.refined > 0 ? ss[0] : 0.0;
-        unaligned = round_away(e);
-        fh = align_first_hsync(ks, ss, lg.refined, unaligned, mll, fhl, cur_lines);
-        line0 = fh - (fhl * mll);
-        ok = true;
-        used_prev = true;
+    bool stop = false;
+    do {
+      long long ws, wl;
+      resolve_window(c, begin, &ws, &wl);
+      rec.begin = begin;
+      rec.window_start = ws;
+      rec.window_length = wl;
+      rec.next_begin = begin;
+      if (ws != in_ws[f] || wl != in_wl[f]) {
+        rec.status = kStatusMispredicted;
+        stop_status = rec.status;
+        stop = true;
+        break;
       }
-    }
-    if (!ok) {
-      recover(rec, 2, c.nominal * 100.0);
-      break;
-    }
-    // UpdateSyncHistory (:6119-6144)
-    if (!used_prev && has_est && isfinite(unaligned)) h.prev_hsync_difference = (unaligned - est) / mll;
-    if (isfinite(fh)) {
-      h.prev_first_hsync_read = ws;
-      h.prev_first_hsync = fh;
-      h.has_prev_first_hsync = 1;
-    }
-    rec.line0 = line0;
-    rec.first_hsync = fh;
-    rec.first_hsync_line = fhl;
-    const int processed = c.output_lines + 10;
-    const double last_line = (((double)wl - line0) / mll) - 1.0;
-    if (last_line < processed) {
-      // ThrowIfInsufficientFieldData (:5961-6008), tape
-      const double suggested = line0 - (mll * 20.0);
-      recover(rec, 3, suggested > c.nominal ? suggested : c.nominal);
-      break;
-    }
-    // ComputeVhsNextFieldOffsetSamples (:2299-2325)
-    const Bounds b = field_bounds(c.is_pal, first);
-    const double next_loc = fh + (mll * ((cur_lines + b.last_hsync) - fhl));
-    double next_off = next_loc - (8.0 * c.nominal) + (double)c.block_cut;
-    if (!(isfinite(next_off) && next_off > 0.0)) {
-      const int fbl = c.output_lines - 7 > 0 ? c.output_lines - 7 : 0;
-      next_off = line0 + (fbl * mll) + (double)c.block_cut;  // line-location fallback, projected
-      next_off = next_off > 1.0 ? next_off : 1.0;
-    }
-    rec.next_offset = next_off;
-    const long long adv = (long long)round_away(next_off);
-    rec.next_begin = ws + (adv > 1 ? adv : 1);
-    rec.status = 0;
-    begin = rec.next_begin;
-    out_count++;
-    // the level state after this field
-    w.level_state_out[0] = f + 1 < nf ? w.level_before[f + 1] : *w.level_state_after_all;
-    *w.history_out = h;
+      const FieldLevels lv = in_levels[f];
+      const FieldLogic lg = in_logic[f];
+      rec.level_source = lv.source;
+      rec.sync_level = lv.sync;
+      rec.blank_level = lv.blank;
+      rec.dc_offset = lv.dc;
+      rec.threshold = lv.threshold;
+      rec.raw_pulses = lg.raw;
+      rec.refined_pulses = lg.refined;
+      rec.hsync_median = lg.hsync_median;
+      rec.mean_line_length = lg.mll;
+      if (in_ovf[f]) {
+        rec.status = kStatusCapacity;
+        stop_status = rec.status;
+        stop = true;
+        break;
+      }
+      if (lg.raw == 0) {
+        const bool missing_start = h.pulse_search_initialized != 0;
+        recover(rec, missing_start ? 2 : 1, missing_start ? c.nominal * 100.0 : (c.fs_mhz * 1000000.0) / 10.0);
+        stop = true;
+        break;
+      }
+      h.pulse_search_initialized = 1;
+      const double mll = lg.mll;
+      // FieldParityDetector.ResolveCadence, no fallback line
+      bool first;
+      int conf = 0;
+      if (h.prev_first >= 0) first = !h.prev_first;
+      else first = lg.detected == 0 || round_away(lg.consensus / (double)lg.detected) == 1.0;
+      int eff = 100;
+      if (lg.detected > 0 && !h.has_prev_first_hsync && eff > 50) eff = 50;
+      if (lg.first_conf >= eff && lg.first_conf > lg.second_conf) {
+        first = true;
+        conf = lg.first_conf;
+      } else if (lg.second_conf >= eff && lg.first_conf < lg.second_conf) {
+        first = false;
+        conf = lg.second_conf;
+      }
+      rec.is_first_field = first ? 1 : 0;
+      rec.parity_confidence = conf;
+      h.prev_first = first ? 1 : 0;
+      const int cur_lines = first ? c.field_lines0 : c.field_lines1;
+      const double fhl = first_hsync_line(c.is_pal, c.num_pulses, first);
+      // EstimateFirstHSyncFromPrevious (:6146-6173)
+      bool has_est = false;
+      double est = 0.0;
+      if (h.has_prev_first_hsync && h.prev_first_hsync > 0.0 && mll > 0.0) {
+        const double cur = first ? c.field_lines0 : c.field_lines1;
+        const double prev = first ? c.field_lines1 : c.field_lines0;
+        const double est_lines = c.is_pal ? cur : prev;
+        const double last_off = (h.prev_first_hsync_read - ws) / mll;
+        est = round_away((last_off + est_lines + (h.prev_first_hsync / mll)) * mll);
+        has_est = true;
+      }
+      // ResolveLine0Location, vhs (:5452-5651)
+      const Transitions& tr = lg.tr[first ? 0 : 1];
+      bool ok = false, used_prev = false;
+      double line0 = 0.0, fh = 0.0, unaligned = 0.0;
+      if (tr.combined.ok) {
+        ok = true;
+        line0 = tr.combined.line0;
+        fh = tr.combined.first_hsync;
+        unaligned = tr.combined.unaligned;
+      } else {
+        const int fc = tr.first.ok ? tr.first.valid_count : 0;
+        const int lc = tr.last.ok ? tr.last.valid_count : 0;
+        const bool has_prev = h.has_prev_first_hsync && h.prev_first_hsync > 0.0;
+        const Estimate* single = nullptr;
+        if (fc == 6 || (!has_prev && fc > lc)) single = tr.first.ok ? &tr.first : nullptr;
+        else if (lc == 6 || (!has_prev && lc > fc)) single = tr.last.ok ? &tr.last : nullptr;
+        if (single) {
+          ok = true;
+          line0 = single->line0;
+          fh = single->first_hsync;
+          unaligned = single->unaligned;
+        } else if (has_est) {
+          double e = est;
+          if (h.prev_hsync_difference >= -0.5 && h.prev_hsync_difference <= 0.5) e += mll * h.prev_hsync_difference;
+          const int* ks = w.ref_kind + (long long)f * c.pulse_cap;
+          const int* ss = w.ref_start + (long long)f * c.pulse_cap;
+          if (e <= 0.0) e = lg.refined > 0 ? ss[0] : 0.0;
+          unaligned = round_away(e);
+          fh = align_first_hsync(ks, ss, lg.refined, unaligned, mll, fhl, cur_lines);
+          line0 = fh - (fhl * mll);
+          ok = true;
+          used_prev = true;
+        }
+      }
+      if (!ok) {
+        recover(rec, 2, c.nominal * 100.0);
+        stop = true;
+        break;
+      }
+      // UpdateSyncHistory (:6119-6144)
+      if (!used_prev && has_est && isfinite(unaligned)) h.prev_hsync_difference = (unaligned - est) / mll;
+      if (isfinite(fh)) {
+        h.prev_first_hsync_read = ws;
+        h.prev_first_hsync = fh;
+        h.has_prev_first_hsync = 1;
+      }
+      rec.line0 = line0;
+      rec.first_hsync = fh;
+      rec.first_hsync_line = fhl;
+      const int processed = c.output_lines + 10;
+      const double last_line = (((double)wl - line0) / mll) - 1.0;
+      if (last_line < processed) {
+        // ThrowIfInsufficientFieldData (:5961-6008), tape
+        const double suggested = line0 - (mll * 20.0);
+        recover(rec, 3, suggested > c.nominal ? suggested : c.nominal);
+        stop = true;
+        break;
+      }
+      // ComputeVhsNextFieldOffsetSamples (:2299-2325)
+      const Bounds b = field_bounds(c.is_pal, first);
+      const double next_loc = fh + (mll * ((cur_lines + b.last_hsync) - fhl));
+      double next_off = next_loc - (8.0 * c.nominal) + (double)c.block_cut;
+      if (!(isfinite(next_off) && next_off > 0.0)) {
+        const int fbl = c.output_lines - 7 > 0 ? c.output_lines - 7 : 0;
+        next_off = line0 + (fbl * mll) + (double)c.block_cut;  // line-location fallback, projected
+        next_off = next_off > 1.0 ? next_off : 1.0;
+      }
+      rec.next_offset = next_off;
+      const long long adv = (long long)round_away(next_off);
+      rec.next_begin = ws + (adv > 1 ? adv : 1);
+      rec.status = 0;
+      begin = rec.next_begin;
+      out_count++;
+      last_done = f;
+    } while (false);
+    w.records[base + f] = rec;
+    if (stop) break;
   }
+  *w.history_out = h;
+  w.level_state_out[0] = last_done < 0 ? *w.level_state_in
+                                        : (last_done + 1 < nf ? w.level_before[last_done + 1] : *w.level_state_after_all);
   // a failed field takes a record; the stop record after the decoded fields does not
   const bool failed = stop_status >= 1 && stop_status <= 3;
   const int used = base + out_count + (failed ? 1 : 0);
