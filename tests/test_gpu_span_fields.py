@@ -104,6 +104,48 @@ def _one_call(dec, video, vlp, n, max_fields, first_begin=0, reset=True, shift=0
     return field.SpanDecode(records, outputs, nxt, None), nxt
 
 
+def test_replayed_launch_graph_matches_launches(span):
+    """A call that repeats the previous call's launch arguments replays them as a captured
+    HIP graph.  The launches made one by one (profiling on) give the reference; then the
+    same call several times on fixed buffers (captured, then replayed), each followed by
+    eager work on the same context and a synchronisation, gives the same records and
+    fields every time."""
+    dem, dec = span["dem"], span["dec"]
+    per = dec.geom.field_lines * dec.geom.out_len
+    mf = 32
+    out = hipmem.DeviceBuffer(mf * per * 2)
+    recs = hipmem.DeviceBuffer(mf * SIZE)
+    meta = hipmem.DeviceBuffer(16)
+
+    def call():
+        out.zero()
+        recs.zero()
+        meta.zero()
+        hipmem.synchronize()
+        dec.decode_async(span["video"], span["vlp"], span["n"], out.ptr, mf, meta.ptr, recs.ptr, meta.ptr + 8)
+        hipmem.synchronize()
+        count = int(meta.download(np.int32, 1)[0])
+        return (count, int(meta.download(np.int64, 1, offset=8)[0]), recs.download(np.uint8, (count + 1) * SIZE),
+                out.download(np.uint16, max(count, 1) * per))
+
+    dem.set_profiling(True)
+    want = call()
+    dem.take_profile()
+    dem.set_profiling(False)
+    assert want[0] >= 4
+    for rep in range(4):
+        got = call()
+        assert got[:2] == want[:2], rep
+        np.testing.assert_array_equal(got[2], want[2])
+        np.testing.assert_array_equal(got[3], want[3])
+        # eager work on the context between replays: a profiled call with other buffers
+        dem.set_profiling(True)
+        _one_call(dec, span["video"], span["vlp"], span["n"], 2)
+        dem.take_profile()
+        dem.set_profiling(False)
+    print(f"[{span['system']}] graph replays equal the launches: {want[0]} records", flush=True)
+
+
 def test_sequence_carried_into_a_later_span(span):
     """Two records decoded on the span, then the rest on the same planes entered at a
     block-grid multiple before the next read window (a later span): the level, parity and

@@ -39,7 +39,7 @@ namespace {
 
 using vhsd_host::set_error;
 constexpr int kBlock = 32768;
-constexpr int kRounds = 4;  // field sequence rounds per call
+constexpr int kRounds = 2;  // field sequence rounds per call (the span loop calls again after a mispredicted window)
 
 #define RUN_HIP(expr)                                                                         \
   do {                                                                                        \

@@ -757,6 +757,7 @@ struct SegJob {
   int* seg_carry;          // [field][segments]
   int* seg_off;
   int* count;              // per field: pulses of the list
+  int* raw_count;          // per field, or null: the count capped at the list capacity (overflow flagged)
   int* starts;             // [field][pulse_cap]
   int* lens;
 };
@@ -844,7 +845,13 @@ __global__ void __launch_bounds__(64) seg_plan_kernel(SyncCfg c, SyncWork w, Seg
   if (r < 0) return;
   int np;
   wave_seg_combine(j, f, r, &np, nullptr, j.seg_carry + (long long)f * j.segs, j.seg_off + (long long)f * j.segs);
-  if (threadIdx.x == 0) j.count[f] = np;
+  if (threadIdx.x == 0) {
+    j.count[f] = np;
+    if (j.raw_count) {
+      if (np > c.pulse_cap) w.overflow[f] = 1;
+      j.raw_count[f] = np < c.pulse_cap ? np : c.pulse_cap;
+    }
+  }
 }
 
 __global__ void __launch_bounds__(kSegT) seg_write_kernel(SyncCfg c, SyncWork w, SegJob j) {
@@ -1122,22 +1129,6 @@ __global__ void __launch_bounds__(kT) refine_second_kernel(SyncCfg c, SyncWork w
   }
 }
 
-// The field thresholds and DC offsets the pulse search of S3 runs at.
-__global__ void field_thresholds_kernel(SyncCfg c, SyncWork w, int max_fields) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= max_fields || f >= *w.spec_count) return;
-  const FieldLevels lv = w.levels[f];
-  w.seg.thr3[f] = lv.threshold;
-  w.seg.dc3[f] = lv.dc;
-}
-
-__global__ void field_raw_count_kernel(SyncCfg c, SyncWork w, int max_fields) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= max_fields || f >= *w.spec_count) return;
-  const int np = w.seg.count3[f];
-  if (np > c.pulse_cap) w.overflow[f] = 1;
-  w.raw_count[f] = np < c.pulse_cap ? np : c.pulse_cap;
-}
 
 // ------------------------------------------------------------------ S2: level chain
 // The moving averages of VsyncSerrationDetector (window 2, watermark 1, unbounded pushes
@@ -1396,7 +1387,12 @@ __global__ void __launch_bounds__(kChainT) level_chain_kernel(SyncCfg c, SyncWor
       lv.dc = 0.0;
       lv.threshold = c.ire0 + (c.hz_ire * (c.vsync_ire / 2.0));
     }
-    if (threadIdx.x == 0) w.levels[f] = lv;
+    if (threadIdx.x == 0) {
+      w.levels[f] = lv;
+      // the field's pulse search runs at this threshold on the reference shifted by dc
+      w.seg.thr3[f] = lv.threshold;
+      w.seg.dc3[f] = lv.dc;
+    }
   }
   if (threadIdx.x == 0) ch.store(w.level_state_after_all);
 }
@@ -2842,7 +2838,6 @@ extern "C" hipError_t vhsd_launch_vhs_sync(const vhsd::vsync::SyncCfg* c, const 
                      span_len, max_fields);
   const SegBuffers& b = w->seg;
   const int segs = b.segs;
-  const int small = (max_fields + 63) / 64;
   // S1: level search
   const double eql = (double)c->ser_eq_len, ll = (double)c->ser_line_len;
   const int pmn = (int)std::ceil(eql / 8.0) > 0 ? (int)std::ceil(eql / 8.0) : 0;
@@ -2850,8 +2845,8 @@ extern "C" hipError_t vhsd_launch_vhs_sync(const vhsd::vsync::SyncCfg* c, const 
   const double min_vsync = (c->vsync_us * c->fs_mhz) * 0.8;
   hipLaunchKernelGGL(seg_min_kernel, dim3(segs, max_fields), dim3(kSegT), 0, stream, *c, *w);
   hipLaunchKernelGGL(search_thresholds_kernel, dim3(max_fields), dim3(64), 0, stream, *c, *w, max_fields);
-  SegJob j1{b.thr, kSpecR, 0, kSpecHalf, nullptr, pmn, pmx, min_vsync, nullptr, b.sum, segs, b.choice, b.seg_carry, b.seg_off,
-            b.count, w->pulse_start, w->pulse_len};
+  SegJob j1{b.thr, kSpecR, 0, kSpecHalf, nullptr, pmn, pmx, min_vsync, nullptr, b.sum, segs, b.choice, b.seg_carry,
+            b.seg_off, b.count, nullptr, w->pulse_start, w->pulse_len};
   hipLaunchKernelGGL(seg_count_kernel, dim3(segs, 1, max_fields), dim3(kSegT), 0, stream, *c, *w, j1);
   hipLaunchKernelGGL(search_decide_kernel, dim3(max_fields), dim3(64 * kSpecR), 0, stream, *c, *w, j1, kSpecHalf, 0);
   SegJob j1b = j1;
@@ -2866,7 +2861,7 @@ extern "C" hipError_t vhsd_launch_vhs_sync(const vhsd::vsync::SyncCfg* c, const 
   hipLaunchKernelGGL(seg_write_kernel, dim3(segs, max_fields), dim3(kSegT), 0, stream, *c, *w, j1);
   hipLaunchKernelGGL(refine_first_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
   SegJob j2{b.thr2, 1, 0, 1, nullptr, pmn, pmx, 1e300, b.active2, b.sum, segs, nullptr, b.seg_carry, b.seg_off,
-            b.count2, w->pulse_start, w->pulse_len};
+            b.count2, nullptr, w->pulse_start, w->pulse_len};
   hipLaunchKernelGGL(seg_count_kernel, dim3(segs, 1, max_fields), dim3(kSegT), 0, stream, *c, *w, j2);
   hipLaunchKernelGGL(seg_plan_kernel, dim3(max_fields), dim3(64), 0, stream, *c, *w, j2, max_fields);
   hipLaunchKernelGGL(seg_write_kernel, dim3(segs, max_fields), dim3(kSegT), 0, stream, *c, *w, j2);
@@ -2874,13 +2869,11 @@ extern "C" hipError_t vhsd_launch_vhs_sync(const vhsd::vsync::SyncCfg* c, const 
   // S2: the level chain
   hipLaunchKernelGGL(level_chain_kernel, dim3(1), dim3(kChainT), 0, stream, *c, *w);
   // S3: the fields' pulses at their thresholds on the DC-shifted reference
-  hipLaunchKernelGGL(field_thresholds_kernel, dim3(small), dim3(64), 0, stream, *c, *w, max_fields);
   SegJob j3{b.thr3, 1, 0, 1, b.dc3, c->pulse_min, c->pulse_max, 1e300, nullptr, b.sum, segs, nullptr, b.seg_carry,
-            b.seg_off, b.count3, w->pulse_start, w->pulse_len};
+            b.seg_off, b.count3, w->raw_count, w->pulse_start, w->pulse_len};
   hipLaunchKernelGGL(seg_count_kernel, dim3(segs, 1, max_fields), dim3(kSegT), 0, stream, *c, *w, j3);
   hipLaunchKernelGGL(seg_plan_kernel, dim3(max_fields), dim3(64), 0, stream, *c, *w, j3, max_fields);
   hipLaunchKernelGGL(seg_write_kernel, dim3(segs, max_fields), dim3(kSegT), 0, stream, *c, *w, j3);
-  hipLaunchKernelGGL(field_raw_count_kernel, dim3(small), dim3(64), 0, stream, *c, *w, max_fields);
   hipLaunchKernelGGL(field_logic_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);
   hipLaunchKernelGGL(field_chain_kernel, dim3(1), dim3(64), 0, stream, *c, *w);
   hipLaunchKernelGGL(field_lines_kernel, dim3(max_fields), dim3(kT), 0, stream, *c, *w);

@@ -288,6 +288,11 @@ struct vhsdecode_hip_demod_context {
   const long long* vs_origins = nullptr;
   const int* vs_first_lines = nullptr;
   int vs_line_stride = 0;
+  // the field sequence's launches captured as a HIP graph (replayed while every argument
+  // of every launch is the same), and those arguments
+  hipGraph_t vs_graph = nullptr;
+  hipGraphExec_t vs_exec = nullptr;
+  std::vector<unsigned char> vs_key;
   double f_pow_alpha = -1.0;
   int f_pow_len = 0;
   void* f_stage = nullptr;   // pinned host staging for the location upload
@@ -325,6 +330,8 @@ struct vhsdecode_hip_demod_context {
     vs_mem.release(); vs_state.release();
     f_loc.release(); f_ref.release(); f_x.release(); f_y0.release(); f_pow.release(); f_fil.release();
     f_fo.release(); f_tab.release();
+    if (vs_exec) (void)hipGraphExecDestroy(vs_exec);
+    if (vs_graph) (void)hipGraphDestroy(vs_graph);
     if (f_stage) (void)hipHostFree(f_stage);
     if (f_stage_done) (void)hipEventDestroy(f_stage_done);
     if (chroma_fork) (void)hipEventDestroy(chroma_fork);
@@ -1703,8 +1710,6 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
     }
     SeqCtl* ctl = reinterpret_cast<SeqCtl*>(m + o_ctl);
     VHSD_HIP(vhsd_launch_vhs_ctl_init(ctl, first_begin, st));
-    // the zero `filled` row (BuildUpstreamLineLocations fills no line); written by a fill kernel
-    VHSD_HIP(vhsd_launch_fill_u32(reinterpret_cast<unsigned*>(m + o_zero), (line_stride + 3) / 4, 0u, st));
     SyncWork w{};
     w.sync_ref = sync_ref;
     w.span_len = span_length;
@@ -1754,12 +1759,6 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
       b.seg_carry = reinterpret_cast<int*>(m + o_scar);
       b.seg_off = reinterpret_cast<int*>(m + o_soff);
     }
-    {
-      ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_PULSES, span_length);
-      VHSD_HIP(vhsd_launch_pulses(sync_ref, span_length, c->bias, p->threshold, cfg.pulse_min, cfg.pulse_max,
-                                  c->tile_counts.p, tiles, c->edge_pos.p, c->edge_kind.p, (int)edge_cap_span,
-                                  c->counters.p, c->p_starts.p, c->p_lens.p, capacity, st));
-    }
     FieldTail t{};
     t.spans = reinterpret_cast<const float**>(m + o_tp);
     t.span_len = reinterpret_cast<long long*>(m + o_tp + 8 * F);
@@ -1805,36 +1804,97 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
     const double line_decay = smooth ? std::pow(1.0 - alpha, (double)p->output_line_length) : 0.0;
     const double out_scale = p->output_scale;
     const double conv_off = (p->output_zero + 0.5) - (p->vsync_ire * out_scale);
-    // Each round speculates the read windows from where the sequence stands, decodes
-    // them in parallel and verifies them in order; a round after a mispredicted window
-    // or a failed field carries on from there, and a halted call's rounds do nothing.
-    for (int r = 0; r < rounds; r++) {
-      SyncWork wr = w;
-      if (r > 0) wr.state_shift = 0;
+    // Everything after the state reset and the start position: the nominal-threshold
+    // pulses of the span, whose vertical-sync groups place the read windows, then the
+    // rounds.  Each round speculates the read windows from where the sequence stands,
+    // decodes them in parallel and verifies them in order; a round after a mispredicted
+    // window or a failed field carries on from there, and a halted call's rounds do nothing.
+    auto enqueue = [&](hipStream_t q) -> int32_t {
+      // the zero `filled` row (BuildUpstreamLineLocations fills no line); written by a fill kernel
+      VHSD_HIP(vhsd_launch_fill_u32(reinterpret_cast<unsigned*>(m + o_zero), (line_stride + 3) / 4, 0u, q));
       {
-        ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_LOCATE, max_fields);
-        VHSD_HIP(vhsd_launch_vhs_sync(&cfg, &wr, c->p_starts.p, c->p_lens.p, c->counters.p + 1, capacity,
-                                      span_length, max_fields, st));
+        ProfScope ps(c, q, VHSDECODE_HIP_KERNEL_PULSES, span_length);
+        VHSD_HIP(vhsd_launch_pulses(sync_ref, span_length, c->bias, p->threshold, cfg.pulse_min, cfg.pulse_max,
+                                    c->tile_counts.p, tiles, c->edge_pos.p, c->edge_kind.p, (int)edge_cap_span,
+                                    c->counters.p, c->p_starts.p, c->p_lens.p, capacity, q));
       }
-      VHSD_HIP(vhsd_launch_vhs_tail_tables(&cfg, &wr, &t, max_fields, st));
-      {
-        ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_HSYNC, max_fields);
-        VHSD_HIP(vhsd_launch_hsync(t.spans, t.span_len, t.locs, t.filled, t.line_counts, &hp, t.refined,
-                                   t.filled_out, max_fields, line_stride, c->bias, c->h_work.p, t.threshold, t.dc,
-                                   st));
+      for (int r = 0; r < rounds; r++) {
+        SyncWork wr = w;
+        if (r > 0) wr.state_shift = 0;
+        {
+          ProfScope ps(c, q, VHSDECODE_HIP_KERNEL_LOCATE, max_fields);
+          VHSD_HIP(vhsd_launch_vhs_sync(&cfg, &wr, c->p_starts.p, c->p_lens.p, c->counters.p + 1, capacity,
+                                        span_length, max_fields, q));
+        }
+        VHSD_HIP(vhsd_launch_vhs_tail_tables(&cfg, &wr, &t, max_fields, q));
+        {
+          ProfScope ps(c, q, VHSDECODE_HIP_KERNEL_HSYNC, max_fields);
+          VHSD_HIP(vhsd_launch_hsync(t.spans, t.span_len, t.locs, t.filled, t.line_counts, &hp, t.refined,
+                                     t.filled_out, max_fields, line_stride, c->bias, c->h_work.p, t.threshold, t.dc,
+                                     q));
+        }
+        VHSD_HIP(vhsd_launch_vhs_render(&cfg, &wr, &t, max_fields, q));
+        VHSD_HIP(vhsd_launch_vhs_export(&cfg, &wr, &t, max_fields, q));
+        {
+          ProfScope ps(c, q, VHSDECODE_HIP_KERNEL_TBC, (long long)max_fields * out_lines * p->output_line_length);
+          VHSD_HIP(vhsd_launch_tbc_fields(video, span_length, c->bias, t.origins, t.render, line_stride, out_lines,
+                                          t.first_lines, max_first, 1, max_fields, p->nominal_line_length,
+                                          line_decay, smooth ? c->f_pow.p : nullptr, c->f_x.p, c->f_y0.p,
+                                          p->output_line_length, smooth ? 1 : 0, p->ire0, out_scale,
+                                          1.0 / p->hz_ire, conv_off, c->lut.p, out, t.out_index, &ctl->decoded, q));
+        }
       }
-      VHSD_HIP(vhsd_launch_vhs_render(&cfg, &wr, &t, max_fields, st));
-      VHSD_HIP(vhsd_launch_vhs_export(&cfg, &wr, &t, max_fields, st));
-      {
-        ProfScope ps(c, st, VHSDECODE_HIP_KERNEL_TBC, (long long)max_fields * out_lines * p->output_line_length);
-        VHSD_HIP(vhsd_launch_tbc_fields(video, span_length, c->bias, t.origins, t.render, line_stride, out_lines,
-                                        t.first_lines, max_first, 1, max_fields, p->nominal_line_length,
-                                        line_decay, smooth ? c->f_pow.p : nullptr, c->f_x.p, c->f_y0.p,
-                                        p->output_line_length, smooth ? 1 : 0, p->ire0, out_scale, 1.0 / p->hz_ire,
-                                        conv_off, c->lut.p, out, t.out_index, &ctl->decoded, st));
+      VHSD_HIP(vhsd_launch_vhs_results(ctl, field_count, next_begin, q));
+      return VHSDECODE_HIP_OK;
+    };
+    if (c->profiling) {
+      // kernel timings need the launches themselves
+      const int32_t erc = enqueue(st);
+      if (erc != VHSDECODE_HIP_OK) return erc;
+    } else {
+      // the launches' arguments: a call that repeats them replays the captured graph
+      std::vector<unsigned char> key;
+      auto put = [&](const void* v, size_t bytes) {
+        const unsigned char* b = static_cast<const unsigned char*>(v);
+        key.insert(key.end(), b, b + bytes);
+      };
+      const void* ptrs[] = {sync_ref, video, out, field_count, next_begin, c->tile_counts.p, c->edge_pos.p,
+                            c->edge_kind.p, c->counters.p, c->p_starts.p, c->p_lens.p, c->h_work.p, c->f_pow.p,
+                            c->f_x.p, c->f_y0.p, c->lut.p, m};
+      put(ptrs, sizeof(ptrs));
+      const long long scal[] = {span_length, capacity, tiles, (long long)edge_cap_span, max_fields, rounds,
+                                line_stride, out_lines, smooth ? 1 : 0};
+      put(scal, sizeof(scal));
+      const double dbl[] = {c->bias, p->threshold, line_decay, out_scale, conv_off, p->ire0, p->hz_ire,
+                            p->nominal_line_length};
+      put(dbl, sizeof(dbl));
+      put(&cfg, sizeof(cfg));
+      put(&w, sizeof(w));
+      put(&t, sizeof(t));
+      put(&hp, sizeof(hp));
+      put(p, sizeof(*p));
+      if (!c->vs_exec || key != c->vs_key) {
+        if (c->vs_exec) (void)hipGraphExecDestroy(c->vs_exec);
+        if (c->vs_graph) (void)hipGraphDestroy(c->vs_graph);
+        c->vs_exec = nullptr;
+        c->vs_graph = nullptr;
+        c->vs_key.clear();
+        VHSD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int32_t erc = enqueue(c->stream);
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+        if (erc != VHSDECODE_HIP_OK) {
+          if (graph) (void)hipGraphDestroy(graph);
+          return erc;
+        }
+        VHSD_HIP(ce);
+        c->vs_graph = graph;
+        VHSD_HIP(hipGraphInstantiate(&c->vs_exec, graph, nullptr, nullptr, 0));
+        c->vs_key = std::move(key);
       }
+      VHSD_HIP(hipGraphLaunch(c->vs_exec, st));
     }
-    VHSD_HIP(vhsd_launch_vhs_results(ctl, field_count, next_begin, st));
+
     c->f_ref_nloc = 0;
     c->vs_lines = t.acc_lines;
     c->vs_refined = t.acc_refined;

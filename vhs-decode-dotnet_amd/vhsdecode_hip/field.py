@@ -133,14 +133,14 @@ class HipVhsFieldDecoder:
 
     def decode_async(self, span_video_ptr: int, span_vlp_ptr: int, span_len: int, out_ptr: int, max_fields: int,
                      count_ptr: int, fields_ptr: int = 0, next_begin_ptr: int = 0, first_begin: int = 0,
-                     reset: bool = True, state_shift: int = 0, rounds: int = 4, stream: int = 0):
+                     reset: bool = True, state_shift: int = 0, rounds: int = 2, stream: int = 0):
         self.demod.decode_vhs_fields_device(span_vlp_ptr, span_video_ptr, span_len, first_begin, self.sync,
                                             self.geom, max_fields, out_ptr, count_ptr, fields_ptr, next_begin_ptr,
                                             reset, state_shift, rounds, stream)
         self.started = True
 
     def decode_span(self, span_video_ptr: int, span_vlp_ptr: int, span_len: int, max_fields: int = 64,
-                    first_begin: int = 0, reset: bool = True, state_shift: int = 0, rounds: int = 4,
+                    first_begin: int = 0, reset: bool = True, state_shift: int = 0, rounds: int = 2,
                     want_tables: bool = False) -> SpanDecode:
         from . import _lib, hipmem
         g = self.geom
