@@ -268,6 +268,27 @@ def test_discriminator_constant_tone():
         assert np.max(np.abs(d[1:] - f)) < 200.0
 
 
+@pytest.mark.parametrize("fs", [20e6, 40e6])
+@pytest.mark.parametrize("system,fmt_name", [("PAL", "VHS"), ("NTSC", "VHS"), ("PAL", "SVHS"), ("PAL", "BETAMAX"),
+                                             ("PAL", "VIDEO8")])
+def test_filter_sets_demodulate_a_constant_carrier(system, fmt_name, fs):
+    """A constant FM carrier demodulates to its own frequency through the whole block chain
+    at either sample rate: the RF band-pass passes it, the discriminator returns it and the
+    video (low-pass and de-emphasis) and sync-reference filters keep DC.  This pins the 20
+    MSPS filter sets, which the reference's fixtures (40 MSPS) do not cover: a design at the
+    wrong rate, a band-pass that cuts the carrier or a filter that does not keep DC fails."""
+    fmt = formats.get_format(system, fmt_name, "sp")
+    sp = fmt.sysparams
+    fl = filters.build_tape_filters(fmt, fs, N)
+    mid = slice(2048, N - 2048)
+    for ire in (-30.0, 50.0, 100.0):
+        f = float(sp["ire0"]) + ire * float(sp["hz_ire"])
+        x = np.round(8000.0 * np.cos(2.0 * np.pi * f * np.arange(N) / fs))
+        r = O.demod_block(x, fl)
+        assert np.max(np.abs(r.video[mid] - f)) < 40.0, (ire, np.max(np.abs(r.video[mid] - f)))
+        assert np.max(np.abs(r.video_lowpass[mid] - f)) < 10.0, (ire, np.max(np.abs(r.video_lowpass[mid] - f)))
+
+
 # ----------------------------------------------------------------- pulses
 
 def _pulse_signal():

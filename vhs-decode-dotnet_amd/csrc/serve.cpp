@@ -4,7 +4,9 @@
 // order into slots (a HIP stream, device buffers sized for the largest window, and
 // one demod context per decode format, so filter tables stay resident), issues the
 // whole chain of an admitted window asynchronously on the slot's stream, and polls
-// the slots' completion events.  Slots are created on demand while the HBM budget
+// the slots' completion events (a window's field sequence may take several calls: after
+// each, the worker reads where it stands and issues the next from there, as the whole run
+// does).  Slots are created on demand while the HBM budget
 // allows, and reused.  Windows in different slots overlap on the device; a window's
 // kernels and inputs do not depend on the other windows in flight, so its result is
 // the same alone and in the mix.
@@ -40,7 +42,7 @@ using Clock = std::chrono::steady_clock;
 using vhsd_host::set_error;
 
 constexpr int kBlock = 32768;
-constexpr int kRounds = 4;               // sequence rounds per window
+constexpr int kRounds = 4;               // sequence rounds per call
 constexpr double kPrerollSeconds = 0.12;  // DecodePreviewSegmentProvider.PrerollSeconds
 
 double ms_between(Clock::time_point a, Clock::time_point b) {
@@ -101,10 +103,21 @@ struct Slot {
   vhsdecode_hip_vhs_field_v1* records = nullptr;
   int32_t* meta = nullptr;     // record count, next begin (int64 at 8), kept count (16), kept records (32)
   int32_t* meta_host = nullptr;  // pinned
+  vhsdecode_hip_vhs_field_v1* records_host = nullptr;  // pinned: the last call's records and stop record
+  int32_t* total_dev = nullptr;  // records of the window's sequence so far (device)
   uint16_t* stage = nullptr;     // pinned staging for pageable outputs
   size_t stage_cap = 0;
   std::map<int, vhsdecode_hip_demod_context*> ctx;
   std::shared_ptr<Request> req;
+  // the window in flight: its field sequence so far
+  vhsdecode_hip_demod_context* cur_ctx = nullptr;
+  int phase = 0;             // 1: a sequence call in flight, 2: the selection and downloads
+  int32_t budget = 0;        // records the window's sequence may take
+  int32_t total = 0;         // records decoded so far
+  int64_t begin = 0;         // where the last call started (span coordinates)
+  int64_t span = 0;          // span samples
+  int64_t origin = 0;        // RF sample of span sample 0
+  int stalls = 0;
 };
 
 }  // namespace
@@ -134,6 +147,9 @@ struct vhsdecode_hip_serve {
   int32_t make_slot(Slot** out);
   void release_slot(Slot* s);
   int32_t launch(Slot* s, const std::shared_ptr<Request>& r);
+  int32_t issue_call(Slot* s, int64_t begin, bool reset);
+  int32_t issue_tail(Slot* s);
+  int32_t advance(Slot* s, bool* finished);
   void finish(Slot* s, int32_t status);
 };
 
@@ -147,6 +163,10 @@ namespace {
 
 // PreviewFrameAssembler.Accept over the window's sequence records: field indices of the
 // kept frames, in order (one thread; a window holds a few hundred records at most).
+__global__ void set_count_kernel(int32_t* p, int32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
 __global__ void window_select_kernel(const vhsdecode_hip_vhs_field_v1* rec, const int32_t* count, long long origin,
                                      long long target, long long half_frame, int max_out, int32_t* kept_count,
                                      int32_t* kept) {
@@ -217,6 +237,10 @@ int32_t vhsdecode_hip_serve::make_slot(Slot** out) {
   if (e == hipSuccess) e = hipMalloc((void**)&s->meta, meta_bytes());
   if (e == hipSuccess) e = hipHostMalloc((void**)&s->meta_host, meta_bytes(), hipHostMallocDefault);
   if (e == hipSuccess)
+    e = hipHostMalloc((void**)&s->records_host, sizeof(vhsdecode_hip_vhs_field_v1) * (size_t)records_cap(),
+                      hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->total_dev, 256);
+  if (e == hipSuccess)
     e = hipMalloc((void**)&s->records, sizeof(vhsdecode_hip_vhs_field_v1) * (size_t)records_cap());
   if (e != hipSuccess) {
     release_slot(s.get());
@@ -240,6 +264,8 @@ void vhsdecode_hip_serve::release_slot(Slot* s) {
   if (s->records) (void)hipFree(s->records);
   if (s->meta) (void)hipFree(s->meta);
   if (s->meta_host) (void)hipHostFree(s->meta_host);
+  if (s->records_host) (void)hipHostFree(s->records_host);
+  if (s->total_dev) (void)hipFree(s->total_dev);
   if (s->stage) (void)hipHostFree(s->stage);
   if (s->done) (void)hipEventDestroy(s->done);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -314,15 +340,45 @@ int32_t vhsdecode_hip_serve::launch(Slot* s, const std::shared_ptr<Request>& r) 
   int32_t rc = vhsdecode_hip_demod_span_device(ctx, s->rf, 0, nb, &sp, s->stream);
   if (rc) return rc;
   // the sequence: DecodePreviewSegmentProvider's field budget (output fields, preroll and 8)
-  const int32_t records = std::min(records_cap(), w.max_fields + f.preroll_fields + 8);
+  s->cur_ctx = ctx;
+  s->budget = std::min(records_cap(), w.max_fields + f.preroll_fields + 8);
+  s->total = 0;
+  s->span = n;
+  s->origin = start + f.cut;  // span sample i is RF sample start + cut + i
+  s->stalls = 0;
+  return issue_call(s, 0, true);
+}
+
+// One call of the window's field sequence from `begin`, its records appended after the
+// ones decoded so far; then the records and where the sequence stands to the host.
+int32_t vhsdecode_hip_serve::issue_call(Slot* s, int64_t begin, bool reset) {
+  const Format& f = *formats[sources[s->req->w.stream_id].format];
+  const float* vlp = s->planes + s->span;
+  const int32_t room = s->budget - s->total;
   unsigned char* meta = reinterpret_cast<unsigned char*>(s->meta);
-  rc = vhsdecode_hip_decode_vhs_fields_device(ctx, sp.video_lowpass, sp.video, n, 0, 1, 0, &f.sync, &f.field,
-                                              records, kRounds, s->fields, s->records, s->meta,
-                                              reinterpret_cast<int64_t*>(meta + 8), s->stream);
+  s->begin = begin;
+  // the sequence's records and fields stay in one place, this call's after the earlier ones
+  int32_t rc = vhsd_decode_vhs_fields_from(s->cur_ctx, vlp, s->planes, s->span, begin, reset ? 1 : 0, 0, &f.sync,
+                                           &f.field, s->budget, kRounds, s->fields, s->records, s->meta,
+                                           reinterpret_cast<int64_t*>(meta + 8), s->stream, s->total);
   if (rc) return rc;
-  // span sample i is RF sample start + cut + i
-  hipLaunchKernelGGL(window_select_kernel, dim3(1), dim3(64), 0, s->stream, s->records, s->meta,
-                     (long long)(start + f.cut), (long long)w.first_sample, (long long)f.half_frame, w.max_fields,
+  SV_HIP(hipMemcpyAsync(s->meta_host, s->meta, 16, hipMemcpyDeviceToHost, s->stream));
+  SV_HIP(hipMemcpyAsync(s->records_host, s->records + s->total, sizeof(vhsdecode_hip_vhs_field_v1) * (size_t)room,
+                        hipMemcpyDeviceToHost, s->stream));
+  SV_HIP(hipEventRecord(s->done, s->stream));
+  s->phase = 1;
+  return VHSDECODE_HIP_OK;
+}
+
+// The frame rule over the whole sequence, the kept fields and their download.
+int32_t vhsdecode_hip_serve::issue_tail(Slot* s) {
+  const std::shared_ptr<Request>& r = s->req;
+  const vhsdecode_hip_window_v1& w = r->w;
+  const Format& f = *formats[sources[w.stream_id].format];
+  unsigned char* meta = reinterpret_cast<unsigned char*>(s->meta);
+  hipLaunchKernelGGL(set_count_kernel, dim3(1), dim3(64), 0, s->stream, s->total_dev, s->total);
+  hipLaunchKernelGGL(window_select_kernel, dim3(1), dim3(64), 0, s->stream, s->records, s->total_dev,
+                     (long long)s->origin, (long long)w.first_sample, (long long)f.half_frame, w.max_fields,
                      reinterpret_cast<int32_t*>(meta + 16), reinterpret_cast<int32_t*>(meta + 32));
   hipLaunchKernelGGL(window_gather_kernel, dim3(64, w.max_fields), dim3(256), 0, s->stream, s->fields, f.per_field,
                      reinterpret_cast<const int32_t*>(meta + 16), reinterpret_cast<const int32_t*>(meta + 32),
@@ -343,12 +399,41 @@ int32_t vhsdecode_hip_serve::launch(Slot* s, const std::shared_ptr<Request>& r) 
   }
   SV_HIP(hipMemcpyAsync(dst, s->kept, sizeof(uint16_t) * out_n, hipMemcpyDeviceToHost, s->stream));
   SV_HIP(hipEventRecord(s->done, s->stream));
+  s->phase = 2;
   return VHSDECODE_HIP_OK;
+}
+
+// A slot whose last issued work has completed: after a sequence call, the next call (the
+// sequence stopped at a window the call did not decode: mispredicted, or not read in its
+// rounds) or the tail; after the tail, the window is done.
+int32_t vhsdecode_hip_serve::advance(Slot* s, bool* finished) {
+  *finished = false;
+  if (s->phase == 2) {
+    *finished = true;
+    return VHSDECODE_HIP_OK;
+  }
+  const int32_t room = s->budget - s->total;
+  const int32_t count = std::max(0, std::min(s->meta_host[0] - s->total, room));
+  int64_t next = 0;
+  std::memcpy(&next, reinterpret_cast<const unsigned char*>(s->meta_host) + 8, sizeof(next));
+  s->total += count;
+  bool more = s->total < s->budget;
+  if (more) {
+    const vhsdecode_hip_vhs_field_v1& stop = s->records_host[count];
+    if (stop.status == 5) more = false;  // a window holds more pulses than the lists: decode no further
+    if (stop.status == 6 && stop.window_start + stop.window_length > s->span) more = false;  // left the span
+  }
+  s->stalls = (count == 0 && next == s->begin) ? s->stalls + 1 : 0;
+  if (s->stalls > 2) more = false;
+  if (more) return issue_call(s, next, false);
+  return issue_tail(s);
 }
 
 void vhsdecode_hip_serve::finish(Slot* s, int32_t status) {
   const std::shared_ptr<Request> r = s->req;
   s->req.reset();
+  s->phase = 0;
+  s->cur_ctx = nullptr;
   r->status = status;
   if (status == 0) {
     r->field_count = s->meta_host[4];
@@ -373,8 +458,21 @@ void vhsdecode_hip_serve::run() {
         (void)hipGetLastError();  // not an error: keep it out of the launchers' error checks
         continue;
       }
-      finish(s.get(), q == hipSuccess ? 0 : VHSDECODE_HIP_E_HIP);
-      finished = true;
+      if (q != hipSuccess) {
+        finish(s.get(), VHSDECODE_HIP_E_HIP);
+        finished = true;
+        continue;
+      }
+      bool done_window = false;
+      const int32_t rc = advance(s.get(), &done_window);
+      if (rc) {
+        (void)hipStreamSynchronize(s->stream);
+        finish(s.get(), rc);
+        finished = true;
+      } else if (done_window) {
+        finish(s.get(), 0);
+        finished = true;
+      }
     }
     if (finished) done_cv.notify_all();
     while (!stop && !queue.empty()) {

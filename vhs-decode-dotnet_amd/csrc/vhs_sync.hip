@@ -2778,16 +2778,17 @@ __global__ void results_kernel(const SeqCtl* ctl, int* record_count, long long* 
   if (next_begin) *next_begin = ctl->begin;
 }
 
-__global__ void ctl_init_kernel(SeqCtl* ctl, long long first_begin) {
+__global__ void ctl_init_kernel(SeqCtl* ctl, long long first_begin, int first_record) {
   if (threadIdx.x != 0) return;
-  *ctl = SeqCtl{first_begin, 0, 0, 0, 0};
+  *ctl = SeqCtl{first_begin, first_record, first_record, 0, 0};
 }
 
 }  // namespace vsync
 }  // namespace vhsd
 
-extern "C" hipError_t vhsd_launch_vhs_ctl_init(vhsd::vsync::SeqCtl* ctl, long long first_begin, hipStream_t stream) {
-  hipLaunchKernelGGL(vhsd::vsync::ctl_init_kernel, dim3(1), dim3(64), 0, stream, ctl, first_begin);
+extern "C" hipError_t vhsd_launch_vhs_ctl_init(vhsd::vsync::SeqCtl* ctl, long long first_begin, int first_record,
+                                               hipStream_t stream) {
+  hipLaunchKernelGGL(vhsd::vsync::ctl_init_kernel, dim3(1), dim3(64), 0, stream, ctl, first_begin, first_record);
   return hipGetLastError();
 }
 

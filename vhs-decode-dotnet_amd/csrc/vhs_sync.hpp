@@ -203,7 +203,8 @@ struct FieldTail {
 
 extern "C" hipError_t vhsd_launch_vhs_results(const vhsd::vsync::SeqCtl* ctl, int* record_count, int64_t* next_begin,
                                               hipStream_t stream);
-extern "C" hipError_t vhsd_launch_vhs_ctl_init(vhsd::vsync::SeqCtl* ctl, long long first_begin, hipStream_t stream);
+extern "C" hipError_t vhsd_launch_vhs_ctl_init(vhsd::vsync::SeqCtl* ctl, long long first_begin, int first_record,
+                                               hipStream_t stream);
 extern "C" hipError_t vhsd_launch_vhs_export(const vhsd::vsync::SyncCfg* c, const vhsd::vsync::SyncWork* w,
                                              const vhsd::vsync::FieldTail* t, int max_fields, hipStream_t stream);
 extern "C" hipError_t vhsd_launch_vhs_sync_reset(vhsd::vsync::LevelStateDev* level, vhsd::vsync::SyncHistoryDev* hist,

@@ -1589,10 +1589,20 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
                                                const vhsdecode_hip_field_params_v1* p, int32_t max_fields,
                                                int32_t rounds, uint16_t* out, vhsdecode_hip_vhs_field_v1* fields,
                                                int32_t* field_count, int64_t* next_begin, void* stream) {
+  return vhsd_decode_vhs_fields_from(c, sync_ref, video, span_length, first_begin, reset_state, state_shift, sy, p,
+                                     max_fields, rounds, out, fields, field_count, next_begin, stream, 0);
+}
+
+int32_t vhsd_decode_vhs_fields_from(vhsdecode_hip_demod_context* c, const float* sync_ref, const float* video,
+                                    int64_t span_length, int64_t first_begin, int32_t reset_state,
+                                    int64_t state_shift, const vhsdecode_hip_vhs_sync_v1* sy,
+                                    const vhsdecode_hip_field_params_v1* p, int32_t max_fields, int32_t rounds,
+                                    uint16_t* out, vhsdecode_hip_vhs_field_v1* fields, int32_t* field_count,
+                                    int64_t* next_begin, void* stream, int32_t first_record) {
   using namespace vhsd::vsync;
   try {
     if (!c || !sync_ref || !video || !sy || !p || !out || !field_count || max_fields <= 0 || span_length <= 0 ||
-        first_begin < 0 || rounds < 1 || rounds > 64)
+        first_begin < 0 || rounds < 1 || rounds > 64 || first_record < 0 || first_record >= max_fields)
       return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid vhs field decode request");
     if (sy->struct_size != sizeof(vhsdecode_hip_vhs_sync_v1) || p->struct_size != sizeof(vhsdecode_hip_field_params_v1))
       return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "vhs_sync or field_params struct_size mismatch");
@@ -1709,7 +1719,7 @@ int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, c
       c->vs_state_ready = true;
     }
     SeqCtl* ctl = reinterpret_cast<SeqCtl*>(m + o_ctl);
-    VHSD_HIP(vhsd_launch_vhs_ctl_init(ctl, first_begin, st));
+    VHSD_HIP(vhsd_launch_vhs_ctl_init(ctl, first_begin, first_record, st));
     SyncWork w{};
     w.sync_ref = sync_ref;
     w.span_len = span_length;
