@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "vhsd_args.hpp"
+#include "vhsd_device.hpp"
 
 namespace vhsd {
 
@@ -18,11 +19,11 @@ namespace vhsd {
 constexpr int kPulseTile = 4096;   // samples per workgroup
 constexpr int kPulseThreads = 256;
 
-__device__ __forceinline__ int in_pulse_at(const float* s, long long i, double bias, double high) {
-  // state after processing sample i
+__device__ __forceinline__ int in_pulse_at(const float* s, long long i, float bound) {
+  // state after processing sample i (bound: pass_bound_f32 of the bias and threshold)
   for (long long k = i; k >= 0; k--) {
     const float v = s[k];
-    if (v == v) return ((double)v + bias) <= high ? 1 : 0;
+    if (v == v) return v <= bound ? 1 : 0;
   }
   return 0;  // all NaN from the start: the initial state `s[0] <= high` is false
 }
@@ -49,17 +50,16 @@ __device__ __forceinline__ void load_run(const float* s, long long i0, long long
 }
 
 // Edge codes of the run (0 none, 1 start, 2 end) as two bits per sample.
-__device__ __forceinline__ unsigned run_edges(const float* s, long long i0, long long n, double bias, double high,
-                                              bool vec) {
+__device__ __forceinline__ unsigned run_edges(const float* s, long long i0, long long n, float bound, bool vec) {
   float x[kPulseRun];
   load_run(s, i0, n, vec, x);
-  int st = i0 > 0 && i0 <= n ? in_pulse_at(s, i0 - 1, bias, high) : 0;
+  int st = i0 > 0 && i0 <= n ? in_pulse_at(s, i0 - 1, bound) : 0;
   unsigned codes = 0;
 #pragma unroll
   for (int k = 0; k < kPulseRun; k++) {
     const float v = x[k];
     if (v == v) {
-      const int cur = ((double)v + bias) <= high ? 1 : 0;
+      const int cur = v <= bound ? 1 : 0;
       if (i0 + k >= 1) {
         const unsigned e = (!st && cur) ? 1u : ((st && !cur) ? 2u : 0u);
         codes |= e << (2 * k);
@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(kPulseThreads) pulse_count_kernel(const float*
   __shared__ int wave_counts[kPulseThreads / 64];
   const bool vec = (reinterpret_cast<uintptr_t>(s) & 15) == 0;
   const long long i0 = (long long)blockIdx.x * kPulseTile + (long long)threadIdx.x * kPulseRun;
-  int c = __popc(run_edges(s, i0, n, bias, high, vec));  // a sample's two bits are never both set
+  int c = __popc(run_edges(s, i0, n, pass_bound_f32(bias, 0.0, high), vec));  // a sample's two bits are never both set
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
   if ((threadIdx.x & 63) == 0) wave_counts[threadIdx.x >> 6] = c;
   __syncthreads();
@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(kPulseThreads) pulse_write_kernel(const float*
   const bool vec = (reinterpret_cast<uintptr_t>(s) & 15) == 0;
   const int tile_off = block_total(tile_counts, blockIdx.x, red);
   const long long i0 = (long long)blockIdx.x * kPulseTile + (long long)threadIdx.x * kPulseRun;
-  const unsigned codes = run_edges(s, i0, n, bias, high, vec);
+  const unsigned codes = run_edges(s, i0, n, pass_bound_f32(bias, 0.0, high), vec);
   const int c = __popc(codes);
   // exclusive scan of the per-thread counts
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

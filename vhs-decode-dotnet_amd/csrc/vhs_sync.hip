@@ -34,6 +34,7 @@
 #include <cmath>
 
 #include "vhs_sync.hpp"
+#include "vhsd_device.hpp"
 
 namespace vhsd {
 namespace vsync {
@@ -621,8 +622,20 @@ struct SegRun {
 __device__ __forceinline__ void seg_load(const Plane& x, long long i0, long long s1, SegRun* run) {
   run->before = __builtin_nanf("");
   if (i0 >= s1) return;
+  if (i0 + kRun <= s1 && (reinterpret_cast<uintptr_t>(x.p + i0) & 15) == 0) {
+    const float4* p4 = reinterpret_cast<const float4*>(x.p + i0);
 #pragma unroll
-  for (int k = 0; k < kRun; k++) run->v[k] = i0 + k < s1 ? x.p[i0 + k] : __builtin_nanf("");
+    for (int q = 0; q < kRun / 4; q++) {
+      const float4 v = p4[q];
+      run->v[4 * q] = v.x;
+      run->v[4 * q + 1] = v.y;
+      run->v[4 * q + 2] = v.z;
+      run->v[4 * q + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kRun; k++) run->v[k] = i0 + k < s1 ? x.p[i0 + k] : __builtin_nanf("");
+  }
   for (long long k = i0 - 1; k >= 0; k--) {
     const float v = x.p[k];
     if (v == v) {
@@ -645,14 +658,16 @@ __device__ SegSummary seg_scan(const Plane& x, const SegRun& run, long long s0, 
   const long long i0 = s0 + (long long)tid * kRun;
   unsigned codes = 0;
   if (i0 < s1) {
+    // x.at(i) <= high as one FP32 comparison per sample
+    const float bound = pass_bound_f32(x.bias, x.dc, high);
     const float bv = run.before;
-    int st = (i0 > 0 && bv == bv) ? ((((double)bv + x.bias) + x.dc) <= high ? 1 : 0) : 0;
+    int st = (i0 > 0 && bv == bv) ? (bv <= bound ? 1 : 0) : 0;
 #pragma unroll
     for (int k = 0; k < kRun; k++) {
       const long long i = i0 + k;
       const float v = run.v[k];
       if (i < s1 && v == v) {
-        const int cur = (((double)v + x.bias) + x.dc) <= high ? 1 : 0;
+        const int cur = v <= bound ? 1 : 0;
         if (i >= 1) codes |= ((!st && cur) ? 1u : ((st && !cur) ? 2u : 0u)) << (2 * k);
         st = cur;
       }

@@ -18,6 +18,25 @@ __device__ __forceinline__ int opaque_tid() {
 
 constexpr int kScratchFloats = 512;
 
+// The pulse searches' test ((double)v + bias) + dc <= high of an FP32 sample v as one FP32
+// comparison v <= T: the two FP64 additions round monotonically, so the samples that pass
+// are those up to the largest float T that passes (found from the nearest float to
+// high - dc - bias and a step or two either way).  NaN when no sample passes, +inf when
+// every one does; NaN samples fail both forms.
+__device__ __forceinline__ float pass_bound_f32(double bias, double dc, double high) {
+#pragma clang fp contract(off)
+  auto pass = [&](float v) { return (((double)v + bias) + dc) <= high; };
+  if (!pass(-INFINITY)) return __builtin_nanf("");
+  if (pass(INFINITY)) return INFINITY;
+  float t = (float)((high - dc) - bias);
+  if (!(t == t)) t = 0.0f;
+  if (t == INFINITY) t = 3.40282347e38f;
+  if (t == -INFINITY) t = -3.40282347e38f;
+  while (!pass(t)) t = nextafterf(t, -INFINITY);
+  while (pass(nextafterf(t, INFINITY))) t = nextafterf(t, INFINITY);
+  return t;
+}
+
 // PortedMath.VhsRustAtan2Approximation (PortedMath.cs:519-544): FP32 polynomial
 // atan2, evaluated with separately rounded operations (no contraction) so that
 // identical inputs give bit-identical angles to the reference.
