@@ -145,7 +145,7 @@ def served_run(requests: int, rate: float, streams: int, window_s: float, seed: 
     from vhsdecode_hip.serve import HipServer
     fs = 40e6
     window = int(window_s * fs) // 2 * 2
-    preroll = int(0.12 * fs) + 2
+    preroll = int(round(0.12 * fs))  # samples the server decodes before each window
     specs = [("NTSC", "VHS", "sp"), ("PAL", "VHS", "sp"), ("PAL", "VHS", "lp")]
     bases, fmts, fls = [], [], []
     for k, (system, tf, speed) in enumerate(specs):
@@ -158,7 +158,7 @@ def served_run(requests: int, rate: float, streams: int, window_s: float, seed: 
         fmts.append(fmt)
         fls.append(fl)
     max_fields = int(window_s * 60) + 2
-    srv = HipServer(max_slots=16, hbm_budget_bytes=64 << 30, max_window_samples=window + preroll,
+    srv = HipServer(max_slots=16, hbm_budget_bytes=64 << 30, max_window_samples=window + preroll + 2,
                     max_fields=max_fields)
     fids = [srv.add_format(fmts[k], fls[k], lut) for k in range(len(specs))]
     sids = [srv.add_stream(fids[i % 3], bases[i % 3].ptr, bases[i % 3].array.size, loop=True) for i in range(streams)]
