@@ -66,15 +66,15 @@ __device__ __forceinline__ void rfft_post(c32* v, BlockLds& s, c32& xm) {
   for (int m = 0; m < 16; m++) s.buf[pj + m * kPadRow] = v[m];
   __syncthreads();
   c32 z0 = s.buf[0];
+  const c32 wj32 = tw32k(s.oct, j);  // roots of k = j + 1024 m: rot32(wj32, m)
 #pragma unroll
   for (int m = 0; m < 16; m++) {
-    const int k = j + m * kFftThreads;
     const c32 zk = v[m];
     const c32 zc = conjc(s.buf[mirror_slot(j, m)]);
     const c32 e = add(zk, zc);
     const c32 o = sub(zk, zc);
     // X[k] = 0.5*(e - i*W^k*o)
-    const c32 wo = mul(tw32k(s.oct, k), o);
+    const c32 wo = mul(rot32(wj32, m), o);
     v[m] = scale(add(e, mul_mi(wo)), 0.5f);
   }
   xm = c32{z0.x - z0.y, 0.0f};
@@ -90,6 +90,7 @@ __device__ __forceinline__ void irfft_pre(c32* v, c32 vm, BlockLds& s) {
 #pragma unroll
   for (int m = 0; m < 16; m++) s.buf[pj + m * kPadRow] = v[m];
   __syncthreads();
+  const c32 wj32 = tw32k(s.oct, j);  // roots of k = j + 1024 m: rot32(wj32, m)
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     const int k = j + m * kFftThreads;
@@ -101,7 +102,7 @@ __device__ __forceinline__ void irfft_pre(c32* v, c32 vm, BlockLds& s) {
       const c32 vk = v[m];
       const c32 vc = conjc(s.buf[mirror_slot(j, m)]);
       const c32 e = add(vk, vc);
-      const c32 o = mul(sub(vk, vc), conjc(tw32k(s.oct, k)));
+      const c32 o = mul(sub(vk, vc), conjc(rot32(wj32, m)));
       y = add(e, mul_pi(o));
     }
     v[m] = y;
@@ -126,10 +127,11 @@ __device__ __forceinline__ bool analytic_demod(c32* F, c32 fm, BlockLds& s, floa
   float2* st_go = stash;            // odd-pass input spectrum
   float2* st_th = stash + kFftM;    // (angle, real part) of the even samples
   // odd output samples' input, parked until the even pass is done
+  const c32 wj32 = tw32k(s.oct, j);  // roots of k = j + 1024 m: rot32(wj32, m)
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     const int k = j + m * kFftThreads;
-    const c32 g = (k == 0) ? sub(F[m], fm) : mul(scale(F[m], 2.0f), conjc(tw32k(s.oct, k)));
+    const c32 g = (k == 0) ? sub(F[m], fm) : mul(scale(F[m], 2.0f), conjc(rot32(wj32, m)));
     row_at(st_go, m) = make_float2(g.x, g.y);
   }
   // even output samples
@@ -503,10 +505,11 @@ __device__ __forceinline__ void analytic_demod_ld(c32* F, c32 fm, BlockLds& s, c
   const float inv_n = 1.0f / 32768.0f;
   float2* st_go = stash;
   float2* st_ev = stash + kFftM;
+  const c32 wj32 = tw32k(s.oct, j);  // roots of k = j + 1024 m: rot32(wj32, m)
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     const int k = j + m * kFftThreads;
-    const c32 g = (k == 0) ? sub(F[m], fm) : mul(scale(F[m], 2.0f), conjc(tw32k(s.oct, k)));
+    const c32 g = (k == 0) ? sub(F[m], fm) : mul(scale(F[m], 2.0f), conjc(rot32(wj32, m)));
     row_at(st_go, m) = make_float2(g.x, g.y);
   }
 #pragma unroll
@@ -756,10 +759,11 @@ __device__ __forceinline__ void vhs_block_subdeemph_body(const SubDeemphArgs& a,
   __syncthreads();  // twiddles and transition matrices in LDS; D read by every thread
   float2* st_go = stash;          // odd-pass input
   float2* st_ev = stash + kFftM;  // even samples (real part, magnitude), then real parts (even, odd)
+  const c32 wj32 = tw32k(s.oct, j);  // roots of k = j + 1024 m: rot32(wj32, m)
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     const int k = j + m * kFftThreads;
-    const c32 g = (k == 0) ? sub(F[m], fm) : mul(scale(F[m], 2.0f), conjc(tw32k(s.oct, k)));
+    const c32 g = (k == 0) ? sub(F[m], fm) : mul(scale(F[m], 2.0f), conjc(rot32(wj32, m)));
     row_at(st_go, m) = make_float2(g.x, g.y);
   }
 #pragma unroll
@@ -955,10 +959,11 @@ __device__ __forceinline__ void vhs_block_repair_body(const RepairArgs& r, const
     const int k = j + m * kFftThreads;
     A[2 * k] = make_float2(w[m].x * inv_n, w[m].y * inv_n);
   }
+  const c32 wj32 = tw32k(s.oct, j);  // roots of k = j + 1024 m: rot32(wj32, m)
 #pragma unroll
   for (int m = 0; m < 16; m++) {
     const int k = j + m * kFftThreads;
-    w[m] = (k == 0) ? sub(v[m], fm) : mul(scale(v[m], 2.0f), conjc(tw32k(s.oct, k)));
+    w[m] = (k == 0) ? sub(v[m], fm) : mul(scale(v[m], 2.0f), conjc(rot32(wj32, m)));
   }
   fft16k_inverse(w, s.buf, s.oct);
 #pragma unroll
@@ -1052,7 +1057,7 @@ __global__ void __launch_bounds__(1024) debug_fft_kernel(float2* data, int inver
 template <typename K, typename... Args>
 static hipError_t launch_block_kernel(K kernel, int count, hipStream_t stream, Args... args) {
   const size_t lds = sizeof(vhsd::BlockLds);
-  hipError_t e = hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = vhsd::lds_limit_once((const void*)kernel, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kernel, dim3(count), dim3(1024), lds, stream, args...);
   return hipGetLastError();
@@ -1062,8 +1067,7 @@ static hipError_t launch_block_kernel(K kernel, int count, hipStream_t stream, A
 template <int S>
 static hipError_t launch_chroma_s(const vhsd::ChromaArgs* a, int nblocks, hipStream_t stream) {
   const size_t lds = sizeof(vhsd::BlockLds);
-  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_chroma_kernel<S>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = vhsd::lds_limit_once((const void*)vhsd::vhs_block_chroma_kernel<S>, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(vhsd::vhs_block_chroma_kernel<S>, dim3(nblocks), dim3(1024), lds, stream, *a);
   return hipGetLastError();
@@ -1075,8 +1079,7 @@ size_t vhsd_block_lds_bytes() { return sizeof(vhsd::BlockLds); }
 
 hipError_t vhsd_launch_block_demod(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream) {
   const size_t lds = sizeof(vhsd::BlockLds);
-  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_demod_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = vhsd::lds_limit_once((const void*)vhsd::vhs_block_demod_kernel, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(vhsd::vhs_block_demod_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
   return hipGetLastError();
@@ -1100,8 +1103,7 @@ hipError_t vhsd_launch_debug_replace_spikes(const float* dm, const float* dd, in
 
 hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const float2* oct, hipStream_t stream) {
   const size_t lds = sizeof(vhsd::BlockLds);
-  hipError_t e = hipFuncSetAttribute((const void*)vhsd::debug_fft_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = vhsd::lds_limit_once((const void*)vhsd::debug_fft_kernel, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(vhsd::debug_fft_kernel, dim3(count), dim3(1024), lds, stream, reinterpret_cast<float2*>(data),
                      inverse, oct);
@@ -1110,8 +1112,7 @@ hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const floa
 
 hipError_t vhsd_launch_block_demod_boost(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream) {
   const size_t lds = sizeof(vhsd::BlockLds);
-  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_demod_boost_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = vhsd::lds_limit_once((const void*)vhsd::vhs_block_demod_boost_kernel, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(vhsd::vhs_block_demod_boost_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
   return hipGetLastError();
@@ -1191,8 +1192,7 @@ hipError_t vhsd_launch_fill_u32(unsigned* p, long long n, unsigned value, hipStr
 
 hipError_t vhsd_launch_block_demod_ld(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream) {
   const size_t lds = sizeof(vhsd::BlockLds);
-  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_demod_ld_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = vhsd::lds_limit_once((const void*)vhsd::vhs_block_demod_ld_kernel, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(vhsd::vhs_block_demod_ld_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
   return hipGetLastError();
@@ -1200,8 +1200,7 @@ hipError_t vhsd_launch_block_demod_ld(const vhsd::DemodArgs* a, int nblocks, hip
 
 hipError_t vhsd_launch_block_video(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream) {
   const size_t lds = sizeof(vhsd::BlockLds);
-  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_video_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = vhsd::lds_limit_once((const void*)vhsd::vhs_block_video_kernel, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(vhsd::vhs_block_video_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
   return hipGetLastError();
@@ -1209,8 +1208,7 @@ hipError_t vhsd_launch_block_video(const vhsd::DemodArgs* a, int nblocks, hipStr
 
 hipError_t vhsd_launch_block_subdeemph(const vhsd::SubDeemphArgs* a, int nblocks, hipStream_t stream) {
   const size_t lds = sizeof(vhsd::BlockLds);
-  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_subdeemph_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = vhsd::lds_limit_once((const void*)vhsd::vhs_block_subdeemph_kernel, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(vhsd::vhs_block_subdeemph_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
   return hipGetLastError();
@@ -1218,8 +1216,7 @@ hipError_t vhsd_launch_block_subdeemph(const vhsd::SubDeemphArgs* a, int nblocks
 
 hipError_t vhsd_launch_block_repair(const vhsd::RepairArgs* a, int nblocks, hipStream_t stream) {
   const size_t lds = sizeof(vhsd::BlockLds);
-  hipError_t e = hipFuncSetAttribute((const void*)vhsd::vhs_block_repair_kernel,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = vhsd::lds_limit_once((const void*)vhsd::vhs_block_repair_kernel, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(vhsd::vhs_block_repair_kernel, dim3(nblocks), dim3(1024), lds, stream, *a);
   return hipGetLastError();

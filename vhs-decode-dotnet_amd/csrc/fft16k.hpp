@@ -13,7 +13,9 @@
 // symmetry, so every 16384-th root is a table value up to exact sign/swap
 // operations.  A radix-16 pass looks up w, w^2, w^4 and w^8 and forms the other
 // eleven powers as products of at most three table values; the radix-4 pass looks
-// up w and w^2.  Odd 32768-th roots take one extra multiply by exp(-i*pi/16384).
+// up the thread's W^j and W^2j once and rotates them by compile-time constants for its
+// four butterflies (rot32).  Odd 32768-th roots take one extra multiply by
+// exp(-i*pi/16384).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -75,6 +77,39 @@ __device__ __forceinline__ c32 tw32k(const float2* __restrict__ oct, int t) {
     w = mul(w, w1);
   }
   return w;
+}
+
+// exp(-2*pi*i*m/32) rounded to float, m = 0..15 (m a compile-time constant after
+// unrolling, so the switch folds to two literals).
+__device__ __forceinline__ c32 w32c(int m) {
+  switch (m & 15) {
+    case 0: return c32{1.0f, 0.0f};
+    case 1: return c32{0.98078528040323044f, -0.19509032201612826f};
+    case 2: return c32{0.92387953251128674f, -0.38268343236508978f};
+    case 3: return c32{0.83146961230254524f, -0.55557023301960218f};
+    case 4: return c32{0.70710678118654752f, -0.70710678118654752f};
+    case 5: return c32{0.55557023301960218f, -0.83146961230254524f};
+    case 6: return c32{0.38268343236508978f, -0.92387953251128674f};
+    case 7: return c32{0.19509032201612826f, -0.98078528040323044f};
+    case 8: return c32{0.0f, -1.0f};
+    case 9: return c32{-0.19509032201612826f, -0.98078528040323044f};
+    case 10: return c32{-0.38268343236508978f, -0.92387953251128674f};
+    case 11: return c32{-0.55557023301960218f, -0.83146961230254524f};
+    case 12: return c32{-0.70710678118654752f, -0.70710678118654752f};
+    case 13: return c32{-0.83146961230254524f, -0.55557023301960218f};
+    case 14: return c32{-0.92387953251128674f, -0.38268343236508978f};
+    default: return c32{-0.98078528040323044f, -0.19509032201612826f};
+  }
+}
+
+// w * exp(-2*pi*i*m/32) for a compile-time m: the root of index k + 1024 m of the
+// 32768-th roots from the root of k (m = 8 is the exact multiply by -i).  One table
+// lookup per thread then serves all sixteen strided slots; the product adds about one
+// ulp to the table value's rounding.
+__device__ __forceinline__ c32 rot32(c32 w, int m) {
+  if (m == 0) return w;
+  if (m == 8) return mul_mi(w);
+  return mul(w, w32c(m));
 }
 
 __device__ __forceinline__ void dft4(c32& a0, c32& a1, c32& a2, c32& a3) {
@@ -172,11 +207,13 @@ __device__ __forceinline__ void fft16k_forward(c32* v, c32* __restrict__ buf, co
     for (int m = 0; m < 16; m++) v[m] = buf[rbase + m * (kFftThreads + kFftThreads / 32)];
     __syncthreads();
   }
-  // radix-4 pass, Ns = 4096: butterfly b = j + 1024 q on elements v[q + 4 r]
+  // radix-4 pass, Ns = 4096: butterfly b = j + 1024 q on elements v[q + 4 r]; its
+  // twiddles W^b and W^2b (W the 16384-th root) are the thread's W^j and W^2j rotated by
+  // the constants exp(-2 pi i q/16) and exp(-2 pi i q/8)
+  const c32 wj1 = tw16k(oct, j), wj2 = tw16k(oct, 2 * j);
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    const int b = j + kFftThreads * q;
-    const c32 w1 = tw16k(oct, b), w2 = tw16k(oct, 2 * b);
+    const c32 w1 = rot32(wj1, 2 * q), w2 = rot32(wj2, 4 * q);
     c32 a0 = v[q], a1 = mul(v[q + 4], w1), a2 = mul(v[q + 8], w2), a3 = mul(v[q + 12], mul(w2, w1));
     dft4(a0, a1, a2, a3);
     v[q] = a0;

@@ -221,14 +221,11 @@ hipError_t vhsd_launch_hifi(const float* in, float* out, float2* work, float* ph
   const int col_lds = (int)vhsd::kHifiColLds;
   const int row_lds = (int)sizeof(vhsd::HifiRowLds);
   hipError_t e;
-  if ((e = hipFuncSetAttribute((const void*)vhsd::hifi_columns_forward_kernel,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, col_lds)) != hipSuccess)
+  if ((e = vhsd::lds_limit_once((const void*)vhsd::hifi_columns_forward_kernel, col_lds)) != hipSuccess)
     return e;
-  if ((e = hipFuncSetAttribute((const void*)vhsd::hifi_columns_phase_kernel,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, col_lds)) != hipSuccess)
+  if ((e = vhsd::lds_limit_once((const void*)vhsd::hifi_columns_phase_kernel, col_lds)) != hipSuccess)
     return e;
-  if ((e = hipFuncSetAttribute((const void*)vhsd::hifi_rows_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               row_lds)) != hipSuccess)
+  if ((e = vhsd::lds_limit_once((const void*)vhsd::hifi_rows_kernel, row_lds)) != hipSuccess)
     return e;
   hipLaunchKernelGGL(vhsd::hifi_columns_forward_kernel, dim3(col_blocks), dim3(vhsd::kHifiColThreads), col_lds,
                      stream, a);

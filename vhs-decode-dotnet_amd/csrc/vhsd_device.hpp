@@ -2,6 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <utility>
+#include <vector>
+
 namespace vhsd {
 
 // threadIdx.x through an opaque register copy.  A kernel inlines several transforms;
@@ -17,6 +21,23 @@ __device__ __forceinline__ int opaque_tid() {
 
 
 constexpr int kScratchFloats = 512;
+
+// Raises kernel `fn`'s dynamic LDS limit to `bytes` on the current device, once: the
+// attribute stays with the function on that device, so later launches skip the call
+// (it took a measurable share of the host's enqueue time when made per launch).
+inline hipError_t lds_limit_once(const void* fn, int bytes) {
+  static std::mutex mu;
+  static std::vector<std::pair<const void*, int>> done;  // (kernel, device) pairs set
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(mu);
+  for (const auto& d : done)
+    if (d.first == fn && d.second == dev) return hipSuccess;
+  e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e == hipSuccess) done.emplace_back(fn, dev);
+  return e;
+}
 
 // The pulse searches' test ((double)v + bias) + dc <= high of an FP32 sample v as one FP32
 // comparison v <= T: the two FP64 additions round monotonically, so the samples that pass
@@ -38,17 +59,22 @@ __device__ __forceinline__ float pass_bound_f32(double bias, double dc, double h
 }
 
 // PortedMath.VhsRustAtan2Approximation (PortedMath.cs:519-544): FP32 polynomial
-// atan2, evaluated with separately rounded operations (no contraction) so that
-// identical inputs give bit-identical angles to the reference.
+// atan2, evaluated with separately rounded operations (no contraction) in the
+// reference's order.  The quotient is the hardware reciprocal's product with one FMA
+// correction step, which gives the IEEE quotient but for rare last-bit ties: the
+// analytic samples it divides are FP32 transform outputs whose own rounding is ~1e-6
+// relative, so the discriminator is judged by tolerance and the 9-instruction IEEE
+// division sequence would buy nothing.
 __device__ __forceinline__ float vhs_atan2(float y, float x) {
 #pragma clang fp contract(off)
   const float min_normal = 1.17549435E-38f;
   x += copysignf(min_normal, x);
   const bool swap = fabsf(x) < fabsf(y);
-  // one division of selected operands (the compiler otherwise emits both quotients
-  // under divergent branches)
+  // one division of selected operands; |den| >= min_normal and |num| <= |den|
   const float num = swap ? x : y, den = swap ? y : x;
-  const float input = num / den;
+  const float rc = __builtin_amdgcn_rcpf(den);
+  const float q0 = num * rc;
+  const float input = __builtin_fmaf(__builtin_fmaf(-q0, den, num), rc, q0);
   const float sq = input * input;
   float r = input * (0.99997726f +
                      (sq * (-0.33262347f +
@@ -57,11 +83,10 @@ __device__ __forceinline__ float vhs_atan2(float y, float x) {
   const float pi = 3.14159265358979323846f;
   const float half_pi = input >= 0.0f ? pi / 2.0f : -pi / 2.0f;
   r = swap ? half_pi - r : r;
+  // quadrant fix without branches: x < 0 adds pi (y >= 0) or -pi (y < 0)
   const bool xp = x >= 0.0f, yp = y >= 0.0f;
-  if (xp && yp) return r;
-  if (!xp && yp) return pi + r;
-  if (!xp && !yp) return -pi + r;
-  return r;
+  const float shifted = (yp ? pi : -pi) + r;
+  return xp ? r : shifted;
 }
 
 // x / (2 pi) in FP32, correctly rounded: a multiply by the correctly rounded
