@@ -489,6 +489,61 @@ VHSDECODE_HIP_API int32_t vhsdecode_hip_decode_vhs_fields_device(
     const vhsdecode_hip_field_params_v1* params, int32_t max_fields, int32_t rounds, uint16_t* out,
     vhsdecode_hip_vhs_field_v1* fields, int32_t* field_count, int64_t* next_begin, void* stream);
 
+/* ---------------------------------------------------------------- VHS colour-under chroma
+ * The chroma of the vhs command for a colour-under format (VHS, S-VHS), the reference's
+ * default with the v0.4.0 behaviour profile: TbcFieldDecodePipeline.DecodePrepared
+ * (Decode/TbcFieldDecodePipeline.cs:1366-1392, 1425-1470, 1869-1920) with
+ * VhsChromaDecoder (Decode/VhsChromaDecoder.cs): the burst phase of every line probed from
+ * the chroma plane resampled at the hsync-refined locations (AnalyzeFieldPhaseWithWorkspace
+ * :574-702, ProbeUpconvertedBurstValue :1660-1713, DemodBurstFloat32 :1521-1626, the
+ * track-phase sequence :1906-1965, 3927-4293, carried from field to field), the line
+ * locations moved by the burst phase (RefineLineLocationsFromBurst :2342-2428) for the
+ * luma render as well, and the chroma field: render resample, NTSC burst de-emphasis,
+ * heterodyne upconversion (NTSC phase compensated, :2009-2045, 2167-2249), the final
+ * band-pass forward and backward in float32, comb and automatic gain to uint16
+ * (CompleteFieldWithPhaseCore :953-1251, ApplyAutomaticChromaGainWithCombToU16
+ * :3504-3656).  Not covered: the colour killer, track-phase detection, chroma AFC
+ * (Betamax PAL), the Video8 chroma de-emphasis and the "current" behaviour profile. */
+typedef struct vhsdecode_hip_chroma_params_v1 {
+  uint32_t struct_size;
+  int32_t is_ntsc;                 /* 1 NTSC (phase-compensated upconversion, 1H comb), 0 PAL (2H comb) */
+  double fsc_mhz;                  /* sysparams fsc_mhz */
+  double color_under_hz;           /* rfparams color_under_carrier */
+  double output_rate_hz;           /* TbcFrameSpec.OutputSampleRateHz (outfreq x 1e6) */
+  int32_t field_lines;             /* output lines per field (frame_lines / 2 + 1) */
+  int32_t output_line_length;      /* outlinelen */
+  int32_t burst_start, burst_end;  /* burst window in output samples (BuildChromaFieldOptions,
+                                      TbcFieldDecodePipeline.cs:2626-2690) */
+  double burst_abs_ref;            /* sysparams burst_abs_ref */
+  int32_t rotation[2];             /* rfparams chroma_rotation */
+  int32_t sections;                /* final band-pass sections, 1..8 */
+  int32_t input_line_length;       /* max(1, round half even (nominal line length, samples)) */
+  double final_sos[8][6];          /* BuildChromaFinalSosFilter (Dsp/DecodeFilterSet.cs:710-751):
+                                      b0 b1 b2 a0 a1 a2 per section */
+} vhsdecode_hip_chroma_params_v1;
+
+/* Sets (params non-NULL) or clears the context's colour-under chroma format; builds the
+ * heterodyne and burst-carrier tables of one field (BuildHeterodyneTable :1967-2007,
+ * BuildCarrierTables :3793-3893) on the device.  No counterpart entry in the reference:
+ * there the chroma options are part of the field decoder's construction. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_set_vhs_chroma(vhsdecode_hip_demod_context* context,
+                                                       const vhsdecode_hip_chroma_params_v1* params);
+
+/* vhsdecode_hip_decode_vhs_fields_device with the chroma of every decoded field: `chroma`
+ * is the span's chroma plane (the block chroma output, same span geometry as sync_ref);
+ * out_chroma: device [max_fields][field_lines * output_line_length], field k's chroma at
+ * slot k as its luma in `out`.  The luma render uses the burst-refined line locations, as
+ * the reference's does whenever it writes chroma (vhsdecode_hip_decode_vhs_fields_device
+ * matches its decode with chroma off).  field_number_base: the number of fields decoded
+ * before this sequence (NtscFieldPhaseTarget's field number), used when reset_state is 1;
+ * a continued sequence counts on.  Requires vhsdecode_hip_set_vhs_chroma. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_decode_vhs_fields_chroma_device(
+    vhsdecode_hip_demod_context* context, const float* sync_ref, const float* video, const float* chroma,
+    int64_t span_length, int64_t first_begin, int32_t reset_state, int64_t state_shift,
+    const vhsdecode_hip_vhs_sync_v1* sync, const vhsdecode_hip_field_params_v1* params, int32_t max_fields,
+    int32_t rounds, uint16_t* out, uint16_t* out_chroma, int32_t field_number_base,
+    vhsdecode_hip_vhs_field_v1* fields, int32_t* field_count, int64_t* next_begin, void* stream);
+
 /* The line tables of the last vhsdecode_hip_decode_vhs_fields_device call on this context,
  * [records][stride] (rows of decoded records only): which 0, BuildUpstreamLineLocations
  * (TbcFieldDecodePipeline.cs:1244-1295); 1, after RefineLineLocationsFromHSync
@@ -698,7 +753,8 @@ enum {
   VHSDECODE_HIP_KERNEL_BLOCK_VIDEO = 7,
   VHSDECODE_HIP_KERNEL_LOCATE = 8,
   VHSDECODE_HIP_KERNEL_DROPOUTS = 9,
-  VHSDECODE_HIP_KERNEL_COUNT = 10
+  VHSDECODE_HIP_KERNEL_CHROMA_FIELD = 10,  /* colour-under burst analysis and chroma field decode */
+  VHSDECODE_HIP_KERNEL_COUNT = 11
 };
 
 typedef struct vhsdecode_hip_profile_v1 {

@@ -522,6 +522,17 @@ void fft_any(cd* x, int n, bool inverse) {
 
 }  // namespace
 
+// The filters and pad rule for the chroma restatement (chroma.cpp, same library).
+namespace oracle_shared {
+void sosfiltfilt_f32(const double* sos, int sections, const double* input, int n, int pad, float* output) {
+  ::sosfiltfilt_f32(sos, sections, input, n, pad, output);
+}
+void sosfiltfilt_f64(const double* sos, int sections, const double* input, int n, int pad, double* output) {
+  ::sosfiltfilt_f64(sos, sections, input, n, pad, output);
+}
+int default_pad_length(const double* sos, int sections) { return ::default_pad_length(sos, sections); }
+}  // namespace oracle_shared
+
 extern "C" {
 
 int oracle_abi_version() { return 1; }

@@ -19,6 +19,7 @@
 
 #include "vhsd_args.hpp"
 #include "vhs_sync.hpp"
+#include "chroma.hpp"
 
 #include "host_common.hpp"
 
@@ -281,6 +282,13 @@ struct vhsdecode_hip_demod_context {
   // at its start and carries from call to call
   DevBuf<unsigned char> vs_mem, vs_state;
   bool vs_state_ready = false;
+  // colour-under chroma (vhsdecode_hip_set_vhs_chroma): the format, its tables, the state
+  // carried with the field sequence and the per-call work buffers
+  bool chroma_on = false;
+  vhsd::chroma::ChromaCfg ccfg{};
+  DevBuf<float> c_het, c_sin, c_cos;
+  DevBuf<unsigned char> c_state, c_work;
+  bool c_state_ready = false;
   const double* vs_lines = nullptr;    // line locations, hsync-refined and render locations of the
   const double* vs_refined = nullptr;  // last vhs field decode ([fields][stride], window coordinates)
   const double* vs_render = nullptr;
@@ -328,6 +336,7 @@ struct vhsdecode_hip_demod_context {
     h_fil.release(); h_fo.release(); h_ptrs.release(); h_work.release(); lut.release(); t_loc.release(); t_adj.release();
     t_f64.release(); t_u16.release(); s_rf.release(); s_planes.release(); s_flags.release();
     vs_mem.release(); vs_state.release();
+    c_het.release(); c_sin.release(); c_cos.release(); c_state.release(); c_work.release();
     f_loc.release(); f_ref.release(); f_x.release(); f_y0.release(); f_pow.release(); f_fil.release();
     f_fo.release(); f_tab.release();
     if (vs_exec) (void)hipGraphExecDestroy(vs_exec);
@@ -1582,6 +1591,134 @@ int32_t vhsdecode_hip_decode_fields_device(vhsdecode_hip_demod_context* c, const
   }
 }
 
+namespace {
+// NumpyTrigFloat32 (Decode/VhsChromaDecoder.cs:3834-3893): numpy's float32 sin/cos, a
+// Cody-Waite reduction by pi/2 and minimax polynomials in FMA form; arguments past the
+// reduction's range take the C library's float sin/cos.
+#pragma clang fp contract(off)
+float numpy_trig_f32(float x, bool cosine) {
+  auto bits = [](uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+  };
+  if (std::isnan(x)) return NAN;
+  if (std::fabs(x) > (cosine ? 71476.0625f : 117435.992f)) return cosine ? std::cos(x) : std::sin(x);
+  float q = x * bits(0x3F22F983u);
+  q = (q + 12582912.0f) - 12582912.0f;
+  float r = std::fma(q, bits(0xBFC90FD8u), x);
+  r = std::fma(q, bits(0xB4A8885Au), r);
+  r = std::fma(q, bits(0xA7C234C4u), r);
+  const float r2 = r * r;
+  float cv = std::fma(bits(0x37CC730Bu), r2, bits(0xBAB6036Eu));
+  cv = std::fma(cv, r2, bits(0x3D2AAA9Eu));
+  cv = std::fma(cv, r2, -0.5f);
+  cv = std::fma(cv, r2, 1.0f);
+  float sv = std::fma(bits(0x363E9DDEu), r2, bits(0xB95035DDu));
+  sv = std::fma(sv, r2, bits(0x3C0888CDu));
+  sv = std::fma(sv, r2, -1.0f / 6.0f);
+  sv = std::fma(sv, r2, 0.0f);
+  sv = std::fma(sv, r, r);
+  int iq = (int)std::nearbyint(q);
+  if (cosine) iq++;
+  const float v = (iq & 1) == 0 ? sv : cv;
+  return (iq & 2) != 0 ? -v : v;
+}
+}  // namespace
+
+int32_t vhsdecode_hip_set_vhs_chroma(vhsdecode_hip_demod_context* c, const vhsdecode_hip_chroma_params_v1* p) {
+#pragma clang fp contract(off)
+  try {
+    if (!c) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid context");
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (!p) {
+      c->chroma_on = false;
+      return VHSDECODE_HIP_OK;
+    }
+    if (p->struct_size != sizeof(vhsdecode_hip_chroma_params_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "chroma_params struct_size mismatch");
+    const int N = p->field_lines, L = p->output_line_length;
+    if (N <= 20 || L <= 0 || (long long)N * L > (1LL << 26) || p->sections < 1 ||
+        p->sections > vhsd::chroma::kMaxSections || !(p->fsc_mhz > 0.0) || !(p->color_under_hz > 0.0) ||
+        !(p->output_rate_hz > 0.0) || p->burst_start < 0 || p->burst_end <= p->burst_start ||
+        p->burst_end > L || p->burst_end + 5 >= L || p->input_line_length <= 0)
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid chroma geometry");
+    VHSD_HIP(hipSetDevice(c->device));
+    vhsd::chroma::ChromaCfg& cc = c->ccfg;
+    cc = vhsd::chroma::ChromaCfg{};
+    cc.is_ntsc = p->is_ntsc ? 1 : 0;
+    cc.lines = N;
+    cc.line_len = L;
+    cc.burst_start = p->burst_start;
+    cc.burst_end = p->burst_end;
+    cc.win_max = p->burst_end + (p->burst_end - p->burst_start);
+    cc.pad = default_pad_length(&p->final_sos[0][0], p->sections);
+    cc.probe_ext = cc.win_max + 2 * cc.pad;
+    if ((size_t)cc.probe_ext * vhsd::chroma::kProbeThreads * sizeof(float) > 160 * 1024)
+      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "burst window too long for the probe kernel");
+    cc.sections = p->sections;
+    float c32[vhsd::kMaxSections][6];
+    for (int s = 0; s < p->sections; s++)
+      for (int q = 0; q < 6; q++) c32[s][q] = cc.sos[s][q] = (float)p->final_sos[s][q];
+    float zi[2 * vhsd::kMaxSections];
+    steady_state_f32(c32, p->sections, zi);
+    for (int s = 0; s < p->sections; s++) {
+      cc.zi[s][0] = zi[2 * s];
+      cc.zi[s][1] = zi[2 * s + 1];
+    }
+    cc.rot[0] = p->rotation[0];
+    cc.rot[1] = p->rotation[1];
+    cc.input_line_length = p->input_line_length;
+    cc.fsc_ratio = (p->output_rate_hz / 1000000.0) / p->fsc_mhz;
+    cc.cuc_hz = p->color_under_hz;
+    cc.fsc_mhz = p->fsc_mhz;
+    cc.burst_abs_ref = p->burst_abs_ref;
+    // BuildHeterodyneTable and BuildCarrierTables at 4 fsc over one field (C library cos,
+    // as Math.Cos on Linux)
+    const size_t n = (size_t)N * L;
+    const double rate = p->fsc_mhz * 4.0;
+    const double het_scale = (p->fsc_mhz + p->color_under_hz / 1000000.0) / rate;
+    const double wave_scale = p->fsc_mhz / rate;
+    std::vector<float> het(4 * n), bs(n), bc(n);
+    for (int ph = 0; ph < 4; ph++) {
+      const double off = ((M_PI / 2.0) * ph) + 0.0;
+      for (size_t i = 0; i < n; i++) het[ph * n + i] = (float)-std::cos(((2.0 * M_PI) * het_scale * (double)i) + off);
+    }
+    for (size_t i = 0; i < n; i++) {
+      const float theta = (float)((2.0 * M_PI) * wave_scale * (double)i);
+      bs[i] = numpy_trig_f32(theta, false);
+      bc[i] = numpy_trig_f32(theta, true);
+    }
+    VHSD_HIP(c->c_het.ensure(4 * n));
+    VHSD_HIP(c->c_sin.ensure(n));
+    VHSD_HIP(c->c_cos.ensure(n));
+    VHSD_HIP(hipMemcpy(c->c_het.p, het.data(), 4 * n * sizeof(float), hipMemcpyHostToDevice));
+    VHSD_HIP(hipMemcpy(c->c_sin.p, bs.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    VHSD_HIP(hipMemcpy(c->c_cos.p, bc.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    VHSD_HIP(c->c_state.ensure(sizeof(vhsd::chroma::ChromaState)));
+    cc.het = c->c_het.p;
+    cc.bsin = c->c_sin.p;
+    cc.bcos = c->c_cos.p;
+    c->chroma_on = true;
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_decode_vhs_fields_chroma_device(
+    vhsdecode_hip_demod_context* c, const float* sync_ref, const float* video, const float* chroma,
+    int64_t span_length, int64_t first_begin, int32_t reset_state, int64_t state_shift,
+    const vhsdecode_hip_vhs_sync_v1* sy, const vhsdecode_hip_field_params_v1* p, int32_t max_fields,
+    int32_t rounds, uint16_t* out, uint16_t* out_chroma, int32_t field_number_base,
+    vhsdecode_hip_vhs_field_v1* fields, int32_t* field_count, int64_t* next_begin, void* stream) {
+  if (!chroma || !out_chroma || field_number_base < 0)
+    return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid chroma plane, output or field number");
+  const vhsd_chroma_io io{chroma, out_chroma, field_number_base};
+  return vhsd_decode_vhs_fields_from(c, sync_ref, video, span_length, first_begin, reset_state, state_shift, sy, p,
+                                     max_fields, rounds, out, fields, field_count, next_begin, stream, 0, &io);
+}
+
 int32_t vhsdecode_hip_decode_vhs_fields_device(vhsdecode_hip_demod_context* c, const float* sync_ref,
                                                const float* video, int64_t span_length, int64_t first_begin,
                                                int32_t reset_state, int64_t state_shift,
@@ -1598,7 +1735,8 @@ int32_t vhsd_decode_vhs_fields_from(vhsdecode_hip_demod_context* c, const float*
                                     int64_t state_shift, const vhsdecode_hip_vhs_sync_v1* sy,
                                     const vhsdecode_hip_field_params_v1* p, int32_t max_fields, int32_t rounds,
                                     uint16_t* out, vhsdecode_hip_vhs_field_v1* fields, int32_t* field_count,
-                                    int64_t* next_begin, void* stream, int32_t first_record) {
+                                    int64_t* next_begin, void* stream, int32_t first_record,
+                                    const vhsd_chroma_io* chroma) {
   using namespace vhsd::vsync;
   try {
     if (!c || !sync_ref || !video || !sy || !p || !out || !field_count || max_fields <= 0 || span_length <= 0 ||
@@ -1626,6 +1764,13 @@ int32_t vhsd_decode_vhs_fields_from(vhsdecode_hip_demod_context* c, const float*
     int32_t rc = ensure_lut(c);
     if (rc) return rc;
     hipStream_t st = pick_stream(c, stream);
+    if (chroma) {
+      if (!c->chroma_on) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "no chroma format set on this context");
+      if (c->ccfg.lines != out_lines || c->ccfg.line_len != p->output_line_length)
+        return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "chroma format and field geometry differ");
+      if (!reset_state && !c->c_state_ready)
+        return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "no chroma state to continue on this context");
+    }
     // nominal-threshold pulses of the span: their vertical-sync groups place the read windows
     const int capacity = (int)std::min<int64_t>(span_length / 16 + 1024, INT32_MAX / 2);
     const int tiles = (int)((span_length + 4095) / 4096);
@@ -1718,6 +1863,11 @@ int32_t vhsd_decode_vhs_fields_from(vhsdecode_hip_demod_context* c, const float*
       VHSD_HIP(vhsd_launch_vhs_sync_reset(level_state, history, st));
       c->vs_state_ready = true;
     }
+    if (chroma && reset_state) {
+      VHSD_HIP(vhsd_launch_chroma_state(reinterpret_cast<vhsd::chroma::ChromaState*>(c->c_state.p), 1,
+                                        chroma->field_number_base, nullptr, 0, st));
+      c->c_state_ready = true;
+    }
     SeqCtl* ctl = reinterpret_cast<SeqCtl*>(m + o_ctl);
     VHSD_HIP(vhsd_launch_vhs_ctl_init(ctl, first_begin, first_record, st));
     SyncWork w{};
@@ -1798,6 +1948,48 @@ int32_t vhsd_decode_vhs_fields_from(vhsdecode_hip_demod_context* c, const float*
     // OutputFirstLine (TbcFieldDecodePipeline.cs:2475-2483)
     t.out_first_line[0] = cfg.is_pal ? 3 : 1;
     t.out_first_line[1] = cfg.is_pal ? 4 : 1;
+    // chroma work buffers (one allocation): probe windows and results, the per-field sequence
+    // choice, analysis level adjusts, the chroma field and the final filter's two buffers
+    vhsd::chroma::ChromaWork cw{};
+    if (chroma) {
+      using vhsd::chroma::kFiltThreads;
+      const vhsd::chroma::ChromaCfg& cc = c->ccfg;
+      const size_t nf = (size_t)cc.lines * cc.line_len;
+      const int lvl_stride = out_lines + max_first;
+      const int mext = (int)nf + 2 * cc.pad;
+      const int chunk = (mext + kFiltThreads - 1) / kFiltThreads;
+      size_t co = 0;
+      auto ctake = [&](size_t bytes) {
+        const size_t o = co;
+        co += (bytes + 255) & ~(size_t)255;
+        return o;
+      };
+      const size_t o_win = ctake(4 * F * cc.lines * cc.win_max), o_probe = ctake(8 * F * 2 * cc.lines * 5);
+      const size_t o_flip = ctake(4 * F * 2), o_summ = ctake(8 * F * 2 * 4), o_ch = ctake(4 * F);
+      const size_t o_fno = ctake(4 * F), o_ones = ctake(4 * F), o_ax = ctake(8 * F * lvl_stride);
+      const size_t o_ay = ctake(8 * F * lvl_stride), o_fld = ctake(4 * F * nf);
+      const size_t o_fa = ctake(4 * F * (size_t)chunk * kFiltThreads), o_fb = ctake(4 * F * (size_t)chunk * kFiltThreads);
+      VHSD_HIP(c->c_work.ensure(co));
+      unsigned char* cm = c->c_work.p;
+      cw.plane = chroma->plane;
+      cw.plane_len = span_length;
+      cw.win = reinterpret_cast<float*>(cm + o_win);
+      cw.probe = reinterpret_cast<double*>(cm + o_probe);
+      cw.flip = reinterpret_cast<int*>(cm + o_flip);
+      cw.summ = reinterpret_cast<double*>(cm + o_summ);
+      cw.chosen = reinterpret_cast<int*>(cm + o_ch);
+      cw.field_no = reinterpret_cast<int*>(cm + o_fno);
+      cw.ones = reinterpret_cast<int*>(cm + o_ones);
+      cw.an_x = reinterpret_cast<double*>(cm + o_ax);
+      cw.an_y0 = reinterpret_cast<double*>(cm + o_ay);
+      cw.lvl_stride = lvl_stride;
+      cw.field = reinterpret_cast<float*>(cm + o_fld);
+      cw.fa = reinterpret_cast<float*>(cm + o_fa);
+      cw.fb = reinterpret_cast<float*>(cm + o_fb);
+      cw.filt_chunk = chunk;
+      cw.state = reinterpret_cast<vhsd::chroma::ChromaState*>(c->c_state.p);
+      cw.out = chroma->out;
+    }
     vhsd::HsyncParams hp{p->threshold, fs_mhz, sy->hsync_pulse_us, p->ire0, p->hz_ire, cfg.is_pal,
                          p->use_right_hsync};
     VHSD_HIP(c->h_work.ensure(vhsd_hsync_work_bytes(max_fields, line_stride)));
@@ -1822,6 +2014,7 @@ int32_t vhsd_decode_vhs_fields_from(vhsdecode_hip_demod_context* c, const float*
     auto enqueue = [&](hipStream_t q) -> int32_t {
       // the zero `filled` row (BuildUpstreamLineLocations fills no line); written by a fill kernel
       VHSD_HIP(vhsd_launch_fill_u32(reinterpret_cast<unsigned*>(m + o_zero), (line_stride + 3) / 4, 0u, q));
+      if (chroma) VHSD_HIP(vhsd_launch_chroma_state(cw.state, 0, -1, cw.ones, max_fields, q));
       {
         ProfScope ps(c, q, VHSDECODE_HIP_KERNEL_PULSES, span_length);
         VHSD_HIP(vhsd_launch_pulses(sync_ref, span_length, c->bias, p->threshold, cfg.pulse_min, cfg.pulse_max,
@@ -1843,7 +2036,19 @@ int32_t vhsd_decode_vhs_fields_from(vhsdecode_hip_demod_context* c, const float*
                                      t.filled_out, max_fields, line_stride, c->bias, c->h_work.p, t.threshold, t.dc,
                                      q));
         }
-        VHSD_HIP(vhsd_launch_vhs_render(&cfg, &wr, &t, max_fields, q));
+        if (chroma) {
+          // the burst phase analysis on the hsync-refined lines, whose burst-refined
+          // locations are the render locations (luma and chroma)
+          ProfScope ps(c, q, VHSDECODE_HIP_KERNEL_CHROMA_FIELD, max_fields);
+          VHSD_HIP(vhsd_launch_tbc_levels(t.refined_base, line_stride, out_lines, cw.ones, cw.lvl_stride,
+                                          p->nominal_line_length, line_decay, cw.an_x, cw.an_y0, &ctl->decoded,
+                                          max_fields, q));
+          VHSD_HIP(vhsd_launch_chroma_analysis(&c->ccfg, &cw, t.refined_base, t.render, line_stride, line_stride,
+                                               t.origins, smooth ? c->f_pow.p : nullptr, p->nominal_line_length,
+                                               smooth ? 1 : 0, c->lut.p, t.fsc_shift, max_fields, &ctl->decoded, q));
+        } else {
+          VHSD_HIP(vhsd_launch_vhs_render(&cfg, &wr, &t, max_fields, q));
+        }
         VHSD_HIP(vhsd_launch_vhs_export(&cfg, &wr, &t, max_fields, q));
         {
           ProfScope ps(c, q, VHSDECODE_HIP_KERNEL_TBC, (long long)max_fields * out_lines * p->output_line_length);
@@ -1852,6 +2057,13 @@ int32_t vhsd_decode_vhs_fields_from(vhsdecode_hip_demod_context* c, const float*
                                           line_decay, smooth ? c->f_pow.p : nullptr, c->f_x.p, c->f_y0.p,
                                           p->output_line_length, smooth ? 1 : 0, p->ire0, out_scale,
                                           1.0 / p->hz_ire, conv_off, c->lut.p, out, t.out_index, &ctl->decoded, q));
+        }
+        if (chroma) {
+          ProfScope ps(c, q, VHSDECODE_HIP_KERNEL_CHROMA_FIELD, max_fields);
+          VHSD_HIP(vhsd_launch_chroma_fields(&c->ccfg, &cw, t.render, line_stride, t.origins, t.first_lines, c->f_x.p,
+                                             c->f_y0.p, smooth ? c->f_pow.p : nullptr, p->nominal_line_length,
+                                             smooth ? 1 : 0, c->lut.p, w.records, ctl, t.out_index, max_fields,
+                                             &ctl->decoded, q));
         }
       }
       VHSD_HIP(vhsd_launch_vhs_results(ctl, field_count, next_begin, q));
@@ -1883,6 +2095,12 @@ int32_t vhsd_decode_vhs_fields_from(vhsdecode_hip_demod_context* c, const float*
       put(&t, sizeof(t));
       put(&hp, sizeof(hp));
       put(p, sizeof(*p));
+      const int chroma_on = chroma ? 1 : 0;
+      put(&chroma_on, sizeof(chroma_on));
+      if (chroma) {
+        put(&cw, sizeof(cw));
+        put(&c->ccfg, sizeof(c->ccfg));
+      }
       if (!c->vs_exec || key != c->vs_key) {
         if (c->vs_exec) (void)hipGraphExecDestroy(c->vs_exec);
         if (c->vs_graph) (void)hipGraphDestroy(c->vs_graph);

@@ -55,10 +55,12 @@ REQUIRED_EXPORTS = (
     "vhsdecode_hip_find_pulses_noalloc",
     "vhsdecode_hip_decode_vhs_fields_device",
     "vhsdecode_hip_vhs_field_lines_device",
+    "vhsdecode_hip_set_vhs_chroma",
+    "vhsdecode_hip_decode_vhs_fields_chroma_device",
 )
 
 KERNEL_NAMES = ("block_demod", "spike_repair", "chroma", "pulses", "hsync", "tbc", "sub_deemphasis", "block_video",
-                "locate", "dropouts")
+                "locate", "dropouts", "chroma_field")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -155,6 +157,15 @@ class FieldParamsV1(ctypes.Structure):
                 ("field_lines", ctypes.c_int32), ("nominal_line_length", ctypes.c_double),
                 ("wow_level_smoothing", ctypes.c_double), ("output_zero", ctypes.c_int32),
                 ("vsync_ire", ctypes.c_double), ("output_scale", ctypes.c_double)]
+
+
+class ChromaParamsV1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("is_ntsc", ctypes.c_int32), ("fsc_mhz", ctypes.c_double),
+                ("color_under_hz", ctypes.c_double), ("output_rate_hz", ctypes.c_double),
+                ("field_lines", ctypes.c_int32), ("output_line_length", ctypes.c_int32),
+                ("burst_start", ctypes.c_int32), ("burst_end", ctypes.c_int32), ("burst_abs_ref", ctypes.c_double),
+                ("rotation", ctypes.c_int32 * 2), ("sections", ctypes.c_int32), ("input_line_length", ctypes.c_int32),
+                ("final_sos", (ctypes.c_double * 6) * 8)]
 
 
 class VhsSyncV1(ctypes.Structure):
@@ -328,6 +339,12 @@ def load():
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
         ctypes.c_int64, ctypes.POINTER(VhsSyncV1), ctypes.POINTER(FieldParamsV1), ctypes.c_int32, ctypes.c_int32,
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.vhsdecode_hip_set_vhs_chroma.argtypes = [ctypes.c_void_p, ctypes.POINTER(ChromaParamsV1)]
+    L.vhsdecode_hip_decode_vhs_fields_chroma_device.argtypes = [
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+        ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(VhsSyncV1), ctypes.POINTER(FieldParamsV1), ctypes.c_int32,
+        ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.c_void_p, ctypes.c_void_p]
     L.vhsdecode_hip_vhs_field_lines_device.argtypes = [ctypes.c_void_p, ctypes.c_int32,
                                                        ctypes.POINTER(ctypes.c_void_p),
                                                        ctypes.POINTER(ctypes.c_int32)]

@@ -344,6 +344,50 @@ class HipBlockDemodulator:
             ctypes.byref(sync), ctypes.byref(p), max_fields, rounds, v(out_ptr), v(fields_ptr or None), v(count_ptr),
             v(next_begin_ptr or None), v(stream)))
 
+    def set_vhs_chroma(self, params, input_line_length: int):
+        """Colour-under chroma for the vhs field sequence (`params` a chroma.ChromaParams, or
+        None to clear): heterodyne and burst-carrier tables built on the device."""
+        if params is None:
+            _lib.check(self._L.vhsdecode_hip_set_vhs_chroma(self._h, None))
+            return
+        sos = np.asarray(params.final_sos, dtype=np.float64)
+        if sos.shape[0] > 8 or len(params.rotation) != 2:
+            raise ValueError("chroma needs at most 8 final band-pass sections and a two-entry rotation")
+        cp = _lib.ChromaParamsV1()
+        cp.struct_size = ctypes.sizeof(_lib.ChromaParamsV1)
+        cp.is_ntsc = 1 if params.is_ntsc else 0
+        cp.fsc_mhz = params.fsc_mhz
+        cp.color_under_hz = params.color_under_hz
+        cp.output_rate_hz = params.frame_rate_hz
+        cp.field_lines = params.lines
+        cp.output_line_length = params.line_len
+        cp.burst_start = params.burst_start
+        cp.burst_end = params.burst_end
+        cp.burst_abs_ref = params.burst_abs_ref
+        cp.rotation[0], cp.rotation[1] = params.rotation
+        cp.sections = sos.shape[0]
+        cp.input_line_length = int(input_line_length)
+        for k in range(sos.shape[0]):
+            for q in range(6):
+                cp.final_sos[k][q] = float(sos[k, q])
+        _lib.check(self._L.vhsdecode_hip_set_vhs_chroma(self._h, ctypes.byref(cp)))
+
+    def decode_vhs_fields_chroma_device(self, sync_ref_ptr: int, video_ptr: int, chroma_ptr: int, span_length: int,
+                                        first_begin: int, sync, geom, max_fields: int, out_ptr: int,
+                                        out_chroma_ptr: int, count_ptr: int, fields_ptr: int = 0,
+                                        next_begin_ptr: int = 0, reset_state: bool = True, state_shift: int = 0,
+                                        rounds: int = 4, field_number_base: int = 0, stream: int = 0):
+        """decode_vhs_fields_device with the colour-under chroma of every decoded field
+        (uint16 at out_chroma_ptr, slot by slot as the luma); the luma renders at the
+        burst-refined line locations."""
+        p = self.field_params(geom)
+        v = ctypes.c_void_p
+        _lib.check(self._L.vhsdecode_hip_decode_vhs_fields_chroma_device(
+            self._h, v(sync_ref_ptr), v(video_ptr), v(chroma_ptr), span_length, first_begin,
+            1 if reset_state else 0, state_shift, ctypes.byref(sync), ctypes.byref(p), max_fields, rounds,
+            v(out_ptr), v(out_chroma_ptr), field_number_base, v(fields_ptr or None), v(count_ptr),
+            v(next_begin_ptr or None), v(stream)))
+
     def vhs_field_lines_device(self, which: int):
         """(device pointer, stride) of a line table of the last vhs field decode, by record:
         0 line locations, 1 hsync-refined, 2 render locations (window coordinates), 3 render

@@ -64,6 +64,18 @@ def sync_geometry(fmt: FormatParameters, sample_rate_hz: float = 40e6) -> SyncGe
         conv=(ire0, hz_ire, int(sp["outputZero"]), vsync_ire, scale))
 
 
+def input_line_length(fmt: FormatParameters, sample_rate_hz: float = 40e6) -> int:
+    """max(1, round half even (SyncAnalyzer.NominalLineLength)): the chroma phase analysis's
+    input line length (TbcFieldDecodePipeline.AnalyzeChromaPhase, :2536-2559), with the
+    line period rounded to whole samples as SyncAnalyzer.FromParameters does
+    (Dsp/SyncAnalysis.cs:88-114)."""
+    fs_mhz = sample_rate_hz / 1e6
+    lp = float(fmt.sysparams["line_period"])
+    if fs_mhz > 0.0:
+        lp = round(lp * fs_mhz) / fs_mhz
+    return max(1, int(round(lp * fs_mhz)))
+
+
 def vhs_sync_params(fmt: FormatParameters, sample_rate_hz: float = 40e6, block_length: int = 32768,
                     block_cut: int = 1024, stride: int = 30720):
     """vhsdecode_hip_vhs_sync_v1 of a tape format: SyncAnalyzer.FromParameters with the vhs
@@ -115,6 +127,7 @@ class SpanDecode:
     outputs: dict                # record index -> uint16 field (decoded records only)
     next_begin: int              # where the sequence continues (span coordinates)
     tables: Optional[list]       # per call: the four line tables [records][stride] (want_tables)
+    chroma: Optional[dict] = None  # record index -> uint16 chroma field (chroma decoders)
 
 
 class HipVhsFieldDecoder:
@@ -124,29 +137,46 @@ class HipVhsFieldDecoder:
     sequence state between calls (and, with `state_shift`, between spans)."""
 
     def __init__(self, demod, fmt: FormatParameters, sample_rate_hz: float = 40e6, block_length: int = 32768,
-                 block_cut: int = 1024, stride: int = 30720):
+                 block_cut: int = 1024, stride: int = 30720, chroma: bool = False):
         self.demod = demod
         self.fmt = fmt
         self.geom = sync_geometry(fmt, sample_rate_hz)
         self.sync = vhs_sync_params(fmt, sample_rate_hz, block_length, block_cut, stride)
         self.started = False
+        self.chroma = None
+        if chroma:
+            from .chroma import chroma_params
+            self.chroma = chroma_params(fmt)
+            demod.set_vhs_chroma(self.chroma, input_line_length(fmt, sample_rate_hz))
 
     def decode_async(self, span_video_ptr: int, span_vlp_ptr: int, span_len: int, out_ptr: int, max_fields: int,
                      count_ptr: int, fields_ptr: int = 0, next_begin_ptr: int = 0, first_begin: int = 0,
-                     reset: bool = True, state_shift: int = 0, rounds: int = 2, stream: int = 0):
-        self.demod.decode_vhs_fields_device(span_vlp_ptr, span_video_ptr, span_len, first_begin, self.sync,
-                                            self.geom, max_fields, out_ptr, count_ptr, fields_ptr, next_begin_ptr,
-                                            reset, state_shift, rounds, stream)
+                     reset: bool = True, state_shift: int = 0, rounds: int = 2, stream: int = 0,
+                     span_chroma_ptr: int = 0, out_chroma_ptr: int = 0, field_number_base: int = 0):
+        if self.chroma is not None:
+            self.demod.decode_vhs_fields_chroma_device(span_vlp_ptr, span_video_ptr, span_chroma_ptr, span_len,
+                                                       first_begin, self.sync, self.geom, max_fields, out_ptr,
+                                                       out_chroma_ptr, count_ptr, fields_ptr, next_begin_ptr, reset,
+                                                       state_shift, rounds, field_number_base, stream)
+        else:
+            self.demod.decode_vhs_fields_device(span_vlp_ptr, span_video_ptr, span_len, first_begin, self.sync,
+                                                self.geom, max_fields, out_ptr, count_ptr, fields_ptr,
+                                                next_begin_ptr, reset, state_shift, rounds, stream)
         self.started = True
 
     def decode_span(self, span_video_ptr: int, span_vlp_ptr: int, span_len: int, max_fields: int = 64,
                     first_begin: int = 0, reset: bool = True, state_shift: int = 0, rounds: int = 2,
-                    want_tables: bool = False) -> SpanDecode:
+                    want_tables: bool = False, span_chroma_ptr: int = 0, field_number_base: int = 0) -> SpanDecode:
         from . import _lib, hipmem
         g = self.geom
         per = g.field_lines * g.out_len
         size = ctypes.sizeof(_lib.VhsFieldV1)
         out = hipmem.DeviceBuffer(max_fields * per * 2)
+        with_chroma = self.chroma is not None
+        if with_chroma and not span_chroma_ptr:
+            raise ValueError("a chroma decoder needs the span's chroma plane")
+        out_c = hipmem.DeviceBuffer(max_fields * per * 2) if with_chroma else None
+        chroma_outputs = {}
         recs = hipmem.DeviceBuffer(max_fields * size)
         meta = hipmem.DeviceBuffer(16)
         records, outputs, tables = [], {}, ([] if want_tables else None)
@@ -154,7 +184,8 @@ class HipVhsFieldDecoder:
         stall = 0
         while True:
             self.decode_async(span_video_ptr, span_vlp_ptr, span_len, out.ptr, max_fields, meta.ptr, recs.ptr,
-                              meta.ptr + 8, begin, reset, state_shift, rounds)
+                              meta.ptr + 8, begin, reset, state_shift, rounds, span_chroma_ptr=span_chroma_ptr,
+                              out_chroma_ptr=out_c.ptr if with_chroma else 0, field_number_base=field_number_base)
             hipmem.synchronize()
             reset, state_shift = False, 0
             count = int(meta.download(np.int32, 1)[0])
@@ -163,9 +194,12 @@ class HipVhsFieldDecoder:
             raw = recs.download(np.uint8, n * size)
             got = [_lib.VhsFieldV1.from_buffer_copy(raw[k * size:(k + 1) * size].tobytes()) for k in range(n)]
             fields = out.download(np.uint16, max(count, 1) * per).reshape(-1, per)
+            cf = out_c.download(np.uint16, max(count, 1) * per).reshape(-1, per) if with_chroma else None
             for k in range(count):
                 if got[k].status == DECODED:
                     outputs[len(records) + k] = fields[k].copy()
+                    if with_chroma:
+                        chroma_outputs[len(records) + k] = cf[k].copy()
             if want_tables:
                 tabs = []
                 for which in range(4):
@@ -181,7 +215,7 @@ class HipVhsFieldDecoder:
             if stop.status == CAPACITY:
                 raise RuntimeError("a read window holds more sync edges or pulses than the device lists")
             if stop.status == NOT_READ and stop.window_start + stop.window_length > span_len:
-                return SpanDecode(records, outputs, next_begin, tables)
+                return SpanDecode(records, outputs, next_begin, tables, chroma_outputs if with_chroma else None)
             stall = stall + 1 if count == 0 and next_begin == begin else 0
             if stall > 2:
                 raise RuntimeError("the field sequence made no progress")

@@ -119,10 +119,16 @@ def _frame_plan(is_pal: bool) -> List[str]:
 def generate(fmt: FormatParameters, fields: int, sample_rate_hz: float = 40e6, snr_db: float = 30.0,
              seed: int = 1, amplitude: float = 8000.0, jitter_ns: float = 0.0, lead_in_lines: int = 20,
              lead_out_lines: int = 20, chroma_under_level: float = 0.12,
-             audio_carriers: tuple = ()) -> SyntheticTape:
+             audio_carriers: tuple = (), chroma_under: str = "carrier") -> SyntheticTape:
     """`audio_carriers`: (carrier Hz, deviation Hz, tone Hz, level relative to the video
     carrier) per FM audio carrier, e.g. the LaserDisc analog sound carriers near 2.3 and
-    2.8 MHz, added with their phase accumulated in FP64."""
+    2.8 MHz, added with their phase accumulated in FP64.
+
+    `chroma_under`: "carrier" adds a constant carrier at the format's colour-under
+    frequency; "modulated" records the lines' chroma (burst and bar colours) under the
+    luma FM as a colour-under recorder does: each video line's chroma moved to the
+    colour-under carrier with the track's phase rotation (the format's chroma_rotation
+    step per line, alternating tracks field by field)."""
     p = dict(fmt.sysparams)
     is_pal = fmt.parent_system == "PAL"
     line_us = float(p["line_period"])
@@ -143,21 +149,44 @@ def generate(fmt: FormatParameters, fields: int, sample_rate_hz: float = 40e6, s
     ire = np.zeros(n, dtype=np.float64)
     sample_idx = np.arange(n, dtype=np.float64)
     first_kind = (frame_lines - lead_in_lines) % frame_lines
+    cu = fmt.rfparams.get("color_under_carrier")
+    modulated = chroma_under == "modulated" and cu and chroma_under_level > 0
+    under = np.zeros(n, dtype=np.float64) if modulated else None
+    rotation = [int(v) for v in (fmt.rfparams.get("chroma_rotation") or [0, 0])]
     for line in range(total_lines):
         a = int(np.ceil(starts[line]))
         b = min(int(np.ceil(starts[line + 1])), n)
         if a >= b:
             continue
         x_us = (sample_idx[a:b] - starts[line]) * 1e6 / sample_rate_hz
-        kind = kinds[(first_kind + line) % frame_lines]
+        frame_line = (first_kind + line) % frame_lines
+        kind = kinds[frame_line]
         ire[a:b] = _line_ire(kind, x_us, line_us, p, line, is_pal)
+        if modulated and kind.startswith("video"):
+            # the line's chroma (amplitude in IRE, phase) on the colour-under carrier
+            amp = np.zeros(b - a)
+            ph = np.zeros(b - a)
+            b0, b1 = p["colorBurstUS"]
+            amp[(x_us >= b0) & (x_us < b1)] = 20.0
+            if kind == "video":
+                a0, a1 = p["activeVideoUS"]
+                act = (x_us >= a0) & (x_us < a1)
+                idx = np.minimum(((x_us[act] - a0) / (a1 - a0) * 8).astype(int), 7)
+                bars = np.array([0.0, 33.0, 47.0, 44.0, 44.0, 47.0, 33.0, 0.0])
+                amp[act] = 0.4 * bars[idx]
+                ph[act] = 0.7 * idx
+            track = rotation[0 if frame_line < frame_lines // 2 else 1]
+            rot = (track * frame_line) % 4
+            under[a:b] = (amp / 20.0) * np.sin(2.0 * np.pi * float(cu) * sample_idx[a:b] / sample_rate_hz
+                                               + rot * (np.pi / 2.0) - ph)
     ire0 = float(fmt.rfparams.get("ire0", p["ire0"])) if "ire0" in fmt.rfparams else float(p["ire0"])
     hz_ire = float(p["hz_ire"])
     freq = ire0 + hz_ire * ire
     phase = 2.0 * np.pi * np.cumsum(freq) / sample_rate_hz
     rf = amplitude * np.cos(phase)
-    cu = fmt.rfparams.get("color_under_carrier")
-    if cu and chroma_under_level > 0:
+    if modulated:
+        rf += chroma_under_level * amplitude * under
+    elif cu and chroma_under_level > 0:
         rf += chroma_under_level * amplitude * np.sin(2.0 * np.pi * float(cu) * sample_idx / sample_rate_hz)
     for fc, dev, tone, level in audio_carriers:
         inst = fc + dev * np.sin(2.0 * np.pi * tone * sample_idx / sample_rate_hz)
