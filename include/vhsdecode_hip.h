@@ -722,6 +722,10 @@ typedef struct vhsdecode_hip_run_config_v1 {
   const char* system_name;
   double colour_burst_start_us;
   double colour_burst_end_us;
+  /* Colour-under chroma (vhsdecode_hip_set_vhs_chroma's parameters): the fields' chroma to
+   * <base>_chroma.tbc beside the luma, which then renders at the burst-refined line
+   * locations (the reference's vhs default); NULL writes no chroma file (its --skip-chroma). */
+  const vhsdecode_hip_chroma_params_v1* chroma;
 } vhsdecode_hip_run_config_v1;
 
 typedef struct vhsdecode_hip_run_result_v1 {

@@ -142,3 +142,41 @@ def test_decoded_burst_level_and_frequency(run):
         assert 0.5 * p.burst_abs_ref < rms < 1.5 * p.burst_abs_ref, rms
         spec = np.abs(np.fft.rfft(burst, axis=1)).mean(axis=0)
         assert abs((np.argmax(spec[1:]) + 1) / burst.shape[1] - 0.25) < 0.05
+
+
+@pytest.mark.parametrize("system", ["NTSC", "PAL"])
+def test_whole_run_writes_the_span_decode_chroma(system, tmp_path):
+    """vhsdecode_hip_run with chroma: spans of about four fields, the field sequence and the
+    chroma track-phase state carried from span to span; <base>.tbc and <base>_chroma.tbc
+    equal one decode of the whole tape as a single span, field for field."""
+    from vhsdecode_hip.run import decode_run
+    fmt = formats.get_format(system, "VHS", "sp")
+    tape = synth.generate(fmt, fields=9, seed=91, snr_db=30.0, jitter_ns=20.0, chroma_under="modulated")
+    fl = filters.build_tape_filters(fmt, 40e6, N)
+    lut = _lut()
+    dem = HipBlockDemodulator(fl, max_blocks=320)
+    dem.set_sinc_table(lut)
+    nb = dem.blocks_for(tape.rf.size)
+    n = nb * STRIDE
+    rf = hipmem.DeviceBuffer.from_array(tape.rf)
+    planes = hipmem.DeviceBuffer(4 * n * 4)
+    flags = hipmem.DeviceBuffer(nb * 4)
+    video, vlp, env, chroma = (planes.ptr + k * n * 4 for k in range(4))
+    dem.demod_span_device(rf.ptr, 0, nb, video, vlp, env, chroma, flags.ptr)
+    dec = field.HipVhsFieldDecoder(dem, fmt, chroma=True)
+    one = dec.decode_span(video, vlp, n, max_fields=16, span_chroma_ptr=chroma)
+    keys = sorted(one.outputs)
+    base = str(tmp_path / "run")
+    field_len = int(round(40e6 / (2 * float(fmt.sysparams["FPS"]))))
+    res = decode_run(tape.rf, fmt, fl, base, sinc_table=lut, dropouts=False, span_samples=4 * field_len,
+                     chroma=True)
+    print(f"\n[chroma run] {system} fields {res.fields_written} spans {res.spans}; one span {len(keys)}", flush=True)
+    assert res.spans >= 2 and res.fields_written == len(keys) >= 6
+    g = dec.geom
+    per = g.field_lines * g.out_len
+    luma = np.fromfile(base + ".tbc", dtype="<u2").reshape(-1, per)
+    chrom = np.fromfile(base + "_chroma.tbc", dtype="<u2").reshape(-1, per)
+    assert luma.shape[0] == chrom.shape[0] == len(keys)
+    for i, k in enumerate(keys):
+        np.testing.assert_array_equal(luma[i], one.outputs[k])
+        np.testing.assert_array_equal(chrom[i], one.chroma[k])

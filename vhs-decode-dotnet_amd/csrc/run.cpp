@@ -5,7 +5,9 @@
 // host memory; each span runs the device chain (vhsdecode_hip_demod_span_device,
 // vhsdecode_hip_decode_vhs_fields_device and, when asked,
 // vhsdecode_hip_detect_dropouts_device) on one stream, and the decoded fields are
-// appended in sequence order to <base>.tbc; <base>.tbc.json is written at the end.
+// appended in sequence order to <base>.tbc (and their colour-under chroma, when the config
+// carries chroma parameters, to <base>_chroma.tbc, through
+// vhsdecode_hip_decode_vhs_fields_chroma_device); <base>.tbc.json is written at the end.
 //
 // Span placement: span k starts at RF sample k * advance, a multiple of lcm(block length,
 // block stride) (491520 samples for 1024/1024 cuts).  Plane sample i of a span starting at
@@ -186,7 +188,9 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
 
   const std::string base(cfg.output_base_utf8);
   const std::string tbc_path = base + ".tbc", json_path = base + ".tbc.json";
-  if (!cfg.overwrite && (file_exists(tbc_path) || file_exists(json_path)))
+  const std::string chroma_path = base + "_chroma.tbc";
+  const bool with_chroma = cfg.chroma != nullptr;
+  if (!cfg.overwrite && (file_exists(tbc_path) || file_exists(json_path) || (with_chroma && file_exists(chroma_path))))
     return set_error(VHSDECODE_HIP_E_IO, "an output file exists and overwrite is off: " + tbc_path);
 
   // the cancel callback runs on this thread; the read wrapper latches a cancel it sees
@@ -212,15 +216,20 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
       const int32_t rs = vhsdecode_hip_set_sinc_table(ctx.c, fm.sinc_table, fm.sinc_count);
       if (rs) return rs;
     }
+    if (with_chroma) {
+      const int32_t rs = vhsdecode_hip_set_vhs_chroma(ctx.c, cfg.chroma);
+      if (rs) return rs;
+    }
   }
   OwnedStream st;
   RUN_HIP(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
   // device: RF, four planes, block flags, fields, records, count and next begin, dropout rows
-  DeviceBuf d_rf, d_planes, d_flags, d_fields, d_records, d_meta, d_drop;
+  DeviceBuf d_rf, d_planes, d_flags, d_fields, d_cfields, d_records, d_meta, d_drop;
   RUN_HIP(hipMalloc(&d_rf.p, sizeof(int16_t) * (size_t)read_max));
   RUN_HIP(hipMalloc(&d_planes.p, sizeof(float) * 4 * (size_t)plane_max));
   RUN_HIP(hipMalloc(&d_flags.p, sizeof(uint32_t) * (size_t)nb_max));
   RUN_HIP(hipMalloc(&d_fields.p, sizeof(uint16_t) * per_field * (size_t)max_fields));
+  if (with_chroma) RUN_HIP(hipMalloc(&d_cfields.p, sizeof(uint16_t) * per_field * (size_t)max_fields));
   const size_t rec_bytes = sizeof(vhsdecode_hip_vhs_field_v1) * (size_t)max_fields;
   RUN_HIP(hipMalloc(&d_records.p, rec_bytes));
   const size_t meta_bytes = 16;  // record count (int32, padded to 8), next begin (int64)
@@ -228,10 +237,13 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
   // dropouts: row counts, then field line / start x / end x, [max_fields][cap] each
   const size_t drop_bytes = sizeof(int32_t) * (size_t)max_fields * (1 + 3 * (size_t)cap);
   if (cfg.dropouts) RUN_HIP(hipMalloc(&d_drop.p, drop_bytes));
-  PinnedBuf h_rf[2], h_fields[2], h_records, h_meta, h_drop;
+  PinnedBuf h_rf[2], h_fields[2], h_cfields[2], h_records, h_meta, h_drop;
   for (PinnedBuf& b : h_rf) RUN_HIP(hipHostMalloc(&b.p, sizeof(int16_t) * (size_t)read_max, hipHostMallocDefault));
   for (PinnedBuf& b : h_fields)
     RUN_HIP(hipHostMalloc(&b.p, sizeof(uint16_t) * per_field * (size_t)max_fields, hipHostMallocDefault));
+  if (with_chroma)
+    for (PinnedBuf& b : h_cfields)
+      RUN_HIP(hipHostMalloc(&b.p, sizeof(uint16_t) * per_field * (size_t)max_fields, hipHostMallocDefault));
   RUN_HIP(hipHostMalloc(&h_records.p, rec_bytes, hipHostMallocDefault));
   RUN_HIP(hipHostMalloc(&h_meta.p, meta_bytes, hipHostMallocDefault));
   if (cfg.dropouts) RUN_HIP(hipHostMalloc(&h_drop.p, drop_bytes, hipHostMallocDefault));
@@ -239,9 +251,13 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
   if (cfg.input_sample_format == VHSDECODE_HIP_INPUT_FLOAT32)
     for (std::vector<float>& v : staging) v.resize((size_t)read_max);
 
-  OwnedFile tbc;
+  OwnedFile tbc, ctbc;
   tbc.f = std::fopen(tbc_path.c_str(), "wb");
   if (!tbc.f) return set_error(VHSDECODE_HIP_E_IO, "cannot create " + tbc_path);
+  if (with_chroma) {
+    ctbc.f = std::fopen(chroma_path.c_str(), "wb");
+    if (!ctbc.f) return set_error(VHSDECODE_HIP_E_IO, "cannot create " + chroma_path);
+  }
 
   const int32_t* h_count = static_cast<const int32_t*>(h_meta.p);
   const int64_t* h_next = reinterpret_cast<const int64_t*>(static_cast<unsigned char*>(h_meta.p) + 8);
@@ -311,8 +327,13 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
   std::future<bool> write_job;
   auto write_fields = [&](int slot, std::vector<int32_t> which) -> bool {
     const uint16_t* base_p = static_cast<const uint16_t*>(h_fields[slot].p);
-    for (const int32_t f : which)
+    const uint16_t* cbase_p = static_cast<const uint16_t*>(h_cfields[slot].p);
+    for (const int32_t f : which) {
       if (std::fwrite(base_p + (size_t)f * per_field, sizeof(uint16_t), per_field, tbc.f) != per_field) return false;
+      if (with_chroma &&
+          std::fwrite(cbase_p + (size_t)f * per_field, sizeof(uint16_t), per_field, ctbc.f) != per_field)
+        return false;
+    }
     return true;
   };
   auto finish_writes = [&]() -> bool { return !write_job.valid() || write_job.get(); };
@@ -351,11 +372,17 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
     prev_pos = pos;
     // calls on this span until the sequence's next read window leaves it
     for (int stall = 0;;) {
-      rc = vhsdecode_hip_decode_vhs_fields_device(ctx.c, sp.video_lowpass, sp.video, plane, begin, fresh ? 1 : 0,
-                                                  shift, &sy, &fp, max_fields, kRounds,
-                                                  static_cast<uint16_t*>(d_fields.p),
-                                                  static_cast<vhsdecode_hip_vhs_field_v1*>(d_records.p), d_count,
-                                                  d_next, st.s);
+      if (with_chroma)
+        rc = vhsdecode_hip_decode_vhs_fields_chroma_device(
+            ctx.c, sp.video_lowpass, sp.video, sp.chroma, plane, begin, fresh ? 1 : 0, shift, &sy, &fp, max_fields,
+            kRounds, static_cast<uint16_t*>(d_fields.p), static_cast<uint16_t*>(d_cfields.p), 0,
+            static_cast<vhsdecode_hip_vhs_field_v1*>(d_records.p), d_count, d_next, st.s);
+      else
+        rc = vhsdecode_hip_decode_vhs_fields_device(ctx.c, sp.video_lowpass, sp.video, plane, begin, fresh ? 1 : 0,
+                                                    shift, &sy, &fp, max_fields, kRounds,
+                                                    static_cast<uint16_t*>(d_fields.p),
+                                                    static_cast<vhsdecode_hip_vhs_field_v1*>(d_records.p), d_count,
+                                                    d_next, st.s);
       if (rc) return rc;
       fresh = false;
       shift = 0;
@@ -378,9 +405,13 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
       const int32_t count = std::min(*h_count, max_fields);
       // staging slot `call & 1` was last written two calls ago; that job has finished
       const int slot = call & 1;
-      if (count > 0)
+      if (count > 0) {
         RUN_HIP(hipMemcpyAsync(h_fields[slot].p, d_fields.p, sizeof(uint16_t) * per_field * (size_t)count,
                                hipMemcpyDeviceToHost, st.s));
+        if (with_chroma)
+          RUN_HIP(hipMemcpyAsync(h_cfields[slot].p, d_cfields.p, sizeof(uint16_t) * per_field * (size_t)count,
+                                 hipMemcpyDeviceToHost, st.s));
+      }
       RUN_HIP(hipStreamSynchronize(st.s));
       std::vector<int32_t> to_write;
       for (int32_t f = 0; f < count && !done; f++) {
@@ -428,6 +459,7 @@ int32_t run_impl(const vhsdecode_hip_run_config_v1& cfg, vhsdecode_hip_run_resul
   if (poll_cancel()) return set_error(VHSDECODE_HIP_E_CANCELLED, "the run was cancelled");
   const auto t1 = std::chrono::steady_clock::now();
   if (std::fflush(tbc.f) != 0) return set_error(VHSDECODE_HIP_E_IO, "cannot write " + tbc_path);
+  if (with_chroma && std::fflush(ctbc.f) != 0) return set_error(VHSDECODE_HIP_E_IO, "cannot write " + chroma_path);
   const double out_rate = sy.output_rate_mhz * 1e6;
   if (!write_json(json_path, fp, out_rate, cfg, written)) return set_error(VHSDECODE_HIP_E_IO, "cannot write " + json_path);
   res.fields_written = (uint32_t)written.size();

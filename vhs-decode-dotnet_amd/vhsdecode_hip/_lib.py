@@ -224,7 +224,8 @@ class RunConfigV1(ctypes.Structure):
                 ("output_base_utf8", ctypes.c_char_p), ("overwrite", ctypes.c_int32),
                 ("read_callback", READ_CALLBACK), ("cancel_callback", CANCEL_CALLBACK),
                 ("user_data", ctypes.c_void_p), ("system_name", ctypes.c_char_p),
-                ("colour_burst_start_us", ctypes.c_double), ("colour_burst_end_us", ctypes.c_double)]
+                ("colour_burst_start_us", ctypes.c_double), ("colour_burst_end_us", ctypes.c_double),
+                ("chroma", ctypes.POINTER(ChromaParamsV1))]
 
 
 class RunResultV1(ctypes.Structure):

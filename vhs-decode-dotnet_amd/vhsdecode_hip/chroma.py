@@ -81,3 +81,31 @@ def chroma_params(fmt: FormatParameters) -> ChromaParams:
         output_rate_mhz=float(sp["fsc_mhz"]) * 4.0, frame_rate_hz=out_per_us * 1_000_000.0, line_len=line_len, lines=lines, burst_start=start,
         burst_end=end, burst_abs_ref=float(sp["burst_abs_ref"]), rotation=rot, final_sos=final_sos(fmt),
         sync_tip_len=int(math.floor(float(sp["hsyncPulseUS"]) * out_per_us)))
+
+
+def to_struct(params: ChromaParams, input_line_length: int):
+    """vhsdecode_hip_chroma_params_v1 of `params`."""
+    import ctypes
+
+    from . import _lib
+    sos = np.asarray(params.final_sos, dtype=np.float64)
+    if sos.shape[0] > 8 or len(params.rotation) != 2:
+        raise ValueError("chroma needs at most 8 final band-pass sections and a two-entry rotation")
+    cp = _lib.ChromaParamsV1()
+    cp.struct_size = ctypes.sizeof(_lib.ChromaParamsV1)
+    cp.is_ntsc = 1 if params.is_ntsc else 0
+    cp.fsc_mhz = params.fsc_mhz
+    cp.color_under_hz = params.color_under_hz
+    cp.output_rate_hz = params.frame_rate_hz
+    cp.field_lines = params.lines
+    cp.output_line_length = params.line_len
+    cp.burst_start = params.burst_start
+    cp.burst_end = params.burst_end
+    cp.burst_abs_ref = params.burst_abs_ref
+    cp.rotation[0], cp.rotation[1] = params.rotation
+    cp.sections = sos.shape[0]
+    cp.input_line_length = int(input_line_length)
+    for k in range(sos.shape[0]):
+        for q in range(6):
+            cp.final_sos[k][q] = float(sos[k, q])
+    return cp

@@ -347,29 +347,11 @@ class HipBlockDemodulator:
     def set_vhs_chroma(self, params, input_line_length: int):
         """Colour-under chroma for the vhs field sequence (`params` a chroma.ChromaParams, or
         None to clear): heterodyne and burst-carrier tables built on the device."""
+        from .chroma import to_struct
         if params is None:
             _lib.check(self._L.vhsdecode_hip_set_vhs_chroma(self._h, None))
             return
-        sos = np.asarray(params.final_sos, dtype=np.float64)
-        if sos.shape[0] > 8 or len(params.rotation) != 2:
-            raise ValueError("chroma needs at most 8 final band-pass sections and a two-entry rotation")
-        cp = _lib.ChromaParamsV1()
-        cp.struct_size = ctypes.sizeof(_lib.ChromaParamsV1)
-        cp.is_ntsc = 1 if params.is_ntsc else 0
-        cp.fsc_mhz = params.fsc_mhz
-        cp.color_under_hz = params.color_under_hz
-        cp.output_rate_hz = params.frame_rate_hz
-        cp.field_lines = params.lines
-        cp.output_line_length = params.line_len
-        cp.burst_start = params.burst_start
-        cp.burst_end = params.burst_end
-        cp.burst_abs_ref = params.burst_abs_ref
-        cp.rotation[0], cp.rotation[1] = params.rotation
-        cp.sections = sos.shape[0]
-        cp.input_line_length = int(input_line_length)
-        for k in range(sos.shape[0]):
-            for q in range(6):
-                cp.final_sos[k][q] = float(sos[k, q])
+        cp = to_struct(params, input_line_length)
         _lib.check(self._L.vhsdecode_hip_set_vhs_chroma(self._h, ctypes.byref(cp)))
 
     def decode_vhs_fields_chroma_device(self, sync_ref_ptr: int, video_ptr: int, chroma_ptr: int, span_length: int,

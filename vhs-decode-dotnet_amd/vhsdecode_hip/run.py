@@ -42,11 +42,13 @@ def decode_run(rf: np.ndarray, fmt: FormatParameters, filters: TapeFilterSet, ou
                sinc_table: Optional[np.ndarray] = None, dropouts: bool = True, dropout_capacity: int = 1024,
                span_samples: int = 0, maximum_output_fields: int = 0, overwrite: bool = True,
                cancel: Optional[Callable[[], bool]] = None, block_cut: int = 1024, block_cut_end: int = 1024,
-               device: int = 0, deliver: Optional[Callable[[int, int], int]] = None) -> RunResult:
+               device: int = 0, deliver: Optional[Callable[[int, int], int]] = None,
+               chroma: bool = False) -> RunResult:
     """Decodes the whole of `rf` (int16, or float32 samples the library rounds to int16)
     read through the library's callback; returns the run's result record.  `deliver`
     (offset, count) -> samples, when given, bounds what each read returns (a short read
-    ends the input, or reports a cancel)."""
+    ends the input, or reports a cancel).  `chroma`: also write the colour-under chroma to
+    <output_base>_chroma.tbc (the luma then renders at the burst-refined locations)."""
     L = _lib.load()
     src = np.ascontiguousarray(rf)
     if src.dtype == np.int16:
@@ -88,6 +90,11 @@ def decode_run(rf: np.ndarray, fmt: FormatParameters, filters: TapeFilterSet, ou
                            ctypes.pointer(dp) if dp is not None else None, src.size, fmt_code,
                            maximum_output_fields, span_samples, output_base.encode("utf-8"), 1 if overwrite else 0,
                            read_cb, cancel_cb, None, _system_name(fmt), *_burst_us(fmt))
+    cp = None
+    if chroma:
+        from .chroma import chroma_params, to_struct
+        cp = to_struct(chroma_params(fmt), field.input_line_length(fmt, filters.sample_rate_hz))
+        cfg.chroma = ctypes.pointer(cp)
     res = _lib.RunResultV1()
     res.struct_size = ctypes.sizeof(_lib.RunResultV1)
     _lib.check(L.vhsdecode_hip_run(ctypes.byref(cfg), ctypes.byref(res)))
