@@ -42,23 +42,29 @@ DROPOUT_ROWS = 1024      # .tbc.json dropout rows kept per field
 SEED = 20240501
 
 
-def build_workload(fields: int):
+def build_workload(fields: int, chroma: bool = True):
     from vhsdecode_hip import filters, formats, synth
     fmt = formats.get_format("NTSC", "VHS", "sp")
-    tape = synth.generate(fmt, fields=fields, seed=SEED, snr_db=28.0, jitter_ns=20.0)
+    # the colour-under chroma recorded with the track-phase rotation when the step decodes it
+    tape = synth.generate(fmt, fields=fields, seed=SEED, snr_db=28.0, jitter_ns=20.0,
+                          chroma_under="modulated" if chroma else "carrier")
     fl = filters.build_tape_filters(fmt, 40e6, N)
     nb = (tape.rf.size - N) // STRIDE + 1
     return fmt, tape, fl, nb
 
 
-def cpu_baseline(fmt, tape, fl, nb, lut, seconds: float, threads: int):
+def cpu_baseline(fmt, tape, fl, nb, lut, seconds: float, threads: int, chroma: bool = True):
     """FP64 oracle on the same tape: block demod (threads over blocks), the vhs field
     sequence (read windows, levels, pulses, parity, line 0, line locations; serial, as the
-    reference's sequence is), then per field hsync refinement, TBC and the envelope dropout
+    reference's sequence is), then per field hsync refinement, with chroma the burst-phase
+    analysis and burst-refined lines (the track-phase start carried from field to field,
+    so this part runs in field order), TBC, the chroma field and the envelope dropout
     detection (threads over fields); repeated for ~`seconds`, then one pass on one thread.
-    Also returns the last pass's records and uint16 fields (the FP64 chain the device's
-    output is compared with)."""
+    Also returns the last pass's records and uint16 luma and chroma fields (the FP64 chain
+    the device's output is compared with)."""
+    from oracle import chroma as OC
     from oracle import oracle as O
+    from vhsdecode_hip import chroma as HC
     from vhsdecode_hip import field
     g = field.sync_geometry(fmt)
     dp = field.dropout_params(fmt)
@@ -66,28 +72,58 @@ def cpu_baseline(fmt, tape, fl, nb, lut, seconds: float, threads: int):
     rf = tape.rf
     fsc_shift = 0.0 if g.is_pal else ((117.25 / 360.0) / ((float(fmt.sysparams["outfreq"]) * 1e6) / 4e6)) * g.fs_mhz
 
+    cp = HC.chroma_params(fmt) if chroma else None
+    ill = field.input_line_length(fmt)
+
     def one_block(b):
         r = O.demod_block(rf[b * STRIDE:b * STRIDE + N].astype(np.float64), fl)
-        return b, r.video[CUT:CUT + STRIDE], r.video_lowpass[CUT:CUT + STRIDE], r.envelope[CUT:CUT + STRIDE]
+        return (b, r.video[CUT:CUT + STRIDE], r.video_lowpass[CUT:CUT + STRIDE], r.envelope[CUT:CUT + STRIDE],
+                r.chroma[CUT:CUT + STRIDE])
 
     def one_pass(pool):
         video = np.empty(nb * STRIDE)
         vlp = np.empty(nb * STRIDE)
         env = np.empty(nb * STRIDE, dtype=np.float32)
-        for b, v, l, e in pool.map(one_block, range(nb)):
+        chro = np.empty(nb * STRIDE)
+        for b, v, l, e, c in pool.map(one_block, range(nb)):
             video[b * STRIDE:(b + 1) * STRIDE] = v
             vlp[b * STRIDE:(b + 1) * STRIDE] = l
             env[b * STRIDE:(b + 1) * STRIDE] = e
+            chro[b * STRIDE:(b + 1) * STRIDE] = c
         recs, locs, shifted = O.vhs_field_sequence(vlp, cfg, 0, 64, want_shifted=True)
         nl = locs.shape[1] if len(recs) else 0
 
-        def one_field(k):
+        def refined(k):
             o = recs[k]
             if o.status != 0:
                 return None
             ref, _ = O.refine_hsync(shifted[k, :o.window_length], locs[k], np.zeros(nl, np.uint8), o.threshold,
                                     g.fs_mhz, g.hsync / g.fs_mhz, g.conv[0], g.conv[1], g.is_pal, True)
-            ref = ref - fsc_shift
+            return ref
+
+        refs = list(pool.map(refined, range(len(recs))))
+        # the chroma phase analysis in field order (the track-phase start is carried)
+        chroma_fields, renders = {}, {}
+        rot, number = 0, 0
+        for k, o in enumerate(recs):
+            if refs[k] is None:
+                continue
+            if cp is None:
+                renders[k] = refs[k] - fsc_shift
+                continue
+            w0, w1 = o.window_start, o.window_start + o.window_length
+            fc = OC.field_chain(chro[w0:w1], refs[k], cp, bool(o.is_first_field), number, rot, ill, lut,
+                                g.nominal, g.smoothing, fsc_shift)
+            rot = fc.phase.next_index
+            number += 1
+            renders[k] = fc.render
+            chroma_fields[k] = fc.chroma
+
+        def one_field(k):
+            o = recs[k]
+            if o.status != 0:
+                return None
+            ref = renders[k]
             first = 1 if not g.is_pal else (3 if o.is_first_field else 4)
             w0, w1 = o.window_start, o.window_start + o.window_length
             out = O.tbc_resample(video[w0:w1], ref, first, g.field_lines, g.out_len, lut, nominal=g.nominal,
@@ -96,13 +132,13 @@ def cpu_baseline(fmt, tape, fl, nb, lut, seconds: float, threads: int):
             O.tape_dropouts(env, ref[first:] + w0, lines, (w0, min(w1, env.size)), out_len=g.out_len)
             return out
 
-        return recs, list(pool.map(one_field, range(len(recs))))
+        return recs, list(pool.map(one_field, range(len(recs)))), chroma_fields
 
     reps = 0
     with ThreadPoolExecutor(max_workers=threads) as pool:
         t0 = time.perf_counter()
         while True:
-            recs, outs = one_pass(pool)
+            recs, outs, couts = one_pass(pool)
             reps += 1
             el = time.perf_counter() - t0
             if el >= seconds:
@@ -112,7 +148,7 @@ def cpu_baseline(fmt, tape, fl, nb, lut, seconds: float, threads: int):
         t1 = time.perf_counter()
         one_pass(pool)
         el1 = time.perf_counter() - t1
-    return samples / el / 1e6, reps, el, ((nb - 1) * STRIDE + N) / el1 / 1e6, recs, outs
+    return samples / el / 1e6, reps, el, ((nb - 1) * STRIDE + N) / el1 / 1e6, recs, outs, couts
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_block_kernels_ntsc_vhs_10fields.md")
@@ -302,7 +338,7 @@ def batched_run(seed: int, segments: int, steps: int):
             "scope": "block demod chain (demod, repair, video, sub-deemphasis, chroma); no TBC (segments are not fields)"}
 
 
-def whole_run(fmt, tape, fl, lut, copies: int, dropouts: bool):
+def whole_run(fmt, tape, fl, lut, copies: int, dropouts: bool, chroma: bool = True):
     """The whole-run entry (vhsdecode_hip_run) over `copies` back-to-back copies of the
     tape held in host RAM: callback reads, PCIe upload, device decode of each span, .tbc and
     .tbc.json written to a temporary directory.  End to end, so never the headline value."""
@@ -313,11 +349,13 @@ def whole_run(fmt, tape, fl, lut, copies: int, dropouts: bool):
     tmp = tempfile.mkdtemp(prefix="vhsd_run_")
     try:
         base = os.path.join(tmp, "capture")
-        decode_run(rf[: 4 * (rf.size // copies)], fmt, fl, base, sinc_table=lut, dropouts=dropouts)  # warm-up
+        decode_run(rf[: 4 * (rf.size // copies)], fmt, fl, base, sinc_table=lut, dropouts=dropouts,
+                   chroma=chroma)  # warm-up
         t0 = time.perf_counter()
-        res = decode_run(rf, fmt, fl, base, sinc_table=lut, dropouts=dropouts)
+        res = decode_run(rf, fmt, fl, base, sinc_table=lut, dropouts=dropouts, chroma=chroma)
         wall = time.perf_counter() - t0
         tbc_bytes = os.path.getsize(base + ".tbc")
+        chroma_bytes = os.path.getsize(base + "_chroma.tbc") if chroma else 0
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     return {"samples": int(rf.size), "fields_written": res.fields_written, "spans": res.spans,
@@ -325,8 +363,9 @@ def whole_run(fmt, tape, fl, lut, copies: int, dropouts: bool):
             "msps": round(rf.size / wall / 1e6, 1),
             "loop_seconds": round(res.elapsed_seconds, 4),
             "loop_msps": round(rf.size / res.elapsed_seconds / 1e6, 1), "tbc_bytes": tbc_bytes,
+            "chroma_tbc_bytes": chroma_bytes,
             "path": "read callback from host RAM -> pinned staging -> PCIe -> device decode per span -> "
-                    ".tbc/.tbc.json on local disk; one span at a time"}
+                    ".tbc/_chroma.tbc/.tbc.json on local disk; one span at a time"}
 
 
 def laserdisc_run(seed: int, fields: int, steps: int, lut):
@@ -416,6 +455,9 @@ def main():
     ap.add_argument("--serve-sweep", type=str, default="16,32,64", help="further arrival rates (req/s)")
     ap.add_argument("--serve-sweep-requests", type=int, default=200)
     ap.add_argument("--no-dropouts", action="store_true", help="skip the per-field dropout detection")
+    ap.add_argument("--no-chroma", action="store_true",
+                    help="skip the colour-under chroma (the reference's --skip-chroma; the luma then renders at "
+                         "the hsync-refined locations)")
     ap.add_argument("--pipeline", type=int, default=3,
                     help="spans in flight: consecutive steps alternate between this many pipelines")
     ap.add_argument("--batch-segments", type=int, default=64, help="batched-segment run (0: skip)")
@@ -433,7 +475,8 @@ def main():
     from vhsdecode_hip import field, hipmem
     from vhsdecode_hip.demod import HipBlockDemodulator
 
-    fmt, tape, fl, nb = build_workload(args.fields)
+    with_chroma = not args.no_chroma
+    fmt, tape, fl, nb = build_workload(args.fields, with_chroma)
     lut = np.load(os.path.join(ROOT, "vhs-decode-dotnet_amd", "vhsdecode_hip", "data",
                                "sinc_lut.npz"))["downscale_sinc_lut"]
     span = nb * STRIDE
@@ -456,11 +499,12 @@ def main():
         def __init__(self, own_stream: bool):
             self.dem = HipBlockDemodulator(fl, max_blocks=nb)
             self.dem.set_sinc_table(lut)
-            self.dec = field.HipVhsFieldDecoder(self.dem, fmt)
+            self.dec = field.HipVhsFieldDecoder(self.dem, fmt, chroma=with_chroma)
             g = self.dec.geom
             self.planes = hipmem.DeviceBuffer(4 * span * 4)
             self.flags = hipmem.DeviceBuffer(nb * 4)
             self.out = hipmem.DeviceBuffer(max_fields * g.field_lines * g.out_len * 2)
+            self.out_c = hipmem.DeviceBuffer(max_fields * g.field_lines * g.out_len * 2) if with_chroma else None
             self.recs = hipmem.DeviceBuffer(max_fields * rec_size)
             self.meta = hipmem.DeviceBuffer(16)  # record count, next begin
             self.drop = hipmem.DeviceBuffer(4 * max_fields * (1 + 3 * DROPOUT_ROWS))
@@ -472,7 +516,8 @@ def main():
             st = self.stream.handle if self.stream else 0
             self.dem.demod_span_device(rf.ptr, 0, nb, *self.p, self.flags.ptr, stream=st)
             self.dec.decode_async(self.p[0], self.p[1], span, self.out.ptr, max_fields, self.meta.ptr,
-                                  self.recs.ptr, self.meta.ptr + 8, rounds=2, stream=st)
+                                  self.recs.ptr, self.meta.ptr + 8, rounds=2, stream=st, span_chroma_ptr=self.p[3],
+                                  out_chroma_ptr=self.out_c.ptr if with_chroma else 0)
             if dropouts is not None:
                 loc_ptr, nloc = self.dem.vhs_field_lines_device(3)
                 d = self.drop.ptr
@@ -527,6 +572,8 @@ def main():
     dem.set_profiling(False)
     dev_fields = pipes[0].out.download(np.uint16, len(records) * pipes[0].dec.geom.field_lines *
                                        pipes[0].dec.geom.out_len).reshape(len(records), -1)
+    dev_chroma = pipes[0].out_c.download(np.uint16, len(records) * pipes[0].dec.geom.field_lines *
+                                         pipes[0].dec.geom.out_len).reshape(len(records), -1) if with_chroma else None
 
     rf_samples = (nb - 1) * STRIDE + N
     elapsed = t1 - t0
@@ -582,6 +629,7 @@ def main():
                    "rf_samples_per_step": rf_samples, "blocks_per_step": nb, "fields_per_step": nfields,
                    "records_per_step": len(records), "block_length": N, "parallelism": "single GPU",
                    "spans_in_flight": len(pipes), "dropout_detection": dropouts is not None,
+                   "chroma": with_chroma,
                    "rf_copies_in_hbm": len(rf_copies),
                    "sync": "vhs field sequence (level detection, pulses, parity, line 0, line locations) on "
                            "the device, the serration search not run"},
@@ -600,7 +648,7 @@ def main():
         print(f"[bench] batched: {bt['segments']} segments / {bt['blocks']} blocks in {bt['batched_ms']} ms "
               f"({bt['batched_msps']} MSPS) vs plain {bt['plain_ms']} ms ({bt['plain_msps']} MSPS)", file=sys.stderr)
     if args.run_copies > 0:
-        wr = whole_run(fmt, tape, fl, lut, args.run_copies, dropouts is not None)
+        wr = whole_run(fmt, tape, fl, lut, args.run_copies, dropouts is not None, with_chroma)
         result["whole_run"] = wr
         print(f"[bench] whole run: {wr['fields_written']} fields in {wr['spans']} spans, {wr['seconds']} s "
               f"({wr['msps']} MSPS end to end with set-up; span loop {wr['loop_msps']} MSPS)", file=sys.stderr)
@@ -625,11 +673,14 @@ def main():
         usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
         # every core this process may run on, up to the 16-core host share of one GPU
         threads = args.cpu_threads or max(1, min(16, usable))
-        msps, reps, el, msps1, orecs, oouts = cpu_baseline(fmt, tape, fl, nb, lut, args.cpu_seconds, threads)
+        msps, reps, el, msps1, orecs, oouts, ochroma = cpu_baseline(fmt, tape, fl, nb, lut, args.cpu_seconds,
+                                                                     threads, with_chroma)
         result["cpu_baseline"] = {"value": round(msps, 3), "unit": "MSPS", "cores": threads, "kind": "port",
                                   "sample": f"the full {args.fields}-field tape x {reps} passes "
-                                            f"({el:.1f} s): FP64 oracle block demod and per-field hsync refine, TBC "
-                                            f"and dropout detection on {threads} threads; the field sequence serial",
+                                            f"({el:.1f} s): FP64 oracle block demod and per-field hsync refine, "
+                                            f"{'chroma burst analysis and decode, ' if with_chroma else ''}TBC "
+                                            f"and dropout detection on {threads} threads; the field sequence"
+                                            f"{' and the chroma track-phase chain' if with_chroma else ''} serial",
                                   "nproc": nproc, "affinity_cores": usable,
                                   "single_thread_value": round(msps1, 3)}
         # the device step's fields against this FP64 chain (TBC +-1 LSB bar)
@@ -647,7 +698,17 @@ def main():
                 total += d.size
         result["parity"] = {"records_match": bool(same), "tbc_max_lsb": worst,
                             "tbc_frac_gt1": (over / total) if total else None, "tbc_samples": total,
-                            "reference": "FP64 oracle chain (demod, field sequence, hsync, TBC) on the same RF"}
+                            "reference": "FP64 oracle chain (demod, field sequence, hsync, "
+                                         f"{'chroma burst analysis, ' if with_chroma else ''}TBC) on the same RF"}
+        if with_chroma and same:
+            cw, cover, ctotal = 0, 0, 0
+            for k, o in ochroma.items():
+                d = np.abs(dev_chroma[k].astype(np.int32) - o.astype(np.int32))
+                cw = max(cw, int(d.max()))
+                cover += int(np.count_nonzero(d > 1))
+                ctotal += d.size
+            result["parity"].update({"chroma_max_lsb": cw, "chroma_frac_gt1": (cover / ctotal) if ctotal else None,
+                                     "chroma_samples": ctotal})
     print(json.dumps(result), flush=True)
 
 

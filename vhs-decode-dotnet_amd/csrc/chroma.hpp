@@ -24,7 +24,7 @@ namespace chroma {
 
 constexpr int kMaxSections = 8;
 constexpr int kProbeThreads = 64;
-constexpr int kFiltThreads = 1024;
+constexpr int kIirChunk = 64;  // whole-field band-pass: samples per chunk
 
 // Per context and format, built on the host (vhsdecode_hip_set_vhs_chroma).
 struct ChromaCfg {
@@ -35,6 +35,8 @@ struct ChromaCfg {
   int win_max;             // longest probe window: burst_end + (burst_end - burst_start)
   int probe_ext;           // win_max + 2 * pad: a probe's filter buffer
   int pad;                 // the final band-pass's odd-extension length (DefaultPadLength)
+  int iir_chunk;           // whole-field band-pass: samples per chunk
+  int iir_warm;            // and the warm-up run before each chunk
   int sections;
   float sos[kMaxSections][6];  // b0 b1 b2 a0 a1 a2, FP32
   float zi[kMaxSections][2];   // steady-state initial conditions (FP32, scaled as the reference)
@@ -71,9 +73,11 @@ struct ChromaWork {
   double* an_y0;
   int lvl_stride;          // level-adjust entries per field (output lines + the largest first line)
   float* field;            // [F][lines * line_len]
-  float* fa;               // [F][filt_chunk * kFiltThreads] filter buffers (chunk-major)
+  double* theta;           // [F][lines * line_len] NTSC upconversion phases
+  float* fa;               // [F][iir_chunks * iir_chunk] the band-pass's sequence buffers (chunk-major)
   float* fb;
-  int filt_chunk;
+  float* iir_state;        // [F][iir_chunks][2][2 kMaxSections] chunk entry and end states
+  int iir_chunks;
   ChromaState* state;
   unsigned short* out;     // [records][lines * line_len]
 };

@@ -90,11 +90,99 @@ __global__ void __launch_bounds__(256) chroma_window_kernel(ChromaCfg cc, Chroma
   }
 }
 
+// One sample through the band-pass's cascade of S biquads (transposed direct form II, the
+// reference's per-sample float32 order), states st[s] = (z1, z2).
+template <int S>
+__device__ __forceinline__ void iir_step(const float (&c)[S][6], float (&st)[S][2], float& v) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    const float fo = (c[s][0] * v) + st[s][0];
+    st[s][0] = ((c[s][1] * v) - (c[s][4] * fo)) + st[s][1];
+    st[s][1] = (c[s][2] * v) - (c[s][5] * fo);
+    v = fo;
+  }
+}
+
+// Section s of the cascade on one sample: returns its output and updates its state.
+template <int S>
+__device__ __forceinline__ float iir_section(const float (&c)[S][6], float (&st)[S][2], int s, float v) {
+#pragma clang fp contract(off)
+  const float fo = (c[s][0] * v) + st[s][0];
+  st[s][0] = ((c[s][1] * v) - (c[s][4] * fo)) + st[s][1];
+  st[s][1] = (c[s][2] * v) - (c[s][5] * fo);
+  return fo;
+}
+
+// `count` samples of a strided sequence (sample j at src[j * step]) through the cascade from
+// `st`; with kStore the outputs go to dst[j * step] (dst may be src).  Every section sees
+// the same samples in the same order as in the sample-by-sample cascade, so the values are
+// identical; only the schedule is skewed: at time k section s takes sample k - s, so the S
+// section updates of one time step are independent.  The inputs are loaded one batch of B
+// time steps ahead; a sample is read before its own output (S - 1 time steps later) or any
+// later output is written.
+template <int S, int B, bool kStore, typename P>
+__device__ __forceinline__ void iir_run(const float (&c)[S][6], float (&st)[S][2], const P* src, P* dst, int step,
+                                        int count) {
+  if (count < S) {
+    for (int j = 0; j < count; j++) {
+      float v = src[j * step];
+      iir_step<S>(c, st, v);
+      if (kStore) dst[j * step] = v;
+    }
+    return;
+  }
+  float pipe[S];  // pipe[s]: section s's latest output
+  // times 0 .. S-2: sections 0 .. k
+#pragma unroll
+  for (int k = 0; k < S - 1; k++) {
+#pragma unroll
+    for (int s = S - 1; s >= 0; s--)
+      if (s <= k) pipe[s] = iir_section<S>(c, st, s, s == 0 ? (float)src[k * step] : pipe[s > 0 ? s - 1 : 0]);
+  }
+  // times S-1 .. count-1: every section; the output of sample k - (S - 1)
+  const int k0 = S - 1;
+  const int steady = count - k0;
+  const int full = steady - steady % B;
+  float cur[B], nxt[B];
+  // the loads past the last sample read the last sample again (unused)
+#pragma unroll
+  for (int j = 0; j < B; j++) cur[j] = src[min(k0 + j, count - 1) * step];
+  for (int base = 0; base < full; base += B) {
+#pragma unroll
+    for (int j = 0; j < B; j++) nxt[j] = src[min(k0 + base + B + j, count - 1) * step];
+#pragma unroll
+    for (int j = 0; j < B; j++) {
+#pragma unroll
+      for (int s = S - 1; s >= 0; s--) pipe[s] = iir_section<S>(c, st, s, s == 0 ? cur[j] : pipe[s > 0 ? s - 1 : 0]);
+      if (kStore) dst[(base + j) * step] = pipe[S - 1];
+    }
+#pragma unroll
+    for (int j = 0; j < B; j++) cur[j] = nxt[j];
+  }
+  for (int j = full; j < steady; j++) {
+    const float x = src[(k0 + j) * step];
+#pragma unroll
+    for (int s = S - 1; s >= 0; s--) pipe[s] = iir_section<S>(c, st, s, s == 0 ? x : pipe[s > 0 ? s - 1 : 0]);
+    if (kStore) dst[j * step] = pipe[S - 1];
+  }
+  // times count .. count+S-2: sections e .. S-1 finish the last samples
+#pragma unroll
+  for (int e = 1; e < S; e++) {
+#pragma unroll
+    for (int s = S - 1; s >= 1; s--)
+      if (s >= e) pipe[s] = iir_section<S>(c, st, s, pipe[s - 1]);
+    if (kStore) dst[(count - S + e) * step] = pipe[S - 1];
+  }
+}
+
 // 2. The burst probe of phase line q for track-phase start `cand`: heterodyne, the final
 // band-pass forward and backward in float32 over the padded window
 // (SosFilter.ApplyForwardBackwardFloat32InPlace: odd extension, steady-state initial
 // conditions, cascaded biquads), and DemodBurstFloat32 (VhsChromaDecoder.cs:1521-1626).
 // One thread per (line, start); its filter buffer is a column of LDS (element k at k * 64).
+// S = the band-pass's section count.
+template <int S>
 __global__ void __launch_bounds__(kProbeThreads) chroma_probe_kernel(ChromaCfg cc, ChromaWork cw,
                                                                      const double* refined, int line_stride,
                                                                      const int* decoded) {
@@ -122,25 +210,18 @@ __global__ void __launch_bounds__(kProbeThreads) chroma_probe_kernel(ChromaCfg c
     for (int i = 0; i < P; i++)
       e[(P + nw + i) * kProbeThreads] = (2.0f * last) - e[(P + nw - 2 - i) * kProbeThreads];
     const int m = nw + 2 * P;
-    float sz[kMaxSections][2];
+    float c[S][6];
+    for (int s = 0; s < S; s++)
+      for (int k = 0; k < 6; k++) c[s][k] = cc.sos[s][k];
+    float sz[S][2];
     for (int dir = 0; dir < 2; dir++) {
       const float x0 = e[(dir == 0 ? 0 : m - 1) * kProbeThreads];
-      for (int s = 0; s < cc.sections; s++) {
+      for (int s = 0; s < S; s++) {
         sz[s][0] = cc.zi[s][0] * x0;
         sz[s][1] = cc.zi[s][1] * x0;
       }
-      for (int step = 0; step < m; step++) {
-        const int idx = dir == 0 ? step : m - 1 - step;
-        float v = e[idx * kProbeThreads];
-        for (int s = 0; s < cc.sections; s++) {
-          const float* c = cc.sos[s];
-          const float fo = (c[0] * v) + sz[s][0];
-          sz[s][0] = ((c[1] * v) - (c[4] * fo)) + sz[s][1];
-          sz[s][1] = (c[2] * v) - (c[5] * fo);
-          v = fo;
-        }
-        e[idx * kProbeThreads] = v;
-      }
+      float* p = e + (dir == 0 ? 0 : (m - 1) * kProbeThreads);
+      iir_run<S, 8, true>(c, sz, p, p, dir == 0 ? kProbeThreads : -kProbeThreads, m);
     }
   }
   // DemodBurstFloat32 over the filtered window less the padding on either side
@@ -192,48 +273,63 @@ __global__ void __launch_bounds__(kProbeThreads) chroma_probe_kernel(ChromaCfg c
 }
 
 // 3. Per field and track-phase start: ShouldFlipTrackPhase (:4164-4217) and the averages of
-// SummarizePhaseSequence (:4219-4293; colour killer off: the burst-detected line is 0).
-__global__ void __launch_bounds__(64) chroma_summary_kernel(ChromaCfg cc, ChromaWork cw, const int* decoded) {
+// SummarizePhaseSequence (:4219-4293; colour killer off: the burst-detected line is 0).  One
+// wave per start: the lanes take the lines (delta buckets, normalised I/Q), the bucket counts
+// are integer sums in any order, and lane 0 adds the FP64 terms in line order.
+constexpr int kSummaryLines = 512;
+__global__ void __launch_bounds__(128) chroma_summary_kernel(ChromaCfg cc, ChromaWork cw, const int* decoded) {
 #pragma clang fp contract(off)
+  __shared__ double ni_s[2][kSummaryLines], nq_s[2][kSummaryLines], mag_s[2][kSummaryLines];
+  __shared__ unsigned char use_s[2][kSummaryLines];
+  __shared__ int buckets[2][4];
   const int f = blockIdx.x;
   if (f >= *decoded) return;
-  const int cand = threadIdx.x;
-  if (cand >= 2) return;
+  const int cand = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   const double* pr = cw.probe + ((long long)f * 2 + cand) * cc.lines * 5;
   const int end = cc.lines + 1;  // lines_out + line offset
   const int check_end = end - kBurstCheckSkipLines;
-  int d[4] = {0, 0, 0, 0};
-  for (int k = 1; k < cc.lines; k++) {
+  if (lane < 4) buckets[cand][lane] = 0;
+  __syncthreads();
+  for (int k = lane; k < cc.lines; k += 64) {
     const int line = k + 1;
-    if (line <= kBurstCheckSkipLines || line >= check_end) continue;
-    const double delta = positive_degrees(pr[k * 5] - pr[(k - 1) * 5]);
-    const int bucket = (int)floor((delta + 45.0) / 90.0) % 4;
-    d[bucket]++;
+    const bool inside = !(line <= kBurstCheckSkipLines || line >= check_end);
+    if (k >= 1 && inside) {
+      const double delta = positive_degrees(pr[k * 5] - pr[(k - 1) * 5]);
+      const int bucket = (int)floor((delta + 45.0) / 90.0) % 4;
+      atomicAdd(&buckets[cand][bucket], 1);
+    }
+    const double m = pr[k * 5 + 2];
+    const bool used = inside && m != 0.0;
+    use_s[cand][k] = used ? 1 : 0;
+    mag_s[cand][k] = m;
+    ni_s[cand][k] = used ? pr[k * 5 + 3] / m : 0.0;
+    nq_s[cand][k] = used ? pr[k * 5 + 4] / m : 0.0;
   }
+  __syncthreads();
+  if (lane != 0) return;
+  const int* d = buckets[cand];
   cw.flip[f * 2 + cand] = cc.is_ntsc ? (d[0] < d[2]) : ((d[1] + d[3]) < (d[0] + d[2]));
   double ei = 0.0, eq = 0.0, oi = 0.0, oq = 0.0, mag = 0.0;
   int count = 0;
   for (int k = 0; k < cc.lines; k++) {
-    const int line = k + 1;
-    if (line <= kBurstCheckSkipLines || line >= check_end) continue;
-    const double m = pr[k * 5 + 2];
-    if (m == 0.0) continue;
-    const double ni = pr[k * 5 + 3] / m, nq = pr[k * 5 + 4] / m;
+    if (!use_s[cand][k]) continue;
+    const double m = mag_s[cand][k];
     count++;
     mag += m;
-    if ((line & 1) == 1) {
-      oi += ni;
-      oq += nq;
+    if (((k + 1) & 1) == 1) {
+      oi += ni_s[cand][k];
+      oq += nq_s[cand][k];
     } else {
-      ei += ni;
-      eq += nq;
+      ei += ni_s[cand][k];
+      eq += nq_s[cand][k];
     }
   }
-  double* s = cw.summ + ((long long)f * 2 + cand) * 4;
-  s[0] = count > 0 ? mag / count : 0.0;
-  s[1] = positive_degrees(atan2(eq + oq, ei + oi) * 180.0 / M_PI);
-  s[2] = positive_degrees(atan2(eq, ei) * 180.0 / M_PI);
-  s[3] = positive_degrees(atan2(oq, oi) * 180.0 / M_PI);
+  double* sm = cw.summ + ((long long)f * 2 + cand) * 4;
+  sm[0] = count > 0 ? mag / count : 0.0;
+  sm[1] = positive_degrees(atan2(eq + oq, ei + oi) * 180.0 / M_PI);
+  sm[2] = positive_degrees(atan2(eq, ei) * 180.0 / M_PI);
+  sm[3] = positive_degrees(atan2(oq, oi) * 180.0 / M_PI);
 }
 
 // 4. The track-phase start carried through the round's fields in order
@@ -292,7 +388,8 @@ __device__ __forceinline__ int phase_line_of(int j, int lines, int lo) {
 
 // 6. The render resample of the chroma plane (same positions and level adjusts as the
 // luma), NTSC burst de-emphasis (ApplyBurstDeemphasisCore :2695-2719, five samples after
-// the burst) and, for PAL, the heterodyne upconversion (UpconvertChroma :2009-2045).
+// the burst) and the upconversion: NTSC phase compensated (the cosine of the sample's phase
+// from the phase kernel), PAL heterodyne (UpconvertChroma :2009-2045).
 __global__ void __launch_bounds__(256) chroma_field_kernel(ChromaCfg cc, ChromaWork cw, const double* render,
                                                            int line_stride, const long long* origins,
                                                            const int* first_lines, const double* line_x,
@@ -317,7 +414,7 @@ __global__ void __launch_bounds__(256) chroma_field_kernel(ChromaCfg cc, ChromaW
   float o;
   if (cc.is_ntsc) {
     if (smp >= cc.burst_end + 5) v *= 2.0;
-    o = (float)v;
+    o = (float)((double)(float)v * -cos(cw.theta[(long long)f * n + d]));
   } else {
     const int k = phase_line_of(j, cc.lines, first);
     if (k < 0) {
@@ -330,12 +427,12 @@ __global__ void __launch_bounds__(256) chroma_field_kernel(ChromaCfg cc, ChromaW
   cw.field[(long long)f * n + d] = o;
 }
 
-// 7. NTSC: UpconvertChromaPhaseCompensated (:2167-2249) for the field's colour-frame target
-// (NtscFieldPhaseTarget :3915-3925): per line, four lanes each carry an FP64 phase in steps
-// of four samples (the vectorised loop), the remainder carried from the line's start.
-__global__ void __launch_bounds__(256) chroma_ntsc_upconvert_kernel(ChromaCfg cc, ChromaWork cw,
-                                                                    const vsync::FieldRecord* records,
-                                                                    const vsync::SeqCtl* ctl, const int* decoded) {
+// 7. NTSC: the phases of UpconvertChromaPhaseCompensated (:2167-2249) for the field's
+// colour-frame target (NtscFieldPhaseTarget :3915-3925): per line, four lanes each carry an
+// FP64 phase in steps of four samples (the vectorised loop), the remainder carried from the
+// line's start.  Only the running sums are sequential; the field kernel takes each sample's
+// cosine.  Runs before the field kernel.
+__global__ void __launch_bounds__(256) chroma_ntsc_phase_kernel(ChromaCfg cc, ChromaWork cw, const int* decoded) {
 #pragma clang fp contract(off)
   const int f = blockIdx.y;
   if (f >= *decoded) return;
@@ -353,8 +450,6 @@ __global__ void __launch_bounds__(256) chroma_ntsc_upconvert_kernel(ChromaCfg cc
   const double four = het * 4.0;
   const int c = cw.chosen[f];
   // the target depends on the colour frame only: (first, second frame) -> 147 or -33
-  (void)records;
-  (void)ctl;
   const bool second_frame = ((cw.field_no[f] / 2) & 1) == 1;
   const double target = second_frame ? 147.0 : -33.0;
   const int rot = pmod(cc.rot[c] * (k + 1), 4);
@@ -365,129 +460,223 @@ __global__ void __launch_bounds__(256) chroma_ntsc_upconvert_kernel(ChromaCfg cc
   theta = __fma_rn(target + burst_phase, deg2rad, theta);
   const int vc = cc.line_len & ~3;
   const int ve = s + vc;
-  float* fld = cw.field + (long long)f * n;
-  double th = theta + coef[lane];
+  double* th = cw.theta + (long long)f * n;
+  double tl = theta + coef[lane];
   for (int i = s + lane; i < ve; i += 4) {
-    fld[i] = (float)((double)fld[i] * -cos(th));
-    th += four;
+    th[i] = tl;
+    tl += four;
   }
   if (lane == 0) {
     double tt = theta + het * vc;
     for (int i = ve; i < e; i++) {
-      fld[i] = (float)((double)fld[i] * -cos(tt));
+      th[i] = tt;
       tt += het;
     }
   }
 }
 
 // 8. The final band-pass forward and backward in float32 over the whole field
-// (SosFilter.ApplyForwardBackwardFloat32InPlace), exactly: the odd-extended sequence is cut
-// into one chunk per thread (chunk-major layout, so each step's loads and stores are
-// coalesced); per section and direction every chunk runs sequentially from an entry state,
-// first from zero, then from its predecessor's end state, until no end state changes.  A
-// chunk whose entry state is exact produces exact values, and the chunk processed first
-// starts from the true initial state, so when the end states stop changing every entry state
-// is the sequential one (the band-pass forgets its entry state well within a chunk, so this
-// takes two passes; at most one pass per chunk in any case).
-__device__ void filt_pass(const float* src, float* dst, int chunk, int m, bool forward, const float* c, float i0,
-                          float i1, float* z0, float* z1, int* changed) {
-#pragma clang fp contract(off)
-  const int t = threadIdx.x;
-  const int T = kFiltThreads;
-  const int last_chunk = (m - 1) / chunk;
-  const int first = forward ? 0 : last_chunk;
-  const int count = max(0, min(chunk, m - t * chunk));
-  auto run = [&](float s1, float s2, bool write, float* e0, float* e1) {
-    for (int kk = 0; kk < count; kk++) {
-      const int k = forward ? kk : count - 1 - kk;
-      const float v = src[(long long)k * T + t];
-      const float fo = (c[0] * v) + s1;
-      s1 = ((c[1] * v) - (c[4] * fo)) + s2;
-      s2 = (c[2] * v) - (c[5] * fo);
-      if (write) dst[(long long)k * T + t] = fo;
-    }
-    *e0 = s1;
-    *e1 = s2;
-  };
-  float a, b;
-  if (t == first)
-    run(i0, i1, false, &a, &b);
-  else
-    run(0.0f, 0.0f, false, &a, &b);
-  z0[t] = a;
-  z1[t] = b;
-  __syncthreads();
-  for (int iter = 0; iter <= T + 1; iter++) {
-    const int pred = forward ? t - 1 : t + 1;
-    float e0 = i0, e1 = i1;
-    if (t != first) {
-      if (pred >= 0 && pred < T) {
-        e0 = z0[pred];
-        e1 = z1[pred];
-      } else {
-        e0 = e1 = 0.0f;
-      }
-    }
-    if (t == 0) *changed = 0;
-    __syncthreads();
-    run(e0, e1, true, &a, &b);
-    // empty chunks (past the sequence's end) are no chunk's predecessor in either direction
-    if (count > 0 && (__float_as_uint(a) != __float_as_uint(z0[t]) || __float_as_uint(b) != __float_as_uint(z1[t])))
-      atomicOr(changed, 1);
-    __syncthreads();
-    z0[t] = a;
-    z1[t] = b;
-    const int any = *changed;
-    __syncthreads();
-    if (!any) break;
-  }
+// (SosFilter.ApplyForwardBackwardFloat32InPlace), exactly, spread over the chip: the
+// odd-extended sequence is cut into chunks of `iir_chunk` samples, one thread each.  A
+// chunk's thread runs the section cascade sample by sample (the values of the reference's
+// per-sample cascade) from a zero state `iir_warm` samples before its chunk (the band-pass
+// forgets its state well within them) and keeps its state on entering the chunk and on
+// leaving it; the chunk processed first starts from the true initial state.  The check
+// kernel then compares every chunk's entry state with its predecessor's end state bit for
+// bit: when all agree, every chunk started from the sequential state (induction from the
+// first), so every value is the sequential one; a chunk that disagrees is run again from
+// its predecessor's end state, in chunk order, until the states agree again.
+__device__ __forceinline__ long long iir_field(const ChromaWork& cw, int f) {
+  return (long long)f * cw.iir_chunks * kIirChunk;
 }
 
-__global__ void __launch_bounds__(kFiltThreads) chroma_filtfilt_kernel(ChromaCfg cc, ChromaWork cw,
-                                                                       const int* decoded) {
+// Chunk-major layout: element idx of chunk idx / C at position idx % C, so that the threads
+// of consecutive chunks read consecutive words at every step.
+__device__ __forceinline__ int iir_at(const ChromaWork& cw, int idx) {
+  return (idx % kIirChunk) * cw.iir_chunks + idx / kIirChunk;
+}
+
+// Runs chunk t of a field (src -> dst) from `st`, which holds the state entering it; on
+// return `st` holds the state leaving it.
+template <int S>
+__device__ __forceinline__ void iir_chunk(const ChromaWork& cw, const float (&c)[S][6], const float* src, float* dst,
+                                          int m, int t, bool forward, float (&st)[S][2]) {
+  const int count = min(m - t * kIirChunk, kIirChunk);
+  const int stride = cw.iir_chunks;
+  const int at = t + (forward ? 0 : (count - 1) * stride);
+  iir_run<S, 16, true>(c, st, src + at, dst + at, forward ? stride : -stride, count);
+}
+
+// The odd-extended field into fa (chunk-major) and, after the backward pass, fa's middle
+// back into the field: tiles of 64 chunks x 64 samples, transposed through LDS so that both
+// sides move whole rows.
+__global__ void __launch_bounds__(256) chroma_ext_kernel(ChromaCfg cc, ChromaWork cw, const int* decoded) {
 #pragma clang fp contract(off)
-  __shared__ float z0[kFiltThreads], z1[kFiltThreads];
-  __shared__ int changed;
-  const int f = blockIdx.x;
+  __shared__ float tile[kIirChunk][kIirChunk + 1];
+  const int f = blockIdx.y;
   if (f >= *decoded) return;
-  const int T = kFiltThreads;
   const int n = cc.lines * cc.line_len;
   const int P = cc.pad;
   const int m = n + 2 * P;
-  const int C = cw.filt_chunk;
-  float* A = cw.fa + (long long)f * C * T;
-  float* B = cw.fb + (long long)f * C * T;
-  float* fld = cw.field + (long long)f * n;
-  auto at = [&](int idx) { return (long long)(idx % C) * T + idx / C; };
-  const float first_v = fld[0], last_v = fld[n - 1];
-  for (int idx = threadIdx.x; idx < m; idx += T) {
-    float v;
+  const int t0 = blockIdx.x * kIirChunk;
+  const float* fld = cw.field + (long long)f * n;
+  for (int r = threadIdx.x; r < kIirChunk * kIirChunk; r += blockDim.x) {
+    const int idx = t0 * kIirChunk + r;
+    float v = 0.0f;
     if (idx < P)
-      v = (2.0f * first_v) - fld[P - idx];
+      v = (2.0f * fld[0]) - fld[P - idx];
     else if (idx < P + n)
       v = fld[idx - P];
-    else
-      v = (2.0f * last_v) - fld[n - 2 - (idx - P - n)];
-    A[at(idx)] = v;
+    else if (idx < m)
+      v = (2.0f * fld[n - 1]) - fld[n - 2 - (idx - P - n)];
+    tile[r / kIirChunk][r % kIirChunk] = v;
   }
   __syncthreads();
-  const float x0f = A[at(0)];
-  for (int s = 0; s < cc.sections; s++) {
-    filt_pass(A, B, C, m, true, cc.sos[s], cc.zi[s][0] * x0f, cc.zi[s][1] * x0f, z0, z1, &changed);
-    float* tmp = A;
-    A = B;
-    B = tmp;
+  float* a = cw.fa + iir_field(cw, f);
+  for (int r = threadIdx.x; r < kIirChunk * kIirChunk; r += blockDim.x) {
+    const int k = r / kIirChunk, tc = r % kIirChunk;
+    if (t0 + tc < cw.iir_chunks) a[(long long)k * cw.iir_chunks + t0 + tc] = tile[tc][k];
+  }
+}
+
+__global__ void __launch_bounds__(256) chroma_out_kernel(ChromaCfg cc, ChromaWork cw, const int* decoded) {
+  __shared__ float tile[kIirChunk][kIirChunk + 1];
+  const int f = blockIdx.y;
+  if (f >= *decoded) return;
+  const int n = cc.lines * cc.line_len;
+  const int P = cc.pad;
+  const int t0 = blockIdx.x * kIirChunk;
+  const float* a = cw.fa + iir_field(cw, f);
+  for (int r = threadIdx.x; r < kIirChunk * kIirChunk; r += blockDim.x) {
+    const int k = r / kIirChunk, tc = r % kIirChunk;
+    tile[tc][k] = t0 + tc < cw.iir_chunks ? a[(long long)k * cw.iir_chunks + t0 + tc] : 0.0f;
+  }
+  __syncthreads();
+  float* fld = cw.field + (long long)f * n;
+  for (int r = threadIdx.x; r < kIirChunk * kIirChunk; r += blockDim.x) {
+    const int i = t0 * kIirChunk + r - P;
+    if (i >= 0 && i < n) fld[i] = tile[r / kIirChunk][r % kIirChunk];
+  }
+}
+
+// One pass: forward reads fa and writes fb, backward reads fb and writes fa.
+template <int S>
+__global__ void __launch_bounds__(256) chroma_iir_pass_kernel(ChromaCfg cc, ChromaWork cw, int forward,
+                                                              const int* decoded) {
+#pragma clang fp contract(off)
+  const int f = blockIdx.y;
+  if (f >= *decoded) return;
+  const int n = cc.lines * cc.line_len;
+  const int m = n + 2 * cc.pad;
+  const int C = kIirChunk;
+  const int chunks = (m + C - 1) / C;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= chunks) return;
+  const float* src = (forward ? cw.fa : cw.fb) + iir_field(cw, f);
+  float* dst = (forward ? cw.fb : cw.fa) + iir_field(cw, f);
+  float c[S][6];
+  for (int s = 0; s < S; s++)
+    for (int q = 0; q < 6; q++) c[s][q] = cc.sos[s][q];
+  const float x0 = forward ? src[iir_at(cw, 0)] : src[iir_at(cw, m - 1)];
+  float st[S][2];
+  const int lo = t * C, hi = min(m, lo + C);
+  // warm-up: from the true initial state when it reaches the sequence's first sample; chunk
+  // by chunk (each a column of the chunk-major buffer)
+  const int stride = cw.iir_chunks;
+  if (forward) {
+    const int w0 = lo - cc.iir_warm;
+    const bool exact = w0 <= 0;
+    for (int s = 0; s < S; s++) {
+      st[s][0] = exact ? cc.zi[s][0] * x0 : 0.0f;
+      st[s][1] = exact ? cc.zi[s][1] * x0 : 0.0f;
+    }
+    for (int i = exact ? 0 : w0; i < lo;) {
+      const int u = i / kIirChunk, k = i % kIirChunk;
+      iir_run<S, 16, false>(c, st, src + u + k * stride, (float*)nullptr, stride, kIirChunk - k);
+      i += kIirChunk - k;
+    }
+  } else {
+    const int w1 = hi - 1 + cc.iir_warm;
+    const bool exact = w1 >= m - 1;
+    for (int s = 0; s < S; s++) {
+      st[s][0] = exact ? cc.zi[s][0] * x0 : 0.0f;
+      st[s][1] = exact ? cc.zi[s][1] * x0 : 0.0f;
+    }
+    for (int i = exact ? m - 1 : w1; i >= hi;) {
+      const int u = i / kIirChunk, k = i % kIirChunk;
+      iir_run<S, 16, false>(c, st, src + u + k * stride, (float*)nullptr, -stride, k + 1);
+      i -= k + 1;
+    }
+  }
+  float* ent = cw.iir_state + (((long long)f * cw.iir_chunks + t) * 2) * (2 * kMaxSections);
+  float* ext = ent + 2 * kMaxSections;
+  for (int s = 0; s < S; s++) {
+    ent[2 * s] = st[s][0];
+    ent[2 * s + 1] = st[s][1];
+  }
+  iir_chunk<S>(cw, c, src, dst, m, t, forward != 0, st);
+  for (int s = 0; s < S; s++) {
+    ext[2 * s] = st[s][0];
+    ext[2 * s + 1] = st[s][1];
+  }
+}
+
+// The check after a pass (one workgroup per field): entry states against the predecessors'
+// end states.  A round takes the chunks that disagree with their predecessors' end states,
+// gives each that state as its entry state and runs them again, all at once; the first
+// disagreeing chunk's predecessor is exact (every chunk before it agrees), so each round
+// makes at least that chunk exact and the rounds end; a warm-up that merged leaves none to
+// run.
+template <int S>
+__global__ void __launch_bounds__(1024) chroma_iir_check_kernel(ChromaCfg cc, ChromaWork cw, int forward,
+                                                                const int* decoded) {
+#pragma clang fp contract(off)
+  __shared__ int any_bad;
+  const int f = blockIdx.x;
+  if (f >= *decoded) return;
+  const int n = cc.lines * cc.line_len;
+  const int m = n + 2 * cc.pad;
+  const int chunks = (m + kIirChunk - 1) / kIirChunk;
+  float* states = cw.iir_state + (long long)f * cw.iir_chunks * 2 * (2 * kMaxSections);
+  auto entry = [&](int t) { return states + (long long)t * 2 * (2 * kMaxSections); };
+  auto leave = [&](int t) { return entry(t) + 2 * kMaxSections; };
+  const float* src = (forward ? cw.fa : cw.fb) + iir_field(cw, f);
+  float* dst = (forward ? cw.fb : cw.fa) + iir_field(cw, f);
+  float c[S][6];
+  for (int s = 0; s < S; s++)
+    for (int q = 0; q < 6; q++) c[s][q] = cc.sos[s][q];
+  for (int round = 0; round <= chunks; round++) {
+    if (threadIdx.x == 0) any_bad = 0;
+    __syncthreads();
+    // entry(t) is read and written by chunk t's thread only in this loop, leave(p) only read
+    unsigned mine = 0;
+    for (int t = threadIdx.x, k = 0; t < chunks; t += blockDim.x, k++) {
+      const int p = forward ? t - 1 : t + 1;
+      if (p < 0 || p >= chunks) continue;
+      bool same = true;
+      for (int q = 0; q < 2 * S; q++) same &= __float_as_uint(entry(t)[q]) == __float_as_uint(leave(p)[q]);
+      if (same) continue;
+      for (int q = 0; q < 2 * S; q++) entry(t)[q] = leave(p)[q];
+      mine |= 1u << k;
+      any_bad = 1;
+    }
+    __syncthreads();
+    if (!any_bad) break;
+    for (int t = threadIdx.x, k = 0; t < chunks; t += blockDim.x, k++) {
+      if (!((mine >> k) & 1u)) continue;
+      float st[S][2];
+      for (int s = 0; s < S; s++) {
+        st[s][0] = entry(t)[2 * s];
+        st[s][1] = entry(t)[2 * s + 1];
+      }
+      iir_chunk<S>(cw, c, src, dst, m, t, forward != 0, st);
+      for (int s = 0; s < S; s++) {
+        leave(t)[2 * s] = st[s][0];
+        leave(t)[2 * s + 1] = st[s][1];
+      }
+    }
     __syncthreads();
   }
-  const float x0b = A[at(m - 1)];
-  for (int s = 0; s < cc.sections; s++) {
-    filt_pass(A, B, C, m, false, cc.sos[s], cc.zi[s][0] * x0b, cc.zi[s][1] * x0b, z0, z1, &changed);
-    float* tmp = A;
-    A = B;
-    B = tmp;
-    __syncthreads();
-  }
-  for (int i = threadIdx.x; i < n; i += T) fld[i] = A[at(P + i)];
 }
 
 // 9. ApplyAutomaticChromaGainWithCombToU16 (:3504-3583) with WriteScaledChromaToU16
@@ -596,11 +785,24 @@ hipError_t vhsd_launch_chroma_analysis(const ChromaCfg* cc, const ChromaWork* cw
   hipLaunchKernelGGL(chroma_window_kernel, dim3(cc->lines, max_fields), dim3(256), 0, stream, *cc, *cw, refined,
                      line_stride, origins, pow_table, nominal, smooth, lut, decoded);
   const size_t lds = (size_t)cc->probe_ext * kProbeThreads * sizeof(float);
-  hipError_t e = vhsd::lds_limit_once((const void*)chroma_probe_kernel, (int)lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(chroma_probe_kernel, dim3((2 * cc->lines + kProbeThreads - 1) / kProbeThreads, max_fields),
-                     dim3(kProbeThreads), lds, stream, *cc, *cw, refined, line_stride, decoded);
-  hipLaunchKernelGGL(chroma_summary_kernel, dim3(max_fields), dim3(64), 0, stream, *cc, *cw, decoded);
+  const dim3 pgrid((2 * cc->lines + kProbeThreads - 1) / kProbeThreads, max_fields);
+  hipError_t e = hipSuccess;
+  switch (cc->sections) {
+#define VHSD_PROBE_CASE(S)                                                                                case S:                                                                                                   e = vhsd::lds_limit_once((const void*)chroma_probe_kernel<S>, (int)lds);                                if (e != hipSuccess) return e;                                                                          hipLaunchKernelGGL(chroma_probe_kernel<S>, pgrid, dim3(kProbeThreads), lds, stream, *cc, *cw, refined,                        line_stride, decoded);                                                               break;
+    VHSD_PROBE_CASE(1)
+    VHSD_PROBE_CASE(2)
+    VHSD_PROBE_CASE(3)
+    VHSD_PROBE_CASE(4)
+    VHSD_PROBE_CASE(5)
+    VHSD_PROBE_CASE(6)
+    VHSD_PROBE_CASE(7)
+    VHSD_PROBE_CASE(8)
+#undef VHSD_PROBE_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+  if (cc->lines > kSummaryLines) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(chroma_summary_kernel, dim3(max_fields), dim3(128), 0, stream, *cc, *cw, decoded);
   hipLaunchKernelGGL(chroma_chain_kernel, dim3(1), dim3(64), 0, stream, *cw, decoded);
   hipLaunchKernelGGL(chroma_render_kernel, dim3((nloc + 255) / 256, max_fields), dim3(256), 0, stream, *cc, *cw,
                      refined, render, line_stride, nloc, fsc_shift, decoded);
@@ -616,12 +818,44 @@ hipError_t vhsd_launch_chroma_fields(const ChromaCfg* cc, const ChromaWork* cw, 
                                      const int* out_index, int max_fields, const int* decoded, hipStream_t stream) {
   if (max_fields <= 0) return hipSuccess;
   const int n = cc->lines * cc->line_len;
+  if (cc->is_ntsc)
+    hipLaunchKernelGGL(chroma_ntsc_phase_kernel, dim3((4 * cc->lines + 255) / 256, max_fields), dim3(256), 0, stream,
+                       *cc, *cw, decoded);
   hipLaunchKernelGGL(chroma_field_kernel, dim3((n + 255) / 256, max_fields), dim3(256), 0, stream, *cc, *cw, render,
                      line_stride, origins, first_lines, line_x, line_y0, pow_table, nominal, smooth, lut, decoded);
-  if (cc->is_ntsc)
-    hipLaunchKernelGGL(chroma_ntsc_upconvert_kernel, dim3((4 * cc->lines + 255) / 256, max_fields), dim3(256), 0,
-                       stream, *cc, *cw, records, ctl, decoded);
-  hipLaunchKernelGGL(chroma_filtfilt_kernel, dim3(max_fields), dim3(kFiltThreads), 0, stream, *cc, *cw, decoded);
+  (void)records;
+  (void)ctl;
+  {
+    const int m = n + 2 * cc->pad;
+    const int chunks = (m + kIirChunk - 1) / kIirChunk;
+    // the check kernel's 1024 threads take at most 32 chunks each (a bit mask)
+    if (cc->iir_chunk != kIirChunk || chunks > cw->iir_chunks || chunks > 32 * 1024) return hipErrorInvalidValue;
+    const dim3 tiles((chunks + kIirChunk - 1) / kIirChunk, max_fields);
+    hipLaunchKernelGGL(chroma_ext_kernel, tiles, dim3(256), 0, stream, *cc, *cw, decoded);
+    switch (cc->sections) {
+#define VHSD_IIR_CASE(S)                                                                                        \
+  case S:                                                                                                       \
+    for (int dir = 1; dir >= 0; dir--) {                                                                        \
+      hipLaunchKernelGGL(chroma_iir_pass_kernel<S>, dim3((chunks + 255) / 256, max_fields), dim3(256), 0, stream, \
+                         *cc, *cw, dir, decoded);                                                               \
+      hipLaunchKernelGGL(chroma_iir_check_kernel<S>, dim3(max_fields), dim3(1024), 0, stream, *cc, *cw, dir,   \
+                         decoded);                                                                              \
+    }                                                                                                           \
+    break;
+      VHSD_IIR_CASE(1)
+      VHSD_IIR_CASE(2)
+      VHSD_IIR_CASE(3)
+      VHSD_IIR_CASE(4)
+      VHSD_IIR_CASE(5)
+      VHSD_IIR_CASE(6)
+      VHSD_IIR_CASE(7)
+      VHSD_IIR_CASE(8)
+#undef VHSD_IIR_CASE
+      default:
+        return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(chroma_out_kernel, tiles, dim3(256), 0, stream, *cc, *cw, decoded);
+  }
   hipLaunchKernelGGL(chroma_acc_kernel, dim3(cc->lines, max_fields), dim3(256),
                      (size_t)cc->line_len * sizeof(float), stream, *cc, *cw, out_index, decoded);
   return hipGetLastError();

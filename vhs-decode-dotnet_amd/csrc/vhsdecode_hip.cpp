@@ -1654,6 +1654,29 @@ int32_t vhsdecode_hip_set_vhs_chroma(vhsdecode_hip_demod_context* c, const vhsde
     cc.win_max = p->burst_end + (p->burst_end - p->burst_start);
     cc.pad = default_pad_length(&p->final_sos[0][0], p->sections);
     cc.probe_ext = cc.win_max + 2 * cc.pad;
+    // the whole-field band-pass's chunks and warm-up: 2.5 times the samples its slowest
+    // pole takes to forget a state to 1e-8, plus a margin (shorter warm-ups leave many chunks
+    // to run again on decoded fields; exactness does not depend on it: a chunk whose warm-up
+    // fell short is run again)
+    {
+      double rmax = 0.0;
+      for (int s = 0; s < p->sections; s++) {
+        const double a0 = p->final_sos[s][3], a1 = p->final_sos[s][4] / a0, a2 = p->final_sos[s][5] / a0;
+        const double disc = a1 * a1 - 4.0 * a2;
+        double r;
+        if (disc < 0.0) {
+          r = std::sqrt(a2);
+        } else {
+          const double q = std::sqrt(disc);
+          r = std::max(std::fabs((-a1 + q) / 2.0), std::fabs((-a1 - q) / 2.0));
+        }
+        rmax = std::max(rmax, r);
+      }
+      if (!(rmax < 1.0)) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "the final chroma band-pass is not stable");
+      const double decay = std::log(1e-8) / std::log(std::max(rmax, 1e-3));
+      cc.iir_warm = (int)std::min(4096.0, std::max(64.0, std::ceil(2.5 * decay) + 64.0));
+      cc.iir_chunk = vhsd::chroma::kIirChunk;
+    }
     if ((size_t)cc.probe_ext * vhsd::chroma::kProbeThreads * sizeof(float) > 160 * 1024)
       return fail(VHSDECODE_HIP_E_UNSUPPORTED, "burst window too long for the probe kernel");
     cc.sections = p->sections;
@@ -1952,12 +1975,11 @@ int32_t vhsd_decode_vhs_fields_from(vhsdecode_hip_demod_context* c, const float*
     // choice, analysis level adjusts, the chroma field and the final filter's two buffers
     vhsd::chroma::ChromaWork cw{};
     if (chroma) {
-      using vhsd::chroma::kFiltThreads;
       const vhsd::chroma::ChromaCfg& cc = c->ccfg;
       const size_t nf = (size_t)cc.lines * cc.line_len;
       const int lvl_stride = out_lines + max_first;
-      const int mext = (int)nf + 2 * cc.pad;
-      const int chunk = (mext + kFiltThreads - 1) / kFiltThreads;
+      const size_t mext = nf + 2 * (size_t)cc.pad;
+      const int chunks = (int)((mext + cc.iir_chunk - 1) / cc.iir_chunk);
       size_t co = 0;
       auto ctake = [&](size_t bytes) {
         const size_t o = co;
@@ -1968,7 +1990,9 @@ int32_t vhsd_decode_vhs_fields_from(vhsdecode_hip_demod_context* c, const float*
       const size_t o_flip = ctake(4 * F * 2), o_summ = ctake(8 * F * 2 * 4), o_ch = ctake(4 * F);
       const size_t o_fno = ctake(4 * F), o_ones = ctake(4 * F), o_ax = ctake(8 * F * lvl_stride);
       const size_t o_ay = ctake(8 * F * lvl_stride), o_fld = ctake(4 * F * nf);
-      const size_t o_fa = ctake(4 * F * (size_t)chunk * kFiltThreads), o_fb = ctake(4 * F * (size_t)chunk * kFiltThreads);
+      const size_t o_fa = ctake(4 * F * (size_t)chunks * cc.iir_chunk), o_fb = ctake(4 * F * (size_t)chunks * cc.iir_chunk);
+      const size_t o_th = ctake(cc.is_ntsc ? 8 * F * nf : 8);
+      const size_t o_ist = ctake(4 * F * (size_t)chunks * 2 * 2 * vhsd::chroma::kMaxSections);
       VHSD_HIP(c->c_work.ensure(co));
       unsigned char* cm = c->c_work.p;
       cw.plane = chroma->plane;
@@ -1986,7 +2010,9 @@ int32_t vhsd_decode_vhs_fields_from(vhsdecode_hip_demod_context* c, const float*
       cw.field = reinterpret_cast<float*>(cm + o_fld);
       cw.fa = reinterpret_cast<float*>(cm + o_fa);
       cw.fb = reinterpret_cast<float*>(cm + o_fb);
-      cw.filt_chunk = chunk;
+      cw.iir_state = reinterpret_cast<float*>(cm + o_ist);
+      cw.theta = reinterpret_cast<double*>(cm + o_th);
+      cw.iir_chunks = chunks;
       cw.state = reinterpret_cast<vhsd::chroma::ChromaState*>(c->c_state.p);
       cw.out = chroma->out;
     }
