@@ -778,6 +778,32 @@ VHSDECODE_HIP_API int32_t vhsdecode_hip_get_profile(vhsdecode_hip_demod_context*
 VHSDECODE_HIP_API int32_t vhsdecode_hip_debug_fft16k(vhsdecode_hip_demod_context* context, float* data,
                                                      int32_t count, int32_t inverse, void* stream);
 
+/* ---------------------------------------------------------------- LaserDisc audio stage 2
+ * LaserDiscAnalogAudioPhase2.Apply (src/VHSDecode.Core/Decode/LaserDiscAnalogAudioPhase2.cs:12-220),
+ * which RfBlockDecodePipeline.ApplyLaserDiscAnalogAudioPhase2 (:159-163) runs on the
+ * assembled analog audio of each stream read (RfBlockStreamDecoder.cs:913-914): per block of
+ * the stage-2 length overlapping by 512, left-channel peak suppression in both channels, the
+ * stage-2 response (BuildLdAnalogAudioStage2Filter, Dsp/DecodeFilterSet.cs:950-970), float32
+ * output.  stage2[ch]: block_length complex values (re, im interleaved, FP64); block_length
+ * must be 32768 (the 40 MSPS filter set's). */
+typedef struct vhsdecode_hip_ld_audio_phase2_v1 {
+  uint32_t struct_size;
+  int32_t block_length;
+  double center_hz[2];             /* LaserDiscAnalogAudioChannelFilter.CenterFrequencyHz, left / right */
+  const double* stage2[2];         /* Stage2Filter, left / right */
+} vhsdecode_hip_ld_audio_phase2_v1;
+
+/* Sets (params non-NULL) or clears the context's stage-2 filters and carriers. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_set_ld_audio_phase2(vhsdecode_hip_demod_context* context,
+                                                            const vhsdecode_hip_ld_audio_phase2_v1* params);
+
+/* Stage 2 of `count` samples per channel (device FP32 Hz at the slice rate, e.g. the span
+ * planes audio_left / audio_right) into out_left / out_right (device FP32, `count` each;
+ * must not overlay the inputs).  stream NULL: the context's stream, returning when done. */
+VHSDECODE_HIP_API int32_t vhsdecode_hip_ld_audio_phase2_device(vhsdecode_hip_demod_context* context,
+                                                               const float* left, const float* right, int64_t count,
+                                                               float* out_left, float* out_right, void* stream);
+
 /* Diagnostic: the block repair kernel's ReplaceSpikes step (RfDemodulator.ReplaceSpikes,
  * src/VHSDecode.Core/Dsp/RfDemodulator.cs:966-1001) on caller-supplied 32768-sample
  * demod and diffed-demod arrays (device FP32); writes the repaired demod to `out`.

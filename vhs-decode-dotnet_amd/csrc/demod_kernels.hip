@@ -1053,6 +1053,121 @@ __global__ void __launch_bounds__(1024) debug_fft_kernel(float2* data, int inver
 
 }  // namespace vhsd
 
+// LaserDisc analog audio stage 2 (LaserDiscAnalogAudioPhase2.Apply,
+// Decode/LaserDiscAnalogAudioPhase2.cs:12-220) over the assembled audio of a read: blocks
+// of the stage-2 length N = 32768 overlapping by 512 samples (the first at 0, the middle
+// ones every N - 512, the last at count - N - 1), one workgroup per (block, channel).  Per
+// block: the float32 offsets from the float32 carrier (PrepareChannel); the left channel's
+// peaks (values from 500 kHz up; a peak at i - 1 wherever sample i exceeds the block's last
+// value and sample i + 1 exceeds sample i) zero samples [p - 8, p + 8) of both channels;
+// the zero-padded block's real transform times the stage-2 response, the inverse
+// transform plus the carrier.  Each output sample is written by the one block whose copy
+// the reference keeps (the last block's copy is placed one sample late, as there).
+namespace vhsd {
+
+constexpr int kLdAudio2Skip = 512;
+__device__ __forceinline__ float ld2_level(float v) { return v < 500000.0f ? 0.0f : v; }
+
+__global__ void __launch_bounds__(1024) ld_audio_phase2_kernel(LdAudio2Args a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  BlockLds& s = *reinterpret_cast<BlockLds*>(lds_raw);
+  const int j = threadIdx.x;
+  const int b = blockIdx.x;
+  const int ch = blockIdx.y;
+  const long long count = a.count;
+  const int N = 2 * kFftM;
+  const long long J = N - kLdAudio2Skip;
+  const bool single = count <= N;
+  // 0 first, 1 middle, 2 last
+  const int kind = single || b == 0 ? 0 : (b <= a.nmid ? 1 : 2);
+  const long long start = kind == 0 ? 0 : (kind == 1 ? (long long)b * J : count - N - 1);
+  const int cnt = (int)min((long long)N, count - start);
+  load_octant(s, a.oct);
+  float* raw = reinterpret_cast<float*>(s.buf);
+  // the left channel's offsets, for the peaks
+  const float cl = (float)a.center[0];
+  for (int i = j; i < N; i += kFftThreads) raw[i] = i < cnt ? a.in[0][start + i] - cl : 0.0f;
+  __syncthreads();
+  // this thread's samples [32 j, 32 j + 32): zeroed where a peak i in [q - 6, q + 9] lies
+  unsigned zero = 0;
+  {
+    const float last = ld2_level(raw[cnt - 1]);
+    const int q0 = 32 * j;
+    float cur = 0.0f;
+    int i = max(0, q0 - 6);
+    if (i < cnt) cur = ld2_level(raw[i]);
+    for (; i <= q0 + 40 && i < cnt - 1; i++) {
+      const float nxt = ld2_level(raw[i + 1]);
+      if (cur > last && nxt > cur) {
+        const int p = i - 1;
+        const int lo = max(max(0, p - 8), q0), hi = min(min(p + 8, cnt), q0 + 32);
+        for (int q = lo; q < hi; q++) zero |= 1u << (q - q0);
+      }
+      cur = nxt;
+    }
+  }
+  __syncthreads();
+  if (ch == 1) {
+    const float cr = (float)a.center[1];
+    for (int i = j; i < N; i += kFftThreads) raw[i] = i < cnt ? a.in[1][start + i] - cr : 0.0f;
+    __syncthreads();
+  }
+  for (int r = 0; r < 32; r++)
+    if ((zero >> r) & 1u) raw[32 * j + r] = 0.0f;
+  __syncthreads();
+  c32 v[16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+    v[m] = c32{raw[2 * k], raw[2 * k + 1]};
+  }
+  __syncthreads();
+  fft16k_forward(v, s.buf, s.oct);
+  c32 xm;
+  rfft_post(v, s, xm);
+  const float2* filt = a.filt[ch];
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const float2 f = row_at(filt, m);
+    v[m] = mul(v[m], c32{f.x, f.y});
+  }
+  const float2 fM = filt[kFftM];
+  const c32 vm = mul(xm, c32{fM.x, fM.y});
+  irfft_pre(v, vm, s);
+  fft16k_inverse(v, s.buf, s.oct);
+  const float inv_n = 1.0f / 32768.0f;
+  const double center = a.center[ch];
+  const long long final_start = count - (N - kLdAudio2Skip);
+  float* out = a.out[ch];
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    const int k = j + m * kFftThreads;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int li = 2 * k + h;
+      const float y = (h == 0 ? v[m].x : v[m].y) * inv_n;
+      long long o;
+      bool own;
+      if (single) {
+        o = li;
+        own = li < cnt;
+      } else if (kind == 0) {
+        o = li;
+        own = o < final_start;
+      } else if (kind == 1) {
+        o = start + li;
+        own = li >= kLdAudio2Skip && li < cnt && o < final_start;
+      } else {
+        o = start + 1 + li;
+        own = li >= kLdAudio2Skip;
+      }
+      if (own) out[o] = (float)((double)y + center);
+    }
+  }
+}
+
+}  // namespace vhsd
+
 // ------------------------------------------------------------------ launchers
 template <typename K, typename... Args>
 static hipError_t launch_block_kernel(K kernel, int count, hipStream_t stream, Args... args) {
@@ -1099,6 +1214,14 @@ hipError_t vhsd_launch_block_chroma(const vhsd::ChromaArgs* a, int nblocks, hipS
 hipError_t vhsd_launch_debug_replace_spikes(const float* dm, const float* dd, int* list, double mx, float* out,
                                            hipStream_t stream) {
   return launch_block_kernel(vhsd::debug_replace_spikes_kernel, 1, stream, dm, dd, list, mx, out);
+}
+
+hipError_t vhsd_launch_ld_audio_phase2(const vhsd::LdAudio2Args* a, int nblocks, hipStream_t stream) {
+  const size_t lds = sizeof(vhsd::BlockLds);
+  hipError_t e = vhsd::lds_limit_once((const void*)vhsd::ld_audio_phase2_kernel, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(vhsd::ld_audio_phase2_kernel, dim3(nblocks, 2), dim3(1024), lds, stream, *a);
+  return hipGetLastError();
 }
 
 hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const float2* oct, hipStream_t stream) {

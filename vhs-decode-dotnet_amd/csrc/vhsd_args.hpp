@@ -209,6 +209,19 @@ struct DropoutArgs {
 
 }  // namespace vhsd
 
+namespace vhsd {
+// LaserDisc analog audio stage 2 over one read's assembled audio (both channels).
+struct LdAudio2Args {
+  const float* in[2];
+  float* out[2];
+  long long count;         // samples per channel
+  int nmid;                // middle blocks (between the first and the last)
+  double center[2];        // carrier of each channel (Hz)
+  const float2* filt[2];   // stage-2 half spectra, 16385 entries each
+  const float2* oct;
+};
+}  // namespace vhsd
+
 extern "C" {
 size_t vhsd_block_lds_bytes();
 hipError_t vhsd_launch_block_demod(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
@@ -223,6 +236,7 @@ hipError_t vhsd_launch_stage2_table(const vhsd::RepairArgs* rrecs, const vhsd::D
                                     const vhsd::SubDeemphArgs* srecs, vhsd::StageTable2 t, int blocks,
                                     hipStream_t stream);
 hipError_t vhsd_launch_fill_u32(unsigned* p, long long n, unsigned value, hipStream_t stream);
+hipError_t vhsd_launch_ld_audio_phase2(const vhsd::LdAudio2Args* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_video(const vhsd::DemodArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_block_subdeemph(const vhsd::SubDeemphArgs* a, int nblocks, hipStream_t stream);
 hipError_t vhsd_launch_debug_fft(float* data, int count, int inverse, const float2* oct, hipStream_t stream);

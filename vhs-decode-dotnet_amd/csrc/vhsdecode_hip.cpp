@@ -287,6 +287,9 @@ struct vhsdecode_hip_demod_context {
   bool chroma_on = false;
   vhsd::chroma::ChromaCfg ccfg{};
   DevBuf<float> c_het, c_sin, c_cos;
+  DevBuf<float2> ld2_filt[2];
+  double ld2_center[2] = {0.0, 0.0};
+  bool ld2_on = false;
   DevBuf<unsigned char> c_state, c_work;
   bool c_state_ready = false;
   const double* vs_lines = nullptr;    // line locations, hsync-refined and render locations of the
@@ -337,6 +340,7 @@ struct vhsdecode_hip_demod_context {
     t_f64.release(); t_u16.release(); s_rf.release(); s_planes.release(); s_flags.release();
     vs_mem.release(); vs_state.release();
     c_het.release(); c_sin.release(); c_cos.release(); c_state.release(); c_work.release();
+    ld2_filt[0].release(); ld2_filt[1].release();
     f_loc.release(); f_ref.release(); f_x.release(); f_y0.release(); f_pow.release(); f_fil.release();
     f_fo.release(); f_tab.release();
     if (vs_exec) (void)hipGraphExecDestroy(vs_exec);
@@ -2300,6 +2304,74 @@ int32_t vhsdecode_hip_debug_replace_spikes(vhsdecode_hip_demod_context* c, const
     VHSD_HIP(hipSetDevice(c->device));
     VHSD_HIP(c->dbg_list.ensure(32768));
     VHSD_HIP(vhsd_launch_debug_replace_spikes(demod, diffed, c->dbg_list.p, max_value, out, pick_stream(c, stream)));
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_set_ld_audio_phase2(vhsdecode_hip_demod_context* c, const vhsdecode_hip_ld_audio_phase2_v1* p) {
+  try {
+    if (!c) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid context");
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (!p) {
+      c->ld2_on = false;
+      return VHSDECODE_HIP_OK;
+    }
+    if (p->struct_size != sizeof(vhsdecode_hip_ld_audio_phase2_v1))
+      return fail(VHSDECODE_HIP_E_STRUCT_SIZE, "ld_audio_phase2 struct_size mismatch");
+    if (!p->stage2[0] || !p->stage2[1] || !std::isfinite(p->center_hz[0]) || !std::isfinite(p->center_hz[1]))
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid stage-2 filters or carriers");
+    if (p->block_length != 2 * vhsd::kFftM)
+      return fail(VHSDECODE_HIP_E_UNSUPPORTED, "stage-2 block length must be 32768");
+    VHSD_HIP(hipSetDevice(c->device));
+    const int M = vhsd::kFftM;
+    std::vector<float2> h(M + 1);
+    for (int ch = 0; ch < 2; ch++) {
+      for (int k = 0; k <= M; k++)
+        h[k] = make_float2((float)p->stage2[ch][2 * k], (float)p->stage2[ch][2 * k + 1]);
+      VHSD_HIP(c->ld2_filt[ch].ensure(M + 1));
+      VHSD_HIP(hipMemcpy(c->ld2_filt[ch].p, h.data(), (M + 1) * sizeof(float2), hipMemcpyHostToDevice));
+      c->ld2_center[ch] = p->center_hz[ch];
+    }
+    c->ld2_on = true;
+    return VHSDECODE_HIP_OK;
+  } catch (const std::exception& e) {
+    return fail(VHSDECODE_HIP_E_INTERNAL, e.what());
+  }
+}
+
+int32_t vhsdecode_hip_ld_audio_phase2_device(vhsdecode_hip_demod_context* c, const float* left, const float* right,
+                                             int64_t count, float* out_left, float* out_right, void* stream) {
+  try {
+    if (!c || count < 0 || (count > 0 && (!left || !right || !out_left || !out_right)))
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "invalid stage-2 audio request");
+    if (count > 0 && (out_left == left || out_left == right || out_right == left || out_right == right))
+      return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "stage-2 output must not overlay its input");
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (!c->ld2_on) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "vhsdecode_hip_set_ld_audio_phase2 not called");
+    if (count == 0) return VHSDECODE_HIP_OK;
+    VHSD_HIP(hipSetDevice(c->device));
+    const long long N = 2 * vhsd::kFftM, J = N - 512;
+    int nmid = 0;
+    if (count > N)
+      for (long long sample = J; sample < count - J; sample += J) nmid++;
+    const int nblocks = count <= N ? 1 : nmid + 2;
+    vhsd::LdAudio2Args a{};
+    a.in[0] = left;
+    a.in[1] = right;
+    a.out[0] = out_left;
+    a.out[1] = out_right;
+    a.count = count;
+    a.nmid = nmid;
+    a.center[0] = c->ld2_center[0];
+    a.center[1] = c->ld2_center[1];
+    a.filt[0] = c->ld2_filt[0].p;
+    a.filt[1] = c->ld2_filt[1].p;
+    a.oct = c->oct.p;
+    const hipStream_t st = pick_stream(c, stream);
+    VHSD_HIP(vhsd_launch_ld_audio_phase2(&a, nblocks, st));
+    if (!stream) VHSD_HIP(hipStreamSynchronize(st));
     return VHSDECODE_HIP_OK;
   } catch (const std::exception& e) {
     return fail(VHSDECODE_HIP_E_INTERNAL, e.what());

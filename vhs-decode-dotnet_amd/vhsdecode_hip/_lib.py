@@ -41,6 +41,7 @@ REQUIRED_EXPORTS = (
     "vhsdecode_hip_get_profile",
     "vhsdecode_hip_debug_fft16k",
     "vhsdecode_hip_debug_replace_spikes",
+    "vhsdecode_hip_set_ld_audio_phase2", "vhsdecode_hip_ld_audio_phase2_device",
     "vhsdecode_hip_batch_create",
     "vhsdecode_hip_batch_info",
     "vhsdecode_hip_batch_run_device",
@@ -157,6 +158,11 @@ class FieldParamsV1(ctypes.Structure):
                 ("field_lines", ctypes.c_int32), ("nominal_line_length", ctypes.c_double),
                 ("wow_level_smoothing", ctypes.c_double), ("output_zero", ctypes.c_int32),
                 ("vsync_ire", ctypes.c_double), ("output_scale", ctypes.c_double)]
+
+
+class LdAudioPhase2V1(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("block_length", ctypes.c_int32),
+                ("center_hz", ctypes.c_double * 2), ("stage2", ctypes.c_void_p * 2)]
 
 
 class ChromaParamsV1(ctypes.Structure):
@@ -329,6 +335,10 @@ def load():
     L.vhsdecode_hip_batch_destroy.restype = None
     L.vhsdecode_hip_debug_replace_spikes.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                                      ctypes.c_void_p, ctypes.c_void_p]
+    L.vhsdecode_hip_set_ld_audio_phase2.argtypes = [ctypes.c_void_p, ctypes.POINTER(LdAudioPhase2V1)]
+    L.vhsdecode_hip_ld_audio_phase2_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                       ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                                       ctypes.c_void_p]
     L.vhsdecode_hip_hifi_create.argtypes = [ctypes.POINTER(HifiConfigV1), ctypes.POINTER(ctypes.c_void_p)]
     L.vhsdecode_hip_hifi_destroy.argtypes = [ctypes.c_void_p]
     L.vhsdecode_hip_hifi_destroy.restype = None

@@ -412,6 +412,30 @@ class HipBlockDemodulator:
         return {name: {"launches": p.launches[i], "ms": p.total_ms[i], "units": p.units[i]}
                 for i, name in enumerate(_lib.KERNEL_NAMES)}
 
+    def set_ld_audio_phase2(self, filters):
+        """LaserDisc analog audio stage 2 (LaserDiscAnalogAudioPhase2.Apply) for this context:
+        `filters` a LaserDiscFilterSet with analog audio (its channels' stage-2 responses and
+        carriers), or None to clear."""
+        if filters is None:
+            _lib.check(self._L.vhsdecode_hip_set_ld_audio_phase2(self._h, None))
+            return
+        chans = (filters.audio_left, filters.audio_right)
+        st = [np.ascontiguousarray(np.asarray(c.stage2, np.complex128)).view(np.float64) for c in chans]
+        p = _lib.LdAudioPhase2V1()
+        p.struct_size = ctypes.sizeof(_lib.LdAudioPhase2V1)
+        p.block_length = int(chans[0].stage2.size)
+        for k, c in enumerate(chans):
+            p.center_hz[k] = float(c.center_hz)
+            p.stage2[k] = st[k].ctypes.data
+        _lib.check(self._L.vhsdecode_hip_set_ld_audio_phase2(self._h, ctypes.byref(p)))
+
+    def ld_audio_phase2_device(self, left_ptr: int, right_ptr: int, count: int, out_left_ptr: int,
+                               out_right_ptr: int, stream: int = 0):
+        """Stage 2 over `count` samples per channel of device FP32 audio (Hz at the slice rate)."""
+        v = ctypes.c_void_p
+        _lib.check(self._L.vhsdecode_hip_ld_audio_phase2_device(self._h, v(left_ptr), v(right_ptr), count,
+                                                                v(out_left_ptr), v(out_right_ptr), v(stream)))
+
     def debug_fft16k(self, data_ptr: int, count: int, inverse: bool = False, stream: int = 0):
         _lib.check(self._L.vhsdecode_hip_debug_fft16k(self._h, ctypes.c_void_p(data_ptr), count,
                                                       1 if inverse else 0, ctypes.c_void_p(stream)))
