@@ -169,7 +169,15 @@ def test_whole_run_writes_the_span_decode_chroma(system, tmp_path):
     base = str(tmp_path / "run")
     field_len = int(round(40e6 / (2 * float(fmt.sysparams["FPS"]))))
     res = decode_run(tape.rf, fmt, fl, base, sinc_table=lut, dropouts=False, span_samples=4 * field_len,
-                     chroma=True)
+                     chroma=True, sqlite=True)
+    import sqlite3
+    con = sqlite3.connect(base + ".tbc.db")
+    try:
+        assert con.execute("SELECT COUNT(*), MIN(field_id), MAX(capture_id) FROM field_record").fetchone() == \
+            (res.fields_written, 0, 0)
+        assert con.execute("SELECT system, field_width FROM capture").fetchone()[0] == system
+    finally:
+        con.close()
     print(f"\n[chroma run] {system} fields {res.fields_written} spans {res.spans}; one span {len(keys)}", flush=True)
     assert res.spans >= 2 and res.fields_written == len(keys) >= 6
     g = dec.geom

@@ -43,12 +43,13 @@ def decode_run(rf: np.ndarray, fmt: FormatParameters, filters: TapeFilterSet, ou
                span_samples: int = 0, maximum_output_fields: int = 0, overwrite: bool = True,
                cancel: Optional[Callable[[], bool]] = None, block_cut: int = 1024, block_cut_end: int = 1024,
                device: int = 0, deliver: Optional[Callable[[int, int], int]] = None,
-               chroma: bool = False) -> RunResult:
+               chroma: bool = False, sqlite: bool = False) -> RunResult:
     """Decodes the whole of `rf` (int16, or float32 samples the library rounds to int16)
     read through the library's callback; returns the run's result record.  `deliver`
     (offset, count) -> samples, when given, bounds what each read returns (a short read
     ends the input, or reports a cancel).  `chroma`: also write the colour-under chroma to
-    <output_base>_chroma.tbc (the luma then renders at the burst-refined locations)."""
+    <output_base>_chroma.tbc (the luma then renders at the burst-refined locations).
+    `sqlite`: also write <output_base>.tbc.db from the run's .tbc.json (tbc_db)."""
     L = _lib.load()
     src = np.ascontiguousarray(rf)
     if src.dtype == np.int16:
@@ -98,5 +99,8 @@ def decode_run(rf: np.ndarray, fmt: FormatParameters, filters: TapeFilterSet, ou
     res = _lib.RunResultV1()
     res.struct_size = ctypes.sizeof(_lib.RunResultV1)
     _lib.check(L.vhsdecode_hip_run(ctypes.byref(cfg), ctypes.byref(res)))
+    if sqlite:
+        from .tbc_db import write_tbc_db
+        write_tbc_db(output_base + ".tbc.json", output_base + ".tbc.db", dropouts_enabled=dropouts)
     return RunResult(res.fields_written, res.output_line_length, res.output_field_lines, res.elapsed_seconds,
                      res.spans, res.dropout_rows)
