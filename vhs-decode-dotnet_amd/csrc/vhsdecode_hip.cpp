@@ -1658,10 +1658,10 @@ int32_t vhsdecode_hip_set_vhs_chroma(vhsdecode_hip_demod_context* c, const vhsde
     cc.win_max = p->burst_end + (p->burst_end - p->burst_start);
     cc.pad = default_pad_length(&p->final_sos[0][0], p->sections);
     cc.probe_ext = cc.win_max + 2 * cc.pad;
-    // the whole-field band-pass's chunks and warm-up: 2.5 times the samples its slowest
-    // pole takes to forget a state to 1e-8, plus a margin (shorter warm-ups leave many chunks
-    // to run again on decoded fields; exactness does not depend on it: a chunk whose warm-up
-    // fell short is run again)
+    // the whole-field band-pass's chunks and warm-up: twice the samples its slowest pole
+    // takes to forget a state to 1e-8, plus a margin (measured fastest on decoded fields:
+    // shorter warm-ups leave many chunks to run again, longer ones lengthen every chunk;
+    // exactness does not depend on it: a chunk whose warm-up fell short is run again)
     {
       double rmax = 0.0;
       for (int s = 0; s < p->sections; s++) {
@@ -1678,7 +1678,7 @@ int32_t vhsdecode_hip_set_vhs_chroma(vhsdecode_hip_demod_context* c, const vhsde
       }
       if (!(rmax < 1.0)) return fail(VHSDECODE_HIP_E_INVALID_ARGUMENT, "the final chroma band-pass is not stable");
       const double decay = std::log(1e-8) / std::log(std::max(rmax, 1e-3));
-      cc.iir_warm = (int)std::min(4096.0, std::max(64.0, std::ceil(2.5 * decay) + 64.0));
+      cc.iir_warm = (int)std::min(4096.0, std::max(64.0, std::ceil(2.0 * decay) + 64.0));
       cc.iir_chunk = vhsd::chroma::kIirChunk;
     }
     if ((size_t)cc.probe_ext * vhsd::chroma::kProbeThreads * sizeof(float) > 160 * 1024)
