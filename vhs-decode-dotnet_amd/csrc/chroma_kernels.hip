@@ -312,18 +312,20 @@ __global__ void __launch_bounds__(128) chroma_summary_kernel(ChromaCfg cc, Chrom
   cw.flip[f * 2 + cand] = cc.is_ntsc ? (d[0] < d[2]) : ((d[1] + d[3]) < (d[0] + d[2]));
   double ei = 0.0, eq = 0.0, oi = 0.0, oq = 0.0, mag = 0.0;
   int count = 0;
+  // branch-free, so the loads of several lines issue together: a skipped line adds +0.0,
+  // which leaves a sum started at +0.0 unchanged bit for bit (round to nearest never
+  // produces -0.0 from it)
+#pragma unroll 8
   for (int k = 0; k < cc.lines; k++) {
-    if (!use_s[cand][k]) continue;
-    const double m = mag_s[cand][k];
-    count++;
-    mag += m;
-    if (((k + 1) & 1) == 1) {
-      oi += ni_s[cand][k];
-      oq += nq_s[cand][k];
-    } else {
-      ei += ni_s[cand][k];
-      eq += nq_s[cand][k];
-    }
+    const bool u = use_s[cand][k] != 0;
+    const bool odd = ((k + 1) & 1) == 1;
+    const double m = mag_s[cand][k], a = ni_s[cand][k], b = nq_s[cand][k];
+    count += u ? 1 : 0;
+    mag += u ? m : 0.0;
+    oi += u && odd ? a : 0.0;
+    oq += u && odd ? b : 0.0;
+    ei += u && !odd ? a : 0.0;
+    eq += u && !odd ? b : 0.0;
   }
   double* sm = cw.summ + ((long long)f * 2 + cand) * 4;
   sm[0] = count > 0 ? mag / count : 0.0;
