@@ -202,6 +202,7 @@ __global__ void __launch_bounds__(kProbeThreads) chroma_probe_kernel(ChromaCfg c
   float* e = probe_lds + threadIdx.x;
   const float* win = cw.win + ((long long)f * cc.lines + q) * cc.win_max;
   const float* het = cc.het + (long long)rot * n;
+#pragma unroll 16
   for (int i = 0; i < nw; i++) e[(P + i) * kProbeThreads] = het[w.ps + i] * win[i];
   if (nw > P) {
     const float first = e[P * kProbeThreads];
