@@ -400,18 +400,24 @@ __global__ void __launch_bounds__(256) chroma_field_kernel(ChromaCfg cc, ChromaW
                                                            double nominal, int smooth,
                                                            const float* __restrict__ lut, const int* decoded) {
 #pragma clang fp contract(off)
+  __shared__ float4 rows[256 / 64][kTbcStage];
   const int f = blockIdx.y;
   if (f >= *decoded) return;
   const int n = cc.lines * cc.line_len;
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= n) return;
+  const int d0 = blockIdx.x * blockDim.x + threadIdx.x;
+  // every lane takes part in the wave's table-row fetch; lanes past the field repeat its
+  // last sample
+  const bool valid = d0 < n;
+  const int d = valid ? d0 : n - 1;
   const int first = first_lines[f];
   const long long org = origins[f];
   double pos, adj;
   position_and_level(render + (long long)f * line_stride, d + first * cc.line_len, cc.line_len, nominal,
                      line_x + (long long)f * cw.lvl_stride, line_y0 + (long long)f * cw.lvl_stride, pow_table, smooth,
                      &pos, &adj);
-  double v = (double)(float)(sample_sinc(cw.plane + org, cw.plane_len - org, 0.0, pos, lut) * adj);
+  const double sv = sample_sinc_wave(cw.plane + org, cw.plane_len - org, 0.0, pos, lut, rows[threadIdx.x >> 6]);
+  if (!valid) return;
+  double v = (double)(float)(sv * adj);
   const int j = d / cc.line_len;
   const int smp = d - j * cc.line_len;
   float o;

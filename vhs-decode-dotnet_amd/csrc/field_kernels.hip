@@ -740,9 +740,6 @@ struct FieldTbcParams {
 // Stage for the cooperative table-row fetch: per wave, 64 pixels x 8 float4 (two rows),
 // slot s of pixel p at p * 8 + (s ^ (p & 7)) so that the 16-byte accesses of eight
 // consecutive pixels fall on distinct banks; then the wave's source window (256 floats).
-constexpr int kTbcWindow = 256;
-constexpr int kTbcStage = 64 * 8 + kTbcWindow / 4;
-__device__ __forceinline__ int tbc_slot(int pix, int s) { return pix * 8 + (s ^ (pix & 7)); }
 
 __global__ void __launch_bounds__(256) tbc_fields_kernel(const float* src, long long src_len, double bias,
                                                          const long long* origins, const double* locs,
